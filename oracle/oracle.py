@@ -1,0 +1,279 @@
+"""ctypes front-end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this module.  The product (``taremincloth_amd``) never
+does.  ``OracleSim`` mirrors the reference ``ClothSimulator`` surface
+(crates/cloth_py/src/lib.rs:1556-1786, 2110-2175) so parity tests read like the
+reference's own tests.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_BUILD = os.path.join(_HERE, "build")
+_LIBS: dict[str, C.CDLL] = {}
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+# orc_mesh_get field ids (xpbd_oracle.c)
+FIELDS = [
+    "dist_v0", "dist_v1", "dist_type", "dist_rest", "dist_tc", "dist_cc", "dist_col_off", "dist_col_cnt",
+    "bend_v0", "bend_v1", "bend_v2", "bend_v3", "bend_rest", "bend_comp", "bend_pad0", "bend_col_off",
+    "bend_col_cnt", "adj_off", "adj_idx", "th_off", "th_idx", "th_rest", "local_len", "star_off",
+    "star_v0", "star_v1", "edge_v0", "edge_v1", "edge_bbits", "island", "scalars", "edge_to_constraint",
+    "layer",
+]
+_FLOAT_FIELDS = {"dist_rest", "dist_tc", "dist_cc", "bend_rest", "bend_comp", "bend_pad0", "th_rest", "local_len"}
+
+
+def build() -> None:
+    """Compile the oracle libraries with its Makefile (gcc only)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _opt(arr, dtype):
+    if arr is None:
+        return None
+    return np.ascontiguousarray(arr, dtype=dtype)
+
+
+def lib(precision: str = "f32") -> C.CDLL:
+    if precision in _LIBS:
+        return _LIBS[precision]
+    path = os.path.join(_BUILD, f"liborc_{precision}.so")
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    L.orc_mesh_build.restype = _vp
+    L.orc_mesh_build.argtypes = [_vp, C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32, _vp, _vp, _vp, C.c_uint32,
+                                 C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.orc_mesh_get.restype = C.c_uint32
+    L.orc_mesh_get.argtypes = [_vp, C.c_int, _vp]
+    L.orc_mesh_free.argtypes = [_vp]
+    L.orc_sim_create.restype = _vp
+    L.orc_sim_create.argtypes = [_vp, C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32, _vp, _vp, _vp, C.c_uint32,
+                                 C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.orc_sim_free.argtypes = [_vp]
+    L.orc_sim_mesh.restype = _vp
+    L.orc_sim_mesh.argtypes = [_vp]
+    L.orc_sim_set_gravity.argtypes = [_vp, C.c_float, C.c_float, C.c_float]
+    L.orc_sim_set_damping.argtypes = [_vp, C.c_float]
+    L.orc_sim_set_damping_all.argtypes = [_vp, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.orc_sim_set_iterations.argtypes = [_vp, C.c_uint32]
+    L.orc_sim_set_self_collision.argtypes = [_vp, C.c_int]
+    L.orc_sim_set_hash_cell.argtypes = [_vp, C.c_float]
+    L.orc_sim_set_self_collision_options.argtypes = [_vp, C.c_float, C.c_float, C.c_int, C.c_uint32]
+    L.orc_sim_set_compliances_raw.argtypes = [_vp, C.c_float, C.c_float, C.c_float]
+    L.orc_sim_add_sphere.argtypes = [_vp, _f32p, C.c_float, C.c_float, C.c_float]
+    L.orc_sim_add_capsule.argtypes = [_vp, _f32p, _f32p, C.c_float, C.c_float, C.c_float]
+    L.orc_sim_add_plane.argtypes = [_vp, _f32p, _f32p, C.c_float, C.c_float]
+    L.orc_sim_clear_colliders.argtypes = [_vp]
+    L.orc_sim_reset.argtypes = [_vp]
+    L.orc_sim_set_state.argtypes = [_vp, _vp, _vp, _vp]
+    L.orc_sim_get.argtypes = [_vp, C.c_int, _f64p]
+    L.orc_sim_get_lambdas.argtypes = [_vp, _f64p, _f64p]
+    L.orc_sim_step.restype = C.c_uint32
+    L.orc_sim_step.argtypes = [_vp, C.c_float, C.c_uint32]
+    L.orc_sim_step_single_substep.argtypes = [_vp, C.c_float]
+    L.orc_sim_substep_parts.argtypes = [_vp, C.c_float, C.c_int, C.c_int, C.c_int]
+    L.orc_sim_residual.argtypes = [_vp, C.c_float, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.orc_areal_inv_masses.argtypes = [_f32p, C.c_uint32, _u32p, C.c_uint32, C.c_float, _vp, _f32p]
+    L.orc_color_edge_pairs.argtypes = [C.c_uint32, _u32p, C.c_uint32, _u32p]
+    L.orc_hash_points.argtypes = [_f32p, C.c_uint32, C.c_float, C.c_uint32, _u32p, _u32p, _u32p]
+    L.orc_real_bytes.restype = C.c_int
+    L.orc_sim_stats.argtypes = [_vp, np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS"), C.c_int]
+    _LIBS[precision] = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def mesh_fields(L: C.CDLL, mh) -> dict[str, np.ndarray]:
+    out = {}
+    for fid, name in enumerate(FIELDS):
+        n = L.orc_mesh_get(mh, fid, None)
+        arr = np.empty(n, dtype=np.float32 if name in _FLOAT_FIELDS else np.uint32)
+        if n:
+            L.orc_mesh_get(mh, fid, arr.ctypes.data_as(C.c_void_p))
+        out[name] = arr
+    s = out["scalars"]
+    out["nv"], out["nd"], out["nb"], out["nd_col"], out["nb_col"], out["ne"], out["ntri"] = (int(x) for x in s[:7])
+    out["n_virtual"], out["table_size"], out["has_multiple_layers"] = int(s[7]), int(s[8]), int(s[9])
+    out["cell_size"], out["mean_edge"], out["max_edge"] = (float(np.array(x, np.uint32).view(np.float32)) for x in s[10:13])
+    return out
+
+
+def build_mesh(positions, edges, faces=None, inv_masses=None, thicknesses=None, layer_ids=None, layer_id=0,
+               thickness=0.005, stiffness=1000.0, bending_stiffness=10.0, compression_stiffness=None,
+               shear_stiffness=None, precision="f32") -> dict[str, np.ndarray]:
+    """ClothMesh::from_raw_with_opts restated (mesh.rs:407-1377), returned as arrays."""
+    L = lib(precision)
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    e = np.ascontiguousarray(edges, np.uint32).reshape(-1, 2)
+    f = _opt(faces, np.uint32)
+    comp = stiffness if compression_stiffness is None else compression_stiffness
+    shear = stiffness * 0.5 if shear_stiffness is None else shear_stiffness
+    im, th, ly = _opt(inv_masses, np.float32), _opt(thicknesses, np.float32), _opt(layer_ids, np.uint32)
+    mh = L.orc_mesh_build(_ptr(pos), len(pos), _ptr(e), len(e), _ptr(f), 0 if f is None else len(f.reshape(-1, 3)),
+                          _ptr(im), _ptr(th), _ptr(ly), layer_id, thickness, stiffness, comp, shear, bending_stiffness)
+    try:
+        return mesh_fields(L, mh)
+    finally:
+        L.orc_mesh_free(mh)
+
+
+def areal_inv_masses(positions, faces, density, pin_weights=None) -> np.ndarray:
+    """compute_combined_inv_masses restated (mesh.rs:237-312), faces only."""
+    L = lib("f32")
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    f = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+    out = np.empty(len(pos), np.float32)
+    pw = _opt(pin_weights, np.float32)
+    L.orc_areal_inv_masses(pos, len(pos), f, len(f), density, _ptr(pw), out)
+    return out
+
+
+def color_edge_pairs(n, pairs) -> np.ndarray:
+    L = lib("f32")
+    p = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+    out = np.empty(len(p), np.uint32)
+    L.orc_color_edge_pairs(n, p, len(p), out)
+    return out
+
+
+def hash_points(points, cell, table):
+    """Spatial-hash keys, cell starts (table+1) and stably sorted indices."""
+    L = lib("f32")
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    keys = np.empty(len(p), np.uint32)
+    starts = np.empty(table + 1, np.uint32)
+    srt = np.empty(len(p), np.uint32)
+    L.orc_hash_points(p, len(p), cell, table, keys, starts, srt)
+    return keys, starts, srt
+
+
+class OracleSim:
+    """CPU restatement of ClothSimulator (decoupled colliders, self-collision DIRECT)."""
+
+    def __init__(self, positions, edges, faces=None, inv_masses=None, thicknesses=None, layer_ids=None,
+                 layer_id=0, thickness=0.005, stiffness=1000.0, bending_stiffness=10.0,
+                 compression_stiffness=None, shear_stiffness=None, precision="f32"):
+        self._L = lib(precision)
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        e = np.ascontiguousarray(edges, np.uint32).reshape(-1, 2)
+        f = _opt(faces, np.uint32)
+        if f is not None and f.size == 0:
+            f = None
+        comp = stiffness if compression_stiffness is None else compression_stiffness
+        shear = stiffness * 0.5 if shear_stiffness is None else shear_stiffness
+        im, th, ly = _opt(inv_masses, np.float32), _opt(thicknesses, np.float32), _opt(layer_ids, np.uint32)
+        self._h = self._L.orc_sim_create(_ptr(pos), len(pos), _ptr(e), len(e), _ptr(f),
+                                         0 if f is None else len(f.reshape(-1, 3)), _ptr(im), _ptr(th), _ptr(ly),
+                                         layer_id, thickness, stiffness, bending_stiffness, comp, shear)
+        self.n = len(pos)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.orc_sim_free(h)
+            self._h = None
+
+    def mesh(self) -> dict[str, np.ndarray]:
+        return mesh_fields(self._L, self._L.orc_sim_mesh(self._h))
+
+    def set_gravity(self, gx, gy, gz):
+        self._L.orc_sim_set_gravity(self._h, gx, gy, gz)
+
+    def set_damping(self, d):
+        self._L.orc_sim_set_damping(self._h, d)
+
+    def set_damping_all(self, t, c, s, b):
+        self._L.orc_sim_set_damping_all(self._h, t, c, s, b)
+
+    def set_solver_iterations(self, it):
+        self._L.orc_sim_set_iterations(self._h, it)
+
+    def set_enable_self_collision(self, on):
+        self._L.orc_sim_set_self_collision(self._h, 1 if on else 0)
+
+    def set_self_collision_options(self, relief_factor, max_displacement_ratio, exclude_neighbors=True,
+                                   enable_normal_untangling=True, max_iterations=256):
+        self._L.orc_sim_set_self_collision_options(self._h, relief_factor, max_displacement_ratio,
+                                                   1 if exclude_neighbors else 0, max_iterations)
+
+    def set_hash_cell_size(self, cell):
+        self._L.orc_sim_set_hash_cell(self._h, cell)
+
+    def set_compliances_raw(self, stretch, shear, bend):
+        self._L.orc_sim_set_compliances_raw(self._h, stretch, shear, bend)
+
+    def add_sphere_collider(self, center, radius, friction=0.0, restitution=0.0):
+        self._L.orc_sim_add_sphere(self._h, np.asarray(center, np.float32), radius, friction, restitution)
+
+    def add_capsule_collider(self, a, b, radius, friction=0.0, restitution=0.0):
+        self._L.orc_sim_add_capsule(self._h, np.asarray(a, np.float32), np.asarray(b, np.float32), radius,
+                                    friction, restitution)
+
+    def add_plane_collider(self, point, normal, friction=0.0, restitution=0.0):
+        self._L.orc_sim_add_plane(self._h, np.asarray(point, np.float32), np.asarray(normal, np.float32),
+                                  friction, restitution)
+
+    def clear_colliders(self):
+        self._L.orc_sim_clear_colliders(self._h)
+
+    def reset(self):
+        self._L.orc_sim_reset(self._h)
+
+    def step(self, dt=1.0 / 60.0, substeps=None, solver_iterations=None):
+        if solver_iterations is not None:
+            self.set_solver_iterations(solver_iterations)
+        return self._L.orc_sim_step(self._h, dt, 0 if substeps is None else substeps)
+
+    def step_single_substep(self, dt_sub):
+        self._L.orc_sim_step_single_substep(self._h, dt_sub)
+
+    def substep_parts(self, dt_sub, collide=True, self_collide=True, update=True):
+        self._L.orc_sim_substep_parts(self._h, dt_sub, int(collide), int(self_collide), int(update))
+
+    def residual(self, dt_sub):
+        a, b = C.c_double(), C.c_double()
+        self._L.orc_sim_residual(self._h, dt_sub, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def state(self, which=0) -> np.ndarray:
+        out = np.empty(self.n * 3, np.float64)
+        self._L.orc_sim_get(self._h, which, out)
+        return out.reshape(-1, 3)
+
+    def set_state(self, x=None, p=None, v=None):
+        cv = [None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1) for a in (x, p, v)]
+        self._L.orc_sim_set_state(self._h, *[_ptr(a) for a in cv])
+
+    def lambdas(self):
+        m = self.mesh()
+        ld, lb = np.empty(m["nd"], np.float64), np.empty(m["nb"], np.float64)
+        self._L.orc_sim_get_lambdas(self._h, ld, lb)
+        return ld, lb
+
+    def contact_stats(self, clear=True) -> dict[str, int]:
+        """Contacts resolved since the last call: vv, vt_ccd, vt_prox, ee_ccd, ee_prox."""
+        out = np.zeros(6, np.uint64)
+        self._L.orc_sim_stats(self._h, out, 1 if clear else 0)
+        return dict(zip(["vv", "vt_ccd", "vt_prox", "ee_ccd", "ee_prox"], (int(x) for x in out[:5])))
+
+    def get_positions(self, out=None):
+        x = self.state(0).astype(np.float32).reshape(-1)
+        if out is not None:
+            out[: x.size] = x
+            return out
+        return x

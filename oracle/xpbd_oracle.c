@@ -1,0 +1,1978 @@
+/*
+ * CPU ORACLE for the XPBD cloth substep -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This file restates, in plain scalar C, the algorithm of the reference
+ * cloth_core substep so that the HIP product can be checked against it.
+ * It is never linked into the product library; only tests/, the smoke check
+ * and the CPU-baseline leg of bench.py load it (through oracle/oracle.py).
+ *
+ * It is compiled twice (oracle/Makefile):
+ *   liborc_f32.so  with ORC_REAL=float   -- same arithmetic type as the
+ *                                           reference shaders (fp32)
+ *   liborc_f64.so  with ORC_REAL=double  -- the FP64 twin used to state the
+ *                                           numerical error of fp32 results
+ * The host-side mesh build (constraint generation, colouring, topology) is
+ * always fp32/integer, as the reference computes it on the host in f32.
+ *
+ * Reference functions restated here (paths relative to the reference root):
+ *   mesh build ............ crates/cloth_core/src/mesh.rs:407-1377
+ *   inverse masses ........ crates/cloth_core/src/mesh.rs:237-312
+ *   Welsh-Powell colouring  crates/cloth_core/src/coloring.rs:6-173
+ *   virtual sample count .. crates/cloth_core/src/virtual_mesh.rs:75-200
+ *   hash cell/table rule .. crates/cloth_core/src/simulation/pipelines.rs:899-912
+ *   substep loop .......... crates/cloth_core/src/simulation/dispatch.rs:7-354
+ *   predict ............... crates/cloth_core/src/shaders/predict.wgsl:33-55
+ *   distance .............. crates/cloth_core/src/shaders/distance.wgsl:60-124
+ *   bending ............... crates/cloth_core/src/shaders/bending.wgsl:59-126
+ *   collider resolve ...... crates/cloth_core/src/shaders/collision.wgsl:83-108, 273-652
+ *   spatial hash .......... crates/cloth_core/src/shaders/spatial_grid_sort.wgsl:25-195
+ *   edge hash ............. crates/cloth_core/src/shaders/spatial_grid_sort_edges.wgsl:51-199
+ *   self collision V-V/V-T  crates/cloth_core/src/shaders/self_collision.wgsl:71-681
+ *   self collision E-E .... crates/cloth_core/src/shaders/self_collision_ee_direct.wgsl:66-776
+ *   self collision apply .. crates/cloth_core/src/shaders/self_collision_apply.wgsl:57-177
+ *   update_vel ............ crates/cloth_core/src/shaders/update_vel.wgsl:33-53
+ *
+ * Where the parity of each function is pinned (golden fixtures, closed
+ * forms, brute force) is listed in the test files under tests/.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef ORC_REAL
+#define ORC_REAL float
+#endif
+typedef ORC_REAL real;
+
+/* Correctly rounded sqrt/floor in the working type.  For float, sqrt in
+ * double then rounding to float is correctly rounded (53 >= 2*24+2). */
+#define RSQRT(x) ((real)sqrt((double)(x)))
+#define RFLOOR(x) ((real)floor((double)(x)))
+
+typedef struct { real x, y, z; } v3;
+
+static inline v3 mk(real x, real y, real z) { v3 r = {x, y, z}; return r; }
+static inline v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 mul(v3 a, real s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline v3 dvs(v3 a, real s) { return mk(a.x / s, a.y / s, a.z / s); }
+static inline v3 neg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+static inline real dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 cross(v3 a, v3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline real len3(v3 a) { return RSQRT(dot(a, a)); }
+static inline real rmax(real a, real b) { return a > b ? a : b; }
+static inline real rmin(real a, real b) { return a < b ? a : b; }
+static inline real rclamp(real x, real lo, real hi) { return rmin(rmax(x, lo), hi); }
+/* WGSL mix(a, b, t) = a * (1 - t) + b * t */
+static inline real rmix(real a, real b, real t) { return a * ((real)1 - t) + b * t; }
+static inline v3 vmix(v3 a, v3 b, real t) {
+    return mk(rmix(a.x, b.x, t), rmix(a.y, b.y, t), rmix(a.z, b.z, t));
+}
+static inline v3 vmin(v3 a, v3 b) { return mk(rmin(a.x, b.x), rmin(a.y, b.y), rmin(a.z, b.z)); }
+static inline v3 vmax(v3 a, v3 b) { return mk(rmax(a.x, b.x), rmax(a.y, b.y), rmax(a.z, b.z)); }
+
+/* ------------------------------------------------------------------ */
+/* small growable arrays                                               */
+/* ------------------------------------------------------------------ */
+typedef struct { uint32_t *d; size_t n, cap; } vu32;
+typedef struct { float *d; size_t n, cap; } vf32;
+static void vu_push(vu32 *v, uint32_t x) {
+    if (v->n == v->cap) { v->cap = v->cap ? v->cap * 2 : 16; v->d = (uint32_t *)realloc(v->d, v->cap * 4); }
+    v->d[v->n++] = x;
+}
+static void vf_push(vf32 *v, float x) {
+    if (v->n == v->cap) { v->cap = v->cap ? v->cap * 2 : 16; v->d = (float *)realloc(v->d, v->cap * 4); }
+    v->d[v->n++] = x;
+}
+
+/* open-addressing set of (u32,u32) keys */
+typedef struct { uint64_t *k; size_t cap, n; } pset;
+#define PSET_EMPTY 0xFFFFFFFFFFFFFFFFull
+static void pset_init(pset *s, size_t expect) {
+    size_t cap = 64;
+    while (cap < expect * 2 + 8) cap <<= 1;
+    s->cap = cap; s->n = 0;
+    s->k = (uint64_t *)malloc(cap * 8);
+    for (size_t i = 0; i < cap; i++) s->k[i] = PSET_EMPTY;
+}
+static uint64_t mix64(uint64_t x) {
+    x ^= x >> 33; x *= 0xff51afd7ed558ccdull; x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ull; x ^= x >> 33;
+    return x;
+}
+static int pset_insert(pset *s, uint32_t a, uint32_t b); /* returns 1 if newly inserted */
+static void pset_grow(pset *s) {
+    pset t; pset_init(&t, s->cap);
+    for (size_t i = 0; i < s->cap; i++)
+        if (s->k[i] != PSET_EMPTY) pset_insert(&t, (uint32_t)(s->k[i] >> 32), (uint32_t)s->k[i]);
+    free(s->k); *s = t;
+}
+static int pset_insert(pset *s, uint32_t a, uint32_t b) {
+    if ((s->n + 1) * 2 > s->cap) pset_grow(s);
+    uint64_t key = ((uint64_t)a << 32) | b;
+    size_t m = s->cap - 1, i = (size_t)mix64(key) & m;
+    while (s->k[i] != PSET_EMPTY) {
+        if (s->k[i] == key) return 0;
+        i = (i + 1) & m;
+    }
+    s->k[i] = key; s->n++;
+    return 1;
+}
+static int pset_has(const pset *s, uint32_t a, uint32_t b) {
+    uint64_t key = ((uint64_t)a << 32) | b;
+    size_t m = s->cap - 1, i = (size_t)mix64(key) & m;
+    while (s->k[i] != PSET_EMPTY) {
+        if (s->k[i] == key) return 1;
+        i = (i + 1) & m;
+    }
+    return 0;
+}
+static void pset_free(pset *s) { free(s->k); s->k = NULL; }
+
+static int cmpu64(const void *a, const void *b) {
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : x > y;
+}
+static int cmp_u32(const void *a, const void *b) {
+    uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+    return x < y ? -1 : x > y;
+}
+/* sort + dedup, returns new length */
+static size_t sort_unique(uint32_t *a, size_t n) {
+    if (!n) return 0;
+    qsort(a, n, 4, cmp_u32);
+    size_t w = 1;
+    for (size_t i = 1; i < n; i++) if (a[i] != a[w - 1]) a[w++] = a[i];
+    return w;
+}
+static int bsearch_u32(const uint32_t *a, size_t n, uint32_t x) {
+    size_t lo = 0, hi = n;
+    while (lo < hi) { size_t mid = lo + (hi - lo) / 2; if (a[mid] == x) return 1; if (a[mid] < x) lo = mid + 1; else hi = mid; }
+    return 0;
+}
+
+static float fdist(const float *a, const float *b) { /* (b - a) as in virtual_mesh.rs:202-207 */
+    float dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+    return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+static float frest(const float *p0, const float *p1) { /* (p0 - p1) as in mesh.rs:480-485 */
+    float dx = p0[0] - p1[0], dy = p0[1] - p1[1], dz = p0[2] - p1[2];
+    return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+static float compliance_of(float k) { /* mesh.rs:450-472 */
+    if (k >= 5000.0f) return 0.0f;
+    if (k > 0.0f) return 1.0f / k;
+    return 1e10f;
+}
+
+/* ------------------------------------------------------------------ */
+/* mesh                                                                */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    uint32_t nv;
+    float *pos0;      /* 3*nv initial positions */
+    float *inv_mass;  /* nv */
+    float *thick;     /* nv */
+    uint32_t *layer;  /* nv */
+
+    uint32_t nd;      /* distance constraints, colour-sorted */
+    uint32_t *d_v0, *d_v1, *d_type;
+    float *d_rest, *d_tc, *d_cc;
+    uint32_t nd_col, *d_col_off, *d_col_cnt;
+    uint32_t *d_remap;          /* generated index -> sorted index */
+    uint32_t ne_in, *edge_to_constraint; /* input edge -> sorted distance index (0xFFFFFFFF invalid) */
+
+    uint32_t nb;      /* bending constraints, colour-sorted */
+    uint32_t *b_v0, *b_v1, *b_v2, *b_v3;
+    float *b_rest, *b_comp, *b_pad0, *b_pad1;
+    uint32_t nb_col, *b_col_off, *b_col_cnt;
+
+    uint32_t *adj_off, *adj_idx;
+    uint32_t *th_off, *th_idx; float *th_rest;
+    float *local_len;
+    uint32_t *star_off, *star_v0, *star_v1;
+    uint32_t ntri, *tri;
+    uint32_t ne, *e_v0, *e_v1, *e_bbits;  /* Morton-sorted unique edges */
+    uint32_t *island;
+    float mean_edge, max_edge;
+    uint32_t n_virtual;
+    float cell_size;
+    uint32_t table_size;
+    int has_multiple_layers;
+} orc_mesh;
+
+/* Welsh-Powell, coloring.rs:6-79 (distance) and 82-173 (bending).
+ * cverts: K vertex slots per constraint (0xFFFFFFFF = unused slot).
+ * reg: which slots are registered in vertex->constraint lists;
+ * deg/used use the same registered slots (see the bending special cases). */
+typedef struct { uint32_t deg; uint32_t idx; } wp_key;
+static int cmp_wp(const void *a, const void *b) {
+    const wp_key *x = (const wp_key *)a, *y = (const wp_key *)b;
+    if (x->deg != y->deg) return x->deg > y->deg ? -1 : 1; /* Reverse(deg) */
+    return x->idx < y->idx ? -1 : x->idx > y->idx;
+}
+
+static void welsh_powell(uint32_t nv, uint32_t nc,
+                         /* registration lists: for each constraint, verts to register */
+                         const uint32_t *reg_v, const uint32_t *reg_n, uint32_t reg_stride,
+                         /* degree verts */
+                         const uint32_t *deg_v, const uint32_t *deg_n, uint32_t deg_stride,
+                         /* verts whose lists give used colours */
+                         const uint32_t *use_v, const uint32_t *use_n, uint32_t use_stride,
+                         uint32_t *colors_out, uint32_t *ncolors_out) {
+    uint32_t *cnt = (uint32_t *)calloc(nv + 1, 4);
+    for (uint32_t i = 0; i < nc; i++)
+        for (uint32_t s = 0; s < reg_n[i]; s++) cnt[reg_v[i * reg_stride + s]]++;
+    uint32_t *off = (uint32_t *)malloc((nv + 1) * 4);
+    off[0] = 0;
+    for (uint32_t v = 0; v < nv; v++) off[v + 1] = off[v] + cnt[v];
+    uint32_t *lst = (uint32_t *)malloc((size_t)(off[nv] ? off[nv] : 1) * 4);
+    uint32_t *fill = (uint32_t *)calloc(nv + 1, 4);
+    for (uint32_t i = 0; i < nc; i++)
+        for (uint32_t s = 0; s < reg_n[i]; s++) {
+            uint32_t v = reg_v[i * reg_stride + s];
+            lst[off[v] + fill[v]++] = i;
+        }
+    wp_key *ord = (wp_key *)malloc((size_t)(nc ? nc : 1) * sizeof(wp_key));
+    for (uint32_t i = 0; i < nc; i++) {
+        uint32_t d = 0;
+        for (uint32_t s = 0; s < deg_n[i]; s++) d += cnt[deg_v[i * deg_stride + s]];
+        ord[i].deg = d; ord[i].idx = i;
+    }
+    qsort(ord, nc, sizeof(wp_key), cmp_wp);
+    for (uint32_t i = 0; i < nc; i++) colors_out[i] = 0xFFFFFFFFu;
+    uint32_t ncol = 0;
+    size_t stamp_cap = 64;
+    uint32_t *stamp = (uint32_t *)calloc(stamp_cap, 4);
+    for (uint32_t o = 0; o < nc; o++) {
+        uint32_t i = ord[o].idx;
+        for (uint32_t s = 0; s < use_n[i]; s++) {
+            uint32_t v = use_v[i * use_stride + s];
+            for (uint32_t k = off[v]; k < off[v + 1]; k++) {
+                uint32_t a = lst[k];
+                if (a != i && colors_out[a] != 0xFFFFFFFFu) {
+                    uint32_t c = colors_out[a];
+                    if (c >= stamp_cap) {
+                        size_t nc2 = stamp_cap; while (nc2 <= c) nc2 *= 2;
+                        stamp = (uint32_t *)realloc(stamp, nc2 * 4);
+                        memset(stamp + stamp_cap, 0, (nc2 - stamp_cap) * 4);
+                        stamp_cap = nc2;
+                    }
+                    stamp[c] = o + 1;
+                }
+            }
+        }
+        uint32_t c = 0;
+        while (c < stamp_cap && stamp[c] == o + 1) c++;
+        colors_out[i] = c;
+        if (c + 1 > ncol) ncol = c + 1;
+    }
+    *ncolors_out = ncol;
+    free(stamp); free(ord); free(fill); free(lst); free(off); free(cnt);
+}
+
+/* colour-bucket reorder (stable within colour) */
+static void bucket_order(uint32_t nc, const uint32_t *col, uint32_t ncol,
+                         uint32_t *perm_sorted_to_orig, uint32_t *remap_orig_to_sorted,
+                         uint32_t *col_off, uint32_t *col_cnt) {
+    for (uint32_t c = 0; c < ncol; c++) col_cnt[c] = 0;
+    for (uint32_t i = 0; i < nc; i++) col_cnt[col[i]]++;
+    uint32_t acc = 0;
+    for (uint32_t c = 0; c < ncol; c++) { col_off[c] = acc; acc += col_cnt[c]; }
+    uint32_t *fill = (uint32_t *)calloc(ncol ? ncol : 1, 4);
+    for (uint32_t i = 0; i < nc; i++) {
+        uint32_t s = col_off[col[i]] + fill[col[i]]++;
+        perm_sorted_to_orig[s] = i;
+        if (remap_orig_to_sorted) remap_orig_to_sorted[i] = s;
+    }
+    free(fill);
+}
+
+typedef struct { uint64_t key; uint32_t opp; uint32_t seq; } eopp;
+static int cmp_eopp(const void *a, const void *b) {
+    const eopp *x = (const eopp *)a, *y = (const eopp *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->seq < y->seq ? -1 : x->seq > y->seq;
+}
+
+static uint32_t expand_bits_10(uint32_t v) { /* mesh.rs:314-320 */
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+static uint32_t f2u_sat(float x) { /* Rust `as u32` (saturating, NaN -> 0) */
+    if (!(x > 0.0f)) return 0;
+    if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+static uint32_t morton_code_3d(const float *p, const float *bmin, const float *inv) { /* mesh.rs:322-327 */
+    uint32_t x = f2u_sat(fminf(fmaxf((p[0] - bmin[0]) * inv[0], 0.0f), 1.0f) * 1023.0f);
+    uint32_t y = f2u_sat(fminf(fmaxf((p[1] - bmin[1]) * inv[1], 0.0f), 1.0f) * 1023.0f);
+    uint32_t z = f2u_sat(fminf(fmaxf((p[2] - bmin[2]) * inv[2], 0.0f), 1.0f) * 1023.0f);
+    return (expand_bits_10(x) << 2) | (expand_bits_10(y) << 1) | expand_bits_10(z);
+}
+typedef struct { uint32_t code; uint32_t idx; } mkey;
+static int cmp_mkey(const void *a, const void *b) {
+    const mkey *x = (const mkey *)a, *y = (const mkey *)b;
+    if (x->code != y->code) return x->code < y->code ? -1 : 1;
+    return x->idx < y->idx ? -1 : x->idx > y->idx; /* sort_by_cached_key is stable */
+}
+
+static uint32_t next_pow2(uint32_t x) { uint32_t p = 1; while (p < x) p <<= 1; return p; }
+
+/* virtual_mesh.rs:75-200 -- only the number of samples is needed (it sizes the hash table) */
+static uint32_t virtual_sample_count(const orc_mesh *m, const float *pos) {
+    if (m->nv == 0 || m->ntri == 0) return 0;
+    float thickness = m->thick[0];
+    float spacing = thickness * 1.5f;
+    if (spacing <= 1e-6f) return 0;
+    /* edge -> first face containing it */
+    pset seen; pset_init(&seen, (size_t)m->ntri * 3);
+    for (uint32_t f = 0; f < m->ntri; f++) {
+        uint32_t a = m->tri[3 * f], b = m->tri[3 * f + 1], c = m->tri[3 * f + 2];
+        pset_insert(&seen, a < b ? a : b, a < b ? b : a);
+        pset_insert(&seen, b < c ? b : c, b < c ? c : b);
+        pset_insert(&seen, c < a ? c : a, c < a ? a : c);
+    }
+    uint64_t total = 0;
+    const uint64_t CAP = 150000;
+    for (uint32_t k = 0; k < m->nd; k++) {
+        uint32_t u = m->d_v0[k], v = m->d_v1[k];
+        if (!pset_has(&seen, u < v ? u : v, u < v ? v : u)) continue;
+        float len = fdist(pos + 3 * u, pos + 3 * v);
+        uint32_t ke = (uint32_t)ceilf(len / spacing);
+        if (ke > 16) ke = 16;
+        if (ke > 1) total += ke - 1;
+    }
+    for (uint32_t f = 0; f < m->ntri; f++) {
+        const float *pa = pos + 3 * m->tri[3 * f], *pb = pos + 3 * m->tri[3 * f + 1], *pc = pos + 3 * m->tri[3 * f + 2];
+        float lab = fdist(pa, pb), lbc = fdist(pb, pc), lca = fdist(pc, pa);
+        float lmax = fmaxf(fmaxf(lab, lbc), lca);
+        uint32_t k = (uint32_t)ceilf(lmax / spacing);
+        if (k > 16) k = 16;
+        if (k >= 3) total += (uint64_t)(k - 1) * (k - 2) / 2;
+    }
+    pset_free(&seen);
+    return (uint32_t)(total < CAP ? total : CAP);
+}
+
+orc_mesh *orc_mesh_build(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne_in,
+                         const uint32_t *faces, uint32_t nf, const float *inv_masses,
+                         const float *thicknesses, const uint32_t *layer_ids, uint32_t default_layer,
+                         float default_thickness, float tension_k, float compression_k, float shear_k,
+                         float bending_k) {
+    orc_mesh *m = (orc_mesh *)calloc(1, sizeof(orc_mesh));
+    m->nv = nv;
+    m->pos0 = (float *)malloc((size_t)(nv ? nv : 1) * 12);
+    memcpy(m->pos0, pos, (size_t)nv * 12);
+    m->inv_mass = (float *)malloc((size_t)(nv ? nv : 1) * 4);
+    m->thick = (float *)malloc((size_t)(nv ? nv : 1) * 4);
+    m->layer = (uint32_t *)malloc((size_t)(nv ? nv : 1) * 4);
+    for (uint32_t i = 0; i < nv; i++) {
+        m->inv_mass[i] = inv_masses ? inv_masses[i] : 1.0f;
+        m->thick[i] = thicknesses ? thicknesses[i] : default_thickness;
+        m->layer[i] = layer_ids ? layer_ids[i] : default_layer;
+    }
+    float tc = compliance_of(tension_k), cc = compliance_of(compression_k), sc = compliance_of(shear_k);
+
+    /* 1. stretch (mesh.rs:474-499) */
+    vu32 dv0 = {0}, dv1 = {0}, dty = {0};
+    vf32 drest = {0}, dtc = {0}, dcc = {0};
+    uint32_t *raw_edge_to_dist = (uint32_t *)malloc((size_t)(ne_in ? ne_in : 1) * 4);
+    pset original; pset_init(&original, ne_in);
+    for (uint32_t e = 0; e < ne_in; e++) {
+        uint32_t a = edges[2 * e], b = edges[2 * e + 1];
+        if (a < nv && b < nv && a != b) {
+            raw_edge_to_dist[e] = (uint32_t)dv0.n;
+            vu_push(&dv0, a); vu_push(&dv1, b); vu_push(&dty, 0);
+            vf_push(&drest, frest(pos + 3 * a, pos + 3 * b)); vf_push(&dtc, tc); vf_push(&dcc, cc);
+            pset_insert(&original, a < b ? a : b, a < b ? b : a);
+        } else {
+            raw_edge_to_dist[e] = 0xFFFFFFFFu;
+        }
+    }
+
+    /* edge -> opposite vertices (face order), mesh.rs:514-527 */
+    eopp *eo = NULL; size_t neo = 0;
+    if (faces && nf) {
+        eo = (eopp *)malloc((size_t)nf * 3 * sizeof(eopp));
+        uint32_t seq = 0;
+        for (uint32_t f = 0; f < nf; f++) {
+            uint32_t f0 = faces[3 * f], f1 = faces[3 * f + 1], f2 = faces[3 * f + 2];
+            if (f0 >= nv || f1 >= nv || f2 >= nv) continue;
+            uint32_t es[3][3] = {{f0 < f1 ? f0 : f1, f0 < f1 ? f1 : f0, f2},
+                                 {f1 < f2 ? f1 : f2, f1 < f2 ? f2 : f1, f0},
+                                 {f2 < f0 ? f2 : f0, f2 < f0 ? f0 : f2, f1}};
+            for (int k = 0; k < 3; k++) {
+                eo[neo].key = ((uint64_t)es[k][0] << 32) | es[k][1];
+                eo[neo].opp = es[k][2];
+                eo[neo].seq = seq++;
+                neo++;
+            }
+        }
+        qsort(eo, neo, sizeof(eopp), cmp_eopp);
+    }
+
+    /* 2. shear (mesh.rs:508-560) */
+    if (neo) {
+        pset existing; pset_init(&existing, ne_in + neo);
+        for (uint32_t e = 0; e < ne_in; e++) {
+            uint32_t a = edges[2 * e], b = edges[2 * e + 1];
+            if (a < nv && b < nv && a != b) pset_insert(&existing, a < b ? a : b, a < b ? b : a);
+        }
+        for (size_t g = 0; g < neo;) {
+            size_t h = g;
+            while (h < neo && eo[h].key == eo[g].key) h++;
+            uint32_t ka = (uint32_t)(eo[g].key >> 32), kb = (uint32_t)eo[g].key;
+            if (!pset_has(&original, ka, kb) && h - g == 2) {
+                uint32_t o0 = eo[g].opp, o1 = eo[g + 1].opp;
+                uint32_t pairs[2][2] = {{ka, kb}, {o0 < o1 ? o0 : o1, o0 < o1 ? o1 : o0}};
+                for (int k = 0; k < 2; k++) {
+                    uint32_t a = pairs[k][0], b = pairs[k][1];
+                    if (a != b && pset_insert(&existing, a, b)) {
+                        vu_push(&dv0, a); vu_push(&dv1, b); vu_push(&dty, 1);
+                        vf_push(&drest, frest(pos + 3 * a, pos + 3 * b)); vf_push(&dtc, sc); vf_push(&dcc, sc);
+                    }
+                }
+            }
+            g = h;
+        }
+        pset_free(&existing);
+    }
+
+    /* colour distance constraints (coloring.rs:6-79) */
+    uint32_t nd = (uint32_t)dv0.n;
+    m->nd = nd;
+    {
+        uint32_t *vv = (uint32_t *)malloc((size_t)(nd ? nd : 1) * 8);
+        uint32_t *two = (uint32_t *)malloc((size_t)(nd ? nd : 1) * 4);
+        for (uint32_t i = 0; i < nd; i++) { vv[2 * i] = dv0.d[i]; vv[2 * i + 1] = dv1.d[i]; two[i] = 2; }
+        uint32_t *col = (uint32_t *)malloc((size_t)(nd ? nd : 1) * 4);
+        uint32_t ncol = 0;
+        welsh_powell(nv, nd, vv, two, 2, vv, two, 2, vv, two, 2, col, &ncol);
+        m->nd_col = ncol;
+        m->d_col_off = (uint32_t *)malloc((size_t)(ncol ? ncol : 1) * 4);
+        m->d_col_cnt = (uint32_t *)malloc((size_t)(ncol ? ncol : 1) * 4);
+        uint32_t *perm = (uint32_t *)malloc((size_t)(nd ? nd : 1) * 4);
+        m->d_remap = (uint32_t *)malloc((size_t)(nd ? nd : 1) * 4);
+        bucket_order(nd, col, ncol, perm, m->d_remap, m->d_col_off, m->d_col_cnt);
+        m->d_v0 = (uint32_t *)malloc((size_t)(nd ? nd : 1) * 4);
+        m->d_v1 = (uint32_t *)malloc((size_t)(nd ? nd : 1) * 4);
+        m->d_type = (uint32_t *)malloc((size_t)(nd ? nd : 1) * 4);
+        m->d_rest = (float *)malloc((size_t)(nd ? nd : 1) * 4);
+        m->d_tc = (float *)malloc((size_t)(nd ? nd : 1) * 4);
+        m->d_cc = (float *)malloc((size_t)(nd ? nd : 1) * 4);
+        for (uint32_t s = 0; s < nd; s++) {
+            uint32_t o = perm[s];
+            m->d_v0[s] = dv0.d[o]; m->d_v1[s] = dv1.d[o]; m->d_type[s] = dty.d[o];
+            m->d_rest[s] = drest.d[o]; m->d_tc[s] = dtc.d[o]; m->d_cc[s] = dcc.d[o];
+        }
+        free(perm); free(col); free(two); free(vv);
+    }
+    m->ne_in = ne_in;
+    m->edge_to_constraint = (uint32_t *)malloc((size_t)(ne_in ? ne_in : 1) * 4);
+    for (uint32_t e = 0; e < ne_in; e++)
+        m->edge_to_constraint[e] = raw_edge_to_dist[e] == 0xFFFFFFFFu ? 0xFFFFFFFFu : m->d_remap[raw_edge_to_dist[e]];
+    free(raw_edge_to_dist);
+
+    /* 3. bending (mesh.rs:648-705), strand bending (mesh.rs:833-892) */
+    vu32 bv0 = {0}, bv1 = {0}, bv2 = {0}, bv3 = {0};
+    vf32 brest = {0}, bcomp = {0}, bpad0 = {0}, bpad1 = {0};
+    float bend_c = bending_k > 0.0f ? 1.0f / bending_k : 1e10f;
+    for (size_t g = 0; g < neo;) {
+        size_t h = g;
+        while (h < neo && eo[h].key == eo[g].key) h++;
+        uint32_t ka = (uint32_t)(eo[g].key >> 32), kb = (uint32_t)eo[g].key;
+        if (pset_has(&original, ka, kb) && h - g == 2 && eo[g].opp != eo[g + 1].opp) {
+            uint32_t v2 = eo[g].opp, v3 = eo[g + 1].opp;
+            vu_push(&bv0, ka); vu_push(&bv1, kb); vu_push(&bv2, v2); vu_push(&bv3, v3);
+            vf_push(&brest, frest(pos + 3 * v2, pos + 3 * v3)); vf_push(&bcomp, bend_c);
+            vf_push(&bpad0, 0.0f); vf_push(&bpad1, 0.0f);
+        }
+        g = h;
+    }
+    {
+        pset face_edges; pset_init(&face_edges, (size_t)nf * 3 + 1);
+        if (faces) for (uint32_t f = 0; f < nf; f++) {
+            uint32_t f0 = faces[3 * f], f1 = faces[3 * f + 1], f2 = faces[3 * f + 2];
+            pset_insert(&face_edges, f0 < f1 ? f0 : f1, f0 < f1 ? f1 : f0);
+            pset_insert(&face_edges, f1 < f2 ? f1 : f2, f1 < f2 ? f2 : f1);
+            pset_insert(&face_edges, f2 < f0 ? f2 : f0, f2 < f0 ? f0 : f2);
+        }
+        /* strand adjacency with insertion order */
+        uint32_t *scnt = (uint32_t *)calloc(nv + 1, 4);
+        for (uint32_t e = 0; e < ne_in; e++) {
+            uint32_t a = edges[2 * e], b = edges[2 * e + 1];
+            if (a < nv && b < nv && a != b && !pset_has(&face_edges, a < b ? a : b, a < b ? b : a)) { scnt[a]++; scnt[b]++; }
+        }
+        uint32_t *soff = (uint32_t *)malloc((nv + 1) * 4);
+        soff[0] = 0;
+        for (uint32_t v = 0; v < nv; v++) soff[v + 1] = soff[v] + scnt[v];
+        uint32_t *sl = (uint32_t *)malloc((size_t)(soff[nv] ? soff[nv] : 1) * 4);
+        uint32_t *sf = (uint32_t *)calloc(nv + 1, 4);
+        for (uint32_t e = 0; e < ne_in; e++) {
+            uint32_t a = edges[2 * e], b = edges[2 * e + 1];
+            if (a < nv && b < nv && a != b && !pset_has(&face_edges, a < b ? a : b, a < b ? b : a)) {
+                sl[soff[a] + sf[a]++] = b;
+                sl[soff[b] + sf[b]++] = a;
+            }
+        }
+        if (soff[nv] > 0) {
+            pset bpairs; pset_init(&bpairs, bv0.n + soff[nv]);
+            for (size_t i = 0; i < bv0.n; i++) {
+                uint32_t a = bv2.d[i], b = bv3.d[i];
+                pset_insert(&bpairs, a < b ? a : b, a < b ? b : a);
+            }
+            float strand_c = fmaxf(bend_c * 10.0f, 0.5f);
+            for (uint32_t u = 0; u < nv; u++) {
+                uint32_t n = scnt[u];
+                if (n < 2) continue;
+                for (uint32_t i = 0; i < n; i++)
+                    for (uint32_t j = i + 1; j < n; j++) {
+                        uint32_t a = sl[soff[u] + i], b = sl[soff[u] + j];
+                        if (a != b && pset_insert(&bpairs, a < b ? a : b, a < b ? b : a)) {
+                            vu_push(&bv0, u); vu_push(&bv1, u); vu_push(&bv2, a); vu_push(&bv3, b);
+                            vf_push(&brest, frest(pos + 3 * a, pos + 3 * b)); vf_push(&bcomp, strand_c);
+                            vf_push(&bpad0, 2.0f); vf_push(&bpad1, 0.0f);
+                        }
+                    }
+            }
+            pset_free(&bpairs);
+        }
+        free(sf); free(sl); free(soff); free(scnt);
+        pset_free(&face_edges);
+    }
+    /* colour bending (coloring.rs:82-173) */
+    uint32_t nb = (uint32_t)bv0.n;
+    m->nb = nb;
+    {
+        uint32_t *reg = (uint32_t *)malloc((size_t)(nb ? nb : 1) * 16);
+        uint32_t *regn = (uint32_t *)malloc((size_t)(nb ? nb : 1) * 4);
+        uint32_t *dgv = (uint32_t *)malloc((size_t)(nb ? nb : 1) * 16);
+        uint32_t *dgn = (uint32_t *)malloc((size_t)(nb ? nb : 1) * 4);
+        uint32_t *usv = (uint32_t *)malloc((size_t)(nb ? nb : 1) * 16);
+        uint32_t *usn = (uint32_t *)malloc((size_t)(nb ? nb : 1) * 4);
+        for (uint32_t i = 0; i < nb; i++) {
+            uint32_t v0 = bv0.d[i], v1 = bv1.d[i], v2 = bv2.d[i], v3 = bv3.d[i];
+            int seam = bpad0.d[i] > 0.5f;
+            uint32_t r = 0;
+            reg[4 * i + r++] = v2; reg[4 * i + r++] = v3;
+            if (seam) {
+                if (v0 < nv && v0 != v2 && v0 != v3) reg[4 * i + r++] = v0;
+                if (v1 < nv && v1 != v2 && v1 != v3 && v1 != v0) reg[4 * i + r++] = v1;
+            }
+            regn[i] = r;
+            uint32_t d = 0;
+            dgv[4 * i + d++] = v2; dgv[4 * i + d++] = v3;
+            if (seam) {
+                if (v0 < nv) dgv[4 * i + d++] = v0;
+                if (v1 < nv) dgv[4 * i + d++] = v1;
+            }
+            dgn[i] = d;
+            uint32_t u = 0;
+            if (seam) {
+                if (v0 < nv) usv[4 * i + u++] = v0;
+                if (v1 < nv) usv[4 * i + u++] = v1;
+            }
+            if (v2 < nv) usv[4 * i + u++] = v2;
+            if (v3 < nv) usv[4 * i + u++] = v3;
+            usn[i] = u;
+        }
+        uint32_t *col = (uint32_t *)malloc((size_t)(nb ? nb : 1) * 4);
+        uint32_t ncol = 0;
+        welsh_powell(nv, nb, reg, regn, 4, dgv, dgn, 4, usv, usn, 4, col, &ncol);
+        m->nb_col = ncol;
+        m->b_col_off = (uint32_t *)malloc((size_t)(ncol ? ncol : 1) * 4);
+        m->b_col_cnt = (uint32_t *)malloc((size_t)(ncol ? ncol : 1) * 4);
+        uint32_t *perm = (uint32_t *)malloc((size_t)(nb ? nb : 1) * 4);
+        bucket_order(nb, col, ncol, perm, NULL, m->b_col_off, m->b_col_cnt);
+#define BALLOC(p, T) p = (T *)malloc((size_t)(nb ? nb : 1) * 4)
+        BALLOC(m->b_v0, uint32_t); BALLOC(m->b_v1, uint32_t); BALLOC(m->b_v2, uint32_t); BALLOC(m->b_v3, uint32_t);
+        BALLOC(m->b_rest, float); BALLOC(m->b_comp, float); BALLOC(m->b_pad0, float); BALLOC(m->b_pad1, float);
+#undef BALLOC
+        for (uint32_t s = 0; s < nb; s++) {
+            uint32_t o = perm[s];
+            m->b_v0[s] = bv0.d[o]; m->b_v1[s] = bv1.d[o]; m->b_v2[s] = bv2.d[o]; m->b_v3[s] = bv3.d[o];
+            m->b_rest[s] = brest.d[o]; m->b_comp[s] = bcomp.d[o]; m->b_pad0[s] = bpad0.d[o]; m->b_pad1[s] = bpad1.d[o];
+        }
+        free(perm); free(col); free(reg); free(regn); free(dgv); free(dgn); free(usv); free(usn);
+    }
+
+    /* 4. adjacency + local edge lengths (mesh.rs:949-1075) */
+    uint32_t *acnt = (uint32_t *)calloc(nv + 1, 4);
+    float *lsum = (float *)calloc(nv + 1, 4);
+    uint32_t *lcnt = (uint32_t *)calloc(nv + 1, 4);
+    float total_len = 0.0f, max_len = 0.0f;
+    uint32_t valid_edges = 0;
+    for (uint32_t e = 0; e < ne_in; e++) {
+        uint32_t a = edges[2 * e], b = edges[2 * e + 1];
+        if (a < nv && b < nv && a != b) {
+            float len = frest(pos + 3 * a, pos + 3 * b);
+            total_len += len;
+            if (len > max_len) max_len = len;
+            valid_edges++;
+            acnt[a]++; acnt[b]++;
+            lsum[a] += len; lcnt[a]++;
+            lsum[b] += len; lcnt[b]++;
+        }
+    }
+    m->mean_edge = valid_edges ? total_len / (float)valid_edges : 0.0f;
+    m->max_edge = max_len;
+    m->local_len = (float *)malloc((size_t)(nv ? nv : 1) * 4);
+    for (uint32_t i = 0; i < nv; i++)
+        m->local_len[i] = lcnt[i] > 0 ? lsum[i] / (float)lcnt[i] : fmaxf(default_thickness, 0.01f);
+    uint32_t *aoff_raw = (uint32_t *)malloc((nv + 1) * 4);
+    aoff_raw[0] = 0;
+    for (uint32_t v = 0; v < nv; v++) aoff_raw[v + 1] = aoff_raw[v] + acnt[v];
+    uint32_t *araw = (uint32_t *)malloc((size_t)(aoff_raw[nv] ? aoff_raw[nv] : 1) * 4);
+    uint32_t *afill = (uint32_t *)calloc(nv + 1, 4);
+    for (uint32_t e = 0; e < ne_in; e++) {
+        uint32_t a = edges[2 * e], b = edges[2 * e + 1];
+        if (a < nv && b < nv && a != b) { araw[aoff_raw[a] + afill[a]++] = b; araw[aoff_raw[b] + afill[b]++] = a; }
+    }
+    m->adj_off = (uint32_t *)malloc((nv + 1) * 4);
+    vu32 aidx = {0};
+    for (uint32_t v = 0; v < nv; v++) {
+        m->adj_off[v] = (uint32_t)aidx.n;
+        size_t n = sort_unique(araw + aoff_raw[v], acnt[v]);
+        for (size_t k = 0; k < n; k++) vu_push(&aidx, araw[aoff_raw[v] + k]);
+        acnt[v] = (uint32_t)n; /* now unique count */
+    }
+    m->adj_off[nv] = (uint32_t)aidx.n;
+    m->adj_idx = aidx.d ? aidx.d : (uint32_t *)calloc(1, 4);
+
+    /* two-hop CSR (mesh.rs:1087-1191); without sewing the contracted graph is the mesh graph */
+    uint8_t *is_face = (uint8_t *)calloc(nv + 1, 1);
+    if (faces) for (uint32_t f = 0; f < nf; f++)
+        for (int k = 0; k < 3; k++) if (faces[3 * f + k] < nv) is_face[faces[3 * f + k]] = 1;
+    m->th_off = (uint32_t *)malloc((nv + 1) * 4);
+    vu32 thi = {0}; vf32 thr = {0};
+    vu32 nb_tmp = {0};
+    for (uint32_t i = 0; i < nv; i++) {
+        nb_tmp.n = 0;
+        vu_push(&nb_tmp, i);
+        for (uint32_t k = m->adj_off[i]; k < m->adj_off[i + 1]; k++) {
+            uint32_t u = m->adj_idx[k];
+            vu_push(&nb_tmp, u);
+            for (uint32_t ku = m->adj_off[u]; ku < m->adj_off[u + 1]; ku++) vu_push(&nb_tmp, m->adj_idx[ku]);
+        }
+        size_t n = sort_unique(nb_tmp.d, nb_tmp.n);
+        m->th_off[i] = (uint32_t)thi.n;
+        const uint32_t *ai = m->adj_idx + m->adj_off[i];
+        size_t an = m->adj_off[i + 1] - m->adj_off[i];
+        for (size_t k = 0; k < n; k++) {
+            uint32_t nbr = nb_tmp.d[k];
+            vu_push(&thi, nbr);
+            int true2 = nbr != i && is_face[i] && is_face[nbr] && !bsearch_u32(ai, an, nbr);
+            vf_push(&thr, true2 ? frest(pos + 3 * i, pos + 3 * nbr) : 0.0f);
+        }
+    }
+    m->th_off[nv] = (uint32_t)thi.n;
+    m->th_idx = thi.d ? thi.d : (uint32_t *)calloc(1, 4);
+    m->th_rest = thr.d ? thr.d : (float *)calloc(1, 4);
+    free(nb_tmp.d); free(is_face);
+
+    /* star lists (mesh.rs:1193-1212), triangles */
+    uint32_t *scount = (uint32_t *)calloc(nv + 1, 4);
+    uint32_t ntri = 0;
+    if (faces) for (uint32_t f = 0; f < nf; f++) {
+        uint32_t f0 = faces[3 * f], f1 = faces[3 * f + 1], f2 = faces[3 * f + 2];
+        if (f0 < nv && f1 < nv && f2 < nv) { scount[f0]++; scount[f1]++; scount[f2]++; ntri++; }
+    }
+    m->star_off = (uint32_t *)malloc((nv + 1) * 4);
+    m->star_off[0] = 0;
+    for (uint32_t v = 0; v < nv; v++) m->star_off[v + 1] = m->star_off[v] + scount[v];
+    uint32_t ns = m->star_off[nv];
+    m->star_v0 = (uint32_t *)malloc((size_t)(ns ? ns : 1) * 4);
+    m->star_v1 = (uint32_t *)malloc((size_t)(ns ? ns : 1) * 4);
+    m->ntri = ntri;
+    m->tri = (uint32_t *)malloc((size_t)(ntri ? ntri : 1) * 12);
+    memset(scount, 0, (nv + 1) * 4);
+    ntri = 0;
+    if (faces) for (uint32_t f = 0; f < nf; f++) {
+        uint32_t f0 = faces[3 * f], f1 = faces[3 * f + 1], f2 = faces[3 * f + 2];
+        if (f0 < nv && f1 < nv && f2 < nv) {
+            uint32_t s;
+            s = m->star_off[f0] + scount[f0]++; m->star_v0[s] = f1; m->star_v1[s] = f2;
+            s = m->star_off[f1] + scount[f1]++; m->star_v0[s] = f2; m->star_v1[s] = f0;
+            s = m->star_off[f2] + scount[f2]++; m->star_v0[s] = f0; m->star_v1[s] = f1;
+            m->tri[3 * ntri] = f0; m->tri[3 * ntri + 1] = f1; m->tri[3 * ntri + 2] = f2; ntri++;
+        }
+    }
+    free(scount);
+
+    /* islands (mesh.rs:1228-1255) */
+    m->island = (uint32_t *)malloc((size_t)(nv ? nv : 1) * 4);
+    for (uint32_t i = 0; i < nv; i++) m->island[i] = 0xFFFFFFFFu;
+    uint32_t *queue = (uint32_t *)malloc((size_t)(nv ? nv : 1) * 4);
+    uint32_t cur_island = 0;
+    for (uint32_t i = 0; i < nv; i++) {
+        if (m->island[i] != 0xFFFFFFFFu) continue;
+        size_t qh = 0, qt = 0;
+        m->island[i] = cur_island; queue[qt++] = i;
+        while (qh < qt) {
+            uint32_t u = queue[qh++];
+            for (uint32_t k = m->adj_off[u]; k < m->adj_off[u + 1]; k++) {
+                uint32_t v = m->adj_idx[k];
+                if (v < nv && m->island[v] == 0xFFFFFFFFu) { m->island[v] = cur_island; queue[qt++] = v; }
+            }
+        }
+        cur_island++;
+    }
+    free(queue);
+
+    /* Morton-sorted unique edges (mesh.rs:1257-1289) + boundary bits (1291-1344) */
+    {
+        pset es; pset_init(&es, ne_in);
+        vu32 e0 = {0}, e1 = {0};
+        for (uint32_t e = 0; e < ne_in; e++) {
+            uint32_t a = edges[2 * e], b = edges[2 * e + 1];
+            if (a < nv && b < nv && a != b) {
+                uint32_t x = a < b ? a : b, y = a < b ? b : a;
+                if (pset_insert(&es, x, y)) { vu_push(&e0, x); vu_push(&e1, y); }
+            }
+        }
+        pset_free(&es);
+        uint32_t ne = (uint32_t)e0.n;
+        m->ne = ne;
+        m->e_v0 = (uint32_t *)malloc((size_t)(ne ? ne : 1) * 4);
+        m->e_v1 = (uint32_t *)malloc((size_t)(ne ? ne : 1) * 4);
+        if (ne && nv) {
+            float bmin[3] = {3.4028235e38f, 3.4028235e38f, 3.4028235e38f};
+            float bmax[3] = {-3.4028235e38f, -3.4028235e38f, -3.4028235e38f};
+            for (uint32_t i = 0; i < nv; i++)
+                for (int a = 0; a < 3; a++) { bmin[a] = fminf(bmin[a], pos[3 * i + a]); bmax[a] = fmaxf(bmax[a], pos[3 * i + a]); }
+            float inv[3];
+            for (int a = 0; a < 3; a++) inv[a] = 1.0f / fmaxf(bmax[a] - bmin[a], 1e-5f);
+            mkey *mk_ = (mkey *)malloc(ne * sizeof(mkey));
+            for (uint32_t k = 0; k < ne; k++) {
+                const float *p0 = pos + 3 * e0.d[k], *p1 = pos + 3 * e1.d[k];
+                float mid[3] = {(p0[0] + p1[0]) * 0.5f, (p0[1] + p1[1]) * 0.5f, (p0[2] + p1[2]) * 0.5f};
+                mk_[k].code = morton_code_3d(mid, bmin, inv); mk_[k].idx = k;
+            }
+            qsort(mk_, ne, sizeof(mkey), cmp_mkey);
+            for (uint32_t k = 0; k < ne; k++) { m->e_v0[k] = e0.d[mk_[k].idx]; m->e_v1[k] = e1.d[mk_[k].idx]; }
+            free(mk_);
+        }
+        free(e0.d); free(e1.d);
+        uint32_t words = (ne + 31) / 32;
+        m->e_bbits = (uint32_t *)calloc(words ? words : 1, 4);
+        if (faces && nf) {
+            /* triangle count per edge key; sorted key array + counting via eo-like list */
+            uint64_t *keys = (uint64_t *)malloc((size_t)nf * 3 * 8);
+            size_t nk = 0;
+            for (uint32_t f = 0; f < nf; f++) {
+                uint32_t f0 = faces[3 * f], f1 = faces[3 * f + 1], f2 = faces[3 * f + 2];
+                uint32_t te[3][2] = {{f0 < f1 ? f0 : f1, f0 < f1 ? f1 : f0}, {f1 < f2 ? f1 : f2, f1 < f2 ? f2 : f1},
+                                     {f2 < f0 ? f2 : f0, f2 < f0 ? f0 : f2}};
+                for (int k = 0; k < 3; k++)
+                    if (te[k][0] < nv && te[k][1] < nv && te[k][0] != te[k][1])
+                        keys[nk++] = ((uint64_t)te[k][0] << 32) | te[k][1];
+            }
+            qsort(keys, nk, 8, cmpu64);
+            for (uint32_t k = 0; k < ne; k++) {
+                uint64_t key = ((uint64_t)m->e_v0[k] << 32) | m->e_v1[k];
+                size_t lo = 0, hi = nk;
+                while (lo < hi) { size_t mid = lo + (hi - lo) / 2; if (keys[mid] < key) lo = mid + 1; else hi = mid; }
+                size_t c = 0;
+                while (lo + c < nk && keys[lo + c] == key) c++;
+                if (c == 0 || c == 1) m->e_bbits[k >> 5] |= 1u << (k & 31);
+            }
+            free(keys);
+        } else {
+            for (uint32_t k = 0; k < ne; k++) m->e_bbits[k >> 5] |= 1u << (k & 31);
+        }
+    }
+
+    /* island layer assignment (pipelines.rs:131-166) */
+    if (nv) {
+        uint32_t first = m->layer[0];
+        int all_same = 1;
+        for (uint32_t i = 0; i < nv; i++) if (m->layer[i] != first) { all_same = 0; break; }
+        uint32_t max_island = 0;
+        for (uint32_t i = 0; i < nv; i++) if (m->island[i] > max_island) max_island = m->island[i];
+        if (all_same && max_island > 0) {
+            float *zs = (float *)calloc(max_island + 1, 4);
+            uint32_t *zc = (uint32_t *)calloc(max_island + 1, 4);
+            for (uint32_t i = 0; i < nv; i++) { zs[m->island[i]] += pos[3 * i + 2]; zc[m->island[i]]++; }
+            float *mz = (float *)malloc((max_island + 1) * 4);
+            uint32_t *ord = (uint32_t *)malloc((max_island + 1) * 4);
+            for (uint32_t k = 0; k <= max_island; k++) { mz[k] = zs[k] / (float)(zc[k] ? zc[k] : 1); ord[k] = k; }
+            for (uint32_t a = 1; a <= max_island; a++) { /* stable insertion sort by mean z */
+                uint32_t x = ord[a]; uint32_t b = a;
+                while (b > 0 && mz[ord[b - 1]] > mz[x]) { ord[b] = ord[b - 1]; b--; }
+                ord[b] = x;
+            }
+            uint32_t *to_layer = (uint32_t *)malloc((max_island + 1) * 4);
+            for (uint32_t r = 0; r <= max_island; r++) to_layer[ord[r]] = r;
+            for (uint32_t i = 0; i < nv; i++) m->layer[i] = to_layer[m->island[i]];
+            free(zs); free(zc); free(mz); free(ord); free(to_layer);
+        }
+        m->has_multiple_layers = 0;
+        for (uint32_t i = 0; i < nv; i++) if (m->layer[i] != m->layer[0]) { m->has_multiple_layers = 1; break; }
+    }
+
+    /* hash parameters (pipelines.rs:899-912) */
+    m->n_virtual = virtual_sample_count(m, pos);
+    float th = nv ? m->thick[0] : 0.005f;
+    float edge_based = fminf(fmaxf(m->max_edge * 0.85f, m->mean_edge * 1.0f), 0.025f);
+    m->cell_size = fmaxf(fmaxf(th * 4.0f, edge_based), 0.01f);
+    uint32_t cap = nv + m->n_virtual;
+    uint32_t ts = next_pow2(cap * 4u);
+    if (ts < 1024) ts = 1024;
+    if (ts > 65536) ts = 65536;
+    m->table_size = ts;
+
+    free(aoff_raw); free(araw); free(afill); free(acnt); free(lsum); free(lcnt);
+    free(eo);
+    pset_free(&original);
+    free(dv0.d); free(dv1.d); free(dty.d); free(drest.d); free(dtc.d); free(dcc.d);
+    free(bv0.d); free(bv1.d); free(bv2.d); free(bv3.d); free(brest.d); free(bcomp.d); free(bpad0.d); free(bpad1.d);
+    return m;
+}
+
+void orc_mesh_free(orc_mesh *m) {
+    if (!m) return;
+    void *ps[] = {m->pos0, m->inv_mass, m->thick, m->layer, m->d_v0, m->d_v1, m->d_type, m->d_rest, m->d_tc, m->d_cc,
+                  m->d_col_off, m->d_col_cnt, m->d_remap, m->edge_to_constraint, m->b_v0, m->b_v1, m->b_v2, m->b_v3,
+                  m->b_rest, m->b_comp, m->b_pad0, m->b_pad1, m->b_col_off, m->b_col_cnt, m->adj_off, m->adj_idx,
+                  m->th_off, m->th_idx, m->th_rest, m->local_len, m->star_off, m->star_v0, m->star_v1, m->tri,
+                  m->e_v0, m->e_v1, m->e_bbits, m->island};
+    for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
+    free(m);
+}
+
+/* mesh accessors for tests: what = field id, returns element count; out may be NULL */
+enum {
+    OM_DIST_V0 = 0, OM_DIST_V1, OM_DIST_TYPE, OM_DIST_REST, OM_DIST_TC, OM_DIST_CC, OM_DIST_COL_OFF, OM_DIST_COL_CNT,
+    OM_BEND_V0, OM_BEND_V1, OM_BEND_V2, OM_BEND_V3, OM_BEND_REST, OM_BEND_COMP, OM_BEND_PAD0, OM_BEND_COL_OFF,
+    OM_BEND_COL_CNT, OM_ADJ_OFF, OM_ADJ_IDX, OM_TH_OFF, OM_TH_IDX, OM_TH_REST, OM_LOCAL_LEN, OM_STAR_OFF,
+    OM_STAR_V0, OM_STAR_V1, OM_EDGE_V0, OM_EDGE_V1, OM_EDGE_BBITS, OM_ISLAND, OM_SCALARS, OM_EDGE_TO_CONSTRAINT,
+    OM_LAYER
+};
+uint32_t orc_mesh_get(const orc_mesh *m, int what, void *out) {
+#define CP(ptr, n) do { if (out && (n)) memcpy(out, ptr, (size_t)(n) * 4); return (uint32_t)(n); } while (0)
+    switch (what) {
+    case OM_DIST_V0: CP(m->d_v0, m->nd);
+    case OM_DIST_V1: CP(m->d_v1, m->nd);
+    case OM_DIST_TYPE: CP(m->d_type, m->nd);
+    case OM_DIST_REST: CP(m->d_rest, m->nd);
+    case OM_DIST_TC: CP(m->d_tc, m->nd);
+    case OM_DIST_CC: CP(m->d_cc, m->nd);
+    case OM_DIST_COL_OFF: CP(m->d_col_off, m->nd_col);
+    case OM_DIST_COL_CNT: CP(m->d_col_cnt, m->nd_col);
+    case OM_BEND_V0: CP(m->b_v0, m->nb);
+    case OM_BEND_V1: CP(m->b_v1, m->nb);
+    case OM_BEND_V2: CP(m->b_v2, m->nb);
+    case OM_BEND_V3: CP(m->b_v3, m->nb);
+    case OM_BEND_REST: CP(m->b_rest, m->nb);
+    case OM_BEND_COMP: CP(m->b_comp, m->nb);
+    case OM_BEND_PAD0: CP(m->b_pad0, m->nb);
+    case OM_BEND_COL_OFF: CP(m->b_col_off, m->nb_col);
+    case OM_BEND_COL_CNT: CP(m->b_col_cnt, m->nb_col);
+    case OM_ADJ_OFF: CP(m->adj_off, m->nv + 1);
+    case OM_ADJ_IDX: CP(m->adj_idx, m->adj_off[m->nv]);
+    case OM_TH_OFF: CP(m->th_off, m->nv + 1);
+    case OM_TH_IDX: CP(m->th_idx, m->th_off[m->nv]);
+    case OM_TH_REST: CP(m->th_rest, m->th_off[m->nv]);
+    case OM_LOCAL_LEN: CP(m->local_len, m->nv);
+    case OM_STAR_OFF: CP(m->star_off, m->nv + 1);
+    case OM_STAR_V0: CP(m->star_v0, m->star_off[m->nv]);
+    case OM_STAR_V1: CP(m->star_v1, m->star_off[m->nv]);
+    case OM_EDGE_V0: CP(m->e_v0, m->ne);
+    case OM_EDGE_V1: CP(m->e_v1, m->ne);
+    case OM_EDGE_BBITS: CP(m->e_bbits, (m->ne + 31) / 32);
+    case OM_ISLAND: CP(m->island, m->nv);
+    case OM_EDGE_TO_CONSTRAINT: CP(m->edge_to_constraint, m->ne_in);
+    case OM_LAYER: CP(m->layer, m->nv);
+    case OM_SCALARS: {
+        /* [nv, nd, nb, nd_col, nb_col, ne, ntri, n_virtual, table_size, has_multi, bits(cell), bits(mean), bits(max)] */
+        uint32_t s[13];
+        s[0] = m->nv; s[1] = m->nd; s[2] = m->nb; s[3] = m->nd_col; s[4] = m->nb_col; s[5] = m->ne; s[6] = m->ntri;
+        s[7] = m->n_virtual; s[8] = m->table_size; s[9] = (uint32_t)m->has_multiple_layers;
+        memcpy(&s[10], &m->cell_size, 4); memcpy(&s[11], &m->mean_edge, 4); memcpy(&s[12], &m->max_edge, 4);
+        CP(s, 13);
+    }
+    default: return 0;
+    }
+#undef CP
+}
+
+/* mesh.rs:237-312 (areal part; no strands, no pins) */
+void orc_areal_inv_masses(const float *pos, uint32_t nv, const uint32_t *faces, uint32_t nf, float density,
+                          const float *pin_weights, float *out) {
+    float *mass = (float *)calloc(nv ? nv : 1, 4);
+    if (density > 0.0f)
+        for (uint32_t f = 0; f < nf; f++) {
+            uint32_t a = faces[3 * f], b = faces[3 * f + 1], c = faces[3 * f + 2];
+            if (a >= nv || b >= nv || c >= nv || a == b || b == c || c == a) continue;
+            const float *pa = pos + 3 * a, *pb = pos + 3 * b, *pc = pos + 3 * c;
+            float ab[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+            float ac[3] = {pc[0] - pa[0], pc[1] - pa[1], pc[2] - pa[2]};
+            float cr[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+            float area = 0.5f * sqrtf(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+            float share = area * density / 3.0f;
+            mass[a] += share; mass[b] += share; mass[c] += share;
+        }
+    for (uint32_t i = 0; i < nv; i++) {
+        float w = pin_weights ? fminf(fmaxf(pin_weights[i], 0.0f), 1.0f) : 0.0f;
+        out[i] = w >= 1.0f ? 0.0f : (1.0f - w) / fmaxf(mass[i], 1e-9f);
+    }
+    free(mass);
+}
+
+/* coloring.rs:180-... generic edge-pair colouring (returns colour per pair) */
+void orc_color_edge_pairs(uint32_t nv, const uint32_t *pairs, uint32_t np, uint32_t *colors_out) {
+    uint32_t *reg = (uint32_t *)calloc((size_t)(np ? np : 1) * 2, 4);
+    uint32_t *regn = (uint32_t *)calloc((size_t)(np ? np : 1), 4);
+    for (uint32_t i = 0; i < np; i++) {
+        uint32_t a = pairs[2 * i], b = pairs[2 * i + 1];
+        if (a < nv && b < nv) { reg[2 * i] = a; reg[2 * i + 1] = b; regn[i] = 2; }
+        else regn[i] = 0;
+    }
+    uint32_t ncol = 0;
+    welsh_powell(nv, np, reg, regn, 2, reg, regn, 2, reg, regn, 2, colors_out, &ncol);
+    for (uint32_t i = 0; i < np; i++) if (regn[i] == 0) colors_out[i] = 0;
+    free(reg); free(regn);
+}
+
+/* ------------------------------------------------------------------ */
+/* spatial hash (spatial_grid_sort.wgsl:25-195)                         */
+/* ------------------------------------------------------------------ */
+static inline int32_t f2i_sat(real x) { /* WGSL i32(f32): truncation, saturating */
+    if (x != x) return 0;
+    if (x >= (real)2147483520.0) return 2147483647;
+    if (x <= (real)-2147483648.0) return (int32_t)0x80000000u;
+    return (int32_t)x;
+}
+static inline void cell_of(v3 p, real cell, int32_t *c) {
+    c[0] = f2i_sat(RFLOOR(p.x / cell));
+    c[1] = f2i_sat(RFLOOR(p.y / cell));
+    c[2] = f2i_sat(RFLOOR(p.z / cell));
+}
+static inline uint32_t hash_coords(const int32_t *c, uint32_t table) {
+    uint32_t n = ((uint32_t)c[0] * 73856093u) ^ ((uint32_t)c[1] * 19349663u) ^ ((uint32_t)c[2] * 83492791u);
+    return n % table;
+}
+
+/* Counting sort; within a bucket elements are in ascending index order. */
+static void hash_build(const v3 *pts, uint32_t n, real cell, uint32_t table, uint32_t *keys, uint32_t *starts,
+                       uint32_t *sorted) {
+    memset(starts, 0, (size_t)(table + 1) * 4);
+    for (uint32_t i = 0; i < n; i++) {
+        int32_t c[3]; cell_of(pts[i], cell, c);
+        keys[i] = hash_coords(c, table);
+        starts[keys[i]]++;
+    }
+    uint32_t acc = 0;
+    for (uint32_t h = 0; h < table; h++) { uint32_t c = starts[h]; starts[h] = acc; acc += c; }
+    starts[table] = acc;
+    uint32_t *fill = (uint32_t *)calloc(table, 4);
+    for (uint32_t i = 0; i < n; i++) sorted[starts[keys[i]] + fill[keys[i]]++] = i;
+    free(fill);
+}
+
+/* standalone: keys/starts/sorted for a point cloud (test entry) */
+void orc_hash_points(const float *pts3, uint32_t n, float cell, uint32_t table, uint32_t *keys, uint32_t *starts,
+                     uint32_t *sorted) {
+    v3 *p = (v3 *)malloc((size_t)(n ? n : 1) * sizeof(v3));
+    for (uint32_t i = 0; i < n; i++) p[i] = mk((real)pts3[3 * i], (real)pts3[3 * i + 1], (real)pts3[3 * i + 2]);
+    hash_build(p, n, (real)cell, table, keys, starts, sorted);
+    free(p);
+}
+
+/* ------------------------------------------------------------------ */
+/* simulator                                                           */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    uint32_t type; real friction, restitution; v3 a; real radius; v3 b;
+} collider;
+
+typedef struct {
+    int32_t dx, dy, dz; uint32_t count;
+    int32_t pdx, pdy, pdz; uint32_t pcount;
+    int32_t cpx, cpy, cpz, cnx, cny, cnz;
+    uint32_t untangle, ee_count;
+} accum_t;
+
+typedef struct {
+    orc_mesh *m;
+    uint32_t nv;
+    v3 *x, *p, *v;
+    real *w;
+    v3 *x0;             /* initial positions (reset) */
+    real *w0;
+    real *lam_d, *lam_b;
+    /* per-constraint compliances in the working type (raw overrides possible) */
+    real *d_tc, *d_cc, *b_comp;
+    real gravity[3];
+    real damping;
+    real tension_damping, compression_damping, shear_damping, bending_damping;
+    uint32_t iterations;
+    int self_collision;
+    collider *cols; uint32_t ncols;
+    real cell_size; uint32_t table_size;
+    accum_t *acc;
+    uint32_t *hkeys, *hstarts, *hsorted;     /* vertex hash */
+    uint32_t *ekeys, *estarts, *esorted;     /* edge hash */
+    uint32_t max_search_iterations;
+    real relief_factor, max_disp_ratio;
+    int enable_relief, exclude_neighbors;
+    uint64_t stats[6]; /* contact counters: vv, vt_ccd, vt_prox, ee_ccd, ee_prox, hash_pairs */
+} orc_sim;
+
+orc_sim *orc_sim_create(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
+                        uint32_t nf, const float *inv_masses, const float *thicknesses, const uint32_t *layer_ids,
+                        uint32_t layer_id, float thickness, float stiffness, float bending_stiffness,
+                        float compression_stiffness, float shear_stiffness) {
+    orc_sim *s = (orc_sim *)calloc(1, sizeof(orc_sim));
+    s->m = orc_mesh_build(pos, nv, edges, ne, faces, nf, inv_masses, thicknesses, layer_ids, layer_id, thickness,
+                          stiffness, compression_stiffness, shear_stiffness, bending_stiffness);
+    orc_mesh *m = s->m;
+    s->nv = nv;
+    size_t N = nv ? nv : 1;
+    s->x = (v3 *)malloc(N * sizeof(v3)); s->p = (v3 *)malloc(N * sizeof(v3)); s->v = (v3 *)calloc(N, sizeof(v3));
+    s->x0 = (v3 *)malloc(N * sizeof(v3));
+    s->w = (real *)malloc(N * sizeof(real)); s->w0 = (real *)malloc(N * sizeof(real));
+    for (uint32_t i = 0; i < nv; i++) {
+        s->x0[i] = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        s->x[i] = s->x0[i]; s->p[i] = s->x0[i];
+        s->w[i] = s->w0[i] = m->inv_mass[i];
+    }
+    s->lam_d = (real *)calloc(m->nd ? m->nd : 1, sizeof(real));
+    s->lam_b = (real *)calloc(m->nb ? m->nb : 1, sizeof(real));
+    s->d_tc = (real *)malloc((m->nd ? m->nd : 1) * sizeof(real));
+    s->d_cc = (real *)malloc((m->nd ? m->nd : 1) * sizeof(real));
+    s->b_comp = (real *)malloc((m->nb ? m->nb : 1) * sizeof(real));
+    for (uint32_t k = 0; k < m->nd; k++) { s->d_tc[k] = m->d_tc[k]; s->d_cc[k] = m->d_cc[k]; }
+    for (uint32_t k = 0; k < m->nb; k++) s->b_comp[k] = m->b_comp[k];
+    s->gravity[0] = 0; s->gravity[1] = 0; s->gravity[2] = (real)-9.81f;
+    s->damping = (real)1.0f;
+    s->iterations = 2;
+    s->self_collision = 1;
+    s->cell_size = m->cell_size; s->table_size = m->table_size;
+    s->acc = (accum_t *)calloc(N, sizeof(accum_t));
+    s->hkeys = (uint32_t *)malloc(N * 4); s->hsorted = (uint32_t *)malloc(N * 4);
+    s->hstarts = (uint32_t *)malloc((size_t)(s->table_size + 1) * 4);
+    size_t NE = m->ne ? m->ne : 1;
+    s->ekeys = (uint32_t *)malloc(NE * 4); s->esorted = (uint32_t *)malloc(NE * 4);
+    s->estarts = (uint32_t *)malloc((size_t)(s->table_size + 1) * 4);
+    s->max_search_iterations = 256;
+    s->relief_factor = (real)0.2f; s->max_disp_ratio = (real)0.2f;
+    s->enable_relief = 1; s->exclude_neighbors = 1;
+    return s;
+}
+
+void orc_sim_free(orc_sim *s) {
+    if (!s) return;
+    orc_mesh_free(s->m);
+    free(s->x); free(s->p); free(s->v); free(s->x0); free(s->w); free(s->w0); free(s->lam_d); free(s->lam_b);
+    free(s->d_tc); free(s->d_cc); free(s->b_comp); free(s->cols); free(s->acc);
+    free(s->hkeys); free(s->hstarts); free(s->hsorted); free(s->ekeys); free(s->estarts); free(s->esorted);
+    free(s);
+}
+
+const orc_mesh *orc_sim_mesh(const orc_sim *s) { return s->m; }
+void orc_sim_set_gravity(orc_sim *s, float gx, float gy, float gz) { s->gravity[0] = gx; s->gravity[1] = gy; s->gravity[2] = gz; }
+void orc_sim_set_damping(orc_sim *s, float d) { s->damping = d > 0.0f ? d : 0.0f; }
+void orc_sim_set_damping_all(orc_sim *s, float t, float c, float sh, float b) {
+    s->tension_damping = t > 0 ? t : 0; s->compression_damping = c > 0 ? c : 0;
+    s->shear_damping = sh > 0 ? sh : 0; s->bending_damping = b > 0 ? b : 0;
+}
+void orc_sim_set_iterations(orc_sim *s, uint32_t it) { s->iterations = it; }
+void orc_sim_set_self_collision(orc_sim *s, int on) { s->self_collision = on; }
+/* mod.rs:920-953 (multi-layer untangling is not part of this build) */
+void orc_sim_set_self_collision_options(orc_sim *s, float relief, float ratio, int exclude_neighbors,
+                                        uint32_t max_iterations) {
+    s->relief_factor = relief; s->max_disp_ratio = ratio;
+    s->enable_relief = relief < 0.999f;
+    s->exclude_neighbors = exclude_neighbors;
+    s->max_search_iterations = max_iterations;
+}
+void orc_sim_set_hash_cell(orc_sim *s, float cell) { if (cell > 0.0f) s->cell_size = cell; }
+/* raw compliance override per constraint kind (no reference counterpart: the
+ * reference maps stiffness >= 5000 N/m to zero compliance) */
+void orc_sim_set_compliances_raw(orc_sim *s, float stretch, float shear, float bend) {
+    for (uint32_t k = 0; k < s->m->nd; k++) {
+        if (s->m->d_type[k] == 1) { s->d_tc[k] = shear; s->d_cc[k] = shear; }
+        else { s->d_tc[k] = stretch; s->d_cc[k] = stretch; }
+    }
+    for (uint32_t k = 0; k < s->m->nb; k++) s->b_comp[k] = bend;
+}
+static void add_col(orc_sim *s, collider c) {
+    s->cols = (collider *)realloc(s->cols, (s->ncols + 1) * sizeof(collider));
+    s->cols[s->ncols++] = c;
+}
+void orc_sim_add_sphere(orc_sim *s, const float *c, float r, float fr, float rest) {
+    collider k = {0, fr, rest, mk(c[0], c[1], c[2]), r, mk(0, 0, 0)}; add_col(s, k);
+}
+void orc_sim_add_capsule(orc_sim *s, const float *a, const float *b, float r, float fr, float rest) {
+    collider k = {1, fr, rest, mk(a[0], a[1], a[2]), r, mk(b[0], b[1], b[2])}; add_col(s, k);
+}
+void orc_sim_add_plane(orc_sim *s, const float *pt, const float *n, float fr, float rest) {
+    collider k = {2, fr, rest, mk(pt[0], pt[1], pt[2]), 0, mk(n[0], n[1], n[2])}; add_col(s, k);
+}
+void orc_sim_clear_colliders(orc_sim *s) { s->ncols = 0; }
+void orc_sim_reset(orc_sim *s) {
+    for (uint32_t i = 0; i < s->nv; i++) { s->x[i] = s->x0[i]; s->p[i] = s->x0[i]; s->v[i] = mk(0, 0, 0); s->w[i] = s->w0[i]; }
+}
+void orc_sim_set_state(orc_sim *s, const double *x, const double *p, const double *v) {
+    for (uint32_t i = 0; i < s->nv; i++) {
+        if (x) s->x[i] = mk((real)x[3 * i], (real)x[3 * i + 1], (real)x[3 * i + 2]);
+        if (p) s->p[i] = mk((real)p[3 * i], (real)p[3 * i + 1], (real)p[3 * i + 2]);
+        if (v) s->v[i] = mk((real)v[3 * i], (real)v[3 * i + 1], (real)v[3 * i + 2]);
+    }
+}
+/* which: 0 = position x, 1 = predicted p, 2 = velocity */
+void orc_sim_get(const orc_sim *s, int which, double *out) {
+    const v3 *a = which == 0 ? s->x : which == 1 ? s->p : s->v;
+    for (uint32_t i = 0; i < s->nv; i++) { out[3 * i] = a[i].x; out[3 * i + 1] = a[i].y; out[3 * i + 2] = a[i].z; }
+}
+void orc_sim_get_lambdas(const orc_sim *s, double *ld, double *lb) {
+    for (uint32_t k = 0; k < s->m->nd; k++) ld[k] = s->lam_d[k];
+    for (uint32_t k = 0; k < s->m->nb; k++) lb[k] = s->lam_b[k];
+}
+
+/* predict.wgsl:33-55 */
+static void k_predict(orc_sim *s, real dt_sub) {
+    real dt = rmax(dt_sub, (real)1e-8f);
+    v3 g = mk(s->gravity[0], s->gravity[1], s->gravity[2]);
+    for (uint32_t i = 0; i < s->nv; i++) {
+        if (s->w[i] <= 0) { s->v[i] = mk(0, 0, 0); s->p[i] = s->x[i]; }
+        else {
+            s->v[i] = add(mul(s->v[i], rmax((real)0, (real)1 - s->damping * dt)), mul(g, dt));
+            s->p[i] = add(s->x[i], mul(s->v[i], dt));
+        }
+    }
+}
+
+/* distance.wgsl:60-124 for one constraint */
+static void k_distance_one(orc_sim *s, uint32_t k, real dt_sub) {
+    const orc_mesh *m = s->m;
+    uint32_t a = m->d_v0[k], b = m->d_v1[k];
+    real w0 = s->w[a], w1 = s->w[b], wsum = w0 + w1;
+    if (wsum <= (real)1e-7f) return;
+    v3 p0 = s->p[a], p1 = s->p[b];
+    v3 delta = sub(p0, p1);
+    real dist = len3(delta);
+    if (dist <= (real)1e-7f) return;
+    real c = dist - (real)m->d_rest[k];
+    real comp = c >= 0 ? s->d_tc[k] : s->d_cc[k];
+    if (comp >= (real)1e9f) return;
+    v3 dir = dvs(delta, dist);
+    real dt = rmax(dt_sub, (real)1e-8f);
+    real alpha = comp / (dt * dt);
+    real damp = m->d_type[k] == 1 ? s->shear_damping : (c >= 0 ? s->tension_damping : s->compression_damping);
+    real tau = damp * (real)1e-5f;
+    real gamma = rmin(tau / dt, (real)1e4f);
+    v3 rel = sub(sub(p0, s->x[a]), sub(p1, s->x[b]));
+    real vproj = dot(dir, rel);
+    real lam = s->lam_d[k];
+    real dl = (-c - alpha * lam - gamma * vproj) / (((real)1 + gamma) * wsum + alpha);
+    s->lam_d[k] = lam + dl;
+    v3 c0 = mul(dir, w0 * dl);
+    v3 c1 = mul(dir, -w1 * dl);
+    s->p[a] = add(s->p[a], c0);
+    s->p[b] = add(s->p[b], c1);
+}
+
+/* bending.wgsl:59-126 */
+static void k_bending_one(orc_sim *s, uint32_t k, real dt_sub) {
+    const orc_mesh *m = s->m;
+    real comp = s->b_comp[k];
+    if (comp >= (real)1e9f) return;
+    uint32_t v2 = m->b_v2[k], v3_ = m->b_v3[k];
+    real w2 = s->w[v2], w3 = s->w[v3_], wsum = w2 + w3;
+    if (wsum <= (real)1e-7f) return;
+    v3 p2 = s->p[v2], p3 = s->p[v3_];
+    v3 delta = sub(p2, p3);
+    real dist = len3(delta);
+    if (dist <= (real)1e-7f) return;
+    float pad0 = m->b_pad0[k];
+    if (pad0 > 0.5f && pad0 < 1.5f) {
+        if (dist >= (real)m->b_rest[k]) return;
+        real gap = len3(sub(s->p[m->b_v0[k]], s->p[m->b_v1[k]]));
+        if (gap > (real)m->b_pad1[k]) return;
+    }
+    v3 dir = dvs(delta, dist);
+    real dt = rmax(dt_sub, (real)1e-8f);
+    real alpha = comp / (dt * dt);
+    real tau = s->bending_damping * (real)1e-5f;
+    real gamma = rmin(tau / dt, (real)1e4f);
+    v3 rel = sub(sub(p2, s->x[v2]), sub(p3, s->x[v3_]));
+    real vproj = dot(dir, rel);
+    real c = dist - (real)m->b_rest[k];
+    real lam = s->lam_b[k];
+    real dl = (-c - alpha * lam - gamma * vproj) / (((real)1 + gamma) * wsum + alpha);
+    s->lam_b[k] = lam + dl;
+    s->p[v2] = add(s->p[v2], mul(dir, w2 * dl));
+    s->p[v3_] = add(s->p[v3_], mul(dir, -w3 * dl));
+}
+
+/* collision.wgsl:83-108 */
+static void contact_response(v3 target, v3 n, real fr, real rest, real dt, v3 p_init, v3 *xn, v3 *p) {
+    v3 total = sub(target, *xn);
+    v3 tang = sub(total, mul(n, dot(total, n)));
+    *p = sub(target, mul(tang, rclamp(fr, 0, 1)));
+    real edt = rmax(dt, (real)1e-6f);
+    real vin = dot(dvs(sub(p_init, *xn), edt), n);
+    real vout = 0;
+    if (vin < 0) vout = -rclamp(rest, 0, 1) * vin;
+    real dxn = dot(sub(target, *xn), n) - vout * edt;
+    *xn = add(*xn, mul(n, dxn));
+}
+
+/* collision.wgsl:377-652 (primitive colliders; mesh/SDF handled elsewhere) */
+static void k_collision(orc_sim *s, real dt) {
+    for (uint32_t i = 0; i < s->nv; i++) {
+        if (s->w[i] <= 0) continue;
+        v3 p = s->p[i], pinit = p, xn = s->x[i];
+        real th = (real)s->m->thick[i];
+        for (uint32_t c = 0; c < s->ncols; c++) {
+            const collider *col = &s->cols[c];
+            if (col->type == 0) {
+                real tr = col->radius + th;
+                v3 d = sub(p, col->a);
+                real dist = len3(d);
+                if (dist < tr) {
+                    v3 n = mk(0, 0, 1);
+                    if (dist > (real)1e-7f) n = dvs(d, dist);
+                    contact_response(add(col->a, mul(n, tr)), n, col->friction, col->restitution, dt, pinit, &xn, &p);
+                }
+            } else if (col->type == 1) {
+                real tr = col->radius + th;
+                v3 ab = sub(col->b, col->a);
+                real l2 = dot(ab, ab);
+                real t = 0;
+                if (l2 > (real)1e-7f) t = rclamp(dot(sub(p, col->a), ab) / l2, 0, 1);
+                v3 cl = add(col->a, mul(ab, t));
+                v3 d = sub(p, cl);
+                real dist = len3(d);
+                if (dist < tr) {
+                    v3 n = mk(0, 0, 1);
+                    if (dist > (real)1e-7f) n = dvs(d, dist);
+                    contact_response(add(cl, mul(n, tr)), n, col->friction, col->restitution, dt, pinit, &xn, &p);
+                }
+            } else if (col->type == 2) {
+                v3 n = dvs(col->b, len3(col->b));
+                real dist = dot(sub(p, col->a), n);
+                if (dist < th) {
+                    contact_response(add(p, mul(n, th - dist)), n, col->friction, col->restitution, dt, pinit, &xn, &p);
+                }
+            }
+        }
+        s->p[i] = p;
+        s->x[i] = xn;
+    }
+}
+
+/* update_vel.wgsl:33-53 */
+static void k_update_vel(orc_sim *s, real dt_sub) {
+    real dt = rmax(dt_sub, (real)1e-8f);
+    for (uint32_t i = 0; i < s->nv; i++) {
+        if (s->w[i] > 0) s->v[i] = dvs(sub(s->p[i], s->x[i]), dt);
+        else s->v[i] = mk(0, 0, 0);
+        s->x[i] = s->p[i];
+    }
+}
+
+/* ---------------- self collision (self_collision.wgsl) ---------------- */
+#define FIXED_SCALE ((real)1000000.0f)
+#define MAX_FIXED ((real)50000000.0f)
+static inline int32_t to_fixed(real x) { return f2i_sat(rclamp(x * FIXED_SCALE, -MAX_FIXED, MAX_FIXED)); }
+static inline void acc_add(int32_t *dst, int32_t v) { *dst = (int32_t)((uint32_t)*dst + (uint32_t)v); }
+static void add_disp(accum_t *a, v3 d) {
+    acc_add(&a->dx, to_fixed(d.x)); acc_add(&a->dy, to_fixed(d.y)); acc_add(&a->dz, to_fixed(d.z)); a->count += 1;
+}
+static void add_pin_disp(accum_t *a, v3 d) {
+    acc_add(&a->pdx, to_fixed(d.x)); acc_add(&a->pdy, to_fixed(d.y)); acc_add(&a->pdz, to_fixed(d.z)); a->pcount += 1;
+}
+static void ccd_axis(int32_t ix, int32_t *pos, int32_t *negv) {
+    if (ix > 0) { if (ix > *pos) *pos = ix; }
+    else if (ix < 0) { if (ix < *negv) *negv = ix; }
+}
+static void add_ccd(accum_t *a, v3 d) {
+    ccd_axis(to_fixed(d.x), &a->cpx, &a->cnx);
+    ccd_axis(to_fixed(d.y), &a->cpy, &a->cny);
+    ccd_axis(to_fixed(d.z), &a->cpz, &a->cnz);
+}
+
+typedef struct { v3 pt, bary; } cbary;
+static cbary closest_bary(v3 p, v3 a, v3 b, v3 c) { /* self_collision.wgsl:134-179 */
+    cbary r;
+    v3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+    real d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) { r.pt = a; r.bary = mk(1, 0, 0); return r; }
+    v3 bp = sub(p, b);
+    real d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) { r.pt = b; r.bary = mk(0, 1, 0); return r; }
+    real vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) { real v = d1 / (d1 - d3); r.pt = add(a, mul(ab, v)); r.bary = mk(1 - v, v, 0); return r; }
+    v3 cp = sub(p, c);
+    real d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) { r.pt = c; r.bary = mk(0, 0, 1); return r; }
+    real vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) { real w = d2 / (d2 - d6); r.pt = add(a, mul(ac, w)); r.bary = mk(1 - w, 0, w); return r; }
+    real va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+        real w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        r.pt = add(b, mul(sub(c, b), w)); r.bary = mk(0, 1 - w, w); return r;
+    }
+    real denom = (real)1 / (va + vb + vc);
+    real v = vb * denom, w = vc * denom;
+    r.pt = add(add(a, mul(ab, v)), mul(ac, w)); r.bary = mk(1 - v - w, v, w);
+    return r;
+}
+
+/* Möller–Trumbore segment/triangle, self_collision.wgsl:182-218 (identical in collision.wgsl:322-361).
+ * returns hit and fills t,u,v */
+static int seg_tri(v3 po, v3 pn, v3 a, v3 b, v3 c, real *t_out, real *u_out, real *v_out) {
+    v3 d = sub(pn, po);
+    real ld = dot(d, d);
+    if (ld < (real)1e-12f) return 0;
+    v3 e1 = sub(b, a), e2 = sub(c, a);
+    v3 pv = cross(d, e2);
+    real det = dot(e1, pv);
+    real tn = dot(e1, e1) * dot(e2, e2);
+    if (det * det <= tn * ld * (real)1e-10f || tn <= (real)1e-20f) return 0;
+    real inv = (real)1 / det;
+    v3 tv = sub(po, a);
+    real u = dot(tv, pv) * inv;
+    if (u < (real)-0.05f || u > (real)1.05f) return 0;
+    v3 qv = cross(tv, e1);
+    real v = dot(d, qv) * inv;
+    if (v < (real)-0.05f || (u + v) > (real)1.05f) return 0;
+    real t = dot(e2, qv) * inv;
+    if (t >= 0 && t <= 1) { *t_out = t; *u_out = u; *v_out = v; return 1; }
+    return 0;
+}
+
+static int within_two_hops(const orc_mesh *m, uint32_t a, uint32_t b) { /* self_collision.wgsl:262-283 */
+    if (a == b) return 1;
+    uint32_t lo = m->th_off[a], hi = m->th_off[a + 1];
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2, val = m->th_idx[mid];
+        if (val == b) return 1;
+        if (val < b) lo = mid + 1; else hi = mid;
+    }
+    return 0;
+}
+static int near_folded(const orc_mesh *m, uint32_t a, uint32_t b, real cur) { /* self_collision.wgsl:287-314 */
+    if (a == b) return 1;
+    uint32_t lo = m->th_off[a], hi = m->th_off[a + 1];
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2, val = m->th_idx[mid];
+        if (val == b) { real l0 = m->th_rest[mid]; return cur < l0 * (real)0.5f ? 0 : 1; }
+        if (val < b) lo = mid + 1; else hi = mid;
+    }
+    return 0;
+}
+static int near_folded_sq(const orc_mesh *m, uint32_t a, uint32_t b, real cur_sq) { /* ee_direct.wgsl:259-283 */
+    if (a == b) return 1;
+    uint32_t lo = m->th_off[a], hi = m->th_off[a + 1];
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2, val = m->th_idx[mid];
+        if (val == b) { real l0 = m->th_rest[mid]; return cur_sq < (l0 * l0) * (real)0.25f ? 0 : 1; }
+        if (val < b) lo = mid + 1; else hi = mid;
+    }
+    return 0;
+}
+
+/* sc_solve: self_collision.wgsl:317-681 (single layer: untangling branches inactive) */
+static void k_sc_solve(orc_sim *s) {
+    const orc_mesh *m = s->m;
+    const real EPS = (real)1e-7f;
+    real cell = s->cell_size;
+    uint32_t T = s->table_size;
+    for (uint32_t index = 0; index < s->nv; index++) {
+        int pinned_i = s->w[index] <= 0;
+        v3 xo_i = s->x[index], p_i = s->p[index];
+        real move_i0 = len3(sub(p_i, xo_i));
+        if (pinned_i && move_i0 < EPS) continue;
+        real thick_i = (real)m->thick[index];
+        real llen_i = (real)m->local_len[index];
+        int32_t cc[3]; cell_of(p_i, cell, cc);
+        for (int32_t cx = cc[0] - 1; cx <= cc[0] + 1; cx++)
+        for (int32_t cy = cc[1] - 1; cy <= cc[1] + 1; cy++)
+        for (int32_t cz = cc[2] - 1; cz <= cc[2] + 1; cz++) {
+            int32_t nc[3] = {cx, cy, cz};
+            uint32_t h = hash_coords(nc, T);
+            for (uint32_t k = s->hstarts[h]; k < s->hstarts[h + 1]; k++) {
+                uint32_t j = s->hsorted[k];
+                if (j >= s->nv || j == index) continue;
+                v3 p_j = s->p[j];
+                int32_t jc[3]; cell_of(p_j, cell, jc);
+                if (!(jc[0] == cx && jc[1] == cy && jc[2] == cz)) continue;
+                v3 dvv = sub(p_i, p_j);
+                real dsq = dot(dvv, dvv);
+                real thick_j = (real)m->thick[j], llen_j = (real)m->local_len[j];
+                real min_dist = thick_i + thick_j;
+                real edge_margin = rmin(llen_i, llen_j) * (real)0.06f;
+                real eff = rmax(min_dist, edge_margin);
+                real max_geom = eff + (llen_i + llen_j) * (real)1.3f;
+                if (!(dsq <= max_geom * max_geom)) continue;
+                real dvv_len = RSQRT(rmax(dsq, 0));
+                if (near_folded(m, index, j, dvv_len)) continue;
+                v3 pj_old = s->x[j];
+                int both_dynamic = s->w[j] > 0;
+                real eff_sq = eff * eff;
+                /* 1. V-V */
+                if (!pinned_i && (!both_dynamic || index < j)) {
+                    if (dsq < eff_sq && dsq > EPS * EPS) {
+                        s->stats[0]++;
+                        v3 n = dvs(dvv, dvv_len);
+                        real pen = eff - dvv_len;
+                        real wi = s->w[index], wj = s->w[j], ws = wi + wj;
+                        if (ws > EPS) {
+                            if (wj <= 0) {
+                                v3 jm = sub(p_j, pj_old);
+                                add_pin_disp(&s->acc[index], add(mul(n, pen), mul(jm, (real)0.5f)));
+                            } else {
+                                v3 disp = mul(n, pen * (real)0.5f);
+                                add_disp(&s->acc[index], mul(disp, wi / ws));
+                                add_disp(&s->acc[j], mul(neg(disp), wj / ws));
+                            }
+                        }
+                    }
+                }
+                /* 2. V-T */
+                real mv_i = len3(sub(p_i, xo_i));
+                real mv_j = len3(sub(p_j, pj_old));
+                real vt_bound = eff + llen_j * (real)1.3f + mv_i + mv_j;
+                if (!(dsq <= vt_bound * vt_bound) || within_two_hops(m, index, j)) continue;
+                for (uint32_t st = m->star_off[j]; st < m->star_off[j + 1]; st++) {
+                    uint32_t v0 = m->star_v0[st], v1 = m->star_v1[st];
+                    if (v0 == index || v1 == index) continue;
+                    v3 p_v0 = s->p[v0], p_v1 = s->p[v1], p_v0o = s->x[v0], p_v1o = s->x[v1];
+                    v3 d0 = sub(p_i, p_v0);
+                    real ll0 = (real)m->local_len[v0];
+                    real et0 = rmax(thick_i + (real)m->thick[v0], rmin(llen_i, ll0) * (real)0.06f);
+                    real b0 = et0 + ll0 * (real)1.3f + mv_i + len3(sub(p_v0, p_v0o));
+                    int32_t c0[3]; cell_of(p_v0, cell, c0);
+                    int in0 = abs(c0[0] - cc[0]) <= 1 && abs(c0[1] - cc[1]) <= 1 && abs(c0[2] - cc[2]) <= 1 &&
+                              dot(d0, d0) <= b0 * b0;
+                    v3 d1 = sub(p_i, p_v1);
+                    real ll1 = (real)m->local_len[v1];
+                    real et1 = rmax(thick_i + (real)m->thick[v1], rmin(llen_i, ll1) * (real)0.06f);
+                    real b1 = et1 + ll1 * (real)1.3f + mv_i + len3(sub(p_v1, p_v1o));
+                    int32_t c1[3]; cell_of(p_v1, cell, c1);
+                    int in1 = abs(c1[0] - cc[0]) <= 1 && abs(c1[1] - cc[1]) <= 1 && abs(c1[2] - cc[2]) <= 1 &&
+                              dot(d1, d1) <= b1 * b1;
+                    if ((in0 && j > v0) || (in1 && j > v1)) continue;
+                    v3 tri_move = dvs(add(add(sub(p_j, pj_old), sub(p_v0, p_v0o)), sub(p_v1, p_v1o)), (real)3.0f);
+                    int tri_pin = s->w[j] <= 0 || s->w[v0] <= 0 || s->w[v1] <= 0;
+                    v3 rel_old = add(xo_i, tri_move);
+                    v3 effv = mk(eff, eff, eff);
+                    v3 smin = sub(vmin(rel_old, p_i), effv), smax = add(vmax(rel_old, p_i), effv);
+                    v3 tmin = vmin(vmin(p_j, p_v0), p_v1), tmax = vmax(vmax(p_j, p_v0), p_v1);
+                    if (smin.x > tmax.x || smin.y > tmax.y || smin.z > tmax.z ||
+                        smax.x < tmin.x || smax.y < tmin.y || smax.z < tmin.z) continue;
+                    if (within_two_hops(m, index, v0) || within_two_hops(m, index, v1)) continue;
+                    v3 move = sub(p_i, rel_old);
+                    real move_sq = dot(move, move);
+                    int hit = 0; real ht = 1, hu = 0, hv = 0;
+                    if (move_sq > (real)1e-12f) hit = seg_tri(rel_old, p_i, p_j, p_v0, p_v1, &ht, &hu, &hv);
+                    if (hit) {
+                        s->stats[1]++;
+                        v3 hit_pt = add(rel_old, mul(move, ht));
+                        v3 tcr = cross(sub(p_v0, p_j), sub(p_v1, p_j));
+                        real tl = len3(tcr);
+                        v3 tn = tl > (real)1e-12f ? dvs(tcr, tl) : mk(0, 1, 0);
+                        real h_old_raw = dot(sub(rel_old, p_j), tn);
+                        real side = fabs((double)h_old_raw) > 1e-6 ? h_old_raw : dot(neg(move), tn);
+                        v3 n_side = side >= 0 ? tn : neg(tn);
+                        real h_old = rmax(dot(sub(rel_old, hit_pt), n_side), 0);
+                        real h_safe = rmax(h_old, eff * (real)0.25f);
+                        v3 p_safe = add(hit_pt, mul(n_side, h_safe));
+                        v3 total = sub(p_safe, p_i);
+                        real pen_soft = rmax(0, eff - h_safe);
+                        if (tri_pin) {
+                            if (!pinned_i) add_ccd(&s->acc[index], add(total, mul(n_side, pen_soft)));
+                        } else if (pinned_i) {
+                            v3 bc = mk(rclamp(1 - hu - hv, 0, 1), rclamp(hu, 0, 1), rclamp(hv, 0, 1));
+                            real wj = s->w[j], w0 = s->w[v0], w1 = s->w[v1];
+                            real wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+                            if (wt > EPS) {
+                                v3 full = add(total, mul(n_side, pen_soft));
+                                if (wj > 0) add_ccd(&s->acc[j], mul(neg(full), bc.x * wj / wt));
+                                if (w0 > 0) add_ccd(&s->acc[v0], mul(neg(full), bc.y * w0 / wt));
+                                if (w1 > 0) add_ccd(&s->acc[v1], mul(neg(full), bc.z * w1 / wt));
+                            }
+                        } else {
+                            v3 bc = mk(rclamp(1 - hu - hv, 0, 1), rclamp(hu, 0, 1), rclamp(hv, 0, 1));
+                            real wi = s->w[index], wj = s->w[j], w0 = s->w[v0], w1 = s->w[v1];
+                            real wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+                            real wtot = wi + wt;
+                            if (wtot > EPS) {
+                                add_ccd(&s->acc[index], mul(total, wi / wtot));
+                                if (wj > 0) add_ccd(&s->acc[j], mul(neg(total), bc.x * wj / wtot));
+                                if (w0 > 0) add_ccd(&s->acc[v0], mul(neg(total), bc.y * w0 / wtot));
+                                if (w1 > 0) add_ccd(&s->acc[v1], mul(neg(total), bc.z * w1 / wtot));
+                                if (pen_soft > EPS) {
+                                    v3 tp = mul(n_side, pen_soft * (real)0.6f);
+                                    add_disp(&s->acc[index], mul(tp, wi / wtot));
+                                    if (wj > 0) add_disp(&s->acc[j], mul(neg(tp), bc.x * wj / wtot));
+                                    if (w0 > 0) add_disp(&s->acc[v0], mul(neg(tp), bc.y * w0 / wtot));
+                                    if (w1 > 0) add_disp(&s->acc[v1], mul(neg(tp), bc.z * w1 / wtot));
+                                }
+                            }
+                        }
+                    } else {
+                        cbary r = closest_bary(p_i, p_j, p_v0, p_v1);
+                        v3 dvt = sub(p_i, r.pt);
+                        real dvsq = dot(dvt, dvt);
+                        if (!pinned_i && dvsq < eff_sq && dvsq > EPS * EPS) {
+                            s->stats[2]++;
+                            real dvt_len = RSQRT(dvsq);
+                            v3 nvt = dvs(dvt, dvt_len);
+                            real pen = eff - dvt_len;
+                            if (tri_pin) {
+                                add_pin_disp(&s->acc[index], add(mul(nvt, pen), mul(tri_move, (real)0.5f)));
+                            } else {
+                                real wi = s->w[index], wj = s->w[j], w0 = s->w[v0], w1 = s->w[v1];
+                                v3 bc = r.bary;
+                                real wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+                                real wtot = wi + wt;
+                                if (wtot > EPS) {
+                                    v3 tp = mul(nvt, pen * (real)0.6f);
+                                    add_disp(&s->acc[index], mul(tp, wi / wtot));
+                                    if (wj > 0) add_disp(&s->acc[j], mul(neg(tp), bc.x * wj / wtot));
+                                    if (w0 > 0) add_disp(&s->acc[v0], mul(neg(tp), bc.y * w0 / wtot));
+                                    if (w1 > 0) add_disp(&s->acc[v1], mul(neg(tp), bc.z * w1 / wtot));
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* ---------------- E-E (self_collision_ee_direct.wgsl) ---------------- */
+static void add_disp_ee(accum_t *a, v3 d) { /* ee_direct.wgsl:70-79: the count is incremented twice */
+    acc_add(&a->dx, to_fixed(d.x)); acc_add(&a->dy, to_fixed(d.y)); acc_add(&a->dz, to_fixed(d.z)); a->count += 2;
+}
+static void add_ccd_ee(accum_t *a, v3 d, int count_ee) { /* ee_direct.wgsl:92-116 */
+    add_ccd(a, d);
+    if (count_ee) {
+        real l = len3(d) * FIXED_SCALE;
+        real lc = rclamp(l, 0, (real)2000000000.0f);
+        uint32_t lf = lc >= (real)4294967296.0 ? 0xFFFFFFFFu : (uint32_t)lc;
+        if (lf > 0) a->ee_count += 1;
+    }
+}
+typedef struct { real s, t; } st2;
+static st2 closest_segs(v3 p1, v3 q1, v3 p2, v3 q2) { /* ee_direct.wgsl:155-193 */
+    st2 r = {0, 0};
+    v3 d1 = sub(q1, p1), d2 = sub(q2, p2), rr = sub(p1, p2);
+    real a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, rr);
+    if (a <= (real)1e-12f && e <= (real)1e-12f) return r;
+    if (a <= (real)1e-12f) { r.t = rclamp(f / e, 0, 1); return r; }
+    real c = dot(d1, rr);
+    if (e <= (real)1e-12f) { r.s = rclamp(-c / a, 0, 1); return r; }
+    real b = dot(d1, d2);
+    real denom = a * e - b * b;
+    real s = 0;
+    if (denom > a * e * (real)1e-7f) s = rclamp((b * f - c * e) / denom, 0, 1);
+    real t = (b * s + f) / e;
+    if (t < 0) { t = 0; s = rclamp(-c / a, 0, 1); }
+    else if (t > 1) { t = 1; s = rclamp((b - c) / a, 0, 1); }
+    r.s = s; r.t = t;
+    return r;
+}
+static real seg_dist_sq(v3 a0o, v3 a0, v3 a1o, v3 a1, v3 b0o, v3 b0, v3 b1o, v3 b1, real tau, st2 *st) {
+    v3 A0 = vmix(a0o, a0, tau), A1 = vmix(a1o, a1, tau), B0 = vmix(b0o, b0, tau), B1 = vmix(b1o, b1, tau);
+    *st = closest_segs(A0, A1, B0, B1);
+    v3 pa = vmix(A0, A1, st->s), pb = vmix(B0, B1, st->t);
+    v3 d = sub(pa, pb);
+    return dot(d, d);
+}
+static real seg_volume(v3 a0o, v3 a0, v3 a1o, v3 a1, v3 b0o, v3 b0, v3 b1o, v3 b1, real tau) {
+    v3 A0 = vmix(a0o, a0, tau), A1 = vmix(a1o, a1, tau), B0 = vmix(b0o, b0, tau), B1 = vmix(b1o, b1, tau);
+    return dot(cross(sub(A1, A0), sub(B1, B0)), sub(B0, A0));
+}
+static int bbit(const orc_mesh *m, uint32_t e) { return (m->e_bbits[e >> 5] >> (e & 31)) & 1; }
+
+static void k_sc_solve_ee(orc_sim *s) {
+    const orc_mesh *m = s->m;
+    const real EPS = (real)1e-7f;
+    real cell = s->cell_size;
+    uint32_t T = s->table_size;
+    for (uint32_t ea = 0; ea < m->ne; ea++) {
+        uint32_t u0 = m->e_v0[ea], u1 = m->e_v1[ea];
+        int pin_u0 = s->w[u0] <= 0, pin_u1 = s->w[u1] <= 0;
+        int a_has_pin = pin_u0 || pin_u1;
+        v3 p_u0 = s->p[u0], p_u1 = s->p[u1], p_u0o = s->x[u0], p_u1o = s->x[u1];
+        real th_u0 = (real)m->thick[u0], th_u1 = (real)m->thick[u1];
+        real max_th_a = rmax(th_u0, th_u1);
+        int bnd_a = bbit(m, ea);
+        v3 mta = mk(max_th_a * 2, max_th_a * 2, max_th_a * 2);
+        v3 amin = sub(vmin(vmin(p_u0, p_u1), vmin(p_u0o, p_u1o)), mta);
+        v3 amax = add(vmax(vmax(p_u0, p_u1), vmax(p_u0o, p_u1o)), mta);
+        v3 pmid = mul(add(p_u0, p_u1), (real)0.5f);
+        int32_t bc[3]; cell_of(pmid, cell, bc);
+        uint32_t search_count = 0;
+        int done = 0;
+        for (int dz = -1; dz <= 1 && !done; dz++)
+        for (int dy = -1; dy <= 1 && !done; dy++)
+        for (int dx = -1; dx <= 1 && !done; dx++) {
+            int32_t nc[3] = {bc[0] + dx, bc[1] + dy, bc[2] + dz};
+            uint32_t h = hash_coords(nc, T);
+            uint32_t start = s->estarts[h], end = s->estarts[h + 1];
+            if (start >= end) continue;
+            for (uint32_t q = start; q < end && !done; q++) {
+                uint32_t eb = s->esorted[q];
+                if (ea >= eb) continue;
+                uint32_t j = m->e_v0[eb], vj = m->e_v1[eb];
+                if (j == u0 || j == u1 || vj == u0 || vj == u1) continue;
+                v3 p_j = s->p[j], p_vj = s->p[vj];
+                v3 bmid = mul(add(p_j, p_vj), (real)0.5f);
+                int32_t bcell[3]; cell_of(bmid, cell, bcell);
+                if (!(bcell[0] == nc[0] && bcell[1] == nc[1] && bcell[2] == nc[2])) continue;
+                v3 p_jo = s->x[j], p_vjo = s->x[vj];
+                v3 bmin_ = vmin(vmin(p_j, p_vj), vmin(p_jo, p_vjo)), bmax_ = vmax(vmax(p_j, p_vj), vmax(p_jo, p_vjo));
+                if (amin.x > bmax_.x || amin.y > bmax_.y || amin.z > bmax_.z ||
+                    amax.x < bmin_.x || amax.y < bmin_.y || amax.z < bmin_.z) continue;
+                search_count++;
+                if (search_count > s->max_search_iterations) { done = 1; break; }
+                int bnd_b = bbit(m, eb);
+                int bnd_pair = bnd_a || bnd_b;
+                int run_ccd = bnd_pair;
+                int has_cached = 0;
+                st2 st1c = {0, 0};
+                v3 pt1ac = mk(0, 0, 0), pt1bc = mk(0, 0, 0), d1c = mk(0, 0, 0);
+                real dsqc = 0, effc = 0;
+                real th_j = (real)m->thick[j], th_vj = (real)m->thick[vj];
+                if (!run_ccd) {
+                    st1c = closest_segs(p_u0, p_u1, p_j, p_vj);
+                    pt1ac = vmix(p_u0, p_u1, st1c.s);
+                    pt1bc = vmix(p_j, p_vj, st1c.t);
+                    d1c = sub(pt1ac, pt1bc);
+                    dsqc = dot(d1c, d1c);
+                    effc = rmix(th_u0, th_u1, st1c.s) + rmix(th_j, th_vj, st1c.t);
+                    real max_eff = max_th_a + rmax(th_j, th_vj);
+                    v3 du0 = sub(p_u0, p_u0o), du1 = sub(p_u1, p_u1o), dj = sub(p_j, p_jo), dvj = sub(p_vj, p_vjo);
+                    v3 va = mul(add(du0, du1), (real)0.5f), vb = mul(add(dj, dvj), (real)0.5f);
+                    real rel = len3(sub(va, vb)) + (real)0.5f * (len3(sub(du1, du0)) + len3(sub(dvj, dj)));
+                    real reach = max_eff + rel;
+                    if (dsqc > reach * reach) continue;
+                    has_cached = 1;
+                    v3 d0g = sub(vmix(p_u0o, p_u1o, st1c.s), vmix(p_jo, p_vjo, st1c.t));
+                    int g1 = rel > effc * (real)0.5f;
+                    int g2 = dot(d0g, d1c) <= 0;
+                    run_ccd = g1 || g2;
+                }
+                if (run_ccd) {
+                    real min_sq = (real)1e20f, best_tau = 0, half_span = (real)0.125f;
+                    st2 min_st = {0, 0};
+                    real prev_vol = seg_volume(p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj, 0);
+                    real zc_tau = -1;
+                    for (uint32_t k = 0; k <= 8; k++) {
+                        real tk = (real)k * (real)0.125f;
+                        st2 stk;
+                        real dk = seg_dist_sq(p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj, tk, &stk);
+                        if (dk < min_sq) { min_sq = dk; best_tau = tk; min_st = stk; }
+                        if (k > 0) {
+                            real vol = seg_volume(p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj, tk);
+                            if (prev_vol * vol <= 0 && zc_tau < 0) zc_tau = tk - (real)0.0625f;
+                            prev_vol = vol;
+                        }
+                    }
+                    if (zc_tau >= 0) {
+                        st2 stz;
+                        real dz_ = seg_dist_sq(p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj, zc_tau, &stz);
+                        if (dz_ < min_sq) { min_sq = dz_; best_tau = zc_tau; half_span = (real)0.0625f; min_st = stz; }
+                    }
+                    real tlo = rmax(0, best_tau - half_span), thi = rmin(1, best_tau + half_span);
+                    real tau_star = best_tau, dsq_star = min_sq;
+                    st2 st_star = min_st;
+                    const real third = (real)(1.0f / 3.0f);
+                    for (int it = 0; it < 6; it++) {
+                        real m1 = tlo + (thi - tlo) * third;
+                        real m2 = thi - (thi - tlo) * third;
+                        st2 s1, s2;
+                        real e1 = seg_dist_sq(p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj, m1, &s1);
+                        real e2 = seg_dist_sq(p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj, m2, &s2);
+                        if (e1 < dsq_star) { dsq_star = e1; tau_star = m1; st_star = s1; }
+                        if (e2 < dsq_star) { dsq_star = e2; tau_star = m2; st_star = s2; }
+                        if (e1 < e2) thi = m2; else tlo = m1;
+                    }
+                    real ss = st_star.s, ts = st_star.t;
+                    real eff_star = rmix(th_u0, th_u1, ss) + rmix(th_j, th_vj, ts);
+                    if (dsq_star >= eff_star * eff_star) continue;
+                    if (s->exclude_neighbors) {
+                        v3 a0s = vmix(p_u0o, p_u0, tau_star), a1s = vmix(p_u1o, p_u1, tau_star);
+                        v3 b0s = vmix(p_jo, p_j, tau_star), b1s = vmix(p_vjo, p_vj, tau_star);
+                        v3 t0 = sub(a0s, b0s), t1 = sub(a0s, b1s), t2 = sub(a1s, b0s), t3 = sub(a1s, b1s);
+                        if (near_folded_sq(m, u0, j, dot(t0, t0)) || near_folded_sq(m, u0, vj, dot(t1, t1)) ||
+                            near_folded_sq(m, u1, j, dot(t2, t2)) || near_folded_sq(m, u1, vj, dot(t3, t3))) continue;
+                    }
+                    st2 st0 = closest_segs(p_u0o, p_u1o, p_jo, p_vjo);
+                    v3 delta0 = sub(vmix(p_u0o, p_u1o, st0.s), vmix(p_jo, p_vjo, st0.t));
+                    real dist0 = len3(delta0);
+                    if (dist0 > EPS) {
+                        v3 n0 = dvs(delta0, dist0);
+                        st2 st1 = closest_segs(p_u0, p_u1, p_j, p_vj);
+                        v3 delta1 = sub(vmix(p_u0, p_u1, st1.s), vmix(p_j, p_vj, st1.t));
+                        real dist1 = len3(delta1);
+                        v3 dend = sub(vmix(p_u0, p_u1, ss), vmix(p_j, p_vj, ts));
+                        real h_end = dot(dend, n0);
+                        int crossover = (h_end <= 0) || (dot(delta1, n0) <= 0) ||
+                                        (bnd_pair && dist1 >= eff_star && tau_star < 1);
+                        if (h_end < eff_star && crossover) {
+                            s->stats[3]++;
+                            real pen = eff_star - h_end;
+                            int b_has_pin = s->w[j] <= 0 || s->w[vj] <= 0;
+                            if (a_has_pin && b_has_pin) continue;
+                            int a_all = s->w[u0] == 0 && s->w[u1] == 0;
+                            int b_all = s->w[j] == 0 && s->w[vj] == 0;
+                            int cnt_pin = !(a_all || b_all);
+                            if (b_has_pin) {
+                                v3 em = mul(add(sub(p_j, p_jo), sub(p_vj, p_vjo)), (real)0.5f);
+                                real a = 1 - ss, b = ss, w0 = s->w[u0], w1 = s->w[u1];
+                                real den = a * a * w0 + b * b * w1;
+                                if (den > EPS) {
+                                    v3 pb = add(mul(n0, pen), mul(em, (real)0.5f));
+                                    if (w0 > 0) add_ccd_ee(&s->acc[u0], mul(pb, rmin((a * w0) / den, 2)), cnt_pin);
+                                    if (w1 > 0) add_ccd_ee(&s->acc[u1], mul(pb, rmin((b * w1) / den, 2)), cnt_pin);
+                                }
+                            } else if (a_has_pin) {
+                                v3 em = mul(add(sub(p_u0, p_u0o), sub(p_u1, p_u1o)), (real)0.5f);
+                                real a = 1 - ts, b = ts, w0 = s->w[j], w1 = s->w[vj];
+                                real den = a * a * w0 + b * b * w1;
+                                if (den > EPS) {
+                                    v3 pb = add(mul(neg(n0), pen), mul(em, (real)0.5f));
+                                    if (w0 > 0) add_ccd_ee(&s->acc[j], mul(pb, rmin((a * w0) / den, 2)), cnt_pin);
+                                    if (w1 > 0) add_ccd_ee(&s->acc[vj], mul(pb, rmin((b * w1) / den, 2)), cnt_pin);
+                                }
+                            } else {
+                                real wu0 = s->w[u0], wu1 = s->w[u1], wj = s->w[j], wvj = s->w[vj];
+                                real we1 = (1 - ss) * (1 - ss) * wu0 + ss * ss * wu1;
+                                real we2 = (1 - ts) * (1 - ts) * wj + ts * ts * wvj;
+                                real wt = we1 + we2;
+                                if (wt > EPS) {
+                                    v3 tot = mul(n0, pen);
+                                    if (wu0 > 0) add_ccd_ee(&s->acc[u0], mul(tot, (1 - ss) * wu0 / wt), 1);
+                                    if (wu1 > 0) add_ccd_ee(&s->acc[u1], mul(tot, ss * wu1 / wt), 1);
+                                    if (wj > 0) add_ccd_ee(&s->acc[j], mul(neg(tot), (1 - ts) * wj / wt), 1);
+                                    if (wvj > 0) add_ccd_ee(&s->acc[vj], mul(neg(tot), ts * wvj / wt), 1);
+                                }
+                            }
+                            continue;
+                        }
+                    }
+                }
+                /* discrete proximity */
+                st2 st; v3 delta_ee; real dsq, eff;
+                if (has_cached) { st = st1c; delta_ee = d1c; dsq = dsqc; eff = effc; }
+                else {
+                    st = closest_segs(p_u0, p_u1, p_j, p_vj);
+                    v3 pt1 = add(p_u0, mul(sub(p_u1, p_u0), st.s));
+                    v3 pt2 = add(p_j, mul(sub(p_vj, p_j), st.t));
+                    delta_ee = sub(pt1, pt2);
+                    dsq = dot(delta_ee, delta_ee);
+                    eff = rmix(th_u0, th_u1, st.s) + rmix(th_j, th_vj, st.t);
+                }
+                if (dsq >= eff * eff || dsq <= EPS * EPS) continue;
+                if (s->exclude_neighbors) {
+                    v3 d0 = sub(p_u0, p_j), d1 = sub(p_u0, p_vj), d2 = sub(p_u1, p_j), d3 = sub(p_u1, p_vj);
+                    if (near_folded_sq(m, u0, j, dot(d0, d0)) || near_folded_sq(m, u0, vj, dot(d1, d1)) ||
+                        near_folded_sq(m, u1, j, dot(d2, d2)) || near_folded_sq(m, u1, vj, dot(d3, d3))) continue;
+                }
+                s->stats[4]++;
+                real dist = RSQRT(dsq);
+                v3 n = dvs(delta_ee, dist);
+                real pen = eff - dist;
+                real sp = st.s, tp = st.t;
+                int b_has_pin = s->w[j] <= 0 || s->w[vj] <= 0;
+                if (a_has_pin && b_has_pin) continue;
+                if (b_has_pin) {
+                    v3 em = mul(add(sub(p_j, p_jo), sub(p_vj, p_vjo)), (real)0.5f);
+                    real a = 1 - sp, b = sp, w0 = s->w[u0], w1 = s->w[u1];
+                    real den = a * a * w0 + b * b * w1;
+                    if (den > EPS) {
+                        v3 pb = add(mul(n, pen), mul(em, (real)0.5f));
+                        if (w0 > 0) add_pin_disp(&s->acc[u0], mul(pb, rmin((a * w0) / den, 2)));
+                        if (w1 > 0) add_pin_disp(&s->acc[u1], mul(pb, rmin((b * w1) / den, 2)));
+                    }
+                } else if (a_has_pin) {
+                    v3 em = mul(add(sub(p_u0, p_u0o), sub(p_u1, p_u1o)), (real)0.5f);
+                    real a = 1 - tp, b = tp, w0 = s->w[j], w1 = s->w[vj];
+                    real den = a * a * w0 + b * b * w1;
+                    if (den > EPS) {
+                        v3 pb = add(mul(neg(n), pen), mul(em, (real)0.5f));
+                        if (w0 > 0) add_pin_disp(&s->acc[j], mul(pb, rmin((a * w0) / den, 2)));
+                        if (w1 > 0) add_pin_disp(&s->acc[vj], mul(pb, rmin((b * w1) / den, 2)));
+                    }
+                } else {
+                    real wu0 = s->w[u0], wu1 = s->w[u1], wj = s->w[j], wvj = s->w[vj];
+                    real we1 = (1 - sp) * (1 - sp) * wu0 + sp * sp * wu1;
+                    real we2 = (1 - tp) * (1 - tp) * wj + tp * tp * wvj;
+                    real wt = we1 + we2;
+                    if (wt > EPS) {
+                        v3 td = mul(n, pen * (real)0.6f);
+                        if (wu0 > 0) add_disp_ee(&s->acc[u0], mul(td, (1 - sp) * wu0 / wt));
+                        if (wu1 > 0) add_disp_ee(&s->acc[u1], mul(td, sp * wu1 / wt));
+                        if (wj > 0) add_disp_ee(&s->acc[j], mul(neg(td), (1 - tp) * wj / wt));
+                        if (wvj > 0) add_disp_ee(&s->acc[vj], mul(neg(td), tp * wvj / wt));
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* self_collision_apply.wgsl:57-177 */
+static void k_sc_apply(orc_sim *s) {
+    const real EPS = (real)1e-7f;
+    const real INV = (real)(1.0f / 1000000.0f);
+    for (uint32_t i = 0; i < s->nv; i++) {
+        accum_t *a = &s->acc[i];
+        int pinned = s->w[i] <= 0;
+        v3 fin = mk(0, 0, 0);
+        if (a->count > 0) {
+            if (!pinned) {
+                v3 tot = mul(mk((real)a->dx, (real)a->dy, (real)a->dz), INV);
+                v3 eff = dvs(tot, rmax((real)1, RSQRT((real)a->count)));
+                if (s->enable_relief) eff = mul(eff, s->relief_factor);
+                real pl = len3(eff);
+                real ratio = s->max_disp_ratio > 0 ? s->max_disp_ratio : (real)0.5f;
+                real ms = rmin((real)s->m->local_len[i] * ratio, (real)0.10f);
+                if (pl > ms && pl > EPS) eff = mul(dvs(eff, pl), ms);
+                fin = add(fin, eff);
+            }
+        }
+        v3 avg_pin = mk(0, 0, 0);
+        if (a->pcount > 0 && !pinned)
+            avg_pin = dvs(mul(mk((real)a->pdx, (real)a->pdy, (real)a->pdz), INV), (real)a->pcount);
+        if (!pinned) {
+            v3 cp = mul(mk((real)a->cpx, (real)a->cpy, (real)a->cpz), INV);
+            v3 cn = mul(mk((real)a->cnx, (real)a->cny, (real)a->cnz), INV);
+            v3 env = add(cp, cn);
+            if (a->ee_count >= 4) {
+                real sc = rmax((real)0.5f, RSQRT((real)3.0f / (real)a->ee_count));
+                env = mul(env, sc);
+            }
+            v3 z = mk(0, 0, 0);
+            v3 hp = vmax(vmax(env, z), vmax(avg_pin, z));
+            v3 hn = vmin(vmin(env, z), vmin(avg_pin, z));
+            fin = add(fin, add(hp, hn));
+        }
+        if (len3(fin) > EPS) s->p[i] = add(s->p[i], fin);
+        memset(a, 0, sizeof(*a));
+    }
+}
+
+static void self_collision_pass(orc_sim *s) {
+    const orc_mesh *m = s->m;
+    hash_build(s->p, s->nv, s->cell_size, s->table_size, s->hkeys, s->hstarts, s->hsorted);
+    if (m->ne > 0) {
+        v3 *mid = (v3 *)malloc(m->ne * sizeof(v3));
+        for (uint32_t e = 0; e < m->ne; e++) mid[e] = mul(add(s->p[m->e_v0[e]], s->p[m->e_v1[e]]), (real)0.5f);
+        hash_build(mid, m->ne, s->cell_size, s->table_size, s->ekeys, s->estarts, s->esorted);
+        free(mid);
+    }
+    k_sc_solve(s);
+    if (m->ne > 0) k_sc_solve_ee(s);
+    k_sc_apply(s);
+}
+
+/* one frame: dispatch.rs:7-354 with adaptive substeps off, decoupled colliders,
+ * self-collision mode 0 / algorithm 0, no pins, no sewing */
+static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
+    if (s->nv == 0) return;
+    real dt_sub = dt / (real)substeps;
+    const orc_mesh *m = s->m;
+    for (uint32_t sub = 0; sub < substeps; sub++) {
+        memset(s->lam_d, 0, (m->nd ? m->nd : 1) * sizeof(real));
+        memset(s->lam_b, 0, (m->nb ? m->nb : 1) * sizeof(real));
+        k_predict(s, dt_sub);
+        for (uint32_t it = 0; it < s->iterations; it++) {
+            for (uint32_t c = 0; c < m->nd_col; c++)
+                for (uint32_t k = m->d_col_off[c]; k < m->d_col_off[c] + m->d_col_cnt[c]; k++) k_distance_one(s, k, dt_sub);
+            for (uint32_t c = 0; c < m->nb_col; c++)
+                for (uint32_t k = m->b_col_off[c]; k < m->b_col_off[c] + m->b_col_cnt[c]; k++) k_bending_one(s, k, dt_sub);
+        }
+        if (s->ncols > 0) k_collision(s, dt_sub);
+        if (s->self_collision) self_collision_pass(s);
+        k_update_vel(s, dt_sub);
+    }
+}
+
+/* ClothSimulator.step (lib.rs:1770-1786 -> dispatch.rs:610-671); returns effective substeps */
+uint32_t orc_sim_step(orc_sim *s, float dt, uint32_t substeps) {
+    uint32_t S = substeps > 0 ? substeps : 20;
+    encode_steps(s, (real)dt, S);
+    return S;
+}
+void orc_sim_step_single_substep(orc_sim *s, float dt_sub) { encode_steps(s, (real)dt_sub, 1); }
+
+/* Pieces for tests: run only the parts of one substep up to (and excluding) the
+ * velocity update, so that the constraint residual can be evaluated. */
+void orc_sim_substep_parts(orc_sim *s, float dt_sub, int do_collide, int do_self, int do_update) {
+    const orc_mesh *m = s->m;
+    real dt = (real)dt_sub;
+    memset(s->lam_d, 0, (m->nd ? m->nd : 1) * sizeof(real));
+    memset(s->lam_b, 0, (m->nb ? m->nb : 1) * sizeof(real));
+    k_predict(s, dt);
+    for (uint32_t it = 0; it < s->iterations; it++) {
+        for (uint32_t c = 0; c < m->nd_col; c++)
+            for (uint32_t k = m->d_col_off[c]; k < m->d_col_off[c] + m->d_col_cnt[c]; k++) k_distance_one(s, k, dt);
+        for (uint32_t c = 0; c < m->nb_col; c++)
+            for (uint32_t k = m->b_col_off[c]; k < m->b_col_off[c] + m->b_col_cnt[c]; k++) k_bending_one(s, k, dt);
+    }
+    if (do_collide && s->ncols > 0) k_collision(s, dt);
+    if (do_self && s->self_collision) self_collision_pass(s);
+    if (do_update) k_update_vel(s, dt);
+}
+
+/* Relative XPBD residual r = ||h|| / ||L0|| with h_j = C_j(p) + alpha~_j lambda_j over
+ * distance, shear and bending constraints (constraints the projection skips are
+ * excluded).  Also returns max |h_j| / L0_j.  No reference counterpart. */
+void orc_sim_residual(const orc_sim *s, float dt_sub, double *out_rel, double *out_maxrel) {
+    const orc_mesh *m = s->m;
+    double dt = dt_sub > 1e-8f ? dt_sub : 1e-8;
+    double hh = 0, ll = 0, mx = 0;
+    for (uint32_t k = 0; k < m->nd; k++) {
+        uint32_t a = m->d_v0[k], b = m->d_v1[k];
+        double ws = (double)s->w[a] + (double)s->w[b];
+        double L = m->d_rest[k];
+        if (ws <= 1e-7) continue;
+        double dx = (double)s->p[a].x - s->p[b].x, dy = (double)s->p[a].y - s->p[b].y, dz = (double)s->p[a].z - s->p[b].z;
+        double d = sqrt(dx * dx + dy * dy + dz * dz);
+        if (d <= 1e-7) continue;
+        double c = d - L;
+        double comp = c >= 0 ? (double)s->d_tc[k] : (double)s->d_cc[k];
+        if (comp >= 1e9) continue;
+        double h = c + comp / (dt * dt) * (double)s->lam_d[k];
+        hh += h * h; ll += L * L;
+        if (L > 0 && fabs(h) / L > mx) mx = fabs(h) / L;
+    }
+    for (uint32_t k = 0; k < m->nb; k++) {
+        double comp = s->b_comp[k];
+        if (comp >= 1e9) continue;
+        uint32_t a = m->b_v2[k], b = m->b_v3[k];
+        double ws = (double)s->w[a] + (double)s->w[b];
+        if (ws <= 1e-7) continue;
+        double L = m->b_rest[k];
+        double dx = (double)s->p[a].x - s->p[b].x, dy = (double)s->p[a].y - s->p[b].y, dz = (double)s->p[a].z - s->p[b].z;
+        double d = sqrt(dx * dx + dy * dy + dz * dz);
+        if (d <= 1e-7) continue;
+        double h = (d - L) + comp / (dt * dt) * (double)s->lam_b[k];
+        hh += h * h; ll += L * L;
+        if (L > 0 && fabs(h) / L > mx) mx = fabs(h) / L;
+    }
+    *out_rel = ll > 0 ? sqrt(hh) / sqrt(ll) : 0;
+    *out_maxrel = mx;
+}
+
+int orc_real_bytes(void) { return (int)sizeof(real); }
+void orc_sim_stats(orc_sim *s, uint64_t *out, int clear) {
+    for (int i = 0; i < 6; i++) { out[i] = s->stats[i]; if (clear) s->stats[i] = 0; }
+}
