@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Substeps/sec of the XPBD cloth substep at a 512x512 grid on one MI355X.
+
+A step is one substep of the hot path over the resident cloth state:
+predict -> 64 iterations of graph-coloured distance/shear/bending sweeps ->
+sphere + capsule resolve -> velocity update (self-collision off: the
+512x512 workload configuration lists none).  Inputs are resident in HBM when the
+timed region starts; frames of 10 substeps are replayed from a captured HIP
+graph.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+BYTES_DIST = 84                 # algorithmic bytes per distance/shear projection
+BYTES_BEND = 80                 # per bending projection
+BYTES_VERTEX_PASS = 52          # predict or update_vel, per vertex
+
+
+def grid_512(n: int, spacing: float = 0.01, height: float = 0.5, seed: int = 0):
+    """n x n cloth (2 triangles per quad, fixed edge order), centred over the colliders,
+    seeded +-1 mm jitter, masses from areal density 0.15 kg/m^2."""
+    j, i = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    half = (n - 1) * spacing * 0.5
+    pos = np.stack([(i * spacing - half).ravel(), (j * spacing - half).ravel(), np.full(n * n, height)], axis=1)
+    rng = np.random.default_rng(seed)
+    pos = (pos + rng.uniform(-1e-3, 1e-3, pos.shape)).astype(np.float32)
+    idx = np.arange(n * n, dtype=np.uint32).reshape(n, n)
+    h = np.stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()], 1)
+    v = np.stack([idx[:-1, :].ravel(), idx[1:, :].ravel()], 1)
+    edges = np.concatenate([h, v]).astype(np.uint32)
+    v00, v10, v01, v11 = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel(), idx[1:, :-1].ravel(), idx[1:, 1:].ravel()
+    faces = np.empty((2 * v00.size, 3), np.uint32)
+    faces[0::2] = np.stack([v00, v10, v11], 1)
+    faces[1::2] = np.stack([v00, v11, v01], 1)
+    return pos, edges, faces
+
+
+def configure(sim, iters):
+    sim.set_gravity(0.0, 0.0, -9.81)
+    sim.set_compliances_raw(1e-8, 1e-6, 1e-5)   # distance, shear, bending
+    sim.set_enable_self_collision(False)
+    sim.add_sphere_collider([0.0, 0.0, 0.0], 0.3, 0.3, 0.0)
+    sim.add_capsule_collider([-0.3, 0.35, 0.1], [0.3, 0.35, 0.1], 0.1, 0.3, 0.0)
+    sim.set_solver_iterations(iters)
+
+
+def run_substeps(sim, count, dt_frame, per_frame):
+    done = 0
+    while done < count:
+        s = min(per_frame, count - done)
+        sim.step(dt_frame * s / per_frame, s, None)
+        done += s
+
+
+def cpu_baseline(pos, edges, faces, inv, iters, dt_sub, budget_s):
+    """The fp32 oracle (scalar C restatement, one core) on the same 512x512 workload:
+    whole substeps until about `budget_s` seconds have passed."""
+    from oracle import oracle as O
+    sim = O.OracleSim(pos, edges, faces, inv, thickness=0.005, stiffness=1000.0, bending_stiffness=10.0)
+    configure(sim, iters)
+    sim.step_single_substep(dt_sub)  # warm caches
+    n, t0 = 0, time.perf_counter()
+    while True:
+        sim.step_single_substep(dt_sub)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 20:
+            break
+    return n / el, n, el
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--iters", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the XPBD substep of one cloth "
+              "does not shard across GPUs in this build. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from taremincloth_amd import taremin_cloth_core as T
+
+    n = args.grid
+    pos, edges, faces = grid_512(n)
+    inv = T.compute_combined_inv_masses(pos, faces, areal_density=0.15)
+    sim = T.ClothSimulator(pos, edges, faces, inv, thickness=0.005, stiffness=1000.0, bending_stiffness=10.0)
+    configure(sim, args.iters)
+    per_frame = 10
+    dt_frame = 1.0 / 60.0
+    dt_sub = dt_frame / per_frame
+    nd, nb = sim.num_distance_constraints(), sim.num_bending_constraints()
+
+    run_substeps(sim, args.warmup, dt_frame, per_frame)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    run_substeps(sim, args.steps, dt_frame, per_frame)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    sps = args.steps / elapsed
+
+    # residual of the plain solve after the last sweep of one more substep
+    sim.substep_parts(dt_sub, collide=False, self_collide=False, update=False)
+    rel, max_rel = sim.residual(dt_sub)
+
+    # per-kernel durations from HIP events on the simulator's stream
+    kt = sim.time_kernels(dt_sub, reps=3)
+    dist_launch_ms = kt["distance"] / max(kt["distance_launches"], 1.0)
+    dist_bytes_per_launch = nd * BYTES_DIST * args.iters / max(kt["distance_launches"], 1.0)
+    achieved = dist_bytes_per_launch / (dist_launch_ms * 1e-3) / 1e9
+    sweep_bytes = (nd * BYTES_DIST + nb * BYTES_BEND) * args.iters
+    sweep_ms = kt["distance"] + kt["bending"]
+    substep_bytes = sweep_bytes + 2 * BYTES_VERTEX_PASS * len(pos)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        v, k, el = cpu_baseline(pos, edges, faces, inv, args.iters, dt_sub, args.cpu_seconds)
+        cpu = {"value": v, "unit": "substeps/s", "cores": 1, "kind": "port",
+               "sample": f"fp32 oracle (scalar C restatement of the reference substep, one thread) on the same "
+                         f"{n}x{n} workload at {args.iters} iterations: {k} whole substeps in {el:.1f} s"}
+
+    line = {
+        "metric": "substeps/sec at 512² grid: learned warm-start vs plain XPBD at residual ≤1e-4",
+        "value": sps,
+        "unit": "substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain XPBD, {n}x{n} grid ({len(pos)} verts, {nd + nb} constraints), {args.iters} "
+                        f"iterations/substep, sphere + capsule colliders, self-collision off; the learned "
+                        f"warm start is not built yet, so value is the plain-XPBD side of the metric",
+            "grid": n, "vertices": len(pos), "distance_constraints": nd, "bending_constraints": nb,
+            "iterations": args.iters, "substeps_per_frame": per_frame, "dt_substep": dt_sub,
+            "relative_residual": rel, "max_relative_residual": max_rel,
+            "substep_ms_breakdown": {k: round(v, 5) for k, v in kt.items()},
+            "sweep_GBps_algorithmic": sweep_bytes / (sweep_ms * 1e-3) / 1e9,
+            "substep_GBps_algorithmic": substep_bytes / (elapsed / args.steps) / 1e9,
+            "constraint_projections_per_s": (nd + nb) * args.iters * sps,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_distance (one colour of distance/shear constraints)",
+            "bytes_per_launch": dist_bytes_per_launch, "launch_ms": dist_launch_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,173 @@
+/*
+ * tc_sim.h -- C ABI of the MI355X XPBD cloth substep library (libtcloth_hip.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, no framework types.
+ * Every entry point returns an int status (0 = TC_OK) unless stated; on error
+ * a thread-local message is available from tc_last_error().
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the reference repository root).  The PyO3 class
+ * `taremin_cloth_core.ClothSimulator` (crates/cloth_py/src/lib.rs:1549-3258)
+ * wraps GpuClothSimulator (crates/cloth_core/src/simulation/mod.rs:57-3223);
+ * this header is the boundary at that level.
+ *
+ * Threading: a handle is used by one thread at a time.  Each simulator owns
+ * its device memory and one HIP stream.  Host inputs are copied at the call
+ * (the reference copies numpy inputs into owned Vecs, lib.rs:1584-1712);
+ * outputs go to caller-owned buffers.
+ */
+#ifndef TC_SIM_H
+#define TC_SIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TC_OK = 0,
+    TC_ERR_VALUE = 1,    /* bad shape / index / argument  (PyValueError in the reference) */
+    TC_ERR_RUNTIME = 2,  /* device or allocation failure  (PyRuntimeError in the reference) */
+    TC_ERR_UNSUPPORTED = 3
+};
+
+typedef struct tc_sim tc_sim;
+
+/* Constructor keyword arguments of ClothSimulator::new (lib.rs:1556-1582).
+ * compression_stiffness / shear_stiffness < 0 mean "not given"
+ * (defaults: stiffness and 0.5 * stiffness, lib.rs:1720-1722). */
+typedef struct {
+    float thickness;              /* default 0.005 */
+    uint32_t layer_id;            /* default 0 */
+    float stiffness;              /* default 1000 */
+    float bending_stiffness;      /* default 10 */
+    float compression_stiffness;  /* < 0: = stiffness */
+    float shear_stiffness;        /* < 0: = 0.5 * stiffness */
+    uint32_t workgroup_size;      /* 0: autotuned; otherwise 64/128/256 */
+    uint32_t solver_mode;         /* 0 = graph colouring (only mode supported) */
+} tc_sim_params;
+
+/* Fills the reference defaults. */
+void tc_sim_params_default(tc_sim_params *p);
+
+/* ClothSimulator::new, lib.rs:1557-1766 -> ClothMesh::from_raw_with_opts (mesh.rs:407)
+ * + GpuClothSimulator::with_options (simulation/mod.rs:363).
+ * faces / inv_masses / thicknesses / layer_ids may be NULL. */
+int tc_sim_create(const float *positions, uint32_t num_vertices,
+                  const uint32_t *edges, uint32_t num_edges,
+                  const uint32_t *faces, uint32_t num_faces,
+                  const float *inv_masses, const float *thicknesses, const uint32_t *layer_ids,
+                  const tc_sim_params *params, tc_sim **out);
+void tc_sim_destroy(tc_sim *sim);
+
+/* ClothSimulator.step(dt, substeps, solver_iterations), lib.rs:1768-1786.
+ * substeps 0 -> 20 (base_substeps); iterations 0 -> keep current.
+ * Asynchronous on the simulator's stream; *effective_substeps may be NULL. */
+int tc_sim_step(tc_sim *sim, float dt, uint32_t substeps, uint32_t iterations, uint32_t *effective_substeps);
+/* ClothSimulator.step_single_substep(dt_sub), lib.rs:1808-1813 */
+int tc_sim_step_single_substep(tc_sim *sim, float dt_sub);
+/* ClothSimulator.get_positions(out), lib.rs:2110-2121 (blocking, n_floats >= 3N) */
+int tc_sim_get_positions(tc_sim *sim, float *out, size_t n_floats);
+/* ClothSimulator.set_positions_and_velocities(pos, vel=None), lib.rs:2124-2153 */
+int tc_sim_set_positions_velocities(tc_sim *sim, const float *positions, const float *velocities, uint32_t n);
+/* ClothSimulator.reset(), dispatch.rs:1411-1427 */
+int tc_sim_reset(tc_sim *sim);
+
+/* Colliders: add_sphere_collider / add_capsule_collider / add_plane_collider /
+ * clear_colliders, lib.rs:2157-2172 -> simulation/mod.rs:1571-1629 */
+int tc_sim_add_sphere(tc_sim *sim, const float center[3], float radius, float friction, float restitution);
+int tc_sim_add_capsule(tc_sim *sim, const float a[3], const float b[3], float radius, float friction,
+                       float restitution);
+int tc_sim_add_plane(tc_sim *sim, const float point[3], const float normal[3], float friction, float restitution);
+int tc_sim_clear_colliders(tc_sim *sim);
+
+/* set_mesh_collider_triangles(tris f32[T,3,3], friction, thickness, restitution, single_sided, attributes=None),
+ * lib.rs:2175-2233 -> upload_colliders (simulation/mod.rs:2582-2765). attrs: NULL or T*4 floats
+ * (friction, thickness, restitution, flags) overriding the scalars per triangle. */
+int tc_sim_set_mesh_triangles(tc_sim *sim, const float *tris, uint32_t num_tris, const float *attrs,
+                              float friction, float thickness, float restitution, int single_sided);
+
+/* Parameter setters, lib.rs:1830-2000 -> simulation/mod.rs:1029-1447 */
+int tc_sim_set_gravity(tc_sim *sim, float gx, float gy, float gz);
+int tc_sim_set_damping(tc_sim *sim, float damping);
+int tc_sim_set_damping_all(tc_sim *sim, float tension, float compression, float shear, float bending);
+int tc_sim_set_solver_iterations(tc_sim *sim, uint32_t iterations);
+int tc_sim_set_enable_self_collision(tc_sim *sim, int enable);
+/* set_self_collision_options, simulation/mod.rs:920-953. Normal untangling only acts on multi-layer
+ * meshes (several islands or layer ids); it is not part of this build, so stepping a multi-layer mesh
+ * with self-collision and untangling enabled returns TC_ERR_UNSUPPORTED. */
+int tc_sim_set_self_collision_options(tc_sim *sim, float relief_factor, float max_displacement_ratio,
+                                      int exclude_neighbors, int enable_normal_untangling, uint32_t max_iterations);
+/* set_stiffness_all, simulation/mod.rs:1106-1168 */
+int tc_sim_set_stiffness_all(tc_sim *sim, float tension, float compression, float shear, float bending);
+
+/* Extensions with no reference counterpart:
+ * raw XPBD compliances per constraint kind (the reference maps k >= 5000 N/m to 0),
+ * and an explicit self-collision hash cell size (the reference clamps cell >= 1 cm). */
+int tc_sim_set_compliances_raw(tc_sim *sim, float stretch, float shear, float bend);
+int tc_sim_set_hash_cell_size(tc_sim *sim, float cell_size);
+
+/* num_vertices / num_distance_constraints / num_bending_constraints getters, lib.rs:2928-2950 */
+uint32_t tc_sim_num_vertices(const tc_sim *sim);
+uint32_t tc_sim_num_distance_constraints(const tc_sim *sim);
+uint32_t tc_sim_num_bending_constraints(const tc_sim *sim);
+
+/* Introspection for parity checks (no reference counterpart; the reference
+ * exposes the same data only to its own Rust tests). `which` selects:
+ *  0 dist v0 (u32, colour order)   1 dist v1          2 dist colour offsets   3 dist colour counts
+ *  4 bend v2                        5 bend v3          6 bend colour offsets   7 bend colour counts
+ *  8 dist rest (f32)                9 bend rest (f32)  10 vertex hash keys of the last build (u32)
+ * 11 vertex hash cell starts (T+1)  12 vertex hash sorted indices            13 predicted positions p (f32 xyz)
+ * 14 velocities (f32 xyz)          15 distance lambdas (f32)                16 bending lambdas (f32)
+ * 17 scalars [cell_size(f32 bits), table_size, n_virtual, num_edges]       18 Morton-sorted edges (u32 pairs)
+ * 19 edge hash keys                 20 edge hash cell starts                 21 edge hash sorted indices
+ * *count receives the element count; out may be NULL to query it. */
+int tc_sim_get_array(tc_sim *sim, int which, void *out, size_t capacity, size_t *count);
+/* Rebuild the spatial hashes from the current predicted positions (parity checks of the hash). */
+int tc_sim_build_hashes(tc_sim *sim);
+/* Relative XPBD residual ||C + alpha~ lambda|| / ||L0|| after the last projection sweep of the
+ * last substep (project definition, no reference counterpart), and max |h_j| / L0_j. */
+int tc_sim_residual(tc_sim *sim, float dt_sub, double *rel, double *max_rel);
+/* Run the parts of one substep up to the velocity update (residual / warm-start studies). */
+int tc_sim_substep_parts(tc_sim *sim, float dt_sub, int collide, int self_collide, int update_velocity);
+/* Times `reps` substeps launched kernel by kernel (no graph) with HIP events recorded on the
+ * simulator's stream and writes per-substep averages (milliseconds) to out[10]:
+ * [0] predict  [1] all distance launches  [2] distance launches per substep  [3] all bending launches
+ * [4] bending launches per substep  [5] collider resolve  [6] self-collision (hash + solve + apply)
+ * [7] update_vel  [8] wall time of the substep between its first and last event  [9] reserved (0).
+ * No reference counterpart (the reference's wgpu-profiler scopes, simulation/profile.rs). */
+int tc_sim_time_kernels(tc_sim *sim, float dt_sub, uint32_t reps, double *out);
+/* Block until all work on the simulator's stream has finished. */
+int tc_sim_synchronize(tc_sim *sim);
+
+/* Host-only mesh build (no device work; usable without a GPU): builds the constraints, colouring
+ * and self-collision topology exactly as tc_sim_create does and returns array `which`:
+ * 0-9, 17, 18 as in tc_sim_get_array, and
+ * 22 two-hop offsets   23 two-hop indices   24 two-hop rest lengths (f32)  25 local edge lengths (f32)
+ * 26 star offsets      27 star pairs (u32 pairs)  28 open-boundary edge bits  29 island ids
+ * 30 input edge -> distance constraint (colour order, UINT32_MAX if invalid)   31 distance type (0 stretch, 1 shear)
+ * 32 tension compliance (f32)   33 compression compliance (f32)   34 bending compliance (f32)
+ * 35 adjacency offsets          36 adjacency indices               37 layer ids */
+int tc_mesh_build_arrays(const float *positions, uint32_t num_vertices, const uint32_t *edges, uint32_t num_edges,
+                         const uint32_t *faces, uint32_t num_faces, const float *inv_masses,
+                         const float *thicknesses, const tc_sim_params *params, int which, void *out,
+                         size_t capacity, size_t *count);
+
+/* Module functions */
+/* color_edge_pairs(n, edges) -> colours, lib.rs:636-648 -> coloring.rs:180 */
+int tc_color_edge_pairs(uint32_t num_vertices, const uint32_t *pairs, uint32_t num_pairs, uint32_t *colors_out);
+/* compute_combined_inv_masses (faces, pins; no strands), lib.rs:552 -> mesh.rs:237-312 */
+int tc_compute_inv_masses(const float *positions, uint32_t num_vertices, const uint32_t *faces, uint32_t num_faces,
+                          const float *pin_weights, float areal_density, float *out);
+
+/* Last error message of the calling thread ("" if none). */
+const char *tc_last_error(void);
+/* Library version string. */
+const char *tc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TC_SIM_H */

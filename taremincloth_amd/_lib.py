@@ -1,0 +1,112 @@
+"""ctypes binding of libtcloth_hip.so (the C ABI declared in include/tc_sim.h).
+
+Loading fails loudly: there is no CPU fallback.  If the library has not been
+built, ``load()`` raises with the build command to run.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libtcloth_hip.so")
+
+# Every symbol include/tc_sim.h declares (checked by the test suite).
+EXPORTS = [
+    "tc_sim_params_default", "tc_sim_create", "tc_sim_destroy", "tc_sim_step", "tc_sim_step_single_substep",
+    "tc_sim_get_positions", "tc_sim_set_positions_velocities", "tc_sim_reset", "tc_sim_add_sphere",
+    "tc_sim_add_capsule", "tc_sim_add_plane", "tc_sim_clear_colliders", "tc_sim_set_mesh_triangles",
+    "tc_sim_set_gravity", "tc_sim_set_damping", "tc_sim_set_damping_all", "tc_sim_set_solver_iterations",
+    "tc_sim_set_enable_self_collision", "tc_sim_set_self_collision_options", "tc_sim_set_stiffness_all",
+    "tc_sim_set_compliances_raw", "tc_sim_set_hash_cell_size", "tc_sim_num_vertices",
+    "tc_sim_num_distance_constraints", "tc_sim_num_bending_constraints", "tc_sim_get_array",
+    "tc_sim_build_hashes", "tc_sim_residual", "tc_sim_substep_parts", "tc_sim_time_kernels", "tc_sim_synchronize",
+    "tc_mesh_build_arrays", "tc_color_edge_pairs", "tc_compute_inv_masses", "tc_last_error", "tc_version",
+]
+
+TC_OK, TC_ERR_VALUE, TC_ERR_RUNTIME, TC_ERR_UNSUPPORTED = 0, 1, 2, 3
+
+
+class TcParams(C.Structure):
+    _fields_ = [
+        ("thickness", C.c_float),
+        ("layer_id", C.c_uint32),
+        ("stiffness", C.c_float),
+        ("bending_stiffness", C.c_float),
+        ("compression_stiffness", C.c_float),
+        ("shear_stiffness", C.c_float),
+        ("workgroup_size", C.c_uint32),
+        ("solver_mode", C.c_uint32),
+    ]
+
+
+_LIB: C.CDLL | None = None
+_vp, _u32, _f32, _i = C.c_void_p, C.c_uint32, C.c_float, C.c_int
+
+
+def load() -> C.CDLL:
+    """Load libtcloth_hip.so and declare every entry point's signature."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m taremincloth_amd.build` "
+            "(there is no CPU fallback for the XPBD substep)")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "tc_sim_params_default": (None, [C.POINTER(TcParams)]),
+        "tc_sim_create": (_i, [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp, C.POINTER(TcParams), C.POINTER(_vp)]),
+        "tc_sim_destroy": (None, [_vp]),
+        "tc_sim_step": (_i, [_vp, _f32, _u32, _u32, C.POINTER(_u32)]),
+        "tc_sim_step_single_substep": (_i, [_vp, _f32]),
+        "tc_sim_get_positions": (_i, [_vp, _vp, C.c_size_t]),
+        "tc_sim_set_positions_velocities": (_i, [_vp, _vp, _vp, _u32]),
+        "tc_sim_reset": (_i, [_vp]),
+        "tc_sim_add_sphere": (_i, [_vp, _vp, _f32, _f32, _f32]),
+        "tc_sim_add_capsule": (_i, [_vp, _vp, _vp, _f32, _f32, _f32]),
+        "tc_sim_add_plane": (_i, [_vp, _vp, _vp, _f32, _f32]),
+        "tc_sim_clear_colliders": (_i, [_vp]),
+        "tc_sim_set_mesh_triangles": (_i, [_vp, _vp, _u32, _vp, _f32, _f32, _f32, _i]),
+        "tc_sim_set_gravity": (_i, [_vp, _f32, _f32, _f32]),
+        "tc_sim_set_damping": (_i, [_vp, _f32]),
+        "tc_sim_set_damping_all": (_i, [_vp, _f32, _f32, _f32, _f32]),
+        "tc_sim_set_solver_iterations": (_i, [_vp, _u32]),
+        "tc_sim_set_enable_self_collision": (_i, [_vp, _i]),
+        "tc_sim_set_self_collision_options": (_i, [_vp, _f32, _f32, _i, _i, _u32]),
+        "tc_sim_set_stiffness_all": (_i, [_vp, _f32, _f32, _f32, _f32]),
+        "tc_sim_set_compliances_raw": (_i, [_vp, _f32, _f32, _f32]),
+        "tc_sim_set_hash_cell_size": (_i, [_vp, _f32]),
+        "tc_sim_num_vertices": (_u32, [_vp]),
+        "tc_sim_num_distance_constraints": (_u32, [_vp]),
+        "tc_sim_num_bending_constraints": (_u32, [_vp]),
+        "tc_sim_get_array": (_i, [_vp, _i, _vp, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "tc_sim_build_hashes": (_i, [_vp]),
+        "tc_sim_residual": (_i, [_vp, _f32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "tc_sim_substep_parts": (_i, [_vp, _f32, _i, _i, _i]),
+        "tc_sim_time_kernels": (_i, [_vp, _f32, _u32, C.POINTER(C.c_double)]),
+        "tc_sim_synchronize": (_i, [_vp]),
+        "tc_mesh_build_arrays": (_i, [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, C.POINTER(TcParams), _i, _vp,
+                                      C.c_size_t, C.POINTER(C.c_size_t)]),
+        "tc_color_edge_pairs": (_i, [_u32, _vp, _u32, _vp]),
+        "tc_compute_inv_masses": (_i, [_vp, _u32, _vp, _u32, _vp, _f32, _vp]),
+        "tc_last_error": (C.c_char_p, []),
+        "tc_version": (C.c_char_p, []),
+    }
+    for name in EXPORTS:
+        fn = getattr(L, name)  # AttributeError if the symbol is missing
+        fn.restype, fn.argtypes = sig[name]
+    _LIB = L
+    return L
+
+
+def check(status: int) -> None:
+    """Raise the reference's exception type for a non-zero status."""
+    if status == TC_OK:
+        return
+    msg = (load().tc_last_error() or b"").decode(errors="replace")
+    if status == TC_ERR_VALUE:
+        raise ValueError(msg)
+    if status == TC_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise RuntimeError(msg)

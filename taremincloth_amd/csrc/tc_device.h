@@ -1,0 +1,81 @@
+// Device-side helpers and layouts shared by the XPBD kernels (gfx950, wave64).
+//
+// Vertex state lives in HBM as three separate float4 streams (structure of
+// arrays, one 16-byte vector per vertex and stream):
+//   X[i] = (x, y, z, thickness)     committed position + contact radius
+//   P[i] = (px, py, pz, inv_mass)   predicted / corrected position + weight
+//   V[i] = (vx, vy, vz, 0)          velocity
+// A constraint gathers P of its two vertices with one 16-byte load each, so
+// position and inverse mass arrive in the same cache line.
+//
+// All arithmetic follows the operation order of the reference WGSL shaders
+// (crates/cloth_core/src/shaders/*.wgsl); the library is compiled with
+// -ffp-contract=off so that no FMA contraction changes the rounding.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tc {
+
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 xyz(const float4 &v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+// WGSL mix(a, b, t) = a * (1 - t) + b * t
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ f3 mix3(f3 a, f3 b, float t) { return {mixf(a.x, b.x, t), mixf(a.y, b.y, t), mixf(a.z, b.z, t)}; }
+__device__ __forceinline__ f3 min3(f3 a, f3 b) { return {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
+__device__ __forceinline__ f3 max3(f3 a, f3 b) { return {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
+
+// WGSL i32(f32) is a saturating truncation.
+__device__ __forceinline__ int32_t f2i_sat(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483520.0f) return 2147483647;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)x;
+}
+struct i3 { int32_t x, y, z; };
+__device__ __forceinline__ i3 cell_of(f3 p, float cell) {
+    return {f2i_sat(floorf(p.x / cell)), f2i_sat(floorf(p.y / cell)), f2i_sat(floorf(p.z / cell))};
+}
+// spatial_grid_sort.wgsl:25-31
+__device__ __forceinline__ uint32_t hash_cell(i3 c, uint32_t table) {
+    const uint32_t n = ((uint32_t)c.x * 73856093u) ^ ((uint32_t)c.y * 19349663u) ^ ((uint32_t)c.z * 83492791u);
+    return n % table;
+}
+
+// Collider primitive (GpuCollider, mesh.rs:65-74): type 0 sphere, 1 capsule, 2 plane.
+struct Collider {
+    uint32_t type;
+    float friction, restitution, pad0;
+    float ax, ay, az, radius;
+    float bx, by, bz, pad1;
+};
+
+// Mesh collider triangle (GpuMeshTriangle, mesh.rs:78-87) + bounding sphere.
+struct MeshTri {
+    float p0x, p0y, p0z, friction;
+    float p1x, p1y, p1z, thickness;
+    float p2x, p2y, p2z, restitution;
+    uint32_t flags; float pad0, pad1, pad2;
+};
+
+// Self-collision accumulator (SelfCollisionAccum, simulation/types.rs:145-178)
+struct Accum {
+    int32_t dx, dy, dz; uint32_t count;
+    int32_t pdx, pdy, pdz; uint32_t pcount;
+    int32_t cpx, cpy, cpz, cnx, cny, cnz;
+    uint32_t untangle, ee_count;
+};
+
+}  // namespace tc

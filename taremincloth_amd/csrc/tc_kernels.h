@@ -1,0 +1,78 @@
+// Launch wrappers of the XPBD kernels (host side).  Every wrapper enqueues on
+// `stream` and returns the hipError_t of the launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tc_device.h"
+
+namespace tc {
+
+struct SubstepArgs {
+    float dt;          // substep length (dt / substeps)
+    float gx, gy, gz;  // gravity
+    float damping;     // air damping
+    float tension_damping, compression_damping, shear_damping, bending_damping;
+};
+
+// --- core substep (tc_xpbd.hip) ---
+hipError_t launch_predict(hipStream_t s, float4 *P, const float4 *X, float4 *V, uint32_t n, const SubstepArgs &a,
+                          uint32_t block);
+// One colour of distance constraints. first: lambda treated as zero (it is
+// cleared at every substep start, dispatch.rs:117-119).
+hipError_t launch_distance(hipStream_t s, float4 *P, const float4 *X, const uint4 *rec, const float *cc, float *lam,
+                           uint32_t off, uint32_t cnt, const SubstepArgs &a, bool first, bool split_compliance,
+                           uint32_t block);
+hipError_t launch_bending(hipStream_t s, float4 *P, const float4 *X, const uint4 *rec, float *lam, uint32_t off,
+                          uint32_t cnt, const SubstepArgs &a, bool first, uint32_t block);
+hipError_t launch_collision(hipStream_t s, float4 *P, float4 *X, uint32_t n, const Collider *cols, uint32_t ncols,
+                            const MeshTri *tris, const float4 *bounds, uint32_t ntris, int single_sided_recovery,
+                            float dt, uint32_t block);
+hipError_t launch_update_vel(hipStream_t s, const float4 *P, float4 *X, float4 *V, uint32_t n, float dt,
+                             uint32_t block);
+hipError_t launch_extract_positions(hipStream_t s, const float4 *X, float *out, uint32_t n);
+hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t value, size_t n);
+// Residual partial sums: out[0] += sum h^2, out[1] += sum L0^2, out[2] = max |h|/L0 (as double).
+hipError_t launch_residual(hipStream_t s, const float4 *P, const uint4 *drec, const float *dcc, const float *dlam,
+                           uint32_t nd, const uint4 *brec, const float *blam, uint32_t nb, float dt, bool split,
+                           double *out);
+
+// --- spatial hash + self collision (tc_selfcol.hip) ---
+struct HashBufs {
+    uint32_t *keys;     // n
+    uint32_t *counts;   // table (also used as scatter cursor)
+    uint32_t *starts;   // table + 1
+    uint32_t *sorted;   // n
+};
+// Vertex hash over P.xyz (spatial_grid_sort.wgsl); within a cell indices are ascending.
+hipError_t launch_hash_vertices(hipStream_t s, const float4 *P, uint32_t n, float cell, uint32_t table,
+                                const HashBufs &h);
+// Edge hash over the midpoint of P (spatial_grid_sort_edges.wgsl).
+hipError_t launch_hash_edges(hipStream_t s, const float4 *P, const uint2 *edges, uint32_t ne, float cell,
+                             uint32_t table, const HashBufs &h);
+
+struct SelfColArgs {
+    float cell;
+    uint32_t table;
+    uint32_t nv, ne;
+    float relief_factor, max_disp_ratio;
+    int enable_relief, exclude_neighbors;
+    uint32_t max_search_iterations;
+};
+struct SelfColTopo {
+    const float *local_len;
+    const uint32_t *th_off, *th_idx;
+    const float *th_rest;
+    const uint32_t *star_off;
+    const uint2 *star;
+    const uint2 *edges;
+    const uint32_t *edge_bbits;
+};
+hipError_t launch_sc_solve(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t,
+                           const HashBufs &vh, Accum *acc, const SelfColArgs &a, uint32_t block);
+hipError_t launch_sc_solve_ee(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t,
+                              const HashBufs &eh, Accum *acc, const SelfColArgs &a, uint32_t block);
+hipError_t launch_sc_apply(hipStream_t s, float4 *P, const float *local_len, Accum *acc, const SelfColArgs &a,
+                           uint32_t block);
+
+}  // namespace tc

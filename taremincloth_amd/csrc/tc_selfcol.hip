@@ -1,0 +1,745 @@
+// Spatial hash and DIRECT self-collision kernels for gfx950.
+//
+// Reference semantics (paths relative to the reference repository root):
+//   hash build        crates/cloth_core/src/shaders/spatial_grid_sort.wgsl:25-195
+//   edge hash build   crates/cloth_core/src/shaders/spatial_grid_sort_edges.wgsl:51-199
+//   V-V / V-T solve   crates/cloth_core/src/shaders/self_collision.wgsl:71-681
+//   E-E solve         crates/cloth_core/src/shaders/self_collision_ee_direct.wgsl:66-776
+//   apply             crates/cloth_core/src/shaders/self_collision_apply.wgsl:57-177
+//
+// Hash build: count (atomics) -> one-workgroup exclusive scan of the table
+// built from wave64 prefix sums (__shfl_up inside a wave, LDS across the 16
+// waves) -> scatter (atomic cursor) -> per-bucket sort, so every bucket lists
+// its elements in ascending index order.  The reference's scatter order inside
+// a bucket depends on atomic timing; the sorted order makes the hash tables,
+// and with them the E-E search cut-off, deterministic.
+//
+// The solve kernels accumulate displacements in 1e-6 m fixed point with
+// integer atomics, which makes the result independent of thread order.
+#include "tc_kernels.h"
+
+namespace tc {
+
+static inline uint32_t grid_for(size_t n, uint32_t block) { return uint32_t((n + block - 1) / block); }
+
+__global__ void __launch_bounds__(256) k_hash_count_vertices(const float4 *__restrict__ P, uint32_t n, float cell,
+                                                             uint32_t table, uint32_t *__restrict__ keys,
+                                                             uint32_t *__restrict__ counts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t h = hash_cell(cell_of(xyz(P[i]), cell), table);
+    keys[i] = h;
+    atomicAdd(&counts[h], 1u);
+}
+
+__global__ void __launch_bounds__(256) k_hash_count_edges(const float4 *__restrict__ P, const uint2 *__restrict__ e,
+                                                          uint32_t n, float cell, uint32_t table,
+                                                          uint32_t *__restrict__ keys, uint32_t *__restrict__ counts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint2 ed = e[i];
+    const f3 mid = (xyz(P[ed.x]) + xyz(P[ed.y])) * 0.5f;
+    const uint32_t h = hash_cell(cell_of(mid, cell), table);
+    keys[i] = h;
+    atomicAdd(&counts[h], 1u);
+}
+
+// Exclusive scan of counts[0..table) into starts[0..table], starts[table] = total;
+// counts are reset to zero for use as scatter cursors.  One workgroup of 1024.
+__global__ void __launch_bounds__(1024) k_scan_table(uint32_t *__restrict__ counts, uint32_t *__restrict__ starts,
+                                                     uint32_t table) {
+    __shared__ uint32_t wave_tot[16];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (table + 1023u) / 1024u;
+    const uint32_t beg = min(tid * per, table), end = min(beg + per, table);
+    uint32_t sum = 0;
+    for (uint32_t k = beg; k < end; ++k) sum += counts[k];
+    // inclusive prefix inside the wave
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    uint32_t incl = sum;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63u) wave_tot[wave] = incl;
+    __syncthreads();
+    uint32_t wave_base = 0;
+    for (uint32_t w = 0; w < wave; ++w) wave_base += wave_tot[w];
+    uint32_t run = wave_base + incl - sum;
+    for (uint32_t k = beg; k < end; ++k) {
+        const uint32_t c = counts[k];
+        starts[k] = run;
+        run += c;
+        counts[k] = 0u;
+    }
+    if (tid == 1023u) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < 16; ++w) total += wave_tot[w];
+        starts[table] = total;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_hash_scatter(const uint32_t *__restrict__ keys, uint32_t n,
+                                                      const uint32_t *__restrict__ starts,
+                                                      uint32_t *__restrict__ cursor, uint32_t *__restrict__ sorted) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t h = keys[i];
+    sorted[starts[h] + atomicAdd(&cursor[h], 1u)] = i;
+}
+
+__global__ void __launch_bounds__(256) k_bucket_sort(const uint32_t *__restrict__ starts, uint32_t table,
+                                                     uint32_t *__restrict__ sorted) {
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= table) return;
+    const uint32_t b = starts[h], e = starts[h + 1];
+    for (uint32_t k = b + 1; k < e; ++k) {
+        const uint32_t x = sorted[k];
+        uint32_t j = k;
+        while (j > b && sorted[j - 1] > x) { sorted[j] = sorted[j - 1]; --j; }
+        sorted[j] = x;
+    }
+}
+
+static hipError_t finish_hash(hipStream_t s, uint32_t n, uint32_t table, const HashBufs &h) {
+    hipLaunchKernelGGL(k_scan_table, dim3(1), dim3(1024), 0, s, h.counts, h.starts, table);
+    hipLaunchKernelGGL(k_hash_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, h.keys, n, h.starts, h.counts,
+                       h.sorted);
+    hipLaunchKernelGGL(k_bucket_sort, dim3(grid_for(table, 256)), dim3(256), 0, s, h.starts, table, h.sorted);
+    return hipGetLastError();
+}
+
+hipError_t launch_hash_vertices(hipStream_t s, const float4 *P, uint32_t n, float cell, uint32_t table,
+                                const HashBufs &h) {
+    hipError_t e = launch_fill_u32(s, h.counts, 0u, table);
+    if (e != hipSuccess || !n) {
+        if (e == hipSuccess) e = launch_fill_u32(s, h.starts, 0u, size_t(table) + 1);
+        return e;
+    }
+    hipLaunchKernelGGL(k_hash_count_vertices, dim3(grid_for(n, 256)), dim3(256), 0, s, P, n, cell, table, h.keys,
+                       h.counts);
+    return finish_hash(s, n, table, h);
+}
+
+hipError_t launch_hash_edges(hipStream_t s, const float4 *P, const uint2 *edges, uint32_t ne, float cell,
+                             uint32_t table, const HashBufs &h) {
+    hipError_t e = launch_fill_u32(s, h.counts, 0u, table);
+    if (e != hipSuccess || !ne) {
+        if (e == hipSuccess) e = launch_fill_u32(s, h.starts, 0u, size_t(table) + 1);
+        return e;
+    }
+    hipLaunchKernelGGL(k_hash_count_edges, dim3(grid_for(ne, 256)), dim3(256), 0, s, P, edges, ne, cell, table,
+                       h.keys, h.counts);
+    return finish_hash(s, ne, table, h);
+}
+
+// ---------------------------------------------------------------------------
+// fixed-point accumulation (self_collision.wgsl:71-116)
+__device__ __forceinline__ int32_t to_fixed(float x) {
+    return f2i_sat(clampf(x * 1000000.0f, -50000000.0f, 50000000.0f));
+}
+__device__ __forceinline__ void add_disp(Accum *a, f3 d) {
+    atomicAdd(&a->dx, to_fixed(d.x));
+    atomicAdd(&a->dy, to_fixed(d.y));
+    atomicAdd(&a->dz, to_fixed(d.z));
+    atomicAdd(&a->count, 1u);
+}
+__device__ __forceinline__ void add_pin_disp(Accum *a, f3 d) {
+    atomicAdd(&a->pdx, to_fixed(d.x));
+    atomicAdd(&a->pdy, to_fixed(d.y));
+    atomicAdd(&a->pdz, to_fixed(d.z));
+    atomicAdd(&a->pcount, 1u);
+}
+__device__ __forceinline__ void ccd_axis(int32_t v, int32_t *pos, int32_t *negv) {
+    if (v > 0) atomicMax(pos, v);
+    else if (v < 0) atomicMin(negv, v);
+}
+__device__ __forceinline__ void add_ccd(Accum *a, f3 d) {
+    ccd_axis(to_fixed(d.x), &a->cpx, &a->cnx);
+    ccd_axis(to_fixed(d.y), &a->cpy, &a->cny);
+    ccd_axis(to_fixed(d.z), &a->cpz, &a->cnz);
+}
+// self_collision_ee_direct.wgsl:70-79: the E-E proximity path counts each contribution twice
+__device__ __forceinline__ void add_disp_ee(Accum *a, f3 d) {
+    atomicAdd(&a->dx, to_fixed(d.x));
+    atomicAdd(&a->dy, to_fixed(d.y));
+    atomicAdd(&a->dz, to_fixed(d.z));
+    atomicAdd(&a->count, 2u);
+}
+__device__ __forceinline__ void add_ccd_ee(Accum *a, f3 d, bool count_ee) {
+    add_ccd(a, d);
+    if (count_ee) {
+        const float l = clampf(length(d) * 1000000.0f, 0.0f, 2000000000.0f);
+        if ((uint32_t)l > 0u) atomicAdd(&a->ee_count, 1u);
+    }
+}
+
+// self_collision.wgsl:134-179
+__device__ __forceinline__ f3 closest_bary(f3 p, f3 a, f3 b, f3 c, f3 &bary) {
+    const f3 ab = b - a, ac = c - a, ap = p - a;
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) { bary = mk3(1.0f, 0.0f, 0.0f); return a; }
+    const f3 bp = p - b;
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) { bary = mk3(0.0f, 1.0f, 0.0f); return b; }
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        const float v = d1 / (d1 - d3);
+        bary = mk3(1.0f - v, v, 0.0f);
+        return a + ab * v;
+    }
+    const f3 cp = p - c;
+    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) { bary = mk3(0.0f, 0.0f, 1.0f); return c; }
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        const float w = d2 / (d2 - d6);
+        bary = mk3(1.0f - w, 0.0f, w);
+        return a + ac * w;
+    }
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        bary = mk3(0.0f, 1.0f - w, w);
+        return b + (c - b) * w;
+    }
+    const float denom = 1.0f / (va + vb + vc);
+    const float v = vb * denom, w = vc * denom;
+    bary = mk3(1.0f - v - w, v, w);
+    return a + ab * v + ac * w;
+}
+
+// self_collision.wgsl:182-218; returns hit with (t, u, v)
+__device__ __forceinline__ bool seg_tri_uv(f3 po, f3 pn, f3 a, f3 b, f3 c, float &t_o, float &u_o, float &v_o) {
+    const f3 d = pn - po;
+    const float ld = dot(d, d);
+    if (ld < 1e-12f) return false;
+    const f3 e1 = b - a, e2 = c - a;
+    const f3 pv = cross(d, e2);
+    const float det = dot(e1, pv);
+    const float tn = dot(e1, e1) * dot(e2, e2);
+    if (det * det <= tn * ld * 1e-10f || tn <= 1e-20f) return false;
+    const float inv = 1.0f / det;
+    const f3 tv = po - a;
+    const float u = dot(tv, pv) * inv;
+    if (u < -0.05f || u > 1.05f) return false;
+    const f3 qv = cross(tv, e1);
+    const float v = dot(d, qv) * inv;
+    if (v < -0.05f || (u + v) > 1.05f) return false;
+    const float t = dot(e2, qv) * inv;
+    if (t >= 0.0f && t <= 1.0f) { t_o = t; u_o = u; v_o = v; return true; }
+    return false;
+}
+
+// two-hop lookups (self_collision.wgsl:262-314, ee_direct.wgsl:259-283):
+// returns -1 if b is not in the two-hop list of a, else its slot
+__device__ __forceinline__ int32_t two_hop_slot(const SelfColTopo &t, uint32_t a, uint32_t b) {
+    uint32_t lo = t.th_off[a], hi = t.th_off[a + 1];
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2, val = t.th_idx[mid];
+        if (val == b) return int32_t(mid);
+        if (val < b) lo = mid + 1; else hi = mid;
+    }
+    return -1;
+}
+__device__ __forceinline__ bool within_two_hops(const SelfColTopo &t, uint32_t a, uint32_t b) {
+    return a == b || two_hop_slot(t, a, b) >= 0;
+}
+__device__ __forceinline__ bool near_folded(const SelfColTopo &t, uint32_t a, uint32_t b, float cur) {
+    if (a == b) return true;
+    const int32_t s = two_hop_slot(t, a, b);
+    if (s < 0) return false;
+    return !(cur < t.th_rest[s] * 0.5f);
+}
+__device__ __forceinline__ bool near_folded_sq(const SelfColTopo &t, uint32_t a, uint32_t b, float cur_sq) {
+    if (a == b) return true;
+    const int32_t s = two_hop_slot(t, a, b);
+    if (s < 0) return false;
+    const float l0 = t.th_rest[s];
+    return !(cur_sq < (l0 * l0) * 0.25f);
+}
+
+// sc_solve: one thread per vertex, 27 neighbour cells (single-layer cloth:
+// the multi-layer untangling branches of the reference are inactive).
+__global__ void __launch_bounds__(256) k_sc_solve(const float4 *__restrict__ P, const float4 *__restrict__ X,
+                                                  SelfColTopo T, const uint32_t *__restrict__ starts,
+                                                  const uint32_t *__restrict__ sorted, Accum *__restrict__ acc,
+                                                  SelfColArgs A) {
+    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= A.nv) return;
+    const float EPS = 1e-7f;
+    const float4 qi = P[index], xi = X[index];
+    const bool pinned_i = qi.w <= 0.0f;
+    const f3 xo_i = xyz(xi), p_i = xyz(qi);
+    const float mv_i = length(p_i - xo_i);
+    if (pinned_i && mv_i < EPS) return;
+    const float thick_i = xi.w;
+    const float llen_i = T.local_len[index];
+    const i3 cc = cell_of(p_i, A.cell);
+    for (int32_t cx = cc.x - 1; cx <= cc.x + 1; ++cx)
+    for (int32_t cy = cc.y - 1; cy <= cc.y + 1; ++cy)
+    for (int32_t cz = cc.z - 1; cz <= cc.z + 1; ++cz) {
+        const uint32_t h = hash_cell({cx, cy, cz}, A.table);
+        const uint32_t kb = starts[h], ke = starts[h + 1];
+        for (uint32_t k = kb; k < ke; ++k) {
+            const uint32_t j = sorted[k];
+            if (j >= A.nv || j == index) continue;
+            const float4 qj = P[j];
+            const f3 p_j = xyz(qj);
+            const i3 jc = cell_of(p_j, A.cell);
+            if (!(jc.x == cx && jc.y == cy && jc.z == cz)) continue;
+            const float4 xj = X[j];
+            const f3 dvv = p_i - p_j;
+            const float dsq = dot(dvv, dvv);
+            const float thick_j = xj.w, llen_j = T.local_len[j];
+            const float eff = fmaxf(thick_i + thick_j, fminf(llen_i, llen_j) * 0.06f);
+            const float max_geom = eff + (llen_i + llen_j) * 1.3f;
+            if (!(dsq <= max_geom * max_geom)) continue;
+            const float dvv_len = sqrtf(fmaxf(dsq, 0.0f));
+            if (near_folded(T, index, j, dvv_len)) continue;
+            const f3 pj_old = xyz(xj);
+            const float wi = qi.w, wj = qj.w;
+            const bool both_dynamic = wj > 0.0f;
+            const float eff_sq = eff * eff;
+            // 1. vertex-vertex
+            if (!pinned_i && (!both_dynamic || index < j)) {
+                if (dsq < eff_sq && dsq > EPS * EPS) {
+                    const f3 n = dvv / dvv_len;
+                    const float pen = eff - dvv_len;
+                    const float ws = wi + wj;
+                    if (ws > EPS) {
+                        if (wj <= 0.0f) {
+                            add_pin_disp(&acc[index], n * pen + (p_j - pj_old) * 0.5f);
+                        } else {
+                            const f3 disp = n * (pen * 0.5f);
+                            add_disp(&acc[index], disp * (wi / ws));
+                            add_disp(&acc[j], (-disp) * (wj / ws));
+                        }
+                    }
+                }
+            }
+            // 2. vertex-triangle over the star of j
+            const float mv_j = length(p_j - pj_old);
+            const float vt_bound = eff + llen_j * 1.3f + mv_i + mv_j;
+            if (!(dsq <= vt_bound * vt_bound) || within_two_hops(T, index, j)) continue;
+            for (uint32_t st = T.star_off[j]; st < T.star_off[j + 1]; ++st) {
+                const uint2 pr = T.star[st];
+                const uint32_t v0 = pr.x, v1 = pr.y;
+                if (v0 == index || v1 == index) continue;
+                const float4 q0 = P[v0], q1 = P[v1], x0 = X[v0], x1 = X[v1];
+                const f3 p_v0 = xyz(q0), p_v1 = xyz(q1), p_v0o = xyz(x0), p_v1o = xyz(x1);
+                const f3 d0 = p_i - p_v0;
+                const float ll0 = T.local_len[v0];
+                const float et0 = fmaxf(thick_i + x0.w, fminf(llen_i, ll0) * 0.06f);
+                const float b0 = et0 + ll0 * 1.3f + mv_i + length(p_v0 - p_v0o);
+                const i3 c0 = cell_of(p_v0, A.cell);
+                const bool in0 = abs(c0.x - cc.x) <= 1 && abs(c0.y - cc.y) <= 1 && abs(c0.z - cc.z) <= 1 &&
+                                 dot(d0, d0) <= b0 * b0;
+                const f3 d1 = p_i - p_v1;
+                const float ll1 = T.local_len[v1];
+                const float et1 = fmaxf(thick_i + x1.w, fminf(llen_i, ll1) * 0.06f);
+                const float b1 = et1 + ll1 * 1.3f + mv_i + length(p_v1 - p_v1o);
+                const i3 c1 = cell_of(p_v1, A.cell);
+                const bool in1 = abs(c1.x - cc.x) <= 1 && abs(c1.y - cc.y) <= 1 && abs(c1.z - cc.z) <= 1 &&
+                                 dot(d1, d1) <= b1 * b1;
+                if ((in0 && j > v0) || (in1 && j > v1)) continue;
+                const f3 tri_move = (((p_j - pj_old) + (p_v0 - p_v0o)) + (p_v1 - p_v1o)) / 3.0f;
+                const bool tri_pin = wj <= 0.0f || q0.w <= 0.0f || q1.w <= 0.0f;
+                const f3 rel_old = xo_i + tri_move;
+                const f3 ev = mk3(eff, eff, eff);
+                const f3 smin = min3(rel_old, p_i) - ev, smax = max3(rel_old, p_i) + ev;
+                const f3 tmin = min3(min3(p_j, p_v0), p_v1), tmax = max3(max3(p_j, p_v0), p_v1);
+                if (smin.x > tmax.x || smin.y > tmax.y || smin.z > tmax.z || smax.x < tmin.x || smax.y < tmin.y ||
+                    smax.z < tmin.z)
+                    continue;
+                if (within_two_hops(T, index, v0) || within_two_hops(T, index, v1)) continue;
+                const f3 move = p_i - rel_old;
+                const float move_sq = dot(move, move);
+                float ht = 1.0f, hu = 0.0f, hv = 0.0f;
+                bool hit = false;
+                if (move_sq > 1e-12f) hit = seg_tri_uv(rel_old, p_i, p_j, p_v0, p_v1, ht, hu, hv);
+                const float w0 = q0.w, w1 = q1.w;
+                if (hit) {
+                    const f3 hit_pt = rel_old + move * ht;
+                    const f3 tcr = cross(p_v0 - p_j, p_v1 - p_j);
+                    const float tl = length(tcr);
+                    const f3 tn = tl > 1e-12f ? tcr / tl : mk3(0.0f, 1.0f, 0.0f);
+                    const float h_old_raw = dot(rel_old - p_j, tn);
+                    const float side = fabsf(h_old_raw) > 1e-6f ? h_old_raw : dot(-move, tn);
+                    const f3 n_side = side >= 0.0f ? tn : -tn;
+                    const float h_old = fmaxf(dot(rel_old - hit_pt, n_side), 0.0f);
+                    const float h_safe = fmaxf(h_old, eff * 0.25f);
+                    const f3 total = (hit_pt + n_side * h_safe) - p_i;
+                    const float pen_soft = fmaxf(0.0f, eff - h_safe);
+                    const f3 bc = mk3(clampf(1.0f - hu - hv, 0.0f, 1.0f), clampf(hu, 0.0f, 1.0f), clampf(hv, 0.0f, 1.0f));
+                    if (tri_pin) {
+                        if (!pinned_i) add_ccd(&acc[index], total + n_side * pen_soft);
+                    } else if (pinned_i) {
+                        const float wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+                        if (wt > EPS) {
+                            const f3 full = total + n_side * pen_soft;
+                            if (wj > 0.0f) add_ccd(&acc[j], (-full) * (bc.x * wj / wt));
+                            if (w0 > 0.0f) add_ccd(&acc[v0], (-full) * (bc.y * w0 / wt));
+                            if (w1 > 0.0f) add_ccd(&acc[v1], (-full) * (bc.z * w1 / wt));
+                        }
+                    } else {
+                        const float wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+                        const float wtot = wi + wt;
+                        if (wtot > EPS) {
+                            add_ccd(&acc[index], total * (wi / wtot));
+                            if (wj > 0.0f) add_ccd(&acc[j], (-total) * (bc.x * wj / wtot));
+                            if (w0 > 0.0f) add_ccd(&acc[v0], (-total) * (bc.y * w0 / wtot));
+                            if (w1 > 0.0f) add_ccd(&acc[v1], (-total) * (bc.z * w1 / wtot));
+                            if (pen_soft > EPS) {
+                                const f3 tp = n_side * (pen_soft * 0.6f);
+                                add_disp(&acc[index], tp * (wi / wtot));
+                                if (wj > 0.0f) add_disp(&acc[j], (-tp) * (bc.x * wj / wtot));
+                                if (w0 > 0.0f) add_disp(&acc[v0], (-tp) * (bc.y * w0 / wtot));
+                                if (w1 > 0.0f) add_disp(&acc[v1], (-tp) * (bc.z * w1 / wtot));
+                            }
+                        }
+                    }
+                } else {
+                    f3 bc;
+                    const f3 cp = closest_bary(p_i, p_j, p_v0, p_v1, bc);
+                    const f3 dvt = p_i - cp;
+                    const float dvsq = dot(dvt, dvt);
+                    if (!pinned_i && dvsq < eff_sq && dvsq > EPS * EPS) {
+                        const float dl = sqrtf(dvsq);
+                        const f3 nvt = dvt / dl;
+                        const float pen = eff - dl;
+                        if (tri_pin) {
+                            add_pin_disp(&acc[index], nvt * pen + tri_move * 0.5f);
+                        } else {
+                            const float wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+                            const float wtot = wi + wt;
+                            if (wtot > EPS) {
+                                const f3 tp = nvt * (pen * 0.6f);
+                                add_disp(&acc[index], tp * (wi / wtot));
+                                if (wj > 0.0f) add_disp(&acc[j], (-tp) * (bc.x * wj / wtot));
+                                if (w0 > 0.0f) add_disp(&acc[v0], (-tp) * (bc.y * w0 / wtot));
+                                if (w1 > 0.0f) add_disp(&acc[v1], (-tp) * (bc.z * w1 / wtot));
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// E-E (self_collision_ee_direct.wgsl)
+struct ST { float s, t; };
+__device__ __forceinline__ ST closest_segs(f3 p1, f3 q1, f3 p2, f3 q2) {  // ee_direct.wgsl:155-193
+    const f3 d1 = q1 - p1, d2 = q2 - p2, r = p1 - p2;
+    const float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+    if (a <= 1e-12f && e <= 1e-12f) return {0.0f, 0.0f};
+    if (a <= 1e-12f) return {0.0f, clampf(f / e, 0.0f, 1.0f)};
+    const float c = dot(d1, r);
+    if (e <= 1e-12f) return {clampf(-c / a, 0.0f, 1.0f), 0.0f};
+    const float b = dot(d1, d2);
+    const float denom = a * e - b * b;
+    float s = 0.0f;
+    if (denom > a * e * 1e-7f) s = clampf((b * f - c * e) / denom, 0.0f, 1.0f);
+    float t = (b * s + f) / e;
+    if (t < 0.0f) { t = 0.0f; s = clampf(-c / a, 0.0f, 1.0f); }
+    else if (t > 1.0f) { t = 1.0f; s = clampf((b - c) / a, 0.0f, 1.0f); }
+    return {s, t};
+}
+struct Sweep { f3 a0o, a0, a1o, a1, b0o, b0, b1o, b1; };
+__device__ __forceinline__ float sweep_dist_sq(const Sweep &w, float tau, ST &st) {
+    const f3 A0 = mix3(w.a0o, w.a0, tau), A1 = mix3(w.a1o, w.a1, tau);
+    const f3 B0 = mix3(w.b0o, w.b0, tau), B1 = mix3(w.b1o, w.b1, tau);
+    st = closest_segs(A0, A1, B0, B1);
+    const f3 d = mix3(A0, A1, st.s) - mix3(B0, B1, st.t);
+    return dot(d, d);
+}
+__device__ __forceinline__ float sweep_volume(const Sweep &w, float tau) {
+    const f3 A0 = mix3(w.a0o, w.a0, tau), A1 = mix3(w.a1o, w.a1, tau);
+    const f3 B0 = mix3(w.b0o, w.b0, tau), B1 = mix3(w.b1o, w.b1, tau);
+    return dot(cross(A1 - A0, B1 - B0), B0 - A0);
+}
+__device__ __forceinline__ bool edge_bit(const uint32_t *bits, uint32_t e) { return ((bits[e >> 5] >> (e & 31u)) & 1u) != 0u; }
+
+__global__ void __launch_bounds__(256) k_sc_solve_ee(const float4 *__restrict__ P, const float4 *__restrict__ X,
+                                                     SelfColTopo T, const uint32_t *__restrict__ starts,
+                                                     const uint32_t *__restrict__ sorted, Accum *__restrict__ acc,
+                                                     SelfColArgs A) {
+    const uint32_t ea = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ea >= A.ne) return;
+    const float EPS = 1e-7f;
+    const uint2 eA = T.edges[ea];
+    const uint32_t u0 = eA.x, u1 = eA.y;
+    const float4 qu0 = P[u0], qu1 = P[u1], xu0 = X[u0], xu1 = X[u1];
+    const float wu0 = qu0.w, wu1 = qu1.w;
+    const bool a_has_pin = wu0 <= 0.0f || wu1 <= 0.0f;
+    const f3 p_u0 = xyz(qu0), p_u1 = xyz(qu1), p_u0o = xyz(xu0), p_u1o = xyz(xu1);
+    const float th_u0 = xu0.w, th_u1 = xu1.w;
+    const float max_th_a = fmaxf(th_u0, th_u1);
+    const bool bnd_a = edge_bit(T.edge_bbits, ea);
+    const f3 mta = mk3(max_th_a * 2.0f, max_th_a * 2.0f, max_th_a * 2.0f);
+    const f3 amin = min3(min3(p_u0, p_u1), min3(p_u0o, p_u1o)) - mta;
+    const f3 amax = max3(max3(p_u0, p_u1), max3(p_u0o, p_u1o)) + mta;
+    const i3 bc = cell_of((p_u0 + p_u1) * 0.5f, A.cell);
+    uint32_t search_count = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+    for (int dx = -1; dx <= 1; ++dx) {
+        const i3 nc = {bc.x + dx, bc.y + dy, bc.z + dz};
+        const uint32_t h = hash_cell(nc, A.table);
+        const uint32_t kb = starts[h], ke = starts[h + 1];
+        for (uint32_t q = kb; q < ke; ++q) {
+            const uint32_t eb = sorted[q];
+            if (ea >= eb) continue;
+            const uint2 eB = T.edges[eb];
+            const uint32_t j = eB.x, vj = eB.y;
+            if (j == u0 || j == u1 || vj == u0 || vj == u1) continue;
+            const float4 qj = P[j], qvj = P[vj];
+            const f3 p_j = xyz(qj), p_vj = xyz(qvj);
+            const i3 bcell = cell_of((p_j + p_vj) * 0.5f, A.cell);
+            if (!(bcell.x == nc.x && bcell.y == nc.y && bcell.z == nc.z)) continue;
+            const float4 xj = X[j], xvj = X[vj];
+            const f3 p_jo = xyz(xj), p_vjo = xyz(xvj);
+            const f3 bmin = min3(min3(p_j, p_vj), min3(p_jo, p_vjo)), bmax = max3(max3(p_j, p_vj), max3(p_jo, p_vjo));
+            if (amin.x > bmax.x || amin.y > bmax.y || amin.z > bmax.z || amax.x < bmin.x || amax.y < bmin.y ||
+                amax.z < bmin.z)
+                continue;
+            ++search_count;
+            if (search_count > A.max_search_iterations) return;
+            const bool bnd_pair = bnd_a || edge_bit(T.edge_bbits, eb);
+            bool run_ccd = bnd_pair, has_cached = false;
+            ST st1c = {0.0f, 0.0f};
+            f3 d1c = mk3(0.0f, 0.0f, 0.0f);
+            float dsqc = 0.0f, effc = 0.0f;
+            const float th_j = xj.w, th_vj = xvj.w;
+            const float wj = qj.w, wvj = qvj.w;
+            if (!run_ccd) {
+                st1c = closest_segs(p_u0, p_u1, p_j, p_vj);
+                d1c = mix3(p_u0, p_u1, st1c.s) - mix3(p_j, p_vj, st1c.t);
+                dsqc = dot(d1c, d1c);
+                effc = mixf(th_u0, th_u1, st1c.s) + mixf(th_j, th_vj, st1c.t);
+                const float max_eff = max_th_a + fmaxf(th_j, th_vj);
+                const f3 du0 = p_u0 - p_u0o, du1 = p_u1 - p_u1o, dj = p_j - p_jo, dvj = p_vj - p_vjo;
+                const f3 va = (du0 + du1) * 0.5f, vb = (dj + dvj) * 0.5f;
+                const float rel = length(va - vb) + 0.5f * (length(du1 - du0) + length(dvj - dj));
+                const float reach = max_eff + rel;
+                if (dsqc > reach * reach) continue;
+                has_cached = true;
+                const f3 d0g = mix3(p_u0o, p_u1o, st1c.s) - mix3(p_jo, p_vjo, st1c.t);
+                run_ccd = (rel > effc * 0.5f) || (dot(d0g, d1c) <= 0.0f);
+            }
+            if (run_ccd) {
+                const Sweep W = {p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj};
+                float min_sq = 1e20f, best_tau = 0.0f, half_span = 0.125f;
+                ST min_st = {0.0f, 0.0f};
+                float prev_vol = sweep_volume(W, 0.0f);
+                float zc_tau = -1.0f;
+                for (uint32_t k = 0; k <= 8u; ++k) {
+                    const float tk = float(k) * 0.125f;
+                    ST stk;
+                    const float dk = sweep_dist_sq(W, tk, stk);
+                    if (dk < min_sq) { min_sq = dk; best_tau = tk; min_st = stk; }
+                    if (k > 0u) {
+                        const float vol = sweep_volume(W, tk);
+                        if (prev_vol * vol <= 0.0f && zc_tau < 0.0f) zc_tau = tk - 0.0625f;
+                        prev_vol = vol;
+                    }
+                }
+                if (zc_tau >= 0.0f) {
+                    ST stz;
+                    const float dzs = sweep_dist_sq(W, zc_tau, stz);
+                    if (dzs < min_sq) { min_sq = dzs; best_tau = zc_tau; half_span = 0.0625f; min_st = stz; }
+                }
+                float tlo = fmaxf(0.0f, best_tau - half_span), thi = fminf(1.0f, best_tau + half_span);
+                float tau_star = best_tau, dsq_star = min_sq;
+                ST st_star = min_st;
+                for (int it = 0; it < 6; ++it) {
+                    const float m1 = tlo + (thi - tlo) * (1.0f / 3.0f);
+                    const float m2 = thi - (thi - tlo) * (1.0f / 3.0f);
+                    ST s1, s2;
+                    const float e1 = sweep_dist_sq(W, m1, s1);
+                    const float e2 = sweep_dist_sq(W, m2, s2);
+                    if (e1 < dsq_star) { dsq_star = e1; tau_star = m1; st_star = s1; }
+                    if (e2 < dsq_star) { dsq_star = e2; tau_star = m2; st_star = s2; }
+                    if (e1 < e2) thi = m2; else tlo = m1;
+                }
+                const float ss = st_star.s, ts = st_star.t;
+                const float eff_star = mixf(th_u0, th_u1, ss) + mixf(th_j, th_vj, ts);
+                if (dsq_star >= eff_star * eff_star) continue;
+                if (A.exclude_neighbors) {
+                    const f3 a0s = mix3(p_u0o, p_u0, tau_star), a1s = mix3(p_u1o, p_u1, tau_star);
+                    const f3 b0s = mix3(p_jo, p_j, tau_star), b1s = mix3(p_vjo, p_vj, tau_star);
+                    const f3 t0 = a0s - b0s, t1 = a0s - b1s, t2 = a1s - b0s, t3 = a1s - b1s;
+                    if (near_folded_sq(T, u0, j, dot(t0, t0)) || near_folded_sq(T, u0, vj, dot(t1, t1)) ||
+                        near_folded_sq(T, u1, j, dot(t2, t2)) || near_folded_sq(T, u1, vj, dot(t3, t3)))
+                        continue;
+                }
+                const ST st0 = closest_segs(p_u0o, p_u1o, p_jo, p_vjo);
+                const f3 delta0 = mix3(p_u0o, p_u1o, st0.s) - mix3(p_jo, p_vjo, st0.t);
+                const float dist0 = length(delta0);
+                if (dist0 > EPS) {
+                    const f3 n0 = delta0 / dist0;
+                    const ST st1 = closest_segs(p_u0, p_u1, p_j, p_vj);
+                    const f3 delta1 = mix3(p_u0, p_u1, st1.s) - mix3(p_j, p_vj, st1.t);
+                    const float dist1 = length(delta1);
+                    const float h_end = dot(mix3(p_u0, p_u1, ss) - mix3(p_j, p_vj, ts), n0);
+                    const bool crossover = (h_end <= 0.0f) || (dot(delta1, n0) <= 0.0f) ||
+                                           (bnd_pair && dist1 >= eff_star && tau_star < 1.0f);
+                    if (h_end < eff_star && crossover) {
+                        const float pen = eff_star - h_end;
+                        const bool b_has_pin = wj <= 0.0f || wvj <= 0.0f;
+                        if (a_has_pin && b_has_pin) continue;
+                        const bool a_all = wu0 == 0.0f && wu1 == 0.0f;
+                        const bool b_all = wj == 0.0f && wvj == 0.0f;
+                        const bool cnt_pin = !(a_all || b_all);
+                        if (b_has_pin) {
+                            const f3 em = ((p_j - p_jo) + (p_vj - p_vjo)) * 0.5f;
+                            const float a = 1.0f - ss, b = ss;
+                            const float den = a * a * wu0 + b * b * wu1;
+                            if (den > EPS) {
+                                const f3 pb = n0 * pen + em * 0.5f;
+                                if (wu0 > 0.0f) add_ccd_ee(&acc[u0], pb * fminf((a * wu0) / den, 2.0f), cnt_pin);
+                                if (wu1 > 0.0f) add_ccd_ee(&acc[u1], pb * fminf((b * wu1) / den, 2.0f), cnt_pin);
+                            }
+                        } else if (a_has_pin) {
+                            const f3 em = ((p_u0 - p_u0o) + (p_u1 - p_u1o)) * 0.5f;
+                            const float a = 1.0f - ts, b = ts;
+                            const float den = a * a * wj + b * b * wvj;
+                            if (den > EPS) {
+                                const f3 pb = (-n0) * pen + em * 0.5f;
+                                if (wj > 0.0f) add_ccd_ee(&acc[j], pb * fminf((a * wj) / den, 2.0f), cnt_pin);
+                                if (wvj > 0.0f) add_ccd_ee(&acc[vj], pb * fminf((b * wvj) / den, 2.0f), cnt_pin);
+                            }
+                        } else {
+                            const float we1 = (1.0f - ss) * (1.0f - ss) * wu0 + ss * ss * wu1;
+                            const float we2 = (1.0f - ts) * (1.0f - ts) * wj + ts * ts * wvj;
+                            const float wt = we1 + we2;
+                            if (wt > EPS) {
+                                const f3 tot = n0 * pen;
+                                if (wu0 > 0.0f) add_ccd_ee(&acc[u0], tot * ((1.0f - ss) * wu0 / wt), true);
+                                if (wu1 > 0.0f) add_ccd_ee(&acc[u1], tot * (ss * wu1 / wt), true);
+                                if (wj > 0.0f) add_ccd_ee(&acc[j], (-tot) * ((1.0f - ts) * wj / wt), true);
+                                if (wvj > 0.0f) add_ccd_ee(&acc[vj], (-tot) * (ts * wvj / wt), true);
+                            }
+                        }
+                        continue;
+                    }
+                }
+            }
+            // discrete proximity
+            ST st;
+            f3 delta_ee;
+            float dsq, eff;
+            if (has_cached) {
+                st = st1c; delta_ee = d1c; dsq = dsqc; eff = effc;
+            } else {
+                st = closest_segs(p_u0, p_u1, p_j, p_vj);
+                delta_ee = (p_u0 + (p_u1 - p_u0) * st.s) - (p_j + (p_vj - p_j) * st.t);
+                dsq = dot(delta_ee, delta_ee);
+                eff = mixf(th_u0, th_u1, st.s) + mixf(th_j, th_vj, st.t);
+            }
+            if (dsq >= eff * eff || dsq <= EPS * EPS) continue;
+            if (A.exclude_neighbors) {
+                const f3 d0 = p_u0 - p_j, d1 = p_u0 - p_vj, d2 = p_u1 - p_j, d3 = p_u1 - p_vj;
+                if (near_folded_sq(T, u0, j, dot(d0, d0)) || near_folded_sq(T, u0, vj, dot(d1, d1)) ||
+                    near_folded_sq(T, u1, j, dot(d2, d2)) || near_folded_sq(T, u1, vj, dot(d3, d3)))
+                    continue;
+            }
+            const float dist = sqrtf(dsq);
+            const f3 n = delta_ee / dist;
+            const float pen = eff - dist;
+            const float sp = st.s, tp = st.t;
+            const bool b_has_pin = wj <= 0.0f || wvj <= 0.0f;
+            if (a_has_pin && b_has_pin) continue;
+            if (b_has_pin) {
+                const f3 em = ((p_j - p_jo) + (p_vj - p_vjo)) * 0.5f;
+                const float a = 1.0f - sp, b = sp;
+                const float den = a * a * wu0 + b * b * wu1;
+                if (den > EPS) {
+                    const f3 pb = n * pen + em * 0.5f;
+                    if (wu0 > 0.0f) add_pin_disp(&acc[u0], pb * fminf((a * wu0) / den, 2.0f));
+                    if (wu1 > 0.0f) add_pin_disp(&acc[u1], pb * fminf((b * wu1) / den, 2.0f));
+                }
+            } else if (a_has_pin) {
+                const f3 em = ((p_u0 - p_u0o) + (p_u1 - p_u1o)) * 0.5f;
+                const float a = 1.0f - tp, b = tp;
+                const float den = a * a * wj + b * b * wvj;
+                if (den > EPS) {
+                    const f3 pb = (-n) * pen + em * 0.5f;
+                    if (wj > 0.0f) add_pin_disp(&acc[j], pb * fminf((a * wj) / den, 2.0f));
+                    if (wvj > 0.0f) add_pin_disp(&acc[vj], pb * fminf((b * wvj) / den, 2.0f));
+                }
+            } else {
+                const float we1 = (1.0f - sp) * (1.0f - sp) * wu0 + sp * sp * wu1;
+                const float we2 = (1.0f - tp) * (1.0f - tp) * wj + tp * tp * wvj;
+                const float wt = we1 + we2;
+                if (wt > EPS) {
+                    const f3 td = n * (pen * 0.6f);
+                    if (wu0 > 0.0f) add_disp_ee(&acc[u0], td * ((1.0f - sp) * wu0 / wt));
+                    if (wu1 > 0.0f) add_disp_ee(&acc[u1], td * (sp * wu1 / wt));
+                    if (wj > 0.0f) add_disp_ee(&acc[j], (-td) * ((1.0f - tp) * wj / wt));
+                    if (wvj > 0.0f) add_disp_ee(&acc[vj], (-td) * (tp * wvj / wt));
+                }
+            }
+        }
+    }
+}
+
+// self_collision_apply.wgsl:57-177; clears the accumulator it reads.
+__global__ void __launch_bounds__(256) k_sc_apply(float4 *__restrict__ P, const float *__restrict__ local_len,
+                                                  Accum *__restrict__ acc, SelfColArgs A) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.nv) return;
+    const float EPS = 1e-7f;
+    const float INV = 1.0f / 1000000.0f;
+    Accum a = acc[i];
+    const float4 q = P[i];
+    const bool pinned = q.w <= 0.0f;
+    f3 fin = mk3(0.0f, 0.0f, 0.0f);
+    if (a.count > 0u && !pinned) {
+        const f3 tot = mk3(float(a.dx), float(a.dy), float(a.dz)) * INV;
+        f3 eff = tot / fmaxf(1.0f, sqrtf(float(a.count)));
+        if (A.enable_relief) eff = eff * A.relief_factor;
+        const float pl = length(eff);
+        const float ratio = A.max_disp_ratio > 0.0f ? A.max_disp_ratio : 0.5f;
+        const float ms = fminf(local_len[i] * ratio, 0.10f);
+        if (pl > ms && pl > EPS) eff = (eff / pl) * ms;
+        fin = fin + eff;
+    }
+    f3 avg_pin = mk3(0.0f, 0.0f, 0.0f);
+    if (a.pcount > 0u && !pinned) avg_pin = (mk3(float(a.pdx), float(a.pdy), float(a.pdz)) * INV) / float(a.pcount);
+    if (!pinned) {
+        f3 env = mk3(float(a.cpx), float(a.cpy), float(a.cpz)) * INV + mk3(float(a.cnx), float(a.cny), float(a.cnz)) * INV;
+        if (a.ee_count >= 4u) env = env * fmaxf(0.5f, sqrtf(3.0f / float(a.ee_count)));
+        const f3 z = mk3(0.0f, 0.0f, 0.0f);
+        fin = fin + (max3(max3(env, z), max3(avg_pin, z)) + min3(min3(env, z), min3(avg_pin, z)));
+    }
+    if (length(fin) > EPS) P[i] = make_float4(q.x + fin.x, q.y + fin.y, q.z + fin.z, q.w);
+    acc[i] = Accum{};
+}
+
+hipError_t launch_sc_solve(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t, const HashBufs &vh,
+                           Accum *acc, const SelfColArgs &a, uint32_t block) {
+    if (!a.nv) return hipSuccess;
+    hipLaunchKernelGGL(k_sc_solve, dim3(grid_for(a.nv, block)), dim3(block), 0, s, P, X, t, vh.starts, vh.sorted,
+                       acc, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sc_solve_ee(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t,
+                              const HashBufs &eh, Accum *acc, const SelfColArgs &a, uint32_t block) {
+    if (!a.ne) return hipSuccess;
+    hipLaunchKernelGGL(k_sc_solve_ee, dim3(grid_for(a.ne, block)), dim3(block), 0, s, P, X, t, eh.starts, eh.sorted,
+                       acc, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sc_apply(hipStream_t s, float4 *P, const float *local_len, Accum *acc, const SelfColArgs &a,
+                           uint32_t block) {
+    if (!a.nv) return hipSuccess;
+    hipLaunchKernelGGL(k_sc_apply, dim3(grid_for(a.nv, block)), dim3(block), 0, s, P, local_len, acc, a);
+    return hipGetLastError();
+}
+
+}  // namespace tc

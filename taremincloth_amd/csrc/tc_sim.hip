@@ -1,0 +1,939 @@
+// Simulator object and C ABI (include/tc_sim.h) of the MI355X XPBD substep library.
+//
+// The frame loop restates GpuClothSimulator::step / encode_simulation_steps
+// (crates/cloth_core/src/simulation/dispatch.rs:7-354, 610-671) with adaptive
+// substeps off, decoupled colliders (the path plain step() + get_positions()
+// always takes, dispatch.rs:94-102) and self-collision mode 0 / DIRECT.
+// One substep enqueues, on the simulator's own HIP stream:
+//   predict -> iterations x (distance colours, bending colours)
+//   -> collider resolve -> [vertex hash, edge hash, V-V/V-T, E-E, apply]
+//   -> update_vel
+// The whole frame is captured once into a hipGraph per (dt, substeps,
+// iterations, parameter epoch) and replayed; the lambda clear of every substep
+// is folded into the first sweep (lambda is read as zero there).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tc_sim.h"
+#include "tc_kernels.h"
+#include "tc_mesh.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define TC_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(TC_ERR_RUNTIME, std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        return hipMalloc(reinterpret_cast<void **>(&p), sizeof(T) * std::max<size_t>(count, 1));
+    }
+    hipError_t upload(const T *src, size_t count, hipStream_t s) {
+        hipError_t e = alloc(count);
+        if (e != hipSuccess || !count) return e;
+        return hipMemcpyAsync(p, src, sizeof(T) * count, hipMemcpyHostToDevice, s);
+    }
+};
+
+}  // namespace
+
+struct tc_sim {
+    tc::Mesh mesh;
+    hipStream_t stream = nullptr;
+    uint32_t block = 256;
+
+    // vertex streams
+    DevBuf<float4> X, P, V;
+    std::vector<float4> X0h, P0h;  // initial state for reset()
+
+    // constraints
+    DevBuf<uint4> drec, brec;
+    DevBuf<float> dcc, dlam, blam;
+    bool split_compliance = false;
+    std::vector<uint4> drec_h, brec_h;
+    std::vector<float> dcc_h;
+
+    // colliders
+    std::vector<tc::Collider> cols_h;
+    DevBuf<tc::Collider> cols;
+    std::vector<tc::MeshTri> tris_h;
+    DevBuf<tc::MeshTri> tris;
+    DevBuf<float4> tri_bounds;
+    int single_sided_recovery = 1;
+
+    // self collision
+    DevBuf<float> local_len, th_rest;
+    DevBuf<uint32_t> th_off, th_idx, star_off, edge_bbits;
+    DevBuf<uint2> star, edges;
+    DevBuf<uint32_t> vkeys, vcounts, vstarts, vsorted;
+    DevBuf<uint32_t> ekeys, ecounts, estarts, esorted;
+    DevBuf<tc::Accum> accum;
+    DevBuf<float> readback;
+    DevBuf<double> resid;
+
+    // parameters (simulation/mod.rs:607-691 defaults)
+    float gravity[3] = {0.0f, 0.0f, -9.81f};
+    float damping = 1.0f;
+    float tension_damping = 0.0f, compression_damping = 0.0f, shear_damping = 0.0f, bending_damping = 0.0f;
+    uint32_t iterations = 2;
+    bool self_collision = true;
+    float relief_factor = 0.2f, max_disp_ratio = 0.2f;
+    int enable_relief = 1, exclude_neighbors = 1;
+    bool normal_untangling = true;  // only effective for multi-layer meshes (pipelines.rs:895-896)
+    uint32_t max_search_iterations = 256;
+    float cell_size = 0.01f;
+    uint32_t table_size = 1024;
+
+    // frame graph cache
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    uint64_t graph_key[4] = {0, 0, 0, 0};
+    uint64_t epoch = 1;  // bumped whenever a parameter or buffer baked into the graph changes
+    bool use_graph = true;
+
+    ~tc_sim() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void touch() { ++epoch; }
+
+    tc::SubstepArgs args(float dt_sub) const {
+        tc::SubstepArgs a;
+        a.dt = dt_sub;
+        a.gx = gravity[0]; a.gy = gravity[1]; a.gz = gravity[2];
+        a.damping = damping;
+        a.tension_damping = tension_damping; a.compression_damping = compression_damping;
+        a.shear_damping = shear_damping; a.bending_damping = bending_damping;
+        return a;
+    }
+    tc::SelfColArgs sc_args() const {
+        tc::SelfColArgs a;
+        a.cell = cell_size; a.table = table_size;
+        a.nv = mesh.nv; a.ne = uint32_t(mesh.e_v0.size());
+        a.relief_factor = relief_factor; a.max_disp_ratio = max_disp_ratio;
+        a.enable_relief = enable_relief; a.exclude_neighbors = exclude_neighbors;
+        a.max_search_iterations = max_search_iterations;
+        return a;
+    }
+    tc::SelfColTopo topo() const {
+        tc::SelfColTopo t;
+        t.local_len = local_len.p; t.th_off = th_off.p; t.th_idx = th_idx.p; t.th_rest = th_rest.p;
+        t.star_off = star_off.p; t.star = star.p; t.edges = edges.p; t.edge_bbits = edge_bbits.p;
+        return t;
+    }
+    tc::HashBufs vhash() const { return {vkeys.p, vcounts.p, vstarts.p, vsorted.p}; }
+    tc::HashBufs ehash() const { return {ekeys.p, ecounts.p, estarts.p, esorted.p}; }
+};
+
+namespace {
+
+hipError_t enqueue_hashes(tc_sim *s) {
+    hipError_t e = tc::launch_hash_vertices(s->stream, s->P.p, s->mesh.nv, s->cell_size, s->table_size, s->vhash());
+    if (e != hipSuccess) return e;
+    return tc::launch_hash_edges(s->stream, s->P.p, s->edges.p, uint32_t(s->mesh.e_v0.size()), s->cell_size,
+                                 s->table_size, s->ehash());
+}
+
+// One substep up to (optionally) the velocity update.
+hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_collide, bool update) {
+    const tc::SubstepArgs a = s->args(dt_sub);
+    const tc::Mesh &m = s->mesh;
+    const uint32_t n = m.nv, B = s->block;
+    hipError_t e = tc::launch_predict(s->stream, s->P.p, s->X.p, s->V.p, n, a, B);
+    if (e != hipSuccess) return e;
+    if (s->iterations == 0) {  // lambdas are still cleared at the substep start (dispatch.rs:117-119)
+        if ((e = tc::launch_fill_u32(s->stream, reinterpret_cast<uint32_t *>(s->dlam.p), 0u, m.d_v0.size())) != hipSuccess) return e;
+        if ((e = tc::launch_fill_u32(s->stream, reinterpret_cast<uint32_t *>(s->blam.p), 0u, m.b_v0.size())) != hipSuccess) return e;
+    }
+    for (uint32_t it = 0; it < s->iterations; ++it) {
+        const bool first = it == 0;
+        for (size_t c = 0; c < m.d_col.count.size(); ++c) {
+            e = tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p, m.d_col.offset[c],
+                                    m.d_col.count[c], a, first, s->split_compliance, B);
+            if (e != hipSuccess) return e;
+        }
+        for (size_t c = 0; c < m.b_col.count.size(); ++c) {
+            e = tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p, m.b_col.offset[c],
+                                   m.b_col.count[c], a, first, B);
+            if (e != hipSuccess) return e;
+        }
+    }
+    if (collide && (!s->cols_h.empty() || !s->tris_h.empty())) {
+        e = tc::launch_collision(s->stream, s->P.p, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
+                                 s->tri_bounds.p, uint32_t(s->tris_h.size()), s->single_sided_recovery, dt_sub, B);
+        if (e != hipSuccess) return e;
+    }
+    if (self_collide && s->self_collision) {
+        if ((e = enqueue_hashes(s)) != hipSuccess) return e;
+        const tc::SelfColArgs sa = s->sc_args();
+        const tc::SelfColTopo t = s->topo();
+        if ((e = tc::launch_sc_solve(s->stream, s->P.p, s->X.p, t, s->vhash(), s->accum.p, sa, B)) != hipSuccess) return e;
+        if ((e = tc::launch_sc_solve_ee(s->stream, s->P.p, s->X.p, t, s->ehash(), s->accum.p, sa, B)) != hipSuccess) return e;
+        if ((e = tc::launch_sc_apply(s->stream, s->P.p, s->local_len.p, s->accum.p, sa, B)) != hipSuccess) return e;
+    }
+    if (update) return tc::launch_update_vel(s->stream, s->P.p, s->X.p, s->V.p, n, dt_sub, B);
+    return hipSuccess;
+}
+
+hipError_t enqueue_frame(tc_sim *s, float dt, uint32_t substeps) {
+    const float dt_sub = dt / float(substeps);
+    for (uint32_t k = 0; k < substeps; ++k) {
+        hipError_t e = enqueue_substep(s, dt_sub, true, true, true);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+int run_frame(tc_sim *s, float dt, uint32_t substeps) {
+    if (s->self_collision && s->normal_untangling && s->mesh.has_multiple_layers)
+        return fail(TC_ERR_UNSUPPORTED,
+                    "multi-layer cloth with normal untangling is not part of this build; disable it with "
+                    "set_self_collision_options(..., enable_normal_untangling=False) or disable self-collision");
+    if (!s->use_graph) {
+        TC_HIP(enqueue_frame(s, dt, substeps));
+        return TC_OK;
+    }
+    uint32_t dtb;
+    std::memcpy(&dtb, &dt, 4);
+    const uint64_t key[4] = {dtb, substeps, s->iterations, s->epoch};
+    if (!s->graph_exec || std::memcmp(key, s->graph_key, sizeof(key)) != 0) {
+        if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
+        if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+        TC_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = enqueue_frame(s, dt, substeps);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+        if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return fail(TC_ERR_RUNTIME, std::string("capture: ") + hipGetErrorString(e)); }
+        TC_HIP(e2);
+        s->graph = g;
+        TC_HIP(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
+        std::memcpy(s->graph_key, key, sizeof(key));
+    }
+    TC_HIP(hipGraphLaunch(s->graph_exec, s->stream));
+    return TC_OK;
+}
+
+void build_records(tc_sim *s) {
+    const tc::Mesh &m = s->mesh;
+    const size_t nd = m.d_v0.size(), nb = m.b_v0.size();
+    s->drec_h.resize(nd);
+    s->dcc_h.resize(nd);
+    s->split_compliance = false;
+    for (size_t k = 0; k < nd; ++k) {
+        uint4 r;
+        r.x = m.d_v0[k];
+        r.y = m.d_v1[k] | (m.d_type[k] == 1u ? 0x80000000u : 0u);
+        std::memcpy(&r.z, &m.d_rest[k], 4);
+        std::memcpy(&r.w, &m.d_tc[k], 4);
+        s->drec_h[k] = r;
+        s->dcc_h[k] = m.d_cc[k];
+        if (m.d_cc[k] != m.d_tc[k]) s->split_compliance = true;
+    }
+    s->brec_h.resize(nb);
+    for (size_t k = 0; k < nb; ++k) {
+        uint4 r;
+        r.x = m.b_v2[k];
+        r.y = m.b_v3[k];
+        std::memcpy(&r.z, &m.b_rest[k], 4);
+        std::memcpy(&r.w, &m.b_comp[k], 4);
+        s->brec_h[k] = r;
+    }
+}
+
+int upload_records(tc_sim *s) {
+    TC_HIP(s->drec.upload(s->drec_h.data(), s->drec_h.size(), s->stream));
+    TC_HIP(s->dcc.upload(s->dcc_h.data(), s->dcc_h.size(), s->stream));
+    TC_HIP(s->brec.upload(s->brec_h.data(), s->brec_h.size(), s->stream));
+    TC_HIP(hipStreamSynchronize(s->stream));
+    s->touch();
+    return TC_OK;
+}
+
+int upload_colliders(tc_sim *s) {
+    TC_HIP(s->cols.upload(s->cols_h.data(), s->cols_h.size(), s->stream));
+    TC_HIP(hipStreamSynchronize(s->stream));
+    s->touch();
+    return TC_OK;
+}
+
+bool finite3(const float *p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
+
+}  // namespace
+
+extern "C" {
+
+void tc_sim_params_default(tc_sim_params *p) {
+    if (!p) return;
+    p->thickness = 0.005f;
+    p->layer_id = 0;
+    p->stiffness = 1000.0f;
+    p->bending_stiffness = 10.0f;
+    p->compression_stiffness = -1.0f;
+    p->shear_stiffness = -1.0f;
+    p->workgroup_size = 0;
+    p->solver_mode = 0;
+}
+
+int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
+                  uint32_t nf, const float *inv_masses, const float *thicknesses, const uint32_t *layer_ids,
+                  const tc_sim_params *params, tc_sim **out) {
+    g_err.clear();
+    if (!out) return fail(TC_ERR_VALUE, "out handle pointer is null");
+    *out = nullptr;
+    tc_sim_params def;
+    tc_sim_params_default(&def);
+    const tc_sim_params &pp = params ? *params : def;
+    if (nv > 0 && !positions) return fail(TC_ERR_VALUE, "positions is null");
+    if (ne > 0 && !edges) return fail(TC_ERR_VALUE, "edges is null");
+    if (nv >= 0x80000000u) return fail(TC_ERR_VALUE, "too many vertices (limit 2^31 - 1)");
+    for (uint32_t e = 0; e < 2 * ne; ++e)
+        if (edges[e] >= nv)
+            return fail(TC_ERR_VALUE, "edges contain an index >= number of vertices (" + std::to_string(nv) + "): " +
+                                          std::to_string(edges[e]));
+    if (faces)
+        for (uint32_t f = 0; f < 3 * nf; ++f)
+            if (faces[f] >= nv)
+                return fail(TC_ERR_VALUE, "faces contain an index >= number of vertices (" + std::to_string(nv) +
+                                              "): " + std::to_string(faces[f]));
+    if (pp.solver_mode != 0) return fail(TC_ERR_UNSUPPORTED, "only solver_mode=0 (graph colouring) is supported");
+
+    tc::MeshInput in;
+    in.positions = positions; in.nv = nv;
+    in.edges = edges; in.ne = ne;
+    in.faces = (faces && nf) ? faces : nullptr; in.nf = (faces && nf) ? nf : 0;
+    in.inv_masses = inv_masses; in.thicknesses = thicknesses; in.layer_ids = layer_ids;
+    in.layer_id = pp.layer_id; in.thickness = pp.thickness;
+    in.tension_k = pp.stiffness;
+    in.compression_k = pp.compression_stiffness >= 0.0f ? pp.compression_stiffness : pp.stiffness;
+    in.shear_k = pp.shear_stiffness >= 0.0f ? pp.shear_stiffness : pp.stiffness * 0.5f;
+    in.bending_k = pp.bending_stiffness;
+
+    tc_sim *s = new tc_sim();
+    std::string err = tc::build_mesh(in, s->mesh);
+    if (!err.empty()) { delete s; return fail(TC_ERR_VALUE, err); }
+    if (pp.workgroup_size == 64 || pp.workgroup_size == 128 || pp.workgroup_size == 256) s->block = pp.workgroup_size;
+    s->cell_size = s->mesh.cell_size;
+    s->table_size = s->mesh.table_size;
+
+    auto bail = [&](int code) { delete s; return code; };
+    hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) return bail(fail(TC_ERR_RUNTIME, std::string("hipStreamCreate: ") + hipGetErrorString(he)));
+
+    const tc::Mesh &m = s->mesh;
+    s->X0h.resize(nv);
+    s->P0h.resize(nv);
+    for (uint32_t i = 0; i < nv; ++i) {
+        s->X0h[i] = make_float4(m.pos0[3 * i], m.pos0[3 * i + 1], m.pos0[3 * i + 2], m.thick[i]);
+        s->P0h[i] = make_float4(m.pos0[3 * i], m.pos0[3 * i + 1], m.pos0[3 * i + 2], m.inv_mass[i]);
+    }
+    std::vector<float4> zeros(nv, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    build_records(s);
+    std::vector<uint2> star(m.star_v0.size()), edg(m.e_v0.size());
+    for (size_t k = 0; k < star.size(); ++k) star[k] = make_uint2(m.star_v0[k], m.star_v1[k]);
+    for (size_t k = 0; k < edg.size(); ++k) edg[k] = make_uint2(m.e_v0[k], m.e_v1[k]);
+    const size_t T = s->table_size, NE = m.e_v0.size();
+#define UP(buf, src, cnt)                                                                        \
+    if ((he = (buf).upload((src), (cnt), s->stream)) != hipSuccess)                              \
+        return bail(fail(TC_ERR_RUNTIME, std::string("device upload: ") + hipGetErrorString(he)));
+#define AL(buf, cnt)                                                                             \
+    if ((he = (buf).alloc(cnt)) != hipSuccess)                                                   \
+        return bail(fail(TC_ERR_RUNTIME, std::string("hipMalloc: ") + hipGetErrorString(he)));
+    UP(s->X, s->X0h.data(), nv);
+    UP(s->P, s->P0h.data(), nv);
+    UP(s->V, zeros.data(), nv);
+    UP(s->drec, s->drec_h.data(), s->drec_h.size());
+    UP(s->dcc, s->dcc_h.data(), s->dcc_h.size());
+    UP(s->brec, s->brec_h.data(), s->brec_h.size());
+    AL(s->dlam, m.d_v0.size());
+    AL(s->blam, m.b_v0.size());
+    UP(s->local_len, m.local_len.data(), m.local_len.size());
+    UP(s->th_off, m.th_off.data(), m.th_off.size());
+    UP(s->th_idx, m.th_idx.data(), m.th_idx.size());
+    UP(s->th_rest, m.th_rest.data(), m.th_rest.size());
+    UP(s->star_off, m.star_off.data(), m.star_off.size());
+    UP(s->star, star.data(), star.size());
+    UP(s->edges, edg.data(), edg.size());
+    UP(s->edge_bbits, m.e_bbits.data(), m.e_bbits.size());
+    AL(s->vkeys, nv); AL(s->vsorted, nv); AL(s->vcounts, T); AL(s->vstarts, T + 1);
+    AL(s->ekeys, NE); AL(s->esorted, NE); AL(s->ecounts, T); AL(s->estarts, T + 1);
+    AL(s->accum, nv);
+    AL(s->readback, size_t(nv) * 3);
+    AL(s->resid, 3);
+#undef UP
+#undef AL
+    if ((he = hipMemsetAsync(s->accum.p, 0, sizeof(tc::Accum) * std::max<size_t>(nv, 1), s->stream)) != hipSuccess ||
+        (he = hipMemsetAsync(s->dlam.p, 0, sizeof(float) * std::max<size_t>(m.d_v0.size(), 1), s->stream)) != hipSuccess ||
+        (he = hipMemsetAsync(s->blam.p, 0, sizeof(float) * std::max<size_t>(m.b_v0.size(), 1), s->stream)) != hipSuccess ||
+        (he = hipStreamSynchronize(s->stream)) != hipSuccess)
+        return bail(fail(TC_ERR_RUNTIME, std::string("device init: ") + hipGetErrorString(he)));
+    *out = s;
+    return TC_OK;
+}
+
+void tc_sim_destroy(tc_sim *sim) {
+    if (!sim) return;
+    if (sim->stream) (void)hipStreamSynchronize(sim->stream);
+    delete sim;
+}
+
+int tc_sim_step(tc_sim *s, float dt, uint32_t substeps, uint32_t iterations, uint32_t *eff) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (iterations > 0) s->iterations = iterations;
+    const uint32_t S = substeps > 0 ? substeps : 20u;
+    if (eff) *eff = S;
+    if (s->mesh.nv == 0) return TC_OK;  // dispatch.rs:611-613
+    return run_frame(s, dt, S);
+}
+
+int tc_sim_step_single_substep(tc_sim *s, float dt_sub) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (s->mesh.nv == 0) return TC_OK;
+    int r = run_frame(s, dt_sub, 1);
+    if (r != TC_OK) return r;
+    TC_HIP(hipStreamSynchronize(s->stream));
+    return TC_OK;
+}
+
+int tc_sim_substep_parts(tc_sim *s, float dt_sub, int collide, int self_collide, int update) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (s->mesh.nv == 0) return TC_OK;
+    if (self_collide && s->self_collision && s->normal_untangling && s->mesh.has_multiple_layers)
+        return fail(TC_ERR_UNSUPPORTED, "multi-layer cloth with normal untangling is not part of this build");
+    TC_HIP(enqueue_substep(s, dt_sub, collide != 0, self_collide != 0, update != 0));
+    return TC_OK;
+}
+
+int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
+    if (!s || !out) return fail(TC_ERR_VALUE, "null argument");
+    for (int k = 0; k < 10; ++k) out[k] = 0.0;
+    if (s->mesh.nv == 0 || reps == 0) return TC_OK;
+    const tc::Mesh &m = s->mesh;
+    const tc::SubstepArgs a = s->args(dt_sub);
+    const uint32_t n = m.nv, B = s->block;
+    const size_t ndc = m.d_col.count.size(), nbc = m.b_col.count.size();
+    const size_t nev = 2 * (4 + size_t(s->iterations) * (ndc + nbc));
+    std::vector<hipEvent_t> ev(nev, nullptr);
+    for (auto &e : ev) TC_HIP(hipEventCreate(&e));
+    struct Rel { size_t a, b; int slot; };
+    std::vector<Rel> rel;
+    auto run = [&]() -> hipError_t {
+        size_t k = 0;
+        auto mark = [&](int slot, auto &&launch) -> hipError_t {
+            hipError_t e = hipEventRecord(ev[k], s->stream);
+            if (e != hipSuccess) return e;
+            if ((e = launch()) != hipSuccess) return e;
+            if ((e = hipEventRecord(ev[k + 1], s->stream)) != hipSuccess) return e;
+            rel.push_back({k, k + 1, slot});
+            k += 2;
+            return hipSuccess;
+        };
+        hipError_t e = mark(0, [&] { return tc::launch_predict(s->stream, s->P.p, s->X.p, s->V.p, n, a, B); });
+        if (e != hipSuccess) return e;
+        for (uint32_t it = 0; it < s->iterations; ++it) {
+            for (size_t c = 0; c < ndc; ++c)
+                if ((e = mark(1, [&] { return tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p,
+                                                                  m.d_col.offset[c], m.d_col.count[c], a, it == 0,
+                                                                  s->split_compliance, B); })) != hipSuccess) return e;
+            for (size_t c = 0; c < nbc; ++c)
+                if ((e = mark(3, [&] { return tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p,
+                                                                 m.b_col.offset[c], m.b_col.count[c], a, it == 0, B); })) != hipSuccess) return e;
+        }
+        if (!s->cols_h.empty() || !s->tris_h.empty())
+            if ((e = mark(5, [&] { return tc::launch_collision(s->stream, s->P.p, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()),
+                                                               s->tris.p, s->tri_bounds.p, uint32_t(s->tris_h.size()),
+                                                               s->single_sided_recovery, dt_sub, B); })) != hipSuccess) return e;
+        if (s->self_collision)
+            if ((e = mark(6, [&] {
+                     hipError_t r = enqueue_hashes(s);
+                     if (r != hipSuccess) return r;
+                     const tc::SelfColArgs sa = s->sc_args();
+                     const tc::SelfColTopo t = s->topo();
+                     if ((r = tc::launch_sc_solve(s->stream, s->P.p, s->X.p, t, s->vhash(), s->accum.p, sa, B)) != hipSuccess) return r;
+                     if ((r = tc::launch_sc_solve_ee(s->stream, s->P.p, s->X.p, t, s->ehash(), s->accum.p, sa, B)) != hipSuccess) return r;
+                     return tc::launch_sc_apply(s->stream, s->P.p, s->local_len.p, s->accum.p, sa, B);
+                 })) != hipSuccess) return e;
+        return mark(7, [&] { return tc::launch_update_vel(s->stream, s->P.p, s->X.p, s->V.p, n, dt_sub, B); });
+    };
+    double acc[10] = {0};
+    for (uint32_t r = 0; r < reps; ++r) {
+        rel.clear();
+        hipError_t e = run();
+        if (e != hipSuccess) {
+            for (auto &x : ev) (void)hipEventDestroy(x);
+            return fail(TC_ERR_RUNTIME, std::string("timed substep: ") + hipGetErrorString(e));
+        }
+        TC_HIP(hipStreamSynchronize(s->stream));
+        for (const Rel &q : rel) {
+            float ms = 0.0f;
+            TC_HIP(hipEventElapsedTime(&ms, ev[q.a], ev[q.b]));
+            acc[q.slot] += ms;
+        }
+        float wall = 0.0f;
+        TC_HIP(hipEventElapsedTime(&wall, ev[rel.front().a], ev[rel.back().b]));
+        acc[8] += wall;
+    }
+    for (auto &x : ev) TC_HIP(hipEventDestroy(x));
+    for (int k = 0; k < 9; ++k) out[k] = acc[k] / double(reps);
+    out[2] = double(ndc) * s->iterations;
+    out[4] = double(nbc) * s->iterations;
+    return TC_OK;
+}
+
+int tc_sim_synchronize(tc_sim *s) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    TC_HIP(hipStreamSynchronize(s->stream));
+    return TC_OK;
+}
+
+int tc_sim_get_positions(tc_sim *s, float *out, size_t n_floats) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    const uint32_t n = s->mesh.nv;
+    if (n == 0) return TC_OK;
+    if (!out || n_floats < size_t(n) * 3) return fail(TC_ERR_VALUE, "output buffer must hold 3 * num_vertices floats");
+    TC_HIP(tc::launch_extract_positions(s->stream, s->X.p, s->readback.p, n));
+    TC_HIP(hipMemcpyAsync(out, s->readback.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s->stream));
+    TC_HIP(hipStreamSynchronize(s->stream));
+    return TC_OK;
+}
+
+int tc_sim_set_positions_velocities(tc_sim *s, const float *pos, const float *vel, uint32_t n) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (n != s->mesh.nv || !pos) return TC_OK;  // dispatch.rs:958-960: silently ignored
+    std::vector<float4> X(n), P(n), V(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        X[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], s->X0h[i].w);
+        P[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], s->P0h[i].w);
+        V[i] = vel ? make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    TC_HIP(hipStreamSynchronize(s->stream));
+    TC_HIP(hipMemcpy(s->X.p, X.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    TC_HIP(hipMemcpy(s->P.p, P.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    TC_HIP(hipMemcpy(s->V.p, V.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    return TC_OK;
+}
+
+int tc_sim_reset(tc_sim *s) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    const uint32_t n = s->mesh.nv;
+    if (!n) return TC_OK;
+    std::vector<float4> V(n, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    TC_HIP(hipStreamSynchronize(s->stream));
+    TC_HIP(hipMemcpy(s->X.p, s->X0h.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    TC_HIP(hipMemcpy(s->P.p, s->P0h.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    TC_HIP(hipMemcpy(s->V.p, V.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    return TC_OK;
+}
+
+int tc_sim_add_sphere(tc_sim *s, const float c[3], float r, float fr, float rest) {
+    if (!s || !c) return fail(TC_ERR_VALUE, "null argument");
+    tc::Collider k{};
+    k.type = 0; k.friction = fr; k.restitution = rest;
+    k.ax = c[0]; k.ay = c[1]; k.az = c[2]; k.radius = r;
+    s->cols_h.push_back(k);
+    return upload_colliders(s);
+}
+
+int tc_sim_add_capsule(tc_sim *s, const float a[3], const float b[3], float r, float fr, float rest) {
+    if (!s || !a || !b) return fail(TC_ERR_VALUE, "null argument");
+    tc::Collider k{};
+    k.type = 1; k.friction = fr; k.restitution = rest;
+    k.ax = a[0]; k.ay = a[1]; k.az = a[2]; k.radius = r;
+    k.bx = b[0]; k.by = b[1]; k.bz = b[2];
+    s->cols_h.push_back(k);
+    return upload_colliders(s);
+}
+
+int tc_sim_add_plane(tc_sim *s, const float p[3], const float n[3], float fr, float rest) {
+    if (!s || !p || !n) return fail(TC_ERR_VALUE, "null argument");
+    tc::Collider k{};
+    k.type = 2; k.friction = fr; k.restitution = rest;
+    k.ax = p[0]; k.ay = p[1]; k.az = p[2];
+    k.bx = n[0]; k.by = n[1]; k.bz = n[2];
+    s->cols_h.push_back(k);
+    return upload_colliders(s);
+}
+
+int tc_sim_clear_colliders(tc_sim *s) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    s->cols_h.clear();
+    s->tris_h.clear();
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_set_mesh_triangles(tc_sim *s, const float *tris, uint32_t nt, const float *attrs, float friction,
+                              float thickness, float restitution, int single_sided) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (nt > 0 && !tris) return fail(TC_ERR_VALUE, "triangles pointer is null");
+    for (size_t k = 0; k < size_t(nt) * 9; ++k)
+        if (!std::isfinite(tris[k])) return fail(TC_ERR_VALUE, "triangle coordinates must be finite");
+    std::vector<tc::MeshTri> T(nt);
+    for (uint32_t t = 0; t < nt; ++t) {
+        const float *v = tris + 9 * size_t(t);
+        tc::MeshTri m{};
+        m.p0x = v[0]; m.p0y = v[1]; m.p0z = v[2];
+        m.p1x = v[3]; m.p1y = v[4]; m.p1z = v[5];
+        m.p2x = v[6]; m.p2y = v[7]; m.p2z = v[8];
+        m.friction = attrs ? attrs[4 * t] : friction;
+        m.thickness = attrs ? attrs[4 * t + 1] : thickness;
+        m.restitution = attrs ? attrs[4 * t + 2] : restitution;
+        m.flags = attrs ? uint32_t(attrs[4 * t + 3]) : (single_sided ? 1u : 0u);
+        T[t] = m;
+    }
+    // Morton order of the triangle centroids (simulation/mod.rs:2607-2632), stable
+    std::vector<tc::MeshTri> sorted = T;
+    if (nt > 0) {
+        float bmin[3] = {3.4028235e38f, 3.4028235e38f, 3.4028235e38f};
+        float bmax[3] = {-3.4028235e38f, -3.4028235e38f, -3.4028235e38f};
+        std::vector<std::array<float, 3>> cen(nt);
+        for (uint32_t t = 0; t < nt; ++t) {
+            const tc::MeshTri &m = T[t];
+            cen[t] = {(m.p0x + m.p1x + m.p2x) / 3.0f, (m.p0y + m.p1y + m.p2y) / 3.0f, (m.p0z + m.p1z + m.p2z) / 3.0f};
+            for (int a = 0; a < 3; ++a) { bmin[a] = std::min(bmin[a], cen[t][a]); bmax[a] = std::max(bmax[a], cen[t][a]); }
+        }
+        float inv[3];
+        for (int a = 0; a < 3; ++a) inv[a] = 1.0f / std::max(bmax[a] - bmin[a], 1e-5f);
+        auto expand = [](uint32_t v) {
+            v = (v * 0x00010001u) & 0xFF0000FFu;
+            v = (v * 0x00000101u) & 0x0F00F00Fu;
+            v = (v * 0x00000011u) & 0xC30C30C3u;
+            v = (v * 0x00000005u) & 0x49249249u;
+            return v;
+        };
+        auto q = [](float x) -> uint32_t { return x > 0.0f ? uint32_t(x) : 0u; };
+        std::vector<uint32_t> code(nt), idx(nt);
+        for (uint32_t t = 0; t < nt; ++t) {
+            uint32_t c[3];
+            for (int a = 0; a < 3; ++a) c[a] = q(std::min(std::max((cen[t][a] - bmin[a]) * inv[a], 0.0f), 1.0f) * 1023.0f);
+            code[t] = (expand(c[0]) << 2) | (expand(c[1]) << 1) | expand(c[2]);
+            idx[t] = t;
+        }
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
+        for (uint32_t t = 0; t < nt; ++t) sorted[t] = T[idx[t]];
+    }
+    // bounding spheres (simulation/mod.rs:2656-2667)
+    std::vector<float4> bounds(nt);
+    for (uint32_t t = 0; t < nt; ++t) {
+        const tc::MeshTri &m = sorted[t];
+        const float cx = (m.p0x + m.p1x + m.p2x) / 3.0f, cy = (m.p0y + m.p1y + m.p2y) / 3.0f,
+                    cz = (m.p0z + m.p1z + m.p2z) / 3.0f;
+        auto d2 = [&](float x, float y, float z) { return (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz); };
+        const float r = std::sqrt(std::max(std::max(d2(m.p0x, m.p0y, m.p0z), d2(m.p1x, m.p1y, m.p1z)),
+                                           d2(m.p2x, m.p2y, m.p2z))) + m.thickness;
+        bounds[t] = make_float4(cx, cy, cz, r);
+    }
+    s->tris_h = sorted;
+    TC_HIP(s->tris.upload(s->tris_h.data(), nt, s->stream));
+    TC_HIP(s->tri_bounds.upload(bounds.data(), nt, s->stream));
+    TC_HIP(hipStreamSynchronize(s->stream));
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_set_gravity(tc_sim *s, float gx, float gy, float gz) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    s->gravity[0] = gx; s->gravity[1] = gy; s->gravity[2] = gz;
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_set_damping(tc_sim *s, float d) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    s->damping = std::max(d, 0.0f);
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_set_damping_all(tc_sim *s, float t, float c, float sh, float b) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    s->tension_damping = std::max(t, 0.0f);
+    s->compression_damping = std::max(c, 0.0f);
+    s->shear_damping = std::max(sh, 0.0f);
+    s->bending_damping = std::max(b, 0.0f);
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_set_solver_iterations(tc_sim *s, uint32_t it) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    s->iterations = it;
+    return TC_OK;
+}
+
+int tc_sim_set_enable_self_collision(tc_sim *s, int on) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    s->self_collision = on != 0;
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_set_self_collision_options(tc_sim *s, float relief, float ratio, int exclude, int untangle,
+                                      uint32_t max_it) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    s->normal_untangling = untangle != 0;
+    s->relief_factor = relief;
+    s->max_disp_ratio = ratio;
+    s->enable_relief = relief < 0.999f ? 1 : 0;  // simulation/mod.rs:941
+    s->exclude_neighbors = exclude != 0;
+    s->max_search_iterations = max_it;
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_set_stiffness_all(tc_sim *s, float tension, float compression, float shear, float bending) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    const float tcmp = tc::compliance_of(tension), ccmp = tc::compliance_of(compression);
+    const float scmp = tc::compliance_of(shear);
+    const float bcmp = bending > 0.0f ? 1.0f / bending : 1e10f;
+    tc::Mesh &m = s->mesh;
+    for (size_t k = 0; k < m.d_v0.size(); ++k) {
+        if (m.d_type[k] == 1u) { m.d_tc[k] = scmp; m.d_cc[k] = scmp; }
+        else { m.d_tc[k] = tcmp; m.d_cc[k] = ccmp; }
+    }
+    for (size_t k = 0; k < m.b_v0.size(); ++k)
+        m.b_comp[k] = m.b_pad0[k] > 1.5f ? std::max(bcmp * 10.0f, 0.5f) : bcmp;
+    build_records(s);
+    return upload_records(s);
+}
+
+int tc_sim_set_compliances_raw(tc_sim *s, float stretch, float shear, float bend) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (!(stretch >= 0.0f) || !(shear >= 0.0f) || !(bend >= 0.0f)) return fail(TC_ERR_VALUE, "compliances must be >= 0");
+    tc::Mesh &m = s->mesh;
+    for (size_t k = 0; k < m.d_v0.size(); ++k) {
+        const float c = m.d_type[k] == 1u ? shear : stretch;
+        m.d_tc[k] = c;
+        m.d_cc[k] = c;
+    }
+    for (size_t k = 0; k < m.b_v0.size(); ++k) m.b_comp[k] = bend;
+    build_records(s);
+    return upload_records(s);
+}
+
+int tc_sim_set_hash_cell_size(tc_sim *s, float cell) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (!(cell > 0.0f) || !std::isfinite(cell)) return fail(TC_ERR_VALUE, "cell size must be positive and finite");
+    s->cell_size = cell;
+    s->touch();
+    return TC_OK;
+}
+
+uint32_t tc_sim_num_vertices(const tc_sim *s) { return s ? s->mesh.nv : 0; }
+uint32_t tc_sim_num_distance_constraints(const tc_sim *s) { return s ? uint32_t(s->mesh.d_v0.size()) : 0; }
+uint32_t tc_sim_num_bending_constraints(const tc_sim *s) { return s ? uint32_t(s->mesh.b_v0.size()) : 0; }
+
+int tc_sim_build_hashes(tc_sim *s) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    TC_HIP(enqueue_hashes(s));
+    TC_HIP(hipStreamSynchronize(s->stream));
+    return TC_OK;
+}
+
+int tc_sim_get_array(tc_sim *s, int which, void *out, size_t cap, size_t *count) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    const tc::Mesh &m = s->mesh;
+    auto host = [&](const void *src, size_t n, size_t elem) -> int {
+        if (count) *count = n;
+        if (!out) return TC_OK;
+        if (cap < n) return fail(TC_ERR_VALUE, "capacity too small");
+        if (n) std::memcpy(out, src, n * elem);
+        return TC_OK;
+    };
+    auto dev = [&](const void *src, size_t n, size_t elem) -> int {
+        if (count) *count = n;
+        if (!out) return TC_OK;
+        if (cap < n) return fail(TC_ERR_VALUE, "capacity too small");
+        TC_HIP(hipStreamSynchronize(s->stream));
+        if (n) TC_HIP(hipMemcpy(out, src, n * elem, hipMemcpyDeviceToHost));
+        return TC_OK;
+    };
+    auto dev_xyz = [&](const float4 *src) -> int {
+        const size_t n = size_t(m.nv) * 3;
+        if (count) *count = n;
+        if (!out) return TC_OK;
+        if (cap < n) return fail(TC_ERR_VALUE, "capacity too small");
+        std::vector<float4> tmp(m.nv);
+        TC_HIP(hipStreamSynchronize(s->stream));
+        if (m.nv) TC_HIP(hipMemcpy(tmp.data(), src, sizeof(float4) * m.nv, hipMemcpyDeviceToHost));
+        float *o = static_cast<float *>(out);
+        for (uint32_t i = 0; i < m.nv; ++i) { o[3 * i] = tmp[i].x; o[3 * i + 1] = tmp[i].y; o[3 * i + 2] = tmp[i].z; }
+        return TC_OK;
+    };
+    switch (which) {
+    case 0: return host(m.d_v0.data(), m.d_v0.size(), 4);
+    case 1: return host(m.d_v1.data(), m.d_v1.size(), 4);
+    case 2: return host(m.d_col.offset.data(), m.d_col.offset.size(), 4);
+    case 3: return host(m.d_col.count.data(), m.d_col.count.size(), 4);
+    case 4: return host(m.b_v2.data(), m.b_v2.size(), 4);
+    case 5: return host(m.b_v3.data(), m.b_v3.size(), 4);
+    case 6: return host(m.b_col.offset.data(), m.b_col.offset.size(), 4);
+    case 7: return host(m.b_col.count.data(), m.b_col.count.size(), 4);
+    case 8: return host(m.d_rest.data(), m.d_rest.size(), 4);
+    case 9: return host(m.b_rest.data(), m.b_rest.size(), 4);
+    case 10: return dev(s->vkeys.p, m.nv, 4);
+    case 11: return dev(s->vstarts.p, size_t(s->table_size) + 1, 4);
+    case 12: return dev(s->vsorted.p, m.nv, 4);
+    case 13: return dev_xyz(s->P.p);
+    case 14: return dev_xyz(s->V.p);
+    case 15: return dev(s->dlam.p, m.d_v0.size(), 4);
+    case 16: return dev(s->blam.p, m.b_v0.size(), 4);
+    case 17: {
+        uint32_t sc[4];
+        std::memcpy(&sc[0], &s->cell_size, 4);
+        sc[1] = s->table_size; sc[2] = m.n_virtual; sc[3] = uint32_t(m.e_v0.size());
+        return host(sc, 4, 4);
+    }
+    case 18: {
+        std::vector<uint32_t> e(2 * m.e_v0.size());
+        for (size_t k = 0; k < m.e_v0.size(); ++k) { e[2 * k] = m.e_v0[k]; e[2 * k + 1] = m.e_v1[k]; }
+        return host(e.data(), e.size(), 4);
+    }
+    case 19: return dev(s->ekeys.p, m.e_v0.size(), 4);
+    case 20: return dev(s->estarts.p, size_t(s->table_size) + 1, 4);
+    case 21: return dev(s->esorted.p, m.e_v0.size(), 4);
+    default: return fail(TC_ERR_VALUE, "unknown array id " + std::to_string(which));
+    }
+}
+
+int tc_sim_residual(tc_sim *s, float dt_sub, double *rel, double *max_rel) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    const tc::Mesh &m = s->mesh;
+    double h[3] = {0.0, 0.0, 0.0};
+    TC_HIP(hipMemcpyAsync(s->resid.p, h, sizeof(h), hipMemcpyHostToDevice, s->stream));
+    TC_HIP(tc::launch_residual(s->stream, s->P.p, s->drec.p, s->dcc.p, s->dlam.p, uint32_t(m.d_v0.size()), s->brec.p,
+                               s->blam.p, uint32_t(m.b_v0.size()), dt_sub, s->split_compliance, s->resid.p));
+    TC_HIP(hipMemcpyAsync(h, s->resid.p, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    TC_HIP(hipStreamSynchronize(s->stream));
+    if (rel) *rel = h[1] > 0.0 ? std::sqrt(h[0]) / std::sqrt(h[1]) : 0.0;
+    if (max_rel) *max_rel = h[2];
+    return TC_OK;
+}
+
+int tc_mesh_build_arrays(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
+                         uint32_t nf, const float *inv_masses, const float *thicknesses, const tc_sim_params *params,
+                         int which, void *out, size_t cap, size_t *count) {
+    g_err.clear();
+    if (nv > 0 && !pos) return fail(TC_ERR_VALUE, "positions is null");
+    if (ne > 0 && !edges) return fail(TC_ERR_VALUE, "edges is null");
+    for (size_t e = 0; e < size_t(ne) * 2; ++e)
+        if (edges[e] >= nv) return fail(TC_ERR_VALUE, "edges contain an index >= number of vertices");
+    if (faces)
+        for (size_t f = 0; f < size_t(nf) * 3; ++f)
+            if (faces[f] >= nv) return fail(TC_ERR_VALUE, "faces contain an index >= number of vertices");
+    tc_sim_params def;
+    tc_sim_params_default(&def);
+    const tc_sim_params &pp = params ? *params : def;
+    tc::MeshInput in;
+    in.positions = pos; in.nv = nv; in.edges = edges; in.ne = ne;
+    in.faces = (faces && nf) ? faces : nullptr; in.nf = (faces && nf) ? nf : 0;
+    in.inv_masses = inv_masses; in.thicknesses = thicknesses;
+    in.layer_id = pp.layer_id; in.thickness = pp.thickness;
+    in.tension_k = pp.stiffness;
+    in.compression_k = pp.compression_stiffness >= 0.0f ? pp.compression_stiffness : pp.stiffness;
+    in.shear_k = pp.shear_stiffness >= 0.0f ? pp.shear_stiffness : pp.stiffness * 0.5f;
+    in.bending_k = pp.bending_stiffness;
+    tc::Mesh m;
+    const std::string err = tc::build_mesh(in, m);
+    if (!err.empty()) return fail(TC_ERR_VALUE, err);
+    auto host = [&](const void *src, size_t n) -> int {
+        if (count) *count = n;
+        if (!out) return TC_OK;
+        if (cap < n) return fail(TC_ERR_VALUE, "capacity too small");
+        if (n) std::memcpy(out, src, n * 4);
+        return TC_OK;
+    };
+    std::vector<uint32_t> tmp;
+    switch (which) {
+    case 0: return host(m.d_v0.data(), m.d_v0.size());
+    case 1: return host(m.d_v1.data(), m.d_v1.size());
+    case 2: return host(m.d_col.offset.data(), m.d_col.offset.size());
+    case 3: return host(m.d_col.count.data(), m.d_col.count.size());
+    case 4: return host(m.b_v2.data(), m.b_v2.size());
+    case 5: return host(m.b_v3.data(), m.b_v3.size());
+    case 6: return host(m.b_col.offset.data(), m.b_col.offset.size());
+    case 7: return host(m.b_col.count.data(), m.b_col.count.size());
+    case 8: return host(m.d_rest.data(), m.d_rest.size());
+    case 9: return host(m.b_rest.data(), m.b_rest.size());
+    case 17: {
+        uint32_t sc[4];
+        std::memcpy(&sc[0], &m.cell_size, 4);
+        sc[1] = m.table_size; sc[2] = m.n_virtual; sc[3] = uint32_t(m.e_v0.size());
+        return host(sc, 4);
+    }
+    case 18:
+        for (size_t k = 0; k < m.e_v0.size(); ++k) { tmp.push_back(m.e_v0[k]); tmp.push_back(m.e_v1[k]); }
+        return host(tmp.data(), tmp.size());
+    case 22: return host(m.th_off.data(), m.th_off.size());
+    case 23: return host(m.th_idx.data(), m.th_idx.size());
+    case 24: return host(m.th_rest.data(), m.th_rest.size());
+    case 25: return host(m.local_len.data(), m.local_len.size());
+    case 26: return host(m.star_off.data(), m.star_off.size());
+    case 27:
+        for (size_t k = 0; k < m.star_v0.size(); ++k) { tmp.push_back(m.star_v0[k]); tmp.push_back(m.star_v1[k]); }
+        return host(tmp.data(), tmp.size());
+    case 28: return host(m.e_bbits.data(), m.e_bbits.size());
+    case 29: return host(m.island.data(), m.island.size());
+    case 30: return host(m.edge_to_constraint.data(), m.edge_to_constraint.size());
+    case 31: return host(m.d_type.data(), m.d_type.size());
+    case 32: return host(m.d_tc.data(), m.d_tc.size());
+    case 33: return host(m.d_cc.data(), m.d_cc.size());
+    case 34: return host(m.b_comp.data(), m.b_comp.size());
+    case 35: return host(m.adj_off.data(), m.adj_off.size());
+    case 36: return host(m.adj_idx.data(), m.adj_idx.size());
+    case 37: return host(m.layer.data(), m.layer.size());
+    default: return fail(TC_ERR_VALUE, "unknown array id " + std::to_string(which));
+    }
+}
+
+int tc_color_edge_pairs(uint32_t nv, const uint32_t *pairs, uint32_t np, uint32_t *colors_out) {
+    if (np && (!pairs || !colors_out)) return fail(TC_ERR_VALUE, "null argument");
+    const std::vector<uint32_t> c = tc::color_edge_pairs(nv, pairs, np);
+    if (np) std::memcpy(colors_out, c.data(), sizeof(uint32_t) * np);
+    return TC_OK;
+}
+
+int tc_compute_inv_masses(const float *pos, uint32_t nv, const uint32_t *faces, uint32_t nf, const float *pins,
+                          float density, float *out) {
+    if (nv && (!pos || !out)) return fail(TC_ERR_VALUE, "null argument");
+    if (faces)
+        for (size_t k = 0; k < size_t(nf) * 3; ++k)
+            if (faces[k] >= nv) return fail(TC_ERR_VALUE, "faces contain an index >= number of vertices");
+    tc::areal_inv_masses(pos, nv, faces, faces ? nf : 0, density, pins, out);
+    return TC_OK;
+}
+
+const char *tc_last_error(void) { return g_err.c_str(); }
+const char *tc_version(void) { return "taremincloth_amd 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,471 @@
+// Core XPBD substep kernels for gfx950: predict, per-colour distance and
+// bending projection, collider resolve, velocity update, readback packing and
+// the constraint-residual reduction.
+//
+// Reference semantics (paths relative to the reference repository root):
+//   predict      crates/cloth_core/src/shaders/predict.wgsl:33-55
+//   distance     crates/cloth_core/src/shaders/distance.wgsl:60-124
+//   bending      crates/cloth_core/src/shaders/bending.wgsl:59-126
+//   collision    crates/cloth_core/src/shaders/collision.wgsl:83-108, 273-652
+//   update_vel   crates/cloth_core/src/shaders/update_vel.wgsl:33-53
+//   extract      crates/cloth_core/src/shaders/extract_positions.wgsl:14-26
+//
+// The projection kernels are HBM/gather bound: one thread per constraint of a
+// colour, a 16-byte constraint record {v0, v1 | type<<31, rest, compliance}
+// and two 16-byte gathers of P.  No two constraints of one colour share a
+// vertex, so the scatter back to P needs no atomics.
+#include "tc_kernels.h"
+
+namespace tc {
+
+__global__ void __launch_bounds__(256) k_predict(float4 *__restrict__ P, const float4 *__restrict__ X,
+                                                 float4 *__restrict__ V, uint32_t n, SubstepArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float dt = fmaxf(a.dt, 1e-8f);
+    const float4 x = X[i];
+    const float w = P[i].w;
+    float4 v = V[i];
+    f3 p;
+    if (w <= 0.0f) {
+        v.x = 0.0f; v.y = 0.0f; v.z = 0.0f;
+        p = xyz(x);
+    } else {
+        const float keep = fmaxf(0.0f, 1.0f - a.damping * dt);
+        v.x = v.x * keep + a.gx * dt;
+        v.y = v.y * keep + a.gy * dt;
+        v.z = v.z * keep + a.gz * dt;
+        p = mk3(x.x + v.x * dt, x.y + v.y * dt, x.z + v.z * dt);
+    }
+    V[i] = v;
+    P[i] = make_float4(p.x, p.y, p.z, w);
+}
+
+template <bool FIRST, bool DAMP, bool SPLIT>
+__global__ void __launch_bounds__(256) k_distance(float4 *__restrict__ P, const float4 *__restrict__ X,
+                                                  const uint4 *__restrict__ rec, const float *__restrict__ ccomp,
+                                                  float *__restrict__ lam, uint32_t off, uint32_t cnt,
+                                                  SubstepArgs a) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const uint32_t k = off + t;
+    const uint4 r = rec[k];
+    const uint32_t i0 = r.x, i1 = r.y & 0x7FFFFFFFu;
+    const float4 q0 = P[i0], q1 = P[i1];
+    const float w0 = q0.w, w1 = q1.w, wsum = w0 + w1;
+    if (wsum <= 1e-7f) return;
+    const f3 delta = xyz(q0) - xyz(q1);
+    const float dist = length(delta);
+    if (dist <= 1e-7f) return;
+    const float c = dist - __uint_as_float(r.z);
+    float comp = __uint_as_float(r.w);
+    if (SPLIT && !(c >= 0.0f)) comp = ccomp[k];
+    if (comp >= 1e9f) return;
+    const f3 dir = delta / dist;
+    const float dt = fmaxf(a.dt, 1e-8f);
+    const float alpha = comp / (dt * dt);
+    const float l = FIRST ? 0.0f : lam[k];
+    float dl;
+    if (DAMP) {
+        const bool shear = (r.y >> 31) != 0u;
+        const float dc = shear ? a.shear_damping : (c >= 0.0f ? a.tension_damping : a.compression_damping);
+        const float gamma = fminf(dc * 1e-5f / dt, 1e4f);
+        const f3 rel = (xyz(q0) - xyz(X[i0])) - (xyz(q1) - xyz(X[i1]));
+        const float vproj = dot(dir, rel);
+        dl = (-c - alpha * l - gamma * vproj) / ((1.0f + gamma) * wsum + alpha);
+    } else {
+        dl = (-c - alpha * l) / (wsum + alpha);
+    }
+    lam[k] = l + dl;
+    const f3 c0 = dir * (w0 * dl);
+    const f3 c1 = dir * (-w1 * dl);
+    P[i0] = make_float4(q0.x + c0.x, q0.y + c0.y, q0.z + c0.z, w0);
+    P[i1] = make_float4(q1.x + c1.x, q1.y + c1.y, q1.z + c1.z, w1);
+}
+
+template <bool FIRST, bool DAMP>
+__global__ void __launch_bounds__(256) k_bending(float4 *__restrict__ P, const float4 *__restrict__ X,
+                                                 const uint4 *__restrict__ rec, float *__restrict__ lam,
+                                                 uint32_t off, uint32_t cnt, SubstepArgs a) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const uint32_t k = off + t;
+    const uint4 r = rec[k];
+    const float comp = __uint_as_float(r.w);
+    if (comp >= 1e9f) return;
+    const uint32_t i2 = r.x, i3 = r.y;
+    const float4 q2 = P[i2], q3 = P[i3];
+    const float w2 = q2.w, w3 = q3.w, wsum = w2 + w3;
+    if (wsum <= 1e-7f) return;
+    const f3 delta = xyz(q2) - xyz(q3);
+    const float dist = length(delta);
+    if (dist <= 1e-7f) return;
+    const f3 dir = delta / dist;
+    const float dt = fmaxf(a.dt, 1e-8f);
+    const float alpha = comp / (dt * dt);
+    const float c = dist - __uint_as_float(r.z);
+    const float l = FIRST ? 0.0f : lam[k];
+    float dl;
+    if (DAMP) {
+        const float gamma = fminf(a.bending_damping * 1e-5f / dt, 1e4f);
+        const f3 rel = (xyz(q2) - xyz(X[i2])) - (xyz(q3) - xyz(X[i3]));
+        const float vproj = dot(dir, rel);
+        dl = (-c - alpha * l - gamma * vproj) / ((1.0f + gamma) * wsum + alpha);
+    } else {
+        dl = (-c - alpha * l) / (wsum + alpha);
+    }
+    lam[k] = l + dl;
+    const f3 c2 = dir * (w2 * dl);
+    const f3 c3 = dir * (-w3 * dl);
+    P[i2] = make_float4(q2.x + c2.x, q2.y + c2.y, q2.z + c2.z, w2);
+    P[i3] = make_float4(q3.x + c3.x, q3.y + c3.y, q3.z + c3.z, w3);
+}
+
+// collision.wgsl:83-108
+__device__ __forceinline__ void contact_response(f3 target, f3 n, float fr, float rest, float dt, f3 p_init, f3 &xn,
+                                                 f3 &p) {
+    const f3 total = target - xn;
+    const f3 tang = total - n * dot(total, n);
+    p = target - tang * clampf(fr, 0.0f, 1.0f);
+    const float edt = fmaxf(dt, 1e-6f);
+    const float vin = dot((p_init - xn) / edt, n);
+    float vout = 0.0f;
+    if (vin < 0.0f) vout = -clampf(rest, 0.0f, 1.0f) * vin;
+    const float dxn = dot(target - xn, n) - vout * edt;
+    xn = xn + n * dxn;
+}
+
+// collision.wgsl:273-319 (closest point and face flag)
+__device__ __forceinline__ f3 closest_on_tri(f3 p, f3 a, f3 b, f3 c, bool &is_face) {
+    is_face = false;
+    const f3 ab = b - a, ac = c - a, ap = p - a;
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) return a;
+    const f3 bp = p - b;
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) return b;
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) return a + ab * (d1 / (d1 - d3));
+    const f3 cp = p - c;
+    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) return c;
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) return a + ac * (d2 / (d2 - d6));
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f)
+        return b + (c - b) * ((d4 - d3) / ((d4 - d3) + (d5 - d6)));
+    const float denom = 1.0f / (va + vb + vc);
+    is_face = true;
+    return a + ab * (vb * denom) + ac * (vc * denom);
+}
+
+// collision.wgsl:322-361 (Moller-Trumbore with 5% edge tolerance)
+__device__ __forceinline__ bool seg_tri(f3 po, f3 pn, f3 a, f3 b, f3 c, float &t_out) {
+    const f3 d = pn - po;
+    const float ld = dot(d, d);
+    if (ld < 1e-12f) return false;
+    const f3 e1 = b - a, e2 = c - a;
+    const f3 pv = cross(d, e2);
+    const float det = dot(e1, pv);
+    const float tn = dot(e1, e1) * dot(e2, e2);
+    if (det * det <= tn * ld * 1e-10f || tn <= 1e-20f) return false;
+    const float inv = 1.0f / det;
+    const f3 tv = po - a;
+    const float u = dot(tv, pv) * inv;
+    if (u < -0.05f || u > 1.05f) return false;
+    const f3 qv = cross(tv, e1);
+    const float v = dot(d, qv) * inv;
+    if (v < -0.05f || (u + v) > 1.05f) return false;
+    const float t = dot(e2, qv) * inv;
+    if (t >= 0.0f && t <= 1.0f) { t_out = t; return true; }
+    return false;
+}
+
+__device__ __forceinline__ float seg_sphere_dist_sq(f3 a, f3 b, f3 c) {  // collision.wgsl:364-374
+    const f3 ab = b - a;
+    const float ab2 = dot(ab, ab);
+    if (ab2 <= 1e-12f) { const f3 d = a - c; return dot(d, d); }
+    const float t = clampf(dot(c - a, ab) / ab2, 0.0f, 1.0f);
+    const f3 d = (a + ab * t) - c;
+    return dot(d, d);
+}
+
+// collision.wgsl:377-652, decoupled (once per substep), direct triangle scan
+// in the Morton order the host sorted the triangles into.
+__global__ void __launch_bounds__(256) k_collision(float4 *__restrict__ P, float4 *__restrict__ X, uint32_t n,
+                                                   const Collider *__restrict__ cols, uint32_t ncols,
+                                                   const MeshTri *__restrict__ tris,
+                                                   const float4 *__restrict__ bounds, uint32_t ntris,
+                                                   int single_sided_recovery, float dt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 q = P[i];
+    if (q.w <= 0.0f) return;
+    const float4 x = X[i];
+    f3 p = xyz(q);
+    const f3 p_init = p, x_old = xyz(x);
+    f3 xn = x_old;
+    const float th = x.w;
+    for (uint32_t c = 0; c < ncols; ++c) {
+        const Collider col = cols[c];
+        const f3 A = mk3(col.ax, col.ay, col.az);
+        if (col.type == 0u) {
+            const float tr = col.radius + th;
+            const f3 d = p - A;
+            const float dist = length(d);
+            if (dist < tr) {
+                f3 nrm = mk3(0.0f, 0.0f, 1.0f);
+                if (dist > 1e-7f) nrm = d / dist;
+                contact_response(A + nrm * tr, nrm, col.friction, col.restitution, dt, p_init, xn, p);
+            }
+        } else if (col.type == 1u) {
+            const f3 B = mk3(col.bx, col.by, col.bz);
+            const float tr = col.radius + th;
+            const f3 ab = B - A;
+            const float l2 = dot(ab, ab);
+            float t = 0.0f;
+            if (l2 > 1e-7f) t = clampf(dot(p - A, ab) / l2, 0.0f, 1.0f);
+            const f3 cl = A + ab * t;
+            const f3 d = p - cl;
+            const float dist = length(d);
+            if (dist < tr) {
+                f3 nrm = mk3(0.0f, 0.0f, 1.0f);
+                if (dist > 1e-7f) nrm = d / dist;
+                contact_response(cl + nrm * tr, nrm, col.friction, col.restitution, dt, p_init, xn, p);
+            }
+        } else if (col.type == 2u) {
+            const f3 B = mk3(col.bx, col.by, col.bz);
+            const f3 nrm = B / length(B);
+            const float dist = dot(p - A, nrm);
+            if (dist < th) contact_response(p + nrm * (th - dist), nrm, col.friction, col.restitution, dt, p_init, xn, p);
+        }
+    }
+    bool front = false, has_rec = false;
+    float best_depth = 999999.0f;
+    f3 best_pos = p, best_n = mk3(0.0f, 0.0f, 0.0f);
+    float best_fr = 0.0f, best_rest = 0.0f;
+    for (uint32_t k = 0; k < ntris; ++k) {
+        const float4 b = bounds[k];
+        const float rr = b.w * 1.5f + th;
+        if (seg_sphere_dist_sq(x_old, p, xyz(b)) > rr * rr) continue;
+        const MeshTri T = tris[k];
+        const f3 t0 = mk3(T.p0x, T.p0y, T.p0z), t1 = mk3(T.p1x, T.p1y, T.p1z), t2 = mk3(T.p2x, T.p2y, T.p2z);
+        const float target = th + T.thickness;
+        const f3 cr = cross(t1 - t0, t2 - t0);
+        const float cl = length(cr);
+        if (cl < 1e-7f) continue;
+        const f3 fn = cr / cl;
+        const bool single = (T.flags & 1u) != 0u;
+        float th_hit;
+        if (seg_tri(x_old, p, t0, t1, t2, th_hit)) {
+            const f3 hit = x_old + (p - x_old) * th_hit;
+            f3 nrm = fn;
+            if (dot(p - x_old, fn) > 0.0f && !single) nrm = -fn;
+            contact_response(hit + nrm * target, nrm, T.friction, T.restitution, dt, p_init, xn, p);
+            front = true;
+        }
+        bool is_face;
+        const f3 qp = closest_on_tri(p, t0, t1, t2, is_face);
+        const f3 d = p - qp;
+        const float dist = length(d);
+        const float sd = dot(d, fn);
+        if (single) {
+            if (sd >= 0.0f) {
+                if (dist < target) {
+                    f3 nrm = fn;
+                    if (dist > 1e-7f) nrm = d / dist;
+                    contact_response(p + nrm * (target - dist), nrm, T.friction, T.restitution, dt, p_init, xn, p);
+                    front = true;
+                }
+            } else if (single_sided_recovery && (T.flags & 2u) == 0u) {
+                if (is_face && dist < target * 3.0f && dist <= fabsf(sd) * 1.1f + 1e-7f && dist < best_depth) {
+                    best_depth = dist;
+                    best_n = fn;
+                    best_pos = p + fn * (target - sd);
+                    best_fr = T.friction;
+                    best_rest = T.restitution;
+                    has_rec = true;
+                }
+            }
+        } else if (dist < target && sd >= -target) {
+            f3 nrm = fn;
+            if (dist > 1e-7f) nrm = d / dist;
+            contact_response(p + nrm * (target - dist), nrm, T.friction, T.restitution, dt, p_init, xn, p);
+            front = true;
+        }
+    }
+    if (!front && has_rec) contact_response(best_pos, best_n, best_fr, best_rest, dt, p_init, xn, p);
+    P[i] = make_float4(p.x, p.y, p.z, q.w);
+    X[i] = make_float4(xn.x, xn.y, xn.z, x.w);
+}
+
+__global__ void __launch_bounds__(256) k_update_vel(const float4 *__restrict__ P, float4 *__restrict__ X,
+                                                    float4 *__restrict__ V, uint32_t n, float dt_sub) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float dt = fmaxf(dt_sub, 1e-8f);
+    const float4 p = P[i];
+    const float4 x = X[i];
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (p.w > 0.0f) {
+        v.x = (p.x - x.x) / dt;
+        v.y = (p.y - x.y) / dt;
+        v.z = (p.z - x.z) / dt;
+    }
+    V[i] = v;
+    X[i] = make_float4(p.x, p.y, p.z, x.w);
+}
+
+__global__ void k_extract(const float4 *__restrict__ X, float *__restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 x = X[i];
+    out[3 * i] = x.x;
+    out[3 * i + 1] = x.y;
+    out[3 * i + 2] = x.z;
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+// Residual h_j = C_j + alpha~_j lambda_j over distance and bending constraints.
+__device__ __forceinline__ void atomic_max_double_nonneg(double *addr, double v) {
+    unsigned long long *a = reinterpret_cast<unsigned long long *>(addr);
+    unsigned long long old = *a, assumed;
+    do {
+        assumed = old;
+        if (__longlong_as_double(assumed) >= v) return;
+        old = atomicCAS(a, assumed, (unsigned long long)__double_as_longlong(v));
+    } while (old != assumed);
+}
+
+__global__ void __launch_bounds__(256) k_residual(const float4 *__restrict__ P, const uint4 *__restrict__ drec,
+                                                  const float *__restrict__ dcc, const float *__restrict__ dlam,
+                                                  uint32_t nd, const uint4 *__restrict__ brec,
+                                                  const float *__restrict__ blam, uint32_t nb, float dt_sub,
+                                                  int split, double *__restrict__ out) {
+    __shared__ double sh[2][256];
+    __shared__ double smax[256];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    double hh = 0.0, ll = 0.0, mx = 0.0;
+    const double dt = dt_sub > 1e-8f ? (double)dt_sub : 1e-8;
+    if (t < nd + nb) {
+        const bool isd = t < nd;
+        const uint32_t k = isd ? t : t - nd;
+        const uint4 r = isd ? drec[k] : brec[k];
+        const uint32_t a = r.x, b = isd ? (r.y & 0x7FFFFFFFu) : r.y;
+        const float4 qa = P[a], qb = P[b];
+        const double L = __uint_as_float(r.z);
+        const double ws = (double)qa.w + (double)qb.w;
+        const double dx = (double)qa.x - qb.x, dy = (double)qa.y - qb.y, dz = (double)qa.z - qb.z;
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        double comp = __uint_as_float(r.w);
+        if (isd && split && d - L < 0.0) comp = dcc[k];
+        if (ws > 1e-7 && d > 1e-7 && comp < 1e9) {
+            const double h = (d - L) + comp / (dt * dt) * (double)(isd ? dlam[k] : blam[k]);
+            hh = h * h;
+            ll = L * L;
+            if (L > 0.0) mx = fabs(h) / L;
+        }
+    }
+    sh[0][threadIdx.x] = hh;
+    sh[1][threadIdx.x] = ll;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&out[0], sh[0][0]);
+        atomicAdd(&out[1], sh[1][0]);
+        atomic_max_double_nonneg(&out[2], smax[0]);
+    }
+}
+
+static inline uint32_t grid_for(size_t n, uint32_t block) { return uint32_t((n + block - 1) / block); }
+
+hipError_t launch_predict(hipStream_t s, float4 *P, const float4 *X, float4 *V, uint32_t n, const SubstepArgs &a,
+                          uint32_t block) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_predict, dim3(grid_for(n, block)), dim3(block), 0, s, P, X, V, n, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_distance(hipStream_t s, float4 *P, const float4 *X, const uint4 *rec, const float *cc, float *lam,
+                           uint32_t off, uint32_t cnt, const SubstepArgs &a, bool first, bool split, uint32_t block) {
+    if (!cnt) return hipSuccess;
+    const bool damp = a.tension_damping != 0.0f || a.compression_damping != 0.0f || a.shear_damping != 0.0f;
+    const dim3 g(grid_for(cnt, block)), b(block);
+#define TC_DIST(F, D, S) hipLaunchKernelGGL((k_distance<F, D, S>), g, b, 0, s, P, X, rec, cc, lam, off, cnt, a)
+    if (first) {
+        if (damp) { if (split) TC_DIST(true, true, true); else TC_DIST(true, true, false); }
+        else { if (split) TC_DIST(true, false, true); else TC_DIST(true, false, false); }
+    } else {
+        if (damp) { if (split) TC_DIST(false, true, true); else TC_DIST(false, true, false); }
+        else { if (split) TC_DIST(false, false, true); else TC_DIST(false, false, false); }
+    }
+#undef TC_DIST
+    return hipGetLastError();
+}
+
+hipError_t launch_bending(hipStream_t s, float4 *P, const float4 *X, const uint4 *rec, float *lam, uint32_t off,
+                          uint32_t cnt, const SubstepArgs &a, bool first, uint32_t block) {
+    if (!cnt) return hipSuccess;
+    const bool damp = a.bending_damping != 0.0f;
+    const dim3 g(grid_for(cnt, block)), b(block);
+    if (first) {
+        if (damp) hipLaunchKernelGGL((k_bending<true, true>), g, b, 0, s, P, X, rec, lam, off, cnt, a);
+        else hipLaunchKernelGGL((k_bending<true, false>), g, b, 0, s, P, X, rec, lam, off, cnt, a);
+    } else {
+        if (damp) hipLaunchKernelGGL((k_bending<false, true>), g, b, 0, s, P, X, rec, lam, off, cnt, a);
+        else hipLaunchKernelGGL((k_bending<false, false>), g, b, 0, s, P, X, rec, lam, off, cnt, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_collision(hipStream_t s, float4 *P, float4 *X, uint32_t n, const Collider *cols, uint32_t ncols,
+                            const MeshTri *tris, const float4 *bounds, uint32_t ntris, int recovery, float dt,
+                            uint32_t block) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_collision, dim3(grid_for(n, block)), dim3(block), 0, s, P, X, n, cols, ncols, tris, bounds,
+                       ntris, recovery, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_vel(hipStream_t s, const float4 *P, float4 *X, float4 *V, uint32_t n, float dt,
+                             uint32_t block) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_update_vel, dim3(grid_for(n, block)), dim3(block), 0, s, P, X, V, n, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_extract_positions(hipStream_t s, const float4 *X, float *out, uint32_t n) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_extract, dim3(grid_for(n, 256)), dim3(256), 0, s, X, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t value, size_t n) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, value, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(hipStream_t s, const float4 *P, const uint4 *drec, const float *dcc, const float *dlam,
+                           uint32_t nd, const uint4 *brec, const float *blam, uint32_t nb, float dt, bool split,
+                           double *out) {
+    const size_t n = size_t(nd) + nb;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_residual, dim3(grid_for(n, 256)), dim3(256), 0, s, P, drec, dcc, dlam, nd, brec, blam, nb,
+                       dt, split ? 1 : 0, out);
+    return hipGetLastError();
+}
+
+}  // namespace tc

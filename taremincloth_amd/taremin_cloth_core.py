@@ -1,0 +1,351 @@
+"""Drop-in mirror of the reference's PyO3 module ``taremin_cloth_core`` for the substep path.
+
+Same class and method names, argument meaning and error behaviour as
+crates/cloth_py/src/lib.rs (ClothSimulator at :1549-3258, module functions at
+:636-648, :552 and :3539-3580); the work runs in libtcloth_hip.so on the GPU.
+Shape and index errors raise ``ValueError`` before any GPU work, device
+failures raise ``RuntimeError`` (lib.rs:1590-1718).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+__all__ = ["ClothSimulator", "color_edge_pairs", "compute_combined_inv_masses"]
+
+
+def _f32(a, shape_tail=None, name="array"):
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    if shape_tail is not None:
+        if arr.ndim != 1 + len(shape_tail) or tuple(arr.shape[1:]) != tuple(shape_tail):
+            raise ValueError(f"{name} must have shape (N, {', '.join(map(str, shape_tail))})")
+    return arr
+
+
+def _u32(a, cols, name):
+    arr = np.asarray(a)
+    if arr.ndim != 2 or arr.shape[1] != cols:
+        raise ValueError(f"{name} must have shape (N, {cols})")
+    if arr.size and np.issubdtype(arr.dtype, np.signedinteger) and (arr < 0).any():
+        raise ValueError(f"{name} contains negative indices")
+    return np.ascontiguousarray(arr, dtype=np.uint32)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _vec3(v, name):
+    a = np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(-1))
+    if a.size != 3:
+        raise ValueError(f"{name} must have 3 components")
+    return a
+
+
+class ClothSimulator:
+    """GPU XPBD cloth simulator (graph-coloured distance/shear/bending, colliders, self-collision)."""
+
+    def __init__(self, positions, edges, faces=None, inv_masses=None, sewing_springs=None, layer_ids=None,
+                 thicknesses=None, layer_id=0, thickness=0.005, stiffness=1000.0, bending_stiffness=10.0,
+                 sewing_shrink_speed=1.0, sewing_stiffness=None, enable_sewing_lock=None, sewing_lock_distance=None,
+                 compression_stiffness=None, shear_stiffness=None, workgroup_size=32, solver_mode=0,
+                 enable_compact_readback=None, enable_pair_cache=None, areal_density=None,
+                 enable_coarse_constraints=None, self_collision_algorithm=None):
+        L = _lib.load()
+        self._L = L
+        self._h = None
+        pos = _f32(positions, (3,), "positions")
+        e = _u32(edges, 2, "edges")
+        n = len(pos)
+        if e.size and int(e.max()) >= n:
+            bad = e[(e >= n).any(axis=1)][0]
+            raise ValueError(f"edges contain an index >= number of vertices ({n}): [{bad[0]}, {bad[1]}]")
+        f = None
+        if faces is not None:
+            f = _u32(faces, 3, "faces")
+            if f.size and int(f.max()) >= n:
+                raise ValueError(f"faces contain an index >= number of vertices ({n})")
+            if f.size == 0:
+                f = None
+        if sewing_springs is not None and np.asarray(sewing_springs).size:
+            raise NotImplementedError("sewing springs are not part of this build")
+        if enable_coarse_constraints:
+            raise NotImplementedError("coarse (two-hop) constraints are not part of this build")
+        if self_collision_algorithm not in (None, 0):
+            raise NotImplementedError("only self_collision_algorithm=0 (DIRECT) is part of this build")
+        if enable_pair_cache:
+            raise NotImplementedError("the pair-cache self-collision path is not part of this build")
+        if solver_mode != 0:
+            raise NotImplementedError("only solver_mode=0 (graph colouring) is part of this build")
+
+        def vec(a, dtype, name):
+            if a is None:
+                return None
+            v = np.ascontiguousarray(np.asarray(a, dtype=dtype).reshape(-1))
+            if len(v) != n:
+                raise ValueError(f"{name} length ({len(v)}) does not match the number of vertices ({n})")
+            return v
+
+        im = vec(inv_masses, np.float32, "inv_masses")
+        ly = vec(layer_ids, np.uint32, "layer_ids")
+        th = vec(thicknesses, np.float32, "thicknesses")
+        p = _lib.TcParams()
+        L.tc_sim_params_default(C.byref(p))
+        p.thickness = float(thickness)
+        p.layer_id = int(layer_id)
+        p.stiffness = float(stiffness)
+        p.bending_stiffness = float(bending_stiffness)
+        p.compression_stiffness = -1.0 if compression_stiffness is None else float(compression_stiffness)
+        p.shear_stiffness = -1.0 if shear_stiffness is None else float(shear_stiffness)
+        # The reference's workgroup_size selects 32/64-wide WGSL groups; wave64 hardware
+        # uses its own tuned block size, so the value is accepted and not forwarded.
+        p.workgroup_size = 0
+        p.solver_mode = 0
+        h = C.c_void_p()
+        _lib.check(L.tc_sim_create(_ptr(pos), n, _ptr(e), len(e), _ptr(f), 0 if f is None else len(f), _ptr(im),
+                                   _ptr(th), _ptr(ly), C.byref(p), C.byref(h)))
+        self._h = h
+        self._n = n
+        self._keep = (pos, e, f)
+        # set_areal_density only records the value (simulation/mod.rs:1241-1245); the substep
+        # uses the inverse masses given at construction.
+        self.areal_density = 0.15 if areal_density is None else float(min(max(areal_density, 0.0), 100.0))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.tc_sim_destroy(h)
+            self._h = None
+
+    # --- stepping ----------------------------------------------------------------------------
+    def step(self, dt=0.016666667, substeps=None, solver_iterations=None) -> int:
+        eff = C.c_uint32(0)
+        it = 0 if solver_iterations is None else int(solver_iterations)
+        if solver_iterations is not None:
+            _lib.check(self._L.tc_sim_set_solver_iterations(self._h, it))
+        _lib.check(self._L.tc_sim_step(self._h, float(dt), 0 if substeps is None else int(substeps), 0,
+                                       C.byref(eff)))
+        return int(eff.value)
+
+    def step_single_substep(self, dt_sub=0.0016666667):
+        _lib.check(self._L.tc_sim_step_single_substep(self._h, float(dt_sub)))
+
+    def get_positions(self, out):
+        if not isinstance(out, np.ndarray) or out.dtype != np.float32 or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float32 numpy array")
+        if out.size < 3 * self._n:
+            raise ValueError(f"out must hold at least {3 * self._n} floats")
+        _lib.check(self._L.tc_sim_get_positions(self._h, out.ctypes.data_as(C.c_void_p), out.size))
+
+    def set_positions_and_velocities(self, positions, velocities=None):
+        pos = _f32(positions).reshape(-1)
+        vel = None if velocities is None else _f32(velocities).reshape(-1)
+        if pos.size % 3 or (vel is not None and vel.size != pos.size):
+            raise ValueError("positions / velocities must be (N, 3)")
+        _lib.check(self._L.tc_sim_set_positions_velocities(self._h, _ptr(pos), _ptr(vel), pos.size // 3))
+
+    def reset(self):
+        _lib.check(self._L.tc_sim_reset(self._h))
+
+    # --- colliders -----------------------------------------------------------------------------
+    def add_sphere_collider(self, center, radius, friction=0.0, restitution=0.0):
+        _lib.check(self._L.tc_sim_add_sphere(self._h, _ptr(_vec3(center, "center")), float(radius), float(friction),
+                                             float(restitution)))
+
+    def add_capsule_collider(self, a, b, radius, friction=0.0, restitution=0.0):
+        _lib.check(self._L.tc_sim_add_capsule(self._h, _ptr(_vec3(a, "a")), _ptr(_vec3(b, "b")), float(radius),
+                                              float(friction), float(restitution)))
+
+    def add_plane_collider(self, point, normal, friction=0.0, restitution=0.0):
+        _lib.check(self._L.tc_sim_add_plane(self._h, _ptr(_vec3(point, "point")), _ptr(_vec3(normal, "normal")),
+                                            float(friction), float(restitution)))
+
+    def set_mesh_collider_triangles(self, tris, friction=0.0, thickness=0.0, restitution=0.0, single_sided=False,
+                                    attributes=None):
+        t = _f32(tris)
+        if t.ndim != 3 or t.shape[1:] != (3, 3):
+            raise ValueError("tris must have shape (T, 3, 3)")
+        at = None
+        if attributes is not None:
+            at = _f32(attributes)
+            if at.shape != (len(t), 4):
+                raise ValueError("attributes must have shape (T, 4)")
+        _lib.check(self._L.tc_sim_set_mesh_triangles(self._h, _ptr(t), len(t), _ptr(at), float(friction),
+                                                     float(thickness), float(restitution), 1 if single_sided else 0))
+
+    def clear_colliders(self):
+        _lib.check(self._L.tc_sim_clear_colliders(self._h))
+
+    # --- parameters ----------------------------------------------------------------------------
+    def set_gravity(self, gx, gy, gz):
+        _lib.check(self._L.tc_sim_set_gravity(self._h, float(gx), float(gy), float(gz)))
+
+    def set_damping(self, damping):
+        _lib.check(self._L.tc_sim_set_damping(self._h, float(damping)))
+
+    def set_damping_all(self, tension, compression, shear, bending):
+        _lib.check(self._L.tc_sim_set_damping_all(self._h, float(tension), float(compression), float(shear),
+                                                  float(bending)))
+
+    def set_solver_iterations(self, iterations):
+        _lib.check(self._L.tc_sim_set_solver_iterations(self._h, int(iterations)))
+
+    def set_enable_self_collision(self, enable):
+        _lib.check(self._L.tc_sim_set_enable_self_collision(self._h, 1 if enable else 0))
+
+    def set_self_collision_options(self, relief_factor, max_displacement_ratio, exclude_neighbors=True,
+                                   enable_normal_untangling=True, max_iterations=256):
+        _lib.check(self._L.tc_sim_set_self_collision_options(self._h, float(relief_factor),
+                                                             float(max_displacement_ratio),
+                                                             1 if exclude_neighbors else 0,
+                                                             1 if enable_normal_untangling else 0,
+                                                             int(max_iterations)))
+
+    def set_stiffness_all(self, tension, compression, shear, bending):
+        _lib.check(self._L.tc_sim_set_stiffness_all(self._h, float(tension), float(compression), float(shear),
+                                                    float(bending)))
+
+    def set_workgroup_size(self, size):
+        """Accepted for API compatibility; the block size is tuned per kernel on gfx950."""
+
+    # extensions (no reference counterpart)
+    def set_compliances_raw(self, stretch, shear, bend):
+        _lib.check(self._L.tc_sim_set_compliances_raw(self._h, float(stretch), float(shear), float(bend)))
+
+    def set_hash_cell_size(self, cell_size):
+        _lib.check(self._L.tc_sim_set_hash_cell_size(self._h, float(cell_size)))
+
+    # --- getters -------------------------------------------------------------------------------
+    def num_vertices(self) -> int:
+        return int(self._L.tc_sim_num_vertices(self._h))
+
+    def num_distance_constraints(self) -> int:
+        return int(self._L.tc_sim_num_distance_constraints(self._h))
+
+    def num_bending_constraints(self) -> int:
+        return int(self._L.tc_sim_num_bending_constraints(self._h))
+
+    # --- introspection used by parity checks ---------------------------------------------------
+    _FLOAT_ARRAYS = {8, 9, 13, 14, 15, 16}
+
+    def _array(self, which: int) -> np.ndarray:
+        cnt = C.c_size_t(0)
+        _lib.check(self._L.tc_sim_get_array(self._h, which, None, 0, C.byref(cnt)))
+        out = np.empty(cnt.value, np.float32 if which in self._FLOAT_ARRAYS else np.uint32)
+        if cnt.value:
+            _lib.check(self._L.tc_sim_get_array(self._h, which, out.ctypes.data_as(C.c_void_p), out.size,
+                                                C.byref(cnt)))
+        return out
+
+    def colouring(self) -> dict[str, np.ndarray]:
+        return {"dist_v0": self._array(0), "dist_v1": self._array(1), "dist_col_off": self._array(2),
+                "dist_col_cnt": self._array(3), "bend_v2": self._array(4), "bend_v3": self._array(5),
+                "bend_col_off": self._array(6), "bend_col_cnt": self._array(7), "dist_rest": self._array(8),
+                "bend_rest": self._array(9)}
+
+    def hash_tables(self, rebuild=True) -> dict[str, np.ndarray]:
+        if rebuild:
+            _lib.check(self._L.tc_sim_build_hashes(self._h))
+        sc = self._array(17)
+        return {"keys": self._array(10), "starts": self._array(11), "sorted": self._array(12),
+                "edge_keys": self._array(19), "edge_starts": self._array(20), "edge_sorted": self._array(21),
+                "edges": self._array(18).reshape(-1, 2), "cell_size": float(sc[:1].view(np.float32)[0]),
+                "table_size": int(sc[1]), "n_virtual": int(sc[2])}
+
+    def predicted_positions(self) -> np.ndarray:
+        return self._array(13).reshape(-1, 3)
+
+    def velocities(self) -> np.ndarray:
+        return self._array(14).reshape(-1, 3)
+
+    def lambdas(self):
+        return self._array(15), self._array(16)
+
+    def substep_parts(self, dt_sub, collide=True, self_collide=True, update=True):
+        _lib.check(self._L.tc_sim_substep_parts(self._h, float(dt_sub), int(collide), int(self_collide), int(update)))
+
+    def residual(self, dt_sub):
+        a, b = C.c_double(), C.c_double()
+        _lib.check(self._L.tc_sim_residual(self._h, float(dt_sub), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def time_kernels(self, dt_sub, reps=5) -> dict:
+        """Per-substep kernel times (ms) from HIP events on the simulator's stream."""
+        out = (C.c_double * 10)()
+        _lib.check(self._L.tc_sim_time_kernels(self._h, float(dt_sub), int(reps), out))
+        keys = ["predict", "distance", "distance_launches", "bending", "bending_launches", "collision",
+                "self_collision", "update_vel", "substep_wall"]
+        return dict(zip(keys, list(out)[:9]))
+
+    def synchronize(self):
+        _lib.check(self._L.tc_sim_synchronize(self._h))
+
+
+def color_edge_pairs(n, edges) -> list:
+    """Welsh-Powell colour per vertex pair (lib.rs:636-648 -> coloring.rs:180)."""
+    L = _lib.load()
+    e = _u32(edges, 2, "edges")
+    out = np.empty(len(e), np.uint32)
+    _lib.check(L.tc_color_edge_pairs(int(n), _ptr(e), len(e), _ptr(out)))
+    return out.tolist()
+
+
+def compute_combined_inv_masses(positions, faces=None, strand_edges=None, pin_weights=None, areal_density=0.15,
+                                linear_density=0.0) -> np.ndarray:
+    """Area/3 mass split with pin weights (mesh.rs:237-312); strands are not part of this build."""
+    if strand_edges is not None and np.asarray(strand_edges).size and linear_density > 0:
+        raise NotImplementedError("strand masses are not part of this build")
+    L = _lib.load()
+    pos = _f32(positions, (3,), "positions")
+    f = None if faces is None else _u32(faces, 3, "faces")
+    pw = None if pin_weights is None else _f32(pin_weights).reshape(-1)
+    if pw is not None and len(pw) != len(pos):
+        raise ValueError("pin_weights length does not match the number of vertices")
+    out = np.empty(len(pos), np.float32)
+    _lib.check(L.tc_compute_inv_masses(_ptr(pos), len(pos), _ptr(f), 0 if f is None else len(f), _ptr(pw),
+                                       float(areal_density), _ptr(out)))
+    return out
+
+
+_MESH_FLOAT_ARRAYS = {8, 9, 24, 25, 32, 33, 34}
+MESH_ARRAYS = {
+    "dist_v0": 0, "dist_v1": 1, "dist_col_off": 2, "dist_col_cnt": 3, "bend_v2": 4, "bend_v3": 5,
+    "bend_col_off": 6, "bend_col_cnt": 7, "dist_rest": 8, "bend_rest": 9, "scalars": 17, "edges": 18,
+    "th_off": 22, "th_idx": 23, "th_rest": 24, "local_len": 25, "star_off": 26, "star": 27, "edge_bbits": 28,
+    "island": 29, "edge_to_constraint": 30, "dist_type": 31, "dist_tc": 32, "dist_cc": 33, "bend_comp": 34,
+    "adj_off": 35, "adj_idx": 36, "layer": 37,
+}
+
+
+def build_mesh_arrays(positions, edges, faces=None, inv_masses=None, thicknesses=None, thickness=0.005,
+                      stiffness=1000.0, bending_stiffness=10.0, compression_stiffness=None,
+                      shear_stiffness=None, layer_id=0) -> dict:
+    """Host-side mesh build of the library (no GPU needed): constraints, colours, topology."""
+    L = _lib.load()
+    pos = _f32(positions, (3,), "positions")
+    e = _u32(edges, 2, "edges")
+    f = None if faces is None else _u32(faces, 3, "faces")
+    im = None if inv_masses is None else _f32(inv_masses).reshape(-1)
+    th = None if thicknesses is None else _f32(thicknesses).reshape(-1)
+    p = _lib.TcParams()
+    L.tc_sim_params_default(C.byref(p))
+    p.thickness, p.layer_id, p.stiffness, p.bending_stiffness = thickness, layer_id, stiffness, bending_stiffness
+    p.compression_stiffness = -1.0 if compression_stiffness is None else compression_stiffness
+    p.shear_stiffness = -1.0 if shear_stiffness is None else shear_stiffness
+    out = {}
+    for name, which in MESH_ARRAYS.items():
+        cnt = C.c_size_t(0)
+        args = (_ptr(pos), len(pos), _ptr(e), len(e), _ptr(f), 0 if f is None else len(f), _ptr(im), _ptr(th),
+                C.byref(p), which)
+        _lib.check(L.tc_mesh_build_arrays(*args, None, 0, C.byref(cnt)))
+        arr = np.empty(cnt.value, np.float32 if which in _MESH_FLOAT_ARRAYS else np.uint32)
+        if cnt.value:
+            _lib.check(L.tc_mesh_build_arrays(*args, arr.ctypes.data_as(C.c_void_p), arr.size, C.byref(cnt)))
+        out[name] = arr
+    sc = out["scalars"]
+    out["cell_size"] = float(sc[:1].view(np.float32)[0])
+    out["table_size"], out["n_virtual"], out["ne"] = int(sc[1]), int(sc[2]), int(sc[3])
+    return out

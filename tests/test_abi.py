@@ -1,0 +1,75 @@
+"""The C-ABI library loads and exports every symbol include/tc_sim.h declares;
+argument errors surface as the reference's exception types before any GPU work."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cases
+from taremincloth_amd import _lib
+from taremincloth_amd import taremin_cloth_core as T
+
+HEADER = os.path.join(cases.ROOT, "include", "tc_sim.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tc_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declarations_are_exported():
+    L = _lib.load()
+    names = _declared()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(_lib.EXPORTS) == names
+
+
+def test_version_and_error_channel():
+    L = _lib.load()
+    assert b"gfx950" in L.tc_version()
+    assert L.tc_last_error() is not None
+
+
+def test_bad_edge_index_raises_value_error():
+    pos, e, f = cases.grid(3, 3, 0.1)
+    bad = e.copy()
+    bad[0, 1] = 99
+    with pytest.raises(ValueError):
+        T.ClothSimulator(pos, bad, f)
+
+
+def test_bad_shapes_raise_value_error():
+    pos, e, f = cases.grid(3, 3, 0.1)
+    with pytest.raises(ValueError):
+        T.ClothSimulator(pos[:, :2], e, f)
+    with pytest.raises(ValueError):
+        T.ClothSimulator(pos, e[:, :1], f)
+    with pytest.raises(ValueError):
+        T.ClothSimulator(pos, e, f, inv_masses=np.ones(3, np.float32))
+    with pytest.raises(ValueError):
+        T.ClothSimulator(pos, e, np.array([[0, 1, 99]], np.uint32))
+
+
+def test_host_mesh_build_rejects_bad_faces_through_abi():
+    L = _lib.load()
+    pos = np.zeros((3, 3), np.float32)
+    e = np.array([[0, 1]], np.uint32)
+    f = np.array([[0, 1, 7]], np.uint32)
+    cnt = C.c_size_t()
+    st = L.tc_mesh_build_arrays(pos.ctypes.data_as(C.c_void_p), 3, e.ctypes.data_as(C.c_void_p), 1,
+                                f.ctypes.data_as(C.c_void_p), 1, None, None, None, 0, None, 0, C.byref(cnt))
+    assert st == _lib.TC_ERR_VALUE
+    assert b"faces" in L.tc_last_error()
+
+
+def test_unsupported_options_raise():
+    pos, e, f = cases.grid(3, 3, 0.1)
+    with pytest.raises(NotImplementedError):
+        T.ClothSimulator(pos, e, f, sewing_springs=np.array([[0, 8]], np.uint32))
+    with pytest.raises(NotImplementedError):
+        T.ClothSimulator(pos, e, f, solver_mode=1)

@@ -1,0 +1,237 @@
+"""Parity of the HIP substep (through the C ABI) with the oracle and the reference's golden fixtures.
+
+Tolerances: fp32 positions within 1e-5 m max-abs of the fp32 oracle after 1 and
+100 substeps; spatial-hash keys, cell starts and sorted indices bit-exact;
+golden single-step fixtures within 1e-5 m max L2 (the reference's own bound).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+from oracle import oracle as O
+from taremincloth_amd import taremin_cloth_core as T
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _say(msg):
+    print(f"[gpu-parity] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(autouse=True)
+def _gpu(require_gpu):
+    yield
+
+
+def _pos(sim):
+    out = np.empty(3 * sim.num_vertices(), np.float32)
+    sim.get_positions(out)
+    return out.reshape(-1, 3)
+
+
+@pytest.mark.parametrize("case", ["case1", "case2", "case3"])
+def test_golden_single_step(case):
+    _say(f"golden single step {case}")
+    c = cases.GOLDEN_CASES[case]
+    step = cases.load_npz(c["step"])
+    sim, _, _ = cases.make_golden_sim(T.ClothSimulator, case, positions=step.get("pos_in"))
+    assert sim.step(1.0 / 60.0, 20, 2) == 20
+    diff = float(np.linalg.norm(_pos(sim) - step["pos_out"], axis=1).max())
+    assert diff < TOL, f"{case}: {diff * 1e6:.3f} um"
+
+
+@pytest.mark.parametrize("case", ["case1", "case4"])
+def test_golden_trajectory_matches_oracle(case):
+    _say(f"golden trajectory vs oracle {case}")
+    g, d, _ = cases.make_golden_sim(T.ClothSimulator, case)
+    o, _, _ = cases.make_golden_sim(O.OracleSim, case)
+    for fi in range(1, 9):
+        g.step(1.0 / 60.0, 20, 2)
+        o.step(1.0 / 60.0, 20, 2)
+        diff = float(np.abs(_pos(g) - o.get_positions().reshape(-1, 3)).max())
+        assert diff < TOL, f"{case} frame {fi}: {diff:.3e}"
+        assert float(np.linalg.norm(_pos(g) - d["positions"][fi], axis=1).max()) < 1e-3
+
+
+def _config1_pair(iters=10):
+    pos, e, f, inv = cases.config1()
+    sims = []
+    for factory in (T.ClothSimulator, O.OracleSim, lambda *a, **k: O.OracleSim(*a, precision="f64", **k)):
+        s = factory(pos, e, f, inv, thickness=0.005, stiffness=1000.0, bending_stiffness=10.0)
+        s.set_compliances_raw(1e-8, 1e-6, 1e-5)
+        s.set_gravity(0.0, 0.0, -9.81)
+        s.set_enable_self_collision(False)
+        s.set_solver_iterations(iters)
+        sims.append(s)
+    return sims
+
+
+def test_config1_one_and_hundred_substeps():
+    """32x32 poked grid, raw compliances 1e-8 / 1e-6 / 1e-5, 10 iterations per substep:
+    after 1 substep and after 100 substeps (10 frames x 10 substeps)."""
+    _say("config1 1 substep")
+    g, o32, o64 = _config1_pair()
+    dt_sub = (1.0 / 60.0) / 10.0
+    for s in (g, o32, o64):
+        s.step_single_substep(dt_sub)
+    e1 = float(np.abs(_pos(g) - o32.get_positions().reshape(-1, 3)).max())
+    e1_64 = float(np.abs(_pos(g).astype(np.float64) - o64.state(0)).max())
+    assert e1 < TOL, e1
+    _say("config1 100 substeps")
+    g, o32, o64 = _config1_pair()
+    for _ in range(10):
+        for s in (g, o32, o64):
+            s.step(1.0 / 60.0, 10, 10)
+    e100 = float(np.abs(_pos(g) - o32.get_positions().reshape(-1, 3)).max())
+    e100_64 = float(np.abs(_pos(g).astype(np.float64) - o64.state(0)).max())
+    _say(f"config1 max-abs vs fp32 oracle: 1 substep {e1:.2e}, 100 substeps {e100:.2e}; "
+         f"vs fp64 oracle: {e1_64:.2e}, {e100_64:.2e}")
+    assert e100 < TOL, e100
+    assert np.abs(g.velocities() - o32.state(2)).max() < 1e-2
+
+
+def test_lambdas_and_residual_match_oracle():
+    _say("lambdas and residual")
+    g, o32, _ = _config1_pair(iters=4)
+    dt_sub = (1.0 / 60.0) / 10.0
+    for _ in range(3):
+        g.step_single_substep(dt_sub)
+        o32.step_single_substep(dt_sub)
+    g.substep_parts(dt_sub, True, True, False)
+    o32.substep_parts(dt_sub, True, True, False)
+    ld_g, lb_g = g.lambdas()
+    ld_o, lb_o = o32.lambdas()
+    scale = max(np.abs(ld_o).max(), 1e-12)
+    assert np.abs(ld_g - ld_o).max() / scale < 1e-3
+    rg, mg = g.residual(dt_sub)
+    ro, mo = o32.residual(dt_sub)
+    assert abs(rg - ro) <= 1e-3 * ro + 1e-12, (rg, ro)
+
+
+def _rand_points(n, seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    return (rng.random((n, 3)) * scale - scale / 2).astype(np.float32)
+
+
+@pytest.mark.parametrize("n,seed", [(500, 1), (4096, 2)])
+def test_hash_tables_bit_exact(n, seed):
+    """Vertex and edge hash keys, cell starts and sorted indices are bit-identical to the oracle."""
+    _say(f"hash tables n={n}")
+    pos, e, f = cases.grid(int(np.sqrt(n)), int(np.sqrt(n)), 0.01)
+    sim = T.ClothSimulator(pos, e, f)
+    pts = _rand_points(len(pos), seed)
+    sim.set_positions_and_velocities(pts)
+    ht = sim.hash_tables()
+    keys, starts, srt = O.hash_points(pts, ht["cell_size"], ht["table_size"])
+    assert np.array_equal(ht["keys"], keys)
+    assert np.array_equal(ht["starts"], starts)
+    assert np.array_equal(ht["sorted"], srt)
+    ed = ht["edges"].astype(np.int64)
+    mid = (pts[ed[:, 0]] + pts[ed[:, 1]]) * np.float32(0.5)
+    k2, s2, o2 = O.hash_points(mid, ht["cell_size"], ht["table_size"])
+    assert np.array_equal(ht["edge_keys"], k2)
+    assert np.array_equal(ht["edge_starts"], s2)
+    assert np.array_equal(ht["edge_sorted"], o2)
+
+
+def test_hash_neighbour_pairs_match_brute_force():
+    """All vertex pairs within one cell size are found among the 27 neighbour cells
+    (canonical i<j pair lists), the same set brute force O(V^2) gives."""
+    _say("hash neighbour pairs")
+    pos, e, f = cases.grid(24, 24, 0.01)
+    sim = T.ClothSimulator(pos, e, f)
+    pts = _rand_points(len(pos), 7, scale=0.12)
+    sim.set_positions_and_velocities(pts)
+    ht = sim.hash_tables()
+    cell, table = ht["cell_size"], ht["table_size"]
+    keys, starts, srt = ht["keys"], ht["starts"], ht["sorted"]
+    cells = np.floor(pts / np.float32(cell)).astype(np.int64)
+    found = set()
+    for i in range(len(pts)):
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    c = cells[i] + (dx, dy, dz)
+                    h = ((np.uint32(c[0] & 0xFFFFFFFF) * np.uint32(73856093)) ^
+                         (np.uint32(c[1] & 0xFFFFFFFF) * np.uint32(19349663)) ^
+                         (np.uint32(c[2] & 0xFFFFFFFF) * np.uint32(83492791))) % np.uint32(table)
+                    for j in srt[starts[h]:starts[h + 1]]:
+                        if j != i and (cells[j] == c).all() and np.linalg.norm(pts[i] - pts[j]) < cell:
+                            found.add((min(i, int(j)), max(i, int(j))))
+    d = np.linalg.norm(pts[:, None, :] - pts[None, :, :], axis=2)
+    ii, jj = np.nonzero(np.triu(d < cell, 1))
+    assert found == set(zip(ii.tolist(), jj.tolist()))
+
+
+def test_self_collision_folded_cloth_matches_oracle():
+    """A cloth folded onto itself: V-V, V-T and E-E contacts in the first substeps."""
+    _say("self collision folded cloth")
+    pos, e, f, inv = cases.folded_cloth()
+    g = T.ClothSimulator(pos, e, f, inv, thickness=0.003, stiffness=1000.0, bending_stiffness=10.0)
+    o = O.OracleSim(pos, e, f, inv, thickness=0.003, stiffness=1000.0, bending_stiffness=10.0)
+    for s in (g, o):
+        s.set_gravity(0.0, 0.0, -9.81)
+        s.set_self_collision_options(1.0, 0.5, True, False, 256)
+    for step in range(3):
+        g.step(1.0 / 60.0, 5, 2)
+        o.step(1.0 / 60.0, 5, 2)
+        diff = float(np.abs(_pos(g) - o.get_positions().reshape(-1, 3)).max())
+        assert diff < TOL, f"frame {step}: {diff:.3e}"
+    st = o.contact_stats()
+    assert st["vv"] + st["vt_prox"] + st["vt_ccd"] > 0 and st["ee_prox"] + st["ee_ccd"] > 0, st
+
+
+def test_drape_on_primitives_with_self_collision_matches_oracle():
+    """64x64 cloth at 4 mm dropping onto a sphere and a capsule with friction 0.3,
+    self-collision on (a reduced version of the 256^2 configuration)."""
+    _say("drape on primitives")
+    pos, e, f, inv = cases.drape_grid(64, 0.004, 0.12, seed=5)
+    g = T.ClothSimulator(pos, e, f, inv, thickness=0.002)
+    o = O.OracleSim(pos, e, f, inv, thickness=0.002)
+    for s in (g, o):
+        s.set_gravity(0.0, 0.0, -9.81)
+        s.add_sphere_collider([0.0, 0.0, 0.0], 0.1, 0.3, 0.0)
+        s.add_capsule_collider([-0.1, 0.05, 0.05], [0.1, 0.05, 0.05], 0.03, 0.3, 0.0)
+        s.set_solver_iterations(20)
+    for fr in range(6):
+        g.step(1.0 / 60.0, 10, 20)
+        o.step(1.0 / 60.0, 10, 20)
+        diff = float(np.abs(_pos(g) - o.get_positions().reshape(-1, 3)).max())
+        assert diff < TOL, f"frame {fr}: {diff:.3e}"
+
+
+def test_reset_and_set_positions():
+    _say("reset / set positions")
+    pos, e, f, inv = cases.config1()
+    g = T.ClothSimulator(pos, e, f, inv)
+    g.step(1.0 / 60.0, 10, 4)
+    g.reset()
+    assert np.abs(_pos(g) - pos).max() <= 1e-6
+    g.set_positions_and_velocities(pos + 0.01)
+    assert np.abs(_pos(g) - (pos + 0.01)).max() <= 1e-6
+
+
+def test_graph_replay_is_stable():
+    """The frame graph replayed many times equals the oracle after each replay."""
+    _say("graph replay")
+    pos, e, f, inv = cases.config1()
+    g = T.ClothSimulator(pos, e, f, inv)
+    o = O.OracleSim(pos, e, f, inv)
+    for s in (g, o):
+        s.set_enable_self_collision(False)
+    for k in range(5):
+        g.step(1.0 / 60.0, 4, 3)
+        o.step(1.0 / 60.0, 4, 3)
+        assert np.abs(_pos(g) - o.get_positions().reshape(-1, 3)).max() < TOL
+
+
+def test_empty_mesh_is_a_no_op():
+    _say("empty mesh")
+    g = T.ClothSimulator(np.zeros((0, 3), np.float32), np.zeros((0, 2), np.uint32))
+    assert g.step(1.0 / 60.0) == 20
+    g.get_positions(np.empty(0, np.float32))
