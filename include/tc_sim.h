@@ -109,6 +109,14 @@ int tc_sim_set_stiffness_all(tc_sim *sim, float tension, float compression, floa
 int tc_sim_set_compliances_raw(tc_sim *sim, float stretch, float shear, float bend);
 int tc_sim_set_hash_cell_size(tc_sim *sim, float cell_size);
 
+/* Solver schedule (no reference counterpart; both give identical results):
+ * 0 auto, 1 one launch per colour (the reference's dispatch pattern, dispatch.rs:1449-1455),
+ * 2 one launch per solver iteration over dependency-cone tiles held in LDS. */
+int tc_sim_set_solver_path(tc_sim *sim, int path);
+/* out[8]: [active path (1|2), tiles, largest cone (vertices), cone vertices total,
+ *          items per iteration, constraints per iteration, plan available (0|1), colour phases] */
+int tc_sim_solver_info(tc_sim *sim, double *out);
+
 /* num_vertices / num_distance_constraints / num_bending_constraints getters, lib.rs:2928-2950 */
 uint32_t tc_sim_num_vertices(const tc_sim *sim);
 uint32_t tc_sim_num_distance_constraints(const tc_sim *sim);

@@ -76,6 +76,7 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.orc_sim_add_capsule.argtypes = [_vp, _f32p, _f32p, C.c_float, C.c_float, C.c_float]
     L.orc_sim_add_plane.argtypes = [_vp, _f32p, _f32p, C.c_float, C.c_float]
     L.orc_sim_clear_colliders.argtypes = [_vp]
+    L.orc_sim_set_mesh_triangles.argtypes = [_vp, _f32p, C.c_uint32, _vp, C.c_float, C.c_float, C.c_float, C.c_int]
     L.orc_sim_reset.argtypes = [_vp]
     L.orc_sim_set_state.argtypes = [_vp, _vp, _vp, _vp]
     L.orc_sim_get.argtypes = [_vp, C.c_int, _f64p]
@@ -230,6 +231,13 @@ class OracleSim:
 
     def clear_colliders(self):
         self._L.orc_sim_clear_colliders(self._h)
+
+    def set_mesh_collider_triangles(self, tris, friction=0.0, thickness=0.0, restitution=0.0, single_sided=False,
+                                    attributes=None):
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        at = _opt(attributes, np.float32)
+        self._L.orc_sim_set_mesh_triangles(self._h, t, len(t), _ptr(at), friction, thickness, restitution,
+                                           1 if single_sided else 0)
 
     def reset(self):
         self._L.orc_sim_reset(self._h)

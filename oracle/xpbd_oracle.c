@@ -994,6 +994,11 @@ typedef struct {
 } collider;
 
 typedef struct {
+    float p0[3], friction, p1[3], thickness, p2[3], restitution;
+    uint32_t flags;
+} mtri;
+
+typedef struct {
     int32_t dx, dy, dz; uint32_t count;
     int32_t pdx, pdy, pdz; uint32_t pcount;
     int32_t cpx, cpy, cpz, cnx, cny, cnz;
@@ -1016,6 +1021,8 @@ typedef struct {
     uint32_t iterations;
     int self_collision;
     collider *cols; uint32_t ncols;
+    mtri *tris; float *tri_bounds; uint32_t ntris;   /* Morton-sorted, bounds = (cx, cy, cz, r) */
+    int single_sided_recovery;
     real cell_size; uint32_t table_size;
     accum_t *acc;
     uint32_t *hkeys, *hstarts, *hsorted;     /* vertex hash */
@@ -1063,6 +1070,7 @@ orc_sim *orc_sim_create(const float *pos, uint32_t nv, const uint32_t *edges, ui
     s->ekeys = (uint32_t *)malloc(NE * 4); s->esorted = (uint32_t *)malloc(NE * 4);
     s->estarts = (uint32_t *)malloc((size_t)(s->table_size + 1) * 4);
     s->max_search_iterations = 256;
+    s->single_sided_recovery = 1;
     s->relief_factor = (real)0.2f; s->max_disp_ratio = (real)0.2f;
     s->enable_relief = 1; s->exclude_neighbors = 1;
     return s;
@@ -1072,7 +1080,7 @@ void orc_sim_free(orc_sim *s) {
     if (!s) return;
     orc_mesh_free(s->m);
     free(s->x); free(s->p); free(s->v); free(s->x0); free(s->w); free(s->w0); free(s->lam_d); free(s->lam_b);
-    free(s->d_tc); free(s->d_cc); free(s->b_comp); free(s->cols); free(s->acc);
+    free(s->d_tc); free(s->d_cc); free(s->b_comp); free(s->cols); free(s->acc); free(s->tris); free(s->tri_bounds);
     free(s->hkeys); free(s->hstarts); free(s->hsorted); free(s->ekeys); free(s->estarts); free(s->esorted);
     free(s);
 }
@@ -1117,7 +1125,59 @@ void orc_sim_add_capsule(orc_sim *s, const float *a, const float *b, float r, fl
 void orc_sim_add_plane(orc_sim *s, const float *pt, const float *n, float fr, float rest) {
     collider k = {2, fr, rest, mk(pt[0], pt[1], pt[2]), 0, mk(n[0], n[1], n[2])}; add_col(s, k);
 }
-void orc_sim_clear_colliders(orc_sim *s) { s->ncols = 0; }
+void orc_sim_clear_colliders(orc_sim *s) { s->ncols = 0; s->ntris = 0; }
+
+typedef struct { uint32_t code, idx; } tkey;
+static int cmp_tkey(const void *a, const void *b) {
+    const tkey *x = (const tkey *)a, *y = (const tkey *)b;
+    if (x->code != y->code) return x->code < y->code ? -1 : 1;
+    return x->idx < y->idx ? -1 : x->idx > y->idx;
+}
+/* set_mesh_collider_triangles (lib.rs:2175-2233) + upload_colliders Morton sort and
+ * bounding spheres (simulation/mod.rs:2582-2667). attrs: NULL or 4 floats per triangle. */
+void orc_sim_set_mesh_triangles(orc_sim *s, const float *t9, uint32_t nt, const float *attrs, float friction,
+                                float thickness, float restitution, int single_sided) {
+    mtri *T = (mtri *)malloc((nt ? nt : 1) * sizeof(mtri));
+    for (uint32_t t = 0; t < nt; t++) {
+        for (int a = 0; a < 3; a++) { T[t].p0[a] = t9[9 * t + a]; T[t].p1[a] = t9[9 * t + 3 + a]; T[t].p2[a] = t9[9 * t + 6 + a]; }
+        if (attrs) {
+            T[t].friction = attrs[4 * t]; T[t].thickness = attrs[4 * t + 1]; T[t].restitution = attrs[4 * t + 2];
+            float r = roundf(attrs[4 * t + 3]);
+            T[t].flags = r > 0.0f ? (r >= 4294967296.0f ? 0xFFFFFFFFu : (uint32_t)r) : 0u;
+        } else {
+            T[t].friction = friction; T[t].thickness = thickness; T[t].restitution = restitution;
+            T[t].flags = single_sided ? 1u : 0u;
+        }
+    }
+    float bmin[3] = {3.4028235e38f, 3.4028235e38f, 3.4028235e38f}, bmax[3] = {-3.4028235e38f, -3.4028235e38f, -3.4028235e38f};
+    float *cen = (float *)malloc((nt ? nt : 1) * 12);
+    for (uint32_t t = 0; t < nt; t++)
+        for (int a = 0; a < 3; a++) {
+            cen[3 * t + a] = (T[t].p0[a] + T[t].p1[a] + T[t].p2[a]) / 3.0f;
+            bmin[a] = fminf(bmin[a], cen[3 * t + a]); bmax[a] = fmaxf(bmax[a], cen[3 * t + a]);
+        }
+    float inv[3];
+    for (int a = 0; a < 3; a++) inv[a] = 1.0f / fmaxf(bmax[a] - bmin[a], 1e-5f);
+    tkey *k = (tkey *)malloc((nt ? nt : 1) * sizeof(tkey));
+    for (uint32_t t = 0; t < nt; t++) { k[t].code = morton_code_3d(cen + 3 * t, bmin, inv); k[t].idx = t; }
+    qsort(k, nt, sizeof(tkey), cmp_tkey);
+    free(s->tris); free(s->tri_bounds);
+    s->tris = (mtri *)malloc((nt ? nt : 1) * sizeof(mtri));
+    s->tri_bounds = (float *)malloc((nt ? nt : 1) * 16);
+    for (uint32_t t = 0; t < nt; t++) {
+        mtri m = T[k[t].idx];
+        s->tris[t] = m;
+        float cx = (m.p0[0] + m.p1[0] + m.p2[0]) / 3.0f, cy = (m.p0[1] + m.p1[1] + m.p2[1]) / 3.0f,
+              cz = (m.p0[2] + m.p1[2] + m.p2[2]) / 3.0f;
+        float d0 = (m.p0[0] - cx) * (m.p0[0] - cx) + (m.p0[1] - cy) * (m.p0[1] - cy) + (m.p0[2] - cz) * (m.p0[2] - cz);
+        float d1 = (m.p1[0] - cx) * (m.p1[0] - cx) + (m.p1[1] - cy) * (m.p1[1] - cy) + (m.p1[2] - cz) * (m.p1[2] - cz);
+        float d2 = (m.p2[0] - cx) * (m.p2[0] - cx) + (m.p2[1] - cy) * (m.p2[1] - cy) + (m.p2[2] - cz) * (m.p2[2] - cz);
+        s->tri_bounds[4 * t] = cx; s->tri_bounds[4 * t + 1] = cy; s->tri_bounds[4 * t + 2] = cz;
+        s->tri_bounds[4 * t + 3] = sqrtf(fmaxf(fmaxf(d0, d1), d2)) + m.thickness;
+    }
+    s->ntris = nt;
+    free(T); free(cen); free(k);
+}
 void orc_sim_reset(orc_sim *s) {
     for (uint32_t i = 0; i < s->nv; i++) { s->x[i] = s->x0[i]; s->p[i] = s->x0[i]; s->v[i] = mk(0, 0, 0); s->w[i] = s->w0[i]; }
 }
@@ -1214,6 +1274,39 @@ static void k_bending_one(orc_sim *s, uint32_t k, real dt_sub) {
     s->p[v3_] = add(s->p[v3_], mul(dir, -w3 * dl));
 }
 
+static int seg_tri(v3 po, v3 pn, v3 a, v3 b, v3 c, real *t_out, real *u_out, real *v_out);
+/* collision.wgsl:364-374 */
+static real seg_sphere_dist_sq(v3 a, v3 b, v3 c) {
+    v3 ab = sub(b, a);
+    real ab2 = dot(ab, ab);
+    if (ab2 <= (real)1e-12f) { v3 d = sub(a, c); return dot(d, d); }
+    real t = rclamp(dot(sub(c, a), ab) / ab2, 0, 1);
+    v3 d = sub(add(a, mul(ab, t)), c);
+    return dot(d, d);
+}
+/* collision.wgsl:273-319 closest point with face-interior flag */
+static v3 closest_face(v3 p, v3 a, v3 b, v3 c, int *is_face) {
+    *is_face = 0;
+    v3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+    real d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) return a;
+    v3 bp = sub(p, b);
+    real d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) return b;
+    real vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) return add(a, mul(ab, d1 / (d1 - d3)));
+    v3 cp = sub(p, c);
+    real d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) return c;
+    real vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) return add(a, mul(ac, d2 / (d2 - d6)));
+    real va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) return add(b, mul(sub(c, b), (d4 - d3) / ((d4 - d3) + (d5 - d6))));
+    real denom = (real)1 / (va + vb + vc);
+    *is_face = 1;
+    return add(add(a, mul(ab, vb * denom)), mul(ac, vc * denom));
+}
+
 /* collision.wgsl:83-108 */
 static void contact_response(v3 target, v3 n, real fr, real rest, real dt, v3 p_init, v3 *xn, v3 *p) {
     v3 total = sub(target, *xn);
@@ -1266,6 +1359,60 @@ static void k_collision(orc_sim *s, real dt) {
                 }
             }
         }
+        /* triangle mesh, direct scan in Morton order (collision.wgsl:447-542) */
+        int front = 0, has_rec = 0;
+        real best_depth = (real)999999.0f;
+        v3 best_pos = p, best_n = mk(0, 0, 0);
+        real best_fr = 0, best_rest = 0;
+        v3 x_old = s->x[i];
+        for (uint32_t k = 0; k < s->ntris; k++) {
+            const float *b = s->tri_bounds + 4 * k;
+            real rr = (real)b[3] * (real)1.5f + th;
+            if (seg_sphere_dist_sq(x_old, p, mk(b[0], b[1], b[2])) > rr * rr) continue;
+            const mtri *T = &s->tris[k];
+            v3 t0 = mk(T->p0[0], T->p0[1], T->p0[2]), t1 = mk(T->p1[0], T->p1[1], T->p1[2]), t2 = mk(T->p2[0], T->p2[1], T->p2[2]);
+            real target = th + (real)T->thickness;
+            v3 cr = cross(sub(t1, t0), sub(t2, t0));
+            real cl = len3(cr);
+            if (cl < (real)1e-7f) continue;
+            v3 fn = dvs(cr, cl);
+            int single = (T->flags & 1u) != 0u;
+            real th_hit, hu, hv;
+            if (seg_tri(x_old, p, t0, t1, t2, &th_hit, &hu, &hv)) {
+                v3 hit = add(x_old, mul(sub(p, x_old), th_hit));
+                v3 nrm = fn;
+                if (dot(sub(p, x_old), fn) > 0 && !single) nrm = neg(fn);
+                contact_response(add(hit, mul(nrm, target)), nrm, T->friction, T->restitution, dt, pinit, &xn, &p);
+                front = 1;
+            }
+            int is_face;
+            v3 q = closest_face(p, t0, t1, t2, &is_face);
+            v3 d = sub(p, q);
+            real dist = len3(d);
+            real sd = dot(d, fn);
+            if (single) {
+                if (sd >= 0) {
+                    if (dist < target) {
+                        v3 nrm = fn;
+                        if (dist > (real)1e-7f) nrm = dvs(d, dist);
+                        contact_response(add(p, mul(nrm, target - dist)), nrm, T->friction, T->restitution, dt, pinit, &xn, &p);
+                        front = 1;
+                    }
+                } else if (s->single_sided_recovery && (T->flags & 2u) == 0u) {
+                    if (is_face && dist < target * (real)3.0f && dist <= (real)fabs((double)sd) * (real)1.1f + (real)1e-7f &&
+                        dist < best_depth) {
+                        best_depth = dist; best_n = fn; best_pos = add(p, mul(fn, target - sd));
+                        best_fr = T->friction; best_rest = T->restitution; has_rec = 1;
+                    }
+                }
+            } else if (dist < target && sd >= -target) {
+                v3 nrm = fn;
+                if (dist > (real)1e-7f) nrm = dvs(d, dist);
+                contact_response(add(p, mul(nrm, target - dist)), nrm, T->friction, T->restitution, dt, pinit, &xn, &p);
+                front = 1;
+            }
+        }
+        if (!front && has_rec) contact_response(best_pos, best_n, best_fr, best_rest, dt, pinit, &xn, &p);
         s->p[i] = p;
         s->x[i] = xn;
     }
@@ -1899,7 +2046,7 @@ static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
             for (uint32_t c = 0; c < m->nb_col; c++)
                 for (uint32_t k = m->b_col_off[c]; k < m->b_col_off[c] + m->b_col_cnt[c]; k++) k_bending_one(s, k, dt_sub);
         }
-        if (s->ncols > 0) k_collision(s, dt_sub);
+        if (s->ncols > 0 || s->ntris > 0) k_collision(s, dt_sub);
         if (s->self_collision) self_collision_pass(s);
         k_update_vel(s, dt_sub);
     }
@@ -1927,7 +2074,7 @@ void orc_sim_substep_parts(orc_sim *s, float dt_sub, int do_collide, int do_self
         for (uint32_t c = 0; c < m->nb_col; c++)
             for (uint32_t k = m->b_col_off[c]; k < m->b_col_off[c] + m->b_col_cnt[c]; k++) k_bending_one(s, k, dt);
     }
-    if (do_collide && s->ncols > 0) k_collision(s, dt);
+    if (do_collide && (s->ncols > 0 || s->ntris > 0)) k_collision(s, dt);
     if (do_self && s->self_collision) self_collision_pass(s);
     if (do_update) k_update_vel(s, dt);
 }

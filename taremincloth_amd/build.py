@@ -16,8 +16,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libtcloth_hip.so"
 ARCH = os.environ.get("TC_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["tc_sim.hip", "tc_xpbd.hip", "tc_selfcol.hip"]
-CXX_SOURCES = ["tc_mesh.cpp"]
+HIP_SOURCES = ["tc_sim.hip", "tc_xpbd.hip", "tc_selfcol.hip", "tc_fused.hip"]
+CXX_SOURCES = ["tc_mesh.cpp", "tc_tiles.cpp"]
 
 # Strict IEEE fp32 (no FMA contraction) so results follow the reference's
 # operation order; packed-fp32 instructions are disabled.

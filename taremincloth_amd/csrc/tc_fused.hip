@@ -1,0 +1,145 @@
+// One solver iteration (all distance colours, then all bending colours) per
+// launch, one workgroup per cone tile (plan in tc_tiles.cpp).
+//
+// Per workgroup: load the tile's cone of P into LDS (16 B per vertex), replay
+// the selected constraints of each colour phase from LDS with a block barrier
+// between phases, write back the tile's own vertices and the lambdas of the
+// constraints it owns.  The projection arithmetic is the same operation
+// sequence as k_distance / k_bending (distance.wgsl:60-124, bending.wgsl:59-126).
+//
+// Positions and lambdas are double-buffered across launches (the cones of
+// neighbouring tiles overlap, so a tile must not read what another tile of the
+// same launch writes).  Items (12 B): word0 = local v0 | local v1 << 16,
+// word1 = constraint index (26 bits) | compliance class << 26 | shear << 30 |
+// owner << 31, word2 = rest length bits.
+//
+// Tiles are Morton-ordered, so consecutive tiles are spatial neighbours; the
+// block -> tile map keeps a contiguous run of tiles on each of the 8 XCDs
+// (workgroups are dealt to XCDs round-robin) so overlapping cone reads of P,
+// lambda and items share that XCD's L2.
+#include "tc_kernels.h"
+
+namespace tc {
+
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks) {
+    const uint32_t per = nblocks / 8u;  // nblocks is a multiple of 8
+    return (b % 8u) * per + b / 8u;
+}
+
+template <bool FIRST, bool DAMP>
+__global__ void __launch_bounds__(512) k_tile_iteration(
+    const float4 *__restrict__ Pin, float4 *__restrict__ Pout, const float4 *__restrict__ X,
+    const uint32_t *__restrict__ ext_off, const uint32_t *__restrict__ ext_vid, const uint32_t *__restrict__ n_own,
+    const uint32_t *__restrict__ ph_off, const uint32_t *__restrict__ items, const float *__restrict__ dlam_in,
+    float *__restrict__ dlam_out, const float *__restrict__ blam_in, float *__restrict__ blam_out, FusedArgs fa,
+    SubstepArgs a) {
+    extern __shared__ float4 sp[];
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= fa.ntiles) return;
+    const uint32_t e0 = ext_off[tile], ne = ext_off[tile + 1] - e0;
+    float4 *sx = sp + fa.max_ext;  // DAMP only: committed positions of the cone
+    for (uint32_t l = threadIdx.x; l < ne; l += blockDim.x) {
+        const uint32_t g = ext_vid[e0 + l];
+        sp[l] = Pin[g];
+        if (DAMP) sx[l] = X[g];
+    }
+    __syncthreads();
+    const float dt = fmaxf(a.dt, 1e-8f);
+    const uint32_t *po = ph_off + size_t(tile) * (fa.nphases + 1);
+    for (uint32_t c = 0; c < fa.nphases; ++c) {
+        const bool dist = c < fa.nd_phases;
+        const uint32_t i1 = po[c + 1];
+        for (uint32_t i = po[c] + threadIdx.x; i < i1; i += blockDim.x) {
+            const uint32_t w0 = items[3 * size_t(i)], w1 = items[3 * size_t(i) + 1];
+            const float rest = __uint_as_float(items[3 * size_t(i) + 2]);
+            const uint32_t l0 = w0 & 0xFFFFu, l1 = w0 >> 16;
+            const uint32_t k = w1 & 0x03FFFFFFu, cls = (w1 >> 26) & 15u;
+            const bool shear = ((w1 >> 30) & 1u) != 0u, owner = (w1 >> 31) != 0u;
+            const float4 q0 = sp[l0], q1 = sp[l1];
+            const float wa = q0.w, wb = q1.w, wsum = wa + wb;
+            if (dist) {
+                // an owner that skips the projection still carries lambda into the output buffer
+                const float l = FIRST ? 0.0f : dlam_in[k];
+                if (wsum <= 1e-7f) { if (owner) dlam_out[k] = l; continue; }
+                const f3 delta = xyz(q0) - xyz(q1);
+                const float d = length(delta);
+                if (d <= 1e-7f) { if (owner) dlam_out[k] = l; continue; }
+                const float cval = d - rest;
+                const float comp = cval >= 0.0f ? fa.dtc[cls] : fa.dcc[cls];
+                if (comp >= 1e9f) { if (owner) dlam_out[k] = l; continue; }
+                const f3 dir = delta / d;
+                const float alpha = comp / (dt * dt);
+                float dl;
+                if (DAMP) {
+                    const float dc = shear ? a.shear_damping : (cval >= 0.0f ? a.tension_damping : a.compression_damping);
+                    const float gamma = fminf(dc * 1e-5f / dt, 1e4f);
+                    const f3 rel = (xyz(q0) - xyz(sx[l0])) - (xyz(q1) - xyz(sx[l1]));
+                    dl = (-cval - alpha * l - gamma * dot(dir, rel)) / ((1.0f + gamma) * wsum + alpha);
+                } else {
+                    dl = (-cval - alpha * l) / (wsum + alpha);
+                }
+                if (owner) dlam_out[k] = l + dl;
+                const f3 c0 = dir * (wa * dl), c1 = dir * (-wb * dl);
+                sp[l0] = make_float4(q0.x + c0.x, q0.y + c0.y, q0.z + c0.z, wa);
+                sp[l1] = make_float4(q1.x + c1.x, q1.y + c1.y, q1.z + c1.z, wb);
+            } else {
+                const float l = FIRST ? 0.0f : blam_in[k];
+                const float comp = fa.bcomp[cls];
+                if (comp >= 1e9f || wsum <= 1e-7f) { if (owner) blam_out[k] = l; continue; }
+                const f3 delta = xyz(q0) - xyz(q1);
+                const float d = length(delta);
+                if (d <= 1e-7f) { if (owner) blam_out[k] = l; continue; }
+                const f3 dir = delta / d;
+                const float alpha = comp / (dt * dt);
+                const float cval = d - rest;
+                float dl;
+                if (DAMP) {
+                    const float gamma = fminf(a.bending_damping * 1e-5f / dt, 1e4f);
+                    const f3 rel = (xyz(q0) - xyz(sx[l0])) - (xyz(q1) - xyz(sx[l1]));
+                    dl = (-cval - alpha * l - gamma * dot(dir, rel)) / ((1.0f + gamma) * wsum + alpha);
+                } else {
+                    dl = (-cval - alpha * l) / (wsum + alpha);
+                }
+                if (owner) blam_out[k] = l + dl;
+                const f3 c0 = dir * (wa * dl), c1 = dir * (-wb * dl);
+                sp[l0] = make_float4(q0.x + c0.x, q0.y + c0.y, q0.z + c0.z, wa);
+                sp[l1] = make_float4(q1.x + c1.x, q1.y + c1.y, q1.z + c1.z, wb);
+            }
+        }
+        __syncthreads();
+    }
+    const uint32_t own = n_own[tile];
+    for (uint32_t l = threadIdx.x; l < own; l += blockDim.x) Pout[ext_vid[e0 + l]] = sp[l];
+}
+
+hipError_t launch_tile_iteration(hipStream_t s, const float4 *Pin, float4 *Pout, const float4 *X,
+                                 const FusedPlanDev &p, const float *dlam_in, float *dlam_out, const float *blam_in,
+                                 float *blam_out, const FusedArgs &fa, const SubstepArgs &a, bool first,
+                                 uint32_t block) {
+    if (!fa.ntiles) return hipSuccess;
+    const bool damp = a.tension_damping != 0.0f || a.compression_damping != 0.0f || a.shear_damping != 0.0f ||
+                      a.bending_damping != 0.0f;
+    const uint32_t nblocks = ((fa.ntiles + 7u) / 8u) * 8u;
+    const size_t lds = size_t(fa.max_ext) * sizeof(float4) * (damp ? 2 : 1);
+#define TC_TILE(F, D)                                                                                            \
+    hipLaunchKernelGGL((k_tile_iteration<F, D>), dim3(nblocks), dim3(block), lds, s, Pin, Pout, X, p.ext_off,   \
+                       p.ext_vid, p.n_own, p.ph_off, p.items, dlam_in, dlam_out, blam_in, blam_out, fa, a)
+    if (first) { if (damp) TC_TILE(true, true); else TC_TILE(true, false); }
+    else { if (damp) TC_TILE(false, true); else TC_TILE(false, false); }
+#undef TC_TILE
+    return hipGetLastError();
+}
+
+hipError_t set_tile_lds_limit(size_t bytes) {
+    hipError_t e;
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_iteration<true, true>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes))) != hipSuccess) return e;
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_iteration<true, false>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes))) != hipSuccess) return e;
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_iteration<false, true>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes))) != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_iteration<false, false>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+}
+
+}  // namespace tc

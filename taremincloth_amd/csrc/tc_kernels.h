@@ -37,6 +37,20 @@ hipError_t launch_residual(hipStream_t s, const float4 *P, const uint4 *drec, co
                            uint32_t nd, const uint4 *brec, const float *blam, uint32_t nb, float dt, bool split,
                            double *out);
 
+// --- one iteration per launch over cone tiles (tc_fused.hip, plan from tc_tiles.cpp) ---
+struct FusedArgs {
+    uint32_t ntiles, nphases, nd_phases, max_ext;
+    float dtc[16], dcc[16], bcomp[16];  // compliance classes
+};
+struct FusedPlanDev {
+    const uint32_t *ext_off, *ext_vid, *n_own, *ph_off, *items;
+};
+hipError_t launch_tile_iteration(hipStream_t s, const float4 *Pin, float4 *Pout, const float4 *X,
+                                 const FusedPlanDev &p, const float *dlam_in, float *dlam_out, const float *blam_in,
+                                 float *blam_out, const FusedArgs &fa, const SubstepArgs &a, bool first,
+                                 uint32_t block);
+hipError_t set_tile_lds_limit(size_t bytes);
+
 // --- spatial hash + self collision (tc_selfcol.hip) ---
 struct HashBufs {
     uint32_t *keys;     // n

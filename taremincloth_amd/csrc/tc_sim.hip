@@ -23,6 +23,7 @@
 #include "../../include/tc_sim.h"
 #include "tc_kernels.h"
 #include "tc_mesh.h"
+#include "tc_tiles.h"
 
 namespace {
 
@@ -72,13 +73,13 @@ struct tc_sim {
     hipStream_t stream = nullptr;
     uint32_t block = 256;
 
-    // vertex streams
-    DevBuf<float4> X, P, V;
+    // vertex streams (P has a second buffer for the one-launch-per-iteration path)
+    DevBuf<float4> X, P, V, Palt;
     std::vector<float4> X0h, P0h;  // initial state for reset()
 
     // constraints
     DevBuf<uint4> drec, brec;
-    DevBuf<float> dcc, dlam, blam;
+    DevBuf<float> dcc, dlam, blam, dlam2, blam2;
     bool split_compliance = false;
     std::vector<uint4> drec_h, brec_h;
     std::vector<float> dcc_h;
@@ -113,6 +114,15 @@ struct tc_sim {
     uint32_t max_search_iterations = 256;
     float cell_size = 0.01f;
     uint32_t table_size = 1024;
+
+    // one-launch-per-iteration plan (tc_tiles.cpp)
+    int solver_path = 0;          // 0 auto, 1 one launch per colour, 2 one launch per iteration
+    bool fused_ready = false;     // plan built and uploaded
+    std::string fused_reason;     // why the plan is unavailable
+    tc::TilePlan plan;
+    DevBuf<uint32_t> pl_ext_off, pl_ext_vid, pl_n_own, pl_ph_off, pl_items;
+    uint32_t fused_block = 512;
+    uint32_t last_p = 0, last_lam = 0;  // buffers holding P and lambda after the last substep
 
     // frame graph cache
     hipGraphExec_t graph_exec = nullptr;
@@ -152,57 +162,94 @@ struct tc_sim {
         t.star_off = star_off.p; t.star = star.p; t.edges = edges.p; t.edge_bbits = edge_bbits.p;
         return t;
     }
+    bool fused_active() const {
+        if (!fused_ready) return false;
+        if (solver_path == 1) return false;
+        if (solver_path == 2) return true;
+        return mesh.nv >= 16384;  // small meshes: the per-colour launches are cheap
+    }
+    float4 *Pbuf(uint32_t k) const { return k ? Palt.p : P.p; }
+    float *dlam_buf(uint32_t k) const { return k ? dlam2.p : dlam.p; }
+    float *blam_buf(uint32_t k) const { return k ? blam2.p : blam.p; }
+    tc::FusedArgs fused_args() const {
+        tc::FusedArgs f{};
+        f.ntiles = plan.ntiles; f.nphases = plan.nphases; f.nd_phases = plan.nd_phases; f.max_ext = plan.max_ext;
+        for (size_t c = 0; c < plan.dcls_tc.size(); ++c) { f.dtc[c] = plan.dcls_tc[c]; f.dcc[c] = plan.dcls_cc[c]; }
+        for (size_t c = 0; c < plan.bcls.size(); ++c) f.bcomp[c] = plan.bcls[c];
+        return f;
+    }
+    tc::FusedPlanDev plan_dev() const {
+        return {pl_ext_off.p, pl_ext_vid.p, pl_n_own.p, pl_ph_off.p, pl_items.p};
+    }
     tc::HashBufs vhash() const { return {vkeys.p, vcounts.p, vstarts.p, vsorted.p}; }
     tc::HashBufs ehash() const { return {ekeys.p, ecounts.p, estarts.p, esorted.p}; }
 };
 
 namespace {
 
-hipError_t enqueue_hashes(tc_sim *s) {
-    hipError_t e = tc::launch_hash_vertices(s->stream, s->P.p, s->mesh.nv, s->cell_size, s->table_size, s->vhash());
+hipError_t enqueue_hashes(tc_sim *s, const float4 *P) {
+    hipError_t e = tc::launch_hash_vertices(s->stream, P, s->mesh.nv, s->cell_size, s->table_size, s->vhash());
     if (e != hipSuccess) return e;
-    return tc::launch_hash_edges(s->stream, s->P.p, s->edges.p, uint32_t(s->mesh.e_v0.size()), s->cell_size,
+    return tc::launch_hash_edges(s->stream, P, s->edges.p, uint32_t(s->mesh.e_v0.size()), s->cell_size,
                                  s->table_size, s->ehash());
 }
 
-// One substep up to (optionally) the velocity update.
+// One substep up to (optionally) the velocity update.  Predict writes P buffer 0;
+// the per-iteration path ping-pongs P and lambda between two buffers, and the
+// rest of the substep works on the buffer holding the latest positions.
 hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_collide, bool update) {
     const tc::SubstepArgs a = s->args(dt_sub);
     const tc::Mesh &m = s->mesh;
     const uint32_t n = m.nv, B = s->block;
     hipError_t e = tc::launch_predict(s->stream, s->P.p, s->X.p, s->V.p, n, a, B);
     if (e != hipSuccess) return e;
+    uint32_t pc = 0, lc = 0;
     if (s->iterations == 0) {  // lambdas are still cleared at the substep start (dispatch.rs:117-119)
         if ((e = tc::launch_fill_u32(s->stream, reinterpret_cast<uint32_t *>(s->dlam.p), 0u, m.d_v0.size())) != hipSuccess) return e;
         if ((e = tc::launch_fill_u32(s->stream, reinterpret_cast<uint32_t *>(s->blam.p), 0u, m.b_v0.size())) != hipSuccess) return e;
-    }
-    for (uint32_t it = 0; it < s->iterations; ++it) {
-        const bool first = it == 0;
-        for (size_t c = 0; c < m.d_col.count.size(); ++c) {
-            e = tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p, m.d_col.offset[c],
-                                    m.d_col.count[c], a, first, s->split_compliance, B);
+    } else if (s->fused_active()) {
+        const tc::FusedArgs fa = s->fused_args();
+        const tc::FusedPlanDev pd = s->plan_dev();
+        for (uint32_t it = 0; it < s->iterations; ++it) {
+            e = tc::launch_tile_iteration(s->stream, s->Pbuf(pc), s->Pbuf(pc ^ 1u), s->X.p, pd, s->dlam_buf(lc),
+                                          s->dlam_buf(lc ^ 1u), s->blam_buf(lc), s->blam_buf(lc ^ 1u), fa, a, it == 0,
+                                          s->fused_block);
             if (e != hipSuccess) return e;
+            pc ^= 1u;
+            lc ^= 1u;
         }
-        for (size_t c = 0; c < m.b_col.count.size(); ++c) {
-            e = tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p, m.b_col.offset[c],
-                                   m.b_col.count[c], a, first, B);
-            if (e != hipSuccess) return e;
+    } else {
+        for (uint32_t it = 0; it < s->iterations; ++it) {
+            const bool first = it == 0;
+            for (size_t c = 0; c < m.d_col.count.size(); ++c) {
+                e = tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p, m.d_col.offset[c],
+                                        m.d_col.count[c], a, first, s->split_compliance, B);
+                if (e != hipSuccess) return e;
+            }
+            for (size_t c = 0; c < m.b_col.count.size(); ++c) {
+                e = tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p, m.b_col.offset[c],
+                                       m.b_col.count[c], a, first, B);
+                if (e != hipSuccess) return e;
+            }
         }
     }
+    float4 *Pc = s->Pbuf(pc);
+    s->last_p = pc;
+    s->last_lam = lc;
     if (collide && (!s->cols_h.empty() || !s->tris_h.empty())) {
-        e = tc::launch_collision(s->stream, s->P.p, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
+        e = tc::launch_collision(s->stream, Pc, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
                                  s->tri_bounds.p, uint32_t(s->tris_h.size()), s->single_sided_recovery, dt_sub, B);
         if (e != hipSuccess) return e;
     }
     if (self_collide && s->self_collision) {
-        if ((e = enqueue_hashes(s)) != hipSuccess) return e;
+        if ((e = enqueue_hashes(s, Pc)) != hipSuccess) return e;
         const tc::SelfColArgs sa = s->sc_args();
         const tc::SelfColTopo t = s->topo();
-        if ((e = tc::launch_sc_solve(s->stream, s->P.p, s->X.p, t, s->vhash(), s->accum.p, sa, B)) != hipSuccess) return e;
-        if ((e = tc::launch_sc_solve_ee(s->stream, s->P.p, s->X.p, t, s->ehash(), s->accum.p, sa, B)) != hipSuccess) return e;
-        if ((e = tc::launch_sc_apply(s->stream, s->P.p, s->local_len.p, s->accum.p, sa, B)) != hipSuccess) return e;
+        if ((e = tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa, B)) != hipSuccess) return e;
+        if ((e = tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa, B)) != hipSuccess) return e;
+        if ((e = tc::launch_sc_apply(s->stream, Pc, s->local_len.p, s->accum.p, sa, B)) != hipSuccess) return e;
     }
-    if (update) return tc::launch_update_vel(s->stream, s->P.p, s->X.p, s->V.p, n, dt_sub, B);
+    if (update) return tc::launch_update_vel(s->stream, Pc, s->X.p, s->V.p, n, dt_sub, B);
     return hipSuccess;
 }
 
@@ -220,13 +267,20 @@ int run_frame(tc_sim *s, float dt, uint32_t substeps) {
         return fail(TC_ERR_UNSUPPORTED,
                     "multi-layer cloth with normal untangling is not part of this build; disable it with "
                     "set_self_collision_options(..., enable_normal_untangling=False) or disable self-collision");
+    if (s->fused_active() && s->iterations > 0) {  // host bookkeeping of the buffer parity the graph bakes in
+        s->last_p = s->iterations & 1u;
+        s->last_lam = s->iterations & 1u;
+    } else {
+        s->last_p = 0;
+        s->last_lam = 0;
+    }
     if (!s->use_graph) {
         TC_HIP(enqueue_frame(s, dt, substeps));
         return TC_OK;
     }
     uint32_t dtb;
     std::memcpy(&dtb, &dt, 4);
-    const uint64_t key[4] = {dtb, substeps, s->iterations, s->epoch};
+    const uint64_t key[4] = {dtb, substeps, uint64_t(s->iterations) | (uint64_t(s->fused_active()) << 32), s->epoch};
     if (!s->graph_exec || std::memcmp(key, s->graph_key, sizeof(key)) != 0) {
         if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
         if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
@@ -242,6 +296,33 @@ int run_frame(tc_sim *s, float dt, uint32_t substeps) {
     }
     TC_HIP(hipGraphLaunch(s->graph_exec, s->stream));
     return TC_OK;
+}
+
+// (Re)builds the one-launch-per-iteration plan; on failure the per-colour path is used.
+void build_plan(tc_sim *s) {
+    s->fused_ready = false;
+    s->fused_reason.clear();
+    int dev = 0;
+    int max_lds = 65536;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && v > 0) max_lds = v;
+    }
+    const uint32_t max_vertices = uint32_t(max_lds / int(2 * sizeof(float4)));  // room for the damped variant
+    std::string why = tc::build_tile_plan(s->mesh, 1024, max_vertices, s->plan);
+    if (!why.empty()) { s->fused_reason = why; return; }
+    hipError_t e;
+    if ((e = s->pl_ext_off.upload(s->plan.ext_off.data(), s->plan.ext_off.size(), s->stream)) != hipSuccess ||
+        (e = s->pl_ext_vid.upload(s->plan.ext_vid.data(), s->plan.ext_vid.size(), s->stream)) != hipSuccess ||
+        (e = s->pl_n_own.upload(s->plan.n_own.data(), s->plan.n_own.size(), s->stream)) != hipSuccess ||
+        (e = s->pl_ph_off.upload(s->plan.ph_off.data(), s->plan.ph_off.size(), s->stream)) != hipSuccess ||
+        (e = s->pl_items.upload(s->plan.items.data(), s->plan.items.size(), s->stream)) != hipSuccess ||
+        (e = tc::set_tile_lds_limit(size_t(s->plan.max_ext) * 2 * sizeof(float4))) != hipSuccess ||
+        (e = hipStreamSynchronize(s->stream)) != hipSuccess) {
+        s->fused_reason = std::string("plan upload: ") + hipGetErrorString(e);
+        return;
+    }
+    s->fused_ready = true;
 }
 
 void build_records(tc_sim *s) {
@@ -276,6 +357,7 @@ int upload_records(tc_sim *s) {
     TC_HIP(s->dcc.upload(s->dcc_h.data(), s->dcc_h.size(), s->stream));
     TC_HIP(s->brec.upload(s->brec_h.data(), s->brec_h.size(), s->stream));
     TC_HIP(hipStreamSynchronize(s->stream));
+    build_plan(s);  // compliance classes are baked into the plan
     s->touch();
     return TC_OK;
 }
@@ -371,12 +453,15 @@ int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, ui
         return bail(fail(TC_ERR_RUNTIME, std::string("hipMalloc: ") + hipGetErrorString(he)));
     UP(s->X, s->X0h.data(), nv);
     UP(s->P, s->P0h.data(), nv);
+    UP(s->Palt, s->P0h.data(), nv);
     UP(s->V, zeros.data(), nv);
     UP(s->drec, s->drec_h.data(), s->drec_h.size());
     UP(s->dcc, s->dcc_h.data(), s->dcc_h.size());
     UP(s->brec, s->brec_h.data(), s->brec_h.size());
     AL(s->dlam, m.d_v0.size());
     AL(s->blam, m.b_v0.size());
+    AL(s->dlam2, m.d_v0.size());
+    AL(s->blam2, m.b_v0.size());
     UP(s->local_len, m.local_len.data(), m.local_len.size());
     UP(s->th_off, m.th_off.data(), m.th_off.size());
     UP(s->th_idx, m.th_idx.data(), m.th_idx.size());
@@ -395,8 +480,11 @@ int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, ui
     if ((he = hipMemsetAsync(s->accum.p, 0, sizeof(tc::Accum) * std::max<size_t>(nv, 1), s->stream)) != hipSuccess ||
         (he = hipMemsetAsync(s->dlam.p, 0, sizeof(float) * std::max<size_t>(m.d_v0.size(), 1), s->stream)) != hipSuccess ||
         (he = hipMemsetAsync(s->blam.p, 0, sizeof(float) * std::max<size_t>(m.b_v0.size(), 1), s->stream)) != hipSuccess ||
+        (he = hipMemsetAsync(s->dlam2.p, 0, sizeof(float) * std::max<size_t>(m.d_v0.size(), 1), s->stream)) != hipSuccess ||
+        (he = hipMemsetAsync(s->blam2.p, 0, sizeof(float) * std::max<size_t>(m.b_v0.size(), 1), s->stream)) != hipSuccess ||
         (he = hipStreamSynchronize(s->stream)) != hipSuccess)
         return bail(fail(TC_ERR_RUNTIME, std::string("device init: ") + hipGetErrorString(he)));
+    build_plan(s);
     *out = s;
     return TC_OK;
 }
@@ -475,7 +563,7 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
                                                                s->single_sided_recovery, dt_sub, B); })) != hipSuccess) return e;
         if (s->self_collision)
             if ((e = mark(6, [&] {
-                     hipError_t r = enqueue_hashes(s);
+                     hipError_t r = enqueue_hashes(s, s->P.p);
                      if (r != hipSuccess) return r;
                      const tc::SelfColArgs sa = s->sc_args();
                      const tc::SelfColTopo t = s->topo();
@@ -540,6 +628,7 @@ int tc_sim_set_positions_velocities(tc_sim *s, const float *pos, const float *ve
     TC_HIP(hipMemcpy(s->X.p, X.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     TC_HIP(hipMemcpy(s->P.p, P.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     TC_HIP(hipMemcpy(s->V.p, V.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    s->last_p = 0;
     return TC_OK;
 }
 
@@ -552,6 +641,7 @@ int tc_sim_reset(tc_sim *s) {
     TC_HIP(hipMemcpy(s->X.p, s->X0h.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     TC_HIP(hipMemcpy(s->P.p, s->P0h.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     TC_HIP(hipMemcpy(s->V.p, V.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    s->last_p = 0;
     return TC_OK;
 }
 
@@ -596,8 +686,6 @@ int tc_sim_set_mesh_triangles(tc_sim *s, const float *tris, uint32_t nt, const f
                               float thickness, float restitution, int single_sided) {
     if (!s) return fail(TC_ERR_VALUE, "null simulator");
     if (nt > 0 && !tris) return fail(TC_ERR_VALUE, "triangles pointer is null");
-    for (size_t k = 0; k < size_t(nt) * 9; ++k)
-        if (!std::isfinite(tris[k])) return fail(TC_ERR_VALUE, "triangle coordinates must be finite");
     std::vector<tc::MeshTri> T(nt);
     for (uint32_t t = 0; t < nt; ++t) {
         const float *v = tris + 9 * size_t(t);
@@ -608,7 +696,12 @@ int tc_sim_set_mesh_triangles(tc_sim *s, const float *tris, uint32_t nt, const f
         m.friction = attrs ? attrs[4 * t] : friction;
         m.thickness = attrs ? attrs[4 * t + 1] : thickness;
         m.restitution = attrs ? attrs[4 * t + 2] : restitution;
-        m.flags = attrs ? uint32_t(attrs[4 * t + 3]) : (single_sided ? 1u : 0u);
+        if (attrs) {  // `row[3].round() as u32` (lib.rs:2210): round half away from zero, saturate
+            const float r = std::round(attrs[4 * t + 3]);
+            m.flags = r > 0.0f ? (r >= 4294967296.0f ? UINT32_MAX : uint32_t(r)) : 0u;
+        } else {
+            m.flags = single_sided ? 1u : 0u;
+        }
         T[t] = m;
     }
     // Morton order of the triangle centroids (simulation/mod.rs:2607-2632), stable
@@ -741,6 +834,29 @@ int tc_sim_set_compliances_raw(tc_sim *s, float stretch, float shear, float bend
     return upload_records(s);
 }
 
+int tc_sim_set_solver_path(tc_sim *s, int path) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (path < 0 || path > 2) return fail(TC_ERR_VALUE, "solver path must be 0 (auto), 1 (per colour) or 2 (per iteration)");
+    if (path == 2 && !s->fused_ready)
+        return fail(TC_ERR_UNSUPPORTED, "one-launch-per-iteration path unavailable: " + s->fused_reason);
+    s->solver_path = path;
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_solver_info(tc_sim *s, double *out) {
+    if (!s || !out) return fail(TC_ERR_VALUE, "null argument");
+    out[0] = s->fused_active() ? 2.0 : 1.0;
+    out[1] = s->plan.ntiles;
+    out[2] = s->plan.max_ext;
+    out[3] = s->plan.ext_vid.size();
+    out[4] = double(s->plan.items.size() / 3);
+    out[5] = double(s->plan.plain_items);
+    out[6] = s->fused_ready ? 1.0 : 0.0;
+    out[7] = s->plan.nphases;
+    return TC_OK;
+}
+
 int tc_sim_set_hash_cell_size(tc_sim *s, float cell) {
     if (!s) return fail(TC_ERR_VALUE, "null simulator");
     if (!(cell > 0.0f) || !std::isfinite(cell)) return fail(TC_ERR_VALUE, "cell size must be positive and finite");
@@ -755,7 +871,7 @@ uint32_t tc_sim_num_bending_constraints(const tc_sim *s) { return s ? uint32_t(s
 
 int tc_sim_build_hashes(tc_sim *s) {
     if (!s) return fail(TC_ERR_VALUE, "null simulator");
-    TC_HIP(enqueue_hashes(s));
+    TC_HIP(enqueue_hashes(s, s->Pbuf(s->last_p)));
     TC_HIP(hipStreamSynchronize(s->stream));
     return TC_OK;
 }
@@ -804,10 +920,10 @@ int tc_sim_get_array(tc_sim *s, int which, void *out, size_t cap, size_t *count)
     case 10: return dev(s->vkeys.p, m.nv, 4);
     case 11: return dev(s->vstarts.p, size_t(s->table_size) + 1, 4);
     case 12: return dev(s->vsorted.p, m.nv, 4);
-    case 13: return dev_xyz(s->P.p);
+    case 13: return dev_xyz(s->Pbuf(s->last_p));
     case 14: return dev_xyz(s->V.p);
-    case 15: return dev(s->dlam.p, m.d_v0.size(), 4);
-    case 16: return dev(s->blam.p, m.b_v0.size(), 4);
+    case 15: return dev(s->dlam_buf(s->last_lam), m.d_v0.size(), 4);
+    case 16: return dev(s->blam_buf(s->last_lam), m.b_v0.size(), 4);
     case 17: {
         uint32_t sc[4];
         std::memcpy(&sc[0], &s->cell_size, 4);
@@ -831,8 +947,9 @@ int tc_sim_residual(tc_sim *s, float dt_sub, double *rel, double *max_rel) {
     const tc::Mesh &m = s->mesh;
     double h[3] = {0.0, 0.0, 0.0};
     TC_HIP(hipMemcpyAsync(s->resid.p, h, sizeof(h), hipMemcpyHostToDevice, s->stream));
-    TC_HIP(tc::launch_residual(s->stream, s->P.p, s->drec.p, s->dcc.p, s->dlam.p, uint32_t(m.d_v0.size()), s->brec.p,
-                               s->blam.p, uint32_t(m.b_v0.size()), dt_sub, s->split_compliance, s->resid.p));
+    TC_HIP(tc::launch_residual(s->stream, s->Pbuf(s->last_p), s->drec.p, s->dcc.p, s->dlam_buf(s->last_lam),
+                               uint32_t(m.d_v0.size()), s->brec.p, s->blam_buf(s->last_lam), uint32_t(m.b_v0.size()),
+                               dt_sub, s->split_compliance, s->resid.p));
     TC_HIP(hipMemcpyAsync(h, s->resid.p, sizeof(h), hipMemcpyDeviceToHost, s->stream));
     TC_HIP(hipStreamSynchronize(s->stream));
     if (rel) *rel = h[1] > 0.0 ? std::sqrt(h[0]) / std::sqrt(h[1]) : 0.0;

@@ -1,0 +1,178 @@
+// Cone-tile planner (see tc_tiles.h).
+#include "tc_tiles.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace tc {
+namespace {
+
+uint32_t spread10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// Vertex order by the Morton code of the rest position (ties by index).
+std::vector<uint32_t> morton_order(const Mesh &m) {
+    const uint32_t nv = m.nv;
+    float bmin[3] = {3.4e38f, 3.4e38f, 3.4e38f}, bmax[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (uint32_t i = 0; i < nv; ++i)
+        for (int a = 0; a < 3; ++a) {
+            bmin[a] = std::min(bmin[a], m.pos0[3 * i + a]);
+            bmax[a] = std::max(bmax[a], m.pos0[3 * i + a]);
+        }
+    std::vector<uint64_t> key(nv);
+    for (uint32_t i = 0; i < nv; ++i) {
+        uint32_t q[3];
+        for (int a = 0; a < 3; ++a) {
+            const float ext = std::max(bmax[a] - bmin[a], 1e-9f);
+            const float u = std::min(std::max((m.pos0[3 * i + a] - bmin[a]) / ext, 0.0f), 1.0f);
+            q[a] = uint32_t(u * 1023.0f);
+        }
+        key[i] = (uint64_t((spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2])) << 32) | i;
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<uint32_t> ord(nv);
+    for (uint32_t i = 0; i < nv; ++i) ord[i] = uint32_t(key[i]);
+    return ord;
+}
+
+}  // namespace
+
+std::string build_tile_plan(const Mesh &m, uint32_t tile_vertices, uint32_t max_lds_vertices, TilePlan &P) {
+    P = TilePlan();
+    const uint32_t nv = m.nv;
+    const size_t nd = m.d_v0.size(), nb = m.b_v0.size();
+    if (nv == 0) return "empty mesh";
+    if (nd >= (1u << 26) || nb >= (1u << 26)) return "too many constraints for the item encoding";
+    // seam-gated bending reads its hinge vertices; this build generates none, but refuse if present
+    for (size_t k = 0; k < nb; ++k)
+        if (m.b_pad0[k] > 0.5f && m.b_pad0[k] < 1.5f) return "seam-gated bending is not supported by the fused path";
+
+    // compliance classes
+    std::vector<uint32_t> dcls(nd), bcls(nb);
+    for (size_t k = 0; k < nd; ++k) {
+        uint32_t c = 0;
+        for (; c < P.dcls_tc.size(); ++c)
+            if (P.dcls_tc[c] == m.d_tc[k] && P.dcls_cc[c] == m.d_cc[k]) break;
+        if (c == P.dcls_tc.size()) {
+            if (c == 16) return "more than 16 distinct distance compliances";
+            P.dcls_tc.push_back(m.d_tc[k]);
+            P.dcls_cc.push_back(m.d_cc[k]);
+        }
+        dcls[k] = c;
+    }
+    for (size_t k = 0; k < nb; ++k) {
+        uint32_t c = 0;
+        for (; c < P.bcls.size(); ++c)
+            if (P.bcls[c] == m.b_comp[k]) break;
+        if (c == P.bcls.size()) {
+            if (c == 16) return "more than 16 distinct bending compliances";
+            P.bcls.push_back(m.b_comp[k]);
+        }
+        bcls[k] = c;
+    }
+
+    // phases of one iteration and, per phase, the constraint touching each vertex
+    const uint32_t ndp = uint32_t(m.d_col.count.size()), nbp = uint32_t(m.b_col.count.size());
+    const uint32_t L = ndp + nbp;
+    P.nphases = L;
+    P.nd_phases = ndp;
+    std::vector<uint32_t> touch(size_t(L) * nv, UINT32_MAX);
+    for (uint32_t c = 0; c < ndp; ++c)
+        for (uint32_t k = m.d_col.offset[c]; k < m.d_col.offset[c] + m.d_col.count[c]; ++k) {
+            touch[size_t(c) * nv + m.d_v0[k]] = k;
+            touch[size_t(c) * nv + m.d_v1[k]] = k;
+        }
+    for (uint32_t c = 0; c < nbp; ++c)
+        for (uint32_t k = m.b_col.offset[c]; k < m.b_col.offset[c] + m.b_col.count[c]; ++k) {
+            touch[size_t(ndp + c) * nv + m.b_v2[k]] = k;
+            touch[size_t(ndp + c) * nv + m.b_v3[k]] = k;
+        }
+
+    const std::vector<uint32_t> order = morton_order(m);
+    const uint32_t T = std::max<uint32_t>(1, tile_vertices);
+    P.ntiles = (nv + T - 1) / T;
+    std::vector<uint32_t> tile_of(nv);
+    for (uint32_t r = 0; r < nv; ++r) tile_of[order[r]] = r / T;
+
+    std::vector<uint32_t> vstamp(nv, UINT32_MAX), local(nv, 0);
+    std::vector<uint32_t> dsel(nd, UINT32_MAX), bsel(nb, UINT32_MAX);
+    std::vector<std::vector<uint32_t>> phase_sel(L);
+    std::vector<uint32_t> cone;
+    P.ext_off.assign(P.ntiles + 1, 0);
+    P.n_own.assign(P.ntiles, 0);
+    P.ph_off.assign(size_t(P.ntiles) * (L + 1), 0);
+    P.plain_items = uint64_t(nd) + nb;
+
+    for (uint32_t t = 0; t < P.ntiles; ++t) {
+        cone.clear();
+        const uint32_t r0 = t * T, r1 = std::min(nv, r0 + T);
+        for (uint32_t r = r0; r < r1; ++r) {
+            const uint32_t v = order[r];
+            vstamp[v] = t;
+            local[v] = uint32_t(cone.size());
+            cone.push_back(v);
+        }
+        P.n_own[t] = r1 - r0;
+        auto need = [&](uint32_t v) {
+            if (vstamp[v] != t) {
+                vstamp[v] = t;
+                local[v] = uint32_t(cone.size());
+                cone.push_back(v);
+            }
+        };
+        for (uint32_t c = L; c-- > 0;) {
+            auto &sel = phase_sel[c];
+            sel.clear();
+            const bool dist = c < ndp;
+            const size_t base = size_t(c) * nv;
+            const size_t n_now = cone.size();  // vertices needed after phase c
+            for (size_t q = 0; q < n_now; ++q) {
+                const uint32_t k = touch[base + cone[q]];
+                if (k == UINT32_MAX) continue;
+                uint32_t &st = dist ? dsel[k] : bsel[k];
+                if (st == t) continue;
+                st = t;
+                sel.push_back(k);
+            }
+            for (uint32_t k : sel) {
+                if (dist) { need(m.d_v0[k]); need(m.d_v1[k]); }
+                else { need(m.b_v2[k]); need(m.b_v3[k]); }
+            }
+        }
+        if (cone.size() > max_lds_vertices || cone.size() > 65535)
+            return "dependency cone of a tile exceeds the LDS budget (" + std::to_string(cone.size()) + " vertices)";
+        P.max_ext = std::max<uint32_t>(P.max_ext, uint32_t(cone.size()));
+        P.ext_vid.insert(P.ext_vid.end(), cone.begin(), cone.end());
+        P.ext_off[t + 1] = uint32_t(P.ext_vid.size());
+        for (uint32_t c = 0; c < L; ++c) {
+            auto &sel = phase_sel[c];
+            std::sort(sel.begin(), sel.end());
+            P.ph_off[size_t(t) * (L + 1) + c] = uint32_t(P.items.size() / 3);
+            const bool dist = c < ndp;
+            for (uint32_t k : sel) {
+                const uint32_t a = dist ? m.d_v0[k] : m.b_v2[k];
+                const uint32_t b = dist ? m.d_v1[k] : m.b_v3[k];
+                const bool owner = tile_of[a] == t;
+                const uint32_t cls = dist ? dcls[k] : bcls[k];
+                const uint32_t shear = dist && m.d_type[k] == 1u ? 1u : 0u;
+                const float rest = dist ? m.d_rest[k] : m.b_rest[k];
+                uint32_t rb;
+                std::memcpy(&rb, &rest, 4);
+                P.items.push_back(local[a] | (local[b] << 16));
+                P.items.push_back(k | (cls << 26) | (shear << 30) | (owner ? 0x80000000u : 0u));
+                P.items.push_back(rb);
+            }
+        }
+        P.ph_off[size_t(t) * (L + 1) + L] = uint32_t(P.items.size() / 3);
+    }
+    return std::string();
+}
+
+}  // namespace tc

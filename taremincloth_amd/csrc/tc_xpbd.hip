@@ -53,14 +53,15 @@ __global__ void __launch_bounds__(256) k_distance(float4 *__restrict__ P, const 
     const uint32_t i0 = r.x, i1 = r.y & 0x7FFFFFFFu;
     const float4 q0 = P[i0], q1 = P[i1];
     const float w0 = q0.w, w1 = q1.w, wsum = w0 + w1;
-    if (wsum <= 1e-7f) return;
+    // lambda is zero at the substep start; the first sweep stores it even when it skips
+    if (wsum <= 1e-7f) { if (FIRST) lam[k] = 0.0f; return; }
     const f3 delta = xyz(q0) - xyz(q1);
     const float dist = length(delta);
-    if (dist <= 1e-7f) return;
+    if (dist <= 1e-7f) { if (FIRST) lam[k] = 0.0f; return; }
     const float c = dist - __uint_as_float(r.z);
     float comp = __uint_as_float(r.w);
     if (SPLIT && !(c >= 0.0f)) comp = ccomp[k];
-    if (comp >= 1e9f) return;
+    if (comp >= 1e9f) { if (FIRST) lam[k] = 0.0f; return; }
     const f3 dir = delta / dist;
     const float dt = fmaxf(a.dt, 1e-8f);
     const float alpha = comp / (dt * dt);
@@ -92,14 +93,14 @@ __global__ void __launch_bounds__(256) k_bending(float4 *__restrict__ P, const f
     const uint32_t k = off + t;
     const uint4 r = rec[k];
     const float comp = __uint_as_float(r.w);
-    if (comp >= 1e9f) return;
+    if (comp >= 1e9f) { if (FIRST) lam[k] = 0.0f; return; }
     const uint32_t i2 = r.x, i3 = r.y;
     const float4 q2 = P[i2], q3 = P[i3];
     const float w2 = q2.w, w3 = q3.w, wsum = w2 + w3;
-    if (wsum <= 1e-7f) return;
+    if (wsum <= 1e-7f) { if (FIRST) lam[k] = 0.0f; return; }
     const f3 delta = xyz(q2) - xyz(q3);
     const float dist = length(delta);
-    if (dist <= 1e-7f) return;
+    if (dist <= 1e-7f) { if (FIRST) lam[k] = 0.0f; return; }
     const f3 dir = delta / dist;
     const float dt = fmaxf(a.dt, 1e-8f);
     const float alpha = comp / (dt * dt);

@@ -215,6 +215,17 @@ class ClothSimulator:
     def set_compliances_raw(self, stretch, shear, bend):
         _lib.check(self._L.tc_sim_set_compliances_raw(self._h, float(stretch), float(shear), float(bend)))
 
+    def set_solver_path(self, path):
+        """0 auto, 1 one launch per colour, 2 one launch per iteration (identical results)."""
+        _lib.check(self._L.tc_sim_set_solver_path(self._h, int(path)))
+
+    def solver_info(self) -> dict:
+        out = (C.c_double * 8)()
+        _lib.check(self._L.tc_sim_solver_info(self._h, out))
+        keys = ["path", "tiles", "max_cone", "cone_vertices", "items_per_iteration", "constraints_per_iteration",
+                "plan_available", "phases"]
+        return dict(zip(keys, list(out)))
+
     def set_hash_cell_size(self, cell_size):
         _lib.check(self._L.tc_sim_set_hash_cell_size(self._h, float(cell_size)))
 

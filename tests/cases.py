@@ -183,3 +183,54 @@ def folded_cloth(n: int = 12, spacing: float = 0.02, gap: float = 0.004):
     pos[:, 0] += (np.arange(len(pos)) % 7) * 1e-4  # break exact symmetry
     inv = areal_inv_masses_np(pos.astype(np.float32), faces, 0.15)
     return pos.astype(np.float32), edges, faces, inv
+
+
+def icosphere(subdiv: int, radius: float = 0.3, noise_amp: float = 0.05, seed: int = 0, center=(0.0, 0.0, 0.0)):
+    """Icosphere (subdivision level `subdiv`: 10*4^s + 2 vertices, 20*4^s triangles) with the
+    radius perturbed by seeded 3-octave value noise of relative amplitude `noise_amp`."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    for _ in range(subdiv):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[k] = len(verts) - 1
+            return cache[k]
+        for a, b, c in f:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        f = nf
+    V = np.array(verts)
+    rng = np.random.default_rng(seed)
+    lat = [rng.uniform(-1, 1, (8 * 2 ** o,) * 3) for o in range(3)]
+
+    def value_noise(p):
+        out = np.zeros(len(p))
+        for o, g in enumerate(lat):
+            n = g.shape[0]
+            q = (p * 0.5 + 0.5) * (n - 1)
+            i0 = np.clip(np.floor(q).astype(int), 0, n - 2)
+            fr = q - i0
+            s = fr * fr * (3 - 2 * fr)
+            acc = 0.0
+            for dx in (0, 1):
+                for dy in (0, 1):
+                    for dz in (0, 1):
+                        w = ((s[:, 0] if dx else 1 - s[:, 0]) * (s[:, 1] if dy else 1 - s[:, 1]) *
+                             (s[:, 2] if dz else 1 - s[:, 2]))
+                        acc = acc + w * g[i0[:, 0] + dx, i0[:, 1] + dy, i0[:, 2] + dz]
+            out += acc / (2 ** o)
+        return out / 1.75
+    r = radius * (1.0 + noise_amp * value_noise(V))
+    pos = (V * r[:, None] + np.asarray(center)).astype(np.float32)
+    return pos, np.array(f, np.uint32)

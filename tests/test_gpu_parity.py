@@ -235,3 +235,77 @@ def test_empty_mesh_is_a_no_op():
     g = T.ClothSimulator(np.zeros((0, 3), np.float32), np.zeros((0, 2), np.uint32))
     assert g.step(1.0 / 60.0) == 20
     g.get_positions(np.empty(0, np.float32))
+
+
+@pytest.mark.parametrize("single_sided", [False, True])
+def test_mesh_collider_matches_oracle(single_sided):
+    """Triangle-mesh collider (Morton-ordered triangles, CCD + closest point + single-sided
+    recovery) with friction 0.4: cloth dropped on a noisy icosphere (level 3)."""
+    _say(f"mesh collider single_sided={single_sided}")
+    sp, sf = cases.icosphere(3, radius=0.15, center=(0.0, 0.0, 0.0))
+    tris = sp[sf.astype(np.int64)]
+    pos, e, f, inv = cases.drape_grid(32, 0.01, 0.17, seed=9)
+    g = T.ClothSimulator(pos, e, f, inv, thickness=0.003)
+    o = O.OracleSim(pos, e, f, inv, thickness=0.003)
+    for s in (g, o):
+        s.set_gravity(0.0, 0.0, -9.81)
+        s.set_mesh_collider_triangles(tris, friction=0.4, thickness=0.002, restitution=0.0, single_sided=single_sided)
+        s.set_solver_iterations(10)
+    for fr in range(8):
+        g.step(1.0 / 60.0, 10, 10)
+        o.step(1.0 / 60.0, 10, 10)
+        diff = float(np.abs(_pos(g) - o.get_positions().reshape(-1, 3)).max())
+        assert diff < TOL, f"frame {fr}: {diff:.3e}"
+    r = np.linalg.norm(_pos(g), axis=1)
+    assert r.min() > 0.13  # resting on the surface, not through it
+
+
+def _run_path(path, pos, e, f, inv, frames, damp=False, iters=7, colliders=True):
+    g = T.ClothSimulator(pos, e, f, inv, thickness=0.004)
+    g.set_compliances_raw(1e-8, 1e-6, 1e-5)
+    g.set_enable_self_collision(False)
+    if damp:
+        g.set_damping_all(2.0, 1.0, 3.0, 4.0)
+    if colliders:
+        g.add_sphere_collider([0.0, 0.0, 0.0], 0.12, 0.3, 0.0)
+    g.set_solver_path(path)
+    for _ in range(frames):
+        g.step(1.0 / 60.0, 5, iters)
+    out = _pos(g)
+    lam = g.lambdas()
+    return g, out, lam
+
+
+@pytest.mark.parametrize("damp", [False, True])
+def test_per_iteration_path_bit_exact_to_per_colour_path(damp):
+    """One launch per iteration over dependency-cone tiles gives bit-identical positions and
+    lambdas to one launch per colour (odd iteration count exercises the buffer ping-pong)."""
+    _say(f"fused vs per-colour damp={damp}")
+    pos, e, f, inv = cases.drape_grid(96, 0.004, 0.14, seed=3)
+    g1, p1, l1 = _run_path(1, pos, e, f, inv, 3, damp)
+    g2, p2, l2 = _run_path(2, pos, e, f, inv, 3, damp)
+    info = g2.solver_info()
+    _say(f"plan: {info}")
+    assert info["path"] == 2 and g1.solver_info()["path"] == 1
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(l1[0], l2[0]) and np.array_equal(l1[1], l2[1])
+    r1, r2 = g1.residual(1 / 300.0), g2.residual(1 / 300.0)
+    assert r1 == r2
+
+
+def test_per_iteration_path_matches_oracle_config1():
+    _say("fused config1 vs oracle")
+    g, o32, _ = _config1_pair()
+    g.set_solver_path(2)
+    for _ in range(10):
+        g.step(1.0 / 60.0, 10, 10)
+        o32.step(1.0 / 60.0, 10, 10)
+    assert float(np.abs(_pos(g) - o32.get_positions().reshape(-1, 3)).max()) < TOL
+
+
+def test_per_iteration_path_poked_and_pinned_bit_exact():
+    _say("fused poked grid")
+    pos, e, f, inv = cases.config1()
+    _, p1, l1 = _run_path(1, pos, e, f, inv, 4, iters=5, colliders=False)
+    _, p2, l2 = _run_path(2, pos, e, f, inv, 4, iters=5, colliders=False)
+    assert np.array_equal(p1, p2) and np.array_equal(l1[0], l2[0])
