@@ -109,6 +109,17 @@ def main():
     dt_sub = dt_frame / per_frame
     nd, nb = sim.num_distance_constraints(), sim.num_bending_constraints()
 
+    # the per-colour schedule, timed briefly for comparison (identical results)
+    sim.set_solver_path(1)
+    run_substeps(sim, max(args.warmup // 2, 10), dt_frame, per_frame)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    run_substeps(sim, max(args.steps // 4, 20), dt_frame, per_frame)
+    sim.synchronize()
+    per_colour_ms = (time.perf_counter() - t0) / max(args.steps // 4, 20) * 1e3
+    sim.set_solver_path(0)
+    info = sim.solver_info()
+
     run_substeps(sim, args.warmup, dt_frame, per_frame)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -123,12 +134,20 @@ def main():
 
     # per-kernel durations from HIP events on the simulator's stream
     kt = sim.time_kernels(dt_sub, reps=3)
-    dist_launch_ms = kt["distance"] / max(kt["distance_launches"], 1.0)
-    dist_bytes_per_launch = nd * BYTES_DIST * args.iters / max(kt["distance_launches"], 1.0)
-    achieved = dist_bytes_per_launch / (dist_launch_ms * 1e-3) / 1e9
     sweep_bytes = (nd * BYTES_DIST + nb * BYTES_BEND) * args.iters
-    sweep_ms = kt["distance"] + kt["bending"]
     substep_bytes = sweep_bytes + 2 * BYTES_VERTEX_PASS * len(pos)
+    if info["path"] == 2:
+        # dominant kernel: one whole solver iteration per launch (all 14 colour phases)
+        launch_ms = kt["fused"] / max(kt["fused_launches"], 1.0)
+        bytes_per_launch = sweep_bytes / args.iters
+        kname = "k_tile_iteration (one solver iteration: all distance and bending colours, cone tiles in LDS)"
+        sweep_ms = kt["fused"]
+    else:
+        launch_ms = kt["distance"] / max(kt["distance_launches"], 1.0)
+        bytes_per_launch = nd * BYTES_DIST * args.iters / max(kt["distance_launches"], 1.0)
+        kname = "k_distance (one colour of distance/shear constraints)"
+        sweep_ms = kt["distance"] + kt["bending"]
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -158,6 +177,12 @@ def main():
             "iterations": args.iters, "substeps_per_frame": per_frame, "dt_substep": dt_sub,
             "relative_residual": rel, "max_relative_residual": max_rel,
             "substep_ms_breakdown": {k: round(v, 5) for k, v in kt.items()},
+            "solver_schedule": {"path": "one launch per iteration (cone tiles)" if info["path"] == 2
+                                else "one launch per colour", "tiles": int(info["tiles"]),
+                                "max_cone_vertices": int(info["max_cone"]),
+                                "items_per_iteration": int(info["items_per_iteration"]),
+                                "constraints_per_iteration": int(info["constraints_per_iteration"]),
+                                "per_colour_path_ms_per_substep": per_colour_ms},
             "sweep_GBps_algorithmic": sweep_bytes / (sweep_ms * 1e-3) / 1e9,
             "substep_GBps_algorithmic": substep_bytes / (elapsed / args.steps) / 1e9,
             "constraint_projections_per_s": (nd + nb) * args.iters * sps,
@@ -165,8 +190,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_distance (one colour of distance/shear constraints)",
-            "bytes_per_launch": dist_bytes_per_launch, "launch_ms": dist_launch_ms,
+            "kernel": kname, "bytes_per_launch": bytes_per_launch, "launch_ms": launch_ms,
         },
         "cpu_baseline": cpu,
     }

@@ -141,11 +141,12 @@ int tc_sim_residual(tc_sim *sim, float dt_sub, double *rel, double *max_rel);
 /* Run the parts of one substep up to the velocity update (residual / warm-start studies). */
 int tc_sim_substep_parts(tc_sim *sim, float dt_sub, int collide, int self_collide, int update_velocity);
 /* Times `reps` substeps launched kernel by kernel (no graph) with HIP events recorded on the
- * simulator's stream and writes per-substep averages (milliseconds) to out[10]:
- * [0] predict  [1] all distance launches  [2] distance launches per substep  [3] all bending launches
- * [4] bending launches per substep  [5] collider resolve  [6] self-collision (hash + solve + apply)
- * [7] update_vel  [8] wall time of the substep between its first and last event  [9] reserved (0).
- * No reference counterpart (the reference's wgpu-profiler scopes, simulation/profile.rs). */
+ * simulator's stream, using the active solver schedule, and writes per-substep averages (milliseconds)
+ * to out[12]: [0] predict  [1] all distance launches  [2] distance launches per substep
+ * [3] all bending launches  [4] bending launches per substep  [5] collider resolve
+ * [6] self-collision (hash + solve + apply)  [7] update_vel  [8] wall time of the substep between its
+ * first and last event  [9] all one-launch-per-iteration launches  [10] those launches per substep
+ * [11] reserved (0).  No reference counterpart (the reference's wgpu-profiler scopes, simulation/profile.rs). */
 int tc_sim_time_kernels(tc_sim *sim, float dt_sub, uint32_t reps, double *out);
 /* Block until all work on the simulator's stream has finished. */
 int tc_sim_synchronize(tc_sim *sim);

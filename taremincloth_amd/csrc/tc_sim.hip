@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -129,7 +130,7 @@ struct tc_sim {
     hipGraph_t graph = nullptr;
     uint64_t graph_key[4] = {0, 0, 0, 0};
     uint64_t epoch = 1;  // bumped whenever a parameter or buffer baked into the graph changes
-    bool use_graph = true;
+    bool use_graph = true;  // TCLOTH_USE_GRAPH=0 launches kernel by kernel (profilers)
 
     ~tc_sim() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -425,6 +426,7 @@ int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, ui
     std::string err = tc::build_mesh(in, s->mesh);
     if (!err.empty()) { delete s; return fail(TC_ERR_VALUE, err); }
     if (pp.workgroup_size == 64 || pp.workgroup_size == 128 || pp.workgroup_size == 256) s->block = pp.workgroup_size;
+    if (const char *g = std::getenv("TCLOTH_USE_GRAPH")) s->use_graph = std::strcmp(g, "0") != 0;
     s->cell_size = s->mesh.cell_size;
     s->table_size = s->mesh.table_size;
 
@@ -522,10 +524,13 @@ int tc_sim_substep_parts(tc_sim *s, float dt_sub, int collide, int self_collide,
     return TC_OK;
 }
 
+static uint32_t pc_final(const tc_sim *s, bool fused) { return fused ? (s->iterations & 1u) : 0u; }
+
 int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
     if (!s || !out) return fail(TC_ERR_VALUE, "null argument");
-    for (int k = 0; k < 10; ++k) out[k] = 0.0;
+    for (int k = 0; k < 12; ++k) out[k] = 0.0;
     if (s->mesh.nv == 0 || reps == 0) return TC_OK;
+    const bool fused = s->fused_active() && s->iterations > 0;
     const tc::Mesh &m = s->mesh;
     const tc::SubstepArgs a = s->args(dt_sub);
     const uint32_t n = m.nv, B = s->block;
@@ -548,7 +553,21 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
         };
         hipError_t e = mark(0, [&] { return tc::launch_predict(s->stream, s->P.p, s->X.p, s->V.p, n, a, B); });
         if (e != hipSuccess) return e;
-        for (uint32_t it = 0; it < s->iterations; ++it) {
+        uint32_t pc = 0;
+        if (fused) {
+            const tc::FusedArgs fa = s->fused_args();
+            const tc::FusedPlanDev pd = s->plan_dev();
+            for (uint32_t it = 0; it < s->iterations; ++it) {
+                if ((e = mark(9, [&] {
+                         return tc::launch_tile_iteration(s->stream, s->Pbuf(pc), s->Pbuf(pc ^ 1u), s->X.p, pd,
+                                                          s->dlam_buf(pc), s->dlam_buf(pc ^ 1u), s->blam_buf(pc),
+                                                          s->blam_buf(pc ^ 1u), fa, a, it == 0, s->fused_block);
+                     })) != hipSuccess) return e;
+                pc ^= 1u;
+            }
+        }
+        float4 *Pc = s->Pbuf(pc);
+        for (uint32_t it = 0; !fused && it < s->iterations; ++it) {
             for (size_t c = 0; c < ndc; ++c)
                 if ((e = mark(1, [&] { return tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p,
                                                                   m.d_col.offset[c], m.d_col.count[c], a, it == 0,
@@ -558,22 +577,22 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
                                                                  m.b_col.offset[c], m.b_col.count[c], a, it == 0, B); })) != hipSuccess) return e;
         }
         if (!s->cols_h.empty() || !s->tris_h.empty())
-            if ((e = mark(5, [&] { return tc::launch_collision(s->stream, s->P.p, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()),
+            if ((e = mark(5, [&] { return tc::launch_collision(s->stream, Pc, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()),
                                                                s->tris.p, s->tri_bounds.p, uint32_t(s->tris_h.size()),
                                                                s->single_sided_recovery, dt_sub, B); })) != hipSuccess) return e;
         if (s->self_collision)
             if ((e = mark(6, [&] {
-                     hipError_t r = enqueue_hashes(s, s->P.p);
+                     hipError_t r = enqueue_hashes(s, Pc);
                      if (r != hipSuccess) return r;
                      const tc::SelfColArgs sa = s->sc_args();
                      const tc::SelfColTopo t = s->topo();
-                     if ((r = tc::launch_sc_solve(s->stream, s->P.p, s->X.p, t, s->vhash(), s->accum.p, sa, B)) != hipSuccess) return r;
-                     if ((r = tc::launch_sc_solve_ee(s->stream, s->P.p, s->X.p, t, s->ehash(), s->accum.p, sa, B)) != hipSuccess) return r;
-                     return tc::launch_sc_apply(s->stream, s->P.p, s->local_len.p, s->accum.p, sa, B);
+                     if ((r = tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa, B)) != hipSuccess) return r;
+                     if ((r = tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa, B)) != hipSuccess) return r;
+                     return tc::launch_sc_apply(s->stream, Pc, s->local_len.p, s->accum.p, sa, B);
                  })) != hipSuccess) return e;
-        return mark(7, [&] { return tc::launch_update_vel(s->stream, s->P.p, s->X.p, s->V.p, n, dt_sub, B); });
+        return mark(7, [&] { return tc::launch_update_vel(s->stream, Pc, s->X.p, s->V.p, n, dt_sub, B); });
     };
-    double acc[10] = {0};
+    double acc[12] = {0};
     for (uint32_t r = 0; r < reps; ++r) {
         rel.clear();
         hipError_t e = run();
@@ -592,9 +611,12 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
         acc[8] += wall;
     }
     for (auto &x : ev) TC_HIP(hipEventDestroy(x));
-    for (int k = 0; k < 9; ++k) out[k] = acc[k] / double(reps);
-    out[2] = double(ndc) * s->iterations;
-    out[4] = double(nbc) * s->iterations;
+    for (int k = 0; k < 12; ++k) out[k] = acc[k] / double(reps);
+    out[2] = fused ? 0.0 : double(ndc) * s->iterations;
+    out[4] = fused ? 0.0 : double(nbc) * s->iterations;
+    out[10] = fused ? double(s->iterations) : 0.0;
+    s->last_p = pc_final(s, fused);
+    s->last_lam = s->last_p;
     return TC_OK;
 }
 

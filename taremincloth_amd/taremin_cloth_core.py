@@ -285,11 +285,11 @@ class ClothSimulator:
 
     def time_kernels(self, dt_sub, reps=5) -> dict:
         """Per-substep kernel times (ms) from HIP events on the simulator's stream."""
-        out = (C.c_double * 10)()
+        out = (C.c_double * 12)()
         _lib.check(self._L.tc_sim_time_kernels(self._h, float(dt_sub), int(reps), out))
         keys = ["predict", "distance", "distance_launches", "bending", "bending_launches", "collision",
-                "self_collision", "update_vel", "substep_wall"]
-        return dict(zip(keys, list(out)[:9]))
+                "self_collision", "update_vel", "substep_wall", "fused", "fused_launches"]
+        return dict(zip(keys, list(out)[:11]))
 
     def synchronize(self):
         _lib.check(self._L.tc_sim_synchronize(self._h))
