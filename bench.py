@@ -118,6 +118,7 @@ def main():
     sim.synchronize()
     per_colour_ms = (time.perf_counter() - t0) / max(args.steps // 4, 20) * 1e3
     sim.set_solver_path(0)
+    tune = sim.autotune(dt_sub)
     info = sim.solver_info()
 
     run_substeps(sim, args.warmup, dt_frame, per_frame)
@@ -182,6 +183,7 @@ def main():
                                 "max_cone_vertices": int(info["max_cone"]),
                                 "items_per_iteration": int(info["items_per_iteration"]),
                                 "constraints_per_iteration": int(info["constraints_per_iteration"]),
+                                "plan_note": info["plan_note"], "autotune": tune,
                                 "per_colour_path_ms_per_substep": per_colour_ms},
             "sweep_GBps_algorithmic": sweep_bytes / (sweep_ms * 1e-3) / 1e9,
             "substep_GBps_algorithmic": substep_bytes / (elapsed / args.steps) / 1e9,

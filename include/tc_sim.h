@@ -117,6 +117,14 @@ int tc_sim_set_solver_path(tc_sim *sim, int path);
  *          items per iteration, constraints per iteration, plan available (0|1), colour phases] */
 int tc_sim_solver_info(tc_sim *sim, double *out);
 
+/* Autotuner (no reference counterpart; the reference exposes a fixed workgroup_size knob,
+ * pipeline_cache.rs:25-48): times 16 solver iterations of the per-colour schedule and of the
+ * per-iteration schedule for tile sizes {256, 512, 1024, 2048} vertices x block sizes {256, 512, 1024}
+ * with HIP events on the simulator's stream, keeps the fastest, and writes
+ * out[4] = [tile vertices (0 = per-colour kept), block size, ms per iteration, per-colour ms per iteration].
+ * Leaves the cloth state unchanged (the trial iterations start from a fresh prediction). */
+int tc_sim_autotune(tc_sim *sim, float dt_sub, double *out);
+
 /* num_vertices / num_distance_constraints / num_bending_constraints getters, lib.rs:2928-2950 */
 uint32_t tc_sim_num_vertices(const tc_sim *sim);
 uint32_t tc_sim_num_distance_constraints(const tc_sim *sim);

@@ -13,7 +13,7 @@
 // word1 = constraint index (26 bits) | compliance class << 26 | shear << 30 |
 // owner << 31, word2 = rest length bits.
 //
-// Tiles are Morton-ordered, so consecutive tiles are spatial neighbours; the
+// Tiles come out of a depth-first coordinate bisection, so consecutive tiles are spatial neighbours; the
 // block -> tile map keeps a contiguous run of tiles on each of the 8 XCDs
 // (workgroups are dealt to XCDs round-robin) so overlapping cone reads of P,
 // lambda and items share that XCD's L2.
@@ -26,8 +26,57 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks) {
     return (b % 8u) * per + b / 8u;
 }
 
+// Projection of one item from LDS (the k_distance / k_bending arithmetic).
+template <bool DIST, bool DAMP>
+__device__ __forceinline__ void project_item(float4 *__restrict__ sp, const float4 *__restrict__ sx, uint32_t w0,
+                                             uint32_t w1, uint32_t w2, float l, float *__restrict__ lam_out,
+                                             const FusedArgs &fa, const SubstepArgs &a, float dt) {
+    const float rest = __uint_as_float(w2);
+    const uint32_t l0 = w0 & 0xFFFFu, l1 = w0 >> 16;
+    const uint32_t k = w1 & 0x03FFFFFFu, cls = (w1 >> 26) & 15u;
+    const bool shear = ((w1 >> 30) & 1u) != 0u, owner = (w1 >> 31) != 0u;
+    const float4 q0 = sp[l0], q1 = sp[l1];
+    const float wa = q0.w, wb = q1.w, wsum = wa + wb;
+    float comp;
+    if (DIST) {
+        // an owner that skips the projection still carries lambda into the output buffer
+        if (wsum <= 1e-7f) { if (owner) lam_out[k] = l; return; }
+    } else {
+        comp = fa.bcomp[cls];
+        if (comp >= 1e9f || wsum <= 1e-7f) { if (owner) lam_out[k] = l; return; }
+    }
+    const f3 delta = xyz(q0) - xyz(q1);
+    const float d = length(delta);
+    if (d <= 1e-7f) { if (owner) lam_out[k] = l; return; }
+    const float cval = d - rest;
+    if (DIST) {
+        comp = cval >= 0.0f ? fa.dtc[cls] : fa.dcc[cls];
+        if (comp >= 1e9f) { if (owner) lam_out[k] = l; return; }
+    }
+    const f3 dir = delta / d;
+    const float alpha = comp / (dt * dt);
+    float dl;
+    if (DAMP) {
+        const float dc = DIST ? (shear ? a.shear_damping : (cval >= 0.0f ? a.tension_damping : a.compression_damping))
+                              : a.bending_damping;
+        const float gamma = fminf(dc * 1e-5f / dt, 1e4f);
+        const f3 rel = (xyz(q0) - xyz(sx[l0])) - (xyz(q1) - xyz(sx[l1]));
+        dl = (-cval - alpha * l - gamma * dot(dir, rel)) / ((1.0f + gamma) * wsum + alpha);
+    } else {
+        dl = (-cval - alpha * l) / (wsum + alpha);
+    }
+    if (owner) lam_out[k] = l + dl;
+    const f3 c0 = dir * (wa * dl), c1 = dir * (-wb * dl);
+    sp[l0] = make_float4(q0.x + c0.x, q0.y + c0.y, q0.z + c0.z, wa);
+    sp[l1] = make_float4(q1.x + c1.x, q1.y + c1.y, q1.z + c1.z, wb);
+}
+
+// The first item of every phase a thread owns is software-pipelined: its
+// words are loaded one phase ahead and its lambda right after the current
+// phase's LDS work, so global latency overlaps the barrier.  Further items of a
+// phase (phases larger than the block) are loaded in place.
 template <bool FIRST, bool DAMP>
-__global__ void __launch_bounds__(512) k_tile_iteration(
+__global__ void __launch_bounds__(1024) k_tile_iteration(
     const float4 *__restrict__ Pin, float4 *__restrict__ Pout, const float4 *__restrict__ X,
     const uint32_t *__restrict__ ext_off, const uint32_t *__restrict__ ext_vid, const uint32_t *__restrict__ n_own,
     const uint32_t *__restrict__ ph_off, const uint32_t *__restrict__ items, const float *__restrict__ dlam_in,
@@ -36,80 +85,55 @@ __global__ void __launch_bounds__(512) k_tile_iteration(
     extern __shared__ float4 sp[];
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= fa.ntiles) return;
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t *po = ph_off + size_t(tile) * (fa.nphases + 1);
+    const uint32_t L = fa.nphases, ndp = fa.nd_phases;
+    // first item of phase 0 (words) and its lambda
+    uint32_t i_cur = po[0] + tid;
+    bool h_cur = i_cur < po[1];
+    uint32_t c0 = 0, c1 = 0, c2 = 0;
+    if (h_cur) { c0 = items[3 * size_t(i_cur)]; c1 = items[3 * size_t(i_cur) + 1]; c2 = items[3 * size_t(i_cur) + 2]; }
     const uint32_t e0 = ext_off[tile], ne = ext_off[tile + 1] - e0;
     float4 *sx = sp + fa.max_ext;  // DAMP only: committed positions of the cone
-    for (uint32_t l = threadIdx.x; l < ne; l += blockDim.x) {
+    for (uint32_t l = tid; l < ne; l += bs) {
         const uint32_t g = ext_vid[e0 + l];
         sp[l] = Pin[g];
         if (DAMP) sx[l] = X[g];
     }
+    float lc = 0.0f;
+    if (!FIRST && h_cur) lc = (ndp > 0 ? dlam_in : blam_in)[c1 & 0x03FFFFFFu];
     __syncthreads();
     const float dt = fmaxf(a.dt, 1e-8f);
-    const uint32_t *po = ph_off + size_t(tile) * (fa.nphases + 1);
-    for (uint32_t c = 0; c < fa.nphases; ++c) {
-        const bool dist = c < fa.nd_phases;
-        const uint32_t i1 = po[c + 1];
-        for (uint32_t i = po[c] + threadIdx.x; i < i1; i += blockDim.x) {
-            const uint32_t w0 = items[3 * size_t(i)], w1 = items[3 * size_t(i) + 1];
-            const float rest = __uint_as_float(items[3 * size_t(i) + 2]);
-            const uint32_t l0 = w0 & 0xFFFFu, l1 = w0 >> 16;
-            const uint32_t k = w1 & 0x03FFFFFFu, cls = (w1 >> 26) & 15u;
-            const bool shear = ((w1 >> 30) & 1u) != 0u, owner = (w1 >> 31) != 0u;
-            const float4 q0 = sp[l0], q1 = sp[l1];
-            const float wa = q0.w, wb = q1.w, wsum = wa + wb;
-            if (dist) {
-                // an owner that skips the projection still carries lambda into the output buffer
-                const float l = FIRST ? 0.0f : dlam_in[k];
-                if (wsum <= 1e-7f) { if (owner) dlam_out[k] = l; continue; }
-                const f3 delta = xyz(q0) - xyz(q1);
-                const float d = length(delta);
-                if (d <= 1e-7f) { if (owner) dlam_out[k] = l; continue; }
-                const float cval = d - rest;
-                const float comp = cval >= 0.0f ? fa.dtc[cls] : fa.dcc[cls];
-                if (comp >= 1e9f) { if (owner) dlam_out[k] = l; continue; }
-                const f3 dir = delta / d;
-                const float alpha = comp / (dt * dt);
-                float dl;
-                if (DAMP) {
-                    const float dc = shear ? a.shear_damping : (cval >= 0.0f ? a.tension_damping : a.compression_damping);
-                    const float gamma = fminf(dc * 1e-5f / dt, 1e4f);
-                    const f3 rel = (xyz(q0) - xyz(sx[l0])) - (xyz(q1) - xyz(sx[l1]));
-                    dl = (-cval - alpha * l - gamma * dot(dir, rel)) / ((1.0f + gamma) * wsum + alpha);
-                } else {
-                    dl = (-cval - alpha * l) / (wsum + alpha);
-                }
-                if (owner) dlam_out[k] = l + dl;
-                const f3 c0 = dir * (wa * dl), c1 = dir * (-wb * dl);
-                sp[l0] = make_float4(q0.x + c0.x, q0.y + c0.y, q0.z + c0.z, wa);
-                sp[l1] = make_float4(q1.x + c1.x, q1.y + c1.y, q1.z + c1.z, wb);
-            } else {
-                const float l = FIRST ? 0.0f : blam_in[k];
-                const float comp = fa.bcomp[cls];
-                if (comp >= 1e9f || wsum <= 1e-7f) { if (owner) blam_out[k] = l; continue; }
-                const f3 delta = xyz(q0) - xyz(q1);
-                const float d = length(delta);
-                if (d <= 1e-7f) { if (owner) blam_out[k] = l; continue; }
-                const f3 dir = delta / d;
-                const float alpha = comp / (dt * dt);
-                const float cval = d - rest;
-                float dl;
-                if (DAMP) {
-                    const float gamma = fminf(a.bending_damping * 1e-5f / dt, 1e4f);
-                    const f3 rel = (xyz(q0) - xyz(sx[l0])) - (xyz(q1) - xyz(sx[l1]));
-                    dl = (-cval - alpha * l - gamma * dot(dir, rel)) / ((1.0f + gamma) * wsum + alpha);
-                } else {
-                    dl = (-cval - alpha * l) / (wsum + alpha);
-                }
-                if (owner) blam_out[k] = l + dl;
-                const f3 c0 = dir * (wa * dl), c1 = dir * (-wb * dl);
-                sp[l0] = make_float4(q0.x + c0.x, q0.y + c0.y, q0.z + c0.z, wa);
-                sp[l1] = make_float4(q1.x + c1.x, q1.y + c1.y, q1.z + c1.z, wb);
-            }
+    for (uint32_t c = 0; c < L; ++c) {
+        const bool dist = c < ndp;
+        // issue the next phase's first item
+        uint32_t n0 = 0, n1 = 0, n2 = 0;
+        bool h_nxt = false;
+        if (c + 1 < L) {
+            const uint32_t i_nxt = po[c + 1] + tid;
+            h_nxt = i_nxt < po[c + 2];
+            if (h_nxt) { n0 = items[3 * size_t(i_nxt)]; n1 = items[3 * size_t(i_nxt) + 1]; n2 = items[3 * size_t(i_nxt) + 2]; }
         }
+        const float *lam_in = dist ? dlam_in : blam_in;
+        float *lam_out = dist ? dlam_out : blam_out;
+        if (h_cur) {
+            if (dist) project_item<true, DAMP>(sp, sx, c0, c1, c2, lc, lam_out, fa, a, dt);
+            else project_item<false, DAMP>(sp, sx, c0, c1, c2, lc, lam_out, fa, a, dt);
+        }
+        for (uint32_t i = po[c] + tid + bs; i < po[c + 1]; i += bs) {
+            const uint32_t w0 = items[3 * size_t(i)], w1 = items[3 * size_t(i) + 1], w2 = items[3 * size_t(i) + 2];
+            const float l = FIRST ? 0.0f : lam_in[w1 & 0x03FFFFFFu];
+            if (dist) project_item<true, DAMP>(sp, sx, w0, w1, w2, l, lam_out, fa, a, dt);
+            else project_item<false, DAMP>(sp, sx, w0, w1, w2, l, lam_out, fa, a, dt);
+        }
+        // the next phase's lambda, now that its item has arrived
+        c0 = n0; c1 = n1; c2 = n2; h_cur = h_nxt;
+        lc = 0.0f;
+        if (!FIRST && h_cur) lc = (c + 1 < ndp ? dlam_in : blam_in)[c1 & 0x03FFFFFFu];
         __syncthreads();
     }
     const uint32_t own = n_own[tile];
-    for (uint32_t l = threadIdx.x; l < own; l += blockDim.x) Pout[ext_vid[e0 + l]] = sp[l];
+    for (uint32_t l = tid; l < own; l += bs) Pout[ext_vid[e0 + l]] = sp[l];
 }
 
 hipError_t launch_tile_iteration(hipStream_t s, const float4 *Pin, float4 *Pout, const float4 *X,

@@ -122,7 +122,8 @@ struct tc_sim {
     std::string fused_reason;     // why the plan is unavailable
     tc::TilePlan plan;
     DevBuf<uint32_t> pl_ext_off, pl_ext_vid, pl_n_own, pl_ph_off, pl_items;
-    uint32_t fused_block = 512;
+    uint32_t fused_block = 1024;
+    uint32_t tile_vertices = 1024;  // TCLOTH_TILE_VERTICES overrides; tc_sim_autotune picks per mesh
     uint32_t last_p = 0, last_lam = 0;  // buffers holding P and lambda after the last substep
 
     // frame graph cache
@@ -310,7 +311,7 @@ void build_plan(tc_sim *s) {
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && v > 0) max_lds = v;
     }
     const uint32_t max_vertices = uint32_t(max_lds / int(2 * sizeof(float4)));  // room for the damped variant
-    std::string why = tc::build_tile_plan(s->mesh, 1024, max_vertices, s->plan);
+    std::string why = tc::build_tile_plan(s->mesh, s->tile_vertices, max_vertices, s->plan);
     if (!why.empty()) { s->fused_reason = why; return; }
     hipError_t e;
     if ((e = s->pl_ext_off.upload(s->plan.ext_off.data(), s->plan.ext_off.size(), s->stream)) != hipSuccess ||
@@ -427,6 +428,14 @@ int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, ui
     if (!err.empty()) { delete s; return fail(TC_ERR_VALUE, err); }
     if (pp.workgroup_size == 64 || pp.workgroup_size == 128 || pp.workgroup_size == 256) s->block = pp.workgroup_size;
     if (const char *g = std::getenv("TCLOTH_USE_GRAPH")) s->use_graph = std::strcmp(g, "0") != 0;
+    if (const char *tv = std::getenv("TCLOTH_TILE_VERTICES")) {
+        const long v = std::strtol(tv, nullptr, 10);
+        if (v >= 64 && v <= 16384) s->tile_vertices = uint32_t(v);
+    }
+    if (const char *fb = std::getenv("TCLOTH_TILE_BLOCK")) {
+        const long v = std::strtol(fb, nullptr, 10);
+        if (v == 256 || v == 512 || v == 1024) s->fused_block = uint32_t(v);
+    }
     s->cell_size = s->mesh.cell_size;
     s->table_size = s->mesh.table_size;
 
@@ -524,6 +533,52 @@ int tc_sim_substep_parts(tc_sim *s, float dt_sub, int collide, int self_collide,
     return TC_OK;
 }
 
+// Times `its` solver iterations of the current schedule from a freshly predicted state.
+static int time_iterations(tc_sim *s, float dt_sub, uint32_t its, bool fused, double *ms_per_iteration) {
+    const tc::SubstepArgs a = s->args(dt_sub);
+    const tc::Mesh &m = s->mesh;
+    hipEvent_t e0, e1;
+    TC_HIP(hipEventCreate(&e0));
+    TC_HIP(hipEventCreate(&e1));
+    hipError_t e = tc::launch_predict(s->stream, s->P.p, s->X.p, s->V.p, m.nv, a, s->block);
+    uint32_t pc = 0;
+    auto one = [&](uint32_t it) -> hipError_t {
+        if (fused) {
+            hipError_t r = tc::launch_tile_iteration(s->stream, s->Pbuf(pc), s->Pbuf(pc ^ 1u), s->X.p, s->plan_dev(),
+                                                     s->dlam_buf(pc), s->dlam_buf(pc ^ 1u), s->blam_buf(pc),
+                                                     s->blam_buf(pc ^ 1u), s->fused_args(), a, it == 0, s->fused_block);
+            pc ^= 1u;
+            return r;
+        }
+        for (size_t c = 0; c < m.d_col.count.size(); ++c) {
+            hipError_t r = tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p,
+                                               m.d_col.offset[c], m.d_col.count[c], a, it == 0, s->split_compliance,
+                                               s->block);
+            if (r != hipSuccess) return r;
+        }
+        for (size_t c = 0; c < m.b_col.count.size(); ++c) {
+            hipError_t r = tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p, m.b_col.offset[c],
+                                              m.b_col.count[c], a, it == 0, s->block);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    };
+    if (e == hipSuccess) e = one(0);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, s->stream);
+    for (uint32_t it = 1; e == hipSuccess && it <= its; ++it) e = one(it);
+    if (e == hipSuccess) e = hipEventRecord(e1, s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    TC_HIP(e);
+    *ms_per_iteration = double(ms) / double(its);
+    s->last_p = 0;
+    s->last_lam = 0;
+    return TC_OK;
+}
+
 static uint32_t pc_final(const tc_sim *s, bool fused) { return fused ? (s->iterations & 1u) : 0u; }
 
 int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
@@ -617,6 +672,49 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
     out[10] = fused ? double(s->iterations) : 0.0;
     s->last_p = pc_final(s, fused);
     s->last_lam = s->last_p;
+    return TC_OK;
+}
+
+int tc_sim_autotune(tc_sim *s, float dt_sub, double *out) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    double best_ms = 0.0;
+    TC_HIP(hipStreamSynchronize(s->stream));
+    int r = time_iterations(s, dt_sub, 16, false, &best_ms);
+    if (r != TC_OK) return r;
+    const double per_colour_ms = best_ms;
+    uint32_t best_tile = 0, best_block = 0;
+    const uint32_t tiles[] = {256, 512, 1024, 2048};
+    const uint32_t blocks[] = {256, 512, 1024};
+    const uint32_t keep_tile = s->tile_vertices, keep_block = s->fused_block;
+    for (uint32_t tv : tiles) {
+        s->tile_vertices = tv;
+        build_plan(s);
+        if (!s->fused_ready) continue;
+        for (uint32_t bk : blocks) {
+            s->fused_block = bk;
+            double ms = 0.0;
+            if ((r = time_iterations(s, dt_sub, 16, true, &ms)) != TC_OK) return r;
+            if (ms < best_ms) { best_ms = ms; best_tile = tv; best_block = bk; }
+        }
+    }
+    if (best_tile) {
+        s->tile_vertices = best_tile;
+        s->fused_block = best_block;
+        build_plan(s);
+        s->solver_path = 2;
+    } else {
+        s->tile_vertices = keep_tile;
+        s->fused_block = keep_block;
+        build_plan(s);
+        s->solver_path = 1;
+    }
+    s->touch();
+    if (out) {
+        out[0] = best_tile;
+        out[1] = best_block;
+        out[2] = best_ms;
+        out[3] = per_colour_ms;
+    }
     return TC_OK;
 }
 
@@ -876,6 +974,7 @@ int tc_sim_solver_info(tc_sim *s, double *out) {
     out[5] = double(s->plan.plain_items);
     out[6] = s->fused_ready ? 1.0 : 0.0;
     out[7] = s->plan.nphases;
+    g_err = s->fused_ready ? std::string() : s->fused_reason;
     return TC_OK;
 }
 

@@ -9,37 +9,34 @@
 namespace tc {
 namespace {
 
-uint32_t spread10(uint32_t v) {
-    v = (v * 0x00010001u) & 0xFF0000FFu;
-    v = (v * 0x00000101u) & 0x0F00F00Fu;
-    v = (v * 0x00000011u) & 0xC30C30C3u;
-    v = (v * 0x00000005u) & 0x49249249u;
-    return v;
-}
-
-// Vertex order by the Morton code of the rest position (ties by index).
-std::vector<uint32_t> morton_order(const Mesh &m) {
-    const uint32_t nv = m.nv;
-    float bmin[3] = {3.4e38f, 3.4e38f, 3.4e38f}, bmax[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
-    for (uint32_t i = 0; i < nv; ++i)
-        for (int a = 0; a < 3; ++a) {
-            bmin[a] = std::min(bmin[a], m.pos0[3 * i + a]);
-            bmax[a] = std::max(bmax[a], m.pos0[3 * i + a]);
-        }
-    std::vector<uint64_t> key(nv);
-    for (uint32_t i = 0; i < nv; ++i) {
-        uint32_t q[3];
-        for (int a = 0; a < 3; ++a) {
-            const float ext = std::max(bmax[a] - bmin[a], 1e-9f);
-            const float u = std::min(std::max((m.pos0[3 * i + a] - bmin[a]) / ext, 0.0f), 1.0f);
-            q[a] = uint32_t(u * 1023.0f);
-        }
-        key[i] = (uint64_t((spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2])) << 32) | i;
+// Recursive coordinate bisection of the rest positions: split at the median of
+// the longest extent until a part holds at most `leaf` vertices.  Parts come
+// out depth first, so consecutive parts are spatial neighbours.
+void rcb(const std::vector<float> &pos, uint32_t *idx, size_t n, uint32_t leaf, std::vector<uint32_t> &order,
+         std::vector<uint32_t> &part_size) {
+    if (n <= leaf) {
+        order.insert(order.end(), idx, idx + n);
+        part_size.push_back(uint32_t(n));
+        return;
     }
-    std::sort(key.begin(), key.end());
-    std::vector<uint32_t> ord(nv);
-    for (uint32_t i = 0; i < nv; ++i) ord[i] = uint32_t(key[i]);
-    return ord;
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (size_t k = 0; k < n; ++k)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], pos[3 * size_t(idx[k]) + a]);
+            hi[a] = std::max(hi[a], pos[3 * size_t(idx[k]) + a]);
+        }
+    int ax = 0;
+    for (int a = 1; a < 3; ++a)
+        if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
+    // split into parts whose sizes are multiples of the leaf size where possible
+    const size_t leaves = (n + leaf - 1) / leaf;
+    const size_t left = ((leaves + 1) / 2) * size_t(leaf) < n ? ((leaves + 1) / 2) * size_t(leaf) : n / 2;
+    std::nth_element(idx, idx + left, idx + n, [&](uint32_t a, uint32_t b) {
+        const float pa = pos[3 * size_t(a) + ax], pb = pos[3 * size_t(b) + ax];
+        return pa < pb || (pa == pb && a < b);
+    });
+    rcb(pos, idx, left, leaf, order, part_size);
+    rcb(pos, idx + left, n - left, leaf, order, part_size);
 }
 
 }  // namespace
@@ -95,11 +92,15 @@ std::string build_tile_plan(const Mesh &m, uint32_t tile_vertices, uint32_t max_
             touch[size_t(ndp + c) * nv + m.b_v3[k]] = k;
         }
 
-    const std::vector<uint32_t> order = morton_order(m);
     const uint32_t T = std::max<uint32_t>(1, tile_vertices);
-    P.ntiles = (nv + T - 1) / T;
-    std::vector<uint32_t> tile_of(nv);
-    for (uint32_t r = 0; r < nv; ++r) tile_of[order[r]] = r / T;
+    std::vector<uint32_t> order, part_size, idx(nv);
+    std::iota(idx.begin(), idx.end(), 0u);
+    rcb(m.pos0, idx.data(), nv, T, order, part_size);
+    P.ntiles = uint32_t(part_size.size());
+    std::vector<uint32_t> part_begin(P.ntiles + 1, 0), tile_of(nv);
+    for (uint32_t t = 0; t < P.ntiles; ++t) part_begin[t + 1] = part_begin[t] + part_size[t];
+    for (uint32_t t = 0; t < P.ntiles; ++t)
+        for (uint32_t r = part_begin[t]; r < part_begin[t + 1]; ++r) tile_of[order[r]] = t;
 
     std::vector<uint32_t> vstamp(nv, UINT32_MAX), local(nv, 0);
     std::vector<uint32_t> dsel(nd, UINT32_MAX), bsel(nb, UINT32_MAX);
@@ -112,7 +113,7 @@ std::string build_tile_plan(const Mesh &m, uint32_t tile_vertices, uint32_t max_
 
     for (uint32_t t = 0; t < P.ntiles; ++t) {
         cone.clear();
-        const uint32_t r0 = t * T, r1 = std::min(nv, r0 + T);
+        const uint32_t r0 = part_begin[t], r1 = part_begin[t + 1];
         for (uint32_t r = r0; r < r1; ++r) {
             const uint32_t v = order[r];
             vstamp[v] = t;

@@ -1,8 +1,8 @@
 // Cone-tile plan for running one full solver iteration (every distance colour,
 // then every bending colour) in a single kernel launch.
 //
-// The vertices are cut into tiles of spatially close vertices (Morton order of
-// the rest positions).  For each tile the planner walks the iteration's colour
+// The vertices are cut into tiles of spatially close vertices (recursive
+// coordinate bisection of the rest positions).  For each tile the planner walks the iteration's colour
 // phases backwards and collects the backward dependency cone of the tile's
 // vertices: the constraints of each phase that touch a vertex whose value is
 // needed later, and the vertices those constraints read.  A workgroup loads

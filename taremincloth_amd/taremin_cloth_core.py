@@ -219,12 +219,21 @@ class ClothSimulator:
         """0 auto, 1 one launch per colour, 2 one launch per iteration (identical results)."""
         _lib.check(self._L.tc_sim_set_solver_path(self._h, int(path)))
 
+    def autotune(self, dt_sub=1.0 / 600.0) -> dict:
+        """Pick the fastest solver schedule / tile size / block size for this mesh (identical results)."""
+        out = (C.c_double * 4)()
+        _lib.check(self._L.tc_sim_autotune(self._h, float(dt_sub), out))
+        return {"tile_vertices": int(out[0]), "block": int(out[1]), "ms_per_iteration": out[2],
+                "per_colour_ms_per_iteration": out[3]}
+
     def solver_info(self) -> dict:
         out = (C.c_double * 8)()
         _lib.check(self._L.tc_sim_solver_info(self._h, out))
         keys = ["path", "tiles", "max_cone", "cone_vertices", "items_per_iteration", "constraints_per_iteration",
                 "plan_available", "phases"]
-        return dict(zip(keys, list(out)))
+        info = dict(zip(keys, list(out)))
+        info["plan_note"] = (self._L.tc_last_error() or b"").decode(errors="replace")
+        return info
 
     def set_hash_cell_size(self, cell_size):
         _lib.check(self._L.tc_sim_set_hash_cell_size(self._h, float(cell_size)))

@@ -310,3 +310,33 @@ def test_per_iteration_path_poked_and_pinned_bit_exact():
     _, p1, l1 = _run_path(1, pos, e, f, inv, 4, iters=5, colliders=False)
     _, p2, l2 = _run_path(2, pos, e, f, inv, 4, iters=5, colliders=False)
     assert np.array_equal(p1, p2) and np.array_equal(l1[0], l2[0])
+
+
+@pytest.mark.parametrize("tile,block", [("256", "256"), ("2048", "1024")])
+def test_per_iteration_path_tile_and_block_sizes(tile, block, monkeypatch):
+    """Small tiles with small blocks (several items per thread and phase) and large tiles."""
+    _say(f"fused tile={tile} block={block}")
+    monkeypatch.setenv("TCLOTH_TILE_VERTICES", tile)
+    monkeypatch.setenv("TCLOTH_TILE_BLOCK", block)
+    pos, e, f, inv = cases.drape_grid(80, 0.004, 0.14, seed=4)
+    _, p1, l1 = _run_path(1, pos, e, f, inv, 2, iters=6)
+    g2, p2, l2 = _run_path(2, pos, e, f, inv, 2, iters=6)
+    assert g2.solver_info()["path"] == 2
+    assert np.array_equal(p1, p2) and np.array_equal(l1[0], l2[0]) and np.array_equal(l1[1], l2[1])
+
+
+def test_autotune_keeps_results_identical():
+    _say("autotune")
+    pos, e, f, inv = cases.drape_grid(96, 0.004, 0.14, seed=6)
+    ref = T.ClothSimulator(pos, e, f, inv, thickness=0.004)
+    tun = T.ClothSimulator(pos, e, f, inv, thickness=0.004)
+    for g in (ref, tun):
+        g.set_compliances_raw(1e-8, 1e-6, 1e-5)
+        g.set_enable_self_collision(False)
+    ref.set_solver_path(1)
+    tune = tun.autotune(1.0 / 300.0)
+    _say(f"autotune: {tune}")
+    for g in (ref, tun):
+        for _ in range(3):
+            g.step(1.0 / 60.0, 5, 9)
+    assert np.array_equal(_pos(ref), _pos(tun))
