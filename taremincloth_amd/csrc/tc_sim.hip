@@ -678,6 +678,11 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
 int tc_sim_autotune(tc_sim *s, float dt_sub, double *out) {
     if (!s) return fail(TC_ERR_VALUE, "null simulator");
     double best_ms = 0.0;
+    // the trial predictions update V in place: keep a copy and put it back afterwards
+    DevBuf<float4> vsave;
+    TC_HIP(vsave.alloc(s->mesh.nv));
+    TC_HIP(hipMemcpyAsync(vsave.p, s->V.p, sizeof(float4) * std::max<size_t>(s->mesh.nv, 1), hipMemcpyDeviceToDevice,
+                          s->stream));
     TC_HIP(hipStreamSynchronize(s->stream));
     int r = time_iterations(s, dt_sub, 16, false, &best_ms);
     if (r != TC_OK) return r;
@@ -708,6 +713,9 @@ int tc_sim_autotune(tc_sim *s, float dt_sub, double *out) {
         build_plan(s);
         s->solver_path = 1;
     }
+    TC_HIP(hipMemcpyAsync(s->V.p, vsave.p, sizeof(float4) * std::max<size_t>(s->mesh.nv, 1), hipMemcpyDeviceToDevice,
+                          s->stream));
+    TC_HIP(hipStreamSynchronize(s->stream));
     s->touch();
     if (out) {
         out[0] = best_tile;
