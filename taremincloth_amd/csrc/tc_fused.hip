@@ -37,24 +37,25 @@ __device__ __forceinline__ void project_item(float4 *__restrict__ sp, const floa
     const bool shear = ((w1 >> 30) & 1u) != 0u, owner = (w1 >> 31) != 0u;
     const float4 q0 = sp[l0], q1 = sp[l1];
     const float wa = q0.w, wb = q1.w, wsum = wa + wb;
-    float comp;
+    // alpha~ = compliance / dt^2 is precomputed per compliance class (same operations, same bits)
+    float alpha;
     if (DIST) {
         // an owner that skips the projection still carries lambda into the output buffer
         if (wsum <= 1e-7f) { if (owner) lam_out[k] = l; return; }
     } else {
-        comp = fa.bcomp[cls];
-        if (comp >= 1e9f || wsum <= 1e-7f) { if (owner) lam_out[k] = l; return; }
+        if (fa.bcomp[cls] >= 1e9f || wsum <= 1e-7f) { if (owner) lam_out[k] = l; return; }
+        alpha = fa.balpha[cls];
     }
     const f3 delta = xyz(q0) - xyz(q1);
     const float d = length(delta);
     if (d <= 1e-7f) { if (owner) lam_out[k] = l; return; }
     const float cval = d - rest;
     if (DIST) {
-        comp = cval >= 0.0f ? fa.dtc[cls] : fa.dcc[cls];
-        if (comp >= 1e9f) { if (owner) lam_out[k] = l; return; }
+        const bool tens = cval >= 0.0f;
+        if ((tens ? fa.dtc[cls] : fa.dcc[cls]) >= 1e9f) { if (owner) lam_out[k] = l; return; }
+        alpha = tens ? fa.dalpha_t[cls] : fa.dalpha_c[cls];
     }
     const f3 dir = delta / d;
-    const float alpha = comp / (dt * dt);
     float dl;
     if (DAMP) {
         const float dc = DIST ? (shear ? a.shear_damping : (cval >= 0.0f ? a.tension_damping : a.compression_damping))
@@ -136,108 +137,38 @@ __global__ void __launch_bounds__(1024) k_tile_iteration(
     for (uint32_t l = tid; l < own; l += bs) Pout[ext_vid[e0 + l]] = sp[l];
 }
 
-// Variant for iterations of at most MAXPH colour phases: every thread issues the
-// loads of its first item of every phase (and then their lambdas) up front,
-// into registers, so the global latency is paid once per launch instead of
-// once per phase.  Items beyond the block size in a phase are loaded in place.
-template <bool FIRST, bool DAMP, int MAXPH>
-__global__ void __launch_bounds__(1024) k_tile_iteration_reg(
-    const float4 *__restrict__ Pin, float4 *__restrict__ Pout, const float4 *__restrict__ X,
-    const uint32_t *__restrict__ ext_off, const uint32_t *__restrict__ ext_vid, const uint32_t *__restrict__ n_own,
-    const uint32_t *__restrict__ ph_off, const uint32_t *__restrict__ items, const float *__restrict__ dlam_in,
-    float *__restrict__ dlam_out, const float *__restrict__ blam_in, float *__restrict__ blam_out, FusedArgs fa,
-    SubstepArgs a) {
-    extern __shared__ float4 sp[];
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    if (tile >= fa.ntiles) return;
-    const uint32_t tid = threadIdx.x, bs = blockDim.x;
-    const uint32_t *po = ph_off + size_t(tile) * (fa.nphases + 1);
-    const uint32_t L = fa.nphases, ndp = fa.nd_phases;
-    uint32_t w0[MAXPH], w1[MAXPH], w2[MAXPH];
-    float lv[MAXPH];
-    uint32_t has = 0u;  // bit c: this thread has a first item in phase c
-#pragma unroll
-    for (int c = 0; c < MAXPH; ++c) {
-        w0[c] = 0u; w1[c] = 0u; w2[c] = 0u;
-        if (uint32_t(c) < L) {
-            const uint32_t i = po[c] + tid;
-            if (i < po[c + 1]) {
-                w0[c] = items[3 * size_t(i)];
-                w1[c] = items[3 * size_t(i) + 1];
-                w2[c] = items[3 * size_t(i) + 2];
-                has |= 1u << c;
-            }
-        }
-    }
-    const uint32_t e0 = ext_off[tile], ne = ext_off[tile + 1] - e0;
-    float4 *sx = sp + fa.max_ext;
-    for (uint32_t l = tid; l < ne; l += bs) {
-        const uint32_t g = ext_vid[e0 + l];
-        sp[l] = Pin[g];
-        if (DAMP) sx[l] = X[g];
-    }
-#pragma unroll
-    for (int c = 0; c < MAXPH; ++c) {
-        lv[c] = 0.0f;
-        if (!FIRST && ((has >> c) & 1u))
-            lv[c] = (uint32_t(c) < ndp ? dlam_in : blam_in)[w1[c] & 0x03FFFFFFu];
-    }
-    __syncthreads();
-    const float dt = fmaxf(a.dt, 1e-8f);
-#pragma unroll
-    for (int c = 0; c < MAXPH; ++c) {
-        if (uint32_t(c) < L) {
-            const bool dist = uint32_t(c) < ndp;
-            float *lam_out = dist ? dlam_out : blam_out;
-            if ((has >> c) & 1u) {
-                if (dist) project_item<true, DAMP>(sp, sx, w0[c], w1[c], w2[c], lv[c], lam_out, fa, a, dt);
-                else project_item<false, DAMP>(sp, sx, w0[c], w1[c], w2[c], lv[c], lam_out, fa, a, dt);
-            }
-            const float *lam_in = dist ? dlam_in : blam_in;
-            for (uint32_t i = po[c] + tid + bs; i < po[c + 1]; i += bs) {
-                const uint32_t x0 = items[3 * size_t(i)], x1 = items[3 * size_t(i) + 1], x2 = items[3 * size_t(i) + 2];
-                const float l = FIRST ? 0.0f : lam_in[x1 & 0x03FFFFFFu];
-                if (dist) project_item<true, DAMP>(sp, sx, x0, x1, x2, l, lam_out, fa, a, dt);
-                else project_item<false, DAMP>(sp, sx, x0, x1, x2, l, lam_out, fa, a, dt);
-            }
-            __syncthreads();
-        }
-    }
-    const uint32_t own = n_own[tile];
-    for (uint32_t l = tid; l < own; l += bs) Pout[ext_vid[e0 + l]] = sp[l];
-}
-
 hipError_t launch_tile_iteration(hipStream_t s, const float4 *Pin, float4 *Pout, const float4 *X,
                                  const FusedPlanDev &p, const float *dlam_in, float *dlam_out, const float *blam_in,
                                  float *blam_out, const FusedArgs &fa, const SubstepArgs &a, bool first,
                                  uint32_t block) {
     if (!fa.ntiles) return hipSuccess;
+    FusedArgs fx = fa;
+    fused_alphas(fx, a.dt);
     const bool damp = a.tension_damping != 0.0f || a.compression_damping != 0.0f || a.shear_damping != 0.0f ||
                       a.bending_damping != 0.0f;
     const uint32_t nblocks = ((fa.ntiles + 7u) / 8u) * 8u;
     const size_t lds = size_t(fa.max_ext) * sizeof(float4) * (damp ? 2 : 1);
 #define TC_TILE(K)                                                                                               \
     hipLaunchKernelGGL(K, dim3(nblocks), dim3(block), lds, s, Pin, Pout, X, p.ext_off, p.ext_vid, p.n_own,        \
-                       p.ph_off, p.items, dlam_in, dlam_out, blam_in, blam_out, fa, a)
-    if (fa.nphases <= 16) {
-        if (first) { if (damp) TC_TILE((k_tile_iteration_reg<true, true, 16>)); else TC_TILE((k_tile_iteration_reg<true, false, 16>)); }
-        else { if (damp) TC_TILE((k_tile_iteration_reg<false, true, 16>)); else TC_TILE((k_tile_iteration_reg<false, false, 16>)); }
-    } else {
-        if (first) { if (damp) TC_TILE((k_tile_iteration<true, true>)); else TC_TILE((k_tile_iteration<true, false>)); }
-        else { if (damp) TC_TILE((k_tile_iteration<false, true>)); else TC_TILE((k_tile_iteration<false, false>)); }
-    }
+                       p.ph_off, p.items, dlam_in, dlam_out, blam_in, blam_out, fx, a)
+    if (first) { if (damp) TC_TILE((k_tile_iteration<true, true>)); else TC_TILE((k_tile_iteration<true, false>)); }
+    else { if (damp) TC_TILE((k_tile_iteration<false, true>)); else TC_TILE((k_tile_iteration<false, false>)); }
 #undef TC_TILE
     return hipGetLastError();
 }
 
+void fused_alphas(FusedArgs &fa, float dt_sub) {
+    const float dt = fmaxf(dt_sub, 1e-8f);
+    const float dt2 = dt * dt;
+    for (int c = 0; c < 16; ++c) {
+        fa.dalpha_t[c] = fa.dtc[c] / dt2;
+        fa.dalpha_c[c] = fa.dcc[c] / dt2;
+        fa.balpha[c] = fa.bcomp[c] / dt2;
+    }
+}
+
 hipError_t set_tile_lds_limit(size_t bytes) {
     hipError_t e;
-    const void *reg[4] = {reinterpret_cast<const void *>(&k_tile_iteration_reg<true, true, 16>),
-                          reinterpret_cast<const void *>(&k_tile_iteration_reg<true, false, 16>),
-                          reinterpret_cast<const void *>(&k_tile_iteration_reg<false, true, 16>),
-                          reinterpret_cast<const void *>(&k_tile_iteration_reg<false, false, 16>)};
-    for (const void *f : reg)
-        if ((e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes))) != hipSuccess) return e;
     if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_iteration<true, true>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes))) != hipSuccess) return e;
     if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_iteration<true, false>),

@@ -40,7 +40,8 @@ hipError_t launch_residual(hipStream_t s, const float4 *P, const uint4 *drec, co
 // --- one iteration per launch over cone tiles (tc_fused.hip, plan from tc_tiles.cpp) ---
 struct FusedArgs {
     uint32_t ntiles, nphases, nd_phases, max_ext;
-    float dtc[16], dcc[16], bcomp[16];  // compliance classes
+    float dtc[16], dcc[16], bcomp[16];           // compliance classes
+    float dalpha_t[16], dalpha_c[16], balpha[16];  // compliance / (dt * dt) per class for this launch
 };
 struct FusedPlanDev {
     const uint32_t *ext_off, *ext_vid, *n_own, *ph_off, *items;
@@ -50,6 +51,8 @@ hipError_t launch_tile_iteration(hipStream_t s, const float4 *Pin, float4 *Pout,
                                  float *blam_out, const FusedArgs &fa, const SubstepArgs &a, bool first,
                                  uint32_t block);
 hipError_t set_tile_lds_limit(size_t bytes);
+// fills the alpha~ tables of `fa` for substep length dt_sub (same float operations as the kernels)
+void fused_alphas(FusedArgs &fa, float dt_sub);
 
 // --- spatial hash + self collision (tc_selfcol.hip) ---
 struct HashBufs {
