@@ -27,10 +27,11 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks) {
 }
 
 // Projection of one item from LDS (the k_distance / k_bending arithmetic).
+// Class table in LDS (per-lane indexed): [0] dtc [16] dcc [32] bcomp [48] dalpha_t [64] dalpha_c [80] balpha
 template <bool DIST, bool DAMP>
 __device__ __forceinline__ void project_item(float4 *__restrict__ sp, const float4 *__restrict__ sx, uint32_t w0,
                                              uint32_t w1, uint32_t w2, float l, float *__restrict__ lam_out,
-                                             const FusedArgs &fa, const SubstepArgs &a, float dt) {
+                                             const float *__restrict__ cls_tab, const SubstepArgs &a, float dt) {
     const float rest = __uint_as_float(w2);
     const uint32_t l0 = w0 & 0xFFFFu, l1 = w0 >> 16;
     const uint32_t k = w1 & 0x03FFFFFFu, cls = (w1 >> 26) & 15u;
@@ -43,8 +44,8 @@ __device__ __forceinline__ void project_item(float4 *__restrict__ sp, const floa
         // an owner that skips the projection still carries lambda into the output buffer
         if (wsum <= 1e-7f) { if (owner) lam_out[k] = l; return; }
     } else {
-        if (fa.bcomp[cls] >= 1e9f || wsum <= 1e-7f) { if (owner) lam_out[k] = l; return; }
-        alpha = fa.balpha[cls];
+        if (cls_tab[32 + cls] >= 1e9f || wsum <= 1e-7f) { if (owner) lam_out[k] = l; return; }
+        alpha = cls_tab[80 + cls];
     }
     const f3 delta = xyz(q0) - xyz(q1);
     const float d = length(delta);
@@ -52,8 +53,8 @@ __device__ __forceinline__ void project_item(float4 *__restrict__ sp, const floa
     const float cval = d - rest;
     if (DIST) {
         const bool tens = cval >= 0.0f;
-        if ((tens ? fa.dtc[cls] : fa.dcc[cls]) >= 1e9f) { if (owner) lam_out[k] = l; return; }
-        alpha = tens ? fa.dalpha_t[cls] : fa.dalpha_c[cls];
+        if (cls_tab[(tens ? 0 : 16) + cls] >= 1e9f) { if (owner) lam_out[k] = l; return; }
+        alpha = cls_tab[(tens ? 48 : 64) + cls];
     }
     const f3 dir = delta / d;
     float dl;
@@ -84,9 +85,18 @@ __global__ void __launch_bounds__(1024) k_tile_iteration(
     float *__restrict__ dlam_out, const float *__restrict__ blam_in, float *__restrict__ blam_out, FusedArgs fa,
     SubstepArgs a) {
     extern __shared__ float4 sp[];
+    __shared__ float cls_tab[96];
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= fa.ntiles) return;
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    if (tid < 16) {
+        cls_tab[tid] = fa.dtc[tid];
+        cls_tab[16 + tid] = fa.dcc[tid];
+        cls_tab[32 + tid] = fa.bcomp[tid];
+        cls_tab[48 + tid] = fa.dalpha_t[tid];
+        cls_tab[64 + tid] = fa.dalpha_c[tid];
+        cls_tab[80 + tid] = fa.balpha[tid];
+    }
     const uint32_t *po = ph_off + size_t(tile) * (fa.nphases + 1);
     const uint32_t L = fa.nphases, ndp = fa.nd_phases;
     // first item of phase 0 (words) and its lambda
@@ -118,14 +128,14 @@ __global__ void __launch_bounds__(1024) k_tile_iteration(
         const float *lam_in = dist ? dlam_in : blam_in;
         float *lam_out = dist ? dlam_out : blam_out;
         if (h_cur) {
-            if (dist) project_item<true, DAMP>(sp, sx, c0, c1, c2, lc, lam_out, fa, a, dt);
-            else project_item<false, DAMP>(sp, sx, c0, c1, c2, lc, lam_out, fa, a, dt);
+            if (dist) project_item<true, DAMP>(sp, sx, c0, c1, c2, lc, lam_out, cls_tab, a, dt);
+            else project_item<false, DAMP>(sp, sx, c0, c1, c2, lc, lam_out, cls_tab, a, dt);
         }
         for (uint32_t i = po[c] + tid + bs; i < po[c + 1]; i += bs) {
             const uint32_t w0 = items[3 * size_t(i)], w1 = items[3 * size_t(i) + 1], w2 = items[3 * size_t(i) + 2];
             const float l = FIRST ? 0.0f : lam_in[w1 & 0x03FFFFFFu];
-            if (dist) project_item<true, DAMP>(sp, sx, w0, w1, w2, l, lam_out, fa, a, dt);
-            else project_item<false, DAMP>(sp, sx, w0, w1, w2, l, lam_out, fa, a, dt);
+            if (dist) project_item<true, DAMP>(sp, sx, w0, w1, w2, l, lam_out, cls_tab, a, dt);
+            else project_item<false, DAMP>(sp, sx, w0, w1, w2, l, lam_out, cls_tab, a, dt);
         }
         // the next phase's lambda, now that its item has arrived
         c0 = n0; c1 = n1; c2 = n2; h_cur = h_nxt;
