@@ -28,6 +28,15 @@ BYTES_DIST = 84                 # algorithmic bytes per distance/shear projectio
 BYTES_BEND = 80                 # per bending projection
 BYTES_VERTEX_PASS = 52          # predict or update_vel, per vertex
 
+# HBM bytes per launch of k_tile_iteration on this workload, from two rocprofv3 --pmc passes
+# (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, averaged over the dispatches of the pass), each
+# scaled by what the same counter reads for an elementwise kernel that reads 1 GiB and writes 1 GiB:
+# FETCH_SIZE reads 0.5 of the bytes read (factor 2.0), WRITE_SIZE reads them exactly (factor 1.0).
+# Keyed by (grid, iterations, tile vertices, block size); the plan (and so the traffic) depends on them.
+MEASURED_TRAFFIC = {
+    (512, 64, 1024, 1024): {"fetch_kib": 23841.0, "write_kib": 10603.4, "fetch_factor": 2.0, "write_factor": 1.0},
+}
+
 
 def grid_512(n: int, spacing: float = 0.01, height: float = 0.5, seed: int = 0):
     """n x n cloth (2 triangles per quad, fixed edge order), centred over the colliders,
@@ -91,7 +100,13 @@ def main():
     ap.add_argument("--iters", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--light", action="store_true",
+                    help="skip the per-colour comparison, the autotuner and the CPU baseline (for counter runs, "
+                         "where every dispatch is serialised); the tile plan comes from TCLOTH_TILE_VERTICES / "
+                         "TCLOTH_TILE_BLOCK or the defaults")
     args = ap.parse_args()
+    if args.light:
+        args.no_cpu_baseline = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the XPBD substep of one cloth "
               "does not shard across GPUs in this build. Run with --gpus 1.", file=sys.stderr)
@@ -109,16 +124,18 @@ def main():
     dt_sub = dt_frame / per_frame
     nd, nb = sim.num_distance_constraints(), sim.num_bending_constraints()
 
-    # the per-colour schedule, timed briefly for comparison (identical results)
-    sim.set_solver_path(1)
-    run_substeps(sim, max(args.warmup // 2, 10), dt_frame, per_frame)
-    sim.synchronize()
-    t0 = time.perf_counter()
-    run_substeps(sim, max(args.steps // 4, 20), dt_frame, per_frame)
-    sim.synchronize()
-    per_colour_ms = (time.perf_counter() - t0) / max(args.steps // 4, 20) * 1e3
-    sim.set_solver_path(0)
-    tune = sim.autotune(dt_sub)
+    per_colour_ms, tune = None, None
+    if not args.light:
+        # the per-colour schedule, timed briefly for comparison (identical results)
+        sim.set_solver_path(1)
+        run_substeps(sim, max(args.warmup // 2, 10), dt_frame, per_frame)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        run_substeps(sim, max(args.steps // 4, 20), dt_frame, per_frame)
+        sim.synchronize()
+        per_colour_ms = (time.perf_counter() - t0) / max(args.steps // 4, 20) * 1e3
+        sim.set_solver_path(0)
+        tune = sim.autotune(dt_sub)
     info = sim.solver_info()
 
     run_substeps(sim, args.warmup, dt_frame, per_frame)
@@ -149,6 +166,11 @@ def main():
         kname = "k_distance (one colour of distance/shear constraints)"
         sweep_ms = kt["distance"] + kt["bending"]
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = None
+    if info["path"] == 2:
+        m = MEASURED_TRAFFIC.get((n, args.iters, int(info["tile_vertices"]), int(info["block"])))
+        if m is not None:
+            traffic = (m["fetch_kib"] * m["fetch_factor"] + m["write_kib"] * m["write_factor"]) * 1024.0
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -183,6 +205,7 @@ def main():
                                 "max_cone_vertices": int(info["max_cone"]),
                                 "items_per_iteration": int(info["items_per_iteration"]),
                                 "constraints_per_iteration": int(info["constraints_per_iteration"]),
+                                "tile_vertices": int(info["tile_vertices"]), "block": int(info["block"]),
                                 "plan_note": info["plan_note"], "autotune": tune,
                                 "per_colour_path_ms_per_substep": per_colour_ms},
             "sweep_GBps_algorithmic": sweep_bytes / (sweep_ms * 1e-3) / 1e9,
@@ -191,7 +214,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": kname, "bytes_per_launch": bytes_per_launch, "launch_ms": launch_ms,
         },
         "cpu_baseline": cpu,

@@ -113,8 +113,9 @@ int tc_sim_set_hash_cell_size(tc_sim *sim, float cell_size);
  * 0 auto, 1 one launch per colour (the reference's dispatch pattern, dispatch.rs:1449-1455),
  * 2 one launch per solver iteration over dependency-cone tiles held in LDS. */
 int tc_sim_set_solver_path(tc_sim *sim, int path);
-/* out[8]: [active path (1|2), tiles, largest cone (vertices), cone vertices total,
- *          items per iteration, constraints per iteration, plan available (0|1), colour phases] */
+/* out[10]: [active path (1|2), tiles, largest cone (vertices), cone vertices total,
+ *           items per iteration, constraints per iteration, plan available (0|1), colour phases,
+ *           tile size (vertices), block size of the per-iteration kernel] */
 int tc_sim_solver_info(tc_sim *sim, double *out);
 
 /* Autotuner (no reference counterpart; the reference exposes a fixed workgroup_size knob,
@@ -149,7 +150,9 @@ int tc_sim_residual(tc_sim *sim, float dt_sub, double *rel, double *max_rel);
 /* Run the parts of one substep up to the velocity update (residual / warm-start studies). */
 int tc_sim_substep_parts(tc_sim *sim, float dt_sub, int collide, int self_collide, int update_velocity);
 /* Times `reps` substeps launched kernel by kernel (no graph) with HIP events recorded on the
- * simulator's stream, using the active solver schedule, and writes per-substep averages (milliseconds)
+ * simulator's stream, using the active solver schedule.  Launches of one kind that follow each other
+ * (the iterations of the per-iteration kernel; one iteration's distance colours; its bending colours)
+ * sit under one event pair, so no event packet separates them.  Writes per-substep averages (milliseconds)
  * to out[12]: [0] predict  [1] all distance launches  [2] distance launches per substep
  * [3] all bending launches  [4] bending launches per substep  [5] collider resolve
  * [6] self-collision (hash + solve + apply)  [7] update_vel  [8] wall time of the substep between its

@@ -610,26 +610,38 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
         if (e != hipSuccess) return e;
         uint32_t pc = 0;
         if (fused) {
+            // the iterations run back to back under one event pair: no event packets between them
             const tc::FusedArgs fa = s->fused_args();
             const tc::FusedPlanDev pd = s->plan_dev();
-            for (uint32_t it = 0; it < s->iterations; ++it) {
-                if ((e = mark(9, [&] {
-                         return tc::launch_tile_iteration(s->stream, s->Pbuf(pc), s->Pbuf(pc ^ 1u), s->X.p, pd,
-                                                          s->dlam_buf(pc), s->dlam_buf(pc ^ 1u), s->blam_buf(pc),
-                                                          s->blam_buf(pc ^ 1u), fa, a, it == 0, s->fused_block);
-                     })) != hipSuccess) return e;
-                pc ^= 1u;
-            }
+            if ((e = mark(9, [&] {
+                     hipError_t r = hipSuccess;
+                     for (uint32_t it = 0; it < s->iterations && r == hipSuccess; ++it) {
+                         r = tc::launch_tile_iteration(s->stream, s->Pbuf(pc), s->Pbuf(pc ^ 1u), s->X.p, pd,
+                                                       s->dlam_buf(pc), s->dlam_buf(pc ^ 1u), s->blam_buf(pc),
+                                                       s->blam_buf(pc ^ 1u), fa, a, it == 0, s->fused_block);
+                         pc ^= 1u;
+                     }
+                     return r;
+                 })) != hipSuccess) return e;
         }
         float4 *Pc = s->Pbuf(pc);
         for (uint32_t it = 0; !fused && it < s->iterations; ++it) {
-            for (size_t c = 0; c < ndc; ++c)
-                if ((e = mark(1, [&] { return tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p,
-                                                                  m.d_col.offset[c], m.d_col.count[c], a, it == 0,
-                                                                  s->split_compliance, B); })) != hipSuccess) return e;
-            for (size_t c = 0; c < nbc; ++c)
-                if ((e = mark(3, [&] { return tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p,
-                                                                 m.b_col.offset[c], m.b_col.count[c], a, it == 0, B); })) != hipSuccess) return e;
+            // one event pair around each iteration's distance colours and one around its bending colours
+            if (ndc && (e = mark(1, [&] {
+                            hipError_t r = hipSuccess;
+                            for (size_t c = 0; c < ndc && r == hipSuccess; ++c)
+                                r = tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p,
+                                                        m.d_col.offset[c], m.d_col.count[c], a, it == 0,
+                                                        s->split_compliance, B);
+                            return r;
+                        })) != hipSuccess) return e;
+            if (nbc && (e = mark(3, [&] {
+                            hipError_t r = hipSuccess;
+                            for (size_t c = 0; c < nbc && r == hipSuccess; ++c)
+                                r = tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p,
+                                                       m.b_col.offset[c], m.b_col.count[c], a, it == 0, B);
+                            return r;
+                        })) != hipSuccess) return e;
         }
         if (!s->cols_h.empty() || !s->tris_h.empty())
             if ((e = mark(5, [&] { return tc::launch_collision(s->stream, Pc, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()),
@@ -982,6 +994,8 @@ int tc_sim_solver_info(tc_sim *s, double *out) {
     out[5] = double(s->plan.plain_items);
     out[6] = s->fused_ready ? 1.0 : 0.0;
     out[7] = s->plan.nphases;
+    out[8] = s->tile_vertices;
+    out[9] = s->fused_block;
     g_err = s->fused_ready ? std::string() : s->fused_reason;
     return TC_OK;
 }

@@ -227,10 +227,10 @@ class ClothSimulator:
                 "per_colour_ms_per_iteration": out[3]}
 
     def solver_info(self) -> dict:
-        out = (C.c_double * 8)()
+        out = (C.c_double * 10)()
         _lib.check(self._L.tc_sim_solver_info(self._h, out))
         keys = ["path", "tiles", "max_cone", "cone_vertices", "items_per_iteration", "constraints_per_iteration",
-                "plan_available", "phases"]
+                "plan_available", "phases", "tile_vertices", "block"]
         info = dict(zip(keys, list(out)))
         info["plan_note"] = (self._L.tc_last_error() or b"").decode(errors="replace")
         return info
