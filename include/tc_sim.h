@@ -162,6 +162,68 @@ int tc_sim_time_kernels(tc_sim *sim, float dt_sub, uint32_t reps, double *out);
 /* Block until all work on the simulator's stream has finished. */
 int tc_sim_synchronize(tc_sim *sim);
 
+/* ---- Learned warm start (no reference counterpart; it runs between predict and the
+ * projection loop of the substep, crates/cloth_core/src/simulation/dispatch.rs:137 -> 169).
+ * A message-passing network over the cloth's stretch-edge graph (hidden width 128, `rounds`
+ * rounds, two-layer MLPs, BF16 MFMA with FP32 accumulation) predicts each free vertex's position
+ * correction for the coming substep from local features of the predicted state and the previous
+ * substep's correction; plain XPBD iterations always follow, and a residual gate runs further
+ * plain iterations when the relative residual after them exceeds a threshold. */
+typedef struct tc_ws_net tc_ws_net;
+
+/* New network with zero parameters; rounds 1..4.  The network must outlive every simulator
+ * it is attached to. */
+int tc_ws_net_create(uint32_t rounds, tc_ws_net **out);
+void tc_ws_net_destroy(tc_ws_net *net);
+uint32_t tc_ws_net_num_params(const tc_ws_net *net);
+uint32_t tc_ws_net_rounds(const tc_ws_net *net);
+/* Flat fp32 parameter vector: per layer W (K x N, row-major) then b (N), layers in the order
+ * encoder 1 (32 x 128), encoder 2 (128 x 128), per round (256 x 128, 128 x 128),
+ * decoder 1 (128 x 128), decoder 2 (128 x 16).  Setting parameters resets the Adam state. */
+int tc_ws_net_set_params(tc_ws_net *net, const float *params, size_t count);
+int tc_ws_net_get_params(tc_ws_net *net, float *params, size_t count);
+/* Seeded He-uniform initialisation; the last layer starts at zero. */
+int tc_ws_net_init(tc_ws_net *net, uint64_t seed);
+/* One forward pass (and, with targets, the loss and its gradient) on host arrays:
+ * features rows x 32 (rounded to BF16), a symmetric neighbour graph in CSR form,
+ * targets rows x 4 (target correction xyz in units of |g| dt^2, weight 0|1) or NULL.
+ * Outputs: y rows x 16, grad (num_params), loss; each may be NULL.  The gradient stays on the
+ * device for tc_ws_net_adam_step. */
+int tc_ws_net_forward_backward(tc_ws_net *net, const float *features, uint32_t rows, const uint32_t *graph_off,
+                               const uint32_t *graph_nbr, const float *targets, float *y, float *grad,
+                               double *loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) with the last computed gradient. */
+int tc_ws_net_adam_step(tc_ws_net *net, float lr);
+/* Trains on the samples the simulators recorded (tc_sim_ws_record): `steps` Adam steps, each on
+ * one recorded substep drawn with a seeded generator; losses[steps] may be NULL. */
+int tc_ws_train(tc_ws_net *net, tc_sim *const *sims, uint32_t nsims, uint32_t steps, float lr, uint64_t seed,
+                float *losses);
+
+/* mode 0: off (plain XPBD with the simulator's iteration count).
+ * mode 1: start the projection from prediction + previous substep's correction.
+ * mode 2: start it from prediction + previous correction + network output.
+ * Modes 1 and 2 run `iterations` plain iterations, then a residual check; when the relative
+ * residual exceeds `residual_threshold`, `fallback_iterations` (rounded up to even) more run. */
+int tc_sim_set_warm_start(tc_sim *sim, tc_ws_net *net, int mode, uint32_t iterations,
+                          uint32_t fallback_iterations, float residual_threshold);
+/* out[6]: [substeps checked, fallbacks taken, relative residual at the last check,
+ *          max |h|/L0 at the last check, mode, warm-start iterations] */
+int tc_sim_warm_start_stats(tc_sim *sim, double *out);
+int tc_sim_ws_reset_stats(tc_sim *sim);
+/* Records training samples from the coming substeps: every `stride`-th substep stores its
+ * features, target correction, prediction and previous correction, up to `capacity` samples.
+ * capacity 0 stops recording and frees the samples.  Recording launches kernel by kernel. */
+int tc_sim_ws_record(tc_sim *sim, uint32_t capacity, uint32_t stride);
+uint32_t tc_sim_ws_num_records(const tc_sim *sim);
+/* Copies sample `idx`: features N x 32, target N x 4, prediction N x 4 (xyz, inverse mass),
+ * previous correction N x 4; each may be NULL. */
+int tc_sim_ws_get_record(tc_sim *sim, uint32_t idx, float *features, float *target, float *prediction,
+                         float *previous);
+/* Network output of the last warm-started substep (N x 16). */
+int tc_sim_ws_last_output(tc_sim *sim, float *y, size_t count);
+/* Reads and/or replaces the previous-substep correction (N x 3 each; either may be NULL). */
+int tc_sim_ws_history(tc_sim *sim, float *get_xyz, const float *set_xyz, uint32_t n);
+
 /* Host-only mesh build (no device work; usable without a GPU): builds the constraints, colouring
  * and self-collision topology exactly as tc_sim_create does and returns array `which`:
  * 0-9, 17, 18 as in tc_sim_get_array, and

@@ -23,6 +23,10 @@ EXPORTS = [
     "tc_sim_num_distance_constraints", "tc_sim_num_bending_constraints", "tc_sim_get_array",
     "tc_sim_build_hashes", "tc_sim_residual", "tc_sim_substep_parts", "tc_sim_time_kernels", "tc_sim_synchronize",
     "tc_mesh_build_arrays", "tc_color_edge_pairs", "tc_compute_inv_masses", "tc_last_error", "tc_version",
+    "tc_ws_net_create", "tc_ws_net_destroy", "tc_ws_net_num_params", "tc_ws_net_rounds", "tc_ws_net_set_params",
+    "tc_ws_net_get_params", "tc_ws_net_init", "tc_ws_net_forward_backward", "tc_ws_net_adam_step", "tc_ws_train",
+    "tc_sim_set_warm_start", "tc_sim_warm_start_stats", "tc_sim_ws_reset_stats", "tc_sim_ws_record",
+    "tc_sim_ws_num_records", "tc_sim_ws_get_record", "tc_sim_ws_last_output", "tc_sim_ws_history",
 ]
 
 TC_OK, TC_ERR_VALUE, TC_ERR_RUNTIME, TC_ERR_UNSUPPORTED = 0, 1, 2, 3
@@ -94,6 +98,24 @@ def load() -> C.CDLL:
                                       C.c_size_t, C.POINTER(C.c_size_t)]),
         "tc_color_edge_pairs": (_i, [_u32, _vp, _u32, _vp]),
         "tc_compute_inv_masses": (_i, [_vp, _u32, _vp, _u32, _vp, _f32, _vp]),
+        "tc_ws_net_create": (_i, [_u32, C.POINTER(_vp)]),
+        "tc_ws_net_destroy": (None, [_vp]),
+        "tc_ws_net_num_params": (_u32, [_vp]),
+        "tc_ws_net_rounds": (_u32, [_vp]),
+        "tc_ws_net_set_params": (_i, [_vp, _vp, C.c_size_t]),
+        "tc_ws_net_get_params": (_i, [_vp, _vp, C.c_size_t]),
+        "tc_ws_net_init": (_i, [_vp, C.c_uint64]),
+        "tc_ws_net_forward_backward": (_i, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double)]),
+        "tc_ws_net_adam_step": (_i, [_vp, _f32]),
+        "tc_ws_train": (_i, [_vp, _vp, _u32, _u32, _f32, C.c_uint64, _vp]),
+        "tc_sim_set_warm_start": (_i, [_vp, _vp, _i, _u32, _u32, _f32]),
+        "tc_sim_warm_start_stats": (_i, [_vp, C.POINTER(C.c_double)]),
+        "tc_sim_ws_reset_stats": (_i, [_vp]),
+        "tc_sim_ws_record": (_i, [_vp, _u32, _u32]),
+        "tc_sim_ws_num_records": (_u32, [_vp]),
+        "tc_sim_ws_get_record": (_i, [_vp, _u32, _vp, _vp, _vp, _vp]),
+        "tc_sim_ws_last_output": (_i, [_vp, _vp, C.c_size_t]),
+        "tc_sim_ws_history": (_i, [_vp, _vp, _vp, _u32]),
         "tc_last_error": (C.c_char_p, []),
         "tc_version": (C.c_char_p, []),
     }

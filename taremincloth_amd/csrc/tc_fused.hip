@@ -86,6 +86,7 @@ __global__ void __launch_bounds__(1024) k_tile_iteration(
     SubstepArgs a) {
     extern __shared__ float4 sp[];
     __shared__ float cls_tab[96];
+    if (a.run_flag && *a.run_flag == 0u) return;
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= fa.ntiles) return;
     const uint32_t tid = threadIdx.x, bs = blockDim.x;

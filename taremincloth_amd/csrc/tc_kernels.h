@@ -13,6 +13,9 @@ struct SubstepArgs {
     float gx, gy, gz;  // gravity
     float damping;     // air damping
     float tension_damping, compression_damping, shear_damping, bending_damping;
+    // Projection kernels return at once when run_flag is set and *run_flag == 0
+    // (the fallback iterations of the warm-started solve); nullptr: always run.
+    const uint32_t *run_flag = nullptr;
 };
 
 // --- core substep (tc_xpbd.hip) ---
@@ -32,10 +35,13 @@ hipError_t launch_update_vel(hipStream_t s, const float4 *P, float4 *X, float4 *
                              uint32_t block);
 hipError_t launch_extract_positions(hipStream_t s, const float4 *X, float *out, uint32_t n);
 hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t value, size_t n);
-// Residual partial sums: out[0] += sum h^2, out[1] += sum L0^2, out[2] = max |h|/L0 (as double).
+// Residual of the XPBD system after the last sweep (double sums, deterministic order):
+// out[0] = sum h^2, out[1] = sum L0^2, out[2] = max |h|/L0, out[3] = sqrt(out[0] / out[1]).
+// part holds 3 * 1024 doubles.  With gate != nullptr the kernel also writes gate[0] = (out[3] >
+// threshold) and counts checks in gate[1] and fallbacks in gate[2].
 hipError_t launch_residual(hipStream_t s, const float4 *P, const uint4 *drec, const float *dcc, const float *dlam,
                            uint32_t nd, const uint4 *brec, const float *blam, uint32_t nb, float dt, bool split,
-                           double *out);
+                           double *part, double *out, uint32_t *gate, float threshold);
 
 // --- one iteration per launch over cone tiles (tc_fused.hip, plan from tc_tiles.cpp) ---
 struct FusedArgs {

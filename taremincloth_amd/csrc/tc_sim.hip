@@ -26,166 +26,12 @@
 #include "tc_mesh.h"
 #include "tc_tiles.h"
 
-namespace {
+#include "tc_sim_impl.h"
 
+namespace tc {
 thread_local std::string g_err;
-
-int fail(int code, const std::string &msg) {
-    g_err = msg;
-    return code;
 }
-
-#define TC_HIP(call)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (call);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            return fail(TC_ERR_RUNTIME, std::string(#call) + ": " + hipGetErrorString(e_));       \
-    } while (0)
-
-template <typename T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    hipError_t alloc(size_t count) {
-        release();
-        n = count;
-        return hipMalloc(reinterpret_cast<void **>(&p), sizeof(T) * std::max<size_t>(count, 1));
-    }
-    hipError_t upload(const T *src, size_t count, hipStream_t s) {
-        hipError_t e = alloc(count);
-        if (e != hipSuccess || !count) return e;
-        return hipMemcpyAsync(p, src, sizeof(T) * count, hipMemcpyHostToDevice, s);
-    }
-};
-
-}  // namespace
-
-struct tc_sim {
-    tc::Mesh mesh;
-    hipStream_t stream = nullptr;
-    uint32_t block = 256;
-
-    // vertex streams (P has a second buffer for the one-launch-per-iteration path)
-    DevBuf<float4> X, P, V, Palt;
-    std::vector<float4> X0h, P0h;  // initial state for reset()
-
-    // constraints
-    DevBuf<uint4> drec, brec;
-    DevBuf<float> dcc, dlam, blam, dlam2, blam2;
-    bool split_compliance = false;
-    std::vector<uint4> drec_h, brec_h;
-    std::vector<float> dcc_h;
-
-    // colliders
-    std::vector<tc::Collider> cols_h;
-    DevBuf<tc::Collider> cols;
-    std::vector<tc::MeshTri> tris_h;
-    DevBuf<tc::MeshTri> tris;
-    DevBuf<float4> tri_bounds;
-    int single_sided_recovery = 1;
-
-    // self collision
-    DevBuf<float> local_len, th_rest;
-    DevBuf<uint32_t> th_off, th_idx, star_off, edge_bbits;
-    DevBuf<uint2> star, edges;
-    DevBuf<uint32_t> vkeys, vcounts, vstarts, vsorted;
-    DevBuf<uint32_t> ekeys, ecounts, estarts, esorted;
-    DevBuf<tc::Accum> accum;
-    DevBuf<float> readback;
-    DevBuf<double> resid;
-
-    // parameters (simulation/mod.rs:607-691 defaults)
-    float gravity[3] = {0.0f, 0.0f, -9.81f};
-    float damping = 1.0f;
-    float tension_damping = 0.0f, compression_damping = 0.0f, shear_damping = 0.0f, bending_damping = 0.0f;
-    uint32_t iterations = 2;
-    bool self_collision = true;
-    float relief_factor = 0.2f, max_disp_ratio = 0.2f;
-    int enable_relief = 1, exclude_neighbors = 1;
-    bool normal_untangling = true;  // only effective for multi-layer meshes (pipelines.rs:895-896)
-    uint32_t max_search_iterations = 256;
-    float cell_size = 0.01f;
-    uint32_t table_size = 1024;
-
-    // one-launch-per-iteration plan (tc_tiles.cpp)
-    int solver_path = 0;          // 0 auto, 1 one launch per colour, 2 one launch per iteration
-    bool fused_ready = false;     // plan built and uploaded
-    std::string fused_reason;     // why the plan is unavailable
-    tc::TilePlan plan;
-    DevBuf<uint32_t> pl_ext_off, pl_ext_vid, pl_n_own, pl_ph_off, pl_items;
-    uint32_t fused_block = 1024;
-    uint32_t tile_vertices = 1024;  // TCLOTH_TILE_VERTICES overrides; tc_sim_autotune picks per mesh
-    uint32_t last_p = 0, last_lam = 0;  // buffers holding P and lambda after the last substep
-
-    // frame graph cache
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
-    uint64_t graph_key[4] = {0, 0, 0, 0};
-    uint64_t epoch = 1;  // bumped whenever a parameter or buffer baked into the graph changes
-    bool use_graph = true;  // TCLOTH_USE_GRAPH=0 launches kernel by kernel (profilers)
-
-    ~tc_sim() {
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-    void touch() { ++epoch; }
-
-    tc::SubstepArgs args(float dt_sub) const {
-        tc::SubstepArgs a;
-        a.dt = dt_sub;
-        a.gx = gravity[0]; a.gy = gravity[1]; a.gz = gravity[2];
-        a.damping = damping;
-        a.tension_damping = tension_damping; a.compression_damping = compression_damping;
-        a.shear_damping = shear_damping; a.bending_damping = bending_damping;
-        return a;
-    }
-    tc::SelfColArgs sc_args() const {
-        tc::SelfColArgs a;
-        a.cell = cell_size; a.table = table_size;
-        a.nv = mesh.nv; a.ne = uint32_t(mesh.e_v0.size());
-        a.relief_factor = relief_factor; a.max_disp_ratio = max_disp_ratio;
-        a.enable_relief = enable_relief; a.exclude_neighbors = exclude_neighbors;
-        a.max_search_iterations = max_search_iterations;
-        return a;
-    }
-    tc::SelfColTopo topo() const {
-        tc::SelfColTopo t;
-        t.local_len = local_len.p; t.th_off = th_off.p; t.th_idx = th_idx.p; t.th_rest = th_rest.p;
-        t.star_off = star_off.p; t.star = star.p; t.edges = edges.p; t.edge_bbits = edge_bbits.p;
-        return t;
-    }
-    bool fused_active() const {
-        if (!fused_ready) return false;
-        if (solver_path == 1) return false;
-        if (solver_path == 2) return true;
-        return mesh.nv >= 16384;  // small meshes: the per-colour launches are cheap
-    }
-    float4 *Pbuf(uint32_t k) const { return k ? Palt.p : P.p; }
-    float *dlam_buf(uint32_t k) const { return k ? dlam2.p : dlam.p; }
-    float *blam_buf(uint32_t k) const { return k ? blam2.p : blam.p; }
-    tc::FusedArgs fused_args() const {
-        tc::FusedArgs f{};
-        f.ntiles = plan.ntiles; f.nphases = plan.nphases; f.nd_phases = plan.nd_phases; f.max_ext = plan.max_ext;
-        for (size_t c = 0; c < plan.dcls_tc.size(); ++c) { f.dtc[c] = plan.dcls_tc[c]; f.dcc[c] = plan.dcls_cc[c]; }
-        for (size_t c = 0; c < plan.bcls.size(); ++c) f.bcomp[c] = plan.bcls[c];
-        return f;
-    }
-    tc::FusedPlanDev plan_dev() const {
-        return {pl_ext_off.p, pl_ext_vid.p, pl_n_own.p, pl_ph_off.p, pl_items.p};
-    }
-    tc::HashBufs vhash() const { return {vkeys.p, vcounts.p, vstarts.p, vsorted.p}; }
-    tc::HashBufs ehash() const { return {ekeys.p, ecounts.p, estarts.p, esorted.p}; }
-};
+using tc::g_err;
 
 namespace {
 
@@ -199,45 +45,74 @@ hipError_t enqueue_hashes(tc_sim *s, const float4 *P) {
 // One substep up to (optionally) the velocity update.  Predict writes P buffer 0;
 // the per-iteration path ping-pongs P and lambda between two buffers, and the
 // rest of the substep works on the buffer holding the latest positions.
+// `count` projection iterations.  The per-iteration path reads P / lambda from buffer
+// pc / lc and leaves them in the other one after each launch; the first iteration
+// (first == true) treats lambda as zero.
+hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t count, bool first, uint32_t &pc,
+                              uint32_t &lc) {
+    const tc::Mesh &m = s->mesh;
+    const uint32_t B = s->block;
+    hipError_t e = hipSuccess;
+    if (s->fused_active()) {
+        const tc::FusedArgs fa = s->fused_args();
+        const tc::FusedPlanDev pd = s->plan_dev();
+        for (uint32_t it = 0; it < count; ++it) {
+            e = tc::launch_tile_iteration(s->stream, s->Pbuf(pc), s->Pbuf(pc ^ 1u), s->X.p, pd, s->dlam_buf(lc),
+                                          s->dlam_buf(lc ^ 1u), s->blam_buf(lc), s->blam_buf(lc ^ 1u), fa, a,
+                                          first && it == 0, s->fused_block);
+            if (e != hipSuccess) return e;
+            pc ^= 1u;
+            lc ^= 1u;
+        }
+        return e;
+    }
+    for (uint32_t it = 0; it < count; ++it) {
+        const bool f = first && it == 0;
+        for (size_t c = 0; c < m.d_col.count.size(); ++c) {
+            e = tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p, m.d_col.offset[c],
+                                    m.d_col.count[c], a, f, s->split_compliance, B);
+            if (e != hipSuccess) return e;
+        }
+        for (size_t c = 0; c < m.b_col.count.size(); ++c) {
+            e = tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p, m.b_col.offset[c],
+                                   m.b_col.count[c], a, f, B);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return e;
+}
+
 hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_collide, bool update) {
     const tc::SubstepArgs a = s->args(dt_sub);
     const tc::Mesh &m = s->mesh;
     const uint32_t n = m.nv, B = s->block;
     hipError_t e = tc::launch_predict(s->stream, s->P.p, s->X.p, s->V.p, n, a, B);
     if (e != hipSuccess) return e;
+    const bool track = s->ws_tracking();
+    if (track && (e = ws_pre(s, dt_sub)) != hipSuccess) return e;
+    const uint32_t iters = s->iters_effective();
     uint32_t pc = 0, lc = 0;
-    if (s->iterations == 0) {  // lambdas are still cleared at the substep start (dispatch.rs:117-119)
+    if (iters == 0) {  // lambdas are still cleared at the substep start (dispatch.rs:117-119)
         if ((e = tc::launch_fill_u32(s->stream, reinterpret_cast<uint32_t *>(s->dlam.p), 0u, m.d_v0.size())) != hipSuccess) return e;
         if ((e = tc::launch_fill_u32(s->stream, reinterpret_cast<uint32_t *>(s->blam.p), 0u, m.b_v0.size())) != hipSuccess) return e;
-    } else if (s->fused_active()) {
-        const tc::FusedArgs fa = s->fused_args();
-        const tc::FusedPlanDev pd = s->plan_dev();
-        for (uint32_t it = 0; it < s->iterations; ++it) {
-            e = tc::launch_tile_iteration(s->stream, s->Pbuf(pc), s->Pbuf(pc ^ 1u), s->X.p, pd, s->dlam_buf(lc),
-                                          s->dlam_buf(lc ^ 1u), s->blam_buf(lc), s->blam_buf(lc ^ 1u), fa, a, it == 0,
-                                          s->fused_block);
-            if (e != hipSuccess) return e;
-            pc ^= 1u;
-            lc ^= 1u;
-        }
     } else {
-        for (uint32_t it = 0; it < s->iterations; ++it) {
-            const bool first = it == 0;
-            for (size_t c = 0; c < m.d_col.count.size(); ++c) {
-                e = tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p, m.d_col.offset[c],
-                                        m.d_col.count[c], a, first, s->split_compliance, B);
-                if (e != hipSuccess) return e;
-            }
-            for (size_t c = 0; c < m.b_col.count.size(); ++c) {
-                e = tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p, m.b_col.offset[c],
-                                       m.b_col.count[c], a, first, B);
-                if (e != hipSuccess) return e;
-            }
+        if ((e = enqueue_iterations(s, a, iters, true, pc, lc)) != hipSuccess) return e;
+        if (s->ws_active() && s->ws_fallback > 0) {
+            // residual gate: the fallback iterations run only when the residual exceeds the threshold
+            e = tc::launch_residual(s->stream, s->Pbuf(pc), s->drec.p, s->dcc.p, s->dlam_buf(lc),
+                                    uint32_t(m.d_v0.size()), s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()),
+                                    dt_sub, s->split_compliance, s->resid_part.p, s->ws_res.p, s->ws_gate.p,
+                                    s->ws_threshold);
+            if (e != hipSuccess) return e;
+            tc::SubstepArgs fa = a;
+            fa.run_flag = s->ws_gate.p;
+            if ((e = enqueue_iterations(s, fa, s->ws_fallback, false, pc, lc)) != hipSuccess) return e;
         }
     }
     float4 *Pc = s->Pbuf(pc);
     s->last_p = pc;
     s->last_lam = lc;
+    if (track && (e = ws_post(s, dt_sub, Pc)) != hipSuccess) return e;
     if (collide && (!s->cols_h.empty() || !s->tris_h.empty())) {
         e = tc::launch_collision(s->stream, Pc, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
                                  s->tri_bounds.p, uint32_t(s->tris_h.size()), s->single_sided_recovery, dt_sub, B);
@@ -255,9 +130,17 @@ hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_coll
     return hipSuccess;
 }
 
+// Picks the training-sample slot the next substep writes (recording only).
+void next_record_slot(tc_sim *s) {
+    s->rec_slot = -1;
+    if (s->rec_cap == 0 || s->rec_count >= s->rec_cap) return;
+    if (s->rec_tick++ % s->rec_stride == 0) s->rec_slot = int(s->rec_count++);
+}
+
 hipError_t enqueue_frame(tc_sim *s, float dt, uint32_t substeps) {
     const float dt_sub = dt / float(substeps);
     for (uint32_t k = 0; k < substeps; ++k) {
+        next_record_slot(s);
         hipError_t e = enqueue_substep(s, dt_sub, true, true, true);
         if (e != hipSuccess) return e;
     }
@@ -269,20 +152,21 @@ int run_frame(tc_sim *s, float dt, uint32_t substeps) {
         return fail(TC_ERR_UNSUPPORTED,
                     "multi-layer cloth with normal untangling is not part of this build; disable it with "
                     "set_self_collision_options(..., enable_normal_untangling=False) or disable self-collision");
-    if (s->fused_active() && s->iterations > 0) {  // host bookkeeping of the buffer parity the graph bakes in
-        s->last_p = s->iterations & 1u;
-        s->last_lam = s->iterations & 1u;
+    const uint32_t iters = s->iters_effective();
+    if (s->fused_active() && iters > 0) {  // host bookkeeping of the buffer parity the graph bakes in
+        s->last_p = iters & 1u;
+        s->last_lam = iters & 1u;
     } else {
         s->last_p = 0;
         s->last_lam = 0;
     }
-    if (!s->use_graph) {
+    if (!s->use_graph || s->rec_cap > 0) {  // recording picks its sample slot per substep on the host
         TC_HIP(enqueue_frame(s, dt, substeps));
         return TC_OK;
     }
     uint32_t dtb;
     std::memcpy(&dtb, &dt, 4);
-    const uint64_t key[4] = {dtb, substeps, uint64_t(s->iterations) | (uint64_t(s->fused_active()) << 32), s->epoch};
+    const uint64_t key[4] = {dtb, substeps, uint64_t(iters) | (uint64_t(s->fused_active()) << 32), s->epoch};
     if (!s->graph_exec || std::memcmp(key, s->graph_key, sizeof(key)) != 0) {
         if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
         if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
@@ -485,7 +369,8 @@ int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, ui
     AL(s->ekeys, NE); AL(s->esorted, NE); AL(s->ecounts, T); AL(s->estarts, T + 1);
     AL(s->accum, nv);
     AL(s->readback, size_t(nv) * 3);
-    AL(s->resid, 3);
+    AL(s->resid, 4);
+    AL(s->resid_part, 3 * 1024);
 #undef UP
 #undef AL
     if ((he = hipMemsetAsync(s->accum.p, 0, sizeof(tc::Accum) * std::max<size_t>(nv, 1), s->stream)) != hipSuccess ||
@@ -529,6 +414,7 @@ int tc_sim_substep_parts(tc_sim *s, float dt_sub, int collide, int self_collide,
     if (s->mesh.nv == 0) return TC_OK;
     if (self_collide && s->self_collision && s->normal_untangling && s->mesh.has_multiple_layers)
         return fail(TC_ERR_UNSUPPORTED, "multi-layer cloth with normal untangling is not part of this build");
+    s->rec_slot = -1;
     TC_HIP(enqueue_substep(s, dt_sub, collide != 0, self_collide != 0, update != 0));
     return TC_OK;
 }
@@ -782,7 +668,7 @@ int tc_sim_reset(tc_sim *s) {
     TC_HIP(hipMemcpy(s->P.p, s->P0h.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     TC_HIP(hipMemcpy(s->V.p, V.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     s->last_p = 0;
-    return TC_OK;
+    return ws_clear_history(s);
 }
 
 int tc_sim_add_sphere(tc_sim *s, const float c[3], float r, float fr, float rest) {
@@ -1088,11 +974,10 @@ int tc_sim_get_array(tc_sim *s, int which, void *out, size_t cap, size_t *count)
 int tc_sim_residual(tc_sim *s, float dt_sub, double *rel, double *max_rel) {
     if (!s) return fail(TC_ERR_VALUE, "null simulator");
     const tc::Mesh &m = s->mesh;
-    double h[3] = {0.0, 0.0, 0.0};
-    TC_HIP(hipMemcpyAsync(s->resid.p, h, sizeof(h), hipMemcpyHostToDevice, s->stream));
+    double h[4] = {0.0, 0.0, 0.0, 0.0};
     TC_HIP(tc::launch_residual(s->stream, s->Pbuf(s->last_p), s->drec.p, s->dcc.p, s->dlam_buf(s->last_lam),
                                uint32_t(m.d_v0.size()), s->brec.p, s->blam_buf(s->last_lam), uint32_t(m.b_v0.size()),
-                               dt_sub, s->split_compliance, s->resid.p));
+                               dt_sub, s->split_compliance, s->resid_part.p, s->resid.p, nullptr, 0.0f));
     TC_HIP(hipMemcpyAsync(h, s->resid.p, sizeof(h), hipMemcpyDeviceToHost, s->stream));
     TC_HIP(hipStreamSynchronize(s->stream));
     if (rel) *rel = h[1] > 0.0 ? std::sqrt(h[0]) / std::sqrt(h[1]) : 0.0;

@@ -1,0 +1,232 @@
+// Internal definition of the simulator object shared by the translation units
+// of the C ABI (tc_sim.hip: simulator, tc_ws_api.hip: learned warm start).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tc_sim.h"
+#include "tc_kernels.h"
+#include "tc_mesh.h"
+#include "tc_tiles.h"
+#include "tc_ws.h"
+
+namespace tc {
+
+extern thread_local std::string g_err;
+
+inline int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define TC_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(TC_ERR_RUNTIME, std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        return hipMalloc(reinterpret_cast<void **>(&p), sizeof(T) * std::max<size_t>(count, 1));
+    }
+    hipError_t upload(const T *src, size_t count, hipStream_t s) {
+        hipError_t e = alloc(count);
+        if (e != hipSuccess || !count) return e;
+        return hipMemcpyAsync(p, src, sizeof(T) * count, hipMemcpyHostToDevice, s);
+    }
+};
+
+}  // namespace tc
+
+using tc::DevBuf;
+using tc::fail;
+
+// Weights, optimiser state and training scratch of one warm-start network.
+struct tc_ws_net {
+    tc::ws::NetLayout nl;
+    DevBuf<float> theta, m, v, grad;
+    DevBuf<tc::ws::bf16> w, wt;
+    uint32_t step = 0;
+    hipStream_t stream = nullptr;
+    // training scratch for up to rows_cap rows
+    uint32_t rows_cap = 0;
+    DevBuf<tc::ws::bf16> h[5], x_r[4], a_r[4], a_enc, a_dec, dyb, dzb, feat, agg;
+    DevBuf<float> y, dy, dh0, dh1, dx, partial;
+    DevBuf<double> loss_part;
+    DevBuf<float4> tgt;
+    DevBuf<uint32_t> goff, gnbr;
+    ~tc_ws_net() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    tc::ws::NetDev dev() const { return {theta.p, w.p, wt.p}; }
+};
+
+struct tc_sim {
+    tc::Mesh mesh;
+    hipStream_t stream = nullptr;
+    uint32_t block = 256;
+
+    // vertex streams (P has a second buffer for the one-launch-per-iteration path)
+    DevBuf<float4> X, P, V, Palt;
+    std::vector<float4> X0h, P0h;  // initial state for reset()
+
+    // constraints
+    DevBuf<uint4> drec, brec;
+    DevBuf<float> dcc, dlam, blam, dlam2, blam2;
+    bool split_compliance = false;
+    std::vector<uint4> drec_h, brec_h;
+    std::vector<float> dcc_h;
+
+    // colliders
+    std::vector<tc::Collider> cols_h;
+    DevBuf<tc::Collider> cols;
+    std::vector<tc::MeshTri> tris_h;
+    DevBuf<tc::MeshTri> tris;
+    DevBuf<float4> tri_bounds;
+    int single_sided_recovery = 1;
+
+    // self collision
+    DevBuf<float> local_len, th_rest;
+    DevBuf<uint32_t> th_off, th_idx, star_off, edge_bbits;
+    DevBuf<uint2> star, edges;
+    DevBuf<uint32_t> vkeys, vcounts, vstarts, vsorted;
+    DevBuf<uint32_t> ekeys, ecounts, estarts, esorted;
+    DevBuf<tc::Accum> accum;
+    DevBuf<float> readback;
+    DevBuf<double> resid, resid_part;
+
+    // parameters (simulation/mod.rs:607-691 defaults)
+    float gravity[3] = {0.0f, 0.0f, -9.81f};
+    float damping = 1.0f;
+    float tension_damping = 0.0f, compression_damping = 0.0f, shear_damping = 0.0f, bending_damping = 0.0f;
+    uint32_t iterations = 2;
+    bool self_collision = true;
+    float relief_factor = 0.2f, max_disp_ratio = 0.2f;
+    int enable_relief = 1, exclude_neighbors = 1;
+    bool normal_untangling = true;  // only effective for multi-layer meshes (pipelines.rs:895-896)
+    uint32_t max_search_iterations = 256;
+    float cell_size = 0.01f;
+    uint32_t table_size = 1024;
+
+    // one-launch-per-iteration plan (tc_tiles.cpp)
+    int solver_path = 0;          // 0 auto, 1 one launch per colour, 2 one launch per iteration
+    bool fused_ready = false;     // plan built and uploaded
+    std::string fused_reason;     // why the plan is unavailable
+    tc::TilePlan plan;
+    DevBuf<uint32_t> pl_ext_off, pl_ext_vid, pl_n_own, pl_ph_off, pl_items;
+    uint32_t fused_block = 1024;
+    uint32_t tile_vertices = 1024;  // TCLOTH_TILE_VERTICES overrides; tc_sim_autotune picks per mesh
+    uint32_t last_p = 0, last_lam = 0;  // buffers holding P and lambda after the last substep
+
+    // learned warm start (tc_ws.h, tc_ws_api.hip)
+    int ws_mode = 0;                   // 0 off, 1 previous correction only, 2 network
+    tc_ws_net *ws_net = nullptr;
+    uint32_t ws_iters = 8, ws_fallback = 56;
+    float ws_threshold = 1e-4f;
+    bool ws_ready = false;             // incidence arrays and buffers on the device
+    DevBuf<uint32_t> ws_doff, ws_dnbr, ws_boff, ws_bnbr, ws_goff, ws_gnbr;
+    DevBuf<float> ws_drest, ws_brest;
+    std::vector<uint32_t> ws_goff_h, ws_gnbr_h;
+    float ws_lscale = 0.01f;
+    DevBuf<float4> ws_ppred, ws_D;
+    DevBuf<tc::ws::bf16> ws_feat, ws_h0, ws_h1, ws_agg;
+    DevBuf<float> ws_y;
+    DevBuf<uint32_t> ws_gate;          // [fallback flag, substeps checked, fallbacks, unused]
+    DevBuf<double> ws_res;             // residual at the gate: sum h^2, sum L0^2, max, relative
+    // training samples recorded from plain substeps
+    uint32_t rec_cap = 0, rec_stride = 1, rec_count = 0, rec_tick = 0;
+    int rec_slot = -1;                 // slot the substep being enqueued writes (-1: none)
+    DevBuf<tc::ws::bf16> rec_feat;
+    DevBuf<float4> rec_T, rec_ppred, rec_D;
+    bool ws_active() const { return ws_mode != 0 && ws_ready && (ws_mode == 1 || ws_net != nullptr); }
+    bool ws_tracking() const { return ws_active() || rec_cap > 0; }
+    uint32_t iters_effective() const { return ws_active() ? ws_iters : iterations; }
+
+    // frame graph cache
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    uint64_t graph_key[4] = {0, 0, 0, 0};
+    uint64_t epoch = 1;  // bumped whenever a parameter or buffer baked into the graph changes
+    bool use_graph = true;  // TCLOTH_USE_GRAPH=0 launches kernel by kernel (profilers)
+
+    ~tc_sim() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void touch() { ++epoch; }
+
+    tc::SubstepArgs args(float dt_sub) const {
+        tc::SubstepArgs a;
+        a.dt = dt_sub;
+        a.gx = gravity[0]; a.gy = gravity[1]; a.gz = gravity[2];
+        a.damping = damping;
+        a.tension_damping = tension_damping; a.compression_damping = compression_damping;
+        a.shear_damping = shear_damping; a.bending_damping = bending_damping;
+        return a;
+    }
+    tc::SelfColArgs sc_args() const {
+        tc::SelfColArgs a;
+        a.cell = cell_size; a.table = table_size;
+        a.nv = mesh.nv; a.ne = uint32_t(mesh.e_v0.size());
+        a.relief_factor = relief_factor; a.max_disp_ratio = max_disp_ratio;
+        a.enable_relief = enable_relief; a.exclude_neighbors = exclude_neighbors;
+        a.max_search_iterations = max_search_iterations;
+        return a;
+    }
+    tc::SelfColTopo topo() const {
+        tc::SelfColTopo t;
+        t.local_len = local_len.p; t.th_off = th_off.p; t.th_idx = th_idx.p; t.th_rest = th_rest.p;
+        t.star_off = star_off.p; t.star = star.p; t.edges = edges.p; t.edge_bbits = edge_bbits.p;
+        return t;
+    }
+    bool fused_active() const {
+        if (!fused_ready) return false;
+        if (solver_path == 1) return false;
+        if (solver_path == 2) return true;
+        return mesh.nv >= 16384;  // small meshes: the per-colour launches are cheap
+    }
+    float4 *Pbuf(uint32_t k) const { return k ? Palt.p : P.p; }
+    float *dlam_buf(uint32_t k) const { return k ? dlam2.p : dlam.p; }
+    float *blam_buf(uint32_t k) const { return k ? blam2.p : blam.p; }
+    tc::FusedArgs fused_args() const {
+        tc::FusedArgs f{};
+        f.ntiles = plan.ntiles; f.nphases = plan.nphases; f.nd_phases = plan.nd_phases; f.max_ext = plan.max_ext;
+        for (size_t c = 0; c < plan.dcls_tc.size(); ++c) { f.dtc[c] = plan.dcls_tc[c]; f.dcc[c] = plan.dcls_cc[c]; }
+        for (size_t c = 0; c < plan.bcls.size(); ++c) f.bcomp[c] = plan.bcls[c];
+        return f;
+    }
+    tc::FusedPlanDev plan_dev() const {
+        return {pl_ext_off.p, pl_ext_vid.p, pl_n_own.p, pl_ph_off.p, pl_items.p};
+    }
+    tc::HashBufs vhash() const { return {vkeys.p, vcounts.p, vstarts.p, vsorted.p}; }
+    tc::HashBufs ehash() const { return {ekeys.p, ecounts.p, estarts.p, esorted.p}; }
+};
+
+// Learned warm start hooks around the projection loop of one substep (tc_ws_api.hip).
+// ws_pre runs after predict: features, network and apply (or only the copy of the
+// prediction when recording plain substeps); ws_post runs after the projection loop.
+hipError_t ws_pre(tc_sim *s, float dt_sub);
+hipError_t ws_post(tc_sim *s, float dt_sub, const float4 *Pc);
+float ws_scale(const tc_sim *s, float dt_sub);
+// Zeroes the previous-substep correction (after the state is set from outside).
+int ws_clear_history(tc_sim *s);

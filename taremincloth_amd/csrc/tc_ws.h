@@ -1,0 +1,127 @@
+// Learned warm start: a message-passing network over the cloth edge graph that
+// predicts each vertex's XPBD position correction for the coming substep.
+// No reference counterpart; it sits between predict and the projection loop of
+// the reference substep (crates/cloth_core/src/simulation/dispatch.rs:137 -> 169).
+//
+// Network (hidden width H = 128, input width 32, R message-passing rounds):
+//   encoder   h0      = W_e2 relu(W_e1 f + b_e1) + b_e2
+//   round r   h_{r+1} = h_r + W_r2 relu(W_r1 [h_r, mean_{j in N(i)} h_r[j]] + b_r1) + b_r2
+//   decoder   y       = W_d2 relu(W_d1 h_R + b_d1) + b_d2          (16 outputs, the first 3 used)
+// Dense layers run on MFMA (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32
+// accumulation); activations are stored in BF16, gradients accumulate in FP32.
+// The correction applied to a free vertex is  D_prev + s * y[0:3],  where
+// D_prev is the correction of the previous substep and s = max(|g|, 1 m/s^2) dt^2.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tc {
+namespace ws {
+
+constexpr int H = 128;        // hidden width
+constexpr int F = 32;         // input features per vertex
+constexpr int O = 16;         // decoder output width
+constexpr int TR = 64;        // rows per tile
+constexpr int MAX_LAYERS = 16;
+
+typedef __bf16 bf16;
+
+// One dense layer inside the flat parameter vector: W (K x N, row-major) then b (N).
+struct Layer { uint32_t off, K, N; };
+
+struct NetLayout {
+    uint32_t rounds = 3;
+    uint32_t nlayers = 0;
+    Layer L[MAX_LAYERS];
+    uint32_t nparams = 0;
+    // layer indices
+    uint32_t enc1() const { return 0; }
+    uint32_t enc2() const { return 1; }
+    uint32_t r1(uint32_t r) const { return 2 + 2 * r; }
+    uint32_t r2(uint32_t r) const { return 3 + 2 * r; }
+    uint32_t dec1() const { return 2 + 2 * rounds; }
+    uint32_t dec2() const { return 3 + 2 * rounds; }
+};
+NetLayout make_layout(uint32_t rounds);
+
+// Device copies of the weights: fp32 master (theta), bf16 row-major (w) and bf16
+// transposed (wt) at the same offsets as theta.
+struct NetDev {
+    const float *theta;
+    const bf16 *w;
+    const bf16 *wt;
+};
+
+// Feature inputs of one cloth.
+struct FeatArgs {
+    const float4 *P;        // predicted positions + inverse mass
+    const float4 *D;        // previous correction (xyz)
+    const uint32_t *d_off, *d_nbr; const float *d_rest;  // distance-constraint incidence (CSR)
+    const uint32_t *b_off, *b_nbr; const float *b_rest;  // bending-pair incidence (CSR)
+    const void *cols; uint32_t ncols;                    // tc::Collider primitives
+    const float4 *X;        // committed positions (w = thickness)
+    uint32_t nv;
+    float s;                // max(|g|, 1) dt^2
+    float lscale;           // collider distance scale (mean edge length)
+};
+
+// Graph of the message-passing rounds (stretch edges, both directions).
+struct Graph { const uint32_t *off, *nbr; };
+
+hipError_t launch_features(hipStream_t st, const FeatArgs &a, bf16 *feat);
+hipError_t launch_copy_pred(hipStream_t st, const float4 *P, float4 *Ppred, uint32_t nv);
+// Applies D_prev + s*y (y == nullptr: D_prev only) to the free vertices of P; keeps P in Ppred.
+hipError_t launch_apply(hipStream_t st, float4 *P, float4 *Ppred, const float4 *D, const float *y, float s, uint32_t nv);
+// D = P - Ppred on free vertices (0 on pinned ones); with T != nullptr also writes the
+// training target T = ((P - Ppred - D_old) / s, free ? 1 : 0) before D is replaced.
+hipError_t launch_update_prev(hipStream_t st, const float4 *P, const float4 *Ppred, float4 *D, float4 *T, float s,
+                              uint32_t nv);
+
+// Forward of the whole network (inference or training).  Buffers (rows = nv):
+//   feat rows x F;  h[0..R] rows x H (h[r] for training, two ping-pong buffers otherwise);
+//   agg rows x H (scratch of the neighbour mean);
+//   y rows x O (fp32).  Training saves (may be null for inference):
+//   a_enc rows x H, x_r[r] rows x 2H, a_r[r] rows x H, a_dec rows x H.
+struct FwdBufs {
+    const bf16 *feat;
+    bf16 *h[8];
+    bf16 *agg;           // rows x H: neighbour mean of the current round
+    float *y;
+    bf16 *a_enc, *a_dec;
+    bf16 *x_r[4], *a_r[4];
+    bool training;
+};
+hipError_t forward(hipStream_t st, const NetLayout &nl, const NetDev &nd, const Graph &g, uint32_t rows,
+                   const FwdBufs &b);
+
+// Backward + weight gradients.  t: rows x 4 (target xyz, mask);  n_free: rows with mask 1.
+struct BwdBufs {
+    float *dy;           // rows x O   (loss gradient)
+    float *dh0, *dh1;    // rows x H fp32 ping-pong
+    float *dx;           // rows x 2H fp32
+    bf16 *dyb;           // rows x H  bf16 gradient of a layer's output
+    bf16 *dzb;           // rows x H  bf16 gradient of a layer's pre-activation
+    float *partial;      // per-block weight-gradient partials
+    uint32_t partial_blocks;
+    float *grad;         // nparams
+    double *loss_part;   // per-block loss partials (>= 1024)
+};
+hipError_t loss_and_backward(hipStream_t st, const NetLayout &nl, const NetDev &nd, const Graph &g, uint32_t rows,
+                             const FwdBufs &f, const float4 *t, float inv_nfree, const BwdBufs &b);
+uint32_t loss_blocks(uint32_t rows);
+
+struct AdamArgs {
+    uint32_t n;
+    float lr, beta1, beta2, eps, bc1, bc2;
+};
+hipError_t launch_adam(hipStream_t st, const NetLayout &nl, const AdamArgs &a, float *theta, float *m, float *v,
+                       const float *g, bf16 *w, bf16 *wt);
+// Refresh the bf16 copies from theta (after the host sets parameters).
+hipError_t launch_refresh(hipStream_t st, const NetLayout &nl, const float *theta, bf16 *w, bf16 *wt);
+
+hipError_t set_lds_limits();
+size_t partial_floats(const NetLayout &nl, uint32_t blocks);
+uint32_t gemm_blocks(uint32_t rows);
+
+}  // namespace ws
+}  // namespace tc

@@ -15,6 +15,7 @@
 // and two 16-byte gathers of P.  No two constraints of one colour share a
 // vertex, so the scatter back to P needs no atomics.
 #include "tc_kernels.h"
+#include <algorithm>
 
 namespace tc {
 
@@ -46,6 +47,7 @@ __global__ void __launch_bounds__(256) k_distance(float4 *__restrict__ P, const 
                                                   const uint4 *__restrict__ rec, const float *__restrict__ ccomp,
                                                   float *__restrict__ lam, uint32_t off, uint32_t cnt,
                                                   SubstepArgs a) {
+    if (a.run_flag && *a.run_flag == 0u) return;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cnt) return;
     const uint32_t k = off + t;
@@ -88,6 +90,7 @@ template <bool FIRST, bool DAMP>
 __global__ void __launch_bounds__(256) k_bending(float4 *__restrict__ P, const float4 *__restrict__ X,
                                                  const uint4 *__restrict__ rec, float *__restrict__ lam,
                                                  uint32_t off, uint32_t cnt, SubstepArgs a) {
+    if (a.run_flag && *a.run_flag == 0u) return;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cnt) return;
     const uint32_t k = off + t;
@@ -331,28 +334,33 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
     if (i < n) p[i] = value;
 }
 
-// Residual h_j = C_j + alpha~_j lambda_j over distance and bending constraints.
-__device__ __forceinline__ void atomic_max_double_nonneg(double *addr, double v) {
-    unsigned long long *a = reinterpret_cast<unsigned long long *>(addr);
-    unsigned long long old = *a, assumed;
-    do {
-        assumed = old;
-        if (__longlong_as_double(assumed) >= v) return;
-        old = atomicCAS(a, assumed, (unsigned long long)__double_as_longlong(v));
-    } while (old != assumed);
+// Residual h_j = C_j + alpha~_j lambda_j over distance and bending constraints,
+// accumulated in double.  Pass 1: each block strides over the constraints and
+// writes (sum h^2, sum L0^2, max |h|/L0) to its slot of `part`; pass 2 (one
+// block) sums the slots in block order, so the result does not depend on
+// scheduling.
+constexpr uint32_t RES_BLOCKS = 1024;
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+    return v;
 }
 
-__global__ void __launch_bounds__(256) k_residual(const float4 *__restrict__ P, const uint4 *__restrict__ drec,
-                                                  const float *__restrict__ dcc, const float *__restrict__ dlam,
-                                                  uint32_t nd, const uint4 *__restrict__ brec,
-                                                  const float *__restrict__ blam, uint32_t nb, float dt_sub,
-                                                  int split, double *__restrict__ out) {
-    __shared__ double sh[2][256];
-    __shared__ double smax[256];
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_residual_part(const float4 *__restrict__ P, const uint4 *__restrict__ drec,
+                                                       const float *__restrict__ dcc, const float *__restrict__ dlam,
+                                                       uint32_t nd, const uint4 *__restrict__ brec,
+                                                       const float *__restrict__ blam, uint32_t nb, float dt_sub,
+                                                       int split, double *__restrict__ part) {
+    __shared__ double sh[3][4];
     double hh = 0.0, ll = 0.0, mx = 0.0;
     const double dt = dt_sub > 1e-8f ? (double)dt_sub : 1e-8;
-    if (t < nd + nb) {
+    const double inv_dt2 = 1.0 / (dt * dt);
+    const uint32_t n = nd + nb;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const bool isd = t < nd;
         const uint32_t k = isd ? t : t - nd;
         const uint4 r = isd ? drec[k] : brec[k];
@@ -365,28 +373,52 @@ __global__ void __launch_bounds__(256) k_residual(const float4 *__restrict__ P, 
         double comp = __uint_as_float(r.w);
         if (isd && split && d - L < 0.0) comp = dcc[k];
         if (ws > 1e-7 && d > 1e-7 && comp < 1e9) {
-            const double h = (d - L) + comp / (dt * dt) * (double)(isd ? dlam[k] : blam[k]);
-            hh = h * h;
-            ll = L * L;
-            if (L > 0.0) mx = fabs(h) / L;
+            const double h = (d - L) + comp * inv_dt2 * (double)(isd ? dlam[k] : blam[k]);
+            hh += h * h;
+            ll += L * L;
+            if (L > 0.0) mx = fmax(mx, fabs(h) / L);
         }
     }
-    sh[0][threadIdx.x] = hh;
-    sh[1][threadIdx.x] = ll;
-    smax[threadIdx.x] = mx;
+    hh = wave_sum(hh);
+    ll = wave_sum(ll);
+    mx = wave_max(mx);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sh[0][w] = hh; sh[1][w] = ll; sh[2][w] = mx; }
     __syncthreads();
-    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
-            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
-        }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        atomicAdd(&out[0], sh[0][0]);
-        atomicAdd(&out[1], sh[1][0]);
-        atomic_max_double_nonneg(&out[2], smax[0]);
+        part[3 * blockIdx.x + 0] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+        part[3 * blockIdx.x + 1] = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
+        part[3 * blockIdx.x + 2] = fmax(fmax(sh[2][0], sh[2][1]), fmax(sh[2][2], sh[2][3]));
+    }
+}
+
+// out[0] = sum h^2, out[1] = sum L0^2, out[2] = max |h|/L0, out[3] = relative residual.
+// With gate != nullptr: gate[0] = 1 if the relative residual exceeds `threshold` (else 0),
+// gate[1] += 1 (substeps checked), gate[2] += gate[0] (substeps that fell back).
+__global__ void __launch_bounds__(64) k_residual_final(const double *__restrict__ part, uint32_t nblocks,
+                                                       double *__restrict__ out, uint32_t *__restrict__ gate,
+                                                       float threshold) {
+    double hh = 0.0, ll = 0.0, mx = 0.0;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += 64) {
+        hh += part[3 * b + 0];
+        ll += part[3 * b + 1];
+        mx = fmax(mx, part[3 * b + 2]);
+    }
+    hh = wave_sum(hh);
+    ll = wave_sum(ll);
+    mx = wave_max(mx);
+    if (threadIdx.x == 0) {
+        const double rel = ll > 0.0 ? sqrt(hh / ll) : 0.0;
+        out[0] = hh;
+        out[1] = ll;
+        out[2] = mx;
+        out[3] = rel;
+        if (gate) {
+            const uint32_t fail = rel > (double)threshold ? 1u : 0u;
+            gate[0] = fail;
+            gate[1] += 1u;
+            gate[2] += fail;
+        }
     }
 }
 
@@ -461,11 +493,12 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t value, size_t n)
 
 hipError_t launch_residual(hipStream_t s, const float4 *P, const uint4 *drec, const float *dcc, const float *dlam,
                            uint32_t nd, const uint4 *brec, const float *blam, uint32_t nb, float dt, bool split,
-                           double *out) {
+                           double *part, double *out, uint32_t *gate, float threshold) {
     const size_t n = size_t(nd) + nb;
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_residual, dim3(grid_for(n, 256)), dim3(256), 0, s, P, drec, dcc, dlam, nd, brec, blam, nb,
-                       dt, split ? 1 : 0, out);
+    const uint32_t blocks = uint32_t(std::min<size_t>(RES_BLOCKS, std::max<size_t>(grid_for(n, 256), 1)));
+    hipLaunchKernelGGL(k_residual_part, dim3(blocks), dim3(256), 0, s, P, drec, dcc, dlam, nd, brec, blam, nb, dt,
+                       split ? 1 : 0, part);
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, s, part, blocks, out, gate, threshold);
     return hipGetLastError();
 }
 

@@ -303,6 +303,50 @@ class ClothSimulator:
     def synchronize(self):
         _lib.check(self._L.tc_sim_synchronize(self._h))
 
+    # --- learned warm start (no reference counterpart; see taremincloth_amd.warmstart) -------
+    def set_warm_start(self, net=None, mode=0, iterations=8, fallback_iterations=56, residual_threshold=1e-4):
+        """mode 0 off, 1 previous correction, 2 network (`net`); the network must outlive its use here."""
+        self._ws_net = net  # keep the network alive while it is attached
+        _lib.check(self._L.tc_sim_set_warm_start(self._h, None if net is None else net.handle, int(mode),
+                                                 int(iterations), int(fallback_iterations),
+                                                 float(residual_threshold)))
+
+    def warm_start_stats(self) -> dict:
+        out = (C.c_double * 6)()
+        _lib.check(self._L.tc_sim_warm_start_stats(self._h, out))
+        keys = ["checked", "fallbacks", "last_relative_residual", "last_max_relative", "mode", "iterations"]
+        return dict(zip(keys, list(out)))
+
+    def ws_reset_stats(self):
+        _lib.check(self._L.tc_sim_ws_reset_stats(self._h))
+
+    def ws_record(self, capacity, stride=1):
+        _lib.check(self._L.tc_sim_ws_record(self._h, int(capacity), int(stride)))
+
+    def ws_num_records(self) -> int:
+        return int(self._L.tc_sim_ws_num_records(self._h))
+
+    def ws_get_record(self, idx) -> dict:
+        n = self._n
+        f = np.empty((n, 32), np.float32)
+        t, p, d = (np.empty((n, 4), np.float32) for _ in range(3))
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        _lib.check(self._L.tc_sim_ws_get_record(self._h, int(idx), vp(f), vp(t), vp(p), vp(d)))
+        return {"features": f, "target": t, "prediction": p, "previous": d[:, :3].copy()}
+
+    def ws_last_output(self) -> np.ndarray:
+        y = np.empty((self._n, 16), np.float32)
+        _lib.check(self._L.tc_sim_ws_last_output(self._h, y.ctypes.data_as(C.c_void_p), y.size))
+        return y
+
+    def ws_history(self, new=None) -> np.ndarray:
+        """Previous-substep correction (N x 3); with `new`, replaces it and returns the old one."""
+        out = np.empty((self._n, 3), np.float32)
+        src = None if new is None else np.ascontiguousarray(new, np.float32).reshape(self._n, 3)
+        _lib.check(self._L.tc_sim_ws_history(self._h, out.ctypes.data_as(C.c_void_p),
+                                             None if src is None else src.ctypes.data_as(C.c_void_p), self._n))
+        return out
+
 
 def color_edge_pairs(n, edges) -> list:
     """Welsh-Powell colour per vertex pair (lib.rs:636-648 -> coloring.rs:180)."""

@@ -1,0 +1,245 @@
+"""Learned warm start: the FP64 oracle pinned by finite differences and closed forms (CPU),
+then the HIP forward / backward / Adam / features / gate checked against it (GPU)."""
+import numpy as np
+import pytest
+
+from oracle import warmstart_oracle as W
+
+import cases
+
+
+def ring_graph(n, extra=7, seed=0):
+    """Symmetric neighbour lists: a ring plus seeded chords (no self loops)."""
+    rng = np.random.default_rng(seed)
+    pairs = {(i, (i + 1) % n) for i in range(n)}
+    for _ in range(extra):
+        a, b = rng.integers(0, n, 2)
+        if a != b:
+            pairs.add((int(a), int(b)))
+    a = np.array([p[0] for p in pairs])
+    b = np.array([p[1] for p in pairs])
+    off, nbr, _ = W.csr_pairs(n, a, b)
+    return off, nbr
+
+
+def random_case(rows, rounds, seed, free_frac=0.8):
+    rng = np.random.default_rng(seed)
+    feat = rng.normal(0, 1, (rows, W.F)).astype(np.float32)
+    feat[:, 16:] = 0.0
+    t = np.zeros((rows, 4), np.float32)
+    t[:, :3] = rng.normal(0, 1, (rows, 3))
+    t[:, 3] = rng.random(rows) < free_frac
+    off, nbr = ring_graph(rows, extra=rows // 2, seed=seed)
+    return W.init_params(rounds, seed), feat, off, nbr, t
+
+
+# ---------------------------------------------------------------- CPU: pin the oracle
+def test_oracle_backward_matches_finite_differences():
+    rounds = 2
+    p, feat, off, nbr, t = random_case(12, rounds, 3)
+    loss, g, _ = W.backward(p, rounds, feat, off, nbr, t, bf16=False)
+    rng = np.random.default_rng(5)
+    lay, _ = W.layout(rounds)
+    picks = []
+    for o, K, N in lay:  # a few weights and a bias of every layer
+        picks += list(o + rng.integers(0, K * N, 3)) + [o + K * N + int(rng.integers(0, N))]
+    pd = p.astype(np.float64)
+    for i in picks:
+        h = 1e-6
+        a, b = pd.copy(), pd.copy()
+        a[i] += h
+        b[i] -= h
+        la = W.loss_grad(W.forward(a, rounds, feat, off, nbr, bf16=False)[0], t)[0]
+        lb = W.loss_grad(W.forward(b, rounds, feat, off, nbr, bf16=False)[0], t)[0]
+        fd = (la - lb) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_oracle_graph_transpose_is_the_adjoint():
+    off, nbr = ring_graph(9, extra=6, seed=1)
+    n = 9
+    A = np.zeros((n, n))
+    deg = np.diff(off)
+    for i in range(n):
+        for k in range(off[i], off[i + 1]):
+            A[i, nbr[k]] += 1.0 / deg[i]
+    x = np.random.default_rng(0).normal(size=(n, 4))
+    assert np.allclose(W.mean_agg(x, off, nbr), A @ x)
+    assert np.allclose(W.mean_agg_transpose(x, off, nbr), A.T @ x)
+
+
+def test_oracle_features_closed_form():
+    """Two free vertices 0.012 m apart with rest 0.010 m: the stretch term pulls each toward the other
+    by (len - L) / s; a sphere 0.05 m below gives the signed gap and the outward normal."""
+    s, L = 1e-4, 0.010
+    ppred = np.array([[0.0, 0.0, 0.1, 1.0], [0.012, 0.0, 0.1, 1.0]])
+    mesh = {"dist_v0": [0], "dist_v1": [1], "dist_rest": [L], "bend_v2": [], "bend_v3": [], "bend_rest": []}
+    cols = [(0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0.05)]
+    f = W.features(ppred, np.zeros((2, 3)), np.full(2, 0.005), mesh, cols, s, 0.01)
+    pull = (0.012 - L) / s
+    assert np.allclose(f[0, 3:6], [W.soft(pull), 0, 0]) and np.allclose(f[1, 3:6], [-W.soft(pull), 0, 0])
+    gap0 = (np.hypot(0.0, 0.1) - 0.05 - 0.005) / 0.01
+    assert np.isclose(f[0, 12], W.soft(min(gap0, 4.0)))
+    assert f[0, 9] == 1 and f[0, 10] == 0.125 and np.allclose(f[:, 16:], 0)
+    assert np.allclose(f[0, 13:16], 0)  # gap of 4.5 lengths: beyond the range, no normal
+    near = W.features(ppred - [0, 0, 0.04, 0], np.zeros((2, 3)), np.full(2, 0.005), mesh, cols, s, 0.01)
+    assert np.allclose(near[0, 13:16], [0, 0, 1])
+
+
+def test_layout_matches_documented_parameter_count():
+    assert W.layout(3)[1] == 187536
+    assert W.layout(1)[1] == 187536 - 2 * (256 * 128 + 128 + 128 * 128 + 128)
+
+
+# ---------------------------------------------------------------- GPU
+def _net(rounds, params):
+    from taremincloth_amd.warmstart import WarmStartNet
+    net = WarmStartNet(rounds)
+    net.set_params(params)
+    return net
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows,rounds", [(200, 3), (1000, 2), (64, 1)])
+def test_gpu_forward_and_gradients_match_oracle(require_gpu, rows, rounds):
+    p, feat, off, nbr, t = random_case(rows, rounds, rows)
+    net = _net(rounds, p)
+    y, g, loss = net.forward_backward(feat, off, nbr, t)
+    lo, go, yo = W.backward(p, rounds, feat, off, nbr, t, bf16=True)
+    ey = np.abs(y - yo).max() / np.abs(yo).max()
+    assert ey < 2e-2, ey
+    assert abs(loss - lo) <= 2e-2 * lo
+    lay, _ = W.layout(rounds)
+    for o, K, N in lay:
+        for sl in (slice(o, o + K * N), slice(o + K * N, o + K * N + N)):
+            num = np.linalg.norm(g[sl] - go[sl])
+            den = np.linalg.norm(go[sl]) + 1e-12
+            assert num / den < 5e-2, (o, K, N, num / den)
+
+
+@pytest.mark.gpu
+def test_gpu_adam_first_step_closed_form(require_gpu):
+    p, feat, off, nbr, t = random_case(128, 1, 9)
+    net = _net(1, p)
+    _, g, _ = net.forward_backward(feat, off, nbr, t)
+    net.adam_step(1e-3)
+    got = net.get_params()
+    want = W.adam_first_step(p.astype(np.float64), g.astype(np.float64), 1e-3)
+    assert np.abs(got - want).max() < 1e-6
+
+
+def _drape(n=24, jitter_seed=0):
+    from taremincloth_amd import taremin_cloth_core as T
+    pos, edges, faces = cases.grid(n, n, 0.02)
+    pos = pos + np.array([-(n - 1) * 0.01, -(n - 1) * 0.01, 0.25], np.float32)
+    inv = cases.areal_inv_masses_np(pos, faces, 0.15)
+    sim = T.ClothSimulator(pos, edges, faces, inv, thickness=0.005, stiffness=1000.0, bending_stiffness=10.0)
+    sim.set_enable_self_collision(False)
+    sim.set_compliances_raw(1e-8, 1e-6, 1e-5)
+    sim.add_sphere_collider([0.0, 0.0, 0.0], 0.15, 0.3, 0.0)
+    sim.set_solver_iterations(16)
+    return sim, pos, edges, faces, inv
+
+
+@pytest.mark.gpu
+def test_gpu_recorded_features_and_targets_match_oracle(require_gpu):
+    from oracle import oracle as O
+    sim, pos, edges, faces, inv = _drape()
+    dt_sub = 1.0 / 600.0
+    for _ in range(12):  # fall onto the sphere first
+        sim.step(1.0 / 60.0, 10, 16)
+    sim.ws_record(4, 1)
+    sim.step(dt_sub * 2, 2, 16)
+    assert sim.ws_num_records() == 2
+    mesh = O.build_mesh(pos, edges, faces, inv, thickness=0.005, stiffness=1000.0, bending_stiffness=10.0)
+    s = 9.81 * dt_sub * dt_sub
+    cols = [(0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), 0.15)]
+    r0, r1 = sim.ws_get_record(0), sim.ws_get_record(1)
+    for r in (r0, r1):
+        want = W.bf16_round(W.features(r["prediction"], r["previous"], np.full(len(pos), 0.005), mesh, cols, s,
+                                       mesh["mean_edge"]))
+        err = np.abs(r["features"] - want)
+        assert (err <= 1e-2 * np.maximum(np.abs(want), 1.0)).all(), err.max()
+    # the second sample's previous correction is the first sample's correction
+    prev1 = r0["previous"] + r0["target"][:, :3] * s
+    assert np.abs(r1["previous"] - prev1).max() < 1e-7
+    assert (r0["target"][:, 3] == (inv > 0)).all()
+
+
+@pytest.mark.gpu
+def test_gpu_gate_fallback_equals_plain_iterations(require_gpu):
+    """Previous-correction mode on the first substep (correction 0): 4 iterations + gate + 6 fallback
+    iterations equal 10 plain iterations bit for bit when the gate fails (threshold 0), and 4 plain
+    iterations when it passes (threshold 1e9)."""
+    dt_sub = 1.0 / 600.0
+    outs = {}
+    for key, mode, th, its in (("plain10", 0, 0.0, 10), ("plain4", 0, 0.0, 4), ("fail", 1, 0.0, 4),
+                               ("pass", 1, 1e9, 4)):
+        sim, pos, *_ = _drape()
+        sim.set_solver_iterations(its)
+        if mode:
+            sim.set_warm_start(None, mode, its, 6, th)
+        sim.step_single_substep(dt_sub)
+        out = np.zeros(pos.size, np.float32)
+        sim.get_positions(out)
+        outs[key] = out
+        if mode:
+            st = sim.warm_start_stats()
+            assert st["checked"] == 1 and st["fallbacks"] == (1 if th == 0.0 else 0)
+    assert np.array_equal(outs["fail"], outs["plain10"])
+    assert np.array_equal(outs["pass"], outs["plain4"])
+
+
+def _stretch_graph(pos, edges, faces, inv):
+    from oracle import oracle as O
+    m = O.build_mesh(pos, edges, faces, inv)
+    stretch = m["dist_type"] == 0
+    off, nbr, _ = W.csr_pairs(len(pos), m["dist_v0"][stretch], m["dist_v1"][stretch])
+    return off, nbr
+
+
+@pytest.mark.gpu
+def test_gpu_training_reduces_loss_on_recorded_samples(require_gpu):
+    from taremincloth_amd.warmstart import WarmStartNet
+    sim, pos, edges, faces, inv = _drape()
+    for _ in range(10):
+        sim.step(1.0 / 60.0, 10, 16)
+    sim.ws_record(40, 1)
+    for _ in range(4):
+        sim.step(1.0 / 60.0, 10, 16)
+    assert sim.ws_num_records() == 40
+    off, nbr = _stretch_graph(pos, edges, faces, inv)
+    recs = [sim.ws_get_record(k) for k in range(0, 40, 4)]
+    net = WarmStartNet(3)
+    net.init(seed=1)
+
+    def mean_loss():
+        return np.mean([net.forward_backward(r["features"], off, nbr, r["target"])[2] for r in recs])
+
+    before = mean_loss()
+    losses = net.train([sim], steps=400, lr=1e-3, seed=2)
+    assert np.isfinite(losses).all()
+    after = mean_loss()
+    assert after < 0.8 * before, (before, after)
+
+
+@pytest.mark.gpu
+def test_gpu_network_inside_the_substep_matches_the_standalone_pass(require_gpu):
+    from taremincloth_amd.warmstart import WarmStartNet
+    sim, pos, edges, faces, inv = _drape()
+    for _ in range(8):
+        sim.step(1.0 / 60.0, 10, 16)
+    net = WarmStartNet(3)
+    net.init(seed=4)
+    p = net.get_params()
+    p[-(128 * 16 + 16):] = np.random.default_rng(0).normal(0, 0.05, 128 * 16 + 16)  # non-zero last layer
+    net.set_params(p)
+    sim.ws_record(1, 1)
+    sim.set_warm_start(net, 2, 8, 0, 1e9)
+    sim.step_single_substep(1.0 / 600.0)
+    r = sim.ws_get_record(0)
+    y_sim = sim.ws_last_output()
+    off, nbr = _stretch_graph(pos, edges, faces, inv)
+    y_ref, _, _ = net.forward_backward(r["features"], off, nbr)
+    assert np.array_equal(y_sim, y_ref)
+    assert np.abs(y_sim[:, :3]).max() > 0
