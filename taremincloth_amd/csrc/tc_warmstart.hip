@@ -57,7 +57,8 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a, bf16 *__restrict__
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.nv) return;
     const float4 pi = a.P[i], di = a.D[i];
-    const f3 p0 = mk3(pi.x + di.x, pi.y + di.y, pi.z + di.z);
+    const float bw = a.beta;
+    const f3 p0 = mk3(pi.x + bw * di.x, pi.y + bw * di.y, pi.z + bw * di.z);
     const float is = 1.0f / a.s;
     float f[F];
     for (int k = 0; k < F; ++k) f[k] = 0.0f;
@@ -69,7 +70,7 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a, bf16 *__restrict__
     for (uint32_t k = d0; k < d1; ++k) {
         const uint32_t j = a.d_nbr[k];
         const float4 pj = a.P[j], dj = a.D[j];
-        const f3 d = mk3(pj.x + dj.x, pj.y + dj.y, pj.z + dj.z) - p0;
+        const f3 d = mk3(pj.x + bw * dj.x, pj.y + bw * dj.y, pj.z + bw * dj.z) - p0;
         const float len = length(d);
         if (len > 1e-12f) acc = acc + d * ((len - a.d_rest[k]) / len);
     }
@@ -81,7 +82,7 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a, bf16 *__restrict__
     for (uint32_t k = b0; k < b1; ++k) {
         const uint32_t j = a.b_nbr[k];
         const float4 pj = a.P[j], dj = a.D[j];
-        const f3 d = mk3(pj.x + dj.x, pj.y + dj.y, pj.z + dj.z) - p0;
+        const f3 d = mk3(pj.x + bw * dj.x, pj.y + bw * dj.y, pj.z + bw * dj.z) - p0;
         const float len = length(d);
         if (len > 1e-12f) acc = acc + d * ((len - a.b_rest[k]) / len);
     }
@@ -139,7 +140,7 @@ __global__ void k_copy_pred(const float4 *__restrict__ P, float4 *__restrict__ P
 }
 
 __global__ void k_apply(float4 *__restrict__ P, float4 *__restrict__ Ppred, const float4 *__restrict__ D,
-                        const float *__restrict__ y, float s, uint32_t nv) {
+                        const float *__restrict__ y, float s, float beta, uint32_t nv) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nv) return;
     float4 p = P[i];
@@ -152,15 +153,15 @@ __global__ void k_apply(float4 *__restrict__ P, float4 *__restrict__ Ppred, cons
             oy = y[size_t(i) * O + 1];
             oz = y[size_t(i) * O + 2];
         }
-        p.x = p.x + (d.x + s * ox);
-        p.y = p.y + (d.y + s * oy);
-        p.z = p.z + (d.z + s * oz);
+        p.x = p.x + (beta * d.x + s * ox);
+        p.y = p.y + (beta * d.y + s * oy);
+        p.z = p.z + (beta * d.z + s * oz);
         P[i] = p;
     }
 }
 
 __global__ void k_update_prev(const float4 *__restrict__ P, const float4 *__restrict__ Ppred, float4 *__restrict__ D,
-                              float4 *__restrict__ T, float s, uint32_t nv) {
+                              float4 *__restrict__ T, float s, float beta, uint32_t nv) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nv) return;
     const float4 p = P[i], q = Ppred[i];
@@ -170,7 +171,8 @@ __global__ void k_update_prev(const float4 *__restrict__ P, const float4 *__rest
     if (T) {
         const float4 od = D[i];
         const float is = 1.0f / s;
-        T[i] = free_v ? make_float4((nd.x - od.x) * is, (nd.y - od.y) * is, (nd.z - od.z) * is, 1.0f)
+        T[i] = free_v ? make_float4((nd.x - beta * od.x) * is, (nd.y - beta * od.y) * is, (nd.z - beta * od.z) * is,
+                                    1.0f)
                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     D[i] = nd;
@@ -218,7 +220,7 @@ struct MlpFwd {
 
 template <int K1, int N2>
 constexpr size_t fwd_lds() {
-    return size_t(2) * (H * (K1 + 8) + N2 * (H + 8) + TR * (K1 + 8) + TR * (H + 8));
+    return size_t(2) * (TR * (K1 + 8) + TR * (H + 8));
 }
 
 // 16-byte chunk q of a tile's input row block: row r0 + q / (K1/8), column chunk q % (K1/8).
@@ -235,27 +237,32 @@ __device__ __forceinline__ bf16x8 load_x_chunk(const MlpFwd &a, uint32_t r0, int
 }
 
 // Fused two-layer block over tiles of 64 rows: X -> relu(X W1 + b1) -> (.) W2 + b2.
-// Weights stay in LDS for the whole (persistent) launch; wave w owns rows 16w..16w+15.
-// The next tile's input rows are loaded into registers while the current tile computes.
+// Both products are computed transposed (weights as the MFMA A operand, activations as
+// B), so a lane's four results are consecutive features of one row.  Wave w owns the
+// 64 output columns 64*(w&1).. of the hidden layer and the 32 rows 32*(w>>1).. of the
+// tile; its weight fragments stay in registers for the whole persistent launch, so LDS
+// only carries the activations.  The next tile's input is loaded into registers while
+// the current tile computes.
 template <int K1, int N2, int MODE>
 __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int LK = K1 + 8, LH = H + 8;
     constexpr int XCH = TR * K1 / 8 / 256;  // 16-byte input chunks per thread and tile
-    bf16 *w1s = reinterpret_cast<bf16 *>(smem);
-    bf16 *w2s = w1s + H * LK;
-    bf16 *xs = w2s + N2 * LH;
-    bf16 *as = xs + TR * LK;
-    // biases in LDS: a global load in the epilogue would wait for the prefetch of the next tile
+    constexpr int KS1 = K1 / 32, KS2 = H / 32;
+    constexpr int CT2 = N2 == H ? 4 : 1;     // layer-2 column tiles per wave
+    bf16 *xs = reinterpret_cast<bf16 *>(smem);  // TR x LK
+    bf16 *as = xs + TR * LK;                    // TR x LH
     __shared__ __align__(16) float b1s[H], b2s[N2];
     const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, g = l >> 4, c = l & 15;
-    for (int q = tid; q < H * K1 / 8; q += 256) {
-        const int n = q / (K1 / 8), k8 = q % (K1 / 8);
-        *reinterpret_cast<bf16x8 *>(&w1s[n * LK + k8 * 8]) = *reinterpret_cast<const bf16x8 *>(&a.w1t[n * K1 + k8 * 8]);
-    }
-    for (int q = tid; q < N2 * H / 8; q += 256) {
-        const int n = q / (H / 8), k8 = q % (H / 8);
-        *reinterpret_cast<bf16x8 *>(&w2s[n * LH + k8 * 8]) = *reinterpret_cast<const bf16x8 *>(&a.w2t[n * H + k8 * 8]);
+    const int cg = w & 1, rg = w >> 1;
+    bf16x8 wf1[4][KS1], wf2[CT2][KS2];
+    for (int ct = 0; ct < 4; ++ct)
+        for (int ks = 0; ks < KS1; ++ks)
+            wf1[ct][ks] = *reinterpret_cast<const bf16x8 *>(&a.w1t[(64 * cg + 16 * ct + c) * K1 + ks * 32 + g * 8]);
+    for (int ct = 0; ct < CT2; ++ct) {
+        const int n = N2 == H ? 64 * cg + 16 * ct + c : c;
+        for (int ks = 0; ks < KS2; ++ks)
+            wf2[ct][ks] = *reinterpret_cast<const bf16x8 *>(&a.w2t[n * H + ks * 32 + g * 8]);
     }
     if (tid < H) b1s[tid] = a.b1[tid];
     if (tid < N2) b2s[tid] = a.b2[tid];
@@ -283,25 +290,25 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
                         *reinterpret_cast<const bf16x8 *>(&xs[rr * LK + ch * 8]);
             }
         }
-        // layer 1, computed transposed (Z^T = W1^T X^T): lane (g, c) holds hidden units
-        // ct*16 + 4g .. 4g+3 of row 16w + c, stored as one 8-byte vector
-        f32x4 acc[H / 16];
-        for (int ct = 0; ct < H / 16; ++ct) acc[ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        for (int ks = 0; ks < K1 / 32; ++ks) {
-            const bf16x8 xf = *reinterpret_cast<const bf16x8 *>(&xs[(16 * w + c) * LK + ks * 32 + g * 8]);
-            for (int ct = 0; ct < H / 16; ++ct) {
-                const bf16x8 wf = *reinterpret_cast<const bf16x8 *>(&w1s[(ct * 16 + c) * LK + ks * 32 + g * 8]);
-                acc[ct] = mfma(wf, xf, acc[ct]);
+        // layer 1: hidden units 64cg + 16ct + 4g + j of rows 32rg + 16rt + c
+        for (int rt = 0; rt < 2; ++rt) {
+            const int rr = 32 * rg + 16 * rt + c;
+            f32x4 acc[4];
+            for (int ct = 0; ct < 4; ++ct) acc[ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int ks = 0; ks < KS1; ++ks) {
+                const bf16x8 xf = *reinterpret_cast<const bf16x8 *>(&xs[rr * LK + ks * 32 + g * 8]);
+                for (int ct = 0; ct < 4; ++ct) acc[ct] = mfma(wf1[ct][ks], xf, acc[ct]);
             }
-        }
-        for (int ct = 0; ct < H / 16; ++ct) {
-            const float4 bias = *reinterpret_cast<const float4 *>(&b1s[ct * 16 + g * 4]);
-            bf16x4 v;
-            v[0] = (bf16)fmaxf(acc[ct][0] + bias.x, 0.0f);
-            v[1] = (bf16)fmaxf(acc[ct][1] + bias.y, 0.0f);
-            v[2] = (bf16)fmaxf(acc[ct][2] + bias.z, 0.0f);
-            v[3] = (bf16)fmaxf(acc[ct][3] + bias.w, 0.0f);
-            *reinterpret_cast<bf16x4 *>(&as[(16 * w + c) * LH + ct * 16 + g * 4]) = v;
+            for (int ct = 0; ct < 4; ++ct) {
+                const int col = 64 * cg + 16 * ct + 4 * g;
+                const float4 bias = *reinterpret_cast<const float4 *>(&b1s[col]);
+                bf16x4 v;
+                v[0] = (bf16)fmaxf(acc[ct][0] + bias.x, 0.0f);
+                v[1] = (bf16)fmaxf(acc[ct][1] + bias.y, 0.0f);
+                v[2] = (bf16)fmaxf(acc[ct][2] + bias.z, 0.0f);
+                v[3] = (bf16)fmaxf(acc[ct][3] + bias.w, 0.0f);
+                *reinterpret_cast<bf16x4 *>(&as[rr * LH + col]) = v;
+            }
         }
         __syncthreads();
         if (a.asave) {
@@ -313,43 +320,46 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
                         *reinterpret_cast<const bf16x8 *>(&as[rr * LH + ch * 8]);
             }
         }
-        // layer 2, transposed as well
-        f32x4 acc2[N2 / 16];
-        for (int ct = 0; ct < N2 / 16; ++ct) acc2[ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        for (int ks = 0; ks < H / 32; ++ks) {
-            const bf16x8 af = *reinterpret_cast<const bf16x8 *>(&as[(16 * w + c) * LH + ks * 32 + g * 8]);
-            for (int ct = 0; ct < N2 / 16; ++ct) {
-                const bf16x8 wf = *reinterpret_cast<const bf16x8 *>(&w2s[(ct * 16 + c) * LH + ks * 32 + g * 8]);
-                acc2[ct] = mfma(wf, af, acc2[ct]);
-            }
-        }
-        __syncthreads();
-        const int rr = 16 * w + c;
-        const uint32_t row = r0 + rr;
         if (MODE == M_DEC) {
-            for (int ct = 0; ct < N2 / 16; ++ct) {
-                const float4 bias = *reinterpret_cast<const float4 *>(&b2s[ct * 16 + g * 4]);
-                if (row < a.rows)
-                    *reinterpret_cast<float4 *>(&a.yout[size_t(row) * O + ct * 16 + g * 4]) =
-                        make_float4(acc2[ct][0] + bias.x, acc2[ct][1] + bias.y, acc2[ct][2] + bias.z,
-                                    acc2[ct][3] + bias.w);
-            }
+            // 16 outputs: wave w takes rows 16w..16w+15
+            const int rr = 16 * w + c;
+            f32x4 acc2 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int ks = 0; ks < KS2; ++ks)
+                acc2 = mfma(wf2[0][ks], *reinterpret_cast<const bf16x8 *>(&as[rr * LH + ks * 32 + g * 8]), acc2);
+            const float4 bias = *reinterpret_cast<const float4 *>(&b2s[4 * g]);
+            const uint32_t row = r0 + rr;
+            if (row < a.rows)
+                *reinterpret_cast<float4 *>(&a.yout[size_t(row) * O + 4 * g]) =
+                    make_float4(acc2[0] + bias.x, acc2[1] + bias.y, acc2[2] + bias.z, acc2[3] + bias.w);
         } else {
-            for (int ct = 0; ct < N2 / 16; ++ct) {
-                const int col = ct * 16 + g * 4;
-                const float4 bias = *reinterpret_cast<const float4 *>(&b2s[col]);
-                float v0 = acc2[ct][0] + bias.x, v1 = acc2[ct][1] + bias.y, v2 = acc2[ct][2] + bias.z,
-                      v3 = acc2[ct][3] + bias.w;
-                if (MODE == M_ROUND) {
-                    const bf16x4 hs = *reinterpret_cast<const bf16x4 *>(&xs[rr * LK + col]);
-                    v0 = (float)hs[0] + v0;
-                    v1 = (float)hs[1] + v1;
-                    v2 = (float)hs[2] + v2;
-                    v3 = (float)hs[3] + v3;
+            f32x4 acc2[2][4];
+            for (int rt = 0; rt < 2; ++rt) {
+                const int rr = 32 * rg + 16 * rt + c;
+                for (int ct = 0; ct < 4; ++ct) acc2[rt][ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+                for (int ks = 0; ks < KS2; ++ks) {
+                    const bf16x8 af = *reinterpret_cast<const bf16x8 *>(&as[rr * LH + ks * 32 + g * 8]);
+                    for (int ct = 0; ct < 4; ++ct) acc2[rt][ct] = mfma(wf2[ct][ks], af, acc2[rt][ct]);
                 }
-                bf16x4 o;
-                o[0] = (bf16)v0; o[1] = (bf16)v1; o[2] = (bf16)v2; o[3] = (bf16)v3;
-                *reinterpret_cast<bf16x4 *>(&as[rr * LH + col]) = o;
+            }
+            __syncthreads();  // the partner wave of this row group has read `as`
+            for (int rt = 0; rt < 2; ++rt) {
+                const int rr = 32 * rg + 16 * rt + c;
+                for (int ct = 0; ct < 4; ++ct) {
+                    const int col = 64 * cg + 16 * ct + 4 * g;
+                    const float4 bias = *reinterpret_cast<const float4 *>(&b2s[col]);
+                    float v0 = acc2[rt][ct][0] + bias.x, v1 = acc2[rt][ct][1] + bias.y, v2 = acc2[rt][ct][2] + bias.z,
+                          v3 = acc2[rt][ct][3] + bias.w;
+                    if (MODE == M_ROUND) {
+                        const bf16x4 hs = *reinterpret_cast<const bf16x4 *>(&xs[rr * LK + col]);
+                        v0 = (float)hs[0] + v0;
+                        v1 = (float)hs[1] + v1;
+                        v2 = (float)hs[2] + v2;
+                        v3 = (float)hs[3] + v3;
+                    }
+                    bf16x4 o;
+                    o[0] = (bf16)v0; o[1] = (bf16)v1; o[2] = (bf16)v2; o[3] = (bf16)v3;
+                    *reinterpret_cast<bf16x4 *>(&as[rr * LH + col]) = o;
+                }
             }
             __syncthreads();
             for (int q = tid; q < TR * (H / 8); q += 256) {
@@ -648,7 +658,9 @@ hipError_t fwd_block(hipStream_t st, const NetDev &nd, const Layer &l1, const La
     a.xsave = xsave;
     a.asave = asave;
     const size_t lds = fwd_lds<K1, N2>();
-    hipLaunchKernelGGL((k_mlp_fwd<K1, N2, MODE>), dim3(std::min(a.ntiles, PERSIST)), dim3(256), lds, st, a);
+    // workgroups resident per CU: the round layer holds 256 VGPRs of weights per lane
+    const uint32_t per_cu = MODE == M_ROUND ? 1u : 2u;
+    hipLaunchKernelGGL((k_mlp_fwd<K1, N2, MODE>), dim3(std::min(a.ntiles, PERSIST * per_cu)), dim3(256), lds, st, a);
     return hipGetLastError();
 }
 
@@ -715,34 +727,48 @@ hipError_t launch_copy_pred(hipStream_t st, const float4 *P, float4 *Ppred, uint
     return hipGetLastError();
 }
 hipError_t launch_apply(hipStream_t st, float4 *P, float4 *Ppred, const float4 *D, const float *y, float s,
-                        uint32_t nv) {
+                        float beta, uint32_t nv) {
     if (!nv) return hipSuccess;
-    hipLaunchKernelGGL(k_apply, dim3(cdiv(nv, 256)), dim3(256), 0, st, P, Ppred, D, y, s, nv);
+    hipLaunchKernelGGL(k_apply, dim3(cdiv(nv, 256)), dim3(256), 0, st, P, Ppred, D, y, s, beta, nv);
     return hipGetLastError();
 }
 hipError_t launch_update_prev(hipStream_t st, const float4 *P, const float4 *Ppred, float4 *D, float4 *T, float s,
-                              uint32_t nv) {
+                              float beta, uint32_t nv) {
     if (!nv) return hipSuccess;
-    hipLaunchKernelGGL(k_update_prev, dim3(cdiv(nv, 256)), dim3(256), 0, st, P, Ppred, D, T, s, nv);
+    hipLaunchKernelGGL(k_update_prev, dim3(cdiv(nv, 256)), dim3(256), 0, st, P, Ppred, D, T, s, beta, nv);
     return hipGetLastError();
 }
 
 hipError_t forward(hipStream_t st, const NetLayout &nl, const NetDev &nd, const Graph &g, uint32_t rows,
-                   const FwdBufs &b) {
+                   const FwdBufs &b, hipEvent_t *ev) {
     if (!rows) return hipSuccess;
     hipError_t e;
+    int k = 0;
+    auto mark = [&]() -> hipError_t { return ev ? hipEventRecord(ev[k++], st) : hipSuccess; };
+    if ((e = mark()) != hipSuccess) return e;
     if ((e = fwd_block<F, H, M_ENC>(st, nd, nl.L[nl.enc1()], nl.L[nl.enc2()], rows, b.feat, nullptr, b.h[0], nullptr,
                                     nullptr, b.training ? b.a_enc : nullptr)) != hipSuccess) return e;
+    if ((e = mark()) != hipSuccess) return e;
     for (uint32_t r = 0; r < nl.rounds; ++r) {
         hipLaunchKernelGGL(k_mean_agg, dim3(cdiv(size_t(rows) * (H / 8), 256)), dim3(256), 0, st, rows, b.h[r], g.off,
                            g.nbr, b.agg);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = mark()) != hipSuccess) return e;
         if ((e = fwd_block<2 * H, H, M_ROUND>(st, nd, nl.L[nl.r1(r)], nl.L[nl.r2(r)], rows, b.h[r], b.agg, b.h[r + 1],
                                               nullptr, b.training ? b.x_r[r] : nullptr,
                                               b.training ? b.a_r[r] : nullptr)) != hipSuccess) return e;
+        if ((e = mark()) != hipSuccess) return e;
     }
-    return fwd_block<H, O, M_DEC>(st, nd, nl.L[nl.dec1()], nl.L[nl.dec2()], rows, b.h[nl.rounds], nullptr, nullptr,
-                                  b.y, nullptr, b.training ? b.a_dec : nullptr);
+    if ((e = fwd_block<H, O, M_DEC>(st, nd, nl.L[nl.dec1()], nl.L[nl.dec2()], rows, b.h[nl.rounds], nullptr, nullptr,
+                                    b.y, nullptr, b.training ? b.a_dec : nullptr)) != hipSuccess) return e;
+    return mark();
+}
+
+hipError_t loss_only(hipStream_t st, uint32_t rows, const float *y, const float4 *t, float inv_nfree, float *dy,
+                     double *loss_part) {
+    if (!rows) return hipSuccess;
+    hipLaunchKernelGGL(k_loss, dim3(loss_blocks(rows)), dim3(256), 0, st, rows, y, t, inv_nfree, dy, loss_part);
+    return hipGetLastError();
 }
 
 hipError_t loss_and_backward(hipStream_t st, const NetLayout &nl, const NetDev &nd, const Graph &g, uint32_t rows,

@@ -9,8 +9,11 @@
 //   decoder   y       = W_d2 relu(W_d1 h_R + b_d1) + b_d2          (16 outputs, the first 3 used)
 // Dense layers run on MFMA (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32
 // accumulation); activations are stored in BF16, gradients accumulate in FP32.
-// The correction applied to a free vertex is  D_prev + s * y[0:3],  where
-// D_prev is the correction of the previous substep and s = max(|g|, 1 m/s^2) dt^2.
+// The correction applied to a free vertex is  beta * D_prev + s * y[0:3],  where
+// D_prev is the correction of the previous substep, beta its weight (the history
+// weight) and s = max(|g|, 1 m/s^2) dt^2.  The previous correction also reaches the
+// next substep through the velocity, so beta = 1 feeds slowly corrected error modes
+// back with a gain above one; beta <= 0.5 keeps that loop contracting.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -63,6 +66,7 @@ struct FeatArgs {
     uint32_t nv;
     float s;                // max(|g|, 1) dt^2
     float lscale;           // collider distance scale (mean edge length)
+    float beta;             // history weight: strains are taken at prediction + beta * D
 };
 
 // Graph of the message-passing rounds (stretch edges, both directions).
@@ -70,12 +74,13 @@ struct Graph { const uint32_t *off, *nbr; };
 
 hipError_t launch_features(hipStream_t st, const FeatArgs &a, bf16 *feat);
 hipError_t launch_copy_pred(hipStream_t st, const float4 *P, float4 *Ppred, uint32_t nv);
-// Applies D_prev + s*y (y == nullptr: D_prev only) to the free vertices of P; keeps P in Ppred.
-hipError_t launch_apply(hipStream_t st, float4 *P, float4 *Ppred, const float4 *D, const float *y, float s, uint32_t nv);
+// Applies beta * D_prev + s*y (y == nullptr: history only) to the free vertices of P; keeps P in Ppred.
+hipError_t launch_apply(hipStream_t st, float4 *P, float4 *Ppred, const float4 *D, const float *y, float s,
+                        float beta, uint32_t nv);
 // D = P - Ppred on free vertices (0 on pinned ones); with T != nullptr also writes the
-// training target T = ((P - Ppred - D_old) / s, free ? 1 : 0) before D is replaced.
+// training target T = ((P - Ppred - beta * D_old) / s, free ? 1 : 0) before D is replaced.
 hipError_t launch_update_prev(hipStream_t st, const float4 *P, const float4 *Ppred, float4 *D, float4 *T, float s,
-                              uint32_t nv);
+                              float beta, uint32_t nv);
 
 // Forward of the whole network (inference or training).  Buffers (rows = nv):
 //   feat rows x F;  h[0..R] rows x H (h[r] for training, two ping-pong buffers otherwise);
@@ -91,8 +96,13 @@ struct FwdBufs {
     bf16 *x_r[4], *a_r[4];
     bool training;
 };
+// ev (optional, 2R + 3 events): recorded before the encoder, after it, after every neighbour
+// mean and round, and after the decoder.
 hipError_t forward(hipStream_t st, const NetLayout &nl, const NetDev &nd, const Graph &g, uint32_t rows,
-                   const FwdBufs &b);
+                   const FwdBufs &b, hipEvent_t *ev = nullptr);
+// Loss partials (and dy) without the backward pass.
+hipError_t loss_only(hipStream_t st, uint32_t rows, const float *y, const float4 *t, float inv_nfree, float *dy,
+                     double *loss_part);
 
 // Backward + weight gradients.  t: rows x 4 (target xyz, mask);  n_free: rows with mask 1.
 struct BwdBufs {

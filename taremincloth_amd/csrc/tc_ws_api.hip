@@ -112,6 +112,7 @@ tc::ws::FeatArgs feat_args(const tc_sim *s, float dt_sub) {
     a.nv = s->mesh.nv;
     a.s = ws_scale(s, dt_sub);
     a.lscale = s->ws_lscale;
+    a.beta = s->ws_beta;
     return a;
 }
 
@@ -161,6 +162,12 @@ tc::ws::FwdBufs train_bufs(tc_ws_net *n, const bf16 *feat) {
     b.a_dec = n->a_dec.p;
     for (uint32_t r = 0; r < n->nl.rounds; ++r) { b.x_r[r] = n->x_r[r].p; b.a_r[r] = n->a_r[r].p; }
     b.training = true;
+    return b;
+}
+
+tc::ws::FwdBufs infer_like(tc_ws_net *n, const bf16 *feat) {
+    tc::ws::FwdBufs b = train_bufs(n, feat);
+    b.training = false;
     return b;
 }
 
@@ -224,9 +231,9 @@ hipError_t ws_pre(tc_sim *s, float dt_sub) {
         const tc::ws::Graph g{s->ws_goff.p, s->ws_gnbr.p};
         if ((e = tc::ws::forward(s->stream, s->ws_net->nl, s->ws_net->dev(), g, nv, infer_bufs(s, feat))) != hipSuccess)
             return e;
-        e = tc::ws::launch_apply(s->stream, s->P.p, s->ws_ppred.p, s->ws_D.p, s->ws_y.p, sc, nv);
+        e = tc::ws::launch_apply(s->stream, s->P.p, s->ws_ppred.p, s->ws_D.p, s->ws_y.p, sc, s->ws_beta, nv);
     } else if (active) {
-        e = tc::ws::launch_apply(s->stream, s->P.p, s->ws_ppred.p, s->ws_D.p, nullptr, sc, nv);
+        e = tc::ws::launch_apply(s->stream, s->P.p, s->ws_ppred.p, s->ws_D.p, nullptr, sc, s->ws_beta, nv);
     } else {
         e = tc::ws::launch_copy_pred(s->stream, s->P.p, s->ws_ppred.p, nv);
     }
@@ -243,7 +250,7 @@ hipError_t ws_pre(tc_sim *s, float dt_sub) {
 hipError_t ws_post(tc_sim *s, float dt_sub, const float4 *Pc) {
     const uint32_t nv = s->mesh.nv;
     float4 *T = s->rec_slot >= 0 ? s->rec_T.p + size_t(s->rec_slot) * nv : nullptr;
-    return tc::ws::launch_update_prev(s->stream, Pc, s->ws_ppred.p, s->ws_D.p, T, ws_scale(s, dt_sub), nv);
+    return tc::ws::launch_update_prev(s->stream, Pc, s->ws_ppred.p, s->ws_D.p, T, ws_scale(s, dt_sub), s->ws_beta, nv);
 }
 
 int ws_clear_history(tc_sim *s) {
@@ -385,13 +392,14 @@ int tc_ws_net_adam_step(tc_ws_net *n, float lr) {
 
 // ------------------------------------------------------------------ simulator side
 int tc_sim_set_warm_start(tc_sim *s, tc_ws_net *net, int mode, uint32_t iterations, uint32_t fallback_iterations,
-                          float residual_threshold) {
+                          float residual_threshold, float history_weight) {
     if (!s) return fail(TC_ERR_VALUE, "null simulator");
     if (mode < 0 || mode > 2) return fail(TC_ERR_VALUE, "mode must be 0 (off), 1 (previous correction) or 2 (network)");
     if (mode == 2 && !net) return fail(TC_ERR_VALUE, "mode 2 needs a network");
     if (mode != 0 && iterations == 0) return fail(TC_ERR_VALUE, "the warm-started solve needs at least one iteration");
     if (!(residual_threshold >= 0.0f) || !std::isfinite(residual_threshold))
         return fail(TC_ERR_VALUE, "residual threshold must be finite and >= 0");
+    if (!(history_weight >= 0.0f && history_weight <= 1.0f)) return fail(TC_ERR_VALUE, "history weight must be in [0, 1]");
     if (mode != 0) {
         int r = ws_prepare(s);
         if (r != TC_OK) return r;
@@ -402,6 +410,7 @@ int tc_sim_set_warm_start(tc_sim *s, tc_ws_net *net, int mode, uint32_t iteratio
     s->ws_iters = iterations;
     s->ws_fallback = fallback_iterations + (fallback_iterations & 1u);  // whole pairs: see enqueue_substep
     s->ws_threshold = residual_threshold;
+    s->ws_beta = history_weight;
     if (mode != 0 && !was_active && s->rec_cap == 0) {
         int r = ws_clear_history(s);
         if (r != TC_OK) return r;
@@ -437,9 +446,12 @@ int tc_sim_ws_reset_stats(tc_sim *s) {
     return TC_OK;
 }
 
-int tc_sim_ws_record(tc_sim *s, uint32_t capacity, uint32_t stride) {
+int tc_sim_ws_record(tc_sim *s, uint32_t capacity, uint32_t stride, float history_weight) {
     if (!s) return fail(TC_ERR_VALUE, "null simulator");
     if (stride == 0) return fail(TC_ERR_VALUE, "stride must be >= 1");
+    if (!(history_weight >= 0.0f && history_weight <= 1.0f)) return fail(TC_ERR_VALUE, "history weight must be in [0, 1]");
+    if (s->ws_active() && history_weight != s->ws_beta)
+        return fail(TC_ERR_VALUE, "the history weight must match the active warm start's");
     TC_HIP(hipStreamSynchronize(s->stream));
     s->rec_feat.release();
     s->rec_T.release();
@@ -463,6 +475,7 @@ int tc_sim_ws_record(tc_sim *s, uint32_t capacity, uint32_t stride) {
     }
     s->rec_cap = capacity;
     s->rec_stride = stride;
+    s->ws_beta = history_weight;
     s->touch();
     return TC_OK;
 }
@@ -549,5 +562,140 @@ int tc_sim_ws_history(tc_sim *s, float *get_xyz, const float *set_xyz, uint32_t 
         for (uint32_t i = 0; i < n; ++i) h[i] = make_float4(set_xyz[3 * i], set_xyz[3 * i + 1], set_xyz[3 * i + 2], 0.0f);
         TC_HIP(hipMemcpy(s->ws_D.p, h.data(), 16 * size_t(n), hipMemcpyHostToDevice));
     }
+    return TC_OK;
+}
+
+int tc_ws_eval(tc_ws_net *n, tc_sim *const *sims, uint32_t nsims, double *mean_loss) {
+    if (!n || !mean_loss || (!sims && nsims)) return fail(TC_ERR_VALUE, "null argument");
+    double total = 0.0;
+    uint64_t count = 0;
+    for (uint32_t k = 0; k < nsims; ++k) {
+        tc_sim *s = sims[k];
+        if (!s) return fail(TC_ERR_VALUE, "null simulator in the list");
+        TC_HIP(hipStreamSynchronize(s->stream));
+        uint32_t nfree = 0;
+        for (float w : s->mesh.inv_mass) nfree += w > 0.0f;
+        const uint32_t rows = s->mesh.nv;
+        int r = net_alloc_scratch(n, rows);
+        if (r != TC_OK) return r;
+        const tc::ws::Graph g{s->ws_goff.p, s->ws_gnbr.p};
+        for (uint32_t idx = 0; idx < s->rec_count; ++idx) {
+            const tc::ws::FwdBufs fw = infer_like(n, s->rec_feat.p + size_t(idx) * rows * tc::ws::F);
+            TC_HIP(tc::ws::forward(n->stream, n->nl, n->dev(), g, rows, fw));
+            TC_HIP(tc::ws::loss_only(n->stream, rows, n->y.p, s->rec_T.p + size_t(idx) * rows,
+                                     1.0f / float(std::max<uint32_t>(nfree, 1)), n->dy.p, n->loss_part.p));
+            hipError_t le = hipSuccess;
+            total += read_loss(n, rows, &le);
+            TC_HIP(le);
+            ++count;
+        }
+    }
+    *mean_loss = count ? total / double(count) : 0.0;
+    return TC_OK;
+}
+
+int tc_sim_ws_time(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
+    if (!s || !out) return fail(TC_ERR_VALUE, "null argument");
+    for (int k = 0; k < 16; ++k) out[k] = 0.0;
+    if (!s->ws_active()) return fail(TC_ERR_VALUE, "warm start is not enabled on this simulator");
+    if (s->mesh.nv == 0 || reps == 0) return TC_OK;
+    if (s->self_collision && s->normal_untangling && s->mesh.has_multiple_layers)
+        return fail(TC_ERR_UNSUPPORTED, "multi-layer cloth with normal untangling is not part of this build");
+    const tc::Mesh &m = s->mesh;
+    const uint32_t nv = m.nv, B = s->block, R = s->ws_net ? s->ws_net->nl.rounds : 0;
+    const bool net = s->ws_mode == 2;
+    std::vector<hipEvent_t> ev(32, nullptr);
+    for (auto &e : ev) TC_HIP(hipEventCreate(&e));
+    struct Guard {
+        std::vector<hipEvent_t> &v;
+        ~Guard() { for (auto e : v) if (e) (void)hipEventDestroy(e); }
+    } guard{ev};
+    const float sc = ws_scale(s, dt_sub);
+    double acc[16] = {0};
+    s->rec_slot = -1;
+    for (uint32_t rep = 0; rep < reps; ++rep) {
+        const tc::SubstepArgs a = s->args(dt_sub);
+        TC_HIP(hipEventRecord(ev[0], s->stream));
+        TC_HIP(tc::launch_predict(s->stream, s->P.p, s->X.p, s->V.p, nv, a, B));
+        TC_HIP(hipEventRecord(ev[1], s->stream));
+        if (net) {
+            TC_HIP(tc::ws::launch_features(s->stream, feat_args(s, dt_sub), s->ws_feat.p));
+            TC_HIP(hipEventRecord(ev[2], s->stream));
+            const tc::ws::Graph g{s->ws_goff.p, s->ws_gnbr.p};
+            TC_HIP(tc::ws::forward(s->stream, s->ws_net->nl, s->ws_net->dev(), g, nv, infer_bufs(s, s->ws_feat.p),
+                                   &ev[3]));  // ev[3 .. 3 + 2R + 2]
+        } else {
+            TC_HIP(hipEventRecord(ev[2], s->stream));
+        }
+        const int fe = net ? 3 + 2 * int(R) + 2 : 2;  // event after the network
+        TC_HIP(tc::ws::launch_apply(s->stream, s->P.p, s->ws_ppred.p, s->ws_D.p, net ? s->ws_y.p : nullptr, sc,
+                                    s->ws_beta, nv));
+        TC_HIP(hipEventRecord(ev[20], s->stream));
+        uint32_t pc = 0, lc = 0;
+        TC_HIP(enqueue_iterations(s, a, s->ws_iters, true, pc, lc));
+        TC_HIP(hipEventRecord(ev[21], s->stream));
+        if (s->ws_fallback > 0) {
+            TC_HIP(tc::launch_residual(s->stream, s->Pbuf(pc), s->drec.p, s->dcc.p, s->dlam_buf(lc),
+                                       uint32_t(m.d_v0.size()), s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()),
+                                       dt_sub, s->split_compliance, s->resid_part.p, s->ws_res.p, s->ws_gate.p,
+                                       s->ws_threshold));
+        }
+        TC_HIP(hipEventRecord(ev[22], s->stream));
+        if (s->ws_fallback > 0) {
+            tc::SubstepArgs fa = a;
+            fa.run_flag = s->ws_gate.p;
+            TC_HIP(enqueue_iterations(s, fa, s->ws_fallback, false, pc, lc));
+        }
+        TC_HIP(hipEventRecord(ev[23], s->stream));
+        float4 *Pc = s->Pbuf(pc);
+        s->last_p = pc;
+        s->last_lam = lc;
+        TC_HIP(tc::ws::launch_update_prev(s->stream, Pc, s->ws_ppred.p, s->ws_D.p, nullptr, sc, s->ws_beta, nv));
+        TC_HIP(hipEventRecord(ev[24], s->stream));
+        if (!s->cols_h.empty() || !s->tris_h.empty())
+            TC_HIP(tc::launch_collision(s->stream, Pc, s->X.p, nv, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
+                                        s->tri_bounds.p, uint32_t(s->tris_h.size()), s->single_sided_recovery, dt_sub,
+                                        B));
+        if (s->self_collision) {
+            TC_HIP(tc::launch_hash_vertices(s->stream, Pc, nv, s->cell_size, s->table_size, s->vhash()));
+            TC_HIP(tc::launch_hash_edges(s->stream, Pc, s->edges.p, uint32_t(m.e_v0.size()), s->cell_size,
+                                         s->table_size, s->ehash()));
+            const tc::SelfColArgs sa = s->sc_args();
+            const tc::SelfColTopo t = s->topo();
+            TC_HIP(tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa, B));
+            TC_HIP(tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa, B));
+            TC_HIP(tc::launch_sc_apply(s->stream, Pc, s->local_len.p, s->accum.p, sa, B));
+        }
+        TC_HIP(hipEventRecord(ev[25], s->stream));
+        TC_HIP(tc::launch_update_vel(s->stream, Pc, s->X.p, s->V.p, nv, dt_sub, B));
+        TC_HIP(hipEventRecord(ev[26], s->stream));
+        TC_HIP(hipStreamSynchronize(s->stream));
+        auto el = [&](int i, int j) -> double {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[i], ev[j]);
+            return ms;
+        };
+        acc[0] += el(0, 1);
+        acc[1] += el(1, 2);
+        if (net) {
+            acc[2] += el(3, 4);
+            for (uint32_t r = 0; r < R; ++r) {
+                acc[3] += el(4 + 2 * r, 5 + 2 * r);
+                acc[4] += el(5 + 2 * r, 6 + 2 * r);
+            }
+            acc[5] += el(4 + 2 * R, 5 + 2 * R);
+        }
+        acc[6] += el(fe, 20);
+        acc[7] += el(20, 21);
+        acc[8] += el(21, 22);
+        acc[9] += el(22, 23);
+        acc[10] += el(23, 24);
+        acc[11] += el(24, 25);
+        acc[12] += el(25, 26);
+        acc[13] += el(0, 26);
+    }
+    for (int k = 0; k < 14; ++k) out[k] = acc[k] / double(reps);
+    out[14] = R;
+    out[15] = s->ws_iters;
     return TC_OK;
 }

@@ -304,12 +304,16 @@ class ClothSimulator:
         _lib.check(self._L.tc_sim_synchronize(self._h))
 
     # --- learned warm start (no reference counterpart; see taremincloth_amd.warmstart) -------
-    def set_warm_start(self, net=None, mode=0, iterations=8, fallback_iterations=56, residual_threshold=1e-4):
-        """mode 0 off, 1 previous correction, 2 network (`net`); the network must outlive its use here."""
+    def set_warm_start(self, net=None, mode=0, iterations=8, fallback_iterations=56, residual_threshold=1e-4,
+                       history_weight=None):
+        """mode 0 off, 1 previous correction, 2 network (`net`).  history_weight defaults to the
+        network's training value (0.5 without a network)."""
+        if history_weight is None:
+            history_weight = getattr(net, "history_weight", 0.5) if net is not None else 0.5
         self._ws_net = net  # keep the network alive while it is attached
         _lib.check(self._L.tc_sim_set_warm_start(self._h, None if net is None else net.handle, int(mode),
                                                  int(iterations), int(fallback_iterations),
-                                                 float(residual_threshold)))
+                                                 float(residual_threshold), float(history_weight)))
 
     def warm_start_stats(self) -> dict:
         out = (C.c_double * 6)()
@@ -320,8 +324,8 @@ class ClothSimulator:
     def ws_reset_stats(self):
         _lib.check(self._L.tc_sim_ws_reset_stats(self._h))
 
-    def ws_record(self, capacity, stride=1):
-        _lib.check(self._L.tc_sim_ws_record(self._h, int(capacity), int(stride)))
+    def ws_record(self, capacity, stride=1, history_weight=0.5):
+        _lib.check(self._L.tc_sim_ws_record(self._h, int(capacity), int(stride), float(history_weight)))
 
     def ws_num_records(self) -> int:
         return int(self._L.tc_sim_ws_num_records(self._h))
@@ -338,6 +342,15 @@ class ClothSimulator:
         y = np.empty((self._n, 16), np.float32)
         _lib.check(self._L.tc_sim_ws_last_output(self._h, y.ctypes.data_as(C.c_void_p), y.size))
         return y
+
+    def ws_time(self, dt_sub, reps=3) -> dict:
+        """Per-part times (ms) of a warm-started substep from HIP events; advances the simulation."""
+        out = (C.c_double * 16)()
+        _lib.check(self._L.tc_sim_ws_time(self._h, float(dt_sub), int(reps), out))
+        keys = ["predict", "features", "encoder", "neighbour_means", "round_mlps", "decoder", "apply",
+                "iterations", "gate", "fallback", "correction_update", "collide", "update_vel", "substep",
+                "rounds", "warm_iterations"]
+        return dict(zip(keys, list(out)))
 
     def ws_history(self, new=None) -> np.ndarray:
         """Previous-substep correction (N x 3); with `new`, replaces it and returns the old one."""

@@ -5,15 +5,19 @@ loop of the reference substep (crates/cloth_core/src/simulation/dispatch.rs:137
 -> 169).  A message-passing network over the cloth's stretch-edge graph (hidden
 width 128, `rounds` rounds of two-layer MLPs, BF16 MFMA with FP32 accumulation)
 predicts every free vertex's position correction for the coming substep, in
-units of s = max(|g|, 1) dt^2, relative to the previous substep's correction.
+units of s = max(|g|, 1) dt^2, relative to w times the previous substep's
+correction (w = history_weight, 0.5 by default: the previous correction also
+reaches the substep through the velocity, and a larger w amplifies the error
+modes that a few iterations correct slowly).
 Plain XPBD iterations always follow; a residual gate adds plain iterations when
 the relative residual after them stays above a threshold.
 
     net = WarmStartNet(rounds=3); net.init(seed=1)
-    sim.ws_record(capacity=400, stride=2); sim.step(...)      # plain 64-iteration solves
+    sim.ws_record(capacity=400, stride=2, history_weight=net.history_weight)
+    sim.step(...)                                             # plain 64-iteration solves
     net.train([sim], steps=2000, lr=1e-3)
     sim.ws_record(0); sim.set_warm_start(net, mode=2, iterations=8, fallback_iterations=56,
-                                         residual_threshold=1e-4)
+                                         residual_threshold=5e-3)
 """
 from __future__ import annotations
 
@@ -32,12 +36,13 @@ OFF, PREVIOUS, NETWORK = 0, 1, 2
 class WarmStartNet:
     """Device-resident weights, Adam state and training scratch of one network."""
 
-    def __init__(self, rounds: int = 3):
+    def __init__(self, rounds: int = 3, history_weight: float = 0.5):
         self._L = _lib.load()
         h = C.c_void_p()
         _lib.check(self._L.tc_ws_net_create(int(rounds), C.byref(h)))
         self._h = h
         self.rounds = int(rounds)
+        self.history_weight = float(history_weight)  # weight of the previous correction it was trained with
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
@@ -64,13 +69,15 @@ class WarmStartNet:
         p = np.ascontiguousarray(params, np.float32).reshape(-1)
         _lib.check(self._L.tc_ws_net_set_params(self._h, p.ctypes.data_as(C.c_void_p), p.size))
 
-    def save(self, path):
-        np.savez(path, params=self.get_params(), rounds=np.int64(self.rounds))
+    def save(self, path, **extra):
+        np.savez(path, params=self.get_params(), rounds=np.int64(self.rounds),
+                 history_weight=np.float64(self.history_weight), **extra)
 
     @classmethod
     def load(cls, path) -> "WarmStartNet":
         with np.load(path, allow_pickle=False) as z:
-            net = cls(int(z["rounds"]))
+            hw = float(z["history_weight"]) if "history_weight" in z.files else 1.0
+            net = cls(int(z["rounds"]), hw)
             net.set_params(z["params"])
         return net
 
@@ -101,3 +108,10 @@ class WarmStartNet:
         _lib.check(self._L.tc_ws_train(self._h, hs, len(sims), int(steps), float(lr), C.c_uint64(seed),
                                        losses.ctypes.data_as(C.c_void_p)))
         return losses
+
+    def eval(self, sims) -> float:
+        """Mean loss over every recorded sample of the simulators (no update)."""
+        hs = (C.c_void_p * len(sims))(*[s._h for s in sims])
+        out = C.c_double(0.0)
+        _lib.check(self._L.tc_ws_eval(self._h, hs, len(sims), C.byref(out)))
+        return out.value

@@ -161,7 +161,7 @@ def test_gpu_recorded_features_and_targets_match_oracle(require_gpu):
         err = np.abs(r["features"] - want)
         assert (err <= 1e-2 * np.maximum(np.abs(want), 1.0)).all(), err.max()
     # the second sample's previous correction is the first sample's correction
-    prev1 = r0["previous"] + r0["target"][:, :3] * s
+    prev1 = 0.5 * r0["previous"] + r0["target"][:, :3] * s
     assert np.abs(r1["previous"] - prev1).max() < 1e-7
     assert (r0["target"][:, 3] == (inv > 0)).all()
 
@@ -178,7 +178,7 @@ def test_gpu_gate_fallback_equals_plain_iterations(require_gpu):
         sim, pos, *_ = _drape()
         sim.set_solver_iterations(its)
         if mode:
-            sim.set_warm_start(None, mode, its, 6, th)
+            sim.set_warm_start(None, mode, its, 6, th, history_weight=0.5)
         sim.step_single_substep(dt_sub)
         out = np.zeros(pos.size, np.float32)
         sim.get_positions(out)
@@ -235,7 +235,7 @@ def test_gpu_network_inside_the_substep_matches_the_standalone_pass(require_gpu)
     p[-(128 * 16 + 16):] = np.random.default_rng(0).normal(0, 0.05, 128 * 16 + 16)  # non-zero last layer
     net.set_params(p)
     sim.ws_record(1, 1)
-    sim.set_warm_start(net, 2, 8, 0, 1e9)
+    sim.set_warm_start(net, 2, 8, 0, 1e9, history_weight=0.5)
     sim.step_single_substep(1.0 / 600.0)
     r = sim.ws_get_record(0)
     y_sim = sim.ws_last_output()
