@@ -82,6 +82,17 @@ int tc_sim_add_capsule(tc_sim *sim, const float a[3], const float b[3], float ra
                        float restitution);
 int tc_sim_add_plane(tc_sim *sim, const float point[3], const float normal[3], float friction, float restitution);
 int tc_sim_clear_colliders(tc_sim *sim);
+/* ClothSimulator.set_bone_sdf_colliders(width, height, depth, texture_bytes, bone_infos),
+ * lib.rs:2245-2284 -> simulation/mod.rs:1630-1727: an Rg16Float volume (4 bytes per texel,
+ * pack2x16float(distance, weight), Z/Y/X order) and nb rows of 20 floats
+ * [aabb_min(3), centre.x, aabb_max(3), centre.y | width, uvw_scale(3), centre.z, uvw_offset(3),
+ *  radius, friction, thickness, restitution, weight threshold].  Transforms start as identity.
+ * Sampled trilinearly in the collider resolve after the primitives and the mesh triangles. */
+int tc_sim_set_sdf_colliders(tc_sim *sim, uint32_t width, uint32_t height, uint32_t depth, const void *texels,
+                             size_t nbytes, const float *infos, uint32_t nb);
+/* ClothSimulator.update_bone_transforms(world, inv_world), lib.rs:2287-2330: n 4x4 matrices each,
+ * row-major as given (the reference uploads them unchanged); n must equal the collider count. */
+int tc_sim_update_bone_transforms(tc_sim *sim, const float *world, const float *inv_world, uint32_t n);
 
 /* set_mesh_collider_triangles(tris f32[T,3,3], friction, thickness, restitution, single_sided, attributes=None),
  * lib.rs:2175-2233 -> upload_colliders (simulation/mod.rs:2582-2765). attrs: NULL or T*4 floats
@@ -237,6 +248,27 @@ int tc_sim_ws_history(tc_sim *sim, float *get_xyz, const float *set_xyz, uint32_
  * [11] collider resolve + self-collision [12] update_vel [13] whole substep [14] rounds
  * [15] warm-start iterations.  The simulation advances by `reps` substeps. */
 int tc_sim_ws_time(tc_sim *sim, float dt_sub, uint32_t reps, double *out);
+
+/* ---- Mesh SDF bake (module functions bake_mesh_sdf_gpu / bake_mesh_sdf_hierarchical_gpu,
+ * lib.rs:1350-1485 -> sdf_baker.rs:1114-1323, 1625-1882).  method 0: dense bake, 1: hierarchical
+ * (coarse top-8 pass + fine pass), 2: the coarse pass alone (bake_mesh_sdf_coarse_ext,
+ * sdf_baker.rs:1350-1600: cell width 4 x voxel_size, 8 nearest triangle indices per cell).
+ * The result holds width x height x depth texels (pack2x16float(signed distance, 1.0), Z/Y/X
+ * order), the 20-float collider info and, for method 2, the nearest-triangle indices. */
+typedef struct tc_sdf_result tc_sdf_result;
+int tc_bake_mesh_sdf(const float *verts, uint32_t num_verts, const int32_t *tris, uint32_t num_tris, float voxel_size,
+                     float margin, float friction, float thickness, float restitution, int close_open_boundaries,
+                     int method, tc_sdf_result **out);
+/* dims[3], bone_info[20], coarse cell width (method 2); each may be NULL. */
+int tc_sdf_result_info(const tc_sdf_result *res, uint32_t *dims, float *bone_info, float *coarse_voxel);
+const uint32_t *tc_sdf_result_texels(const tc_sdf_result *res);
+const uint32_t *tc_sdf_result_nearest(const tc_sdf_result *res);
+void tc_sdf_result_free(tc_sdf_result *res);
+/* close_open_boundary_loops_mesh (sdf_baker.rs:455-462): caps the closed boundary loops of a solid
+ * mesh.  With NULL outputs only the counts are written. */
+int tc_close_open_boundary_loops(const float *verts, uint32_t num_verts, const int32_t *tris, uint32_t num_tris,
+                                 float *out_verts, size_t cap_verts, int32_t *out_tris, size_t cap_tris,
+                                 uint32_t *num_verts_out, uint32_t *num_tris_out, uint32_t *loops_closed);
 
 /* Host-only mesh build (no device work; usable without a GPU): builds the constraints, colouring
  * and self-collision topology exactly as tc_sim_create does and returns array `which`:

@@ -21,6 +21,7 @@ _LIBS: dict[str, C.CDLL] = {}
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
 _vp = C.c_void_p
 
 # orc_mesh_get field ids (xpbd_oracle.c)
@@ -91,6 +92,21 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.orc_hash_points.argtypes = [_f32p, C.c_uint32, C.c_float, C.c_uint32, _u32p, _u32p, _u32p]
     L.orc_real_bytes.restype = C.c_int
     L.orc_sim_stats.argtypes = [_vp, np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS"), C.c_int]
+    # SDF bake and SDF colliders (sdf_oracle.c)
+    L.orc_sdf_bake.restype = _vp
+    L.orc_sdf_bake.argtypes = [_f32p, C.c_uint32, _vp, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float,
+                               C.c_float, C.c_int, C.c_int]
+    L.orc_sdf_info.argtypes = [_vp, _u32p, _f32p, C.POINTER(C.c_float)]
+    L.orc_sdf_texels.argtypes = [_vp, _u32p]
+    L.orc_sdf_nearest.argtypes = [_vp, _u32p]
+    L.orc_sdf_free.argtypes = [_vp]
+    L.orc_f32_to_f16_n.argtypes = [_f32p, _u16p, C.c_size_t]
+    L.orc_f16_to_f32_n.argtypes = [_u16p, _f32p, C.c_size_t]
+    L.orc_close_loops_copy.restype = C.c_uint32
+    L.orc_close_loops_copy.argtypes = [_f32p, C.c_uint32, _vp, C.c_uint32, _vp, C.c_size_t, _vp, C.c_size_t,
+                                       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    L.orc_sim_set_sdf.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _u32p, _f32p, C.c_uint32]
+    L.orc_sim_set_bone_transforms.argtypes = [_vp, _f32p, _f32p, C.c_uint32]
     _LIBS[precision] = L
     return L
 
@@ -239,6 +255,16 @@ class OracleSim:
         self._L.orc_sim_set_mesh_triangles(self._h, t, len(t), _ptr(at), friction, thickness, restitution,
                                            1 if single_sided else 0)
 
+    def set_bone_sdf_colliders(self, width, height, depth, texture_bytes, bone_infos):
+        tex = np.frombuffer(bytes(texture_bytes), dtype=np.uint32).copy()
+        infos = np.ascontiguousarray(np.asarray(bone_infos, np.float32)[:, :20])
+        self._L.orc_sim_set_sdf(self._h, width, height, depth, tex, infos, len(infos))
+
+    def update_bone_transforms(self, world_matrices, inv_world_matrices):
+        w = np.ascontiguousarray(world_matrices, np.float32)
+        iw = np.ascontiguousarray(inv_world_matrices, np.float32)
+        self._L.orc_sim_set_bone_transforms(self._h, w, iw, len(w))
+
     def reset(self):
         self._L.orc_sim_reset(self._h)
 
@@ -285,3 +311,63 @@ class OracleSim:
             out[: x.size] = x
             return out
         return x
+
+
+def f32_to_f16_bits(x, precision: str = "f32") -> np.ndarray:
+    """binary16 bits of float32 values, round to nearest even (sdf_oracle.c)."""
+    a = np.ascontiguousarray(x, np.float32).reshape(-1)
+    out = np.empty(a.size, np.uint16)
+    lib(precision).orc_f32_to_f16_n(a, out, a.size)
+    return out
+
+
+def f16_bits_to_f32(h, precision: str = "f32") -> np.ndarray:
+    a = np.ascontiguousarray(h, np.uint16).reshape(-1)
+    out = np.empty(a.size, np.float32)
+    lib(precision).orc_f16_to_f32_n(a, out, a.size)
+    return out
+
+
+def close_open_boundary_loops(verts, tris, precision: str = "f32"):
+    """sdf_baker.rs:244-462 restated: (verts, tris, loops closed)."""
+    L = lib(precision)
+    v = np.ascontiguousarray(np.asarray(verts, np.float32).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(tris, np.int32).reshape(-1, 3))
+    nv, nt = C.c_uint32(), C.c_uint32()
+    L.orc_close_loops_copy(v.reshape(-1), len(v), _ptr(t), len(t), None, 0, None, 0, C.byref(nv), C.byref(nt))
+    ov = np.empty((nv.value, 3), np.float32)
+    ot = np.empty((nt.value, 3), np.int32)
+    loops = L.orc_close_loops_copy(v.reshape(-1), len(v), _ptr(t), len(t), _ptr(ov), ov.size, _ptr(ot), ot.size,
+                                   C.byref(nv), C.byref(nt))
+    return ov, ot, int(loops)
+
+
+def bake_mesh_sdf(verts, tris, voxel_size=0.004, margin=0.02, friction=0.5, thickness=0.005, restitution=0.0,
+                  close_open_boundaries=True, method=0, precision: str = "f32"):
+    """Brute-force restatement of the reference bakes (method 0 dense, 1 hierarchical, 2 coarse pass);
+    returns the same dict as the product's bake functions, or None where the reference bake fails."""
+    L = lib(precision)
+    v = np.ascontiguousarray(np.asarray(verts, np.float32).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(tris, np.int32).reshape(-1, 3))
+    h = L.orc_sdf_bake(v.reshape(-1), len(v), _ptr(t), len(t), voxel_size, margin, friction, thickness,
+                       restitution, 1 if close_open_boundaries else 0, method)
+    if not h:
+        return None
+    try:
+        dims = np.zeros(3, np.uint32)
+        info = np.zeros(20, np.float32)
+        cv = C.c_float()
+        L.orc_sdf_info(h, dims, info, C.byref(cv))
+        n = int(dims[0]) * int(dims[1]) * int(dims[2])
+        tex = np.empty(n, np.uint32)
+        L.orc_sdf_texels(h, tex)
+        out = {"texture_bytes": tex.tobytes(), "width": int(dims[0]), "height": int(dims[1]),
+               "depth": int(dims[2]), "bone_info": info, "texels": tex}
+        if method == 2:
+            near = np.empty(n * 8, np.uint32)
+            L.orc_sdf_nearest(h, near)
+            out["nearest_tri"] = near
+            out["coarse_voxel"] = float(cv.value)
+        return out
+    finally:
+        L.orc_sdf_free(h)

@@ -1031,6 +1031,8 @@ typedef struct {
     real relief_factor, max_disp_ratio;
     int enable_relief, exclude_neighbors;
     uint64_t stats[6]; /* contact counters: vv, vt_ccd, vt_prox, ee_ccd, ee_prox, hash_pairs */
+    uint32_t *sdf_tex; float *sdf_info, *sdf_xf;     /* SDF colliders (sdf_oracle.c) */
+    uint32_t sdf_w, sdf_h, sdf_d, sdf_nb;
 } orc_sim;
 
 orc_sim *orc_sim_create(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
@@ -1082,6 +1084,7 @@ void orc_sim_free(orc_sim *s) {
     free(s->x); free(s->p); free(s->v); free(s->x0); free(s->w); free(s->w0); free(s->lam_d); free(s->lam_b);
     free(s->d_tc); free(s->d_cc); free(s->b_comp); free(s->cols); free(s->acc); free(s->tris); free(s->tri_bounds);
     free(s->hkeys); free(s->hstarts); free(s->hsorted); free(s->ekeys); free(s->estarts); free(s->esorted);
+    free(s->sdf_tex); free(s->sdf_info); free(s->sdf_xf);
     free(s);
 }
 
@@ -1126,6 +1129,31 @@ void orc_sim_add_plane(orc_sim *s, const float *pt, const float *n, float fr, fl
     collider k = {2, fr, rest, mk(pt[0], pt[1], pt[2]), 0, mk(n[0], n[1], n[2])}; add_col(s, k);
 }
 void orc_sim_clear_colliders(orc_sim *s) { s->ncols = 0; s->ntris = 0; }
+
+/* set_bone_sdf_colliders (lib.rs:2235-2285): texels as pack2x16float words, Z, Y, X order; 20
+ * floats of collider info each; transforms start as identity (simulation/mod.rs:1700-1719). */
+void orc_sim_set_sdf(orc_sim *s, uint32_t w, uint32_t h, uint32_t d, const uint32_t *tex, const float *info, uint32_t nb) {
+    free(s->sdf_tex); free(s->sdf_info); free(s->sdf_xf);
+    s->sdf_tex = NULL; s->sdf_info = NULL; s->sdf_xf = NULL; s->sdf_nb = 0;
+    if (!nb) return;
+    const size_t nvox = (size_t)w * h * d;
+    s->sdf_tex = (uint32_t *)malloc(4 * nvox);
+    memcpy(s->sdf_tex, tex, 4 * nvox);
+    s->sdf_info = (float *)malloc(sizeof(float) * 20 * nb);
+    memcpy(s->sdf_info, info, sizeof(float) * 20 * nb);
+    s->sdf_xf = (float *)calloc((size_t)32 * nb, sizeof(float));
+    for (uint32_t b = 0; b < nb; b++)
+        for (int m = 0; m < 2; m++)
+            for (int k = 0; k < 4; k++) s->sdf_xf[(size_t)b * 32 + m * 16 + k * 5] = 1.0f;
+    s->sdf_w = w; s->sdf_h = h; s->sdf_d = d; s->sdf_nb = nb;
+}
+/* update_bone_transforms (lib.rs:2287-2330): world and inverse-world 4x4 matrices, uploaded as given */
+void orc_sim_set_bone_transforms(orc_sim *s, const float *world, const float *inv, uint32_t n) {
+    for (uint32_t b = 0; b < n && b < s->sdf_nb; b++) {
+        memcpy(&s->sdf_xf[(size_t)b * 32], world + (size_t)b * 16, 64);
+        memcpy(&s->sdf_xf[(size_t)b * 32 + 16], inv + (size_t)b * 16, 64);
+    }
+}
 
 typedef struct { uint32_t code, idx; } tkey;
 static int cmp_tkey(const void *a, const void *b) {
@@ -1320,7 +1348,11 @@ static void contact_response(v3 target, v3 n, real fr, real rest, real dt, v3 p_
     *xn = add(*xn, mul(n, dxn));
 }
 
-/* collision.wgsl:377-652 (primitive colliders; mesh/SDF handled elsewhere) */
+int orc_sdf_contact(const uint32_t *tex, uint32_t w, uint32_t h, uint32_t d, const float *info, const float *xf,
+                    uint32_t nb, uint32_t ntris_mesh, const real *p3, real vth, real *target3, real *normal3,
+                    real *fr, real *rest);
+
+/* collision.wgsl:377-652: primitives, mesh triangles, then SDF colliders (sdf_oracle.c) */
 static void k_collision(orc_sim *s, real dt) {
     for (uint32_t i = 0; i < s->nv; i++) {
         if (s->w[i] <= 0) continue;
@@ -1413,6 +1445,13 @@ static void k_collision(orc_sim *s, real dt) {
             }
         }
         if (!front && has_rec) contact_response(best_pos, best_n, best_fr, best_rest, dt, pinit, &xn, &p);
+        /* SDF colliders last, velocity reference = the position at this point (collision.wgsl:645-647) */
+        if (s->sdf_nb) {
+            real pa[3] = {p.x, p.y, p.z}, tg[3], nn[3], fr, rest;
+            if (orc_sdf_contact(s->sdf_tex, s->sdf_w, s->sdf_h, s->sdf_d, s->sdf_info, s->sdf_xf, s->sdf_nb, s->ntris,
+                                pa, th, tg, nn, &fr, &rest))
+                contact_response(mk(tg[0], tg[1], tg[2]), mk(nn[0], nn[1], nn[2]), fr, rest, dt, p, &xn, &p);
+        }
         s->p[i] = p;
         s->x[i] = xn;
     }
@@ -2046,7 +2085,7 @@ static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
             for (uint32_t c = 0; c < m->nb_col; c++)
                 for (uint32_t k = m->b_col_off[c]; k < m->b_col_off[c] + m->b_col_cnt[c]; k++) k_bending_one(s, k, dt_sub);
         }
-        if (s->ncols > 0 || s->ntris > 0) k_collision(s, dt_sub);
+        if (s->ncols > 0 || s->ntris > 0 || s->sdf_nb > 0) k_collision(s, dt_sub);
         if (s->self_collision) self_collision_pass(s);
         k_update_vel(s, dt_sub);
     }
@@ -2074,7 +2113,7 @@ void orc_sim_substep_parts(orc_sim *s, float dt_sub, int do_collide, int do_self
         for (uint32_t c = 0; c < m->nb_col; c++)
             for (uint32_t k = m->b_col_off[c]; k < m->b_col_off[c] + m->b_col_cnt[c]; k++) k_bending_one(s, k, dt);
     }
-    if (do_collide && (s->ncols > 0 || s->ntris > 0)) k_collision(s, dt);
+    if (do_collide && (s->ncols > 0 || s->ntris > 0 || s->sdf_nb > 0)) k_collision(s, dt);
     if (do_self && s->self_collision) self_collision_pass(s);
     if (do_update) k_update_vel(s, dt);
 }

@@ -28,9 +28,20 @@ hipError_t launch_distance(hipStream_t s, float4 *P, const float4 *X, const uint
                            uint32_t block);
 hipError_t launch_bending(hipStream_t s, float4 *P, const float4 *X, const uint4 *rec, float *lam, uint32_t off,
                           uint32_t cnt, const SubstepArgs &a, bool first, uint32_t block);
+// SDF colliders (set_bone_sdf_colliders, crates/cloth_core/src/simulation/mod.rs:1630-1727): an
+// Rg16Float volume (pack2x16float(distance, weight) per texel, Z/Y/X order), 20 floats of
+// GpuBoneInfo per collider and its world / inverse-world matrices (16 + 16 floats, each given
+// as matrix[row][column] the way the reference uploads them).
+struct SdfArgs {
+    const uint32_t *tex = nullptr;
+    uint32_t w = 0, h = 0, d = 0;
+    const float *info = nullptr;
+    const float *xf = nullptr;
+    uint32_t nb = 0;
+};
 hipError_t launch_collision(hipStream_t s, float4 *P, float4 *X, uint32_t n, const Collider *cols, uint32_t ncols,
                             const MeshTri *tris, const float4 *bounds, uint32_t ntris, int single_sided_recovery,
-                            float dt, uint32_t block);
+                            float dt, uint32_t block, const SdfArgs &sdf);
 hipError_t launch_update_vel(hipStream_t s, const float4 *P, float4 *X, float4 *V, uint32_t n, float dt,
                              uint32_t block);
 hipError_t launch_extract_positions(hipStream_t s, const float4 *X, float *out, uint32_t n);

@@ -1,0 +1,78 @@
+// Mesh SDF bake: host-side preparation (domain, 20-float collider info, bake
+// triangles, open-boundary closing) and the device bakes (tc_sdf_bake.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace tc {
+namespace sdf {
+
+// Bake triangle (GpuBakeMeshTriangle, sdf_baker.rs): three corners and the unit face normal.
+struct BakeTri {
+    float p0x, p0y, p0z, pad0;
+    float p1x, p1y, p1z, pad1;
+    float p2x, p2y, p2z, pad2;
+    float nx, ny, nz, pad3;
+};
+
+struct Domain {
+    float min[3], max[3], size[3];
+    uint32_t w, h, d;
+};
+
+// close_open_boundary_loops_mesh (sdf_baker.rs:244-462): caps closed boundary loops of a solid mesh.
+// Returns the number of loops closed.
+size_t close_open_boundary_loops(std::vector<float> &verts, std::vector<int32_t> &tris);
+// mesh_sdf_domain (sdf_baker.rs:939-980)
+Domain mesh_sdf_domain(const std::vector<float> &verts, float voxel_size, float margin, float thickness);
+// mesh_sdf_bone_info (sdf_baker.rs:984-1013)
+void mesh_sdf_bone_info(const Domain &dm, float friction, float thickness, float restitution, float out[20]);
+// collect_mesh_bake_triangles (sdf_baker.rs:1016-1065): invalid indices skipped, dummy when empty.
+std::vector<BakeTri> collect_bake_triangles(const std::vector<float> &verts, const std::vector<int32_t> &tris);
+
+// Triangles in Morton order of their centroids, grouped into clusters of up to CLUSTER
+// consecutive triangles with an axis-aligned box each (the culling hierarchy of the dense bake).
+constexpr uint32_t CLUSTER = 32;
+struct Clusters {
+    std::vector<BakeTri> tris;      // Morton order
+    std::vector<uint32_t> index;    // original index of each sorted triangle
+    std::vector<float4> lo, hi;     // per cluster box (w unused)
+    std::vector<uint32_t> code;     // Morton code of each cluster's first triangle
+    float origin[3], scale;         // Morton quantisation of the domain
+};
+Clusters build_clusters(const std::vector<BakeTri> &tris, const Domain &dm);
+
+// Device bakes; every output voxel is pack2x16float(signed distance, 1.0) in Z, Y, X order.
+struct DenseArgs {
+    const BakeTri *tris;        // Morton order
+    const BakeTri *otris;       // original order (coarse pass: replay when more than 8 ties)
+    const uint32_t *index;      // original index per sorted triangle
+    const float4 *clo, *chi;    // cluster boxes
+    const uint32_t *ccode;      // cluster Morton codes
+    uint32_t ntris, nclusters;
+    float mo[3], ms;            // Morton quantisation
+    float lmin[3], lmax[3];
+    uint32_t w, h, d;
+    uint32_t topk;              // 1 (dense bake) or 8 (coarse pass of the hierarchical bake)
+    uint32_t *out;              // packed distances
+    uint32_t *nearest;          // topk == 8: 8 triangle indices per voxel
+};
+hipError_t launch_bake_dense(hipStream_t st, const DenseArgs &a);
+
+struct FineArgs {
+    const BakeTri *tris;        // original order
+    uint32_t ntris;
+    const uint32_t *coarse_dist, *coarse_tri;
+    uint32_t cw, ch, cd;
+    float band;
+    float lmin[3], lmax[3];
+    uint32_t w, h, d;
+    uint32_t *out;
+};
+hipError_t launch_bake_fine(hipStream_t st, const FineArgs &a);
+
+}  // namespace sdf
+}  // namespace tc

@@ -117,9 +117,9 @@ hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_coll
     s->last_p = pc;
     s->last_lam = lc;
     if (track && (e = ws_post(s, dt_sub, Pc)) != hipSuccess) return e;
-    if (collide && (!s->cols_h.empty() || !s->tris_h.empty())) {
+    if (collide && (!s->cols_h.empty() || !s->tris_h.empty() || s->sdf_nb > 0)) {
         e = tc::launch_collision(s->stream, Pc, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
-                                 s->tri_bounds.p, uint32_t(s->tris_h.size()), s->single_sided_recovery, dt_sub, B);
+                                 s->tri_bounds.p, uint32_t(s->tris_h.size()), s->single_sided_recovery, dt_sub, B, s->sdf_args());
         if (e != hipSuccess) return e;
     }
     if (self_collide && s->self_collision) {
@@ -533,10 +533,10 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
                             return r;
                         })) != hipSuccess) return e;
         }
-        if (!s->cols_h.empty() || !s->tris_h.empty())
+        if (!s->cols_h.empty() || !s->tris_h.empty() || s->sdf_nb > 0)
             if ((e = mark(5, [&] { return tc::launch_collision(s->stream, Pc, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()),
                                                                s->tris.p, s->tri_bounds.p, uint32_t(s->tris_h.size()),
-                                                               s->single_sided_recovery, dt_sub, B); })) != hipSuccess) return e;
+                                                               s->single_sided_recovery, dt_sub, B, s->sdf_args()); })) != hipSuccess) return e;
         if (s->self_collision)
             if ((e = mark(6, [&] {
                      hipError_t r = enqueue_hashes(s, Pc);
@@ -709,6 +709,52 @@ int tc_sim_clear_colliders(tc_sim *s) {
     s->cols_h.clear();
     s->tris_h.clear();
     s->touch();
+    return TC_OK;
+}
+
+int tc_sim_set_sdf_colliders(tc_sim *s, uint32_t w, uint32_t h, uint32_t d, const void *texels, size_t nbytes,
+                             const float *infos, uint32_t nb) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (nb > 0 && !infos) return fail(TC_ERR_VALUE, "collider infos pointer is null");
+    const size_t nvox = size_t(w) * h * d;
+    if (nb > 0 && (w == 0 || h == 0 || d == 0)) return fail(TC_ERR_VALUE, "texture dimensions must be positive");
+    if (nb > 0 && (!texels || nbytes < nvox * 4))
+        return fail(TC_ERR_VALUE, "texture data holds " + std::to_string(nbytes) + " bytes, " +
+                                      std::to_string(nvox * 4) + " needed (Rg16Float, 4 bytes per texel)");
+    TC_HIP(hipStreamSynchronize(s->stream));
+    s->sdf_nb = 0;
+    if (nb > 0) {
+        TC_HIP(s->sdf_tex.alloc(nvox));
+        TC_HIP(hipMemcpy(s->sdf_tex.p, texels, nvox * 4, hipMemcpyHostToDevice));
+        TC_HIP(s->sdf_info.alloc(size_t(nb) * 20));
+        TC_HIP(hipMemcpy(s->sdf_info.p, infos, size_t(nb) * 20 * 4, hipMemcpyHostToDevice));
+        // identity transforms until update_bone_transforms (simulation/mod.rs:1700-1719)
+        std::vector<float> xf(size_t(nb) * 32, 0.0f);
+        for (uint32_t b = 0; b < nb; ++b)
+            for (int m = 0; m < 2; ++m)
+                for (int k = 0; k < 4; ++k) xf[size_t(b) * 32 + m * 16 + k * 5] = 1.0f;
+        TC_HIP(s->sdf_xf.alloc(xf.size()));
+        TC_HIP(hipMemcpy(s->sdf_xf.p, xf.data(), 4 * xf.size(), hipMemcpyHostToDevice));
+        s->sdf_w = w; s->sdf_h = h; s->sdf_d = d;
+        s->sdf_nb = nb;
+    }
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_update_bone_transforms(tc_sim *s, const float *world, const float *inv_world, uint32_t n) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (!world || !inv_world) return fail(TC_ERR_VALUE, "null matrix pointer");
+    if (n != s->sdf_nb)
+        return fail(TC_ERR_VALUE, "expected " + std::to_string(s->sdf_nb) + " transforms (one per SDF collider), got " +
+                                      std::to_string(n));
+    std::vector<float> xf(size_t(n) * 32);
+    for (uint32_t b = 0; b < n; ++b) {
+        std::memcpy(&xf[size_t(b) * 32], world + size_t(b) * 16, 64);
+        std::memcpy(&xf[size_t(b) * 32 + 16], inv_world + size_t(b) * 16, 64);
+    }
+    TC_HIP(hipStreamSynchronize(s->stream));
+    if (n) TC_HIP(hipMemcpy(s->sdf_xf.p, xf.data(), 4 * xf.size(), hipMemcpyHostToDevice));
     return TC_OK;
 }
 

@@ -104,6 +104,17 @@ struct tc_sim {
     DevBuf<float4> tri_bounds;
     int single_sided_recovery = 1;
 
+    // SDF colliders (set_bone_sdf_colliders)
+    DevBuf<uint32_t> sdf_tex;
+    DevBuf<float> sdf_info, sdf_xf;
+    uint32_t sdf_w = 0, sdf_h = 0, sdf_d = 0, sdf_nb = 0;
+    tc::SdfArgs sdf_args() const {
+        tc::SdfArgs a;
+        a.tex = sdf_tex.p; a.w = sdf_w; a.h = sdf_h; a.d = sdf_d;
+        a.info = sdf_info.p; a.xf = sdf_xf.p; a.nb = sdf_nb;
+        return a;
+    }
+
     // self collision
     DevBuf<float> local_len, th_rest;
     DevBuf<uint32_t> th_off, th_idx, star_off, edge_bbits;

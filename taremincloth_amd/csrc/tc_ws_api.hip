@@ -652,10 +652,10 @@ int tc_sim_ws_time(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
         s->last_lam = lc;
         TC_HIP(tc::ws::launch_update_prev(s->stream, Pc, s->ws_ppred.p, s->ws_D.p, nullptr, sc, s->ws_beta, nv));
         TC_HIP(hipEventRecord(ev[24], s->stream));
-        if (!s->cols_h.empty() || !s->tris_h.empty())
+        if (!s->cols_h.empty() || !s->tris_h.empty() || s->sdf_nb > 0)
             TC_HIP(tc::launch_collision(s->stream, Pc, s->X.p, nv, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
                                         s->tri_bounds.p, uint32_t(s->tris_h.size()), s->single_sided_recovery, dt_sub,
-                                        B));
+                                        B, s->sdf_args()));
         if (s->self_collision) {
             TC_HIP(tc::launch_hash_vertices(s->stream, Pc, nv, s->cell_size, s->table_size, s->vhash()));
             TC_HIP(tc::launch_hash_edges(s->stream, Pc, s->edges.p, uint32_t(m.e_v0.size()), s->cell_size,

@@ -196,11 +196,125 @@ __device__ __forceinline__ float seg_sphere_dist_sq(f3 a, f3 b, f3 c) {  // coll
 
 // collision.wgsl:377-652, decoupled (once per substep), direct triangle scan
 // in the Morton order the host sorted the triangles into.
+// Trilinear sample of the (distance, weight) volume at normalised coordinates, clamp-to-edge
+// addressing with texel centres at (i + 0.5) / dim -- the filtering a linear sampler applies.
+__device__ __forceinline__ float2 sdf_sample(const SdfArgs &s, float u, float v, float w) {
+    const float x = u * float(s.w) - 0.5f, y = v * float(s.h) - 0.5f, z = w * float(s.d) - 0.5f;
+    const float fx0 = floorf(x), fy0 = floorf(y), fz0 = floorf(z);
+    const float tx = x - fx0, ty = y - fy0, tz = z - fz0;
+    const int32_t xi = f2i_sat(fx0), yi = f2i_sat(fy0), zi = f2i_sat(fz0);
+    const int32_t x0 = min(max(xi, 0), int32_t(s.w) - 1), x1 = min(max(xi + 1, 0), int32_t(s.w) - 1);
+    const int32_t y0 = min(max(yi, 0), int32_t(s.h) - 1), y1 = min(max(yi + 1, 0), int32_t(s.h) - 1);
+    const int32_t z0 = min(max(zi, 0), int32_t(s.d) - 1), z1 = min(max(zi + 1, 0), int32_t(s.d) - 1);
+    auto half_bits = [](uint32_t b) {
+        const uint16_t u = uint16_t(b);
+        _Float16 h;
+        __builtin_memcpy(&h, &u, 2);
+        return (float)h;
+    };
+    auto texel = [&](int32_t i, int32_t j, int32_t k) {
+        const uint32_t t = s.tex[(size_t(k) * s.h + uint32_t(j)) * s.w + uint32_t(i)];
+        return make_float2(half_bits(t & 0xFFFFu), half_bits(t >> 16));
+    };
+    auto lerp2 = [](float2 a, float2 b, float t) { return make_float2(a.x + (b.x - a.x) * t, a.y + (b.y - a.y) * t); };
+    const float2 c00 = lerp2(texel(x0, y0, z0), texel(x1, y0, z0), tx);
+    const float2 c10 = lerp2(texel(x0, y1, z0), texel(x1, y1, z0), tx);
+    const float2 c01 = lerp2(texel(x0, y0, z1), texel(x1, y0, z1), tx);
+    const float2 c11 = lerp2(texel(x0, y1, z1), texel(x1, y1, z1), tx);
+    return lerp2(lerp2(c00, c10, ty), lerp2(c01, c11, ty), tz);
+}
+
+// matrix (given as m[row][column]) applied the way the reference's shader applies the uploaded
+// array: result[k] = sum_j m[j][k] * v[j]
+__device__ __forceinline__ f3 sdf_xform(const float *m, f3 v, float w) {
+    return mk3(m[0] * v.x + m[4] * v.y + m[8] * v.z + m[12] * w, m[1] * v.x + m[5] * v.y + m[9] * v.z + m[13] * w,
+               m[2] * v.x + m[6] * v.y + m[10] * v.z + m[14] * w);
+}
+
+// solve_bone_sdf_collision (collision.wgsl:116-265)
+__device__ void sdf_collision(const SdfArgs &s, uint32_t ntris_mesh, f3 &p, f3 &xn, float dt, f3 p_initial, float vth) {
+    if (s.nb == 0) return;
+    float best_dist = 1e6f, best_fr = 0.0f, best_rest = 0.0f, best_target = 0.0f;
+    f3 best_n = mk3(0.0f, 0.0f, 0.0f);
+    uint32_t hits = 0;
+    for (uint32_t b = 0; b < s.nb; ++b) {
+        const float *I = s.info + 20 * size_t(b);
+        const float radius = I[15];
+        if (radius > 0.0f) {
+            const f3 dw = p - mk3(I[3], I[7], I[11]);
+            const float rt = radius + vth;
+            if (dot(dw, dw) > rt * rt) continue;
+        }
+        const float *wm = s.xf + 32 * size_t(b), *im = wm + 16;
+        const f3 pl = sdf_xform(im, p, 1.0f);
+        const float target = vth + I[17];
+        const float margin = target + 0.05f;
+        if (pl.x < I[0] - margin || pl.x > I[4] + margin || pl.y < I[1] - margin || pl.y > I[5] + margin ||
+            pl.z < I[2] - margin || pl.z > I[6] + margin)
+            continue;
+        const f3 sc = mk3(I[8], I[9], I[10]), of = mk3(I[12], I[13], I[14]);
+        const f3 uvw = mk3(pl.x * sc.x + of.x, pl.y * sc.y + of.y, pl.z * sc.z + of.z);
+        const f3 umin = mk3(I[0] * sc.x + of.x, I[1] * sc.y + of.y, I[2] * sc.z + of.z);
+        const f3 umax = mk3(I[4] * sc.x + of.x, I[5] * sc.y + of.y, I[6] * sc.z + of.z);
+        if (uvw.x < umin.x || uvw.x > umax.x || uvw.y < umin.y || uvw.y > umax.y || uvw.z < umin.z || uvw.z > umax.z)
+            continue;
+        const f3 eps_slice = (umax - umin) * (s.nb == 1u ? 0.002f : 0.015f);
+        const f3 lo = umin + eps_slice, hi = umax - eps_slice;
+        const f3 us = mk3(clampf(uvw.x, lo.x, hi.x), clampf(uvw.y, lo.y, hi.y), clampf(uvw.z, lo.z, hi.z));
+        const float2 sv = sdf_sample(s, us.x, us.y, us.z);
+        const float raw = sv.x, alpha = sv.y;
+        const float threshold = fmaxf(I[19], 0.01f);
+        float retract = 0.0f;
+        if (ntris_mesh > 0u) {
+            const float t = clampf((alpha - threshold) / (0.7f - threshold), 0.0f, 1.0f);
+            retract = (1.0f - t * t * (3.0f - 2.0f * t)) * 0.06f;
+        }
+        const float eff = alpha > threshold ? raw + retract : 10.0f;
+        if (eff < target) {
+            const f3 eu = sc * 0.005f;
+            const float dxp = sdf_sample(s, clampf(uvw.x + eu.x, lo.x, hi.x), us.y, us.z).x;
+            const float dxn = sdf_sample(s, clampf(uvw.x - eu.x, lo.x, hi.x), us.y, us.z).x;
+            const float dyp = sdf_sample(s, us.x, clampf(uvw.y + eu.y, lo.y, hi.y), us.z).x;
+            const float dyn = sdf_sample(s, us.x, clampf(uvw.y - eu.y, lo.y, hi.y), us.z).x;
+            const float dzp = sdf_sample(s, us.x, us.y, clampf(uvw.z + eu.z, lo.z, hi.z)).x;
+            const float dzn = sdf_sample(s, us.x, us.y, clampf(uvw.z - eu.z, lo.z, hi.z)).x;
+            float dx = dxp - dxn, dy = dyp - dyn, dz = dzp - dzn;
+            if (fabsf(dx) < 1e-6f) dx = uvw.x < (lo.x + hi.x) * 0.5f ? dxp - raw : raw - dxn;
+            if (fabsf(dy) < 1e-6f) dy = uvw.y < (lo.y + hi.y) * 0.5f ? dyp - raw : raw - dyn;
+            if (fabsf(dz) < 1e-6f) dz = uvw.z < (lo.z + hi.z) * 0.5f ? dzp - raw : raw - dzn;
+            const f3 g = mk3(dx, dy, dz);
+            const float gl = length(g);
+            f3 nl = mk3(0.0f, 0.0f, 1.0f);
+            if (gl > 1e-6f) {
+                nl = g / gl;
+            } else {
+                const f3 dc = pl - mk3((I[0] + I[4]) * 0.5f, (I[1] + I[5]) * 0.5f, (I[2] + I[6]) * 0.5f);
+                const float lc = length(dc);
+                if (lc > 1e-6f) nl = dc / lc;
+            }
+            const f3 nw0 = sdf_xform(wm, nl, 0.0f);
+            const f3 nw = nw0 / length(nw0);
+            if (hits == 0u || eff < best_dist) {
+                best_dist = eff;
+                best_n = nw;
+                best_fr = I[16];
+                best_rest = I[18];
+                best_target = target;
+            }
+            ++hits;
+        }
+    }
+    if (hits > 0u && best_dist < best_target) {
+        const float pen = best_target - best_dist;
+        contact_response(p + best_n * pen, best_n, best_fr, best_rest, dt, p_initial, xn, p);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_collision(float4 *__restrict__ P, float4 *__restrict__ X, uint32_t n,
                                                    const Collider *__restrict__ cols, uint32_t ncols,
                                                    const MeshTri *__restrict__ tris,
                                                    const float4 *__restrict__ bounds, uint32_t ntris,
-                                                   int single_sided_recovery, float dt) {
+                                                   int single_sided_recovery, float dt, SdfArgs sdf) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 q = P[i];
@@ -299,6 +413,8 @@ __global__ void __launch_bounds__(256) k_collision(float4 *__restrict__ P, float
         }
     }
     if (!front && has_rec) contact_response(best_pos, best_n, best_fr, best_rest, dt, p_init, xn, p);
+    // SDF colliders last, with the current p as the reference for the velocity (collision.wgsl:645-647)
+    sdf_collision(sdf, ntris, p, xn, dt, p, th);
     P[i] = make_float4(p.x, p.y, p.z, q.w);
     X[i] = make_float4(xn.x, xn.y, xn.z, x.w);
 }
@@ -465,10 +581,10 @@ hipError_t launch_bending(hipStream_t s, float4 *P, const float4 *X, const uint4
 
 hipError_t launch_collision(hipStream_t s, float4 *P, float4 *X, uint32_t n, const Collider *cols, uint32_t ncols,
                             const MeshTri *tris, const float4 *bounds, uint32_t ntris, int recovery, float dt,
-                            uint32_t block) {
+                            uint32_t block, const SdfArgs &sdf) {
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(k_collision, dim3(grid_for(n, block)), dim3(block), 0, s, P, X, n, cols, ncols, tris, bounds,
-                       ntris, recovery, dt);
+                       ntris, recovery, dt, sdf);
     return hipGetLastError();
 }
 

@@ -14,7 +14,8 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ["ClothSimulator", "color_edge_pairs", "compute_combined_inv_masses"]
+__all__ = ["ClothSimulator", "color_edge_pairs", "compute_combined_inv_masses", "bake_mesh_sdf_gpu",
+           "bake_mesh_sdf_hierarchical_gpu"]
 
 
 def _f32(a, shape_tail=None, name="array"):
@@ -303,6 +304,27 @@ class ClothSimulator:
     def synchronize(self):
         _lib.check(self._L.tc_sim_synchronize(self._h))
 
+    def set_bone_sdf_colliders(self, width, height, depth, texture_bytes, bone_infos):
+        """SDF colliders from a baked Rg16Float volume and rows of 20 floats (lib.rs:2245-2284)."""
+        infos = np.ascontiguousarray(np.asarray(bone_infos, dtype=np.float32))
+        if infos.ndim != 2 or (infos.size and infos.shape[1] < 20):
+            raise ValueError("each bone_infos row needs at least 20 elements")
+        infos = np.ascontiguousarray(infos[:, :20])
+        tex = np.frombuffer(bytes(texture_bytes), dtype=np.uint8)
+        _lib.check(self._L.tc_sim_set_sdf_colliders(self._h, int(width), int(height), int(depth),
+                                                    tex.ctypes.data_as(C.c_void_p), tex.size,
+                                                    infos.ctypes.data_as(C.c_void_p), len(infos)))
+        self._sdf_texture = tex
+
+    def update_bone_transforms(self, world_matrices, inv_world_matrices, dirty_bone_indices=None):
+        """Per-collider world and inverse-world 4x4 matrices (lib.rs:2287-2330)."""
+        w = np.ascontiguousarray(world_matrices, np.float32)
+        iw = np.ascontiguousarray(inv_world_matrices, np.float32)
+        if w.ndim != 3 or w.shape[1:] != (4, 4) or iw.shape != w.shape:
+            raise ValueError("world_matrices and inv_world_matrices must both have shape (N, 4, 4)")
+        _lib.check(self._L.tc_sim_update_bone_transforms(self._h, w.ctypes.data_as(C.c_void_p),
+                                                         iw.ctypes.data_as(C.c_void_p), len(w)))
+
     # --- learned warm start (no reference counterpart; see taremincloth_amd.warmstart) -------
     def set_warm_start(self, net=None, mode=0, iterations=8, fallback_iterations=56, residual_threshold=1e-4,
                        history_weight=None):
@@ -385,6 +407,74 @@ def compute_combined_inv_masses(positions, faces=None, strand_edges=None, pin_we
     _lib.check(L.tc_compute_inv_masses(_ptr(pos), len(pos), _ptr(f), 0 if f is None else len(f), _ptr(pw),
                                        float(areal_density), _ptr(out)))
     return out
+
+
+def _bake(mesh_verts, mesh_tris, voxel_size, margin, friction, thickness, restitution, close_open_boundaries,
+          method):
+    L = _lib.load()
+    v = np.ascontiguousarray(np.asarray(mesh_verts, dtype=np.float32))
+    t = np.ascontiguousarray(np.asarray(mesh_tris, dtype=np.int32))
+    if v.ndim != 2 or v.shape[1] < 3 or t.ndim != 2 or (t.size and t.shape[1] < 3):
+        raise ValueError("mesh_verts must be (N, 3) and mesh_tris (T, 3)")
+    v = np.ascontiguousarray(v[:, :3])
+    t = np.ascontiguousarray(t[:, :3]) if t.size else np.zeros((0, 3), np.int32)
+    res = C.c_void_p()
+    status = L.tc_bake_mesh_sdf(_ptr(v), len(v), _ptr(t), len(t), float(voxel_size), float(margin), float(friction),
+                                float(thickness), float(restitution), 1 if close_open_boundaries else 0, int(method),
+                                C.byref(res))
+    if status == _lib.TC_ERR_RUNTIME:  # the reference raises RuntimeError for every bake failure (lib.rs:1410)
+        raise RuntimeError("GPU mesh SDF bake failed: " + (L.tc_last_error() or b"").decode(errors="replace"))
+    _lib.check(status)
+    try:
+        dims = (C.c_uint32 * 3)()
+        info = (C.c_float * 20)()
+        cv = C.c_float(0.0)
+        _lib.check(L.tc_sdf_result_info(res, dims, info, C.byref(cv)))
+        n = dims[0] * dims[1] * dims[2]
+        tex = np.ctypeslib.as_array(C.cast(L.tc_sdf_result_texels(res), C.POINTER(C.c_uint32)), (n,)).copy()
+        out = {"texture_bytes": tex.tobytes(), "width": int(dims[0]), "height": int(dims[1]), "depth": int(dims[2]),
+               "bone_info": np.array(info, dtype=np.float32)}
+        if method == 2:
+            near = np.ctypeslib.as_array(C.cast(L.tc_sdf_result_nearest(res), C.POINTER(C.c_uint32)), (n * 8,))
+            out["nearest_tri"] = near.copy()
+            out["dist_packed"] = tex
+            out["coarse_voxel"] = float(cv.value)
+        return out
+    finally:
+        L.tc_sdf_result_free(res)
+
+
+def bake_mesh_sdf_gpu(mesh_verts, mesh_tris, voxel_size=0.004, margin=0.02, friction=0.5, thickness=0.005,
+                      restitution=0.0, close_open_boundaries=True) -> dict:
+    """Dense mesh SDF bake (lib.rs:1350-1417 -> sdf_baker.rs:1114-1323)."""
+    return _bake(mesh_verts, mesh_tris, voxel_size, margin, friction, thickness, restitution, close_open_boundaries, 0)
+
+
+def bake_mesh_sdf_hierarchical_gpu(mesh_verts, mesh_tris, voxel_size=0.004, margin=0.02, friction=0.5,
+                                   thickness=0.005, restitution=0.0, close_open_boundaries=True) -> dict:
+    """Hierarchical mesh SDF bake: coarse top-8 pass + fine pass (lib.rs:1419-1485 -> sdf_baker.rs:1625-1882)."""
+    return _bake(mesh_verts, mesh_tris, voxel_size, margin, friction, thickness, restitution, close_open_boundaries, 1)
+
+
+def bake_mesh_sdf_coarse(mesh_verts, mesh_tris, voxel_size=0.004, margin=0.02, friction=0.5, thickness=0.005,
+                         restitution=0.0, close_open_boundaries=True) -> dict:
+    """The coarse pass alone (sdf_baker.rs:1328-1600): adds nearest_tri (8 per cell) and coarse_voxel."""
+    return _bake(mesh_verts, mesh_tris, voxel_size, margin, friction, thickness, restitution, close_open_boundaries, 2)
+
+
+def close_open_boundary_loops(mesh_verts, mesh_tris):
+    """Caps closed boundary loops of a solid mesh (sdf_baker.rs:244-462): (verts, tris, loops_closed)."""
+    L = _lib.load()
+    v = np.ascontiguousarray(np.asarray(mesh_verts, dtype=np.float32).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(mesh_tris, dtype=np.int32).reshape(-1, 3))
+    nv, nt, loops = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    _lib.check(L.tc_close_open_boundary_loops(_ptr(v), len(v), _ptr(t), len(t), None, 0, None, 0, C.byref(nv),
+                                              C.byref(nt), C.byref(loops)))
+    ov = np.empty((nv.value, 3), np.float32)
+    ot = np.empty((nt.value, 3), np.int32)
+    _lib.check(L.tc_close_open_boundary_loops(_ptr(v), len(v), _ptr(t), len(t), _ptr(ov), ov.size, _ptr(ot), ot.size,
+                                              C.byref(nv), C.byref(nt), C.byref(loops)))
+    return ov, ot, int(loops.value)
 
 
 _MESH_FLOAT_ARRAYS = {8, 9, 24, 25, 32, 33, 34}
