@@ -261,6 +261,12 @@ int tc_bake_mesh_sdf(const float *verts, uint32_t num_verts, const int32_t *tris
                      int method, tc_sdf_result **out);
 /* dims[3], bone_info[20], coarse cell width (method 2); each may be NULL. */
 int tc_sdf_result_info(const tc_sdf_result *res, uint32_t *dims, float *bone_info, float *coarse_voxel);
+/* Device time (ms, HIP events on the launching stream) of the bake's kernels: kernel_ms[0] the dense
+ * traversal or the coarse pass, kernel_ms[1] the fine pass with its fallback (0 unless hierarchical);
+ * fallback_voxels: voxels the fine pass left to its exhaustive fallback.  Either may be NULL.
+ * Diagnostics only; no reference counterpart (bake_mesh_sdf_gpu_ext, sdf_baker.rs:1114-1323, reports
+ * neither). */
+int tc_sdf_result_timing(const tc_sdf_result *res, float *kernel_ms, uint32_t *fallback_voxels);
 const uint32_t *tc_sdf_result_texels(const tc_sdf_result *res);
 const uint32_t *tc_sdf_result_nearest(const tc_sdf_result *res);
 void tc_sdf_result_free(tc_sdf_result *res);

@@ -18,6 +18,8 @@ struct BakeTri {
     float nx, ny, nz, pad3;
 };
 
+static_assert(sizeof(BakeTri) == 64, "a cluster of 32 triangles is staged as 128 float4");
+
 struct Domain {
     float min[3], max[3], size[3];
     uint32_t w, h, d;
@@ -40,8 +42,6 @@ struct Clusters {
     std::vector<BakeTri> tris;      // Morton order
     std::vector<uint32_t> index;    // original index of each sorted triangle
     std::vector<float4> lo, hi;     // per cluster box (w unused)
-    std::vector<uint32_t> code;     // Morton code of each cluster's first triangle
-    float origin[3], scale;         // Morton quantisation of the domain
 };
 Clusters build_clusters(const std::vector<BakeTri> &tris, const Domain &dm);
 
@@ -51,15 +51,16 @@ struct DenseArgs {
     const BakeTri *otris;       // original order (coarse pass: replay when more than 8 ties)
     const uint32_t *index;      // original index per sorted triangle
     const float4 *clo, *chi;    // cluster boxes
-    const uint32_t *ccode;      // cluster Morton codes
     uint32_t ntris, nclusters;
-    float mo[3], ms;            // Morton quantisation
     float lmin[3], lmax[3];
     uint32_t w, h, d;
     uint32_t topk;              // 1 (dense bake) or 8 (coarse pass of the hierarchical bake)
     uint32_t *out;              // packed distances
     uint32_t *nearest;          // topk == 8: 8 triangle indices per voxel
+    unsigned long long *keys;   // topk == 1: per voxel (squared distance bits << 32 | original index)
+    unsigned long long item0;   // first (brick, segment) item of a launch of the segmented traversal
 };
+// topk 1: three launches (upper bound, brick x cluster-segment traversal, finish); topk 8: one.
 hipError_t launch_bake_dense(hipStream_t st, const DenseArgs &a);
 
 struct FineArgs {
@@ -71,8 +72,25 @@ struct FineArgs {
     float lmin[3], lmax[3];
     uint32_t w, h, d;
     uint32_t *out;
+    uint32_t *amb, *amb_count;  // voxels left to the exhaustive fallback (capacity w*h*d) and their count
 };
 hipError_t launch_bake_fine(hipStream_t st, const FineArgs &a);
+
+// The fine pass's exhaustive fallback for the listed voxels: one wavefront per voxel walks the
+// clusters under an upper bound and keeps the (d_sq, index) minimum, the first strict minimum
+// of the reference's in-order scan.
+struct ResolveArgs {
+    const BakeTri *tris;        // Morton order
+    const uint32_t *index;      // original index per sorted triangle
+    const float4 *clo, *chi;    // cluster boxes
+    uint32_t ntris, nclusters;
+    const uint32_t *list;       // flat voxel indices
+    uint32_t count;
+    float lmin[3], lmax[3];
+    uint32_t w, h, d;
+    uint32_t *out;
+};
+hipError_t launch_bake_resolve(hipStream_t st, const ResolveArgs &a);
 
 }  // namespace sdf
 }  // namespace tc

@@ -5,21 +5,23 @@
 // nearest triangle (Ericson's closest point, sign from the face normal, first strict minimum
 // in triangle order wins), stored as pack2x16float(d, 1.0) in Z, Y, X order.
 //
-// Here one wavefront owns a brick of 8x8x8 voxels (8 per lane).  Triangles are sorted along
-// a Morton curve and grouped into clusters of 32 with a box each; a wavefront walks the
-// clusters outward from its brick's Morton position and skips every cluster whose box lies
-// farther from the brick than the worst current best of its voxels.  Candidates compare by
-// (squared distance, original triangle index), so the result is the same argmin the
+// Here one wavefront owns a brick of 8x8x8 voxels (8 per lane).  Triangles are sorted along a
+// Morton curve and grouped into clusters of 32 with a box each.  A wavefront first bounds the
+// nearest distance of all its voxels from above, then evaluates only the clusters and triangles
+// whose boxes can still hold a nearer triangle for one of its voxels (k_bake_brick).  Candidates
+// compare by (squared distance, original triangle index), so the result is the same argmin the
 // reference's in-order scan returns, whatever order the clusters are visited in.
 //
 // The coarse pass of the hierarchical bake (bake_mesh_sdf_hier_coarse_main.wgsl) is the same
-// walk with one voxel per lane (4x4x4 bricks) and the eight nearest triangles per voxel;
-// the fine pass (bake_mesh_sdf_hier_fine_main.wgsl) is restated voxel by voxel.
+// traversal with one voxel per lane (4x4x4 bricks) and the 16 nearest triangles per voxel, from
+// which the reference's eight-entry list is rebuilt; the fine pass
+// (bake_mesh_sdf_hier_fine_main.wgsl) is restated voxel by voxel.
 #include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <numeric>
 #include <set>
 #include <unordered_map>
@@ -234,15 +236,14 @@ std::vector<BakeTri> collect_bake_triangles(const std::vector<float> &verts, con
 Clusters build_clusters(const std::vector<BakeTri> &tris, const Domain &dm) {
     Clusters c;
     const float ext = std::max(dm.size[0], std::max(dm.size[1], dm.size[2]));
-    for (int a = 0; a < 3; ++a) c.origin[a] = dm.min[a];
-    c.scale = 1023.0f / std::max(ext, 1e-6f);
+    const float scale = 1023.0f / std::max(ext, 1e-6f);
     std::vector<uint32_t> code(tris.size());
     for (size_t t = 0; t < tris.size(); ++t) {
         const BakeTri &b = tris[t];
         const float cx = (b.p0x + b.p1x + b.p2x) / 3.0f, cy = (b.p0y + b.p1y + b.p2y) / 3.0f,
                     cz = (b.p0z + b.p1z + b.p2z) / 3.0f;
         auto q = [&](float v, int a) {
-            const float f = (v - c.origin[a]) * c.scale;
+            const float f = (v - dm.min[a]) * scale;
             return uint32_t(std::min(1023.0f, std::max(0.0f, f)));
         };
         code[t] = spread10(q(cx, 0)) | (spread10(q(cy, 1)) << 1) | (spread10(q(cz, 2)) << 2);
@@ -256,7 +257,6 @@ Clusters build_clusters(const std::vector<BakeTri> &tris, const Domain &dm) {
     const size_t nc = (tris.size() + CLUSTER - 1) / CLUSTER;
     c.lo.resize(nc);
     c.hi.resize(nc);
-    c.code.resize(nc);
     for (size_t ci = 0; ci < nc; ++ci) {
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (size_t k = ci * CLUSTER; k < std::min(tris.size(), (ci + 1) * CLUSTER); ++k) {
@@ -270,7 +270,6 @@ Clusters build_clusters(const std::vector<BakeTri> &tris, const Domain &dm) {
         }
         c.lo[ci] = make_float4(lo[0], lo[1], lo[2], 0.0f);
         c.hi[ci] = make_float4(hi[0], hi[1], hi[2], 0.0f);
-        c.code[ci] = code[order[ci * CLUSTER]];
     }
     return c;
 }
@@ -325,25 +324,13 @@ __device__ __forceinline__ float voxel_coord(uint32_t i, uint32_t dim, float lo,
     return lo + uvw * (hi - lo);
 }
 
+__device__ __forceinline__ float wave_min_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
 __device__ __forceinline__ float wave_max_f(float v) {
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
     return v;
-}
-
-__device__ __forceinline__ uint32_t morton_of(f3 p, const DenseArgs &a) {
-    auto q = [&](float v, int ax) {
-        const float f = (v - a.mo[ax]) * a.ms;
-        return uint32_t(fminf(1023.0f, fmaxf(0.0f, f)));
-    };
-    auto spread = [](uint32_t v) {
-        v &= 1023u;
-        v = (v | (v << 16)) & 0x030000FFu;
-        v = (v | (v << 8)) & 0x0300F00Fu;
-        v = (v | (v << 4)) & 0x030C30C3u;
-        v = (v | (v << 2)) & 0x09249249u;
-        return v;
-    };
-    return spread(q(p.x, 0)) | (spread(q(p.y, 1)) << 1) | (spread(q(p.z, 2)) << 2);
 }
 
 // (d_sq, index) lexicographic "less": the in-order first strict minimum
@@ -367,17 +354,59 @@ __device__ __forceinline__ void ref_insert(float (&sq)[8], float (&sg)[8], uint3
 }
 
 constexpr uint32_t NO_TRI = 0xFFFFFFFFu;
+constexpr float FAR = 3.0e38f;
 
-// VPL voxels per lane (brick = 8 x 8 x VPL for VPL = 8, 4 x 4 x 4 for VPL = 1), K nearest kept.
-// K = 1: the dense bake.  K = 16: the coarse pass; the walk keeps the 16 nearest in (d_sq, index)
-// order, which holds every triangle at or below the eighth distance D8 unless more than 16 do.
-// The reference's list is the result of its update rule applied in triangle order, and only
-// triangles at or below D8 can be in it or move an entry of it, so replaying exactly those in
-// index order gives the reference's list; with no equal distances among them the replay is
-// the identity and is skipped.  More than 16 at or below D8 replays every triangle.
+// Lower bound of the squared distance from p to a box, shortened by 1e-5 m per axis and 1e-4
+// relative so that the rounding of an exact point-triangle distance can never fall below it.
+__device__ __forceinline__ float lb_box_point(f3 lo, f3 hi, f3 p) {
+    const float gx = fmaxf(0.0f, fmaxf(lo.x - p.x, p.x - hi.x) - 1e-5f);
+    const float gy = fmaxf(0.0f, fmaxf(lo.y - p.y, p.y - hi.y) - 1e-5f);
+    const float gz = fmaxf(0.0f, fmaxf(lo.z - p.z, p.z - hi.z) - 1e-5f);
+    return (gx * gx + gy * gy + gz * gz) * 0.9999f;
+}
+__device__ __forceinline__ float lb_box_box(f3 lo, f3 hi, f3 bmin, f3 bmax) {
+    const float gx = fmaxf(0.0f, fmaxf(lo.x - bmax.x, bmin.x - hi.x) - 1e-5f);
+    const float gy = fmaxf(0.0f, fmaxf(lo.y - bmax.y, bmin.y - hi.y) - 1e-5f);
+    const float gz = fmaxf(0.0f, fmaxf(lo.z - bmax.z, bmin.z - hi.z) - 1e-5f);
+    return (gx * gx + gy * gy + gz * gz) * 0.9999f;
+}
+// Squared distance from the farthest point of the box [bmin, bmax] to q (to any point of box
+// [lo, hi] for the second form): an upper bound on the distance to whatever lies there.
+__device__ __forceinline__ float ub_box_point(f3 bmin, f3 bmax, f3 q) {
+    const float dx = fmaxf(fabsf(q.x - bmin.x), fabsf(q.x - bmax.x));
+    const float dy = fmaxf(fabsf(q.y - bmin.y), fabsf(q.y - bmax.y));
+    const float dz = fmaxf(fabsf(q.z - bmin.z), fabsf(q.z - bmax.z));
+    return dx * dx + dy * dy + dz * dz;
+}
+__device__ __forceinline__ float ub_box_box(f3 bmin, f3 bmax, f3 lo, f3 hi) {
+    const float dx = fmaxf(fabsf(hi.x - bmin.x), fabsf(bmax.x - lo.x));
+    const float dy = fmaxf(fabsf(hi.y - bmin.y), fabsf(bmax.y - lo.y));
+    const float dz = fmaxf(fabsf(hi.z - bmin.z), fabsf(bmax.z - lo.z));
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// One wavefront per brick of voxels: VPL voxels per lane (8 x 8 x VPL bricks for VPL > 1,
+// 4 x 4 x 4 for VPL = 1), the K nearest triangles per voxel in (d_sq, index) order.
+// K = 1: the dense bake.  K = 16: the coarse pass of the hierarchical bake.
+//   1. Upper bound U on the K-th nearest distance of every voxel in the brick: for K = 1 the
+//      farthest brick point from a vertex of some triangle, for K = 16 the farthest pair of
+//      points between the brick and the box of a cluster holding at least 16 triangles; the
+//      cluster giving the smallest U is evaluated first.
+//   2. Clusters (32 Morton-consecutive triangles) whose box lies farther than U (or than the
+//      worst current K-th best of the brick) from the brick are skipped, 64 at a time by ballot,
+//      walking outward from the seed cluster along the Morton order.
+//   3. Within a cluster, a triangle is evaluated only if its box is not farther than the current
+//      K-th best of some voxel of the wavefront, and only for the voxels where that holds.
+// Every skipped triangle is farther than a K-th best it could replace, so the lists equal those
+// of the exhaustive scan.
+// For K = 16 the 16 nearest hold every triangle at or below the eighth distance D8 unless more
+// than 16 do.  The reference's list is the result of its update rule applied in triangle order,
+// and only triangles at or below D8 can be in it or move an entry of it, so replaying exactly
+// those in index order gives the reference's list; with no equal distances among them the
+// replay is the identity and is skipped.  More than 16 at or below D8 replays every triangle.
 template <int VPL, int K>
-__global__ void __launch_bounds__(256) k_bake_walk(DenseArgs a) {
-    constexpr int BX = VPL == 8 ? 8 : 4, BY = VPL == 8 ? 8 : 4, BZ = VPL == 8 ? 8 : 4;
+__global__ void __launch_bounds__(256) k_bake_brick(DenseArgs a) {
+    constexpr int BX = VPL == 1 ? 4 : 8, BY = VPL == 1 ? 4 : 8, BZ = VPL == 1 ? 4 : VPL;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t brick = blockIdx.x * 4u + (threadIdx.x >> 6);
     const uint32_t nbx = (a.w + BX - 1) / BX, nby = (a.h + BY - 1) / BY, nbz = (a.d + BZ - 1) / BZ;
@@ -385,7 +414,7 @@ __global__ void __launch_bounds__(256) k_bake_walk(DenseArgs a) {
     const uint32_t bx = brick % nbx, by = (brick / nbx) % nby, bz = brick / (nbx * nby);
     // lane's voxels
     uint32_t vx, vy, vz0;
-    if (VPL == 8) { vx = bx * 8 + (lane & 7u); vy = by * 8 + (lane >> 3); vz0 = bz * 8; }
+    if (VPL > 1) { vx = bx * 8 + (lane & 7u); vy = by * 8 + (lane >> 3); vz0 = bz * VPL; }
     else { vx = bx * 4 + (lane & 3u); vy = by * 4 + ((lane >> 2) & 3u); vz0 = bz * 4 + (lane >> 4); }
     const bool col_ok = vx < a.w && vy < a.h;
     f3 p[VPL];
@@ -407,41 +436,62 @@ __global__ void __launch_bounds__(256) k_bake_walk(DenseArgs a) {
     uint32_t bid[VPL][K];
     for (int j = 0; j < VPL; ++j)
         for (int k = 0; k < K; ++k) { bsq[j][k] = 1e10f; bsg[j][k] = 10.0f; bid[j][k] = NO_TRI; }
-    // start at the brick centre's position along the Morton curve
-    const uint32_t mc = morton_of((bmin + bmax) * 0.5f, a);
-    uint32_t lo = 0, hi = a.nclusters;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a.ccode[mid] < mc) lo = mid + 1; else hi = mid;
-    }
-    const int32_t c0 = int32_t(min(lo, a.nclusters - 1));
-    float bound = 3.0e38f;
-    int32_t up = c0, down = c0 - 1;
-    for (uint32_t step = 0; up < int32_t(a.nclusters) || down >= 0; ++step) {
-        int32_t c;
-        if ((step & 1u) == 0u) { if (up < int32_t(a.nclusters)) c = up++; else c = down--; }
-        else { if (down >= 0) c = down--; else c = up++; }
-        if ((step & 3u) == 0u) {
-            float worst = -1.0f;
-            for (int j = 0; j < VPL; ++j)
-                if (ok[j]) worst = fmaxf(worst, bsq[j][K - 1]);
-            bound = wave_max_f(worst);
+    const uint32_t ncl = a.nclusters;
+
+    // 1. upper bound and seed cluster
+    float ub = FAR;
+    uint32_t uc = 0;
+    for (uint32_t c = lane; c < ncl; c += 64u) {
+        float u = FAR;
+        if (K == 1) {
+            const BakeTri &t0 = a.tris[size_t(c) * CLUSTER];
+            u = ub_box_point(bmin, bmax, mk3(t0.p0x, t0.p0y, t0.p0z));
+        } else if (min(a.ntris - c * CLUSTER, CLUSTER) >= uint32_t(K)) {
+            const float4 lo = a.clo[c], hi = a.chi[c];
+            u = ub_box_box(bmin, bmax, mk3(lo.x, lo.y, lo.z), mk3(hi.x, hi.y, hi.z));
         }
-        const float4 cl = a.clo[c], ch = a.chi[c];
-        // gap between the boxes, shortened by 1e-5 m and 1e-4 relative so that rounding of the
-        // exact distances can never make a skipped triangle the better candidate
-        const float gx = fmaxf(0.0f, fmaxf(cl.x - bmax.x, bmin.x - ch.x) - 1e-5f);
-        const float gy = fmaxf(0.0f, fmaxf(cl.y - bmax.y, bmin.y - ch.y) - 1e-5f);
-        const float gz = fmaxf(0.0f, fmaxf(cl.z - bmax.z, bmin.z - ch.z) - 1e-5f);
-        const float lb = (gx * gx + gy * gy + gz * gz) * 0.9999f;
-        if (lb > bound) continue;
-        const uint32_t t1 = min(a.ntris, uint32_t(c + 1) * CLUSTER);
-        for (uint32_t t = uint32_t(c) * CLUSTER; t < t1; ++t) {
-            const BakeTri tr = a.tris[t];
-            const uint32_t idx = a.index[t];
-            const f3 A = mk3(tr.p0x, tr.p0y, tr.p0z), B = mk3(tr.p1x, tr.p1y, tr.p1z), C = mk3(tr.p2x, tr.p2y, tr.p2z);
-            const f3 N = mk3(tr.nx, tr.ny, tr.nz);
+        if (u < ub) { ub = u; uc = c; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const float ou = __shfl_xor(ub, o, 64);
+        const uint32_t oc = __shfl_xor(uc, o, 64);
+        if (ou < ub || (ou == ub && oc < uc)) { ub = ou; uc = oc; }
+    }
+    const float ubound = ub >= FAR ? FAR : ub * 1.0001f + 1e-12f;
+
+    // the cluster being evaluated, staged in LDS by the whole wavefront (p0, p1, p2, n per triangle)
+    __shared__ float4 s_tri[4][CLUSTER * 4];
+    __shared__ uint32_t s_idx[4][CLUSTER];
+    const uint32_t wv = threadIdx.x >> 6;
+    auto eval_cluster = [&](uint32_t c) {
+        const float4 cl4 = a.clo[c], ch4 = a.chi[c];
+        const f3 cl = mk3(cl4.x, cl4.y, cl4.z), ch = mk3(ch4.x, ch4.y, ch4.z);
+        bool need = false;
+        for (int j = 0; j < VPL; ++j) need = need || (ok[j] && !(lb_box_point(cl, ch, p[j]) > bsq[j][K - 1]));
+        if (!__any(need)) return;
+        const uint32_t t0 = c * CLUSTER, cnt = min(a.ntris - t0, CLUSTER);
+        const float4 *src = reinterpret_cast<const float4 *>(a.tris + t0);
+        __builtin_amdgcn_wave_barrier();  // the previous cluster's reads come first
+        if (lane < cnt * 4u) s_tri[wv][lane] = src[lane];
+        if (lane + 64u < cnt * 4u) s_tri[wv][lane + 64u] = src[lane + 64u];
+        if (lane < cnt) s_idx[wv][lane] = a.index[t0 + lane];
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t t = 0; t < cnt; ++t) {
+            const float4 q0 = s_tri[wv][4 * t], q1 = s_tri[wv][4 * t + 1], q2 = s_tri[wv][4 * t + 2];
+            const f3 A = mk3(q0.x, q0.y, q0.z), B = mk3(q1.x, q1.y, q1.z), C = mk3(q2.x, q2.y, q2.z);
+            const f3 tmin = min3(A, min3(B, C)), tmax = max3(A, max3(B, C));
+            bool tn[VPL];
+            bool any = false;
             for (int j = 0; j < VPL; ++j) {
+                tn[j] = ok[j] && !(lb_box_point(tmin, tmax, p[j]) > bsq[j][K - 1]);
+                any = any || tn[j];
+            }
+            if (!__any(any)) continue;
+            const uint32_t idx = s_idx[wv][t];
+            const float4 q3 = s_tri[wv][4 * t + 3];
+            const f3 N = mk3(q3.x, q3.y, q3.z);
+            for (int j = 0; j < VPL; ++j) {
+                if (!tn[j]) continue;
                 const f3 q = closest_on_tri(p[j], A, B, C);
                 const f3 diff = p[j] - q;
                 const float dsq = dot(diff, diff);
@@ -461,6 +511,32 @@ __global__ void __launch_bounds__(256) k_bake_walk(DenseArgs a) {
                         }
                 }
             }
+        }
+    };
+
+    // 2. seed cluster, then the rest outward along the Morton order, 64 clusters per ballot
+    eval_cluster(uc);
+    const int32_t ng = int32_t((ncl + 63u) / 64u), g0 = int32_t(uc / 64u);
+    int32_t up = g0, down = g0 - 1;
+    for (uint32_t step = 0; up < ng || down >= 0; ++step) {
+        int32_t g;
+        if ((step & 1u) == 0u) { if (up < ng) g = up++; else g = down--; }
+        else { if (down >= 0) g = down--; else g = up++; }
+        float worst = -1.0f;
+        for (int j = 0; j < VPL; ++j)
+            if (ok[j]) worst = fmaxf(worst, bsq[j][K - 1]);
+        const float bound = fminf(ubound, wave_max_f(worst));
+        const uint32_t c = uint32_t(g) * 64u + lane;
+        bool cand = false;
+        if (c < ncl && c != uc) {
+            const float4 lo = a.clo[c], hi = a.chi[c];
+            cand = !(lb_box_box(mk3(lo.x, lo.y, lo.z), mk3(hi.x, hi.y, hi.z), bmin, bmax) > bound);
+        }
+        uint64_t m = __ballot(cand);
+        while (m) {
+            const uint32_t l = uint32_t(__ffsll((unsigned long long)m) - 1);
+            m &= m - 1u;
+            eval_cluster(uint32_t(g) * 64u + l);
         }
     }
     for (int j = 0; j < VPL; ++j) {
@@ -508,6 +584,168 @@ __global__ void __launch_bounds__(256) k_bake_walk(DenseArgs a) {
             for (int k = 0; k < 8; ++k) a.nearest[flat * 8 + k] = rid[k] == NO_TRI ? 0u : rid[k];
         }
     }
+}
+
+// ------------------------------------------------------------------ dense bake, segmented
+// The nearest triangle of a voxel is the minimum of the 64-bit key (squared distance bits << 32 |
+// original index): squared distances are non-negative, so their bits order like the values and
+// the key orders like (d_sq, index), the first strict minimum of the in-order scan.  Work items
+// are (brick of 8 x 8 x DZ voxels, segment of SEG clusters) so that bricks whose voxels see many
+// nearly equidistant triangles (the middle of a closed surface) spread over many wavefronts;
+// each keeps the current key minima of its voxels as culling bounds and merges with atomicMin.
+constexpr uint32_t DZ = 2, SEG = 128;
+
+__device__ __forceinline__ unsigned long long dense_key(float dsq, uint32_t idx) {
+    return (static_cast<unsigned long long>(__float_as_uint(dsq)) << 32) | idx;
+}
+
+struct Brick {
+    uint32_t vx, vy, vz0;
+    f3 p[DZ];
+    bool ok[DZ];
+    f3 bmin, bmax;
+};
+
+__device__ __forceinline__ Brick dense_brick(const DenseArgs &a, uint32_t brick, uint32_t lane) {
+    Brick b;
+    const uint32_t nbx = (a.w + 7) / 8, nby = (a.h + 7) / 8;
+    const uint32_t bx = brick % nbx, by = (brick / nbx) % nby, bz = brick / (nbx * nby);
+    b.vx = bx * 8 + (lane & 7u);
+    b.vy = by * 8 + (lane >> 3);
+    b.vz0 = bz * DZ;
+    const bool col_ok = b.vx < a.w && b.vy < a.h;
+    for (uint32_t j = 0; j < DZ; ++j) {
+        const uint32_t vz = b.vz0 + j;
+        b.ok[j] = col_ok && vz < a.d;
+        b.p[j] = mk3(voxel_coord(b.vx, a.w, a.lmin[0], a.lmax[0]), voxel_coord(b.vy, a.h, a.lmin[1], a.lmax[1]),
+                     voxel_coord(vz, a.d, a.lmin[2], a.lmax[2]));
+    }
+    const uint32_t x0 = bx * 8, y0 = by * 8, z0 = bz * DZ;
+    const uint32_t x1 = min(a.w, x0 + 8) - 1, y1 = min(a.h, y0 + 8) - 1, z1 = min(a.d, z0 + DZ) - 1;
+    b.bmin = mk3(voxel_coord(x0, a.w, a.lmin[0], a.lmax[0]), voxel_coord(y0, a.h, a.lmin[1], a.lmax[1]),
+                 voxel_coord(z0, a.d, a.lmin[2], a.lmax[2]));
+    b.bmax = mk3(voxel_coord(x1, a.w, a.lmin[0], a.lmax[0]), voxel_coord(y1, a.h, a.lmin[1], a.lmax[1]),
+                 voxel_coord(z1, a.d, a.lmin[2], a.lmax[2]));
+    return b;
+}
+
+__device__ __forceinline__ uint32_t dense_bricks(const DenseArgs &a) {
+    return ((a.w + 7) / 8) * ((a.h + 7) / 8) * ((a.d + DZ - 1) / DZ);
+}
+
+// Upper bound for every voxel of a brick: the farthest brick point from the first vertex of the
+// nearest cluster; the keys start there with no triangle.
+__global__ void __launch_bounds__(256) k_bake_dense_bound(DenseArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t brick = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (brick >= dense_bricks(a)) return;  // whole wavefront
+    const Brick b = dense_brick(a, brick, lane);
+    float ub = FAR;
+    for (uint32_t c = lane; c < a.nclusters; c += 64u) {
+        const BakeTri &t0 = a.tris[size_t(c) * CLUSTER];
+        ub = fminf(ub, ub_box_point(b.bmin, b.bmax, mk3(t0.p0x, t0.p0y, t0.p0z)));
+    }
+    const unsigned long long key = dense_key(wave_min_f(ub) * 1.0001f + 1e-12f, NO_TRI);
+    for (uint32_t j = 0; j < DZ; ++j)
+        if (b.ok[j]) a.keys[(size_t(b.vz0 + j) * a.h + b.vy) * a.w + b.vx] = key;
+}
+
+__global__ void __launch_bounds__(256) k_bake_dense_seg(DenseArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long nseg = (a.nclusters + SEG - 1) / SEG;
+    const unsigned long long item = a.item0 + blockIdx.x * 4ull + (threadIdx.x >> 6);
+    if (item >= (unsigned long long)dense_bricks(a) * nseg) return;  // whole wavefront
+    const uint32_t brick = uint32_t(item / nseg), seg = uint32_t(item % nseg);
+    const Brick b = dense_brick(a, brick, lane);
+    size_t flat[DZ];
+    float bsq[DZ], bsg_unused[DZ];
+    uint32_t bid[DZ];
+    unsigned long long start[DZ];
+    (void)bsg_unused;
+    for (uint32_t j = 0; j < DZ; ++j) {
+        flat[j] = (size_t(b.vz0 + j) * a.h + b.vy) * a.w + b.vx;
+        start[j] = b.ok[j] ? a.keys[flat[j]] : ~0ull;
+        bsq[j] = __uint_as_float(uint32_t(start[j] >> 32));
+        bid[j] = uint32_t(start[j]);
+    }
+    __shared__ float4 s_tri[4][CLUSTER * 4];
+    __shared__ uint32_t s_idx[4][CLUSTER];
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t c0 = seg * SEG, c1 = min(a.nclusters, c0 + SEG);
+    for (uint32_t base = c0; base < c1; base += 64u) {
+        float worst = -1.0f;
+        for (uint32_t j = 0; j < DZ; ++j)
+            if (b.ok[j]) worst = fmaxf(worst, bsq[j]);
+        const float bound = wave_max_f(worst);
+        const uint32_t c = base + lane;
+        bool cand = false;
+        if (c < c1) {
+            const float4 lo = a.clo[c], hi = a.chi[c];
+            cand = !(lb_box_box(mk3(lo.x, lo.y, lo.z), mk3(hi.x, hi.y, hi.z), b.bmin, b.bmax) > bound);
+        }
+        uint64_t m = __ballot(cand);
+        while (m) {
+            const uint32_t cc = base + uint32_t(__ffsll((unsigned long long)m) - 1);
+            m &= m - 1u;
+            const float4 cl4 = a.clo[cc], ch4 = a.chi[cc];
+            const f3 cl = mk3(cl4.x, cl4.y, cl4.z), ch = mk3(ch4.x, ch4.y, ch4.z);
+            bool need = false;
+            for (uint32_t j = 0; j < DZ; ++j) need = need || (b.ok[j] && !(lb_box_point(cl, ch, b.p[j]) > bsq[j]));
+            if (!__any(need)) continue;
+            const uint32_t t0 = cc * CLUSTER, cnt = min(a.ntris - t0, CLUSTER);
+            const float4 *src = reinterpret_cast<const float4 *>(a.tris + t0);
+            __builtin_amdgcn_wave_barrier();  // the previous cluster's reads come first
+            if (lane < cnt * 4u) s_tri[wv][lane] = src[lane];
+            if (lane + 64u < cnt * 4u) s_tri[wv][lane + 64u] = src[lane + 64u];
+            if (lane < cnt) s_idx[wv][lane] = a.index[t0 + lane];
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t t = 0; t < cnt; ++t) {
+                const float4 q0 = s_tri[wv][4 * t], q1 = s_tri[wv][4 * t + 1], q2 = s_tri[wv][4 * t + 2];
+                const f3 A = mk3(q0.x, q0.y, q0.z), B = mk3(q1.x, q1.y, q1.z), C = mk3(q2.x, q2.y, q2.z);
+                const f3 tmin = min3(A, min3(B, C)), tmax = max3(A, max3(B, C));
+                bool tn[DZ];
+                bool any = false;
+                for (uint32_t j = 0; j < DZ; ++j) {
+                    tn[j] = b.ok[j] && !(lb_box_point(tmin, tmax, b.p[j]) > bsq[j]);
+                    any = any || tn[j];
+                }
+                if (!__any(any)) continue;
+                const uint32_t idx = s_idx[wv][t];
+                for (uint32_t j = 0; j < DZ; ++j) {
+                    if (!tn[j]) continue;
+                    const f3 q = closest_on_tri(b.p[j], A, B, C);
+                    const f3 diff = b.p[j] - q;
+                    const float dsq = dot(diff, diff);
+                    if (better(dsq, idx, bsq[j], bid[j])) { bsq[j] = dsq; bid[j] = idx; }
+                }
+            }
+        }
+    }
+    for (uint32_t j = 0; j < DZ; ++j) {
+        if (!b.ok[j]) continue;
+        const unsigned long long k = dense_key(bsq[j], bid[j]);
+        if (k < start[j]) atomicMin(&a.keys[flat[j]], k);
+    }
+}
+
+// The winning triangle of each voxel, evaluated once more for the sign of its distance.
+__global__ void __launch_bounds__(256) k_bake_dense_finish(DenseArgs a) {
+    const size_t i = size_t(blockIdx.x) * 256u + threadIdx.x;
+    const size_t n = size_t(a.w) * a.h * a.d;
+    if (i >= n) return;
+    const uint32_t vx = uint32_t(i % a.w), vy = uint32_t((i / a.w) % a.h), vz = uint32_t(i / (size_t(a.w) * a.h));
+    const uint32_t idx = uint32_t(a.keys[i]);
+    if (idx >= a.ntris) {  // unreachable: the bound comes from a triangle that is always evaluated
+        a.out[i] = pack_dist(10.0f);
+        return;
+    }
+    const f3 p = mk3(voxel_coord(vx, a.w, a.lmin[0], a.lmax[0]), voxel_coord(vy, a.h, a.lmin[1], a.lmax[1]),
+                     voxel_coord(vz, a.d, a.lmin[2], a.lmax[2]));
+    const BakeTri tr = a.otris[idx];
+    const f3 q = closest_on_tri(p, mk3(tr.p0x, tr.p0y, tr.p0z), mk3(tr.p1x, tr.p1y, tr.p1z), mk3(tr.p2x, tr.p2y, tr.p2z));
+    const f3 diff = p - q;
+    const float dsq = dot(diff, diff);
+    a.out[i] = pack_dist((dot(diff, mk3(tr.nx, tr.ny, tr.nz)) >= 0.0f ? 1.0f : -1.0f) * sqrtf(dsq));
 }
 
 __device__ __forceinline__ void fine_candidate(f3 p, const BakeTri *tris, uint32_t ntris, uint32_t ti, uint32_t &best_i,
@@ -572,37 +810,94 @@ __global__ void __launch_bounds__(256) k_bake_fine(FineArgs a) {
             }
     bool ambiguous = sqrtf(second_sq) - sqrtf(best_sq) <= 0.0002f;
     if (ambiguous) ambiguous = (best_sg >= 0.0f) != (second_sg >= 0.0f);
-    if (ambiguous) {
-        best_sq = 1e10f;
-        best_sg = 10.0f;
-        for (uint32_t i = 0; i < a.ntris; ++i) {
-            const BakeTri tr = a.tris[i];
-            const f3 A = mk3(tr.p0x, tr.p0y, tr.p0z), B = mk3(tr.p1x, tr.p1y, tr.p1z), C = mk3(tr.p2x, tr.p2y, tr.p2z);
-            const f3 q = closest_on_tri(p, A, B, C);
-            const f3 diff = p - q;
-            const float dsq = dot(diff, diff);
-            if (dsq < best_sq) {
-                best_sq = dsq;
-                best_sg = sqrtf(dsq) * (dot(diff, mk3(tr.nx, tr.ny, tr.nz)) >= 0.0f ? 1.0f : -1.0f);
-            }
-        }
+    if (ambiguous) {  // exhaustive scan in triangle order: k_bake_resolve
+        a.amb[atomicAdd(a.amb_count, 1u)] = uint32_t(flat);
+        return;
     }
     a.out[flat] = pack_dist(best_sg);
 }
 
+
+__global__ void __launch_bounds__(256) k_bake_resolve(ResolveArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t item = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (item >= a.count) return;  // whole wavefront
+    const uint32_t flat = a.list[item];
+    const uint32_t vx = flat % a.w, vy = (flat / a.w) % a.h, vz = flat / (a.w * a.h);
+    // the voxel centre as the fine pass computes it
+    const f3 lmin = mk3(a.lmin[0], a.lmin[1], a.lmin[2]), lmax = mk3(a.lmax[0], a.lmax[1], a.lmax[2]);
+    const f3 size = lmax - lmin;
+    const f3 uvw = mk3((float(vx) + 0.5f) * (1.0f / float(a.w)), (float(vy) + 0.5f) * (1.0f / float(a.h)),
+                       (float(vz) + 0.5f) * (1.0f / float(a.d)));
+    const f3 p = mk3(lmin.x + uvw.x * size.x, lmin.y + uvw.y * size.y, lmin.z + uvw.z * size.z);
+    // upper bound: the nearest first vertex of a cluster
+    float ub = FAR;
+    for (uint32_t c = lane; c < a.nclusters; c += 64u) {
+        const BakeTri &t0 = a.tris[size_t(c) * CLUSTER];
+        const f3 d = p - mk3(t0.p0x, t0.p0y, t0.p0z);
+        ub = fminf(ub, dot(d, d));
+    }
+    float bound = wave_min_f(ub) * 1.0001f + 1e-12f;
+    float bsq = 1e10f, bsg = 10.0f;
+    uint32_t bid = NO_TRI;
+    const uint32_t rounds = (a.nclusters + 63u) / 64u;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        if ((r & 3u) == 0u) bound = fminf(bound, wave_min_f(bsq));
+        const uint32_t c = r * 64u + lane;
+        if (c >= a.nclusters) continue;
+        const float4 lo = a.clo[c], hi = a.chi[c];
+        if (lb_box_point(mk3(lo.x, lo.y, lo.z), mk3(hi.x, hi.y, hi.z), p) > bound) continue;
+        const uint32_t t1 = min(a.ntris, (c + 1u) * CLUSTER);
+        for (uint32_t t = c * CLUSTER; t < t1; ++t) {
+            const BakeTri tr = a.tris[t];
+            const f3 A = mk3(tr.p0x, tr.p0y, tr.p0z), B = mk3(tr.p1x, tr.p1y, tr.p1z), C = mk3(tr.p2x, tr.p2y, tr.p2z);
+            if (lb_box_point(min3(A, min3(B, C)), max3(A, max3(B, C)), p) > fminf(bound, bsq)) continue;
+            const f3 q = closest_on_tri(p, A, B, C);
+            const f3 diff = p - q;
+            const float dsq = dot(diff, diff);
+            const uint32_t idx = a.index[t];
+            if (!better(dsq, idx, bsq, bid)) continue;
+            bsq = dsq;
+            bid = idx;
+            bsg = sqrtf(dsq) * (dot(diff, mk3(tr.nx, tr.ny, tr.nz)) >= 0.0f ? 1.0f : -1.0f);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const float os = __shfl_xor(bsq, o, 64), og = __shfl_xor(bsg, o, 64);
+        const uint32_t oi = __shfl_xor(bid, o, 64);
+        if (better(os, oi, bsq, bid)) { bsq = os; bsg = og; bid = oi; }
+    }
+    if (lane == 0) a.out[flat] = pack_dist(bsg);
+}
+
 hipError_t launch_bake_dense(hipStream_t st, const DenseArgs &a) {
     if (a.topk == 1) {
-        const size_t nb = size_t((a.w + 7) / 8) * ((a.h + 7) / 8) * ((a.d + 7) / 8);
-        hipLaunchKernelGGL((k_bake_walk<8, 1>), dim3(uint32_t((nb + 3) / 4)), dim3(256), 0, st, a);
+        const size_t nb = size_t((a.w + 7) / 8) * ((a.h + 7) / 8) * ((a.d + DZ - 1) / DZ);
+        const size_t nseg = (a.nclusters + SEG - 1) / SEG, nvox = size_t(a.w) * a.h * a.d;
+        hipLaunchKernelGGL(k_bake_dense_bound, dim3(uint32_t((nb + 3) / 4)), dim3(256), 0, st, a);
+        const size_t items = nb * nseg, chunk = size_t(1) << 30;  // items per launch
+        for (size_t i0 = 0; i0 < items; i0 += chunk) {
+            DenseArgs c = a;
+            c.item0 = i0;
+            const size_t n = std::min(chunk, items - i0);
+            hipLaunchKernelGGL(k_bake_dense_seg, dim3(uint32_t((n + 3) / 4)), dim3(256), 0, st, c);
+        }
+        hipLaunchKernelGGL(k_bake_dense_finish, dim3(uint32_t((nvox + 255) / 256)), dim3(256), 0, st, a);
     } else {
         const size_t nb = size_t((a.w + 3) / 4) * ((a.h + 3) / 4) * ((a.d + 3) / 4);
-        hipLaunchKernelGGL((k_bake_walk<1, 16>), dim3(uint32_t((nb + 3) / 4)), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_bake_brick<1, 16>), dim3(uint32_t((nb + 3) / 4)), dim3(256), 0, st, a);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_bake_fine(hipStream_t st, const FineArgs &a) {
     hipLaunchKernelGGL(k_bake_fine, dim3((a.w + 7) / 8, (a.h + 7) / 8, (a.d + 3) / 4), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bake_resolve(hipStream_t st, const ResolveArgs &a) {
+    if (a.count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bake_resolve, dim3((a.count + 3u) / 4u), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
@@ -616,12 +911,30 @@ struct tc_sdf_result {
     uint32_t w = 0, h = 0, d = 0;
     float bone_info[20] = {0};
     float coarse_voxel = 0.0f;
+    float kernel_ms[2] = {0.0f, 0.0f};  // device time of the walk (dense or coarse) and of the fine pass
+    uint32_t fallback_voxels = 0;       // voxels the fine pass left to the exhaustive fallback
     std::vector<uint32_t> texels, nearest;
 };
 
 namespace {
 
 using tc::sdf::BakeTri;
+
+struct EventPair {  // device-side timing of one launch on the null stream
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    hipError_t init() {
+        hipError_t e = hipEventCreate(&a);
+        return e == hipSuccess ? hipEventCreate(&b) : e;
+    }
+    float ms() const {
+        float t = 0.0f;
+        return hipEventElapsedTime(&t, a, b) == hipSuccess ? t : 0.0f;
+    }
+};
 
 struct DevAlloc {  // device buffers of one bake, freed on every exit path
     std::vector<void *> ptrs;
@@ -634,57 +947,148 @@ struct DevAlloc {  // device buffers of one bake, freed on every exit path
     }
 };
 
-// Runs the walk bake (topk 1: dense, 8: coarse) of `tris` over domain `dm`; fills r.
-int run_walk(const std::vector<BakeTri> &tris, const tc::sdf::Domain &dm, uint32_t topk, tc_sdf_result &r) {
+// Bake triangles on the device: the Morton-ordered clusters the traversals walk and the original
+// order the fine pass indexes.
+struct DevTris {
+    DevAlloc mem;
+    BakeTri *sorted = nullptr, *orig = nullptr;
+    uint32_t *index = nullptr;
+    float4 *lo = nullptr, *hi = nullptr;
+    uint32_t n = 0, ncl = 0;
+};
+
+int upload_tris(const std::vector<BakeTri> &tris, const tc::sdf::Domain &dm, DevTris &d) {
+    const tc::sdf::Clusters cl = tc::sdf::build_clusters(tris, dm);
+    d.n = uint32_t(tris.size());
+    d.ncl = uint32_t(cl.lo.size());
+    TC_HIP(d.mem.get(&d.sorted, tris.size()));
+    TC_HIP(d.mem.get(&d.orig, tris.size()));
+    TC_HIP(d.mem.get(&d.index, tris.size()));
+    TC_HIP(d.mem.get(&d.lo, cl.lo.size()));
+    TC_HIP(d.mem.get(&d.hi, cl.hi.size()));
+    TC_HIP(hipMemcpy(d.sorted, cl.tris.data(), sizeof(BakeTri) * tris.size(), hipMemcpyHostToDevice));
+    TC_HIP(hipMemcpy(d.orig, tris.data(), sizeof(BakeTri) * tris.size(), hipMemcpyHostToDevice));
+    TC_HIP(hipMemcpy(d.index, cl.index.data(), 4 * tris.size(), hipMemcpyHostToDevice));
+    TC_HIP(hipMemcpy(d.lo, cl.lo.data(), 16 * cl.lo.size(), hipMemcpyHostToDevice));
+    TC_HIP(hipMemcpy(d.hi, cl.hi.data(), 16 * cl.hi.size(), hipMemcpyHostToDevice));
+    return TC_OK;
+}
+
+int check_memory(const tc::sdf::Domain &dm, size_t bytes_per_voxel, size_t ntris) {
     const size_t nvox = size_t(dm.w) * dm.h * dm.d;
     size_t free_b = 0, total_b = 0;
     TC_HIP(hipMemGetInfo(&free_b, &total_b));
-    const size_t need = nvox * 4 * (topk == 8 ? 9 : 1) + tris.size() * 80;
+    const size_t need = nvox * bytes_per_voxel + ntris * 160;
     if (need > free_b / 10 * 9)
         return fail(TC_ERR_RUNTIME, "SDF resolution " + std::to_string(dm.w) + "x" + std::to_string(dm.h) + "x" +
                                         std::to_string(dm.d) + " needs " + std::to_string(need >> 20) +
                                         " MiB of device memory; use a larger voxel size");
-    const tc::sdf::Clusters cl = tc::sdf::build_clusters(tris, dm);
-    DevAlloc dev;
-    BakeTri *dt = nullptr;
-    BakeTri *dot_ = nullptr;
-    uint32_t *didx = nullptr, *dcode = nullptr, *dout = nullptr, *dnear = nullptr;
-    float4 *dlo = nullptr, *dhi = nullptr;
-    TC_HIP(dev.get(&dt, cl.tris.size()));
-    TC_HIP(dev.get(&didx, cl.index.size()));
-    TC_HIP(dev.get(&dcode, cl.code.size()));
-    TC_HIP(dev.get(&dlo, cl.lo.size()));
-    TC_HIP(dev.get(&dhi, cl.hi.size()));
-    TC_HIP(dev.get(&dout, nvox));
-    if (topk == 8) {
-        TC_HIP(dev.get(&dnear, nvox * 8));
-        TC_HIP(dev.get(&dot_, tris.size()));
-        TC_HIP(hipMemcpy(dot_, tris.data(), sizeof(BakeTri) * tris.size(), hipMemcpyHostToDevice));
-    }
-    TC_HIP(hipMemcpy(dt, cl.tris.data(), sizeof(BakeTri) * cl.tris.size(), hipMemcpyHostToDevice));
-    TC_HIP(hipMemcpy(didx, cl.index.data(), 4 * cl.index.size(), hipMemcpyHostToDevice));
-    TC_HIP(hipMemcpy(dcode, cl.code.data(), 4 * cl.code.size(), hipMemcpyHostToDevice));
-    TC_HIP(hipMemcpy(dlo, cl.lo.data(), 16 * cl.lo.size(), hipMemcpyHostToDevice));
-    TC_HIP(hipMemcpy(dhi, cl.hi.data(), 16 * cl.hi.size(), hipMemcpyHostToDevice));
+    return TC_OK;
+}
+
+// The brick traversal (topk 1: dense bake, 8: coarse pass) over domain dm into device buffers.
+int launch_walk(const DevTris &t, const tc::sdf::Domain &dm, uint32_t topk, uint32_t *dout, uint32_t *dnear,
+                unsigned long long *keys, float &ms) {
     tc::sdf::DenseArgs a;
-    a.tris = dt; a.otris = dot_; a.index = didx; a.clo = dlo; a.chi = dhi; a.ccode = dcode;
-    a.ntris = uint32_t(cl.tris.size()); a.nclusters = uint32_t(cl.lo.size());
-    for (int k = 0; k < 3; ++k) { a.mo[k] = cl.origin[k]; a.lmin[k] = dm.min[k]; a.lmax[k] = dm.max[k]; }
-    a.ms = cl.scale;
+    a.tris = t.sorted; a.otris = t.orig; a.index = t.index; a.clo = t.lo; a.chi = t.hi;
+    a.ntris = t.n; a.nclusters = t.ncl;
+    for (int k = 0; k < 3; ++k) { a.lmin[k] = dm.min[k]; a.lmax[k] = dm.max[k]; }
     a.w = dm.w; a.h = dm.h; a.d = dm.d;
     a.topk = topk;
     a.out = dout;
     a.nearest = dnear;
+    a.keys = keys;
+    a.item0 = 0;
+    EventPair ev;
+    TC_HIP(ev.init());
+    TC_HIP(hipEventRecord(ev.a, nullptr));
     TC_HIP(tc::sdf::launch_bake_dense(nullptr, a));
+    TC_HIP(hipEventRecord(ev.b, nullptr));
     TC_HIP(hipDeviceSynchronize());
+    ms = ev.ms();
+    return TC_OK;
+}
+
+int download(tc_sdf_result &r, const tc::sdf::Domain &dm, const uint32_t *dout, const uint32_t *dnear) {
+    const size_t nvox = size_t(dm.w) * dm.h * dm.d;
     r.w = dm.w; r.h = dm.h; r.d = dm.d;
     r.texels.resize(nvox);
     TC_HIP(hipMemcpy(r.texels.data(), dout, 4 * nvox, hipMemcpyDeviceToHost));
-    if (topk == 8) {
+    if (dnear) {
         r.nearest.resize(nvox * 8);
         TC_HIP(hipMemcpy(r.nearest.data(), dnear, 4 * nvox * 8, hipMemcpyDeviceToHost));
     }
     return TC_OK;
+}
+
+int bake_dense(const std::vector<BakeTri> &bt, const tc::sdf::Domain &dm, uint32_t topk, tc_sdf_result &r) {
+    const size_t nvox = size_t(dm.w) * dm.h * dm.d;
+    if (int rc = check_memory(dm, topk == 8 ? 36 : 12, bt.size())) return rc;
+    DevTris t;
+    if (int rc = upload_tris(bt, dm, t)) return rc;
+    DevAlloc mem;
+    uint32_t *dout = nullptr, *dnear = nullptr;
+    unsigned long long *keys = nullptr;
+    TC_HIP(mem.get(&dout, nvox));
+    if (topk == 8) TC_HIP(mem.get(&dnear, nvox * 8));
+    else TC_HIP(mem.get(&keys, nvox));
+    if (int rc = launch_walk(t, dm, topk, dout, dnear, keys, r.kernel_ms[0])) return rc;
+    return download(r, dm, dout, dnear);
+}
+
+// bake_mesh_sdf_hierarchical_ext (sdf_baker.rs:1625-1882): coarse top-8 pass, fine pass, and the
+// fine pass's exhaustive fallback for the voxels it leaves.
+int bake_hierarchical(const std::vector<BakeTri> &bt, const tc::sdf::Domain &dm, const tc::sdf::Domain &cdm,
+                      tc_sdf_result &r) {
+    const float h_fine = std::fmax(std::fmax(dm.size[0] / float(dm.w), dm.size[1] / float(dm.h)), dm.size[2] / float(dm.d));
+    const float band = 4.0f * h_fine;
+    const float h_coarse_min =
+        std::fmin(std::fmin(cdm.size[0] / float(cdm.w), cdm.size[1] / float(cdm.h)), cdm.size[2] / float(cdm.d));
+    if (band > 2.0f * h_coarse_min)
+        return fail(TC_ERR_RUNTIME, "hierarchical bake: band " + std::to_string(band) +
+                                        " m is too wide for coarse cells of " + std::to_string(h_coarse_min) + " m");
+    const size_t nvox = size_t(dm.w) * dm.h * dm.d, ncvox = size_t(cdm.w) * cdm.h * cdm.d;
+    if (int rc = check_memory(dm, 9, bt.size())) return rc;  // out + fallback list, coarse grid ~1/64
+    DevTris t;
+    if (int rc = upload_tris(bt, dm, t)) return rc;
+    DevAlloc mem;
+    uint32_t *dcd = nullptr, *dct = nullptr, *dout = nullptr, *damb = nullptr, *dcount = nullptr;
+    TC_HIP(mem.get(&dcd, ncvox));
+    TC_HIP(mem.get(&dct, ncvox * 8));
+    TC_HIP(mem.get(&dout, nvox));
+    TC_HIP(mem.get(&damb, nvox));
+    TC_HIP(mem.get(&dcount, 1));
+    TC_HIP(hipMemset(dcount, 0, 4));
+    if (int rc = launch_walk(t, cdm, 8, dcd, dct, nullptr, r.kernel_ms[0])) return rc;
+    tc::sdf::FineArgs fa;
+    fa.tris = t.orig; fa.ntris = t.n;
+    fa.coarse_dist = dcd; fa.coarse_tri = dct;
+    fa.cw = cdm.w; fa.ch = cdm.h; fa.cd = cdm.d;
+    fa.band = band;
+    for (int k = 0; k < 3; ++k) { fa.lmin[k] = dm.min[k]; fa.lmax[k] = dm.max[k]; }
+    fa.w = dm.w; fa.h = dm.h; fa.d = dm.d;
+    fa.out = dout;
+    fa.amb = damb;
+    fa.amb_count = dcount;
+    EventPair ev;
+    TC_HIP(ev.init());
+    TC_HIP(hipEventRecord(ev.a, nullptr));
+    TC_HIP(tc::sdf::launch_bake_fine(nullptr, fa));
+    uint32_t count = 0;
+    TC_HIP(hipMemcpy(&count, dcount, 4, hipMemcpyDeviceToHost));
+    tc::sdf::ResolveArgs ra;
+    ra.tris = t.sorted; ra.index = t.index; ra.clo = t.lo; ra.chi = t.hi;
+    ra.ntris = t.n; ra.nclusters = t.ncl;
+    ra.list = damb; ra.count = count;
+    for (int k = 0; k < 3; ++k) { ra.lmin[k] = dm.min[k]; ra.lmax[k] = dm.max[k]; }
+    ra.w = dm.w; ra.h = dm.h; ra.d = dm.d;
+    ra.out = dout;
+    TC_HIP(tc::sdf::launch_bake_resolve(nullptr, ra));
+    TC_HIP(hipEventRecord(ev.b, nullptr));
+    TC_HIP(hipDeviceSynchronize());
+    r.kernel_ms[1] = ev.ms();
+    r.fallback_voxels = count;
+    return download(r, dm, dout, nullptr);
 }
 
 }  // namespace
@@ -702,70 +1106,26 @@ int tc_bake_mesh_sdf(const float *verts, uint32_t nv, const int32_t *tris, uint3
     for (float x : v)
         if (!std::isfinite(x)) return fail(TC_ERR_VALUE, "vertex coordinates must be finite");
     if (close_open_boundaries) tc::sdf::close_open_boundary_loops(v, t);
-    tc_sdf_result *r = new tc_sdf_result();
+    std::unique_ptr<tc_sdf_result> r(new tc_sdf_result());
     const std::vector<BakeTri> bt = tc::sdf::collect_bake_triangles(v, t);
+    const float cv = std::fmax(voxel_size * 4.0f, 0.001f);  // coarse cell: 4 voxels
     int rc = TC_OK;
     if (method == 0) {
         const tc::sdf::Domain dm = tc::sdf::mesh_sdf_domain(v, voxel_size, margin, thickness);
         tc::sdf::mesh_sdf_bone_info(dm, friction, thickness, restitution, r->bone_info);
-        rc = run_walk(bt, dm, 1, *r);
+        rc = bake_dense(bt, dm, 1, *r);
     } else if (method == 2) {
-        const float cv = std::fmax(voxel_size * 4.0f, 0.001f);
         const tc::sdf::Domain dm = tc::sdf::mesh_sdf_domain(v, cv, margin, thickness);
         tc::sdf::mesh_sdf_bone_info(dm, friction, thickness, restitution, r->bone_info);
         r->coarse_voxel = cv;
-        rc = run_walk(bt, dm, 8, *r);
+        rc = bake_dense(bt, dm, 8, *r);
     } else {
-        // bake_mesh_sdf_hierarchical_ext (sdf_baker.rs:1625-1882)
         const tc::sdf::Domain dm = tc::sdf::mesh_sdf_domain(v, voxel_size, margin, thickness);
         tc::sdf::mesh_sdf_bone_info(dm, friction, thickness, restitution, r->bone_info);
-        const float cv = std::fmax(voxel_size * 4.0f, 0.001f);
-        const tc::sdf::Domain cdm = tc::sdf::mesh_sdf_domain(v, cv, margin, thickness);
-        tc_sdf_result coarse;
-        rc = run_walk(bt, cdm, 8, coarse);
-        if (rc == TC_OK) {
-            const float h_fine = std::fmax(std::fmax(dm.size[0] / float(dm.w), dm.size[1] / float(dm.h)),
-                                           dm.size[2] / float(dm.d));
-            const float band = 4.0f * h_fine;
-            const float h_coarse_min = std::fmin(std::fmin(cdm.size[0] / float(cdm.w), cdm.size[1] / float(cdm.h)),
-                                                 cdm.size[2] / float(cdm.d));
-            if (band > 2.0f * h_coarse_min) {
-                rc = fail(TC_ERR_RUNTIME, "hierarchical bake: band " + std::to_string(band) +
-                                              " m is too wide for coarse cells of " + std::to_string(h_coarse_min) + " m");
-            } else {
-                const size_t nvox = size_t(dm.w) * dm.h * dm.d;
-                DevAlloc dev;
-                BakeTri *dt = nullptr;
-                uint32_t *dcd = nullptr, *dct = nullptr, *dout = nullptr;
-                hipError_t e = hipSuccess;
-                if ((e = dev.get(&dt, bt.size())) == hipSuccess && (e = dev.get(&dcd, coarse.texels.size())) == hipSuccess &&
-                    (e = dev.get(&dct, coarse.nearest.size())) == hipSuccess && (e = dev.get(&dout, nvox)) == hipSuccess &&
-                    (e = hipMemcpy(dt, bt.data(), sizeof(BakeTri) * bt.size(), hipMemcpyHostToDevice)) == hipSuccess &&
-                    (e = hipMemcpy(dcd, coarse.texels.data(), 4 * coarse.texels.size(), hipMemcpyHostToDevice)) == hipSuccess &&
-                    (e = hipMemcpy(dct, coarse.nearest.data(), 4 * coarse.nearest.size(), hipMemcpyHostToDevice)) == hipSuccess) {
-                    tc::sdf::FineArgs fa;
-                    fa.tris = dt; fa.ntris = uint32_t(bt.size());
-                    fa.coarse_dist = dcd; fa.coarse_tri = dct;
-                    fa.cw = cdm.w; fa.ch = cdm.h; fa.cd = cdm.d;
-                    fa.band = band;
-                    for (int k = 0; k < 3; ++k) { fa.lmin[k] = dm.min[k]; fa.lmax[k] = dm.max[k]; }
-                    fa.w = dm.w; fa.h = dm.h; fa.d = dm.d;
-                    fa.out = dout;
-                    if ((e = tc::sdf::launch_bake_fine(nullptr, fa)) == hipSuccess && (e = hipDeviceSynchronize()) == hipSuccess) {
-                        r->w = dm.w; r->h = dm.h; r->d = dm.d;
-                        r->texels.resize(nvox);
-                        e = hipMemcpy(r->texels.data(), dout, 4 * nvox, hipMemcpyDeviceToHost);
-                    }
-                }
-                if (e != hipSuccess) rc = fail(TC_ERR_RUNTIME, std::string("hierarchical bake: ") + hipGetErrorString(e));
-            }
-        }
+        rc = bake_hierarchical(bt, dm, tc::sdf::mesh_sdf_domain(v, cv, margin, thickness), *r);
     }
-    if (rc != TC_OK) {
-        delete r;
-        return rc;
-    }
-    *out = r;
+    if (rc != TC_OK) return rc;
+    *out = r.release();
     return TC_OK;
 }
 
@@ -774,6 +1134,16 @@ int tc_sdf_result_info(const tc_sdf_result *r, uint32_t *dims, float *bone_info,
     if (dims) { dims[0] = r->w; dims[1] = r->h; dims[2] = r->d; }
     if (bone_info) std::memcpy(bone_info, r->bone_info, sizeof(r->bone_info));
     if (coarse_voxel) *coarse_voxel = r->coarse_voxel;
+    return TC_OK;
+}
+
+int tc_sdf_result_timing(const tc_sdf_result *r, float *kernel_ms, uint32_t *fallback_voxels) {
+    if (!r) return fail(TC_ERR_VALUE, "null result");
+    if (kernel_ms) {
+        kernel_ms[0] = r->kernel_ms[0];
+        kernel_ms[1] = r->kernel_ms[1];
+    }
+    if (fallback_voxels) *fallback_voxels = r->fallback_voxels;
     return TC_OK;
 }
 

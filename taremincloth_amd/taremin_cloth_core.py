@@ -430,10 +430,14 @@ def _bake(mesh_verts, mesh_tris, voxel_size, margin, friction, thickness, restit
         info = (C.c_float * 20)()
         cv = C.c_float(0.0)
         _lib.check(L.tc_sdf_result_info(res, dims, info, C.byref(cv)))
+        kms = (C.c_float * 2)()
+        nfb = C.c_uint32(0)
+        _lib.check(L.tc_sdf_result_timing(res, kms, C.byref(nfb)))
         n = dims[0] * dims[1] * dims[2]
         tex = np.ctypeslib.as_array(C.cast(L.tc_sdf_result_texels(res), C.POINTER(C.c_uint32)), (n,)).copy()
         out = {"texture_bytes": tex.tobytes(), "width": int(dims[0]), "height": int(dims[1]), "depth": int(dims[2]),
-               "bone_info": np.array(info, dtype=np.float32)}
+               "bone_info": np.array(info, dtype=np.float32), "kernel_ms": (float(kms[0]), float(kms[1])),
+               "fallback_voxels": int(nfb.value)}
         if method == 2:
             near = np.ctypeslib.as_array(C.cast(L.tc_sdf_result_nearest(res), C.POINTER(C.c_uint32)), (n * 8,))
             out["nearest_tri"] = near.copy()

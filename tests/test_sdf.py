@@ -371,6 +371,7 @@ def test_gpu_bake_bit_exact(require_gpu, method, mesh, voxel, margin, thickness,
             _product_bake(T, method, v, t, voxel, margin, 0.4, thickness, 0.1, close)
         return
     got = _product_bake(T, method, v, t, voxel, margin, 0.4, thickness, 0.1, close)
+    _say(f"  fallback voxels {got.get('fallback_voxels')}, kernel ms {got.get('kernel_ms')}")
     assert (got["width"], got["height"], got["depth"]) == (want["width"], want["height"], want["depth"])
     np.testing.assert_array_equal(np.asarray(got["bone_info"], np.float32), want["bone_info"])
     gt = np.frombuffer(got["texture_bytes"], np.uint32)
