@@ -192,6 +192,8 @@ def main():
                          "baseline (for counter runs, where every dispatch is serialised); the tile plan comes "
                          "from TCLOTH_TILE_VERTICES / TCLOTH_TILE_BLOCK or the defaults")
     ap.add_argument("--no-sdf", action="store_true", help="skip the mesh SDF bake measurement")
+    ap.add_argument("--sdf-only", action="store_true",
+                    help="measure only the mesh SDF bakes and print their JSON object (for profiler runs)")
     args = ap.parse_args()
     if args.light:
         args.no_cpu_baseline = True
@@ -202,6 +204,10 @@ def main():
 
     from taremincloth_amd import taremin_cloth_core as T
     from taremincloth_amd.warmstart import WarmStartNet
+
+    if args.sdf_only:
+        print(json.dumps(sdf_bake_bench(T, not args.no_cpu_baseline)), flush=True)
+        return
 
     n = args.grid
     pos, edges, faces = grid_512(n)
