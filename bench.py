@@ -174,6 +174,68 @@ def sdf_bake_bench(T, cpu_sample: bool):
     return out
 
 
+def batch_bench(T, frames: int, warmup: int, cpu_sample: bool):
+    """64 independent curtains of side U[16, 256] (seed 0, 1 cm spacing, top corners pinned) in one
+    simulator: 10 substeps x 20 iterations per frame, one launch per iteration over all
+    instances, frames replayed from captured HIP graphs, tile and block size autotuned."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import cases
+    inst = cases.batch_instances(64, 16, 256, seed=0)
+    b = T.ClothBatch(inst)
+    b.set_gravity(0.0, 0.0, -9.81)
+    b.set_compliances_raw(1e-8, 1e-6, 1e-5)
+    b.set_solver_iterations(20)
+    per_frame, dt_frame = 10, 1.0 / 60.0
+    dt_sub = dt_frame / per_frame
+    tune = b.autotune(dt_sub)
+    info = b.solver_info()
+    for _ in range(warmup):
+        b.step(dt_frame, per_frame)
+    b.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        b.step(dt_frame, per_frame)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    sps = frames * per_frame / el
+    nd, nb, nv = b.num_distance_constraints(), b.num_bending_constraints(), b.num_vertices()
+    kt = b.time_kernels(dt_sub, reps=3)
+    sides = [int(round(len(d["positions"]) ** 0.5)) for d in inst]
+    out = {"workload": f"64 curtains, side U[16, 256] (seed 0; {min(sides)}..{max(sides)}), {nv} vertices, "
+                       f"{nd + nb} constraints; 10 substeps x 20 iterations per frame; {frames} timed frames "
+                       f"after {warmup}",
+           "vertices": nv, "constraints": nd + nb, "substeps_per_s": sps, "ms_per_substep": 1e3 / sps,
+           "constraint_projections_per_s": (nd + nb) * 20 * sps,
+           "schedule": {"path": "one launch per iteration (cone tiles)" if info["path"] == 2 else "one launch per colour",
+                        "tiles": int(info["tiles"]), "tile_vertices": int(info["tile_vertices"]),
+                        "block": int(info["block"]), "autotune": tune},
+           "substep_ms_breakdown": {k: round(v, 5) for k, v in kt.items()}}
+    if info["path"] == 2 and kt["fused_launches"] > 0:
+        launch_ms = kt["fused"] / kt["fused_launches"]
+        by = nd * BYTES_DIST + nb * BYTES_BEND
+        out["roofline"] = {"bound": "hbm", "kernel": "k_tile_iteration over all instances", "bytes_per_launch": by,
+                           "launch_ms": launch_ms, "achieved": by / (launch_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": by / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if cpu_sample:
+        from oracle import oracle as O
+        d = inst[0]
+        o = O.OracleSim(d["positions"], d["edges"], d["faces"], d["inv_masses"])
+        o.set_compliances_raw(1e-8, 1e-6, 1e-5)
+        o.set_enable_self_collision(False)
+        o.set_solver_iterations(20)
+        k, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < 5.0:
+            o.step_single_substep(dt_sub)
+            k += 1
+        cel = time.perf_counter() - t1
+        rate = k * (len(d["positions"]) / nv) / cel  # whole-batch substeps/s at this per-vertex cost
+        out["cpu_baseline"] = {"value": rate, "unit": "substeps/s", "cores": 1, "kind": "port",
+                               "sample": f"fp32 oracle, one thread: {k} substeps of instance 0 "
+                                         f"({len(d['positions'])} vertices) in {cel:.1f} s, scaled to the batch's "
+                                         f"{nv} vertices"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +254,9 @@ def main():
                          "baseline (for counter runs, where every dispatch is serialised); the tile plan comes "
                          "from TCLOTH_TILE_VERTICES / TCLOTH_TILE_BLOCK or the defaults")
     ap.add_argument("--no-sdf", action="store_true", help="skip the mesh SDF bake measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances measurement")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="measure only the batch of 64 instances and print its JSON object (for profiler runs)")
     ap.add_argument("--sdf-only", action="store_true",
                     help="measure only the mesh SDF bakes and print their JSON object (for profiler runs)")
     args = ap.parse_args()
@@ -207,6 +272,9 @@ def main():
 
     if args.sdf_only:
         print(json.dumps(sdf_bake_bench(T, not args.no_cpu_baseline)), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps(batch_bench(T, 30, 5, not args.no_cpu_baseline)), flush=True)
         return
 
     n = args.grid
@@ -329,6 +397,7 @@ def main():
     roofline = net_rl if wt["round_mlps"] >= wt["iterations"] else solver_rl
 
     sdf = None if args.no_sdf else sdf_bake_bench(T, not args.no_cpu_baseline)
+    batch = None if args.no_batch else batch_bench(T, 30, 5, not args.no_cpu_baseline)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -386,6 +455,7 @@ def main():
         "roofline_solver": solver_rl,
         "cpu_baseline": cpu,
         "sdf_bake": sdf,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
 

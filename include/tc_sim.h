@@ -62,6 +62,32 @@ int tc_sim_create(const float *positions, uint32_t num_vertices,
                   const tc_sim_params *params, tc_sim **out);
 void tc_sim_destroy(tc_sim *sim);
 
+/* One cloth instance of a batch: the arguments of tc_sim_create for that cloth (vertex indices are
+ * local to the instance; inv_masses / thicknesses / faces may be NULL). */
+typedef struct {
+    const float *positions;
+    uint32_t num_vertices;
+    const uint32_t *edges;
+    uint32_t num_edges;
+    const uint32_t *faces;
+    uint32_t num_faces;
+    const float *inv_masses;
+    const float *thicknesses;
+} tc_instance;
+
+/* Batch of independent cloth instances in one simulator: no reference counterpart (the reference
+ * runs one GpuClothSimulator per cloth, simulation/mod.rs:363).  Each instance is built and
+ * coloured on its own; the simulator holds their disjoint union, colour c of the union being
+ * colour c of every instance, so one launch per colour (or per iteration) advances all of them
+ * with each instance's own Gauss-Seidel order.  Tiles of the per-iteration path never mix two
+ * instances.  Self-collision starts disabled (instances that overlap in space would collide);
+ * colliders, gravity and damping are shared.  Every other tc_sim_* call works on the union;
+ * tc_sim_instance_range gives each instance's vertex range in it. */
+int tc_sim_create_batch(const tc_instance *instances, uint32_t num_instances, const tc_sim_params *params,
+                        tc_sim **out);
+int tc_sim_instance_count(const tc_sim *sim, uint32_t *num_instances);
+int tc_sim_instance_range(const tc_sim *sim, uint32_t instance, uint32_t *first_vertex, uint32_t *num_vertices);
+
 /* ClothSimulator.step(dt, substeps, solver_iterations), lib.rs:1768-1786.
  * substeps 0 -> 20 (base_substeps); iterations 0 -> keep current.
  * Asynchronous on the simulator's stream; *effective_substeps may be NULL. */

@@ -13,7 +13,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libtcloth_hip.so")
 
 # Every symbol include/tc_sim.h declares (checked by the test suite).
 EXPORTS = [
-    "tc_sim_params_default", "tc_sim_create", "tc_sim_destroy", "tc_sim_step", "tc_sim_step_single_substep",
+    "tc_sim_params_default", "tc_sim_create", "tc_sim_create_batch", "tc_sim_instance_count",
+    "tc_sim_instance_range", "tc_sim_destroy", "tc_sim_step", "tc_sim_step_single_substep",
     "tc_sim_get_positions", "tc_sim_set_positions_velocities", "tc_sim_reset", "tc_sim_add_sphere",
     "tc_sim_add_capsule", "tc_sim_add_plane", "tc_sim_clear_colliders", "tc_sim_set_mesh_triangles",
     "tc_sim_set_gravity", "tc_sim_set_damping", "tc_sim_set_damping_all", "tc_sim_set_solver_iterations",
@@ -33,6 +34,12 @@ EXPORTS = [
 ]
 
 TC_OK, TC_ERR_VALUE, TC_ERR_RUNTIME, TC_ERR_UNSUPPORTED = 0, 1, 2, 3
+
+
+class TcInstance(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("num_vertices", C.c_uint32), ("edges", C.c_void_p),
+                ("num_edges", C.c_uint32), ("faces", C.c_void_p), ("num_faces", C.c_uint32),
+                ("inv_masses", C.c_void_p), ("thicknesses", C.c_void_p)]
 
 
 class TcParams(C.Structure):
@@ -65,6 +72,9 @@ def load() -> C.CDLL:
     sig = {
         "tc_sim_params_default": (None, [C.POINTER(TcParams)]),
         "tc_sim_create": (_i, [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp, C.POINTER(TcParams), C.POINTER(_vp)]),
+        "tc_sim_create_batch": (_i, [_vp, _u32, C.POINTER(TcParams), C.POINTER(_vp)]),
+        "tc_sim_instance_count": (_i, [_vp, C.POINTER(_u32)]),
+        "tc_sim_instance_range": (_i, [_vp, _u32, C.POINTER(_u32), C.POINTER(_u32)]),
         "tc_sim_destroy": (None, [_vp]),
         "tc_sim_step": (_i, [_vp, _f32, _u32, _u32, C.POINTER(_u32)]),
         "tc_sim_step_single_substep": (_i, [_vp, _f32]),

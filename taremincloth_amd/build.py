@@ -17,7 +17,7 @@ LIBNAME = "libtcloth_hip.so"
 ARCH = os.environ.get("TC_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["tc_sim.hip", "tc_xpbd.hip", "tc_selfcol.hip", "tc_fused.hip", "tc_warmstart.hip", "tc_ws_api.hip", "tc_sdf_bake.hip"]
-CXX_SOURCES = ["tc_mesh.cpp", "tc_tiles.cpp"]
+CXX_SOURCES = ["tc_mesh.cpp", "tc_tiles.cpp", "tc_batch.cpp"]
 
 # Strict IEEE fp32 (no FMA contraction) so results follow the reference's
 # operation order; packed-fp32 instructions are disabled.

@@ -57,7 +57,16 @@ struct Mesh {
     float cell_size = 0.01f;
     uint32_t table_size = 1024;
     bool has_multiple_layers = false;
+
+    // independent parts (instances of a batch): vertex offsets, nparts + 1 entries; empty = one part
+    std::vector<uint32_t> part_off;
 };
+
+// Disjoint union of independently built meshes (a batch of cloth instances).  Vertices are
+// concatenated; colour c of the union holds colour c of every instance, instance by instance, so
+// each instance keeps its own colouring and its own Gauss-Seidel order (constraints of one colour
+// share no vertex, so their order inside the colour does not matter).
+void merge_meshes(const std::vector<Mesh> &parts, Mesh &out);
 
 // Builds the mesh. Returns an empty string on success, else an error message.
 std::string build_mesh(const MeshInput &in, Mesh &out);

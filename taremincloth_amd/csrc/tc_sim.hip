@@ -277,15 +277,26 @@ void tc_sim_params_default(tc_sim_params *p) {
     p->solver_mode = 0;
 }
 
-int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
-                  uint32_t nf, const float *inv_masses, const float *thicknesses, const uint32_t *layer_ids,
-                  const tc_sim_params *params, tc_sim **out) {
-    g_err.clear();
-    if (!out) return fail(TC_ERR_VALUE, "out handle pointer is null");
-    *out = nullptr;
-    tc_sim_params def;
-    tc_sim_params_default(&def);
-    const tc_sim_params &pp = params ? *params : def;
+static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out);
+
+static tc::MeshInput mesh_input(const float *positions, uint32_t nv, const uint32_t *edges, uint32_t ne,
+                                const uint32_t *faces, uint32_t nf, const float *inv_masses, const float *thicknesses,
+                                const uint32_t *layer_ids, const tc_sim_params &pp) {
+    tc::MeshInput in;
+    in.positions = positions; in.nv = nv;
+    in.edges = edges; in.ne = ne;
+    in.faces = (faces && nf) ? faces : nullptr; in.nf = (faces && nf) ? nf : 0;
+    in.inv_masses = inv_masses; in.thicknesses = thicknesses; in.layer_ids = layer_ids;
+    in.layer_id = pp.layer_id; in.thickness = pp.thickness;
+    in.tension_k = pp.stiffness;
+    in.compression_k = pp.compression_stiffness >= 0.0f ? pp.compression_stiffness : pp.stiffness;
+    in.shear_k = pp.shear_stiffness >= 0.0f ? pp.shear_stiffness : pp.stiffness * 0.5f;
+    in.bending_k = pp.bending_stiffness;
+    return in;
+}
+
+static int check_topology(const float *positions, uint32_t nv, const uint32_t *edges, uint32_t ne,
+                          const uint32_t *faces, uint32_t nf) {
     if (nv > 0 && !positions) return fail(TC_ERR_VALUE, "positions is null");
     if (ne > 0 && !edges) return fail(TC_ERR_VALUE, "edges is null");
     if (nv >= 0x80000000u) return fail(TC_ERR_VALUE, "too many vertices (limit 2^31 - 1)");
@@ -298,22 +309,31 @@ int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, ui
             if (faces[f] >= nv)
                 return fail(TC_ERR_VALUE, "faces contain an index >= number of vertices (" + std::to_string(nv) +
                                               "): " + std::to_string(faces[f]));
-    if (pp.solver_mode != 0) return fail(TC_ERR_UNSUPPORTED, "only solver_mode=0 (graph colouring) is supported");
+    return TC_OK;
+}
 
-    tc::MeshInput in;
-    in.positions = positions; in.nv = nv;
-    in.edges = edges; in.ne = ne;
-    in.faces = (faces && nf) ? faces : nullptr; in.nf = (faces && nf) ? nf : 0;
-    in.inv_masses = inv_masses; in.thicknesses = thicknesses; in.layer_ids = layer_ids;
-    in.layer_id = pp.layer_id; in.thickness = pp.thickness;
-    in.tension_k = pp.stiffness;
-    in.compression_k = pp.compression_stiffness >= 0.0f ? pp.compression_stiffness : pp.stiffness;
-    in.shear_k = pp.shear_stiffness >= 0.0f ? pp.shear_stiffness : pp.stiffness * 0.5f;
-    in.bending_k = pp.bending_stiffness;
+int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
+                  uint32_t nf, const float *inv_masses, const float *thicknesses, const uint32_t *layer_ids,
+                  const tc_sim_params *params, tc_sim **out) {
+    g_err.clear();
+    if (!out) return fail(TC_ERR_VALUE, "out handle pointer is null");
+    *out = nullptr;
+    tc_sim_params def;
+    tc_sim_params_default(&def);
+    const tc_sim_params &pp = params ? *params : def;
+    if (int rc = check_topology(positions, nv, edges, ne, faces, nf)) return rc;
+    if (pp.solver_mode != 0) return fail(TC_ERR_UNSUPPORTED, "only solver_mode=0 (graph colouring) is supported");
+    const tc::MeshInput in = mesh_input(positions, nv, edges, ne, faces, nf, inv_masses, thicknesses, layer_ids, pp);
 
     tc_sim *s = new tc_sim();
     std::string err = tc::build_mesh(in, s->mesh);
     if (!err.empty()) { delete s; return fail(TC_ERR_VALUE, err); }
+    return finish_create(s, pp, out);
+}
+
+// Device state of a simulator whose mesh is built; takes ownership of s.
+static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
+    const uint32_t nv = s->mesh.nv;
     if (pp.workgroup_size == 64 || pp.workgroup_size == 128 || pp.workgroup_size == 256) s->block = pp.workgroup_size;
     if (const char *g = std::getenv("TCLOTH_USE_GRAPH")) s->use_graph = std::strcmp(g, "0") != 0;
     if (const char *tv = std::getenv("TCLOTH_TILE_VERTICES")) {
@@ -386,6 +406,53 @@ int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, ui
         return bail(fail(TC_ERR_RUNTIME, std::string("device init: ") + hipGetErrorString(he)));
     build_plan(s);
     *out = s;
+    return TC_OK;
+}
+
+int tc_sim_create_batch(const tc_instance *instances, uint32_t n, const tc_sim_params *params, tc_sim **out) {
+    g_err.clear();
+    if (!out) return fail(TC_ERR_VALUE, "out handle pointer is null");
+    *out = nullptr;
+    if (n > 0 && !instances) return fail(TC_ERR_VALUE, "instances is null");
+    tc_sim_params def;
+    tc_sim_params_default(&def);
+    const tc_sim_params &pp = params ? *params : def;
+    if (pp.solver_mode != 0) return fail(TC_ERR_UNSUPPORTED, "only solver_mode=0 (graph colouring) is supported");
+    std::vector<tc::Mesh> parts(n);
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const tc_instance &I = instances[k];
+        if (int rc = check_topology(I.positions, I.num_vertices, I.edges, I.num_edges, I.faces, I.num_faces)) {
+            const std::string why = g_err;
+            return fail(rc, "instance " + std::to_string(k) + ": " + why);
+        }
+        total += I.num_vertices;
+        if (total >= 0x80000000ull) return fail(TC_ERR_VALUE, "too many vertices in the batch (limit 2^31 - 1)");
+        const tc::MeshInput in = mesh_input(I.positions, I.num_vertices, I.edges, I.num_edges, I.faces, I.num_faces,
+                                            I.inv_masses, I.thicknesses, nullptr, pp);
+        const std::string err = tc::build_mesh(in, parts[k]);
+        if (!err.empty()) return fail(TC_ERR_VALUE, "instance " + std::to_string(k) + ": " + err);
+    }
+    tc_sim *s = new tc_sim();
+    tc::merge_meshes(parts, s->mesh);
+    s->self_collision = false;  // instances are independent; overlapping ones would collide
+    return finish_create(s, pp, out);
+}
+
+int tc_sim_instance_count(const tc_sim *s, uint32_t *n) {
+    if (!s || !n) return fail(TC_ERR_VALUE, "null argument");
+    *n = s->mesh.part_off.empty() ? 1u : uint32_t(s->mesh.part_off.size() - 1);
+    return TC_OK;
+}
+
+int tc_sim_instance_range(const tc_sim *s, uint32_t i, uint32_t *first_vertex, uint32_t *num_vertices) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    const auto &po = s->mesh.part_off;
+    const uint32_t count = po.empty() ? 1u : uint32_t(po.size() - 1);
+    if (i >= count) return fail(TC_ERR_VALUE, "instance " + std::to_string(i) + " out of range (" + std::to_string(count) + ")");
+    const uint32_t a = po.empty() ? 0u : po[i], b = po.empty() ? s->mesh.nv : po[i + 1];
+    if (first_vertex) *first_vertex = a;
+    if (num_vertices) *num_vertices = b - a;
     return TC_OK;
 }
 

@@ -95,7 +95,14 @@ std::string build_tile_plan(const Mesh &m, uint32_t tile_vertices, uint32_t max_
     const uint32_t T = std::max<uint32_t>(1, tile_vertices);
     std::vector<uint32_t> order, part_size, idx(nv);
     std::iota(idx.begin(), idx.end(), 0u);
-    rcb(m.pos0, idx.data(), nv, T, order, part_size);
+    if (m.part_off.size() >= 2 && m.part_off.back() == nv) {
+        // independent instances: tiles never mix two of them (their rest positions may overlap)
+        for (size_t k = 0; k + 1 < m.part_off.size(); ++k)
+            if (m.part_off[k + 1] > m.part_off[k])
+                rcb(m.pos0, idx.data() + m.part_off[k], m.part_off[k + 1] - m.part_off[k], T, order, part_size);
+    } else {
+        rcb(m.pos0, idx.data(), nv, T, order, part_size);
+    }
     P.ntiles = uint32_t(part_size.size());
     std::vector<uint32_t> part_begin(P.ntiles + 1, 0), tile_of(nv);
     for (uint32_t t = 0; t < P.ntiles; ++t) part_begin[t + 1] = part_begin[t] + part_size[t];

@@ -409,6 +409,85 @@ def compute_combined_inv_masses(positions, faces=None, strand_edges=None, pin_we
     return out
 
 
+class ClothBatch(ClothSimulator):
+    """Many independent cloth instances advanced together (no reference counterpart: the reference
+    runs one simulator per cloth).  ``instances`` is a sequence of dicts with the keys
+    ``positions``, ``edges`` and optionally ``faces``, ``inv_masses`` and ``thicknesses`` (the
+    ClothSimulator arguments of that cloth, indices local to it).  Each instance keeps its own
+    colouring; one launch per colour (or per iteration) advances all of them.  The shared
+    keyword arguments are those of ClothSimulator.  Every ClothSimulator method works on the
+    concatenation of the instances' vertices; ``instance_range`` gives each one's slice.
+    Self-collision starts disabled."""
+
+    def __init__(self, instances, thickness=0.005, stiffness=1000.0, bending_stiffness=10.0,
+                 compression_stiffness=None, shear_stiffness=None, areal_density=None):
+        L = _lib.load()
+        self._L = L
+        self._h = None
+        descs = (_lib.TcInstance * max(len(instances), 1))()
+        keep = []
+        total = 0
+        for k, inst in enumerate(instances):
+            pos = _f32(inst["positions"], (3,), f"instances[{k}].positions")
+            e = _u32(inst["edges"], 2, f"instances[{k}].edges")
+            n = len(pos)
+            if e.size and int(e.max()) >= n:
+                raise ValueError(f"instances[{k}]: edges contain an index >= number of vertices ({n})")
+            f = inst.get("faces")
+            if f is not None:
+                f = _u32(f, 3, f"instances[{k}].faces")
+                if f.size and int(f.max()) >= n:
+                    raise ValueError(f"instances[{k}]: faces contain an index >= number of vertices ({n})")
+                if f.size == 0:
+                    f = None
+            arrs = []
+            for name in ("inv_masses", "thicknesses"):
+                a = inst.get(name)
+                if a is not None:
+                    a = np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1))
+                    if len(a) != n:
+                        raise ValueError(f"instances[{k}].{name} length ({len(a)}) does not match the number of "
+                                         f"vertices ({n})")
+                arrs.append(a)
+            d = descs[k]
+            d.positions, d.num_vertices = _ptr(pos), n
+            d.edges, d.num_edges = _ptr(e), len(e)
+            d.faces, d.num_faces = _ptr(f), 0 if f is None else len(f)
+            d.inv_masses, d.thicknesses = _ptr(arrs[0]), _ptr(arrs[1])
+            keep.append((pos, e, f, arrs))
+            total += n
+        p = _lib.TcParams()
+        L.tc_sim_params_default(C.byref(p))
+        p.thickness = float(thickness)
+        p.stiffness = float(stiffness)
+        p.bending_stiffness = float(bending_stiffness)
+        p.compression_stiffness = -1.0 if compression_stiffness is None else float(compression_stiffness)
+        p.shear_stiffness = -1.0 if shear_stiffness is None else float(shear_stiffness)
+        h = C.c_void_p()
+        _lib.check(L.tc_sim_create_batch(descs, len(instances), C.byref(p), C.byref(h)))
+        self._h = h
+        self._n = total
+        self._keep = keep
+        self.areal_density = 0.15 if areal_density is None else float(min(max(areal_density, 0.0), 100.0))
+
+    def num_instances(self) -> int:
+        n = C.c_uint32(0)
+        _lib.check(self._L.tc_sim_instance_count(self._h, C.byref(n)))
+        return int(n.value)
+
+    def instance_range(self, i):
+        """(first vertex, vertex count) of instance i in the batch's vertex arrays."""
+        a, n = C.c_uint32(0), C.c_uint32(0)
+        _lib.check(self._L.tc_sim_instance_range(self._h, int(i), C.byref(a), C.byref(n)))
+        return int(a.value), int(n.value)
+
+    def get_instance_positions(self, i) -> np.ndarray:
+        a, n = self.instance_range(i)
+        out = np.empty(3 * self._n, np.float32)
+        self.get_positions(out)
+        return out.reshape(-1, 3)[a:a + n].copy()
+
+
 def _bake(mesh_verts, mesh_tris, voxel_size, margin, friction, thickness, restitution, close_open_boundaries,
           method):
     L = _lib.load()

@@ -234,3 +234,23 @@ def icosphere(subdiv: int, radius: float = 0.3, noise_amp: float = 0.05, seed: i
     r = radius * (1.0 + noise_amp * value_noise(V))
     pos = (V * r[:, None] + np.asarray(center)).astype(np.float32)
     return pos, np.array(f, np.uint32)
+
+
+def batch_instances(count: int, side_lo: int, side_hi: int, seed: int = 0, spacing: float = 0.01,
+                    jitter: float = 1e-4):
+    """`count` vertical curtains (XZ lattices, 2 triangles per quad) with sides drawn uniformly
+    from [side_lo, side_hi] by splitmix64(seed), laid side by side along x; the top two corners
+    of each are pinned, masses from areal density 0.15, positions jittered by +-`jitter`."""
+    g = splitmix64(seed)
+    out = []
+    x0 = 0.0
+    for k in range(count):
+        side = side_lo + int(next(g) % (side_hi - side_lo + 1))
+        pos, edges, faces = grid(side, side, spacing, "xz", origin=(x0, 0.0, 0.0))
+        pos = pos + uniform_jitter(len(pos), jitter, seed * 1000 + k)
+        inv = areal_inv_masses_np(pos, faces, 0.15)
+        inv[side * (side - 1)] = 0.0
+        inv[side * side - 1] = 0.0
+        out.append({"positions": pos.astype(np.float32), "edges": edges, "faces": faces, "inv_masses": inv})
+        x0 += side * spacing + 0.05
+    return out
