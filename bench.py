@@ -66,6 +66,25 @@ def grid_512(n: int, spacing: float = 0.01, height: float = 0.5, seed: int = 0):
     return pos, edges, faces
 
 
+EQUIV_ITERS = (8, 16, 32)
+
+
+def equivalent_iterations(curve: dict, r: float) -> float:
+    """Plain iteration count whose residual equals r, interpolating log(residual) against
+    log(iterations) between the measured counts (clamped to the measured range)."""
+    ks = sorted(curve)
+    rs = [curve[k] for k in ks]
+    if r >= rs[0]:
+        return float(ks[0])
+    if r <= rs[-1]:
+        return float(ks[-1])
+    for (k0, r0), (k1, r1) in zip(zip(ks, rs), zip(ks[1:], rs[1:])):
+        if r1 <= r <= r0 and r0 > r1:
+            f = (np.log(r0) - np.log(r)) / (np.log(r0) - np.log(r1))
+            return float(np.exp(np.log(k0) + f * (np.log(k1) - np.log(k0))))
+    return float(ks[-1])
+
+
 def configure(sim, iters):
     sim.set_gravity(0.0, 0.0, -9.81)
     sim.set_compliances_raw(1e-8, 1e-6, 1e-5)   # distance, shear, bending
@@ -333,8 +352,20 @@ def main():
             sim.substep_parts(dt_sub, collide=False, self_collide=False, update=False)
             r_learn, m_learn = sim.residual(dt_sub)
             restore(sim, st)
+            # plain XPBD at fewer iterations from the same state: how many plain iterations the
+            # network plus its 8 iterations are worth
+            curve = {}
+            for k in EQUIV_ITERS:
+                plain_on()
+                sim.set_solver_iterations(k)
+                sim.substep_parts(dt_sub, collide=False, self_collide=False, update=False)
+                curve[k], _ = sim.residual(dt_sub)
+                restore(sim, st)
+            curve[args.iters] = r_plain
             checks.append({"substep": args.warmup + done, "plain": r_plain, "learned": r_learn,
-                           "plain_max": m_plain, "learned_max": m_learn})
+                           "plain_max": m_plain, "learned_max": m_learn,
+                           "plain_by_iterations": {str(k): v for k, v in sorted(curve.items())},
+                           "learned_equals_plain_iterations": equivalent_iterations(curve, r_learn)})
     threshold = float(np.median([c["plain"] for c in checks])) if checks else 1e9
 
     # timed arm 1: plain XPBD at 64 iterations
@@ -407,6 +438,22 @@ def main():
                          f"{n}x{n} workload, plain XPBD at {args.iters} iterations: {k} whole substeps in {el:.1f} s"}
 
     eq = [c["learned"] <= c["plain"] for c in checks]
+    # the equal-residual comparison: plain XPBD at the iteration count that matches the learned
+    # arm's residual, priced at the measured per-iteration launch time plus the rest of a substep
+    equal_residual = None
+    if checks:
+        k_eq = float(np.median([c["learned_equals_plain_iterations"] for c in checks]))
+        it_ms = (kt["fused"] / max(kt["fused_launches"], 1.0)) if info["path"] == 2 else \
+            (kt["distance"] + kt["bending"]) / max(args.iters, 1)
+        rest_ms = 1e3 / plain_sps - args.iters * it_ms
+        t_eq = rest_ms + k_eq * it_ms
+        equal_residual = {
+            "plain_iterations_matching_learned_residual": k_eq,
+            "plain_ms_per_substep_at_that_count": t_eq,
+            "learned_ms_per_substep": 1e3 / learned_sps,
+            "learned_speed_relative_to_plain_at_equal_residual": t_eq / (1e3 / learned_sps),
+            "note": "the network plus its iterations is worth this many plain iterations from the same state "
+                    "(median over the checked states); below 1 the warm start costs more than it saves"}
     line = {
         "metric": "substeps/sec at 512² grid: learned warm-start vs plain XPBD at residual ≤1e-4",
         "value": learned_sps,
@@ -435,8 +482,10 @@ def main():
                          "threshold is the plain 64-iteration arm's median r over the checked states (neither arm "
                          "reaches 1e-4 once the cloth is in contact)",
             "gate_checked": stats["checked"], "gate_fallbacks": stats["fallbacks"],
+            "gate_fallback_chunks": stats.get("fallback_chunks"),
             "equal_residual_checks": checks,
             "learned_at_or_below_plain_residual": f"{sum(eq)}/{len(eq)}",
+            "equal_residual_comparison": equal_residual,
             "plain_relative_residual_end": r_plain_end,
             "learned_substep_ms_breakdown": {k: round(v, 5) for k, v in wt.items()},
             "plain_substep_ms_breakdown": {k: round(v, 5) for k, v in kt.items()},

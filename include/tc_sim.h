@@ -245,12 +245,14 @@ int tc_ws_eval(tc_ws_net *net, tc_sim *const *sims, uint32_t nsims, double *mean
  * w = history_weight in [0, 1]; the previous correction also reaches the substep through the
  * velocity, so w near 1 amplifies error modes the iterations correct slowly (0.5 or less keeps
  * them decaying).  A network must be used with the weight it was trained for.
- * Modes 1 and 2 run `iterations` plain iterations, then a residual check; when the relative
- * residual exceeds `residual_threshold`, `fallback_iterations` (rounded up to even) more run. */
+ * Modes 1 and 2 run `iterations` plain iterations, then a residual check; while the relative
+ * residual exceeds `residual_threshold`, fallback iterations run in chunks of `iterations`
+ * (rounded up to even), each followed by another check, up to `fallback_iterations` (rounded up
+ * to even) in all. */
 int tc_sim_set_warm_start(tc_sim *sim, tc_ws_net *net, int mode, uint32_t iterations,
                           uint32_t fallback_iterations, float residual_threshold, float history_weight);
-/* out[6]: [substeps checked, fallbacks taken, relative residual at the last check,
- *          max |h|/L0 at the last check, mode, warm-start iterations] */
+/* out[7]: [substeps checked, substeps that fell back, relative residual at the last check,
+ *          max |h|/L0 at the last check, mode, warm-start iterations, fallback chunks run] */
 int tc_sim_warm_start_stats(tc_sim *sim, double *out);
 int tc_sim_ws_reset_stats(tc_sim *sim);
 /* Records training samples from the coming substeps: every `stride`-th substep stores its

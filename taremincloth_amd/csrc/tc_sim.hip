@@ -84,6 +84,27 @@ hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t coun
     return e;
 }
 
+// Residual-gated fallback: chunks of the warm-start iteration count (rounded up to even, so that
+// the P / lambda buffer parity is the same whether a chunk runs or not), each preceded by a
+// residual check; the first check that passes closes the gate for the rest of the substep.
+hipError_t enqueue_gated_fallback(tc_sim *s, const tc::SubstepArgs &a, float dt_sub, uint32_t &pc, uint32_t &lc) {
+    const tc::Mesh &m = s->mesh;
+    const uint32_t chunk = std::max<uint32_t>(2u, s->ws_iters + (s->ws_iters & 1u));
+    tc::SubstepArgs fa = a;
+    fa.run_flag = s->ws_gate.p;
+    hipError_t e = hipSuccess;
+    for (uint32_t done = 0; done < s->ws_fallback; done += chunk) {
+        e = tc::launch_residual(s->stream, s->Pbuf(pc), s->drec.p, s->dcc.p, s->dlam_buf(lc), uint32_t(m.d_v0.size()),
+                                s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()), dt_sub, s->split_compliance,
+                                s->resid_part.p, s->ws_res.p, s->ws_gate.p, s->ws_threshold,
+                                done == 0 ? nullptr : s->ws_gate.p);
+        if (e != hipSuccess) return e;
+        if ((e = enqueue_iterations(s, fa, std::min(chunk, s->ws_fallback - done), false, pc, lc)) != hipSuccess)
+            return e;
+    }
+    return e;
+}
+
 namespace {
 
 hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_collide, bool update) {
@@ -101,17 +122,8 @@ hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_coll
         if ((e = tc::launch_fill_u32(s->stream, reinterpret_cast<uint32_t *>(s->blam.p), 0u, m.b_v0.size())) != hipSuccess) return e;
     } else {
         if ((e = enqueue_iterations(s, a, iters, true, pc, lc)) != hipSuccess) return e;
-        if (s->ws_active() && s->ws_fallback > 0) {
-            // residual gate: the fallback iterations run only when the residual exceeds the threshold
-            e = tc::launch_residual(s->stream, s->Pbuf(pc), s->drec.p, s->dcc.p, s->dlam_buf(lc),
-                                    uint32_t(m.d_v0.size()), s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()),
-                                    dt_sub, s->split_compliance, s->resid_part.p, s->ws_res.p, s->ws_gate.p,
-                                    s->ws_threshold);
-            if (e != hipSuccess) return e;
-            tc::SubstepArgs fa = a;
-            fa.run_flag = s->ws_gate.p;
-            if ((e = enqueue_iterations(s, fa, s->ws_fallback, false, pc, lc)) != hipSuccess) return e;
-        }
+        if (s->ws_active() && s->ws_fallback > 0 && (e = enqueue_gated_fallback(s, a, dt_sub, pc, lc)) != hipSuccess)
+            return e;
     }
     float4 *Pc = s->Pbuf(pc);
     s->last_p = pc;

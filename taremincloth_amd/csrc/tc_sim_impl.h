@@ -162,7 +162,7 @@ struct tc_sim {
     DevBuf<float4> ws_ppred, ws_D;
     DevBuf<tc::ws::bf16> ws_feat, ws_h0, ws_h1, ws_agg;
     DevBuf<float> ws_y;
-    DevBuf<uint32_t> ws_gate;          // [fallback flag, substeps checked, fallbacks, unused]
+    DevBuf<uint32_t> ws_gate;          // [fallback flag, substeps checked, fallbacks, fallback chunks run]
     DevBuf<double> ws_res;             // residual at the gate: sum h^2, sum L0^2, max, relative
     // training samples recorded from plain substeps
     uint32_t rec_cap = 0, rec_stride = 1, rec_count = 0, rec_tick = 0;
@@ -244,5 +244,6 @@ float ws_scale(const tc_sim *s, float dt_sub);
 int ws_clear_history(tc_sim *s);
 // `count` projection iterations of the active schedule (tc_sim.hip); pc / lc track the
 // buffers holding P and lambda on the per-iteration path.
+hipError_t enqueue_gated_fallback(tc_sim *s, const tc::SubstepArgs &a, float dt_sub, uint32_t &pc, uint32_t &lc);
 hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t count, bool first, uint32_t &pc,
                               uint32_t &lc);

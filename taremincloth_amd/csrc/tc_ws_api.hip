@@ -421,7 +421,7 @@ int tc_sim_set_warm_start(tc_sim *s, tc_ws_net *net, int mode, uint32_t iteratio
 
 int tc_sim_warm_start_stats(tc_sim *s, double *out) {
     if (!s || !out) return fail(TC_ERR_VALUE, "null argument");
-    for (int k = 0; k < 6; ++k) out[k] = 0.0;
+    for (int k = 0; k < 7; ++k) out[k] = 0.0;
     out[4] = s->ws_mode;
     out[5] = s->ws_iters;
     if (!s->ws_ready) return TC_OK;
@@ -434,6 +434,7 @@ int tc_sim_warm_start_stats(tc_sim *s, double *out) {
     out[1] = g[2];
     out[2] = r[3];
     out[3] = r[2];
+    out[6] = g[3];
     return TC_OK;
 }
 
@@ -634,18 +635,8 @@ int tc_sim_ws_time(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
         uint32_t pc = 0, lc = 0;
         TC_HIP(enqueue_iterations(s, a, s->ws_iters, true, pc, lc));
         TC_HIP(hipEventRecord(ev[21], s->stream));
-        if (s->ws_fallback > 0) {
-            TC_HIP(tc::launch_residual(s->stream, s->Pbuf(pc), s->drec.p, s->dcc.p, s->dlam_buf(lc),
-                                       uint32_t(m.d_v0.size()), s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()),
-                                       dt_sub, s->split_compliance, s->resid_part.p, s->ws_res.p, s->ws_gate.p,
-                                       s->ws_threshold));
-        }
-        TC_HIP(hipEventRecord(ev[22], s->stream));
-        if (s->ws_fallback > 0) {
-            tc::SubstepArgs fa = a;
-            fa.run_flag = s->ws_gate.p;
-            TC_HIP(enqueue_iterations(s, fa, s->ws_fallback, false, pc, lc));
-        }
+        TC_HIP(hipEventRecord(ev[22], s->stream));  // gate and fallback chunks are timed together
+        if (s->ws_fallback > 0) TC_HIP(enqueue_gated_fallback(s, a, dt_sub, pc, lc));
         TC_HIP(hipEventRecord(ev[23], s->stream));
         float4 *Pc = s->Pbuf(pc);
         s->last_p = pc;

@@ -470,7 +470,9 @@ __global__ void __launch_bounds__(256) k_residual_part(const float4 *__restrict_
                                                        const float *__restrict__ dcc, const float *__restrict__ dlam,
                                                        uint32_t nd, const uint4 *__restrict__ brec,
                                                        const float *__restrict__ blam, uint32_t nb, float dt_sub,
-                                                       int split, double *__restrict__ part) {
+                                                       int split, double *__restrict__ part,
+                                                       const uint32_t *__restrict__ run_flag) {
+    if (run_flag && *run_flag == 0u) return;  // the previous check passed: nothing to measure
     __shared__ double sh[3][4];
     double hh = 0.0, ll = 0.0, mx = 0.0;
     const double dt = dt_sub > 1e-8f ? (double)dt_sub : 1e-8;
@@ -513,7 +515,9 @@ __global__ void __launch_bounds__(256) k_residual_part(const float4 *__restrict_
 // gate[1] += 1 (substeps checked), gate[2] += gate[0] (substeps that fell back).
 __global__ void __launch_bounds__(64) k_residual_final(const double *__restrict__ part, uint32_t nblocks,
                                                        double *__restrict__ out, uint32_t *__restrict__ gate,
-                                                       float threshold) {
+                                                       float threshold, const uint32_t *run_flag) {
+    // a chained check (run_flag set) after a check that passed keeps the gate closed
+    if (run_flag && *run_flag == 0u) return;
     double hh = 0.0, ll = 0.0, mx = 0.0;
     for (uint32_t b = threadIdx.x; b < nblocks; b += 64) {
         hh += part[3 * b + 0];
@@ -532,8 +536,13 @@ __global__ void __launch_bounds__(64) k_residual_final(const double *__restrict_
         if (gate) {
             const uint32_t fail = rel > (double)threshold ? 1u : 0u;
             gate[0] = fail;
-            gate[1] += 1u;
-            gate[2] += fail;
+            if (run_flag) {
+                gate[3] += fail;  // another fallback chunk
+            } else {
+                gate[1] += 1u;    // substeps checked
+                gate[2] += fail;  // substeps that fell back
+                gate[3] += fail;
+            }
         }
     }
 }
@@ -609,12 +618,12 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t value, size_t n)
 
 hipError_t launch_residual(hipStream_t s, const float4 *P, const uint4 *drec, const float *dcc, const float *dlam,
                            uint32_t nd, const uint4 *brec, const float *blam, uint32_t nb, float dt, bool split,
-                           double *part, double *out, uint32_t *gate, float threshold) {
+                           double *part, double *out, uint32_t *gate, float threshold, const uint32_t *run_flag) {
     const size_t n = size_t(nd) + nb;
     const uint32_t blocks = uint32_t(std::min<size_t>(RES_BLOCKS, std::max<size_t>(grid_for(n, 256), 1)));
     hipLaunchKernelGGL(k_residual_part, dim3(blocks), dim3(256), 0, s, P, drec, dcc, dlam, nd, brec, blam, nb, dt,
-                       split ? 1 : 0, part);
-    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, s, part, blocks, out, gate, threshold);
+                       split ? 1 : 0, part, run_flag);
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, s, part, blocks, out, gate, threshold, run_flag);
     return hipGetLastError();
 }
 

@@ -76,12 +76,12 @@ def record(seeds, frames, stride, n, history_weight):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
-    ap.add_argument("--scenes", type=int, default=32)
+    ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--frames", type=int, default=62)
     ap.add_argument("--stride", type=int, default=4)
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--rounds", type=int, default=3)
-    ap.add_argument("--schedule", default="1e-3:6000,3e-4:3000,1e-4:1000")
+    ap.add_argument("--schedule", default="1e-3:30000,3e-4:20000,1e-4:10000")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--history-weight", type=float, default=0.5,
                     help="weight of the previous correction in the warm start the network is trained for")

@@ -186,8 +186,47 @@ def test_gpu_gate_fallback_equals_plain_iterations(require_gpu):
         if mode:
             st = sim.warm_start_stats()
             assert st["checked"] == 1 and st["fallbacks"] == (1 if th == 0.0 else 0)
+            # chunks of 4, then the remaining 2
+            assert st["fallback_chunks"] == (2 if th == 0.0 else 0)
     assert np.array_equal(outs["fail"], outs["plain10"])
     assert np.array_equal(outs["pass"], outs["plain4"])
+
+
+@pytest.mark.gpu
+def test_gpu_gate_stops_after_the_chunk_that_passes(require_gpu):
+    """With the threshold between the residuals after 4 and after 8 plain iterations, the first
+    check fails, the chained check after one 4-iteration chunk passes, and the remaining chunks do
+    not run: the substep equals 8 plain iterations bit for bit."""
+    dt_sub = 1.0 / 600.0
+
+    def settled():  # cloth in contact with the sphere, identical in every call
+        sim, pos, *_ = _drape()
+        for _ in range(12):
+            sim.step(1.0 / 60.0, 10, 16)
+        return sim, pos
+
+    res = {}
+    for its in (4, 8):
+        sim, pos = settled()
+        sim.set_solver_iterations(its)
+        sim.substep_parts(dt_sub, collide=False, self_collide=False, update=False)
+        res[its], _ = sim.residual(dt_sub)
+    assert res[4] > res[8]
+    th = float(np.sqrt(res[4] * res[8]))
+    sim, pos = settled()
+    sim.set_solver_iterations(8)
+    sim.step_single_substep(dt_sub)
+    want = np.zeros(pos.size, np.float32)
+    sim.get_positions(want)
+    sim, pos = settled()
+    sim.set_solver_iterations(4)
+    sim.set_warm_start(None, 1, 4, 12, th, history_weight=0.5)
+    sim.step_single_substep(dt_sub)
+    got = np.zeros(pos.size, np.float32)
+    sim.get_positions(got)
+    st = sim.warm_start_stats()
+    assert st["checked"] == 1 and st["fallbacks"] == 1 and st["fallback_chunks"] == 1
+    assert np.array_equal(got, want)
 
 
 def _stretch_graph(pos, edges, faces, inv):
