@@ -193,6 +193,43 @@ def sdf_bake_bench(T, cpu_sample: bool):
     return out
 
 
+def drape_bench(T, frames: int = 200, warmup: int = 5):
+    """A 128 x 128 cloth (1 cm) draped for `frames` frames on the icosphere mesh collider
+    (subdivision 6, 81,920 triangles, radius 0.3 m x (1 + 5% value noise, seed 0)), triangle-mesh
+    collision with friction 0.4, 10 substeps x 20 iterations per frame, self-collision off."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import cases
+    v, f = cases.icosphere(6, radius=0.3, noise_amp=0.05, seed=0)
+    tris = v[f.astype(np.int64)]
+    pos, edges, faces = grid_512(128, 0.01, 0.45, seed=3)
+    inv = T.compute_combined_inv_masses(pos, faces, areal_density=0.15)
+    sim = T.ClothSimulator(pos, edges, faces, inv, thickness=0.005, stiffness=1000.0, bending_stiffness=10.0)
+    sim.set_gravity(0.0, 0.0, -9.81)
+    sim.set_compliances_raw(1e-8, 1e-6, 1e-5)
+    sim.set_enable_self_collision(False)
+    sim.set_mesh_collider_triangles(tris, friction=0.4, thickness=0.002, restitution=0.0)
+    sim.set_solver_iterations(20)
+    per_frame, dt_frame = 10, 1.0 / 60.0
+    for _ in range(warmup):
+        sim.step(dt_frame, per_frame)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sim.step(dt_frame, per_frame)
+    sim.synchronize()
+    el = time.perf_counter() - t0
+    kt = sim.time_kernels(dt_frame / per_frame, reps=3)
+    out = np.zeros(3 * len(pos), np.float32)
+    sim.get_positions(out)
+    r = np.linalg.norm(out.reshape(-1, 3), axis=1)
+    return {"workload": f"128x128 cloth (1 cm) on an icosphere mesh collider (subdivision 6, {len(f)} triangles, "
+                        f"r 0.3 m, 5% noise), friction 0.4, 10 substeps x 20 iterations; {frames} timed frames after "
+                        f"{warmup}", "frames_per_s": frames / el, "substeps_per_s": frames * per_frame / el,
+            "ms_per_substep": el / (frames * per_frame) * 1e3,
+            "collision_ms_per_substep": kt["collision"], "substep_ms_breakdown": {k: round(x, 5) for k, x in kt.items()},
+            "min_distance_to_centre_m": float(r.min())}
+
+
 def batch_bench(T, frames: int, warmup: int, cpu_sample: bool):
     """64 independent curtains of side U[16, 256] (seed 0, 1 cm spacing, top corners pinned) in one
     simulator: 10 substeps x 20 iterations per frame, one launch per iteration over all
@@ -274,6 +311,9 @@ def main():
                          "from TCLOTH_TILE_VERTICES / TCLOTH_TILE_BLOCK or the defaults")
     ap.add_argument("--no-sdf", action="store_true", help="skip the mesh SDF bake measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances measurement")
+    ap.add_argument("--no-drape", action="store_true", help="skip the mesh-collider drape measurement")
+    ap.add_argument("--drape-only", action="store_true",
+                    help="measure only the drape on the icosphere mesh collider and print its JSON object")
     ap.add_argument("--batch-only", action="store_true",
                     help="measure only the batch of 64 instances and print its JSON object (for profiler runs)")
     ap.add_argument("--sdf-only", action="store_true",
@@ -291,6 +331,9 @@ def main():
 
     if args.sdf_only:
         print(json.dumps(sdf_bake_bench(T, not args.no_cpu_baseline)), flush=True)
+        return
+    if args.drape_only:
+        print(json.dumps(drape_bench(T)), flush=True)
         return
     if args.batch_only:
         print(json.dumps(batch_bench(T, 30, 5, not args.no_cpu_baseline)), flush=True)
@@ -429,6 +472,7 @@ def main():
 
     sdf = None if args.no_sdf else sdf_bake_bench(T, not args.no_cpu_baseline)
     batch = None if args.no_batch else batch_bench(T, 30, 5, not args.no_cpu_baseline)
+    drape = None if args.no_drape else drape_bench(T)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -505,6 +549,7 @@ def main():
         "cpu_baseline": cpu,
         "sdf_bake": sdf,
         "batch": batch,
+        "mesh_collider_drape": drape,
     }
     print(json.dumps(line), flush=True)
 

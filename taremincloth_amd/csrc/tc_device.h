@@ -69,6 +69,7 @@ struct MeshTri {
     float p2x, p2y, p2z, restitution;
     uint32_t flags; float pad0, pad1, pad2;
 };
+static_assert(sizeof(MeshTri) == 64, "the collider stages a triangle as 4 float4");
 
 // Self-collision accumulator (SelfCollisionAccum, simulation/types.rs:145-178)
 struct Accum {

@@ -131,7 +131,7 @@ hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_coll
     if (track && (e = ws_post(s, dt_sub, Pc)) != hipSuccess) return e;
     if (collide && (!s->cols_h.empty() || !s->tris_h.empty() || s->sdf_nb > 0)) {
         e = tc::launch_collision(s->stream, Pc, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
-                                 s->tri_bounds.p, uint32_t(s->tris_h.size()), s->single_sided_recovery, dt_sub, B, s->sdf_args());
+                                 s->tri_bounds.p, uint32_t(s->tris_h.size()), s->tri_cbox.p, s->single_sided_recovery, dt_sub, B, s->sdf_args());
         if (e != hipSuccess) return e;
     }
     if (self_collide && s->self_collision) {
@@ -614,7 +614,7 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
         }
         if (!s->cols_h.empty() || !s->tris_h.empty() || s->sdf_nb > 0)
             if ((e = mark(5, [&] { return tc::launch_collision(s->stream, Pc, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()),
-                                                               s->tris.p, s->tri_bounds.p, uint32_t(s->tris_h.size()),
+                                                               s->tris.p, s->tri_bounds.p, uint32_t(s->tris_h.size()), s->tri_cbox.p,
                                                                s->single_sided_recovery, dt_sub, B, s->sdf_args()); })) != hipSuccess) return e;
         if (s->self_collision)
             if ((e = mark(6, [&] {
@@ -901,9 +901,32 @@ int tc_sim_set_mesh_triangles(tc_sim *s, const float *tris, uint32_t nt, const f
                                            d2(m.p2x, m.p2y, m.p2z))) + m.thickness;
         bounds[t] = make_float4(cx, cy, cz, r);
     }
+    // culling boxes: each triangle's bounding sphere scaled by 1.5 (the kernel's per-triangle test),
+    // widened by a relative 1e-6 and 1e-7 m against rounding, gathered into clusters of 32
+    // triangles and super-clusters of 32 clusters
+    const uint32_t ncl = (nt + 31u) / 32u, nsc = (ncl + 31u) / 32u;
+    std::vector<float4> cbox(2 * (size_t(nsc) + ncl));
+    auto grow = [](float4 &lo, float4 &hi, float x0, float y0, float z0, float x1, float y1, float z1) {
+        lo.x = std::min(lo.x, x0); lo.y = std::min(lo.y, y0); lo.z = std::min(lo.z, z0);
+        hi.x = std::max(hi.x, x1); hi.y = std::max(hi.y, y1); hi.z = std::max(hi.z, z1);
+    };
+    const float4 empty_lo = make_float4(3.4e38f, 3.4e38f, 3.4e38f, 0.0f), empty_hi = make_float4(-3.4e38f, -3.4e38f, -3.4e38f, 0.0f);
+    for (uint32_t c = 0; c < nsc + ncl; ++c) { cbox[2 * c] = empty_lo; cbox[2 * c + 1] = empty_hi; }
+    for (uint32_t t = 0; t < nt; ++t) {
+        const float4 b = bounds[t];
+        const float e = 1.5f * b.w;
+        const float sx = 1e-6f * (std::fabs(b.x) + e) + 1e-7f, sy = 1e-6f * (std::fabs(b.y) + e) + 1e-7f,
+                    sz = 1e-6f * (std::fabs(b.z) + e) + 1e-7f;
+        const uint32_t c = t / 32u, sc = c / 32u;
+        float4 &clo = cbox[2 * (size_t(nsc) + c)], &chi = cbox[2 * (size_t(nsc) + c) + 1];
+        grow(clo, chi, b.x - e - sx, b.y - e - sy, b.z - e - sz, b.x + e + sx, b.y + e + sy, b.z + e + sz);
+        float4 &slo = cbox[2 * size_t(sc)], &shi = cbox[2 * size_t(sc) + 1];
+        grow(slo, shi, b.x - e - sx, b.y - e - sy, b.z - e - sz, b.x + e + sx, b.y + e + sy, b.z + e + sz);
+    }
     s->tris_h = sorted;
     TC_HIP(s->tris.upload(s->tris_h.data(), nt, s->stream));
     TC_HIP(s->tri_bounds.upload(bounds.data(), nt, s->stream));
+    TC_HIP(s->tri_cbox.upload(cbox.data(), cbox.size(), s->stream));
     TC_HIP(hipStreamSynchronize(s->stream));
     s->touch();
     return TC_OK;

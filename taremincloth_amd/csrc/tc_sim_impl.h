@@ -102,6 +102,7 @@ struct tc_sim {
     std::vector<tc::MeshTri> tris_h;
     DevBuf<tc::MeshTri> tris;
     DevBuf<float4> tri_bounds;
+    DevBuf<float4> tri_cbox;           // super-cluster then cluster boxes (lo, hi) of the Morton-ordered triangles
     int single_sided_recovery = 1;
 
     // SDF colliders (set_bone_sdf_colliders)

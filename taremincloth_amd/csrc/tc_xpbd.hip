@@ -310,21 +310,36 @@ __device__ void sdf_collision(const SdfArgs &s, uint32_t ntris_mesh, f3 &p, f3 &
     }
 }
 
+// Whether the segment a->b may pass within `th` of the box [lo, hi]: the gap between the
+// segment's box and [lo, hi], compared with a slack of 1e-5 relative + 1e-5 m so that no
+// triangle whose own bounding-sphere test (below) would pass is skipped.
+__device__ __forceinline__ bool seg_box_near(f3 a, f3 b, float4 lo, float4 hi, float th) {
+    const f3 smin = min3(a, b), smax = max3(a, b);
+    const float gx = fmaxf(0.0f, fmaxf(lo.x - smax.x, smin.x - hi.x));
+    const float gy = fmaxf(0.0f, fmaxf(lo.y - smax.y, smin.y - hi.y));
+    const float gz = fmaxf(0.0f, fmaxf(lo.z - smax.z, smin.z - hi.z));
+    const float lim = th * 1.00001f + 1e-5f;
+    return !(gx * gx + gy * gy + gz * gz > lim * lim);
+}
+
 __global__ void __launch_bounds__(256) k_collision(float4 *__restrict__ P, float4 *__restrict__ X, uint32_t n,
                                                    const Collider *__restrict__ cols, uint32_t ncols,
                                                    const MeshTri *__restrict__ tris,
                                                    const float4 *__restrict__ bounds, uint32_t ntris,
-                                                   int single_sided_recovery, float dt, SdfArgs sdf) {
+                                                   const float4 *__restrict__ cbox, int single_sided_recovery,
+                                                   float dt, SdfArgs sdf) {
+    // Every lane of a wavefront stays to the end: lanes without a vertex (or with a pinned one)
+    // still take part in staging the triangle clusters and in the cluster votes.
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 q = P[i];
-    if (q.w <= 0.0f) return;
-    const float4 x = X[i];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const float4 q = i < n ? P[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const bool active = i < n && q.w > 0.0f;
+    const float4 x = active ? X[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     f3 p = xyz(q);
     const f3 p_init = p, x_old = xyz(x);
     f3 xn = x_old;
     const float th = x.w;
-    for (uint32_t c = 0; c < ncols; ++c) {
+    for (uint32_t c = 0; active && c < ncols; ++c) {
         const Collider col = cols[c];
         const f3 A = mk3(col.ax, col.ay, col.az);
         if (col.type == 0u) {
@@ -362,56 +377,85 @@ __global__ void __launch_bounds__(256) k_collision(float4 *__restrict__ P, float
     float best_depth = 999999.0f;
     f3 best_pos = p, best_n = mk3(0.0f, 0.0f, 0.0f);
     float best_fr = 0.0f, best_rest = 0.0f;
-    for (uint32_t k = 0; k < ntris; ++k) {
-        const float4 b = bounds[k];
-        const float rr = b.w * 1.5f + th;
-        if (seg_sphere_dist_sq(x_old, p, xyz(b)) > rr * rr) continue;
-        const MeshTri T = tris[k];
-        const f3 t0 = mk3(T.p0x, T.p0y, T.p0z), t1 = mk3(T.p1x, T.p1y, T.p1z), t2 = mk3(T.p2x, T.p2y, T.p2z);
-        const float target = th + T.thickness;
-        const f3 cr = cross(t1 - t0, t2 - t0);
-        const float cl = length(cr);
-        if (cl < 1e-7f) continue;
-        const f3 fn = cr / cl;
-        const bool single = (T.flags & 1u) != 0u;
-        float th_hit;
-        if (seg_tri(x_old, p, t0, t1, t2, th_hit)) {
-            const f3 hit = x_old + (p - x_old) * th_hit;
-            f3 nrm = fn;
-            if (dot(p - x_old, fn) > 0.0f && !single) nrm = -fn;
-            contact_response(hit + nrm * target, nrm, T.friction, T.restitution, dt, p_init, xn, p);
-            front = true;
-        }
-        bool is_face;
-        const f3 qp = closest_on_tri(p, t0, t1, t2, is_face);
-        const f3 d = p - qp;
-        const float dist = length(d);
-        const float sd = dot(d, fn);
-        if (single) {
-            if (sd >= 0.0f) {
-                if (dist < target) {
+    // Triangles in Morton order, in clusters of 32 and super-clusters of 32 clusters with boxes
+    // around their bounding spheres scaled by 1.5 (cbox: super-cluster boxes, then cluster
+    // boxes, lo/hi pairs).  A (super-)cluster is skipped when no lane's current segment comes
+    // near its box; the triangles are still visited in Morton order, so every contact is applied
+    // in the same order as in a scan of all triangles.
+    // A cluster that passes the vote is staged in LDS by the whole wavefront (bounding spheres and
+    // triangles), so its 32 triangles cost one round trip to memory instead of 32.
+    __shared__ float4 s_bound[4][32];
+    __shared__ MeshTri s_tri[4][32];
+    const uint32_t ncl = (ntris + 31u) / 32u, nsc = (ncl + 31u) / 32u;
+    const float4 *sbox = cbox, *clbox = cbox + 2 * size_t(nsc);
+    for (uint32_t sc = 0; sc < nsc; ++sc) {
+        if (!__any(active && seg_box_near(x_old, p, sbox[2 * sc], sbox[2 * sc + 1], th))) continue;
+        const uint32_t c1 = min(ncl, (sc + 1u) * 32u);
+        for (uint32_t ci = sc * 32u; ci < c1; ++ci) {
+            if (!__any(active && seg_box_near(x_old, p, clbox[2 * ci], clbox[2 * ci + 1], th))) continue;
+            const uint32_t t0i = ci * 32u, cnt = min(ntris - t0i, 32u);
+            __builtin_amdgcn_wave_barrier();  // the previous cluster's reads come first
+            if (lane < cnt) s_bound[wv][lane] = bounds[t0i + lane];
+            {
+                const float4 *src = reinterpret_cast<const float4 *>(tris + t0i);
+                float4 *dst = reinterpret_cast<float4 *>(&s_tri[wv][0]);
+                if (lane < cnt * 4u) dst[lane] = src[lane];
+                if (lane + 64u < cnt * 4u) dst[lane + 64u] = src[lane + 64u];
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t kk = 0; kk < cnt; ++kk) {
+                const float4 b = s_bound[wv][kk];
+                const float rr = b.w * 1.5f + th;
+                if (!active || seg_sphere_dist_sq(x_old, p, xyz(b)) > rr * rr) continue;
+                const MeshTri T = s_tri[wv][kk];
+                const f3 t0 = mk3(T.p0x, T.p0y, T.p0z), t1 = mk3(T.p1x, T.p1y, T.p1z), t2 = mk3(T.p2x, T.p2y, T.p2z);
+                const float target = th + T.thickness;
+                const f3 cr = cross(t1 - t0, t2 - t0);
+                const float cl = length(cr);
+                if (cl < 1e-7f) continue;
+                const f3 fn = cr / cl;
+                const bool single = (T.flags & 1u) != 0u;
+                float th_hit;
+                if (seg_tri(x_old, p, t0, t1, t2, th_hit)) {
+                    const f3 hit = x_old + (p - x_old) * th_hit;
+                    f3 nrm = fn;
+                    if (dot(p - x_old, fn) > 0.0f && !single) nrm = -fn;
+                    contact_response(hit + nrm * target, nrm, T.friction, T.restitution, dt, p_init, xn, p);
+                    front = true;
+                }
+                bool is_face;
+                const f3 qp = closest_on_tri(p, t0, t1, t2, is_face);
+                const f3 d = p - qp;
+                const float dist = length(d);
+                const float sd = dot(d, fn);
+                if (single) {
+                    if (sd >= 0.0f) {
+                        if (dist < target) {
+                            f3 nrm = fn;
+                            if (dist > 1e-7f) nrm = d / dist;
+                            contact_response(p + nrm * (target - dist), nrm, T.friction, T.restitution, dt, p_init, xn, p);
+                            front = true;
+                        }
+                    } else if (single_sided_recovery && (T.flags & 2u) == 0u) {
+                        if (is_face && dist < target * 3.0f && dist <= fabsf(sd) * 1.1f + 1e-7f && dist < best_depth) {
+                            best_depth = dist;
+                            best_n = fn;
+                            best_pos = p + fn * (target - sd);
+                            best_fr = T.friction;
+                            best_rest = T.restitution;
+                            has_rec = true;
+                        }
+                    }
+                } else if (dist < target && sd >= -target) {
                     f3 nrm = fn;
                     if (dist > 1e-7f) nrm = d / dist;
                     contact_response(p + nrm * (target - dist), nrm, T.friction, T.restitution, dt, p_init, xn, p);
                     front = true;
                 }
-            } else if (single_sided_recovery && (T.flags & 2u) == 0u) {
-                if (is_face && dist < target * 3.0f && dist <= fabsf(sd) * 1.1f + 1e-7f && dist < best_depth) {
-                    best_depth = dist;
-                    best_n = fn;
-                    best_pos = p + fn * (target - sd);
-                    best_fr = T.friction;
-                    best_rest = T.restitution;
-                    has_rec = true;
-                }
             }
-        } else if (dist < target && sd >= -target) {
-            f3 nrm = fn;
-            if (dist > 1e-7f) nrm = d / dist;
-            contact_response(p + nrm * (target - dist), nrm, T.friction, T.restitution, dt, p_init, xn, p);
-            front = true;
         }
     }
+    if (!active) return;
     if (!front && has_rec) contact_response(best_pos, best_n, best_fr, best_rest, dt, p_init, xn, p);
     // SDF colliders last, with the current p as the reference for the velocity (collision.wgsl:645-647)
     sdf_collision(sdf, ntris, p, xn, dt, p, th);
@@ -589,11 +633,12 @@ hipError_t launch_bending(hipStream_t s, float4 *P, const float4 *X, const uint4
 }
 
 hipError_t launch_collision(hipStream_t s, float4 *P, float4 *X, uint32_t n, const Collider *cols, uint32_t ncols,
-                            const MeshTri *tris, const float4 *bounds, uint32_t ntris, int recovery, float dt,
-                            uint32_t block, const SdfArgs &sdf) {
+                            const MeshTri *tris, const float4 *bounds, uint32_t ntris, const float4 *cbox,
+                            int recovery, float dt, uint32_t block, const SdfArgs &sdf) {
     if (!n) return hipSuccess;
+    if (block % 64u != 0u || block > 256u) return hipErrorInvalidValue;  // whole wavefronts, 4 LDS slots
     hipLaunchKernelGGL(k_collision, dim3(grid_for(n, block)), dim3(block), 0, s, P, X, n, cols, ncols, tris, bounds,
-                       ntris, recovery, dt, sdf);
+                       ntris, cbox, recovery, dt, sdf);
     return hipGetLastError();
 }
 

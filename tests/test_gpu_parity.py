@@ -261,6 +261,29 @@ def test_mesh_collider_matches_oracle(single_sided):
     assert r.min() > 0.13  # resting on the surface, not through it
 
 
+def test_mesh_collider_culling_large_mesh():
+    """5,120 triangles (160 clusters in 5 super-clusters): the culled traversal against the
+    oracle's scan of every triangle, bit for bit, with the cloth sliding over the mesh."""
+    _say("mesh collider culling, 5120 triangles")
+    sp, sf = cases.icosphere(4, radius=0.15, center=(0.0, 0.0, 0.0), seed=4)
+    tris = sp[sf.astype(np.int64)]
+    pos, e, f, inv = cases.drape_grid(40, 0.01, 0.17, seed=2)
+    g = T.ClothSimulator(pos, e, f, inv, thickness=0.003)
+    o = O.OracleSim(pos, e, f, inv, thickness=0.003)
+    for s in (g, o):
+        s.set_gravity(0.0, 0.0, -9.81)
+        s.set_enable_self_collision(False)
+        s.set_mesh_collider_triangles(tris, friction=0.4, thickness=0.002, restitution=0.0)
+        s.set_solver_iterations(10)
+    for fr in range(10):
+        g.step(1.0 / 60.0, 10, 10)
+        o.step(1.0 / 60.0, 10, 10)
+        diff = float(np.abs(_pos(g) - o.get_positions().reshape(-1, 3)).max())
+        _say(f"  frame {fr}: {diff:.3e}")
+        assert diff == 0.0, f"frame {fr}: {diff:.3e}"
+    assert np.linalg.norm(_pos(g), axis=1).min() > 0.13
+
+
 def _run_path(path, pos, e, f, inv, frames, damp=False, iters=7, colliders=True):
     g = T.ClothSimulator(pos, e, f, inv, thickness=0.004)
     g.set_compliances_raw(1e-8, 1e-6, 1e-5)
