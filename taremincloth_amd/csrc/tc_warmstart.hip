@@ -295,10 +295,12 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
             const int rr = 32 * rg + 16 * rt + c;
             f32x4 acc[4];
             for (int ct = 0; ct < 4; ++ct) acc[ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            for (int ks = 0; ks < KS1; ++ks) {
-                const bf16x8 xf = *reinterpret_cast<const bf16x8 *>(&xs[rr * LK + ks * 32 + g * 8]);
-                for (int ct = 0; ct < 4; ++ct) acc[ct] = mfma(wf1[ct][ks], xf, acc[ct]);
-            }
+            // all of the row's B fragments first, so that the LDS reads overlap instead of each
+            // waiting in front of its four products
+            bf16x8 xf[KS1];
+            for (int ks = 0; ks < KS1; ++ks) xf[ks] = *reinterpret_cast<const bf16x8 *>(&xs[rr * LK + ks * 32 + g * 8]);
+            for (int ks = 0; ks < KS1; ++ks)
+                for (int ct = 0; ct < 4; ++ct) acc[ct] = mfma(wf1[ct][ks], xf[ks], acc[ct]);
             for (int ct = 0; ct < 4; ++ct) {
                 const int col = 64 * cg + 16 * ct + 4 * g;
                 const float4 bias = *reinterpret_cast<const float4 *>(&b1s[col]);
@@ -333,13 +335,14 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
                     make_float4(acc2[0] + bias.x, acc2[1] + bias.y, acc2[2] + bias.z, acc2[3] + bias.w);
         } else {
             f32x4 acc2[2][4];
+            bf16x8 af[2][KS2];
+            for (int rt = 0; rt < 2; ++rt)
+                for (int ks = 0; ks < KS2; ++ks)
+                    af[rt][ks] = *reinterpret_cast<const bf16x8 *>(&as[(32 * rg + 16 * rt + c) * LH + ks * 32 + g * 8]);
             for (int rt = 0; rt < 2; ++rt) {
-                const int rr = 32 * rg + 16 * rt + c;
                 for (int ct = 0; ct < 4; ++ct) acc2[rt][ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-                for (int ks = 0; ks < KS2; ++ks) {
-                    const bf16x8 af = *reinterpret_cast<const bf16x8 *>(&as[rr * LH + ks * 32 + g * 8]);
-                    for (int ct = 0; ct < 4; ++ct) acc2[rt][ct] = mfma(wf2[ct][ks], af, acc2[rt][ct]);
-                }
+                for (int ks = 0; ks < KS2; ++ks)
+                    for (int ct = 0; ct < 4; ++ct) acc2[rt][ct] = mfma(wf2[ct][ks], af[rt][ks], acc2[rt][ct]);
             }
             __syncthreads();  // the partner wave of this row group has read `as`
             for (int rt = 0; rt < 2; ++rt) {
@@ -362,12 +365,16 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
                 }
             }
             __syncthreads();
-            for (int q = tid; q < TR * (H / 8); q += 256) {
-                const int r2 = q / (H / 8), ch = q % (H / 8);
+            constexpr int OCH = TR * (H / 8) / 256;  // 16-byte output chunks per thread
+            bf16x8 ov[OCH];
+            for (int u = 0; u < OCH; ++u) {
+                const int q = tid + 256 * u, r2 = q / (H / 8), ch = q % (H / 8);
+                ov[u] = *reinterpret_cast<const bf16x8 *>(&as[r2 * LH + ch * 8]);
+            }
+            for (int u = 0; u < OCH; ++u) {
+                const int q = tid + 256 * u, r2 = q / (H / 8), ch = q % (H / 8);
                 const uint32_t row2 = r0 + r2;
-                if (row2 < a.rows)
-                    *reinterpret_cast<bf16x8 *>(&a.hout[size_t(row2) * H + ch * 8]) =
-                        *reinterpret_cast<const bf16x8 *>(&as[r2 * LH + ch * 8]);
+                if (row2 < a.rows) *reinterpret_cast<bf16x8 *>(&a.hout[size_t(row2) * H + ch * 8]) = ov[u];
             }
         }
     }
@@ -658,7 +665,7 @@ hipError_t fwd_block(hipStream_t st, const NetDev &nd, const Layer &l1, const La
     a.xsave = xsave;
     a.asave = asave;
     const size_t lds = fwd_lds<K1, N2>();
-    // workgroups resident per CU: the round layer holds 256 VGPRs of weights per lane
+    // workgroups per CU of the persistent grid (measured: more resident workgroups are slower)
     const uint32_t per_cu = MODE == M_ROUND ? 1u : 2u;
     hipLaunchKernelGGL((k_mlp_fwd<K1, N2, MODE>), dim3(std::min(a.ntiles, PERSIST * per_cu)), dim3(256), lds, st, a);
     return hipGetLastError();
