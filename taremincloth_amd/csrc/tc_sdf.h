@@ -57,10 +57,16 @@ struct DenseArgs {
     uint32_t topk;              // 1 (dense bake) or 8 (coarse pass of the hierarchical bake)
     uint32_t *out;              // packed distances
     uint32_t *nearest;          // topk == 8: 8 triangle indices per voxel
-    unsigned long long *keys;   // topk == 1: per voxel (squared distance bits << 32 | original index)
+    unsigned long long *keys;   // per voxel (squared distance bits << 32 | original index): the nearest
+                                // (topk 1) or an upper bound on the 16th nearest (topk 8)
     unsigned long long item0;   // first (brick, segment) item of a launch of the segmented traversal
+    unsigned long long *part;   // topk == 8: 16 keys per cell and cluster segment, part_bricks bricks
+    uint32_t part_bricks;       // bricks of 4x4x4 cells whose lists fit in `part`
+    uint32_t brick0, brick_count;  // the chunk of bricks of one coarse launch
 };
-// topk 1: three launches (upper bound, brick x cluster-segment traversal, finish); topk 8: one.
+constexpr uint32_t SEG_CLUSTERS = 128;  // clusters per segment of the segmented traversals
+// topk 1: upper bound, (brick, cluster segment) traversal, finish; topk 8: upper bound, then per
+// chunk of bricks the (brick, segment) lists and their merge.
 hipError_t launch_bake_dense(hipStream_t st, const DenseArgs &a);
 
 struct FineArgs {

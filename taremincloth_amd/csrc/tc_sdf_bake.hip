@@ -5,17 +5,17 @@
 // nearest triangle (Ericson's closest point, sign from the face normal, first strict minimum
 // in triangle order wins), stored as pack2x16float(d, 1.0) in Z, Y, X order.
 //
-// Here one wavefront owns a brick of 8x8x8 voxels (8 per lane).  Triangles are sorted along a
-// Morton curve and grouped into clusters of 32 with a box each.  A wavefront first bounds the
-// nearest distance of all its voxels from above, then evaluates only the clusters and triangles
-// whose boxes can still hold a nearer triangle for one of its voxels (k_bake_brick).  Candidates
-// compare by (squared distance, original triangle index), so the result is the same argmin the
-// reference's in-order scan returns, whatever order the clusters are visited in.
+// Here the triangles are sorted along a Morton curve and grouped into clusters of 32 with a box
+// each.  Work items are (brick of voxels, segment of 128 clusters): a wavefront bounds its
+// voxels' nearest distances from above, then evaluates only the clusters and triangles whose
+// boxes can still hold a nearer triangle for one of its voxels, and merges per voxel through a
+// 64-bit key (squared distance bits, original triangle index).  The key orders like (d_sq,
+// index), so the result is the same argmin the reference's in-order scan returns, whatever
+// order the work is done in.
 //
-// The coarse pass of the hierarchical bake (bake_mesh_sdf_hier_coarse_main.wgsl) is the same
-// traversal with one voxel per lane (4x4x4 bricks) and the 16 nearest triangles per voxel, from
-// which the reference's eight-entry list is rebuilt; the fine pass
-// (bake_mesh_sdf_hier_fine_main.wgsl) is restated voxel by voxel.
+// The coarse pass of the hierarchical bake (bake_mesh_sdf_hier_coarse_main.wgsl) keeps the 16
+// nearest per cell the same way and rebuilds the reference's eight-entry list from them; the
+// fine pass (bake_mesh_sdf_hier_fine_main.wgsl) is restated voxel by voxel.
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -336,23 +336,6 @@ __device__ __forceinline__ float wave_max_f(float v) {
 // (d_sq, index) lexicographic "less": the in-order first strict minimum
 __device__ __forceinline__ bool better(float d, uint32_t i, float bd, uint32_t bi) { return d < bd || (d == bd && i < bi); }
 
-// The coarse pass's list update (bake_mesh_sdf_hier_coarse_main.wgsl:41-60): strict comparisons,
-// so an entry pushed down the list passes over the entries of equal distance below it.
-__device__ __forceinline__ void ref_insert(float (&sq)[8], float (&sg)[8], uint32_t (&id)[8], float d, float s,
-                                           uint32_t i) {
-    if (!(d < sq[7])) return;
-    float is = d, ig = s;
-    uint32_t ii = i;
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-        if (is < sq[k]) {
-            const float ts = sq[k], tg = sg[k];
-            const uint32_t ti = id[k];
-            sq[k] = is; sg[k] = ig; id[k] = ii;
-            is = ts; ig = tg; ii = ti;
-        }
-}
-
 constexpr uint32_t NO_TRI = 0xFFFFFFFFu;
 constexpr float FAR = 3.0e38f;
 
@@ -385,207 +368,6 @@ __device__ __forceinline__ float ub_box_box(f3 bmin, f3 bmax, f3 lo, f3 hi) {
     return dx * dx + dy * dy + dz * dz;
 }
 
-// One wavefront per brick of voxels: VPL voxels per lane (8 x 8 x VPL bricks for VPL > 1,
-// 4 x 4 x 4 for VPL = 1), the K nearest triangles per voxel in (d_sq, index) order.
-// K = 1: the dense bake.  K = 16: the coarse pass of the hierarchical bake.
-//   1. Upper bound U on the K-th nearest distance of every voxel in the brick: for K = 1 the
-//      farthest brick point from a vertex of some triangle, for K = 16 the farthest pair of
-//      points between the brick and the box of a cluster holding at least 16 triangles; the
-//      cluster giving the smallest U is evaluated first.
-//   2. Clusters (32 Morton-consecutive triangles) whose box lies farther than U (or than the
-//      worst current K-th best of the brick) from the brick are skipped, 64 at a time by ballot,
-//      walking outward from the seed cluster along the Morton order.
-//   3. Within a cluster, a triangle is evaluated only if its box is not farther than the current
-//      K-th best of some voxel of the wavefront, and only for the voxels where that holds.
-// Every skipped triangle is farther than a K-th best it could replace, so the lists equal those
-// of the exhaustive scan.
-// For K = 16 the 16 nearest hold every triangle at or below the eighth distance D8 unless more
-// than 16 do.  The reference's list is the result of its update rule applied in triangle order,
-// and only triangles at or below D8 can be in it or move an entry of it, so replaying exactly
-// those in index order gives the reference's list; with no equal distances among them the
-// replay is the identity and is skipped.  More than 16 at or below D8 replays every triangle.
-template <int VPL, int K>
-__global__ void __launch_bounds__(256) k_bake_brick(DenseArgs a) {
-    constexpr int BX = VPL == 1 ? 4 : 8, BY = VPL == 1 ? 4 : 8, BZ = VPL == 1 ? 4 : VPL;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t brick = blockIdx.x * 4u + (threadIdx.x >> 6);
-    const uint32_t nbx = (a.w + BX - 1) / BX, nby = (a.h + BY - 1) / BY, nbz = (a.d + BZ - 1) / BZ;
-    if (brick >= nbx * nby * nbz) return;  // whole wavefront
-    const uint32_t bx = brick % nbx, by = (brick / nbx) % nby, bz = brick / (nbx * nby);
-    // lane's voxels
-    uint32_t vx, vy, vz0;
-    if (VPL > 1) { vx = bx * 8 + (lane & 7u); vy = by * 8 + (lane >> 3); vz0 = bz * VPL; }
-    else { vx = bx * 4 + (lane & 3u); vy = by * 4 + ((lane >> 2) & 3u); vz0 = bz * 4 + (lane >> 4); }
-    const bool col_ok = vx < a.w && vy < a.h;
-    f3 p[VPL];
-    bool ok[VPL];
-    for (int j = 0; j < VPL; ++j) {
-        const uint32_t vz = vz0 + j;
-        ok[j] = col_ok && vz < a.d;
-        p[j] = mk3(voxel_coord(vx, a.w, a.lmin[0], a.lmax[0]), voxel_coord(vy, a.h, a.lmin[1], a.lmax[1]),
-                   voxel_coord(vz, a.d, a.lmin[2], a.lmax[2]));
-    }
-    // brick box over its valid voxel centres
-    const uint32_t x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
-    const uint32_t x1 = min(a.w, x0 + BX) - 1, y1 = min(a.h, y0 + BY) - 1, z1 = min(a.d, z0 + BZ) - 1;
-    const f3 bmin = mk3(voxel_coord(x0, a.w, a.lmin[0], a.lmax[0]), voxel_coord(y0, a.h, a.lmin[1], a.lmax[1]),
-                        voxel_coord(z0, a.d, a.lmin[2], a.lmax[2]));
-    const f3 bmax = mk3(voxel_coord(x1, a.w, a.lmin[0], a.lmax[0]), voxel_coord(y1, a.h, a.lmin[1], a.lmax[1]),
-                        voxel_coord(z1, a.d, a.lmin[2], a.lmax[2]));
-    float bsq[VPL][K], bsg[VPL][K];
-    uint32_t bid[VPL][K];
-    for (int j = 0; j < VPL; ++j)
-        for (int k = 0; k < K; ++k) { bsq[j][k] = 1e10f; bsg[j][k] = 10.0f; bid[j][k] = NO_TRI; }
-    const uint32_t ncl = a.nclusters;
-
-    // 1. upper bound and seed cluster
-    float ub = FAR;
-    uint32_t uc = 0;
-    for (uint32_t c = lane; c < ncl; c += 64u) {
-        float u = FAR;
-        if (K == 1) {
-            const BakeTri &t0 = a.tris[size_t(c) * CLUSTER];
-            u = ub_box_point(bmin, bmax, mk3(t0.p0x, t0.p0y, t0.p0z));
-        } else if (min(a.ntris - c * CLUSTER, CLUSTER) >= uint32_t(K)) {
-            const float4 lo = a.clo[c], hi = a.chi[c];
-            u = ub_box_box(bmin, bmax, mk3(lo.x, lo.y, lo.z), mk3(hi.x, hi.y, hi.z));
-        }
-        if (u < ub) { ub = u; uc = c; }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        const float ou = __shfl_xor(ub, o, 64);
-        const uint32_t oc = __shfl_xor(uc, o, 64);
-        if (ou < ub || (ou == ub && oc < uc)) { ub = ou; uc = oc; }
-    }
-    const float ubound = ub >= FAR ? FAR : ub * 1.0001f + 1e-12f;
-
-    // the cluster being evaluated, staged in LDS by the whole wavefront (p0, p1, p2, n per triangle)
-    __shared__ float4 s_tri[4][CLUSTER * 4];
-    __shared__ uint32_t s_idx[4][CLUSTER];
-    const uint32_t wv = threadIdx.x >> 6;
-    auto eval_cluster = [&](uint32_t c) {
-        const float4 cl4 = a.clo[c], ch4 = a.chi[c];
-        const f3 cl = mk3(cl4.x, cl4.y, cl4.z), ch = mk3(ch4.x, ch4.y, ch4.z);
-        bool need = false;
-        for (int j = 0; j < VPL; ++j) need = need || (ok[j] && !(lb_box_point(cl, ch, p[j]) > bsq[j][K - 1]));
-        if (!__any(need)) return;
-        const uint32_t t0 = c * CLUSTER, cnt = min(a.ntris - t0, CLUSTER);
-        const float4 *src = reinterpret_cast<const float4 *>(a.tris + t0);
-        __builtin_amdgcn_wave_barrier();  // the previous cluster's reads come first
-        if (lane < cnt * 4u) s_tri[wv][lane] = src[lane];
-        if (lane + 64u < cnt * 4u) s_tri[wv][lane + 64u] = src[lane + 64u];
-        if (lane < cnt) s_idx[wv][lane] = a.index[t0 + lane];
-        __builtin_amdgcn_wave_barrier();
-        for (uint32_t t = 0; t < cnt; ++t) {
-            const float4 q0 = s_tri[wv][4 * t], q1 = s_tri[wv][4 * t + 1], q2 = s_tri[wv][4 * t + 2];
-            const f3 A = mk3(q0.x, q0.y, q0.z), B = mk3(q1.x, q1.y, q1.z), C = mk3(q2.x, q2.y, q2.z);
-            const f3 tmin = min3(A, min3(B, C)), tmax = max3(A, max3(B, C));
-            bool tn[VPL];
-            bool any = false;
-            for (int j = 0; j < VPL; ++j) {
-                tn[j] = ok[j] && !(lb_box_point(tmin, tmax, p[j]) > bsq[j][K - 1]);
-                any = any || tn[j];
-            }
-            if (!__any(any)) continue;
-            const uint32_t idx = s_idx[wv][t];
-            const float4 q3 = s_tri[wv][4 * t + 3];
-            const f3 N = mk3(q3.x, q3.y, q3.z);
-            for (int j = 0; j < VPL; ++j) {
-                if (!tn[j]) continue;
-                const f3 q = closest_on_tri(p[j], A, B, C);
-                const f3 diff = p[j] - q;
-                const float dsq = dot(diff, diff);
-                if (!better(dsq, idx, bsq[j][K - 1], bid[j][K - 1])) continue;
-                const float sg = (dot(diff, N) >= 0.0f ? 1.0f : -1.0f) * sqrtf(dsq);
-                if (K == 1) {
-                    bsq[j][0] = dsq; bsg[j][0] = sg; bid[j][0] = idx;
-                } else {
-                    float is = dsq, ig = sg;
-                    uint32_t ii = idx;
-                    for (int k = 0; k < K; ++k)
-                        if (better(is, ii, bsq[j][k], bid[j][k])) {
-                            const float ts = bsq[j][k], tg = bsg[j][k];
-                            const uint32_t ti = bid[j][k];
-                            bsq[j][k] = is; bsg[j][k] = ig; bid[j][k] = ii;
-                            is = ts; ig = tg; ii = ti;
-                        }
-                }
-            }
-        }
-    };
-
-    // 2. seed cluster, then the rest outward along the Morton order, 64 clusters per ballot
-    eval_cluster(uc);
-    const int32_t ng = int32_t((ncl + 63u) / 64u), g0 = int32_t(uc / 64u);
-    int32_t up = g0, down = g0 - 1;
-    for (uint32_t step = 0; up < ng || down >= 0; ++step) {
-        int32_t g;
-        if ((step & 1u) == 0u) { if (up < ng) g = up++; else g = down--; }
-        else { if (down >= 0) g = down--; else g = up++; }
-        float worst = -1.0f;
-        for (int j = 0; j < VPL; ++j)
-            if (ok[j]) worst = fmaxf(worst, bsq[j][K - 1]);
-        const float bound = fminf(ubound, wave_max_f(worst));
-        const uint32_t c = uint32_t(g) * 64u + lane;
-        bool cand = false;
-        if (c < ncl && c != uc) {
-            const float4 lo = a.clo[c], hi = a.chi[c];
-            cand = !(lb_box_box(mk3(lo.x, lo.y, lo.z), mk3(hi.x, hi.y, hi.z), bmin, bmax) > bound);
-        }
-        uint64_t m = __ballot(cand);
-        while (m) {
-            const uint32_t l = uint32_t(__ffsll((unsigned long long)m) - 1);
-            m &= m - 1u;
-            eval_cluster(uint32_t(g) * 64u + l);
-        }
-    }
-    for (int j = 0; j < VPL; ++j) {
-        if (!ok[j]) continue;
-        const size_t flat = (size_t(vz0 + j) * a.h + vy) * a.w + vx;
-        a.out[flat] = pack_dist(bsg[j][0]);
-        if constexpr (K > 1) {
-            if (!a.nearest) continue;
-            float rsq[8], rsg[8];
-            uint32_t rid[8];
-            for (int k = 0; k < 8; ++k) { rsq[k] = bsq[j][k]; rsg[k] = bsg[j][k]; rid[k] = bid[j][k]; }
-            bool tie = false;
-            for (int k = 0; k < 8; ++k) tie = tie || bsq[j][k] == bsq[j][k + 1];
-            if (tie) {
-                const float d8 = bsq[j][7];
-                for (int k = 0; k < 8; ++k) { rsq[k] = 1e10f; rsg[k] = 10.0f; rid[k] = 0u; }
-                if (bsq[j][K - 1] > d8) {
-                    uint32_t used = 0u;
-                    for (int n = 0; n < K; ++n) {  // entries at or below D8 in increasing triangle index
-                        uint32_t pick = NO_TRI;
-                        float pd = 0.0f, ps = 0.0f;
-                        int pk = -1;
-                        for (int k = 0; k < K; ++k)
-                            if (!((used >> k) & 1u) && bsq[j][k] <= d8 && bid[j][k] < pick) {
-                                pick = bid[j][k]; pd = bsq[j][k]; ps = bsg[j][k]; pk = k;
-                            }
-                        if (pk < 0) break;
-                        used |= 1u << pk;
-                        ref_insert(rsq, rsg, rid, pd, ps, pick);
-                    }
-                } else {
-                    for (uint32_t i = 0; i < a.ntris; ++i) {
-                        const BakeTri tr = a.otris[i];
-                        const f3 A = mk3(tr.p0x, tr.p0y, tr.p0z), B = mk3(tr.p1x, tr.p1y, tr.p1z);
-                        const f3 C = mk3(tr.p2x, tr.p2y, tr.p2z);
-                        const f3 q = closest_on_tri(p[j], A, B, C);
-                        const f3 diff = p[j] - q;
-                        const float dsq = dot(diff, diff);
-                        if (!(dsq < rsq[7])) continue;
-                        ref_insert(rsq, rsg, rid, dsq, (dot(diff, mk3(tr.nx, tr.ny, tr.nz)) >= 0.0f ? 1.0f : -1.0f) * sqrtf(dsq),
-                                   i);
-                    }
-                }
-            }
-            for (int k = 0; k < 8; ++k) a.nearest[flat * 8 + k] = rid[k] == NO_TRI ? 0u : rid[k];
-        }
-    }
-}
-
 // ------------------------------------------------------------------ dense bake, segmented
 // The nearest triangle of a voxel is the minimum of the 64-bit key (squared distance bits << 32 |
 // original index): squared distances are non-negative, so their bits order like the values and
@@ -593,7 +375,7 @@ __global__ void __launch_bounds__(256) k_bake_brick(DenseArgs a) {
 // are (brick of 8 x 8 x DZ voxels, segment of SEG clusters) so that bricks whose voxels see many
 // nearly equidistant triangles (the middle of a closed surface) spread over many wavefronts;
 // each keeps the current key minima of its voxels as culling bounds and merges with atomicMin.
-constexpr uint32_t DZ = 2, SEG = 128;
+constexpr uint32_t DZ = 2, SEG = SEG_CLUSTERS;
 
 __device__ __forceinline__ unsigned long long dense_key(float dsq, uint32_t idx) {
     return (static_cast<unsigned long long>(__float_as_uint(dsq)) << 32) | idx;
@@ -748,6 +530,228 @@ __global__ void __launch_bounds__(256) k_bake_dense_finish(DenseArgs a) {
     a.out[i] = pack_dist((dot(diff, mk3(tr.nx, tr.ny, tr.nz)) >= 0.0f ? 1.0f : -1.0f) * sqrtf(dsq));
 }
 
+// ------------------------------------------------------------------ coarse pass, segmented
+// bake_mesh_sdf_hier_coarse_main.wgsl keeps each cell's eight nearest triangles with its own
+// update rule (strict comparisons, so an entry pushed down the list passes the entries of equal
+// distance below it), applied in triangle order.  Here the 16 nearest of each cell are found in
+// (d_sq, index) order by (brick of 4 x 4 x 4 cells, segment of SEG clusters) work items, each
+// keeping a sorted list of 64-bit keys, and merged per cell.  Only triangles at or below the
+// eighth distance D8 can be in the reference's list or move an entry of it, so replaying exactly
+// those in index order gives the reference's list; with no equal distances among them the
+// replay is the identity and is skipped.  More than 16 at or below D8 replays every triangle.
+constexpr uint32_t KC = 16;
+constexpr unsigned long long NO_KEY = ~0ull;
+
+__device__ __forceinline__ float key_dsq(unsigned long long k) {
+    return k == NO_KEY ? FAR : __uint_as_float(uint32_t(k >> 32));
+}
+
+struct CBrick {
+    uint32_t vx, vy, vz;
+    bool ok;
+    f3 p, bmin, bmax;
+};
+
+__device__ __forceinline__ uint32_t coarse_bricks(const DenseArgs &a) {
+    return ((a.w + 3) / 4) * ((a.h + 3) / 4) * ((a.d + 3) / 4);
+}
+
+__device__ __forceinline__ CBrick coarse_brick(const DenseArgs &a, uint32_t brick, uint32_t lane) {
+    CBrick b;
+    const uint32_t nbx = (a.w + 3) / 4, nby = (a.h + 3) / 4;
+    const uint32_t bx = brick % nbx, by = (brick / nbx) % nby, bz = brick / (nbx * nby);
+    b.vx = bx * 4 + (lane & 3u);
+    b.vy = by * 4 + ((lane >> 2) & 3u);
+    b.vz = bz * 4 + (lane >> 4);
+    b.ok = b.vx < a.w && b.vy < a.h && b.vz < a.d;
+    b.p = mk3(voxel_coord(b.vx, a.w, a.lmin[0], a.lmax[0]), voxel_coord(b.vy, a.h, a.lmin[1], a.lmax[1]),
+              voxel_coord(b.vz, a.d, a.lmin[2], a.lmax[2]));
+    const uint32_t x0 = bx * 4, y0 = by * 4, z0 = bz * 4;
+    const uint32_t x1 = min(a.w, x0 + 4) - 1, y1 = min(a.h, y0 + 4) - 1, z1 = min(a.d, z0 + 4) - 1;
+    b.bmin = mk3(voxel_coord(x0, a.w, a.lmin[0], a.lmax[0]), voxel_coord(y0, a.h, a.lmin[1], a.lmax[1]),
+                 voxel_coord(z0, a.d, a.lmin[2], a.lmax[2]));
+    b.bmax = mk3(voxel_coord(x1, a.w, a.lmin[0], a.lmax[0]), voxel_coord(y1, a.h, a.lmin[1], a.lmax[1]),
+                 voxel_coord(z1, a.d, a.lmin[2], a.lmax[2]));
+    return b;
+}
+
+// Upper bound on every cell's 16th nearest distance: the farthest pair of points between the
+// brick and a cluster holding at least 16 triangles.  The cells' bound keys start there.
+__global__ void __launch_bounds__(256) k_bake_coarse_bound(DenseArgs a) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t brick = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (brick >= coarse_bricks(a)) return;  // whole wavefront
+    const CBrick b = coarse_brick(a, brick, lane);
+    float ub = FAR;
+    for (uint32_t c = lane; c < a.nclusters; c += 64u)
+        if (min(a.ntris - c * CLUSTER, CLUSTER) >= KC) {
+            const float4 lo = a.clo[c], hi = a.chi[c];
+            ub = fminf(ub, ub_box_box(b.bmin, b.bmax, mk3(lo.x, lo.y, lo.z), mk3(hi.x, hi.y, hi.z)));
+        }
+    ub = wave_min_f(ub);
+    const unsigned long long key = ub >= FAR ? NO_KEY : dense_key(ub * 1.0001f + 1e-12f, NO_TRI);
+    if (b.ok) a.keys[(size_t(b.vz) * a.h + b.vy) * a.w + b.vx] = key;
+}
+
+// Insert `key` into the ascending list `l` (the largest falls off).
+__device__ __forceinline__ void insert_key(unsigned long long (&l)[KC], unsigned long long key) {
+#pragma unroll
+    for (uint32_t k = 0; k < KC; ++k)
+        if (key < l[k]) {
+            const unsigned long long t = l[k];
+            l[k] = key;
+            key = t;
+        }
+}
+
+__global__ void __launch_bounds__(256) k_bake_coarse_seg(DenseArgs a) {
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint32_t nseg = (a.nclusters + SEG - 1) / SEG;
+    const unsigned long long item = blockIdx.x * 4ull + wv;
+    if (item >= (unsigned long long)a.brick_count * nseg) return;  // whole wavefront
+    const uint32_t lb = uint32_t(item / nseg), seg = uint32_t(item % nseg);
+    const CBrick b = coarse_brick(a, a.brick0 + lb, lane);
+    const size_t flat = (size_t(b.vz) * a.h + b.vy) * a.w + b.vx;
+    unsigned long long l[KC];
+    for (uint32_t k = 0; k < KC; ++k) l[k] = NO_KEY;
+    unsigned long long gb = b.ok ? a.keys[flat] : NO_KEY;
+    float th = key_dsq(gb);
+    __shared__ float4 s_tri[4][CLUSTER * 4];
+    __shared__ uint32_t s_idx[4][CLUSTER];
+    const uint32_t c0 = seg * SEG, c1 = min(a.nclusters, c0 + SEG);
+    for (uint32_t base = c0; base < c1; base += 64u) {
+        if (b.ok && base != c0) {  // other work items of this cell may have tightened its bound
+            gb = min(gb, a.keys[flat]);
+            th = fminf(key_dsq(l[KC - 1]), key_dsq(gb));
+        }
+        const float bound = wave_max_f(b.ok ? th : -1.0f);
+        const uint32_t c = base + lane;
+        bool cand = false;
+        if (c < c1) {
+            const float4 lo = a.clo[c], hi = a.chi[c];
+            cand = !(lb_box_box(mk3(lo.x, lo.y, lo.z), mk3(hi.x, hi.y, hi.z), b.bmin, b.bmax) > bound);
+        }
+        uint64_t m = __ballot(cand);
+        while (m) {
+            const uint32_t cc = base + uint32_t(__ffsll((unsigned long long)m) - 1);
+            m &= m - 1u;
+            const float4 cl4 = a.clo[cc], ch4 = a.chi[cc];
+            const bool need = b.ok && !(lb_box_point(mk3(cl4.x, cl4.y, cl4.z), mk3(ch4.x, ch4.y, ch4.z), b.p) > th);
+            if (!__any(need)) continue;
+            const uint32_t t0 = cc * CLUSTER, cnt = min(a.ntris - t0, CLUSTER);
+            const float4 *src = reinterpret_cast<const float4 *>(a.tris + t0);
+            __builtin_amdgcn_wave_barrier();  // the previous cluster's reads come first
+            if (lane < cnt * 4u) s_tri[wv][lane] = src[lane];
+            if (lane + 64u < cnt * 4u) s_tri[wv][lane + 64u] = src[lane + 64u];
+            if (lane < cnt) s_idx[wv][lane] = a.index[t0 + lane];
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t t = 0; t < cnt; ++t) {
+                const float4 q0 = s_tri[wv][4 * t], q1 = s_tri[wv][4 * t + 1], q2 = s_tri[wv][4 * t + 2];
+                const f3 A = mk3(q0.x, q0.y, q0.z), B = mk3(q1.x, q1.y, q1.z), C = mk3(q2.x, q2.y, q2.z);
+                const bool tn = b.ok && !(lb_box_point(min3(A, min3(B, C)), max3(A, max3(B, C)), b.p) > th);
+                if (!__any(tn)) continue;
+                if (!tn) continue;
+                const f3 q = closest_on_tri(b.p, A, B, C);
+                const f3 diff = b.p - q;
+                const unsigned long long key = dense_key(dot(diff, diff), s_idx[wv][t]);
+                if (key < l[KC - 1]) {
+                    insert_key(l, key);
+                    th = fminf(key_dsq(l[KC - 1]), key_dsq(gb));
+                }
+            }
+        }
+    }
+    if (!b.ok) return;
+    unsigned long long *dst = a.part + (size_t(lb) * 64u + lane) * nseg * KC + size_t(seg) * KC;
+    for (uint32_t k = 0; k < KC; ++k) dst[k] = l[k];
+    if (l[KC - 1] != NO_KEY && l[KC - 1] < gb) atomicMin(&a.keys[flat], l[KC - 1]);
+}
+
+// The reference's list update on (squared distance, index) pairs (signs are not needed: the
+// cell's texel comes from its nearest triangle, the first entry either way).
+__device__ __forceinline__ void ref_insert_pair(float (&sq)[8], uint32_t (&id)[8], float d, uint32_t i) {
+    if (!(d < sq[7])) return;
+    float is = d;
+    uint32_t ii = i;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (is < sq[k]) {
+            const float ts = sq[k];
+            const uint32_t ti = id[k];
+            sq[k] = is; id[k] = ii;
+            is = ts; ii = ti;
+        }
+}
+
+// One thread per cell: merge the work items' lists, write the cell's texel and the reference's
+// eight-entry list.
+__global__ void __launch_bounds__(256) k_bake_coarse_merge(DenseArgs a) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (g >= a.brick_count * 64u) return;
+    const uint32_t lb = g / 64u, lane = g % 64u;
+    const CBrick b = coarse_brick(a, a.brick0 + lb, lane);
+    if (!b.ok) return;
+    const size_t flat = (size_t(b.vz) * a.h + b.vy) * a.w + b.vx;
+    const uint32_t nseg = (a.nclusters + SEG - 1) / SEG;
+    unsigned long long l[KC];
+    for (uint32_t k = 0; k < KC; ++k) l[k] = NO_KEY;
+    const unsigned long long *src = a.part + size_t(g) * nseg * KC;
+    for (uint32_t sg = 0; sg < nseg; ++sg)
+        for (uint32_t k = 0; k < KC; ++k) {
+            const unsigned long long key = src[size_t(sg) * KC + k];
+            if (key >= l[KC - 1]) break;  // each list is ascending
+            insert_key(l, key);
+        }
+    // texel: the nearest triangle, evaluated again for its sign
+    const uint32_t i0 = uint32_t(l[0]);
+    if (l[0] == NO_KEY || i0 >= a.ntris) {
+        a.out[flat] = pack_dist(10.0f);
+    } else {
+        const BakeTri tr = a.otris[i0];
+        const f3 q = closest_on_tri(b.p, mk3(tr.p0x, tr.p0y, tr.p0z), mk3(tr.p1x, tr.p1y, tr.p1z),
+                                    mk3(tr.p2x, tr.p2y, tr.p2z));
+        const f3 diff = b.p - q;
+        a.out[flat] = pack_dist((dot(diff, mk3(tr.nx, tr.ny, tr.nz)) >= 0.0f ? 1.0f : -1.0f) * sqrtf(dot(diff, diff)));
+    }
+    uint32_t rid[8];
+    bool tie = false;
+    for (int k = 0; k < 8; ++k) tie = tie || key_dsq(l[k]) == key_dsq(l[k + 1]);
+    if (!tie) {
+        for (int k = 0; k < 8; ++k) rid[k] = l[k] == NO_KEY ? 0u : uint32_t(l[k]);
+    } else {
+        float rsq[8];
+        for (int k = 0; k < 8; ++k) { rsq[k] = 1e10f; rid[k] = 0u; }
+        const float d8 = key_dsq(l[7]);
+        if (l[KC - 1] == NO_KEY || key_dsq(l[KC - 1]) > d8) {
+            // every triangle at or below D8 is in the list: replay those in index order
+            uint32_t used = 0u;
+            for (uint32_t n = 0; n < KC; ++n) {
+                uint32_t pick = NO_TRI;
+                float pd = 0.0f;
+                int pk = -1;
+                for (uint32_t k = 0; k < KC; ++k)
+                    if (!((used >> k) & 1u) && l[k] != NO_KEY && key_dsq(l[k]) <= d8 && uint32_t(l[k]) < pick) {
+                        pick = uint32_t(l[k]);
+                        pd = key_dsq(l[k]);
+                        pk = int(k);
+                    }
+                if (pk < 0) break;
+                used |= 1u << pk;
+                ref_insert_pair(rsq, rid, pd, pick);
+            }
+        } else {
+            for (uint32_t i = 0; i < a.ntris; ++i) {
+                const BakeTri tr = a.otris[i];
+                const f3 q = closest_on_tri(b.p, mk3(tr.p0x, tr.p0y, tr.p0z), mk3(tr.p1x, tr.p1y, tr.p1z),
+                                            mk3(tr.p2x, tr.p2y, tr.p2z));
+                const f3 diff = b.p - q;
+                ref_insert_pair(rsq, rid, dot(diff, diff), i);
+            }
+        }
+    }
+    for (int k = 0; k < 8; ++k) a.nearest[flat * 8 + k] = rid[k];
+}
+
 __device__ __forceinline__ void fine_candidate(f3 p, const BakeTri *tris, uint32_t ntris, uint32_t ti, uint32_t &best_i,
                                                float &best_sq, float &best_sg, float &second_sq, float &second_sg) {
     if (ti >= ntris || ti == best_i) return;
@@ -884,8 +888,19 @@ hipError_t launch_bake_dense(hipStream_t st, const DenseArgs &a) {
         }
         hipLaunchKernelGGL(k_bake_dense_finish, dim3(uint32_t((nvox + 255) / 256)), dim3(256), 0, st, a);
     } else {
+        // coarse pass: bounds, then bricks in chunks whose per-segment lists fit `part`
         const size_t nb = size_t((a.w + 3) / 4) * ((a.h + 3) / 4) * ((a.d + 3) / 4);
-        hipLaunchKernelGGL((k_bake_brick<1, 16>), dim3(uint32_t((nb + 3) / 4)), dim3(256), 0, st, a);
+        const size_t nseg = (a.nclusters + SEG - 1) / SEG;
+        if (a.part_bricks == 0) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_bake_coarse_bound, dim3(uint32_t((nb + 3) / 4)), dim3(256), 0, st, a);
+        for (size_t b0 = 0; b0 < nb; b0 += a.part_bricks) {
+            DenseArgs c = a;
+            c.brick0 = uint32_t(b0);
+            c.brick_count = uint32_t(std::min<size_t>(a.part_bricks, nb - b0));
+            hipLaunchKernelGGL(k_bake_coarse_seg, dim3(uint32_t((size_t(c.brick_count) * nseg + 3) / 4)), dim3(256), 0,
+                               st, c);
+            hipLaunchKernelGGL(k_bake_coarse_merge, dim3((c.brick_count * 64u + 255u) / 256u), dim3(256), 0, st, c);
+        }
     }
     return hipGetLastError();
 }
@@ -987,8 +1002,21 @@ int check_memory(const tc::sdf::Domain &dm, size_t bytes_per_voxel, size_t ntris
 }
 
 // The brick traversal (topk 1: dense bake, 8: coarse pass) over domain dm into device buffers.
+// Per-segment candidate lists of the coarse pass: 16 keys per cell and cluster segment, for as
+// many 4x4x4 bricks at a time as fit in 1 GiB.
+int alloc_part(DevAlloc &mem, const tc::sdf::Domain &cdm, uint32_t ncl, unsigned long long *&part,
+               uint32_t &part_bricks) {
+    const size_t nb = size_t((cdm.w + 3) / 4) * ((cdm.h + 3) / 4) * ((cdm.d + 3) / 4);
+    const size_t nseg = std::max<size_t>(1, (ncl + tc::sdf::SEG_CLUSTERS - 1) / tc::sdf::SEG_CLUSTERS);
+    const size_t per_brick = size_t(64) * nseg * 16;  // keys
+    const size_t budget = (size_t(1) << 30) / 8;
+    part_bricks = uint32_t(std::max<size_t>(1, std::min(nb, budget / per_brick)));
+    TC_HIP(mem.get(&part, size_t(part_bricks) * per_brick));
+    return TC_OK;
+}
+
 int launch_walk(const DevTris &t, const tc::sdf::Domain &dm, uint32_t topk, uint32_t *dout, uint32_t *dnear,
-                unsigned long long *keys, float &ms) {
+                unsigned long long *keys, unsigned long long *part, uint32_t part_bricks, float &ms) {
     tc::sdf::DenseArgs a;
     a.tris = t.sorted; a.otris = t.orig; a.index = t.index; a.clo = t.lo; a.chi = t.hi;
     a.ntris = t.n; a.nclusters = t.ncl;
@@ -999,6 +1027,10 @@ int launch_walk(const DevTris &t, const tc::sdf::Domain &dm, uint32_t topk, uint
     a.nearest = dnear;
     a.keys = keys;
     a.item0 = 0;
+    a.part = part;
+    a.part_bricks = part_bricks;
+    a.brick0 = 0;
+    a.brick_count = 0;
     EventPair ev;
     TC_HIP(ev.init());
     TC_HIP(hipEventRecord(ev.a, nullptr));
@@ -1023,16 +1055,20 @@ int download(tc_sdf_result &r, const tc::sdf::Domain &dm, const uint32_t *dout, 
 
 int bake_dense(const std::vector<BakeTri> &bt, const tc::sdf::Domain &dm, uint32_t topk, tc_sdf_result &r) {
     const size_t nvox = size_t(dm.w) * dm.h * dm.d;
-    if (int rc = check_memory(dm, topk == 8 ? 36 : 12, bt.size())) return rc;
+    if (int rc = check_memory(dm, topk == 8 ? 44 : 12, bt.size())) return rc;
     DevTris t;
     if (int rc = upload_tris(bt, dm, t)) return rc;
     DevAlloc mem;
     uint32_t *dout = nullptr, *dnear = nullptr;
-    unsigned long long *keys = nullptr;
+    unsigned long long *keys = nullptr, *part = nullptr;
+    uint32_t part_bricks = 0;
     TC_HIP(mem.get(&dout, nvox));
-    if (topk == 8) TC_HIP(mem.get(&dnear, nvox * 8));
-    else TC_HIP(mem.get(&keys, nvox));
-    if (int rc = launch_walk(t, dm, topk, dout, dnear, keys, r.kernel_ms[0])) return rc;
+    TC_HIP(mem.get(&keys, nvox));
+    if (topk == 8) {
+        TC_HIP(mem.get(&dnear, nvox * 8));
+        if (int rc = alloc_part(mem, dm, t.ncl, part, part_bricks)) return rc;
+    }
+    if (int rc = launch_walk(t, dm, topk, dout, dnear, keys, part, part_bricks, r.kernel_ms[0])) return rc;
     return download(r, dm, dout, dnear);
 }
 
@@ -1053,13 +1089,17 @@ int bake_hierarchical(const std::vector<BakeTri> &bt, const tc::sdf::Domain &dm,
     if (int rc = upload_tris(bt, dm, t)) return rc;
     DevAlloc mem;
     uint32_t *dcd = nullptr, *dct = nullptr, *dout = nullptr, *damb = nullptr, *dcount = nullptr;
+    unsigned long long *ckeys = nullptr, *part = nullptr;
+    uint32_t part_bricks = 0;
     TC_HIP(mem.get(&dcd, ncvox));
     TC_HIP(mem.get(&dct, ncvox * 8));
+    TC_HIP(mem.get(&ckeys, ncvox));
+    if (int rc = alloc_part(mem, cdm, t.ncl, part, part_bricks)) return rc;
     TC_HIP(mem.get(&dout, nvox));
     TC_HIP(mem.get(&damb, nvox));
     TC_HIP(mem.get(&dcount, 1));
     TC_HIP(hipMemset(dcount, 0, 4));
-    if (int rc = launch_walk(t, cdm, 8, dcd, dct, nullptr, r.kernel_ms[0])) return rc;
+    if (int rc = launch_walk(t, cdm, 8, dcd, dct, ckeys, part, part_bricks, r.kernel_ms[0])) return rc;
     tc::sdf::FineArgs fa;
     fa.tris = t.orig; fa.ntris = t.n;
     fa.coarse_dist = dcd; fa.coarse_tri = dct;
