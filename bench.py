@@ -230,6 +230,37 @@ def drape_bench(T, frames: int = 200, warmup: int = 5):
             "min_distance_to_centre_m": float(r.min())}
 
 
+def selfcol_bench(T, frames: int = 300, warmup: int = 5):
+    """A 256 x 256 cloth (4 mm spacing, 65,536 vertices) dropped from 0.5 m with +-1 mm jitter on a
+    sphere (r 0.3) and a capsule (r 0.1, length 0.6), friction 0.3, with self-collision (thickness
+    2 mm, hash cell 4 mm): 10 substeps x 20 iterations per frame."""
+    pos, edges, faces = grid_512(256, 0.004, 0.5, seed=5)
+    inv = T.compute_combined_inv_masses(pos, faces, areal_density=0.15)
+    sim = T.ClothSimulator(pos, edges, faces, inv, thickness=0.002, stiffness=1000.0, bending_stiffness=10.0)
+    sim.set_gravity(0.0, 0.0, -9.81)
+    sim.set_compliances_raw(1e-8, 1e-6, 1e-5)
+    sim.add_sphere_collider([0.0, 0.0, 0.0], 0.3, 0.3, 0.0)
+    sim.add_capsule_collider([-0.3, 0.35, 0.1], [0.3, 0.35, 0.1], 0.1, 0.3, 0.0)
+    sim.set_enable_self_collision(True)
+    sim.set_hash_cell_size(0.004)
+    sim.set_solver_iterations(20)
+    per_frame, dt_frame = 10, 1.0 / 60.0
+    for _ in range(warmup):
+        sim.step(dt_frame, per_frame)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sim.step(dt_frame, per_frame)
+    sim.synchronize()
+    el = time.perf_counter() - t0
+    kt = sim.time_kernels(dt_frame / per_frame, reps=3)
+    return {"workload": f"256x256 cloth (4 mm) with self-collision (thickness 2 mm, cell 4 mm) on a sphere and a "
+                        f"capsule, friction 0.3, 10 substeps x 20 iterations; {frames} timed frames after {warmup}",
+            "substeps_per_s": frames * per_frame / el, "ms_per_substep": el / (frames * per_frame) * 1e3,
+            "self_collision_ms_per_substep": kt["self_collision"],
+            "substep_ms_breakdown": {k: round(x, 5) for k, x in kt.items()}}
+
+
 def batch_bench(T, frames: int, warmup: int, cpu_sample: bool):
     """64 independent curtains of side U[16, 256] (seed 0, 1 cm spacing, top corners pinned) in one
     simulator: 10 substeps x 20 iterations per frame, one launch per iteration over all
@@ -312,6 +343,9 @@ def main():
     ap.add_argument("--no-sdf", action="store_true", help="skip the mesh SDF bake measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances measurement")
     ap.add_argument("--no-drape", action="store_true", help="skip the mesh-collider drape measurement")
+    ap.add_argument("--no-selfcol", action="store_true", help="skip the self-collision measurement")
+    ap.add_argument("--selfcol-only", action="store_true",
+                    help="measure only the 256x256 self-collision drop and print its JSON object")
     ap.add_argument("--drape-only", action="store_true",
                     help="measure only the drape on the icosphere mesh collider and print its JSON object")
     ap.add_argument("--batch-only", action="store_true",
@@ -331,6 +365,9 @@ def main():
 
     if args.sdf_only:
         print(json.dumps(sdf_bake_bench(T, not args.no_cpu_baseline)), flush=True)
+        return
+    if args.selfcol_only:
+        print(json.dumps(selfcol_bench(T)), flush=True)
         return
     if args.drape_only:
         print(json.dumps(drape_bench(T)), flush=True)
@@ -473,6 +510,7 @@ def main():
     sdf = None if args.no_sdf else sdf_bake_bench(T, not args.no_cpu_baseline)
     batch = None if args.no_batch else batch_bench(T, 30, 5, not args.no_cpu_baseline)
     drape = None if args.no_drape else drape_bench(T)
+    selfcol = None if args.no_selfcol else selfcol_bench(T)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -550,6 +588,7 @@ def main():
         "sdf_bake": sdf,
         "batch": batch,
         "mesh_collider_drape": drape,
+        "self_collision_drop": selfcol,
     }
     print(json.dumps(line), flush=True)
 
