@@ -75,7 +75,7 @@ void fused_alphas(FusedArgs &fa, float dt_sub);
 // --- spatial hash + self collision (tc_selfcol.hip) ---
 struct HashBufs {
     uint32_t *keys;     // n
-    uint32_t *counts;   // table (also used as scatter cursor)
+    uint32_t *counts;   // table + 64: per-bucket counts, then scatter cursors; the tail holds the scan's row totals
     uint32_t *starts;   // table + 1
     uint32_t *sorted;   // n
 };
@@ -104,9 +104,9 @@ struct SelfColTopo {
     const uint32_t *edge_bbits;
 };
 hipError_t launch_sc_solve(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t,
-                           const HashBufs &vh, Accum *acc, const SelfColArgs &a, uint32_t block);
+                           const HashBufs &vh, Accum *acc, const SelfColArgs &a);
 hipError_t launch_sc_solve_ee(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t,
-                              const HashBufs &eh, Accum *acc, const SelfColArgs &a, uint32_t block);
+                              const HashBufs &eh, Accum *acc, const SelfColArgs &a);
 hipError_t launch_sc_apply(hipStream_t s, float4 *P, const float *local_len, Accum *acc, const SelfColArgs &a,
                            uint32_t block);
 

@@ -7,12 +7,12 @@
 //   E-E solve         crates/cloth_core/src/shaders/self_collision_ee_direct.wgsl:66-776
 //   apply             crates/cloth_core/src/shaders/self_collision_apply.wgsl:57-177
 //
-// Hash build: count (atomics) -> one-workgroup exclusive scan of the table
-// built from wave64 prefix sums (__shfl_up inside a wave, LDS across the 16
-// waves) -> scatter (atomic cursor) -> per-bucket sort, so every bucket lists
-// its elements in ascending index order.  The reference's scatter order inside
-// a bucket depends on atomic timing; the sorted order makes the hash tables,
-// and with them the E-E search cut-off, deterministic.
+// Hash build: count (atomics) -> exclusive scan of the table (one workgroup per
+// row of 1024 entries, then the row totals) -> scatter (atomic cursor) ->
+// per-bucket sort, so every bucket lists its elements in ascending index order.  The
+// reference's scatter order inside a bucket depends on atomic timing; the sorted
+// order makes the hash tables, and with them the E-E search cut-off,
+// deterministic.
 //
 // The solve kernels accumulate displacements in 1e-6 m fixed point with
 // integer atomics, which makes the result independent of thread order.
@@ -44,39 +44,43 @@ __global__ void __launch_bounds__(256) k_hash_count_edges(const float4 *__restri
     atomicAdd(&counts[h], 1u);
 }
 
-// Exclusive scan of counts[0..table) into starts[0..table], starts[table] = total;
-// counts are reset to zero for use as scatter cursors.  One workgroup of 1024.
-__global__ void __launch_bounds__(1024) k_scan_table(uint32_t *__restrict__ counts, uint32_t *__restrict__ starts,
-                                                     uint32_t table) {
+// Exclusive scan of counts[0..table) into starts[0..table], starts[table] = total,
+// in two launches over rows of 1024 entries (at most SCAN_ROWS rows), one
+// workgroup per row, coalesced throughout.  k_scan_rows scans each row (wave64
+// prefix sums, then the 16 wavefront totals), writes the in-row exclusive sums,
+// keeps the row total in counts[table + row] and resets counts[0..table) to zero
+// for use as scatter cursors; k_scan_add adds the totals of the rows in front.
+constexpr uint32_t SCAN_ROWS = 64;
+__global__ void __launch_bounds__(1024) k_scan_rows(uint32_t *__restrict__ counts, uint32_t *__restrict__ starts,
+                                                    uint32_t table) {
     __shared__ uint32_t wave_tot[16];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t per = (table + 1023u) / 1024u;
-    const uint32_t beg = min(tid * per, table), end = min(beg + per, table);
-    uint32_t sum = 0;
-    for (uint32_t k = beg; k < end; ++k) sum += counts[k];
-    // inclusive prefix inside the wave
-    const uint32_t lane = tid & 63u, wave = tid >> 6;
-    uint32_t incl = sum;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t k = (blockIdx.x << 10) + tid;
+    const uint32_t c = k < table ? counts[k] : 0u;
+    uint32_t incl = c;
     for (uint32_t o = 1; o < 64; o <<= 1) {
         const uint32_t y = __shfl_up(incl, o, 64);
         if (lane >= o) incl += y;
     }
     if (lane == 63u) wave_tot[wave] = incl;
     __syncthreads();
-    uint32_t wave_base = 0;
-    for (uint32_t w = 0; w < wave; ++w) wave_base += wave_tot[w];
-    uint32_t run = wave_base + incl - sum;
-    for (uint32_t k = beg; k < end; ++k) {
-        const uint32_t c = counts[k];
-        starts[k] = run;
-        run += c;
+    uint32_t base = 0;
+    for (uint32_t w = 0; w < wave; ++w) base += wave_tot[w];
+    if (k < table) {
+        starts[k] = base + incl - c;
         counts[k] = 0u;
     }
-    if (tid == 1023u) {
-        uint32_t total = 0;
-        for (uint32_t w = 0; w < 16; ++w) total += wave_tot[w];
-        starts[table] = total;
-    }
+    if (tid == 1023u) counts[table + blockIdx.x] = base + incl;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_add(const uint32_t *__restrict__ counts, uint32_t *__restrict__ starts,
+                                                   uint32_t table) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, row = blockIdx.x;
+    uint32_t base = lane < row ? counts[table + lane] : 0u;  // row < SCAN_ROWS = 64 lanes
+    for (uint32_t o = 32; o != 0u; o >>= 1) base += __shfl_xor(base, int(o), 64);
+    const uint32_t k = (row << 10) + tid;
+    if (k < table) starts[k] += base;
+    if (row == gridDim.x - 1u && tid == 0u) starts[table] = base + counts[table + row];
 }
 
 __global__ void __launch_bounds__(256) k_hash_scatter(const uint32_t *__restrict__ keys, uint32_t n,
@@ -102,7 +106,10 @@ __global__ void __launch_bounds__(256) k_bucket_sort(const uint32_t *__restrict_
 }
 
 static hipError_t finish_hash(hipStream_t s, uint32_t n, uint32_t table, const HashBufs &h) {
-    hipLaunchKernelGGL(k_scan_table, dim3(1), dim3(1024), 0, s, h.counts, h.starts, table);
+    if (table == 0u || table > SCAN_ROWS * 1024u) return hipErrorInvalidValue;
+    const uint32_t rows = (table + 1023u) >> 10;
+    hipLaunchKernelGGL(k_scan_rows, dim3(rows), dim3(1024), 0, s, h.counts, h.starts, table);
+    hipLaunchKernelGGL(k_scan_add, dim3(rows), dim3(1024), 0, s, h.counts, h.starts, table);
     hipLaunchKernelGGL(k_hash_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, h.keys, n, h.starts, h.counts,
                        h.sorted);
     hipLaunchKernelGGL(k_bucket_sort, dim3(grid_for(table, 256)), dim3(256), 0, s, h.starts, table, h.sorted);
@@ -259,173 +266,252 @@ __device__ __forceinline__ bool near_folded_sq(const SelfColTopo &t, uint32_t a,
     return !(cur_sq < (l0 * l0) * 0.25f);
 }
 
-// sc_solve: one thread per vertex, 27 neighbour cells (single-layer cloth:
-// the multi-layer untangling branches of the reference are inactive).
-__global__ void __launch_bounds__(256) k_sc_solve(const float4 *__restrict__ P, const float4 *__restrict__ X,
-                                                  SelfColTopo T, const uint32_t *__restrict__ starts,
-                                                  const uint32_t *__restrict__ sorted, Accum *__restrict__ acc,
-                                                  SelfColArgs A) {
-    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
-    if (index >= A.nv) return;
+// Work distribution of the two solve kernels.  A wavefront takes two elements
+// (vertices or edges); lanes 0-26 and 32-58 each walk the hash bucket of one of
+// the element's 27 neighbour cells, numbered in the order of the reference's
+// E-E loop (dz outer, dx inner: cell = 9 (dz+1) + 3 (dy+1) + (dx+1)).  The
+// candidates that pass the cheap tests go into a per-wavefront LDS queue, which
+// the wavefront works off 64 pairs at a time, one pair per lane, so that the
+// expensive pair tests (CCD sweeps, two-hop lookups) run with full lanes.  The
+// fixed-point accumulation makes the result independent of this order.
+constexpr uint32_t SC_BLOCK = 256;
+constexpr uint32_t SC_QCAP = 128;  // queue entries per wavefront: < 64 left over + up to 64 pushed per step
+constexpr uint32_t SC_NONE = 0xFFFFFFFFu, SC_STOP = 0xFFFFFFFEu;
+
+__device__ __forceinline__ i3 neighbour_cell(i3 c, uint32_t k) {
+    return {c.x + (int32_t(k % 3u) - 1), c.y + (int32_t((k / 3u) % 3u) - 1), c.z + (int32_t(k / 9u) - 1)};
+}
+
+// Lockstep walk of every lane's bucket range [q, qe).  accept(k) tests entry k of
+// the bucket list and returns the partner element, SC_NONE to skip it, or SC_STOP
+// to end this lane's walk; pair(self, partner) processes one queued pair.  Every
+// lane of the wavefront must call this (the ballots and the queue are shared).
+template <class Accept, class Pair>
+__device__ __forceinline__ void sc_walk(uint2 *queue, uint32_t lane, uint32_t self, uint32_t q, uint32_t qe,
+                                        Accept accept, Pair pair) {
+    uint32_t qn = 0;  // queued pairs (the same value in every lane)
+    for (;;) {
+        const bool more = __any(q < qe) != 0;
+        bool push = false;
+        uint32_t other = 0;
+        if (q < qe) {
+            other = accept(q);
+            ++q;
+            if (other == SC_STOP) q = qe;
+            else push = other != SC_NONE;
+        }
+        const uint64_t m = __ballot(push);
+        if (push) queue[qn + uint32_t(__popcll(m & ((uint64_t(1) << lane) - 1u)))] = make_uint2(self, other);
+        qn += uint32_t(__popcll(m));
+        if (qn >= 64u || (!more && qn != 0u)) {
+            const uint32_t take = min(qn, 64u), rest = qn - take;
+            __builtin_amdgcn_wave_barrier();  // the pushes above come first
+            uint2 pr = make_uint2(0u, 0u), mv = make_uint2(0u, 0u);
+            if (lane < take) pr = queue[lane];
+            if (lane < rest) mv = queue[64u + lane];
+            __builtin_amdgcn_wave_barrier();
+            if (lane < rest) queue[lane] = mv;
+            qn = rest;
+            if (lane < take) pair(pr.x, pr.y);
+        }
+        if (!more && qn == 0u) break;
+    }
+}
+
+// V-V and V-T response of vertex `index` against vertex j and j's star, for a
+// pair that passed the cell and reach tests (self_collision.wgsl:388-681).
+__device__ __forceinline__ void vv_pair(const float4 *__restrict__ P, const float4 *__restrict__ X,
+                                        const SelfColTopo &T, Accum *__restrict__ acc, const SelfColArgs &A,
+                                        uint32_t index, uint32_t j) {
     const float EPS = 1e-7f;
     const float4 qi = P[index], xi = X[index];
     const bool pinned_i = qi.w <= 0.0f;
     const f3 xo_i = xyz(xi), p_i = xyz(qi);
     const float mv_i = length(p_i - xo_i);
-    if (pinned_i && mv_i < EPS) return;
     const float thick_i = xi.w;
     const float llen_i = T.local_len[index];
     const i3 cc = cell_of(p_i, A.cell);
-    for (int32_t cx = cc.x - 1; cx <= cc.x + 1; ++cx)
-    for (int32_t cy = cc.y - 1; cy <= cc.y + 1; ++cy)
-    for (int32_t cz = cc.z - 1; cz <= cc.z + 1; ++cz) {
-        const uint32_t h = hash_cell({cx, cy, cz}, A.table);
-        const uint32_t kb = starts[h], ke = starts[h + 1];
-        for (uint32_t k = kb; k < ke; ++k) {
-            const uint32_t j = sorted[k];
-            if (j >= A.nv || j == index) continue;
-            const float4 qj = P[j];
-            const f3 p_j = xyz(qj);
-            const i3 jc = cell_of(p_j, A.cell);
-            if (!(jc.x == cx && jc.y == cy && jc.z == cz)) continue;
-            const float4 xj = X[j];
-            const f3 dvv = p_i - p_j;
-            const float dsq = dot(dvv, dvv);
-            const float thick_j = xj.w, llen_j = T.local_len[j];
-            const float eff = fmaxf(thick_i + thick_j, fminf(llen_i, llen_j) * 0.06f);
-            const float max_geom = eff + (llen_i + llen_j) * 1.3f;
-            if (!(dsq <= max_geom * max_geom)) continue;
-            const float dvv_len = sqrtf(fmaxf(dsq, 0.0f));
-            if (near_folded(T, index, j, dvv_len)) continue;
-            const f3 pj_old = xyz(xj);
-            const float wi = qi.w, wj = qj.w;
-            const bool both_dynamic = wj > 0.0f;
-            const float eff_sq = eff * eff;
-            // 1. vertex-vertex
-            if (!pinned_i && (!both_dynamic || index < j)) {
-                if (dsq < eff_sq && dsq > EPS * EPS) {
-                    const f3 n = dvv / dvv_len;
-                    const float pen = eff - dvv_len;
-                    const float ws = wi + wj;
-                    if (ws > EPS) {
-                        if (wj <= 0.0f) {
-                            add_pin_disp(&acc[index], n * pen + (p_j - pj_old) * 0.5f);
-                        } else {
-                            const f3 disp = n * (pen * 0.5f);
-                            add_disp(&acc[index], disp * (wi / ws));
-                            add_disp(&acc[j], (-disp) * (wj / ws));
-                        }
+    const float4 qj = P[j], xj = X[j];
+    const f3 p_j = xyz(qj);
+    const f3 dvv = p_i - p_j;
+    const float dsq = dot(dvv, dvv);
+    const float thick_j = xj.w, llen_j = T.local_len[j];
+    const float eff = fmaxf(thick_i + thick_j, fminf(llen_i, llen_j) * 0.06f);
+    const float dvv_len = sqrtf(fmaxf(dsq, 0.0f));
+    if (near_folded(T, index, j, dvv_len)) return;
+    const f3 pj_old = xyz(xj);
+    const float wi = qi.w, wj = qj.w;
+    const bool both_dynamic = wj > 0.0f;
+    const float eff_sq = eff * eff;
+    // 1. vertex-vertex
+    if (!pinned_i && (!both_dynamic || index < j)) {
+        if (dsq < eff_sq && dsq > EPS * EPS) {
+            const f3 n = dvv / dvv_len;
+            const float pen = eff - dvv_len;
+            const float ws = wi + wj;
+            if (ws > EPS) {
+                if (wj <= 0.0f) {
+                    add_pin_disp(&acc[index], n * pen + (p_j - pj_old) * 0.5f);
+                } else {
+                    const f3 disp = n * (pen * 0.5f);
+                    add_disp(&acc[index], disp * (wi / ws));
+                    add_disp(&acc[j], (-disp) * (wj / ws));
+                }
+            }
+        }
+    }
+    // 2. vertex-triangle over the star of j
+    const float mv_j = length(p_j - pj_old);
+    const float vt_bound = eff + llen_j * 1.3f + mv_i + mv_j;
+    if (!(dsq <= vt_bound * vt_bound) || within_two_hops(T, index, j)) return;
+    for (uint32_t st = T.star_off[j]; st < T.star_off[j + 1]; ++st) {
+        const uint2 pr = T.star[st];
+        const uint32_t v0 = pr.x, v1 = pr.y;
+        if (v0 == index || v1 == index) continue;
+        const float4 q0 = P[v0], q1 = P[v1], x0 = X[v0], x1 = X[v1];
+        const f3 p_v0 = xyz(q0), p_v1 = xyz(q1), p_v0o = xyz(x0), p_v1o = xyz(x1);
+        const f3 d0 = p_i - p_v0;
+        const float ll0 = T.local_len[v0];
+        const float et0 = fmaxf(thick_i + x0.w, fminf(llen_i, ll0) * 0.06f);
+        const float b0 = et0 + ll0 * 1.3f + mv_i + length(p_v0 - p_v0o);
+        const i3 c0 = cell_of(p_v0, A.cell);
+        const bool in0 = abs(c0.x - cc.x) <= 1 && abs(c0.y - cc.y) <= 1 && abs(c0.z - cc.z) <= 1 &&
+                         dot(d0, d0) <= b0 * b0;
+        const f3 d1 = p_i - p_v1;
+        const float ll1 = T.local_len[v1];
+        const float et1 = fmaxf(thick_i + x1.w, fminf(llen_i, ll1) * 0.06f);
+        const float b1 = et1 + ll1 * 1.3f + mv_i + length(p_v1 - p_v1o);
+        const i3 c1 = cell_of(p_v1, A.cell);
+        const bool in1 = abs(c1.x - cc.x) <= 1 && abs(c1.y - cc.y) <= 1 && abs(c1.z - cc.z) <= 1 &&
+                         dot(d1, d1) <= b1 * b1;
+        if ((in0 && j > v0) || (in1 && j > v1)) continue;
+        const f3 tri_move = (((p_j - pj_old) + (p_v0 - p_v0o)) + (p_v1 - p_v1o)) / 3.0f;
+        const bool tri_pin = wj <= 0.0f || q0.w <= 0.0f || q1.w <= 0.0f;
+        const f3 rel_old = xo_i + tri_move;
+        const f3 ev = mk3(eff, eff, eff);
+        const f3 smin = min3(rel_old, p_i) - ev, smax = max3(rel_old, p_i) + ev;
+        const f3 tmin = min3(min3(p_j, p_v0), p_v1), tmax = max3(max3(p_j, p_v0), p_v1);
+        if (smin.x > tmax.x || smin.y > tmax.y || smin.z > tmax.z || smax.x < tmin.x || smax.y < tmin.y ||
+            smax.z < tmin.z)
+            continue;
+        if (within_two_hops(T, index, v0) || within_two_hops(T, index, v1)) continue;
+        const f3 move = p_i - rel_old;
+        const float move_sq = dot(move, move);
+        float ht = 1.0f, hu = 0.0f, hv = 0.0f;
+        bool hit = false;
+        if (move_sq > 1e-12f) hit = seg_tri_uv(rel_old, p_i, p_j, p_v0, p_v1, ht, hu, hv);
+        const float w0 = q0.w, w1 = q1.w;
+        if (hit) {
+            const f3 hit_pt = rel_old + move * ht;
+            const f3 tcr = cross(p_v0 - p_j, p_v1 - p_j);
+            const float tl = length(tcr);
+            const f3 tn = tl > 1e-12f ? tcr / tl : mk3(0.0f, 1.0f, 0.0f);
+            const float h_old_raw = dot(rel_old - p_j, tn);
+            const float side = fabsf(h_old_raw) > 1e-6f ? h_old_raw : dot(-move, tn);
+            const f3 n_side = side >= 0.0f ? tn : -tn;
+            const float h_old = fmaxf(dot(rel_old - hit_pt, n_side), 0.0f);
+            const float h_safe = fmaxf(h_old, eff * 0.25f);
+            const f3 total = (hit_pt + n_side * h_safe) - p_i;
+            const float pen_soft = fmaxf(0.0f, eff - h_safe);
+            const f3 bc = mk3(clampf(1.0f - hu - hv, 0.0f, 1.0f), clampf(hu, 0.0f, 1.0f), clampf(hv, 0.0f, 1.0f));
+            if (tri_pin) {
+                if (!pinned_i) add_ccd(&acc[index], total + n_side * pen_soft);
+            } else if (pinned_i) {
+                const float wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+                if (wt > EPS) {
+                    const f3 full = total + n_side * pen_soft;
+                    if (wj > 0.0f) add_ccd(&acc[j], (-full) * (bc.x * wj / wt));
+                    if (w0 > 0.0f) add_ccd(&acc[v0], (-full) * (bc.y * w0 / wt));
+                    if (w1 > 0.0f) add_ccd(&acc[v1], (-full) * (bc.z * w1 / wt));
+                }
+            } else {
+                const float wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+                const float wtot = wi + wt;
+                if (wtot > EPS) {
+                    add_ccd(&acc[index], total * (wi / wtot));
+                    if (wj > 0.0f) add_ccd(&acc[j], (-total) * (bc.x * wj / wtot));
+                    if (w0 > 0.0f) add_ccd(&acc[v0], (-total) * (bc.y * w0 / wtot));
+                    if (w1 > 0.0f) add_ccd(&acc[v1], (-total) * (bc.z * w1 / wtot));
+                    if (pen_soft > EPS) {
+                        const f3 tp = n_side * (pen_soft * 0.6f);
+                        add_disp(&acc[index], tp * (wi / wtot));
+                        if (wj > 0.0f) add_disp(&acc[j], (-tp) * (bc.x * wj / wtot));
+                        if (w0 > 0.0f) add_disp(&acc[v0], (-tp) * (bc.y * w0 / wtot));
+                        if (w1 > 0.0f) add_disp(&acc[v1], (-tp) * (bc.z * w1 / wtot));
                     }
                 }
             }
-            // 2. vertex-triangle over the star of j
-            const float mv_j = length(p_j - pj_old);
-            const float vt_bound = eff + llen_j * 1.3f + mv_i + mv_j;
-            if (!(dsq <= vt_bound * vt_bound) || within_two_hops(T, index, j)) continue;
-            for (uint32_t st = T.star_off[j]; st < T.star_off[j + 1]; ++st) {
-                const uint2 pr = T.star[st];
-                const uint32_t v0 = pr.x, v1 = pr.y;
-                if (v0 == index || v1 == index) continue;
-                const float4 q0 = P[v0], q1 = P[v1], x0 = X[v0], x1 = X[v1];
-                const f3 p_v0 = xyz(q0), p_v1 = xyz(q1), p_v0o = xyz(x0), p_v1o = xyz(x1);
-                const f3 d0 = p_i - p_v0;
-                const float ll0 = T.local_len[v0];
-                const float et0 = fmaxf(thick_i + x0.w, fminf(llen_i, ll0) * 0.06f);
-                const float b0 = et0 + ll0 * 1.3f + mv_i + length(p_v0 - p_v0o);
-                const i3 c0 = cell_of(p_v0, A.cell);
-                const bool in0 = abs(c0.x - cc.x) <= 1 && abs(c0.y - cc.y) <= 1 && abs(c0.z - cc.z) <= 1 &&
-                                 dot(d0, d0) <= b0 * b0;
-                const f3 d1 = p_i - p_v1;
-                const float ll1 = T.local_len[v1];
-                const float et1 = fmaxf(thick_i + x1.w, fminf(llen_i, ll1) * 0.06f);
-                const float b1 = et1 + ll1 * 1.3f + mv_i + length(p_v1 - p_v1o);
-                const i3 c1 = cell_of(p_v1, A.cell);
-                const bool in1 = abs(c1.x - cc.x) <= 1 && abs(c1.y - cc.y) <= 1 && abs(c1.z - cc.z) <= 1 &&
-                                 dot(d1, d1) <= b1 * b1;
-                if ((in0 && j > v0) || (in1 && j > v1)) continue;
-                const f3 tri_move = (((p_j - pj_old) + (p_v0 - p_v0o)) + (p_v1 - p_v1o)) / 3.0f;
-                const bool tri_pin = wj <= 0.0f || q0.w <= 0.0f || q1.w <= 0.0f;
-                const f3 rel_old = xo_i + tri_move;
-                const f3 ev = mk3(eff, eff, eff);
-                const f3 smin = min3(rel_old, p_i) - ev, smax = max3(rel_old, p_i) + ev;
-                const f3 tmin = min3(min3(p_j, p_v0), p_v1), tmax = max3(max3(p_j, p_v0), p_v1);
-                if (smin.x > tmax.x || smin.y > tmax.y || smin.z > tmax.z || smax.x < tmin.x || smax.y < tmin.y ||
-                    smax.z < tmin.z)
-                    continue;
-                if (within_two_hops(T, index, v0) || within_two_hops(T, index, v1)) continue;
-                const f3 move = p_i - rel_old;
-                const float move_sq = dot(move, move);
-                float ht = 1.0f, hu = 0.0f, hv = 0.0f;
-                bool hit = false;
-                if (move_sq > 1e-12f) hit = seg_tri_uv(rel_old, p_i, p_j, p_v0, p_v1, ht, hu, hv);
-                const float w0 = q0.w, w1 = q1.w;
-                if (hit) {
-                    const f3 hit_pt = rel_old + move * ht;
-                    const f3 tcr = cross(p_v0 - p_j, p_v1 - p_j);
-                    const float tl = length(tcr);
-                    const f3 tn = tl > 1e-12f ? tcr / tl : mk3(0.0f, 1.0f, 0.0f);
-                    const float h_old_raw = dot(rel_old - p_j, tn);
-                    const float side = fabsf(h_old_raw) > 1e-6f ? h_old_raw : dot(-move, tn);
-                    const f3 n_side = side >= 0.0f ? tn : -tn;
-                    const float h_old = fmaxf(dot(rel_old - hit_pt, n_side), 0.0f);
-                    const float h_safe = fmaxf(h_old, eff * 0.25f);
-                    const f3 total = (hit_pt + n_side * h_safe) - p_i;
-                    const float pen_soft = fmaxf(0.0f, eff - h_safe);
-                    const f3 bc = mk3(clampf(1.0f - hu - hv, 0.0f, 1.0f), clampf(hu, 0.0f, 1.0f), clampf(hv, 0.0f, 1.0f));
-                    if (tri_pin) {
-                        if (!pinned_i) add_ccd(&acc[index], total + n_side * pen_soft);
-                    } else if (pinned_i) {
-                        const float wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
-                        if (wt > EPS) {
-                            const f3 full = total + n_side * pen_soft;
-                            if (wj > 0.0f) add_ccd(&acc[j], (-full) * (bc.x * wj / wt));
-                            if (w0 > 0.0f) add_ccd(&acc[v0], (-full) * (bc.y * w0 / wt));
-                            if (w1 > 0.0f) add_ccd(&acc[v1], (-full) * (bc.z * w1 / wt));
-                        }
-                    } else {
-                        const float wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
-                        const float wtot = wi + wt;
-                        if (wtot > EPS) {
-                            add_ccd(&acc[index], total * (wi / wtot));
-                            if (wj > 0.0f) add_ccd(&acc[j], (-total) * (bc.x * wj / wtot));
-                            if (w0 > 0.0f) add_ccd(&acc[v0], (-total) * (bc.y * w0 / wtot));
-                            if (w1 > 0.0f) add_ccd(&acc[v1], (-total) * (bc.z * w1 / wtot));
-                            if (pen_soft > EPS) {
-                                const f3 tp = n_side * (pen_soft * 0.6f);
-                                add_disp(&acc[index], tp * (wi / wtot));
-                                if (wj > 0.0f) add_disp(&acc[j], (-tp) * (bc.x * wj / wtot));
-                                if (w0 > 0.0f) add_disp(&acc[v0], (-tp) * (bc.y * w0 / wtot));
-                                if (w1 > 0.0f) add_disp(&acc[v1], (-tp) * (bc.z * w1 / wtot));
-                            }
-                        }
-                    }
+        } else {
+            f3 bc;
+            const f3 cp = closest_bary(p_i, p_j, p_v0, p_v1, bc);
+            const f3 dvt = p_i - cp;
+            const float dvsq = dot(dvt, dvt);
+            if (!pinned_i && dvsq < eff_sq && dvsq > EPS * EPS) {
+                const float dl = sqrtf(dvsq);
+                const f3 nvt = dvt / dl;
+                const float pen = eff - dl;
+                if (tri_pin) {
+                    add_pin_disp(&acc[index], nvt * pen + tri_move * 0.5f);
                 } else {
-                    f3 bc;
-                    const f3 cp = closest_bary(p_i, p_j, p_v0, p_v1, bc);
-                    const f3 dvt = p_i - cp;
-                    const float dvsq = dot(dvt, dvt);
-                    if (!pinned_i && dvsq < eff_sq && dvsq > EPS * EPS) {
-                        const float dl = sqrtf(dvsq);
-                        const f3 nvt = dvt / dl;
-                        const float pen = eff - dl;
-                        if (tri_pin) {
-                            add_pin_disp(&acc[index], nvt * pen + tri_move * 0.5f);
-                        } else {
-                            const float wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
-                            const float wtot = wi + wt;
-                            if (wtot > EPS) {
-                                const f3 tp = nvt * (pen * 0.6f);
-                                add_disp(&acc[index], tp * (wi / wtot));
-                                if (wj > 0.0f) add_disp(&acc[j], (-tp) * (bc.x * wj / wtot));
-                                if (w0 > 0.0f) add_disp(&acc[v0], (-tp) * (bc.y * w0 / wtot));
-                                if (w1 > 0.0f) add_disp(&acc[v1], (-tp) * (bc.z * w1 / wtot));
-                            }
-                        }
+                    const float wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+                    const float wtot = wi + wt;
+                    if (wtot > EPS) {
+                        const f3 tp = nvt * (pen * 0.6f);
+                        add_disp(&acc[index], tp * (wi / wtot));
+                        if (wj > 0.0f) add_disp(&acc[j], (-tp) * (bc.x * wj / wtot));
+                        if (w0 > 0.0f) add_disp(&acc[v0], (-tp) * (bc.y * w0 / wtot));
+                        if (w1 > 0.0f) add_disp(&acc[v1], (-tp) * (bc.z * w1 / wtot));
                     }
                 }
             }
         }
     }
+}
+
+// sc_solve over the 27 neighbour cells of each vertex (single-layer cloth: the
+// multi-layer untangling branches of the reference are inactive).
+__global__ void __launch_bounds__(SC_BLOCK) k_sc_solve(const float4 *__restrict__ P, const float4 *__restrict__ X,
+                                                       SelfColTopo T, const uint32_t *__restrict__ starts,
+                                                       const uint32_t *__restrict__ sorted, Accum *__restrict__ acc,
+                                                       SelfColArgs A) {
+    __shared__ uint2 s_queue[SC_BLOCK / 64][SC_QCAP];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, ci = lane & 31u;
+    const uint32_t index = (blockIdx.x * (SC_BLOCK / 64) + wv) * 2u + (lane >> 5);
+    const float EPS = 1e-7f;
+    uint32_t q = 0, qe = 0;
+    f3 p_i = mk3(0.0f, 0.0f, 0.0f);
+    float thick_i = 0.0f, llen_i = 0.0f;
+    i3 nc = {0, 0, 0};
+    if (index < A.nv && ci < 27u) {
+        const float4 qi = P[index], xi = X[index];
+        p_i = xyz(qi);
+        const float mv_i = length(p_i - xyz(xi));
+        if (!(qi.w <= 0.0f && mv_i < EPS)) {  // a pinned vertex that did not move takes no part
+            thick_i = xi.w;
+            llen_i = T.local_len[index];
+            nc = neighbour_cell(cell_of(p_i, A.cell), ci);
+            const uint32_t h = hash_cell(nc, A.table);
+            q = starts[h];
+            qe = starts[h + 1];
+        }
+    }
+    auto accept = [&](uint32_t k) -> uint32_t {
+        const uint32_t j = sorted[k];
+        if (j >= A.nv || j == index) return SC_NONE;
+        const float4 qj = P[j];
+        const f3 p_j = xyz(qj);
+        const i3 jc = cell_of(p_j, A.cell);
+        if (!(jc.x == nc.x && jc.y == nc.y && jc.z == nc.z)) return SC_NONE;
+        const float4 xj = X[j];
+        const f3 dvv = p_i - p_j;
+        const float dsq = dot(dvv, dvv);
+        const float thick_j = xj.w, llen_j = T.local_len[j];
+        const float eff = fmaxf(thick_i + thick_j, fminf(llen_i, llen_j) * 0.06f);
+        const float max_geom = eff + (llen_i + llen_j) * 1.3f;
+        return dsq <= max_geom * max_geom ? j : SC_NONE;
+    };
+    sc_walk(s_queue[wv], lane, index, q, qe, accept,
+            [&](uint32_t i, uint32_t j) { vv_pair(P, X, T, acc, A, i, j); });
 }
 
 // ---------------------------------------------------------------------------
@@ -462,12 +548,12 @@ __device__ __forceinline__ float sweep_volume(const Sweep &w, float tau) {
 }
 __device__ __forceinline__ bool edge_bit(const uint32_t *bits, uint32_t e) { return ((bits[e >> 5] >> (e & 31u)) & 1u) != 0u; }
 
-__global__ void __launch_bounds__(256) k_sc_solve_ee(const float4 *__restrict__ P, const float4 *__restrict__ X,
-                                                     SelfColTopo T, const uint32_t *__restrict__ starts,
-                                                     const uint32_t *__restrict__ sorted, Accum *__restrict__ acc,
-                                                     SelfColArgs A) {
-    const uint32_t ea = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ea >= A.ne) return;
+// E-E response of edge ea against edge eb for a pair that passed the box test
+// (self_collision_ee_direct.wgsl:266-776, the candidate loop's body after the
+// search counter).
+__device__ __forceinline__ void ee_pair(const float4 *__restrict__ P, const float4 *__restrict__ X,
+                                        const SelfColTopo &T, Accum *__restrict__ acc, const SelfColArgs &A,
+                                        uint32_t ea, uint32_t eb) {
     const float EPS = 1e-7f;
     const uint2 eA = T.edges[ea];
     const uint32_t u0 = eA.x, u1 = eA.y;
@@ -478,212 +564,253 @@ __global__ void __launch_bounds__(256) k_sc_solve_ee(const float4 *__restrict__ 
     const float th_u0 = xu0.w, th_u1 = xu1.w;
     const float max_th_a = fmaxf(th_u0, th_u1);
     const bool bnd_a = edge_bit(T.edge_bbits, ea);
-    const f3 mta = mk3(max_th_a * 2.0f, max_th_a * 2.0f, max_th_a * 2.0f);
-    const f3 amin = min3(min3(p_u0, p_u1), min3(p_u0o, p_u1o)) - mta;
-    const f3 amax = max3(max3(p_u0, p_u1), max3(p_u0o, p_u1o)) + mta;
-    const i3 bc = cell_of((p_u0 + p_u1) * 0.5f, A.cell);
-    uint32_t search_count = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-    for (int dx = -1; dx <= 1; ++dx) {
-        const i3 nc = {bc.x + dx, bc.y + dy, bc.z + dz};
-        const uint32_t h = hash_cell(nc, A.table);
-        const uint32_t kb = starts[h], ke = starts[h + 1];
-        for (uint32_t q = kb; q < ke; ++q) {
-            const uint32_t eb = sorted[q];
-            if (ea >= eb) continue;
-            const uint2 eB = T.edges[eb];
-            const uint32_t j = eB.x, vj = eB.y;
-            if (j == u0 || j == u1 || vj == u0 || vj == u1) continue;
-            const float4 qj = P[j], qvj = P[vj];
-            const f3 p_j = xyz(qj), p_vj = xyz(qvj);
-            const i3 bcell = cell_of((p_j + p_vj) * 0.5f, A.cell);
-            if (!(bcell.x == nc.x && bcell.y == nc.y && bcell.z == nc.z)) continue;
-            const float4 xj = X[j], xvj = X[vj];
-            const f3 p_jo = xyz(xj), p_vjo = xyz(xvj);
-            const f3 bmin = min3(min3(p_j, p_vj), min3(p_jo, p_vjo)), bmax = max3(max3(p_j, p_vj), max3(p_jo, p_vjo));
-            if (amin.x > bmax.x || amin.y > bmax.y || amin.z > bmax.z || amax.x < bmin.x || amax.y < bmin.y ||
-                amax.z < bmin.z)
-                continue;
-            ++search_count;
-            if (search_count > A.max_search_iterations) return;
-            const bool bnd_pair = bnd_a || edge_bit(T.edge_bbits, eb);
-            bool run_ccd = bnd_pair, has_cached = false;
-            ST st1c = {0.0f, 0.0f};
-            f3 d1c = mk3(0.0f, 0.0f, 0.0f);
-            float dsqc = 0.0f, effc = 0.0f;
-            const float th_j = xj.w, th_vj = xvj.w;
-            const float wj = qj.w, wvj = qvj.w;
-            if (!run_ccd) {
-                st1c = closest_segs(p_u0, p_u1, p_j, p_vj);
-                d1c = mix3(p_u0, p_u1, st1c.s) - mix3(p_j, p_vj, st1c.t);
-                dsqc = dot(d1c, d1c);
-                effc = mixf(th_u0, th_u1, st1c.s) + mixf(th_j, th_vj, st1c.t);
-                const float max_eff = max_th_a + fmaxf(th_j, th_vj);
-                const f3 du0 = p_u0 - p_u0o, du1 = p_u1 - p_u1o, dj = p_j - p_jo, dvj = p_vj - p_vjo;
-                const f3 va = (du0 + du1) * 0.5f, vb = (dj + dvj) * 0.5f;
-                const float rel = length(va - vb) + 0.5f * (length(du1 - du0) + length(dvj - dj));
-                const float reach = max_eff + rel;
-                if (dsqc > reach * reach) continue;
-                has_cached = true;
-                const f3 d0g = mix3(p_u0o, p_u1o, st1c.s) - mix3(p_jo, p_vjo, st1c.t);
-                run_ccd = (rel > effc * 0.5f) || (dot(d0g, d1c) <= 0.0f);
+    const uint2 eB = T.edges[eb];
+    const uint32_t j = eB.x, vj = eB.y;
+    const float4 qj = P[j], qvj = P[vj], xj = X[j], xvj = X[vj];
+    const f3 p_j = xyz(qj), p_vj = xyz(qvj), p_jo = xyz(xj), p_vjo = xyz(xvj);
+    const bool bnd_pair = bnd_a || edge_bit(T.edge_bbits, eb);
+    bool run_ccd = bnd_pair, has_cached = false;
+    ST st1c = {0.0f, 0.0f};
+    f3 d1c = mk3(0.0f, 0.0f, 0.0f);
+    float dsqc = 0.0f, effc = 0.0f;
+    const float th_j = xj.w, th_vj = xvj.w;
+    const float wj = qj.w, wvj = qvj.w;
+    if (!run_ccd) {
+        st1c = closest_segs(p_u0, p_u1, p_j, p_vj);
+        d1c = mix3(p_u0, p_u1, st1c.s) - mix3(p_j, p_vj, st1c.t);
+        dsqc = dot(d1c, d1c);
+        effc = mixf(th_u0, th_u1, st1c.s) + mixf(th_j, th_vj, st1c.t);
+        const float max_eff = max_th_a + fmaxf(th_j, th_vj);
+        const f3 du0 = p_u0 - p_u0o, du1 = p_u1 - p_u1o, dj = p_j - p_jo, dvj = p_vj - p_vjo;
+        const f3 va = (du0 + du1) * 0.5f, vb = (dj + dvj) * 0.5f;
+        const float rel = length(va - vb) + 0.5f * (length(du1 - du0) + length(dvj - dj));
+        const float reach = max_eff + rel;
+        if (dsqc > reach * reach) return;
+        has_cached = true;
+        const f3 d0g = mix3(p_u0o, p_u1o, st1c.s) - mix3(p_jo, p_vjo, st1c.t);
+        run_ccd = (rel > effc * 0.5f) || (dot(d0g, d1c) <= 0.0f);
+    }
+    if (run_ccd) {
+        const Sweep W = {p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj};
+        float min_sq = 1e20f, best_tau = 0.0f, half_span = 0.125f;
+        ST min_st = {0.0f, 0.0f};
+        float prev_vol = sweep_volume(W, 0.0f);
+        float zc_tau = -1.0f;
+        for (uint32_t k = 0; k <= 8u; ++k) {
+            const float tk = float(k) * 0.125f;
+            ST stk;
+            const float dk = sweep_dist_sq(W, tk, stk);
+            if (dk < min_sq) { min_sq = dk; best_tau = tk; min_st = stk; }
+            if (k > 0u) {
+                const float vol = sweep_volume(W, tk);
+                if (prev_vol * vol <= 0.0f && zc_tau < 0.0f) zc_tau = tk - 0.0625f;
+                prev_vol = vol;
             }
-            if (run_ccd) {
-                const Sweep W = {p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj};
-                float min_sq = 1e20f, best_tau = 0.0f, half_span = 0.125f;
-                ST min_st = {0.0f, 0.0f};
-                float prev_vol = sweep_volume(W, 0.0f);
-                float zc_tau = -1.0f;
-                for (uint32_t k = 0; k <= 8u; ++k) {
-                    const float tk = float(k) * 0.125f;
-                    ST stk;
-                    const float dk = sweep_dist_sq(W, tk, stk);
-                    if (dk < min_sq) { min_sq = dk; best_tau = tk; min_st = stk; }
-                    if (k > 0u) {
-                        const float vol = sweep_volume(W, tk);
-                        if (prev_vol * vol <= 0.0f && zc_tau < 0.0f) zc_tau = tk - 0.0625f;
-                        prev_vol = vol;
+        }
+        if (zc_tau >= 0.0f) {
+            ST stz;
+            const float dzs = sweep_dist_sq(W, zc_tau, stz);
+            if (dzs < min_sq) { min_sq = dzs; best_tau = zc_tau; half_span = 0.0625f; min_st = stz; }
+        }
+        float tlo = fmaxf(0.0f, best_tau - half_span), thi = fminf(1.0f, best_tau + half_span);
+        float tau_star = best_tau, dsq_star = min_sq;
+        ST st_star = min_st;
+        for (int it = 0; it < 6; ++it) {
+            const float m1 = tlo + (thi - tlo) * (1.0f / 3.0f);
+            const float m2 = thi - (thi - tlo) * (1.0f / 3.0f);
+            ST s1, s2;
+            const float e1 = sweep_dist_sq(W, m1, s1);
+            const float e2 = sweep_dist_sq(W, m2, s2);
+            if (e1 < dsq_star) { dsq_star = e1; tau_star = m1; st_star = s1; }
+            if (e2 < dsq_star) { dsq_star = e2; tau_star = m2; st_star = s2; }
+            if (e1 < e2) thi = m2; else tlo = m1;
+        }
+        const float ss = st_star.s, ts = st_star.t;
+        const float eff_star = mixf(th_u0, th_u1, ss) + mixf(th_j, th_vj, ts);
+        if (dsq_star >= eff_star * eff_star) return;
+        if (A.exclude_neighbors) {
+            const f3 a0s = mix3(p_u0o, p_u0, tau_star), a1s = mix3(p_u1o, p_u1, tau_star);
+            const f3 b0s = mix3(p_jo, p_j, tau_star), b1s = mix3(p_vjo, p_vj, tau_star);
+            const f3 t0 = a0s - b0s, t1 = a0s - b1s, t2 = a1s - b0s, t3 = a1s - b1s;
+            if (near_folded_sq(T, u0, j, dot(t0, t0)) || near_folded_sq(T, u0, vj, dot(t1, t1)) ||
+                near_folded_sq(T, u1, j, dot(t2, t2)) || near_folded_sq(T, u1, vj, dot(t3, t3)))
+                return;
+        }
+        const ST st0 = closest_segs(p_u0o, p_u1o, p_jo, p_vjo);
+        const f3 delta0 = mix3(p_u0o, p_u1o, st0.s) - mix3(p_jo, p_vjo, st0.t);
+        const float dist0 = length(delta0);
+        if (dist0 > EPS) {
+            const f3 n0 = delta0 / dist0;
+            const ST st1 = closest_segs(p_u0, p_u1, p_j, p_vj);
+            const f3 delta1 = mix3(p_u0, p_u1, st1.s) - mix3(p_j, p_vj, st1.t);
+            const float dist1 = length(delta1);
+            const float h_end = dot(mix3(p_u0, p_u1, ss) - mix3(p_j, p_vj, ts), n0);
+            const bool crossover = (h_end <= 0.0f) || (dot(delta1, n0) <= 0.0f) ||
+                                   (bnd_pair && dist1 >= eff_star && tau_star < 1.0f);
+            if (h_end < eff_star && crossover) {
+                const float pen = eff_star - h_end;
+                const bool b_has_pin = wj <= 0.0f || wvj <= 0.0f;
+                if (a_has_pin && b_has_pin) return;
+                const bool a_all = wu0 == 0.0f && wu1 == 0.0f;
+                const bool b_all = wj == 0.0f && wvj == 0.0f;
+                const bool cnt_pin = !(a_all || b_all);
+                if (b_has_pin) {
+                    const f3 em = ((p_j - p_jo) + (p_vj - p_vjo)) * 0.5f;
+                    const float a = 1.0f - ss, b = ss;
+                    const float den = a * a * wu0 + b * b * wu1;
+                    if (den > EPS) {
+                        const f3 pb = n0 * pen + em * 0.5f;
+                        if (wu0 > 0.0f) add_ccd_ee(&acc[u0], pb * fminf((a * wu0) / den, 2.0f), cnt_pin);
+                        if (wu1 > 0.0f) add_ccd_ee(&acc[u1], pb * fminf((b * wu1) / den, 2.0f), cnt_pin);
+                    }
+                } else if (a_has_pin) {
+                    const f3 em = ((p_u0 - p_u0o) + (p_u1 - p_u1o)) * 0.5f;
+                    const float a = 1.0f - ts, b = ts;
+                    const float den = a * a * wj + b * b * wvj;
+                    if (den > EPS) {
+                        const f3 pb = (-n0) * pen + em * 0.5f;
+                        if (wj > 0.0f) add_ccd_ee(&acc[j], pb * fminf((a * wj) / den, 2.0f), cnt_pin);
+                        if (wvj > 0.0f) add_ccd_ee(&acc[vj], pb * fminf((b * wvj) / den, 2.0f), cnt_pin);
+                    }
+                } else {
+                    const float we1 = (1.0f - ss) * (1.0f - ss) * wu0 + ss * ss * wu1;
+                    const float we2 = (1.0f - ts) * (1.0f - ts) * wj + ts * ts * wvj;
+                    const float wt = we1 + we2;
+                    if (wt > EPS) {
+                        const f3 tot = n0 * pen;
+                        if (wu0 > 0.0f) add_ccd_ee(&acc[u0], tot * ((1.0f - ss) * wu0 / wt), true);
+                        if (wu1 > 0.0f) add_ccd_ee(&acc[u1], tot * (ss * wu1 / wt), true);
+                        if (wj > 0.0f) add_ccd_ee(&acc[j], (-tot) * ((1.0f - ts) * wj / wt), true);
+                        if (wvj > 0.0f) add_ccd_ee(&acc[vj], (-tot) * (ts * wvj / wt), true);
                     }
                 }
-                if (zc_tau >= 0.0f) {
-                    ST stz;
-                    const float dzs = sweep_dist_sq(W, zc_tau, stz);
-                    if (dzs < min_sq) { min_sq = dzs; best_tau = zc_tau; half_span = 0.0625f; min_st = stz; }
-                }
-                float tlo = fmaxf(0.0f, best_tau - half_span), thi = fminf(1.0f, best_tau + half_span);
-                float tau_star = best_tau, dsq_star = min_sq;
-                ST st_star = min_st;
-                for (int it = 0; it < 6; ++it) {
-                    const float m1 = tlo + (thi - tlo) * (1.0f / 3.0f);
-                    const float m2 = thi - (thi - tlo) * (1.0f / 3.0f);
-                    ST s1, s2;
-                    const float e1 = sweep_dist_sq(W, m1, s1);
-                    const float e2 = sweep_dist_sq(W, m2, s2);
-                    if (e1 < dsq_star) { dsq_star = e1; tau_star = m1; st_star = s1; }
-                    if (e2 < dsq_star) { dsq_star = e2; tau_star = m2; st_star = s2; }
-                    if (e1 < e2) thi = m2; else tlo = m1;
-                }
-                const float ss = st_star.s, ts = st_star.t;
-                const float eff_star = mixf(th_u0, th_u1, ss) + mixf(th_j, th_vj, ts);
-                if (dsq_star >= eff_star * eff_star) continue;
-                if (A.exclude_neighbors) {
-                    const f3 a0s = mix3(p_u0o, p_u0, tau_star), a1s = mix3(p_u1o, p_u1, tau_star);
-                    const f3 b0s = mix3(p_jo, p_j, tau_star), b1s = mix3(p_vjo, p_vj, tau_star);
-                    const f3 t0 = a0s - b0s, t1 = a0s - b1s, t2 = a1s - b0s, t3 = a1s - b1s;
-                    if (near_folded_sq(T, u0, j, dot(t0, t0)) || near_folded_sq(T, u0, vj, dot(t1, t1)) ||
-                        near_folded_sq(T, u1, j, dot(t2, t2)) || near_folded_sq(T, u1, vj, dot(t3, t3)))
-                        continue;
-                }
-                const ST st0 = closest_segs(p_u0o, p_u1o, p_jo, p_vjo);
-                const f3 delta0 = mix3(p_u0o, p_u1o, st0.s) - mix3(p_jo, p_vjo, st0.t);
-                const float dist0 = length(delta0);
-                if (dist0 > EPS) {
-                    const f3 n0 = delta0 / dist0;
-                    const ST st1 = closest_segs(p_u0, p_u1, p_j, p_vj);
-                    const f3 delta1 = mix3(p_u0, p_u1, st1.s) - mix3(p_j, p_vj, st1.t);
-                    const float dist1 = length(delta1);
-                    const float h_end = dot(mix3(p_u0, p_u1, ss) - mix3(p_j, p_vj, ts), n0);
-                    const bool crossover = (h_end <= 0.0f) || (dot(delta1, n0) <= 0.0f) ||
-                                           (bnd_pair && dist1 >= eff_star && tau_star < 1.0f);
-                    if (h_end < eff_star && crossover) {
-                        const float pen = eff_star - h_end;
-                        const bool b_has_pin = wj <= 0.0f || wvj <= 0.0f;
-                        if (a_has_pin && b_has_pin) continue;
-                        const bool a_all = wu0 == 0.0f && wu1 == 0.0f;
-                        const bool b_all = wj == 0.0f && wvj == 0.0f;
-                        const bool cnt_pin = !(a_all || b_all);
-                        if (b_has_pin) {
-                            const f3 em = ((p_j - p_jo) + (p_vj - p_vjo)) * 0.5f;
-                            const float a = 1.0f - ss, b = ss;
-                            const float den = a * a * wu0 + b * b * wu1;
-                            if (den > EPS) {
-                                const f3 pb = n0 * pen + em * 0.5f;
-                                if (wu0 > 0.0f) add_ccd_ee(&acc[u0], pb * fminf((a * wu0) / den, 2.0f), cnt_pin);
-                                if (wu1 > 0.0f) add_ccd_ee(&acc[u1], pb * fminf((b * wu1) / den, 2.0f), cnt_pin);
-                            }
-                        } else if (a_has_pin) {
-                            const f3 em = ((p_u0 - p_u0o) + (p_u1 - p_u1o)) * 0.5f;
-                            const float a = 1.0f - ts, b = ts;
-                            const float den = a * a * wj + b * b * wvj;
-                            if (den > EPS) {
-                                const f3 pb = (-n0) * pen + em * 0.5f;
-                                if (wj > 0.0f) add_ccd_ee(&acc[j], pb * fminf((a * wj) / den, 2.0f), cnt_pin);
-                                if (wvj > 0.0f) add_ccd_ee(&acc[vj], pb * fminf((b * wvj) / den, 2.0f), cnt_pin);
-                            }
-                        } else {
-                            const float we1 = (1.0f - ss) * (1.0f - ss) * wu0 + ss * ss * wu1;
-                            const float we2 = (1.0f - ts) * (1.0f - ts) * wj + ts * ts * wvj;
-                            const float wt = we1 + we2;
-                            if (wt > EPS) {
-                                const f3 tot = n0 * pen;
-                                if (wu0 > 0.0f) add_ccd_ee(&acc[u0], tot * ((1.0f - ss) * wu0 / wt), true);
-                                if (wu1 > 0.0f) add_ccd_ee(&acc[u1], tot * (ss * wu1 / wt), true);
-                                if (wj > 0.0f) add_ccd_ee(&acc[j], (-tot) * ((1.0f - ts) * wj / wt), true);
-                                if (wvj > 0.0f) add_ccd_ee(&acc[vj], (-tot) * (ts * wvj / wt), true);
-                            }
-                        }
-                        continue;
-                    }
-                }
-            }
-            // discrete proximity
-            ST st;
-            f3 delta_ee;
-            float dsq, eff;
-            if (has_cached) {
-                st = st1c; delta_ee = d1c; dsq = dsqc; eff = effc;
-            } else {
-                st = closest_segs(p_u0, p_u1, p_j, p_vj);
-                delta_ee = (p_u0 + (p_u1 - p_u0) * st.s) - (p_j + (p_vj - p_j) * st.t);
-                dsq = dot(delta_ee, delta_ee);
-                eff = mixf(th_u0, th_u1, st.s) + mixf(th_j, th_vj, st.t);
-            }
-            if (dsq >= eff * eff || dsq <= EPS * EPS) continue;
-            if (A.exclude_neighbors) {
-                const f3 d0 = p_u0 - p_j, d1 = p_u0 - p_vj, d2 = p_u1 - p_j, d3 = p_u1 - p_vj;
-                if (near_folded_sq(T, u0, j, dot(d0, d0)) || near_folded_sq(T, u0, vj, dot(d1, d1)) ||
-                    near_folded_sq(T, u1, j, dot(d2, d2)) || near_folded_sq(T, u1, vj, dot(d3, d3)))
-                    continue;
-            }
-            const float dist = sqrtf(dsq);
-            const f3 n = delta_ee / dist;
-            const float pen = eff - dist;
-            const float sp = st.s, tp = st.t;
-            const bool b_has_pin = wj <= 0.0f || wvj <= 0.0f;
-            if (a_has_pin && b_has_pin) continue;
-            if (b_has_pin) {
-                const f3 em = ((p_j - p_jo) + (p_vj - p_vjo)) * 0.5f;
-                const float a = 1.0f - sp, b = sp;
-                const float den = a * a * wu0 + b * b * wu1;
-                if (den > EPS) {
-                    const f3 pb = n * pen + em * 0.5f;
-                    if (wu0 > 0.0f) add_pin_disp(&acc[u0], pb * fminf((a * wu0) / den, 2.0f));
-                    if (wu1 > 0.0f) add_pin_disp(&acc[u1], pb * fminf((b * wu1) / den, 2.0f));
-                }
-            } else if (a_has_pin) {
-                const f3 em = ((p_u0 - p_u0o) + (p_u1 - p_u1o)) * 0.5f;
-                const float a = 1.0f - tp, b = tp;
-                const float den = a * a * wj + b * b * wvj;
-                if (den > EPS) {
-                    const f3 pb = (-n) * pen + em * 0.5f;
-                    if (wj > 0.0f) add_pin_disp(&acc[j], pb * fminf((a * wj) / den, 2.0f));
-                    if (wvj > 0.0f) add_pin_disp(&acc[vj], pb * fminf((b * wvj) / den, 2.0f));
-                }
-            } else {
-                const float we1 = (1.0f - sp) * (1.0f - sp) * wu0 + sp * sp * wu1;
-                const float we2 = (1.0f - tp) * (1.0f - tp) * wj + tp * tp * wvj;
-                const float wt = we1 + we2;
-                if (wt > EPS) {
-                    const f3 td = n * (pen * 0.6f);
-                    if (wu0 > 0.0f) add_disp_ee(&acc[u0], td * ((1.0f - sp) * wu0 / wt));
-                    if (wu1 > 0.0f) add_disp_ee(&acc[u1], td * (sp * wu1 / wt));
-                    if (wj > 0.0f) add_disp_ee(&acc[j], (-td) * ((1.0f - tp) * wj / wt));
-                    if (wvj > 0.0f) add_disp_ee(&acc[vj], (-td) * (tp * wvj / wt));
-                }
+                return;
             }
         }
     }
+    // discrete proximity
+    ST st;
+    f3 delta_ee;
+    float dsq, eff;
+    if (has_cached) {
+        st = st1c; delta_ee = d1c; dsq = dsqc; eff = effc;
+    } else {
+        st = closest_segs(p_u0, p_u1, p_j, p_vj);
+        delta_ee = (p_u0 + (p_u1 - p_u0) * st.s) - (p_j + (p_vj - p_j) * st.t);
+        dsq = dot(delta_ee, delta_ee);
+        eff = mixf(th_u0, th_u1, st.s) + mixf(th_j, th_vj, st.t);
+    }
+    if (dsq >= eff * eff || dsq <= EPS * EPS) return;
+    if (A.exclude_neighbors) {
+        const f3 d0 = p_u0 - p_j, d1 = p_u0 - p_vj, d2 = p_u1 - p_j, d3 = p_u1 - p_vj;
+        if (near_folded_sq(T, u0, j, dot(d0, d0)) || near_folded_sq(T, u0, vj, dot(d1, d1)) ||
+            near_folded_sq(T, u1, j, dot(d2, d2)) || near_folded_sq(T, u1, vj, dot(d3, d3)))
+            return;
+    }
+    const float dist = sqrtf(dsq);
+    const f3 n = delta_ee / dist;
+    const float pen = eff - dist;
+    const float sp = st.s, tp = st.t;
+    const bool b_has_pin = wj <= 0.0f || wvj <= 0.0f;
+    if (a_has_pin && b_has_pin) return;
+    if (b_has_pin) {
+        const f3 em = ((p_j - p_jo) + (p_vj - p_vjo)) * 0.5f;
+        const float a = 1.0f - sp, b = sp;
+        const float den = a * a * wu0 + b * b * wu1;
+        if (den > EPS) {
+            const f3 pb = n * pen + em * 0.5f;
+            if (wu0 > 0.0f) add_pin_disp(&acc[u0], pb * fminf((a * wu0) / den, 2.0f));
+            if (wu1 > 0.0f) add_pin_disp(&acc[u1], pb * fminf((b * wu1) / den, 2.0f));
+        }
+    } else if (a_has_pin) {
+        const f3 em = ((p_u0 - p_u0o) + (p_u1 - p_u1o)) * 0.5f;
+        const float a = 1.0f - tp, b = tp;
+        const float den = a * a * wj + b * b * wvj;
+        if (den > EPS) {
+            const f3 pb = (-n) * pen + em * 0.5f;
+            if (wj > 0.0f) add_pin_disp(&acc[j], pb * fminf((a * wj) / den, 2.0f));
+            if (wvj > 0.0f) add_pin_disp(&acc[vj], pb * fminf((b * wvj) / den, 2.0f));
+        }
+    } else {
+        const float we1 = (1.0f - sp) * (1.0f - sp) * wu0 + sp * sp * wu1;
+        const float we2 = (1.0f - tp) * (1.0f - tp) * wj + tp * tp * wvj;
+        const float wt = we1 + we2;
+        if (wt > EPS) {
+            const f3 td = n * (pen * 0.6f);
+            if (wu0 > 0.0f) add_disp_ee(&acc[u0], td * ((1.0f - sp) * wu0 / wt));
+            if (wu1 > 0.0f) add_disp_ee(&acc[u1], td * (sp * wu1 / wt));
+            if (wj > 0.0f) add_disp_ee(&acc[j], (-td) * ((1.0f - tp) * wj / wt));
+            if (wvj > 0.0f) add_disp_ee(&acc[vj], (-td) * (tp * wvj / wt));
+        }
+    }
+}
+
+__global__ void __launch_bounds__(SC_BLOCK) k_sc_solve_ee(const float4 *__restrict__ P, const float4 *__restrict__ X,
+                                                          SelfColTopo T, const uint32_t *__restrict__ starts,
+                                                          const uint32_t *__restrict__ sorted,
+                                                          Accum *__restrict__ acc, SelfColArgs A) {
+    __shared__ uint2 s_queue[SC_BLOCK / 64][SC_QCAP];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, ci = lane & 31u;
+    const uint32_t ea = (blockIdx.x * (SC_BLOCK / 64) + wv) * 2u + (lane >> 5);
+    uint32_t q = 0, qe = 0, u0 = 0, u1 = 0;
+    f3 amin = mk3(0.0f, 0.0f, 0.0f), amax = amin;
+    i3 nc = {0, 0, 0};
+    if (ea < A.ne && ci < 27u) {
+        const uint2 eA = T.edges[ea];
+        u0 = eA.x;
+        u1 = eA.y;
+        const float4 qu0 = P[u0], qu1 = P[u1], xu0 = X[u0], xu1 = X[u1];
+        const f3 p_u0 = xyz(qu0), p_u1 = xyz(qu1), p_u0o = xyz(xu0), p_u1o = xyz(xu1);
+        const float max_th_a = fmaxf(xu0.w, xu1.w);
+        const f3 mta = mk3(max_th_a * 2.0f, max_th_a * 2.0f, max_th_a * 2.0f);
+        amin = min3(min3(p_u0, p_u1), min3(p_u0o, p_u1o)) - mta;
+        amax = max3(max3(p_u0, p_u1), max3(p_u0o, p_u1o)) + mta;
+        nc = neighbour_cell(cell_of((p_u0 + p_u1) * 0.5f, A.cell), ci);
+        const uint32_t h = hash_cell(nc, A.table);
+        q = starts[h];
+        qe = starts[h + 1];
+    }
+    // candidate tests ahead of the reference's search counter
+    auto boxed = [&](uint32_t eb) -> bool {
+        if (ea >= eb) return false;
+        const uint2 eB = T.edges[eb];
+        const uint32_t j = eB.x, vj = eB.y;
+        if (j == u0 || j == u1 || vj == u0 || vj == u1) return false;
+        const float4 qj = P[j], qvj = P[vj];
+        const f3 p_j = xyz(qj), p_vj = xyz(qvj);
+        const i3 bcell = cell_of((p_j + p_vj) * 0.5f, A.cell);
+        if (!(bcell.x == nc.x && bcell.y == nc.y && bcell.z == nc.z)) return false;
+        const float4 xj = X[j], xvj = X[vj];
+        const f3 p_jo = xyz(xj), p_vjo = xyz(xvj);
+        const f3 bmin = min3(min3(p_j, p_vj), min3(p_jo, p_vjo)), bmax = max3(max3(p_j, p_vj), max3(p_jo, p_vjo));
+        return !(amin.x > bmax.x || amin.y > bmax.y || amin.z > bmax.z || amax.x < bmin.x || amax.y < bmin.y ||
+                 amax.z < bmin.z);
+    };
+    // The reference ends an edge's search once max_search_iterations candidates
+    // have passed the box test, counting through the cells in order.  When the 27
+    // buckets of an edge hold more entries than that, each lane first counts its
+    // cell's passes so that it knows how many the cells before it contribute.
+    const uint32_t cap = A.max_search_iterations;
+    uint32_t ord = 0;  // box-test passes of this edge before the current candidate
+    uint32_t n = qe - q;
+    for (uint32_t o = 16u; o != 0u; o >>= 1) n += __shfl_xor(n, int(o), 32);
+    if (__any(n > cap)) {
+        uint32_t cnt = 0;
+        for (uint32_t k = q; k < qe; ++k) cnt += boxed(sorted[k]) ? 1u : 0u;
+        uint32_t incl = cnt;
+        for (uint32_t o = 1u; o < 32u; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 32);
+            if (ci >= o) incl += y;
+        }
+        ord = incl - cnt;
+    }
+    auto accept = [&](uint32_t k) -> uint32_t {
+        const uint32_t eb = sorted[k];
+        if (!boxed(eb)) return SC_NONE;
+        return ++ord <= cap ? eb : SC_STOP;
+    };
+    sc_walk(s_queue[wv], lane, ea, q, qe, accept,
+            [&](uint32_t a, uint32_t b) { ee_pair(P, X, T, acc, A, a, b); });
 }
 
 // self_collision_apply.wgsl:57-177; clears the accumulator it reads.
@@ -720,18 +847,18 @@ __global__ void __launch_bounds__(256) k_sc_apply(float4 *__restrict__ P, const 
 }
 
 hipError_t launch_sc_solve(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t, const HashBufs &vh,
-                           Accum *acc, const SelfColArgs &a, uint32_t block) {
+                           Accum *acc, const SelfColArgs &a) {
     if (!a.nv) return hipSuccess;
-    hipLaunchKernelGGL(k_sc_solve, dim3(grid_for(a.nv, block)), dim3(block), 0, s, P, X, t, vh.starts, vh.sorted,
-                       acc, a);
+    hipLaunchKernelGGL(k_sc_solve, dim3(grid_for(a.nv, SC_BLOCK / 32)), dim3(SC_BLOCK), 0, s, P, X, t, vh.starts,
+                       vh.sorted, acc, a);
     return hipGetLastError();
 }
 
 hipError_t launch_sc_solve_ee(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t,
-                              const HashBufs &eh, Accum *acc, const SelfColArgs &a, uint32_t block) {
+                              const HashBufs &eh, Accum *acc, const SelfColArgs &a) {
     if (!a.ne) return hipSuccess;
-    hipLaunchKernelGGL(k_sc_solve_ee, dim3(grid_for(a.ne, block)), dim3(block), 0, s, P, X, t, eh.starts, eh.sorted,
-                       acc, a);
+    hipLaunchKernelGGL(k_sc_solve_ee, dim3(grid_for(a.ne, SC_BLOCK / 32)), dim3(SC_BLOCK), 0, s, P, X, t, eh.starts,
+                       eh.sorted, acc, a);
     return hipGetLastError();
 }
 

@@ -138,8 +138,8 @@ hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_coll
         if ((e = enqueue_hashes(s, Pc)) != hipSuccess) return e;
         const tc::SelfColArgs sa = s->sc_args();
         const tc::SelfColTopo t = s->topo();
-        if ((e = tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa, B)) != hipSuccess) return e;
-        if ((e = tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa, B)) != hipSuccess) return e;
+        if ((e = tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa)) != hipSuccess) return e;
+        if ((e = tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa)) != hipSuccess) return e;
         if ((e = tc::launch_sc_apply(s->stream, Pc, s->local_len.p, s->accum.p, sa, B)) != hipSuccess) return e;
     }
     if (update) return tc::launch_update_vel(s->stream, Pc, s->X.p, s->V.p, n, dt_sub, B);
@@ -401,8 +401,8 @@ static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
     UP(s->star, star.data(), star.size());
     UP(s->edges, edg.data(), edg.size());
     UP(s->edge_bbits, m.e_bbits.data(), m.e_bbits.size());
-    AL(s->vkeys, nv); AL(s->vsorted, nv); AL(s->vcounts, T); AL(s->vstarts, T + 1);
-    AL(s->ekeys, NE); AL(s->esorted, NE); AL(s->ecounts, T); AL(s->estarts, T + 1);
+    AL(s->vkeys, nv); AL(s->vsorted, nv); AL(s->vcounts, T + 64); AL(s->vstarts, T + 1);
+    AL(s->ekeys, NE); AL(s->esorted, NE); AL(s->ecounts, T + 64); AL(s->estarts, T + 1);
     AL(s->accum, nv);
     AL(s->readback, size_t(nv) * 3);
     AL(s->resid, 4);
@@ -622,8 +622,8 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
                      if (r != hipSuccess) return r;
                      const tc::SelfColArgs sa = s->sc_args();
                      const tc::SelfColTopo t = s->topo();
-                     if ((r = tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa, B)) != hipSuccess) return r;
-                     if ((r = tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa, B)) != hipSuccess) return r;
+                     if ((r = tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa)) != hipSuccess) return r;
+                     if ((r = tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa)) != hipSuccess) return r;
                      return tc::launch_sc_apply(s->stream, Pc, s->local_len.p, s->accum.p, sa, B);
                  })) != hipSuccess) return e;
         return mark(7, [&] { return tc::launch_update_vel(s->stream, Pc, s->X.p, s->V.p, n, dt_sub, B); });

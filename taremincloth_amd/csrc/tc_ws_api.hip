@@ -653,8 +653,8 @@ int tc_sim_ws_time(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
                                          s->table_size, s->ehash()));
             const tc::SelfColArgs sa = s->sc_args();
             const tc::SelfColTopo t = s->topo();
-            TC_HIP(tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa, B));
-            TC_HIP(tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa, B));
+            TC_HIP(tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa));
+            TC_HIP(tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa));
             TC_HIP(tc::launch_sc_apply(s->stream, Pc, s->local_len.p, s->accum.p, sa, B));
         }
         TC_HIP(hipEventRecord(ev[25], s->stream));

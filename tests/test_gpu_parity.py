@@ -186,6 +186,28 @@ def test_self_collision_folded_cloth_matches_oracle():
     assert st["vv"] + st["vt_prox"] + st["vt_ccd"] > 0 and st["ee_prox"] + st["ee_ccd"] > 0, st
 
 
+@pytest.mark.parametrize("cap,cell", [(2, 0.02), (3, 0.03), (256, 0.05)])
+def test_self_collision_search_cap_matches_oracle(cap, cell):
+    """The E-E search stops an edge after `cap` box-test passes, counted through the 27 cells in
+    order. Small caps cut most edges short; the large cells fill the buckets, so that wavefronts
+    queue more than 64 candidate pairs at once."""
+    _say(f"self collision search cap {cap}, cell {cell}")
+    pos, e, f, inv = cases.folded_cloth()
+    g = T.ClothSimulator(pos, e, f, inv, thickness=0.003, stiffness=1000.0, bending_stiffness=10.0)
+    o = O.OracleSim(pos, e, f, inv, thickness=0.003, stiffness=1000.0, bending_stiffness=10.0)
+    for s in (g, o):
+        s.set_gravity(0.0, 0.0, -9.81)
+        s.set_self_collision_options(1.0, 0.5, True, False, cap)
+        s.set_hash_cell_size(cell)
+    for step in range(3):
+        g.step(1.0 / 60.0, 5, 2)
+        o.step(1.0 / 60.0, 5, 2)
+        diff = float(np.abs(_pos(g) - o.get_positions().reshape(-1, 3)).max())
+        assert diff < TOL, f"frame {step}: {diff:.3e}"
+    st = o.contact_stats()
+    assert st["ee_prox"] + st["ee_ccd"] > 0, st
+
+
 def test_drape_on_primitives_with_self_collision_matches_oracle():
     """64x64 cloth at 4 mm dropping onto a sphere and a capsule with friction 0.3,
     self-collision on (a reduced version of the 256^2 configuration)."""
