@@ -266,15 +266,19 @@ __device__ __forceinline__ bool near_folded_sq(const SelfColTopo &t, uint32_t a,
     return !(cur_sq < (l0 * l0) * 0.25f);
 }
 
-// Work distribution of the two solve kernels.  A wavefront takes two elements
-// (vertices or edges); lanes 0-26 and 32-58 each walk the hash bucket of one of
-// the element's 27 neighbour cells, numbered in the order of the reference's
-// E-E loop (dz outer, dx inner: cell = 9 (dz+1) + 3 (dy+1) + (dx+1)).  The
-// candidates that pass the cheap tests go into a per-wavefront LDS queue, which
-// the wavefront works off 64 pairs at a time, one pair per lane, so that the
-// expensive pair tests (CCD sweeps, two-hop lookups) run with full lanes.  The
-// fixed-point accumulation makes the result independent of this order.
+// Work distribution of the two solve kernels.  A wavefront takes SC_GROUPS
+// groups of two elements (vertices or edges) one after the other; in each group,
+// lanes 0-26 and 32-58 walk the hash bucket of one of the element's 27 neighbour
+// cells, numbered in the order of the reference's E-E loop (dz outer, dx inner:
+// cell = 9 (dz+1) + 3 (dy+1) + (dx+1)).  The candidates that pass the cheap
+// tests go into a per-wavefront LDS queue, which the wavefront works off 64 pairs
+// at a time, one pair per lane, so that the expensive pair tests (CCD sweeps,
+// two-hop lookups) run with full lanes; the queue carries over from one group to
+// the next.  The fixed-point accumulation makes the result independent of this
+// order.
 constexpr uint32_t SC_BLOCK = 256;
+constexpr uint32_t SC_GROUPS = 4;
+constexpr uint32_t SC_PER_BLOCK = (SC_BLOCK / 64) * SC_GROUPS * 2;  // elements per workgroup
 constexpr uint32_t SC_QCAP = 128;  // queue entries per wavefront: < 64 left over + up to 64 pushed per step
 constexpr uint32_t SC_NONE = 0xFFFFFFFFu, SC_STOP = 0xFFFFFFFEu;
 
@@ -284,12 +288,13 @@ __device__ __forceinline__ i3 neighbour_cell(i3 c, uint32_t k) {
 
 // Lockstep walk of every lane's bucket range [q, qe).  accept(k) tests entry k of
 // the bucket list and returns the partner element, SC_NONE to skip it, or SC_STOP
-// to end this lane's walk; pair(self, partner) processes one queued pair.  Every
+// to end this lane's walk; pair(self, partner) processes one queued pair.  qn is
+// the number of queued pairs (the same in every lane); a full batch of 64 is
+// processed as soon as it is there, the rest only when `flush` is set.  Every
 // lane of the wavefront must call this (the ballots and the queue are shared).
 template <class Accept, class Pair>
-__device__ __forceinline__ void sc_walk(uint2 *queue, uint32_t lane, uint32_t self, uint32_t q, uint32_t qe,
-                                        Accept accept, Pair pair) {
-    uint32_t qn = 0;  // queued pairs (the same value in every lane)
+__device__ __forceinline__ void sc_walk(uint2 *queue, uint32_t &qn, bool flush, uint32_t lane, uint32_t self,
+                                        uint32_t q, uint32_t qe, Accept accept, Pair pair) {
     for (;;) {
         const bool more = __any(q < qe) != 0;
         bool push = false;
@@ -303,7 +308,7 @@ __device__ __forceinline__ void sc_walk(uint2 *queue, uint32_t lane, uint32_t se
         const uint64_t m = __ballot(push);
         if (push) queue[qn + uint32_t(__popcll(m & ((uint64_t(1) << lane) - 1u)))] = make_uint2(self, other);
         qn += uint32_t(__popcll(m));
-        if (qn >= 64u || (!more && qn != 0u)) {
+        if (qn >= 64u || (flush && !more && qn != 0u)) {
             const uint32_t take = min(qn, 64u), rest = qn - take;
             __builtin_amdgcn_wave_barrier();  // the pushes above come first
             uint2 pr = make_uint2(0u, 0u), mv = make_uint2(0u, 0u);
@@ -314,7 +319,7 @@ __device__ __forceinline__ void sc_walk(uint2 *queue, uint32_t lane, uint32_t se
             qn = rest;
             if (lane < take) pair(pr.x, pr.y);
         }
-        if (!more && qn == 0u) break;
+        if (!more && (qn == 0u || !flush)) break;
     }
 }
 
@@ -476,42 +481,45 @@ __global__ void __launch_bounds__(SC_BLOCK) k_sc_solve(const float4 *__restrict_
                                                        SelfColArgs A) {
     __shared__ uint2 s_queue[SC_BLOCK / 64][SC_QCAP];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, ci = lane & 31u;
-    const uint32_t index = (blockIdx.x * (SC_BLOCK / 64) + wv) * 2u + (lane >> 5);
     const float EPS = 1e-7f;
-    uint32_t q = 0, qe = 0;
-    f3 p_i = mk3(0.0f, 0.0f, 0.0f);
-    float thick_i = 0.0f, llen_i = 0.0f;
-    i3 nc = {0, 0, 0};
-    if (index < A.nv && ci < 27u) {
-        const float4 qi = P[index], xi = X[index];
-        p_i = xyz(qi);
-        const float mv_i = length(p_i - xyz(xi));
-        if (!(qi.w <= 0.0f && mv_i < EPS)) {  // a pinned vertex that did not move takes no part
-            thick_i = xi.w;
-            llen_i = T.local_len[index];
-            nc = neighbour_cell(cell_of(p_i, A.cell), ci);
-            const uint32_t h = hash_cell(nc, A.table);
-            q = starts[h];
-            qe = starts[h + 1];
+    uint32_t qn = 0;
+    for (uint32_t g = 0; g < SC_GROUPS; ++g) {
+        const uint32_t index = ((blockIdx.x * (SC_BLOCK / 64) + wv) * SC_GROUPS + g) * 2u + (lane >> 5);
+        uint32_t q = 0, qe = 0;
+        f3 p_i = mk3(0.0f, 0.0f, 0.0f);
+        float thick_i = 0.0f, llen_i = 0.0f;
+        i3 nc = {0, 0, 0};
+        if (index < A.nv && ci < 27u) {
+            const float4 qi = P[index], xi = X[index];
+            p_i = xyz(qi);
+            const float mv_i = length(p_i - xyz(xi));
+            if (!(qi.w <= 0.0f && mv_i < EPS)) {  // a pinned vertex that did not move takes no part
+                thick_i = xi.w;
+                llen_i = T.local_len[index];
+                nc = neighbour_cell(cell_of(p_i, A.cell), ci);
+                const uint32_t h = hash_cell(nc, A.table);
+                q = starts[h];
+                qe = starts[h + 1];
+            }
         }
+        auto accept = [&](uint32_t k) -> uint32_t {
+            const uint32_t j = sorted[k];
+            if (j >= A.nv || j == index) return SC_NONE;
+            const float4 qj = P[j];
+            const f3 p_j = xyz(qj);
+            const i3 jc = cell_of(p_j, A.cell);
+            if (!(jc.x == nc.x && jc.y == nc.y && jc.z == nc.z)) return SC_NONE;
+            const float4 xj = X[j];
+            const f3 dvv = p_i - p_j;
+            const float dsq = dot(dvv, dvv);
+            const float thick_j = xj.w, llen_j = T.local_len[j];
+            const float eff = fmaxf(thick_i + thick_j, fminf(llen_i, llen_j) * 0.06f);
+            const float max_geom = eff + (llen_i + llen_j) * 1.3f;
+            return dsq <= max_geom * max_geom ? j : SC_NONE;
+        };
+        sc_walk(s_queue[wv], qn, g + 1u == SC_GROUPS, lane, index, q, qe, accept,
+                [&](uint32_t i, uint32_t j) { vv_pair(P, X, T, acc, A, i, j); });
     }
-    auto accept = [&](uint32_t k) -> uint32_t {
-        const uint32_t j = sorted[k];
-        if (j >= A.nv || j == index) return SC_NONE;
-        const float4 qj = P[j];
-        const f3 p_j = xyz(qj);
-        const i3 jc = cell_of(p_j, A.cell);
-        if (!(jc.x == nc.x && jc.y == nc.y && jc.z == nc.z)) return SC_NONE;
-        const float4 xj = X[j];
-        const f3 dvv = p_i - p_j;
-        const float dsq = dot(dvv, dvv);
-        const float thick_j = xj.w, llen_j = T.local_len[j];
-        const float eff = fmaxf(thick_i + thick_j, fminf(llen_i, llen_j) * 0.06f);
-        const float max_geom = eff + (llen_i + llen_j) * 1.3f;
-        return dsq <= max_geom * max_geom ? j : SC_NONE;
-    };
-    sc_walk(s_queue[wv], lane, index, q, qe, accept,
-            [&](uint32_t i, uint32_t j) { vv_pair(P, X, T, acc, A, i, j); });
 }
 
 // ---------------------------------------------------------------------------
@@ -596,6 +604,7 @@ __device__ __forceinline__ void ee_pair(const float4 *__restrict__ P, const floa
         ST min_st = {0.0f, 0.0f};
         float prev_vol = sweep_volume(W, 0.0f);
         float zc_tau = -1.0f;
+#pragma unroll 1  // unrolled, the sweeps need 215 VGPRs (2 wavefronts per SIMD)
         for (uint32_t k = 0; k <= 8u; ++k) {
             const float tk = float(k) * 0.125f;
             ST stk;
@@ -615,6 +624,7 @@ __device__ __forceinline__ void ee_pair(const float4 *__restrict__ P, const floa
         float tlo = fmaxf(0.0f, best_tau - half_span), thi = fminf(1.0f, best_tau + half_span);
         float tau_star = best_tau, dsq_star = min_sq;
         ST st_star = min_st;
+#pragma unroll 1
         for (int it = 0; it < 6; ++it) {
             const float m1 = tlo + (thi - tlo) * (1.0f / 3.0f);
             const float m2 = thi - (thi - tlo) * (1.0f / 3.0f);
@@ -745,72 +755,76 @@ __device__ __forceinline__ void ee_pair(const float4 *__restrict__ P, const floa
     }
 }
 
-__global__ void __launch_bounds__(SC_BLOCK) k_sc_solve_ee(const float4 *__restrict__ P, const float4 *__restrict__ X,
+__global__ void __launch_bounds__(SC_BLOCK, 4) k_sc_solve_ee(const float4 *__restrict__ P, const float4 *__restrict__ X,
                                                           SelfColTopo T, const uint32_t *__restrict__ starts,
                                                           const uint32_t *__restrict__ sorted,
                                                           Accum *__restrict__ acc, SelfColArgs A) {
     __shared__ uint2 s_queue[SC_BLOCK / 64][SC_QCAP];
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, ci = lane & 31u;
-    const uint32_t ea = (blockIdx.x * (SC_BLOCK / 64) + wv) * 2u + (lane >> 5);
-    uint32_t q = 0, qe = 0, u0 = 0, u1 = 0;
-    f3 amin = mk3(0.0f, 0.0f, 0.0f), amax = amin;
-    i3 nc = {0, 0, 0};
-    if (ea < A.ne && ci < 27u) {
-        const uint2 eA = T.edges[ea];
-        u0 = eA.x;
-        u1 = eA.y;
-        const float4 qu0 = P[u0], qu1 = P[u1], xu0 = X[u0], xu1 = X[u1];
-        const f3 p_u0 = xyz(qu0), p_u1 = xyz(qu1), p_u0o = xyz(xu0), p_u1o = xyz(xu1);
-        const float max_th_a = fmaxf(xu0.w, xu1.w);
-        const f3 mta = mk3(max_th_a * 2.0f, max_th_a * 2.0f, max_th_a * 2.0f);
-        amin = min3(min3(p_u0, p_u1), min3(p_u0o, p_u1o)) - mta;
-        amax = max3(max3(p_u0, p_u1), max3(p_u0o, p_u1o)) + mta;
-        nc = neighbour_cell(cell_of((p_u0 + p_u1) * 0.5f, A.cell), ci);
-        const uint32_t h = hash_cell(nc, A.table);
-        q = starts[h];
-        qe = starts[h + 1];
-    }
-    // candidate tests ahead of the reference's search counter
-    auto boxed = [&](uint32_t eb) -> bool {
-        if (ea >= eb) return false;
-        const uint2 eB = T.edges[eb];
-        const uint32_t j = eB.x, vj = eB.y;
-        if (j == u0 || j == u1 || vj == u0 || vj == u1) return false;
-        const float4 qj = P[j], qvj = P[vj];
-        const f3 p_j = xyz(qj), p_vj = xyz(qvj);
-        const i3 bcell = cell_of((p_j + p_vj) * 0.5f, A.cell);
-        if (!(bcell.x == nc.x && bcell.y == nc.y && bcell.z == nc.z)) return false;
-        const float4 xj = X[j], xvj = X[vj];
-        const f3 p_jo = xyz(xj), p_vjo = xyz(xvj);
-        const f3 bmin = min3(min3(p_j, p_vj), min3(p_jo, p_vjo)), bmax = max3(max3(p_j, p_vj), max3(p_jo, p_vjo));
-        return !(amin.x > bmax.x || amin.y > bmax.y || amin.z > bmax.z || amax.x < bmin.x || amax.y < bmin.y ||
-                 amax.z < bmin.z);
-    };
-    // The reference ends an edge's search once max_search_iterations candidates
-    // have passed the box test, counting through the cells in order.  When the 27
-    // buckets of an edge hold more entries than that, each lane first counts its
-    // cell's passes so that it knows how many the cells before it contribute.
     const uint32_t cap = A.max_search_iterations;
-    uint32_t ord = 0;  // box-test passes of this edge before the current candidate
-    uint32_t n = qe - q;
-    for (uint32_t o = 16u; o != 0u; o >>= 1) n += __shfl_xor(n, int(o), 32);
-    if (__any(n > cap)) {
-        uint32_t cnt = 0;
-        for (uint32_t k = q; k < qe; ++k) cnt += boxed(sorted[k]) ? 1u : 0u;
-        uint32_t incl = cnt;
-        for (uint32_t o = 1u; o < 32u; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o, 32);
-            if (ci >= o) incl += y;
+    uint32_t qn = 0;
+    for (uint32_t g = 0; g < SC_GROUPS; ++g) {
+        const uint32_t ea = ((blockIdx.x * (SC_BLOCK / 64) + wv) * SC_GROUPS + g) * 2u + (lane >> 5);
+        uint32_t q = 0, qe = 0, u0 = 0, u1 = 0;
+        f3 amin = mk3(0.0f, 0.0f, 0.0f), amax = amin;
+        i3 nc = {0, 0, 0};
+        if (ea < A.ne && ci < 27u) {
+            const uint2 eA = T.edges[ea];
+            u0 = eA.x;
+            u1 = eA.y;
+            const float4 qu0 = P[u0], qu1 = P[u1], xu0 = X[u0], xu1 = X[u1];
+            const f3 p_u0 = xyz(qu0), p_u1 = xyz(qu1), p_u0o = xyz(xu0), p_u1o = xyz(xu1);
+            const float max_th_a = fmaxf(xu0.w, xu1.w);
+            const f3 mta = mk3(max_th_a * 2.0f, max_th_a * 2.0f, max_th_a * 2.0f);
+            amin = min3(min3(p_u0, p_u1), min3(p_u0o, p_u1o)) - mta;
+            amax = max3(max3(p_u0, p_u1), max3(p_u0o, p_u1o)) + mta;
+            nc = neighbour_cell(cell_of((p_u0 + p_u1) * 0.5f, A.cell), ci);
+            const uint32_t h = hash_cell(nc, A.table);
+            q = starts[h];
+            qe = starts[h + 1];
         }
-        ord = incl - cnt;
+        // candidate tests ahead of the reference's search counter
+        auto boxed = [&](uint32_t eb) -> bool {
+            if (ea >= eb) return false;
+            const uint2 eB = T.edges[eb];
+            const uint32_t j = eB.x, vj = eB.y;
+            if (j == u0 || j == u1 || vj == u0 || vj == u1) return false;
+            const float4 qj = P[j], qvj = P[vj];
+            const f3 p_j = xyz(qj), p_vj = xyz(qvj);
+            const i3 bcell = cell_of((p_j + p_vj) * 0.5f, A.cell);
+            if (!(bcell.x == nc.x && bcell.y == nc.y && bcell.z == nc.z)) return false;
+            const float4 xj = X[j], xvj = X[vj];
+            const f3 p_jo = xyz(xj), p_vjo = xyz(xvj);
+            const f3 bmin = min3(min3(p_j, p_vj), min3(p_jo, p_vjo));
+            const f3 bmax = max3(max3(p_j, p_vj), max3(p_jo, p_vjo));
+            return !(amin.x > bmax.x || amin.y > bmax.y || amin.z > bmax.z || amax.x < bmin.x ||
+                     amax.y < bmin.y || amax.z < bmin.z);
+        };
+        // The reference ends an edge's search once max_search_iterations candidates
+        // have passed the box test, counting through the cells in order.  When the 27
+        // buckets of an edge hold more entries than that, each lane first counts its
+        // cell's passes so that it knows how many the cells before it contribute.
+        uint32_t ord = 0;  // box-test passes of this edge before the current candidate
+        uint32_t n = qe - q;
+        for (uint32_t o = 16u; o != 0u; o >>= 1) n += __shfl_xor(n, int(o), 32);
+        if (__any(n > cap)) {
+            uint32_t cnt = 0;
+            for (uint32_t k = q; k < qe; ++k) cnt += boxed(sorted[k]) ? 1u : 0u;
+            uint32_t incl = cnt;
+            for (uint32_t o = 1u; o < 32u; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o, 32);
+                if (ci >= o) incl += y;
+            }
+            ord = incl - cnt;
+        }
+        auto accept = [&](uint32_t k) -> uint32_t {
+            const uint32_t eb = sorted[k];
+            if (!boxed(eb)) return SC_NONE;
+            return ++ord <= cap ? eb : SC_STOP;
+        };
+        sc_walk(s_queue[wv], qn, g + 1u == SC_GROUPS, lane, ea, q, qe, accept,
+                [&](uint32_t a, uint32_t b) { ee_pair(P, X, T, acc, A, a, b); });
     }
-    auto accept = [&](uint32_t k) -> uint32_t {
-        const uint32_t eb = sorted[k];
-        if (!boxed(eb)) return SC_NONE;
-        return ++ord <= cap ? eb : SC_STOP;
-    };
-    sc_walk(s_queue[wv], lane, ea, q, qe, accept,
-            [&](uint32_t a, uint32_t b) { ee_pair(P, X, T, acc, A, a, b); });
 }
 
 // self_collision_apply.wgsl:57-177; clears the accumulator it reads.
@@ -849,7 +863,7 @@ __global__ void __launch_bounds__(256) k_sc_apply(float4 *__restrict__ P, const 
 hipError_t launch_sc_solve(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t, const HashBufs &vh,
                            Accum *acc, const SelfColArgs &a) {
     if (!a.nv) return hipSuccess;
-    hipLaunchKernelGGL(k_sc_solve, dim3(grid_for(a.nv, SC_BLOCK / 32)), dim3(SC_BLOCK), 0, s, P, X, t, vh.starts,
+    hipLaunchKernelGGL(k_sc_solve, dim3(grid_for(a.nv, SC_PER_BLOCK)), dim3(SC_BLOCK), 0, s, P, X, t, vh.starts,
                        vh.sorted, acc, a);
     return hipGetLastError();
 }
@@ -857,7 +871,7 @@ hipError_t launch_sc_solve(hipStream_t s, const float4 *P, const float4 *X, cons
 hipError_t launch_sc_solve_ee(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t,
                               const HashBufs &eh, Accum *acc, const SelfColArgs &a) {
     if (!a.ne) return hipSuccess;
-    hipLaunchKernelGGL(k_sc_solve_ee, dim3(grid_for(a.ne, SC_BLOCK / 32)), dim3(SC_BLOCK), 0, s, P, X, t, eh.starts,
+    hipLaunchKernelGGL(k_sc_solve_ee, dim3(grid_for(a.ne, SC_PER_BLOCK)), dim3(SC_BLOCK), 0, s, P, X, t, eh.starts,
                        eh.sorted, acc, a);
     return hipGetLastError();
 }
