@@ -75,7 +75,7 @@ void fused_alphas(FusedArgs &fa, float dt_sub);
 // --- spatial hash + self collision (tc_selfcol.hip) ---
 struct HashBufs {
     uint32_t *keys;     // n
-    uint32_t *counts;   // table + 64: per-bucket counts, then scatter cursors; the tail holds the scan's row totals
+    uint32_t *counts;   // table + table/1024: bucket counts, then scatter cursors; the tail holds the scan's row totals
     uint32_t *starts;   // table + 1
     uint32_t *sorted;   // n
 };

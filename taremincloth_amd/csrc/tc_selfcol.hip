@@ -8,8 +8,8 @@
 //   apply             crates/cloth_core/src/shaders/self_collision_apply.wgsl:57-177
 //
 // Hash build: count (atomics) -> exclusive scan of the table (one workgroup per
-// row of 1024 entries, then the row totals) -> scatter (atomic cursor) ->
-// per-bucket sort, so every bucket lists its elements in ascending index order.  The
+// row of 1024 entries, one over the row totals, one add) -> scatter (atomic
+// cursor) -> per-bucket sort, so every bucket lists its elements in ascending index order.  The
 // reference's scatter order inside a bucket depends on atomic timing; the sorted
 // order makes the hash tables, and with them the E-E search cut-off,
 // deterministic.
@@ -45,23 +45,29 @@ __global__ void __launch_bounds__(256) k_hash_count_edges(const float4 *__restri
 }
 
 // Exclusive scan of counts[0..table) into starts[0..table], starts[table] = total,
-// in two launches over rows of 1024 entries (at most SCAN_ROWS rows), one
-// workgroup per row, coalesced throughout.  k_scan_rows scans each row (wave64
-// prefix sums, then the 16 wavefront totals), writes the in-row exclusive sums,
+// in three launches over rows of 1024 entries (at most SCAN_ROWS rows), coalesced
+// throughout.  k_scan_rows (one workgroup per row) scans each row with wave64
+// prefix sums and the 16 wavefront totals, writes the in-row exclusive sums,
 // keeps the row total in counts[table + row] and resets counts[0..table) to zero
-// for use as scatter cursors; k_scan_add adds the totals of the rows in front.
-constexpr uint32_t SCAN_ROWS = 64;
+// for use as scatter cursors; k_scan_totals (one workgroup) turns the row totals
+// into their exclusive prefix and writes starts[table]; k_scan_add adds each
+// row's prefix.
+constexpr uint32_t SCAN_ROWS = 4096;
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
 __global__ void __launch_bounds__(1024) k_scan_rows(uint32_t *__restrict__ counts, uint32_t *__restrict__ starts,
                                                     uint32_t table) {
     __shared__ uint32_t wave_tot[16];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t k = (blockIdx.x << 10) + tid;
     const uint32_t c = k < table ? counts[k] : 0u;
-    uint32_t incl = c;
-    for (uint32_t o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
-    }
+    const uint32_t incl = wave_incl_scan(c, lane);
     if (lane == 63u) wave_tot[wave] = incl;
     __syncthreads();
     uint32_t base = 0;
@@ -73,14 +79,38 @@ __global__ void __launch_bounds__(1024) k_scan_rows(uint32_t *__restrict__ count
     if (tid == 1023u) counts[table + blockIdx.x] = base + incl;
 }
 
+// rows <= SCAN_ROWS = 4 x 1024: thread t owns rows 4t .. 4t+3
+__global__ void __launch_bounds__(1024) k_scan_totals(uint32_t *__restrict__ counts, uint32_t *__restrict__ starts,
+                                                      uint32_t table, uint32_t rows) {
+    __shared__ uint32_t wave_tot[16];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint32_t *tot = counts + table;
+    uint32_t v[4], sum = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t r = 4u * tid + u;
+        v[u] = r < rows ? tot[r] : 0u;
+        sum += v[u];
+    }
+    const uint32_t incl = wave_incl_scan(sum, lane);
+    if (lane == 63u) wave_tot[wave] = incl;
+    __syncthreads();
+    uint32_t run = 0;
+    for (uint32_t w = 0; w < wave; ++w) run += wave_tot[w];
+    run += incl - sum;
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t r = 4u * tid + u;
+        if (r < rows) tot[r] = run;
+        run += v[u];
+    }
+    if (tid == 1023u) starts[table] = run;
+}
+
 __global__ void __launch_bounds__(1024) k_scan_add(const uint32_t *__restrict__ counts, uint32_t *__restrict__ starts,
                                                    uint32_t table) {
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, row = blockIdx.x;
-    uint32_t base = lane < row ? counts[table + lane] : 0u;  // row < SCAN_ROWS = 64 lanes
-    for (uint32_t o = 32; o != 0u; o >>= 1) base += __shfl_xor(base, int(o), 64);
-    const uint32_t k = (row << 10) + tid;
-    if (k < table) starts[k] += base;
-    if (row == gridDim.x - 1u && tid == 0u) starts[table] = base + counts[table + row];
+    const uint32_t k = (blockIdx.x << 10) + threadIdx.x;
+    if (k < table) starts[k] += counts[table + blockIdx.x];
 }
 
 __global__ void __launch_bounds__(256) k_hash_scatter(const uint32_t *__restrict__ keys, uint32_t n,
@@ -109,6 +139,7 @@ static hipError_t finish_hash(hipStream_t s, uint32_t n, uint32_t table, const H
     if (table == 0u || table > SCAN_ROWS * 1024u) return hipErrorInvalidValue;
     const uint32_t rows = (table + 1023u) >> 10;
     hipLaunchKernelGGL(k_scan_rows, dim3(rows), dim3(1024), 0, s, h.counts, h.starts, table);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, s, h.counts, h.starts, table, rows);
     hipLaunchKernelGGL(k_scan_add, dim3(rows), dim3(1024), 0, s, h.counts, h.starts, table);
     hipLaunchKernelGGL(k_hash_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, h.keys, n, h.starts, h.counts,
                        h.sorted);
@@ -252,12 +283,6 @@ __device__ __forceinline__ int32_t two_hop_slot(const SelfColTopo &t, uint32_t a
 __device__ __forceinline__ bool within_two_hops(const SelfColTopo &t, uint32_t a, uint32_t b) {
     return a == b || two_hop_slot(t, a, b) >= 0;
 }
-__device__ __forceinline__ bool near_folded(const SelfColTopo &t, uint32_t a, uint32_t b, float cur) {
-    if (a == b) return true;
-    const int32_t s = two_hop_slot(t, a, b);
-    if (s < 0) return false;
-    return !(cur < t.th_rest[s] * 0.5f);
-}
 __device__ __forceinline__ bool near_folded_sq(const SelfColTopo &t, uint32_t a, uint32_t b, float cur_sq) {
     if (a == b) return true;
     const int32_t s = two_hop_slot(t, a, b);
@@ -343,7 +368,9 @@ __device__ __forceinline__ void vv_pair(const float4 *__restrict__ P, const floa
     const float thick_j = xj.w, llen_j = T.local_len[j];
     const float eff = fmaxf(thick_i + thick_j, fminf(llen_i, llen_j) * 0.06f);
     const float dvv_len = sqrtf(fmaxf(dsq, 0.0f));
-    if (near_folded(T, index, j, dvv_len)) return;
+    // one lookup serves both tests on (index, j): near_folded and within_two_hops
+    const int32_t s_ij = two_hop_slot(T, index, j);
+    if (s_ij >= 0 && !(dvv_len < T.th_rest[s_ij] * 0.5f)) return;
     const f3 pj_old = xyz(xj);
     const float wi = qi.w, wj = qj.w;
     const bool both_dynamic = wj > 0.0f;
@@ -368,7 +395,7 @@ __device__ __forceinline__ void vv_pair(const float4 *__restrict__ P, const floa
     // 2. vertex-triangle over the star of j
     const float mv_j = length(p_j - pj_old);
     const float vt_bound = eff + llen_j * 1.3f + mv_i + mv_j;
-    if (!(dsq <= vt_bound * vt_bound) || within_two_hops(T, index, j)) return;
+    if (!(dsq <= vt_bound * vt_bound) || s_ij >= 0) return;
     for (uint32_t st = T.star_off[j]; st < T.star_off[j + 1]; ++st) {
         const uint2 pr = T.star[st];
         const uint32_t v0 = pr.x, v1 = pr.y;

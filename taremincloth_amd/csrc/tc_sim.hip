@@ -344,6 +344,20 @@ int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, ui
 }
 
 // Device state of a simulator whose mesh is built; takes ownership of s.
+// Bucket count of the device hash tables.  The reference caps its tables at
+// 65,536 buckets (crates/cloth_core/src/spatial_hash.rs:48); above that many
+// elements most buckets hold several cells, and every neighbour-cell walk reads
+// entries of cells that it then rejects.  Buckets list their elements in index
+// order, so the candidates of a cell, and their order, do not depend on the
+// bucket count: a larger table changes no result, only the number of rejected
+// reads.  Four buckets per element, at most 2^22 (32 MiB for both tables).
+static uint32_t device_table_size(const tc::Mesh &m) {
+    const uint64_t n = std::max<uint64_t>(uint64_t(m.nv) + m.n_virtual, m.e_v0.size());
+    uint32_t ts = std::max<uint32_t>(m.table_size, 1024u);
+    while (ts < 4 * n && ts < (1u << 22)) ts <<= 1;
+    return ts;
+}
+
 static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
     const uint32_t nv = s->mesh.nv;
     if (pp.workgroup_size == 64 || pp.workgroup_size == 128 || pp.workgroup_size == 256) s->block = pp.workgroup_size;
@@ -357,7 +371,7 @@ static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
         if (v == 256 || v == 512 || v == 1024) s->fused_block = uint32_t(v);
     }
     s->cell_size = s->mesh.cell_size;
-    s->table_size = s->mesh.table_size;
+    s->table_size = device_table_size(s->mesh);
 
     auto bail = [&](int code) { delete s; return code; };
     hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -401,8 +415,8 @@ static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
     UP(s->star, star.data(), star.size());
     UP(s->edges, edg.data(), edg.size());
     UP(s->edge_bbits, m.e_bbits.data(), m.e_bbits.size());
-    AL(s->vkeys, nv); AL(s->vsorted, nv); AL(s->vcounts, T + 64); AL(s->vstarts, T + 1);
-    AL(s->ekeys, NE); AL(s->esorted, NE); AL(s->ecounts, T + 64); AL(s->estarts, T + 1);
+    AL(s->vkeys, nv); AL(s->vsorted, nv); AL(s->vcounts, T + (T + 1023) / 1024); AL(s->vstarts, T + 1);
+    AL(s->ekeys, NE); AL(s->esorted, NE); AL(s->ecounts, T + (T + 1023) / 1024); AL(s->estarts, T + 1);
     AL(s->accum, nv);
     AL(s->readback, size_t(nv) * 3);
     AL(s->resid, 4);

@@ -118,7 +118,7 @@ def _rand_points(n, seed, scale=0.3):
     return (rng.random((n, 3)) * scale - scale / 2).astype(np.float32)
 
 
-@pytest.mark.parametrize("n,seed", [(500, 1), (4096, 2)])
+@pytest.mark.parametrize("n,seed", [(500, 1), (4096, 2), (40000, 3)])
 def test_hash_tables_bit_exact(n, seed):
     """Vertex and edge hash keys, cell starts and sorted indices are bit-identical to the oracle."""
     _say(f"hash tables n={n}")
