@@ -39,9 +39,11 @@ struct SdfArgs {
     const float *xf = nullptr;
     uint32_t nb = 0;
 };
-hipError_t launch_collision(hipStream_t s, float4 *P, float4 *X, uint32_t n, const Collider *cols, uint32_t ncols,
-                            const MeshTri *tris, const float4 *bounds, uint32_t ntris, const float4 *cbox, int single_sided_recovery,
-                            float dt, uint32_t block, const SdfArgs &sdf);
+// order: the vertices in the order the lanes take them (a permutation of [0, n), or null for 0..n-1)
+hipError_t launch_collision(hipStream_t s, float4 *P, float4 *X, uint32_t n, const uint32_t *order,
+                            const Collider *cols, uint32_t ncols, const MeshTri *tris, const float4 *bounds,
+                            uint32_t ntris, const float4 *cbox, int single_sided_recovery, float dt, uint32_t block,
+                            const SdfArgs &sdf);
 hipError_t launch_update_vel(hipStream_t s, const float4 *P, float4 *X, float4 *V, uint32_t n, float dt,
                              uint32_t block);
 hipError_t launch_extract_positions(hipStream_t s, const float4 *X, float *out, uint32_t n);

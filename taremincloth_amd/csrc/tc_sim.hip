@@ -130,7 +130,7 @@ hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_coll
     s->last_lam = lc;
     if (track && (e = ws_post(s, dt_sub, Pc)) != hipSuccess) return e;
     if (collide && (!s->cols_h.empty() || !s->tris_h.empty() || s->sdf_nb > 0)) {
-        e = tc::launch_collision(s->stream, Pc, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
+        e = tc::launch_collision(s->stream, Pc, s->X.p, n, s->col_order.p, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
                                  s->tri_bounds.p, uint32_t(s->tris_h.size()), s->tri_cbox.p, s->single_sided_recovery, dt_sub, B, s->sdf_args());
         if (e != hipSuccess) return e;
     }
@@ -358,6 +358,43 @@ static uint32_t device_table_size(const tc::Mesh &m) {
     return ts;
 }
 
+// Vertices in the Morton order of their rest positions (10 bits per axis over the bounding
+// cube, ties by index).  The collider kernel takes its vertices in this order, so that the
+// lanes of a wavefront lie close together on the cloth and need the same triangle clusters.
+// It changes no result: each vertex's collision pass reads and writes only that vertex.
+static std::vector<uint32_t> vertex_morton_order(const std::vector<float> &pos, uint32_t nv) {
+    std::vector<uint32_t> idx(nv);
+    for (uint32_t i = 0; i < nv; ++i) idx[i] = i;
+    if (nv == 0) return idx;
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (uint32_t i = 0; i < nv; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], pos[3 * size_t(i) + a]);
+            hi[a] = std::max(hi[a], pos[3 * size_t(i) + a]);
+        }
+    auto spread = [](uint32_t v) {
+        v = (v * 0x00010001u) & 0xFF0000FFu;
+        v = (v * 0x00000101u) & 0x0F00F00Fu;
+        v = (v * 0x00000011u) & 0xC30C30C3u;
+        v = (v * 0x00000005u) & 0x49249249u;
+        return v;
+    };
+    // one scale for all axes: a flat cloth's jitter along its thin axis must not fill that
+    // axis's 10 bits and scatter neighbours far apart in the order
+    const float ext = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), std::max(hi[2] - lo[2], 1e-9f));
+    std::vector<uint32_t> code(nv);
+    for (uint32_t i = 0; i < nv; ++i) {
+        uint32_t c[3];
+        for (int a = 0; a < 3; ++a) {
+            const float u = std::min(std::max((pos[3 * size_t(i) + a] - lo[a]) / ext, 0.0f), 1.0f);
+            c[a] = uint32_t(u * 1023.0f);
+        }
+        code[i] = (spread(c[0]) << 2) | (spread(c[1]) << 1) | spread(c[2]);
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
+    return idx;
+}
+
 static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
     const uint32_t nv = s->mesh.nv;
     if (pp.workgroup_size == 64 || pp.workgroup_size == 128 || pp.workgroup_size == 256) s->block = pp.workgroup_size;
@@ -389,6 +426,7 @@ static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
     std::vector<uint2> star(m.star_v0.size()), edg(m.e_v0.size());
     for (size_t k = 0; k < star.size(); ++k) star[k] = make_uint2(m.star_v0[k], m.star_v1[k]);
     for (size_t k = 0; k < edg.size(); ++k) edg[k] = make_uint2(m.e_v0[k], m.e_v1[k]);
+    const std::vector<uint32_t> col_order = vertex_morton_order(m.pos0, nv);
     const size_t T = s->table_size, NE = m.e_v0.size();
 #define UP(buf, src, cnt)                                                                        \
     if ((he = (buf).upload((src), (cnt), s->stream)) != hipSuccess)                              \
@@ -415,6 +453,7 @@ static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
     UP(s->star, star.data(), star.size());
     UP(s->edges, edg.data(), edg.size());
     UP(s->edge_bbits, m.e_bbits.data(), m.e_bbits.size());
+    UP(s->col_order, col_order.data(), col_order.size());
     AL(s->vkeys, nv); AL(s->vsorted, nv); AL(s->vcounts, T + (T + 1023) / 1024); AL(s->vstarts, T + 1);
     AL(s->ekeys, NE); AL(s->esorted, NE); AL(s->ecounts, T + (T + 1023) / 1024); AL(s->estarts, T + 1);
     AL(s->accum, nv);
@@ -627,7 +666,7 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
                         })) != hipSuccess) return e;
         }
         if (!s->cols_h.empty() || !s->tris_h.empty() || s->sdf_nb > 0)
-            if ((e = mark(5, [&] { return tc::launch_collision(s->stream, Pc, s->X.p, n, s->cols.p, uint32_t(s->cols_h.size()),
+            if ((e = mark(5, [&] { return tc::launch_collision(s->stream, Pc, s->X.p, n, s->col_order.p, s->cols.p, uint32_t(s->cols_h.size()),
                                                                s->tris.p, s->tri_bounds.p, uint32_t(s->tris_h.size()), s->tri_cbox.p,
                                                                s->single_sided_recovery, dt_sub, B, s->sdf_args()); })) != hipSuccess) return e;
         if (s->self_collision)

@@ -103,6 +103,7 @@ struct tc_sim {
     DevBuf<tc::MeshTri> tris;
     DevBuf<float4> tri_bounds;
     DevBuf<float4> tri_cbox;           // super-cluster then cluster boxes (lo, hi) of the Morton-ordered triangles
+    DevBuf<uint32_t> col_order;        // vertices in Morton order of their rest positions (collider lanes)
     int single_sided_recovery = 1;
 
     // SDF colliders (set_bone_sdf_colliders)

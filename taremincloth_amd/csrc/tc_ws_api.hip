@@ -644,7 +644,7 @@ int tc_sim_ws_time(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
         TC_HIP(tc::ws::launch_update_prev(s->stream, Pc, s->ws_ppred.p, s->ws_D.p, nullptr, sc, s->ws_beta, nv));
         TC_HIP(hipEventRecord(ev[24], s->stream));
         if (!s->cols_h.empty() || !s->tris_h.empty() || s->sdf_nb > 0)
-            TC_HIP(tc::launch_collision(s->stream, Pc, s->X.p, nv, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
+            TC_HIP(tc::launch_collision(s->stream, Pc, s->X.p, nv, s->col_order.p, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
                                         s->tri_bounds.p, uint32_t(s->tris_h.size()), s->tri_cbox.p, s->single_sided_recovery, dt_sub,
                                         B, s->sdf_args()));
         if (s->self_collision) {

@@ -323,17 +323,22 @@ __device__ __forceinline__ bool seg_box_near(f3 a, f3 b, float4 lo, float4 hi, f
 }
 
 __global__ void __launch_bounds__(256) k_collision(float4 *__restrict__ P, float4 *__restrict__ X, uint32_t n,
+                                                   const uint32_t *__restrict__ order,
                                                    const Collider *__restrict__ cols, uint32_t ncols,
                                                    const MeshTri *__restrict__ tris,
                                                    const float4 *__restrict__ bounds, uint32_t ntris,
                                                    const float4 *__restrict__ cbox, int single_sided_recovery,
-                                                   float dt, SdfArgs sdf) {
-    // Every lane of a wavefront stays to the end: lanes without a vertex (or with a pinned one)
-    // still take part in staging the triangle clusters and in the cluster votes.
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                   float dt, SdfArgs sdf, uint32_t vpw) {
+    // A wavefront takes vpw vertices (lanes 0..vpw-1); slot k handles vertex order[k], a spatially
+    // compact order, so that the lanes of a wavefront need the same triangle clusters.  Every lane
+    // of a wavefront stays to the end: lanes without a vertex (or with a pinned one) still take
+    // part in staging and in the cluster votes.
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    const float4 q = i < n ? P[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const bool active = i < n && q.w > 0.0f;
+    const uint32_t slot = (blockIdx.x * (blockDim.x >> 6) + wv) * vpw + lane;
+    const bool has = lane < vpw && slot < n;
+    const uint32_t i = has ? (order ? order[slot] : slot) : 0u;
+    const float4 q = has ? P[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const bool active = has && q.w > 0.0f;
     const float4 x = active ? X[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     f3 p = xyz(q);
     const f3 p_init = p, x_old = xyz(x);
@@ -373,90 +378,130 @@ __global__ void __launch_bounds__(256) k_collision(float4 *__restrict__ P, float
             if (dist < th) contact_response(p + nrm * (th - dist), nrm, col.friction, col.restitution, dt, p_init, xn, p);
         }
     }
-    bool front = false, has_rec = false;
+    // bit 0: a front contact was applied; bit 1: a recovery candidate exists (one word: as two
+    // separate flags the compiler merged their stores through a pointer select into scratch)
+    uint32_t contact_bits = 0u;
     float best_depth = 999999.0f;
     f3 best_pos = p, best_n = mk3(0.0f, 0.0f, 0.0f);
     float best_fr = 0.0f, best_rest = 0.0f;
+    // collision.wgsl:463-542 for one triangle, in the scan's order
+    auto triangle = [&](const MeshTri &T) {
+        const f3 t0 = mk3(T.p0x, T.p0y, T.p0z), t1 = mk3(T.p1x, T.p1y, T.p1z), t2 = mk3(T.p2x, T.p2y, T.p2z);
+        const float target = th + T.thickness;
+        const f3 cr = cross(t1 - t0, t2 - t0);
+        const float cl = length(cr);
+        if (cl < 1e-7f) return;
+        const f3 fn = cr / cl;
+        const bool single = (T.flags & 1u) != 0u;
+        float th_hit;
+        if (seg_tri(x_old, p, t0, t1, t2, th_hit)) {
+            const f3 hit = x_old + (p - x_old) * th_hit;
+            f3 nrm = fn;
+            if (dot(p - x_old, fn) > 0.0f && !single) nrm = -fn;
+            contact_response(hit + nrm * target, nrm, T.friction, T.restitution, dt, p_init, xn, p);
+            contact_bits |= 1u;
+        }
+        bool is_face;
+        const f3 qp = closest_on_tri(p, t0, t1, t2, is_face);
+        const f3 d = p - qp;
+        const float dist = length(d);
+        const float sd = dot(d, fn);
+        if (single) {
+            if (sd >= 0.0f) {
+                if (dist < target) {
+                    f3 nrm = fn;
+                    if (dist > 1e-7f) nrm = d / dist;
+                    contact_response(p + nrm * (target - dist), nrm, T.friction, T.restitution, dt, p_init, xn, p);
+                    contact_bits |= 1u;
+                }
+            } else if (single_sided_recovery && (T.flags & 2u) == 0u) {
+                if (is_face && dist < target * 3.0f && dist <= fabsf(sd) * 1.1f + 1e-7f && dist < best_depth) {
+                    best_depth = dist;
+                    best_n = fn;
+                    best_pos = p + fn * (target - sd);
+                    best_fr = T.friction;
+                    best_rest = T.restitution;
+                    contact_bits |= 2u;
+                }
+            }
+        } else if (dist < target && sd >= -target) {
+            f3 nrm = fn;
+            if (dist > 1e-7f) nrm = d / dist;
+            contact_response(p + nrm * (target - dist), nrm, T.friction, T.restitution, dt, p_init, xn, p);
+            contact_bits |= 1u;
+        }
+    };
     // Triangles in Morton order, in clusters of 32 and super-clusters of 32 clusters with boxes
     // around their bounding spheres scaled by 1.5 (cbox: super-cluster boxes, then cluster
     // boxes, lo/hi pairs).  A (super-)cluster is skipped when no lane's current segment comes
-    // near its box; the triangles are still visited in Morton order, so every contact is applied
-    // in the same order as in a scan of all triangles.
-    // A cluster that passes the vote is staged in LDS by the whole wavefront (bounding spheres and
-    // triangles), so its 32 triangles cost one round trip to memory instead of 32.
+    // near its box.  The boxes of 32 super-clusters, the boxes of one super-cluster's clusters,
+    // and a cluster's bounding spheres and triangles are staged in LDS by the whole wavefront.
+    // Inside a cluster each lane marks the triangles whose bounding-sphere test it passes and
+    // works through its own marks in order, all lanes in step, so that the wavefront pays for
+    // the longest list instead of every triangle that any lane needs.  A contact that moves p
+    // renews the lane's marks after that triangle.  Each vertex therefore meets its triangles in
+    // the scan's order with the p the scan would have, and every contact is applied as in a scan
+    // of all triangles.
+    __shared__ float4 s_sbox[4][64];
+    __shared__ float4 s_cbox[4][64];
     __shared__ float4 s_bound[4][32];
     __shared__ MeshTri s_tri[4][32];
     const uint32_t ncl = (ntris + 31u) / 32u, nsc = (ncl + 31u) / 32u;
     const float4 *sbox = cbox, *clbox = cbox + 2 * size_t(nsc);
-    for (uint32_t sc = 0; sc < nsc; ++sc) {
-        if (!__any(active && seg_box_near(x_old, p, sbox[2 * sc], sbox[2 * sc + 1], th))) continue;
-        const uint32_t c1 = min(ncl, (sc + 1u) * 32u);
-        for (uint32_t ci = sc * 32u; ci < c1; ++ci) {
-            if (!__any(active && seg_box_near(x_old, p, clbox[2 * ci], clbox[2 * ci + 1], th))) continue;
-            const uint32_t t0i = ci * 32u, cnt = min(ntris - t0i, 32u);
-            __builtin_amdgcn_wave_barrier();  // the previous cluster's reads come first
-            if (lane < cnt) s_bound[wv][lane] = bounds[t0i + lane];
-            {
-                const float4 *src = reinterpret_cast<const float4 *>(tris + t0i);
-                float4 *dst = reinterpret_cast<float4 *>(&s_tri[wv][0]);
-                if (lane < cnt * 4u) dst[lane] = src[lane];
-                if (lane + 64u < cnt * 4u) dst[lane + 64u] = src[lane + 64u];
-            }
+    auto marks = [&](uint32_t from, uint32_t cnt) -> uint32_t {
+        uint32_t m = 0;
+        for (uint32_t kk = from; kk < cnt; ++kk) {
+            const float4 b = s_bound[wv][kk];
+            const float rr = b.w * 1.5f + th;
+            if (!(seg_sphere_dist_sq(x_old, p, xyz(b)) > rr * rr)) m |= 1u << kk;
+        }
+        return m;
+    };
+    for (uint32_t s0 = 0; s0 < nsc; s0 += 32u) {
+        const uint32_t s1 = min(nsc, s0 + 32u);
+        __builtin_amdgcn_wave_barrier();  // the previous chunk's reads come first
+        if (lane < 2u * (s1 - s0)) s_sbox[wv][lane] = sbox[2 * size_t(s0) + lane];
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t sc = s0; sc < s1; ++sc) {
+            if (!__any(active && seg_box_near(x_old, p, s_sbox[wv][2u * (sc - s0)], s_sbox[wv][2u * (sc - s0) + 1u],
+                                              th)))
+                continue;
+            const uint32_t c0 = sc * 32u, c1 = min(ncl, c0 + 32u);
             __builtin_amdgcn_wave_barrier();
-            for (uint32_t kk = 0; kk < cnt; ++kk) {
-                const float4 b = s_bound[wv][kk];
-                const float rr = b.w * 1.5f + th;
-                if (!active || seg_sphere_dist_sq(x_old, p, xyz(b)) > rr * rr) continue;
-                const MeshTri T = s_tri[wv][kk];
-                const f3 t0 = mk3(T.p0x, T.p0y, T.p0z), t1 = mk3(T.p1x, T.p1y, T.p1z), t2 = mk3(T.p2x, T.p2y, T.p2z);
-                const float target = th + T.thickness;
-                const f3 cr = cross(t1 - t0, t2 - t0);
-                const float cl = length(cr);
-                if (cl < 1e-7f) continue;
-                const f3 fn = cr / cl;
-                const bool single = (T.flags & 1u) != 0u;
-                float th_hit;
-                if (seg_tri(x_old, p, t0, t1, t2, th_hit)) {
-                    const f3 hit = x_old + (p - x_old) * th_hit;
-                    f3 nrm = fn;
-                    if (dot(p - x_old, fn) > 0.0f && !single) nrm = -fn;
-                    contact_response(hit + nrm * target, nrm, T.friction, T.restitution, dt, p_init, xn, p);
-                    front = true;
+            if (lane < 2u * (c1 - c0)) s_cbox[wv][lane] = clbox[2 * size_t(c0) + lane];
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t ci = c0; ci < c1; ++ci) {
+                if (!__any(active && seg_box_near(x_old, p, s_cbox[wv][2u * (ci - c0)], s_cbox[wv][2u * (ci - c0) + 1u],
+                                                  th)))
+                    continue;
+                const uint32_t t0i = ci * 32u, cnt = min(ntris - t0i, 32u);
+                __builtin_amdgcn_wave_barrier();  // the previous cluster's reads come first
+                if (lane < cnt) s_bound[wv][lane] = bounds[t0i + lane];
+                {
+                    const float4 *src = reinterpret_cast<const float4 *>(tris + t0i);
+                    float4 *dst = reinterpret_cast<float4 *>(&s_tri[wv][0]);
+                    if (lane < cnt * 4u) dst[lane] = src[lane];
+                    if (lane + 64u < cnt * 4u) dst[lane + 64u] = src[lane + 64u];
                 }
-                bool is_face;
-                const f3 qp = closest_on_tri(p, t0, t1, t2, is_face);
-                const f3 d = p - qp;
-                const float dist = length(d);
-                const float sd = dot(d, fn);
-                if (single) {
-                    if (sd >= 0.0f) {
-                        if (dist < target) {
-                            f3 nrm = fn;
-                            if (dist > 1e-7f) nrm = d / dist;
-                            contact_response(p + nrm * (target - dist), nrm, T.friction, T.restitution, dt, p_init, xn, p);
-                            front = true;
-                        }
-                    } else if (single_sided_recovery && (T.flags & 2u) == 0u) {
-                        if (is_face && dist < target * 3.0f && dist <= fabsf(sd) * 1.1f + 1e-7f && dist < best_depth) {
-                            best_depth = dist;
-                            best_n = fn;
-                            best_pos = p + fn * (target - sd);
-                            best_fr = T.friction;
-                            best_rest = T.restitution;
-                            has_rec = true;
-                        }
+                __builtin_amdgcn_wave_barrier();
+                uint32_t m = active ? marks(0u, cnt) : 0u;
+                while (__any(m != 0u)) {
+                    if (m != 0u) {
+                        const uint32_t kk = uint32_t(__ffs(m) - 1);
+                        m &= m - 1u;
+                        const f3 pb = p;
+                        triangle(s_tri[wv][kk]);
+                        if (__float_as_uint(p.x) != __float_as_uint(pb.x) ||
+                            __float_as_uint(p.y) != __float_as_uint(pb.y) ||
+                            __float_as_uint(p.z) != __float_as_uint(pb.z))
+                            m = marks(kk + 1u, cnt);
                     }
-                } else if (dist < target && sd >= -target) {
-                    f3 nrm = fn;
-                    if (dist > 1e-7f) nrm = d / dist;
-                    contact_response(p + nrm * (target - dist), nrm, T.friction, T.restitution, dt, p_init, xn, p);
-                    front = true;
                 }
             }
         }
     }
     if (!active) return;
-    if (!front && has_rec) contact_response(best_pos, best_n, best_fr, best_rest, dt, p_init, xn, p);
+    if (contact_bits == 2u) contact_response(best_pos, best_n, best_fr, best_rest, dt, p_init, xn, p);
     // SDF colliders last, with the current p as the reference for the velocity (collision.wgsl:645-647)
     sdf_collision(sdf, ntris, p, xn, dt, p, th);
     P[i] = make_float4(p.x, p.y, p.z, q.w);
@@ -632,13 +677,25 @@ hipError_t launch_bending(hipStream_t s, float4 *P, const float4 *X, const uint4
     return hipGetLastError();
 }
 
-hipError_t launch_collision(hipStream_t s, float4 *P, float4 *X, uint32_t n, const Collider *cols, uint32_t ncols,
-                            const MeshTri *tris, const float4 *bounds, uint32_t ntris, const float4 *cbox,
-                            int recovery, float dt, uint32_t block, const SdfArgs &sdf) {
+hipError_t launch_collision(hipStream_t s, float4 *P, float4 *X, uint32_t n, const uint32_t *order,
+                            const Collider *cols, uint32_t ncols, const MeshTri *tris, const float4 *bounds,
+                            uint32_t ntris, const float4 *cbox, int recovery, float dt, uint32_t block,
+                            const SdfArgs &sdf) {
     if (!n) return hipSuccess;
     if (block % 64u != 0u || block > 256u) return hipErrorInvalidValue;  // whole wavefronts, 4 LDS slots
-    hipLaunchKernelGGL(k_collision, dim3(grid_for(n, block)), dim3(block), 0, s, P, X, n, cols, ncols, tris, bounds,
-                       ntris, cbox, recovery, dt, sdf);
+    // Vertices per wavefront.  Against a triangle mesh each wavefront is a latency-bound chain
+    // (cluster votes, then every lane's candidate triangles in order), and the chain grows with
+    // the cloth area its lanes cover: a 16k-vertex cloth on 81,920 triangles takes 1.11 ms at 64
+    // vertices per wavefront, 0.44 ms at 16 and 0.27 ms at 4.  So the wavefronts get fewer
+    // vertices until there are about 8,192 of them (at least 4 vertices each).  Primitives and
+    // SDFs are per-lane work: there, 64.
+    uint32_t vpw = 64;
+    if (ntris) {
+        vpw = 4;
+        while (vpw < 64u && size_t(vpw) * 2u * 8192u <= n) vpw *= 2u;
+    }
+    hipLaunchKernelGGL(k_collision, dim3(grid_for(n, (block / 64u) * vpw)), dim3(block), 0, s, P, X, n, order, cols,
+                       ncols, tris, bounds, ntris, cbox, recovery, dt, sdf, vpw);
     return hipGetLastError();
 }
 
