@@ -399,6 +399,13 @@ static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
     const uint32_t nv = s->mesh.nv;
     if (pp.workgroup_size == 64 || pp.workgroup_size == 128 || pp.workgroup_size == 256) s->block = pp.workgroup_size;
     if (const char *g = std::getenv("TCLOTH_USE_GRAPH")) s->use_graph = std::strcmp(g, "0") != 0;
+    // Default tile size: small enough for one tile per CU or more.  An iteration's 14 colour
+    // phases cost about 13-15 us of barriers and LDS round trips however little a tile holds, so
+    // small meshes want many small tiles (measured at 128^2 and 256^2: 256-vertex tiles 14.3 and
+    // 15.0 us per iteration, 1024-vertex tiles 19.2 and 19.6 us); from 512^2 on, 1024-vertex
+    // tiles keep the cone halos small.  tc_sim_autotune times the alternatives per mesh.
+    s->tile_vertices = 256;
+    while (s->tile_vertices < 1024u && size_t(s->tile_vertices) * 2u * 256u <= nv) s->tile_vertices *= 2u;
     if (const char *tv = std::getenv("TCLOTH_TILE_VERTICES")) {
         const long v = std::strtol(tv, nullptr, 10);
         if (v >= 64 && v <= 16384) s->tile_vertices = uint32_t(v);

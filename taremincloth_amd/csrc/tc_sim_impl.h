@@ -147,7 +147,7 @@ struct tc_sim {
     tc::TilePlan plan;
     DevBuf<uint32_t> pl_ext_off, pl_ext_vid, pl_n_own, pl_ph_off, pl_items;
     uint32_t fused_block = 1024;
-    uint32_t tile_vertices = 1024;  // TCLOTH_TILE_VERTICES overrides; tc_sim_autotune picks per mesh
+    uint32_t tile_vertices = 1024;  // set per mesh at creation; TCLOTH_TILE_VERTICES overrides; tc_sim_autotune picks per mesh
     uint32_t last_p = 0, last_lam = 0;  // buffers holding P and lambda after the last substep
 
     // learned warm start (tc_ws.h, tc_ws_api.hip)
