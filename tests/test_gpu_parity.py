@@ -306,6 +306,28 @@ def test_mesh_collider_culling_large_mesh():
     assert np.linalg.norm(_pos(g), axis=1).min() > 0.13
 
 
+def test_mesh_collider_65k_vertex_cloth():
+    """A 256 x 256 cloth (65,536 vertices) starting 1 mm above a 5,120-triangle sphere: the
+    collider packs 8 vertices per wavefront at this size (4 on the smaller test meshes). Two
+    substeps, bit for bit against the oracle's scan of every triangle."""
+    _say("mesh collider, 65,536-vertex cloth")
+    sp, sf = cases.icosphere(4, radius=0.3, center=(0.0, 0.0, 0.0), seed=4)
+    tris = sp[sf.astype(np.int64)]
+    pos, e, f, inv = cases.drape_grid(256, 0.004, 0.301, seed=6)
+    g = T.ClothSimulator(pos, e, f, inv, thickness=0.003)
+    o = O.OracleSim(pos, e, f, inv, thickness=0.003)
+    for s in (g, o):
+        s.set_gravity(0.0, 0.0, -9.81)
+        s.set_enable_self_collision(False)
+        s.set_mesh_collider_triangles(tris, friction=0.4, thickness=0.002, restitution=0.0)
+        s.set_solver_iterations(4)
+    g.step(1.0 / 60.0, 2, 4)
+    o.step(1.0 / 60.0, 2, 4)
+    po = o.get_positions().reshape(-1, 3)
+    assert float(np.abs(_pos(g) - po).max()) == 0.0
+    assert int((po[:, 2] > pos[:, 2] + 1e-4).sum()) > 100  # the sphere pushed the centre up
+
+
 def _run_path(path, pos, e, f, inv, frames, damp=False, iters=7, colliders=True):
     g = T.ClothSimulator(pos, e, f, inv, thickness=0.004)
     g.set_compliances_raw(1e-8, 1e-6, 1e-5)
