@@ -100,11 +100,18 @@ __global__ void __launch_bounds__(1024) k_tile_iteration(
     }
     const uint32_t *po = ph_off + size_t(tile) * (fa.nphases + 1);
     const uint32_t L = fa.nphases, ndp = fa.nd_phases;
-    // first item of phase 0 (words) and its lambda
-    uint32_t i_cur = po[0] + tid;
-    bool h_cur = i_cur < po[1];
-    uint32_t c0 = 0, c1 = 0, c2 = 0;
-    if (h_cur) { c0 = items[3 * size_t(i_cur)]; c1 = items[3 * size_t(i_cur) + 1]; c2 = items[3 * size_t(i_cur) + 2]; }
+    // A thread's first item of a phase moves through a two-deep pipeline: its words are loaded
+    // two phases ahead and its lambda one phase ahead, so neither load waits at a phase start.
+    auto load_item = [&](uint32_t c, uint32_t &w0, uint32_t &w1, uint32_t &w2) -> bool {
+        if (c >= L) return false;
+        const uint32_t i = po[c] + tid;
+        if (i >= po[c + 1]) return false;
+        w0 = items[3 * size_t(i)]; w1 = items[3 * size_t(i) + 1]; w2 = items[3 * size_t(i) + 2];
+        return true;
+    };
+    uint32_t c0 = 0, c1 = 0, c2 = 0, n0 = 0, n1 = 0, n2 = 0;
+    bool h_cur = load_item(0, c0, c1, c2);
+    bool h_nxt = load_item(1, n0, n1, n2);
     const uint32_t e0 = ext_off[tile], ne = ext_off[tile + 1] - e0;
     float4 *sx = sp + fa.max_ext;  // DAMP only: committed positions of the cone
     for (uint32_t l = tid; l < ne; l += bs) {
@@ -118,14 +125,11 @@ __global__ void __launch_bounds__(1024) k_tile_iteration(
     const float dt = fmaxf(a.dt, 1e-8f);
     for (uint32_t c = 0; c < L; ++c) {
         const bool dist = c < ndp;
-        // issue the next phase's first item
-        uint32_t n0 = 0, n1 = 0, n2 = 0;
-        bool h_nxt = false;
-        if (c + 1 < L) {
-            const uint32_t i_nxt = po[c + 1] + tid;
-            h_nxt = i_nxt < po[c + 2];
-            if (h_nxt) { n0 = items[3 * size_t(i_nxt)]; n1 = items[3 * size_t(i_nxt) + 1]; n2 = items[3 * size_t(i_nxt) + 2]; }
-        }
+        // the next phase's lambda (its item arrived during the previous phase) and the item after
+        float ln = 0.0f;
+        if (!FIRST && h_nxt) ln = (c + 1 < ndp ? dlam_in : blam_in)[n1 & 0x03FFFFFFu];
+        uint32_t m0 = 0, m1 = 0, m2 = 0;
+        const bool h_nn = load_item(c + 2, m0, m1, m2);
         const float *lam_in = dist ? dlam_in : blam_in;
         float *lam_out = dist ? dlam_out : blam_out;
         if (h_cur) {
@@ -138,10 +142,8 @@ __global__ void __launch_bounds__(1024) k_tile_iteration(
             if (dist) project_item<true, DAMP>(sp, sx, w0, w1, w2, l, lam_out, cls_tab, a, dt);
             else project_item<false, DAMP>(sp, sx, w0, w1, w2, l, lam_out, cls_tab, a, dt);
         }
-        // the next phase's lambda, now that its item has arrived
-        c0 = n0; c1 = n1; c2 = n2; h_cur = h_nxt;
-        lc = 0.0f;
-        if (!FIRST && h_cur) lc = (c + 1 < ndp ? dlam_in : blam_in)[c1 & 0x03FFFFFFu];
+        c0 = n0; c1 = n1; c2 = n2; h_cur = h_nxt; lc = ln;
+        n0 = m0; n1 = m1; n2 = m2; h_nxt = h_nn;
         __syncthreads();
     }
     const uint32_t own = n_own[tile];
