@@ -504,8 +504,10 @@ def main():
     solver_rl = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname,
                  "bytes_per_launch": bytes_per_launch, "launch_ms": launch_ms}
-    # the dominant kernel of the learned substep by time
-    roofline = net_rl if wt["round_mlps"] >= wt["iterations"] else solver_rl
+    # the dominant kernel of the learned substep by time: its solver iterations are the warm
+    # iterations plus the fallback iterations, in the share of substeps whose gate failed
+    fb_share = stats["fallbacks"] / max(stats["checked"], 1)
+    roofline = net_rl if wt["round_mlps"] >= wt["iterations"] + wt["fallback"] * fb_share else solver_rl
 
     sdf = None if args.no_sdf else sdf_bake_bench(T, not args.no_cpu_baseline)
     batch = None if args.no_batch else batch_bench(T, 30, 5, not args.no_cpu_baseline)
