@@ -81,12 +81,11 @@ struct HashBufs {
     uint32_t *starts;   // table + 1
     uint32_t *sorted;   // n
 };
-// Vertex hash over P.xyz (spatial_grid_sort.wgsl); within a cell indices are ascending.
-hipError_t launch_hash_vertices(hipStream_t s, const float4 *P, uint32_t n, float cell, uint32_t table,
-                                const HashBufs &h);
-// Edge hash over the midpoint of P (spatial_grid_sort_edges.wgsl).
-hipError_t launch_hash_edges(hipStream_t s, const float4 *P, const uint2 *edges, uint32_t ne, float cell,
-                             uint32_t table, const HashBufs &h);
+// Vertex hash over P.xyz (spatial_grid_sort.wgsl) and edge hash over the edge midpoints
+// (spatial_grid_sort_edges.wgsl), built together, table buckets each; within a cell indices
+// are ascending.
+hipError_t launch_hashes(hipStream_t s, const float4 *P, uint32_t nv, const uint2 *edges, uint32_t ne, float cell,
+                         uint32_t table, const HashBufs &vh, const HashBufs &eh);
 
 struct SelfColArgs {
     float cell;

@@ -7,9 +7,9 @@
 //   E-E solve         crates/cloth_core/src/shaders/self_collision_ee_direct.wgsl:66-776
 //   apply             crates/cloth_core/src/shaders/self_collision_apply.wgsl:57-177
 //
-// Hash build: count (atomics) -> exclusive scan of the table (one workgroup per
-// row of 1024 entries, one over the row totals, one add) -> scatter (atomic
-// cursor) -> per-bucket sort, so every bucket lists its elements in ascending index order.  The
+// Hash build, vertices and edge midpoints together: count (atomics) -> exclusive
+// scan of each table (one workgroup per row of 1024 entries, one over the row
+// totals, one add) -> scatter (atomic cursor) -> per-bucket sort, so every bucket lists its elements in ascending index order.  The
 // reference's scatter order inside a bucket depends on atomic timing; the sorted
 // order makes the hash tables, and with them the E-E search cut-off,
 // deterministic.
@@ -22,36 +22,46 @@ namespace tc {
 
 static inline uint32_t grid_for(size_t n, uint32_t block) { return uint32_t((n + block - 1) / block); }
 
-__global__ void __launch_bounds__(256) k_hash_count_vertices(const float4 *__restrict__ P, uint32_t n, float cell,
-                                                             uint32_t table, uint32_t *__restrict__ keys,
-                                                             uint32_t *__restrict__ counts) {
+// The vertex table (0) and the edge-midpoint table (1) have the same bucket count and are built
+// together: every launch below serves both, the table picked by the thread or workgroup index.
+struct HashPair {
+    HashBufs t[2];
+    uint32_t n[2];      // elements: vertices, edges
+    uint32_t table;     // buckets per table
+};
+
+__global__ void __launch_bounds__(256) k_hash_clear(HashPair hp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t h = hash_cell(cell_of(xyz(P[i]), cell), table);
-    keys[i] = h;
-    atomicAdd(&counts[h], 1u);
+    if (i < 2u * hp.table) hp.t[i / hp.table].counts[i % hp.table] = 0u;
 }
 
-__global__ void __launch_bounds__(256) k_hash_count_edges(const float4 *__restrict__ P, const uint2 *__restrict__ e,
-                                                          uint32_t n, float cell, uint32_t table,
-                                                          uint32_t *__restrict__ keys, uint32_t *__restrict__ counts) {
+__global__ void __launch_bounds__(256) k_hash_count(HashPair hp, const float4 *__restrict__ P,
+                                                    const uint2 *__restrict__ e, float cell) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint2 ed = e[i];
-    const f3 mid = (xyz(P[ed.x]) + xyz(P[ed.y])) * 0.5f;
-    const uint32_t h = hash_cell(cell_of(mid, cell), table);
-    keys[i] = h;
-    atomicAdd(&counts[h], 1u);
+    if (i >= hp.n[0] + hp.n[1]) return;
+    uint32_t h, k;
+    if (i < hp.n[0]) {
+        k = i;
+        h = hash_cell(cell_of(xyz(P[k]), cell), hp.table);
+        hp.t[0].keys[k] = h;
+        atomicAdd(&hp.t[0].counts[h], 1u);
+    } else {
+        k = i - hp.n[0];
+        const uint2 ed = e[k];
+        const f3 mid = (xyz(P[ed.x]) + xyz(P[ed.y])) * 0.5f;
+        h = hash_cell(cell_of(mid, cell), hp.table);
+        hp.t[1].keys[k] = h;
+        atomicAdd(&hp.t[1].counts[h], 1u);
+    }
 }
 
 // Exclusive scan of counts[0..table) into starts[0..table], starts[table] = total,
-// in three launches over rows of 1024 entries (at most SCAN_ROWS rows), coalesced
-// throughout.  k_scan_rows (one workgroup per row) scans each row with wave64
-// prefix sums and the 16 wavefront totals, writes the in-row exclusive sums,
-// keeps the row total in counts[table + row] and resets counts[0..table) to zero
-// for use as scatter cursors; k_scan_totals (one workgroup) turns the row totals
-// into their exclusive prefix and writes starts[table]; k_scan_add adds each
-// row's prefix.
+// for both tables in three launches over rows of 1024 entries (at most SCAN_ROWS rows per
+// table), coalesced throughout.  k_scan_rows (one workgroup per row and table) scans each row
+// with wave64 prefix sums and the 16 wavefront totals, writes the in-row exclusive sums, keeps
+// the row total in counts[table + row] and resets counts[0..table) to zero for use as scatter
+// cursors; k_scan_totals (one workgroup per table) turns the row totals into their exclusive
+// prefix and writes starts[table]; k_scan_add adds each row's prefix.
 constexpr uint32_t SCAN_ROWS = 4096;
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
     for (uint32_t o = 1; o < 64; o <<= 1) {
@@ -61,11 +71,13 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
     return x;
 }
 
-__global__ void __launch_bounds__(1024) k_scan_rows(uint32_t *__restrict__ counts, uint32_t *__restrict__ starts,
-                                                    uint32_t table) {
+__global__ void __launch_bounds__(1024) k_scan_rows(HashPair hp, uint32_t rows) {
     __shared__ uint32_t wave_tot[16];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const uint32_t k = (blockIdx.x << 10) + tid;
+    const uint32_t tb = blockIdx.x / rows, row = blockIdx.x % rows, table = hp.table;
+    uint32_t *__restrict__ counts = hp.t[tb].counts;
+    uint32_t *__restrict__ starts = hp.t[tb].starts;
+    const uint32_t k = (row << 10) + tid;
     const uint32_t c = k < table ? counts[k] : 0u;
     const uint32_t incl = wave_incl_scan(c, lane);
     if (lane == 63u) wave_tot[wave] = incl;
@@ -76,15 +88,14 @@ __global__ void __launch_bounds__(1024) k_scan_rows(uint32_t *__restrict__ count
         starts[k] = base + incl - c;
         counts[k] = 0u;
     }
-    if (tid == 1023u) counts[table + blockIdx.x] = base + incl;
+    if (tid == 1023u) counts[table + row] = base + incl;
 }
 
 // rows <= SCAN_ROWS = 4 x 1024: thread t owns rows 4t .. 4t+3
-__global__ void __launch_bounds__(1024) k_scan_totals(uint32_t *__restrict__ counts, uint32_t *__restrict__ starts,
-                                                      uint32_t table, uint32_t rows) {
+__global__ void __launch_bounds__(1024) k_scan_totals(HashPair hp, uint32_t rows) {
     __shared__ uint32_t wave_tot[16];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    uint32_t *tot = counts + table;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6, table = hp.table;
+    uint32_t *tot = hp.t[blockIdx.x].counts + table;
     uint32_t v[4], sum = 0;
 #pragma unroll
     for (uint32_t u = 0; u < 4; ++u) {
@@ -104,28 +115,30 @@ __global__ void __launch_bounds__(1024) k_scan_totals(uint32_t *__restrict__ cou
         if (r < rows) tot[r] = run;
         run += v[u];
     }
-    if (tid == 1023u) starts[table] = run;
+    if (tid == 1023u) hp.t[blockIdx.x].starts[table] = run;
 }
 
-__global__ void __launch_bounds__(1024) k_scan_add(const uint32_t *__restrict__ counts, uint32_t *__restrict__ starts,
-                                                   uint32_t table) {
-    const uint32_t k = (blockIdx.x << 10) + threadIdx.x;
-    if (k < table) starts[k] += counts[table + blockIdx.x];
+__global__ void __launch_bounds__(1024) k_scan_add(HashPair hp, uint32_t rows) {
+    const uint32_t tb = blockIdx.x / rows, row = blockIdx.x % rows;
+    const uint32_t k = (row << 10) + threadIdx.x;
+    if (k < hp.table) hp.t[tb].starts[k] += hp.t[tb].counts[hp.table + row];
 }
 
-__global__ void __launch_bounds__(256) k_hash_scatter(const uint32_t *__restrict__ keys, uint32_t n,
-                                                      const uint32_t *__restrict__ starts,
-                                                      uint32_t *__restrict__ cursor, uint32_t *__restrict__ sorted) {
+__global__ void __launch_bounds__(256) k_hash_scatter(HashPair hp) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t h = keys[i];
-    sorted[starts[h] + atomicAdd(&cursor[h], 1u)] = i;
+    if (i >= hp.n[0] + hp.n[1]) return;
+    const uint32_t tb = i < hp.n[0] ? 0u : 1u, k = tb ? i - hp.n[0] : i;
+    const HashBufs &h = hp.t[tb];
+    const uint32_t b = h.keys[k];
+    h.sorted[h.starts[b] + atomicAdd(&h.counts[b], 1u)] = k;
 }
 
-__global__ void __launch_bounds__(256) k_bucket_sort(const uint32_t *__restrict__ starts, uint32_t table,
-                                                     uint32_t *__restrict__ sorted) {
-    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= table) return;
+__global__ void __launch_bounds__(256) k_bucket_sort(HashPair hp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2u * hp.table) return;
+    const uint32_t tb = i / hp.table, h = i % hp.table;
+    const uint32_t *__restrict__ starts = hp.t[tb].starts;
+    uint32_t *__restrict__ sorted = hp.t[tb].sorted;
     const uint32_t b = starts[h], e = starts[h + 1];
     for (uint32_t k = b + 1; k < e; ++k) {
         const uint32_t x = sorted[k];
@@ -135,40 +148,24 @@ __global__ void __launch_bounds__(256) k_bucket_sort(const uint32_t *__restrict_
     }
 }
 
-static hipError_t finish_hash(hipStream_t s, uint32_t n, uint32_t table, const HashBufs &h) {
+hipError_t launch_hashes(hipStream_t s, const float4 *P, uint32_t nv, const uint2 *edges, uint32_t ne, float cell,
+                         uint32_t table, const HashBufs &vh, const HashBufs &eh) {
     if (table == 0u || table > SCAN_ROWS * 1024u) return hipErrorInvalidValue;
-    const uint32_t rows = (table + 1023u) >> 10;
-    hipLaunchKernelGGL(k_scan_rows, dim3(rows), dim3(1024), 0, s, h.counts, h.starts, table);
-    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, s, h.counts, h.starts, table, rows);
-    hipLaunchKernelGGL(k_scan_add, dim3(rows), dim3(1024), 0, s, h.counts, h.starts, table);
-    hipLaunchKernelGGL(k_hash_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, h.keys, n, h.starts, h.counts,
-                       h.sorted);
-    hipLaunchKernelGGL(k_bucket_sort, dim3(grid_for(table, 256)), dim3(256), 0, s, h.starts, table, h.sorted);
+    HashPair hp;
+    hp.t[0] = vh;
+    hp.t[1] = eh;
+    hp.n[0] = nv;
+    hp.n[1] = ne;
+    hp.table = table;
+    const uint32_t rows = (table + 1023u) >> 10, nall = nv + ne;
+    hipLaunchKernelGGL(k_hash_clear, dim3(grid_for(2 * size_t(table), 256)), dim3(256), 0, s, hp);
+    if (nall) hipLaunchKernelGGL(k_hash_count, dim3(grid_for(nall, 256)), dim3(256), 0, s, hp, P, edges, cell);
+    hipLaunchKernelGGL(k_scan_rows, dim3(2 * rows), dim3(1024), 0, s, hp, rows);
+    hipLaunchKernelGGL(k_scan_totals, dim3(2), dim3(1024), 0, s, hp, rows);
+    hipLaunchKernelGGL(k_scan_add, dim3(2 * rows), dim3(1024), 0, s, hp, rows);
+    if (nall) hipLaunchKernelGGL(k_hash_scatter, dim3(grid_for(nall, 256)), dim3(256), 0, s, hp);
+    hipLaunchKernelGGL(k_bucket_sort, dim3(grid_for(2 * size_t(table), 256)), dim3(256), 0, s, hp);
     return hipGetLastError();
-}
-
-hipError_t launch_hash_vertices(hipStream_t s, const float4 *P, uint32_t n, float cell, uint32_t table,
-                                const HashBufs &h) {
-    hipError_t e = launch_fill_u32(s, h.counts, 0u, table);
-    if (e != hipSuccess || !n) {
-        if (e == hipSuccess) e = launch_fill_u32(s, h.starts, 0u, size_t(table) + 1);
-        return e;
-    }
-    hipLaunchKernelGGL(k_hash_count_vertices, dim3(grid_for(n, 256)), dim3(256), 0, s, P, n, cell, table, h.keys,
-                       h.counts);
-    return finish_hash(s, n, table, h);
-}
-
-hipError_t launch_hash_edges(hipStream_t s, const float4 *P, const uint2 *edges, uint32_t ne, float cell,
-                             uint32_t table, const HashBufs &h) {
-    hipError_t e = launch_fill_u32(s, h.counts, 0u, table);
-    if (e != hipSuccess || !ne) {
-        if (e == hipSuccess) e = launch_fill_u32(s, h.starts, 0u, size_t(table) + 1);
-        return e;
-    }
-    hipLaunchKernelGGL(k_hash_count_edges, dim3(grid_for(ne, 256)), dim3(256), 0, s, P, edges, ne, cell, table,
-                       h.keys, h.counts);
-    return finish_hash(s, ne, table, h);
 }
 
 // ---------------------------------------------------------------------------

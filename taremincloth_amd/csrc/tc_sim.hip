@@ -36,10 +36,8 @@ using tc::g_err;
 namespace {
 
 hipError_t enqueue_hashes(tc_sim *s, const float4 *P) {
-    hipError_t e = tc::launch_hash_vertices(s->stream, P, s->mesh.nv, s->cell_size, s->table_size, s->vhash());
-    if (e != hipSuccess) return e;
-    return tc::launch_hash_edges(s->stream, P, s->edges.p, uint32_t(s->mesh.e_v0.size()), s->cell_size,
-                                 s->table_size, s->ehash());
+    return tc::launch_hashes(s->stream, P, s->mesh.nv, s->edges.p, uint32_t(s->mesh.e_v0.size()), s->cell_size,
+                             s->table_size, s->vhash(), s->ehash());
 }
 
 // One substep up to (optionally) the velocity update.  Predict writes P buffer 0;

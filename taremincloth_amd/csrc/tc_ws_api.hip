@@ -648,9 +648,8 @@ int tc_sim_ws_time(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
                                         s->tri_bounds.p, uint32_t(s->tris_h.size()), s->tri_cbox.p, s->single_sided_recovery, dt_sub,
                                         B, s->sdf_args()));
         if (s->self_collision) {
-            TC_HIP(tc::launch_hash_vertices(s->stream, Pc, nv, s->cell_size, s->table_size, s->vhash()));
-            TC_HIP(tc::launch_hash_edges(s->stream, Pc, s->edges.p, uint32_t(m.e_v0.size()), s->cell_size,
-                                         s->table_size, s->ehash()));
+            TC_HIP(tc::launch_hashes(s->stream, Pc, nv, s->edges.p, uint32_t(m.e_v0.size()), s->cell_size,
+                                     s->table_size, s->vhash(), s->ehash()));
             const tc::SelfColArgs sa = s->sc_args();
             const tc::SelfColTopo t = s->topo();
             TC_HIP(tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa));
