@@ -423,25 +423,28 @@ def main():
         while done < args.steps:
             run_substeps(sim, every, dt_frame, per_frame)
             done += every
+            # switching from plain back to the warm start clears the previous correction, so
+            # every restore follows the mode switch
             st = snapshot(sim, nv)
             plain_on()
             sim.substep_parts(dt_sub, collide=False, self_collide=False, update=False)
             r_plain, m_plain = sim.residual(dt_sub)
-            restore(sim, st)
             learned_on(1e9)
+            restore(sim, st)
             sim.substep_parts(dt_sub, collide=False, self_collide=False, update=False)
             r_learn, m_learn = sim.residual(dt_sub)
-            restore(sim, st)
             # plain XPBD at fewer iterations from the same state: how many plain iterations the
             # network plus its 8 iterations are worth
             curve = {}
             for k in EQUIV_ITERS:
                 plain_on()
+                restore(sim, st)
                 sim.set_solver_iterations(k)
                 sim.substep_parts(dt_sub, collide=False, self_collide=False, update=False)
                 curve[k], _ = sim.residual(dt_sub)
-                restore(sim, st)
             curve[args.iters] = r_plain
+            learned_on(1e9)  # the trajectory goes on with the warm start and its history
+            restore(sim, st)
             checks.append({"substep": args.warmup + done, "plain": r_plain, "learned": r_learn,
                            "plain_max": m_plain, "learned_max": m_learn,
                            "plain_by_iterations": {str(k): v for k, v in sorted(curve.items())},
@@ -473,6 +476,24 @@ def main():
     learned_s = time.perf_counter() - t0
     stats = sim.warm_start_stats()
     wt = sim.ws_time(dt_sub, reps=3)
+
+    # reference arm 3: the previous substep's correction alone (no network) + the same
+    # iterations and gate, to separate what the network adds from what the history adds
+    history = None
+    if not args.light:
+        sim.reset()
+        sim.set_solver_iterations(args.iters)
+        sim.set_warm_start(None, 1, args.ws_iters, args.ws_fallback, threshold, net.history_weight)
+        run_substeps(sim, args.warmup, dt_frame, per_frame)
+        sim.synchronize()
+        sim.ws_reset_stats()
+        t0 = time.perf_counter()
+        run_substeps(sim, args.steps, dt_frame, per_frame)
+        sim.synchronize()
+        hs = sim.warm_start_stats()
+        history = {"substeps_per_s": args.steps / (time.perf_counter() - t0),
+                   "gate_fallbacks": hs["fallbacks"], "gate_checked": hs["checked"],
+                   "gate_fallback_chunks": hs.get("fallback_chunks")}
 
     plain_sps, learned_sps = args.steps / plain_s, args.steps / learned_s
 
@@ -570,6 +591,7 @@ def main():
             "equal_residual_checks": checks,
             "learned_at_or_below_plain_residual": f"{sum(eq)}/{len(eq)}",
             "equal_residual_comparison": equal_residual,
+            "history_only_arm": history,
             "plain_relative_residual_end": r_plain_end,
             "learned_substep_ms_breakdown": {k: round(v, 5) for k, v in wt.items()},
             "plain_substep_ms_breakdown": {k: round(v, 5) for k, v in kt.items()},

@@ -8,6 +8,10 @@ seeded sphere and, in half of the scenes, a capsule; friction seeded in [0.2, 0.
 scene runs `--frames` frames of 10 substeps at 64 plain iterations and records every
 `--stride`-th substep (features, target correction relative to `--history-weight` times
 the previous correction).  One extra scene is held out for the validation loss.  Training and data stay on the device (tc_sim_ws_record, tc_ws_train).
+
+After training, the output gain is calibrated in closed loop on four more scenes (see
+calibrate_gain) and folded into the output layer; `--calibrate-from` runs only that step on
+existing weights.
 """
 from __future__ import annotations
 
@@ -73,6 +77,79 @@ def record(seeds, frames, stride, n, history_weight):
     return sims
 
 
+def _snapshot(sim):
+    n = sim.num_vertices()
+    x = np.zeros(3 * n, np.float32)
+    sim.get_positions(x)
+    return x.reshape(-1, 3).copy(), sim.velocities().copy(), sim.ws_history()
+
+
+def _restore(sim, st):
+    sim.set_positions_and_velocities(st[0], st[1])
+    sim.ws_history(st[2])
+
+
+def scene_threshold(seed, n, warm_frames, frames, checks, history_weight, ws_iters=8, iters=64):
+    """The gate threshold the bench uses, on one scene and window: along the warm-started
+    trajectory (previous correction only, so that it does not depend on the network), the median
+    relative residual after the plain `iters`-iteration solve from `checks` of its states."""
+    sim = make_scene(seed, n, iters)
+    sim.set_warm_start(None, 1, ws_iters, 0, 1e9, history_weight)
+    dt_sub = 1.0 / 600.0
+    for _ in range(warm_frames):
+        sim.step(1.0 / 60.0, 10)
+    every = max(frames // checks, 1)
+    rs = []
+    for _ in range(checks):
+        for _ in range(every):
+            sim.step(1.0 / 60.0, 10)
+        st = _snapshot(sim)
+        sim.set_warm_start(None, 0, ws_iters, 0, 1e9, history_weight)
+        sim.set_solver_iterations(iters)
+        sim.substep_parts(dt_sub, collide=False, self_collide=False, update=False)
+        rs.append(sim.residual(dt_sub)[0])
+        # switching back to the warm start clears the previous correction: restore after it
+        sim.set_warm_start(None, 1, ws_iters, 0, 1e9, history_weight)
+        _restore(sim, st)
+    return float(np.median(rs))
+
+
+def calibrate_gain(net, seeds, n, gains, warm_frames=25, frames=30, ws_iters=8, fallback=56):
+    """Closed-loop calibration of the network's output gain.  Trained on states of plain
+    64-iteration solves, the network runs at inference on its own warm-started trajectory, where
+    its corrections come back through the history term; there a full-strength output overshoots.
+    For each gain, the scenes (seeds unused by training and validation) run `frames` frames from
+    frame `warm_frames` (the cloth on the colliders) with the network, the warm-start iterations
+    and the residual gate at each scene's threshold (scene_threshold, same window); the gain that
+    needs the fewest fallback chunks wins (ties: the gain nearest 1)."""
+    base = net.get_params()
+    thr = {s: scene_threshold(s, n, warm_frames, frames, 10, net.history_weight, ws_iters) for s in seeds}
+    table = []
+    for g in gains:
+        net.set_params(base)
+        net.scale_output(g)
+        chunks = fallbacks = checked = 0.0
+        for s in seeds:
+            sim = make_scene(s, n)
+            sim.set_warm_start(net, 2, ws_iters, fallback, thr[s], net.history_weight)
+            for _ in range(warm_frames):
+                sim.step(1.0 / 60.0, 10)
+            sim.synchronize()
+            sim.ws_reset_stats()
+            for _ in range(frames):
+                sim.step(1.0 / 60.0, 10)
+            st = sim.warm_start_stats()
+            chunks += st["fallback_chunks"]
+            fallbacks += st["fallbacks"]
+            checked += st["checked"]
+        table.append({"gain": g, "fallback_chunks": chunks, "fallbacks": fallbacks, "checked": checked})
+        print(f"gain {g}: {int(fallbacks)}/{int(checked)} substeps fell back, {int(chunks)} chunks", flush=True)
+    best = min(table, key=lambda r: (r["fallback_chunks"], abs(r["gain"] - 1.0)))
+    net.set_params(base)
+    net.scale_output(best["gain"])
+    return best["gain"], table, thr
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
@@ -85,7 +162,26 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--history-weight", type=float, default=0.5,
                     help="weight of the previous correction in the warm start the network is trained for")
+    ap.add_argument("--calibrate-from", default=None,
+                    help="skip training: load these weights, calibrate the output gain, save to --out")
+    ap.add_argument("--gains", default="0.25,0.375,0.5,0.625,0.75,1.0")
+    ap.add_argument("--calibration-seeds", default="9100,9101,9102,9103")
     args = ap.parse_args(argv)
+    gains = [float(g) for g in args.gains.split(",")]
+    cal_seeds = [int(x) for x in args.calibration_seeds.split(",")]
+
+    if args.calibrate_from:
+        t0 = time.time()
+        net = WarmStartNet.load(args.calibrate_from)
+        with np.load(args.calibrate_from, allow_pickle=False) as z:
+            meta = json.loads(str(z["meta"])) if "meta" in z.files else {}
+        gain, table, thr = calibrate_gain(net, cal_seeds, args.grid, gains)
+        meta["output_gain"] = float(meta.get("output_gain", 1.0)) * gain
+        meta["calibration"] = {"seeds": cal_seeds, "thresholds": {str(k): v for k, v in thr.items()}, "table": table,
+                               "chosen": gain}
+        net.save(args.out, meta=np.array(json.dumps(meta)))
+        print(f"gain {gain}: saved {args.out} ({time.time() - t0:.1f} s)", flush=True)
+        return
 
     t0 = time.time()
     hw = args.history_weight
@@ -110,10 +206,13 @@ def main(argv=None):
         log["phases"].append({"lr": float(lr), "steps": int(steps), "seconds": el, "last_batch_loss": float(losses[-200:].mean()),
                               "train": tr, "val": va})
         print(f"lr {lr} x {steps}: {int(steps) / el:.0f} steps/s, train {tr:.3f} val {va:.3f}", flush=True)
+    gain, table, thr = calibrate_gain(net, cal_seeds, args.grid, gains)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     meta = {"grid": args.grid, "scenes": args.scenes, "frames": args.frames, "stride": args.stride,
             "training_substeps": substeps, "samples": nrec, "schedule": args.schedule, "history_weight": hw,
-            "log": log}
+            "log": log, "output_gain": gain,
+            "calibration": {"seeds": cal_seeds, "thresholds": {str(k): v for k, v in thr.items()}, "table": table,
+                            "chosen": gain}}
     net.save(args.out, meta=np.array(json.dumps(meta)))
     print(f"saved {args.out} ({time.time() - t0:.1f} s total)", flush=True)
 

@@ -69,6 +69,14 @@ class WarmStartNet:
         p = np.ascontiguousarray(params, np.float32).reshape(-1)
         _lib.check(self._L.tc_ws_net_set_params(self._h, p.ctypes.data_as(C.c_void_p), p.size))
 
+    def scale_output(self, gain: float):
+        """Multiply the output layer (weights and biases) by `gain`, which scales the predicted
+        correction by `gain`; exact for powers of two."""
+        p = self.get_params()
+        n_out = HIDDEN * OUTPUTS + OUTPUTS  # the last layer: H x O weights, then O biases
+        p[-n_out:] *= np.float32(gain)
+        self.set_params(p)
+
     def save(self, path, **extra):
         np.savez(path, params=self.get_params(), rounds=np.int64(self.rounds),
                  history_weight=np.float64(self.history_weight), **extra)

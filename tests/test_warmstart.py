@@ -282,3 +282,36 @@ def test_gpu_network_inside_the_substep_matches_the_standalone_pass(require_gpu)
     y_ref, _, _ = net.forward_backward(r["features"], off, nbr)
     assert np.array_equal(y_sim, y_ref)
     assert np.abs(y_sim[:, :3]).max() > 0
+
+
+@pytest.mark.gpu
+def test_gpu_calibrated_network_gains_over_the_history_alone(require_gpu):
+    """What the trained network adds to the warm start it is built on: on a 128 x 128 drape that
+    neither training, validation nor the gain calibration saw (seed 9104), with the gate at that
+    scene's plain 64-iteration median residual, the network + previous correction needs fewer
+    fallback chunks than the previous correction alone."""
+    from taremincloth_amd import train_warmstart as TW
+    from taremincloth_amd.warmstart import WarmStartNet
+
+    net = WarmStartNet.load(TW.DEFAULT_WEIGHTS)
+    seed, warm, frames = 9104, 25, 20
+    thr = TW.scene_threshold(seed, 128, warm, frames, 10, net.history_weight)
+
+    def run(mode):
+        sim = TW.make_scene(seed, 128)
+        sim.set_warm_start(net if mode == 2 else None, mode, 8, 56, thr, net.history_weight)
+        for _ in range(warm):
+            sim.step(1.0 / 60.0, 10)
+        sim.synchronize()
+        sim.ws_reset_stats()
+        for _ in range(frames):
+            sim.step(1.0 / 60.0, 10)
+        st = sim.warm_start_stats()
+        return st["fallback_chunks"], st["fallbacks"], st["checked"]
+
+    net_chunks, net_fb, checked = run(2)
+    hist_chunks, hist_fb, _ = run(1)
+    print(f"[warm start] threshold {thr:.3e}: network {net_fb:.0f}/{checked:.0f} fallbacks, {net_chunks:.0f} chunks; "
+          f"history alone {hist_fb:.0f} fallbacks, {hist_chunks:.0f} chunks", flush=True)
+    assert checked == frames * 10
+    assert net_chunks < hist_chunks
