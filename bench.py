@@ -576,7 +576,8 @@ def main():
             "workload": f"{n}x{n} grid ({nv} verts, {nd + nb} constraints) dropped on a sphere + capsule, "
                         f"self-collision off; learned arm: network ({R} rounds, hidden {HIDDEN}) + "
                         f"{args.ws_iters} iterations + residual gate with {args.ws_fallback} fallback iterations; "
-                        f"plain arm: {args.iters} iterations; same {args.steps}-substep window after "
+                        f"plain arm: {args.iters} iterations; reference arm (history_only_arm): the previous "
+                        f"correction alone + the same iterations and gate; same {args.steps}-substep window after "
                         f"{args.warmup} warm-up substeps",
             "grid": n, "vertices": nv, "distance_constraints": nd, "bending_constraints": nb,
             "plain_iterations": args.iters, "warm_start_iterations": args.ws_iters,
