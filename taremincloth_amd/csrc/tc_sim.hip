@@ -105,28 +105,61 @@ hipError_t enqueue_gated_fallback(tc_sim *s, const tc::SubstepArgs &a, float dt_
 
 namespace {
 
-hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_collide, bool update) {
+// The substep up to the end of the warm iterations.  With a warm start and a fallback, either the
+// whole gated fallback follows on the device (gate_to_host false) or only the first residual
+// check, with the gate's fallback flag copied to s->gate_host for the host to read.
+hipError_t enqueue_substep_head(tc_sim *s, float dt_sub, bool gate_to_host, uint32_t &pc, uint32_t &lc) {
     const tc::SubstepArgs a = s->args(dt_sub);
     const tc::Mesh &m = s->mesh;
     const uint32_t n = m.nv, B = s->block;
     hipError_t e = tc::launch_predict(s->stream, s->P.p, s->X.p, s->V.p, n, a, B);
     if (e != hipSuccess) return e;
-    const bool track = s->ws_tracking();
-    if (track && (e = ws_pre(s, dt_sub)) != hipSuccess) return e;
+    if (s->ws_tracking() && (e = ws_pre(s, dt_sub)) != hipSuccess) return e;
     const uint32_t iters = s->iters_effective();
-    uint32_t pc = 0, lc = 0;
+    pc = 0;
+    lc = 0;
     if (iters == 0) {  // lambdas are still cleared at the substep start (dispatch.rs:117-119)
         if ((e = tc::launch_fill_u32(s->stream, reinterpret_cast<uint32_t *>(s->dlam.p), 0u, m.d_v0.size())) != hipSuccess) return e;
         if ((e = tc::launch_fill_u32(s->stream, reinterpret_cast<uint32_t *>(s->blam.p), 0u, m.b_v0.size())) != hipSuccess) return e;
-    } else {
-        if ((e = enqueue_iterations(s, a, iters, true, pc, lc)) != hipSuccess) return e;
-        if (s->ws_active() && s->ws_fallback > 0 && (e = enqueue_gated_fallback(s, a, dt_sub, pc, lc)) != hipSuccess)
-            return e;
+        return e;
     }
+    if ((e = enqueue_iterations(s, a, iters, true, pc, lc)) != hipSuccess) return e;
+    if (!s->ws_active() || s->ws_fallback == 0) return e;
+    if (!gate_to_host) return enqueue_gated_fallback(s, a, dt_sub, pc, lc);
+    e = tc::launch_residual(s->stream, s->Pbuf(pc), s->drec.p, s->dcc.p, s->dlam_buf(lc), uint32_t(m.d_v0.size()),
+                            s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()), dt_sub, s->split_compliance,
+                            s->resid_part.p, s->ws_res.p, s->ws_gate.p, s->ws_threshold, nullptr);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(s->gate_host, s->ws_gate.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream);
+}
+
+// One fallback chunk of `count` iterations (gated on the device flag as well), then, unless it
+// is the last chunk of the budget, the next residual check and the flag's copy to the host.  The
+// same kernels in the same order as enqueue_gated_fallback, minus those the gate would skip.
+hipError_t enqueue_fallback_chunk(tc_sim *s, float dt_sub, uint32_t count, bool last, uint32_t &pc, uint32_t &lc) {
+    const tc::Mesh &m = s->mesh;
+    tc::SubstepArgs fa = s->args(dt_sub);
+    fa.run_flag = s->ws_gate.p;
+    hipError_t e = enqueue_iterations(s, fa, count, false, pc, lc);
+    if (e != hipSuccess || last) return e;
+    e = tc::launch_residual(s->stream, s->Pbuf(pc), s->drec.p, s->dcc.p, s->dlam_buf(lc), uint32_t(m.d_v0.size()),
+                            s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()), dt_sub, s->split_compliance,
+                            s->resid_part.p, s->ws_res.p, s->ws_gate.p, s->ws_threshold, s->ws_gate.p);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(s->gate_host, s->ws_gate.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream);
+}
+
+// The substep after the projection loop: the warm start's bookkeeping, colliders, self-collision
+// and the velocity update, on the buffers (pc, lc) that hold the latest positions and lambdas.
+hipError_t enqueue_substep_tail(tc_sim *s, float dt_sub, uint32_t pc, uint32_t lc, bool collide, bool self_collide,
+                                bool update) {
+    const tc::Mesh &m = s->mesh;
+    const uint32_t n = m.nv, B = s->block;
     float4 *Pc = s->Pbuf(pc);
     s->last_p = pc;
     s->last_lam = lc;
-    if (track && (e = ws_post(s, dt_sub, Pc)) != hipSuccess) return e;
+    hipError_t e = hipSuccess;
+    if (s->ws_tracking() && (e = ws_post(s, dt_sub, Pc)) != hipSuccess) return e;
     if (collide && (!s->cols_h.empty() || !s->tris_h.empty() || s->sdf_nb > 0)) {
         e = tc::launch_collision(s->stream, Pc, s->X.p, n, s->col_order.p, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
                                  s->tri_bounds.p, uint32_t(s->tris_h.size()), s->tri_cbox.p, s->single_sided_recovery, dt_sub, B, s->sdf_args());
@@ -142,6 +175,13 @@ hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_coll
     }
     if (update) return tc::launch_update_vel(s->stream, Pc, s->X.p, s->V.p, n, dt_sub, B);
     return hipSuccess;
+}
+
+hipError_t enqueue_substep(tc_sim *s, float dt_sub, bool collide, bool self_collide, bool update) {
+    uint32_t pc = 0, lc = 0;
+    hipError_t e = enqueue_substep_head(s, dt_sub, false, pc, lc);
+    if (e != hipSuccess) return e;
+    return enqueue_substep_tail(s, dt_sub, pc, lc, collide, self_collide, update);
 }
 
 // Picks the training-sample slot the next substep writes (recording only).
@@ -161,6 +201,64 @@ hipError_t enqueue_frame(tc_sim *s, float dt, uint32_t substeps) {
     return hipSuccess;
 }
 
+// Replays `pg` (captured from `enqueue` when its key changes).
+template <class Enqueue>
+int replay_part(tc_sim *s, tc_sim::PartGraph &pg, const uint64_t key[4], Enqueue enqueue) {
+    if (!pg.x || std::memcmp(key, pg.key, sizeof(pg.key)) != 0) {
+        pg.clear();
+        TC_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = enqueue();
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+        if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return fail(TC_ERR_RUNTIME, std::string("capture: ") + hipGetErrorString(e)); }
+        TC_HIP(e2);
+        pg.g = g;
+        TC_HIP(hipGraphInstantiate(&pg.x, pg.g, nullptr, nullptr, 0));
+        std::memcpy(pg.key, key, sizeof(pg.key));
+    }
+    TC_HIP(hipGraphLaunch(pg.x, s->stream));
+    return TC_OK;
+}
+
+// A frame of warm-started substeps with a fallback, from graphs of the substeps' parts.  Captured
+// whole, a substep carries every fallback chunk, and a chunk whose gate has passed still costs its
+// launches (about 2.7 us each for the per-iteration kernel, 56 of them); here the host waits for the
+// first check and launches chunks only while the gate stays open.  The kernels that run, and their
+// order, are those of the captured frame.
+int run_frame_host_gated(tc_sim *s, float dt, uint32_t substeps) {
+    if (!s->gate_host) TC_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->gate_host), 16, hipHostMallocDefault));
+    const float dt_sub = dt / float(substeps);
+    uint32_t dtb;
+    std::memcpy(&dtb, &dt_sub, 4);
+    const uint64_t fused = s->fused_active() ? 1u : 0u;
+    const uint32_t chunk = std::max<uint32_t>(2u, s->ws_iters + (s->ws_iters & 1u));
+    uint32_t p0 = 0, l0 = 0;
+    for (uint32_t k = 0; k < substeps; ++k) {
+        const uint64_t kh[4] = {dtb, s->ws_iters, fused, s->epoch};
+        int r = replay_part(s, s->pg_head, kh, [&] { return enqueue_substep_head(s, dt_sub, true, p0, l0); });
+        if (r != TC_OK) return r;
+        // the buffers holding P and lambda after the warm iterations (chunks have an even size)
+        p0 = l0 = (fused && s->ws_iters) ? (s->ws_iters & 1u) : 0u;
+        TC_HIP(hipStreamSynchronize(s->stream));
+        for (uint32_t done = 0; done < s->ws_fallback && *s->gate_host != 0u;) {
+            const uint32_t cnt = std::min(chunk, s->ws_fallback - done);
+            const bool last = done + cnt >= s->ws_fallback;
+            const uint64_t kc[4] = {dtb, cnt, fused | (uint64_t(p0) << 1), s->epoch};
+            uint32_t pc = p0, lc = l0;
+            r = replay_part(s, last ? s->pg_last : s->pg_chunk, kc,
+                            [&] { return enqueue_fallback_chunk(s, dt_sub, cnt, last, pc, lc); });
+            if (r != TC_OK) return r;
+            done += cnt;
+            if (last) break;
+            TC_HIP(hipStreamSynchronize(s->stream));
+        }
+        const uint64_t kt[4] = {dtb, uint64_t(p0) | (uint64_t(l0) << 1), fused, s->epoch};
+        r = replay_part(s, s->pg_tail, kt, [&] { return enqueue_substep_tail(s, dt_sub, p0, l0, true, true, true); });
+        if (r != TC_OK) return r;
+    }
+    return TC_OK;
+}
+
 int run_frame(tc_sim *s, float dt, uint32_t substeps) {
     if (s->self_collision && s->normal_untangling && s->mesh.has_multiple_layers)
         return fail(TC_ERR_UNSUPPORTED,
@@ -178,6 +276,7 @@ int run_frame(tc_sim *s, float dt, uint32_t substeps) {
         TC_HIP(enqueue_frame(s, dt, substeps));
         return TC_OK;
     }
+    if (s->ws_active() && s->ws_fallback > 0 && iters > 0) return run_frame_host_gated(s, dt, substeps);
     uint32_t dtb;
     std::memcpy(&dtb, &dt, 4);
     const uint64_t key[4] = {dtb, substeps, uint64_t(iters) | (uint64_t(s->fused_active()) << 32), s->epoch};

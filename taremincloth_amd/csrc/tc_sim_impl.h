@@ -179,12 +179,32 @@ struct tc_sim {
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     uint64_t graph_key[4] = {0, 0, 0, 0};
+    // Warm start with a fallback, replayed from graphs: the host reads the gate after the warm
+    // iterations and launches fallback chunks only when they run (graphs of the substep's parts).
+    struct PartGraph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        uint64_t key[4] = {0, 0, 0, 0};
+        void clear() {
+            if (x) (void)hipGraphExecDestroy(x);
+            if (g) (void)hipGraphDestroy(g);
+            x = nullptr;
+            g = nullptr;
+        }
+    };
+    PartGraph pg_head, pg_chunk, pg_last, pg_tail;
+    uint32_t *gate_host = nullptr;  // pinned host copy of the gate's fallback flag
     uint64_t epoch = 1;  // bumped whenever a parameter or buffer baked into the graph changes
     bool use_graph = true;  // TCLOTH_USE_GRAPH=0 launches kernel by kernel (profilers)
 
     ~tc_sim() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
+        pg_head.clear();
+        pg_chunk.clear();
+        pg_last.clear();
+        pg_tail.clear();
+        if (gate_host) (void)hipHostFree(gate_host);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void touch() { ++epoch; }
