@@ -9,10 +9,10 @@
 //
 // Hash build, vertices and edge midpoints together: count (atomics) -> exclusive
 // scan of each table (one workgroup per row of 1024 entries, one over the row
-// totals, one add) -> scatter (atomic cursor) -> per-bucket sort, so every bucket lists its elements in ascending index order.  The
-// reference's scatter order inside a bucket depends on atomic timing; the sorted
-// order makes the hash tables, and with them the E-E search cut-off,
-// deterministic.
+// totals, one add) -> scatter (atomic cursor) -> per-bucket sort, so every
+// bucket lists its elements in ascending index order.  The reference's scatter
+// order inside a bucket depends on atomic timing; the sorted order makes the
+// hash tables, and with them the E-E search cut-off, deterministic.
 //
 // The solve kernels accumulate displacements in 1e-6 m fixed point with
 // integer atomics, which makes the result independent of thread order.
