@@ -229,6 +229,48 @@ def test_gpu_gate_stops_after_the_chunk_that_passes(require_gpu):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [24, 128])
+def test_gpu_host_read_gate_equals_the_device_gated_frame(require_gpu, n, monkeypatch):
+    """With graph replay, a warm-started frame reads the gate on the host and launches fallback
+    chunks only while it stays open; without (TCLOTH_USE_GRAPH=0) every chunk is enqueued and
+    gated on the device. Both run the same kernels in the same order: 3 frames of 10 substeps
+    from a settled drape, with the threshold between the residuals after 4 and 8 iterations,
+    give the same positions bit for bit and the same gate statistics. n = 128 runs the
+    one-launch-per-iteration schedule, n = 24 the per-colour one."""
+    dt_sub = 1.0 / 600.0
+
+    def settled():
+        sim, pos, *_ = _drape(n)
+        for _ in range(6):
+            sim.step(1.0 / 60.0, 10, 16)
+        return sim, pos
+
+    monkeypatch.setenv("TCLOTH_USE_GRAPH", "0")
+    res = {}
+    for its in (4, 8):
+        sim, pos = settled()
+        sim.set_solver_iterations(its)
+        sim.substep_parts(dt_sub, collide=False, self_collide=False, update=False)
+        res[its], _ = sim.residual(dt_sub)
+    th = float(np.sqrt(res[4] * res[8]))
+    outs, stats = {}, {}
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("TCLOTH_USE_GRAPH", graphs)
+        sim, pos = settled()
+        sim.set_solver_iterations(4)
+        sim.set_warm_start(None, 1, 4, 12, th, history_weight=0.5)
+        for _ in range(3):
+            sim.step(1.0 / 60.0, 10)
+        out = np.zeros(pos.size, np.float32)
+        sim.get_positions(out)
+        outs[graphs] = out
+        st = sim.warm_start_stats()
+        stats[graphs] = (st["checked"], st["fallbacks"], st["fallback_chunks"])
+    assert stats["1"] == stats["0"] and stats["1"][0] == 30 and stats["1"][1] > 0
+    assert np.array_equal(outs["1"], outs["0"])
+
+
 def _stretch_graph(pos, edges, faces, inv):
     from oracle import oracle as O
     m = O.build_mesh(pos, edges, faces, inv)
