@@ -43,7 +43,7 @@ SDF_FLOPS_PER_TEST = 60         # point-triangle closest point + squared distanc
 MEASURED_TRAFFIC = {
     (512, 64, 1024, 1024): {"fetch_kib": 23841.0, "write_kib": 10603.4, "fetch_factor": 2.0, "write_factor": 1.0},
 }
-# The same for one launch of the message-passing round layer (k_mlp_fwd<256,128>), keyed by grid.
+# The same for one launch of the message-passing round kernel (k_round_fwd), keyed by grid.
 MEASURED_TRAFFIC_ROUND = {}
 
 
@@ -497,16 +497,26 @@ def main():
 
     plain_sps, learned_sps = args.steps / plain_s, args.steps / learned_s
 
-    # rooflines: the round layer of the network (MFMA) and the plain solver's per-iteration kernel (HBM)
+    # rooflines: one message-passing round of the network and the plain solver's per-iteration
+    # kernel.  A round reads each row of h once and writes h' once (the neighbour rows it gathers
+    # are rows already counted); its MFMA work is the two dense layers.  Its bound is whichever
+    # of the two takes longer at peak.
     R = int(wt["rounds"])
     round_flops = nv * 2.0 * (2 * HIDDEN * HIDDEN + HIDDEN * HIDDEN)
-    round_ms = wt["round_mlps"] / max(R, 1)
-    net_rl = {"bound": "mfma", "achieved": round_flops / (round_ms * 1e-3) / 1e12, "peak": BF16_PEAK_TFLOPS,
-              "unit": "TFLOP/s", "traffic": MEASURED_TRAFFIC_ROUND.get(n),
-              "kernel": "k_mlp_fwd<256,128> (one message-passing round: [h, neighbour mean] -> 128 -> 128, "
-                        "BF16 MFMA, FP32 accumulate)",
-              "flops_per_launch": round_flops, "launch_ms": round_ms}
-    net_rl["frac"] = net_rl["achieved"] / BF16_PEAK_TFLOPS
+    round_bytes = nv * 2.0 * HIDDEN * 2
+    round_ms = wt["round_layers"] / max(R, 1)
+    mfma_tflops = round_flops / (round_ms * 1e-3) / 1e12
+    hbm_gbs = round_bytes / (round_ms * 1e-3) / 1e9
+    round_kernel = ("k_round_fwd (one message-passing round: neighbour mean gathered in-kernel, "
+                    "[h, mean] -> 128 -> 128 + residual, BF16 MFMA, FP32 accumulate)")
+    if round_bytes / (HBM_PEAK_GBS * 1e9) >= round_flops / (BF16_PEAK_TFLOPS * 1e12):
+        net_rl = {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": hbm_gbs / HBM_PEAK_GBS, "mfma_tflops": mfma_tflops}
+    else:
+        net_rl = {"bound": "mfma", "achieved": mfma_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                  "frac": mfma_tflops / BF16_PEAK_TFLOPS, "hbm_gbs": hbm_gbs}
+    net_rl.update({"traffic": MEASURED_TRAFFIC_ROUND.get(n), "kernel": round_kernel, "bytes_per_launch": round_bytes,
+                   "flops_per_launch": round_flops, "launch_ms": round_ms})
     sweep_bytes = nd * BYTES_DIST + nb * BYTES_BEND
     if info["path"] == 2:
         launch_ms = kt["fused"] / max(kt["fused_launches"], 1.0)
@@ -528,7 +538,7 @@ def main():
     # the dominant kernel of the learned substep by time: its solver iterations are the warm
     # iterations plus the fallback iterations, in the share of substeps whose gate failed
     fb_share = stats["fallbacks"] / max(stats["checked"], 1)
-    roofline = net_rl if wt["round_mlps"] >= wt["iterations"] + wt["fallback"] * fb_share else solver_rl
+    roofline = net_rl if wt["round_layers"] >= wt["iterations"] + wt["fallback"] * fb_share else solver_rl
 
     sdf = None if args.no_sdf else sdf_bake_bench(T, not args.no_cpu_baseline)
     batch = None if args.no_batch else batch_bench(T, 30, 5, not args.no_cpu_baseline)

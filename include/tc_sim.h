@@ -271,10 +271,10 @@ int tc_sim_ws_last_output(tc_sim *sim, float *y, size_t count);
 int tc_sim_ws_history(tc_sim *sim, float *get_xyz, const float *set_xyz, uint32_t n);
 /* Runs `reps` warm-started substeps kernel by kernel with HIP events on the simulator's stream
  * and writes per-substep averages (ms) to out[16]: [0] predict [1] features [2] encoder
- * [3] neighbour means (all rounds) [4] round MLPs (all rounds) [5] decoder [6] apply
- * [7] warm-start iterations [8] residual gate [9] fallback launches [10] correction update
- * [11] collider resolve + self-collision [12] update_vel [13] whole substep [14] rounds
- * [15] warm-start iterations.  The simulation advances by `reps` substeps. */
+ * [3] message-passing rounds (all rounds) [4] decoder [5] apply [6] warm-start iterations
+ * [7] residual gate [8] fallback launches [9] correction update [10] collider resolve +
+ * self-collision [11] update_vel [12] whole substep [13] rounds [14] warm-start iterations
+ * ([15] unused).  The simulation advances by `reps` substeps. */
 int tc_sim_ws_time(tc_sim *sim, float dt_sub, uint32_t reps, double *out);
 
 /* ---- Mesh SDF bake (module functions bake_mesh_sdf_gpu / bake_mesh_sdf_hierarchical_gpu,

@@ -69,7 +69,7 @@ struct tc_ws_net {
     hipStream_t stream = nullptr;
     // training scratch for up to rows_cap rows
     uint32_t rows_cap = 0;
-    DevBuf<tc::ws::bf16> h[5], x_r[4], a_r[4], a_enc, a_dec, dyb, dzb, feat, agg;
+    DevBuf<tc::ws::bf16> h[5], x_r[4], a_r[4], a_enc, a_dec, dyb, dzb, feat;
     DevBuf<float> y, dy, dh0, dh1, dx, partial;
     DevBuf<double> loss_part;
     DevBuf<float4> tgt;
@@ -162,7 +162,7 @@ struct tc_sim {
     std::vector<uint32_t> ws_goff_h, ws_gnbr_h;
     float ws_lscale = 0.01f;
     DevBuf<float4> ws_ppred, ws_D;
-    DevBuf<tc::ws::bf16> ws_feat, ws_h0, ws_h1, ws_agg;
+    DevBuf<tc::ws::bf16> ws_feat, ws_h0, ws_h1;
     DevBuf<float> ws_y;
     DevBuf<uint32_t> ws_gate;          // [fallback flag, substeps checked, fallbacks, fallback chunks run]
     DevBuf<double> ws_res;             // residual at the gate: sum h^2, sum L0^2, max, relative

@@ -179,36 +179,11 @@ __global__ void k_update_prev(const float4 *__restrict__ P, const float4 *__rest
 }
 
 // ---------------------------------------------------------------- forward
-enum { M_ENC = 0, M_ROUND = 1, M_DEC = 2 };
-
-// agg[i] = mean over the neighbours j of h[j] (FP32 sum in neighbour order, then BF16);
-// one thread per row and 8-column chunk.
-__global__ void __launch_bounds__(256) k_mean_agg(uint32_t rows, const bf16 *__restrict__ h,
-                                                  const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr,
-                                                  bf16 *__restrict__ agg) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= rows * (H / 8)) return;
-    const uint32_t row = q / (H / 8), ch = q % (H / 8);
-    float acc[8];
-    for (int i = 0; i < 8; ++i) acc[i] = 0.0f;
-    const uint32_t b = off[row], e = off[row + 1];
-    for (uint32_t k = b; k < e; ++k) {
-        const bf16x8 u = *reinterpret_cast<const bf16x8 *>(&h[size_t(nbr[k]) * H + ch * 8]);
-        for (int i = 0; i < 8; ++i) acc[i] += (float)u[i];
-    }
-    if (e > b) {
-        const float inv = 1.0f / float(e - b);
-        for (int i = 0; i < 8; ++i) acc[i] *= inv;
-    }
-    bf16x8 m;
-    for (int i = 0; i < 8; ++i) m[i] = (bf16)acc[i];
-    *reinterpret_cast<bf16x8 *>(&agg[size_t(row) * H + ch * 8]) = m;
-}
+enum { M_ENC = 0, M_DEC = 2 };
 
 struct MlpFwd {
     uint32_t rows, ntiles;
-    const bf16 *xin;     // ENC: features (rows x F); ROUND / DEC: h (rows x H)
-    const bf16 *xagg;    // ROUND: mean of the neighbours' h (rows x H)
+    const bf16 *xin;     // ENC: features (rows x F); DEC: h (rows x H)
     const bf16 *w1t;
     const float *b1;
     const bf16 *w2t;
@@ -229,10 +204,6 @@ __device__ __forceinline__ bf16x8 load_x_chunk(const MlpFwd &a, uint32_t r0, int
     const int rr = q / (K1 / 8), ch = q % (K1 / 8);
     const uint32_t row = r0 + rr;
     if (row >= a.rows) return zero8();
-    if (MODE == M_ROUND) {
-        const bf16 *src = ch < H / 8 ? a.xin + size_t(row) * H + ch * 8 : a.xagg + size_t(row) * H + (ch - H / 8) * 8;
-        return *reinterpret_cast<const bf16x8 *>(src);
-    }
     return *reinterpret_cast<const bf16x8 *>(&a.xin[size_t(row) * K1 + ch * 8]);
 }
 
@@ -352,13 +323,6 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
                     const float4 bias = *reinterpret_cast<const float4 *>(&b2s[col]);
                     float v0 = acc2[rt][ct][0] + bias.x, v1 = acc2[rt][ct][1] + bias.y, v2 = acc2[rt][ct][2] + bias.z,
                           v3 = acc2[rt][ct][3] + bias.w;
-                    if (MODE == M_ROUND) {
-                        const bf16x4 hs = *reinterpret_cast<const bf16x4 *>(&xs[rr * LK + col]);
-                        v0 = (float)hs[0] + v0;
-                        v1 = (float)hs[1] + v1;
-                        v2 = (float)hs[2] + v2;
-                        v3 = (float)hs[3] + v3;
-                    }
                     bf16x4 o;
                     o[0] = (bf16)v0; o[1] = (bf16)v1; o[2] = (bf16)v2; o[3] = (bf16)v3;
                     *reinterpret_cast<bf16x4 *>(&as[rr * LH + col]) = o;
@@ -377,6 +341,220 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
                 if (row2 < a.rows) *reinterpret_cast<bf16x8 *>(&a.hout[size_t(row2) * H + ch * 8]) = ov[u];
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------- message-passing round
+// h' = h + W2 relu(W1 [h, m] + b1) + b2 with m[i] = mean over the neighbours j of h[j],
+// in one persistent launch over tiles of 64 rows.  The mean is formed while the tile's
+// input is staged (FP32 sum in neighbour order, times 1/deg, rounded to BF16), so it is
+// never written to or read back from HBM.
+//
+// 512 threads, 8 waves: wave w owns the 32 output columns 32*(w&3).. of both layers and
+// the 32 rows 32*(w>>2).. of the tile, so its weight fragments (96 registers) leave room
+// for two waves per SIMD.  Thread q stages the 16-byte column chunk q&15 of rows q>>4 and
+// (q>>4)+32.  The neighbour gather runs three tiles ahead in three stages, one per
+// tile: the CSR range, then the neighbour ids, then the rows themselves, so every load
+// depends only on data that arrived during an earlier tile's compute.  The rows are
+// requested right after staging; the ids and ranges after the tile's output stores, so
+// that waiting for them never waits for those stores.  Up to RND_DMAX neighbours per row
+// are prefetched; any beyond are read when the tile is staged.
+struct RoundFwd {
+    uint32_t rows, ntiles;
+    const bf16 *h;               // rows x H
+    const uint32_t *off, *nbr;   // neighbour lists (CSR)
+    const bf16 *w1t;
+    const float *b1;
+    const bf16 *w2t;
+    const float *b2;
+    bf16 *hout;
+    bf16 *xsave, *asave;         // training saves of [h, m] and of the hidden activation (or null)
+};
+constexpr int RND_THREADS = 512;
+constexpr int RND_DMAX = 6;
+constexpr int RND_U = TR * (H / 8) / RND_THREADS;  // staged (row, chunk) pairs per thread
+constexpr size_t round_lds() { return size_t(2) * (TR * (2 * H + 8) + TR * (H + 8)); }
+
+__global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int K1 = 2 * H, LK = K1 + 8, LH = H + 8, KS1 = K1 / 32, KS2 = H / 32;
+    bf16 *xs = reinterpret_cast<bf16 *>(smem);  // TR x LK: [h, m]
+    bf16 *as = xs + TR * LK;                    // TR x LH: hidden, then output
+    __shared__ __align__(16) float b1s[H], b2s[H];
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, g = l >> 4, c = l & 15;
+    const int cg = w & 3, rg = w >> 2;
+    bf16x8 wf1[2][KS1], wf2[2][KS2];
+    for (int ct = 0; ct < 2; ++ct) {
+        const int n = 32 * cg + 16 * ct + c;
+        for (int ks = 0; ks < KS1; ++ks)
+            wf1[ct][ks] = *reinterpret_cast<const bf16x8 *>(&a.w1t[n * K1 + ks * 32 + g * 8]);
+        for (int ks = 0; ks < KS2; ++ks)
+            wf2[ct][ks] = *reinterpret_cast<const bf16x8 *>(&a.w2t[n * H + ks * 32 + g * 8]);
+    }
+    if (tid < H) {
+        b1s[tid] = a.b1[tid];
+        b2s[tid] = a.b2[tid];
+    }
+    const int qc = tid & 15, qr = tid >> 4;
+    const uint32_t G = gridDim.x;
+    // stage A: CSR range; stage B: neighbour ids; stage C: own row chunk and neighbour chunks.
+    // Every load is issued unconditionally from an address that is always valid (row 0 for a
+    // row past the end, the row itself for a neighbour slot past its degree), and a stage
+    // only selects on the previous stage's values once they have arrived, so no load sits
+    // in a branch and the compiler waits only for the data it is about to use.
+    uint32_t sa_b[RND_U], sa_e[RND_U], sa_ok = 0;
+    uint32_t sb_id[RND_U][RND_DMAX], sb_b[RND_U], sb_n[RND_U];
+    uint32_t sc_b[RND_U], sc_n[RND_U], sc_ok = 0;
+    bf16x8 sc_self[RND_U], sc_nb[RND_U][RND_DMAX];
+    auto stage_a = [&](uint32_t t) {
+        sa_ok = 0;
+        for (int u = 0; u < RND_U; ++u) {
+            const uint32_t row = t * TR + qr + 32 * u;
+            const bool ok = t < a.ntiles && row < a.rows;
+            const uint32_t r = ok ? row : 0u;
+            sa_b[u] = a.off[r];
+            sa_e[u] = a.off[r + 1];
+            sa_ok |= uint32_t(ok) << u;
+        }
+    };
+    auto stage_b = [&]() {
+        for (int u = 0; u < RND_U; ++u) {
+            const bool ok = (sa_ok >> u) & 1u;
+            const uint32_t b0 = ok ? sa_b[u] : 0u, n = ok ? sa_e[u] - sa_b[u] : 0u;
+            sb_b[u] = b0;
+            sb_n[u] = n;
+            for (int k = 0; k < RND_DMAX; ++k) sb_id[u][k] = a.nbr[uint32_t(k) < n ? b0 + k : 0u];
+        }
+    };
+    auto stage_c = [&](uint32_t t) {
+        sc_ok = 0;
+        for (int u = 0; u < RND_U; ++u) {
+            const uint32_t row = t * TR + qr + 32 * u;
+            const bool ok = t < a.ntiles && row < a.rows;
+            const uint32_t r = ok ? row : 0u;
+            sc_ok |= uint32_t(ok) << u;
+            sc_b[u] = sb_b[u];
+            sc_n[u] = sb_n[u];
+            sc_self[u] = *reinterpret_cast<const bf16x8 *>(&a.h[size_t(r) * H + qc * 8]);
+            for (int k = 0; k < RND_DMAX; ++k) {
+                const uint32_t j = uint32_t(k) < sb_n[u] ? sb_id[u][k] : r;
+                sc_nb[u][k] = *reinterpret_cast<const bf16x8 *>(&a.h[size_t(j) * H + qc * 8]);
+            }
+        }
+    };
+    uint32_t t = blockIdx.x;
+    stage_a(t);
+    stage_b();
+    stage_c(t);
+    stage_a(t + G);
+    stage_b();
+    stage_a(t + 2 * G);
+    for (; t < a.ntiles; t += G) {
+        const uint32_t r0 = t * TR;
+        // the previous tile read `xs` last before its final barrier, so staging needs none
+        for (int u = 0; u < RND_U; ++u) {
+            const int rr = qr + 32 * u;
+            float acc[8];
+            for (int i = 0; i < 8; ++i) acc[i] = 0.0f;
+            const uint32_t n = sc_n[u];
+            for (int k = 0; k < RND_DMAX; ++k)
+                if (uint32_t(k) < n)
+                    for (int i = 0; i < 8; ++i) acc[i] += (float)sc_nb[u][k][i];
+            for (uint32_t k = RND_DMAX; k < n; ++k) {
+                const bf16x8 v = *reinterpret_cast<const bf16x8 *>(&a.h[size_t(a.nbr[sc_b[u] + k]) * H + qc * 8]);
+                for (int i = 0; i < 8; ++i) acc[i] += (float)v[i];
+            }
+            if (n > 0) {
+                const float inv = 1.0f / float(n);
+                for (int i = 0; i < 8; ++i) acc[i] *= inv;
+            }
+            bf16x8 m;
+            for (int i = 0; i < 8; ++i) m[i] = (bf16)acc[i];
+            *reinterpret_cast<bf16x8 *>(&xs[rr * LK + qc * 8]) = (sc_ok >> u) & 1u ? sc_self[u] : zero8();
+            *reinterpret_cast<bf16x8 *>(&xs[rr * LK + H + qc * 8]) = m;
+        }
+        stage_c(t + G);
+        __syncthreads();
+        if (a.xsave) {
+            // constant trip counts: a store loop of unknown length would make every later
+            // wait for an earlier load a wait for all memory operations
+            for (int u = 0; u < TR * (K1 / 8) / RND_THREADS; ++u) {
+                const int q = tid + RND_THREADS * u, rr = q / (K1 / 8), ch = q % (K1 / 8);
+                const uint32_t row = r0 + rr;
+                if (row < a.rows)
+                    *reinterpret_cast<bf16x8 *>(&a.xsave[size_t(row) * K1 + ch * 8]) =
+                        *reinterpret_cast<const bf16x8 *>(&xs[rr * LK + ch * 8]);
+            }
+        }
+        // layer 1: hidden units 32cg + 16ct + 4g + j of rows 32rg + 16rt + c
+        for (int rt = 0; rt < 2; ++rt) {
+            const int rr = 32 * rg + 16 * rt + c;
+            f32x4 acc[2];
+            for (int ct = 0; ct < 2; ++ct) acc[ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            bf16x8 xf[KS1];
+            for (int ks = 0; ks < KS1; ++ks) xf[ks] = *reinterpret_cast<const bf16x8 *>(&xs[rr * LK + ks * 32 + g * 8]);
+            for (int ks = 0; ks < KS1; ++ks)
+                for (int ct = 0; ct < 2; ++ct) acc[ct] = mfma(wf1[ct][ks], xf[ks], acc[ct]);
+            for (int ct = 0; ct < 2; ++ct) {
+                const int col = 32 * cg + 16 * ct + 4 * g;
+                const float4 bias = *reinterpret_cast<const float4 *>(&b1s[col]);
+                bf16x4 v;
+                v[0] = (bf16)fmaxf(acc[ct][0] + bias.x, 0.0f);
+                v[1] = (bf16)fmaxf(acc[ct][1] + bias.y, 0.0f);
+                v[2] = (bf16)fmaxf(acc[ct][2] + bias.z, 0.0f);
+                v[3] = (bf16)fmaxf(acc[ct][3] + bias.w, 0.0f);
+                *reinterpret_cast<bf16x4 *>(&as[rr * LH + col]) = v;
+            }
+        }
+        __syncthreads();
+        if (a.asave) {
+            for (int u = 0; u < TR * (H / 8) / RND_THREADS; ++u) {
+                const int q = tid + RND_THREADS * u, rr = q / (H / 8), ch = q % (H / 8);
+                const uint32_t row = r0 + rr;
+                if (row < a.rows)
+                    *reinterpret_cast<bf16x8 *>(&a.asave[size_t(row) * H + ch * 8]) =
+                        *reinterpret_cast<const bf16x8 *>(&as[rr * LH + ch * 8]);
+            }
+        }
+        // layer 2 and the residual
+        f32x4 acc2[2][2];
+        for (int rt = 0; rt < 2; ++rt) {
+            bf16x8 af[KS2];
+            for (int ks = 0; ks < KS2; ++ks)
+                af[ks] = *reinterpret_cast<const bf16x8 *>(&as[(32 * rg + 16 * rt + c) * LH + ks * 32 + g * 8]);
+            for (int ct = 0; ct < 2; ++ct) acc2[rt][ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int ks = 0; ks < KS2; ++ks)
+                for (int ct = 0; ct < 2; ++ct) acc2[rt][ct] = mfma(wf2[ct][ks], af[ks], acc2[rt][ct]);
+        }
+        __syncthreads();  // every wave of this row group has read `as`
+        for (int rt = 0; rt < 2; ++rt) {
+            const int rr = 32 * rg + 16 * rt + c;
+            for (int ct = 0; ct < 2; ++ct) {
+                const int col = 32 * cg + 16 * ct + 4 * g;
+                const float4 bias = *reinterpret_cast<const float4 *>(&b2s[col]);
+                const bf16x4 hs = *reinterpret_cast<const bf16x4 *>(&xs[rr * LK + col]);
+                bf16x4 o;
+                o[0] = (bf16)((float)hs[0] + (acc2[rt][ct][0] + bias.x));
+                o[1] = (bf16)((float)hs[1] + (acc2[rt][ct][1] + bias.y));
+                o[2] = (bf16)((float)hs[2] + (acc2[rt][ct][2] + bias.z));
+                o[3] = (bf16)((float)hs[3] + (acc2[rt][ct][3] + bias.w));
+                *reinterpret_cast<bf16x4 *>(&as[rr * LH + col]) = o;
+            }
+        }
+        __syncthreads();
+        constexpr int OCH = TR * (H / 8) / RND_THREADS;
+        bf16x8 ov[OCH];
+        for (int u = 0; u < OCH; ++u) {
+            const int q = tid + RND_THREADS * u, r2 = q / (H / 8), ch = q % (H / 8);
+            ov[u] = *reinterpret_cast<const bf16x8 *>(&as[r2 * LH + ch * 8]);
+        }
+        for (int u = 0; u < OCH; ++u) {
+            const int q = tid + RND_THREADS * u, r2 = q / (H / 8), ch = q % (H / 8);
+            const uint32_t row2 = r0 + r2;
+            if (row2 < a.rows) *reinterpret_cast<bf16x8 *>(&a.hout[size_t(row2) * H + ch * 8]) = ov[u];
+        }
+        stage_b();
+        stage_a(t + 3 * G);
     }
 }
 
@@ -650,12 +828,11 @@ constexpr uint32_t PERSIST = 256;  // one workgroup per CU for the LDS-heavy lay
 
 template <int K1, int N2, int MODE>
 hipError_t fwd_block(hipStream_t st, const NetDev &nd, const Layer &l1, const Layer &l2, uint32_t rows,
-                     const bf16 *xin, const bf16 *xagg, bf16 *hout, float *yout, bf16 *xsave, bf16 *asave) {
+                     const bf16 *xin, bf16 *hout, float *yout, bf16 *xsave, bf16 *asave) {
     MlpFwd a;
     a.rows = rows;
     a.ntiles = cdiv(rows, TR);
     a.xin = xin;
-    a.xagg = xagg;
     a.w1t = nd.wt + l1.off;
     a.b1 = nd.theta + l1.off + l1.K * l1.N;
     a.w2t = nd.wt + l2.off;
@@ -665,9 +842,28 @@ hipError_t fwd_block(hipStream_t st, const NetDev &nd, const Layer &l1, const La
     a.xsave = xsave;
     a.asave = asave;
     const size_t lds = fwd_lds<K1, N2>();
-    // workgroups per CU of the persistent grid (measured: more resident workgroups are slower)
-    const uint32_t per_cu = MODE == M_ROUND ? 1u : 2u;
-    hipLaunchKernelGGL((k_mlp_fwd<K1, N2, MODE>), dim3(std::min(a.ntiles, PERSIST * per_cu)), dim3(256), lds, st, a);
+    // two workgroups per CU of the persistent grid
+    hipLaunchKernelGGL((k_mlp_fwd<K1, N2, MODE>), dim3(std::min(a.ntiles, PERSIST * 2)), dim3(256), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t round_fwd(hipStream_t st, const NetDev &nd, const Layer &l1, const Layer &l2, const Graph &g,
+                     uint32_t rows, const bf16 *h, bf16 *hout, bf16 *xsave, bf16 *asave) {
+    RoundFwd a;
+    a.rows = rows;
+    a.ntiles = cdiv(rows, TR);
+    a.h = h;
+    a.off = g.off;
+    a.nbr = g.nbr;
+    a.w1t = nd.wt + l1.off;
+    a.b1 = nd.theta + l1.off + l1.K * l1.N;
+    a.w2t = nd.wt + l2.off;
+    a.b2 = nd.theta + l2.off + l2.K * l2.N;
+    a.hout = hout;
+    a.xsave = xsave;
+    a.asave = asave;
+    // one workgroup (8 waves) per CU: the weight fragments take about half of each wave's registers
+    hipLaunchKernelGGL(k_round_fwd, dim3(std::min(a.ntiles, PERSIST)), dim3(RND_THREADS), round_lds(), st, a);
     return hipGetLastError();
 }
 
@@ -712,7 +908,7 @@ hipError_t allow_lds(Kern k, size_t bytes) {
 hipError_t set_lds_limits() {
     hipError_t e;
     if ((e = allow_lds(&k_mlp_fwd<F, H, M_ENC>, fwd_lds<F, H>())) != hipSuccess) return e;
-    if ((e = allow_lds(&k_mlp_fwd<2 * H, H, M_ROUND>, fwd_lds<2 * H, H>())) != hipSuccess) return e;
+    if ((e = allow_lds(&k_round_fwd, round_lds())) != hipSuccess) return e;
     if ((e = allow_lds(&k_mlp_fwd<H, O, M_DEC>, fwd_lds<H, O>())) != hipSuccess) return e;
     if ((e = allow_lds(&k_mlp_bwd<H, O>, bwd_lds<H, O>())) != hipSuccess) return e;
     if ((e = allow_lds(&k_mlp_bwd<2 * H, H>, bwd_lds<2 * H, H>())) != hipSuccess) return e;
@@ -753,20 +949,15 @@ hipError_t forward(hipStream_t st, const NetLayout &nl, const NetDev &nd, const 
     int k = 0;
     auto mark = [&]() -> hipError_t { return ev ? hipEventRecord(ev[k++], st) : hipSuccess; };
     if ((e = mark()) != hipSuccess) return e;
-    if ((e = fwd_block<F, H, M_ENC>(st, nd, nl.L[nl.enc1()], nl.L[nl.enc2()], rows, b.feat, nullptr, b.h[0], nullptr,
+    if ((e = fwd_block<F, H, M_ENC>(st, nd, nl.L[nl.enc1()], nl.L[nl.enc2()], rows, b.feat, b.h[0], nullptr,
                                     nullptr, b.training ? b.a_enc : nullptr)) != hipSuccess) return e;
     if ((e = mark()) != hipSuccess) return e;
     for (uint32_t r = 0; r < nl.rounds; ++r) {
-        hipLaunchKernelGGL(k_mean_agg, dim3(cdiv(size_t(rows) * (H / 8), 256)), dim3(256), 0, st, rows, b.h[r], g.off,
-                           g.nbr, b.agg);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = mark()) != hipSuccess) return e;
-        if ((e = fwd_block<2 * H, H, M_ROUND>(st, nd, nl.L[nl.r1(r)], nl.L[nl.r2(r)], rows, b.h[r], b.agg, b.h[r + 1],
-                                              nullptr, b.training ? b.x_r[r] : nullptr,
-                                              b.training ? b.a_r[r] : nullptr)) != hipSuccess) return e;
+        if ((e = round_fwd(st, nd, nl.L[nl.r1(r)], nl.L[nl.r2(r)], g, rows, b.h[r], b.h[r + 1],
+                           b.training ? b.x_r[r] : nullptr, b.training ? b.a_r[r] : nullptr)) != hipSuccess) return e;
         if ((e = mark()) != hipSuccess) return e;
     }
-    if ((e = fwd_block<H, O, M_DEC>(st, nd, nl.L[nl.dec1()], nl.L[nl.dec2()], rows, b.h[nl.rounds], nullptr, nullptr,
+    if ((e = fwd_block<H, O, M_DEC>(st, nd, nl.L[nl.dec1()], nl.L[nl.dec2()], rows, b.h[nl.rounds], nullptr,
                                     b.y, nullptr, b.training ? b.a_dec : nullptr)) != hipSuccess) return e;
     return mark();
 }

@@ -84,20 +84,18 @@ hipError_t launch_update_prev(hipStream_t st, const float4 *P, const float4 *Ppr
 
 // Forward of the whole network (inference or training).  Buffers (rows = nv):
 //   feat rows x F;  h[0..R] rows x H (h[r] for training, two ping-pong buffers otherwise);
-//   agg rows x H (scratch of the neighbour mean);
 //   y rows x O (fp32).  Training saves (may be null for inference):
 //   a_enc rows x H, x_r[r] rows x 2H, a_r[r] rows x H, a_dec rows x H.
 struct FwdBufs {
     const bf16 *feat;
     bf16 *h[8];
-    bf16 *agg;           // rows x H: neighbour mean of the current round
     float *y;
     bf16 *a_enc, *a_dec;
     bf16 *x_r[4], *a_r[4];
     bool training;
 };
-// ev (optional, 2R + 3 events): recorded before the encoder, after it, after every neighbour
-// mean and round, and after the decoder.
+// ev (optional, R + 3 events): recorded before the encoder, after it, after every round
+// (neighbour mean and round MLP in one launch), and after the decoder.
 hipError_t forward(hipStream_t st, const NetLayout &nl, const NetDev &nd, const Graph &g, uint32_t rows,
                    const FwdBufs &b, hipEvent_t *ev = nullptr);
 // Loss partials (and dy) without the backward pass.

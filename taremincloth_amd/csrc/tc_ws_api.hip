@@ -88,7 +88,6 @@ int ws_prepare(tc_sim *s) {
     TC_HIP(s->ws_feat.alloc(size_t(nv) * tc::ws::F));
     TC_HIP(s->ws_h0.alloc(size_t(nv) * tc::ws::H));
     TC_HIP(s->ws_h1.alloc(size_t(nv) * tc::ws::H));
-    TC_HIP(s->ws_agg.alloc(size_t(nv) * tc::ws::H));
     TC_HIP(s->ws_y.alloc(size_t(nv) * tc::ws::O));
     TC_HIP(s->ws_gate.alloc(4));
     TC_HIP(hipMemsetAsync(s->ws_gate.p, 0, 16, s->stream));
@@ -120,7 +119,6 @@ tc::ws::FwdBufs infer_bufs(tc_sim *s, const bf16 *feat) {
     tc::ws::FwdBufs b{};
     b.feat = feat;
     for (int r = 0; r < 8; ++r) b.h[r] = (r & 1) ? s->ws_h1.p : s->ws_h0.p;
-    b.agg = s->ws_agg.p;
     b.y = s->ws_y.p;
     b.training = false;
     return b;
@@ -139,7 +137,6 @@ int net_alloc_scratch(tc_ws_net *n, uint32_t rows) {
     TC_HIP(n->dyb.alloc(R * H));
     TC_HIP(n->dzb.alloc(R * H));
     TC_HIP(n->feat.alloc(R * tc::ws::F));
-    TC_HIP(n->agg.alloc(R * H));
     TC_HIP(n->y.alloc(R * tc::ws::O));
     TC_HIP(n->dy.alloc(R * tc::ws::O));
     TC_HIP(n->dh0.alloc(R * H));
@@ -156,7 +153,6 @@ tc::ws::FwdBufs train_bufs(tc_ws_net *n, const bf16 *feat) {
     tc::ws::FwdBufs b{};
     b.feat = feat;
     for (uint32_t r = 0; r <= n->nl.rounds; ++r) b.h[r] = n->h[r].p;
-    b.agg = n->agg.p;
     b.y = n->y.p;
     b.a_enc = n->a_enc.p;
     b.a_dec = n->a_dec.p;
@@ -624,11 +620,11 @@ int tc_sim_ws_time(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
             TC_HIP(hipEventRecord(ev[2], s->stream));
             const tc::ws::Graph g{s->ws_goff.p, s->ws_gnbr.p};
             TC_HIP(tc::ws::forward(s->stream, s->ws_net->nl, s->ws_net->dev(), g, nv, infer_bufs(s, s->ws_feat.p),
-                                   &ev[3]));  // ev[3 .. 3 + 2R + 2]
+                                   &ev[3]));  // ev[3 .. 3 + R + 2]
         } else {
             TC_HIP(hipEventRecord(ev[2], s->stream));
         }
-        const int fe = net ? 3 + 2 * int(R) + 2 : 2;  // event after the network
+        const int fe = net ? 3 + int(R) + 2 : 2;  // event after the network
         TC_HIP(tc::ws::launch_apply(s->stream, s->P.p, s->ws_ppred.p, s->ws_D.p, net ? s->ws_y.p : nullptr, sc,
                                     s->ws_beta, nv));
         TC_HIP(hipEventRecord(ev[20], s->stream));
@@ -669,23 +665,20 @@ int tc_sim_ws_time(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
         acc[1] += el(1, 2);
         if (net) {
             acc[2] += el(3, 4);
-            for (uint32_t r = 0; r < R; ++r) {
-                acc[3] += el(4 + 2 * r, 5 + 2 * r);
-                acc[4] += el(5 + 2 * r, 6 + 2 * r);
-            }
-            acc[5] += el(4 + 2 * R, 5 + 2 * R);
+            for (uint32_t r = 0; r < R; ++r) acc[3] += el(4 + r, 5 + r);
+            acc[4] += el(4 + R, 5 + R);
         }
-        acc[6] += el(fe, 20);
-        acc[7] += el(20, 21);
-        acc[8] += el(21, 22);
-        acc[9] += el(22, 23);
-        acc[10] += el(23, 24);
-        acc[11] += el(24, 25);
-        acc[12] += el(25, 26);
-        acc[13] += el(0, 26);
+        acc[5] += el(fe, 20);
+        acc[6] += el(20, 21);
+        acc[7] += el(21, 22);
+        acc[8] += el(22, 23);
+        acc[9] += el(23, 24);
+        acc[10] += el(24, 25);
+        acc[11] += el(25, 26);
+        acc[12] += el(0, 26);
     }
-    for (int k = 0; k < 14; ++k) out[k] = acc[k] / double(reps);
-    out[14] = R;
-    out[15] = s->ws_iters;
+    for (int k = 0; k < 13; ++k) out[k] = acc[k] / double(reps);
+    out[13] = R;
+    out[14] = s->ws_iters;
     return TC_OK;
 }

@@ -370,7 +370,7 @@ class ClothSimulator:
         """Per-part times (ms) of a warm-started substep from HIP events; advances the simulation."""
         out = (C.c_double * 16)()
         _lib.check(self._L.tc_sim_ws_time(self._h, float(dt_sub), int(reps), out))
-        keys = ["predict", "features", "encoder", "neighbour_means", "round_mlps", "decoder", "apply",
+        keys = ["predict", "features", "encoder", "round_layers", "decoder", "apply",
                 "iterations", "gate", "fallback", "correction_update", "collide", "update_vel", "substep",
                 "rounds", "warm_iterations"]
         return dict(zip(keys, list(out)))
