@@ -53,6 +53,48 @@ NetLayout make_layout(uint32_t rounds) {
 // ---------------------------------------------------------------- features
 __device__ __forceinline__ float soft(float x) { return x / (1.0f + fabsf(x) * (1.0f / 16.0f)); }
 
+// Sum over the incident pairs k of vertex i of d_k (|d_k| - rest_k) / |d_k|, d_k = p_j - p0, in
+// incidence order.  The first FEAT_NB neighbours' ids, rest lengths and positions are all
+// requested before any is used (ids from index 0 and positions of vertex i stand in for the
+// slots past the degree, so every load is in bounds), which keeps the chain of dependent
+// loads two deep; further neighbours follow one by one.
+constexpr int FEAT_NB = 8;
+__device__ __forceinline__ f3 strain_sum(const FeatArgs &a, uint32_t i, f3 p0, uint32_t k0, uint32_t k1,
+                                         const uint32_t *__restrict__ nbr, const float *__restrict__ rest) {
+    const float bw = a.beta;
+    const uint32_t n = k1 - k0;
+    uint32_t js[FEAT_NB];
+    float rs[FEAT_NB];
+#pragma unroll
+    for (int k = 0; k < FEAT_NB; ++k) {
+        const uint32_t q = uint32_t(k) < n ? k0 + k : 0u;
+        js[k] = nbr[q];
+        rs[k] = rest[q];
+    }
+    float4 pj[FEAT_NB], dj[FEAT_NB];
+#pragma unroll
+    for (int k = 0; k < FEAT_NB; ++k) {
+        const uint32_t j = uint32_t(k) < n ? js[k] : i;
+        pj[k] = a.P[j];
+        dj[k] = a.D[j];
+    }
+    f3 acc = mk3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int k = 0; k < FEAT_NB; ++k) {
+        const f3 d = mk3(pj[k].x + bw * dj[k].x, pj[k].y + bw * dj[k].y, pj[k].z + bw * dj[k].z) - p0;
+        const float len = length(d);
+        if (uint32_t(k) < n && len > 1e-12f) acc = acc + d * ((len - rs[k]) / len);
+    }
+    for (uint32_t k = k0 + FEAT_NB; k < k1; ++k) {
+        const uint32_t j = nbr[k];
+        const float4 p = a.P[j], dd = a.D[j];
+        const f3 d = mk3(p.x + bw * dd.x, p.y + bw * dd.y, p.z + bw * dd.z) - p0;
+        const float len = length(d);
+        if (len > 1e-12f) acc = acc + d * ((len - rest[k]) / len);
+    }
+    return acc;
+}
+
 __global__ void __launch_bounds__(256) k_features(FeatArgs a, bf16 *__restrict__ feat) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.nv) return;
@@ -65,30 +107,16 @@ __global__ void __launch_bounds__(256) k_features(FeatArgs a, bf16 *__restrict__
     f[0] = soft(di.x * is);
     f[1] = soft(di.y * is);
     f[2] = soft(di.z * is);
-    f3 acc = mk3(0.0f, 0.0f, 0.0f);
     const uint32_t d0 = a.d_off[i], d1 = a.d_off[i + 1];
-    for (uint32_t k = d0; k < d1; ++k) {
-        const uint32_t j = a.d_nbr[k];
-        const float4 pj = a.P[j], dj = a.D[j];
-        const f3 d = mk3(pj.x + bw * dj.x, pj.y + bw * dj.y, pj.z + bw * dj.z) - p0;
-        const float len = length(d);
-        if (len > 1e-12f) acc = acc + d * ((len - a.d_rest[k]) / len);
-    }
-    f[3] = soft(acc.x * is);
-    f[4] = soft(acc.y * is);
-    f[5] = soft(acc.z * is);
-    acc = mk3(0.0f, 0.0f, 0.0f);
+    const f3 sd = strain_sum(a, i, p0, d0, d1, a.d_nbr, a.d_rest);
+    f[3] = soft(sd.x * is);
+    f[4] = soft(sd.y * is);
+    f[5] = soft(sd.z * is);
     const uint32_t b0 = a.b_off[i], b1 = a.b_off[i + 1];
-    for (uint32_t k = b0; k < b1; ++k) {
-        const uint32_t j = a.b_nbr[k];
-        const float4 pj = a.P[j], dj = a.D[j];
-        const f3 d = mk3(pj.x + bw * dj.x, pj.y + bw * dj.y, pj.z + bw * dj.z) - p0;
-        const float len = length(d);
-        if (len > 1e-12f) acc = acc + d * ((len - a.b_rest[k]) / len);
-    }
-    f[6] = soft(acc.x * is);
-    f[7] = soft(acc.y * is);
-    f[8] = soft(acc.z * is);
+    const f3 sb = strain_sum(a, i, p0, b0, b1, a.b_nbr, a.b_rest);
+    f[6] = soft(sb.x * is);
+    f[7] = soft(sb.y * is);
+    f[8] = soft(sb.z * is);
     f[9] = pi.w > 0.0f ? 1.0f : 0.0f;
     f[10] = float(d1 - d0) * 0.125f;
     f[11] = float(b1 - b0) * 0.125f;
@@ -238,9 +266,28 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
     if (tid < H) b1s[tid] = a.b1[tid];
     if (tid < N2) b2s[tid] = a.b2[tid];
     bf16x8 nx[XCH];
-    uint32_t t = blockIdx.x;
+    // A tile's output is stored during the next tile, right after that tile's input load is
+    // issued, so the wait for the input never waits for stores issued just before it.  The
+    // first pass stores zeros into its own tile's rows, which it overwrites later.
+    constexpr int OCH = TR * (H / 8) / 256;  // 16-byte output chunks per thread (hidden output)
+    bf16x8 ov[OCH];
+    for (int u = 0; u < OCH; ++u) ov[u] = zero8();
+    float4 yv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // decoder output of this lane's row
+    uint32_t t = blockIdx.x, tp = t;
     if (t < a.ntiles)
         for (int u = 0; u < XCH; ++u) nx[u] = load_x_chunk<K1, MODE>(a, t * TR, tid + 256 * u);
+    auto store_out = [&](uint32_t tile) {
+        if (MODE == M_DEC) {
+            const uint32_t row = tile * TR + 16 * w + c;
+            if (row < a.rows) *reinterpret_cast<float4 *>(&a.yout[size_t(row) * O + 4 * g]) = yv;
+        } else {
+            for (int u = 0; u < OCH; ++u) {
+                const int q = tid + 256 * u, r2 = q / (H / 8), ch = q % (H / 8);
+                const uint32_t row2 = tile * TR + r2;
+                if (row2 < a.rows) *reinterpret_cast<bf16x8 *>(&a.hout[size_t(row2) * H + ch * 8]) = ov[u];
+            }
+        }
+    };
     for (; t < a.ntiles; t += gridDim.x) {
         const uint32_t r0 = t * TR;
         __syncthreads();
@@ -251,6 +298,8 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
         const uint32_t tn = t + gridDim.x;
         if (tn < a.ntiles)
             for (int u = 0; u < XCH; ++u) nx[u] = load_x_chunk<K1, MODE>(a, tn * TR, tid + 256 * u);
+        store_out(tp);
+        tp = t;
         __syncthreads();
         if (a.xsave) {
             for (int q = tid; q < TR * (K1 / 8); q += 256) {
@@ -300,10 +349,7 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
             for (int ks = 0; ks < KS2; ++ks)
                 acc2 = mfma(wf2[0][ks], *reinterpret_cast<const bf16x8 *>(&as[rr * LH + ks * 32 + g * 8]), acc2);
             const float4 bias = *reinterpret_cast<const float4 *>(&b2s[4 * g]);
-            const uint32_t row = r0 + rr;
-            if (row < a.rows)
-                *reinterpret_cast<float4 *>(&a.yout[size_t(row) * O + 4 * g]) =
-                    make_float4(acc2[0] + bias.x, acc2[1] + bias.y, acc2[2] + bias.z, acc2[3] + bias.w);
+            yv = make_float4(acc2[0] + bias.x, acc2[1] + bias.y, acc2[2] + bias.z, acc2[3] + bias.w);
         } else {
             f32x4 acc2[2][4];
             bf16x8 af[2][KS2];
@@ -329,19 +375,13 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
                 }
             }
             __syncthreads();
-            constexpr int OCH = TR * (H / 8) / 256;  // 16-byte output chunks per thread
-            bf16x8 ov[OCH];
             for (int u = 0; u < OCH; ++u) {
                 const int q = tid + 256 * u, r2 = q / (H / 8), ch = q % (H / 8);
                 ov[u] = *reinterpret_cast<const bf16x8 *>(&as[r2 * LH + ch * 8]);
             }
-            for (int u = 0; u < OCH; ++u) {
-                const int q = tid + 256 * u, r2 = q / (H / 8), ch = q % (H / 8);
-                const uint32_t row2 = r0 + r2;
-                if (row2 < a.rows) *reinterpret_cast<bf16x8 *>(&a.hout[size_t(row2) * H + ch * 8]) = ov[u];
-            }
         }
     }
+    if (blockIdx.x < a.ntiles) store_out(tp);
 }
 
 // ---------------------------------------------------------------- message-passing round
@@ -355,10 +395,11 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
 // for two waves per SIMD.  Thread q stages the 16-byte column chunk q&15 of rows q>>4 and
 // (q>>4)+32.  The neighbour gather runs three tiles ahead in three stages, one per
 // tile: the CSR range, then the neighbour ids, then the rows themselves, so every load
-// depends only on data that arrived during an earlier tile's compute.  The rows are
-// requested right after staging; the ids and ranges after the tile's output stores, so
-// that waiting for them never waits for those stores.  Up to RND_DMAX neighbours per row
-// are prefetched; any beyond are read when the tile is staged.
+// depends only on data that arrived during an earlier tile's compute.  Up to RND_DMAX
+// neighbours per row are prefetched; any beyond are read when the tile is staged.
+// The buffers the kernel writes (hout, xsave, asave) hold whole tiles (padded_rows), so
+// every store is unconditional: the compiler then knows how many memory operations follow
+// each prefetch and waits for the prefetched data alone, never for the stores.
 struct RoundFwd {
     uint32_t rows, ntiles;
     const bf16 *h;               // rows x H
@@ -368,13 +409,14 @@ struct RoundFwd {
     const bf16 *w2t;
     const float *b2;
     bf16 *hout;
-    bf16 *xsave, *asave;         // training saves of [h, m] and of the hidden activation (or null)
+    bf16 *xsave, *asave;         // training saves of [h, m] and of the hidden activation
 };
 constexpr int RND_THREADS = 512;
 constexpr int RND_DMAX = 6;
 constexpr int RND_U = TR * (H / 8) / RND_THREADS;  // staged (row, chunk) pairs per thread
 constexpr size_t round_lds() { return size_t(2) * (TR * (2 * H + 8) + TR * (H + 8)); }
 
+template <bool SAVE>
 __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int K1 = 2 * H, LK = K1 + 8, LH = H + 8, KS1 = K1 / 32, KS2 = H / 32;
@@ -442,7 +484,13 @@ __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
             }
         }
     };
-    uint32_t t = blockIdx.x;
+    // A tile's output is stored during the next tile, right after that tile's prefetch is
+    // issued, so no wait for prefetched data ever waits for stores issued just before it.
+    // The first pass stores zeros into its own tile's rows, which it overwrites later.
+    constexpr int OCH = TR * (H / 8) / RND_THREADS;
+    bf16x8 ov[OCH];
+    for (int u = 0; u < OCH; ++u) ov[u] = zero8();
+    uint32_t t = blockIdx.x, tp = t;
     stage_a(t);
     stage_b();
     stage_c(t);
@@ -474,16 +522,19 @@ __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
             *reinterpret_cast<bf16x8 *>(&xs[rr * LK + H + qc * 8]) = m;
         }
         stage_c(t + G);
+        stage_b();
+        stage_a(t + 3 * G);
+        for (int u = 0; u < OCH; ++u) {
+            const int q = tid + RND_THREADS * u, r2 = q / (H / 8), ch = q % (H / 8);
+            *reinterpret_cast<bf16x8 *>(&a.hout[size_t(tp * TR + r2) * H + ch * 8]) = ov[u];
+        }
+        tp = t;
         __syncthreads();
-        if (a.xsave) {
-            // constant trip counts: a store loop of unknown length would make every later
-            // wait for an earlier load a wait for all memory operations
+        if (SAVE) {
             for (int u = 0; u < TR * (K1 / 8) / RND_THREADS; ++u) {
                 const int q = tid + RND_THREADS * u, rr = q / (K1 / 8), ch = q % (K1 / 8);
-                const uint32_t row = r0 + rr;
-                if (row < a.rows)
-                    *reinterpret_cast<bf16x8 *>(&a.xsave[size_t(row) * K1 + ch * 8]) =
-                        *reinterpret_cast<const bf16x8 *>(&xs[rr * LK + ch * 8]);
+                *reinterpret_cast<bf16x8 *>(&a.xsave[size_t(r0 + rr) * K1 + ch * 8]) =
+                    *reinterpret_cast<const bf16x8 *>(&xs[rr * LK + ch * 8]);
             }
         }
         // layer 1: hidden units 32cg + 16ct + 4g + j of rows 32rg + 16rt + c
@@ -507,13 +558,11 @@ __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
             }
         }
         __syncthreads();
-        if (a.asave) {
+        if (SAVE) {
             for (int u = 0; u < TR * (H / 8) / RND_THREADS; ++u) {
                 const int q = tid + RND_THREADS * u, rr = q / (H / 8), ch = q % (H / 8);
-                const uint32_t row = r0 + rr;
-                if (row < a.rows)
-                    *reinterpret_cast<bf16x8 *>(&a.asave[size_t(row) * H + ch * 8]) =
-                        *reinterpret_cast<const bf16x8 *>(&as[rr * LH + ch * 8]);
+                *reinterpret_cast<bf16x8 *>(&a.asave[size_t(r0 + rr) * H + ch * 8]) =
+                    *reinterpret_cast<const bf16x8 *>(&as[rr * LH + ch * 8]);
             }
         }
         // layer 2 and the residual
@@ -542,19 +591,14 @@ __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
             }
         }
         __syncthreads();
-        constexpr int OCH = TR * (H / 8) / RND_THREADS;
-        bf16x8 ov[OCH];
         for (int u = 0; u < OCH; ++u) {
             const int q = tid + RND_THREADS * u, r2 = q / (H / 8), ch = q % (H / 8);
             ov[u] = *reinterpret_cast<const bf16x8 *>(&as[r2 * LH + ch * 8]);
         }
-        for (int u = 0; u < OCH; ++u) {
-            const int q = tid + RND_THREADS * u, r2 = q / (H / 8), ch = q % (H / 8);
-            const uint32_t row2 = r0 + r2;
-            if (row2 < a.rows) *reinterpret_cast<bf16x8 *>(&a.hout[size_t(row2) * H + ch * 8]) = ov[u];
-        }
-        stage_b();
-        stage_a(t + 3 * G);
+    }
+    for (int u = 0; u < OCH; ++u) {
+        const int q = tid + RND_THREADS * u, r2 = q / (H / 8), ch = q % (H / 8);
+        *reinterpret_cast<bf16x8 *>(&a.hout[size_t(tp * TR + r2) * H + ch * 8]) = ov[u];
     }
 }
 
@@ -863,7 +907,10 @@ hipError_t round_fwd(hipStream_t st, const NetDev &nd, const Layer &l1, const La
     a.xsave = xsave;
     a.asave = asave;
     // one workgroup (8 waves) per CU: the weight fragments take about half of each wave's registers
-    hipLaunchKernelGGL(k_round_fwd, dim3(std::min(a.ntiles, PERSIST)), dim3(RND_THREADS), round_lds(), st, a);
+    if (xsave && asave)
+        hipLaunchKernelGGL(k_round_fwd<true>, dim3(std::min(a.ntiles, PERSIST)), dim3(RND_THREADS), round_lds(), st, a);
+    else
+        hipLaunchKernelGGL(k_round_fwd<false>, dim3(std::min(a.ntiles, PERSIST)), dim3(RND_THREADS), round_lds(), st, a);
     return hipGetLastError();
 }
 
@@ -908,7 +955,8 @@ hipError_t allow_lds(Kern k, size_t bytes) {
 hipError_t set_lds_limits() {
     hipError_t e;
     if ((e = allow_lds(&k_mlp_fwd<F, H, M_ENC>, fwd_lds<F, H>())) != hipSuccess) return e;
-    if ((e = allow_lds(&k_round_fwd, round_lds())) != hipSuccess) return e;
+    if ((e = allow_lds(&k_round_fwd<false>, round_lds())) != hipSuccess) return e;
+    if ((e = allow_lds(&k_round_fwd<true>, round_lds())) != hipSuccess) return e;
     if ((e = allow_lds(&k_mlp_fwd<H, O, M_DEC>, fwd_lds<H, O>())) != hipSuccess) return e;
     if ((e = allow_lds(&k_mlp_bwd<H, O>, bwd_lds<H, O>())) != hipSuccess) return e;
     if ((e = allow_lds(&k_mlp_bwd<2 * H, H>, bwd_lds<2 * H, H>())) != hipSuccess) return e;

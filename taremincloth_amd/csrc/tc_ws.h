@@ -26,6 +26,9 @@ constexpr int F = 32;         // input features per vertex
 constexpr int O = 16;         // decoder output width
 constexpr int TR = 64;        // rows per tile
 constexpr int MAX_LAYERS = 16;
+// Rows of the buffers a message-passing round writes (h, and the training saves x_r, a_r):
+// whole tiles, so that the round kernel stores without bounds checks.
+constexpr size_t padded_rows(size_t rows) { return (rows + TR - 1) / TR * TR; }
 
 typedef __bf16 bf16;
 
@@ -83,9 +86,10 @@ hipError_t launch_update_prev(hipStream_t st, const float4 *P, const float4 *Ppr
                               float beta, uint32_t nv);
 
 // Forward of the whole network (inference or training).  Buffers (rows = nv):
-//   feat rows x F;  h[0..R] rows x H (h[r] for training, two ping-pong buffers otherwise);
+//   feat rows x F;  h[0..R] padded_rows(rows) x H (h[r] for training, two ping-pong buffers
+//   otherwise);
 //   y rows x O (fp32).  Training saves (may be null for inference):
-//   a_enc rows x H, x_r[r] rows x 2H, a_r[r] rows x H, a_dec rows x H.
+//   a_enc rows x H, x_r[r] padded_rows(rows) x 2H, a_r[r] padded_rows(rows) x H, a_dec rows x H.
 struct FwdBufs {
     const bf16 *feat;
     bf16 *h[8];

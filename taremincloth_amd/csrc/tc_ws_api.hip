@@ -86,8 +86,8 @@ int ws_prepare(tc_sim *s) {
     TC_HIP(s->ws_D.alloc(nv));
     TC_HIP(hipMemsetAsync(s->ws_D.p, 0, sizeof(float4) * std::max<size_t>(nv, 1), s->stream));
     TC_HIP(s->ws_feat.alloc(size_t(nv) * tc::ws::F));
-    TC_HIP(s->ws_h0.alloc(size_t(nv) * tc::ws::H));
-    TC_HIP(s->ws_h1.alloc(size_t(nv) * tc::ws::H));
+    TC_HIP(s->ws_h0.alloc(tc::ws::padded_rows(nv) * tc::ws::H));
+    TC_HIP(s->ws_h1.alloc(tc::ws::padded_rows(nv) * tc::ws::H));
     TC_HIP(s->ws_y.alloc(size_t(nv) * tc::ws::O));
     TC_HIP(s->ws_gate.alloc(4));
     TC_HIP(hipMemsetAsync(s->ws_gate.p, 0, 16, s->stream));
@@ -127,10 +127,11 @@ tc::ws::FwdBufs infer_bufs(tc_sim *s, const bf16 *feat) {
 int net_alloc_scratch(tc_ws_net *n, uint32_t rows) {
     if (rows <= n->rows_cap) return TC_OK;
     const size_t R = rows, H = tc::ws::H;
-    for (uint32_t r = 0; r <= n->nl.rounds; ++r) TC_HIP(n->h[r].alloc(R * H));
+    const size_t Rp = tc::ws::padded_rows(rows);
+    for (uint32_t r = 0; r <= n->nl.rounds; ++r) TC_HIP(n->h[r].alloc(Rp * H));
     for (uint32_t r = 0; r < n->nl.rounds; ++r) {
-        TC_HIP(n->x_r[r].alloc(R * 2 * H));
-        TC_HIP(n->a_r[r].alloc(R * H));
+        TC_HIP(n->x_r[r].alloc(Rp * 2 * H));
+        TC_HIP(n->a_r[r].alloc(Rp * H));
     }
     TC_HIP(n->a_enc.alloc(R * H));
     TC_HIP(n->a_dec.alloc(R * H));
