@@ -117,6 +117,40 @@ def test_gpu_forward_and_gradients_match_oracle(require_gpu, rows, rounds):
             assert num / den < 5e-2, (o, K, N, num / den)
 
 
+def hub_graph(n, seed=0):
+    """A ring over the first n - 1 vertices, hubs 0..4 joined to 10..60 others, vertex n - 1
+    isolated: degrees from 0 to about 60, beyond what the round kernel prefetches per row."""
+    rng = np.random.default_rng(seed)
+    pairs = {(i, (i + 1) % (n - 1)) for i in range(n - 1)}
+    for hub, k in zip(range(5), (10, 20, 30, 45, 60)):
+        for j in rng.choice(np.arange(5, n - 1), k, replace=False):
+            pairs.add((hub, int(j)))
+    a = np.array([p[0] for p in pairs])
+    b = np.array([p[1] for p in pairs])
+    off, nbr, _ = W.csr_pairs(n, a, b)
+    return off, nbr
+
+
+@pytest.mark.gpu
+def test_gpu_forward_with_hub_and_isolated_vertices_matches_oracle(require_gpu):
+    """Rows of degree above the prefetched neighbours finish their mean in a loop; an isolated
+    row's mean is zero; 301 rows leave a ragged last tile."""
+    rows, rounds = 301, 3
+    p, feat, _, _, t = random_case(rows, rounds, 77)
+    off, nbr = hub_graph(rows, seed=77)
+    deg = np.diff(off)
+    assert deg.max() > 40 and deg[-1] == 0
+    net = _net(rounds, p)
+    y, g, loss = net.forward_backward(feat, off, nbr, t)
+    lo, go, yo = W.backward(p, rounds, feat, off, nbr, t, bf16=True)
+    for rows_sel in (slice(None), slice(0, 5), slice(rows - 1, rows)):
+        ey = np.abs(y[rows_sel] - yo[rows_sel]).max() / np.abs(yo).max()
+        assert ey < 2e-2, (rows_sel, ey)
+    assert abs(loss - lo) <= 2e-2 * lo
+    num = np.linalg.norm(g - go)
+    assert num / np.linalg.norm(go) < 5e-2
+
+
 @pytest.mark.gpu
 def test_gpu_adam_first_step_closed_form(require_gpu):
     p, feat, off, nbr, t = random_case(128, 1, 9)
