@@ -398,8 +398,8 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
 // depends only on data that arrived during an earlier tile's compute.  Up to RND_DMAX
 // neighbours per row are prefetched; any beyond are read when the tile is staged.
 // The buffers the kernel writes (hout, xsave, asave) hold whole tiles (padded_rows), so
-// every store is unconditional: the compiler then knows how many memory operations follow
-// each prefetch and waits for the prefetched data alone, never for the stores.
+// every store is unconditional.  Two barriers per tile: after staging, and between the
+// layers; the outputs leave from the layer-2 accumulators.
 struct RoundFwd {
     uint32_t rows, ntiles;
     const bf16 *h;               // rows x H
@@ -484,13 +484,7 @@ __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
             }
         }
     };
-    // A tile's output is stored during the next tile, right after that tile's prefetch is
-    // issued, so no wait for prefetched data ever waits for stores issued just before it.
-    // The first pass stores zeros into its own tile's rows, which it overwrites later.
-    constexpr int OCH = TR * (H / 8) / RND_THREADS;
-    bf16x8 ov[OCH];
-    for (int u = 0; u < OCH; ++u) ov[u] = zero8();
-    uint32_t t = blockIdx.x, tp = t;
+    uint32_t t = blockIdx.x;
     stage_a(t);
     stage_b();
     stage_c(t);
@@ -524,11 +518,6 @@ __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
         stage_c(t + G);
         stage_b();
         stage_a(t + 3 * G);
-        for (int u = 0; u < OCH; ++u) {
-            const int q = tid + RND_THREADS * u, r2 = q / (H / 8), ch = q % (H / 8);
-            *reinterpret_cast<bf16x8 *>(&a.hout[size_t(tp * TR + r2) * H + ch * 8]) = ov[u];
-        }
-        tp = t;
         __syncthreads();
         if (SAVE) {
             for (int u = 0; u < TR * (K1 / 8) / RND_THREADS; ++u) {
@@ -537,7 +526,10 @@ __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
                     *reinterpret_cast<const bf16x8 *>(&xs[rr * LK + ch * 8]);
             }
         }
-        // layer 1: hidden units 32cg + 16ct + 4g + j of rows 32rg + 16rt + c
+        // layer 1: hidden units 32cg + 16ct + 4g + j of rows 32rg + 16rt + c.  The residual
+        // inputs of the lane's outputs are read here too: after the next barrier no wave reads
+        // `xs` again, so the next tile may be staged over it without another barrier.
+        bf16x4 hs[2][2];
         for (int rt = 0; rt < 2; ++rt) {
             const int rr = 32 * rg + 16 * rt + c;
             f32x4 acc[2];
@@ -555,6 +547,7 @@ __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
                 v[2] = (bf16)fmaxf(acc[ct][2] + bias.z, 0.0f);
                 v[3] = (bf16)fmaxf(acc[ct][3] + bias.w, 0.0f);
                 *reinterpret_cast<bf16x4 *>(&as[rr * LH + col]) = v;
+                hs[rt][ct] = *reinterpret_cast<const bf16x4 *>(&xs[rr * LK + col]);
             }
         }
         __syncthreads();
@@ -575,30 +568,21 @@ __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
             for (int ks = 0; ks < KS2; ++ks)
                 for (int ct = 0; ct < 2; ++ct) acc2[rt][ct] = mfma(wf2[ct][ks], af[ks], acc2[rt][ct]);
         }
-        __syncthreads();  // every wave of this row group has read `as`
+        // layer-2 outputs go straight from the accumulators to HBM (the rows are padded to whole
+        // tiles); `as` is next written by the next tile's layer 1, after the staging barrier
         for (int rt = 0; rt < 2; ++rt) {
             const int rr = 32 * rg + 16 * rt + c;
             for (int ct = 0; ct < 2; ++ct) {
                 const int col = 32 * cg + 16 * ct + 4 * g;
                 const float4 bias = *reinterpret_cast<const float4 *>(&b2s[col]);
-                const bf16x4 hs = *reinterpret_cast<const bf16x4 *>(&xs[rr * LK + col]);
                 bf16x4 o;
-                o[0] = (bf16)((float)hs[0] + (acc2[rt][ct][0] + bias.x));
-                o[1] = (bf16)((float)hs[1] + (acc2[rt][ct][1] + bias.y));
-                o[2] = (bf16)((float)hs[2] + (acc2[rt][ct][2] + bias.z));
-                o[3] = (bf16)((float)hs[3] + (acc2[rt][ct][3] + bias.w));
-                *reinterpret_cast<bf16x4 *>(&as[rr * LH + col]) = o;
+                o[0] = (bf16)((float)hs[rt][ct][0] + (acc2[rt][ct][0] + bias.x));
+                o[1] = (bf16)((float)hs[rt][ct][1] + (acc2[rt][ct][1] + bias.y));
+                o[2] = (bf16)((float)hs[rt][ct][2] + (acc2[rt][ct][2] + bias.z));
+                o[3] = (bf16)((float)hs[rt][ct][3] + (acc2[rt][ct][3] + bias.w));
+                *reinterpret_cast<bf16x4 *>(&a.hout[size_t(r0 + rr) * H + col]) = o;
             }
         }
-        __syncthreads();
-        for (int u = 0; u < OCH; ++u) {
-            const int q = tid + RND_THREADS * u, r2 = q / (H / 8), ch = q % (H / 8);
-            ov[u] = *reinterpret_cast<const bf16x8 *>(&as[r2 * LH + ch * 8]);
-        }
-    }
-    for (int u = 0; u < OCH; ++u) {
-        const int q = tid + RND_THREADS * u, r2 = q / (H / 8), ch = q % (H / 8);
-        *reinterpret_cast<bf16x8 *>(&a.hout[size_t(tp * TR + r2) * H + ch * 8]) = ov[u];
     }
 }
 
