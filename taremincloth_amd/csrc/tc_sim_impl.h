@@ -160,6 +160,7 @@ struct tc_sim {
     DevBuf<uint32_t> ws_doff, ws_dnbr, ws_boff, ws_bnbr, ws_goff, ws_gnbr;
     DevBuf<float> ws_drest, ws_brest;
     std::vector<uint32_t> ws_goff_h, ws_gnbr_h;
+    uint32_t ws_gmaxdeg = 0;
     float ws_lscale = 0.01f;
     DevBuf<float4> ws_ppred, ws_D;
     DevBuf<tc::ws::bf16> ws_feat, ws_h0, ws_h1;

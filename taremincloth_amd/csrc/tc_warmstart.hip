@@ -395,8 +395,9 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(MlpFwd a) {
 // for two waves per SIMD.  Thread q stages the 16-byte column chunk q&15 of rows q>>4 and
 // (q>>4)+32.  The neighbour gather runs three tiles ahead in three stages, one per
 // tile: the CSR range, then the neighbour ids, then the rows themselves, so every load
-// depends only on data that arrived during an earlier tile's compute.  Up to RND_DMAX
-// neighbours per row are prefetched; any beyond are read when the tile is staged.
+// depends only on data that arrived during an earlier tile's compute.  Up to DMAX (4 or 6,
+// the smallest that covers the graph's largest degree) neighbours per row are prefetched;
+// any beyond are read when the tile is staged.
 // The buffers the kernel writes (hout, xsave, asave) hold whole tiles (padded_rows), so
 // every store is unconditional.  Two barriers per tile: after staging, and between the
 // layers; the outputs leave from the layer-2 accumulators.
@@ -412,11 +413,10 @@ struct RoundFwd {
     bf16 *xsave, *asave;         // training saves of [h, m] and of the hidden activation
 };
 constexpr int RND_THREADS = 512;
-constexpr int RND_DMAX = 6;
 constexpr int RND_U = TR * (H / 8) / RND_THREADS;  // staged (row, chunk) pairs per thread
 constexpr size_t round_lds() { return size_t(2) * (TR * (2 * H + 8) + TR * (H + 8)); }
 
-template <bool SAVE>
+template <bool SAVE, int DMAX>
 __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int K1 = 2 * H, LK = K1 + 8, LH = H + 8, KS1 = K1 / 32, KS2 = H / 32;
@@ -445,9 +445,9 @@ __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
     // only selects on the previous stage's values once they have arrived, so no load sits
     // in a branch and the compiler waits only for the data it is about to use.
     uint32_t sa_b[RND_U], sa_e[RND_U], sa_ok = 0;
-    uint32_t sb_id[RND_U][RND_DMAX], sb_b[RND_U], sb_n[RND_U];
+    uint32_t sb_id[RND_U][DMAX], sb_b[RND_U], sb_n[RND_U];
     uint32_t sc_b[RND_U], sc_n[RND_U], sc_ok = 0;
-    bf16x8 sc_self[RND_U], sc_nb[RND_U][RND_DMAX];
+    bf16x8 sc_self[RND_U], sc_nb[RND_U][DMAX];
     auto stage_a = [&](uint32_t t) {
         sa_ok = 0;
         for (int u = 0; u < RND_U; ++u) {
@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
             const uint32_t b0 = ok ? sa_b[u] : 0u, n = ok ? sa_e[u] - sa_b[u] : 0u;
             sb_b[u] = b0;
             sb_n[u] = n;
-            for (int k = 0; k < RND_DMAX; ++k) sb_id[u][k] = a.nbr[uint32_t(k) < n ? b0 + k : 0u];
+            for (int k = 0; k < DMAX; ++k) sb_id[u][k] = a.nbr[uint32_t(k) < n ? b0 + k : 0u];
         }
     };
     auto stage_c = [&](uint32_t t) {
@@ -478,7 +478,7 @@ __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
             sc_b[u] = sb_b[u];
             sc_n[u] = sb_n[u];
             sc_self[u] = *reinterpret_cast<const bf16x8 *>(&a.h[size_t(r) * H + qc * 8]);
-            for (int k = 0; k < RND_DMAX; ++k) {
+            for (int k = 0; k < DMAX; ++k) {
                 const uint32_t j = uint32_t(k) < sb_n[u] ? sb_id[u][k] : r;
                 sc_nb[u][k] = *reinterpret_cast<const bf16x8 *>(&a.h[size_t(j) * H + qc * 8]);
             }
@@ -499,10 +499,10 @@ __global__ void __launch_bounds__(RND_THREADS) k_round_fwd(RoundFwd a) {
             float acc[8];
             for (int i = 0; i < 8; ++i) acc[i] = 0.0f;
             const uint32_t n = sc_n[u];
-            for (int k = 0; k < RND_DMAX; ++k)
+            for (int k = 0; k < DMAX; ++k)
                 if (uint32_t(k) < n)
                     for (int i = 0; i < 8; ++i) acc[i] += (float)sc_nb[u][k][i];
-            for (uint32_t k = RND_DMAX; k < n; ++k) {
+            for (uint32_t k = DMAX; k < n; ++k) {
                 const bf16x8 v = *reinterpret_cast<const bf16x8 *>(&a.h[size_t(a.nbr[sc_b[u] + k]) * H + qc * 8]);
                 for (int i = 0; i < 8; ++i) acc[i] += (float)v[i];
             }
@@ -891,10 +891,12 @@ hipError_t round_fwd(hipStream_t st, const NetDev &nd, const Layer &l1, const La
     a.xsave = xsave;
     a.asave = asave;
     // one workgroup (8 waves) per CU: the weight fragments take about half of each wave's registers
-    if (xsave && asave)
-        hipLaunchKernelGGL(k_round_fwd<true>, dim3(std::min(a.ntiles, PERSIST)), dim3(RND_THREADS), round_lds(), st, a);
-    else
-        hipLaunchKernelGGL(k_round_fwd<false>, dim3(std::min(a.ntiles, PERSIST)), dim3(RND_THREADS), round_lds(), st, a);
+    const dim3 grid(std::min(a.ntiles, PERSIST)), block(RND_THREADS);
+    const bool save = xsave && asave, d4 = g.max_degree <= 4;
+    if (save && d4) hipLaunchKernelGGL((k_round_fwd<true, 4>), grid, block, round_lds(), st, a);
+    else if (save) hipLaunchKernelGGL((k_round_fwd<true, 6>), grid, block, round_lds(), st, a);
+    else if (d4) hipLaunchKernelGGL((k_round_fwd<false, 4>), grid, block, round_lds(), st, a);
+    else hipLaunchKernelGGL((k_round_fwd<false, 6>), grid, block, round_lds(), st, a);
     return hipGetLastError();
 }
 
@@ -939,8 +941,10 @@ hipError_t allow_lds(Kern k, size_t bytes) {
 hipError_t set_lds_limits() {
     hipError_t e;
     if ((e = allow_lds(&k_mlp_fwd<F, H, M_ENC>, fwd_lds<F, H>())) != hipSuccess) return e;
-    if ((e = allow_lds(&k_round_fwd<false>, round_lds())) != hipSuccess) return e;
-    if ((e = allow_lds(&k_round_fwd<true>, round_lds())) != hipSuccess) return e;
+    if ((e = allow_lds(&k_round_fwd<false, 4>, round_lds())) != hipSuccess) return e;
+    if ((e = allow_lds(&k_round_fwd<false, 6>, round_lds())) != hipSuccess) return e;
+    if ((e = allow_lds(&k_round_fwd<true, 4>, round_lds())) != hipSuccess) return e;
+    if ((e = allow_lds(&k_round_fwd<true, 6>, round_lds())) != hipSuccess) return e;
     if ((e = allow_lds(&k_mlp_fwd<H, O, M_DEC>, fwd_lds<H, O>())) != hipSuccess) return e;
     if ((e = allow_lds(&k_mlp_bwd<H, O>, bwd_lds<H, O>())) != hipSuccess) return e;
     if ((e = allow_lds(&k_mlp_bwd<2 * H, H>, bwd_lds<2 * H, H>())) != hipSuccess) return e;

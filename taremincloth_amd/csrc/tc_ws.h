@@ -72,8 +72,14 @@ struct FeatArgs {
     float beta;             // history weight: strains are taken at prediction + beta * D
 };
 
-// Graph of the message-passing rounds (stretch edges, both directions).
-struct Graph { const uint32_t *off, *nbr; };
+// Graph of the message-passing rounds (stretch edges, both directions); max_degree is the
+// largest row length (it picks how many neighbours a round prefetches per row).
+struct Graph { const uint32_t *off, *nbr; uint32_t max_degree; };
+inline uint32_t max_degree(const uint32_t *off, uint32_t rows) {
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < rows; ++i) m = off[i + 1] - off[i] > m ? off[i + 1] - off[i] : m;
+    return m;
+}
 
 hipError_t launch_features(hipStream_t st, const FeatArgs &a, bf16 *feat);
 hipError_t launch_copy_pred(hipStream_t st, const float4 *P, float4 *Ppred, uint32_t nv);

@@ -80,6 +80,7 @@ int ws_prepare(tc_sim *s) {
     TC_HIP(upf(s->ws_brest, rr));
     build_csr(nv, m.d_v0, m.d_v1, nullptr, &m.d_type, s->ws_goff_h, s->ws_gnbr_h, rr);
     TC_HIP(up32(s->ws_goff, s->ws_goff_h));
+    s->ws_gmaxdeg = tc::ws::max_degree(s->ws_goff_h.data(), nv);
     TC_HIP(up32(s->ws_gnbr, s->ws_gnbr_h));
     s->ws_lscale = m.mean_edge > 0.0f ? m.mean_edge : 0.01f;
     TC_HIP(s->ws_ppred.alloc(nv));
@@ -225,7 +226,7 @@ hipError_t ws_pre(tc_sim *s, float dt_sub) {
     hipError_t e = hipSuccess;
     if (need_feat && (e = tc::ws::launch_features(s->stream, feat_args(s, dt_sub), feat)) != hipSuccess) return e;
     if (active && s->ws_mode == 2) {
-        const tc::ws::Graph g{s->ws_goff.p, s->ws_gnbr.p};
+        const tc::ws::Graph g{s->ws_goff.p, s->ws_gnbr.p, s->ws_gmaxdeg};
         if ((e = tc::ws::forward(s->stream, s->ws_net->nl, s->ws_net->dev(), g, nv, infer_bufs(s, feat))) != hipSuccess)
             return e;
         e = tc::ws::launch_apply(s->stream, s->P.p, s->ws_ppred.p, s->ws_D.p, s->ws_y.p, sc, s->ws_beta, nv);
@@ -358,7 +359,7 @@ int tc_ws_net_forward_backward(tc_ws_net *n, const float *feat, uint32_t rows, c
     TC_HIP(n->gnbr.alloc(std::max<uint32_t>(nn, 1)));
     TC_HIP(hipMemcpyAsync(n->goff.p, graph_off, 4 * (size_t(rows) + 1), hipMemcpyHostToDevice, n->stream));
     if (nn) TC_HIP(hipMemcpyAsync(n->gnbr.p, graph_nbr, 4 * size_t(nn), hipMemcpyHostToDevice, n->stream));
-    const tc::ws::Graph g{n->goff.p, n->gnbr.p};
+    const tc::ws::Graph g{n->goff.p, n->gnbr.p, tc::ws::max_degree(graph_off, rows)};
     const tc::ws::FwdBufs fw = train_bufs(n, n->feat.p);
     TC_HIP(tc::ws::forward(n->stream, n->nl, n->dev(), g, rows, fw));
     if (targets) {
@@ -526,7 +527,7 @@ int tc_ws_train(tc_ws_net *n, tc_sim *const *sims, uint32_t nsims, uint32_t step
         tc_sim *s = sims[k];
         const uint32_t idx = uint32_t(splitmix64(st) % s->rec_count), rows = s->mesh.nv;
         const bf16 *feat = s->rec_feat.p + size_t(idx) * rows * tc::ws::F;
-        const tc::ws::Graph g{s->ws_goff.p, s->ws_gnbr.p};
+        const tc::ws::Graph g{s->ws_goff.p, s->ws_gnbr.p, s->ws_gmaxdeg};
         const tc::ws::FwdBufs fw = train_bufs(n, feat);
         TC_HIP(tc::ws::forward(n->stream, n->nl, n->dev(), g, rows, fw));
         TC_HIP(tc::ws::loss_and_backward(n->stream, n->nl, n->dev(), g, rows, fw, s->rec_T.p + size_t(idx) * rows,
@@ -576,7 +577,7 @@ int tc_ws_eval(tc_ws_net *n, tc_sim *const *sims, uint32_t nsims, double *mean_l
         const uint32_t rows = s->mesh.nv;
         int r = net_alloc_scratch(n, rows);
         if (r != TC_OK) return r;
-        const tc::ws::Graph g{s->ws_goff.p, s->ws_gnbr.p};
+        const tc::ws::Graph g{s->ws_goff.p, s->ws_gnbr.p, s->ws_gmaxdeg};
         for (uint32_t idx = 0; idx < s->rec_count; ++idx) {
             const tc::ws::FwdBufs fw = infer_like(n, s->rec_feat.p + size_t(idx) * rows * tc::ws::F);
             TC_HIP(tc::ws::forward(n->stream, n->nl, n->dev(), g, rows, fw));
@@ -619,7 +620,7 @@ int tc_sim_ws_time(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
         if (net) {
             TC_HIP(tc::ws::launch_features(s->stream, feat_args(s, dt_sub), s->ws_feat.p));
             TC_HIP(hipEventRecord(ev[2], s->stream));
-            const tc::ws::Graph g{s->ws_goff.p, s->ws_gnbr.p};
+            const tc::ws::Graph g{s->ws_goff.p, s->ws_gnbr.p, s->ws_gmaxdeg};
             TC_HIP(tc::ws::forward(s->stream, s->ws_net->nl, s->ws_net->dev(), g, nv, infer_bufs(s, s->ws_feat.p),
                                    &ev[3]));  // ev[3 .. 3 + R + 2]
         } else {
