@@ -55,7 +55,7 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t value, size_t n)
 hipError_t launch_residual(hipStream_t s, const float4 *P, const uint4 *drec, const float *dcc, const float *dlam,
                            uint32_t nd, const uint4 *brec, const float *blam, uint32_t nb, float dt, bool split,
                            double *part, double *out, uint32_t *gate, float threshold,
-                           const uint32_t *run_flag = nullptr);
+                           const uint32_t *run_flag = nullptr, uint32_t *gate_host = nullptr);
 
 // --- one iteration per launch over cone tiles (tc_fused.hip, plan from tc_tiles.cpp) ---
 struct FusedArgs {

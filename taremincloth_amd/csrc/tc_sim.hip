@@ -107,7 +107,7 @@ namespace {
 
 // The substep up to the end of the warm iterations.  With a warm start and a fallback, either the
 // whole gated fallback follows on the device (gate_to_host false) or only the first residual
-// check, with the gate's fallback flag copied to s->gate_host for the host to read.
+// check, whose kernel also writes the gate's fallback flag to s->gate_host for the host to read.
 hipError_t enqueue_substep_head(tc_sim *s, float dt_sub, bool gate_to_host, uint32_t &pc, uint32_t &lc) {
     const tc::SubstepArgs a = s->args(dt_sub);
     const tc::Mesh &m = s->mesh;
@@ -128,9 +128,8 @@ hipError_t enqueue_substep_head(tc_sim *s, float dt_sub, bool gate_to_host, uint
     if (!gate_to_host) return enqueue_gated_fallback(s, a, dt_sub, pc, lc);
     e = tc::launch_residual(s->stream, s->Pbuf(pc), s->drec.p, s->dcc.p, s->dlam_buf(lc), uint32_t(m.d_v0.size()),
                             s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()), dt_sub, s->split_compliance,
-                            s->resid_part.p, s->ws_res.p, s->ws_gate.p, s->ws_threshold, nullptr);
-    if (e != hipSuccess) return e;
-    return hipMemcpyAsync(s->gate_host, s->ws_gate.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream);
+                            s->resid_part.p, s->ws_res.p, s->ws_gate.p, s->ws_threshold, nullptr, s->gate_host);
+    return e;
 }
 
 // One fallback chunk of `count` iterations (gated on the device flag as well), then, unless it
@@ -144,9 +143,9 @@ hipError_t enqueue_fallback_chunk(tc_sim *s, float dt_sub, uint32_t count, bool 
     if (e != hipSuccess || last) return e;
     e = tc::launch_residual(s->stream, s->Pbuf(pc), s->drec.p, s->dcc.p, s->dlam_buf(lc), uint32_t(m.d_v0.size()),
                             s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()), dt_sub, s->split_compliance,
-                            s->resid_part.p, s->ws_res.p, s->ws_gate.p, s->ws_threshold, s->ws_gate.p);
-    if (e != hipSuccess) return e;
-    return hipMemcpyAsync(s->gate_host, s->ws_gate.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream);
+                            s->resid_part.p, s->ws_res.p, s->ws_gate.p, s->ws_threshold, s->ws_gate.p,
+                            s->gate_host);
+    return e;
 }
 
 // The substep after the projection loop: the warm start's bookkeeping, colliders, self-collision

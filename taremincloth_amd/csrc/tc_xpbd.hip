@@ -604,9 +604,13 @@ __global__ void __launch_bounds__(256) k_residual_part(const float4 *__restrict_
 // gate[1] += 1 (substeps checked), gate[2] += gate[0] (substeps that fell back).
 __global__ void __launch_bounds__(64) k_residual_final(const double *__restrict__ part, uint32_t nblocks,
                                                        double *__restrict__ out, uint32_t *__restrict__ gate,
-                                                       float threshold, const uint32_t *run_flag) {
+                                                       float threshold, const uint32_t *run_flag,
+                                                       uint32_t *gate_host) {
     // a chained check (run_flag set) after a check that passed keeps the gate closed
-    if (run_flag && *run_flag == 0u) return;
+    if (run_flag && *run_flag == 0u) {
+        if (gate_host && threadIdx.x == 0) *gate_host = 0u;
+        return;
+    }
     double hh = 0.0, ll = 0.0, mx = 0.0;
     for (uint32_t b = threadIdx.x; b < nblocks; b += 64) {
         hh += part[3 * b + 0];
@@ -625,6 +629,8 @@ __global__ void __launch_bounds__(64) k_residual_final(const double *__restrict_
         if (gate) {
             const uint32_t fail = rel > (double)threshold ? 1u : 0u;
             gate[0] = fail;
+            // pinned host copy of the flag (host-coherent at the kernel's end)
+            if (gate_host) *gate_host = fail;
             if (run_flag) {
                 gate[3] += fail;  // another fallback chunk
             } else {
@@ -720,12 +726,14 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t value, size_t n)
 
 hipError_t launch_residual(hipStream_t s, const float4 *P, const uint4 *drec, const float *dcc, const float *dlam,
                            uint32_t nd, const uint4 *brec, const float *blam, uint32_t nb, float dt, bool split,
-                           double *part, double *out, uint32_t *gate, float threshold, const uint32_t *run_flag) {
+                           double *part, double *out, uint32_t *gate, float threshold, const uint32_t *run_flag,
+                           uint32_t *gate_host) {
     const size_t n = size_t(nd) + nb;
     const uint32_t blocks = uint32_t(std::min<size_t>(RES_BLOCKS, std::max<size_t>(grid_for(n, 256), 1)));
     hipLaunchKernelGGL(k_residual_part, dim3(blocks), dim3(256), 0, s, P, drec, dcc, dlam, nd, brec, blam, nb, dt,
                        split ? 1 : 0, part, run_flag);
-    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, s, part, blocks, out, gate, threshold, run_flag);
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, s, part, blocks, out, gate, threshold, run_flag,
+                       gate_host);
     return hipGetLastError();
 }
 
