@@ -131,6 +131,29 @@ def hub_graph(n, seed=0):
     return off, nbr
 
 
+def lattice_graph(w, h):
+    """4-neighbour lattice of w x h rows: degree 2 to 4, the shallow prefetch of the round kernel."""
+    idx = np.arange(w * h).reshape(h, w)
+    a = np.concatenate([idx[:, :-1].ravel(), idx[:-1, :].ravel()])
+    b = np.concatenate([idx[:, 1:].ravel(), idx[1:, :].ravel()])
+    off, nbr, _ = W.csr_pairs(w * h, a, b)
+    return off, nbr
+
+
+@pytest.mark.gpu
+def test_gpu_forward_on_a_degree_4_lattice_matches_oracle(require_gpu):
+    rows, rounds = 12 * 13, 3
+    p, feat, _, _, t = random_case(rows, rounds, 31)
+    off, nbr = lattice_graph(12, 13)
+    assert np.diff(off).max() == 4
+    net = _net(rounds, p)
+    y, g, loss = net.forward_backward(feat, off, nbr, t)
+    lo, go, yo = W.backward(p, rounds, feat, off, nbr, t, bf16=True)
+    assert np.abs(y - yo).max() / np.abs(yo).max() < 2e-2
+    assert abs(loss - lo) <= 2e-2 * lo
+    assert np.linalg.norm(g - go) / np.linalg.norm(go) < 5e-2
+
+
 @pytest.mark.gpu
 def test_gpu_forward_with_hub_and_isolated_vertices_matches_oracle(require_gpu):
     """Rows of degree above the prefetched neighbours finish their mean in a loop; an isolated
