@@ -1,6 +1,6 @@
 #!/usr/bin/env python3
 """Substeps/sec of the XPBD cloth substep at a 512x512 grid on one MI355X:
-learned warm start (network + 8 plain iterations, residual-gated fallback) against
+learned warm start (network + 4 plain iterations, residual-gated fallback) against
 plain XPBD at 64 iterations, compared at equal relative constraint residual.
 
 A step is one substep over the resident cloth state: predict -> [features ->
@@ -66,7 +66,7 @@ def grid_512(n: int, spacing: float = 0.01, height: float = 0.5, seed: int = 0):
     return pos, edges, faces
 
 
-EQUIV_ITERS = (8, 16, 32)
+EQUIV_ITERS = (2, 4, 8, 16, 32)
 
 
 def equivalent_iterations(curve: dict, r: float) -> float:
@@ -330,8 +330,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--iters", type=int, default=64, help="iterations of the plain arm")
-    ap.add_argument("--ws-iters", type=int, default=8, help="plain iterations after the network")
-    ap.add_argument("--ws-fallback", type=int, default=56, help="fallback iterations when the gate fails")
+    ap.add_argument("--ws-iters", type=int, default=4, help="plain iterations after the network")
+    ap.add_argument("--ws-fallback", type=int, default=60, help="fallback iterations when the gate fails")
     ap.add_argument("--weights", default=os.path.join(HERE, "taremincloth_amd", "weights", "warmstart_r3.npz"))
     ap.add_argument("--checks", type=int, default=10, help="states compared at equal residual")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -411,7 +411,7 @@ def main():
         sim.set_solver_iterations(args.iters)
 
     # equal-residual calibration over the trajectory window: at `checks` states, the relative
-    # residual after the plain 64-iteration solve and after network + 8 iterations, both from
+    # residual after the plain 64-iteration solve and after network + the warm iterations, both from
     # the same state and previous correction
     checks = []
     if not args.light:
@@ -434,7 +434,7 @@ def main():
             sim.substep_parts(dt_sub, collide=False, self_collide=False, update=False)
             r_learn, m_learn = sim.residual(dt_sub)
             # plain XPBD at fewer iterations from the same state: how many plain iterations the
-            # network plus its 8 iterations are worth
+            # network plus its warm iterations are worth
             curve = {}
             for k in EQUIV_ITERS:
                 plain_on()
