@@ -285,7 +285,7 @@ def batch_bench(T, frames: int, warmup: int, cpu_sample: bool):
     b.synchronize()
     el = time.perf_counter() - t0
     sps = frames * per_frame / el
-    nd, nb, nv = b.num_distance_constraints(), b.num_bending_constraints(), b.num_vertices()
+    nd, nb, nv = b.num_distance_constraints, b.num_bending_constraints, b.num_vertices
     kt = b.time_kernels(dt_sub, reps=3)
     sides = [int(round(len(d["positions"]) ** 0.5)) for d in inst]
     out = {"workload": f"64 curtains, side U[16, 256] (seed 0; {min(sides)}..{max(sides)}), {nv} vertices, "
@@ -386,7 +386,7 @@ def main():
     per_frame = 10
     dt_frame = 1.0 / 60.0
     dt_sub = dt_frame / per_frame
-    nd, nb = sim.num_distance_constraints(), sim.num_bending_constraints()
+    nd, nb = sim.num_distance_constraints, sim.num_bending_constraints
 
     per_colour_ms, tune = None, None
     if not args.light:

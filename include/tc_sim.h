@@ -30,7 +30,8 @@ enum {
     TC_OK = 0,
     TC_ERR_VALUE = 1,    /* bad shape / index / argument  (PyValueError in the reference) */
     TC_ERR_RUNTIME = 2,  /* device or allocation failure  (PyRuntimeError in the reference) */
-    TC_ERR_UNSUPPORTED = 3
+    TC_ERR_UNSUPPORTED = 3,
+    TC_ERR_IO = 4        /* file could not be written       (PyOSError in the reference) */
 };
 
 typedef struct tc_sim tc_sim;
@@ -196,6 +197,28 @@ int tc_sim_substep_parts(tc_sim *sim, float dt_sub, int collide, int self_collid
  * first and last event  [9] all one-launch-per-iteration launches  [10] those launches per substep
  * [11] reserved (0).  No reference counterpart (the reference's wgpu-profiler scopes, simulation/profile.rs). */
 int tc_sim_time_kernels(tc_sim *sim, float dt_sub, uint32_t reps, double *out);
+/* ClothSimulator.set_profiling_enabled(enable) -> bool, lib.rs:1975-1978 -> GpuProfiler::set_enabled
+ * (simulation/profile.rs:76-83).  While on, frames launch kernel by kernel and HIP events on the
+ * simulator's stream bracket each pass under the reference's pass names: predict, solver_iter (one per
+ * iteration), collider_decoupled, self_collision_Outer (a scope holding hash_build, sc_solve,
+ * sc_solve_ee, sc_apply), update_vel, plus warm_start, residual_gate and warm_start_update for the
+ * learned warm start.  *enabled (may be NULL) receives the resulting state; HIP events are always
+ * available, so it equals the request.  Turning it on or off clears what was recorded. */
+int tc_sim_set_profiling_enabled(tc_sim *sim, int enable, int *enabled);
+/* ClothSimulator.take_profile() -> [(pass, ms)], lib.rs:1980-1983 -> take_ms (profile.rs:206-230):
+ * waits for the simulator's stream and keeps the passes of the last frame stepped; *count receives
+ * their number (0 when profiling is off or nothing ran).  A later call without a new frame returns the
+ * same passes, as the reference returns its previous values.  Entries, in recording order (a scope
+ * before its passes), come from tc_sim_profile_entry: name (owned by the simulator, valid until the
+ * next take), milliseconds, start relative to the frame's first pass, nesting depth. */
+int tc_sim_take_profile(tc_sim *sim, size_t *count);
+int tc_sim_profile_entry(const tc_sim *sim, size_t index, const char **name, float *ms, float *start_ms,
+                         uint32_t *depth);
+/* ClothSimulator.save_profile_trace(path), lib.rs:1985-1990 -> save_chrometrace (profile.rs:232-237):
+ * the passes of the last take as Chrome trace JSON (complete events in microseconds).  TC_ERR_IO when
+ * the file cannot be written (the reference raises OSError). */
+int tc_sim_save_profile_trace(tc_sim *sim, const char *path);
+
 /* Block until all work on the simulator's stream has finished. */
 int tc_sim_synchronize(tc_sim *sim);
 

@@ -82,6 +82,7 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.orc_sim_set_state.argtypes = [_vp, _vp, _vp, _vp]
     L.orc_sim_get.argtypes = [_vp, C.c_int, _f64p]
     L.orc_sim_get_lambdas.argtypes = [_vp, _f64p, _f64p]
+    L.orc_sim_set_lambdas.argtypes = [_vp, _vp, _vp]
     L.orc_sim_step.restype = C.c_uint32
     L.orc_sim_step.argtypes = [_vp, C.c_float, C.c_uint32]
     L.orc_sim_step_single_substep.argtypes = [_vp, C.c_float]
@@ -298,6 +299,12 @@ class OracleSim:
         ld, lb = np.empty(m["nd"], np.float64), np.empty(m["nb"], np.float64)
         self._L.orc_sim_get_lambdas(self._h, ld, lb)
         return ld, lb
+
+    def set_lambdas(self, ld=None, lb=None):
+        """Overwrite the lambdas (distance in colour order, bending); a test hook."""
+        a = None if ld is None else np.ascontiguousarray(ld, np.float64)
+        b = None if lb is None else np.ascontiguousarray(lb, np.float64)
+        self._L.orc_sim_set_lambdas(self._h, _ptr(a), _ptr(b))
 
     def contact_stats(self, clear=True) -> dict[str, int]:
         """Contacts resolved since the last call: vv, vt_ccd, vt_prox, ee_ccd, ee_prox."""

@@ -1225,6 +1225,11 @@ void orc_sim_get_lambdas(const orc_sim *s, double *ld, double *lb) {
     for (uint32_t k = 0; k < s->m->nd; k++) ld[k] = s->lam_d[k];
     for (uint32_t k = 0; k < s->m->nb; k++) lb[k] = s->lam_b[k];
 }
+/* Test hook: overwrite the lambdas (distance in colour order, then bending); either may be NULL. */
+void orc_sim_set_lambdas(orc_sim *s, const double *ld, const double *lb) {
+    for (uint32_t k = 0; ld && k < s->m->nd; k++) s->lam_d[k] = (real)ld[k];
+    for (uint32_t k = 0; lb && k < s->m->nb; k++) s->lam_b[k] = (real)lb[k];
+}
 
 /* predict.wgsl:33-55 */
 static void k_predict(orc_sim *s, real dt_sub) {
@@ -2118,9 +2123,12 @@ void orc_sim_substep_parts(orc_sim *s, float dt_sub, int do_collide, int do_self
     if (do_update) k_update_vel(s, dt);
 }
 
-/* Relative XPBD residual r = ||h|| / ||L0|| with h_j = C_j(p) + alpha~_j lambda_j over
- * distance, shear and bending constraints (constraints the projection skips are
- * excluded).  Also returns max |h_j| / L0_j.  No reference counterpart. */
+/* Relative XPBD residual r = ||h|| / ||L0|| over distance, shear and bending constraints,
+ * h_j = C_j(p) + alpha~_j lambda_j + gamma_j grad C_j . (p - x): the quantity the damped
+ * projection of distance.wgsl:60-124 / bending.wgsl:59-126 drives to zero (its numerator),
+ * with alpha~ = alpha / dt^2 and gamma = min(damping * 1e-5 / dt, 1e4) (zero without
+ * constraint damping).  Constraints the projection skips (w0 + w1 <= 1e-7, |p0 - p1| <= 1e-7,
+ * compliance >= 1e9) are excluded.  Also returns max |h_j| / L0_j.  No reference counterpart. */
 void orc_sim_residual(const orc_sim *s, float dt_sub, double *out_rel, double *out_maxrel) {
     const orc_mesh *m = s->m;
     double dt = dt_sub > 1e-8f ? dt_sub : 1e-8;
@@ -2137,6 +2145,15 @@ void orc_sim_residual(const orc_sim *s, float dt_sub, double *out_rel, double *o
         double comp = c >= 0 ? (double)s->d_tc[k] : (double)s->d_cc[k];
         if (comp >= 1e9) continue;
         double h = c + comp / (dt * dt) * (double)s->lam_d[k];
+        double damp = m->d_type[k] == 1 ? (double)s->shear_damping
+                                        : (c >= 0 ? (double)s->tension_damping : (double)s->compression_damping);
+        if (damp > 0) {
+            double gamma = fmin(damp * (double)1e-5f / dt, 1e4);
+            double rx = ((double)s->p[a].x - s->x[a].x) - ((double)s->p[b].x - s->x[b].x);
+            double ry = ((double)s->p[a].y - s->x[a].y) - ((double)s->p[b].y - s->x[b].y);
+            double rz = ((double)s->p[a].z - s->x[a].z) - ((double)s->p[b].z - s->x[b].z);
+            h += gamma * (dx * rx + dy * ry + dz * rz) / d;
+        }
         hh += h * h; ll += L * L;
         if (L > 0 && fabs(h) / L > mx) mx = fabs(h) / L;
     }
@@ -2151,6 +2168,13 @@ void orc_sim_residual(const orc_sim *s, float dt_sub, double *out_rel, double *o
         double d = sqrt(dx * dx + dy * dy + dz * dz);
         if (d <= 1e-7) continue;
         double h = (d - L) + comp / (dt * dt) * (double)s->lam_b[k];
+        if (s->bending_damping > 0) {
+            double gamma = fmin((double)s->bending_damping * (double)1e-5f / dt, 1e4);
+            double rx = ((double)s->p[a].x - s->x[a].x) - ((double)s->p[b].x - s->x[b].x);
+            double ry = ((double)s->p[a].y - s->x[a].y) - ((double)s->p[b].y - s->x[b].y);
+            double rz = ((double)s->p[a].z - s->x[a].z) - ((double)s->p[b].z - s->x[b].z);
+            h += gamma * (dx * rx + dy * ry + dz * rz) / d;
+        }
         hh += h * h; ll += L * L;
         if (L > 0 && fabs(h) / L > mx) mx = fabs(h) / L;
     }

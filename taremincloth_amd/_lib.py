@@ -23,6 +23,7 @@ EXPORTS = [
     "tc_sim_num_vertices",
     "tc_sim_num_distance_constraints", "tc_sim_num_bending_constraints", "tc_sim_get_array",
     "tc_sim_build_hashes", "tc_sim_residual", "tc_sim_substep_parts", "tc_sim_time_kernels", "tc_sim_synchronize",
+    "tc_sim_set_profiling_enabled", "tc_sim_take_profile", "tc_sim_profile_entry", "tc_sim_save_profile_trace",
     "tc_mesh_build_arrays", "tc_color_edge_pairs", "tc_compute_inv_masses", "tc_last_error", "tc_version",
     "tc_ws_net_create", "tc_ws_net_destroy", "tc_ws_net_num_params", "tc_ws_net_rounds", "tc_ws_net_set_params",
     "tc_ws_net_get_params", "tc_ws_net_init", "tc_ws_net_forward_backward", "tc_ws_net_adam_step", "tc_ws_train",
@@ -33,7 +34,7 @@ EXPORTS = [
     "tc_sdf_result_timing", "tc_sdf_result_texels", "tc_sdf_result_nearest", "tc_sdf_result_free", "tc_close_open_boundary_loops",
 ]
 
-TC_OK, TC_ERR_VALUE, TC_ERR_RUNTIME, TC_ERR_UNSUPPORTED = 0, 1, 2, 3
+TC_OK, TC_ERR_VALUE, TC_ERR_RUNTIME, TC_ERR_UNSUPPORTED, TC_ERR_IO = 0, 1, 2, 3, 4
 
 
 class TcInstance(C.Structure):
@@ -107,6 +108,11 @@ def load() -> C.CDLL:
         "tc_sim_substep_parts": (_i, [_vp, _f32, _i, _i, _i]),
         "tc_sim_time_kernels": (_i, [_vp, _f32, _u32, C.POINTER(C.c_double)]),
         "tc_sim_synchronize": (_i, [_vp]),
+        "tc_sim_set_profiling_enabled": (_i, [_vp, _i, C.POINTER(_i)]),
+        "tc_sim_take_profile": (_i, [_vp, C.POINTER(C.c_size_t)]),
+        "tc_sim_profile_entry": (_i, [_vp, C.c_size_t, C.POINTER(C.c_char_p), C.POINTER(_f32), C.POINTER(_f32),
+                                      C.POINTER(_u32)]),
+        "tc_sim_save_profile_trace": (_i, [_vp, C.c_char_p]),
         "tc_mesh_build_arrays": (_i, [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, C.POINTER(TcParams), _i, _vp,
                                       C.c_size_t, C.POINTER(C.c_size_t)]),
         "tc_color_edge_pairs": (_i, [_u32, _vp, _u32, _vp]),
@@ -160,4 +166,6 @@ def check(status: int) -> None:
         raise ValueError(msg)
     if status == TC_ERR_UNSUPPORTED:
         raise NotImplementedError(msg)
+    if status == TC_ERR_IO:
+        raise OSError(msg)
     raise RuntimeError(msg)

@@ -61,6 +61,7 @@ struct Collider {
     float ax, ay, az, radius;
     float bx, by, bz, pad1;
 };
+static_assert(sizeof(Collider) == 48, "GpuCollider is 48 bytes (mesh.rs:65-74)");
 
 // Mesh collider triangle (GpuMeshTriangle, mesh.rs:78-87) + bounding sphere.
 struct MeshTri {
@@ -78,5 +79,6 @@ struct Accum {
     int32_t cpx, cpy, cpz, cnx, cny, cnz;
     uint32_t untangle, ee_count;
 };
+static_assert(sizeof(Accum) == 64, "SelfCollisionAccum is 64 bytes (simulation/types.rs:145-178)");
 
 }  // namespace tc

@@ -12,6 +12,7 @@
 // constraints are generated in, so the generation order below is the
 // reference's: stretch in input-edge order, then shear per sorted triangle-edge
 // key followed by its cross pair, then bending per sorted original-edge key.
+#include "tc_layout.h"
 #include "tc_mesh.h"
 
 #include <algorithm>

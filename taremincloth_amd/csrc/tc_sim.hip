@@ -15,12 +15,15 @@
 
 #include <algorithm>
 #include <array>
+#include <cerrno>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "tc_layout.h"
 #include "../../include/tc_sim.h"
 #include "tc_kernels.h"
 #include "tc_mesh.h"
@@ -57,9 +60,11 @@ hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t coun
         const tc::FusedArgs fa = s->fused_args();
         const tc::FusedPlanDev pd = s->plan_dev();
         for (uint32_t it = 0; it < count; ++it) {
+            const int q = s->prof_begin("solver_iter");
             e = tc::launch_tile_iteration(s->stream, s->Pbuf(pc), s->Pbuf(pc ^ 1u), s->X.p, pd, s->dlam_buf(lc),
                                           s->dlam_buf(lc ^ 1u), s->blam_buf(lc), s->blam_buf(lc ^ 1u), fa, a,
                                           first && it == 0, s->fused_block);
+            s->prof_end(q);
             if (e != hipSuccess) return e;
             pc ^= 1u;
             lc ^= 1u;
@@ -68,6 +73,7 @@ hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t coun
     }
     for (uint32_t it = 0; it < count; ++it) {
         const bool f = first && it == 0;
+        const int q = s->prof_begin("solver_iter");
         for (size_t c = 0; c < m.d_col.count.size(); ++c) {
             e = tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p, m.d_col.offset[c],
                                     m.d_col.count[c], a, f, s->split_compliance, B);
@@ -78,6 +84,7 @@ hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t coun
                                    m.b_col.count[c], a, f, B);
             if (e != hipSuccess) return e;
         }
+        s->prof_end(q);
     }
     return e;
 }
@@ -92,10 +99,12 @@ hipError_t enqueue_gated_fallback(tc_sim *s, const tc::SubstepArgs &a, float dt_
     fa.run_flag = s->ws_gate.p;
     hipError_t e = hipSuccess;
     for (uint32_t done = 0; done < s->ws_fallback; done += chunk) {
-        e = tc::launch_residual(s->stream, s->Pbuf(pc), s->drec.p, s->dcc.p, s->dlam_buf(lc), uint32_t(m.d_v0.size()),
-                                s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()), dt_sub, s->split_compliance,
+        const int q = s->prof_begin("residual_gate");
+        e = tc::launch_residual(s->stream, s->Pbuf(pc), s->X.p, s->drec.p, s->dcc.p, s->dlam_buf(lc), uint32_t(m.d_v0.size()),
+                                s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()), a, s->split_compliance,
                                 s->resid_part.p, s->ws_res.p, s->ws_gate.p, s->ws_threshold,
                                 done == 0 ? nullptr : s->ws_gate.p);
+        s->prof_end(q);
         if (e != hipSuccess) return e;
         if ((e = enqueue_iterations(s, fa, std::min(chunk, s->ws_fallback - done), false, pc, lc)) != hipSuccess)
             return e;
@@ -112,9 +121,16 @@ hipError_t enqueue_substep_head(tc_sim *s, float dt_sub, bool gate_to_host, uint
     const tc::SubstepArgs a = s->args(dt_sub);
     const tc::Mesh &m = s->mesh;
     const uint32_t n = m.nv, B = s->block;
+    int q = s->prof_begin("predict");
     hipError_t e = tc::launch_predict(s->stream, s->P.p, s->X.p, s->V.p, n, a, B);
+    s->prof_end(q);
     if (e != hipSuccess) return e;
-    if (s->ws_tracking() && (e = ws_pre(s, dt_sub)) != hipSuccess) return e;
+    if (s->ws_tracking()) {
+        q = s->prof_begin("warm_start");
+        e = ws_pre(s, dt_sub);
+        s->prof_end(q);
+        if (e != hipSuccess) return e;
+    }
     const uint32_t iters = s->iters_effective();
     pc = 0;
     lc = 0;
@@ -126,9 +142,11 @@ hipError_t enqueue_substep_head(tc_sim *s, float dt_sub, bool gate_to_host, uint
     if ((e = enqueue_iterations(s, a, iters, true, pc, lc)) != hipSuccess) return e;
     if (!s->ws_active() || s->ws_fallback == 0) return e;
     if (!gate_to_host) return enqueue_gated_fallback(s, a, dt_sub, pc, lc);
-    e = tc::launch_residual(s->stream, s->Pbuf(pc), s->drec.p, s->dcc.p, s->dlam_buf(lc), uint32_t(m.d_v0.size()),
-                            s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()), dt_sub, s->split_compliance,
+    q = s->prof_begin("residual_gate");
+    e = tc::launch_residual(s->stream, s->Pbuf(pc), s->X.p, s->drec.p, s->dcc.p, s->dlam_buf(lc), uint32_t(m.d_v0.size()),
+                            s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()), a, s->split_compliance,
                             s->resid_part.p, s->ws_res.p, s->ws_gate.p, s->ws_threshold, nullptr, s->gate_host);
+    s->prof_end(q);
     return e;
 }
 
@@ -141,8 +159,8 @@ hipError_t enqueue_fallback_chunk(tc_sim *s, float dt_sub, uint32_t count, bool 
     fa.run_flag = s->ws_gate.p;
     hipError_t e = enqueue_iterations(s, fa, count, false, pc, lc);
     if (e != hipSuccess || last) return e;
-    e = tc::launch_residual(s->stream, s->Pbuf(pc), s->drec.p, s->dcc.p, s->dlam_buf(lc), uint32_t(m.d_v0.size()),
-                            s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()), dt_sub, s->split_compliance,
+    e = tc::launch_residual(s->stream, s->Pbuf(pc), s->X.p, s->drec.p, s->dcc.p, s->dlam_buf(lc), uint32_t(m.d_v0.size()),
+                            s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()), fa, s->split_compliance,
                             s->resid_part.p, s->ws_res.p, s->ws_gate.p, s->ws_threshold, s->ws_gate.p,
                             s->gate_host);
     return e;
@@ -158,21 +176,48 @@ hipError_t enqueue_substep_tail(tc_sim *s, float dt_sub, uint32_t pc, uint32_t l
     s->last_p = pc;
     s->last_lam = lc;
     hipError_t e = hipSuccess;
-    if (s->ws_tracking() && (e = ws_post(s, dt_sub, Pc)) != hipSuccess) return e;
+    int q;
+    if (s->ws_tracking()) {
+        q = s->prof_begin("warm_start_update");
+        e = ws_post(s, dt_sub, Pc);
+        s->prof_end(q);
+        if (e != hipSuccess) return e;
+    }
     if (collide && (!s->cols_h.empty() || !s->tris_h.empty() || s->sdf_nb > 0)) {
+        q = s->prof_begin("collider_decoupled");
         e = tc::launch_collision(s->stream, Pc, s->X.p, n, s->col_order.p, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
                                  s->tri_bounds.p, uint32_t(s->tris_h.size()), s->tri_cbox.p, s->single_sided_recovery, dt_sub, B, s->sdf_args());
+        s->prof_end(q);
         if (e != hipSuccess) return e;
     }
     if (self_collide && s->self_collision) {
-        if ((e = enqueue_hashes(s, Pc)) != hipSuccess) return e;
+        const int scope = s->prof_begin("self_collision_Outer");
+        q = s->prof_begin("hash_build");
+        e = enqueue_hashes(s, Pc);
+        s->prof_end(q);
+        if (e != hipSuccess) return e;
         const tc::SelfColArgs sa = s->sc_args();
         const tc::SelfColTopo t = s->topo();
-        if ((e = tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa)) != hipSuccess) return e;
-        if ((e = tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa)) != hipSuccess) return e;
-        if ((e = tc::launch_sc_apply(s->stream, Pc, s->local_len.p, s->accum.p, sa, B)) != hipSuccess) return e;
+        q = s->prof_begin("sc_solve");
+        e = tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa);
+        s->prof_end(q);
+        if (e != hipSuccess) return e;
+        q = s->prof_begin("sc_solve_ee");
+        e = tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa);
+        s->prof_end(q);
+        if (e != hipSuccess) return e;
+        q = s->prof_begin("sc_apply");
+        e = tc::launch_sc_apply(s->stream, Pc, s->local_len.p, s->accum.p, sa, B);
+        s->prof_end(q);
+        s->prof_end(scope);
+        if (e != hipSuccess) return e;
     }
-    if (update) return tc::launch_update_vel(s->stream, Pc, s->X.p, s->V.p, n, dt_sub, B);
+    if (update) {
+        q = s->prof_begin("update_vel");
+        e = tc::launch_update_vel(s->stream, Pc, s->X.p, s->V.p, n, dt_sub, B);
+        s->prof_end(q);
+        return e;
+    }
     return hipSuccess;
 }
 
@@ -270,6 +315,14 @@ int run_frame(tc_sim *s, float dt, uint32_t substeps) {
     } else {
         s->last_p = 0;
         s->last_lam = 0;
+    }
+    if (s->prof.enabled) {  // pass events are recorded between the launches, kernel by kernel
+        s->prof.frame_begin();
+        s->prof.active = true;
+        const hipError_t e = enqueue_frame(s, dt, substeps);
+        s->prof.active = false;
+        TC_HIP(e);
+        return TC_OK;
     }
     if (!s->use_graph || s->rec_cap > 0) {  // recording picks its sample slot per substep on the host
         TC_HIP(enqueue_frame(s, dt, substeps));
@@ -863,6 +916,67 @@ int tc_sim_autotune(tc_sim *s, float dt_sub, double *out) {
     return TC_OK;
 }
 
+int tc_sim_set_profiling_enabled(tc_sim *s, int enable, int *enabled) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    s->prof.enabled = enable != 0;
+    s->prof.frame_begin();
+    s->prof.last.clear();
+    if (enabled) *enabled = s->prof.enabled ? 1 : 0;
+    return TC_OK;
+}
+
+int tc_sim_take_profile(tc_sim *s, size_t *count) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (s->prof.enabled && !s->prof.cur.empty()) {
+        TC_HIP(hipStreamSynchronize(s->stream));
+        TcProfiler &P = s->prof;
+        std::vector<TcProfiler::Entry> out;
+        out.reserve(P.cur.size());
+        const hipEvent_t t0 = P.pool[P.cur.front().ev0];
+        for (const TcProfiler::Scope &c : P.cur) {
+            float ms = 0.0f, start = 0.0f;
+            if (c.ev1 != c.ev0) TC_HIP(hipEventElapsedTime(&ms, P.pool[c.ev0], P.pool[c.ev1]));
+            TC_HIP(hipEventElapsedTime(&start, t0, P.pool[c.ev0]));
+            out.push_back({c.name, c.depth, start, ms});
+        }
+        P.last.swap(out);
+        P.cur.clear();
+    }
+    if (!s->prof.enabled) s->prof.last.clear();
+    if (count) *count = s->prof.last.size();
+    return TC_OK;
+}
+
+int tc_sim_profile_entry(const tc_sim *s, size_t index, const char **name, float *ms, float *start_ms,
+                         uint32_t *depth) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (index >= s->prof.last.size()) return fail(TC_ERR_VALUE, "profile entry index out of range");
+    const TcProfiler::Entry &e = s->prof.last[index];
+    if (name) *name = e.name.c_str();
+    if (ms) *ms = e.ms;
+    if (start_ms) *start_ms = e.start_ms;
+    if (depth) *depth = e.depth;
+    return TC_OK;
+}
+
+int tc_sim_save_profile_trace(tc_sim *s, const char *path) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (!path) return fail(TC_ERR_VALUE, "path is null");
+    FILE *f = std::fopen(path, "w");
+    if (!f) return fail(TC_ERR_IO, std::string("cannot open ") + path + ": " + std::strerror(errno));
+    // Chrome trace event format: complete events ("ph": "X") in microseconds, nested by time.
+    std::fputs("{\"traceEvents\": [", f);
+    for (size_t i = 0; i < s->prof.last.size(); ++i) {
+        const TcProfiler::Entry &e = s->prof.last[i];
+        std::fprintf(f, "%s\n{\"name\": \"%s\", \"ph\": \"X\", \"pid\": 0, \"tid\": %u, \"ts\": %.3f, \"dur\": %.3f}",
+                     i ? "," : "", e.name.c_str(), e.depth, double(e.start_ms) * 1e3, double(e.ms) * 1e3);
+    }
+    std::fputs("\n], \"displayTimeUnit\": \"ms\"}\n", f);
+    const bool bad = std::ferror(f) != 0;
+    if (std::fclose(f) != 0 || bad) return fail(TC_ERR_IO, std::string("cannot write ") + path);
+    return TC_OK;
+}
+
 int tc_sim_synchronize(tc_sim *s) {
     if (!s) return fail(TC_ERR_VALUE, "null simulator");
     TC_HIP(hipStreamSynchronize(s->stream));
@@ -1283,9 +1397,9 @@ int tc_sim_residual(tc_sim *s, float dt_sub, double *rel, double *max_rel) {
     if (!s) return fail(TC_ERR_VALUE, "null simulator");
     const tc::Mesh &m = s->mesh;
     double h[4] = {0.0, 0.0, 0.0, 0.0};
-    TC_HIP(tc::launch_residual(s->stream, s->Pbuf(s->last_p), s->drec.p, s->dcc.p, s->dlam_buf(s->last_lam),
+    TC_HIP(tc::launch_residual(s->stream, s->Pbuf(s->last_p), s->X.p, s->drec.p, s->dcc.p, s->dlam_buf(s->last_lam),
                                uint32_t(m.d_v0.size()), s->brec.p, s->blam_buf(s->last_lam), uint32_t(m.b_v0.size()),
-                               dt_sub, s->split_compliance, s->resid_part.p, s->resid.p, nullptr, 0.0f));
+                               s->args(dt_sub), s->split_compliance, s->resid_part.p, s->resid.p, nullptr, 0.0f));
     TC_HIP(hipMemcpyAsync(h, s->resid.p, sizeof(h), hipMemcpyDeviceToHost, s->stream));
     TC_HIP(hipStreamSynchronize(s->stream));
     if (rel) *rel = h[1] > 0.0 ? std::sqrt(h[0]) / std::sqrt(h[1]) : 0.0;

@@ -80,6 +80,60 @@ struct tc_ws_net {
     tc::ws::NetDev dev() const { return {theta.p, w.p, wt.p}; }
 };
 
+// Pass timing with HIP events under the reference's pass names (simulation/profile.rs): a
+// scope's start and end events are recorded on the simulator's stream around the pass, with its
+// nesting depth; take() waits for the stream and turns the frame's events into milliseconds.
+struct TcProfiler {
+    struct Scope {
+        const char *name;
+        uint32_t depth;
+        size_t ev0, ev1;
+    };
+    struct Entry {
+        std::string name;
+        uint32_t depth;
+        float start_ms, ms;
+    };
+    bool enabled = false;
+    bool active = false;           // inside a frame of run_frame (other entry points record nothing)
+    std::vector<hipEvent_t> pool;  // events of the frame being recorded, reused frame after frame
+    size_t used = 0;
+    uint32_t depth = 0;
+    std::vector<Scope> cur;        // scopes of the last frame enqueued
+    std::vector<Entry> last;       // the last frame taken
+    ~TcProfiler() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+    void frame_begin() {
+        used = 0;
+        depth = 0;
+        cur.clear();
+    }
+    // Records a start event; returns the scope index (or -1 when disabled / out of events).
+    int begin(hipStream_t s, const char *name) {
+        if (!enabled || !active) return -1;
+        const size_t e = next();
+        if (e == SIZE_MAX || hipEventRecord(pool[e], s) != hipSuccess) return -1;
+        cur.push_back({name, depth++, e, e});
+        return int(cur.size() - 1);
+    }
+    void end(hipStream_t s, int idx) {
+        if (!enabled || !active || idx < 0) return;
+        const size_t e = next();
+        if (depth) --depth;
+        if (e == SIZE_MAX || hipEventRecord(pool[e], s) != hipSuccess) return;
+        cur[size_t(idx)].ev1 = e;
+    }
+    size_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return SIZE_MAX;
+            pool.push_back(e);
+        }
+        return used++;
+    }
+};
+
 struct tc_sim {
     tc::Mesh mesh;
     hipStream_t stream = nullptr;
@@ -197,6 +251,9 @@ struct tc_sim {
     uint32_t *gate_host = nullptr;  // pinned host copy of the gate's fallback flag
     uint64_t epoch = 1;  // bumped whenever a parameter or buffer baked into the graph changes
     bool use_graph = true;  // TCLOTH_USE_GRAPH=0 launches kernel by kernel (profilers)
+    TcProfiler prof;        // set_profiling_enabled: frames launch kernel by kernel with pass events
+    int prof_begin(const char *name) { return prof.begin(stream, name); }
+    void prof_end(int idx) { prof.end(stream, idx); }
 
     ~tc_sim() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);

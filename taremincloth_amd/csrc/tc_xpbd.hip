@@ -539,11 +539,13 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n) {
     if (i < n) p[i] = value;
 }
 
-// Residual h_j = C_j + alpha~_j lambda_j over distance and bending constraints,
-// accumulated in double.  Pass 1: each block strides over the constraints and
-// writes (sum h^2, sum L0^2, max |h|/L0) to its slot of `part`; pass 2 (one
-// block) sums the slots in block order, so the result does not depend on
-// scheduling.
+// Residual h_j = C_j + alpha~_j lambda_j + gamma_j grad C_j . (p - x) over distance and
+// bending constraints, accumulated in double: the numerator of the damped projection's
+// update (distance.wgsl:60-124, bending.wgsl:59-126), which the iterations drive to zero.
+// gamma = min(damping * 1e-5 / dt, 1e4) is zero without constraint damping, and then X is not
+// read.  Pass 1: each block strides over the constraints and writes (sum h^2, sum L0^2,
+// max |h|/L0) to its slot of `part`; pass 2 (one block) sums the slots in block order, so the
+// result does not depend on scheduling.
 constexpr uint32_t RES_BLOCKS = 1024;
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -555,12 +557,15 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-__global__ void __launch_bounds__(256) k_residual_part(const float4 *__restrict__ P, const uint4 *__restrict__ drec,
+template <bool DAMP>
+__global__ void __launch_bounds__(256) k_residual_part(const float4 *__restrict__ P, const float4 *__restrict__ X,
+                                                       const uint4 *__restrict__ drec,
                                                        const float *__restrict__ dcc, const float *__restrict__ dlam,
                                                        uint32_t nd, const uint4 *__restrict__ brec,
-                                                       const float *__restrict__ blam, uint32_t nb, float dt_sub,
+                                                       const float *__restrict__ blam, uint32_t nb, SubstepArgs sa,
                                                        int split, double *__restrict__ part,
                                                        const uint32_t *__restrict__ run_flag) {
+    const float dt_sub = sa.dt;
     if (run_flag && *run_flag == 0u) return;  // the previous check passed: nothing to measure
     __shared__ double sh[3][4];
     double hh = 0.0, ll = 0.0, mx = 0.0;
@@ -580,7 +585,21 @@ __global__ void __launch_bounds__(256) k_residual_part(const float4 *__restrict_
         double comp = __uint_as_float(r.w);
         if (isd && split && d - L < 0.0) comp = dcc[k];
         if (ws > 1e-7 && d > 1e-7 && comp < 1e9) {
-            const double h = (d - L) + comp * inv_dt2 * (double)(isd ? dlam[k] : blam[k]);
+            double h = (d - L) + comp * inv_dt2 * (double)(isd ? dlam[k] : blam[k]);
+            if (DAMP) {
+                const bool shear = isd && (r.y >> 31) != 0u;
+                const float dc = !isd ? sa.bending_damping
+                                      : (shear ? sa.shear_damping
+                                               : (d - L >= 0.0 ? sa.tension_damping : sa.compression_damping));
+                if (dc > 0.0f) {
+                    const double gamma = fmin((double)dc * (double)1e-5f / dt, 1e4);
+                    const float4 xa = X[a], xb = X[b];
+                    const double rx = ((double)qa.x - xa.x) - ((double)qb.x - xb.x);
+                    const double ry = ((double)qa.y - xa.y) - ((double)qb.y - xb.y);
+                    const double rz = ((double)qa.z - xa.z) - ((double)qb.z - xb.z);
+                    h += gamma * (dx * rx + dy * ry + dz * rz) / d;
+                }
+            }
             hh += h * h;
             ll += L * L;
             if (L > 0.0) mx = fmax(mx, fabs(h) / L);
@@ -724,14 +743,22 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t value, size_t n)
     return hipGetLastError();
 }
 
-hipError_t launch_residual(hipStream_t s, const float4 *P, const uint4 *drec, const float *dcc, const float *dlam,
-                           uint32_t nd, const uint4 *brec, const float *blam, uint32_t nb, float dt, bool split,
-                           double *part, double *out, uint32_t *gate, float threshold, const uint32_t *run_flag,
-                           uint32_t *gate_host) {
+hipError_t launch_residual(hipStream_t s, const float4 *P, const float4 *X, const uint4 *drec, const float *dcc,
+                           const float *dlam, uint32_t nd, const uint4 *brec, const float *blam, uint32_t nb,
+                           const SubstepArgs &a, bool split, double *part, double *out, uint32_t *gate,
+                           float threshold, const uint32_t *run_flag, uint32_t *gate_host) {
     const size_t n = size_t(nd) + nb;
     const uint32_t blocks = uint32_t(std::min<size_t>(RES_BLOCKS, std::max<size_t>(grid_for(n, 256), 1)));
-    hipLaunchKernelGGL(k_residual_part, dim3(blocks), dim3(256), 0, s, P, drec, dcc, dlam, nd, brec, blam, nb, dt,
-                       split ? 1 : 0, part, run_flag);
+    SubstepArgs ra = a;
+    ra.run_flag = nullptr;
+    const bool damp = a.tension_damping > 0.0f || a.compression_damping > 0.0f || a.shear_damping > 0.0f ||
+                      a.bending_damping > 0.0f;
+    if (damp)
+        hipLaunchKernelGGL(k_residual_part<true>, dim3(blocks), dim3(256), 0, s, P, X, drec, dcc, dlam, nd, brec,
+                           blam, nb, ra, split ? 1 : 0, part, run_flag);
+    else
+        hipLaunchKernelGGL(k_residual_part<false>, dim3(blocks), dim3(256), 0, s, P, X, drec, dcc, dlam, nd, brec,
+                           blam, nb, ra, split ? 1 : 0, part, run_flag);
     hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, s, part, blocks, out, gate, threshold, run_flag,
                        gate_host);
     return hipGetLastError();

@@ -239,15 +239,45 @@ class ClothSimulator:
     def set_hash_cell_size(self, cell_size):
         _lib.check(self._L.tc_sim_set_hash_cell_size(self._h, float(cell_size)))
 
-    # --- getters -------------------------------------------------------------------------------
+    # --- getters (PyO3 #[getter] properties, lib.rs:2927-2950) ---------------------------------
+    @property
     def num_vertices(self) -> int:
         return int(self._L.tc_sim_num_vertices(self._h))
 
+    @property
     def num_distance_constraints(self) -> int:
         return int(self._L.tc_sim_num_distance_constraints(self._h))
 
+    @property
     def num_bending_constraints(self) -> int:
         return int(self._L.tc_sim_num_bending_constraints(self._h))
+
+    @property
+    def num_sewing_constraints(self) -> int:
+        """Sewing springs are rejected at construction, so there are none."""
+        return 0
+
+    # --- diagnostics: GPU pass timing (lib.rs:1975-1990, simulation/profile.rs) -----------------
+    def set_profiling_enabled(self, enable) -> bool:
+        """Times each pass of the coming frames with HIP events; returns whether it is on."""
+        on = C.c_int(0)
+        _lib.check(self._L.tc_sim_set_profiling_enabled(self._h, 1 if enable else 0, C.byref(on)))
+        return bool(on.value)
+
+    def take_profile(self) -> list:
+        """(pass name, milliseconds) of the last frame stepped, scopes before their passes."""
+        n = C.c_size_t(0)
+        _lib.check(self._L.tc_sim_take_profile(self._h, C.byref(n)))
+        out = []
+        name, ms = C.c_char_p(), C.c_float()
+        for i in range(n.value):
+            _lib.check(self._L.tc_sim_profile_entry(self._h, i, C.byref(name), C.byref(ms), None, None))
+            out.append((name.value.decode(), float(ms.value)))
+        return out
+
+    def save_profile_trace(self, path):
+        """Chrome trace JSON of the last frame taken; OSError when the file cannot be written."""
+        _lib.check(self._L.tc_sim_save_profile_trace(self._h, str(path).encode()))
 
     # --- introspection used by parity checks ---------------------------------------------------
     _FLOAT_ARRAYS = {8, 9, 13, 14, 15, 16}

@@ -78,7 +78,7 @@ def record(seeds, frames, stride, n, history_weight):
 
 
 def _snapshot(sim):
-    n = sim.num_vertices()
+    n = sim.num_vertices
     x = np.zeros(3 * n, np.float32)
     sim.get_positions(x)
     return x.reshape(-1, 3).copy(), sim.velocities().copy(), sim.ws_history()

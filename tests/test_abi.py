@@ -73,3 +73,28 @@ def test_unsupported_options_raise():
         T.ClothSimulator(pos, e, f, sewing_springs=np.array([[0, 8]], np.uint32))
     with pytest.raises(NotImplementedError):
         T.ClothSimulator(pos, e, f, solver_mode=1)
+
+
+def test_abi_struct_layout_matches_ctypes():
+    """The sizes and offsets that tc_layout.h asserts at compile time (g++ and hipcc) are those
+    of the ctypes structures the Python binding passes."""
+    text = open(os.path.join(cases.ROOT, "taremincloth_amd", "csrc", "tc_layout.h")).read()
+    sizes = dict(re.findall(r"static_assert\(sizeof\((\w+)\) == (\d+)", text))
+    offs = re.findall(r"static_assert\(offsetof\((\w+), (\w+)\) == (\d+)", text)
+    classes = {"tc_sim_params": _lib.TcParams, "tc_instance": _lib.TcInstance}
+    assert set(sizes) == set(classes)
+    for name, cls in classes.items():
+        assert C.sizeof(cls) == int(sizes[name]), name
+        fields = [f for f, *_ in cls._fields_]
+        asserted = [(f, int(o)) for s, f, o in offs if s == name]
+        assert [f for f, _ in asserted] == fields, name   # every field, in order
+        for f, o in asserted:
+            assert getattr(cls, f).offset == o, (name, f)
+
+
+def test_getters_are_properties_as_in_pyo3():
+    """lib.rs:2927-2950 exposes the counts as #[getter] properties."""
+    for name in ("num_vertices", "num_distance_constraints", "num_bending_constraints", "num_sewing_constraints"):
+        assert isinstance(getattr(T.ClothSimulator, name), property), name
+    for name in ("set_profiling_enabled", "take_profile", "save_profile_trace"):
+        assert callable(getattr(T.ClothSimulator, name)), name

@@ -61,7 +61,7 @@ def test_instance_ranges_and_errors():
         a, n = b.instance_range(k)
         assert (a, n) == (first, len(d["positions"]))
         first += n
-    assert b.num_vertices() == first
+    assert b.num_vertices == first
     with pytest.raises(ValueError):
         b.instance_range(4)
     bad = [dict(inst[0]), dict(inst[1])]
@@ -114,7 +114,7 @@ def test_batch_autotune_and_reset():
     b = T.ClothBatch(inst)
     _configure(b)
     _run(b, 1)
-    before = np.empty(3 * b.num_vertices(), np.float32)
+    before = np.empty(3 * b.num_vertices, np.float32)
     b.get_positions(before)
     b.reset()
     tune = b.autotune(1.0 / 600.0)

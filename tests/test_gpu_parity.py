@@ -29,7 +29,7 @@ def _gpu(require_gpu):
 
 
 def _pos(sim):
-    out = np.empty(3 * sim.num_vertices(), np.float32)
+    out = np.empty(3 * sim.num_vertices, np.float32)
     sim.get_positions(out)
     return out.reshape(-1, 3)
 
@@ -111,6 +111,25 @@ def test_lambdas_and_residual_match_oracle():
     rg, mg = g.residual(dt_sub)
     ro, mo = o32.residual(dt_sub)
     assert abs(rg - ro) <= 1e-3 * ro + 1e-12, (rg, ro)
+
+
+def test_damped_residual_matches_oracle():
+    """With constraint damping (5, 5, 5, 0.5: the add-on's defaults) the residual carries the
+    damping term gamma grad C . (p - x): GPU equals the oracle (whose damped residual is pinned by
+    closed forms and by a single projection driving it to zero, tests/test_residual_pins.py)."""
+    _say("damped residual")
+    for iters in (4, 32):
+        g, o32, _ = _config1_pair(iters=iters)
+        dt_sub = (1.0 / 60.0) / 10.0
+        for s in (g, o32):
+            s.set_damping_all(5.0, 5.0, 5.0, 0.5)
+            for _ in range(3):
+                s.step_single_substep(dt_sub)
+            s.substep_parts(dt_sub, True, True, False)
+        rg, mg = g.residual(dt_sub)
+        ro, mo = o32.residual(dt_sub)
+        assert abs(rg - ro) <= 1e-3 * ro + 1e-12, (iters, rg, ro)
+        assert abs(mg - mo) <= 1e-3 * mo + 1e-12, (iters, mg, mo)
 
 
 def _rand_points(n, seed, scale=0.3):

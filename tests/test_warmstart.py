@@ -86,6 +86,118 @@ def test_oracle_features_closed_form():
     assert np.allclose(near[0, 13:16], [0, 0, 1])
 
 
+def _torch_forward(params, rounds, feat, off, nbr):
+    """The network restated with torch on the CPU in float64: torch.nn.functional.linear / relu
+    and an index_add_ neighbour mean, from the documented layer layout (W stored K x N)."""
+    import torch
+    import torch.nn.functional as Fn
+    p = torch.as_tensor(np.asarray(params, np.float64))
+    layers, o = [], 0
+    dims = [(W.F, W.H), (W.H, W.H)] + [(2 * W.H, W.H), (W.H, W.H)] * rounds + [(W.H, W.H), (W.H, W.O)]
+    for K, N in dims:
+        layers.append((p[o:o + K * N].reshape(K, N).T.contiguous(), p[o + K * N:o + K * N + N]))
+        o += K * N + N
+    x = torch.as_tensor(np.asarray(feat, np.float64))
+    offt = torch.as_tensor(np.asarray(off, np.int64))
+    deg = offt[1:] - offt[:-1]
+    rows = torch.repeat_interleave(torch.arange(len(deg)), deg)
+    cols = torch.as_tensor(np.asarray(nbr, np.int64))
+
+    def mlp(v, a, b):
+        return Fn.linear(Fn.relu(Fn.linear(v, *layers[a])), *layers[b])
+
+    h = mlp(x, 0, 1)
+    for r in range(rounds):
+        s_ = torch.zeros_like(h).index_add_(0, rows, h[cols])
+        mean = s_ / deg.clamp(min=1).unsqueeze(1).to(h.dtype)
+        h = h + mlp(torch.cat([h, mean], dim=1), 2 + 2 * r, 3 + 2 * r)
+    return mlp(h, 2 + 2 * rounds, 3 + 2 * rounds).numpy()
+
+
+@pytest.mark.parametrize("rounds", [1, 3])
+def test_oracle_forward_matches_a_torch_restatement(rounds):
+    p, feat, off, nbr, _ = random_case(40, rounds, 11 + rounds)
+    # one isolated vertex: its neighbour mean is zero
+    deg = np.diff(off).astype(np.int64)
+    keep = np.repeat(np.arange(40), deg) != 7
+    keep &= nbr != 7
+    rows = np.repeat(np.arange(40), deg)[keep]
+    nbr2 = nbr[keep]
+    off2 = np.zeros(41, np.uint32)
+    off2[1:] = np.cumsum(np.bincount(rows, minlength=40))
+    y, _ = W.forward(p, rounds, feat, off2, nbr2, bf16=False)
+    yt = _torch_forward(p, rounds, feat, off2, nbr2)
+    assert np.abs(y - yt).max() <= 1e-10 * max(1.0, np.abs(yt).max())
+
+
+def test_oracle_forward_is_permutation_equivariant():
+    """Relabelling the vertices permutes the output rows and nothing else."""
+    rounds = 2
+    p, feat, off, nbr, _ = random_case(30, rounds, 4)
+    perm = np.random.default_rng(9).permutation(30)
+    inv = np.argsort(perm)
+    deg = np.diff(off).astype(np.int64)
+    rows = np.repeat(np.arange(30), deg)
+    a, b = inv[rows], inv[nbr]          # new labels of each directed pair
+    order = np.lexsort((np.arange(len(a)), a))
+    off_p = np.zeros(31, np.uint32)
+    off_p[1:] = np.cumsum(np.bincount(a, minlength=30))
+    y, _ = W.forward(p, rounds, feat, off, nbr, bf16=False)
+    yp, _ = W.forward(p, rounds, feat[perm], off_p, b[order].astype(np.uint32), bf16=False)
+    assert np.allclose(yp, y[perm], rtol=1e-12, atol=1e-12)
+
+
+def _feat(ppred, mesh, cols, dprev=None, s=1e-4, lscale=0.01, th=0.005, beta=0.5):
+    n = len(ppred)
+    return W.features(np.asarray(ppred, np.float64), np.zeros((n, 3)) if dprev is None else np.asarray(dprev),
+                      np.full(n, th), mesh, cols, s, lscale, beta)
+
+
+_NO_PAIRS = {"dist_v0": [], "dist_v1": [], "dist_rest": [], "bend_v2": [], "bend_v3": [], "bend_rest": []}
+
+
+def test_oracle_features_capsule_and_plane_closed_forms():
+    # capsule along x from -0.3 to 0.3, radius 0.1: above its middle, and beyond its end cap
+    cap = [(1, (-0.3, 0.0, 0.0), (0.3, 0.0, 0.0), 0.1)]
+    f = _feat([[0.1, 0.0, 0.13, 1.0], [0.35, 0.0, 0.05, 1.0]], _NO_PAIRS, cap)
+    r0 = (0.13 - 0.1 - 0.005) / 0.01
+    assert np.isclose(f[0, 12], W.soft(r0)) and np.allclose(f[0, 13:16], [0, 0, 1])
+    r1 = (np.hypot(0.05, 0.05) - 0.1 - 0.005) / 0.01
+    assert np.isclose(f[1, 12], W.soft(r1)) and np.allclose(f[1, 13:16], [np.sqrt(0.5), 0, np.sqrt(0.5)])
+    # plane through (0, 0, 0.1) with an unnormalised normal (0, 0, 2): signed height, also below it
+    pl = [(2, (0.0, 0.0, 0.1), (0.0, 0.0, 2.0), 0.0)]
+    f = _feat([[0.3, -0.2, 0.12, 1.0], [0.0, 0.0, 0.09, 1.0]], _NO_PAIRS, pl)
+    assert np.isclose(f[0, 12], W.soft((0.02 - 0.005) / 0.01)) and np.allclose(f[0, 13:16], [0, 0, 1])
+    assert np.isclose(f[1, 12], W.soft((-0.01 - 0.005) / 0.01)) and np.allclose(f[1, 13:16], [0, 0, 1])
+    # nearest of two colliders wins: the plane is nearer than the capsule here
+    f = _feat([[0.1, 0.0, 0.13, 1.0]], _NO_PAIRS, cap + [(2, (0.0, 0.0, 0.12), (0.0, 0.0, 1.0), 0.0)])
+    assert np.isclose(f[0, 12], W.soft((0.01 - 0.005) / 0.01))
+    # no collider: the range cap
+    f = _feat([[0.0, 0.0, 0.0, 1.0]], _NO_PAIRS, [])
+    assert np.isclose(f[0, 12], W.soft(4.0)) and np.allclose(f[0, 13:16], 0)
+
+
+def test_oracle_features_bending_pull_history_and_degrees():
+    s, Lb = 1e-4, 0.02
+    mesh = {"dist_v0": [0, 1], "dist_v1": [1, 2], "dist_rest": [0.0125, 0.0125],
+            "bend_v2": [0], "bend_v3": [2], "bend_rest": [Lb]}
+    pp = [[0.0, 0.0, 0.0, 1.0], [0.0125, 0.0, 0.0, 1.0], [0.025, 0.0, 0.0, 0.0]]
+    f = _feat(pp, mesh, [], s=s)
+    pull = (0.025 - Lb) / s          # bending pair stretched by 5 mm, toward each other
+    assert np.allclose(f[0, 6:9], [W.soft(pull), 0, 0]) and np.allclose(f[2, 6:9], [-W.soft(pull), 0, 0])
+    assert np.allclose(f[1, 6:9], 0) and np.allclose(f[:, 3:6], 0)        # distance pairs at rest
+    assert np.allclose(f[:, 10], [0.125, 0.25, 0.125]) and np.allclose(f[:, 11], [0.125, 0.0, 0.125])
+    assert np.allclose(f[:, 9], [1, 1, 0])                                 # pinned vertex 2
+    # the strains are taken at prediction + beta * previous correction; the correction itself is a feature
+    d = np.array([[0.0, 0.0, 0.0], [0.002, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    for beta in (0.5, 1.0):
+        f = _feat(pp, mesh, [], dprev=d, s=s, beta=beta)
+        assert np.allclose(f[1, 0:3], [W.soft(0.002 / s), 0, 0])
+        # vertex 1 moved by beta * 2 mm: pair (0,1) stretched, pair (1,2) compressed by the same amount
+        assert np.allclose(f[0, 3:6], [W.soft(beta * 0.002 / s), 0, 0])
+        assert np.allclose(f[1, 3:6], W.soft(np.array([-2 * beta * 0.002 / s, 0.0, 0.0])))
+
+
 def test_layout_matches_documented_parameter_count():
     assert W.layout(3)[1] == 187536
     assert W.layout(1)[1] == 187536 - 2 * (256 * 128 + 128 + 128 * 128 + 128)
@@ -250,14 +362,18 @@ def test_gpu_gate_fallback_equals_plain_iterations(require_gpu):
 
 
 @pytest.mark.gpu
-def test_gpu_gate_stops_after_the_chunk_that_passes(require_gpu):
+@pytest.mark.parametrize("damped", [False, True])
+def test_gpu_gate_stops_after_the_chunk_that_passes(require_gpu, damped):
     """With the threshold between the residuals after 4 and after 8 plain iterations, the first
     check fails, the chained check after one 4-iteration chunk passes, and the remaining chunks do
-    not run: the substep equals 8 plain iterations bit for bit."""
+    not run: the substep equals 8 plain iterations bit for bit.  `damped` runs it with constraint
+    damping (5, 5, 5, 0.5), where the residual includes the damping term."""
     dt_sub = 1.0 / 600.0
 
     def settled():  # cloth in contact with the sphere, identical in every call
         sim, pos, *_ = _drape()
+        if damped:
+            sim.set_damping_all(5.0, 5.0, 5.0, 0.5)
         for _ in range(12):
             sim.step(1.0 / 60.0, 10, 16)
         return sim, pos
