@@ -254,6 +254,22 @@ int tc_ws_net_forward_backward(tc_ws_net *net, const float *features, uint32_t r
                                double *loss);
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) with the last computed gradient. */
 int tc_ws_net_adam_step(tc_ws_net *net, float lr);
+/* Allocation-free forward pass.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same pass as tc_ws_net_forward_backward with targets NULL (features rounded to
+ * BF16 the same way, the same kernels, bit-identical y), but every device buffer it needs -- the
+ * features, the two hidden-state buffers, the outputs and the graph -- lives in one static device
+ * array of TC_WS_NOALLOC_SCRATCH_BYTES held by each GPU until the process ends; it calls no
+ * hipMalloc or hipFree.  Limits: rows <= TC_WS_NOALLOC_MAX_ROWS (a 128 x 128 cloth) and
+ * graph_off[rows] <= TC_WS_NOALLOC_MAX_NBR; larger inputs return TC_ERR_VALUE.  The graph need not
+ * be symmetric (there is no backward pass).  Inputs and y are host arrays; the call blocks until y
+ * is written.  Calls on one GPU share the array (it keeps the last call's contents, none of which a
+ * call reads before writing), so calls that overlap in time on one GPU race. */
+#define TC_WS_NOALLOC_MAX_ROWS 16384u
+#define TC_WS_NOALLOC_MAX_NBR 131072u
+#define TC_WS_NOALLOC_SCRATCH_BYTES 13172992u
+int tc_ws_net_forward_noalloc(tc_ws_net *net, const float *features, uint32_t rows, const uint32_t *graph_off,
+                              const uint32_t *graph_nbr, float *y);
 /* Trains on the samples the simulators recorded (tc_sim_ws_record): `steps` Adam steps, each on
  * one recorded substep drawn with a seeded generator; losses[steps] may be NULL. */
 int tc_ws_train(tc_ws_net *net, tc_sim *const *sims, uint32_t nsims, uint32_t steps, float lr, uint64_t seed,

@@ -29,7 +29,7 @@ EXPORTS = [
     "tc_ws_net_get_params", "tc_ws_net_init", "tc_ws_net_forward_backward", "tc_ws_net_adam_step", "tc_ws_train",
     "tc_sim_set_warm_start", "tc_sim_warm_start_stats", "tc_sim_ws_reset_stats", "tc_sim_ws_record",
     "tc_sim_ws_num_records", "tc_sim_ws_get_record", "tc_sim_ws_last_output", "tc_sim_ws_history",
-    "tc_ws_eval", "tc_sim_ws_time",
+    "tc_ws_eval", "tc_sim_ws_time", "tc_ws_net_forward_noalloc",
     "tc_sim_set_sdf_colliders", "tc_sim_update_bone_transforms", "tc_bake_mesh_sdf", "tc_sdf_result_info",
     "tc_sdf_result_timing", "tc_sdf_result_texels", "tc_sdf_result_nearest", "tc_sdf_result_free", "tc_close_open_boundary_loops",
 ]
@@ -126,6 +126,7 @@ def load() -> C.CDLL:
         "tc_ws_net_init": (_i, [_vp, C.c_uint64]),
         "tc_ws_net_forward_backward": (_i, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double)]),
         "tc_ws_net_adam_step": (_i, [_vp, _f32]),
+        "tc_ws_net_forward_noalloc": (_i, [_vp, _vp, _u32, _vp, _vp, _vp]),
         "tc_ws_train": (_i, [_vp, _vp, _u32, _u32, _f32, C.c_uint64, _vp]),
         "tc_sim_set_warm_start": (_i, [_vp, _vp, _i, _u32, _u32, _f32, _f32]),
         "tc_sim_warm_start_stats": (_i, [_vp, C.POINTER(C.c_double)]),

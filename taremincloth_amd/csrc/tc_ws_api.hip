@@ -5,6 +5,8 @@
 // loop, crates/cloth_core/src/simulation/dispatch.rs:137 -> 169).
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "tc_sim_impl.h"
 
@@ -682,5 +684,105 @@ int tc_sim_ws_time(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
     for (int k = 0; k < 13; ++k) out[k] = acc[k] / double(reps);
     out[13] = R;
     out[14] = s->ws_iters;
+    return TC_OK;
+}
+
+// ------------------------------------------------------------------ forward pass without allocation
+// tc_ws_net_forward_noalloc: the inference half of tc_ws_net_forward_backward with every device
+// buffer carved from one file-scope device array, for callers that cannot allocate (graph capture,
+// serving loops).  Host code never names `ws_scratch` in a launch or a copy: each device's address
+// of it comes from hipGetSymbolAddress once, on the first call on that device, and only that
+// address is used.
+namespace {
+
+constexpr size_t NA_ALIGN = 256;
+constexpr size_t na_round(size_t b) { return (b + NA_ALIGN - 1) / NA_ALIGN * NA_ALIGN; }
+constexpr size_t NA_ROWS = TC_WS_NOALLOC_MAX_ROWS;
+constexpr size_t NA_NBR = TC_WS_NOALLOC_MAX_NBR;
+// layout: fp32 feature staging | BF16 features | hidden state h0 | h1 | outputs y | graph offsets | neighbours
+constexpr size_t NA_F32 = 0;
+constexpr size_t NA_FEAT = NA_F32 + na_round(NA_ROWS * tc::ws::F * sizeof(float));
+constexpr size_t NA_H0 = NA_FEAT + na_round(NA_ROWS * tc::ws::F * sizeof(bf16));
+constexpr size_t NA_H1 = NA_H0 + na_round(tc::ws::padded_rows(NA_ROWS) * tc::ws::H * sizeof(bf16));
+constexpr size_t NA_Y = NA_H1 + na_round(tc::ws::padded_rows(NA_ROWS) * tc::ws::H * sizeof(bf16));
+constexpr size_t NA_OFF = NA_Y + na_round(NA_ROWS * tc::ws::O * sizeof(float));
+constexpr size_t NA_NBR_OFF = NA_OFF + na_round((NA_ROWS + 1) * sizeof(uint32_t));
+constexpr size_t NA_BYTES = NA_NBR_OFF + na_round(NA_NBR * sizeof(uint32_t));
+static_assert(NA_BYTES == TC_WS_NOALLOC_SCRATCH_BYTES, "the header states the scratch size");
+
+__device__ __align__(256) unsigned char ws_scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mutex;
+unsigned char *na_ptr[NA_MAX_DEVICES] = {};
+
+// This device's address of ws_scratch (looked up once per device).
+int na_scratch(unsigned char **out) {
+    int dev = 0;
+    TC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return fail(TC_ERR_RUNTIME, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(na_mutex);
+    if (!na_ptr[dev]) {
+        void *p = nullptr;
+        TC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(ws_scratch)));
+        if (!p) return fail(TC_ERR_RUNTIME, "hipGetSymbolAddress returned null for the scratch array");
+        na_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = na_ptr[dev];
+    return TC_OK;
+}
+
+// fp32 -> BF16, round to nearest even: the same bit operation as f2bf on the host.
+__global__ void __launch_bounds__(256) k_f32_to_bf16(const float *__restrict__ in, uint16_t *__restrict__ out,
+                                                     size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t u = __float_as_uint(in[i]);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    out[i] = uint16_t(u >> 16);
+}
+
+}  // namespace
+
+int tc_ws_net_forward_noalloc(tc_ws_net *n, const float *feat, uint32_t rows, const uint32_t *graph_off,
+                              const uint32_t *graph_nbr, float *out_y) {
+    if (!n || !feat || !graph_off || !out_y) return fail(TC_ERR_VALUE, "null argument");
+    if (rows == 0) return fail(TC_ERR_VALUE, "rows must be positive");
+    if (rows > NA_ROWS)
+        return fail(TC_ERR_VALUE, "rows (" + std::to_string(rows) + ") exceed the allocation-free limit of " +
+                                      std::to_string(NA_ROWS));
+    const uint32_t nn = graph_off[rows];
+    if (nn > NA_NBR)
+        return fail(TC_ERR_VALUE, "neighbour entries (" + std::to_string(nn) + ") exceed the allocation-free limit of " +
+                                      std::to_string(NA_NBR));
+    if (nn && !graph_nbr) return fail(TC_ERR_VALUE, "null argument");
+    for (uint32_t i = 0; i < rows; ++i)
+        if (graph_off[i] > graph_off[i + 1]) return fail(TC_ERR_VALUE, "graph offsets must be non-decreasing");
+    for (uint32_t k = 0; k < nn; ++k)
+        if (graph_nbr[k] >= rows) return fail(TC_ERR_VALUE, "graph neighbour index out of range");
+    unsigned char *base = nullptr;
+    int r = na_scratch(&base);
+    if (r != TC_OK) return r;
+    float *f32 = reinterpret_cast<float *>(base + NA_F32);
+    bf16 *fb = reinterpret_cast<bf16 *>(base + NA_FEAT);
+    uint32_t *goff = reinterpret_cast<uint32_t *>(base + NA_OFF);
+    uint32_t *gnbr = reinterpret_cast<uint32_t *>(base + NA_NBR_OFF);
+    const size_t nf = size_t(rows) * tc::ws::F;
+    TC_HIP(hipMemcpyAsync(f32, feat, 4 * nf, hipMemcpyHostToDevice, n->stream));
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3(unsigned((nf + 255) / 256)), dim3(256), 0, n->stream, f32,
+                       reinterpret_cast<uint16_t *>(fb), nf);
+    TC_HIP(hipGetLastError());
+    TC_HIP(hipMemcpyAsync(goff, graph_off, 4 * (size_t(rows) + 1), hipMemcpyHostToDevice, n->stream));
+    if (nn) TC_HIP(hipMemcpyAsync(gnbr, graph_nbr, 4 * size_t(nn), hipMemcpyHostToDevice, n->stream));
+    tc::ws::FwdBufs b{};
+    b.feat = fb;
+    bf16 *h0 = reinterpret_cast<bf16 *>(base + NA_H0), *h1 = reinterpret_cast<bf16 *>(base + NA_H1);
+    for (int k = 0; k < 8; ++k) b.h[k] = (k & 1) ? h1 : h0;
+    b.y = reinterpret_cast<float *>(base + NA_Y);
+    b.training = false;
+    const tc::ws::Graph g{goff, gnbr, tc::ws::max_degree(graph_off, rows)};
+    TC_HIP(tc::ws::forward(n->stream, n->nl, n->dev(), g, rows, b));
+    TC_HIP(hipMemcpyAsync(out_y, b.y, 4 * size_t(rows) * tc::ws::O, hipMemcpyDeviceToHost, n->stream));
+    TC_HIP(hipStreamSynchronize(n->stream));
     return TC_OK;
 }
