@@ -290,8 +290,11 @@ int tc_ws_eval(tc_ws_net *net, tc_sim *const *sims, uint32_t nsims, double *mean
  * to even) in all. */
 int tc_sim_set_warm_start(tc_sim *sim, tc_ws_net *net, int mode, uint32_t iterations,
                           uint32_t fallback_iterations, float residual_threshold, float history_weight);
-/* out[7]: [substeps checked, substeps that fell back, relative residual at the last check,
- *          max |h|/L0 at the last check, mode, warm-start iterations, fallback chunks run] */
+/* out[8]: [substeps checked, substeps that fell back, relative residual at the last check,
+ *          max |h|/L0 at the last check, mode, warm-start iterations, fallback chunks run,
+ *          substeps that spent the whole fallback budget and still ended above the threshold].
+ * A substep whose last fallback chunk runs gets one more check after it, so "the last check" is the
+ * residual the substep ended with. */
 int tc_sim_warm_start_stats(tc_sim *sim, double *out);
 int tc_sim_ws_reset_stats(tc_sim *sim);
 /* Records training samples from the coming substeps: every `stride`-th substep stores its

@@ -92,6 +92,7 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.orc_color_edge_pairs.argtypes = [C.c_uint32, _u32p, C.c_uint32, _u32p]
     L.orc_hash_points.argtypes = [_f32p, C.c_uint32, C.c_float, C.c_uint32, _u32p, _u32p, _u32p]
     L.orc_real_bytes.restype = C.c_int
+    L.orc_num_threads.restype = C.c_int
     L.orc_sim_stats.argtypes = [_vp, np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS"), C.c_int]
     # SDF bake and SDF colliders (sdf_oracle.c)
     L.orc_sdf_bake.restype = _vp

@@ -40,6 +40,16 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Built with -fopenmp (liborc_f32_omp.so, the bench's multithreaded CPU baseline) the per-vertex
+ * passes and each colour's constraints are split over threads.  Constraints of one colour share no
+ * vertex and the per-vertex passes write only their own vertex, so every thread count gives the
+ * serial result bit for bit.  Self-collision stays serial (shared counters). */
+#ifdef _OPENMP
+#define ORC_PARALLEL_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define ORC_PARALLEL_FOR
+#endif
+
 #ifndef ORC_REAL
 #define ORC_REAL float
 #endif
@@ -1235,6 +1245,7 @@ void orc_sim_set_lambdas(orc_sim *s, const double *ld, const double *lb) {
 static void k_predict(orc_sim *s, real dt_sub) {
     real dt = rmax(dt_sub, (real)1e-8f);
     v3 g = mk(s->gravity[0], s->gravity[1], s->gravity[2]);
+    ORC_PARALLEL_FOR
     for (uint32_t i = 0; i < s->nv; i++) {
         if (s->w[i] <= 0) { s->v[i] = mk(0, 0, 0); s->p[i] = s->x[i]; }
         else {
@@ -1359,6 +1370,7 @@ int orc_sdf_contact(const uint32_t *tex, uint32_t w, uint32_t h, uint32_t d, con
 
 /* collision.wgsl:377-652: primitives, mesh triangles, then SDF colliders (sdf_oracle.c) */
 static void k_collision(orc_sim *s, real dt) {
+    ORC_PARALLEL_FOR
     for (uint32_t i = 0; i < s->nv; i++) {
         if (s->w[i] <= 0) continue;
         v3 p = s->p[i], pinit = p, xn = s->x[i];
@@ -1465,6 +1477,7 @@ static void k_collision(orc_sim *s, real dt) {
 /* update_vel.wgsl:33-53 */
 static void k_update_vel(orc_sim *s, real dt_sub) {
     real dt = rmax(dt_sub, (real)1e-8f);
+    ORC_PARALLEL_FOR
     for (uint32_t i = 0; i < s->nv; i++) {
         if (s->w[i] > 0) s->v[i] = dvs(sub(s->p[i], s->x[i]), dt);
         else s->v[i] = mk(0, 0, 0);
@@ -2085,10 +2098,14 @@ static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
         memset(s->lam_b, 0, (m->nb ? m->nb : 1) * sizeof(real));
         k_predict(s, dt_sub);
         for (uint32_t it = 0; it < s->iterations; it++) {
-            for (uint32_t c = 0; c < m->nd_col; c++)
+            for (uint32_t c = 0; c < m->nd_col; c++) {
+                ORC_PARALLEL_FOR
                 for (uint32_t k = m->d_col_off[c]; k < m->d_col_off[c] + m->d_col_cnt[c]; k++) k_distance_one(s, k, dt_sub);
-            for (uint32_t c = 0; c < m->nb_col; c++)
+            }
+            for (uint32_t c = 0; c < m->nb_col; c++) {
+                ORC_PARALLEL_FOR
                 for (uint32_t k = m->b_col_off[c]; k < m->b_col_off[c] + m->b_col_cnt[c]; k++) k_bending_one(s, k, dt_sub);
+            }
         }
         if (s->ncols > 0 || s->ntris > 0 || s->sdf_nb > 0) k_collision(s, dt_sub);
         if (s->self_collision) self_collision_pass(s);
@@ -2113,10 +2130,14 @@ void orc_sim_substep_parts(orc_sim *s, float dt_sub, int do_collide, int do_self
     memset(s->lam_b, 0, (m->nb ? m->nb : 1) * sizeof(real));
     k_predict(s, dt);
     for (uint32_t it = 0; it < s->iterations; it++) {
-        for (uint32_t c = 0; c < m->nd_col; c++)
+        for (uint32_t c = 0; c < m->nd_col; c++) {
+            ORC_PARALLEL_FOR
             for (uint32_t k = m->d_col_off[c]; k < m->d_col_off[c] + m->d_col_cnt[c]; k++) k_distance_one(s, k, dt);
-        for (uint32_t c = 0; c < m->nb_col; c++)
+        }
+        for (uint32_t c = 0; c < m->nb_col; c++) {
+            ORC_PARALLEL_FOR
             for (uint32_t k = m->b_col_off[c]; k < m->b_col_off[c] + m->b_col_cnt[c]; k++) k_bending_one(s, k, dt);
+        }
     }
     if (do_collide && (s->ncols > 0 || s->ntris > 0 || s->sdf_nb > 0)) k_collision(s, dt);
     if (do_self && s->self_collision) self_collision_pass(s);
@@ -2183,6 +2204,12 @@ void orc_sim_residual(const orc_sim *s, float dt_sub, double *out_rel, double *o
 }
 
 int orc_real_bytes(void) { return (int)sizeof(real); }
+#ifdef _OPENMP
+#include <omp.h>
+int orc_num_threads(void) { return omp_get_max_threads(); }
+#else
+int orc_num_threads(void) { return 1; }
+#endif
 void orc_sim_stats(orc_sim *s, uint64_t *out, int clear) {
     for (int i = 0; i < 6; i++) { out[i] = s->stats[i]; if (clear) s->stats[i] = 0; }
 }

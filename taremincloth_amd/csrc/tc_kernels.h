@@ -51,12 +51,14 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t value, size_t n)
 // Residual of the XPBD system after the last sweep (double sums, deterministic order):
 // out[0] = sum h^2, out[1] = sum L0^2, out[2] = max |h|/L0, out[3] = sqrt(out[0] / out[1]).
 // part holds 3 * 1024 doubles.  With gate != nullptr the kernel also writes gate[0] = (out[3] >
-// threshold) and counts checks in gate[1] and fallbacks in gate[2].
+// threshold) and counts checks in gate[1], fallbacks in gate[2], fallback chunks in gate[3] and, for
+// the final check after the last chunk (final_check), substeps that end above the threshold in gate[4].
 // X (the committed positions) is read only with constraint damping in `a`, for the damping term.
 hipError_t launch_residual(hipStream_t s, const float4 *P, const float4 *X, const uint4 *drec, const float *dcc,
                            const float *dlam, uint32_t nd, const uint4 *brec, const float *blam, uint32_t nb,
                            const SubstepArgs &a, bool split, double *part, double *out, uint32_t *gate,
-                           float threshold, const uint32_t *run_flag = nullptr, uint32_t *gate_host = nullptr);
+                           float threshold, const uint32_t *run_flag = nullptr, uint32_t *gate_host = nullptr,
+                           bool final_check = false);
 
 // --- one iteration per launch over cone tiles (tc_fused.hip, plan from tc_tiles.cpp) ---
 struct FusedArgs {

@@ -89,6 +89,19 @@ hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t coun
     return e;
 }
 
+// The check after the budget's last chunk: it runs only when that chunk ran (the gate still open)
+// and counts the substeps that end above the threshold; nothing reads its flag.
+hipError_t enqueue_final_check(tc_sim *s, const tc::SubstepArgs &a, uint32_t pc, uint32_t lc) {
+    const tc::Mesh &m = s->mesh;
+    const int q = s->prof_begin("residual_gate");
+    hipError_t e = tc::launch_residual(s->stream, s->Pbuf(pc), s->X.p, s->drec.p, s->dcc.p, s->dlam_buf(lc),
+                                       uint32_t(m.d_v0.size()), s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()),
+                                       a, s->split_compliance, s->resid_part.p, s->ws_res.p, s->ws_gate.p,
+                                       s->ws_threshold, s->ws_gate.p, nullptr, true);
+    s->prof_end(q);
+    return e;
+}
+
 // Residual-gated fallback: chunks of the warm-start iteration count (rounded up to even, so that
 // the P / lambda buffer parity is the same whether a chunk runs or not), each preceded by a
 // residual check; the first check that passes closes the gate for the rest of the substep.
@@ -109,7 +122,7 @@ hipError_t enqueue_gated_fallback(tc_sim *s, const tc::SubstepArgs &a, float dt_
         if ((e = enqueue_iterations(s, fa, std::min(chunk, s->ws_fallback - done), false, pc, lc)) != hipSuccess)
             return e;
     }
-    return e;
+    return enqueue_final_check(s, a, pc, lc);
 }
 
 namespace {
@@ -158,7 +171,8 @@ hipError_t enqueue_fallback_chunk(tc_sim *s, float dt_sub, uint32_t count, bool 
     tc::SubstepArgs fa = s->args(dt_sub);
     fa.run_flag = s->ws_gate.p;
     hipError_t e = enqueue_iterations(s, fa, count, false, pc, lc);
-    if (e != hipSuccess || last) return e;
+    if (e != hipSuccess) return e;
+    if (last) return enqueue_final_check(s, fa, pc, lc);
     e = tc::launch_residual(s->stream, s->Pbuf(pc), s->X.p, s->drec.p, s->dcc.p, s->dlam_buf(lc), uint32_t(m.d_v0.size()),
                             s->brec.p, s->blam_buf(lc), uint32_t(m.b_v0.size()), fa, s->split_compliance,
                             s->resid_part.p, s->ws_res.p, s->ws_gate.p, s->ws_threshold, s->ws_gate.p,

@@ -92,8 +92,8 @@ int ws_prepare(tc_sim *s) {
     TC_HIP(s->ws_h0.alloc(tc::ws::padded_rows(nv) * tc::ws::H));
     TC_HIP(s->ws_h1.alloc(tc::ws::padded_rows(nv) * tc::ws::H));
     TC_HIP(s->ws_y.alloc(size_t(nv) * tc::ws::O));
-    TC_HIP(s->ws_gate.alloc(4));
-    TC_HIP(hipMemsetAsync(s->ws_gate.p, 0, 16, s->stream));
+    TC_HIP(s->ws_gate.alloc(8));
+    TC_HIP(hipMemsetAsync(s->ws_gate.p, 0, 32, s->stream));
     TC_HIP(s->ws_res.alloc(4));
     TC_HIP(hipMemsetAsync(s->ws_res.p, 0, 32, s->stream));
     TC_HIP(tc::ws::set_lds_limits());
@@ -421,13 +421,13 @@ int tc_sim_set_warm_start(tc_sim *s, tc_ws_net *net, int mode, uint32_t iteratio
 
 int tc_sim_warm_start_stats(tc_sim *s, double *out) {
     if (!s || !out) return fail(TC_ERR_VALUE, "null argument");
-    for (int k = 0; k < 7; ++k) out[k] = 0.0;
+    for (int k = 0; k < 8; ++k) out[k] = 0.0;
     out[4] = s->ws_mode;
     out[5] = s->ws_iters;
     if (!s->ws_ready) return TC_OK;
-    uint32_t g[4];
+    uint32_t g[8];
     double r[4];
-    TC_HIP(hipMemcpyAsync(g, s->ws_gate.p, 16, hipMemcpyDeviceToHost, s->stream));
+    TC_HIP(hipMemcpyAsync(g, s->ws_gate.p, 32, hipMemcpyDeviceToHost, s->stream));
     TC_HIP(hipMemcpyAsync(r, s->ws_res.p, 32, hipMemcpyDeviceToHost, s->stream));
     TC_HIP(hipStreamSynchronize(s->stream));
     out[0] = g[1];
@@ -435,13 +435,14 @@ int tc_sim_warm_start_stats(tc_sim *s, double *out) {
     out[2] = r[3];
     out[3] = r[2];
     out[6] = g[3];
+    out[7] = g[4];
     return TC_OK;
 }
 
 int tc_sim_ws_reset_stats(tc_sim *s) {
     if (!s) return fail(TC_ERR_VALUE, "null simulator");
     if (!s->ws_ready) return TC_OK;
-    TC_HIP(hipMemsetAsync(s->ws_gate.p, 0, 16, s->stream));
+    TC_HIP(hipMemsetAsync(s->ws_gate.p, 0, 32, s->stream));
     TC_HIP(hipMemsetAsync(s->ws_res.p, 0, 32, s->stream));
     TC_HIP(hipStreamSynchronize(s->stream));
     return TC_OK;

@@ -619,12 +619,14 @@ __global__ void __launch_bounds__(256) k_residual_part(const float4 *__restrict_
 }
 
 // out[0] = sum h^2, out[1] = sum L0^2, out[2] = max |h|/L0, out[3] = relative residual.
-// With gate != nullptr: gate[0] = 1 if the relative residual exceeds `threshold` (else 0),
-// gate[1] += 1 (substeps checked), gate[2] += gate[0] (substeps that fell back).
+// With gate != nullptr: gate[0] = 1 if the relative residual exceeds `threshold` (else 0); the
+// first check of a substep adds gate[1] += 1 (substeps checked), gate[2] += gate[0] (substeps that
+// fell back) and gate[3] += gate[0] (fallback chunks); a chained check adds gate[3] += gate[0]; the
+// final check after the budget's last chunk adds gate[4] += gate[0] (substeps that end above it).
 __global__ void __launch_bounds__(64) k_residual_final(const double *__restrict__ part, uint32_t nblocks,
                                                        double *__restrict__ out, uint32_t *__restrict__ gate,
                                                        float threshold, const uint32_t *run_flag,
-                                                       uint32_t *gate_host) {
+                                                       uint32_t *gate_host, int final_check) {
     // a chained check (run_flag set) after a check that passed keeps the gate closed
     if (run_flag && *run_flag == 0u) {
         if (gate_host && threadIdx.x == 0) *gate_host = 0u;
@@ -650,7 +652,9 @@ __global__ void __launch_bounds__(64) k_residual_final(const double *__restrict_
             gate[0] = fail;
             // pinned host copy of the flag (host-coherent at the kernel's end)
             if (gate_host) *gate_host = fail;
-            if (run_flag) {
+            if (final_check) {
+                gate[4] += fail;  // the budget is spent and the substep ends above the threshold
+            } else if (run_flag) {
                 gate[3] += fail;  // another fallback chunk
             } else {
                 gate[1] += 1u;    // substeps checked
@@ -746,7 +750,7 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t value, size_t n)
 hipError_t launch_residual(hipStream_t s, const float4 *P, const float4 *X, const uint4 *drec, const float *dcc,
                            const float *dlam, uint32_t nd, const uint4 *brec, const float *blam, uint32_t nb,
                            const SubstepArgs &a, bool split, double *part, double *out, uint32_t *gate,
-                           float threshold, const uint32_t *run_flag, uint32_t *gate_host) {
+                           float threshold, const uint32_t *run_flag, uint32_t *gate_host, bool final_check) {
     const size_t n = size_t(nd) + nb;
     const uint32_t blocks = uint32_t(std::min<size_t>(RES_BLOCKS, std::max<size_t>(grid_for(n, 256), 1)));
     SubstepArgs ra = a;
@@ -760,7 +764,7 @@ hipError_t launch_residual(hipStream_t s, const float4 *P, const float4 *X, cons
         hipLaunchKernelGGL(k_residual_part<false>, dim3(blocks), dim3(256), 0, s, P, X, drec, dcc, dlam, nd, brec,
                            blam, nb, ra, split ? 1 : 0, part, run_flag);
     hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, s, part, blocks, out, gate, threshold, run_flag,
-                       gate_host);
+                       gate_host, final_check ? 1 : 0);
     return hipGetLastError();
 }
 

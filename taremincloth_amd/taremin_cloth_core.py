@@ -368,10 +368,10 @@ class ClothSimulator:
                                                  float(residual_threshold), float(history_weight)))
 
     def warm_start_stats(self) -> dict:
-        out = (C.c_double * 7)()
+        out = (C.c_double * 8)()
         _lib.check(self._L.tc_sim_warm_start_stats(self._h, out))
         keys = ["checked", "fallbacks", "last_relative_residual", "last_max_relative", "mode", "iterations",
-                "fallback_chunks"]
+                "fallback_chunks", "ended_above_threshold"]
         return dict(zip(keys, list(out)))
 
     def ws_reset_stats(self):

@@ -106,3 +106,24 @@ def test_fp32_vs_fp64_oracle_one_frame():
     s64.step(1.0 / 60.0, 20, 2)
     err = np.abs(s32.state(0) - s64.state(0)).max()
     assert err < 1e-6
+
+
+def test_openmp_oracle_equals_the_serial_oracle():
+    """The multithreaded build of the fp32 restatement (the bench's CPU baseline) splits each
+    colour and each per-vertex pass over threads; the result is the serial one bit for bit."""
+    from oracle import oracle as O
+    pos, edges, faces = cases.grid(24, 24, 0.02)
+    pos = pos + np.array([-0.23, -0.23, 0.2], np.float32)
+    sims = []
+    for prec in ("f32", "f32_omp"):
+        s = O.OracleSim(pos, edges, faces, precision=prec)
+        s.set_enable_self_collision(False)
+        s.set_compliances_raw(1e-8, 1e-6, 1e-5)
+        s.add_sphere_collider([0.0, 0.0, 0.0], 0.15, 0.3, 0.0)
+        s.add_capsule_collider([-0.2, 0.1, 0.05], [0.2, 0.1, 0.05], 0.05, 0.3, 0.0)
+        s.set_solver_iterations(8)
+        for _ in range(12):
+            s.step(1.0 / 60.0, 10)
+        sims.append(s)
+    assert np.array_equal(sims[0].state(0), sims[1].state(0))
+    assert np.array_equal(sims[0].state(2), sims[1].state(2))
