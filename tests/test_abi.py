@@ -98,3 +98,73 @@ def test_getters_are_properties_as_in_pyo3():
         assert isinstance(getattr(T.ClothSimulator, name), property), name
     for name in ("set_profiling_enabled", "take_profile", "save_profile_trace"):
         assert callable(getattr(T.ClothSimulator, name)), name
+
+
+_C_SCALAR = {"int": "i32", "uint32_t": "u32", "int32_t": "i32", "float": "f32", "double": "f64", "size_t": "usize",
+             "uint64_t": "u64"}
+
+
+def _c_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"#.*", "", text)
+    out = []
+    for ret, name, params in re.findall(r"([A-Za-z_][\w\s\*]*?)\b(tc_[a-z_0-9]+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S):
+        ps = [p.strip() for p in params.split(",")] if params.strip() not in ("", "void") else []
+        out.append((name, ret.strip(), ps))
+    return out
+
+
+def _rust_functions():
+    text = open(os.path.join(cases.ROOT, "include", "tc_ffi.rs")).read()
+    block = text[text.index('extern "C" {'):]
+    out = []
+    for name, params, ret in re.findall(r"pub fn (tc_[a-z_0-9]+)\((.*?)\)(?:\s*->\s*([^;]+))?;", block):
+        ps = [p.split(":", 1)[1].strip() for p in params.split(",")] if params.strip() else []
+        out.append((name, (ret or "").strip(), ps))
+    return out
+
+
+def _shape_c(p):
+    """(pointer depth, outermost pointee const, scalar Rust type or None) of a C parameter or return."""
+    depth = p.count("*") + p.count("[")
+    if "[" in p:  # an array parameter is a pointer to its element type
+        const = p.startswith("const")
+    else:
+        # the qualifier just before the last '*' is the one on what the outermost pointer points to
+        segs = p.split("*")
+        const = depth > 0 and "const" in segs[-2].split()
+    base = re.sub(r"\bconst\b|\*|\[.*?\]", " ", p).split()
+    scalar = _C_SCALAR.get(base[0]) if base else None
+    return depth, const if depth else False, scalar if depth == 0 else None
+
+
+def _shape_rust(t):
+    depth = t.count("*const") + t.count("*mut")
+    const = t.startswith("*const")
+    return depth, const if depth else False, t if depth == 0 else None
+
+
+def test_rust_extern_block_matches_the_header():
+    cf, rf = _c_functions(), _rust_functions()
+    assert [n for n, _, _ in cf] == [n for n, _, _ in rf]
+    assert sorted(n for n, _, _ in cf) == sorted(_lib.EXPORTS)
+    for (name, cret, cps), (_, rret, rps) in zip(cf, rf):
+        assert len(cps) == len(rps), name
+        for cp, rp in zip(cps, rps):
+            assert _shape_c(cp) == _shape_rust(rp), (name, cp, rp)
+        if cret == "void":
+            assert rret == "", name
+        else:
+            assert _shape_c(cret) == _shape_rust(rret), (name, cret, rret)
+
+
+def test_rust_structs_and_constants_match_the_header():
+    rs = open(os.path.join(cases.ROOT, "include", "tc_ffi.rs")).read()
+    for rname, cls in (("TcParams", _lib.TcParams), ("TcInstance", _lib.TcInstance)):
+        body = re.search(r"pub struct %s \{(.*?)\}" % rname, rs, flags=re.S).group(1)
+        assert re.findall(r"pub (\w+):", body) == [f for f, *_ in cls._fields_], rname
+    hdr = open(HEADER).read()
+    for name, val in re.findall(r"pub const (TC_\w+): \w+ = (\d+);", rs):
+        m = re.search(r"\b%s\s*=\s*(\d+)" % name, hdr) or re.search(r"#define %s (\d+)u?" % name, hdr)
+        assert m and int(m.group(1)) == int(val), name
