@@ -27,6 +27,14 @@
 typedef ORC_REAL real;
 
 typedef struct { real x, y, z; } s3;
+/* With -fopenmp (liborc_f32_omp.so) the voxel slabs of a bake are split over threads; every voxel
+ * is computed alone, so the texels are the serial ones bit for bit. */
+#ifdef _OPENMP
+#define ORC_PARALLEL_FOR _Pragma("omp parallel for schedule(dynamic, 1)")
+#else
+#define ORC_PARALLEL_FOR
+#endif
+
 static inline s3 sk(real x, real y, real z) { s3 r = {x, y, z}; return r; }
 static inline s3 sadd(s3 a, s3 b) { return sk(a.x + b.x, a.y + b.y, a.z + b.z); }
 static inline s3 ssub(s3 a, s3 b) { return sk(a.x - b.x, a.y - b.y, a.z - b.z); }
@@ -392,6 +400,7 @@ static void bone_info(const odomain *dm, float friction, float thickness, float 
 
 /* bake_mesh_sdf.wgsl (K = 1) / the coarse pass (K = 8) over every voxel of dm */
 static void scan(const odomain *dm, const otri *tr, uint32_t nt, int K, uint32_t *out, uint32_t *nearest) {
+    ORC_PARALLEL_FOR
     for (uint32_t z = 0; z < dm->d; z++)
         for (uint32_t y = 0; y < dm->h; y++)
             for (uint32_t x = 0; x < dm->w; x++) {
@@ -429,6 +438,7 @@ static void fine(const odomain *dm, const odomain *cd, const otri *tr, uint32_t 
     const s3 size = ssub(lmax, lmin);
     const s3 spacing = sk(size.x / (real)cd->w, size.y / (real)cd->h, size.z / (real)cd->d);
     const real halfdiag = (real)0.5 * ssqrt(sdot(spacing, spacing));
+    ORC_PARALLEL_FOR
     for (uint32_t z = 0; z < dm->d; z++)
         for (uint32_t y = 0; y < dm->h; y++)
             for (uint32_t x = 0; x < dm->w; x++) {

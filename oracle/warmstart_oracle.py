@@ -172,16 +172,23 @@ def soft(x):
     return x / (1.0 + np.abs(x) / 16.0)
 
 
-def features(ppred, dprev, thickness, mesh, colliders, s, lscale, beta=0.5):
+def features(ppred, dprev, thickness, mesh, colliders, s, lscale, beta=0.5, fp32_points=False):
     """Input features of every vertex (rows x 32, FP64 before BF16 rounding).
 
     ppred: N x 4 (predicted xyz, inverse mass); dprev: N x 3 previous correction;
     thickness: N; mesh: the mesh arrays (dist_v0/v1/rest, bend_v2/v3/rest);
     colliders: list of (type, A(3), B(3), radius); s = max(|g|, 1) dt^2;
     lscale = mean edge length; beta = history weight (strains at prediction + beta * dprev).
+    fp32_points: form the points prediction + beta * dprev as the GPU stores them, in fp32
+    (beta * dprev rounded, then the sum rounded); on a large cloth the coordinates' fp32 spacing
+    (2.4e-7 m at 2.5 m) is not small against s, so FP64 points would differ by that much.
     """
     nv = len(ppred)
-    p0 = ppred[:, :3].astype(np.float64) + beta * dprev.astype(np.float64)
+    if fp32_points:
+        b32 = np.float32(beta)
+        p0 = (ppred[:, :3].astype(np.float32) + (b32 * dprev.astype(np.float32))).astype(np.float64)
+    else:
+        p0 = ppred[:, :3].astype(np.float64) + beta * dprev.astype(np.float64)
     f = np.zeros((nv, F))
     f[:, 0:3] = soft(dprev / s)
     for col, (a, b, rest) in ((3, ("dist_v0", "dist_v1", "dist_rest")), (6, ("bend_v2", "bend_v3", "bend_rest"))):

@@ -485,3 +485,15 @@ def test_gpu_cloth_on_transformed_sdf_matches_oracle(require_gpu):
     # the sheet was caught: nothing fell through the collider's centre region
     p = out.reshape(-1, 3)
     assert p[:, 2].max() > 0.1
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_openmp_oracle_bake_equals_the_serial_bake(method):
+    """The multithreaded oracle build bakes voxel slabs on several threads; texels (and the coarse
+    pass's nearest lists) are the serial bake's bit for bit."""
+    v, t = lumpy_sphere(2)
+    a = O.bake_mesh_sdf(v, t, 0.02, 0.02, 0.5, 0.005, 0.0, True, method)
+    b = O.bake_mesh_sdf(v, t, 0.02, 0.02, 0.5, 0.005, 0.0, True, method, precision="f32_omp")
+    assert np.array_equal(a["texels"], b["texels"])
+    if method == 2:
+        assert np.array_equal(a["nearest_tri"], b["nearest_tri"])
