@@ -248,7 +248,8 @@ int tc_ws_net_init(tc_ws_net *net, uint64_t seed);
  * features rows x 32 (rounded to BF16), a symmetric neighbour graph in CSR form,
  * targets rows x 4 (target correction xyz in units of |g| dt^2, weight 0|1) or NULL.
  * Outputs: y rows x 16, grad (num_params), loss; each may be NULL.  The gradient stays on the
- * device for tc_ws_net_adam_step. */
+ * device for tc_ws_net_adam_step.  Without targets only the forward pass runs (no activations are
+ * kept, no gradient is formed). */
 int tc_ws_net_forward_backward(tc_ws_net *net, const float *features, uint32_t rows, const uint32_t *graph_off,
                                const uint32_t *graph_nbr, const float *targets, float *y, float *grad,
                                double *loss);

@@ -60,8 +60,10 @@ int ws_prepare(tc_sim *s) {
     if (s->ws_ready) return TC_OK;
     const tc::Mesh &m = s->mesh;
     const uint32_t nv = m.nv;
-    std::vector<uint32_t> off, nbr;
-    std::vector<float> rr;
+    // Each upload reads its own host array, and every array lives until the stream is synchronized
+    // below: an asynchronous copy from pageable memory may still be reading it after the call returns.
+    std::vector<uint32_t> doff, dnbr, boff, bnbr;
+    std::vector<float> drr, brr, grr;
     auto up32 = [&](DevBuf<uint32_t> &d, const std::vector<uint32_t> &h) -> hipError_t {
         hipError_t e = d.alloc(std::max<size_t>(h.size(), 1));
         if (e != hipSuccess || h.empty()) return e;
@@ -72,15 +74,15 @@ int ws_prepare(tc_sim *s) {
         if (e != hipSuccess || h.empty()) return e;
         return hipMemcpyAsync(d.p, h.data(), 4 * h.size(), hipMemcpyHostToDevice, s->stream);
     };
-    build_csr(nv, m.d_v0, m.d_v1, &m.d_rest, nullptr, off, nbr, rr);
-    TC_HIP(up32(s->ws_doff, off));
-    TC_HIP(up32(s->ws_dnbr, nbr));
-    TC_HIP(upf(s->ws_drest, rr));
-    build_csr(nv, m.b_v2, m.b_v3, &m.b_rest, nullptr, off, nbr, rr);
-    TC_HIP(up32(s->ws_boff, off));
-    TC_HIP(up32(s->ws_bnbr, nbr));
-    TC_HIP(upf(s->ws_brest, rr));
-    build_csr(nv, m.d_v0, m.d_v1, nullptr, &m.d_type, s->ws_goff_h, s->ws_gnbr_h, rr);
+    build_csr(nv, m.d_v0, m.d_v1, &m.d_rest, nullptr, doff, dnbr, drr);
+    TC_HIP(up32(s->ws_doff, doff));
+    TC_HIP(up32(s->ws_dnbr, dnbr));
+    TC_HIP(upf(s->ws_drest, drr));
+    build_csr(nv, m.b_v2, m.b_v3, &m.b_rest, nullptr, boff, bnbr, brr);
+    TC_HIP(up32(s->ws_boff, boff));
+    TC_HIP(up32(s->ws_bnbr, bnbr));
+    TC_HIP(upf(s->ws_brest, brr));
+    build_csr(nv, m.d_v0, m.d_v1, nullptr, &m.d_type, s->ws_goff_h, s->ws_gnbr_h, grr);
     TC_HIP(up32(s->ws_goff, s->ws_goff_h));
     s->ws_gmaxdeg = tc::ws::max_degree(s->ws_goff_h.data(), nv);
     TC_HIP(up32(s->ws_gnbr, s->ws_gnbr_h));
@@ -362,7 +364,8 @@ int tc_ws_net_forward_backward(tc_ws_net *n, const float *feat, uint32_t rows, c
     TC_HIP(hipMemcpyAsync(n->goff.p, graph_off, 4 * (size_t(rows) + 1), hipMemcpyHostToDevice, n->stream));
     if (nn) TC_HIP(hipMemcpyAsync(n->gnbr.p, graph_nbr, 4 * size_t(nn), hipMemcpyHostToDevice, n->stream));
     const tc::ws::Graph g{n->goff.p, n->gnbr.p, tc::ws::max_degree(graph_off, rows)};
-    const tc::ws::FwdBufs fw = train_bufs(n, n->feat.p);
+    // without targets there is no backward pass, so the forward keeps no activations for it
+    const tc::ws::FwdBufs fw = targets ? train_bufs(n, n->feat.p) : infer_like(n, n->feat.p);
     TC_HIP(tc::ws::forward(n->stream, n->nl, n->dev(), g, rows, fw));
     if (targets) {
         uint32_t nfree = 0;
