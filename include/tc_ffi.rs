@@ -59,6 +59,18 @@ extern "C" {
     pub fn tc_sim_instance_count(sim: *const TcSim, num_instances: *mut u32) -> i32;
     pub fn tc_sim_instance_range(sim: *const TcSim, instance: u32, first_vertex: *mut u32, num_vertices: *mut u32) -> i32;
     pub fn tc_sim_step(sim: *mut TcSim, dt: f32, substeps: u32, iterations: u32, effective_substeps: *mut u32) -> i32;
+    pub fn tc_sim_set_adaptive_substep_options(sim: *mut TcSim, enabled: i32, min_substeps: u32, max_substeps: u32) -> i32;
+    pub fn tc_sim_set_auto_compensate_iterations(sim: *mut TcSim, enabled: i32) -> i32;
+    pub fn tc_sim_set_strain_adaptive_options(sim: *mut TcSim, enabled: i32, tolerance: f32) -> i32;
+    pub fn tc_sim_set_enable_compact_readback(sim: *mut TcSim, enabled: i32) -> i32;
+    pub fn tc_sim_note_external_displacement(sim: *mut TcSim, displacement: f32) -> i32;
+    pub fn tc_sim_frame_policy_state(sim: *const TcSim, out: *mut f64) -> i32;
+    pub fn tc_sim_step_async(sim: *mut TcSim, dt: f32, substeps: u32, iterations: u32) -> i32;
+    pub fn tc_sim_fetch_positions(sim: *mut TcSim, out: *mut f32, n_floats: usize, ok: *mut i32) -> i32;
+    pub fn tc_sim_step_buffered(sim: *mut TcSim, dt: f32, substeps: u32, iterations: u32, count: *mut u32) -> i32;
+    pub fn tc_sim_fetch_buffered_positions(sim: *mut TcSim, out: *mut f32, n_floats: usize, count: *mut u32) -> i32;
+    pub fn tc_sim_get_buffered_frame_count(sim: *const TcSim, count: *mut u32) -> i32;
+    pub fn tc_sim_clear_frame_buffer(sim: *mut TcSim) -> i32;
     pub fn tc_sim_step_single_substep(sim: *mut TcSim, dt_sub: f32) -> i32;
     pub fn tc_sim_get_positions(sim: *mut TcSim, out: *mut f32, n_floats: usize) -> i32;
     pub fn tc_sim_set_positions_velocities(sim: *mut TcSim, positions: *const f32, velocities: *const f32, n: u32) -> i32;

@@ -89,10 +89,47 @@ int tc_sim_create_batch(const tc_instance *instances, uint32_t num_instances, co
 int tc_sim_instance_count(const tc_sim *sim, uint32_t *num_instances);
 int tc_sim_instance_range(const tc_sim *sim, uint32_t instance, uint32_t *first_vertex, uint32_t *num_vertices);
 
-/* ClothSimulator.step(dt, substeps, solver_iterations), lib.rs:1768-1786.
- * substeps 0 -> 20 (base_substeps); iterations 0 -> keep current.
+/* ClothSimulator.step(dt, substeps, solver_iterations), lib.rs:1768-1786 -> dispatch.rs:610-671.
+ * substeps 0 -> 20 (base_substeps), or the adaptive policy's choice when it is on; iterations 0 -> keep current.
  * Asynchronous on the simulator's stream; *effective_substeps may be NULL. */
 int tc_sim_step(tc_sim *sim, float dt, uint32_t substeps, uint32_t iterations, uint32_t *effective_substeps);
+/* Adaptive substeps and iterations: set_adaptive_substep_options(enabled, min, max), lib.rs:1797-1800
+ * -> simulation/mod.rs:723-744; set_auto_compensate_iterations, lib.rs:1950 -> mod.rs:760;
+ * set_strain_adaptive_options(enabled, tolerance), lib.rs:1961 -> mod.rs:775 (tolerance clamped to
+ * [1e-4, 0.1]); note_external_displacement(disp), lib.rs:1803 -> mod.rs:787.  With the policy on,
+ * tc_sim_step / _step_async / _step_buffered pick the frame's substeps (CFL on the largest per-axis
+ * displacement of the last frame read back or noted, a free-fall floor, a strain floor, hysteresis of
+ * +-2 per frame with an emergency jump, dispatch.rs:476-570) and its iterations (a propagation budget
+ * spread over the substeps, dispatch.rs:573-607); the effective substeps come back through
+ * *effective_substeps.  The feedback is read from the last frame's positions, copied asynchronously
+ * after the frame to pinned host memory and consumed at the next step once it has landed. */
+int tc_sim_set_adaptive_substep_options(tc_sim *sim, int enabled, uint32_t min_substeps, uint32_t max_substeps);
+int tc_sim_set_auto_compensate_iterations(tc_sim *sim, int enabled);
+int tc_sim_set_strain_adaptive_options(tc_sim *sim, int enabled, float tolerance);
+/* set_enable_compact_readback(enabled): whether step copies the positions for the policy's feedback. */
+int tc_sim_set_enable_compact_readback(tc_sim *sim, int enabled);
+int tc_sim_note_external_displacement(tc_sim *sim, float displacement);
+/* get_effective_substeps / get_effective_solver_iterations / get_last_step_max_strain, lib.rs:1787-1795,
+ * 1955: out[8] = [effective substeps of the last step, its iterations, P95 strain of the last feedback,
+ * largest displacement of the last feedback, displacement the last decision used, CFL characteristic
+ * length, mesh diameter in edge hops, mean edge length]. */
+int tc_sim_frame_policy_state(const tc_sim *sim, double *out);
+/* ClothSimulator.step_async(dt, substeps, solver_iterations), lib.rs:1823-1831 -> dispatch.rs:738-807:
+ * the frame is enqueued and its positions copied to pinned host memory without waiting; a step_async
+ * still uncollected is collected first.  fetch_positions(out), lib.rs:1833-1845 -> dispatch.rs:809-863:
+ * waits for that copy, writes min(n_floats, 3N) floats and sets *ok = 1; *ok = 0 when nothing is pending. */
+int tc_sim_step_async(tc_sim *sim, float dt, uint32_t substeps, uint32_t iterations);
+int tc_sim_fetch_positions(tc_sim *sim, float *out, size_t n_floats, int *ok);
+/* step_buffered(dt, substeps, solver_iterations) -> frames held, lib.rs:1847-1855 -> dispatch.rs:1293-1339:
+ * each frame's positions go to a device ring of 8 frames (the last slot is overwritten once full);
+ * fetch_buffered_positions(out) -> frames, lib.rs:1857-1870 -> dispatch.rs:1345-1398: one copy of all
+ * held frames (frame-major, 3N floats each), then the ring is empty; get_buffered_frame_count,
+ * clear_frame_buffer, lib.rs:1872-1880. */
+int tc_sim_step_buffered(tc_sim *sim, float dt, uint32_t substeps, uint32_t iterations, uint32_t *count);
+int tc_sim_fetch_buffered_positions(tc_sim *sim, float *out, size_t n_floats, uint32_t *count);
+int tc_sim_get_buffered_frame_count(const tc_sim *sim, uint32_t *count);
+int tc_sim_clear_frame_buffer(tc_sim *sim);
+
 /* ClothSimulator.step_single_substep(dt_sub), lib.rs:1808-1813 */
 int tc_sim_step_single_substep(tc_sim *sim, float dt_sub);
 /* ClothSimulator.get_positions(out), lib.rs:2110-2121 (blocking, n_floats >= 3N) */

@@ -15,7 +15,11 @@ LIB_PATH = os.path.join(HERE, "lib", "libtcloth_hip.so")
 EXPORTS = [
     "tc_sim_params_default", "tc_sim_create", "tc_sim_create_batch", "tc_sim_instance_count",
     "tc_sim_instance_range", "tc_sim_destroy", "tc_sim_step", "tc_sim_step_single_substep",
-    "tc_sim_get_positions", "tc_sim_set_positions_velocities", "tc_sim_reset", "tc_sim_add_sphere",
+    "tc_sim_get_positions", "tc_sim_set_positions_velocities",
+    "tc_sim_set_adaptive_substep_options", "tc_sim_set_auto_compensate_iterations", "tc_sim_set_strain_adaptive_options",
+    "tc_sim_set_enable_compact_readback", "tc_sim_note_external_displacement", "tc_sim_frame_policy_state",
+    "tc_sim_step_async", "tc_sim_fetch_positions", "tc_sim_step_buffered", "tc_sim_fetch_buffered_positions",
+    "tc_sim_get_buffered_frame_count", "tc_sim_clear_frame_buffer", "tc_sim_reset", "tc_sim_add_sphere",
     "tc_sim_add_capsule", "tc_sim_add_plane", "tc_sim_clear_colliders", "tc_sim_set_mesh_triangles",
     "tc_sim_set_gravity", "tc_sim_set_damping", "tc_sim_set_damping_all", "tc_sim_set_solver_iterations",
     "tc_sim_set_enable_self_collision", "tc_sim_set_self_collision_options", "tc_sim_set_stiffness_all",
@@ -80,6 +84,18 @@ def load() -> C.CDLL:
         "tc_sim_step": (_i, [_vp, _f32, _u32, _u32, C.POINTER(_u32)]),
         "tc_sim_step_single_substep": (_i, [_vp, _f32]),
         "tc_sim_get_positions": (_i, [_vp, _vp, C.c_size_t]),
+        "tc_sim_set_adaptive_substep_options": (_i, [_vp, _i, _u32, _u32]),
+        "tc_sim_set_auto_compensate_iterations": (_i, [_vp, _i]),
+        "tc_sim_set_strain_adaptive_options": (_i, [_vp, _i, _f32]),
+        "tc_sim_set_enable_compact_readback": (_i, [_vp, _i]),
+        "tc_sim_note_external_displacement": (_i, [_vp, _f32]),
+        "tc_sim_frame_policy_state": (_i, [_vp, C.POINTER(C.c_double)]),
+        "tc_sim_step_async": (_i, [_vp, _f32, _u32, _u32]),
+        "tc_sim_fetch_positions": (_i, [_vp, _vp, C.c_size_t, C.POINTER(_i)]),
+        "tc_sim_step_buffered": (_i, [_vp, _f32, _u32, _u32, C.POINTER(_u32)]),
+        "tc_sim_fetch_buffered_positions": (_i, [_vp, _vp, C.c_size_t, C.POINTER(_u32)]),
+        "tc_sim_get_buffered_frame_count": (_i, [_vp, C.POINTER(_u32)]),
+        "tc_sim_clear_frame_buffer": (_i, [_vp]),
         "tc_sim_set_positions_velocities": (_i, [_vp, _vp, _vp, _u32]),
         "tc_sim_reset": (_i, [_vp]),
         "tc_sim_add_sphere": (_i, [_vp, _vp, _f32, _f32, _f32]),

@@ -438,6 +438,8 @@ bool finite3(const float *p) { return std::isfinite(p[0]) && std::isfinite(p[1])
 
 }  // namespace
 
+int tc_run_frame(tc_sim *s, float dt, uint32_t substeps) { return run_frame(s, dt, substeps); }
+
 extern "C" {
 
 void tc_sim_params_default(tc_sim_params *p) {
@@ -640,6 +642,7 @@ static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
         (he = hipStreamSynchronize(s->stream)) != hipSuccess)
         return bail(fail(TC_ERR_RUNTIME, std::string("device init: ") + hipGetErrorString(he)));
     build_plan(s);
+    frames_init(s);
     *out = s;
     return TC_OK;
 }
@@ -695,15 +698,6 @@ void tc_sim_destroy(tc_sim *sim) {
     if (!sim) return;
     if (sim->stream) (void)hipStreamSynchronize(sim->stream);
     delete sim;
-}
-
-int tc_sim_step(tc_sim *s, float dt, uint32_t substeps, uint32_t iterations, uint32_t *eff) {
-    if (!s) return fail(TC_ERR_VALUE, "null simulator");
-    if (iterations > 0) s->iterations = iterations;
-    const uint32_t S = substeps > 0 ? substeps : 20u;
-    if (eff) *eff = S;
-    if (s->mesh.nv == 0) return TC_OK;  // dispatch.rs:611-613
-    return run_frame(s, dt, S);
 }
 
 int tc_sim_step_single_substep(tc_sim *s, float dt_sub) {

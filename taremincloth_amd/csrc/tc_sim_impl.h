@@ -228,7 +228,7 @@ struct tc_sim {
     DevBuf<float4> rec_T, rec_ppred, rec_D;
     bool ws_active() const { return ws_mode != 0 && ws_ready && (ws_mode == 1 || ws_net != nullptr); }
     bool ws_tracking() const { return ws_active() || rec_cap > 0; }
-    uint32_t iters_effective() const { return ws_active() ? ws_iters : iterations; }
+    uint32_t iters_effective() const { return ws_active() ? ws_iters : (eff_iters ? eff_iters : iterations); }
 
     // frame graph cache
     hipGraphExec_t graph_exec = nullptr;
@@ -252,6 +252,33 @@ struct tc_sim {
     uint64_t epoch = 1;  // bumped whenever a parameter or buffer baked into the graph changes
     bool use_graph = true;  // TCLOTH_USE_GRAPH=0 launches kernel by kernel (profilers)
     TcProfiler prof;        // set_profiling_enabled: frames launch kernel by kernel with pass events
+
+    // Adaptive substeps and iterations (dispatch.rs:477-607, simulation/mod.rs:332-348, 723-790):
+    // state of the host policy and of the feedback it reads from the last frame's positions.
+    bool adaptive = false;
+    uint32_t min_substeps = 4, max_substeps = 64, base_substeps = 20;
+    uint32_t eff_substeps = 0;         // 0: no history (the next step takes its target directly)
+    uint32_t eff_iters = 0;            // iterations of the current frame when the policy sets them (0: iterations)
+    bool auto_compensate = true, strain_adaptive = true, compact_readback = true;
+    float strain_tolerance = 0.008f;
+    float cloth_thickness = 0.005f, mesh_char_len = 0.01f, mesh_avg_edge_len = 0.02f;
+    uint32_t mesh_diameter_hops = 10;
+    float pending_external = 0.0f, last_max_disp = 0.0f, last_eff_disp = 0.0f, last_max_strain = 0.0f;
+    std::vector<float> prev_step_pos;  // 3N, positions of the last frame read back
+    // pinned host buffers for the positions copied after a frame (two each, so a copy in flight
+    // is never overwritten): the policy's feedback, step_async, and the buffered ring's download
+    struct HostReadback {
+        float *host = nullptr;
+        hipEvent_t ev = nullptr;
+        bool pending = false;
+    };
+    HostReadback fb[2], as[2];
+    uint32_t fb_next = 0, as_next = 0;
+    bool async_pending = false;
+    uint32_t as_ready = 0;             // buffer holding the positions of the last step_async
+    DevBuf<float> ring;                // step_buffered: up to ring_frames frames of 3N floats on the device
+    uint32_t ring_frames = 8, ring_count = 0;
+    float *ring_host = nullptr;
     int prof_begin(const char *name) { return prof.begin(stream, name); }
     void prof_end(int idx) { prof.end(stream, idx); }
 
@@ -263,6 +290,11 @@ struct tc_sim {
         pg_last.clear();
         pg_tail.clear();
         if (gate_host) (void)hipHostFree(gate_host);
+        for (HostReadback *b : {&fb[0], &fb[1], &as[0], &as[1]}) {
+            if (b->host) (void)hipHostFree(b->host);
+            if (b->ev) (void)hipEventDestroy(b->ev);
+        }
+        if (ring_host) (void)hipHostFree(ring_host);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void touch() { ++epoch; }
@@ -313,6 +345,11 @@ struct tc_sim {
     tc::HashBufs vhash() const { return {vkeys.p, vcounts.p, vstarts.p, vsorted.p}; }
     tc::HashBufs ehash() const { return {ekeys.p, ecounts.p, estarts.p, esorted.p}; }
 };
+
+// One frame of `substeps` substeps on the simulator's stream (graph replay when it applies).
+int tc_run_frame(tc_sim *s, float dt, uint32_t substeps);
+// The policy's mesh constants (simulation/mod.rs:380-420), set once at creation (tc_frames.hip).
+void frames_init(tc_sim *s);
 
 // Learned warm start hooks around the projection loop of one substep (tc_ws_api.hip).
 // ws_pre runs after predict: features, network and apply (or only the copy of the

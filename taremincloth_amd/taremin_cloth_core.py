@@ -13,6 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
+from . import sim_config as _sc
 
 __all__ = ["ClothSimulator", "color_edge_pairs", "compute_combined_inv_masses", "bake_mesh_sdf_gpu",
            "bake_mesh_sdf_hierarchical_gpu"]
@@ -114,6 +115,26 @@ class ClothSimulator:
         # set_areal_density only records the value (simulation/mod.rs:1241-1245); the substep
         # uses the inverse masses given at construction.
         self.areal_density = 0.15 if areal_density is None else float(min(max(areal_density, 0.0), 100.0))
+        self._init_config(stiffness, compression_stiffness, shear_stiffness, bending_stiffness, workgroup_size)
+        self._ctor = dict(positions=pos, edges=e, faces=f, inv_masses=im, thicknesses=th, layer_ids=ly,
+                          layer_id=int(layer_id), thickness=float(thickness), stiffness=float(stiffness),
+                          bending_stiffness=float(bending_stiffness), compression_stiffness=compression_stiffness,
+                          shear_stiffness=shear_stiffness)
+        self._colliders = []
+        self._recorder = None
+
+    def _init_config(self, stiffness, compression_stiffness, shear_stiffness, bending_stiffness, workgroup_size):
+        """The simulator's parameters as SimConfig fields, with the runtime defaults of
+        simulation/mod.rs:607-691 (not SimConfig's own defaults, config.rs:156-213)."""
+        cfg = _sc.defaults()
+        cfg.update(damping=1.0, tension_damping=0.0, compression_damping=0.0, shear_damping=0.0,
+                   bending_damping=0.0, tension_stiffness=float(stiffness),
+                   compression_stiffness=float(stiffness if compression_stiffness is None else compression_stiffness),
+                   shear_stiffness=float(stiffness * 0.5 if shear_stiffness is None else shear_stiffness),
+                   bending_stiffness=float(bending_stiffness), solver_iterations=2, solver_mode=0,
+                   workgroup_size=int(workgroup_size), enable_self_collision=True, relief_factor=0.2,
+                   max_displacement_ratio=0.2, post_relaxation_iters=1, areal_density=self.areal_density)
+        self._cfg = cfg
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -123,13 +144,156 @@ class ClothSimulator:
 
     # --- stepping ----------------------------------------------------------------------------
     def step(self, dt=0.016666667, substeps=None, solver_iterations=None) -> int:
+        """One frame; returns the substeps it ran (the adaptive policy's choice when it is on)."""
         eff = C.c_uint32(0)
-        it = 0 if solver_iterations is None else int(solver_iterations)
         if solver_iterations is not None:
-            _lib.check(self._L.tc_sim_set_solver_iterations(self._h, it))
+            self.set_solver_iterations(solver_iterations)
         _lib.check(self._L.tc_sim_step(self._h, float(dt), 0 if substeps is None else int(substeps), 0,
                                        C.byref(eff)))
+        if self._recorder is not None:
+            self._recorder.record(self, float(dt), int(eff.value))
         return int(eff.value)
+
+    def step_async(self, dt=0.016666667, substeps=None, solver_iterations=None):
+        """Enqueues a frame and the copy of its positions without waiting (lib.rs:1823-1831)."""
+        if solver_iterations is not None:
+            self.set_solver_iterations(solver_iterations)
+        _lib.check(self._L.tc_sim_step_async(self._h, float(dt), 0 if substeps is None else int(substeps), 0))
+
+    def fetch_positions(self, out_array) -> bool:
+        """Positions of the last step_async into out_array; False when none is pending (lib.rs:1833-1845)."""
+        out = self._writable_f32(out_array)
+        ok = C.c_int(0)
+        _lib.check(self._L.tc_sim_fetch_positions(self._h, out.ctypes.data_as(C.c_void_p), out.size, C.byref(ok)))
+        return bool(ok.value)
+
+    def step_buffered(self, dt=0.016666667, substeps=None, solver_iterations=None) -> int:
+        """A frame whose positions stay in a device ring of 8; returns the frames held (lib.rs:1847-1855)."""
+        if solver_iterations is not None:
+            self.set_solver_iterations(solver_iterations)
+        cnt = C.c_uint32(0)
+        _lib.check(self._L.tc_sim_step_buffered(self._h, float(dt), 0 if substeps is None else int(substeps), 0,
+                                                C.byref(cnt)))
+        return int(cnt.value)
+
+    def fetch_buffered_positions(self, out_array) -> int:
+        """All held frames (frame-major) in one copy; returns their number and empties the ring."""
+        out = self._writable_f32(out_array)
+        cnt = C.c_uint32(0)
+        _lib.check(self._L.tc_sim_fetch_buffered_positions(self._h, out.ctypes.data_as(C.c_void_p), out.size,
+                                                           C.byref(cnt)))
+        return int(cnt.value)
+
+    def get_buffered_frame_count(self) -> int:
+        cnt = C.c_uint32(0)
+        _lib.check(self._L.tc_sim_get_buffered_frame_count(self._h, C.byref(cnt)))
+        return int(cnt.value)
+
+    def clear_frame_buffer(self):
+        _lib.check(self._L.tc_sim_clear_frame_buffer(self._h))
+
+    @staticmethod
+    def _writable_f32(a):
+        if not isinstance(a, np.ndarray) or a.dtype != np.float32 or not a.flags.c_contiguous or not a.flags.writeable:
+            raise ValueError("out_array must be a writable C-contiguous float32 numpy array")
+        return a
+
+    # --- adaptive substeps (lib.rs:1787-1805, 1950-1962) ----------------------------------------
+    def set_adaptive_substep_options(self, enabled, min_substeps, max_substeps):
+        lo = max(int(min_substeps), 1)
+        hi = max(int(max_substeps), lo)
+        _lib.check(self._L.tc_sim_set_adaptive_substep_options(self._h, 1 if enabled else 0, lo, hi))
+        self._cfg.update(enable_adaptive_substep=bool(enabled), min_substeps=lo, max_substeps=hi)
+
+    def set_auto_compensate_iterations(self, enabled):
+        _lib.check(self._L.tc_sim_set_auto_compensate_iterations(self._h, 1 if enabled else 0))
+        self._cfg["auto_compensate_iterations"] = bool(enabled)
+
+    def set_strain_adaptive_options(self, enabled, tolerance):
+        tol = float(np.float32(min(max(float(tolerance), 0.0001), 0.1)))
+        _lib.check(self._L.tc_sim_set_strain_adaptive_options(self._h, 1 if enabled else 0, tol))
+        self._cfg.update(enable_strain_adaptive=bool(enabled), strain_tolerance=tol)
+
+    def set_enable_compact_readback(self, enabled):
+        _lib.check(self._L.tc_sim_set_enable_compact_readback(self._h, 1 if enabled else 0))
+
+    def note_external_displacement(self, disp):
+        _lib.check(self._L.tc_sim_note_external_displacement(self._h, float(disp)))
+
+    def _policy(self):
+        out = (C.c_double * 8)()
+        _lib.check(self._L.tc_sim_frame_policy_state(self._h, out))
+        return list(out)
+
+    def get_effective_substeps(self) -> int:
+        return int(self._policy()[0])
+
+    def get_effective_solver_iterations(self) -> int:
+        return int(self._policy()[1])
+
+    def get_last_step_max_strain(self) -> float:
+        return float(self._policy()[2])
+
+    # --- SimConfig (lib.rs:3028-3048, simulation/config_impl.rs) ---------------------------------
+    def get_config_json(self) -> str:
+        return _sc.to_json(self._cfg)
+
+    def get_config_hash(self) -> int:
+        return _sc.fnv1a64(self.get_config_json())
+
+    def apply_config_json(self, json_str):
+        """Applies every field through its setter (config_impl.rs:91-166).  Options whose solver is not
+        part of this build raise NotImplementedError unless they keep their inert value."""
+        c = _sc.parse(json_str)
+        inert = {"coupled_mode": 0, "coupled_collider": False, "substep_interval": 1, "ee_substep_interval": 1,
+                 "self_collision_algorithm": 0, "enable_pair_cache": False, "enable_edge_collision": False,
+                 "sewing_priority_enabled": False, "solver_mode": 0}
+        for k, v in inert.items():
+            if c[k] != v:
+                raise NotImplementedError(f"SimConfig {k}={c[k]!r}: that solver option is not part of this build")
+        self.set_gravity(*c["gravity"])
+        self.set_damping(c["damping"])
+        self.set_damping_all(c["tension_damping"], c["compression_damping"], c["shear_damping"], c["bending_damping"])
+        self.set_stiffness_all(c["tension_stiffness"], c["compression_stiffness"], c["shear_stiffness"],
+                               c["bending_stiffness"])
+        self.set_solver_iterations(c["solver_iterations"])
+        self.set_workgroup_size(c["workgroup_size"])
+        self.set_enable_self_collision(c["enable_self_collision"])
+        self.set_self_collision_options(c["relief_factor"], c["max_displacement_ratio"], c["exclude_neighbors"],
+                                        c["enable_normal_untangling"], c["self_collision_max_iterations"])
+        self.set_adaptive_substep_options(c["enable_adaptive_substep"], c["min_substeps"], c["max_substeps"])
+        self.set_auto_compensate_iterations(c["auto_compensate_iterations"])
+        self.set_strain_adaptive_options(c["enable_strain_adaptive"], c["strain_tolerance"])
+        self.areal_density = float(min(max(c["areal_density"], 0.0), 100.0))
+        # recorded as given: options of solvers this build does not have, kept at their inert values
+        for k in ("post_relaxation_iters", "auto_coupled_on_low_substeps", "pair_margin_mode", "pair_safety_margin",
+                  "pair_horizon_scale", "pair_max_horizon", "pair_max_pairs", "enable_pair_final_fallback",
+                  "edge_margin_scale", "edge_margin_offset", "sewing_stiffness", "sewing_shrink_speed",
+                  "enable_sewing_lock", "sewing_lock_distance", "sewing_priority_threshold",
+                  "sewing_priority_merge_dist", "sewing_priority_ramp_frames", "sewing_priority_max_frames",
+                  "areal_density", "version"):
+            self._cfg[k] = c[k]
+
+    # --- debug recording (lib.rs:2956-3012, debug_recorder.rs) ----------------------------------
+    def start_debug_recording(self, object_name, max_frames=None):
+        from .debug_recorder import DebugRecorder
+        self._recorder = DebugRecorder(self, str(object_name), 3600 if max_frames is None else int(max_frames))
+
+    def is_debug_recording(self) -> bool:
+        return self._recorder is not None
+
+    def stop_debug_recording(self):
+        self._recording_kept = self._recorder
+        self._recorder = None
+
+    def save_debug_recording(self, file_path) -> str:
+        rec = self._recorder if self._recorder is not None else getattr(self, "_recording_kept", None)
+        if rec is None:
+            raise RuntimeError("no debug recording to save")
+        try:
+            return rec.save(str(file_path))
+        except OSError as e:
+            raise OSError(str(e)) from None
 
     def step_single_substep(self, dt_sub=0.0016666667):
         _lib.check(self._L.tc_sim_step_single_substep(self._h, float(dt_sub)))
@@ -153,16 +317,23 @@ class ClothSimulator:
 
     # --- colliders -----------------------------------------------------------------------------
     def add_sphere_collider(self, center, radius, friction=0.0, restitution=0.0):
-        _lib.check(self._L.tc_sim_add_sphere(self._h, _ptr(_vec3(center, "center")), float(radius), float(friction),
-                                             float(restitution)))
+        c = _vec3(center, "center")
+        _lib.check(self._L.tc_sim_add_sphere(self._h, _ptr(c), float(radius), float(friction), float(restitution)))
+        self._colliders.append({"type": "sphere", "center": c.tolist(), "radius": float(radius),
+                                "friction": float(friction), "restitution": float(restitution)})
 
     def add_capsule_collider(self, a, b, radius, friction=0.0, restitution=0.0):
-        _lib.check(self._L.tc_sim_add_capsule(self._h, _ptr(_vec3(a, "a")), _ptr(_vec3(b, "b")), float(radius),
-                                              float(friction), float(restitution)))
+        pa, pb = _vec3(a, "a"), _vec3(b, "b")
+        _lib.check(self._L.tc_sim_add_capsule(self._h, _ptr(pa), _ptr(pb), float(radius), float(friction),
+                                              float(restitution)))
+        self._colliders.append({"type": "capsule", "point_a": pa.tolist(), "point_b": pb.tolist(),
+                                "radius": float(radius), "friction": float(friction), "restitution": float(restitution)})
 
     def add_plane_collider(self, point, normal, friction=0.0, restitution=0.0):
-        _lib.check(self._L.tc_sim_add_plane(self._h, _ptr(_vec3(point, "point")), _ptr(_vec3(normal, "normal")),
-                                            float(friction), float(restitution)))
+        pt, nm = _vec3(point, "point"), _vec3(normal, "normal")
+        _lib.check(self._L.tc_sim_add_plane(self._h, _ptr(pt), _ptr(nm), float(friction), float(restitution)))
+        self._colliders.append({"type": "plane", "point": pt.tolist(), "normal": nm.tolist(),
+                                "friction": float(friction), "restitution": float(restitution)})
 
     def set_mesh_collider_triangles(self, tris, friction=0.0, thickness=0.0, restitution=0.0, single_sided=False,
                                     attributes=None):
@@ -176,26 +347,39 @@ class ClothSimulator:
                 raise ValueError("attributes must have shape (T, 4)")
         _lib.check(self._L.tc_sim_set_mesh_triangles(self._h, _ptr(t), len(t), _ptr(at), float(friction),
                                                      float(thickness), float(restitution), 1 if single_sided else 0))
+        self._colliders = [c for c in self._colliders if c["type"] != "mesh"]
+        if len(t):
+            self._colliders.append({"type": "mesh", "triangles": t.reshape(-1, 9).tolist(), "friction": float(friction),
+                                    "thickness": float(thickness), "restitution": float(restitution),
+                                    "single_sided": bool(single_sided)})
 
     def clear_colliders(self):
         _lib.check(self._L.tc_sim_clear_colliders(self._h))
+        self._colliders = []
 
     # --- parameters ----------------------------------------------------------------------------
     def set_gravity(self, gx, gy, gz):
         _lib.check(self._L.tc_sim_set_gravity(self._h, float(gx), float(gy), float(gz)))
+        self._cfg["gravity"] = [float(np.float32(gx)), float(np.float32(gy)), float(np.float32(gz))]
 
     def set_damping(self, damping):
         _lib.check(self._L.tc_sim_set_damping(self._h, float(damping)))
+        self._cfg["damping"] = float(np.float32(max(float(damping), 0.0)))
 
     def set_damping_all(self, tension, compression, shear, bending):
         _lib.check(self._L.tc_sim_set_damping_all(self._h, float(tension), float(compression), float(shear),
                                                   float(bending)))
+        for k, v in zip(("tension_damping", "compression_damping", "shear_damping", "bending_damping"),
+                        (tension, compression, shear, bending)):
+            self._cfg[k] = float(np.float32(max(float(v), 0.0)))
 
     def set_solver_iterations(self, iterations):
         _lib.check(self._L.tc_sim_set_solver_iterations(self._h, int(iterations)))
+        self._cfg["solver_iterations"] = int(iterations)
 
     def set_enable_self_collision(self, enable):
         _lib.check(self._L.tc_sim_set_enable_self_collision(self._h, 1 if enable else 0))
+        self._cfg["enable_self_collision"] = bool(enable)
 
     def set_self_collision_options(self, relief_factor, max_displacement_ratio, exclude_neighbors=True,
                                    enable_normal_untangling=True, max_iterations=256):
@@ -204,13 +388,27 @@ class ClothSimulator:
                                                              1 if exclude_neighbors else 0,
                                                              1 if enable_normal_untangling else 0,
                                                              int(max_iterations)))
+        self._cfg.update(relief_factor=float(np.float32(relief_factor)),
+                         max_displacement_ratio=float(np.float32(max_displacement_ratio)),
+                         exclude_neighbors=bool(exclude_neighbors),
+                         enable_normal_untangling=bool(enable_normal_untangling),
+                         self_collision_max_iterations=int(max_iterations))
 
     def set_stiffness_all(self, tension, compression, shear, bending):
         _lib.check(self._L.tc_sim_set_stiffness_all(self._h, float(tension), float(compression), float(shear),
                                                     float(bending)))
+        for k, v in zip(("tension_stiffness", "compression_stiffness", "shear_stiffness", "bending_stiffness"),
+                        (tension, compression, shear, bending)):
+            self._cfg[k] = float(np.float32(v))
 
     def set_workgroup_size(self, size):
         """Accepted for API compatibility; the block size is tuned per kernel on gfx950."""
+        self._cfg["workgroup_size"] = int(size)
+
+    def set_areal_density(self, density):
+        """Recorded only (simulation/mod.rs:1241-1245); the masses are those given at construction."""
+        self.areal_density = float(min(max(float(density), 0.0), 100.0))
+        self._cfg["areal_density"] = float(np.float32(self.areal_density))
 
     # extensions (no reference counterpart)
     def set_compliances_raw(self, stretch, shear, bend):
@@ -500,6 +698,10 @@ class ClothBatch(ClothSimulator):
         self._n = total
         self._keep = keep
         self.areal_density = 0.15 if areal_density is None else float(min(max(areal_density, 0.0), 100.0))
+        self._init_config(stiffness, compression_stiffness, shear_stiffness, bending_stiffness, 32)
+        self._ctor = None
+        self._colliders = []
+        self._recorder = None
 
     def num_instances(self) -> int:
         n = C.c_uint32(0)

@@ -1,0 +1,217 @@
+"""Frame-level host policies: adaptive substeps (dispatch.rs:476-607), asynchronous and buffered
+readback (dispatch.rs:738-863, 1293-1403), SimConfig JSON (config.rs, config_impl.rs) and the debug
+recording with its replay (debug_recorder.rs, simulation/recording.rs)."""
+import json
+
+import numpy as np
+import pytest
+
+import cases
+from taremincloth_amd import sim_config as SC
+
+
+# ---------------------------------------------------------------- CPU
+def test_fnv1a64_known_vectors():
+    assert SC.fnv1a64("") == 0xCBF29CE484222325
+    assert SC.fnv1a64("a") == 0xAF63DC4C8601EC8C
+    assert SC.fnv1a64("foobar") == 0x85944171F73967E8
+
+
+def test_config_defaults_and_missing_fields():
+    d = SC.parse("{}")   # config.rs:238-246
+    assert d == SC.defaults()
+    assert d["version"] == 1 and d["enable_adaptive_substep"] is False
+    assert d["min_substeps"] == 4 and d["max_substeps"] == 64
+    assert SC.parse(SC.to_json(d)) == d
+    js = SC.to_json(d)
+    assert js.startswith('{"version":1,"gravity":[0.0,0.0,-9.81],"damping":1.0,')
+    assert '"strain_tolerance":0.008}' in js and '"pair_max_pairs":65536' in js
+    assert list(json.loads(js)) == [n for n, _, _ in SC.FIELDS]
+
+
+def test_config_f32_text():
+    assert SC._f32_text(1.0) == "1.0" and SC._f32_text(-9.81) == "-9.81"
+    assert SC._f32_text(0.005) == "0.005" and SC._f32_text(1500.0) == "1500.0"
+    assert SC._f32_text(1e-7) == "1e-7" and SC._f32_text(0.0) == "0.0"
+    for x in (0.1 + 0.2, 3.3e-5, 123456.7, 1e20, -2.5e-9):
+        assert np.float32(float(SC._f32_text(x))) == np.float32(x), x
+
+
+def test_config_rejects_malformed_json():
+    with pytest.raises(ValueError):
+        SC.parse("{invalid json")
+    with pytest.raises(ValueError):
+        SC.parse('{"solver_iterations": -1}')
+    with pytest.raises(ValueError):
+        SC.parse('{"gravity": [0, 1]}')
+
+
+# ---------------------------------------------------------------- GPU
+def _T():
+    from taremincloth_amd import taremin_cloth_core as T
+    return T
+
+
+def _grid4(pin_all=True):
+    pos, e, f = cases.grid(4, 4, 1.0 / 3.0)
+    inv = np.zeros(len(pos), np.float32) if pin_all else np.ones(len(pos), np.float32)
+    return pos, e, f, inv
+
+
+@pytest.mark.gpu
+def test_gpu_adaptive_disabled_returns_requested_or_base(require_gpu):
+    T = _T()
+    pos, e, f, inv = _grid4()
+    sim = T.ClothSimulator(pos, e, f, inv)
+    assert sim.step(1 / 60, 15) == 15 and sim.get_effective_substeps() == 15
+    assert sim.step(1 / 60) == 20
+
+
+@pytest.mark.gpu
+def test_gpu_adaptive_hysteresis_smooth_decay_from_peak(require_gpu):
+    """tests/physics/test_adaptive_substep.py:99-125: min 4, max 20, all vertices pinned: the first
+    step takes the target 4; a noted 0.2 m displacement jumps to 20; then -2 per step, kept across a
+    stiffness change, down to 4."""
+    T = _T()
+    pos, e, f, inv = _grid4()
+    sim = T.ClothSimulator(pos, e, f, inv, thickness=0.005)
+    sim.set_adaptive_substep_options(True, 4, 20)
+    out = np.zeros(pos.size, np.float32)
+
+    def frame():
+        n = sim.step(1.0 / 60.0, 20)
+        sim.get_positions(out)
+        return n
+
+    assert frame() == 4
+    sim.note_external_displacement(0.20)
+    assert frame() == 20
+    assert frame() == 18
+    sim.set_stiffness_all(1600.0, 1600.0, 800.0, 10.0)
+    assert frame() == 16
+    assert [frame() for _ in range(8)] == [14, 12, 10, 8, 6, 4, 4, 4]
+    # re-applying the options while enabled keeps the history; disabling clears it
+    sim.note_external_displacement(0.2)
+    assert frame() == 20
+    sim.set_adaptive_substep_options(True, 4, 12)
+    assert sim.get_effective_substeps() == 12
+    sim.set_adaptive_substep_options(False, 4, 12)
+    assert sim.get_effective_substeps() == 0
+
+
+@pytest.mark.gpu
+def test_gpu_adaptive_rapid_motion_boosts_substeps(require_gpu):
+    """A free cloth falling at twice gravity: its own displacement raises the substeps within the cap;
+    a large noted displacement jumps to the cap; iterations are compensated for the substeps."""
+    T = _T()
+    pos, e, f, inv = _grid4(pin_all=False)
+    sim = T.ClothSimulator(pos, e, f, inv, thickness=0.005)
+    sim.set_gravity(0.0, 0.0, -19.62)
+    sim.set_adaptive_substep_options(True, 4, 20)
+    out = np.zeros(pos.size, np.float32)
+    seen = []
+    for _ in range(16):
+        seen.append(sim.step(1.0 / 60.0, 20))
+        sim.get_positions(out)
+    assert max(seen) > 4 and max(seen) <= 20, seen
+    sim.note_external_displacement(1.0)
+    assert sim.step(1.0 / 60.0, 20) == 20
+    # iteration budget (dispatch.rs:573-607): min(20 * 2, max(hops clamp(15, 30), 20)) = 20 solves over 20
+    # substeps -> 1, clamped to [2, 8]
+    assert sim.get_effective_solver_iterations() == 2
+
+
+@pytest.mark.gpu
+def test_gpu_async_and_buffered_readback_equal_synchronous_steps(require_gpu):
+    T = _T()
+    pos, e, f, inv = cases.drape_grid(24, 0.02, 0.3, seed=1)
+
+    def make():
+        s = T.ClothSimulator(pos, e, f, inv)
+        s.add_sphere_collider([0.0, 0.0, 0.0], 0.2, 0.3, 0.0)
+        return s
+
+    ref = make()
+    want = []
+    for _ in range(5):
+        ref.step(1 / 60, 10, 4)
+        o = np.zeros(pos.size, np.float32)
+        ref.get_positions(o)
+        want.append(o)
+    a = make()
+    got = np.zeros(pos.size, np.float32)
+    assert a.fetch_positions(got) is False          # nothing pending
+    a.step_async(1 / 60, 10, 4)
+    assert a.fetch_positions(got) is True and np.array_equal(got, want[0])
+    a.step_async(1 / 60, 10, 4)
+    a.step_async(1 / 60, 10, 4)                      # collects the previous one first
+    assert a.fetch_positions(got) is True and np.array_equal(got, want[2])
+    with pytest.raises(ValueError):
+        a.fetch_positions(np.zeros(pos.size, np.float64))
+    b = make()
+    assert [b.step_buffered(1 / 60, 10, 4) for _ in range(3)] == [1, 2, 3]
+    assert b.get_buffered_frame_count() == 3
+    ring = np.zeros(3 * pos.size, np.float32)
+    assert b.fetch_buffered_positions(ring) == 3
+    assert np.array_equal(ring.reshape(3, -1), np.stack(want[:3]))
+    assert b.get_buffered_frame_count() == 0 and b.fetch_buffered_positions(ring) == 0
+    b.step_buffered(1 / 60, 10, 4)
+    b.clear_frame_buffer()
+    assert b.get_buffered_frame_count() == 0
+
+
+@pytest.mark.gpu
+def test_gpu_config_json_roundtrip_and_apply(require_gpu):
+    """tests/core/test_config_parity.py:87-177 for the options this build has."""
+    T = _T()
+    pos, e, f, inv = _grid4(pin_all=False)
+    sim = T.ClothSimulator(pos, e, f, inv)
+    cfg = json.loads(sim.get_config_json())
+    for key in ("coupled_mode", "post_relaxation_iters", "substep_interval", "sewing_stiffness", "enable_sewing_lock",
+                "sewing_lock_distance", "sewing_priority_enabled", "solver_mode", "workgroup_size",
+                "enable_pair_cache", "pair_margin_mode"):
+        assert key in cfg
+    assert cfg["solver_iterations"] == 2 and cfg["enable_self_collision"] is True and cfg["damping"] == 1.0
+    h0 = sim.get_config_hash()
+    cfg["solver_iterations"] = 7
+    cfg["sewing_stiffness"] = 12345.0
+    cfg["tension_damping"] = 5.0
+    sim.apply_config_json(json.dumps(cfg))
+    cfg2 = json.loads(sim.get_config_json())
+    assert cfg2["solver_iterations"] == 7 and cfg2["sewing_stiffness"] == 12345.0 and cfg2["tension_damping"] == 5.0
+    assert sim.get_config_hash() != h0
+    with pytest.raises(ValueError):
+        sim.apply_config_json("{invalid json")
+    cfg2["coupled_mode"] = 1
+    with pytest.raises(NotImplementedError):
+        sim.apply_config_json(json.dumps(cfg2))
+
+
+@pytest.mark.gpu
+def test_gpu_debug_recording_replays_bit_for_bit(require_gpu, tmp_path):
+    from taremincloth_amd import debug_recorder as DR
+    T = _T()
+    pos, e, f, inv = cases.drape_grid(20, 0.02, 0.25, seed=4)
+    sim = T.ClothSimulator(pos, e, f, inv)
+    sim.add_sphere_collider([0.0, 0.0, 0.0], 0.15, 0.3, 0.0)
+    sim.set_solver_iterations(3)
+    sim.start_debug_recording("cloth", max_frames=100)
+    assert sim.is_debug_recording()
+    for k in range(6):
+        if k == 3:
+            sim.set_stiffness_all(800.0, 800.0, 400.0, 5.0)
+            sim.add_plane_collider([0.0, 0.0, -0.05], [0.0, 0.0, 1.0], 0.5, 0.0)
+        sim.step(1 / 60, 10)
+    sim.stop_debug_recording()
+    path = sim.save_debug_recording(str(tmp_path / "rec.jsonl.gz"))
+    meta, frames = DR.load(path)
+    assert meta["num_vertices"] == len(pos) and len(frames) == 6
+    assert frames[0]["param_deltas"] is None and frames[3]["param_deltas"] is not None
+    assert frames[3]["param_deltas"]["tension_stiffness"] == 800.0
+    assert [c["type"] for c in frames[3]["colliders"]] == ["sphere", "plane"]
+    assert not frames[5]["stats"]["has_nan_or_inf"] and frames[5]["stats"]["config_hash"] == sim.get_config_hash()
+    n = 0
+    for fr, p in DR.replay(path):
+        assert np.array_equal(p, np.asarray(fr["positions"], np.float32)), fr["frame_index"]
+        n += 1
+    assert n == 6
