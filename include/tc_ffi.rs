@@ -132,6 +132,7 @@ extern "C" {
     pub fn tc_sdf_result_info(res: *const TcSdfResult, dims: *mut u32, bone_info: *mut f32, coarse_voxel: *mut f32) -> i32;
     pub fn tc_sdf_result_timing(res: *const TcSdfResult, kernel_ms: *mut f32, fallback_voxels: *mut u32) -> i32;
     pub fn tc_sdf_result_texels(res: *const TcSdfResult) -> *const u32;
+    pub fn tc_sdf_result_atlas(res: *const TcSdfResult, brick_dims: *mut u32) -> *const f32;
     pub fn tc_sdf_result_nearest(res: *const TcSdfResult) -> *const u32;
     pub fn tc_sdf_result_free(res: *mut TcSdfResult);
     pub fn tc_close_open_boundary_loops(verts: *const f32, num_verts: u32, tris: *const i32, num_tris: u32, out_verts: *mut f32, cap_verts: usize, out_tris: *mut i32, cap_tris: usize, num_verts_out: *mut u32, num_tris_out: *mut u32, loops_closed: *mut u32) -> i32;

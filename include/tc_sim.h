@@ -362,7 +362,8 @@ int tc_sim_ws_time(tc_sim *sim, float dt_sub, uint32_t reps, double *out);
  * (coarse top-8 pass + fine pass), 2: the coarse pass alone (bake_mesh_sdf_coarse_ext,
  * sdf_baker.rs:1350-1600: cell width 4 x voxel_size, 8 nearest triangle indices per cell).
  * The result holds width x height x depth texels (pack2x16float(signed distance, 1.0), Z/Y/X
- * order), the 20-float collider info and, for method 2, the nearest-triangle indices. */
+ * order), the 20-float collider info and, for method 2, the nearest-triangle indices; method 3 is the
+ * dense bake with an fp32 brick atlas as well (tc_sdf_result_atlas). */
 typedef struct tc_sdf_result tc_sdf_result;
 int tc_bake_mesh_sdf(const float *verts, uint32_t num_verts, const int32_t *tris, uint32_t num_tris, float voxel_size,
                      float margin, float friction, float thickness, float restitution, int close_open_boundaries,
@@ -376,6 +377,13 @@ int tc_sdf_result_info(const tc_sdf_result *res, uint32_t *dims, float *bone_inf
  * neither). */
 int tc_sdf_result_timing(const tc_sdf_result *res, float *kernel_ms, uint32_t *fallback_voxels);
 const uint32_t *tc_sdf_result_texels(const tc_sdf_result *res);
+/* method 3 (no reference counterpart; the reference stores only pack2x16float texels,
+ * bake_mesh_sdf.wgsl:120): the dense bake's texels plus an fp32 brick atlas of the same voxels,
+ * ceil(w/8) x ceil(h/8) x ceil(d/8) bricks of 8^3 floats (signed distance to the nearest triangle,
+ * sign from its face normal), bricks and voxels within a brick in Z/Y/X order, voxels past the
+ * domain's edge holding the nearest voxel's value.  Returns NULL for other methods; brick_dims
+ * (may be NULL) receives the brick counts. */
+const float *tc_sdf_result_atlas(const tc_sdf_result *res, uint32_t *brick_dims);
 const uint32_t *tc_sdf_result_nearest(const tc_sdf_result *res);
 void tc_sdf_result_free(tc_sdf_result *res);
 /* close_open_boundary_loops_mesh (sdf_baker.rs:455-462): caps the closed boundary loops of a solid

@@ -35,7 +35,7 @@ EXPORTS = [
     "tc_sim_ws_num_records", "tc_sim_ws_get_record", "tc_sim_ws_last_output", "tc_sim_ws_history",
     "tc_ws_eval", "tc_sim_ws_time", "tc_ws_net_forward_noalloc",
     "tc_sim_set_sdf_colliders", "tc_sim_update_bone_transforms", "tc_bake_mesh_sdf", "tc_sdf_result_info",
-    "tc_sdf_result_timing", "tc_sdf_result_texels", "tc_sdf_result_nearest", "tc_sdf_result_free", "tc_close_open_boundary_loops",
+    "tc_sdf_result_timing", "tc_sdf_result_texels", "tc_sdf_result_atlas", "tc_sdf_result_nearest", "tc_sdf_result_free", "tc_close_open_boundary_loops",
 ]
 
 TC_OK, TC_ERR_VALUE, TC_ERR_RUNTIME, TC_ERR_UNSUPPORTED, TC_ERR_IO = 0, 1, 2, 3, 4
@@ -160,6 +160,7 @@ def load() -> C.CDLL:
         "tc_sdf_result_info": (_i, [_vp, _vp, _vp, C.POINTER(_f32)]),
         "tc_sdf_result_timing": (_i, [_vp, _vp, _vp]),
         "tc_sdf_result_texels": (_vp, [_vp]),
+        "tc_sdf_result_atlas": (_vp, [_vp, _vp]),
         "tc_sdf_result_nearest": (_vp, [_vp]),
         "tc_sdf_result_free": (None, [_vp]),
         "tc_close_open_boundary_loops": (_i, [_vp, _u32, _vp, _u32, _vp, C.c_size_t, _vp, C.c_size_t,

@@ -65,6 +65,12 @@ struct DenseArgs {
     uint32_t brick0, brick_count;  // the chunk of bricks of one coarse launch
 };
 constexpr uint32_t SEG_CLUSTERS = 128;  // clusters per segment of the segmented traversals
+// fp32 brick atlas of a dense bake: ceil(w/8) x ceil(h/8) x ceil(d/8) bricks of 8^3 voxels, bricks
+// and the voxels inside a brick both in Z/Y/X order; a voxel past the domain's edge holds the value
+// of the nearest voxel inside it (clamp to edge).  One wavefront per brick reads the dense bake's
+// (d^2, index) keys and writes the signed fp32 distance to that triangle.
+constexpr uint32_t ATLAS_BRICK = 8;
+hipError_t launch_bake_atlas(hipStream_t st, const DenseArgs &a, float *atlas);
 // topk 1: upper bound, (brick, cluster segment) traversal, finish; topk 8: upper bound, then per
 // chunk of bricks the (brick, segment) lists and their merge.
 hipError_t launch_bake_dense(hipStream_t st, const DenseArgs &a);

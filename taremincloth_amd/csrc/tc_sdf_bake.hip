@@ -530,6 +530,43 @@ __global__ void __launch_bounds__(256) k_bake_dense_finish(DenseArgs a) {
     a.out[i] = pack_dist((dot(diff, mk3(tr.nx, tr.ny, tr.nz)) >= 0.0f ? 1.0f : -1.0f) * sqrtf(dsq));
 }
 
+// The dense bake's nearest triangles as an fp32 brick atlas: one wavefront per 8^3 brick, each
+// lane eight voxels (lane + 64 k), so every store of a wavefront covers 64 consecutive floats.
+__global__ void __launch_bounds__(256) k_bake_atlas(DenseArgs a, float *__restrict__ atlas, uint32_t bx_n,
+                                                    uint32_t by_n, uint32_t nbricks) {
+    const uint32_t brick = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (brick >= nbricks) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t bx = brick % bx_n, by = (brick / bx_n) % by_n, bz = brick / (bx_n * by_n);
+    for (uint32_t k = 0; k < 8; ++k) {
+        const uint32_t v = lane + 64u * k;  // (lz * 8 + ly) * 8 + lx
+        const uint32_t lx = v & 7u, ly = (v >> 3) & 7u, lz = v >> 6;
+        const uint32_t vx = min(bx * ATLAS_BRICK + lx, a.w - 1), vy = min(by * ATLAS_BRICK + ly, a.h - 1),
+                       vz = min(bz * ATLAS_BRICK + lz, a.d - 1);
+        const size_t i = (size_t(vz) * a.h + vy) * a.w + vx;
+        const uint32_t idx = uint32_t(a.keys[i]);
+        float val = 10.0f;
+        if (idx < a.ntris) {
+            const f3 p = mk3(voxel_coord(vx, a.w, a.lmin[0], a.lmax[0]), voxel_coord(vy, a.h, a.lmin[1], a.lmax[1]),
+                             voxel_coord(vz, a.d, a.lmin[2], a.lmax[2]));
+            const BakeTri tr = a.otris[idx];
+            const f3 q = closest_on_tri(p, mk3(tr.p0x, tr.p0y, tr.p0z), mk3(tr.p1x, tr.p1y, tr.p1z),
+                                        mk3(tr.p2x, tr.p2y, tr.p2z));
+            const f3 diff = p - q;
+            val = (dot(diff, mk3(tr.nx, tr.ny, tr.nz)) >= 0.0f ? 1.0f : -1.0f) * sqrtf(dot(diff, diff));
+        }
+        atlas[size_t(brick) * 512u + v] = val;
+    }
+}
+
+hipError_t launch_bake_atlas(hipStream_t st, const DenseArgs &a, float *atlas) {
+    const uint32_t bx = (a.w + 7) / 8, by = (a.h + 7) / 8, bz = (a.d + 7) / 8;
+    const uint32_t nb = bx * by * bz;
+    if (!nb) return hipSuccess;
+    hipLaunchKernelGGL(k_bake_atlas, dim3((nb + 3) / 4), dim3(256), 0, st, a, atlas, bx, by, nb);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ coarse pass, segmented
 // bake_mesh_sdf_hier_coarse_main.wgsl keeps each cell's eight nearest triangles with its own
 // update rule (strict comparisons, so an entry pushed down the list passes the entries of equal
@@ -929,6 +966,8 @@ struct tc_sdf_result {
     float kernel_ms[2] = {0.0f, 0.0f};  // device time of the walk (dense or coarse) and of the fine pass
     uint32_t fallback_voxels = 0;       // voxels the fine pass left to the exhaustive fallback
     std::vector<uint32_t> texels, nearest;
+    std::vector<float> atlas;           // method 3: fp32 brick atlas
+    uint32_t brick_dims[3] = {0, 0, 0};
 };
 
 namespace {
@@ -1016,7 +1055,8 @@ int alloc_part(DevAlloc &mem, const tc::sdf::Domain &cdm, uint32_t ncl, unsigned
 }
 
 int launch_walk(const DevTris &t, const tc::sdf::Domain &dm, uint32_t topk, uint32_t *dout, uint32_t *dnear,
-                unsigned long long *keys, unsigned long long *part, uint32_t part_bricks, float &ms) {
+                unsigned long long *keys, unsigned long long *part, uint32_t part_bricks, float &ms,
+                float *atlas = nullptr) {
     tc::sdf::DenseArgs a;
     a.tris = t.sorted; a.otris = t.orig; a.index = t.index; a.clo = t.lo; a.chi = t.hi;
     a.ntris = t.n; a.nclusters = t.ncl;
@@ -1035,6 +1075,7 @@ int launch_walk(const DevTris &t, const tc::sdf::Domain &dm, uint32_t topk, uint
     TC_HIP(ev.init());
     TC_HIP(hipEventRecord(ev.a, nullptr));
     TC_HIP(tc::sdf::launch_bake_dense(nullptr, a));
+    if (atlas) TC_HIP(tc::sdf::launch_bake_atlas(nullptr, a, atlas));
     TC_HIP(hipEventRecord(ev.b, nullptr));
     TC_HIP(hipDeviceSynchronize());
     ms = ev.ms();
@@ -1053,22 +1094,32 @@ int download(tc_sdf_result &r, const tc::sdf::Domain &dm, const uint32_t *dout, 
     return TC_OK;
 }
 
-int bake_dense(const std::vector<BakeTri> &bt, const tc::sdf::Domain &dm, uint32_t topk, tc_sdf_result &r) {
+int bake_dense(const std::vector<BakeTri> &bt, const tc::sdf::Domain &dm, uint32_t topk, tc_sdf_result &r,
+               bool with_atlas = false) {
     const size_t nvox = size_t(dm.w) * dm.h * dm.d;
-    if (int rc = check_memory(dm, topk == 8 ? 44 : 12, bt.size())) return rc;
+    const uint32_t bx = (dm.w + 7) / 8, by = (dm.h + 7) / 8, bz = (dm.d + 7) / 8;
+    const size_t natlas = with_atlas ? size_t(bx) * by * bz * 512 : 0;
+    if (int rc = check_memory(dm, (topk == 8 ? 44 : 12) + (with_atlas ? 8 : 0), bt.size())) return rc;
     DevTris t;
     if (int rc = upload_tris(bt, dm, t)) return rc;
     DevAlloc mem;
     uint32_t *dout = nullptr, *dnear = nullptr;
     unsigned long long *keys = nullptr, *part = nullptr;
+    float *datlas = nullptr;
     uint32_t part_bricks = 0;
     TC_HIP(mem.get(&dout, nvox));
     TC_HIP(mem.get(&keys, nvox));
+    if (with_atlas) TC_HIP(mem.get(&datlas, natlas));
     if (topk == 8) {
         TC_HIP(mem.get(&dnear, nvox * 8));
         if (int rc = alloc_part(mem, dm, t.ncl, part, part_bricks)) return rc;
     }
-    if (int rc = launch_walk(t, dm, topk, dout, dnear, keys, part, part_bricks, r.kernel_ms[0])) return rc;
+    if (int rc = launch_walk(t, dm, topk, dout, dnear, keys, part, part_bricks, r.kernel_ms[0], datlas)) return rc;
+    if (with_atlas) {
+        r.atlas.resize(natlas);
+        r.brick_dims[0] = bx; r.brick_dims[1] = by; r.brick_dims[2] = bz;
+        TC_HIP(hipMemcpy(r.atlas.data(), datlas, sizeof(float) * natlas, hipMemcpyDeviceToHost));
+    }
     return download(r, dm, dout, dnear);
 }
 
@@ -1138,7 +1189,8 @@ int tc_bake_mesh_sdf(const float *verts, uint32_t nv, const int32_t *tris, uint3
                      int method, tc_sdf_result **out) {
     if (!out) return fail(TC_ERR_VALUE, "null output pointer");
     *out = nullptr;
-    if (method < 0 || method > 2) return fail(TC_ERR_VALUE, "method must be 0 (dense), 1 (hierarchical) or 2 (coarse)");
+    if (method < 0 || method > 3)
+        return fail(TC_ERR_VALUE, "method must be 0 (dense), 1 (hierarchical), 2 (coarse) or 3 (dense + fp32 atlas)");
     if (nv == 0 || !verts) return fail(TC_ERR_RUNTIME, "the mesh has no vertices");
     if (nt && !tris) return fail(TC_ERR_VALUE, "null triangle array");
     std::vector<float> v(verts, verts + 3 * size_t(nv));
@@ -1150,10 +1202,10 @@ int tc_bake_mesh_sdf(const float *verts, uint32_t nv, const int32_t *tris, uint3
     const std::vector<BakeTri> bt = tc::sdf::collect_bake_triangles(v, t);
     const float cv = std::fmax(voxel_size * 4.0f, 0.001f);  // coarse cell: 4 voxels
     int rc = TC_OK;
-    if (method == 0) {
+    if (method == 0 || method == 3) {
         const tc::sdf::Domain dm = tc::sdf::mesh_sdf_domain(v, voxel_size, margin, thickness);
         tc::sdf::mesh_sdf_bone_info(dm, friction, thickness, restitution, r->bone_info);
-        rc = bake_dense(bt, dm, 1, *r);
+        rc = bake_dense(bt, dm, 1, *r, method == 3);
     } else if (method == 2) {
         const tc::sdf::Domain dm = tc::sdf::mesh_sdf_domain(v, cv, margin, thickness);
         tc::sdf::mesh_sdf_bone_info(dm, friction, thickness, restitution, r->bone_info);
@@ -1167,6 +1219,12 @@ int tc_bake_mesh_sdf(const float *verts, uint32_t nv, const int32_t *tris, uint3
     if (rc != TC_OK) return rc;
     *out = r.release();
     return TC_OK;
+}
+
+const float *tc_sdf_result_atlas(const tc_sdf_result *r, uint32_t *brick_dims) {
+    if (!r || r->atlas.empty()) return nullptr;
+    if (brick_dims) { brick_dims[0] = r->brick_dims[0]; brick_dims[1] = r->brick_dims[1]; brick_dims[2] = r->brick_dims[2]; }
+    return r->atlas.data();
 }
 
 int tc_sdf_result_info(const tc_sdf_result *r, uint32_t *dims, float *bone_info, float *coarse_voxel) {

@@ -755,6 +755,14 @@ def _bake(mesh_verts, mesh_tris, voxel_size, margin, friction, thickness, restit
             out["nearest_tri"] = near.copy()
             out["dist_packed"] = tex
             out["coarse_voxel"] = float(cv.value)
+        if method == 3:
+            bd = (C.c_uint32 * 3)()
+            ptr = L.tc_sdf_result_atlas(res, bd)
+            if not ptr:
+                raise RuntimeError("the bake returned no fp32 atlas")
+            nb = bd[0] * bd[1] * bd[2]
+            out["atlas"] = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_float)), (nb * 512,)).copy()
+            out["brick_dims"] = (int(bd[0]), int(bd[1]), int(bd[2]))
         return out
     finally:
         L.tc_sdf_result_free(res)
@@ -776,6 +784,23 @@ def bake_mesh_sdf_coarse(mesh_verts, mesh_tris, voxel_size=0.004, margin=0.02, f
                          restitution=0.0, close_open_boundaries=True) -> dict:
     """The coarse pass alone (sdf_baker.rs:1328-1600): adds nearest_tri (8 per cell) and coarse_voxel."""
     return _bake(mesh_verts, mesh_tris, voxel_size, margin, friction, thickness, restitution, close_open_boundaries, 2)
+
+
+def bake_mesh_sdf_atlas_fp32(mesh_verts, mesh_tris, voxel_size=0.004, margin=0.02, friction=0.5, thickness=0.005,
+                             restitution=0.0, close_open_boundaries=True) -> dict:
+    """Dense mesh SDF bake that also returns the signed distances as an fp32 brick atlas ("atlas":
+    bx*by*bz bricks of 8^3 floats, "brick_dims": (bx, by, bz)); brick b = (bz*by_n + by)*bx_n + bx holds
+    voxel (8bx + lx, 8by + ly, 8bz + lz) at offset (lz*8 + ly)*8 + lx, clamped to the domain's edge.
+    The texels are those of bake_mesh_sdf_gpu; the reference keeps only those (bake_mesh_sdf.wgsl:120)."""
+    return _bake(mesh_verts, mesh_tris, voxel_size, margin, friction, thickness, restitution, close_open_boundaries, 3)
+
+
+def atlas_to_dense(atlas, brick_dims, dims):
+    """The fp32 brick atlas as a dense (depth, height, width) array."""
+    bx, by, bz = brick_dims
+    w, h, d = dims
+    a = np.asarray(atlas, np.float32).reshape(bz, by, bx, 8, 8, 8)
+    return a.transpose(0, 3, 1, 4, 2, 5).reshape(bz * 8, by * 8, bx * 8)[:d, :h, :w]
 
 
 def close_open_boundary_loops(mesh_verts, mesh_tris):

@@ -6,7 +6,8 @@
   fp32 oracle, and the hash tables bit for bit at the device table size.
 - 128 x 128 cloth (1 cm) on the icosphere-6 collider (81,920 triangles, 5% noise), friction 0.4:
   substeps in contact against the oracle's scan of every triangle.
-- The 128^3 dense and hierarchical bakes of that icosphere: the dense bake on 10,000 sampled voxels
+- The 128^3 dense (with its fp32 brick atlas, 16^3 bricks) and hierarchical bakes of that
+  icosphere: the dense bake and the atlas (within 1e-3 voxel) on 10,000 sampled voxels
   against an independent FP64 brute force to half a binary16 ulp; the hierarchical bake bit for bit
   against the oracle's two passes, with its signs and error bound against the same brute force.
 - 512 x 512 learned warm start: the network run inside the substep equals the standalone pass on the
@@ -191,13 +192,15 @@ def test_cfg3_bakes_128_against_fp64_brute_force():
     size = np.maximum(hi - lo, 0.05)
     padded = size + 2 * (np.maximum(size * margin, 0.04) + thickness + 0.02)
     voxel = float(padded.max()) / 128.0 * (1.0 + 1e-6)
-    dense = T.bake_mesh_sdf_gpu(v, fi, voxel_size=voxel, margin=margin, thickness=thickness)
+    dense = T.bake_mesh_sdf_atlas_fp32(v, fi, voxel_size=voxel, margin=margin, thickness=thickness)
     hier = T.bake_mesh_sdf_hierarchical_gpu(v, fi, voxel_size=voxel, margin=margin, thickness=thickness)
     dims = np.array([dense["width"], dense["height"], dense["depth"]])
     _say(f"cfg3 bakes: dims {dims.tolist()}, voxel {voxel * 1e3:.4f} mm, {len(fc)} triangles")
     assert dims.max() == 128
     sys.path.insert(0, cases.ROOT + "/tests")
-    from test_sdf import domain, f16_decode, f16_halfulp
+    from test_sdf import check_atlas, domain, f16_decode, f16_halfulp
+    assert dense["brick_dims"] == (16, 16, 16)  # 4,096 bricks of 8^3
+    atlas = check_atlas(dense, dims).ravel()
     dlo, dhi, _, ddims = domain(v, voxel, margin, thickness)
     assert np.array_equal(np.asarray(ddims), dims)
     rng = np.random.default_rng(0)
@@ -214,6 +217,11 @@ def test_cfg3_bakes_128_against_fp64_brute_force():
     assert np.all(err <= f16_halfulp(dist) + 1e-6)
     gs = np.where(got >= 0, 1, -1)
     assert np.array_equal(gs[~amb], sign[~amb].astype(int))
+    vsz = float(np.min((np.asarray(dhi, np.float64) - np.asarray(dlo, np.float64)) / dims))
+    aerr = np.abs(np.abs(atlas[idx].astype(np.float64)) - dist)
+    _say(f"  fp32 atlas: max | |d| - fp64 | = {aerr.max():.3e} m = {aerr.max() / vsz:.2e} voxel")
+    assert aerr.max() <= 1e-3 * vsz
+    assert np.array_equal(np.where(atlas[idx] >= 0, 1, -1)[~amb], sign[~amb].astype(int))
     # hierarchical: bit-equal to the oracle's restatement of the reference's two passes at full size.
     # Its near field is not the dense bake's on this mesh: the fine pass sees only the 8 nearest
     # triangles of each coarse cell within +-2 cells, which on 81,920 small triangles can miss the

@@ -15,7 +15,8 @@ CPU tests pin the oracle (oracle/sdf_oracle.c):
   * the reference's falling-vertex-on-cube SDF collider scene (tests/collider/
     test_mesh_sdf_baker.py:172-235) on the oracle simulator.
 GPU tests: every bake of the HIP path bit-equal to the oracle's (texels and nearest lists), the
-capping host code equal to the oracle's, a >256 MiB bake against the closed form, and the SDF
+fp32 brick atlas within 1e-3 voxel of an FP64 brute force at every voxel (its values rounding to
+the dense texels, its padding clamped to the edge), the capping host code equal to the oracle's, a >256 MiB bake against the closed form, and the SDF
 collider in the simulator against the oracle (1e-5 m) and the reference scene's bounds.
 """
 import sys
@@ -380,6 +381,71 @@ def test_gpu_bake_bit_exact(require_gpu, method, mesh, voxel, margin, thickness,
     if method == 2:
         np.testing.assert_array_equal(got["nearest_tri"], want["nearest_tri"])
         assert abs(got["coarse_voxel"] - want["coarse_voxel"]) == 0.0
+
+
+def check_atlas(r, voxel_dims):
+    """The fp32 brick atlas of a method-3 bake: brick counts, clamp-to-edge padding, and every voxel's
+    value rounding (binary16, nearest even) to the dense texel beside it.  Returns the atlas as a
+    dense (d, h, w) array."""
+    from taremincloth_amd.taremin_cloth_core import atlas_to_dense
+    w, h, d = r["width"], r["height"], r["depth"]
+    bd = r["brick_dims"]
+    assert bd == ((w + 7) // 8, (h + 7) // 8, (d + 7) // 8)
+    a = np.asarray(r["atlas"], np.float32)
+    assert a.size == bd[0] * bd[1] * bd[2] * 512 and np.isfinite(a).all()
+    full = a.reshape(bd[2], bd[1], bd[0], 8, 8, 8).transpose(0, 3, 1, 4, 2, 5).reshape(bd[2] * 8, bd[1] * 8,
+                                                                                     bd[0] * 8)
+    dense = atlas_to_dense(a, bd, (w, h, d))
+    zi = np.minimum(np.arange(bd[2] * 8), d - 1)
+    yi = np.minimum(np.arange(bd[1] * 8), h - 1)
+    xi = np.minimum(np.arange(bd[0] * 8), w - 1)
+    np.testing.assert_array_equal(full, dense[zi][:, yi][:, :, xi])
+    tex = np.frombuffer(r["texture_bytes"], np.uint32)
+    assert np.all((tex >> 16) == 0x3C00)
+    packed = dense.ravel().astype(np.float16).view(np.uint16).astype(np.uint32)
+    mism = np.flatnonzero(packed != (tex & 0xFFFF))
+    assert mism.size == 0, f"{mism.size} atlas values do not round to their texel, first at {mism[:5]}"
+    return dense
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mesh,voxel,close", [("lumpy", 0.02, True), ("double_wall", 0.004, True),
+                                              ("cylinder", 0.004, False), ("cube", 0.013, True)])
+def test_gpu_atlas_fp32_against_fp64(require_gpu, mesh, voxel, close):
+    """The fp32 atlas within 1e-3 voxel of the FP64 distance to the nearest triangle at every voxel,
+    with the FP64 sign wherever the nearest triangles agree on it; its texels equal the dense bake's."""
+    T = _T()
+    v, t = _mesh(mesh)
+    _say(f"atlas mesh={mesh} voxel={voxel} close={close}")
+    r = T.bake_mesh_sdf_atlas_fp32(v, t, voxel_size=voxel, margin=0.02, thickness=0.005,
+                                   close_open_boundaries=close)
+    d0 = T.bake_mesh_sdf_gpu(v, t, voxel_size=voxel, margin=0.02, thickness=0.005, close_open_boundaries=close)
+    assert r["texture_bytes"] == d0["texture_bytes"]
+    lo, hi, size, dims = domain(v, voxel, 0.02, 0.005)
+    assert (r["width"], r["height"], r["depth"]) == tuple(dims)
+    dense = check_atlas(r, dims).ravel().astype(np.float64)
+    vsz = float(np.min((hi.astype(np.float64) - lo.astype(np.float64)) / dims))
+    vv, tt = v, t  # closed meshes, or the open cylinder baked uncapped: the bake sees exactly (v, t)
+    p = voxel_centres(lo.astype(np.float64), hi.astype(np.float64), dims)
+    err = np.empty(len(p))
+    amb = np.zeros(len(p), bool)
+    ok_sign = np.ones(len(p), bool)
+    for c0 in range(0, len(p), 4096):
+        sl = slice(c0, c0 + 4096)
+        d, q, nn = brute_force(p[sl], vv, tt)
+        m = np.arange(d.shape[0])
+        best = np.argmin(d, axis=1)
+        dmin = d[m, best]
+        err[sl] = np.abs(np.abs(dense[sl]) - dmin)
+        sgn = np.sign(np.einsum("mtk,mtk->mt", p[sl][:, None, :] - q, nn))
+        sgn[sgn == 0] = 1
+        near = d <= dmin[:, None] + 1e-5
+        amb[sl] = ~np.all(np.where(near, sgn == sgn[m, best][:, None], True), axis=1) | (dmin <= 1e-6)
+        ok_sign[sl] = np.where(dense[sl] >= 0, 1, -1) == sgn[m, best]
+    _say(f"  {len(p)} voxels, max | |d| - fp64 | = {err.max():.3e} m = {err.max() / vsz:.2e} voxel; "
+         f"{amb.sum()} sign-ambiguous")
+    assert err.max() <= 1e-3 * vsz
+    assert ok_sign[~amb].all()
 
 
 @pytest.mark.gpu
