@@ -75,6 +75,13 @@ extern "C" {
     pub fn tc_sim_get_positions(sim: *mut TcSim, out: *mut f32, n_floats: usize) -> i32;
     pub fn tc_sim_set_positions_velocities(sim: *mut TcSim, positions: *const f32, velocities: *const f32, n: u32) -> i32;
     pub fn tc_sim_reset(sim: *mut TcSim) -> i32;
+    pub fn tc_sim_set_pins(sim: *mut TcSim, indices: *const u32, targets: *const f32, weights: *const f32, n: u32) -> i32;
+    pub fn tc_sim_release_pins(sim: *mut TcSim, indices: *const u32, n: u32) -> i32;
+    pub fn tc_sim_clear_pins(sim: *mut TcSim) -> i32;
+    pub fn tc_sim_pin_count(sim: *const TcSim, n: *mut u32) -> i32;
+    pub fn tc_sim_get_pins(sim: *const TcSim, indices: *mut u32, targets: *mut f32, weights: *mut f32, cap: u32, count: *mut u32) -> i32;
+    pub fn tc_sim_set_vertex_inv_masses(sim: *mut TcSim, indices: *const u32, inv_masses: *const f32, n: u32) -> i32;
+    pub fn tc_sim_get_vertex_inv_masses(sim: *const TcSim, out: *mut f32, cap: usize) -> i32;
     pub fn tc_sim_add_sphere(sim: *mut TcSim, center: *const f32, radius: f32, friction: f32, restitution: f32) -> i32;
     pub fn tc_sim_add_capsule(sim: *mut TcSim, a: *const f32, b: *const f32, radius: f32, friction: f32, restitution: f32) -> i32;
     pub fn tc_sim_add_plane(sim: *mut TcSim, point: *const f32, normal: *const f32, friction: f32, restitution: f32) -> i32;

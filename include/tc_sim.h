@@ -139,6 +139,27 @@ int tc_sim_set_positions_velocities(tc_sim *sim, const float *positions, const f
 /* ClothSimulator.reset(), dispatch.rs:1411-1427 */
 int tc_sim_reset(tc_sim *sim);
 
+/* Dynamic pins, lib.rs:2616-2675 -> simulation/mod.rs:2768-2905.  set_pins: set_pin (n = 1) and
+ * set_pins_batch; targets hold 3 floats per pin; an index >= num_vertices is ignored; a vertex's
+ * inverse mass becomes its base inverse mass * (1 - clamp(weight, 0, 1)) (a non-finite weight
+ * counts as 0).  Each iteration, and once after the collider pass, a pin pulls its vertex towards
+ * the target with compliance 1e-3 (1 - w) / w, or snaps it there when its inverse mass is 0
+ * (pin.wgsl:55-92).  release_pins: release_pin (n = 1) and release_pins_batch, restoring the base
+ * inverse masses.  clear_pins: clear_pins. */
+int tc_sim_set_pins(tc_sim *sim, const uint32_t *indices, const float *targets, const float *weights, uint32_t n);
+int tc_sim_release_pins(tc_sim *sim, const uint32_t *indices, uint32_t n);
+int tc_sim_clear_pins(tc_sim *sim);
+int tc_sim_pin_count(const tc_sim *sim, uint32_t *n);
+/* The pin table in ascending vertex order: up to cap entries, *count = number of pins (each output
+ * may be NULL). */
+int tc_sim_get_pins(const tc_sim *sim, uint32_t *indices, float *targets, float *weights, uint32_t cap,
+                    uint32_t *count);
+/* set_vertex_inv_masses(indices, inv_masses) / get_vertex_inv_masses(), lib.rs:2660-2676: sets the
+ * base inverse masses (the device value keeps a pin's scaling); get returns the base values
+ * (cap >= num_vertices). */
+int tc_sim_set_vertex_inv_masses(tc_sim *sim, const uint32_t *indices, const float *inv_masses, uint32_t n);
+int tc_sim_get_vertex_inv_masses(const tc_sim *sim, float *out, size_t cap);
+
 /* Colliders: add_sphere_collider / add_capsule_collider / add_plane_collider /
  * clear_colliders, lib.rs:2157-2172 -> simulation/mod.rs:1571-1629 */
 int tc_sim_add_sphere(tc_sim *sim, const float center[3], float radius, float friction, float restitution);
@@ -215,6 +236,7 @@ uint32_t tc_sim_num_bending_constraints(const tc_sim *sim);
  * 14 velocities (f32 xyz)          15 distance lambdas (f32)                16 bending lambdas (f32)
  * 17 scalars [cell_size(f32 bits), table_size, n_virtual, num_edges]       18 Morton-sorted edges (u32 pairs)
  * 19 edge hash keys                 20 edge hash cell starts                 21 edge hash sorted indices
+ * 22 inverse masses the solver reads (f32; the base values scaled by the pins)
  * *count receives the element count; out may be NULL to query it. */
 int tc_sim_get_array(tc_sim *sim, int which, void *out, size_t capacity, size_t *count);
 /* Rebuild the spatial hashes from the current predicted positions (parity checks of the hash). */

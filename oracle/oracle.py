@@ -109,6 +109,9 @@ def lib(precision: str = "f32") -> C.CDLL:
                                        C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.orc_sim_set_sdf.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _u32p, _f32p, C.c_uint32]
     L.orc_sim_set_bone_transforms.argtypes = [_vp, _f32p, _f32p, C.c_uint32]
+    L.orc_sim_set_pin_table.argtypes = [_vp, _vp, _vp, _vp, C.c_uint32]
+    L.orc_sim_set_inv_masses.argtypes = [_vp, _vp, _vp, C.c_uint32]
+    L.orc_sim_get_inv_masses.argtypes = [_vp, C.c_int, _f32p]
     _LIBS[precision] = L
     return L
 
@@ -200,6 +203,8 @@ class OracleSim:
                                          0 if f is None else len(f.reshape(-1, 3)), _ptr(im), _ptr(th), _ptr(ly),
                                          layer_id, thickness, stiffness, bending_stiffness, comp, shear)
         self.n = len(pos)
+        self._pins = {}  # vertex -> (target float32[3], weight float32)
+        self._inv_orig = self.inv_masses(1)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -268,7 +273,95 @@ class OracleSim:
         self._L.orc_sim_set_bone_transforms(self._h, w, iw, len(w))
 
     def reset(self):
+        """dispatch.rs:1411-1427: initial state and inverse masses, no pins; the base inverse masses
+        of set_vertex_inv_masses stay."""
         self._L.orc_sim_reset(self._h)
+        self._pins.clear()
+        self._upload_pins()
+
+    # --- dynamic pins: the host policy of simulation/mod.rs:2768-2905 -----------------------------
+    @staticmethod
+    def _clamped(w):
+        w = np.float32(w)
+        return np.float32(min(max(w, np.float32(0)), np.float32(1))) if np.isfinite(w) else np.float32(0)
+
+    def inv_masses(self, which=0) -> np.ndarray:
+        """which 0: the working inverse masses (what the solver reads), 1: the mesh's at creation."""
+        out = np.empty(self.n, np.float32)
+        self._L.orc_sim_get_inv_masses(self._h, which, out)
+        return out
+
+    def _upload_pins(self):
+        keys = sorted(self._pins)
+        v = np.array(keys, np.uint32)
+        t = np.array([self._pins[k][0] for k in keys], np.float32).reshape(-1, 3)
+        w = np.array([self._pins[k][1] for k in keys], np.float32)
+        self._L.orc_sim_set_pin_table(self._h, _ptr(v), _ptr(t), _ptr(w), len(keys))
+
+    def _write_inv(self, idx, vals):
+        i = np.asarray(idx, np.uint32)
+        v = np.asarray(vals, np.float32)
+        self._L.orc_sim_set_inv_masses(self._h, _ptr(i), _ptr(v), len(i))
+
+    def _effective(self, i):
+        if i in self._pins:
+            return np.float32(self._inv_orig[i] * (np.float32(1) - self._clamped(self._pins[i][1])))
+        return self._inv_orig[i]
+
+    def set_pin(self, vertex_idx, target_pos, weight=1.0):
+        self.set_pins_batch([vertex_idx], [target_pos], [weight])
+
+    def set_pins_batch(self, indices, targets, weights):
+        n = min(len(indices), len(targets), len(weights))
+        idx, vals = [], []
+        for k in range(n):
+            i = int(indices[k])
+            if i >= self.n:
+                continue
+            self._pins[i] = (np.asarray(targets[k], np.float32).reshape(3), np.float32(weights[k]))
+            idx.append(i)
+            vals.append(np.float32(self._inv_orig[i] * (np.float32(1) - self._clamped(weights[k]))))
+        self._upload_pins()
+        self._write_inv(idx, vals)
+
+    def release_pin(self, vertex_idx):
+        self.release_pins_batch([vertex_idx])
+
+    def release_pins_batch(self, indices):
+        idx = [int(i) for i in indices if self._pins.pop(int(i), None) is not None]
+        if idx:
+            self._upload_pins()
+            self._write_inv(idx, [self._inv_orig[i] for i in idx])
+
+    def clear_pins(self):
+        idx = list(self._pins)
+        self._pins.clear()
+        self._upload_pins()
+        self._write_inv(idx, [self._inv_orig[i] for i in idx])
+
+    def set_vertex_inv_masses(self, indices, inv_masses):
+        idx, vals = [], []
+        for i, m in zip(indices, inv_masses):
+            i = int(i)
+            if i >= self.n:
+                continue
+            self._inv_orig[i] = np.float32(m)
+            idx.append(i)
+            vals.append(self._effective(i))
+        self._write_inv(idx, vals)
+
+    def get_vertex_inv_masses(self) -> np.ndarray:
+        return self._inv_orig.copy()
+
+    def set_positions_and_velocities(self, positions, velocities=None):
+        """dispatch.rs:952-986: x = p = positions, v = velocities (or zero), inverse masses from the
+        base values and the pins."""
+        pos = np.asarray(positions, np.float64).reshape(-1, 3)
+        if len(pos) != self.n:
+            return
+        vel = np.zeros_like(pos) if velocities is None else np.asarray(velocities, np.float64).reshape(-1, 3)
+        self.set_state(x=pos, p=pos, v=vel)
+        self._write_inv(range(self.n), [self._effective(i) for i in range(self.n)])
 
     def step(self, dt=1.0 / 60.0, substeps=None, solver_iterations=None):
         if solver_iterations is not None:

@@ -1043,6 +1043,7 @@ typedef struct {
     uint64_t stats[6]; /* contact counters: vv, vt_ccd, vt_prox, ee_ccd, ee_prox, hash_pairs */
     uint32_t *sdf_tex; float *sdf_info, *sdf_xf;     /* SDF colliders (sdf_oracle.c) */
     uint32_t sdf_w, sdf_h, sdf_d, sdf_nb;
+    uint32_t npins, *pin_v; v3 *pin_t; real *pin_w;  /* dynamic pin table (distinct vertices) */
 } orc_sim;
 
 orc_sim *orc_sim_create(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
@@ -1095,6 +1096,7 @@ void orc_sim_free(orc_sim *s) {
     free(s->d_tc); free(s->d_cc); free(s->b_comp); free(s->cols); free(s->acc); free(s->tris); free(s->tri_bounds);
     free(s->hkeys); free(s->hstarts); free(s->hsorted); free(s->ekeys); free(s->estarts); free(s->esorted);
     free(s->sdf_tex); free(s->sdf_info); free(s->sdf_xf);
+    free(s->pin_v); free(s->pin_t); free(s->pin_w);
     free(s);
 }
 
@@ -1239,6 +1241,49 @@ void orc_sim_get_lambdas(const orc_sim *s, double *ld, double *lb) {
 void orc_sim_set_lambdas(orc_sim *s, const double *ld, const double *lb) {
     for (uint32_t k = 0; ld && k < s->m->nd; k++) s->lam_d[k] = (real)ld[k];
     for (uint32_t k = 0; lb && k < s->m->nb; k++) s->lam_b[k] = (real)lb[k];
+}
+
+/* The pin table (replaced whole; the host policy of simulation/mod.rs:2768-2875 that builds it,
+ * and the inverse masses it implies, is restated in oracle.py) and the inverse masses. */
+void orc_sim_set_pin_table(orc_sim *s, const uint32_t *v, const float *t, const float *w, uint32_t n) {
+    free(s->pin_v); free(s->pin_t); free(s->pin_w);
+    s->npins = n;
+    s->pin_v = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    s->pin_t = (v3 *)malloc((n ? n : 1) * sizeof(v3));
+    s->pin_w = (real *)malloc((n ? n : 1) * sizeof(real));
+    for (uint32_t k = 0; k < n; k++) {
+        s->pin_v[k] = v[k];
+        s->pin_t[k] = mk(t[3 * k], t[3 * k + 1], t[3 * k + 2]);
+        s->pin_w[k] = w[k];
+    }
+}
+void orc_sim_set_inv_masses(orc_sim *s, const uint32_t *idx, const float *val, uint32_t n) {
+    for (uint32_t k = 0; k < n; k++)
+        if (idx[k] < s->nv) s->w[idx[k]] = val[k];
+}
+/* which: 0 = the working inverse masses, 1 = those of the mesh at creation */
+void orc_sim_get_inv_masses(const orc_sim *s, int which, float *out) {
+    for (uint32_t i = 0; i < s->nv; i++) out[i] = (float)(which ? s->w0[i] : s->w[i]);
+}
+
+/* pin.wgsl:55-92, every pin of the table */
+static void k_pins(orc_sim *s, real dt_sub) {
+    for (uint32_t k = 0; k < s->npins; k++) {
+        real w = rclamp(s->pin_w[k], (real)0, (real)1); /* NaN -> 0 */
+        if (w <= 0) continue;
+        uint32_t v = s->pin_v[k];
+        real inv_m = s->w[v];
+        if (inv_m <= 0) { s->p[v] = s->pin_t[k]; continue; }
+        v3 d = sub(s->p[v], s->pin_t[k]);
+        real dist = len3(d);
+        if (!(dist > (real)1e-7f)) continue;
+        v3 n = dvs(d, dist);
+        real dt = rmax(dt_sub, (real)1e-8f);
+        real compliance = (real)1e-3f * ((real)1 - w) / rmax(w, (real)1e-6f);
+        real alpha = compliance / (dt * dt);
+        real corr = -dist / (inv_m + alpha);
+        s->p[v] = add(s->p[v], mul(n, inv_m * corr));
+    }
 }
 
 /* predict.wgsl:33-55 */
@@ -2088,7 +2133,8 @@ static void self_collision_pass(orc_sim *s) {
 }
 
 /* one frame: dispatch.rs:7-354 with adaptive substeps off, decoupled colliders,
- * self-collision mode 0 / algorithm 0, no pins, no sewing */
+ * self-collision mode 0 / algorithm 0, no sewing; the pins run at the start of every
+ * iteration and once after the collider (dispatch.rs:176-181, 238-250) */
 static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
     if (s->nv == 0) return;
     real dt_sub = dt / (real)substeps;
@@ -2098,6 +2144,7 @@ static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
         memset(s->lam_b, 0, (m->nb ? m->nb : 1) * sizeof(real));
         k_predict(s, dt_sub);
         for (uint32_t it = 0; it < s->iterations; it++) {
+            k_pins(s, dt_sub);
             for (uint32_t c = 0; c < m->nd_col; c++) {
                 ORC_PARALLEL_FOR
                 for (uint32_t k = m->d_col_off[c]; k < m->d_col_off[c] + m->d_col_cnt[c]; k++) k_distance_one(s, k, dt_sub);
@@ -2108,6 +2155,7 @@ static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
             }
         }
         if (s->ncols > 0 || s->ntris > 0 || s->sdf_nb > 0) k_collision(s, dt_sub);
+        k_pins(s, dt_sub);
         if (s->self_collision) self_collision_pass(s);
         k_update_vel(s, dt_sub);
     }
@@ -2130,6 +2178,7 @@ void orc_sim_substep_parts(orc_sim *s, float dt_sub, int do_collide, int do_self
     memset(s->lam_b, 0, (m->nb ? m->nb : 1) * sizeof(real));
     k_predict(s, dt);
     for (uint32_t it = 0; it < s->iterations; it++) {
+        k_pins(s, dt);
         for (uint32_t c = 0; c < m->nd_col; c++) {
             ORC_PARALLEL_FOR
             for (uint32_t k = m->d_col_off[c]; k < m->d_col_off[c] + m->d_col_cnt[c]; k++) k_distance_one(s, k, dt);
@@ -2140,6 +2189,7 @@ void orc_sim_substep_parts(orc_sim *s, float dt_sub, int do_collide, int do_self
         }
     }
     if (do_collide && (s->ncols > 0 || s->ntris > 0 || s->sdf_nb > 0)) k_collision(s, dt);
+    k_pins(s, dt);
     if (do_self && s->self_collision) self_collision_pass(s);
     if (do_update) k_update_vel(s, dt);
 }

@@ -19,7 +19,9 @@ EXPORTS = [
     "tc_sim_set_adaptive_substep_options", "tc_sim_set_auto_compensate_iterations", "tc_sim_set_strain_adaptive_options",
     "tc_sim_set_enable_compact_readback", "tc_sim_note_external_displacement", "tc_sim_frame_policy_state",
     "tc_sim_step_async", "tc_sim_fetch_positions", "tc_sim_step_buffered", "tc_sim_fetch_buffered_positions",
-    "tc_sim_get_buffered_frame_count", "tc_sim_clear_frame_buffer", "tc_sim_reset", "tc_sim_add_sphere",
+    "tc_sim_get_buffered_frame_count", "tc_sim_clear_frame_buffer", "tc_sim_reset", "tc_sim_set_pins", "tc_sim_release_pins",
+    "tc_sim_clear_pins", "tc_sim_pin_count", "tc_sim_get_pins", "tc_sim_set_vertex_inv_masses",
+    "tc_sim_get_vertex_inv_masses", "tc_sim_add_sphere",
     "tc_sim_add_capsule", "tc_sim_add_plane", "tc_sim_clear_colliders", "tc_sim_set_mesh_triangles",
     "tc_sim_set_gravity", "tc_sim_set_damping", "tc_sim_set_damping_all", "tc_sim_set_solver_iterations",
     "tc_sim_set_enable_self_collision", "tc_sim_set_self_collision_options", "tc_sim_set_stiffness_all",
@@ -98,6 +100,13 @@ def load() -> C.CDLL:
         "tc_sim_clear_frame_buffer": (_i, [_vp]),
         "tc_sim_set_positions_velocities": (_i, [_vp, _vp, _vp, _u32]),
         "tc_sim_reset": (_i, [_vp]),
+        "tc_sim_set_pins": (_i, [_vp, _vp, _vp, _vp, _u32]),
+        "tc_sim_release_pins": (_i, [_vp, _vp, _u32]),
+        "tc_sim_clear_pins": (_i, [_vp]),
+        "tc_sim_pin_count": (_i, [_vp, _vp]),
+        "tc_sim_get_pins": (_i, [_vp, _vp, _vp, _vp, _u32, _vp]),
+        "tc_sim_set_vertex_inv_masses": (_i, [_vp, _vp, _vp, _u32]),
+        "tc_sim_get_vertex_inv_masses": (_i, [_vp, _vp, C.c_size_t]),
         "tc_sim_add_sphere": (_i, [_vp, _vp, _f32, _f32, _f32]),
         "tc_sim_add_capsule": (_i, [_vp, _vp, _vp, _f32, _f32, _f32]),
         "tc_sim_add_plane": (_i, [_vp, _vp, _vp, _f32, _f32]),

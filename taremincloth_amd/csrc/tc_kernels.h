@@ -47,6 +47,9 @@ hipError_t launch_collision(hipStream_t s, float4 *P, float4 *X, uint32_t n, con
 hipError_t launch_update_vel(hipStream_t s, const float4 *P, float4 *X, float4 *V, uint32_t n, float dt,
                              uint32_t block);
 hipError_t launch_extract_positions(hipStream_t s, const float4 *X, float *out, uint32_t n);
+// Compliant pins (pin.wgsl:55-92) on P: pin_t = (target xyz, weight), one distinct vertex per pin.
+hipError_t launch_pins(hipStream_t s, float4 *P, const uint32_t *pin_v, const float4 *pin_t, uint32_t npins,
+                       const SubstepArgs &a);
 hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t value, size_t n);
 // Residual of the XPBD system after the last sweep (double sums, deterministic order):
 // out[0] = sum h^2, out[1] = sum L0^2, out[2] = max |h|/L0, out[3] = sqrt(out[0] / out[1]).

@@ -61,6 +61,7 @@ hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t coun
         const tc::FusedPlanDev pd = s->plan_dev();
         for (uint32_t it = 0; it < count; ++it) {
             const int q = s->prof_begin("solver_iter");
+            if (s->npins && (e = enqueue_pins(s, s->Pbuf(pc), a)) != hipSuccess) return e;
             e = tc::launch_tile_iteration(s->stream, s->Pbuf(pc), s->Pbuf(pc ^ 1u), s->X.p, pd, s->dlam_buf(lc),
                                           s->dlam_buf(lc ^ 1u), s->blam_buf(lc), s->blam_buf(lc ^ 1u), fa, a,
                                           first && it == 0, s->fused_block);
@@ -74,6 +75,7 @@ hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t coun
     for (uint32_t it = 0; it < count; ++it) {
         const bool f = first && it == 0;
         const int q = s->prof_begin("solver_iter");
+        if (s->npins && (e = enqueue_pins(s, s->P.p, a)) != hipSuccess) return e;  // dispatch.rs:176-181
         for (size_t c = 0; c < m.d_col.count.size(); ++c) {
             e = tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p, m.d_col.offset[c],
                                     m.d_col.count[c], a, f, s->split_compliance, B);
@@ -201,6 +203,12 @@ hipError_t enqueue_substep_tail(tc_sim *s, float dt_sub, uint32_t pc, uint32_t l
         q = s->prof_begin("collider_decoupled");
         e = tc::launch_collision(s->stream, Pc, s->X.p, n, s->col_order.p, s->cols.p, uint32_t(s->cols_h.size()), s->tris.p,
                                  s->tri_bounds.p, uint32_t(s->tris_h.size()), s->tri_cbox.p, s->single_sided_recovery, dt_sub, B, s->sdf_args());
+        s->prof_end(q);
+        if (e != hipSuccess) return e;
+    }
+    if (s->npins) {  // the final pin after the iterations and the decoupled collider (dispatch.rs:238-250)
+        q = s->prof_begin("final_pin");
+        e = enqueue_pins(s, Pc, s->args(dt_sub));
         s->prof_end(q);
         if (e != hipSuccess) return e;
     }
@@ -593,6 +601,8 @@ static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
         s->X0h[i] = make_float4(m.pos0[3 * i], m.pos0[3 * i + 1], m.pos0[3 * i + 2], m.thick[i]);
         s->P0h[i] = make_float4(m.pos0[3 * i], m.pos0[3 * i + 1], m.pos0[3 * i + 2], m.inv_mass[i]);
     }
+    s->inv_orig.assign(m.inv_mass.begin(), m.inv_mass.end());
+    s->inv_eff = s->inv_orig;
     std::vector<float4> zeros(nv, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     build_records(s);
     std::vector<uint2> star(m.star_v0.size()), edg(m.e_v0.size());
@@ -1008,13 +1018,14 @@ int tc_sim_set_positions_velocities(tc_sim *s, const float *pos, const float *ve
     std::vector<float4> X(n), P(n), V(n);
     for (uint32_t i = 0; i < n; ++i) {
         X[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], s->X0h[i].w);
-        P[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], s->P0h[i].w);
+        P[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], effective_inv_mass(s, i));
         V[i] = vel ? make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     TC_HIP(hipStreamSynchronize(s->stream));
     TC_HIP(hipMemcpy(s->X.p, X.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     TC_HIP(hipMemcpy(s->P.p, P.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     TC_HIP(hipMemcpy(s->V.p, V.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    for (uint32_t i = 0; i < n; ++i) s->inv_eff[i] = P[i].w;
     s->last_p = 0;
     return TC_OK;
 }
@@ -1028,6 +1039,11 @@ int tc_sim_reset(tc_sim *s) {
     TC_HIP(hipMemcpy(s->X.p, s->X0h.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     TC_HIP(hipMemcpy(s->P.p, s->P0h.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
     TC_HIP(hipMemcpy(s->V.p, V.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+    // dispatch.rs:1411-1422: the pins go, the initial inverse masses come back (the base inverse
+    // masses of set_vertex_inv_masses are kept for later calls)
+    s->pins.clear();
+    if (int rc = upload_pins(s)) return rc;
+    for (uint32_t i = 0; i < n; ++i) s->inv_eff[i] = s->P0h[i].w;
     s->last_p = 0;
     return ws_clear_history(s);
 }
@@ -1397,6 +1413,17 @@ int tc_sim_get_array(tc_sim *s, int which, void *out, size_t cap, size_t *count)
     case 19: return dev(s->ekeys.p, m.e_v0.size(), 4);
     case 20: return dev(s->estarts.p, size_t(s->table_size) + 1, 4);
     case 21: return dev(s->esorted.p, m.e_v0.size(), 4);
+    case 22: {  // the inverse masses the solver reads (P.w)
+        if (count) *count = m.nv;
+        if (!out) return TC_OK;
+        if (cap < m.nv) return fail(TC_ERR_VALUE, "capacity too small");
+        std::vector<float4> tmp(m.nv);
+        TC_HIP(hipStreamSynchronize(s->stream));
+        if (m.nv) TC_HIP(hipMemcpy(tmp.data(), s->P.p, sizeof(float4) * m.nv, hipMemcpyDeviceToHost));
+        float *o = static_cast<float *>(out);
+        for (uint32_t i = 0; i < m.nv; ++i) o[i] = tmp[i].w;
+        return TC_OK;
+    }
     default: return fail(TC_ERR_VALUE, "unknown array id " + std::to_string(which));
     }
 }

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -142,6 +143,18 @@ struct tc_sim {
     // vertex streams (P has a second buffer for the one-launch-per-iteration path)
     DevBuf<float4> X, P, V, Palt;
     std::vector<float4> X0h, P0h;  // initial state for reset()
+
+    // dynamic pins (vertex -> target, weight) and inverse masses: inv_orig is the base inverse mass
+    // of each vertex (set_vertex_inv_masses changes it), inv_eff what P.w holds (orig * (1 - w) when
+    // pinned); the device table lists the pins in ascending vertex order
+    struct PinRec { float target[3]; float weight; };
+    std::map<uint32_t, PinRec> pins;
+    std::vector<float> inv_orig, inv_eff;
+    DevBuf<uint32_t> pin_v;
+    DevBuf<float4> pin_t;  // target xyz, weight
+    uint32_t npins = 0, pin_cap = 0;
+    DevBuf<uint32_t> scat_idx;
+    DevBuf<float> scat_val;
 
     // constraints
     DevBuf<uint4> drec, brec;
@@ -345,6 +358,12 @@ struct tc_sim {
     tc::HashBufs vhash() const { return {vkeys.p, vcounts.p, vstarts.p, vsorted.p}; }
     tc::HashBufs ehash() const { return {ekeys.p, ecounts.p, estarts.p, esorted.p}; }
 };
+
+// Dynamic pins (tc_pins.hip): the pin projection of the current table on P, the table's upload,
+// and the inverse mass a vertex takes when its state is set from outside.
+hipError_t enqueue_pins(tc_sim *s, float4 *P, const tc::SubstepArgs &a);
+int upload_pins(tc_sim *s);
+float effective_inv_mass(const tc_sim *s, uint32_t i);
 
 // One frame of `substeps` substeps on the simulator's stream (graph replay when it applies).
 int tc_run_frame(tc_sim *s, float dt, uint32_t substeps);

@@ -5,8 +5,11 @@ Follows the record layout of crates/cloth_core/src/debug_recorder.rs (JsonlRecor
 FrameStats, ColliderRecord tagged by "type", SimulationMetadata) and the capture of
 simulation/recording.rs:110-230: after each step the positions and velocities, the colliders, the
 frame statistics (largest speed and displacement, NaN/Inf, bounding box, largest stretch strain,
-config hash) and, on the frames where the SimConfig changed, the whole SimConfig as param_deltas.
-What this build does not simulate (pins, sewing, bone SDFs, pair counters) is written empty.
+config hash), the dynamic pins (PinRecord: vertex_idx, target_pos, weight), on the frames where the
+SimConfig changed the whole SimConfig as param_deltas, and on the frames where base inverse masses
+changed their new values as inv_mass_deltas (debug_recorder.rs:478-505: a value counts as changed
+when its bits differ, +0 and -0 being the same).  What this build does not simulate (sewing, bone
+SDFs, pair counters) is written empty.
 
 `load(path)` reads such a file; `replay(path)` builds a simulator from the metadata and steps it
 through the recorded frames (dt, substeps, iterations, configuration changes), for replay-driven
@@ -54,7 +57,8 @@ class DebugRecorder:
         self._edges = edges.astype(np.int64)
         self._rest = np.linalg.norm(pos[self._edges[:, 0]] - pos[self._edges[:, 1]], axis=1) if len(edges) else \
             np.zeros(0)
-        inv = c["inv_masses"] if c["inv_masses"] is not None else np.ones(len(pos), np.float32)
+        inv = sim.get_vertex_inv_masses()
+        self._prev_inv = inv.copy()
         self.metadata = {
             "object_name": object_name, "num_vertices": int(len(pos)), "num_edges": int(len(edges)),
             "num_faces": 0 if c["faces"] is None else int(len(c["faces"])),
@@ -79,14 +83,22 @@ class DebugRecorder:
         pos = flat.reshape(-1, 3)
         vel = sim.velocities()
         h = sim.get_config_hash()
+        pidx, ptgt, pw = sim.get_pins()
+        pins = [{"vertex_idx": int(i), "target_pos": [float(x) for x in t], "weight": float(w)}
+                for i, t, w in zip(pidx, ptgt, pw)]
+        inv = sim.get_vertex_inv_masses()
+        changed = (inv.view(np.uint32) != self._prev_inv.view(np.uint32)) & ~((inv == 0) & (self._prev_inv == 0))
+        deltas = [{"vertex_idx": int(i), "inv_mass": float(inv[i])} for i in np.flatnonzero(changed)]
+        self._prev_inv = inv.copy()
         rec = {"frame_index": self.seq, "dt": dt, "substeps": substeps,
                "solver_iterations": sim.get_effective_solver_iterations(),
-               "positions": pos.tolist(), "velocities": vel.tolist(), "pinned_indices": [], "pins": [],
+               "positions": pos.tolist(), "velocities": vel.tolist(),
+               "pinned_indices": [p["vertex_idx"] for p in pins], "pins": pins,
                "colliders": [dict(c) for c in sim._colliders],
                "stats": _stats(pos, vel, self._prev_pos, self._edges, self._rest, h),
                "param_deltas": json.loads(sim.get_config_json()) if h != self._prev_hash else None,
                "elastic_scales": None, "bone_transforms": None, "sewing_rest_lengths": None, "sewing_priority": None,
-               "inv_mass_deltas": None}
+               "inv_mass_deltas": deltas or None}
         self._prev_pos = pos.copy()
         self._prev_hash = h
         self.seq += 1
@@ -150,6 +162,19 @@ def replay(path: str):
                     sim.set_mesh_collider_triangles(np.asarray(c["triangles"], np.float32).reshape(-1, 3, 3),
                                                     c["friction"], c["thickness"], c["restitution"], c["single_sided"])
             colliders = fr["colliders"]
+        if fr.get("inv_mass_deltas"):
+            sim.set_vertex_inv_masses(np.array([d["vertex_idx"] for d in fr["inv_mass_deltas"]], np.uint32),
+                                      np.array([d["inv_mass"] for d in fr["inv_mass_deltas"]], np.float32))
+        pins = fr.get("pins") or []
+        gidx, gt, gw = sim.get_pins()
+        want = ([p["vertex_idx"] for p in pins], [p["target_pos"] for p in pins], [p["weight"] for p in pins])
+        if (list(gidx) != want[0] or not np.array_equal(gt, np.asarray(want[1], np.float32).reshape(-1, 3))
+                or not np.array_equal(gw.view(np.uint32), np.asarray(want[2], np.float32).view(np.uint32))):
+            gone = sorted(set(int(i) for i in gidx) - set(want[0]))
+            if gone:
+                sim.release_pins_batch(gone)
+            if pins:
+                sim.set_pins_batch(want[0], np.asarray(want[1], np.float32).reshape(-1, 3), want[2])
         # the recorded substeps and iterations are what ran; the adaptive policy is not re-run
         sim.set_adaptive_substep_options(False, sim._cfg["min_substeps"], sim._cfg["max_substeps"])
         sim.set_solver_iterations(fr["solver_iterations"])

@@ -315,6 +315,69 @@ class ClothSimulator:
     def reset(self):
         _lib.check(self._L.tc_sim_reset(self._h))
 
+    # --- dynamic pins and inverse masses (lib.rs:2616-2676) ----------------------------------------
+    def set_pin(self, vertex_idx, target_pos, weight=1.0):
+        idx = np.array([int(vertex_idx)], np.uint32) if 0 <= int(vertex_idx) < 2 ** 32 else None
+        if idx is None:
+            raise OverflowError("vertex_idx must fit an unsigned 32-bit integer")
+        t = _vec3(target_pos, "target_pos")
+        w = np.array([weight], np.float32)
+        _lib.check(self._L.tc_sim_set_pins(self._h, _ptr(idx), _ptr(t), _ptr(w), 1))
+
+    def set_pins_batch(self, indices, targets, weights):
+        """lib.rs:2637-2657: the shortest of the three lengths counts; targets is (N, 3)."""
+        idx = np.ascontiguousarray(np.asarray(indices, dtype=np.int64).reshape(-1))
+        if idx.size and (idx.min() < 0 or idx.max() >= 2 ** 32):
+            raise OverflowError("indices must fit unsigned 32-bit integers")
+        t = np.asarray(targets, dtype=np.float32)
+        if t.ndim != 2:
+            raise ValueError("targets must be a 2-D array")
+        if t.shape[0] and t.shape[1] != 3:
+            raise ValueError("each target must have 3 components")
+        w = np.ascontiguousarray(np.asarray(weights, dtype=np.float32).reshape(-1))
+        n = min(idx.size, t.shape[0], w.size)
+        idx32 = np.ascontiguousarray(idx[:n].astype(np.uint32))
+        t = np.ascontiguousarray(t[:n])
+        _lib.check(self._L.tc_sim_set_pins(self._h, _ptr(idx32), _ptr(t), _ptr(w[:n].copy()), n))
+
+    def release_pin(self, vertex_idx):
+        self.release_pins_batch([vertex_idx])
+
+    def release_pins_batch(self, indices):
+        idx = np.asarray(indices, dtype=np.int64).reshape(-1)
+        if idx.size and (idx.min() < 0 or idx.max() >= 2 ** 32):
+            raise OverflowError("indices must fit unsigned 32-bit integers")
+        idx32 = np.ascontiguousarray(idx.astype(np.uint32))
+        _lib.check(self._L.tc_sim_release_pins(self._h, _ptr(idx32), idx32.size))
+
+    def clear_pins(self):
+        _lib.check(self._L.tc_sim_clear_pins(self._h))
+
+    def get_pins(self):
+        """The pins as (indices uint32, targets (N, 3) float32, weights float32), ascending index."""
+        n = C.c_uint32(0)
+        _lib.check(self._L.tc_sim_get_pins(self._h, None, None, None, 0, C.byref(n)))
+        idx = np.empty(n.value, np.uint32)
+        t = np.empty((n.value, 3), np.float32)
+        w = np.empty(n.value, np.float32)
+        _lib.check(self._L.tc_sim_get_pins(self._h, _ptr(idx), _ptr(t), _ptr(w), n.value, C.byref(n)))
+        return idx, t, w
+
+    def set_vertex_inv_masses(self, indices, inv_masses):
+        idx = np.asarray(indices)
+        inv = np.asarray(inv_masses)
+        if idx.dtype != np.uint32 or inv.dtype != np.float32 or idx.ndim != 1 or inv.ndim != 1:
+            raise TypeError("indices must be a 1-D uint32 array and inv_masses a 1-D float32 array")
+        if not idx.flags.c_contiguous or not inv.flags.c_contiguous:
+            raise TypeError("indices and inv_masses must be contiguous")
+        n = min(idx.size, inv.size)
+        _lib.check(self._L.tc_sim_set_vertex_inv_masses(self._h, _ptr(idx), _ptr(inv), n))
+
+    def get_vertex_inv_masses(self) -> np.ndarray:
+        out = np.empty(self._n, np.float32)
+        _lib.check(self._L.tc_sim_get_vertex_inv_masses(self._h, _ptr(out), out.size))
+        return out
+
     # --- colliders -----------------------------------------------------------------------------
     def add_sphere_collider(self, center, radius, friction=0.0, restitution=0.0):
         c = _vec3(center, "center")
@@ -478,7 +541,11 @@ class ClothSimulator:
         _lib.check(self._L.tc_sim_save_profile_trace(self._h, str(path).encode()))
 
     # --- introspection used by parity checks ---------------------------------------------------
-    _FLOAT_ARRAYS = {8, 9, 13, 14, 15, 16}
+    _FLOAT_ARRAYS = {8, 9, 13, 14, 15, 16, 22}
+
+    def device_inv_masses(self) -> np.ndarray:
+        """The inverse masses the solver reads (base values scaled by the pins)."""
+        return self._array(22)
 
     def _array(self, which: int) -> np.ndarray:
         cnt = C.c_size_t(0)

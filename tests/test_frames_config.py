@@ -198,9 +198,15 @@ def test_gpu_debug_recording_replays_bit_for_bit(require_gpu, tmp_path):
     sim.start_debug_recording("cloth", max_frames=100)
     assert sim.is_debug_recording()
     for k in range(6):
+        if k == 1:
+            sim.set_pins_batch([0, 19], pos[[0, 19]] + np.float32([0.0, 0.0, 0.03]), [1.0, 0.6])
+        if k == 2:
+            sim.set_vertex_inv_masses(np.array([5, 6], np.uint32), np.array([0.5, 0.0], np.float32))
         if k == 3:
             sim.set_stiffness_all(800.0, 800.0, 400.0, 5.0)
             sim.add_plane_collider([0.0, 0.0, -0.05], [0.0, 0.0, 1.0], 0.5, 0.0)
+        if k == 4:
+            sim.release_pin(19)
         sim.step(1 / 60, 10)
     sim.stop_debug_recording()
     path = sim.save_debug_recording(str(tmp_path / "rec.jsonl.gz"))
@@ -209,6 +215,11 @@ def test_gpu_debug_recording_replays_bit_for_bit(require_gpu, tmp_path):
     assert frames[0]["param_deltas"] is None and frames[3]["param_deltas"] is not None
     assert frames[3]["param_deltas"]["tension_stiffness"] == 800.0
     assert [c["type"] for c in frames[3]["colliders"]] == ["sphere", "plane"]
+    assert frames[0]["pins"] == [] and frames[1]["pinned_indices"] == [0, 19] and frames[4]["pinned_indices"] == [0]
+    assert frames[1]["pins"][1]["weight"] == np.float32(0.6)
+    assert frames[1]["inv_mass_deltas"] is None
+    assert frames[2]["inv_mass_deltas"] == [{"vertex_idx": 5, "inv_mass": 0.5}, {"vertex_idx": 6, "inv_mass": 0.0}]
+    assert frames[3]["inv_mass_deltas"] is None
     assert not frames[5]["stats"]["has_nan_or_inf"] and frames[5]["stats"]["config_hash"] == sim.get_config_hash()
     n = 0
     for fr, p in DR.replay(path):
