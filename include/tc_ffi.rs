@@ -99,6 +99,8 @@ extern "C" {
     pub fn tc_sim_set_compliances_raw(sim: *mut TcSim, stretch: f32, shear: f32, bend: f32) -> i32;
     pub fn tc_sim_set_hash_cell_size(sim: *mut TcSim, cell_size: f32) -> i32;
     pub fn tc_sim_set_solver_path(sim: *mut TcSim, path: i32) -> i32;
+    pub fn tc_sim_set_solver_mode(sim: *mut TcSim, mode: u32) -> i32;
+    pub fn tc_sim_get_solver_mode(sim: *const TcSim, mode: *mut u32) -> i32;
     pub fn tc_sim_solver_info(sim: *mut TcSim, out: *mut f64) -> i32;
     pub fn tc_sim_autotune(sim: *mut TcSim, dt_sub: f32, out: *mut f64) -> i32;
     pub fn tc_sim_num_vertices(sim: *const TcSim) -> u32;

@@ -47,7 +47,7 @@ typedef struct {
     float compression_stiffness;  /* < 0: = stiffness */
     float shear_stiffness;        /* < 0: = 0.5 * stiffness */
     uint32_t workgroup_size;      /* 0: autotuned; otherwise 64/128/256 */
-    uint32_t solver_mode;         /* 0 = graph colouring (only mode supported) */
+    uint32_t solver_mode;         /* 0 = graph colouring (Gauss-Seidel per colour), 1 = fixed-point Jacobi */
 } tc_sim_params;
 
 /* Fills the reference defaults. */
@@ -209,6 +209,12 @@ int tc_sim_set_hash_cell_size(tc_sim *sim, float cell_size);
  * 0 auto, 1 one launch per colour (the reference's dispatch pattern, dispatch.rs:1449-1455),
  * 2 one launch per solver iteration over dependency-cone tiles held in LDS. */
 int tc_sim_set_solver_path(tc_sim *sim, int path);
+/* set_solver_mode / get_solver_mode, lib.rs:1887-1895 -> simulation/mod.rs:2975-2984: 1 solves the
+ * distance constraints of an iteration in one pass with fixed-point Jacobi sums, averaged and
+ * over-relaxed by 1.25 (distance_atomic.wgsl:78-182; lambdas accumulate a quarter of each step);
+ * any other value colours.  Mode 1 always takes the one-launch-per-colour path. */
+int tc_sim_set_solver_mode(tc_sim *sim, uint32_t mode);
+int tc_sim_get_solver_mode(const tc_sim *sim, uint32_t *mode);
 /* out[10]: [active path (1|2), tiles, largest cone (vertices), cone vertices total,
  *           items per iteration, constraints per iteration, plan available (0|1), colour phases,
  *           tile size (vertices), block size of the per-iteration kernel] */

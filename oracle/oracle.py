@@ -112,6 +112,7 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.orc_sim_set_pin_table.argtypes = [_vp, _vp, _vp, _vp, C.c_uint32]
     L.orc_sim_set_inv_masses.argtypes = [_vp, _vp, _vp, C.c_uint32]
     L.orc_sim_get_inv_masses.argtypes = [_vp, C.c_int, _f32p]
+    L.orc_sim_set_solver_mode.argtypes = [_vp, C.c_uint32]
     _LIBS[precision] = L
     return L
 
@@ -226,6 +227,10 @@ class OracleSim:
 
     def set_solver_iterations(self, it):
         self._L.orc_sim_set_iterations(self._h, it)
+
+    def set_solver_mode(self, mode):
+        """0: colours; 1: fixed-point Jacobi distance pass (distance_atomic.wgsl)."""
+        self._L.orc_sim_set_solver_mode(self._h, int(mode))
 
     def set_enable_self_collision(self, on):
         self._L.orc_sim_set_self_collision(self._h, 1 if on else 0)

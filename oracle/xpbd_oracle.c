@@ -1044,6 +1044,8 @@ typedef struct {
     uint32_t *sdf_tex; float *sdf_info, *sdf_xf;     /* SDF colliders (sdf_oracle.c) */
     uint32_t sdf_w, sdf_h, sdf_d, sdf_nb;
     uint32_t npins, *pin_v; v3 *pin_t; real *pin_w;  /* dynamic pin table (distinct vertices) */
+    uint32_t solver_mode;                            /* 1: fixed-point Jacobi distance pass */
+    int32_t *jac;                                    /* per-vertex sums {dx, dy, dz, count} */
 } orc_sim;
 
 orc_sim *orc_sim_create(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
@@ -1077,6 +1079,7 @@ orc_sim *orc_sim_create(const float *pos, uint32_t nv, const uint32_t *edges, ui
     s->self_collision = 1;
     s->cell_size = m->cell_size; s->table_size = m->table_size;
     s->acc = (accum_t *)calloc(N, sizeof(accum_t));
+    s->jac = (int32_t *)calloc(4 * N, sizeof(int32_t));
     s->hkeys = (uint32_t *)malloc(N * 4); s->hsorted = (uint32_t *)malloc(N * 4);
     s->hstarts = (uint32_t *)malloc((size_t)(s->table_size + 1) * 4);
     size_t NE = m->ne ? m->ne : 1;
@@ -1096,7 +1099,7 @@ void orc_sim_free(orc_sim *s) {
     free(s->d_tc); free(s->d_cc); free(s->b_comp); free(s->cols); free(s->acc); free(s->tris); free(s->tri_bounds);
     free(s->hkeys); free(s->hstarts); free(s->hsorted); free(s->ekeys); free(s->estarts); free(s->esorted);
     free(s->sdf_tex); free(s->sdf_info); free(s->sdf_xf);
-    free(s->pin_v); free(s->pin_t); free(s->pin_w);
+    free(s->pin_v); free(s->pin_t); free(s->pin_w); free(s->jac);
     free(s);
 }
 
@@ -1300,6 +1303,8 @@ static void k_predict(orc_sim *s, real dt_sub) {
     }
 }
 
+void orc_sim_set_solver_mode(orc_sim *s, uint32_t mode) { s->solver_mode = mode; }
+
 /* distance.wgsl:60-124 for one constraint */
 static void k_distance_one(orc_sim *s, uint32_t k, real dt_sub) {
     const orc_mesh *m = s->m;
@@ -1361,6 +1366,65 @@ static void k_bending_one(orc_sim *s, uint32_t k, real dt_sub) {
     s->lam_b[k] = lam + dl;
     s->p[v2] = add(s->p[v2], mul(dir, w2 * dl));
     s->p[v3_] = add(s->p[v3_], mul(dir, -w3 * dl));
+}
+
+/* distance_atomic.wgsl:78-158 (solve_distance_atomic) for one constraint: the XPBD step with the
+ * Rayleigh term always present, a quarter of it kept in lambda, the corrections added to the
+ * per-vertex sums in 1e-6 m fixed point (saturated at +-5e7, truncated toward zero, wrapping adds) */
+static inline int32_t jac_fixed(real x) {
+    return f2i_sat(rclamp(x * (real)1000000.0f, (real)-50000000.0f, (real)50000000.0f));
+}
+static inline void jac_add(int32_t *a, int32_t v) { *a = (int32_t)((uint32_t)*a + (uint32_t)v); }
+static void k_distance_jacobi_one(orc_sim *s, uint32_t k, real dt_sub) {
+    const orc_mesh *m = s->m;
+    uint32_t a = m->d_v0[k], b = m->d_v1[k];
+    real w0 = s->w[a], w1 = s->w[b], ws = w0 + w1;
+    if (ws <= (real)1e-7f) return;
+    v3 p0 = s->p[a], p1 = s->p[b];
+    v3 delta = sub(p0, p1);
+    real dist = len3(delta);
+    if (dist <= (real)1e-7f) return;
+    real c = dist - (real)m->d_rest[k];
+    real comp = c >= 0 ? s->d_tc[k] : s->d_cc[k];
+    if (comp >= (real)1e9f) return;
+    v3 dir = dvs(delta, dist);
+    real dt = rmax(dt_sub, (real)1e-8f);
+    real alpha = comp / (dt * dt);
+    real dc = m->d_type[k] == 1 ? s->shear_damping : (c >= 0 ? s->tension_damping : s->compression_damping);
+    real gamma = rmin(dc * (real)1e-5f / dt, (real)1e4f);
+    v3 rel = sub(sub(p0, s->x[a]), sub(p1, s->x[b]));
+    real vproj = dot(dir, rel);
+    real l = s->lam_d[k];
+    real dl = (-c - alpha * l - gamma * vproj) / (((real)1 + gamma) * ws + alpha);
+    s->lam_d[k] = l + dl * (real)0.25f;
+    v3 c0 = mul(dir, w0 * dl), c1 = mul(dir, -w1 * dl);
+    int32_t *A = s->jac + 4 * (size_t)a, *B = s->jac + 4 * (size_t)b;
+    jac_add(&A[0], jac_fixed(c0.x)); jac_add(&A[1], jac_fixed(c0.y)); jac_add(&A[2], jac_fixed(c0.z)); A[3] += 1;
+    jac_add(&B[0], jac_fixed(c1.x)); jac_add(&B[1], jac_fixed(c1.y)); jac_add(&B[2], jac_fixed(c1.z)); B[3] += 1;
+}
+/* distance_atomic.wgsl:162-182 (apply_atomic_accum): the mean correction times 1.25, sums cleared */
+static void k_jacobi_apply(orc_sim *s) {
+    for (uint32_t v = 0; v < s->nv; v++) {
+        int32_t *A = s->jac + 4 * (size_t)v;
+        uint32_t count = (uint32_t)A[3];
+        if (count == 0) continue;
+        real scale = (real)(1.0f / 1000000.0f) / (real)(float)count;
+        s->p[v] = add(s->p[v], mk((real)(float)A[0] * scale * (real)1.25f, (real)(float)A[1] * scale * (real)1.25f,
+                                  (real)(float)A[2] * scale * (real)1.25f));
+        A[0] = A[1] = A[2] = A[3] = 0;
+    }
+}
+static void distance_pass(orc_sim *s, real dt_sub) {
+    const orc_mesh *m = s->m;
+    if (s->solver_mode == 1) {
+        for (uint32_t k = 0; k < m->nd; k++) k_distance_jacobi_one(s, k, dt_sub);
+        k_jacobi_apply(s);
+        return;
+    }
+    for (uint32_t c = 0; c < m->nd_col; c++) {
+        ORC_PARALLEL_FOR
+        for (uint32_t k = m->d_col_off[c]; k < m->d_col_off[c] + m->d_col_cnt[c]; k++) k_distance_one(s, k, dt_sub);
+    }
 }
 
 static int seg_tri(v3 po, v3 pn, v3 a, v3 b, v3 c, real *t_out, real *u_out, real *v_out);
@@ -2145,10 +2209,7 @@ static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
         k_predict(s, dt_sub);
         for (uint32_t it = 0; it < s->iterations; it++) {
             k_pins(s, dt_sub);
-            for (uint32_t c = 0; c < m->nd_col; c++) {
-                ORC_PARALLEL_FOR
-                for (uint32_t k = m->d_col_off[c]; k < m->d_col_off[c] + m->d_col_cnt[c]; k++) k_distance_one(s, k, dt_sub);
-            }
+            distance_pass(s, dt_sub);
             for (uint32_t c = 0; c < m->nb_col; c++) {
                 ORC_PARALLEL_FOR
                 for (uint32_t k = m->b_col_off[c]; k < m->b_col_off[c] + m->b_col_cnt[c]; k++) k_bending_one(s, k, dt_sub);
@@ -2179,10 +2240,7 @@ void orc_sim_substep_parts(orc_sim *s, float dt_sub, int do_collide, int do_self
     k_predict(s, dt);
     for (uint32_t it = 0; it < s->iterations; it++) {
         k_pins(s, dt);
-        for (uint32_t c = 0; c < m->nd_col; c++) {
-            ORC_PARALLEL_FOR
-            for (uint32_t k = m->d_col_off[c]; k < m->d_col_off[c] + m->d_col_cnt[c]; k++) k_distance_one(s, k, dt);
-        }
+        distance_pass(s, dt);
         for (uint32_t c = 0; c < m->nb_col; c++) {
             ORC_PARALLEL_FOR
             for (uint32_t k = m->b_col_off[c]; k < m->b_col_off[c] + m->b_col_cnt[c]; k++) k_bending_one(s, k, dt);

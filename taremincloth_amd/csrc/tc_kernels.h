@@ -26,6 +26,11 @@ hipError_t launch_predict(hipStream_t s, float4 *P, const float4 *X, float4 *V, 
 hipError_t launch_distance(hipStream_t s, float4 *P, const float4 *X, const uint4 *rec, const float *cc, float *lam,
                            uint32_t off, uint32_t cnt, const SubstepArgs &a, bool first, bool split_compliance,
                            uint32_t block);
+// solver_mode 1 (distance_atomic.wgsl:78-182): all distance constraints at once into fixed-point
+// sums `acc` (zero on entry, left zero), then the averaged, over-relaxed corrections on P.
+hipError_t launch_distance_jacobi(hipStream_t s, float4 *P, const float4 *X, const uint4 *rec, const float *cc,
+                                  float *lam, uint32_t nd, int4 *acc, uint32_t nv, const SubstepArgs &a, bool first,
+                                  bool split);
 hipError_t launch_bending(hipStream_t s, float4 *P, const float4 *X, const uint4 *rec, float *lam, uint32_t off,
                           uint32_t cnt, const SubstepArgs &a, bool first, uint32_t block);
 // SDF colliders (set_bone_sdf_colliders, crates/cloth_core/src/simulation/mod.rs:1630-1727): an

@@ -76,10 +76,16 @@ hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t coun
         const bool f = first && it == 0;
         const int q = s->prof_begin("solver_iter");
         if (s->npins && (e = enqueue_pins(s, s->P.p, a)) != hipSuccess) return e;  // dispatch.rs:176-181
-        for (size_t c = 0; c < m.d_col.count.size(); ++c) {
-            e = tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p, m.d_col.offset[c],
-                                    m.d_col.count[c], a, f, s->split_compliance, B);
+        if (s->solver_mode == 1) {  // dispatch.rs:1436-1446
+            e = tc::launch_distance_jacobi(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p,
+                                           uint32_t(m.d_v0.size()), s->jacobi_acc.p, m.nv, a, f, s->split_compliance);
             if (e != hipSuccess) return e;
+        } else {
+            for (size_t c = 0; c < m.d_col.count.size(); ++c) {
+                e = tc::launch_distance(s->stream, s->P.p, s->X.p, s->drec.p, s->dcc.p, s->dlam.p, m.d_col.offset[c],
+                                        m.d_col.count[c], a, f, s->split_compliance, B);
+                if (e != hipSuccess) return e;
+            }
         }
         for (size_t c = 0; c < m.b_col.count.size(); ++c) {
             e = tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p, m.b_col.offset[c],
@@ -507,7 +513,6 @@ int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, ui
     tc_sim_params_default(&def);
     const tc_sim_params &pp = params ? *params : def;
     if (int rc = check_topology(positions, nv, edges, ne, faces, nf)) return rc;
-    if (pp.solver_mode != 0) return fail(TC_ERR_UNSUPPORTED, "only solver_mode=0 (graph colouring) is supported");
     const tc::MeshInput in = mesh_input(positions, nv, edges, ne, faces, nf, inv_masses, thicknesses, layer_ids, pp);
 
     tc_sim *s = new tc_sim();
@@ -587,6 +592,7 @@ static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
         const long v = std::strtol(fb, nullptr, 10);
         if (v == 256 || v == 512 || v == 1024) s->fused_block = uint32_t(v);
     }
+    s->solver_mode = pp.solver_mode;
     s->cell_size = s->mesh.cell_size;
     s->table_size = device_table_size(s->mesh);
 
@@ -639,12 +645,14 @@ static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
     AL(s->vkeys, nv); AL(s->vsorted, nv); AL(s->vcounts, T + (T + 1023) / 1024); AL(s->vstarts, T + 1);
     AL(s->ekeys, NE); AL(s->esorted, NE); AL(s->ecounts, T + (T + 1023) / 1024); AL(s->estarts, T + 1);
     AL(s->accum, nv);
+    AL(s->jacobi_acc, nv);
     AL(s->readback, size_t(nv) * 3);
     AL(s->resid, 4);
     AL(s->resid_part, 3 * 1024);
 #undef UP
 #undef AL
     if ((he = hipMemsetAsync(s->accum.p, 0, sizeof(tc::Accum) * std::max<size_t>(nv, 1), s->stream)) != hipSuccess ||
+        (he = hipMemsetAsync(s->jacobi_acc.p, 0, sizeof(int4) * std::max<size_t>(nv, 1), s->stream)) != hipSuccess ||
         (he = hipMemsetAsync(s->dlam.p, 0, sizeof(float) * std::max<size_t>(m.d_v0.size(), 1), s->stream)) != hipSuccess ||
         (he = hipMemsetAsync(s->blam.p, 0, sizeof(float) * std::max<size_t>(m.b_v0.size(), 1), s->stream)) != hipSuccess ||
         (he = hipMemsetAsync(s->dlam2.p, 0, sizeof(float) * std::max<size_t>(m.d_v0.size(), 1), s->stream)) != hipSuccess ||
@@ -665,7 +673,6 @@ int tc_sim_create_batch(const tc_instance *instances, uint32_t n, const tc_sim_p
     tc_sim_params def;
     tc_sim_params_default(&def);
     const tc_sim_params &pp = params ? *params : def;
-    if (pp.solver_mode != 0) return fail(TC_ERR_UNSUPPORTED, "only solver_mode=0 (graph colouring) is supported");
     std::vector<tc::Mesh> parts(n);
     uint64_t total = 0;
     for (uint32_t k = 0; k < n; ++k) {
@@ -1313,6 +1320,19 @@ int tc_sim_set_solver_path(tc_sim *s, int path) {
         return fail(TC_ERR_UNSUPPORTED, "one-launch-per-iteration path unavailable: " + s->fused_reason);
     s->solver_path = path;
     s->touch();
+    return TC_OK;
+}
+
+int tc_sim_set_solver_mode(tc_sim *s, uint32_t mode) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    s->solver_mode = mode;  // 1: Jacobi; every other value colours (dispatch.rs:1436)
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_get_solver_mode(const tc_sim *s, uint32_t *mode) {
+    if (!s || !mode) return fail(TC_ERR_VALUE, "null argument");
+    *mode = s->solver_mode;
     return TC_OK;
 }
 

@@ -209,6 +209,8 @@ struct tc_sim {
 
     // one-launch-per-iteration plan (tc_tiles.cpp)
     int solver_path = 0;          // 0 auto, 1 one launch per colour, 2 one launch per iteration
+    uint32_t solver_mode = 0;     // 1: distance constraints by fixed-point Jacobi sums (tc_jacobi.hip)
+    DevBuf<int4> jacobi_acc;      // per-vertex sums of solver_mode 1, zero between iterations
     bool fused_ready = false;     // plan built and uploaded
     std::string fused_reason;     // why the plan is unavailable
     tc::TilePlan plan;
@@ -338,7 +340,7 @@ struct tc_sim {
     }
     bool fused_active() const {
         if (!fused_ready) return false;
-        if (solver_path == 1) return false;
+        if (solver_mode == 1 || solver_path == 1) return false;
         if (solver_path == 2) return true;
         return mesh.nv >= 16384;  // small meshes: the per-colour launches are cheap
     }

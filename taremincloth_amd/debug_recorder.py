@@ -70,7 +70,7 @@ class DebugRecorder:
             "sewing_springs": None, "stiffness": c["stiffness"], "compression_stiffness": c["compression_stiffness"],
             "shear_stiffness": c["shear_stiffness"], "bending_stiffness": c["bending_stiffness"],
             "thickness": c["thickness"], "layer_id": c["layer_id"],
-            "gravity": list(sim._cfg["gravity"]), "damping": sim._cfg["damping"], "solver_mode": 0,
+            "gravity": list(sim._cfg["gravity"]), "damping": sim._cfg["damping"], "solver_mode": sim.get_solver_mode(),
             "config": json.loads(sim.get_config_json()),
         }
         self._prev_pos = None

@@ -80,8 +80,9 @@ class ClothSimulator:
             raise NotImplementedError("only self_collision_algorithm=0 (DIRECT) is part of this build")
         if enable_pair_cache:
             raise NotImplementedError("the pair-cache self-collision path is not part of this build")
-        if solver_mode != 0:
-            raise NotImplementedError("only solver_mode=0 (graph colouring) is part of this build")
+        solver_mode = int(solver_mode)
+        if not 0 <= solver_mode < 2 ** 32:
+            raise OverflowError("solver_mode must fit an unsigned 32-bit integer")
 
         def vec(a, dtype, name):
             if a is None:
@@ -105,7 +106,7 @@ class ClothSimulator:
         # The reference's workgroup_size selects 32/64-wide WGSL groups; wave64 hardware
         # uses its own tuned block size, so the value is accepted and not forwarded.
         p.workgroup_size = 0
-        p.solver_mode = 0
+        p.solver_mode = solver_mode
         h = C.c_void_p()
         _lib.check(L.tc_sim_create(_ptr(pos), n, _ptr(e), len(e), _ptr(f), 0 if f is None else len(f), _ptr(im),
                                    _ptr(th), _ptr(ly), C.byref(p), C.byref(h)))
@@ -116,6 +117,7 @@ class ClothSimulator:
         # uses the inverse masses given at construction.
         self.areal_density = 0.15 if areal_density is None else float(min(max(areal_density, 0.0), 100.0))
         self._init_config(stiffness, compression_stiffness, shear_stiffness, bending_stiffness, workgroup_size)
+        self._cfg["solver_mode"] = solver_mode
         self._ctor = dict(positions=pos, edges=e, faces=f, inv_masses=im, thicknesses=th, layer_ids=ly,
                           layer_id=int(layer_id), thickness=float(thickness), stiffness=float(stiffness),
                           bending_stiffness=float(bending_stiffness), compression_stiffness=compression_stiffness,
@@ -247,7 +249,7 @@ class ClothSimulator:
         c = _sc.parse(json_str)
         inert = {"coupled_mode": 0, "coupled_collider": False, "substep_interval": 1, "ee_substep_interval": 1,
                  "self_collision_algorithm": 0, "enable_pair_cache": False, "enable_edge_collision": False,
-                 "sewing_priority_enabled": False, "solver_mode": 0}
+                 "sewing_priority_enabled": False}
         for k, v in inert.items():
             if c[k] != v:
                 raise NotImplementedError(f"SimConfig {k}={c[k]!r}: that solver option is not part of this build")
@@ -257,6 +259,7 @@ class ClothSimulator:
         self.set_stiffness_all(c["tension_stiffness"], c["compression_stiffness"], c["shear_stiffness"],
                                c["bending_stiffness"])
         self.set_solver_iterations(c["solver_iterations"])
+        self.set_solver_mode(c["solver_mode"])
         self.set_workgroup_size(c["workgroup_size"])
         self.set_enable_self_collision(c["enable_self_collision"])
         self.set_self_collision_options(c["relief_factor"], c["max_displacement_ratio"], c["exclude_neighbors"],
@@ -314,6 +317,19 @@ class ClothSimulator:
 
     def reset(self):
         _lib.check(self._L.tc_sim_reset(self._h))
+
+    def set_solver_mode(self, mode):
+        """0: graph colouring; 1: fixed-point Jacobi for the distance constraints (lib.rs:1887-1890)."""
+        mode = int(mode)
+        if not 0 <= mode < 2 ** 32:
+            raise OverflowError("mode must fit an unsigned 32-bit integer")
+        _lib.check(self._L.tc_sim_set_solver_mode(self._h, mode))
+        self._cfg["solver_mode"] = mode
+
+    def get_solver_mode(self) -> int:
+        m = C.c_uint32(0)
+        _lib.check(self._L.tc_sim_get_solver_mode(self._h, C.byref(m)))
+        return int(m.value)
 
     # --- dynamic pins and inverse masses (lib.rs:2616-2676) ----------------------------------------
     def set_pin(self, vertex_idx, target_pos, weight=1.0):

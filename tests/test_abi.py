@@ -71,8 +71,6 @@ def test_unsupported_options_raise():
     pos, e, f = cases.grid(3, 3, 0.1)
     with pytest.raises(NotImplementedError):
         T.ClothSimulator(pos, e, f, sewing_springs=np.array([[0, 8]], np.uint32))
-    with pytest.raises(NotImplementedError):
-        T.ClothSimulator(pos, e, f, solver_mode=1)
 
 
 def test_abi_struct_layout_matches_ctypes():
