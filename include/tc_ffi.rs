@@ -101,6 +101,14 @@ extern "C" {
     pub fn tc_sim_set_solver_path(sim: *mut TcSim, path: i32) -> i32;
     pub fn tc_sim_set_solver_mode(sim: *mut TcSim, mode: u32) -> i32;
     pub fn tc_sim_get_solver_mode(sim: *const TcSim, mode: *mut u32) -> i32;
+    pub fn tc_sim_set_enable_pair_cache(sim: *mut TcSim, enable: i32) -> i32;
+    pub fn tc_sim_get_enable_pair_cache(sim: *const TcSim, enable: *mut i32) -> i32;
+    pub fn tc_sim_set_self_collision_algorithm(sim: *mut TcSim, algo: u32) -> i32;
+    pub fn tc_sim_get_self_collision_algorithm(sim: *const TcSim, algo: *mut u32) -> i32;
+    pub fn tc_sim_set_pair_cache_options(sim: *mut TcSim, max_vt_pairs: u32, max_ee_pairs: u32, margin_mode: u32, safety_margin: f32, horizon_scale: f32, max_horizon: f32) -> i32;
+    pub fn tc_sim_set_enable_pair_cache_final_fallback(sim: *mut TcSim, enable: i32) -> i32;
+    pub fn tc_sim_pair_cache_state(sim: *const TcSim, out: *mut f64) -> i32;
+    pub fn tc_sim_get_active_pairs(sim: *mut TcSim, counts: *mut u32, vt: *mut u32, vt_cap: usize, n_vt: *mut usize, ee: *mut u32, ee_cap: usize, n_ee: *mut usize) -> i32;
     pub fn tc_sim_solver_info(sim: *mut TcSim, out: *mut f64) -> i32;
     pub fn tc_sim_autotune(sim: *mut TcSim, dt_sub: f32, out: *mut f64) -> i32;
     pub fn tc_sim_num_vertices(sim: *const TcSim) -> u32;

@@ -215,6 +215,29 @@ int tc_sim_set_solver_path(tc_sim *sim, int path);
  * any other value colours.  Mode 1 always takes the one-launch-per-colour path. */
 int tc_sim_set_solver_mode(tc_sim *sim, uint32_t mode);
 int tc_sim_get_solver_mode(const tc_sim *sim, uint32_t *mode);
+/* Pair-cache self-collision (self_collision_algorithm 1), lib.rs:2029-2107 -> simulation/mod.rs:703-870,
+ * dispatch.rs:1554-1645.  On the first substep of a frame a broadphase keeps, per vertex, the
+ * quota = clamp(budget / num_vertices, 1, 8) closest V-T triples and E-E quadruples
+ * (self_collision_collect_pairs.wgsl); every substep then resolves only those
+ * (self_collision_solve_vt.wgsl, self_collision_solve_ee.wgsl) and applies the sums; with the final
+ * fallback (default on) the last substep takes the direct path instead.  Algorithm 2 (virtual
+ * vertices) returns TC_ERR_UNSUPPORTED.  set_pair_cache_options clamps as the reference: budgets to
+ * [64, 262144], safety margin to [0, 0.05], horizon scale to [0, 4], max horizon to [0, 0.1];
+ * margin_mode 0 drops the velocity horizon.  pair_cache_state out[9] = [max V-T, max E-E, quota V-T,
+ * quota E-E, margin mode, safety margin, horizon scale, max horizon, final fallback]. */
+int tc_sim_set_enable_pair_cache(tc_sim *sim, int enable);
+int tc_sim_get_enable_pair_cache(const tc_sim *sim, int *enable);
+int tc_sim_set_self_collision_algorithm(tc_sim *sim, uint32_t algo);
+int tc_sim_get_self_collision_algorithm(const tc_sim *sim, uint32_t *algo);
+int tc_sim_set_pair_cache_options(tc_sim *sim, uint32_t max_vt_pairs, uint32_t max_ee_pairs, uint32_t margin_mode,
+                                  float safety_margin, float horizon_scale, float max_horizon);
+int tc_sim_set_enable_pair_cache_final_fallback(tc_sim *sim, int enable);
+int tc_sim_pair_cache_state(const tc_sim *sim, double *out);
+/* get_active_pairs, dispatch.rs:1255-1291: counts[4] = [V-T evaluated, E-E evaluated, V-T dropped,
+ * E-E dropped] of the last collect; the held pairs in slot order (vertex-major), 4 indices each,
+ * up to *_cap pairs written, *n_* = pairs held (outputs may be NULL to count). */
+int tc_sim_get_active_pairs(tc_sim *sim, uint32_t *counts, uint32_t *vt, size_t vt_cap, size_t *n_vt, uint32_t *ee,
+                            size_t ee_cap, size_t *n_ee);
 /* out[10]: [active path (1|2), tiles, largest cone (vertices), cone vertices total,
  *           items per iteration, constraints per iteration, plan available (0|1), colour phases,
  *           tile size (vertices), block size of the per-iteration kernel] */

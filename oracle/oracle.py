@@ -113,6 +113,10 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.orc_sim_set_inv_masses.argtypes = [_vp, _vp, _vp, C.c_uint32]
     L.orc_sim_get_inv_masses.argtypes = [_vp, C.c_int, _f32p]
     L.orc_sim_set_solver_mode.argtypes = [_vp, C.c_uint32]
+    L.orc_sim_set_pair_cache.argtypes = [_vp, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
+                                         C.c_float, C.c_float, C.c_int]
+    L.orc_sim_active_pairs.restype = C.c_uint32
+    L.orc_sim_active_pairs.argtypes = [_vp, C.c_int, _vp, _vp]
     _LIBS[precision] = L
     return L
 
@@ -227,6 +231,28 @@ class OracleSim:
 
     def set_solver_iterations(self, it):
         self._L.orc_sim_set_iterations(self._h, it)
+
+    def set_pair_cache(self, on=True, final_fallback=True, quota_vt=None, quota_ee=None, margin_mode=1,
+                       safety_margin=0.005, horizon_scale=1.3, max_horizon=0.02, brute=False):
+        """Pair-cache self-collision (self_collision_algorithm 1); quotas default to the reference's
+        clamp(clamp(8 N, 8192, 262144) / N, 1, 8).  brute: candidates from a scan of every vertex
+        instead of the hash tables (the same lists)."""
+        budget = min(max(8 * self.n, 8192), 262144)
+        q = min(max(budget // max(self.n, 1), 1), 8)
+        self._L.orc_sim_set_pair_cache(self._h, int(on), int(final_fallback), q if quota_vt is None else quota_vt,
+                                       q if quota_ee is None else quota_ee, margin_mode, safety_margin,
+                                       horizon_scale, max_horizon, int(brute))
+
+    def active_pairs(self):
+        """(counters [V-T evaluated, E-E evaluated, V-T dropped, E-E dropped], V-T (n, 4), E-E (n, 4))."""
+        cnt = np.zeros(4, np.uint32)
+        nvt = self._L.orc_sim_active_pairs(self._h, 0, None, None)
+        nee = self._L.orc_sim_active_pairs(self._h, 1, None, None)
+        vt = np.zeros((nvt, 4), np.uint32)
+        ee = np.zeros((nee, 4), np.uint32)
+        self._L.orc_sim_active_pairs(self._h, 0, _ptr(vt), _ptr(cnt))
+        self._L.orc_sim_active_pairs(self._h, 1, _ptr(ee), None)
+        return cnt, vt, ee
 
     def set_solver_mode(self, mode):
         """0: colours; 1: fixed-point Jacobi distance pass (distance_atomic.wgsl)."""

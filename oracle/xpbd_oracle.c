@@ -1045,6 +1045,11 @@ typedef struct {
     uint32_t sdf_w, sdf_h, sdf_d, sdf_nb;
     uint32_t npins, *pin_v; v3 *pin_t; real *pin_w;  /* dynamic pin table (distinct vertices) */
     uint32_t solver_mode;                            /* 1: fixed-point Jacobi distance pass */
+    /* pair cache (self_collision_algorithm 1): vertex-major slots of 4 indices, counters */
+    int pair_cache, pair_final_fallback, pair_brute;
+    uint32_t pair_quota_vt, pair_quota_ee, pair_margin_mode;
+    real pair_safety, pair_hscale, pair_hmax, pair_dt_frame;
+    uint32_t *pair_vt, *pair_ee, pair_counters[4];
     int32_t *jac;                                    /* per-vertex sums {dx, dy, dz, count} */
 } orc_sim;
 
@@ -1080,6 +1085,18 @@ orc_sim *orc_sim_create(const float *pos, uint32_t nv, const uint32_t *edges, ui
     s->cell_size = m->cell_size; s->table_size = m->table_size;
     s->acc = (accum_t *)calloc(N, sizeof(accum_t));
     s->jac = (int32_t *)calloc(4 * N, sizeof(int32_t));
+    s->pair_vt = (uint32_t *)malloc(N * 8 * 4 * sizeof(uint32_t));
+    s->pair_ee = (uint32_t *)malloc(N * 8 * 4 * sizeof(uint32_t));
+    for (size_t k = 0; k < N * 8; k++) { s->pair_vt[4 * k] = 0xFFFFFFFFu; s->pair_ee[4 * k] = 0xFFFFFFFFu; }
+    {   /* simulation/mod.rs:583-597 */
+        uint64_t b = (uint64_t)nv * 8u;
+        uint32_t budget = (uint32_t)(b < 8192 ? 8192 : b > 262144 ? 262144 : b);
+        uint32_t q = nv ? budget / nv : 1;
+        s->pair_quota_vt = s->pair_quota_ee = q < 1 ? 1 : q > 8 ? 8 : q;
+    }
+    s->pair_final_fallback = 1; s->pair_margin_mode = 1;
+    s->pair_safety = (real)0.005f; s->pair_hscale = (real)1.3f; s->pair_hmax = (real)0.02f;
+    s->pair_dt_frame = (real)(1.0f / 60.0f);
     s->hkeys = (uint32_t *)malloc(N * 4); s->hsorted = (uint32_t *)malloc(N * 4);
     s->hstarts = (uint32_t *)malloc((size_t)(s->table_size + 1) * 4);
     size_t NE = m->ne ? m->ne : 1;
@@ -1099,7 +1116,7 @@ void orc_sim_free(orc_sim *s) {
     free(s->d_tc); free(s->d_cc); free(s->b_comp); free(s->cols); free(s->acc); free(s->tris); free(s->tri_bounds);
     free(s->hkeys); free(s->hstarts); free(s->hsorted); free(s->ekeys); free(s->estarts); free(s->esorted);
     free(s->sdf_tex); free(s->sdf_info); free(s->sdf_xf);
-    free(s->pin_v); free(s->pin_t); free(s->pin_w); free(s->jac);
+    free(s->pin_v); free(s->pin_t); free(s->pin_w); free(s->jac); free(s->pair_vt); free(s->pair_ee);
     free(s);
 }
 
@@ -2182,6 +2199,447 @@ static void k_sc_apply(orc_sim *s) {
     }
 }
 
+/* ---------------- pair cache (self_collision_algorithm 1) ---------------- */
+void orc_sim_set_pair_cache(orc_sim *s, int on, int final_fallback, uint32_t quota_vt, uint32_t quota_ee,
+                            uint32_t margin_mode, float safety, float hscale, float hmax, int brute) {
+    s->pair_cache = on; s->pair_final_fallback = final_fallback;
+    s->pair_quota_vt = quota_vt < 1 ? 1 : quota_vt > 8 ? 8 : quota_vt;
+    s->pair_quota_ee = quota_ee < 1 ? 1 : quota_ee > 8 ? 8 : quota_ee;
+    s->pair_margin_mode = margin_mode ? 1 : 0;
+    s->pair_safety = safety; s->pair_hscale = hscale; s->pair_hmax = hmax;
+    s->pair_brute = brute;
+}
+/* The held pairs in slot order and the counters of the last collect: returns the number of V-T
+ * (which 0) or E-E (which 1) pairs, written to out (4 indices each) when out is not NULL. */
+uint32_t orc_sim_active_pairs(const orc_sim *s, int which, uint32_t *out, uint32_t *counters) {
+    const uint32_t *src = which ? s->pair_ee : s->pair_vt;
+    uint32_t q = which ? s->pair_quota_ee : s->pair_quota_vt, n = 0;
+    for (size_t k = 0; k < (size_t)s->nv * q; k++) {
+        if (src[4 * k] == 0xFFFFFFFFu) continue;
+        if (out) memcpy(out + 4 * (size_t)n, src + 4 * k, 16);
+        n++;
+    }
+    if (counters) memcpy(counters, s->pair_counters, 16);
+    return n;
+}
+/* self_collision_collect_pairs.wgsl:122-156 (no a == b shortcut) */
+static int near_folded_pairs(const orc_mesh *m, uint32_t a, uint32_t b, real cur) {
+    uint32_t lo = m->th_off[a], hi = m->th_off[a + 1];
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2, val = m->th_idx[mid];
+        if (val == b) { real l0 = m->th_rest[mid]; return cur < l0 * (real)0.5f ? 0 : 1; }
+        if (val < b) lo = mid + 1; else hi = mid;
+    }
+    return 0;
+}
+static real pair_horizon(const orc_sim *s, uint32_t v) {
+    return rmin(len3(s->v[v]) * s->pair_dt_frame * s->pair_hscale, s->pair_hmax);
+}
+static int32_t iabs_wrap(int32_t d) { return d < 0 ? (int32_t)(0u - (uint32_t)d) : d; }
+/* can_ee_anchor_reach, collect_pairs.wgsl:250-279 */
+static int ee_anchor_reach(const orc_sim *s, uint32_t u, uint32_t w) {
+    const orc_mesh *m = s->m;
+    if (s->w[u] <= 0) return 0;
+    int32_t cu[3], cw[3];
+    cell_of(s->p[u], s->cell_size, cu); cell_of(s->p[w], s->cell_size, cw);
+    for (int a = 0; a < 3; a++) if (iabs_wrap((int32_t)((uint32_t)cu[a] - (uint32_t)cw[a])) > 1) return 0;
+    v3 d = sub(s->p[u], s->p[w]);
+    real dsq = dot(d, d);
+    real mf = (real)(float)s->pair_margin_mode;
+    real hu = pair_horizon(s, u) * mf, hw = pair_horizon(s, w) * mf;
+    real bound = (real)m->thick[u] + (real)m->thick[w] + ((real)m->local_len[u] + (real)m->local_len[w]) * (real)1.3f +
+                 s->pair_safety + hu + hw;
+    if (dsq > bound * bound) return 0;
+    if (m->island[u] == m->island[w] && s->exclude_neighbors)
+        if (near_folded_pairs(m, u, w, RSQRT(rmax(dsq, 0)))) return 0;
+    return 1;
+}
+static int smaller_anchor(const orc_sim *s, uint32_t u, uint32_t w, uint32_t cu, uint32_t cw) {
+    if (u > cu || (u == cu && w >= cw)) return 0;
+    return ee_anchor_reach(s, u, w);
+}
+typedef struct { real d[8]; uint32_t e[8][4]; uint32_t n; } topk_t;
+/* collect_pairs.wgsl:427-476 */
+static void topk_offer(topk_t *L, uint32_t quota, real dist, const uint32_t ent[4], uint32_t *eval, uint32_t *drop) {
+    for (uint32_t q = 0; q < L->n; q++)
+        if (L->e[q][1] == ent[1] && L->e[q][2] == ent[2] && L->e[q][3] == ent[3]) return;
+    (*eval)++;
+    uint32_t pos = L->n;
+    for (uint32_t q = 0; q < L->n; q++) {
+        const uint32_t *o = L->e[q];
+        int tie = dist == L->d[q] && (ent[1] < o[1] || (ent[1] == o[1] && (ent[2] < o[2] || (ent[2] == o[2] && ent[3] < o[3]))));
+        if (dist < L->d[q] || tie) { pos = q; break; }
+    }
+    if (pos >= quota) { (*drop)++; return; }
+    uint32_t up = L->n < quota - 1 ? L->n : quota - 1;
+    for (uint32_t q = up; q > pos; q--) { L->d[q] = L->d[q - 1]; memcpy(L->e[q], L->e[q - 1], 16); }
+    L->d[pos] = dist; memcpy(L->e[pos], ent, 16);
+    if (L->n < quota) L->n++;
+}
+/* self_collision_collect_pairs.wgsl:294-661.  The candidates j are the vertices of the 27 cells
+ * around vertex i, cell by cell (x outer, z inner), in ascending index within a cell: from the
+ * hash tables, or (pair_brute) from a scan of every vertex, which must give the same lists. */
+static void k_pair_collect(orc_sim *s) {
+    const orc_mesh *m = s->m;
+    const real EPS = (real)1e-7f;
+    const real cell = s->cell_size;
+    const real mf = (real)(float)s->pair_margin_mode;
+    uint32_t nv = s->nv;
+    memset(s->pair_counters, 0, sizeof(s->pair_counters));
+    uint32_t *cand = (uint32_t *)malloc((nv ? nv : 1) * 4);
+    for (uint32_t index = 0; index < nv; index++) {
+        int pinned_i = s->w[index] <= 0;
+        v3 p_i = s->p[index];
+        real thick_i = (real)m->thick[index], lli = (real)m->local_len[index];
+        int32_t base[3]; cell_of(p_i, cell, base);
+        topk_t VT, EE; VT.n = EE.n = 0;
+        uint32_t vt_eval = 0, vt_drop = 0, ee_eval = 0, ee_drop = 0;
+        real hi_eff = pair_horizon(s, index) * mf;
+        for (int gx = -1; gx <= 1; gx++)
+        for (int gy = -1; gy <= 1; gy++)
+        for (int gz = -1; gz <= 1; gz++) {
+            int32_t cc[3] = {(int32_t)((uint32_t)base[0] + (uint32_t)gx), (int32_t)((uint32_t)base[1] + (uint32_t)gy),
+                             (int32_t)((uint32_t)base[2] + (uint32_t)gz)};
+            uint32_t nc = 0;
+            if (s->pair_brute) {
+                for (uint32_t j = 0; j < nv; j++) cand[nc++] = j;
+            } else {
+                uint32_t h = hash_coords(cc, s->table_size);
+                for (uint32_t k = s->hstarts[h]; k < s->hstarts[h + 1]; k++) cand[nc++] = s->hsorted[k];
+            }
+            for (uint32_t kk = 0; kk < nc; kk++) {
+                uint32_t j = cand[kk];
+                if (j >= nv || j == index) continue;
+                v3 p_j = s->p[j];
+                int32_t jc[3]; cell_of(p_j, cell, jc);
+                if (jc[0] != cc[0] || jc[1] != cc[1] || jc[2] != cc[2]) continue;
+                int pinned_j = s->w[j] <= 0;
+                if (pinned_i && pinned_j) continue;
+                real hj = pair_horizon(s, j) * mf;
+                v3 dvv = sub(p_i, p_j);
+                real dsq = dot(dvv, dvv);
+                real eff = thick_i + (real)m->thick[j];
+                real llj = (real)m->local_len[j];
+                real broad = eff + (lli + llj) * (real)1.3f + s->pair_safety + hi_eff + hj;
+                if (!(dsq <= broad * broad)) continue;
+                int same_island = m->island[index] == m->island[j];
+                real dvv_len = RSQRT(rmax(dsq, 0));
+                if (same_island && s->exclude_neighbors && near_folded_pairs(m, index, j, dvv_len)) continue;
+                real check = eff + s->pair_safety + hi_eff + hj;
+                /* V-T */
+                real vt_bound = eff + llj * (real)1.3f + s->pair_safety + hi_eff + hj;
+                int near2 = s->exclude_neighbors && within_two_hops(m, index, j);
+                if (dsq <= vt_bound * vt_bound && !near2) {
+                    uint32_t ss = m->star_off[j], se = m->star_off[j + 1];
+                    uint32_t ns = se > ss ? (se - ss < 8 ? se - ss : 8) : 0;
+                    for (uint32_t st = 0; st < ns; st++) {
+                        uint32_t v0 = m->star_v0[ss + st], v1 = m->star_v1[ss + st];
+                        if (index == v0 || index == v1 || v0 >= nv || v1 >= nv) continue;
+                        uint32_t c0 = j, c1 = v0, c2 = v1;
+                        if (v0 < j && v0 < v1) { c0 = v0; c1 = v1; c2 = j; }
+                        else if (v1 < j && v1 < v0) { c0 = v1; c1 = j; c2 = v0; }
+                        cbary cb = closest_bary(p_i, s->p[c0], s->p[c1], s->p[c2]);
+                        v3 dvt = sub(p_i, cb.pt);
+                        real dsq_vt = dot(dvt, dvt);
+                        if (dsq_vt < check * check) {
+                            if (s->exclude_neighbors && (within_two_hops(m, index, v0) || within_two_hops(m, index, v1)))
+                                continue;
+                            uint32_t ent[4] = {index, c0, c1, c2};
+                            topk_offer(&VT, s->pair_quota_vt, dsq_vt, ent, &vt_eval, &vt_drop);
+                        }
+                    }
+                }
+                /* E-E (its bound equals the broad bound) */
+                uint32_t ai0 = m->adj_off[index], ai1 = m->adj_off[index + 1];
+                uint32_t aj0 = m->adj_off[j], aj1 = m->adj_off[j + 1];
+                if (!(ai0 < ai1 && aj0 < aj1)) continue;
+                uint32_t nei = ai1 - ai0 < 8 ? ai1 - ai0 : 8, nej = aj1 - aj0 < 8 ? aj1 - aj0 : 8;
+                for (uint32_t ei = 0; ei < nei; ei++) {
+                    uint32_t ui = m->adj_idx[ai0 + ei];
+                    if (ui >= nv) continue;
+                    v3 p_ui = s->p[ui];
+                    real llui = (real)m->local_len[ui];
+                    v3 dui = sub(p_ui, p_i);
+                    real len_sq_i = dot(dui, dui);
+                    real max_len_i = (lli + llui) * (real)1.5f;
+                    if (len_sq_i > max_len_i * max_len_i || len_sq_i <= (real)1e-12f) continue;
+                    for (uint32_t ej = 0; ej < nej; ej++) {
+                        uint32_t vj = m->adj_idx[aj0 + ej];
+                        if (vj >= nv) continue;
+                        if (ui == j || ui == vj || index == vj) continue;
+                        uint32_t c_j = j < vj ? j : vj, c_vj = j < vj ? vj : j;
+                        int smaller = smaller_anchor(s, index, vj, index, j) || smaller_anchor(s, ui, j, index, j) ||
+                                      smaller_anchor(s, ui, vj, index, j) || smaller_anchor(s, j, index, index, j) ||
+                                      smaller_anchor(s, j, ui, index, j) || smaller_anchor(s, vj, index, index, j) ||
+                                      smaller_anchor(s, vj, ui, index, j);
+                        if (pinned_i || smaller) continue;
+                        v3 p_vj = s->p[vj];
+                        real llvj = (real)m->local_len[vj];
+                        v3 djv = sub(p_vj, p_j);
+                        real len_sq_j = dot(djv, djv);
+                        real max_len_j = (llj + llvj) * (real)1.5f;
+                        if (len_sq_j > max_len_j * max_len_j || len_sq_j <= (real)1e-12f) continue;
+                        v3 p_cj = j < vj ? p_j : p_vj, p_cvj = j < vj ? p_vj : p_j;
+                        st2 st = closest_segs(p_i, p_ui, p_cj, p_cvj);
+                        v3 pt1 = add(p_i, mul(sub(p_ui, p_i), st.s));
+                        v3 pt2 = add(p_cj, mul(sub(p_cvj, p_cj), st.t));
+                        v3 dee = sub(pt1, pt2);
+                        real dsq_ee = dot(dee, dee);
+                        if (!(dsq_ee < check * check && dsq_ee > EPS * EPS)) continue;
+                        if (s->exclude_neighbors) {
+                            if (near_folded_pairs(m, index, vj, len3(sub(p_i, p_vj))) ||
+                                near_folded_pairs(m, ui, j, len3(sub(p_ui, p_j))) ||
+                                near_folded_pairs(m, ui, vj, len3(sub(p_ui, p_vj))))
+                                continue;
+                        }
+                        uint32_t ent[4] = {index, ui, c_j, c_vj};
+                        topk_offer(&EE, s->pair_quota_ee, dsq_ee, ent, &ee_eval, &ee_drop);
+                    }
+                }
+            }
+        }
+        for (uint32_t k = 0; k < s->pair_quota_vt; k++) {
+            uint32_t *o = s->pair_vt + 4 * ((size_t)index * s->pair_quota_vt + k);
+            if (k < VT.n) memcpy(o, VT.e[k], 16); else { o[0] = 0xFFFFFFFFu; o[1] = o[2] = o[3] = 0; }
+        }
+        for (uint32_t k = 0; k < s->pair_quota_ee; k++) {
+            uint32_t *o = s->pair_ee + 4 * ((size_t)index * s->pair_quota_ee + k);
+            if (k < EE.n) memcpy(o, EE.e[k], 16); else { o[0] = 0xFFFFFFFFu; o[1] = o[2] = o[3] = 0; }
+        }
+        s->pair_counters[0] += vt_eval; s->pair_counters[1] += ee_eval;
+        s->pair_counters[2] += vt_drop; s->pair_counters[3] += ee_drop;
+    }
+    free(cand);
+}
+/* self_collision_solve_vt.wgsl:227-414 (single layer) */
+static void k_pair_solve_vt(orc_sim *s) {
+    const orc_mesh *m = s->m;
+    const real EPS = (real)1e-7f;
+    uint32_t nv = s->nv;
+    for (size_t k = 0; k < (size_t)nv * s->pair_quota_vt; k++) {
+        const uint32_t *pr = s->pair_vt + 4 * k;
+        if (pr[0] == 0xFFFFFFFFu) continue;
+        uint32_t index = pr[0], j = pr[1], v0 = pr[2], v1 = pr[3];
+        if (index >= nv || j >= nv || v0 >= nv || v1 >= nv) continue;
+        if (index == j || index == v0 || index == v1 || v0 == v1 || j == v0 || j == v1) continue;
+        int pinned_i = s->w[index] <= 0;
+        v3 p_i = s->p[index], xo_i = s->x[index];
+        v3 p_j = s->p[j], p_v0 = s->p[v0], p_v1 = s->p[v1];
+        v3 pj_old = s->x[j], p_v0o = s->x[v0], p_v1o = s->x[v1];
+        real eff = (real)m->thick[index] + (real)m->thick[j];
+        v3 tri_move = dvs(add(add(sub(p_j, pj_old), sub(p_v0, p_v0o)), sub(p_v1, p_v1o)), (real)3.0f);
+        real wi = s->w[index], wj = s->w[j], w0 = s->w[v0], w1 = s->w[v1];
+        int tri_pin = wj <= 0 || w0 <= 0 || w1 <= 0;
+        v3 rel_old = add(xo_i, tri_move);
+        real ht = 0, hu = 0, hv = 0;
+        if (seg_tri(rel_old, p_i, p_j, p_v0, p_v1, &ht, &hu, &hv)) {
+            v3 move = sub(p_i, rel_old);
+            v3 hit_pt = add(rel_old, mul(move, ht));
+            v3 tcr = cross(sub(p_v0, p_j), sub(p_v1, p_j));
+            real tl = len3(tcr);
+            v3 tn = tl > (real)1e-12f ? dvs(tcr, tl) : mk(0, 1, 0);
+            real h_old_raw = dot(sub(rel_old, p_j), tn);
+            real side = (real)fabs((double)h_old_raw) > (real)1e-6f ? h_old_raw : dot(neg(move), tn);
+            v3 n_side = side >= 0 ? tn : neg(tn);
+            real h_old = rmax(dot(sub(rel_old, hit_pt), n_side), 0);
+            real h_safe = rmax(h_old, eff * (real)0.25f);
+            v3 total = sub(add(hit_pt, mul(n_side, h_safe)), p_i);
+            real pen_soft = rmax(0, eff - h_safe);
+            v3 bc = mk(rclamp(1 - hu - hv, 0, 1), rclamp(hu, 0, 1), rclamp(hv, 0, 1));
+            if (tri_pin) {
+                if (!pinned_i) add_ccd(&s->acc[index], add(total, mul(n_side, pen_soft)));
+            } else if (pinned_i) {
+                real wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+                if (wt > EPS) {
+                    v3 full = add(total, mul(n_side, pen_soft));
+                    if (wj > 0) add_ccd(&s->acc[j], mul(neg(full), bc.x * wj / wt));
+                    if (w0 > 0) add_ccd(&s->acc[v0], mul(neg(full), bc.y * w0 / wt));
+                    if (w1 > 0) add_ccd(&s->acc[v1], mul(neg(full), bc.z * w1 / wt));
+                }
+            } else {
+                real wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+                real wtot = wi + wt;
+                if (wtot > EPS) {
+                    add_ccd(&s->acc[index], mul(total, wi / wtot));
+                    if (wj > 0) add_ccd(&s->acc[j], mul(neg(total), bc.x * wj / wtot));
+                    if (w0 > 0) add_ccd(&s->acc[v0], mul(neg(total), bc.y * w0 / wtot));
+                    if (w1 > 0) add_ccd(&s->acc[v1], mul(neg(total), bc.z * w1 / wtot));
+                    if (pen_soft > EPS) {
+                        v3 tp = mul(n_side, pen_soft * (real)0.6f);
+                        add_disp(&s->acc[index], mul(tp, wi / wtot));
+                        if (wj > 0) add_disp(&s->acc[j], mul(neg(tp), bc.x * wj / wtot));
+                        if (w0 > 0) add_disp(&s->acc[v0], mul(neg(tp), bc.y * w0 / wtot));
+                        if (w1 > 0) add_disp(&s->acc[v1], mul(neg(tp), bc.z * w1 / wtot));
+                    }
+                }
+            }
+            continue;
+        }
+        cbary cb = closest_bary(p_i, p_j, p_v0, p_v1);
+        v3 dvt = sub(p_i, cb.pt);
+        real dist = len3(dvt);
+        if (pinned_i || !(dist < eff) || !(dist > EPS)) continue;
+        v3 nvt = dvs(dvt, dist);
+        real pen = eff - dist;
+        if (tri_pin) { add_pin_disp(&s->acc[index], add(mul(nvt, pen), mul(tri_move, (real)0.5f))); continue; }
+        v3 bc = cb.bary;
+        real wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+        real wtot = wi + wt;
+        if (wtot > EPS) {
+            v3 tp = mul(nvt, pen * (real)0.6f);
+            add_disp(&s->acc[index], mul(tp, wi / wtot));
+            if (wj > 0) add_disp(&s->acc[j], mul(neg(tp), bc.x * wj / wtot));
+            if (w0 > 0) add_disp(&s->acc[v0], mul(neg(tp), bc.y * w0 / wtot));
+            if (w1 > 0) add_disp(&s->acc[v1], mul(neg(tp), bc.z * w1 / wtot));
+        }
+    }
+}
+/* self_collision_solve_ee.wgsl:231-589 (single layer) */
+static void k_pair_solve_ee(orc_sim *s) {
+    const orc_mesh *m = s->m;
+    const real EPS = (real)1e-7f;
+    uint32_t nv = s->nv;
+    for (size_t k = 0; k < (size_t)nv * s->pair_quota_ee; k++) {
+        const uint32_t *pr = s->pair_ee + 4 * k;
+        if (pr[0] == 0xFFFFFFFFu) continue;
+        uint32_t u0 = pr[0], u1 = pr[1], j = pr[2], vj = pr[3];
+        if (u0 >= nv || u1 >= nv || j >= nv || vj >= nv) continue;
+        if (u0 == u1 || j == vj || u0 == j || u0 == vj || u1 == j || u1 == vj) continue;
+        real wu0 = s->w[u0], wu1 = s->w[u1], wj = s->w[j], wvj = s->w[vj];
+        int a_has_pin = wu0 <= 0 || wu1 <= 0, b_has_pin = wj <= 0 || wvj <= 0;
+        if (a_has_pin && b_has_pin) continue;
+        v3 p_u0 = s->p[u0], p_u1 = s->p[u1], p_j = s->p[j], p_vj = s->p[vj];
+        v3 p_u0o = s->x[u0], p_u1o = s->x[u1], p_jo = s->x[j], p_vjo = s->x[vj];
+        real th_u0 = m->thick[u0], th_u1 = m->thick[u1], th_j = m->thick[j], th_vj = m->thick[vj];
+        st2 st1c = closest_segs(p_u0, p_u1, p_j, p_vj);
+        v3 d1c = sub(vmix(p_u0, p_u1, st1c.s), vmix(p_j, p_vj, st1c.t));
+        real dsqc = dot(d1c, d1c);
+        real effc = rmix(th_u0, th_u1, st1c.s) + rmix(th_j, th_vj, st1c.t);
+        real max_eff = rmax(th_u0, th_u1) + rmax(th_j, th_vj);
+        v3 du0 = sub(p_u0, p_u0o), du1 = sub(p_u1, p_u1o), dj = sub(p_j, p_jo), dvj = sub(p_vj, p_vjo);
+        v3 va = mul(add(du0, du1), (real)0.5f), vb = mul(add(dj, dvj), (real)0.5f);
+        real rel = len3(sub(va, vb)) + (real)0.5f * (len3(sub(du1, du0)) + len3(sub(dvj, dj)));
+        real reach = max_eff + rel;
+        if (dsqc > reach * reach) continue;
+        v3 d0g = sub(vmix(p_u0o, p_u1o, st1c.s), vmix(p_jo, p_vjo, st1c.t));
+        int done = 0;
+        if ((rel > effc * (real)0.5f) || (dot(d0g, d1c) <= 0)) {
+            real min_sq = (real)1e20f, best_tau = 0, half_span = (real)0.125f;
+            st2 min_st = {0, 0};
+            real prev_vol = seg_volume(p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj, 0);
+            real zc = -1;
+            for (uint32_t kk = 0; kk <= 8; kk++) {
+                real tk = (real)(float)kk * (real)0.125f;
+                st2 stk;
+                real dk = seg_dist_sq(p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj, tk, &stk);
+                if (dk < min_sq) { min_sq = dk; best_tau = tk; min_st = stk; }
+                if (kk > 0) {
+                    real vol = seg_volume(p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj, tk);
+                    if (prev_vol * vol <= 0 && zc < 0) zc = tk - (real)0.0625f;
+                    prev_vol = vol;
+                }
+            }
+            if (zc >= 0) {
+                st2 stz;
+                real dzs = seg_dist_sq(p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj, zc, &stz);
+                if (dzs < min_sq) { min_sq = dzs; best_tau = zc; half_span = (real)0.0625f; min_st = stz; }
+            }
+            real tlo = rmax(0, best_tau - half_span), thi = rmin(1, best_tau + half_span);
+            real dsq_star = min_sq;
+            st2 st_star = min_st;
+            for (int it = 0; it < 6; it++) {
+                real m1 = tlo + (thi - tlo) * (real)(1.0f / 3.0f);
+                real m2 = thi - (thi - tlo) * (real)(1.0f / 3.0f);
+                st2 s1, s2;
+                real e1 = seg_dist_sq(p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj, m1, &s1);
+                real e2 = seg_dist_sq(p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj, m2, &s2);
+                if (e1 < dsq_star) { dsq_star = e1; st_star = s1; }
+                if (e2 < dsq_star) { dsq_star = e2; st_star = s2; }
+                if (e1 < e2) thi = m2; else tlo = m1;
+            }
+            real ss = st_star.s, ts = st_star.t;
+            real eff_star = rmix(th_u0, th_u1, ss) + rmix(th_j, th_vj, ts);
+            if (dsq_star >= eff_star * eff_star) continue;
+            st2 st0 = closest_segs(p_u0o, p_u1o, p_jo, p_vjo);
+            v3 delta0 = sub(vmix(p_u0o, p_u1o, st0.s), vmix(p_jo, p_vjo, st0.t));
+            real dist0 = len3(delta0);
+            if (dist0 > EPS) {
+                v3 n0 = dvs(delta0, dist0);
+                real h_end = dot(sub(vmix(p_u0, p_u1, ss), vmix(p_j, p_vj, ts)), n0);
+                int crossover = (h_end <= 0) || (dot(d1c, n0) <= 0);
+                if (h_end < eff_star && crossover) {
+                    real pen = eff_star - h_end;
+                    int a_all = wu0 == 0 && wu1 == 0, b_all = wj == 0 && wvj == 0;
+                    int cnt_pin = !(a_all || b_all);
+                    if (b_has_pin) {
+                        v3 em = mul(add(sub(p_j, p_jo), sub(p_vj, p_vjo)), (real)0.5f);
+                        real a = 1 - ss, b = ss, den = a * a * wu0 + b * b * wu1;
+                        if (den > EPS) {
+                            v3 pb = add(mul(n0, pen), mul(em, (real)0.5f));
+                            if (wu0 > 0) add_ccd_ee(&s->acc[u0], mul(pb, rmin((a * wu0) / den, 2)), cnt_pin);
+                            if (wu1 > 0) add_ccd_ee(&s->acc[u1], mul(pb, rmin((b * wu1) / den, 2)), cnt_pin);
+                        }
+                    } else if (a_has_pin) {
+                        v3 em = mul(add(sub(p_u0, p_u0o), sub(p_u1, p_u1o)), (real)0.5f);
+                        real a = 1 - ts, b = ts, den = a * a * wj + b * b * wvj;
+                        if (den > EPS) {
+                            v3 pb = add(mul(neg(n0), pen), mul(em, (real)0.5f));
+                            if (wj > 0) add_ccd_ee(&s->acc[j], mul(pb, rmin((a * wj) / den, 2)), cnt_pin);
+                            if (wvj > 0) add_ccd_ee(&s->acc[vj], mul(pb, rmin((b * wvj) / den, 2)), cnt_pin);
+                        }
+                    } else {
+                        real we1 = (1 - ss) * (1 - ss) * wu0 + ss * ss * wu1;
+                        real we2 = (1 - ts) * (1 - ts) * wj + ts * ts * wvj;
+                        real wt = we1 + we2;
+                        if (wt > EPS) {
+                            v3 tot = mul(n0, pen);
+                            if (wu0 > 0) add_ccd_ee(&s->acc[u0], mul(tot, (1 - ss) * wu0 / wt), 1);
+                            if (wu1 > 0) add_ccd_ee(&s->acc[u1], mul(tot, ss * wu1 / wt), 1);
+                            if (wj > 0) add_ccd_ee(&s->acc[j], mul(neg(tot), (1 - ts) * wj / wt), 1);
+                            if (wvj > 0) add_ccd_ee(&s->acc[vj], mul(neg(tot), ts * wvj / wt), 1);
+                        }
+                    }
+                    done = 1;
+                }
+            }
+        }
+        if (done) continue;
+        if (dsqc >= effc * effc || dsqc <= EPS * EPS) continue;
+        real dist = RSQRT(dsqc);
+        v3 n = dvs(d1c, dist);
+        real pen = effc - dist;
+        real sp = st1c.s, tp = st1c.t;
+        if (b_has_pin) {
+            v3 em = mul(add(sub(p_j, p_jo), sub(p_vj, p_vjo)), (real)0.5f);
+            real a = 1 - sp, b = sp, den = a * a * wu0 + b * b * wu1;
+            if (den > EPS) {
+                v3 pb = add(mul(n, pen), mul(em, (real)0.5f));
+                if (wu0 > 0) add_pin_disp(&s->acc[u0], mul(pb, rmin((a * wu0) / den, 2)));
+                if (wu1 > 0) add_pin_disp(&s->acc[u1], mul(pb, rmin((b * wu1) / den, 2)));
+            }
+        } else if (a_has_pin) {
+            v3 em = mul(add(sub(p_u0, p_u0o), sub(p_u1, p_u1o)), (real)0.5f);
+            real a = 1 - tp, b = tp, den = a * a * wj + b * b * wvj;
+            if (den > EPS) {
+                v3 pb = add(mul(neg(n), pen), mul(em, (real)0.5f));
+                if (wj > 0) add_pin_disp(&s->acc[j], mul(pb, rmin((a * wj) / den, 2)));
+                if (wvj > 0) add_pin_disp(&s->acc[vj], mul(pb, rmin((b * wvj) / den, 2)));
+            }
+        } else {
+            real we1 = (1 - sp) * (1 - sp) * wu0 + sp * sp * wu1;
+            real we2 = (1 - tp) * (1 - tp) * wj + tp * tp * wvj;
+            real wt = we1 + we2;
+            if (wt > EPS) {
+                v3 td = mul(n, pen * (real)0.6f);
+                if (wu0 > 0) add_disp(&s->acc[u0], mul(td, (1 - sp) * wu0 / wt));
+                if (wu1 > 0) add_disp(&s->acc[u1], mul(td, sp * wu1 / wt));
+                if (wj > 0) add_disp(&s->acc[j], mul(neg(td), (1 - tp) * wj / wt));
+                if (wvj > 0) add_disp(&s->acc[vj], mul(neg(td), tp * wvj / wt));
+            }
+        }
+    }
+}
+
 static void self_collision_pass(orc_sim *s) {
     const orc_mesh *m = s->m;
     hash_build(s->p, s->nv, s->cell_size, s->table_size, s->hkeys, s->hstarts, s->hsorted);
@@ -2199,10 +2657,25 @@ static void self_collision_pass(orc_sim *s) {
 /* one frame: dispatch.rs:7-354 with adaptive substeps off, decoupled colliders,
  * self-collision mode 0 / algorithm 0, no sewing; the pins run at the start of every
  * iteration and once after the collider (dispatch.rs:176-181, 238-250) */
+/* The outer self-collision pass of a substep (dispatch.rs:1554-1645): the pair cache collects on
+ * the frame's first substep and solves the held pairs on the others; with the final fallback the
+ * last substep takes the direct path. */
+static void self_collision_outer(orc_sim *s, int first, int last) {
+    if (!s->pair_cache || (s->pair_final_fallback && last)) { self_collision_pass(s); return; }
+    if (first) {
+        hash_build(s->p, s->nv, s->cell_size, s->table_size, s->hkeys, s->hstarts, s->hsorted);
+        k_pair_collect(s);
+    }
+    k_pair_solve_vt(s);
+    k_pair_solve_ee(s);
+    k_sc_apply(s);
+}
+
 static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
     if (s->nv == 0) return;
     real dt_sub = dt / (real)substeps;
     const orc_mesh *m = s->m;
+    s->pair_dt_frame = dt;
     for (uint32_t sub = 0; sub < substeps; sub++) {
         memset(s->lam_d, 0, (m->nd ? m->nd : 1) * sizeof(real));
         memset(s->lam_b, 0, (m->nb ? m->nb : 1) * sizeof(real));
@@ -2217,7 +2690,7 @@ static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
         }
         if (s->ncols > 0 || s->ntris > 0 || s->sdf_nb > 0) k_collision(s, dt_sub);
         k_pins(s, dt_sub);
-        if (s->self_collision) self_collision_pass(s);
+        if (s->self_collision) self_collision_outer(s, sub == 0, sub + 1 == substeps);
         k_update_vel(s, dt_sub);
     }
 }
@@ -2248,7 +2721,8 @@ void orc_sim_substep_parts(orc_sim *s, float dt_sub, int do_collide, int do_self
     }
     if (do_collide && (s->ncols > 0 || s->ntris > 0 || s->sdf_nb > 0)) k_collision(s, dt);
     k_pins(s, dt);
-    if (do_self && s->self_collision) self_collision_pass(s);
+    s->pair_dt_frame = dt;
+    if (do_self && s->self_collision) self_collision_outer(s, 1, 1);
     if (do_update) k_update_vel(s, dt);
 }
 

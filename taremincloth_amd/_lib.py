@@ -21,7 +21,10 @@ EXPORTS = [
     "tc_sim_step_async", "tc_sim_fetch_positions", "tc_sim_step_buffered", "tc_sim_fetch_buffered_positions",
     "tc_sim_get_buffered_frame_count", "tc_sim_clear_frame_buffer", "tc_sim_reset", "tc_sim_set_pins", "tc_sim_release_pins",
     "tc_sim_clear_pins", "tc_sim_pin_count", "tc_sim_get_pins", "tc_sim_set_vertex_inv_masses",
-    "tc_sim_get_vertex_inv_masses", "tc_sim_set_solver_mode", "tc_sim_get_solver_mode", "tc_sim_add_sphere",
+    "tc_sim_get_vertex_inv_masses", "tc_sim_set_solver_mode", "tc_sim_get_solver_mode",
+    "tc_sim_set_enable_pair_cache", "tc_sim_get_enable_pair_cache", "tc_sim_set_self_collision_algorithm",
+    "tc_sim_get_self_collision_algorithm", "tc_sim_set_pair_cache_options", "tc_sim_set_enable_pair_cache_final_fallback",
+    "tc_sim_pair_cache_state", "tc_sim_get_active_pairs", "tc_sim_add_sphere",
     "tc_sim_add_capsule", "tc_sim_add_plane", "tc_sim_clear_colliders", "tc_sim_set_mesh_triangles",
     "tc_sim_set_gravity", "tc_sim_set_damping", "tc_sim_set_damping_all", "tc_sim_set_solver_iterations",
     "tc_sim_set_enable_self_collision", "tc_sim_set_self_collision_options", "tc_sim_set_stiffness_all",
@@ -109,6 +112,15 @@ def load() -> C.CDLL:
         "tc_sim_get_vertex_inv_masses": (_i, [_vp, _vp, C.c_size_t]),
         "tc_sim_set_solver_mode": (_i, [_vp, _u32]),
         "tc_sim_get_solver_mode": (_i, [_vp, C.POINTER(_u32)]),
+        "tc_sim_set_enable_pair_cache": (_i, [_vp, _i]),
+        "tc_sim_get_enable_pair_cache": (_i, [_vp, C.POINTER(_i)]),
+        "tc_sim_set_self_collision_algorithm": (_i, [_vp, _u32]),
+        "tc_sim_get_self_collision_algorithm": (_i, [_vp, C.POINTER(_u32)]),
+        "tc_sim_set_pair_cache_options": (_i, [_vp, _u32, _u32, _u32, _f32, _f32, _f32]),
+        "tc_sim_set_enable_pair_cache_final_fallback": (_i, [_vp, _i]),
+        "tc_sim_pair_cache_state": (_i, [_vp, C.POINTER(C.c_double)]),
+        "tc_sim_get_active_pairs": (_i, [_vp, _vp, _vp, C.c_size_t, C.POINTER(C.c_size_t), _vp, C.c_size_t,
+                                         C.POINTER(C.c_size_t)]),
         "tc_sim_add_sphere": (_i, [_vp, _vp, _f32, _f32, _f32]),
         "tc_sim_add_capsule": (_i, [_vp, _vp, _vp, _f32, _f32, _f32]),
         "tc_sim_add_plane": (_i, [_vp, _vp, _vp, _f32, _f32]),

@@ -122,4 +122,29 @@ hipError_t launch_sc_solve_ee(hipStream_t s, const float4 *P, const float4 *X, c
 hipError_t launch_sc_apply(hipStream_t s, float4 *P, const float *local_len, Accum *acc, const SelfColArgs &a,
                            uint32_t block);
 
+// Pair cache (self_collision_algorithm 1).  Slots are vertex-major: vertex i owns
+// [i * quota, (i + 1) * quota) of each list; an unused slot has 0xFFFFFFFF in its first word.
+// V-T slot: (vertex, triangle c0, c1, c2 with c0 the smallest index); E-E slot: (vertex, its
+// neighbour, min(j, vj), max(j, vj)).  counters: [V-T evaluated, E-E evaluated, V-T dropped,
+// E-E dropped] (the caller clears them before a collect).
+struct PairArgs {
+    float cell;
+    uint32_t table, nv;
+    float safety_margin;
+    int exclude_neighbors;
+    float mode_f;  // 1 with the velocity horizon (margin mode 1), 0 without
+    float dt_frame, horizon_scale, max_horizon;
+    uint32_t quota_vt, quota_ee;  // 1..8
+};
+struct PairTopo {
+    const uint32_t *adj_off, *adj_idx, *island;
+};
+hipError_t launch_pair_collect(hipStream_t s, const float4 *P, const float4 *X, const float4 *V, const SelfColTopo &T,
+                               const PairTopo &PT, const HashBufs &vh, uint4 *vt, uint4 *ee, uint32_t *counters,
+                               const PairArgs &a);
+hipError_t launch_pair_solve_vt(hipStream_t s, const float4 *P, const float4 *X, const uint4 *vt, uint32_t slots,
+                                Accum *acc, uint32_t nv);
+hipError_t launch_pair_solve_ee(hipStream_t s, const float4 *P, const float4 *X, const uint4 *ee, uint32_t slots,
+                                Accum *acc, uint32_t nv);
+
 }  // namespace tc

@@ -907,4 +907,501 @@ hipError_t launch_sc_apply(hipStream_t s, float4 *P, const float *local_len, Acc
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Pair cache (self_collision_algorithm 1): a broadphase that keeps, per vertex, the `quota`
+// closest V-T triples and E-E quadruples (self_collision_collect_pairs.wgsl:294-661), and
+// narrowphase kernels that resolve the cached pairs with the current positions
+// (self_collision_solve_vt.wgsl:227-414, self_collision_solve_ee.wgsl:231-589).  The slots are
+// vertex-major (vertex i owns slots [i * quota, (i + 1) * quota)); unused slots hold the sentinel
+// 0xFFFFFFFF in their first word.  One thread per vertex walks the 27 cells in the reference's
+// order (x outer, z inner; buckets in ascending index), so the evaluation counters match too; the
+// per-thread top-K lists are unrolled so they stay in registers.
+
+constexpr uint32_t PAIR_SENTINEL = 0xFFFFFFFFu;
+
+// self_collision_collect_pairs.wgsl:122-156: compares the distance itself (not its square)
+__device__ __forceinline__ bool near_folded_dist(const SelfColTopo &t, uint32_t a, uint32_t b, float cur) {
+    const int32_t s = two_hop_slot(t, a, b);
+    if (s < 0) return false;
+    return !(cur < t.th_rest[s] * 0.5f);
+}
+__device__ __forceinline__ int32_t iabs_wrap(int32_t d) { return d < 0 ? int32_t(0u - uint32_t(d)) : d; }
+__device__ __forceinline__ int32_t isub_wrap(int32_t a, int32_t b) { return int32_t(uint32_t(a) - uint32_t(b)); }
+__device__ __forceinline__ float pair_horizon(const float4 *__restrict__ V, uint32_t v, const PairArgs &A) {
+    const float4 u = V[v];
+    return fminf(length(mk3(u.x, u.y, u.z)) * A.dt_frame * A.horizon_scale, A.max_horizon);
+}
+
+// can_ee_anchor_reach (self_collision_collect_pairs.wgsl:250-279)
+__device__ __forceinline__ bool ee_anchor_reach(const float4 *__restrict__ P, const float4 *__restrict__ X,
+                                                const float4 *__restrict__ V, const SelfColTopo &T,
+                                                const PairTopo &PT, const PairArgs &A, uint32_t u, uint32_t w) {
+    const float4 qu = P[u], qw = P[w];
+    if (qu.w <= 0.0f) return false;
+    const f3 pu = xyz(qu), pw = xyz(qw);
+    const i3 cu = cell_of(pu, A.cell), cw = cell_of(pw, A.cell);
+    if (iabs_wrap(isub_wrap(cu.x, cw.x)) > 1 || iabs_wrap(isub_wrap(cu.y, cw.y)) > 1 ||
+        iabs_wrap(isub_wrap(cu.z, cw.z)) > 1)
+        return false;
+    const f3 d = pu - pw;
+    const float dsq = dot(d, d);
+    const float hu = pair_horizon(V, u, A) * A.mode_f, hw = pair_horizon(V, w, A) * A.mode_f;
+    const float bound = X[u].w + X[w].w + (T.local_len[u] + T.local_len[w]) * 1.3f + A.safety_margin + hu + hw;
+    if (dsq > bound * bound) return false;
+    if (PT.island[u] == PT.island[w] && A.exclude_neighbors) {
+        if (near_folded_dist(T, u, w, sqrtf(fmaxf(dsq, 0.0f)))) return false;
+    }
+    return true;
+}
+__device__ __forceinline__ bool smaller_anchor(const float4 *__restrict__ P, const float4 *__restrict__ X,
+                                               const float4 *__restrict__ V, const SelfColTopo &T,
+                                               const PairTopo &PT, const PairArgs &A, uint32_t u, uint32_t w,
+                                               uint32_t cu, uint32_t cw) {
+    if (u > cu || (u == cu && w >= cw)) return false;
+    return ee_anchor_reach(P, X, V, T, PT, A, u, w);
+}
+
+// Vertex-local top-K list ordered by (distance^2, key), keys compared lexicographically.
+struct TopK {
+    float d[8];
+    uint4 e[8];
+    uint32_t n;
+};
+// Inserts (dist, entry) unless an entry with the same key (words y, z, w) is held; counts the
+// evaluation and, when the list is full of closer entries, the drop (collect_pairs.wgsl:427-476).
+__device__ __forceinline__ void topk_offer(TopK &L, uint32_t quota, float dist, uint4 ent, uint32_t &eval,
+                                           uint32_t &drop) {
+    bool dup = false;
+#pragma unroll
+    for (uint32_t q = 0; q < 8u; ++q)
+        if (q < L.n && L.e[q].y == ent.y && L.e[q].z == ent.z && L.e[q].w == ent.w) dup = true;
+    if (dup) return;
+    ++eval;
+    uint32_t pos = L.n;
+#pragma unroll
+    for (int q = 7; q >= 0; --q) {
+        if (uint32_t(q) < L.n) {
+            const uint4 o = L.e[q];
+            const bool tie_lt = dist == L.d[q] &&
+                                (ent.y < o.y || (ent.y == o.y && (ent.z < o.z || (ent.z == o.z && ent.w < o.w))));
+            if (dist < L.d[q] || tie_lt) pos = uint32_t(q);  // the first such slot wins: scanned from the top
+        }
+    }
+    if (pos >= quota) { ++drop; return; }
+    const uint32_t up = min(L.n, quota - 1u);
+#pragma unroll
+    for (int q = 7; q >= 1; --q) {
+        if (uint32_t(q) <= up && uint32_t(q) > pos) { L.d[q] = L.d[q - 1]; L.e[q] = L.e[q - 1]; }
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < 8u; ++q) {
+        if (q == pos) { L.d[q] = dist; L.e[q] = ent; }
+    }
+    if (L.n < quota) ++L.n;
+}
+
+// One list per launch (VT: the V-T triples, else the E-E quadruples): both walk the same cells
+// and candidates, and one list's state per thread keeps the top-K in registers.
+template <bool VT>
+__global__ void __launch_bounds__(64) k_pair_collect(const float4 *__restrict__ P, const float4 *__restrict__ X,
+                                                     const float4 *__restrict__ V, SelfColTopo T, PairTopo PT,
+                                                     const uint32_t *__restrict__ starts,
+                                                     const uint32_t *__restrict__ sorted, uint4 *__restrict__ out,
+                                                     uint32_t *__restrict__ counters, PairArgs A) {
+    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= A.nv) return;
+    const float EPS = 1e-7f;
+    const float4 qi = P[index];
+    const bool pinned_i = qi.w <= 0.0f;
+    const f3 p_i = xyz(qi);
+    const float thick_i = X[index].w, lli = T.local_len[index];
+    const i3 base = cell_of(p_i, A.cell);
+    TopK L;
+    L.n = 0;
+    uint32_t eval = 0, drop = 0;
+    const uint32_t quota = VT ? A.quota_vt : A.quota_ee;
+    const float hi_eff = pair_horizon(V, index, A) * A.mode_f;
+    const uint32_t island_i = PT.island[index];
+    for (int gx = -1; gx <= 1; ++gx)
+        for (int gy = -1; gy <= 1; ++gy)
+            for (int gz = -1; gz <= 1; ++gz) {
+                const i3 cc = {int32_t(uint32_t(base.x) + uint32_t(gx)), int32_t(uint32_t(base.y) + uint32_t(gy)),
+                               int32_t(uint32_t(base.z) + uint32_t(gz))};
+                const uint32_t h = hash_cell(cc, A.table);
+                const uint32_t start = starts[h], end = starts[h + 1];
+                for (uint32_t k = start; k < end; ++k) {
+                    const uint32_t j = sorted[k];
+                    if (j >= A.nv || j == index) continue;
+                    const float4 qj = P[j];
+                    const f3 p_j = xyz(qj);
+                    const i3 jc = cell_of(p_j, A.cell);
+                    if (!(jc.x == cc.x && jc.y == cc.y && jc.z == cc.z)) continue;
+                    const bool pinned_j = qj.w <= 0.0f;
+                    if (pinned_i && pinned_j) continue;
+                    const float hj = pair_horizon(V, j, A) * A.mode_f;
+                    const f3 dvv = p_i - p_j;
+                    const float dsq = dot(dvv, dvv);
+                    const float eff = thick_i + X[j].w;
+                    const float llj = T.local_len[j];
+                    const float broad = eff + (lli + llj) * 1.3f + A.safety_margin + hi_eff + hj;
+                    if (!(dsq <= broad * broad)) continue;
+                    const bool same_island = island_i == PT.island[j];
+                    const float dvv_len = sqrtf(fmaxf(dsq, 0.0f));
+                    if (same_island && A.exclude_neighbors && near_folded_dist(T, index, j, dvv_len)) continue;
+                    const float check = eff + A.safety_margin + hi_eff + hj;
+                    if (VT) {  // 1. V-T triples over the first 8 star triangles of j
+                    const float vt_bound = eff + llj * 1.3f + A.safety_margin + hi_eff + hj;
+                    const bool near2 = A.exclude_neighbors && within_two_hops(T, index, j);
+                    if (dsq <= vt_bound * vt_bound && !near2) {
+                        const uint32_t ss = T.star_off[j], se = T.star_off[j + 1];
+                        const uint32_t ns = se > ss ? min(se - ss, 8u) : 0u;
+                        for (uint32_t st = 0; st < ns; ++st) {
+                            const uint2 pr = T.star[ss + st];
+                            const uint32_t v0 = pr.x, v1 = pr.y;
+                            if (index == v0 || index == v1 || v0 >= A.nv || v1 >= A.nv) continue;
+                            uint32_t c0 = j, c1 = v0, c2 = v1;  // rotate the smallest index first (orientation kept)
+                            if (v0 < j && v0 < v1) { c0 = v0; c1 = v1; c2 = j; }
+                            else if (v1 < j && v1 < v0) { c0 = v1; c1 = j; c2 = v0; }
+                            f3 bc;
+                            const f3 cp = closest_bary(p_i, xyz(P[c0]), xyz(P[c1]), xyz(P[c2]), bc);
+                            const f3 dvt = p_i - cp;
+                            const float dsq_vt = dot(dvt, dvt);
+                            if (dsq_vt < check * check) {
+                                if (A.exclude_neighbors &&
+                                    (within_two_hops(T, index, v0) || within_two_hops(T, index, v1)))
+                                    continue;
+                                topk_offer(L, quota, dsq_vt, make_uint4(index, c0, c1, c2), eval, drop);
+                            }
+                        }
+                    }
+                    continue;
+                    }
+                    // 2. E-E quadruples over the first 8 neighbours of index and of j (the E-E bound
+                    // equals the broad bound)
+                    const uint32_t ai0 = PT.adj_off[index], ai1 = PT.adj_off[index + 1];
+                    const uint32_t aj0 = PT.adj_off[j], aj1 = PT.adj_off[j + 1];
+                    if (!(ai0 < ai1 && aj0 < aj1)) continue;
+                    const uint32_t nei = min(ai1 - ai0, 8u), nej = min(aj1 - aj0, 8u);
+                    for (uint32_t ei = 0; ei < nei; ++ei) {
+                        const uint32_t ui = PT.adj_idx[ai0 + ei];
+                        if (ui >= A.nv) continue;
+                        const f3 p_ui = xyz(P[ui]);
+                        const float llui = T.local_len[ui];
+                        const f3 dui = p_ui - p_i;
+                        const float len_sq_i = dot(dui, dui);
+                        const float max_len_i = (lli + llui) * 1.5f;
+                        if (len_sq_i > max_len_i * max_len_i || len_sq_i <= 1e-12f) continue;
+                        for (uint32_t ej = 0; ej < nej; ++ej) {
+                            const uint32_t vj = PT.adj_idx[aj0 + ej];
+                            if (vj >= A.nv) continue;
+                            if (ui == j || ui == vj || index == vj) continue;
+                            const uint32_t c_j = min(j, vj), c_vj = max(j, vj);
+                            const bool smaller =
+                                smaller_anchor(P, X, V, T, PT, A, index, vj, index, j) ||
+                                smaller_anchor(P, X, V, T, PT, A, ui, j, index, j) ||
+                                smaller_anchor(P, X, V, T, PT, A, ui, vj, index, j) ||
+                                smaller_anchor(P, X, V, T, PT, A, j, index, index, j) ||
+                                smaller_anchor(P, X, V, T, PT, A, j, ui, index, j) ||
+                                smaller_anchor(P, X, V, T, PT, A, vj, index, index, j) ||
+                                smaller_anchor(P, X, V, T, PT, A, vj, ui, index, j);
+                            if (pinned_i || smaller) continue;
+                            const f3 p_vj = xyz(P[vj]);
+                            const float llvj = T.local_len[vj];
+                            const f3 djv = p_vj - p_j;
+                            const float len_sq_j = dot(djv, djv);
+                            const float max_len_j = (llj + llvj) * 1.5f;
+                            if (len_sq_j > max_len_j * max_len_j || len_sq_j <= 1e-12f) continue;
+                            const f3 p_cj = j < vj ? p_j : p_vj, p_cvj = j < vj ? p_vj : p_j;
+                            const ST stp = closest_segs(p_i, p_ui, p_cj, p_cvj);
+                            const f3 pt1 = p_i + (p_ui - p_i) * stp.s;
+                            const f3 pt2 = p_cj + (p_cvj - p_cj) * stp.t;
+                            const f3 dee = pt1 - pt2;
+                            const float dsq_ee = dot(dee, dee);
+                            if (!(dsq_ee < check * check && dsq_ee > EPS * EPS)) continue;
+                            if (A.exclude_neighbors) {
+                                if (near_folded_dist(T, index, vj, length(p_i - p_vj)) ||
+                                    near_folded_dist(T, ui, j, length(p_ui - p_j)) ||
+                                    near_folded_dist(T, ui, vj, length(p_ui - p_vj)))
+                                    continue;
+                            }
+                            topk_offer(L, quota, dsq_ee, make_uint4(index, ui, c_j, c_vj), eval, drop);
+                        }
+                    }
+                }
+            }
+    const uint4 none = make_uint4(PAIR_SENTINEL, 0u, 0u, 0u);
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; ++k)
+        if (k < quota) out[size_t(index) * quota + k] = k < L.n ? L.e[k] : none;
+    if (eval) atomicAdd(&counters[VT ? 0 : 1], eval);
+    if (drop) atomicAdd(&counters[VT ? 2 : 3], drop);
+}
+
+// self_collision_solve_vt.wgsl:227-414 for one cached triple (single-layer: no untangling)
+__global__ void __launch_bounds__(256) k_pair_solve_vt(const float4 *__restrict__ P, const float4 *__restrict__ X,
+                                                       const uint4 *__restrict__ pairs, uint32_t slots,
+                                                       Accum *__restrict__ acc, uint32_t nv) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= slots) return;
+    const float EPS = 1e-7f;
+    const uint4 pr = pairs[t];
+    if (pr.x == PAIR_SENTINEL) return;
+    const uint32_t index = pr.x, j = pr.y, v0 = pr.z, v1 = pr.w;
+    if (index >= nv || j >= nv || v0 >= nv || v1 >= nv) return;
+    if (index == j || index == v0 || index == v1 || v0 == v1 || j == v0 || j == v1) return;
+    const float4 qi = P[index], xi = X[index], qj = P[j], xj = X[j], q0 = P[v0], x0 = X[v0], q1 = P[v1], x1 = X[v1];
+    const bool pinned_i = qi.w <= 0.0f;
+    const f3 p_i = xyz(qi), xo_i = xyz(xi);
+    const f3 p_j = xyz(qj), p_v0 = xyz(q0), p_v1 = xyz(q1);
+    const f3 pj_old = xyz(xj), p_v0o = xyz(x0), p_v1o = xyz(x1);
+    const float eff = xi.w + xj.w;
+    const f3 tri_move = (((p_j - pj_old) + (p_v0 - p_v0o)) + (p_v1 - p_v1o)) / 3.0f;
+    const float wi = qi.w, wj = qj.w, w0 = q0.w, w1 = q1.w;
+    const bool tri_pin = wj <= 0.0f || w0 <= 0.0f || w1 <= 0.0f;
+    const f3 rel_old = xo_i + tri_move;
+    float ht = 0.0f, hu = 0.0f, hv = 0.0f;
+    if (seg_tri_uv(rel_old, p_i, p_j, p_v0, p_v1, ht, hu, hv)) {
+        const f3 move = p_i - rel_old;
+        const f3 hit_pt = rel_old + move * ht;
+        const f3 tcr = cross(p_v0 - p_j, p_v1 - p_j);
+        const float tl = length(tcr);
+        const f3 tn = tl > 1e-12f ? tcr / tl : mk3(0.0f, 1.0f, 0.0f);
+        const float h_old_raw = dot(rel_old - p_j, tn);
+        const float side = fabsf(h_old_raw) > 1e-6f ? h_old_raw : dot(-move, tn);
+        const f3 n_side = side >= 0.0f ? tn : -tn;
+        const float h_old = fmaxf(dot(rel_old - hit_pt, n_side), 0.0f);
+        const float h_safe = fmaxf(h_old, eff * 0.25f);
+        const f3 total = (hit_pt + n_side * h_safe) - p_i;
+        const float pen_soft = fmaxf(0.0f, eff - h_safe);
+        const f3 bc = mk3(clampf(1.0f - hu - hv, 0.0f, 1.0f), clampf(hu, 0.0f, 1.0f), clampf(hv, 0.0f, 1.0f));
+        if (tri_pin) {
+            if (!pinned_i) add_ccd(&acc[index], total + n_side * pen_soft);
+        } else if (pinned_i) {
+            const float wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+            if (wt > EPS) {
+                const f3 full = total + n_side * pen_soft;
+                if (wj > 0.0f) add_ccd(&acc[j], (-full) * (bc.x * wj / wt));
+                if (w0 > 0.0f) add_ccd(&acc[v0], (-full) * (bc.y * w0 / wt));
+                if (w1 > 0.0f) add_ccd(&acc[v1], (-full) * (bc.z * w1 / wt));
+            }
+        } else {
+            const float wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+            const float wtot = wi + wt;
+            if (wtot > EPS) {
+                add_ccd(&acc[index], total * (wi / wtot));
+                if (wj > 0.0f) add_ccd(&acc[j], (-total) * (bc.x * wj / wtot));
+                if (w0 > 0.0f) add_ccd(&acc[v0], (-total) * (bc.y * w0 / wtot));
+                if (w1 > 0.0f) add_ccd(&acc[v1], (-total) * (bc.z * w1 / wtot));
+                if (pen_soft > EPS) {
+                    const f3 tp = n_side * (pen_soft * 0.6f);
+                    add_disp(&acc[index], tp * (wi / wtot));
+                    if (wj > 0.0f) add_disp(&acc[j], (-tp) * (bc.x * wj / wtot));
+                    if (w0 > 0.0f) add_disp(&acc[v0], (-tp) * (bc.y * w0 / wtot));
+                    if (w1 > 0.0f) add_disp(&acc[v1], (-tp) * (bc.z * w1 / wtot));
+                }
+            }
+        }
+        return;
+    }
+    f3 bc;
+    const f3 cp = closest_bary(p_i, p_j, p_v0, p_v1, bc);
+    const f3 dvt = p_i - cp;
+    const float dist = length(dvt);
+    if (pinned_i || !(dist < eff) || !(dist > EPS)) return;
+    const f3 nvt = dvt / dist;
+    const float pen = eff - dist;
+    if (tri_pin) {
+        add_pin_disp(&acc[index], nvt * pen + tri_move * 0.5f);
+        return;
+    }
+    const float wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+    const float wtot = wi + wt;
+    if (wtot > EPS) {
+        const f3 tp = nvt * (pen * 0.6f);
+        add_disp(&acc[index], tp * (wi / wtot));
+        if (wj > 0.0f) add_disp(&acc[j], (-tp) * (bc.x * wj / wtot));
+        if (w0 > 0.0f) add_disp(&acc[v0], (-tp) * (bc.y * w0 / wtot));
+        if (w1 > 0.0f) add_disp(&acc[v1], (-tp) * (bc.z * w1 / wtot));
+    }
+}
+
+// self_collision_solve_ee.wgsl:231-589 for one cached quadruple (single-layer: no untangling)
+__global__ void __launch_bounds__(256) k_pair_solve_ee(const float4 *__restrict__ P, const float4 *__restrict__ X,
+                                                       const uint4 *__restrict__ pairs, uint32_t slots,
+                                                       Accum *__restrict__ acc, uint32_t nv) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= slots) return;
+    const float EPS = 1e-7f;
+    const uint4 pr = pairs[t];
+    if (pr.x == PAIR_SENTINEL) return;
+    const uint32_t u0 = pr.x, u1 = pr.y, j = pr.z, vj = pr.w;
+    if (u0 >= nv || u1 >= nv || j >= nv || vj >= nv) return;
+    if (u0 == u1 || j == vj || u0 == j || u0 == vj || u1 == j || u1 == vj) return;
+    const float4 qu0 = P[u0], qu1 = P[u1], qj = P[j], qvj = P[vj];
+    const float wu0 = qu0.w, wu1 = qu1.w, wj = qj.w, wvj = qvj.w;
+    const bool a_has_pin = wu0 <= 0.0f || wu1 <= 0.0f, b_has_pin = wj <= 0.0f || wvj <= 0.0f;
+    if (a_has_pin && b_has_pin) return;
+    const float4 xu0 = X[u0], xu1 = X[u1], xj = X[j], xvj = X[vj];
+    const f3 p_u0 = xyz(qu0), p_u1 = xyz(qu1), p_j = xyz(qj), p_vj = xyz(qvj);
+    const f3 p_u0o = xyz(xu0), p_u1o = xyz(xu1), p_jo = xyz(xj), p_vjo = xyz(xvj);
+    const float th_u0 = xu0.w, th_u1 = xu1.w, th_j = xj.w, th_vj = xvj.w;
+    const ST st1c = closest_segs(p_u0, p_u1, p_j, p_vj);
+    const f3 d1c = mix3(p_u0, p_u1, st1c.s) - mix3(p_j, p_vj, st1c.t);
+    const float dsqc = dot(d1c, d1c);
+    const float effc = mixf(th_u0, th_u1, st1c.s) + mixf(th_j, th_vj, st1c.t);
+    const float max_eff = fmaxf(th_u0, th_u1) + fmaxf(th_j, th_vj);
+    const f3 du0 = p_u0 - p_u0o, du1 = p_u1 - p_u1o, dj = p_j - p_jo, dvj = p_vj - p_vjo;
+    const f3 va = (du0 + du1) * 0.5f, vb = (dj + dvj) * 0.5f;
+    const float rel = length(va - vb) + 0.5f * (length(du1 - du0) + length(dvj - dj));
+    const float reach = max_eff + rel;
+    if (dsqc > reach * reach) return;
+    const f3 d0g = mix3(p_u0o, p_u1o, st1c.s) - mix3(p_jo, p_vjo, st1c.t);
+    if ((rel > effc * 0.5f) || (dot(d0g, d1c) <= 0.0f)) {
+        const Sweep W = {p_u0o, p_u0, p_u1o, p_u1, p_jo, p_j, p_vjo, p_vj};
+        float min_sq = 1e20f, best_tau = 0.0f, half_span = 0.125f;
+        ST min_st = {0.0f, 0.0f};
+        float prev_vol = sweep_volume(W, 0.0f);
+        float zc_tau = -1.0f;
+#pragma unroll 1
+        for (uint32_t k = 0; k <= 8u; ++k) {
+            const float tk = float(k) * 0.125f;
+            ST stk;
+            const float dk = sweep_dist_sq(W, tk, stk);
+            if (dk < min_sq) { min_sq = dk; best_tau = tk; min_st = stk; }
+            if (k > 0u) {
+                const float vol = sweep_volume(W, tk);
+                if (prev_vol * vol <= 0.0f && zc_tau < 0.0f) zc_tau = tk - 0.0625f;
+                prev_vol = vol;
+            }
+        }
+        if (zc_tau >= 0.0f) {
+            ST stz;
+            const float dzs = sweep_dist_sq(W, zc_tau, stz);
+            if (dzs < min_sq) { min_sq = dzs; best_tau = zc_tau; half_span = 0.0625f; min_st = stz; }
+        }
+        float tlo = fmaxf(0.0f, best_tau - half_span), thi = fminf(1.0f, best_tau + half_span);
+        float dsq_star = min_sq;
+        ST st_star = min_st;
+#pragma unroll 1
+        for (int it = 0; it < 6; ++it) {
+            const float m1 = tlo + (thi - tlo) * (1.0f / 3.0f);
+            const float m2 = thi - (thi - tlo) * (1.0f / 3.0f);
+            ST s1, s2;
+            const float e1 = sweep_dist_sq(W, m1, s1);
+            const float e2 = sweep_dist_sq(W, m2, s2);
+            if (e1 < dsq_star) { dsq_star = e1; st_star = s1; }
+            if (e2 < dsq_star) { dsq_star = e2; st_star = s2; }
+            if (e1 < e2) thi = m2; else tlo = m1;
+        }
+        const float ss = st_star.s, ts = st_star.t;
+        const float eff_star = mixf(th_u0, th_u1, ss) + mixf(th_j, th_vj, ts);
+        if (dsq_star >= eff_star * eff_star) return;
+        const ST st0 = closest_segs(p_u0o, p_u1o, p_jo, p_vjo);
+        const f3 delta0 = mix3(p_u0o, p_u1o, st0.s) - mix3(p_jo, p_vjo, st0.t);
+        const float dist0 = length(delta0);
+        if (dist0 > EPS) {
+            const f3 n0 = delta0 / dist0;
+            const float h_end = dot(mix3(p_u0, p_u1, ss) - mix3(p_j, p_vj, ts), n0);
+            const bool crossover = (h_end <= 0.0f) || (dot(d1c, n0) <= 0.0f);
+            if (h_end < eff_star && crossover) {
+                const float pen = eff_star - h_end;
+                const bool a_all = wu0 == 0.0f && wu1 == 0.0f;
+                const bool b_all = wj == 0.0f && wvj == 0.0f;
+                const bool cnt_pin = !(a_all || b_all);
+                if (b_has_pin) {
+                    const f3 em = ((p_j - p_jo) + (p_vj - p_vjo)) * 0.5f;
+                    const float a = 1.0f - ss, b = ss;
+                    const float den = a * a * wu0 + b * b * wu1;
+                    if (den > EPS) {
+                        const f3 pb = n0 * pen + em * 0.5f;
+                        if (wu0 > 0.0f) add_ccd_ee(&acc[u0], pb * fminf((a * wu0) / den, 2.0f), cnt_pin);
+                        if (wu1 > 0.0f) add_ccd_ee(&acc[u1], pb * fminf((b * wu1) / den, 2.0f), cnt_pin);
+                    }
+                } else if (a_has_pin) {
+                    const f3 em = ((p_u0 - p_u0o) + (p_u1 - p_u1o)) * 0.5f;
+                    const float a = 1.0f - ts, b = ts;
+                    const float den = a * a * wj + b * b * wvj;
+                    if (den > EPS) {
+                        const f3 pb = (-n0) * pen + em * 0.5f;
+                        if (wj > 0.0f) add_ccd_ee(&acc[j], pb * fminf((a * wj) / den, 2.0f), cnt_pin);
+                        if (wvj > 0.0f) add_ccd_ee(&acc[vj], pb * fminf((b * wvj) / den, 2.0f), cnt_pin);
+                    }
+                } else {
+                    const float we1 = (1.0f - ss) * (1.0f - ss) * wu0 + ss * ss * wu1;
+                    const float we2 = (1.0f - ts) * (1.0f - ts) * wj + ts * ts * wvj;
+                    const float wt = we1 + we2;
+                    if (wt > EPS) {
+                        const f3 tot = n0 * pen;
+                        if (wu0 > 0.0f) add_ccd_ee(&acc[u0], tot * ((1.0f - ss) * wu0 / wt), true);
+                        if (wu1 > 0.0f) add_ccd_ee(&acc[u1], tot * (ss * wu1 / wt), true);
+                        if (wj > 0.0f) add_ccd_ee(&acc[j], (-tot) * ((1.0f - ts) * wj / wt), true);
+                        if (wvj > 0.0f) add_ccd_ee(&acc[vj], (-tot) * (ts * wvj / wt), true);
+                    }
+                }
+                return;
+            }
+        }
+    }
+    if (dsqc >= effc * effc || dsqc <= EPS * EPS) return;
+    const float dist = sqrtf(dsqc);
+    const f3 n = d1c / dist;
+    const float pen = effc - dist;
+    const float sp = st1c.s, tp = st1c.t;
+    if (b_has_pin) {
+        const f3 em = ((p_j - p_jo) + (p_vj - p_vjo)) * 0.5f;
+        const float a = 1.0f - sp, b = sp;
+        const float den = a * a * wu0 + b * b * wu1;
+        if (den > EPS) {
+            const f3 pb = n * pen + em * 0.5f;
+            if (wu0 > 0.0f) add_pin_disp(&acc[u0], pb * fminf((a * wu0) / den, 2.0f));
+            if (wu1 > 0.0f) add_pin_disp(&acc[u1], pb * fminf((b * wu1) / den, 2.0f));
+        }
+    } else if (a_has_pin) {
+        const f3 em = ((p_u0 - p_u0o) + (p_u1 - p_u1o)) * 0.5f;
+        const float a = 1.0f - tp, b = tp;
+        const float den = a * a * wj + b * b * wvj;
+        if (den > EPS) {
+            const f3 pb = (-n) * pen + em * 0.5f;
+            if (wj > 0.0f) add_pin_disp(&acc[j], pb * fminf((a * wj) / den, 2.0f));
+            if (wvj > 0.0f) add_pin_disp(&acc[vj], pb * fminf((b * wvj) / den, 2.0f));
+        }
+    } else {
+        const float we1 = (1.0f - sp) * (1.0f - sp) * wu0 + sp * sp * wu1;
+        const float we2 = (1.0f - tp) * (1.0f - tp) * wj + tp * tp * wvj;
+        const float wt = we1 + we2;
+        if (wt > EPS) {
+            const f3 td = n * (pen * 0.6f);
+            if (wu0 > 0.0f) add_disp(&acc[u0], td * ((1.0f - sp) * wu0 / wt));
+            if (wu1 > 0.0f) add_disp(&acc[u1], td * (sp * wu1 / wt));
+            if (wj > 0.0f) add_disp(&acc[j], (-td) * ((1.0f - tp) * wj / wt));
+            if (wvj > 0.0f) add_disp(&acc[vj], (-td) * (tp * wvj / wt));
+        }
+    }
+}
+
+hipError_t launch_pair_collect(hipStream_t s, const float4 *P, const float4 *X, const float4 *V, const SelfColTopo &T,
+                               const PairTopo &PT, const HashBufs &vh, uint4 *vt, uint4 *ee, uint32_t *counters,
+                               const PairArgs &a) {
+    if (!a.nv) return hipSuccess;
+    hipLaunchKernelGGL(k_pair_collect<true>, dim3(grid_for(a.nv, 64)), dim3(64), 0, s, P, X, V, T, PT, vh.starts,
+                       vh.sorted, vt, counters, a);
+    hipLaunchKernelGGL(k_pair_collect<false>, dim3(grid_for(a.nv, 64)), dim3(64), 0, s, P, X, V, T, PT, vh.starts,
+                       vh.sorted, ee, counters, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_solve_vt(hipStream_t s, const float4 *P, const float4 *X, const uint4 *vt, uint32_t slots,
+                                Accum *acc, uint32_t nv) {
+    if (!slots) return hipSuccess;
+    hipLaunchKernelGGL(k_pair_solve_vt, dim3(grid_for(slots, 256)), dim3(256), 0, s, P, X, vt, slots, acc, nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_solve_ee(hipStream_t s, const float4 *P, const float4 *X, const uint4 *ee, uint32_t slots,
+                                Accum *acc, uint32_t nv) {
+    if (!slots) return hipSuccess;
+    hipLaunchKernelGGL(k_pair_solve_ee, dim3(grid_for(slots, 256)), dim3(256), 0, s, P, X, ee, slots, acc, nv);
+    return hipGetLastError();
+}
+
 }  // namespace tc

@@ -218,7 +218,38 @@ hipError_t enqueue_substep_tail(tc_sim *s, float dt_sub, uint32_t pc, uint32_t l
         s->prof_end(q);
         if (e != hipSuccess) return e;
     }
-    if (self_collide && s->self_collision) {
+    if (self_collide && s->self_collision && s->pair_cache && !(s->pair_final_fallback && s->sub_last)) {
+        // dispatch.rs:1554-1645: collect on the frame's first substep, then solve the cached pairs
+        const int scope = s->prof_begin("self_collision_Outer");
+        const tc::SelfColArgs sa = s->sc_args();
+        const tc::SelfColTopo t = s->topo();
+        if (s->sub_first) {
+            q = s->prof_begin("hash_build");
+            e = enqueue_hashes(s, Pc);
+            s->prof_end(q);
+            if (e != hipSuccess) return e;
+            if ((e = tc::launch_fill_u32(s->stream, s->pair_counters.p, 0u, 4)) != hipSuccess) return e;
+            q = s->prof_begin("pair_collect");
+            e = tc::launch_pair_collect(s->stream, Pc, s->X.p, s->V.p, t, {s->adj_off.p, s->adj_idx.p, s->island.p},
+                                        s->vhash(), s->pair_vt.p, s->pair_ee.p, s->pair_counters.p, s->pair_args());
+            s->prof_end(q);
+            if (e != hipSuccess) return e;
+        }
+        const tc::PairArgs pa = s->pair_args();
+        q = s->prof_begin("pair_vt");
+        e = tc::launch_pair_solve_vt(s->stream, Pc, s->X.p, s->pair_vt.p, n * pa.quota_vt, s->accum.p, n);
+        s->prof_end(q);
+        if (e != hipSuccess) return e;
+        q = s->prof_begin("pair_ee");
+        e = tc::launch_pair_solve_ee(s->stream, Pc, s->X.p, s->pair_ee.p, n * pa.quota_ee, s->accum.p, n);
+        s->prof_end(q);
+        if (e != hipSuccess) return e;
+        q = s->prof_begin("pair_apply");
+        e = tc::launch_sc_apply(s->stream, Pc, s->local_len.p, s->accum.p, sa, B);
+        s->prof_end(q);
+        s->prof_end(scope);
+        if (e != hipSuccess) return e;
+    } else if (self_collide && s->self_collision) {
         const int scope = s->prof_begin("self_collision_Outer");
         q = s->prof_begin("hash_build");
         e = enqueue_hashes(s, Pc);
@@ -265,7 +296,10 @@ void next_record_slot(tc_sim *s) {
 
 hipError_t enqueue_frame(tc_sim *s, float dt, uint32_t substeps) {
     const float dt_sub = dt / float(substeps);
+    s->pair_dt_frame = dt;
     for (uint32_t k = 0; k < substeps; ++k) {
+        s->sub_first = k == 0;
+        s->sub_last = k + 1 == substeps;
         next_record_slot(s);
         hipError_t e = enqueue_substep(s, dt_sub, true, true, true);
         if (e != hipSuccess) return e;
@@ -305,7 +339,14 @@ int run_frame_host_gated(tc_sim *s, float dt, uint32_t substeps) {
     const uint64_t fused = s->fused_active() ? 1u : 0u;
     const uint32_t chunk = std::max<uint32_t>(2u, s->ws_iters + (s->ws_iters & 1u));
     uint32_t p0 = 0, l0 = 0;
+    s->pair_dt_frame = dt;
+    uint32_t dtf;
+    std::memcpy(&dtf, &dt, 4);
     for (uint32_t k = 0; k < substeps; ++k) {
+        s->sub_first = k == 0;
+        s->sub_last = k + 1 == substeps;
+        // the tail of the frame's first and last substeps differs with the pair cache
+        const uint64_t kind = s->pair_cache ? (uint64_t(s->sub_first) | (uint64_t(s->sub_last) << 1) | (uint64_t(dtf) << 2)) : 0u;
         const uint64_t kh[4] = {dtb, s->ws_iters, fused, s->epoch};
         int r = replay_part(s, s->pg_head, kh, [&] { return enqueue_substep_head(s, dt_sub, true, p0, l0); });
         if (r != TC_OK) return r;
@@ -324,7 +365,7 @@ int run_frame_host_gated(tc_sim *s, float dt, uint32_t substeps) {
             if (last) break;
             TC_HIP(hipStreamSynchronize(s->stream));
         }
-        const uint64_t kt[4] = {dtb, uint64_t(p0) | (uint64_t(l0) << 1), fused, s->epoch};
+        const uint64_t kt[4] = {dtb, uint64_t(p0) | (uint64_t(l0) << 1) | (kind << 2), fused, s->epoch};
         r = replay_part(s, s->pg_tail, kt, [&] { return enqueue_substep_tail(s, dt_sub, p0, l0, true, true, true); });
         if (r != TC_OK) return r;
     }
@@ -593,6 +634,8 @@ static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
         if (v == 256 || v == 512 || v == 1024) s->fused_block = uint32_t(v);
     }
     s->solver_mode = pp.solver_mode;
+    s->pair_max_vt = s->pair_max_ee = uint32_t(std::min<uint64_t>(std::max<uint64_t>(uint64_t(nv) * 8u, 8192u), 262144u));
+    s->pair_quota_vt = s->pair_quota_ee = nv ? std::min<uint32_t>(std::max<uint32_t>(s->pair_max_vt / nv, 1u), 8u) : 1u;
     s->cell_size = s->mesh.cell_size;
     s->table_size = device_table_size(s->mesh);
 
@@ -732,6 +775,8 @@ int tc_sim_substep_parts(tc_sim *s, float dt_sub, int collide, int self_collide,
     if (self_collide && s->self_collision && s->normal_untangling && s->mesh.has_multiple_layers)
         return fail(TC_ERR_UNSUPPORTED, "multi-layer cloth with normal untangling is not part of this build");
     s->rec_slot = -1;
+    s->sub_first = s->sub_last = true;  // one substep: a frame of its own
+    s->pair_dt_frame = dt_sub;
     TC_HIP(enqueue_substep(s, dt_sub, collide != 0, self_collide != 0, update != 0));
     return TC_OK;
 }
@@ -1333,6 +1378,119 @@ int tc_sim_set_solver_mode(tc_sim *s, uint32_t mode) {
 int tc_sim_get_solver_mode(const tc_sim *s, uint32_t *mode) {
     if (!s || !mode) return fail(TC_ERR_VALUE, "null argument");
     *mode = s->solver_mode;
+    return TC_OK;
+}
+
+// Device buffers of the pair cache, allocated on first use: slots for 8 pairs per vertex (the
+// largest quota), the counters, and the adjacency and island tables the collect kernel reads.
+static int pair_cache_alloc(tc_sim *s) {
+    if (s->pair_vt.p) return TC_OK;
+    const tc::Mesh &m = s->mesh;
+    const size_t slots = size_t(std::max<uint32_t>(m.nv, 1u)) * 8u;
+    TC_HIP(hipStreamSynchronize(s->stream));
+    TC_HIP(s->pair_vt.alloc(slots));
+    TC_HIP(s->pair_ee.alloc(slots));
+    TC_HIP(s->pair_counters.alloc(4));
+    TC_HIP(s->adj_off.upload(m.adj_off.data(), m.adj_off.size(), s->stream));
+    TC_HIP(s->adj_idx.upload(m.adj_idx.data(), m.adj_idx.size(), s->stream));
+    TC_HIP(s->island.upload(m.island.data(), m.island.size(), s->stream));
+    std::vector<uint4> none(slots, make_uint4(0xFFFFFFFFu, 0u, 0u, 0u));
+    TC_HIP(hipMemcpyAsync(s->pair_vt.p, none.data(), sizeof(uint4) * slots, hipMemcpyHostToDevice, s->stream));
+    TC_HIP(hipMemcpyAsync(s->pair_ee.p, none.data(), sizeof(uint4) * slots, hipMemcpyHostToDevice, s->stream));
+    TC_HIP(hipMemsetAsync(s->pair_counters.p, 0, 16, s->stream));
+    TC_HIP(hipStreamSynchronize(s->stream));
+    return TC_OK;
+}
+
+int tc_sim_set_enable_pair_cache(tc_sim *s, int enable) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (enable)
+        if (int rc = pair_cache_alloc(s)) return rc;
+    s->pair_cache = enable != 0;
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_get_enable_pair_cache(const tc_sim *s, int *enable) {
+    if (!s || !enable) return fail(TC_ERR_VALUE, "null argument");
+    *enable = s->pair_cache ? 1 : 0;
+    return TC_OK;
+}
+
+int tc_sim_set_self_collision_algorithm(tc_sim *s, uint32_t algo) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    if (algo == 2)
+        return fail(TC_ERR_UNSUPPORTED, "self_collision_algorithm=2 (virtual-vertex V-V) is not part of this build");
+    if (algo == 1) {
+        if (int rc = pair_cache_alloc(s)) return rc;
+    }
+    s->pair_cache = algo == 1;  // simulation/mod.rs:703-715
+    s->sc_algorithm = algo;
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_get_self_collision_algorithm(const tc_sim *s, uint32_t *algo) {
+    if (!s || !algo) return fail(TC_ERR_VALUE, "null argument");
+    *algo = s->sc_algorithm;
+    return TC_OK;
+}
+
+int tc_sim_set_pair_cache_options(tc_sim *s, uint32_t max_vt, uint32_t max_ee, uint32_t margin_mode, float safety,
+                                  float horizon_scale, float max_horizon) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    const uint32_t nv = s->mesh.nv;
+    auto clampu = [](uint32_t v, uint32_t lo, uint32_t hi) { return std::min(std::max(v, lo), hi); };
+    auto clampf = [](float v, float lo, float hi) { return std::min(std::max(v, lo), hi); };
+    s->pair_max_vt = clampu(max_vt, 64u, 262144u);
+    s->pair_max_ee = clampu(max_ee, 64u, 262144u);
+    s->pair_quota_vt = nv ? clampu(s->pair_max_vt / nv, 1u, 8u) : 1u;
+    s->pair_quota_ee = nv ? clampu(s->pair_max_ee / nv, 1u, 8u) : 1u;
+    s->pair_margin_mode = margin_mode == 0 ? 0u : 1u;
+    s->pair_safety = clampf(safety, 0.0f, 0.05f);
+    s->pair_hscale = clampf(horizon_scale, 0.0f, 4.0f);
+    s->pair_hmax = clampf(max_horizon, 0.0f, 0.1f);
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_set_enable_pair_cache_final_fallback(tc_sim *s, int enable) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    s->pair_final_fallback = enable != 0;
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_pair_cache_state(const tc_sim *s, double *out) {
+    if (!s || !out) return fail(TC_ERR_VALUE, "null argument");
+    out[0] = s->pair_max_vt; out[1] = s->pair_max_ee; out[2] = s->pair_quota_vt; out[3] = s->pair_quota_ee;
+    out[4] = s->pair_margin_mode; out[5] = s->pair_safety; out[6] = s->pair_hscale; out[7] = s->pair_hmax;
+    out[8] = s->pair_final_fallback ? 1.0 : 0.0;
+    return TC_OK;
+}
+
+int tc_sim_get_active_pairs(tc_sim *s, uint32_t *counts, uint32_t *vt, size_t vt_cap, size_t *n_vt, uint32_t *ee,
+                            size_t ee_cap, size_t *n_ee) {
+    if (!s || !counts || !n_vt || !n_ee) return fail(TC_ERR_VALUE, "null argument");
+    std::memset(counts, 0, 16);
+    *n_vt = *n_ee = 0;
+    if (!s->pair_vt.p) return TC_OK;
+    const uint32_t nv = s->mesh.nv;
+    const tc::PairArgs pa = s->pair_args();
+    std::vector<uint4> hv(size_t(nv) * pa.quota_vt), he(size_t(nv) * pa.quota_ee);
+    TC_HIP(hipStreamSynchronize(s->stream));
+    TC_HIP(hipMemcpy(counts, s->pair_counters.p, 16, hipMemcpyDeviceToHost));
+    if (!hv.empty()) TC_HIP(hipMemcpy(hv.data(), s->pair_vt.p, sizeof(uint4) * hv.size(), hipMemcpyDeviceToHost));
+    if (!he.empty()) TC_HIP(hipMemcpy(he.data(), s->pair_ee.p, sizeof(uint4) * he.size(), hipMemcpyDeviceToHost));
+    auto emit = [](const std::vector<uint4> &src, uint32_t *dst, size_t cap, size_t *n) {
+        for (const uint4 &p : src) {
+            if (p.x == 0xFFFFFFFFu) continue;
+            if (dst && *n < cap) { dst[4 * *n] = p.x; dst[4 * *n + 1] = p.y; dst[4 * *n + 2] = p.z; dst[4 * *n + 3] = p.w; }
+            ++*n;
+        }
+    };
+    emit(hv, vt, vt_cap, n_vt);
+    emit(he, ee, ee_cap, n_ee);
     return TC_OK;
 }
 

@@ -210,6 +210,20 @@ struct tc_sim {
     // one-launch-per-iteration plan (tc_tiles.cpp)
     int solver_path = 0;          // 0 auto, 1 one launch per colour, 2 one launch per iteration
     uint32_t solver_mode = 0;     // 1: distance constraints by fixed-point Jacobi sums (tc_jacobi.hip)
+
+    // pair cache (self_collision_algorithm 1, simulation/mod.rs:583-597, 703-870): the first
+    // substep of a frame collects per-vertex top-K candidate pairs, the others solve only those;
+    // with the final fallback the last substep takes the direct path on fresh hash tables
+    uint32_t sc_algorithm = 0;
+    bool pair_cache = false, pair_final_fallback = true;
+    uint32_t pair_max_vt = 8192, pair_max_ee = 8192, pair_quota_vt = 1, pair_quota_ee = 1;
+    uint32_t pair_margin_mode = 1;
+    float pair_safety = 0.005f, pair_hscale = 1.3f, pair_hmax = 0.02f;
+    float pair_dt_frame = 1.0f / 60.0f;  // frame length of the step being recorded (velocity horizon)
+    bool sub_first = true, sub_last = true;  // position of the substep being recorded in its frame
+    DevBuf<uint4> pair_vt, pair_ee;     // nv * 8 slots each (quota slots in use per vertex)
+    DevBuf<uint32_t> pair_counters;     // 4: V-T / E-E evaluated, V-T / E-E dropped
+    DevBuf<uint32_t> adj_off, adj_idx, island;
     DevBuf<int4> jacobi_acc;      // per-vertex sums of solver_mode 1, zero between iterations
     bool fused_ready = false;     // plan built and uploaded
     std::string fused_reason;     // why the plan is unavailable
@@ -356,6 +370,16 @@ struct tc_sim {
     }
     tc::FusedPlanDev plan_dev() const {
         return {pl_ext_off.p, pl_ext_vid.p, pl_n_own.p, pl_ph_off.p, pl_items.p};
+    }
+    tc::PairArgs pair_args() const {
+        tc::PairArgs a;
+        a.cell = cell_size; a.table = table_size; a.nv = mesh.nv;
+        a.safety_margin = pair_safety; a.exclude_neighbors = exclude_neighbors;
+        a.mode_f = float(pair_margin_mode);
+        a.dt_frame = pair_dt_frame; a.horizon_scale = pair_hscale; a.max_horizon = pair_hmax;
+        a.quota_vt = std::min<uint32_t>(std::max<uint32_t>(pair_quota_vt, 1u), 8u);
+        a.quota_ee = std::min<uint32_t>(std::max<uint32_t>(pair_quota_ee, 1u), 8u);
+        return a;
     }
     tc::HashBufs vhash() const { return {vkeys.p, vcounts.p, vstarts.p, vsorted.p}; }
     tc::HashBufs ehash() const { return {ekeys.p, ecounts.p, estarts.p, esorted.p}; }

@@ -76,10 +76,8 @@ class ClothSimulator:
             raise NotImplementedError("sewing springs are not part of this build")
         if enable_coarse_constraints:
             raise NotImplementedError("coarse (two-hop) constraints are not part of this build")
-        if self_collision_algorithm not in (None, 0):
-            raise NotImplementedError("only self_collision_algorithm=0 (DIRECT) is part of this build")
-        if enable_pair_cache:
-            raise NotImplementedError("the pair-cache self-collision path is not part of this build")
+        if self_collision_algorithm == 2:
+            raise NotImplementedError("self_collision_algorithm=2 (virtual-vertex V-V) is not part of this build")
         solver_mode = int(solver_mode)
         if not 0 <= solver_mode < 2 ** 32:
             raise OverflowError("solver_mode must fit an unsigned 32-bit integer")
@@ -118,6 +116,11 @@ class ClothSimulator:
         self.areal_density = 0.15 if areal_density is None else float(min(max(areal_density, 0.0), 100.0))
         self._init_config(stiffness, compression_stiffness, shear_stiffness, bending_stiffness, workgroup_size)
         self._cfg["solver_mode"] = solver_mode
+        self._sync_pair_cfg()
+        if enable_pair_cache is not None:  # lib.rs:1755-1760
+            self.set_enable_pair_cache(enable_pair_cache)
+        if self_collision_algorithm is not None:
+            self.set_self_collision_algorithm(self_collision_algorithm)
         self._ctor = dict(positions=pos, edges=e, faces=f, inv_masses=im, thicknesses=th, layer_ids=ly,
                           layer_id=int(layer_id), thickness=float(thickness), stiffness=float(stiffness),
                           bending_stiffness=float(bending_stiffness), compression_stiffness=compression_stiffness,
@@ -248,8 +251,9 @@ class ClothSimulator:
         part of this build raise NotImplementedError unless they keep their inert value."""
         c = _sc.parse(json_str)
         inert = {"coupled_mode": 0, "coupled_collider": False, "substep_interval": 1, "ee_substep_interval": 1,
-                 "self_collision_algorithm": 0, "enable_pair_cache": False, "enable_edge_collision": False,
-                 "sewing_priority_enabled": False}
+                 "enable_edge_collision": False, "sewing_priority_enabled": False}
+        if c["self_collision_algorithm"] == 2:
+            raise NotImplementedError("SimConfig self_collision_algorithm=2: that solver option is not part of this build")
         for k, v in inert.items():
             if c[k] != v:
                 raise NotImplementedError(f"SimConfig {k}={c[k]!r}: that solver option is not part of this build")
@@ -264,14 +268,17 @@ class ClothSimulator:
         self.set_enable_self_collision(c["enable_self_collision"])
         self.set_self_collision_options(c["relief_factor"], c["max_displacement_ratio"], c["exclude_neighbors"],
                                         c["enable_normal_untangling"], c["self_collision_max_iterations"])
+        self.set_self_collision_algorithm(c["self_collision_algorithm"])
+        self.set_enable_pair_cache(c["enable_pair_cache"])
+        self.set_pair_cache_options(c["pair_max_pairs"], c["pair_max_pairs"], c["pair_margin_mode"],
+                                    c["pair_safety_margin"], c["pair_horizon_scale"], c["pair_max_horizon"])
+        self.set_enable_pair_cache_final_fallback(c["enable_pair_final_fallback"])
         self.set_adaptive_substep_options(c["enable_adaptive_substep"], c["min_substeps"], c["max_substeps"])
         self.set_auto_compensate_iterations(c["auto_compensate_iterations"])
         self.set_strain_adaptive_options(c["enable_strain_adaptive"], c["strain_tolerance"])
         self.areal_density = float(min(max(c["areal_density"], 0.0), 100.0))
         # recorded as given: options of solvers this build does not have, kept at their inert values
-        for k in ("post_relaxation_iters", "auto_coupled_on_low_substeps", "pair_margin_mode", "pair_safety_margin",
-                  "pair_horizon_scale", "pair_max_horizon", "pair_max_pairs", "enable_pair_final_fallback",
-                  "edge_margin_scale", "edge_margin_offset", "sewing_stiffness", "sewing_shrink_speed",
+        for k in ("post_relaxation_iters", "auto_coupled_on_low_substeps", "edge_margin_scale", "edge_margin_offset", "sewing_stiffness", "sewing_shrink_speed",
                   "enable_sewing_lock", "sewing_lock_distance", "sewing_priority_threshold",
                   "sewing_priority_merge_dist", "sewing_priority_ramp_frames", "sewing_priority_max_frames",
                   "areal_density", "version"):
@@ -317,6 +324,92 @@ class ClothSimulator:
 
     def reset(self):
         _lib.check(self._L.tc_sim_reset(self._h))
+
+    # --- pair-cache self-collision (lib.rs:2029-2107) ----------------------------------------------
+    def _pair_state(self):
+        out = (C.c_double * 9)()
+        _lib.check(self._L.tc_sim_pair_cache_state(self._h, out))
+        return list(out)
+
+    def _sync_pair_cfg(self):
+        st = self._pair_state()
+        e = C.c_int(0)
+        _lib.check(self._L.tc_sim_get_enable_pair_cache(self._h, C.byref(e)))
+        a = C.c_uint32(0)
+        _lib.check(self._L.tc_sim_get_self_collision_algorithm(self._h, C.byref(a)))
+        self._cfg.update(enable_pair_cache=bool(e.value), self_collision_algorithm=int(a.value),
+                         pair_max_pairs=int(max(st[0], st[1])), pair_margin_mode=int(st[4]),
+                         pair_safety_margin=float(np.float32(st[5])), pair_horizon_scale=float(np.float32(st[6])),
+                         pair_max_horizon=float(np.float32(st[7])), enable_pair_final_fallback=bool(st[8]))
+
+    def set_enable_pair_cache(self, enable):
+        _lib.check(self._L.tc_sim_set_enable_pair_cache(self._h, 1 if enable else 0))
+        self._sync_pair_cfg()
+
+    def get_enable_pair_cache(self) -> bool:
+        e = C.c_int(0)
+        _lib.check(self._L.tc_sim_get_enable_pair_cache(self._h, C.byref(e)))
+        return bool(e.value)
+
+    def set_self_collision_algorithm(self, algo):
+        """0: DIRECT, 1: PAIR_CACHE; 2 (virtual-vertex V-V) raises NotImplementedError."""
+        algo = int(algo)
+        if not 0 <= algo < 2 ** 32:
+            raise OverflowError("algo must fit an unsigned 32-bit integer")
+        if algo == 2:
+            raise NotImplementedError("self_collision_algorithm=2 (virtual-vertex V-V) is not part of this build")
+        _lib.check(self._L.tc_sim_set_self_collision_algorithm(self._h, algo))
+        self._sync_pair_cfg()
+
+    def get_self_collision_algorithm(self) -> int:
+        a = C.c_uint32(0)
+        _lib.check(self._L.tc_sim_get_self_collision_algorithm(self._h, C.byref(a)))
+        return int(a.value)
+
+    def set_pair_cache_options(self, max_vt_pairs=65536, max_ee_pairs=65536, margin_mode=1, safety_margin=0.005,
+                               horizon_scale=1.3, max_horizon=0.02):
+        _lib.check(self._L.tc_sim_set_pair_cache_options(self._h, int(max_vt_pairs), int(max_ee_pairs),
+                                                         int(margin_mode), float(safety_margin), float(horizon_scale),
+                                                         float(max_horizon)))
+        self._sync_pair_cfg()
+
+    def set_enable_pair_cache_final_fallback(self, enable):
+        _lib.check(self._L.tc_sim_set_enable_pair_cache_final_fallback(self._h, 1 if enable else 0))
+        self._sync_pair_cfg()
+
+    def get_enable_pair_cache_final_fallback(self) -> bool:
+        return bool(self._pair_state()[8])
+
+    def get_pair_cache_stats(self):
+        """(V-T evaluated, E-E evaluated, V-T budget, E-E budget), dispatch.rs:1111-1119."""
+        cnt = np.zeros(4, np.uint32)
+        a, b = C.c_size_t(0), C.c_size_t(0)
+        _lib.check(self._L.tc_sim_get_active_pairs(self._h, _ptr(cnt), None, 0, C.byref(a), None, 0, C.byref(b)))
+        st = self._pair_state()
+        return int(cnt[0]), int(cnt[1]), int(st[0]), int(st[1])
+
+    def get_active_pairs(self):
+        """(vt_count, ee_count, V-T pairs, E-E pairs) as lists of 4 indices, slot order
+        (dispatch.rs:1255-1291)."""
+        cnt = np.zeros(4, np.uint32)
+        a, b = C.c_size_t(0), C.c_size_t(0)
+        _lib.check(self._L.tc_sim_get_active_pairs(self._h, _ptr(cnt), None, 0, C.byref(a), None, 0, C.byref(b)))
+        vt = np.zeros((a.value, 4), np.uint32)
+        ee = np.zeros((b.value, 4), np.uint32)
+        _lib.check(self._L.tc_sim_get_active_pairs(self._h, _ptr(cnt), _ptr(vt), a.value, C.byref(a), _ptr(ee),
+                                                   b.value, C.byref(b)))
+        return int(cnt[0]), int(cnt[1]), vt.tolist(), ee.tolist()
+
+    def active_pairs_arrays(self):
+        """(counters [V-T evaluated, E-E evaluated, V-T dropped, E-E dropped], V-T (n, 4), E-E (n, 4))."""
+        cnt = np.zeros(4, np.uint32)
+        a, b = C.c_size_t(0), C.c_size_t(0)
+        _lib.check(self._L.tc_sim_get_active_pairs(self._h, _ptr(cnt), None, 0, C.byref(a), None, 0, C.byref(b)))
+        vt = np.zeros((a.value, 4), np.uint32)
+        ee = np.zeros((b.value, 4), np.uint32)
+        _lib.check(self._L.tc_sim_get_active_pairs(self._h, _ptr(cnt), _ptr(vt), a.value, C.byref(a), _ptr(ee),
+                                                   b.value, C.byref(b)))
+        return cnt, vt, ee
 
     def set_solver_mode(self, mode):
         """0: graph colouring; 1: fixed-point Jacobi for the distance constraints (lib.rs:1887-1890)."""
@@ -782,6 +875,7 @@ class ClothBatch(ClothSimulator):
         self._keep = keep
         self.areal_density = 0.15 if areal_density is None else float(min(max(areal_density, 0.0), 100.0))
         self._init_config(stiffness, compression_stiffness, shear_stiffness, bending_stiffness, 32)
+        self._sync_pair_cfg()
         self._ctor = None
         self._colliders = []
         self._recorder = None
