@@ -109,6 +109,10 @@ extern "C" {
     pub fn tc_sim_set_enable_pair_cache_final_fallback(sim: *mut TcSim, enable: i32) -> i32;
     pub fn tc_sim_pair_cache_state(sim: *const TcSim, out: *mut f64) -> i32;
     pub fn tc_sim_get_active_pairs(sim: *mut TcSim, counts: *mut u32, vt: *mut u32, vt_cap: usize, n_vt: *mut usize, ee: *mut u32, ee_cap: usize, n_ee: *mut usize) -> i32;
+    pub fn tc_sim_set_enable_edge_collision(sim: *mut TcSim, enable: i32) -> i32;
+    pub fn tc_sim_set_edge_margin(sim: *mut TcSim, scale: f32, offset: f32) -> i32;
+    pub fn tc_sim_collider_edge_count(sim: *const TcSim, n: *mut u32) -> i32;
+    pub fn tc_sim_get_collider_edges(sim: *mut TcSim, out: *mut f32, cap: u32) -> i32;
     pub fn tc_sim_solver_info(sim: *mut TcSim, out: *mut f64) -> i32;
     pub fn tc_sim_autotune(sim: *mut TcSim, dt_sub: f32, out: *mut f64) -> i32;
     pub fn tc_sim_num_vertices(sim: *const TcSim) -> u32;

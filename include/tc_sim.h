@@ -238,6 +238,17 @@ int tc_sim_pair_cache_state(const tc_sim *sim, double *out);
  * up to *_cap pairs written, *n_* = pairs held (outputs may be NULL to count). */
 int tc_sim_get_active_pairs(tc_sim *sim, uint32_t *counts, uint32_t *vt, size_t vt_cap, size_t *n_vt, uint32_t *ee,
                             size_t ee_cap, size_t *n_ee);
+/* Collider-edge collision, lib.rs:2735-2749 -> simulation/mod.rs:1007-1025: cloth edges against the
+ * unique edges of the mesh collider (edge_collision.wgsl), after the collider pass and the final pin,
+ * pushing into the self-collision accumulators' hard channel and applied at once.  set_edge_margin:
+ * target distance = (cloth + collider thickness) * scale + offset, scale >= 1, offset >= 0
+ * (set_edge_margin_scale / set_edge_margin_offset).  collider_edge_count: unique collider edges. */
+int tc_sim_set_enable_edge_collision(tc_sim *sim, int enable);
+int tc_sim_set_edge_margin(tc_sim *sim, float scale, float offset);
+int tc_sim_collider_edge_count(const tc_sim *sim, uint32_t *n);
+/* The collider edges, 12 words each: p0 xyz, thickness, p1 xyz, friction, normal xyz, flags (u32 bits);
+ * cap = edges the buffer holds (no reference counterpart: exposed for parity checks). */
+int tc_sim_get_collider_edges(tc_sim *sim, float *out, uint32_t cap);
 /* out[10]: [active path (1|2), tiles, largest cone (vertices), cone vertices total,
  *           items per iteration, constraints per iteration, plan available (0|1), colour phases,
  *           tile size (vertices), block size of the per-iteration kernel] */

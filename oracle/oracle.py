@@ -117,6 +117,9 @@ def lib(precision: str = "f32") -> C.CDLL:
                                          C.c_float, C.c_float, C.c_int]
     L.orc_sim_active_pairs.restype = C.c_uint32
     L.orc_sim_active_pairs.argtypes = [_vp, C.c_int, _vp, _vp]
+    L.orc_sim_set_edge_collision.argtypes = [_vp, C.c_int, C.c_float, C.c_float]
+    L.orc_sim_collider_edges.restype = C.c_uint32
+    L.orc_sim_collider_edges.argtypes = [_vp, _vp, _vp]
     _LIBS[precision] = L
     return L
 
@@ -253,6 +256,17 @@ class OracleSim:
         self._L.orc_sim_active_pairs(self._h, 0, _ptr(vt), _ptr(cnt))
         self._L.orc_sim_active_pairs(self._h, 1, _ptr(ee), None)
         return cnt, vt, ee
+
+    def set_edge_collision(self, on=True, margin_scale=1.0, margin_offset=0.0):
+        self._L.orc_sim_set_edge_collision(self._h, int(on), margin_scale, margin_offset)
+
+    def collider_edges(self):
+        """The unique collider edges: (n, 11) [p0, thickness, p1, friction, normal] and flags."""
+        n = self._L.orc_sim_collider_edges(self._h, None, None)
+        e = np.zeros((n, 11), np.float32)
+        fl = np.zeros(n, np.uint32)
+        self._L.orc_sim_collider_edges(self._h, _ptr(e), _ptr(fl))
+        return e, fl
 
     def set_solver_mode(self, mode):
         """0: colours; 1: fixed-point Jacobi distance pass (distance_atomic.wgsl)."""

@@ -1050,6 +1050,10 @@ typedef struct {
     uint32_t pair_quota_vt, pair_quota_ee, pair_margin_mode;
     real pair_safety, pair_hscale, pair_hmax, pair_dt_frame;
     uint32_t *pair_vt, *pair_ee, pair_counters[4];
+    /* collider-edge collision: unique collider edges {p0, thickness, p1, friction, n, flags} and
+     * their grid (32768 buckets) */
+    int edge_collision; real edge_scale, edge_offset;
+    float *cedges; uint32_t *cedge_flags, nce, *cstarts, *csorted;
     int32_t *jac;                                    /* per-vertex sums {dx, dy, dz, count} */
 } orc_sim;
 
@@ -1097,6 +1101,7 @@ orc_sim *orc_sim_create(const float *pos, uint32_t nv, const uint32_t *edges, ui
     s->pair_final_fallback = 1; s->pair_margin_mode = 1;
     s->pair_safety = (real)0.005f; s->pair_hscale = (real)1.3f; s->pair_hmax = (real)0.02f;
     s->pair_dt_frame = (real)(1.0f / 60.0f);
+    s->edge_scale = 1; s->edge_offset = 0;
     s->hkeys = (uint32_t *)malloc(N * 4); s->hsorted = (uint32_t *)malloc(N * 4);
     s->hstarts = (uint32_t *)malloc((size_t)(s->table_size + 1) * 4);
     size_t NE = m->ne ? m->ne : 1;
@@ -1117,6 +1122,7 @@ void orc_sim_free(orc_sim *s) {
     free(s->hkeys); free(s->hstarts); free(s->hsorted); free(s->ekeys); free(s->estarts); free(s->esorted);
     free(s->sdf_tex); free(s->sdf_info); free(s->sdf_xf);
     free(s->pin_v); free(s->pin_t); free(s->pin_w); free(s->jac); free(s->pair_vt); free(s->pair_ee);
+    free(s->cedges); free(s->cedge_flags); free(s->cstarts); free(s->csorted);
     free(s);
 }
 
@@ -1160,7 +1166,8 @@ void orc_sim_add_capsule(orc_sim *s, const float *a, const float *b, float r, fl
 void orc_sim_add_plane(orc_sim *s, const float *pt, const float *n, float fr, float rest) {
     collider k = {2, fr, rest, mk(pt[0], pt[1], pt[2]), 0, mk(n[0], n[1], n[2])}; add_col(s, k);
 }
-void orc_sim_clear_colliders(orc_sim *s) { s->ncols = 0; s->ntris = 0; }
+static void build_collider_edges(orc_sim *s);
+void orc_sim_clear_colliders(orc_sim *s) { s->ncols = 0; s->ntris = 0; build_collider_edges(s); }
 
 /* set_bone_sdf_colliders (lib.rs:2235-2285): texels as pack2x16float words, Z, Y, X order; 20
  * floats of collider info each; transforms start as identity (simulation/mod.rs:1700-1719). */
@@ -1237,6 +1244,137 @@ void orc_sim_set_mesh_triangles(orc_sim *s, const float *t9, uint32_t nt, const 
     }
     s->ntris = nt;
     free(T); free(cen); free(k);
+    build_collider_edges(s);
+}
+
+/* extract_collider_edges (mesh.rs:1492-1567) over the Morton-ordered triangles: the unique edges
+ * keyed by the bits of their end points (smaller triple first), in order of first appearance;
+ * thickness = max, friction averaged pairwise in triangle order, normal = normalised sum of the
+ * triangle normals, single-sided only if all are.  Then build_collider_edge_grid
+ * (spatial_hash.rs:473-529): each edge in every bucket of its half-cell samples. */
+typedef struct { uint32_t key[6]; uint32_t seq; } ekey_t;
+static int cmp_ekey(const void *a, const void *b) {
+    const ekey_t *x = (const ekey_t *)a, *y = (const ekey_t *)b;
+    for (int i = 0; i < 6; i++) if (x->key[i] != y->key[i]) return x->key[i] < y->key[i] ? -1 : 1;
+    return x->seq < y->seq ? -1 : x->seq > y->seq;
+}
+static int cmp_u32pair(const void *a, const void *b) {
+    const uint32_t *x = (const uint32_t *)a, *y = (const uint32_t *)b;
+    return x[0] < y[0] ? -1 : x[0] > y[0];
+}
+static uint32_t fbits(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
+static int32_t f2i_sat_f(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483520.0f) return 2147483647;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)x;
+}
+static void build_collider_edges(orc_sim *s) {
+    uint32_t nt = s->ntris, n3 = 3 * nt;
+    free(s->cedges); free(s->cedge_flags); free(s->cstarts); free(s->csorted);
+    ekey_t *K = (ekey_t *)malloc((n3 ? n3 : 1) * sizeof(ekey_t));
+    float (*tn)[3] = (float (*)[3])malloc((nt ? nt : 1) * 12);
+    for (uint32_t t = 0; t < nt; t++) {
+        const mtri *T = &s->tris[t];
+        float e1[3] = {T->p1[0] - T->p0[0], T->p1[1] - T->p0[1], T->p1[2] - T->p0[2]};
+        float e2[3] = {T->p2[0] - T->p0[0], T->p2[1] - T->p0[1], T->p2[2] - T->p0[2]};
+        float cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        float cl = sqrtf(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+        for (int a = 0; a < 3; a++) tn[t][a] = cl > 1e-12f ? cr[a] / cl : 0.0f;
+        const float *P3[3] = {T->p0, T->p1, T->p2};
+        for (int k = 0; k < 3; k++) {
+            const float *a = P3[k], *b = P3[(k + 1) % 3];
+            uint32_t ab[3] = {fbits(a[0]), fbits(a[1]), fbits(a[2])}, bb[3] = {fbits(b[0]), fbits(b[1]), fbits(b[2])};
+            int a_first = ab[0] != bb[0] ? ab[0] < bb[0] : ab[1] != bb[1] ? ab[1] < bb[1] : ab[2] < bb[2];
+            ekey_t *e = &K[3 * t + k];
+            memcpy(e->key, a_first ? ab : bb, 12); memcpy(e->key + 3, a_first ? bb : ab, 12);
+            e->seq = 3 * t + k;
+        }
+    }
+    qsort(K, n3, sizeof(ekey_t), cmp_ekey);
+    /* groups in order of first appearance */
+    uint32_t *first = (uint32_t *)malloc(2 * (n3 ? n3 : 1) * 4);  /* (first seq, group start) */
+    uint32_t ng = 0;
+    for (uint32_t i = 0; i < n3; i++)
+        if (i == 0 || memcmp(K[i].key, K[i - 1].key, 24) != 0) { first[2 * ng] = K[i].seq; first[2 * ng + 1] = i; ng++; }
+    qsort(first, ng, 8, cmp_u32pair);
+    s->cedges = (float *)malloc((ng ? ng : 1) * 11 * sizeof(float));
+    s->cedge_flags = (uint32_t *)malloc((ng ? ng : 1) * 4);
+    for (uint32_t g = 0; g < ng; g++) {
+        uint32_t i0 = first[2 * g + 1];
+        float *E = s->cedges + 11 * (size_t)g;
+        float ns[3] = {0, 0, 0};
+        uint32_t flags = 0;
+        for (uint32_t i = i0; i < n3 && memcmp(K[i].key, K[i0].key, 24) == 0; i++) {
+            uint32_t t = K[i].seq / 3, k = K[i].seq % 3;
+            const mtri *T = &s->tris[t];
+            if (i == i0) {
+                const float *P3[3] = {T->p0, T->p1, T->p2};
+                for (int a = 0; a < 3; a++) { E[a] = P3[k][a]; E[4 + a] = P3[(k + 1) % 3][a]; }
+                E[3] = T->thickness; E[7] = T->friction;
+                ns[0] = tn[t][0]; ns[1] = tn[t][1]; ns[2] = tn[t][2];
+                flags = T->flags & 1u;
+            } else {
+                E[3] = fmaxf(E[3], T->thickness);
+                E[7] = (E[7] + T->friction) * 0.5f;
+                ns[0] += tn[t][0]; ns[1] += tn[t][1]; ns[2] += tn[t][2];
+                flags &= T->flags & 1u;
+            }
+        }
+        float nl = sqrtf(ns[0] * ns[0] + ns[1] * ns[1] + ns[2] * ns[2]);
+        for (int a = 0; a < 3; a++) E[8 + a] = nl > 1e-6f ? ns[a] / nl : 0.0f;
+        s->cedge_flags[g] = flags;
+    }
+    s->nce = ng;
+    /* grid */
+    const uint32_t table = 32768;
+    float cell = (float)s->m->cell_size, safe = fmaxf(cell, 1e-4f), pitch = safe * 0.5f;
+    s->cstarts = (uint32_t *)calloc(table + 1, 4);
+    uint32_t *counts = (uint32_t *)calloc(table, 4);
+    size_t cap = 64, ne = 0;
+    uint32_t *ent = (uint32_t *)malloc(cap * 8);
+    uint32_t *hs = (uint32_t *)malloc(4098 * 4);
+    for (uint32_t g = 0; g < ng; g++) {
+        const float *E = s->cedges + 11 * (size_t)g;
+        float dx = E[4] - E[0], dy = E[5] - E[1], dz = E[6] - E[2];
+        float len = sqrtf(dx * dx + dy * dy + dz * dz);
+        float cf = ceilf(len / pitch);
+        size_t steps = (cf != cf || cf <= 0.0f) ? 0 : (cf >= 1.8446744e19f ? (size_t)-1 : (size_t)cf);
+        if (steps < 1) steps = 1;
+        if (steps > 4096) steps = 4096;
+        uint32_t nh = 0;
+        for (size_t k = 0; k <= steps; k++) {
+            float t = (float)k / (float)steps;
+            float px = E[0] + dx * t, py = E[1] + dy * t, pz = E[2] + dz * t;
+            uint32_t c0 = (uint32_t)f2i_sat_f(floorf(px / safe)), c1 = (uint32_t)f2i_sat_f(floorf(py / safe)),
+                     c2 = (uint32_t)f2i_sat_f(floorf(pz / safe));
+            hs[nh++] = ((c0 * 73856093u) ^ (c1 * 19349663u) ^ (c2 * 83492791u)) % table;
+        }
+        qsort(hs, nh, 4, cmp_u32);
+        for (uint32_t q = 0; q < nh; q++) {
+            if (q > 0 && hs[q] == hs[q - 1]) continue;
+            if (ne == cap) { cap *= 2; ent = (uint32_t *)realloc(ent, cap * 8); }
+            ent[2 * ne] = hs[q]; ent[2 * ne + 1] = g; ne++;
+            counts[hs[q]]++;
+        }
+    }
+    uint32_t sum = 0;
+    for (uint32_t h = 0; h < table; h++) { s->cstarts[h] = sum; sum += counts[h]; }
+    s->cstarts[table] = sum;
+    s->csorted = (uint32_t *)malloc((ne ? ne : 1) * 4);
+    memcpy(counts, s->cstarts, table * 4);
+    for (size_t i = 0; i < ne; i++) s->csorted[counts[ent[2 * i]]++] = ent[2 * i + 1];
+    free(K); free(tn); free(first); free(counts); free(ent); free(hs);
+}
+void orc_sim_set_edge_collision(orc_sim *s, int on, float scale, float offset) {
+    s->edge_collision = on;
+    s->edge_scale = scale < 1.0f ? 1.0f : scale;
+    s->edge_offset = offset < 0.0f ? 0.0f : offset;
+}
+uint32_t orc_sim_collider_edges(const orc_sim *s, float *out, uint32_t *flags) {
+    if (out) memcpy(out, s->cedges, (size_t)s->nce * 11 * sizeof(float));
+    if (flags) memcpy(flags, s->cedge_flags, (size_t)s->nce * 4);
+    return s->nce;
 }
 void orc_sim_reset(orc_sim *s) {
     for (uint32_t i = 0; i < s->nv; i++) { s->x[i] = s->x0[i]; s->p[i] = s->x0[i]; s->v[i] = mk(0, 0, 0); s->w[i] = s->w0[i]; }
@@ -2657,6 +2795,90 @@ static void self_collision_pass(orc_sim *s) {
 /* one frame: dispatch.rs:7-354 with adaptive substeps off, decoupled colliders,
  * self-collision mode 0 / algorithm 0, no sewing; the pins run at the start of every
  * iteration and once after the collider (dispatch.rs:176-181, 238-250) */
+/* edge_collision.wgsl:98-150 */
+typedef struct { real s, t, dist; } eseg;
+static eseg edge_closest(v3 p0, v3 p1, v3 q0, v3 q1) {
+    v3 d1 = sub(p1, p0), d2 = sub(q1, q0), r = sub(p0, q0);
+    real a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r), s_ = 0, t = 0;
+    eseg res;
+    if (a <= (real)1e-12f && e <= (real)1e-12f) { res.s = 0; res.t = 0; res.dist = len3(sub(p0, q0)); return res; }
+    if (a <= (real)1e-12f) {
+        t = rclamp(f / e, 0, 1);
+    } else {
+        real c = dot(d1, r);
+        if (e <= (real)1e-12f) {
+            s_ = rclamp(-c / a, 0, 1);
+        } else {
+            real b = dot(d1, d2), denom = a * e - b * b;
+            s_ = denom > a * e * (real)1e-7f ? rclamp((b * f - c * e) / denom, 0, 1) : 0;
+            t = (b * s_ + f) / e;
+            if (t < 0) { t = 0; s_ = rclamp(-c / a, 0, 1); }
+            else if (t > 1) { t = 1; s_ = rclamp((b - c) / a, 0, 1); }
+        }
+    }
+    res.s = s_; res.t = t; res.dist = len3(sub(add(p0, mul(d1, s_)), add(q0, mul(d2, t))));
+    return res;
+}
+/* edge_collision.wgsl:153-276: every cloth edge over the 27 cells (z outer) of its midpoint */
+static void k_edge_collision(orc_sim *s) {
+    const orc_mesh *m = s->m;
+    const real EPS = (real)1e-7f;
+    const real cell = (real)m->cell_size;
+    if (s->nce == 0) return;
+    for (uint32_t k = 0; k < m->ne; k++) {
+        uint32_t u0 = m->e_v0[k], u1 = m->e_v1[k];
+        real w0 = s->w[u0], w1 = s->w[u1];
+        if (w0 + w1 <= EPS) continue;
+        v3 p0 = s->p[u0], p1 = s->p[u1];
+        v3 mid = mul(add(p0, p1), (real)0.5f);
+        real cth = rmax((real)m->thick[u0], (real)m->thick[u1]);
+        int32_t cc[3]; cell_of(mid, cell, cc);
+        v3 amin = vmin(p0, p1), amax = vmax(p0, p1);
+        for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+            int32_t nc[3] = {(int32_t)((uint32_t)cc[0] + (uint32_t)dx), (int32_t)((uint32_t)cc[1] + (uint32_t)dy),
+                             (int32_t)((uint32_t)cc[2] + (uint32_t)dz)};
+            uint32_t h = hash_coords(nc, 32768);
+            for (uint32_t i = s->cstarts[h]; i < s->cstarts[h + 1]; i++) {
+                uint32_t ci = s->csorted[i];
+                const float *E = s->cedges + 11 * (size_t)ci;
+                v3 q0 = mk(E[0], E[1], E[2]), q1 = mk(E[4], E[5], E[6]);
+                real target = (cth + (real)E[3]) * s->edge_scale + s->edge_offset;
+                v3 cmin = vmin(q0, q1), cmax = vmax(q0, q1);
+                if (amax.x + target < cmin.x || amin.x - target > cmax.x || amax.y + target < cmin.y ||
+                    amin.y - target > cmax.y || amax.z + target < cmin.z || amin.z - target > cmax.z)
+                    continue;
+                eseg r = edge_closest(p0, p1, q0, q1);
+                if (!(r.dist < target)) continue;
+                v3 diff = sub(add(p0, mul(sub(p1, p0), r.s)), add(q0, mul(sub(q1, q0), r.t)));
+                v3 en = mk(E[8], E[9], E[10]);
+                int single = (s->cedge_flags[ci] & 1u) != 0, has_n = dot(en, en) > (real)0.25f;
+                v3 n = mk(0, 0, 1);
+                real pen = target - r.dist;
+                if (single && has_n) {
+                    real sp = dot(diff, en);
+                    if (sp >= 0) { n = r.dist > EPS ? dvs(diff, r.dist) : en; pen = target - r.dist; }
+                    else { n = en; pen = rmin(target - sp, target * (real)2.0f); }
+                } else if (r.dist > EPS) {
+                    n = dvs(diff, r.dist);
+                } else {
+                    v3 cr = cross(sub(p1, p0), sub(q1, q0));
+                    real cl = len3(cr);
+                    if (cl > EPS) n = dvs(cr, cl);
+                }
+                v3 push = mul(n, pen);
+                real a = 1 - r.s, b = r.s, denom = a * a * w0 + b * b * w1;
+                if (denom > EPS) {
+                    real f0 = rmin((a * w0) / denom, 2), f1 = rmin((b * w1) / denom, 2);
+                    if (w0 > 0) add_pin_disp(&s->acc[u0], mul(push, f0));
+                    if (w1 > 0) add_pin_disp(&s->acc[u1], mul(push, f1));
+                }
+            }
+        }
+    }
+}
+
 /* The outer self-collision pass of a substep (dispatch.rs:1554-1645): the pair cache collects on
  * the frame's first substep and solves the held pairs on the others; with the final fallback the
  * last substep takes the direct path. */
@@ -2690,6 +2912,10 @@ static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
         }
         if (s->ncols > 0 || s->ntris > 0 || s->sdf_nb > 0) k_collision(s, dt_sub);
         k_pins(s, dt_sub);
+        if (s->edge_collision && m->ne > 0 && (s->ncols > 0 || s->ntris > 0 || s->sdf_nb > 0)) {
+            k_edge_collision(s);
+            k_sc_apply(s);
+        }
         if (s->self_collision) self_collision_outer(s, sub == 0, sub + 1 == substeps);
         k_update_vel(s, dt_sub);
     }
@@ -2721,6 +2947,10 @@ void orc_sim_substep_parts(orc_sim *s, float dt_sub, int do_collide, int do_self
     }
     if (do_collide && (s->ncols > 0 || s->ntris > 0 || s->sdf_nb > 0)) k_collision(s, dt);
     k_pins(s, dt);
+    if (do_collide && s->edge_collision && m->ne > 0 && (s->ncols > 0 || s->ntris > 0 || s->sdf_nb > 0)) {
+        k_edge_collision(s);
+        k_sc_apply(s);
+    }
     s->pair_dt_frame = dt;
     if (do_self && s->self_collision) self_collision_outer(s, 1, 1);
     if (do_update) k_update_vel(s, dt);

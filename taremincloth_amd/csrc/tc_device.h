@@ -72,6 +72,18 @@ struct MeshTri {
 };
 static_assert(sizeof(MeshTri) == 64, "the collider stages a triangle as 4 float4");
 
+// Unique edge of the mesh collider (GpuColliderEdge, mesh.rs): end points, the largest
+// thickness and mean friction of its triangles, the normalised sum of their normals, bit 0 of
+// flags set when every one of them is single-sided.
+struct ColliderEdge {
+    float p0x, p0y, p0z, thickness;
+    float p1x, p1y, p1z, friction;
+    float nx, ny, nz;
+    uint32_t flags;
+};
+static_assert(sizeof(ColliderEdge) == 48, "GpuColliderEdge is 48 bytes");
+constexpr uint32_t COLLIDER_EDGE_TABLE = 32768;  // spatial_hash.rs:7 (DEFAULT_HASH_TABLE_SIZE)
+
 // Self-collision accumulator (SelfCollisionAccum, simulation/types.rs:145-178)
 struct Accum {
     int32_t dx, dy, dz; uint32_t count;

@@ -122,6 +122,16 @@ hipError_t launch_sc_solve_ee(hipStream_t s, const float4 *P, const float4 *X, c
 hipError_t launch_sc_apply(hipStream_t s, float4 *P, const float *local_len, Accum *acc, const SelfColArgs &a,
                            uint32_t block);
 
+// Collider-edge collision (edge_collision.wgsl:153-276): pushes into the accumulators' pin channel.
+struct EdgeColArgs {
+    float cell;
+    uint32_t table;
+    float margin_scale, margin_offset;
+};
+hipError_t launch_edge_collision(hipStream_t s, const float4 *P, const float4 *X, const uint2 *edges, uint32_t ne,
+                                 const ColliderEdge *ce, uint32_t nce, const uint32_t *starts, const uint32_t *sorted,
+                                 const EdgeColArgs &a, Accum *acc);
+
 // Pair cache (self_collision_algorithm 1).  Slots are vertex-major: vertex i owns
 // [i * quota, (i + 1) * quota) of each list; an unused slot has 0xFFFFFFFF in its first word.
 // V-T slot: (vertex, triangle c0, c1, c2 with c0 the smallest index); E-E slot: (vertex, its

@@ -218,6 +218,14 @@ hipError_t enqueue_substep_tail(tc_sim *s, float dt_sub, uint32_t pc, uint32_t l
         s->prof_end(q);
         if (e != hipSuccess) return e;
     }
+    // collider-edge collision and its apply (dispatch.rs:252-278)
+    if (collide && s->edge_collision && !s->mesh.e_v0.empty() &&
+        (!s->cols_h.empty() || !s->tris_h.empty() || s->sdf_nb > 0)) {
+        q = s->prof_begin("edge_collision");
+        e = enqueue_edge_collision(s, Pc);
+        s->prof_end(q);
+        if (e != hipSuccess) return e;
+    }
     if (self_collide && s->self_collision && s->pair_cache && !(s->pair_final_fallback && s->sub_last)) {
         // dispatch.rs:1554-1645: collect on the frame's first substep, then solve the cached pairs
         const int scope = s->prof_begin("self_collision_Outer");
@@ -1134,7 +1142,7 @@ int tc_sim_clear_colliders(tc_sim *s) {
     s->cols_h.clear();
     s->tris_h.clear();
     s->touch();
-    return TC_OK;
+    return upload_collider_edges(s);
 }
 
 int tc_sim_set_sdf_colliders(tc_sim *s, uint32_t w, uint32_t h, uint32_t d, const void *texels, size_t nbytes,
@@ -1275,7 +1283,7 @@ int tc_sim_set_mesh_triangles(tc_sim *s, const float *tris, uint32_t nt, const f
     TC_HIP(s->tri_cbox.upload(cbox.data(), cbox.size(), s->stream));
     TC_HIP(hipStreamSynchronize(s->stream));
     s->touch();
-    return TC_OK;
+    return upload_collider_edges(s);
 }
 
 int tc_sim_set_gravity(tc_sim *s, float gx, float gy, float gz) {

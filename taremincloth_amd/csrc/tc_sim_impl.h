@@ -170,6 +170,12 @@ struct tc_sim {
     DevBuf<tc::MeshTri> tris;
     DevBuf<float4> tri_bounds;
     DevBuf<float4> tri_cbox;           // super-cluster then cluster boxes (lo, hi) of the Morton-ordered triangles
+    // collider-edge collision (tc_edge_collision.hip): the triangles' unique edges and their grid
+    bool edge_collision = false;
+    float edge_margin_scale = 1.0f, edge_margin_offset = 0.0f;
+    DevBuf<tc::ColliderEdge> cedges;
+    DevBuf<uint32_t> cedge_starts, cedge_sorted;
+    uint32_t ncedges = 0;
     DevBuf<uint32_t> col_order;        // vertices in Morton order of their rest positions (collider lanes)
     int single_sided_recovery = 1;
 
@@ -384,6 +390,11 @@ struct tc_sim {
     tc::HashBufs vhash() const { return {vkeys.p, vcounts.p, vstarts.p, vsorted.p}; }
     tc::HashBufs ehash() const { return {ekeys.p, ecounts.p, estarts.p, esorted.p}; }
 };
+
+// Collider-edge collision (tc_edge_collision.hip): the edges' rebuild after the triangles change,
+// and the pass (kernel + apply) on P.
+int upload_collider_edges(tc_sim *s);
+hipError_t enqueue_edge_collision(tc_sim *s, float4 *P);
 
 // Dynamic pins (tc_pins.hip): the pin projection of the current table on P, the table's upload,
 // and the inverse mass a vertex takes when its state is set from outside.

@@ -251,7 +251,7 @@ class ClothSimulator:
         part of this build raise NotImplementedError unless they keep their inert value."""
         c = _sc.parse(json_str)
         inert = {"coupled_mode": 0, "coupled_collider": False, "substep_interval": 1, "ee_substep_interval": 1,
-                 "enable_edge_collision": False, "sewing_priority_enabled": False}
+                 "sewing_priority_enabled": False}
         if c["self_collision_algorithm"] == 2:
             raise NotImplementedError("SimConfig self_collision_algorithm=2: that solver option is not part of this build")
         for k, v in inert.items():
@@ -273,12 +273,15 @@ class ClothSimulator:
         self.set_pair_cache_options(c["pair_max_pairs"], c["pair_max_pairs"], c["pair_margin_mode"],
                                     c["pair_safety_margin"], c["pair_horizon_scale"], c["pair_max_horizon"])
         self.set_enable_pair_cache_final_fallback(c["enable_pair_final_fallback"])
+        self.set_enable_edge_collision(c["enable_edge_collision"])
+        self.set_edge_margin_scale(c["edge_margin_scale"])
+        self.set_edge_margin_offset(c["edge_margin_offset"])
         self.set_adaptive_substep_options(c["enable_adaptive_substep"], c["min_substeps"], c["max_substeps"])
         self.set_auto_compensate_iterations(c["auto_compensate_iterations"])
         self.set_strain_adaptive_options(c["enable_strain_adaptive"], c["strain_tolerance"])
         self.areal_density = float(min(max(c["areal_density"], 0.0), 100.0))
         # recorded as given: options of solvers this build does not have, kept at their inert values
-        for k in ("post_relaxation_iters", "auto_coupled_on_low_substeps", "edge_margin_scale", "edge_margin_offset", "sewing_stiffness", "sewing_shrink_speed",
+        for k in ("post_relaxation_iters", "auto_coupled_on_low_substeps", "sewing_stiffness", "sewing_shrink_speed",
                   "enable_sewing_lock", "sewing_lock_distance", "sewing_priority_threshold",
                   "sewing_priority_merge_dist", "sewing_priority_ramp_frames", "sewing_priority_max_frames",
                   "areal_density", "version"):
@@ -324,6 +327,35 @@ class ClothSimulator:
 
     def reset(self):
         _lib.check(self._L.tc_sim_reset(self._h))
+
+    # --- collider-edge collision (lib.rs:2735-2749) ------------------------------------------------
+    def set_enable_edge_collision(self, enable):
+        _lib.check(self._L.tc_sim_set_enable_edge_collision(self._h, 1 if enable else 0))
+        self._cfg["enable_edge_collision"] = bool(enable)
+
+    def _edge_margin(self, scale, offset):
+        sc, off = float(np.float32(max(scale, 1.0))), float(np.float32(max(offset, 0.0)))
+        _lib.check(self._L.tc_sim_set_edge_margin(self._h, sc, off))
+        self._cfg["edge_margin_scale"], self._cfg["edge_margin_offset"] = sc, off
+
+    def set_edge_margin_scale(self, scale):
+        self._edge_margin(float(scale), self._cfg["edge_margin_offset"])
+
+    def set_edge_margin_offset(self, offset):
+        self._edge_margin(self._cfg["edge_margin_scale"], float(offset))
+
+    def collider_edge_count(self) -> int:
+        n = C.c_uint32(0)
+        _lib.check(self._L.tc_sim_collider_edge_count(self._h, C.byref(n)))
+        return int(n.value)
+
+    def collider_edges(self):
+        """The unique collider edges: (n, 11) [p0, thickness, p1, friction, normal] and their flags."""
+        n = self.collider_edge_count()
+        buf = np.zeros((n, 12), np.float32)
+        if n:
+            _lib.check(self._L.tc_sim_get_collider_edges(self._h, _ptr(buf), n))
+        return buf[:, :11].copy(), buf[:, 11].view(np.uint32).copy()
 
     # --- pair-cache self-collision (lib.rs:2029-2107) ----------------------------------------------
     def _pair_state(self):
