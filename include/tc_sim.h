@@ -249,6 +249,21 @@ int tc_sim_collider_edge_count(const tc_sim *sim, uint32_t *n);
 /* The collider edges, 12 words each: p0 xyz, thickness, p1 xyz, friction, normal xyz, flags (u32 bits);
  * cap = edges the buffer holds (no reference counterpart: exposed for parity checks). */
 int tc_sim_get_collider_edges(tc_sim *sim, float *out, uint32_t cap);
+/* Wrinkle field (drape guide), lib.rs:3049-3137 -> simulation/mod.rs:2997-3130, wrinkle_field.wgsl:
+ * after predict, a radial acceleration around a bone read from an RGBA8 texture over the bone's
+ * cylinder (R valley strength, G crest strength, B / A valley / crest radius in [r_min, r_max]).
+ * set_wrinkle_field_params: frame[16] = origin xyz + influence radius, axis xyz + bone radius,
+ * normal xyz + stiffness, binormal xyz + blend weight; the z and r ranges and the two windows.
+ * get_wrinkle_field_params: out[22] in that order, then z_min, z_max, r_min, r_max, windows.
+ * set_wrinkle_field_texture_2d: width * height * 4 bytes (rows of RGBA, v = row); a zero size drops
+ * the texture (the pass then does nothing); also sets the z and r ranges. */
+int tc_sim_set_enable_wrinkle_field(tc_sim *sim, int enable);
+int tc_sim_wrinkle_field_enabled(const tc_sim *sim, int *enabled);
+int tc_sim_set_wrinkle_field_params(tc_sim *sim, const float *frame, float z_min, float z_max, float r_min,
+                                    float r_max, float valley_window, float crest_window);
+int tc_sim_get_wrinkle_field_params(const tc_sim *sim, float *out);
+int tc_sim_set_wrinkle_field_texture_2d(tc_sim *sim, uint32_t width, uint32_t height, const uint8_t *rgba,
+                                        size_t nbytes, float z_min, float z_max, float r_min, float r_max);
 /* out[10]: [active path (1|2), tiles, largest cone (vertices), cone vertices total,
  *           items per iteration, constraints per iteration, plan available (0|1), colour phases,
  *           tile size (vertices), block size of the per-iteration kernel] */

@@ -118,6 +118,7 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.orc_sim_active_pairs.restype = C.c_uint32
     L.orc_sim_active_pairs.argtypes = [_vp, C.c_int, _vp, _vp]
     L.orc_sim_set_edge_collision.argtypes = [_vp, C.c_int, C.c_float, C.c_float]
+    L.orc_sim_set_wrinkle.argtypes = [_vp, C.c_int, _f32p, _vp, C.c_uint32, C.c_uint32]
     L.orc_sim_collider_edges.restype = C.c_uint32
     L.orc_sim_collider_edges.argtypes = [_vp, _vp, _vp]
     _LIBS[precision] = L
@@ -256,6 +257,15 @@ class OracleSim:
         self._L.orc_sim_active_pairs(self._h, 0, _ptr(vt), _ptr(cnt))
         self._L.orc_sim_active_pairs(self._h, 1, _ptr(ee), None)
         return cnt, vt, ee
+
+    def set_wrinkle(self, on, params22, texture_bytes=None, width=0, height=0):
+        """The wrinkle field: on, its 22 parameters (origin + influence radius, axis + bone radius,
+        normal + stiffness, binormal + blend, z_min, z_max, r_min, r_max, valley / crest window), the
+        RGBA8 texture."""
+        tex = None if not texture_bytes else np.frombuffer(bytes(texture_bytes), np.uint8).copy()
+        self._wtex = tex
+        self._L.orc_sim_set_wrinkle(self._h, int(on), np.ascontiguousarray(params22, np.float32), _ptr(tex),
+                                    width, height)
 
     def set_edge_collision(self, on=True, margin_scale=1.0, margin_offset=0.0):
         self._L.orc_sim_set_edge_collision(self._h, int(on), margin_scale, margin_offset)

@@ -1053,6 +1053,8 @@ typedef struct {
     /* collider-edge collision: unique collider edges {p0, thickness, p1, friction, n, flags} and
      * their grid (32768 buckets) */
     int edge_collision; real edge_scale, edge_offset;
+    /* wrinkle field: parameters (origin4, axis4, normal4, binormal4, z/r ranges, windows) and texture */
+    int wrinkle_on; float wrinkle[22]; uint8_t *wtex; uint32_t ww, wh;
     float *cedges; uint32_t *cedge_flags, nce, *cstarts, *csorted;
     int32_t *jac;                                    /* per-vertex sums {dx, dy, dz, count} */
 } orc_sim;
@@ -1122,7 +1124,7 @@ void orc_sim_free(orc_sim *s) {
     free(s->hkeys); free(s->hstarts); free(s->hsorted); free(s->ekeys); free(s->estarts); free(s->esorted);
     free(s->sdf_tex); free(s->sdf_info); free(s->sdf_xf);
     free(s->pin_v); free(s->pin_t); free(s->pin_w); free(s->jac); free(s->pair_vt); free(s->pair_ee);
-    free(s->cedges); free(s->cedge_flags); free(s->cstarts); free(s->csorted);
+    free(s->cedges); free(s->cedge_flags); free(s->cstarts); free(s->csorted); free(s->wtex);
     free(s);
 }
 
@@ -1441,6 +1443,97 @@ static void k_pins(orc_sim *s, real dt_sub) {
         real alpha = compliance / (dt * dt);
         real corr = -dist / (inv_m + alpha);
         s->p[v] = add(s->p[v], mul(n, inv_m * corr));
+    }
+}
+
+/* The wrinkle field's parameters (on, 22 floats as tc_sim_get_wrinkle_field_params) and its RGBA8
+ * texture (w x h, rows along v; NULL or a zero size: none). */
+void orc_sim_set_wrinkle(orc_sim *s, int on, const float *params, const uint8_t *tex, uint32_t w, uint32_t h) {
+    s->wrinkle_on = on;
+    memcpy(s->wrinkle, params, sizeof(s->wrinkle));
+    free(s->wtex); s->wtex = NULL; s->ww = s->wh = 0;
+    if (tex && w && h) {
+        s->wtex = (uint8_t *)malloc((size_t)w * h * 4);
+        memcpy(s->wtex, tex, (size_t)w * h * 4);
+        s->ww = w; s->wh = h;
+    }
+}
+/* atan2 by octant reduction and the odd Taylor series of atan to a^15 on |a| <= tan(pi/8) (the
+ * product's atan2_det, operation for operation) */
+static float atan2_det(float y, float x) {
+    float ax = fabsf(x), ay = fabsf(y);
+    float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    if (!(mx > 0.0f)) return 0.0f;
+    float a = mn / mx, off = 0.0f;
+    if (a > 0.41421356f) { a = (a - 1.0f) / (a + 1.0f); off = 0.78539816f; }
+    float sq = a * a;
+    float p = -1.0f / 15.0f;
+    p = p * sq + 1.0f / 13.0f; p = p * sq - 1.0f / 11.0f; p = p * sq + 1.0f / 9.0f; p = p * sq - 1.0f / 7.0f;
+    p = p * sq + 1.0f / 5.0f; p = p * sq - 1.0f / 3.0f; p = p * sq + 1.0f;
+    float r = off + a * p;
+    if (ay > ax) r = 1.5707964f - r;
+    if (x < 0.0f) r = 3.1415927f - r;
+    if (y < 0.0f) r = -r;
+    return r;
+}
+float orc_atan2_det(float y, float x) { return atan2_det(y, x); }
+/* wrinkle_field.wgsl:58-154 with a bilinear sample (x = u W - 0.5, u repeating, v clamped) */
+static void k_wrinkle(orc_sim *s, real dt_sub) {
+    if (!s->wrinkle_on || !s->wtex) return;
+    const float *W = s->wrinkle;
+    const real EPS = (real)1e-7f, TWO_PI = (real)6.2831855f;
+    v3 org = mk(W[0], W[1], W[2]), axis = mk(W[4], W[5], W[6]), nrm = mk(W[8], W[9], W[10]), bin = mk(W[12], W[13], W[14]);
+    real infl = W[3], bone_r = W[7], stiff = W[11], blend = W[15];
+    real z_min = W[16], z_max = W[17], r_min = W[18], r_max = W[19], vwin = W[20], cwin = W[21];
+    int TW = (int)s->ww, TH = (int)s->wh;
+    for (uint32_t i = 0; i < s->nv; i++) {
+        if (s->w[i] <= 0) continue;
+        v3 diff = sub(s->p[i], org);
+        real z = dot(diff, axis);
+        if (z < z_min || z > z_max) continue;
+        v3 rv = sub(diff, mul(axis, z));
+        real r = len3(rv);
+        if (r <= (real)1e-5f) continue;
+        if (infl > 0 && r_max > 0 && r > r_max + infl) continue;
+        real theta = (real)atan2_det((float)dot(rv, bin), (float)dot(rv, nrm));
+        if (theta < 0) theta += TWO_PI;
+        real uq = theta / TWO_PI;
+        real u = uq - (real)trunc((double)uq);
+        real zl = rmax(z_max - z_min, EPS);
+        real vv = rclamp((z - z_min) / zl, 0, 1);
+        real fx = u * (real)TW - (real)0.5f, fy = vv * (real)TH - (real)0.5f;
+        real x0f = (real)floor((double)fx), y0f = (real)floor((double)fy);
+        real tx = fx - x0f, ty = fy - y0f;
+        int x0 = (int)x0f, y0 = (int)y0f;
+        int xa = ((x0 % TW) + TW) % TW, xb = (((x0 + 1) % TW) + TW) % TW;
+        int ya = y0 < 0 ? 0 : (y0 > TH - 1 ? TH - 1 : y0), yb = y0 + 1 < 0 ? 0 : (y0 + 1 > TH - 1 ? TH - 1 : y0 + 1);
+        real t[4];
+        for (int c = 0; c < 4; c++) {
+            real t00 = (real)((float)s->wtex[4 * ((size_t)ya * TW + xa) + c] / 255.0f);
+            real t10 = (real)((float)s->wtex[4 * ((size_t)ya * TW + xb) + c] / 255.0f);
+            real t01 = (real)((float)s->wtex[4 * ((size_t)yb * TW + xa) + c] / 255.0f);
+            real t11 = (real)((float)s->wtex[4 * ((size_t)yb * TW + xb) + c] / 255.0f);
+            t[c] = (t00 * (1 - tx) + t10 * tx) * (1 - ty) + (t01 * (1 - tx) + t11 * tx) * ty;
+        }
+        real rl = rmax(r_max - r_min, EPS);
+        real tv_r = r_min + t[2] * rl, tc_r = r_min + t[3] * rl;
+        real dt = rmax(dt_sub, (real)1e-8f);
+        real a_pull = 0, a_push = 0;
+        if (t[0] > 0) {
+            real vr = rmax(tv_r, bone_r + (real)0.002f);
+            real margin = rmax(0, r - vr);
+            a_pull = t[0] * blend * stiff * rclamp(margin / rmax(vwin, (real)0.001f), 0, 1);
+        }
+        if (t[1] > 0) {
+            real ex = rmax(0, tc_r - r);
+            a_push = t[1] * blend * stiff * rclamp(ex / rmax(cwin, (real)0.001f), 0, 1);
+        }
+        real a = a_push - a_pull;
+        if (!((real)fabs((double)a) > EPS)) continue;
+        v3 dir = r > EPS ? dvs(rv, r) : nrm;
+        v3 dv = mul(dir, a * dt);
+        s->v[i] = add(s->v[i], dv);
+        s->p[i] = add(s->p[i], mul(dv, dt));
     }
 }
 
@@ -2902,6 +2995,7 @@ static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
         memset(s->lam_d, 0, (m->nd ? m->nd : 1) * sizeof(real));
         memset(s->lam_b, 0, (m->nb ? m->nb : 1) * sizeof(real));
         k_predict(s, dt_sub);
+        k_wrinkle(s, dt_sub);
         for (uint32_t it = 0; it < s->iterations; it++) {
             k_pins(s, dt_sub);
             distance_pass(s, dt_sub);
@@ -2937,6 +3031,7 @@ void orc_sim_substep_parts(orc_sim *s, float dt_sub, int do_collide, int do_self
     memset(s->lam_d, 0, (m->nd ? m->nd : 1) * sizeof(real));
     memset(s->lam_b, 0, (m->nb ? m->nb : 1) * sizeof(real));
     k_predict(s, dt);
+    k_wrinkle(s, dt);
     for (uint32_t it = 0; it < s->iterations; it++) {
         k_pins(s, dt);
         distance_pass(s, dt);

@@ -25,7 +25,9 @@ EXPORTS = [
     "tc_sim_set_enable_pair_cache", "tc_sim_get_enable_pair_cache", "tc_sim_set_self_collision_algorithm",
     "tc_sim_get_self_collision_algorithm", "tc_sim_set_pair_cache_options", "tc_sim_set_enable_pair_cache_final_fallback",
     "tc_sim_pair_cache_state", "tc_sim_get_active_pairs", "tc_sim_set_enable_edge_collision", "tc_sim_set_edge_margin",
-    "tc_sim_collider_edge_count", "tc_sim_get_collider_edges", "tc_sim_add_sphere",
+    "tc_sim_collider_edge_count", "tc_sim_get_collider_edges", "tc_sim_set_enable_wrinkle_field",
+    "tc_sim_wrinkle_field_enabled", "tc_sim_set_wrinkle_field_params", "tc_sim_get_wrinkle_field_params",
+    "tc_sim_set_wrinkle_field_texture_2d", "tc_sim_add_sphere",
     "tc_sim_add_capsule", "tc_sim_add_plane", "tc_sim_clear_colliders", "tc_sim_set_mesh_triangles",
     "tc_sim_set_gravity", "tc_sim_set_damping", "tc_sim_set_damping_all", "tc_sim_set_solver_iterations",
     "tc_sim_set_enable_self_collision", "tc_sim_set_self_collision_options", "tc_sim_set_stiffness_all",
@@ -124,6 +126,11 @@ def load() -> C.CDLL:
         "tc_sim_set_edge_margin": (_i, [_vp, _f32, _f32]),
         "tc_sim_collider_edge_count": (_i, [_vp, C.POINTER(_u32)]),
         "tc_sim_get_collider_edges": (_i, [_vp, _vp, _u32]),
+        "tc_sim_set_enable_wrinkle_field": (_i, [_vp, _i]),
+        "tc_sim_wrinkle_field_enabled": (_i, [_vp, C.POINTER(_i)]),
+        "tc_sim_set_wrinkle_field_params": (_i, [_vp, _vp, _f32, _f32, _f32, _f32, _f32, _f32]),
+        "tc_sim_get_wrinkle_field_params": (_i, [_vp, _vp]),
+        "tc_sim_set_wrinkle_field_texture_2d": (_i, [_vp, _u32, _u32, _vp, C.c_size_t, _f32, _f32, _f32, _f32]),
         "tc_sim_get_active_pairs": (_i, [_vp, _vp, _vp, C.c_size_t, C.POINTER(C.c_size_t), _vp, C.c_size_t,
                                          C.POINTER(C.c_size_t)]),
         "tc_sim_add_sphere": (_i, [_vp, _vp, _f32, _f32, _f32]),

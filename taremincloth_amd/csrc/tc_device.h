@@ -54,6 +54,32 @@ __device__ __forceinline__ uint32_t hash_cell(i3 c, uint32_t table) {
     return n % table;
 }
 
+// atan2 by octant reduction: the ratio min/max of |x|, |y| in [0, 1], folded to |a| <= tan(pi/8)
+// by atan(a) = pi/4 + atan((a - 1) / (a + 1)), then the odd Taylor series of atan to a^15
+// (truncation below 2e-8 rad), and the octant restored.  The same operations on the host give the
+// same bits, which a library atan2f does not promise.
+__host__ __device__ __forceinline__ float atan2_det(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    if (!(mx > 0.0f)) return 0.0f;
+    float a = mn / mx, off = 0.0f;
+    if (a > 0.41421356f) { a = (a - 1.0f) / (a + 1.0f); off = 0.78539816f; }
+    const float s = a * a;
+    float p = -1.0f / 15.0f;
+    p = p * s + 1.0f / 13.0f;
+    p = p * s - 1.0f / 11.0f;
+    p = p * s + 1.0f / 9.0f;
+    p = p * s - 1.0f / 7.0f;
+    p = p * s + 1.0f / 5.0f;
+    p = p * s - 1.0f / 3.0f;
+    p = p * s + 1.0f;
+    float r = off + a * p;
+    if (ay > ax) r = 1.5707964f - r;
+    if (x < 0.0f) r = 3.1415927f - r;
+    if (y < 0.0f) r = -r;
+    return r;
+}
+
 // Collider primitive (GpuCollider, mesh.rs:65-74): type 0 sphere, 1 capsule, 2 plane.
 struct Collider {
     uint32_t type;

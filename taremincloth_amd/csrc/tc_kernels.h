@@ -122,6 +122,16 @@ hipError_t launch_sc_solve_ee(hipStream_t s, const float4 *P, const float4 *X, c
 hipError_t launch_sc_apply(hipStream_t s, float4 *P, const float *local_len, Accum *acc, const SelfColArgs &a,
                            uint32_t block);
 
+// Wrinkle field (WrinkleFieldParams, simulation/types.rs:110-139)
+struct WrinkleArgs {
+    float origin[3] = {0.0f, 0.0f, 0.0f}, influence_radius = 0.0f;
+    float axis[3] = {0.0f, 0.0f, 0.0f}, bone_radius = 0.0f;
+    float normal[3] = {0.0f, 0.0f, 0.0f}, stiffness = 0.0f;
+    float binormal[3] = {0.0f, 0.0f, 0.0f}, blend = 0.0f;
+    float z_min = 0.0f, z_max = 0.0f, r_min = 0.0f, r_max = 0.0f;
+    float valley_window = 0.015f, crest_window = 0.020f;
+};
+
 // Collider-edge collision (edge_collision.wgsl:153-276): pushes into the accumulators' pin channel.
 struct EdgeColArgs {
     float cell;

@@ -146,6 +146,12 @@ hipError_t enqueue_substep_head(tc_sim *s, float dt_sub, bool gate_to_host, uint
     hipError_t e = tc::launch_predict(s->stream, s->P.p, s->X.p, s->V.p, n, a, B);
     s->prof_end(q);
     if (e != hipSuccess) return e;
+    if (s->wrinkle_on && s->wrinkle_tex.p) {  // dispatch.rs:151-165
+        q = s->prof_begin("wrinkle_field");
+        e = enqueue_wrinkle(s, dt_sub);
+        s->prof_end(q);
+        if (e != hipSuccess) return e;
+    }
     if (s->ws_tracking()) {
         q = s->prof_begin("warm_start");
         e = ws_pre(s, dt_sub);

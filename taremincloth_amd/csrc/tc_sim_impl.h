@@ -170,6 +170,11 @@ struct tc_sim {
     DevBuf<tc::MeshTri> tris;
     DevBuf<float4> tri_bounds;
     DevBuf<float4> tri_cbox;           // super-cluster then cluster boxes (lo, hi) of the Morton-ordered triangles
+    // wrinkle field (tc_wrinkle.hip): parameters and the RGBA8 texture
+    bool wrinkle_on = false;
+    tc::WrinkleArgs wrinkle;
+    DevBuf<uchar4> wrinkle_tex;
+    uint32_t wrinkle_w = 0, wrinkle_h = 0;
     // collider-edge collision (tc_edge_collision.hip): the triangles' unique edges and their grid
     bool edge_collision = false;
     float edge_margin_scale = 1.0f, edge_margin_offset = 0.0f;
@@ -390,6 +395,9 @@ struct tc_sim {
     tc::HashBufs vhash() const { return {vkeys.p, vcounts.p, vstarts.p, vsorted.p}; }
     tc::HashBufs ehash() const { return {ekeys.p, ecounts.p, estarts.p, esorted.p}; }
 };
+
+// Wrinkle field after predict (tc_wrinkle.hip); nothing when off or without a texture.
+hipError_t enqueue_wrinkle(tc_sim *s, float dt_sub);
 
 // Collider-edge collision (tc_edge_collision.hip): the edges' rebuild after the triangles change,
 // and the pass (kernel + apply) on P.

@@ -328,6 +328,44 @@ class ClothSimulator:
     def reset(self):
         _lib.check(self._L.tc_sim_reset(self._h))
 
+    # --- wrinkle field (lib.rs:3049-3137) ----------------------------------------------------------
+    def set_enable_wrinkle_field(self, enable):
+        _lib.check(self._L.tc_sim_set_enable_wrinkle_field(self._h, 1 if enable else 0))
+
+    def wrinkle_field_enabled(self) -> bool:
+        e = C.c_int(0)
+        _lib.check(self._L.tc_sim_wrinkle_field_enabled(self._h, C.byref(e)))
+        return bool(e.value)
+
+    def _wrinkle_params(self):
+        out = np.zeros(22, np.float32)
+        _lib.check(self._L.tc_sim_get_wrinkle_field_params(self._h, _ptr(out)))
+        return out
+
+    def set_wrinkle_field_params(self, origin, axis, normal, binormal, influence_radius=0.05, bone_radius=0.05,
+                                 stiffness=1.0, blend_weight=1.0, z_min=None, z_max=None, r_min=None, r_max=None,
+                                 valley_window=0.015, crest_window=0.020):
+        cur = self._wrinkle_params()
+        frame = np.concatenate([_vec3(origin, "origin"), [influence_radius], _vec3(axis, "axis"), [bone_radius],
+                                _vec3(normal, "normal"), [stiffness], _vec3(binormal, "binormal"),
+                                [blend_weight]]).astype(np.float32)
+        zr = (cur[16] if z_min is None else z_min, cur[17] if z_max is None else z_max)
+        rr = (cur[18] if r_min is None else r_min, cur[19] if r_max is None else r_max)
+        vw = valley_window if valley_window > 0.0 else 0.015  # lib.rs:3093-3094
+        cw = crest_window if crest_window > 0.0 else 0.020
+        _lib.check(self._L.tc_sim_set_wrinkle_field_params(self._h, _ptr(frame), float(zr[0]), float(zr[1]),
+                                                           float(rr[0]), float(rr[1]), float(vw), float(cw)))
+
+    def set_wrinkle_field_texture_2d(self, width, height, texture_bytes, z_min, z_max, r_min, r_max):
+        data = bytes(texture_bytes)
+        if len(data) != int(width) * int(height) * 4:
+            raise ValueError(f"texture_bytes has the wrong size: actual={len(data)}, "
+                             f"expected={int(width) * int(height) * 4} (width={width}, height={height})")
+        buf = np.frombuffer(data, np.uint8) if data else np.zeros(0, np.uint8)
+        _lib.check(self._L.tc_sim_set_wrinkle_field_texture_2d(self._h, int(width), int(height), _ptr(buf),
+                                                               len(data), float(z_min), float(z_max),
+                                                               float(r_min), float(r_max)))
+
     # --- collider-edge collision (lib.rs:2735-2749) ------------------------------------------------
     def set_enable_edge_collision(self, enable):
         _lib.check(self._L.tc_sim_set_enable_edge_collision(self._h, 1 if enable else 0))
