@@ -23,6 +23,7 @@ pub const TC_WS_NOALLOC_SCRATCH_BYTES: u32 = 13172992;
 #[repr(C)] pub struct TcSim { _private: [u8; 0] }
 #[repr(C)] pub struct TcWsNet { _private: [u8; 0] }
 #[repr(C)] pub struct TcSdfResult { _private: [u8; 0] }
+#[repr(C)] pub struct TcBoneSdfResult { _private: [u8; 0] }
 
 #[repr(C)]
 #[derive(Clone, Copy, Debug)]
@@ -161,6 +162,10 @@ extern "C" {
     pub fn tc_sdf_result_atlas(res: *const TcSdfResult, brick_dims: *mut u32) -> *const f32;
     pub fn tc_sdf_result_nearest(res: *const TcSdfResult) -> *const u32;
     pub fn tc_sdf_result_free(res: *mut TcSdfResult);
+    pub fn tc_bake_bone_sdf(verts: *const f32, num_verts: u32, tris: *const i32, num_tris: u32, weights: *const f32, binds: *const f32, num_bones: u32, resolution: u32, margin: f32, weight_threshold: f32, blend_k: f32, friction: f32, thickness: f32, restitution: f32, close_open_boundaries: i32, out: *mut *mut TcBoneSdfResult) -> i32;
+    pub fn tc_bone_sdf_result_info(res: *const TcBoneSdfResult, dims: *mut u32, n_active: *mut u32, kernel_ms: *mut f32) -> i32;
+    pub fn tc_bone_sdf_result_arrays(res: *const TcBoneSdfResult, texels: *mut u32, infos: *mut f32, active: *mut u32, binds: *mut f32) -> i32;
+    pub fn tc_bone_sdf_result_free(res: *mut TcBoneSdfResult);
     pub fn tc_close_open_boundary_loops(verts: *const f32, num_verts: u32, tris: *const i32, num_tris: u32, out_verts: *mut f32, cap_verts: usize, out_tris: *mut i32, cap_tris: usize, num_verts_out: *mut u32, num_tris_out: *mut u32, loops_closed: *mut u32) -> i32;
     pub fn tc_mesh_build_arrays(positions: *const f32, num_vertices: u32, edges: *const u32, num_edges: u32, faces: *const u32, num_faces: u32, inv_masses: *const f32, thicknesses: *const f32, params: *const TcParams, which: i32, out: *mut c_void, capacity: usize, count: *mut usize) -> i32;
     pub fn tc_color_edge_pairs(num_vertices: u32, pairs: *const u32, num_pairs: u32, colors_out: *mut u32) -> i32;

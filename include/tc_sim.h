@@ -463,6 +463,29 @@ const uint32_t *tc_sdf_result_texels(const tc_sdf_result *res);
 const float *tc_sdf_result_atlas(const tc_sdf_result *res, uint32_t *brick_dims);
 const uint32_t *tc_sdf_result_nearest(const tc_sdf_result *res);
 void tc_sdf_result_free(tc_sdf_result *res);
+/* ---- Bone SDF atlas bake (module function bake_bone_sdf_gpu, lib.rs:1190-1348 ->
+ * bake_bone_sdf_gpu_ext, sdf_baker.rs:494-930, kernel bake_sdf.wgsl).  weights: num_bones x
+ * num_verts floats, bone-major; binds: num_bones row-major 4x4 bind (world -> bone) matrices.
+ * Bones whose largest weight exceeds weight_threshold each get a res^3 tile of one 3D atlas
+ * (compute_3d_atlas_layout, sdf_baker.rs:137-163, at most 2048 texels per axis); each texel holds
+ * pack2x16float(signed distance to the bone's skinned triangles in bone space, bone weight at the
+ * nearest point) in Z/Y/X order.  TC_ERR_VALUE for resolution 0 or null arrays; TC_ERR_RUNTIME when
+ * no bone passes the threshold or the atlas does not fit. */
+typedef struct tc_bone_sdf_result tc_bone_sdf_result;
+int tc_bake_bone_sdf(const float *verts, uint32_t num_verts, const int32_t *tris, uint32_t num_tris,
+                     const float *weights, const float *binds, uint32_t num_bones, uint32_t resolution, float margin,
+                     float weight_threshold, float blend_k, float friction, float thickness, float restitution,
+                     int close_open_boundaries, tc_bone_sdf_result **out);
+/* dims[3] = width, height, depth; n_active = active bones; kernel_ms = device time of the bake
+ * kernel (HIP events; no reference counterpart).  Each may be NULL. */
+int tc_bone_sdf_result_info(const tc_bone_sdf_result *res, uint32_t *dims, uint32_t *n_active, float *kernel_ms);
+/* texels (w*h*d u32), infos (20 floats per active bone: local min, tile index, local max, res,
+ * atlas uvw scale + blend_k, atlas uvw offset + 0, friction, thickness, restitution, weight
+ * threshold; sdf_baker.rs:603-615), active (input index of each active bone), binds (16 floats per
+ * active bone).  Each may be NULL. */
+int tc_bone_sdf_result_arrays(const tc_bone_sdf_result *res, uint32_t *texels, float *infos, uint32_t *active,
+                              float *binds);
+void tc_bone_sdf_result_free(tc_bone_sdf_result *res);
 /* close_open_boundary_loops_mesh (sdf_baker.rs:455-462): caps the closed boundary loops of a solid
  * mesh.  With NULL outputs only the counts are written. */
 int tc_close_open_boundary_loops(const float *verts, uint32_t num_verts, const int32_t *tris, uint32_t num_tris,

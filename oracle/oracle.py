@@ -118,6 +118,13 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.orc_sim_active_pairs.restype = C.c_uint32
     L.orc_sim_active_pairs.argtypes = [_vp, C.c_int, _vp, _vp]
     L.orc_sim_set_edge_collision.argtypes = [_vp, C.c_int, C.c_float, C.c_float]
+    L.orc_atlas_layout.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, _u32p]
+    L.orc_bone_sdf_bake.restype = _vp
+    L.orc_bone_sdf_bake.argtypes = [_f32p, C.c_uint32, _vp, C.c_uint32, _f32p, _f32p, C.c_uint32, C.c_uint32,
+                                    C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]
+    L.orc_bone_sdf_info.argtypes = [_vp, _u32p, C.POINTER(C.c_uint32)]
+    L.orc_bone_sdf_arrays.argtypes = [_vp, _u32p, _f32p, _u32p, _f32p]
+    L.orc_bone_sdf_free.argtypes = [_vp]
     L.orc_sim_set_wrinkle.argtypes = [_vp, C.c_int, _f32p, _vp, C.c_uint32, C.c_uint32]
     L.orc_sim_collider_edges.restype = C.c_uint32
     L.orc_sim_collider_edges.argtypes = [_vp, _vp, _vp]
@@ -496,6 +503,46 @@ def close_open_boundary_loops(verts, tris, precision: str = "f32"):
     loops = L.orc_close_loops_copy(v.reshape(-1), len(v), _ptr(t), len(t), _ptr(ov), ov.size, _ptr(ot), ot.size,
                                    C.byref(nv), C.byref(nt))
     return ov, ot, int(loops)
+
+
+def atlas_layout(n_bones, resolution, max_dim=2048):
+    """compute_3d_atlas_layout (sdf_baker.rs:137-163): (cols, rows, layers, res, width, height, depth)."""
+    o = np.zeros(7, np.uint32)
+    lib("f32").orc_atlas_layout(int(n_bones), int(resolution), int(max_dim), o)
+    return tuple(int(x) for x in o)
+
+
+def bake_bone_sdf(verts, tris, weights, binds, resolution=64, margin=0.2, weight_threshold=0.02, blend_k=0.05,
+                  friction=0.5, thickness=0.005, restitution=0.0, close_open_boundaries=True,
+                  precision: str = "f32"):
+    """Bone SDF atlas bake restated (sdf_baker.rs:494-930, bake_sdf.wgsl).  weights (n_bones, n_verts),
+    binds (n_bones, 4, 4).  Returns texels, width, height, depth, bone_infos, active (input indices),
+    bind_matrices, or None where the reference bake fails (no bone above the threshold)."""
+    L = lib(precision)
+    v = np.ascontiguousarray(np.asarray(verts, np.float32).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(tris, np.int32).reshape(-1, 3))
+    w = np.ascontiguousarray(np.asarray(weights, np.float32).reshape(-1))
+    b = np.ascontiguousarray(np.asarray(binds, np.float32).reshape(-1))
+    nb = b.size // 16
+    h = L.orc_bone_sdf_bake(v.reshape(-1), len(v), _ptr(t), len(t), w if w.size else np.zeros(1, np.float32),
+                            b if b.size else np.zeros(1, np.float32), nb, int(resolution), margin, weight_threshold,
+                            blend_k, friction, thickness, restitution, 1 if close_open_boundaries else 0)
+    if not h:
+        return None
+    try:
+        dims = np.zeros(3, np.uint32)
+        na = C.c_uint32()
+        L.orc_bone_sdf_info(h, dims, C.byref(na))
+        n = int(na.value)
+        tex = np.empty(int(dims[0]) * int(dims[1]) * int(dims[2]), np.uint32)
+        infos = np.empty((n, 20), np.float32)
+        act = np.empty(n, np.uint32)
+        mats = np.empty((n, 4, 4), np.float32)
+        L.orc_bone_sdf_arrays(h, tex, infos.reshape(-1), act, mats.reshape(-1))
+        return {"texels": tex, "width": int(dims[0]), "height": int(dims[1]), "depth": int(dims[2]),
+                "bone_infos": infos, "active": act, "bind_matrices": mats}
+    finally:
+        L.orc_bone_sdf_free(h)
 
 
 def bake_mesh_sdf(verts, tris, voxel_size=0.004, margin=0.02, friction=0.5, thickness=0.005, restitution=0.0,

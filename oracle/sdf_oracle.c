@@ -138,8 +138,11 @@ static uint32_t edge_count(const ecount *ec, size_t n, int32_t u, int32_t v) {
 }
 
 /* Caps closed boundary loops.  verts (3 x nv floats) and tris (3 x nt) are grown in place through
- * the returned arrays; *nv_io / *nt_io are updated.  Returns the loops closed. */
-uint32_t orc_close_open_boundary_loops(float **verts_io, uint32_t *nv_io, int32_t **tris_io, uint32_t *nt_io) {
+ * the returned arrays; *nv_io / *nt_io are updated.  With bone weights (wb[b] holds nv floats for
+ * each of nbones bones, grown in place) each cap centre takes the mean weight of its loop
+ * (sdf_baker.rs:427-439).  Returns the loops closed. */
+static uint32_t close_loops_weighted(float **verts_io, uint32_t *nv_io, int32_t **tris_io, uint32_t *nt_io,
+                                     float **wb, uint32_t nbones) {
     const uint32_t n = *nv_io, m = *nt_io;
     if (m < 4 || n < 4) return 0;
     float *V = *verts_io;
@@ -262,6 +265,12 @@ uint32_t orc_close_open_boundary_loops(float **verts_io, uint32_t *nv_io, int32_
             const int32_t ci = (int32_t)*nv_io;
             V = (float *)realloc(V, sizeof(float) * 3 * (*nv_io + 1));
             memcpy(&V[3 * (size_t)ci], center, sizeof(center));
+            for (uint32_t b = 0; b < nbones; b++) {
+                float ws = 0.0f;
+                for (size_t i = 0; i < k; i++) ws += wb[b][loop[i]];
+                wb[b] = (float *)realloc(wb[b], sizeof(float) * (*nv_io + 1));
+                wb[b][ci] = ws * inv_k;
+            }
             (*nv_io)++;
             T = (int32_t *)realloc(T, sizeof(int32_t) * 3 * (*nt_io + k));
             for (size_t i = 0; i < k; i++) {
@@ -276,6 +285,10 @@ uint32_t orc_close_open_boundary_loops(float **verts_io, uint32_t *nv_io, int32_
     *verts_io = V;
     *tris_io = T;
     return closed;
+}
+
+uint32_t orc_close_open_boundary_loops(float **verts_io, uint32_t *nv_io, int32_t **tris_io, uint32_t *nt_io) {
+    return close_loops_weighted(verts_io, nv_io, tris_io, nt_io, NULL, 0);
 }
 
 typedef struct { float min[3], max[3], size[3]; uint32_t w, h, d; } odomain;
@@ -720,4 +733,280 @@ uint32_t orc_close_loops_copy(const float *verts, uint32_t nv, const int32_t *tr
     free(V);
     free(T);
     return loops;
+}
+
+/* ------------------------------------------------------------------ bone SDF atlas bake */
+/* compute_3d_atlas_layout (sdf_baker.rs:137-163): o = cols, rows, layers, res, width, height, depth */
+void orc_atlas_layout(uint32_t n, uint32_t res, uint32_t max_dim, uint32_t *o) {
+    if (!n) { o[0] = o[1] = o[2] = 0; o[3] = res; o[4] = o[5] = o[6] = 0; return; }
+    uint32_t cols = (uint32_t)ceil(pow((double)n, 1.0 / 3.0));
+    if (cols < 1) cols = 1;
+    uint32_t rows = (uint32_t)ceil(sqrt((double)((n + cols - 1) / cols)));
+    if (rows < 1) rows = 1;
+    uint32_t layers = (n + cols * rows - 1) / (cols * rows);
+    if (layers < 1) layers = 1;
+    uint32_t mt = cols > rows ? cols : rows;
+    if (layers > mt) mt = layers;
+    uint32_t r = res;
+    if ((uint64_t)mt * res > max_dim) { r = max_dim / mt; if (r < 16) r = 16; }
+    o[0] = cols; o[1] = rows; o[2] = layers; o[3] = r; o[4] = cols * r; o[5] = rows * r; o[6] = layers * r;
+}
+
+/* transform_point (sdf_baker.rs:167-178) */
+static void bone_xform(const float *m, const float *p, float *o) {
+    const float x = m[0] * p[0] + m[1] * p[1] + m[2] * p[2] + m[3];
+    const float y = m[4] * p[0] + m[5] * p[1] + m[6] * p[2] + m[7];
+    const float z = m[8] * p[0] + m[9] * p[1] + m[10] * p[2] + m[11];
+    const float w = m[12] * p[0] + m[13] * p[1] + m[14] * p[2] + m[15];
+    if (fabsf(w) > 1e-7f && fabsf(w - 1.0f) > 1e-5f) {
+        const float iw = 1.0f / w;
+        o[0] = x * iw; o[1] = y * iw; o[2] = z * iw;
+    } else {
+        o[0] = x; o[1] = y; o[2] = z;
+    }
+}
+
+typedef struct { float p[3][3], w[3], n[3], bmask; } obtri;
+typedef struct { float lmin[3], lmax[3]; uint32_t start, count, col, row, layer; } obparams;
+typedef struct orc_bone_sdf {
+    uint32_t w, h, d, nact;
+    uint32_t *texels, *active;
+    float *infos, *binds;
+} orc_bone_sdf;
+
+static inline s3 fv(const float *p) { return sk(p[0], p[1], p[2]); }
+static inline s3 scross(s3 a, s3 b) { return sk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline s3 snorm(s3 v) { const real l = ssqrt(sdot(v, v)); return sk(v.x / l, v.y / l, v.z / l); }
+static inline real smax(real a, real b) { return (a != a) ? b : (b != b) ? a : (a > b ? a : b); }
+
+/* bake_sdf.wgsl:34-196 for one voxel of one bone: the nearest triangle by Ericson's region test
+ * (Real-Time Collision Detection 5.1.5) in triangle order, near ties within 1e-7 max(1, d^2) broken
+ * toward the larger |normal projection|, the open-rim sign fix, and the weight at the nearest point. */
+static uint32_t bone_voxel(const obparams *P, const obtri *tr, uint32_t res, uint32_t vx, uint32_t vy, uint32_t vz) {
+    const real fr = (real)res;
+    const real u = ((real)vx + (real)0.5) / fr, v = ((real)vy + (real)0.5) / fr, ww = ((real)vz + (real)0.5) / fr;
+    const s3 lmin = fv(P->lmin), lmax = fv(P->lmax);
+    const s3 lp = ssub(lmax, lmin);
+    const s3 p = sk(lmin.x + u * lp.x, lmin.y + v * lp.y, lmin.z + ww * lp.z);
+    real min_dsq = (real)1e10f, best_abs = -1, best_sd = 10, best_alpha = 0;
+    for (uint32_t i = 0; i < P->count; i++) {
+        const obtri *T = &tr[P->start + i];
+        const s3 a = fv(T->p[0]), b = fv(T->p[1]), c = fv(T->p[2]);
+        const s3 ab = ssub(b, a), ac = ssub(c, a), ap = ssub(p, a);
+        const real d1 = sdot(ab, ap), d2 = sdot(ac, ap);
+        s3 cp = a, bary = sk(1, 0, 0);
+        if (!(d1 <= 0 && d2 <= 0)) {
+            const s3 bp = ssub(p, b);
+            const real d3 = sdot(ab, bp), d4 = sdot(ac, bp);
+            const real vc = d1 * d4 - d3 * d2;
+            if (d3 >= 0 && d4 <= d3) { cp = b; bary = sk(0, 1, 0); }
+            else if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+                const real t = d1 / (d1 - d3);
+                cp = sadd(a, smul(ab, t)); bary = sk(1 - t, t, 0);
+            } else {
+                const s3 cq = ssub(p, c);
+                const real d5 = sdot(ab, cq), d6 = sdot(ac, cq);
+                const real vb = d5 * d2 - d1 * d6;
+                if (d6 >= 0 && d5 <= d6) { cp = c; bary = sk(0, 0, 1); }
+                else if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+                    const real t = d2 / (d2 - d6);
+                    cp = sadd(a, smul(ac, t)); bary = sk(1 - t, 0, t);
+                } else {
+                    const real va = d3 * d6 - d5 * d4;
+                    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+                        const real t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+                        cp = sadd(b, smul(ssub(c, b), t)); bary = sk(0, 1 - t, t);
+                    } else {
+                        const real den = 1 / (va + vb + vc);
+                        const real sv = vb * den, sw = vc * den;
+                        cp = sadd(sadd(a, smul(ab, sv)), smul(ac, sw)); bary = sk(1 - sv - sw, sv, sw);
+                    }
+                }
+            }
+        }
+        const s3 diff = ssub(p, cp);
+        const real dsq = sdot(diff, diff);
+        const s3 tn = fv(T->n);
+        const real proj = sdot(diff, tn), aproj = (real)fabs((double)proj);
+        const real eps_sq = (real)1e-7f * (min_dsq > 1 ? min_dsq : (real)1);
+        const int closer = dsq + eps_sq < min_dsq;
+        const int tie = (real)fabs((double)(dsq - min_dsq)) <= eps_sq && aproj > best_abs;
+        if (!(closer || tie)) continue;
+        if (dsq < min_dsq) min_dsq = dsq;
+        best_abs = aproj;
+        const real d = ssqrt(dsq);
+        real sign = proj >= 0 ? 1 : -1;
+        const uint32_t bm = (uint32_t)(T->bmask + 0.5f);
+        if (bm && sign < 0) {
+            real op = -1;
+            if ((bm & 1u) && bary.z <= (real)1e-5f) op = smax(op, sdot(diff, snorm(scross(ssub(b, a), tn))));
+            if ((bm & 2u) && bary.x <= (real)1e-5f) op = smax(op, sdot(diff, snorm(scross(ssub(c, b), tn))));
+            if ((bm & 4u) && bary.y <= (real)1e-5f) op = smax(op, sdot(diff, snorm(scross(ssub(a, c), tn))));
+            if (op > 0 && op >= aproj * (real)0.25) sign = 1;
+        }
+        best_sd = d * sign;
+        best_alpha = bary.x * T->w[0] + bary.y * T->w[1] + bary.z * T->w[2];
+    }
+    return (uint32_t)orc_f32_to_f16((float)best_sd) | ((uint32_t)orc_f32_to_f16((float)best_alpha) << 16);
+}
+
+/* bake_bone_sdf_gpu_ext (sdf_baker.rs:494-930).  weights: nbones x nv (bone-major); binds: nbones
+ * row-major 4x4 bind (world -> bone) matrices.  Returns NULL when no bone has a weight above the
+ * threshold or resolution is 0. */
+orc_bone_sdf *orc_bone_sdf_bake(const float *verts, uint32_t nv, const int32_t *tris, uint32_t nt,
+                                const float *weights, const float *binds, uint32_t nbones, uint32_t resolution,
+                                float margin, float wth, float blend_k, float friction, float thickness,
+                                float restitution, int close_open) {
+    if (!resolution) return NULL;
+    float *V = (float *)malloc(sizeof(float) * 3 * (nv ? nv : 1));
+    int32_t *T = (int32_t *)malloc(sizeof(int32_t) * 3 * (nt ? nt : 1));
+    if (nv) memcpy(V, verts, sizeof(float) * 3 * nv);
+    if (nt) memcpy(T, tris, sizeof(int32_t) * 3 * nt);
+    float **W = (float **)malloc(sizeof(float *) * (nbones ? nbones : 1));
+    for (uint32_t b = 0; b < nbones; b++) {
+        W[b] = (float *)malloc(sizeof(float) * (nv ? nv : 1));
+        if (nv) memcpy(W[b], weights + (size_t)b * nv, sizeof(float) * nv);
+    }
+    uint32_t n = nv, m = nt;
+    if (close_open) close_loops_weighted(&V, &n, &T, &m, W, nbones);
+    /* undirected edge multiplicities over the valid triangles (sdf_baker.rs:203-240) */
+    ecount *ec = (ecount *)malloc(sizeof(ecount) * 3 * (m ? m : 1));
+    size_t nec = 0;
+    for (uint32_t t = 0; t < m; t++) {
+        const int32_t *tr = &T[3 * t];
+        if (!tri_ok(tr, n)) continue;
+        for (int e = 0; e < 3; e++) {
+            const int32_t u = tr[e], v = tr[(e + 1) % 3];
+            ecount c = {u < v ? u : v, u < v ? v : u, 1};
+            ec[nec++] = c;
+        }
+    }
+    qsort(ec, nec, sizeof(ecount), cmp_ecount);
+    size_t q = 0;
+    for (size_t i = 0; i < nec; i++) {
+        if (q && ec[q - 1].a == ec[i].a && ec[q - 1].b == ec[i].b) ec[q - 1].count++;
+        else ec[q++] = ec[i];
+    }
+    nec = q;
+    uint32_t *act = (uint32_t *)malloc(sizeof(uint32_t) * (nbones ? nbones : 1)), na = 0;
+    for (uint32_t b = 0; b < nbones; b++) {
+        float mx = 0.0f;
+        for (uint32_t v = 0; v < n; v++) mx = fmaxf(mx, W[b][v]);
+        if (mx > wth) act[na++] = b;
+    }
+    orc_bone_sdf *r = NULL;
+    if (na) {
+        uint32_t L[7];
+        orc_atlas_layout(na, resolution, 2048, L);
+        const uint32_t cols = L[0], rows = L[1], layers = L[2], res = L[3];
+        r = (orc_bone_sdf *)calloc(1, sizeof(orc_bone_sdf));
+        r->w = L[4]; r->h = L[5]; r->d = L[6]; r->nact = na;
+        r->active = act;
+        act = NULL;
+        r->infos = (float *)malloc(sizeof(float) * 20 * na);
+        r->binds = (float *)malloc(sizeof(float) * 16 * na);
+        obparams *P = (obparams *)malloc(sizeof(obparams) * na);
+        obtri *tr = (obtri *)malloc(sizeof(obtri) * ((size_t)na * m + 1));
+        uint32_t ntr = 0;
+        for (uint32_t ob = 0; ob < na; ob++) {
+            const uint32_t sb = r->active[ob];
+            const float *mat = binds + 16 * (size_t)sb;
+            memcpy(&r->binds[16 * (size_t)ob], mat, 16 * sizeof(float));
+            float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY}, size[3];
+            uint32_t valid = 0;
+            for (uint32_t v = 0; v < n; v++)
+                if (W[sb][v] > wth) {
+                    float pl[3];
+                    bone_xform(mat, &V[3 * (size_t)v], pl);
+                    for (int k = 0; k < 3; k++) { mn[k] = fminf(mn[k], pl[k]); mx[k] = fmaxf(mx[k], pl[k]); }
+                    valid++;
+                }
+            if (!valid)
+                for (int k = 0; k < 3; k++) { mn[k] = -0.1f; mx[k] = 0.1f; }
+            for (int k = 0; k < 3; k++) size[k] = fmaxf(mx[k] - mn[k], 0.05f);
+            for (int k = 0; k < 3; k++) {
+                const float pad = size[k] * margin + fmaxf(thickness, 0.0f);
+                mn[k] -= pad; mx[k] += pad;
+                size[k] = mx[k] - mn[k];
+            }
+            const uint32_t bpl = cols * rows, l = ob / bpl, rem = ob % bpl, row = rem / cols, col = rem % cols;
+            float *info = &r->infos[20 * (size_t)ob];
+            const float nn[3] = {(float)cols, (float)rows, (float)layers}, ix[3] = {(float)col, (float)row, (float)l};
+            for (int k = 0; k < 3; k++) {
+                info[k] = mn[k];
+                info[4 + k] = mx[k];
+                info[8 + k] = (1.0f / size[k]) / nn[k];
+                info[12 + k] = (-mn[k] / size[k] + ix[k]) / nn[k];
+            }
+            info[3] = (float)ob; info[7] = (float)res; info[11] = blend_k; info[15] = 0.0f;
+            info[16] = friction; info[17] = thickness; info[18] = restitution; info[19] = wth;
+            const uint32_t start = ntr;
+            for (uint32_t t = 0; t < m; t++) {
+                const int32_t *tt = &T[3 * t];
+                if ((uint32_t)tt[0] >= n || (uint32_t)tt[1] >= n || (uint32_t)tt[2] >= n) continue;
+                const float w0 = W[sb][tt[0]], w1 = W[sb][tt[1]], w2 = W[sb][tt[2]];
+                if (!(w0 > 0.0f || w1 > 0.0f || w2 > 0.0f)) continue;
+                obtri o;
+                for (int c = 0; c < 3; c++) bone_xform(mat, &V[3 * (size_t)tt[c]], o.p[c]);
+                int ov = 1;
+                for (int k = 0; k < 3; k++) {
+                    const float tmin = fminf(fminf(o.p[0][k], o.p[1][k]), o.p[2][k]);
+                    const float tmax = fmaxf(fmaxf(o.p[0][k], o.p[1][k]), o.p[2][k]);
+                    ov = ov && tmin <= mx[k] && tmax >= mn[k];
+                }
+                if (!ov) continue;
+                const float e01[3] = {o.p[1][0] - o.p[0][0], o.p[1][1] - o.p[0][1], o.p[1][2] - o.p[0][2]};
+                const float e02[3] = {o.p[2][0] - o.p[0][0], o.p[2][1] - o.p[0][1], o.p[2][2] - o.p[0][2]};
+                float cr[3];
+                cross3(e01, e02, cr);
+                normalize3(cr, o.n);
+                uint32_t bm = 0;
+                if (edge_count(ec, nec, tt[0], tt[1]) == 1u) bm |= 1u;
+                if (edge_count(ec, nec, tt[1], tt[2]) == 1u) bm |= 2u;
+                if (edge_count(ec, nec, tt[2], tt[0]) == 1u) bm |= 4u;
+                o.w[0] = w0; o.w[1] = w1; o.w[2] = w2;
+                o.bmask = (float)bm;
+                tr[ntr++] = o;
+            }
+            P[ob].start = start; P[ob].count = ntr - start;
+            P[ob].col = col; P[ob].row = row; P[ob].layer = l;
+            memcpy(P[ob].lmin, mn, sizeof(mn));
+            memcpy(P[ob].lmax, mx, sizeof(mx));
+        }
+        const size_t nvox = (size_t)r->w * r->h * r->d;
+        r->texels = (uint32_t *)calloc(nvox ? nvox : 1, sizeof(uint32_t));
+        for (uint32_t ob = 0; ob < na; ob++) {
+            const obparams *pp = &P[ob];
+            ORC_PARALLEL_FOR
+            for (int32_t vz = 0; vz < (int32_t)res; vz++)
+                for (uint32_t vy = 0; vy < res; vy++)
+                    for (uint32_t vx = 0; vx < res; vx++) {
+                        const size_t gx = pp->col * res + vx, gy = pp->row * res + vy, gz = pp->layer * res + (uint32_t)vz;
+                        r->texels[(gz * r->h + gy) * r->w + gx] = bone_voxel(pp, tr, res, vx, vy, (uint32_t)vz);
+                    }
+        }
+        free(P);
+        free(tr);
+    }
+    free(act);
+    free(ec);
+    for (uint32_t b = 0; b < nbones; b++) free(W[b]);
+    free(W); free(V); free(T);
+    return r;
+}
+
+void orc_bone_sdf_info(const orc_bone_sdf *r, uint32_t *dims, uint32_t *n_active) {
+    dims[0] = r->w; dims[1] = r->h; dims[2] = r->d;
+    *n_active = r->nact;
+}
+void orc_bone_sdf_arrays(const orc_bone_sdf *r, uint32_t *texels, float *infos, uint32_t *active, float *binds) {
+    if (texels) memcpy(texels, r->texels, 4 * (size_t)r->w * r->h * r->d);
+    if (infos) memcpy(infos, r->infos, sizeof(float) * 20 * r->nact);
+    if (active) memcpy(active, r->active, sizeof(uint32_t) * r->nact);
+    if (binds) memcpy(binds, r->binds, sizeof(float) * 16 * r->nact);
+}
+void orc_bone_sdf_free(orc_bone_sdf *r) {
+    if (!r) return;
+    free(r->texels); free(r->active); free(r->infos); free(r->binds);
+    free(r);
 }

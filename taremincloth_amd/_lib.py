@@ -44,6 +44,7 @@ EXPORTS = [
     "tc_ws_eval", "tc_sim_ws_time", "tc_ws_net_forward_noalloc",
     "tc_sim_set_sdf_colliders", "tc_sim_update_bone_transforms", "tc_bake_mesh_sdf", "tc_sdf_result_info",
     "tc_sdf_result_timing", "tc_sdf_result_texels", "tc_sdf_result_atlas", "tc_sdf_result_nearest", "tc_sdf_result_free", "tc_close_open_boundary_loops",
+    "tc_bake_bone_sdf", "tc_bone_sdf_result_info", "tc_bone_sdf_result_arrays", "tc_bone_sdf_result_free",
 ]
 
 TC_OK, TC_ERR_VALUE, TC_ERR_RUNTIME, TC_ERR_UNSUPPORTED, TC_ERR_IO = 0, 1, 2, 3, 4
@@ -198,6 +199,11 @@ def load() -> C.CDLL:
         "tc_sdf_result_atlas": (_vp, [_vp, _vp]),
         "tc_sdf_result_nearest": (_vp, [_vp]),
         "tc_sdf_result_free": (None, [_vp]),
+        "tc_bake_bone_sdf": (_i, [_vp, _u32, _vp, _u32, _vp, _vp, _u32, _u32, _f32, _f32, _f32, _f32, _f32, _f32, _i,
+                                 C.POINTER(_vp)]),
+        "tc_bone_sdf_result_info": (_i, [_vp, _vp, _vp, _vp]),
+        "tc_bone_sdf_result_arrays": (_i, [_vp, _vp, _vp, _vp, _vp]),
+        "tc_bone_sdf_result_free": (None, [_vp]),
         "tc_close_open_boundary_loops": (_i, [_vp, _u32, _vp, _u32, _vp, C.c_size_t, _vp, C.c_size_t,
                                               C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
         "tc_last_error": (C.c_char_p, []),

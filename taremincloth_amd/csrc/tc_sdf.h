@@ -25,9 +25,11 @@ struct Domain {
     uint32_t w, h, d;
 };
 
-// close_open_boundary_loops_mesh (sdf_baker.rs:244-462): caps closed boundary loops of a solid mesh.
+// close_open_boundary_loops (sdf_baker.rs:244-462): caps closed boundary loops of a solid mesh;
+// with `weights` (one vector per bone) each new cap centre takes the loop's mean weight.
 // Returns the number of loops closed.
-size_t close_open_boundary_loops(std::vector<float> &verts, std::vector<int32_t> &tris);
+size_t close_open_boundary_loops(std::vector<float> &verts, std::vector<int32_t> &tris,
+                                 std::vector<std::vector<float>> *weights = nullptr);
 // mesh_sdf_domain (sdf_baker.rs:939-980)
 Domain mesh_sdf_domain(const std::vector<float> &verts, float voxel_size, float margin, float thickness);
 // mesh_sdf_bone_info (sdf_baker.rs:984-1013)

@@ -62,7 +62,8 @@ inline uint32_t spread10(uint32_t v) {
 }
 }  // namespace
 
-size_t close_open_boundary_loops(std::vector<float> &verts, std::vector<int32_t> &tris) {
+size_t close_open_boundary_loops(std::vector<float> &verts, std::vector<int32_t> &tris,
+                                 std::vector<std::vector<float>> *weights) {
     const size_t n = verts.size() / 3, m = tris.size() / 3;
     if (m < 4 || n < 4) return 0;
     std::map<std::pair<int32_t, int32_t>, uint32_t> count;
@@ -168,6 +169,14 @@ size_t close_open_boundary_loops(std::vector<float> &verts, std::vector<int32_t>
         }
         const int32_t ci = int32_t(verts.size() / 3);
         verts.insert(verts.end(), center, center + 3);
+        if (weights)  // each bone's weight at the cap centre: the loop's mean (sdf_baker.rs:427-439)
+            for (std::vector<float> &w : *weights) {
+                float ws = 0.0f;
+                for (int32_t vi : loop)
+                    if (size_t(vi) < w.size()) ws += w[size_t(vi)];
+                if (w.size() < size_t(ci)) w.resize(size_t(ci), 0.0f);
+                w.push_back(ws * inv_k);
+            }
         for (size_t i = 0; i < k; ++i) tris.insert(tris.end(), {loop[(i + 1) % k], loop[i], ci});
         ++closed;
     }

@@ -1050,6 +1050,63 @@ def atlas_to_dense(atlas, brick_dims, dims):
     return a.transpose(0, 3, 1, 4, 2, 5).reshape(bz * 8, by * 8, bx * 8)[:d, :h, :w]
 
 
+def bake_bone_sdf_gpu(mesh_verts, mesh_tris, bone_names, bone_weights, bone_bind_matrices, resolution=64, margin=0.2,
+                      weight_threshold=0.02, blend_k=0.05, friction=0.5, thickness=0.005, restitution=0.0,
+                      close_open_boundaries=True) -> dict:
+    """Bone SDF atlas bake (lib.rs:1190-1348 -> sdf_baker.rs:494-930): one res^3 tile per bone whose
+    largest weight exceeds weight_threshold, each texel pack2x16float(signed distance in bone space,
+    interpolated bone weight).  Returns texture_bytes, width, height, depth, bone_infos (N, 20),
+    active_bones (names) and bind_matrices (N, 4, 4); "kernel_ms" is the bake kernel's device time."""
+    if int(resolution) <= 0:
+        raise ValueError("resolution must be 1 or more")
+    v = np.asarray(mesh_verts, dtype=np.float32)
+    if v.ndim != 2 or v.shape[1] < 3:
+        raise ValueError("mesh_verts must have shape (N, 3)")
+    t = np.asarray(mesh_tris, dtype=np.int32)
+    if t.ndim != 2 or t.shape[1] < 3:
+        raise ValueError("mesh_tris must have shape (M, 3)")
+    v = np.ascontiguousarray(v[:, :3])
+    t = np.ascontiguousarray(t[:, :3])
+    names = [str(n) for n in bone_names]
+    nb = len(names)
+    if len(bone_weights) != nb:
+        raise ValueError("bone_names and bone_weights differ in length")
+    bm = np.asarray(bone_bind_matrices, dtype=np.float32)
+    if bm.ndim != 3 or bm.shape[0] < nb or bm.shape[1:] != (4, 4):
+        raise ValueError("bone_bind_matrices must have shape (N_bones, 4, 4)")
+    w = np.zeros((nb, len(v)), np.float32)
+    for i, bw in enumerate(bone_weights):
+        a = np.asarray(bw, dtype=np.float32).reshape(-1)
+        if a.size != len(v):
+            raise ValueError(f"bone_weights[{i}] has length {a.size}, the mesh has {len(v)} vertices")
+        w[i] = a
+    binds = np.ascontiguousarray(bm[:nb].reshape(nb, 16))
+    L = _lib.load()
+    res = C.c_void_p()
+    try:
+        _lib.check(L.tc_bake_bone_sdf(_ptr(v), len(v), _ptr(t), len(t), _ptr(w), _ptr(binds), nb, int(resolution),
+                                      float(margin), float(weight_threshold), float(blend_k), float(friction),
+                                      float(thickness), float(restitution), 1 if close_open_boundaries else 0,
+                                      C.byref(res)))
+    except RuntimeError as e:
+        raise RuntimeError(f"GPU SDF bake failed: {e}") from None
+    try:
+        dims = np.zeros(3, np.uint32)
+        na, ms = C.c_uint32(), C.c_float()
+        _lib.check(L.tc_bone_sdf_result_info(res, _ptr(dims), C.byref(na), C.byref(ms)))
+        n = int(na.value)
+        tex = np.empty(int(dims[0]) * int(dims[1]) * int(dims[2]), np.uint32)
+        infos = np.empty((n, 20), np.float32)
+        act = np.empty(n, np.uint32)
+        mats = np.empty((n, 4, 4), np.float32)
+        _lib.check(L.tc_bone_sdf_result_arrays(res, _ptr(tex), _ptr(infos), _ptr(act), _ptr(mats)))
+    finally:
+        L.tc_bone_sdf_result_free(res)
+    return {"texture_bytes": tex.tobytes(), "width": int(dims[0]), "height": int(dims[1]), "depth": int(dims[2]),
+            "bone_infos": infos, "active_bones": [names[int(i)] for i in act], "bind_matrices": mats,
+            "kernel_ms": float(ms.value)}
+
+
 def close_open_boundary_loops(mesh_verts, mesh_tris):
     """Caps closed boundary loops of a solid mesh (sdf_baker.rs:244-462): (verts, tris, loops_closed)."""
     L = _lib.load()
