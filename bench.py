@@ -46,6 +46,11 @@ MEASURED_TRAFFIC = {
     (512, 1024, 1024): {"fetch_kib": 23888.3, "write_kib": 10608.3, "fetch_factor": 2.0, "write_factor": 1.0,
                         "profile": "profiles/k_tile_iteration_grid512_counters.md"},
 }
+# k_tile_iteration with the fast projection, keyed as MEASURED_TRAFFIC.
+MEASURED_TRAFFIC_FAST = {
+    (512, 1024, 1024): {"fetch_kib": 23822.2, "write_kib": 10626.6, "fetch_factor": 2.0, "write_factor": 1.0,
+                        "profile": "profiles/k_tile_iteration_fast_projection_grid512.md"},
+}
 MEASURED_TRAFFIC_ROUND = {
     512: {"fetch_kib": 37182.9, "write_kib": 65536.0, "fetch_factor": 2.0, "write_factor": 1.0,
           "profile": "profiles/k_round_fwd_grid512_counters.md"},
@@ -431,16 +436,18 @@ def batch_bench(T, frames: int, warmup: int, cpu_sample: bool):
 
 def kernel_pass(T, kind: str, substeps: int = 3):
     """One kernel path alone, launched kernel by kernel (no graphs), for counter passes:
-    'tile': plain 64-iteration substeps of the 512x512 workload (k_tile_iteration);
+    'tile': plain 64-iteration substeps of the 512x512 workload (k_tile_iteration); 'tile_fast': the
+    same with the fast projection switched on;
     'round': the network's forward pass over the 512x512 stretch-edge graph (encoder, rounds,
     decoder) on seeded features; 'batch': 20-iteration substeps of the 64-instance batch."""
     from taremincloth_amd.warmstart import WarmStartNet
     dt_sub = float(np.float32(np.float32(1.0 / 60.0) / np.float32(10.0)))
-    if kind == "tile":
+    if kind in ("tile", "tile_fast"):
         pos, edges, faces = grid_512(512)
         inv = T.compute_combined_inv_masses(pos, faces, areal_density=0.15)
         sim = T.ClothSimulator(pos, edges, faces, inv, thickness=0.005, stiffness=1000.0, bending_stiffness=10.0)
         configure(sim, 64)
+        sim.set_fast_projection(kind == "tile_fast")
         for _ in range(substeps):
             sim.substep_parts(dt_sub, True, False, True)
         sim.synchronize()
@@ -512,7 +519,7 @@ def main():
                     help="measure only the batch of 64 instances and print its JSON object (for profiler runs)")
     ap.add_argument("--sdf-only", action="store_true",
                     help="measure only the mesh SDF bakes and print their JSON object (for profiler runs)")
-    ap.add_argument("--kernel-pass", choices=["tile", "round", "batch"],
+    ap.add_argument("--kernel-pass", choices=["tile", "tile_fast", "round", "batch"],
                     help="run one kernel path alone, kernel by kernel, and print what ran (for counter passes)")
     args = ap.parse_args()
     if args.light:
@@ -668,9 +675,22 @@ def main():
         arms[f"learned_{alt}_warm_iterations"] = timed_arm(lambda: gated(2, alt, net.history_weight, tau))
         arms["history_only"] = timed_arm(lambda: gated(1, args.ws_iters, net.history_weight, tau, with_net=False))
         arms["plain_64_fixed"] = timed_arm(plain_fixed, gate=False)
+        # the opt-in fast projection (refined hardware reciprocals; not bit-identical to the reference
+        # arithmetic) under the same gate and threshold
+        sim.set_fast_projection(True)
+        arms["learned_fast_projection"] = timed_arm(lambda: gated(2, args.ws_iters, net.history_weight, tau))
+        arms["plain_gated_fast_projection"] = timed_arm(lambda: gated(1, args.ws_iters, 0.0, tau, with_net=False))
+        sim.set_fast_projection(False)
     plain_fixed()
     restore(sim, S)
     kt = sim.time_kernels(dt_sub, reps=3)
+    kt_fast = None
+    if info["path"] == 2:
+        sim.set_fast_projection(True)
+        restore(sim, S)
+        kt_fast = sim.time_kernels(dt_sub, reps=3)
+        sim.set_fast_projection(False)
+        restore(sim, S)
 
     # ---- how many plain iterations the network plus its warm iterations are worth, at `checks`
     # states of the learned arm's window (no gate: the residual right after the warm iterations)
@@ -740,6 +760,16 @@ def main():
     solver_rl = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname,
                  "bytes_per_launch": bytes_per_launch, "launch_ms": launch_ms}
+    solver_fast_rl = None
+    if kt_fast is not None:
+        fast_ms = kt_fast["fused"] / max(kt_fast["fused_launches"], 1.0)
+        fast_gbs = sweep_bytes / (fast_ms * 1e-3) / 1e9
+        solver_fast_rl = {"bound": "hbm", "achieved": fast_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": fast_gbs / HBM_PEAK_GBS,
+                          "traffic": measured_traffic(MEASURED_TRAFFIC_FAST,
+                                                      (n, int(info["tile_vertices"]), int(info["block"]))),
+                          "kernel": "k_tile_iteration with the fast projection (opt-in)",
+                          "bytes_per_launch": sweep_bytes, "launch_ms": fast_ms}
     # the dominant kernel of the learned substep by time: its solver iterations are the warm
     # iterations plus the fallback iterations the gate ran
     roofline = net_rl if wt["round_layers"] >= wt["iterations"] + wt["fallback"] else solver_rl
@@ -826,6 +856,7 @@ def main():
         "roofline": roofline,
         "roofline_network": net_rl,
         "roofline_solver": solver_rl,
+        "roofline_solver_fast_projection": solver_fast_rl,
         "cpu_baseline": cpu,
         "config1_curtain": cfg1,
         "sdf_bake": sdf,

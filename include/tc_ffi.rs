@@ -119,6 +119,8 @@ extern "C" {
     pub fn tc_sim_set_wrinkle_field_params(sim: *mut TcSim, frame: *const f32, z_min: f32, z_max: f32, r_min: f32, r_max: f32, valley_window: f32, crest_window: f32) -> i32;
     pub fn tc_sim_get_wrinkle_field_params(sim: *const TcSim, out: *mut f32) -> i32;
     pub fn tc_sim_set_wrinkle_field_texture_2d(sim: *mut TcSim, width: u32, height: u32, rgba: *const u8, nbytes: usize, z_min: f32, z_max: f32, r_min: f32, r_max: f32) -> i32;
+    pub fn tc_sim_set_fast_projection(sim: *mut TcSim, enable: i32) -> i32;
+    pub fn tc_sim_get_fast_projection(sim: *const TcSim, enabled: *mut i32) -> i32;
     pub fn tc_sim_solver_info(sim: *mut TcSim, out: *mut f64) -> i32;
     pub fn tc_sim_autotune(sim: *mut TcSim, dt_sub: f32, out: *mut f64) -> i32;
     pub fn tc_sim_num_vertices(sim: *const TcSim) -> u32;

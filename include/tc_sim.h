@@ -264,6 +264,14 @@ int tc_sim_set_wrinkle_field_params(tc_sim *sim, const float *frame, float z_min
 int tc_sim_get_wrinkle_field_params(const tc_sim *sim, float *out);
 int tc_sim_set_wrinkle_field_texture_2d(tc_sim *sim, uint32_t width, uint32_t height, const uint8_t *rgba,
                                         size_t nbytes, float z_min, float z_max, float r_min, float r_max);
+/* Fast projection (no reference counterpart; opt-in, off by default): the per-iteration tile path
+ * computes each projection's length, direction and lambda step with the hardware reciprocal square
+ * root and reciprocal, each refined by one Newton-Raphson step, in place of the IEEE square root and
+ * divisions of distance.wgsl:60-124 / bending.wgsl:59-126.  Results are no longer bit-identical to
+ * the exact path (within 1e-5 m of it over 100 substeps on the tested drapes); the per-colour and
+ * Jacobi paths are unaffected. */
+int tc_sim_set_fast_projection(tc_sim *sim, int enable);
+int tc_sim_get_fast_projection(const tc_sim *sim, int *enabled);
 /* out[10]: [active path (1|2), tiles, largest cone (vertices), cone vertices total,
  *           items per iteration, constraints per iteration, plan available (0|1), colour phases,
  *           tile size (vertices), block size of the per-iteration kernel] */

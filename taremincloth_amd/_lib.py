@@ -31,7 +31,7 @@ EXPORTS = [
     "tc_sim_add_capsule", "tc_sim_add_plane", "tc_sim_clear_colliders", "tc_sim_set_mesh_triangles",
     "tc_sim_set_gravity", "tc_sim_set_damping", "tc_sim_set_damping_all", "tc_sim_set_solver_iterations",
     "tc_sim_set_enable_self_collision", "tc_sim_set_self_collision_options", "tc_sim_set_stiffness_all",
-    "tc_sim_set_compliances_raw", "tc_sim_set_hash_cell_size", "tc_sim_set_solver_path", "tc_sim_solver_info", "tc_sim_autotune",
+    "tc_sim_set_compliances_raw", "tc_sim_set_hash_cell_size", "tc_sim_set_solver_path", "tc_sim_solver_info", "tc_sim_set_fast_projection", "tc_sim_get_fast_projection", "tc_sim_autotune",
     "tc_sim_num_vertices",
     "tc_sim_num_distance_constraints", "tc_sim_num_bending_constraints", "tc_sim_get_array",
     "tc_sim_build_hashes", "tc_sim_residual", "tc_sim_substep_parts", "tc_sim_time_kernels", "tc_sim_synchronize",
@@ -149,6 +149,8 @@ def load() -> C.CDLL:
         "tc_sim_set_compliances_raw": (_i, [_vp, _f32, _f32, _f32]),
         "tc_sim_set_hash_cell_size": (_i, [_vp, _f32]),
         "tc_sim_set_solver_path": (_i, [_vp, _i]),
+        "tc_sim_set_fast_projection": (_i, [_vp, _i]),
+        "tc_sim_get_fast_projection": (_i, [_vp, C.POINTER(_i)]),
         "tc_sim_solver_info": (_i, [_vp, C.POINTER(C.c_double)]),
         "tc_sim_autotune": (_i, [_vp, _f32, C.POINTER(C.c_double)]),
         "tc_sim_num_vertices": (_u32, [_vp]),

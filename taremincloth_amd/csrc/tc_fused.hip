@@ -5,7 +5,9 @@
 // the selected constraints of each colour phase from LDS with a block barrier
 // between phases, write back the tile's own vertices and the lambdas of the
 // constraints it owns.  The projection arithmetic is the same operation
-// sequence as k_distance / k_bending (distance.wgsl:60-124, bending.wgsl:59-126).
+// sequence as k_distance / k_bending (distance.wgsl:60-124, bending.wgsl:59-126), or, with the
+// fast projection switched on, the same formula with the square root and the divisions replaced
+// by refined hardware reciprocals (not bit-identical; within 1e-6 relative per projection).
 //
 // Positions and lambdas are double-buffered across launches (the cones of
 // neighbouring tiles overlap, so a tile must not read what another tile of the
@@ -26,9 +28,20 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks) {
     return (b % 8u) * per + b / 8u;
 }
 
+// 1/sqrt(x) and 1/x from the hardware approximations (about 1 ulp) refined by one Newton-Raphson
+// step each: y <- y (3 - x y^2) / 2 and r <- r (2 - x r).
+__device__ __forceinline__ float rsqrt_nr(float x) {
+    const float y = __builtin_amdgcn_rsqf(x);
+    return y * (1.5f - (0.5f * x) * (y * y));
+}
+__device__ __forceinline__ float rcp_nr(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return r * (2.0f - x * r);
+}
+
 // Projection of one item from LDS (the k_distance / k_bending arithmetic).
 // Class table in LDS (per-lane indexed): [0] dtc [16] dcc [32] bcomp [48] dalpha_t [64] dalpha_c [80] balpha
-template <bool DIST, bool DAMP>
+template <bool DIST, bool DAMP, bool FAST>
 __device__ __forceinline__ void project_item(float4 *__restrict__ sp, const float4 *__restrict__ sx, uint32_t w0,
                                              uint32_t w1, uint32_t w2, float l, float *__restrict__ lam_out,
                                              const float *__restrict__ cls_tab, const SubstepArgs &a, float dt) {
@@ -48,24 +61,35 @@ __device__ __forceinline__ void project_item(float4 *__restrict__ sp, const floa
         alpha = cls_tab[80 + cls];
     }
     const f3 delta = xyz(q0) - xyz(q1);
-    const float d = length(delta);
-    if (d <= 1e-7f) { if (owner) lam_out[k] = l; return; }
+    float d, inv_d = 0.0f;
+    if (FAST) {
+        // one reciprocal square root gives the length and the direction; |delta| <= 1e-7 as d^2 <= 1e-14
+        const float dsq = dot(delta, delta);
+        if (dsq <= 1e-14f) { if (owner) lam_out[k] = l; return; }
+        inv_d = rsqrt_nr(dsq);
+        d = dsq * inv_d;
+    } else {
+        d = length(delta);
+        if (d <= 1e-7f) { if (owner) lam_out[k] = l; return; }
+    }
     const float cval = d - rest;
     if (DIST) {
         const bool tens = cval >= 0.0f;
         if (cls_tab[(tens ? 0 : 16) + cls] >= 1e9f) { if (owner) lam_out[k] = l; return; }
         alpha = cls_tab[(tens ? 48 : 64) + cls];
     }
-    const f3 dir = delta / d;
+    const f3 dir = FAST ? delta * inv_d : delta / d;
     float dl;
     if (DAMP) {
         const float dc = DIST ? (shear ? a.shear_damping : (cval >= 0.0f ? a.tension_damping : a.compression_damping))
                               : a.bending_damping;
         const float gamma = fminf(dc * 1e-5f / dt, 1e4f);
         const f3 rel = (xyz(q0) - xyz(sx[l0])) - (xyz(q1) - xyz(sx[l1]));
-        dl = (-cval - alpha * l - gamma * dot(dir, rel)) / ((1.0f + gamma) * wsum + alpha);
+        const float num = -cval - alpha * l - gamma * dot(dir, rel), den = (1.0f + gamma) * wsum + alpha;
+        dl = FAST ? num * rcp_nr(den) : num / den;
     } else {
-        dl = (-cval - alpha * l) / (wsum + alpha);
+        const float num = -cval - alpha * l, den = wsum + alpha;
+        dl = FAST ? num * rcp_nr(den) : num / den;
     }
     if (owner) lam_out[k] = l + dl;
     const f3 c0 = dir * (wa * dl), c1 = dir * (-wb * dl);
@@ -77,7 +101,7 @@ __device__ __forceinline__ void project_item(float4 *__restrict__ sp, const floa
 // words are loaded one phase ahead and its lambda right after the current
 // phase's LDS work, so global latency overlaps the barrier.  Further items of a
 // phase (phases larger than the block) are loaded in place.
-template <bool FIRST, bool DAMP>
+template <bool FIRST, bool DAMP, bool FAST>
 __global__ void __launch_bounds__(1024) k_tile_iteration(
     const float4 *__restrict__ Pin, float4 *__restrict__ Pout, const float4 *__restrict__ X,
     const uint32_t *__restrict__ ext_off, const uint32_t *__restrict__ ext_vid, const uint32_t *__restrict__ n_own,
@@ -133,14 +157,14 @@ __global__ void __launch_bounds__(1024) k_tile_iteration(
         const float *lam_in = dist ? dlam_in : blam_in;
         float *lam_out = dist ? dlam_out : blam_out;
         if (h_cur) {
-            if (dist) project_item<true, DAMP>(sp, sx, c0, c1, c2, lc, lam_out, cls_tab, a, dt);
-            else project_item<false, DAMP>(sp, sx, c0, c1, c2, lc, lam_out, cls_tab, a, dt);
+            if (dist) project_item<true, DAMP, FAST>(sp, sx, c0, c1, c2, lc, lam_out, cls_tab, a, dt);
+            else project_item<false, DAMP, FAST>(sp, sx, c0, c1, c2, lc, lam_out, cls_tab, a, dt);
         }
         for (uint32_t i = po[c] + tid + bs; i < po[c + 1]; i += bs) {
             const uint32_t w0 = items[3 * size_t(i)], w1 = items[3 * size_t(i) + 1], w2 = items[3 * size_t(i) + 2];
             const float l = FIRST ? 0.0f : lam_in[w1 & 0x03FFFFFFu];
-            if (dist) project_item<true, DAMP>(sp, sx, w0, w1, w2, l, lam_out, cls_tab, a, dt);
-            else project_item<false, DAMP>(sp, sx, w0, w1, w2, l, lam_out, cls_tab, a, dt);
+            if (dist) project_item<true, DAMP, FAST>(sp, sx, w0, w1, w2, l, lam_out, cls_tab, a, dt);
+            else project_item<false, DAMP, FAST>(sp, sx, w0, w1, w2, l, lam_out, cls_tab, a, dt);
         }
         c0 = n0; c1 = n1; c2 = n2; h_cur = h_nxt; lc = ln;
         n0 = m0; n1 = m1; n2 = m2; h_nxt = h_nn;
@@ -164,8 +188,13 @@ hipError_t launch_tile_iteration(hipStream_t s, const float4 *Pin, float4 *Pout,
 #define TC_TILE(K)                                                                                               \
     hipLaunchKernelGGL(K, dim3(nblocks), dim3(block), lds, s, Pin, Pout, X, p.ext_off, p.ext_vid, p.n_own,        \
                        p.ph_off, p.items, dlam_in, dlam_out, blam_in, blam_out, fx, a)
-    if (first) { if (damp) TC_TILE((k_tile_iteration<true, true>)); else TC_TILE((k_tile_iteration<true, false>)); }
-    else { if (damp) TC_TILE((k_tile_iteration<false, true>)); else TC_TILE((k_tile_iteration<false, false>)); }
+    if (fa.fast) {
+        if (first) { if (damp) TC_TILE((k_tile_iteration<true, true, true>)); else TC_TILE((k_tile_iteration<true, false, true>)); }
+        else { if (damp) TC_TILE((k_tile_iteration<false, true, true>)); else TC_TILE((k_tile_iteration<false, false, true>)); }
+    } else {
+        if (first) { if (damp) TC_TILE((k_tile_iteration<true, true, false>)); else TC_TILE((k_tile_iteration<true, false, false>)); }
+        else { if (damp) TC_TILE((k_tile_iteration<false, true, false>)); else TC_TILE((k_tile_iteration<false, false, false>)); }
+    }
 #undef TC_TILE
     return hipGetLastError();
 }
@@ -181,15 +210,21 @@ void fused_alphas(FusedArgs &fa, float dt_sub) {
 }
 
 hipError_t set_tile_lds_limit(size_t bytes) {
-    hipError_t e;
-    if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_iteration<true, true>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes))) != hipSuccess) return e;
-    if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_iteration<true, false>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes))) != hipSuccess) return e;
-    if ((e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_iteration<false, true>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes))) != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_iteration<false, false>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+    const void *ks[] = {
+        reinterpret_cast<const void *>(&k_tile_iteration<true, true, false>),
+        reinterpret_cast<const void *>(&k_tile_iteration<true, false, false>),
+        reinterpret_cast<const void *>(&k_tile_iteration<false, true, false>),
+        reinterpret_cast<const void *>(&k_tile_iteration<false, false, false>),
+        reinterpret_cast<const void *>(&k_tile_iteration<true, true, true>),
+        reinterpret_cast<const void *>(&k_tile_iteration<true, false, true>),
+        reinterpret_cast<const void *>(&k_tile_iteration<false, true, true>),
+        reinterpret_cast<const void *>(&k_tile_iteration<false, false, true>),
+    };
+    for (const void *k : ks) {
+        const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace tc

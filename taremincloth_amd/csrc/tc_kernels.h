@@ -71,6 +71,9 @@ hipError_t launch_residual(hipStream_t s, const float4 *P, const float4 *X, cons
 // --- one iteration per launch over cone tiles (tc_fused.hip, plan from tc_tiles.cpp) ---
 struct FusedArgs {
     uint32_t ntiles, nphases, nd_phases, max_ext;
+    // 1: fast projection (hardware reciprocal square root and reciprocal, one Newton-Raphson step
+    // each, instead of the IEEE square root and divisions); 0: the exact operation sequence
+    uint32_t fast;
     float dtc[16], dcc[16], bcomp[16];           // compliance classes
     float dalpha_t[16], dalpha_c[16], balpha[16];  // compliance / (dt * dt) per class for this launch
 };

@@ -1382,6 +1382,19 @@ int tc_sim_set_solver_path(tc_sim *s, int path) {
     return TC_OK;
 }
 
+int tc_sim_set_fast_projection(tc_sim *s, int enable) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    s->fast_projection = enable != 0;
+    s->touch();  // captured graphs hold the kernel variant
+    return TC_OK;
+}
+
+int tc_sim_get_fast_projection(const tc_sim *s, int *enabled) {
+    if (!s || !enabled) return fail(TC_ERR_VALUE, "null argument");
+    *enabled = s->fast_projection ? 1 : 0;
+    return TC_OK;
+}
+
 int tc_sim_set_solver_mode(tc_sim *s, uint32_t mode) {
     if (!s) return fail(TC_ERR_VALUE, "null simulator");
     s->solver_mode = mode;  // 1: Jacobi; every other value colours (dispatch.rs:1436)

@@ -241,6 +241,7 @@ struct tc_sim {
     tc::TilePlan plan;
     DevBuf<uint32_t> pl_ext_off, pl_ext_vid, pl_n_own, pl_ph_off, pl_items;
     uint32_t fused_block = 1024;
+    bool fast_projection = false;  // the tile path's refined-reciprocal projection (opt-in)
     uint32_t tile_vertices = 1024;  // set per mesh at creation; TCLOTH_TILE_VERTICES overrides; tc_sim_autotune picks per mesh
     uint32_t last_p = 0, last_lam = 0;  // buffers holding P and lambda after the last substep
 
@@ -375,6 +376,7 @@ struct tc_sim {
     tc::FusedArgs fused_args() const {
         tc::FusedArgs f{};
         f.ntiles = plan.ntiles; f.nphases = plan.nphases; f.nd_phases = plan.nd_phases; f.max_ext = plan.max_ext;
+        f.fast = fast_projection ? 1u : 0u;
         for (size_t c = 0; c < plan.dcls_tc.size(); ++c) { f.dtc[c] = plan.dcls_tc[c]; f.dcc[c] = plan.dcls_cc[c]; }
         for (size_t c = 0; c < plan.bcls.size(); ++c) f.bcomp[c] = plan.bcls[c];
         return f;

@@ -656,6 +656,16 @@ class ClothSimulator:
     def set_compliances_raw(self, stretch, shear, bend):
         _lib.check(self._L.tc_sim_set_compliances_raw(self._h, float(stretch), float(shear), float(bend)))
 
+    def set_fast_projection(self, enable: bool):
+        """Opt-in fast projection on the per-iteration tile path: refined hardware reciprocals in place
+        of the IEEE square root and divisions (not bit-identical; see tc_sim_set_fast_projection)."""
+        _lib.check(self._L.tc_sim_set_fast_projection(self._h, 1 if enable else 0))
+
+    def get_fast_projection(self) -> bool:
+        e = C.c_int(0)
+        _lib.check(self._L.tc_sim_get_fast_projection(self._h, C.byref(e)))
+        return bool(e.value)
+
     def set_solver_path(self, path):
         """0 auto, 1 one launch per colour, 2 one launch per iteration (identical results)."""
         _lib.check(self._L.tc_sim_set_solver_path(self._h, int(path)))
