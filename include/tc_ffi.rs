@@ -5,8 +5,8 @@
 //
 // Unverified by a Rust compiler (cargo is not available where this library is built).  The test
 // suite checks that every function of tc_sim.h appears here once, in the same order, with the same
-// number of parameters and the same pointer / scalar shape for each, and that the two #[repr(C)]
-// structures list the fields of tc_sim_params and tc_instance in order.
+// number of parameters and the same pointer / scalar shape for each, and that the three #[repr(C)]
+// structures list the fields of tc_sim_params, tc_instance and tc_sewing_params in order.
 #![allow(non_camel_case_types, dead_code)]
 
 use std::os::raw::{c_char, c_void};
@@ -51,11 +51,29 @@ pub struct TcInstance {
     pub thicknesses: *const f32,
 }
 
+#[repr(C)]
+#[derive(Clone, Copy, Debug)]
+pub struct TcSewingParams {
+    pub shrink_speed: f32,
+    pub stiffness: f32,
+    pub lock: i32,
+    pub lock_distance: f32,
+}
+
 #[link(name = "tcloth_hip")]
 extern "C" {
     pub fn tc_sim_params_default(p: *mut TcParams);
+    pub fn tc_sewing_params_default(p: *mut TcSewingParams);
     pub fn tc_sim_create(positions: *const f32, num_vertices: u32, edges: *const u32, num_edges: u32, faces: *const u32, num_faces: u32, inv_masses: *const f32, thicknesses: *const f32, layer_ids: *const u32, params: *const TcParams, out: *mut *mut TcSim) -> i32;
     pub fn tc_sim_destroy(sim: *mut TcSim);
+    pub fn tc_sim_create_sewn(positions: *const f32, num_vertices: u32, edges: *const u32, num_edges: u32, faces: *const u32, num_faces: u32, inv_masses: *const f32, thicknesses: *const f32, layer_ids: *const u32, sewing: *const u32, num_sewing: u32, sewing_params: *const TcSewingParams, params: *const TcParams, out: *mut *mut TcSim) -> i32;
+    pub fn tc_sim_set_sewing_stiffness(sim: *mut TcSim, stiffness: f32) -> i32;
+    pub fn tc_sim_set_sewing_shrink_speed(sim: *mut TcSim, speed: f32) -> i32;
+    pub fn tc_sim_set_enable_sewing_lock(sim: *mut TcSim, enable: i32) -> i32;
+    pub fn tc_sim_set_sewing_lock_distance(sim: *mut TcSim, distance: f32) -> i32;
+    pub fn tc_sim_get_sewing_state(sim: *const TcSim, out: *mut f32) -> i32;
+    pub fn tc_sim_get_sewing_rest_lengths(sim: *mut TcSim, out: *mut f32, n: u32) -> i32;
+    pub fn tc_sim_set_sewing_rest_lengths(sim: *mut TcSim, values: *const f32, n: u32) -> i32;
     pub fn tc_sim_create_batch(instances: *const TcInstance, num_instances: u32, params: *const TcParams, out: *mut *mut TcSim) -> i32;
     pub fn tc_sim_instance_count(sim: *const TcSim, num_instances: *mut u32) -> i32;
     pub fn tc_sim_instance_range(sim: *const TcSim, instance: u32, first_vertex: *mut u32, num_vertices: *mut u32) -> i32;
@@ -170,6 +188,7 @@ extern "C" {
     pub fn tc_bone_sdf_result_free(res: *mut TcBoneSdfResult);
     pub fn tc_close_open_boundary_loops(verts: *const f32, num_verts: u32, tris: *const i32, num_tris: u32, out_verts: *mut f32, cap_verts: usize, out_tris: *mut i32, cap_tris: usize, num_verts_out: *mut u32, num_tris_out: *mut u32, loops_closed: *mut u32) -> i32;
     pub fn tc_mesh_build_arrays(positions: *const f32, num_vertices: u32, edges: *const u32, num_edges: u32, faces: *const u32, num_faces: u32, inv_masses: *const f32, thicknesses: *const f32, params: *const TcParams, which: i32, out: *mut c_void, capacity: usize, count: *mut usize) -> i32;
+    pub fn tc_mesh_build_arrays_sewn(positions: *const f32, num_vertices: u32, edges: *const u32, num_edges: u32, faces: *const u32, num_faces: u32, inv_masses: *const f32, thicknesses: *const f32, sewing: *const u32, num_sewing: u32, sewing_params: *const TcSewingParams, params: *const TcParams, which: i32, out: *mut c_void, capacity: usize, count: *mut usize) -> i32;
     pub fn tc_color_edge_pairs(num_vertices: u32, pairs: *const u32, num_pairs: u32, colors_out: *mut u32) -> i32;
     pub fn tc_compute_inv_masses(positions: *const f32, num_vertices: u32, faces: *const u32, num_faces: u32, pin_weights: *const f32, areal_density: f32, out: *mut f32) -> i32;
     pub fn tc_last_error() -> *const c_char;

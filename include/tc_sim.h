@@ -53,6 +53,20 @@ typedef struct {
 /* Fills the reference defaults. */
 void tc_sim_params_default(tc_sim_params *p);
 
+/* Sewing keyword arguments of ClothSimulator::new (lib.rs:1557-1582): sewing_shrink_speed (m/s,
+ * default 1.0), sewing_stiffness (< 0: not given -> 10000), enable_sewing_lock (< 0: not given ->
+ * on), sewing_lock_distance (< 0: not given -> 0.02 m).  As in the reference, the stiffness and lock
+ * given here are stored in the sewing records only (mesh.rs:911-941); the solver uses the simulator's
+ * values, which start at 10000 and on (mod.rs:637-640) and follow tc_sim_set_sewing_stiffness /
+ * tc_sim_set_enable_sewing_lock.  The lock distance is applied (lib.rs:1749-1751). */
+typedef struct {
+    float shrink_speed;
+    float stiffness;
+    int32_t lock;
+    float lock_distance;
+} tc_sewing_params;
+void tc_sewing_params_default(tc_sewing_params *p);
+
 /* ClothSimulator::new, lib.rs:1557-1766 -> ClothMesh::from_raw_with_opts (mesh.rs:407)
  * + GpuClothSimulator::with_options (simulation/mod.rs:363).
  * faces / inv_masses / thicknesses / layer_ids may be NULL. */
@@ -62,6 +76,29 @@ int tc_sim_create(const float *positions, uint32_t num_vertices,
                   const float *inv_masses, const float *thicknesses, const uint32_t *layer_ids,
                   const tc_sim_params *params, tc_sim **out);
 void tc_sim_destroy(tc_sim *sim);
+/* tc_sim_create with sewing springs (lib.rs:1643-1665 -> mesh.rs:706-1191): `sewing` holds
+ * num_sewing vertex pairs; an index >= num_vertices is TC_ERR_VALUE.  Sewn vertices join one node of
+ * the self-collision topology, seams get gated bending constraints, and the sewing springs shrink
+ * and project every iteration (sewing.wgsl, sewing_shrink.wgsl). */
+int tc_sim_create_sewn(const float *positions, uint32_t num_vertices, const uint32_t *edges, uint32_t num_edges,
+                       const uint32_t *faces, uint32_t num_faces, const float *inv_masses, const float *thicknesses,
+                       const uint32_t *layer_ids, const uint32_t *sewing, uint32_t num_sewing,
+                       const tc_sewing_params *sewing_params, const tc_sim_params *params, tc_sim **out);
+/* ClothSimulator.set_sewing_stiffness (lib.rs:2800, clamped to >= 1), set_sewing_shrink_speed
+ * (lib.rs:2806, non-finite -> 1, >= 0; the current rest lengths are kept), set_enable_sewing_lock
+ * (lib.rs:2817), set_sewing_lock_distance (lib.rs:2823, >= 0). */
+int tc_sim_set_sewing_stiffness(tc_sim *sim, float stiffness);
+int tc_sim_set_sewing_shrink_speed(tc_sim *sim, float speed);
+int tc_sim_set_enable_sewing_lock(tc_sim *sim, int enable);
+int tc_sim_set_sewing_lock_distance(tc_sim *sim, float distance);
+/* out[5]: stiffness, shrink speed, lock (0|1), lock distance, number of sewing constraints
+ * (get_sewing_shrink_speed, get_sewing_lock_distance, num_sewing_constraints: lib.rs:2811-2947). */
+int tc_sim_get_sewing_state(const tc_sim *sim, float *out);
+/* get_sewing_current_rest_lengths / set_sewing_current_rest_lengths (lib.rs:2901-2908 ->
+ * dispatch.rs:912-950), in the constraints' colour order.  get: n must equal the constraint count;
+ * set: a list of another length is ignored, as in the reference. */
+int tc_sim_get_sewing_rest_lengths(tc_sim *sim, float *out, uint32_t n);
+int tc_sim_set_sewing_rest_lengths(tc_sim *sim, const float *values, uint32_t n);
 
 /* One cloth instance of a batch: the arguments of tc_sim_create for that cloth (vertex indices are
  * local to the instance; inv_masses / thicknesses / faces may be NULL). */
@@ -512,6 +549,14 @@ int tc_mesh_build_arrays(const float *positions, uint32_t num_vertices, const ui
                          const uint32_t *faces, uint32_t num_faces, const float *inv_masses,
                          const float *thicknesses, const tc_sim_params *params, int which, void *out,
                          size_t capacity, size_t *count);
+/* The same with sewing springs; further array ids:
+ * 38 bending v0   39 bending v1   40 bending _pad0 (f32)   41 bending _pad1 (f32)
+ * 42 sewing v0    43 sewing v1    44 sewing rest lengths (f32)   45 sewing colour offsets   46 sewing colour counts */
+int tc_mesh_build_arrays_sewn(const float *positions, uint32_t num_vertices, const uint32_t *edges,
+                              uint32_t num_edges, const uint32_t *faces, uint32_t num_faces,
+                              const float *inv_masses, const float *thicknesses, const uint32_t *sewing,
+                              uint32_t num_sewing, const tc_sewing_params *sewing_params,
+                              const tc_sim_params *params, int which, void *out, size_t capacity, size_t *count);
 
 /* Module functions */
 /* color_edge_pairs(n, edges) -> colours, lib.rs:636-648 -> coloring.rs:180 */

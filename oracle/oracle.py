@@ -30,9 +30,10 @@ FIELDS = [
     "bend_v0", "bend_v1", "bend_v2", "bend_v3", "bend_rest", "bend_comp", "bend_pad0", "bend_col_off",
     "bend_col_cnt", "adj_off", "adj_idx", "th_off", "th_idx", "th_rest", "local_len", "star_off",
     "star_v0", "star_v1", "edge_v0", "edge_v1", "edge_bbits", "island", "scalars", "edge_to_constraint",
-    "layer",
+    "layer", "bend_pad1", "sew_v0", "sew_v1", "sew_rest", "sew_col_off", "sew_col_cnt",
 ]
-_FLOAT_FIELDS = {"dist_rest", "dist_tc", "dist_cc", "bend_rest", "bend_comp", "bend_pad0", "th_rest", "local_len"}
+_FLOAT_FIELDS = {"dist_rest", "dist_tc", "dist_cc", "bend_rest", "bend_comp", "bend_pad0", "bend_pad1", "th_rest",
+                 "local_len", "sew_rest"}
 
 
 def build() -> None:
@@ -56,6 +57,16 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.orc_mesh_build.restype = _vp
     L.orc_mesh_build.argtypes = [_vp, C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32, _vp, _vp, _vp, C.c_uint32,
                                  C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.orc_mesh_build_sewn.restype = _vp
+    L.orc_mesh_build_sewn.argtypes = [_vp, C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32, _vp, _vp, _vp, C.c_uint32,
+                                      C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, _vp, C.c_uint32,
+                                      C.c_float]
+    L.orc_sim_create_sewn.restype = _vp
+    L.orc_sim_create_sewn.argtypes = [_vp, C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32, _vp, _vp, _vp, C.c_uint32,
+                                      C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, _vp, C.c_uint32,
+                                      C.c_float, C.c_float]
+    L.orc_sim_set_sewing.argtypes = [_vp, C.c_float, C.c_int, C.c_float, C.c_float]
+    L.orc_sim_sewing_rest_lengths.argtypes = [_vp, _vp, _vp]
     L.orc_mesh_get.restype = C.c_uint32
     L.orc_mesh_get.argtypes = [_vp, C.c_int, _vp]
     L.orc_mesh_free.argtypes = [_vp]
@@ -153,7 +164,8 @@ def mesh_fields(L: C.CDLL, mh) -> dict[str, np.ndarray]:
 
 def build_mesh(positions, edges, faces=None, inv_masses=None, thicknesses=None, layer_ids=None, layer_id=0,
                thickness=0.005, stiffness=1000.0, bending_stiffness=10.0, compression_stiffness=None,
-               shear_stiffness=None, precision="f32") -> dict[str, np.ndarray]:
+               shear_stiffness=None, sewing_springs=None, sewing_shrink_speed=1.0,
+               precision="f32") -> dict[str, np.ndarray]:
     """ClothMesh::from_raw_with_opts restated (mesh.rs:407-1377), returned as arrays."""
     L = lib(precision)
     pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
@@ -162,8 +174,11 @@ def build_mesh(positions, edges, faces=None, inv_masses=None, thicknesses=None, 
     comp = stiffness if compression_stiffness is None else compression_stiffness
     shear = stiffness * 0.5 if shear_stiffness is None else shear_stiffness
     im, th, ly = _opt(inv_masses, np.float32), _opt(thicknesses, np.float32), _opt(layer_ids, np.uint32)
-    mh = L.orc_mesh_build(_ptr(pos), len(pos), _ptr(e), len(e), _ptr(f), 0 if f is None else len(f.reshape(-1, 3)),
-                          _ptr(im), _ptr(th), _ptr(ly), layer_id, thickness, stiffness, comp, shear, bending_stiffness)
+    sw = None if sewing_springs is None else np.ascontiguousarray(sewing_springs, np.uint32).reshape(-1, 2)
+    mh = L.orc_mesh_build_sewn(_ptr(pos), len(pos), _ptr(e), len(e), _ptr(f),
+                               0 if f is None else len(f.reshape(-1, 3)), _ptr(im), _ptr(th), _ptr(ly), layer_id,
+                               thickness, stiffness, comp, shear, bending_stiffness, _ptr(sw),
+                               0 if sw is None else len(sw), sewing_shrink_speed)
     try:
         return mesh_fields(L, mh)
     finally:
@@ -205,7 +220,8 @@ class OracleSim:
 
     def __init__(self, positions, edges, faces=None, inv_masses=None, thicknesses=None, layer_ids=None,
                  layer_id=0, thickness=0.005, stiffness=1000.0, bending_stiffness=10.0,
-                 compression_stiffness=None, shear_stiffness=None, precision="f32"):
+                 compression_stiffness=None, shear_stiffness=None, sewing_springs=None, sewing_shrink_speed=1.0,
+                 sewing_lock_distance=None, precision="f32"):
         self._L = lib(precision)
         pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
         e = np.ascontiguousarray(edges, np.uint32).reshape(-1, 2)
@@ -215,9 +231,15 @@ class OracleSim:
         comp = stiffness if compression_stiffness is None else compression_stiffness
         shear = stiffness * 0.5 if shear_stiffness is None else shear_stiffness
         im, th, ly = _opt(inv_masses, np.float32), _opt(thicknesses, np.float32), _opt(layer_ids, np.uint32)
-        self._h = self._L.orc_sim_create(_ptr(pos), len(pos), _ptr(e), len(e), _ptr(f),
-                                         0 if f is None else len(f.reshape(-1, 3)), _ptr(im), _ptr(th), _ptr(ly),
-                                         layer_id, thickness, stiffness, bending_stiffness, comp, shear)
+        sw = None if sewing_springs is None or np.asarray(sewing_springs).size == 0 else \
+            np.ascontiguousarray(sewing_springs, np.uint32).reshape(-1, 2)
+        self._h = self._L.orc_sim_create_sewn(_ptr(pos), len(pos), _ptr(e), len(e), _ptr(f),
+                                              0 if f is None else len(f.reshape(-1, 3)), _ptr(im), _ptr(th), _ptr(ly),
+                                              layer_id, thickness, stiffness, bending_stiffness, comp, shear, _ptr(sw),
+                                              0 if sw is None else len(sw), sewing_shrink_speed,
+                                              -1.0 if sewing_lock_distance is None else sewing_lock_distance)
+        self._sew = [10000.0, 1, 0.02 if sewing_lock_distance is None else max(sewing_lock_distance, 0.0),
+                     max(sewing_shrink_speed, 0.0)]
         self.n = len(pos)
         self._pins = {}  # vertex -> (target float32[3], weight float32)
         self._inv_orig = self.inv_masses(1)
@@ -230,6 +252,34 @@ class OracleSim:
 
     def mesh(self) -> dict[str, np.ndarray]:
         return mesh_fields(self._L, self._L.orc_sim_mesh(self._h))
+
+    # sewing: stiffness, lock, lock distance, shrink speed (the simulator's values, mod.rs:1211-1255)
+    def _apply_sewing(self):
+        st, lk, ld, sp = self._sew
+        self._L.orc_sim_set_sewing(self._h, st, int(lk), ld, sp)
+
+    def set_sewing_stiffness(self, v):
+        self._sew[0] = max(float(v), 1.0)
+        self._apply_sewing()
+
+    def set_enable_sewing_lock(self, on):
+        self._sew[1] = 1 if on else 0
+        self._apply_sewing()
+
+    def set_sewing_lock_distance(self, d):
+        self._sew[2] = max(float(d), 0.0)
+        self._apply_sewing()
+
+    def set_sewing_shrink_speed(self, v):
+        self._sew[3] = max(float(v), 0.0) if np.isfinite(v) else 1.0
+        self._apply_sewing()
+
+    def sewing_rest_lengths(self, values=None) -> np.ndarray:
+        n = len(self.mesh()["sew_v0"])
+        out = np.empty(n, np.float64)
+        vin = None if values is None else np.ascontiguousarray(values, np.float64)
+        self._L.orc_sim_sewing_rest_lengths(self._h, out.ctypes.data_as(C.c_void_p), _ptr(vin))
+        return out
 
     def set_gravity(self, gx, gy, gz):
         self._L.orc_sim_set_gravity(self._h, gx, gy, gz)

@@ -200,6 +200,12 @@ typedef struct {
     float *b_rest, *b_comp, *b_pad0, *b_pad1;
     uint32_t nb_col, *b_col_off, *b_col_cnt;
 
+    uint32_t ns;      /* sewing constraints, colour-sorted (mesh.rs:909-947) */
+    uint32_t *s_v0, *s_v1;
+    float *s_rest;    /* initial (current) rest lengths; the target is 0 */
+    uint32_t ns_col, *s_col_off, *s_col_cnt;
+    float sew_speed;
+
     uint32_t *adj_off, *adj_idx;
     uint32_t *th_off, *th_idx; float *th_rest;
     float *local_len;
@@ -371,11 +377,115 @@ static uint32_t virtual_sample_count(const orc_mesh *m, const float *pos) {
     return (uint32_t)(total < CAP ? total : CAP);
 }
 
+/* Sewing groups: union-find over the valid sewing pairs, each vertex labelled by the smallest
+ * vertex of its group (mesh.rs:709-739 and 998-1030 build the same partition). */
+static void sew_reps(uint32_t nv, const uint32_t *sew, uint32_t nsew, uint32_t *rep) {
+    for (uint32_t i = 0; i < nv; i++) rep[i] = i;
+    for (uint32_t k = 0; k < nsew; k++) {
+        uint32_t u = sew[2 * k], v = sew[2 * k + 1];
+        if (u >= nv || v >= nv || u == v) continue;
+        while (rep[u] != u) u = rep[u];
+        while (rep[v] != v) v = rep[v];
+        if (u != v) rep[u > v ? u : v] = u < v ? u : v;
+    }
+    for (uint32_t i = 0; i < nv; i++) {
+        uint32_t r = i;
+        while (rep[r] != r) r = rep[r];
+        rep[i] = r;
+    }
+}
+
+/* Unfolded distance of q (over edge e0-e1) and r (over f0-f1) about a shared seam axis:
+ * sqrt((s_q - s_r)^2 + (h_q + h_r)^2) with s the along-edge and h the perpendicular offset from
+ * each edge's first end; -1 when either edge is not longer than 1e-6 (mesh.rs:789-822, 1436-1458). */
+static float seam_unfold(const float *pos, uint32_t e0, uint32_t e1, uint32_t q, uint32_t f0, uint32_t f1, uint32_t r) {
+    const float *A0 = pos + 3 * e0, *A1 = pos + 3 * e1, *B0 = pos + 3 * f0, *B1 = pos + 3 * f1;
+    const float *Q = pos + 3 * q, *R = pos + 3 * r;
+    float ea[3] = {A1[0] - A0[0], A1[1] - A0[1], A1[2] - A0[2]}, eb[3] = {B1[0] - B0[0], B1[1] - B0[1], B1[2] - B0[2]};
+    float la = sqrtf(ea[0] * ea[0] + ea[1] * ea[1] + ea[2] * ea[2]);
+    float lb = sqrtf(eb[0] * eb[0] + eb[1] * eb[1] + eb[2] * eb[2]);
+    if (!(la > 1e-6f && lb > 1e-6f)) return -1.0f;
+    float ua[3] = {ea[0] / la, ea[1] / la, ea[2] / la}, ub[3] = {eb[0] / lb, eb[1] / lb, eb[2] / lb};
+    float da[3] = {Q[0] - A0[0], Q[1] - A0[1], Q[2] - A0[2]}, db[3] = {R[0] - B0[0], R[1] - B0[1], R[2] - B0[2]};
+    float sa = da[0] * ua[0] + da[1] * ua[1] + da[2] * ua[2];
+    float sb = db[0] * ub[0] + db[1] * ub[1] + db[2] * ub[2];
+    float pa[3] = {da[0] - sa * ua[0], da[1] - sa * ua[1], da[2] - sa * ua[2]};
+    float pb[3] = {db[0] - sb * ub[0], db[1] - sb * ub[1], db[2] - sb * ub[2]};
+    float ha = sqrtf(pa[0] * pa[0] + pa[1] * pa[1] + pa[2] * pa[2]);
+    float hb = sqrtf(pb[0] * pb[0] + pb[1] * pb[1] + pb[2] * pb[2]);
+    float ds = sa - sb, dh = ha + hb;
+    return sqrtf(ds * ds + dh * dh);
+}
+
+/* Rest length of a two-hop pair joined only through sewing (mesh.rs:1382-1490): the smallest of,
+ * over original neighbours m1 of i and m2 of nbr in one sewing group (not i's or nbr's group), the
+ * direct distance when m1 == m2, the unfolded distance over each seam edge m1-k1 / m2-k2 whose far
+ * ends share a group, or |i - m1| + |nbr - m2| when no seam edge gives a smaller value; 0 unless
+ * above 1e-5.  orig_off / orig: the original adjacency. */
+static float seam_two_hop(uint32_t i, uint32_t nbr, const float *pos, const uint32_t *orig_off, const uint32_t *orig,
+                          const uint32_t *rep) {
+    uint32_t ri = rep[i], rn = rep[nbr];
+    float best = 3.4028235e38f;
+    for (uint32_t a = orig_off[i]; a < orig_off[i + 1]; a++) {
+        uint32_t m1 = orig[a], rm = rep[m1];
+        if (rm == ri || rm == rn) continue;
+        for (uint32_t b = orig_off[nbr]; b < orig_off[nbr + 1]; b++) {
+            uint32_t m2 = orig[b];
+            if (rep[m2] != rm) continue;
+            if (m1 == m2) {
+                float d = frest(pos + 3 * i, pos + 3 * nbr);
+                if (d < best) best = d;
+                continue;
+            }
+            int found = 0;
+            for (uint32_t c = orig_off[m1]; c < orig_off[m1 + 1]; c++) {
+                uint32_t k1 = orig[c], rk = rep[k1];
+                if (rk == rm) continue;
+                for (uint32_t d = orig_off[m2]; d < orig_off[m2 + 1]; d++) {
+                    uint32_t k2 = orig[d];
+                    if (rep[k2] != rk) continue;
+                    float fl = seam_unfold(pos, m1, k1, i, m2, k2, nbr);
+                    if (fl >= 0.0f && fl < best) { best = fl; found = 1; }
+                }
+            }
+            if (!found) {
+                float slit = frest(pos + 3 * i, pos + 3 * m1) + frest(pos + 3 * nbr, pos + 3 * m2);
+                if (slit < best) best = slit;
+            }
+        }
+    }
+    return (best < 3.4028235e38f && isfinite(best) && best > 1e-5f) ? best : 0.0f;
+}
+
+typedef struct { uint64_t ckey; uint32_t u, v, opp, ord; } seament;
+static int cmp_seament(const void *a, const void *b) {
+    const seament *x = (const seament *)a, *y = (const seament *)b;
+    if (x->ckey != y->ckey) return x->ckey < y->ckey ? -1 : 1;
+    return x->ord < y->ord ? -1 : x->ord > y->ord;
+}
+
+orc_mesh *orc_mesh_build_sewn(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne_in,
+                              const uint32_t *faces, uint32_t nf, const float *inv_masses,
+                              const float *thicknesses, const uint32_t *layer_ids, uint32_t default_layer,
+                              float default_thickness, float tension_k, float compression_k, float shear_k,
+                              float bending_k, const uint32_t *sew, uint32_t nsew, float sew_speed);
 orc_mesh *orc_mesh_build(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne_in,
                          const uint32_t *faces, uint32_t nf, const float *inv_masses,
                          const float *thicknesses, const uint32_t *layer_ids, uint32_t default_layer,
                          float default_thickness, float tension_k, float compression_k, float shear_k,
                          float bending_k) {
+    return orc_mesh_build_sewn(pos, nv, edges, ne_in, faces, nf, inv_masses, thicknesses, layer_ids, default_layer,
+                               default_thickness, tension_k, compression_k, shear_k, bending_k, NULL, 0, 1.0f);
+}
+
+orc_mesh *orc_mesh_build_sewn(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne_in,
+                              const uint32_t *faces, uint32_t nf, const float *inv_masses,
+                              const float *thicknesses, const uint32_t *layer_ids, uint32_t default_layer,
+                              float default_thickness, float tension_k, float compression_k, float shear_k,
+                              float bending_k, const uint32_t *sew, uint32_t nsew, float sew_speed) {
+    const int sewing = sew && nsew;
+    uint32_t *rep = (uint32_t *)malloc((size_t)(nv ? nv : 1) * 4);
+    sew_reps(nv, sewing ? sew : NULL, sewing ? nsew : 0, rep);
     orc_mesh *m = (orc_mesh *)calloc(1, sizeof(orc_mesh));
     m->nv = nv;
     m->pos0 = (float *)malloc((size_t)(nv ? nv : 1) * 12);
@@ -506,6 +616,49 @@ orc_mesh *orc_mesh_build(const float *pos, uint32_t nv, const uint32_t *edges, u
         }
         g = h;
     }
+    /* seam-gated bending (mesh.rs:706-831): boundary edges (one opposite vertex) with both ends
+     * sewn, grouped by their pair of sewing groups; a pair of exactly two such edges bends about the
+     * seam, at the unfolded distance of its opposite vertices, gated at half of it (>= 1 cm) */
+    if (sewing && neo) {
+        uint8_t *has_sew = (uint8_t *)calloc(nv + 1, 1);
+        for (uint32_t k = 0; k < nsew; k++) {
+            uint32_t u = sew[2 * k], v = sew[2 * k + 1];
+            if (u < nv && v < nv && u != v) { has_sew[u] = 1; has_sew[v] = 1; }
+        }
+        seament *se = (seament *)malloc(neo * sizeof(seament));
+        size_t nse = 0;
+        for (size_t g = 0; g < neo;) {
+            size_t h = g;
+            while (h < neo && eo[h].key == eo[g].key) h++;
+            uint32_t ea = (uint32_t)(eo[g].key >> 32), eb = (uint32_t)eo[g].key;
+            if (h - g == 1 && has_sew[ea] && has_sew[eb] && rep[ea] != rep[eb]) {
+                uint32_t ra = rep[ea], rb = rep[eb];
+                seament x;
+                x.ckey = ra < rb ? ((uint64_t)ra << 32) | rb : ((uint64_t)rb << 32) | ra;
+                x.u = ra < rb ? ea : eb; x.v = ra < rb ? eb : ea; x.opp = eo[g].opp; x.ord = (uint32_t)nse;
+                se[nse++] = x;
+            }
+            g = h;
+        }
+        qsort(se, nse, sizeof(seament), cmp_seament);
+        for (size_t g = 0; g < nse;) {
+            size_t h = g;
+            while (h < nse && se[h].ckey == se[g].ckey) h++;
+            if (h - g == 2) {
+                uint32_t a0 = se[g].u, a1 = se[g].v, v2 = se[g].opp, b0 = se[g + 1].u, b1 = se[g + 1].v, v3 = se[g + 1].opp;
+                if (!(a0 == b0 || a1 == b1 || v2 == v3 || rep[v2] == rep[v3])) {
+                    float flat = seam_unfold(pos, a0, a1, v2, b0, b1, v3);
+                    if (flat >= 0.0f && flat > 1e-5f) {
+                        vu_push(&bv0, a0); vu_push(&bv1, b0); vu_push(&bv2, v2); vu_push(&bv3, v3);
+                        vf_push(&brest, flat); vf_push(&bcomp, bend_c);
+                        vf_push(&bpad0, 1.0f); vf_push(&bpad1, fmaxf(flat * 0.5f, 0.01f));
+                    }
+                }
+            }
+            g = h;
+        }
+        free(se); free(has_sew);
+    }
     {
         pset face_edges; pset_init(&face_edges, (size_t)nf * 3 + 1);
         if (faces) for (uint32_t f = 0; f < nf; f++) {
@@ -613,6 +766,36 @@ orc_mesh *orc_mesh_build(const float *pos, uint32_t nv, const uint32_t *edges, u
         free(perm); free(col); free(reg); free(regn); free(dgv); free(dgn); free(usv); free(usn);
     }
 
+    /* sewing constraints in input order (mesh.rs:909-944), coloured like the distance pairs
+     * (coloring.rs:236-300) */
+    m->sew_speed = sew_speed > 0.0f ? sew_speed : 0.0f;
+    {
+        vu32 s0 = {0}, s1 = {0}; vf32 sr = {0};
+        if (sewing)
+            for (uint32_t k = 0; k < nsew; k++) {
+                uint32_t a = sew[2 * k], b = sew[2 * k + 1];
+                if (a < nv && b < nv && a != b) { vu_push(&s0, a); vu_push(&s1, b); vf_push(&sr, frest(pos + 3 * a, pos + 3 * b)); }
+            }
+        uint32_t ns = (uint32_t)s0.n;
+        m->ns = ns;
+        uint32_t *vv = (uint32_t *)malloc((size_t)(ns ? ns : 1) * 8);
+        uint32_t *two = (uint32_t *)malloc((size_t)(ns ? ns : 1) * 4);
+        for (uint32_t i = 0; i < ns; i++) { vv[2 * i] = s0.d[i]; vv[2 * i + 1] = s1.d[i]; two[i] = 2; }
+        uint32_t *col = (uint32_t *)malloc((size_t)(ns ? ns : 1) * 4);
+        uint32_t ncol = 0;
+        welsh_powell(nv, ns, vv, two, 2, vv, two, 2, vv, two, 2, col, &ncol);
+        m->ns_col = ncol;
+        m->s_col_off = (uint32_t *)malloc((size_t)(ncol ? ncol : 1) * 4);
+        m->s_col_cnt = (uint32_t *)malloc((size_t)(ncol ? ncol : 1) * 4);
+        uint32_t *perm = (uint32_t *)malloc((size_t)(ns ? ns : 1) * 4);
+        bucket_order(ns, col, ncol, perm, NULL, m->s_col_off, m->s_col_cnt);
+        m->s_v0 = (uint32_t *)malloc((size_t)(ns ? ns : 1) * 4);
+        m->s_v1 = (uint32_t *)malloc((size_t)(ns ? ns : 1) * 4);
+        m->s_rest = (float *)malloc((size_t)(ns ? ns : 1) * 4);
+        for (uint32_t q = 0; q < ns; q++) { m->s_v0[q] = s0.d[perm[q]]; m->s_v1[q] = s1.d[perm[q]]; m->s_rest[q] = sr.d[perm[q]]; }
+        free(perm); free(col); free(two); free(vv); free(s0.d); free(s1.d); free(sr.d);
+    }
+
     /* 4. adjacency + local edge lengths (mesh.rs:949-1075) */
     uint32_t *acnt = (uint32_t *)calloc(nv + 1, 4);
     float *lsum = (float *)calloc(nv + 1, 4);
@@ -645,18 +828,70 @@ orc_mesh *orc_mesh_build(const float *pos, uint32_t nv, const uint32_t *edges, u
         uint32_t a = edges[2 * e], b = edges[2 * e + 1];
         if (a < nv && b < nv && a != b) { araw[aoff_raw[a] + afill[a]++] = b; araw[aoff_raw[b] + afill[b]++] = a; }
     }
+    /* the original adjacency, sorted and unique (orig_off / orig) */
+    uint32_t *orig_off = (uint32_t *)malloc((nv + 1) * 4);
+    vu32 origv = {0};
+    for (uint32_t v = 0; v < nv; v++) {
+        orig_off[v] = (uint32_t)origv.n;
+        size_t n = sort_unique(araw + aoff_raw[v], acnt[v]);
+        for (size_t k = 0; k < n; k++) vu_push(&origv, araw[aoff_raw[v] + k]);
+    }
+    orig_off[nv] = (uint32_t)origv.n;
+    uint32_t *orig = origv.d ? origv.d : (uint32_t *)calloc(1, 4);
+    /* with sewing, a group's vertices neighbour each other and share every outside neighbour of the
+     * group, both ways (mesh.rs:1031-1063); the adjacency CSR is then sorted and unique */
+    vu32 *cl = (vu32 *)calloc(nv + 1, sizeof(vu32));
+    for (uint32_t v = 0; v < nv; v++)
+        for (uint32_t k = orig_off[v]; k < orig_off[v + 1]; k++) vu_push(&cl[v], orig[k]);
+    if (sewing) {
+        uint32_t *gcnt = (uint32_t *)calloc(nv + 1, 4), *goff = (uint32_t *)malloc((nv + 1) * 4);
+        for (uint32_t i = 0; i < nv; i++) gcnt[rep[i]]++;
+        goff[0] = 0;
+        for (uint32_t r = 0; r < nv; r++) goff[r + 1] = goff[r] + gcnt[r];
+        uint32_t *gm = (uint32_t *)malloc((size_t)(nv ? nv : 1) * 4), *gf = (uint32_t *)calloc(nv + 1, 4);
+        for (uint32_t i = 0; i < nv; i++) gm[goff[rep[i]] + gf[rep[i]]++] = i;
+        for (uint32_t r = 0; r < nv; r++) {
+            if (gcnt[r] < 2) continue;
+            const uint32_t *g = gm + goff[r];
+            for (uint32_t a = 0; a < gcnt[r]; a++)
+                for (uint32_t b = 0; b < gcnt[r]; b++)
+                    if (a != b) vu_push(&cl[g[a]], g[b]);
+            vu32 shared = {0};
+            for (uint32_t a = 0; a < gcnt[r]; a++)
+                for (uint32_t k = orig_off[g[a]]; k < orig_off[g[a] + 1]; k++)
+                    if (rep[orig[k]] != r) vu_push(&shared, orig[k]);
+            size_t nsh = sort_unique(shared.d, shared.n);
+            for (uint32_t a = 0; a < gcnt[r]; a++)
+                for (size_t k = 0; k < nsh; k++) { vu_push(&cl[g[a]], shared.d[k]); vu_push(&cl[shared.d[k]], g[a]); }
+            free(shared.d);
+        }
+        free(gcnt); free(goff); free(gm); free(gf);
+    }
     m->adj_off = (uint32_t *)malloc((nv + 1) * 4);
     vu32 aidx = {0};
     for (uint32_t v = 0; v < nv; v++) {
         m->adj_off[v] = (uint32_t)aidx.n;
-        size_t n = sort_unique(araw + aoff_raw[v], acnt[v]);
-        for (size_t k = 0; k < n; k++) vu_push(&aidx, araw[aoff_raw[v] + k]);
-        acnt[v] = (uint32_t)n; /* now unique count */
+        size_t n = sort_unique(cl[v].d, cl[v].n);
+        for (size_t k = 0; k < n; k++) vu_push(&aidx, cl[v].d[k]);
+        free(cl[v].d);
     }
+    free(cl);
     m->adj_off[nv] = (uint32_t)aidx.n;
     m->adj_idx = aidx.d ? aidx.d : (uint32_t *)calloc(1, 4);
 
-    /* two-hop CSR (mesh.rs:1087-1191); without sewing the contracted graph is the mesh graph */
+    /* two-hop CSR (mesh.rs:1087-1191) over the adjacency above.  A true two-hop pair is of face
+     * vertices in different sewing groups, joined by no original edge, whose groups are not
+     * neighbours in the contracted graph (c1, in group labels); its rest length is the distance when
+     * the pair is two hops apart in the original mesh too, else the seam rest length. */
+    vu32 *c1 = (vu32 *)calloc(nv + 1, sizeof(vu32));
+    if (sewing) {
+        for (uint32_t e = 0; e < ne_in; e++) {
+            uint32_t a = edges[2 * e], b = edges[2 * e + 1];
+            if (a < nv && b < nv && a != b && rep[a] != rep[b]) { vu_push(&c1[rep[a]], rep[b]); vu_push(&c1[rep[b]], rep[a]); }
+        }
+        for (uint32_t v = 0; v < nv; v++) c1[v].n = sort_unique(c1[v].d, c1[v].n);
+    }
+    vu32 onb = {0};
     uint8_t *is_face = (uint8_t *)calloc(nv + 1, 1);
     if (faces) for (uint32_t f = 0; f < nf; f++)
         for (int k = 0; k < 3; k++) if (faces[3 * f + k] < nv) is_face[faces[3 * f + k]] = 1;
@@ -675,17 +910,41 @@ orc_mesh *orc_mesh_build(const float *pos, uint32_t nv, const uint32_t *edges, u
         m->th_off[i] = (uint32_t)thi.n;
         const uint32_t *ai = m->adj_idx + m->adj_off[i];
         size_t an = m->adj_off[i + 1] - m->adj_off[i];
+        size_t on = 0;
+        if (sewing) {
+            onb.n = 0;
+            vu_push(&onb, i);
+            for (uint32_t k = orig_off[i]; k < orig_off[i + 1]; k++) {
+                uint32_t u = orig[k];
+                vu_push(&onb, u);
+                for (uint32_t ku = orig_off[u]; ku < orig_off[u + 1]; ku++) vu_push(&onb, orig[ku]);
+            }
+            on = sort_unique(onb.d, onb.n);
+        }
         for (size_t k = 0; k < n; k++) {
             uint32_t nbr = nb_tmp.d[k];
             vu_push(&thi, nbr);
-            int true2 = nbr != i && is_face[i] && is_face[nbr] && !bsearch_u32(ai, an, nbr);
-            vf_push(&thr, true2 ? frest(pos + 3 * i, pos + 3 * nbr) : 0.0f);
+            float rest = 0.0f;
+            if (!sewing) {
+                int true2 = nbr != i && is_face[i] && is_face[nbr] && !bsearch_u32(ai, an, nbr);
+                if (true2) rest = frest(pos + 3 * i, pos + 3 * nbr);
+            } else {
+                int true2 = nbr != i && is_face[i] && is_face[nbr] && rep[i] != rep[nbr] &&
+                            !bsearch_u32(orig + orig_off[i], orig_off[i + 1] - orig_off[i], nbr) &&
+                            !bsearch_u32(c1[rep[i]].d, c1[rep[i]].n, rep[nbr]);
+                if (true2)
+                    rest = bsearch_u32(onb.d, on, nbr) ? frest(pos + 3 * i, pos + 3 * nbr)
+                                                       : seam_two_hop(i, nbr, pos, orig_off, orig, rep);
+            }
+            vf_push(&thr, rest);
         }
     }
     m->th_off[nv] = (uint32_t)thi.n;
     m->th_idx = thi.d ? thi.d : (uint32_t *)calloc(1, 4);
     m->th_rest = thr.d ? thr.d : (float *)calloc(1, 4);
-    free(nb_tmp.d); free(is_face);
+    free(nb_tmp.d); free(is_face); free(onb.d);
+    for (uint32_t v = 0; v < nv; v++) free(c1[v].d);
+    free(c1);
 
     /* star lists (mesh.rs:1193-1212), triangles */
     uint32_t *scount = (uint32_t *)calloc(nv + 1, 4);
@@ -785,17 +1044,37 @@ orc_mesh *orc_mesh_build(const float *pos, uint32_t nv, const uint32_t *edges, u
                         keys[nk++] = ((uint64_t)te[k][0] << 32) | te[k][1];
             }
             qsort(keys, nk, 8, cmpu64);
+            /* the same triangle edges in sewing-group labels (pairs within a group dropped) */
+            uint64_t *ckeys = (uint64_t *)malloc((nk ? nk : 1) * 8);
+            size_t nck = 0;
+            for (size_t k = 0; k < nk; k++) {
+                uint32_t ra = rep[(uint32_t)(keys[k] >> 32)], rb = rep[(uint32_t)keys[k]];
+                if (ra != rb) ckeys[nck++] = ra < rb ? ((uint64_t)ra << 32) | rb : ((uint64_t)rb << 32) | ra;
+            }
+            qsort(ckeys, nck, 8, cmpu64);
             for (uint32_t k = 0; k < ne; k++) {
+                uint32_t ra = rep[m->e_v0[k]], rb = rep[m->e_v1[k]];
+                if (ra == rb) continue;  /* an edge inside a sewing group is never open */
                 uint64_t key = ((uint64_t)m->e_v0[k] << 32) | m->e_v1[k];
                 size_t lo = 0, hi = nk;
                 while (lo < hi) { size_t mid = lo + (hi - lo) / 2; if (keys[mid] < key) lo = mid + 1; else hi = mid; }
                 size_t c = 0;
                 while (lo + c < nk && keys[lo + c] == key) c++;
-                if (c == 0 || c == 1) m->e_bbits[k >> 5] |= 1u << (k & 31);
+                int open = c == 0;
+                if (c == 1) {  /* open only if its contracted edge is in one triangle too (mesh.rs:1323-1330) */
+                    uint64_t ck = ra < rb ? ((uint64_t)ra << 32) | rb : ((uint64_t)rb << 32) | ra;
+                    lo = 0; hi = nck;
+                    while (lo < hi) { size_t mid = lo + (hi - lo) / 2; if (ckeys[mid] < ck) lo = mid + 1; else hi = mid; }
+                    size_t cc = 0;
+                    while (lo + cc < nck && ckeys[lo + cc] == ck) cc++;
+                    open = cc == 1;
+                }
+                if (open) m->e_bbits[k >> 5] |= 1u << (k & 31);
             }
-            free(keys);
+            free(keys); free(ckeys);
         } else {
-            for (uint32_t k = 0; k < ne; k++) m->e_bbits[k >> 5] |= 1u << (k & 31);
+            for (uint32_t k = 0; k < ne; k++)
+                if (rep[m->e_v0[k]] != rep[m->e_v1[k]]) m->e_bbits[k >> 5] |= 1u << (k & 31);
         }
     }
 
@@ -839,6 +1118,7 @@ orc_mesh *orc_mesh_build(const float *pos, uint32_t nv, const uint32_t *edges, u
     m->table_size = ts;
 
     free(aoff_raw); free(araw); free(afill); free(acnt); free(lsum); free(lcnt);
+    free(orig_off); free(orig); free(rep);
     free(eo);
     pset_free(&original);
     free(dv0.d); free(dv1.d); free(dty.d); free(drest.d); free(dtc.d); free(dcc.d);
@@ -852,7 +1132,8 @@ void orc_mesh_free(orc_mesh *m) {
                   m->d_col_off, m->d_col_cnt, m->d_remap, m->edge_to_constraint, m->b_v0, m->b_v1, m->b_v2, m->b_v3,
                   m->b_rest, m->b_comp, m->b_pad0, m->b_pad1, m->b_col_off, m->b_col_cnt, m->adj_off, m->adj_idx,
                   m->th_off, m->th_idx, m->th_rest, m->local_len, m->star_off, m->star_v0, m->star_v1, m->tri,
-                  m->e_v0, m->e_v1, m->e_bbits, m->island};
+                  m->e_v0, m->e_v1, m->e_bbits, m->island, m->s_v0, m->s_v1, m->s_rest, m->s_col_off,
+                  m->s_col_cnt};
     for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
     free(m);
 }
@@ -863,7 +1144,7 @@ enum {
     OM_BEND_V0, OM_BEND_V1, OM_BEND_V2, OM_BEND_V3, OM_BEND_REST, OM_BEND_COMP, OM_BEND_PAD0, OM_BEND_COL_OFF,
     OM_BEND_COL_CNT, OM_ADJ_OFF, OM_ADJ_IDX, OM_TH_OFF, OM_TH_IDX, OM_TH_REST, OM_LOCAL_LEN, OM_STAR_OFF,
     OM_STAR_V0, OM_STAR_V1, OM_EDGE_V0, OM_EDGE_V1, OM_EDGE_BBITS, OM_ISLAND, OM_SCALARS, OM_EDGE_TO_CONSTRAINT,
-    OM_LAYER
+    OM_LAYER, OM_BEND_PAD1, OM_SEW_V0, OM_SEW_V1, OM_SEW_REST, OM_SEW_COL_OFF, OM_SEW_COL_CNT
 };
 uint32_t orc_mesh_get(const orc_mesh *m, int what, void *out) {
 #define CP(ptr, n) do { if (out && (n)) memcpy(out, ptr, (size_t)(n) * 4); return (uint32_t)(n); } while (0)
@@ -900,6 +1181,12 @@ uint32_t orc_mesh_get(const orc_mesh *m, int what, void *out) {
     case OM_ISLAND: CP(m->island, m->nv);
     case OM_EDGE_TO_CONSTRAINT: CP(m->edge_to_constraint, m->ne_in);
     case OM_LAYER: CP(m->layer, m->nv);
+    case OM_BEND_PAD1: CP(m->b_pad1, m->nb);
+    case OM_SEW_V0: CP(m->s_v0, m->ns);
+    case OM_SEW_V1: CP(m->s_v1, m->ns);
+    case OM_SEW_REST: CP(m->s_rest, m->ns);
+    case OM_SEW_COL_OFF: CP(m->s_col_off, m->ns_col);
+    case OM_SEW_COL_CNT: CP(m->s_col_cnt, m->ns_col);
     case OM_SCALARS: {
         /* [nv, nd, nb, nd_col, nb_col, ne, ntri, n_virtual, table_size, has_multi, bits(cell), bits(mean), bits(max)] */
         uint32_t s[13];
@@ -1057,16 +1344,44 @@ typedef struct {
     int wrinkle_on; float wrinkle[22]; uint8_t *wtex; uint32_t ww, wh;
     float *cedges; uint32_t *cedge_flags, nce, *cstarts, *csorted;
     int32_t *jac;                                    /* per-vertex sums {dx, dy, dz, count} */
+    /* sewing: current and target rest lengths, lambdas; the simulator's stiffness, lock and lock
+     * distance (mod.rs:637-640) and shrink speed */
+    real *s_cur, *s_tgt, *lam_s;
+    float sew_stiffness, sew_lock_dist, sew_speed;
+    int sew_lock;
 } orc_sim;
 
+orc_sim *orc_sim_create_sewn(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne,
+                             const uint32_t *faces, uint32_t nf, const float *inv_masses, const float *thicknesses,
+                             const uint32_t *layer_ids, uint32_t layer_id, float thickness, float stiffness,
+                             float bending_stiffness, float compression_stiffness, float shear_stiffness,
+                             const uint32_t *sew, uint32_t nsew, float sew_speed, float sew_lock_distance);
 orc_sim *orc_sim_create(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
                         uint32_t nf, const float *inv_masses, const float *thicknesses, const uint32_t *layer_ids,
                         uint32_t layer_id, float thickness, float stiffness, float bending_stiffness,
                         float compression_stiffness, float shear_stiffness) {
+    return orc_sim_create_sewn(pos, nv, edges, ne, faces, nf, inv_masses, thicknesses, layer_ids, layer_id, thickness,
+                               stiffness, bending_stiffness, compression_stiffness, shear_stiffness, NULL, 0, 1.0f, -1.0f);
+}
+
+/* ClothSimulator::new with sewing_springs (lib.rs:1643-1751); sew_lock_distance < 0: not given. */
+orc_sim *orc_sim_create_sewn(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne,
+                             const uint32_t *faces, uint32_t nf, const float *inv_masses, const float *thicknesses,
+                             const uint32_t *layer_ids, uint32_t layer_id, float thickness, float stiffness,
+                             float bending_stiffness, float compression_stiffness, float shear_stiffness,
+                             const uint32_t *sew, uint32_t nsew, float sew_speed, float sew_lock_distance) {
     orc_sim *s = (orc_sim *)calloc(1, sizeof(orc_sim));
-    s->m = orc_mesh_build(pos, nv, edges, ne, faces, nf, inv_masses, thicknesses, layer_ids, layer_id, thickness,
-                          stiffness, compression_stiffness, shear_stiffness, bending_stiffness);
+    s->m = orc_mesh_build_sewn(pos, nv, edges, ne, faces, nf, inv_masses, thicknesses, layer_ids, layer_id, thickness,
+                               stiffness, compression_stiffness, shear_stiffness, bending_stiffness, sew, nsew,
+                               sew_speed);
     orc_mesh *m = s->m;
+    s->s_cur = (real *)malloc((m->ns ? m->ns : 1) * sizeof(real));
+    s->s_tgt = (real *)calloc(m->ns ? m->ns : 1, sizeof(real));
+    s->lam_s = (real *)calloc(m->ns ? m->ns : 1, sizeof(real));
+    for (uint32_t k = 0; k < m->ns; k++) s->s_cur[k] = (real)m->s_rest[k];
+    s->sew_stiffness = 10000.0f; s->sew_lock = 1;
+    s->sew_lock_dist = sew_lock_distance >= 0.0f ? sew_lock_distance : 0.02f;
+    s->sew_speed = m->sew_speed;
     s->nv = nv;
     size_t N = nv ? nv : 1;
     s->x = (v3 *)malloc(N * sizeof(v3)); s->p = (v3 *)malloc(N * sizeof(v3)); s->v = (v3 *)calloc(N, sizeof(v3));
@@ -1125,6 +1440,7 @@ void orc_sim_free(orc_sim *s) {
     free(s->sdf_tex); free(s->sdf_info); free(s->sdf_xf);
     free(s->pin_v); free(s->pin_t); free(s->pin_w); free(s->jac); free(s->pair_vt); free(s->pair_ee);
     free(s->cedges); free(s->cedge_flags); free(s->cstarts); free(s->csorted); free(s->wtex);
+    free(s->s_cur); free(s->s_tgt); free(s->lam_s);
     free(s);
 }
 
@@ -1662,6 +1978,64 @@ static void k_jacobi_apply(orc_sim *s) {
         A[0] = A[1] = A[2] = A[3] = 0;
     }
 }
+/* sewing_shrink.wgsl:40-49 (once per substep) and the sewing lambdas' clear (dispatch.rs:117-119) */
+static void k_sew_shrink(orc_sim *s, real dt_sub) {
+    real dt = rmax(dt_sub, (real)1e-8f);
+    for (uint32_t k = 0; k < s->m->ns; k++) {
+        s->s_cur[k] = rmax(s->s_tgt[k], s->s_cur[k] - (real)s->sew_speed * dt);
+        s->lam_s[k] = 0;
+    }
+}
+
+/* sewing.wgsl:56-110: XPBD at the simulator's sewing compliance; a pair within
+ * max(thickness sum, lock distance) with the lock on projects at zero compliance to the target
+ * length and keeps that as its current rest length */
+static void k_sewing_one(orc_sim *s, uint32_t k, real dt_sub) {
+    const orc_mesh *m = s->m;
+    uint32_t a = m->s_v0[k], b = m->s_v1[k];
+    real w0 = s->w[a], w1 = s->w[b], wsum = w0 + w1;
+    if (wsum <= (real)1e-7f) return;
+    v3 delta = sub(s->p[a], s->p[b]);
+    real dist = len3(delta);
+    if (dist <= (real)1e-7f) return;
+    real dt = rmax(dt_sub, (real)1e-8f);
+    v3 dir = dvs(delta, dist);
+    float compf = s->sew_stiffness >= 5000.0f ? 0.0f : 1.0f / fmaxf(s->sew_stiffness, 1.0f);
+    real target = s->s_cur[k], comp = (real)compf;
+    real thresh = rmax((real)(m->thick[a] + m->thick[b]), (real)s->sew_lock_dist);
+    if (s->sew_lock && dist <= thresh) {
+        comp = 0;
+        target = s->s_tgt[k];
+        s->s_cur[k] = s->s_tgt[k];
+    }
+    real alpha = comp / (dt * dt);
+    real c = dist - target;
+    real lam = s->lam_s[k];
+    real dl = (-c - alpha * lam) / (wsum + alpha);
+    s->lam_s[k] = lam + dl;
+    s->p[a] = add(s->p[a], mul(dir, w0 * dl));
+    s->p[b] = sub(s->p[b], mul(dir, w1 * dl));
+}
+
+static void sewing_pass(orc_sim *s, real dt_sub) {
+    const orc_mesh *m = s->m;
+    for (uint32_t c = 0; c < m->ns_col; c++)
+        for (uint32_t k = m->s_col_off[c]; k < m->s_col_off[c] + m->s_col_cnt[c]; k++) k_sewing_one(s, k, dt_sub);
+}
+
+void orc_sim_set_sewing(orc_sim *s, float stiffness, int lock, float lock_distance, float speed) {
+    s->sew_stiffness = fmaxf(stiffness, 1.0f);
+    s->sew_lock = lock;
+    s->sew_lock_dist = fmaxf(lock_distance, 0.0f);
+    s->sew_speed = isfinite(speed) ? fmaxf(speed, 0.0f) : 1.0f;
+}
+void orc_sim_sewing_rest_lengths(orc_sim *s, double *out, const double *in) {
+    for (uint32_t k = 0; k < s->m->ns; k++) {
+        if (out) out[k] = (double)s->s_cur[k];
+        if (in) s->s_cur[k] = (real)in[k];
+    }
+}
+
 static void distance_pass(orc_sim *s, real dt_sub) {
     const orc_mesh *m = s->m;
     if (s->solver_mode == 1) {
@@ -2994,11 +3368,13 @@ static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
     for (uint32_t sub = 0; sub < substeps; sub++) {
         memset(s->lam_d, 0, (m->nd ? m->nd : 1) * sizeof(real));
         memset(s->lam_b, 0, (m->nb ? m->nb : 1) * sizeof(real));
+        k_sew_shrink(s, dt_sub);
         k_predict(s, dt_sub);
         k_wrinkle(s, dt_sub);
         for (uint32_t it = 0; it < s->iterations; it++) {
             k_pins(s, dt_sub);
             distance_pass(s, dt_sub);
+            sewing_pass(s, dt_sub);
             for (uint32_t c = 0; c < m->nb_col; c++) {
                 ORC_PARALLEL_FOR
                 for (uint32_t k = m->b_col_off[c]; k < m->b_col_off[c] + m->b_col_cnt[c]; k++) k_bending_one(s, k, dt_sub);
@@ -3011,6 +3387,7 @@ static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
             k_sc_apply(s);
         }
         if (s->self_collision) self_collision_outer(s, sub == 0, sub + 1 == substeps);
+        sewing_pass(s, dt_sub);
         k_update_vel(s, dt_sub);
     }
 }
@@ -3030,11 +3407,13 @@ void orc_sim_substep_parts(orc_sim *s, float dt_sub, int do_collide, int do_self
     real dt = (real)dt_sub;
     memset(s->lam_d, 0, (m->nd ? m->nd : 1) * sizeof(real));
     memset(s->lam_b, 0, (m->nb ? m->nb : 1) * sizeof(real));
+    k_sew_shrink(s, dt);
     k_predict(s, dt);
     k_wrinkle(s, dt);
     for (uint32_t it = 0; it < s->iterations; it++) {
         k_pins(s, dt);
         distance_pass(s, dt);
+        sewing_pass(s, dt);
         for (uint32_t c = 0; c < m->nb_col; c++) {
             ORC_PARALLEL_FOR
             for (uint32_t k = m->b_col_off[c]; k < m->b_col_off[c] + m->b_col_cnt[c]; k++) k_bending_one(s, k, dt);
@@ -3048,6 +3427,7 @@ void orc_sim_substep_parts(orc_sim *s, float dt_sub, int do_collide, int do_self
     }
     s->pair_dt_frame = dt;
     if (do_self && s->self_collision) self_collision_outer(s, 1, 1);
+    sewing_pass(s, dt);
     if (do_update) k_update_vel(s, dt);
 }
 

@@ -45,6 +45,9 @@ EXPORTS = [
     "tc_sim_set_sdf_colliders", "tc_sim_update_bone_transforms", "tc_bake_mesh_sdf", "tc_sdf_result_info",
     "tc_sdf_result_timing", "tc_sdf_result_texels", "tc_sdf_result_atlas", "tc_sdf_result_nearest", "tc_sdf_result_free", "tc_close_open_boundary_loops",
     "tc_bake_bone_sdf", "tc_bone_sdf_result_info", "tc_bone_sdf_result_arrays", "tc_bone_sdf_result_free",
+    "tc_sewing_params_default", "tc_sim_create_sewn", "tc_sim_set_sewing_stiffness", "tc_sim_set_sewing_shrink_speed",
+    "tc_sim_set_enable_sewing_lock", "tc_sim_set_sewing_lock_distance", "tc_sim_get_sewing_state",
+    "tc_sim_get_sewing_rest_lengths", "tc_sim_set_sewing_rest_lengths", "tc_mesh_build_arrays_sewn",
 ]
 
 TC_OK, TC_ERR_VALUE, TC_ERR_RUNTIME, TC_ERR_UNSUPPORTED, TC_ERR_IO = 0, 1, 2, 3, 4
@@ -66,6 +69,15 @@ class TcParams(C.Structure):
         ("shear_stiffness", C.c_float),
         ("workgroup_size", C.c_uint32),
         ("solver_mode", C.c_uint32),
+    ]
+
+
+class TcSewingParams(C.Structure):
+    _fields_ = [
+        ("shrink_speed", C.c_float),
+        ("stiffness", C.c_float),
+        ("lock", C.c_int32),
+        ("lock_distance", C.c_float),
     ]
 
 
@@ -206,6 +218,18 @@ def load() -> C.CDLL:
         "tc_bone_sdf_result_info": (_i, [_vp, _vp, _vp, _vp]),
         "tc_bone_sdf_result_arrays": (_i, [_vp, _vp, _vp, _vp, _vp]),
         "tc_bone_sdf_result_free": (None, [_vp]),
+        "tc_sewing_params_default": (None, [_vp]),
+        "tc_sim_create_sewn": (_i, [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _u32, _vp, _vp,
+                                   C.POINTER(_vp)]),
+        "tc_sim_set_sewing_stiffness": (_i, [_vp, _f32]),
+        "tc_sim_set_sewing_shrink_speed": (_i, [_vp, _f32]),
+        "tc_sim_set_enable_sewing_lock": (_i, [_vp, _i]),
+        "tc_sim_set_sewing_lock_distance": (_i, [_vp, _f32]),
+        "tc_sim_get_sewing_state": (_i, [_vp, _vp]),
+        "tc_sim_get_sewing_rest_lengths": (_i, [_vp, _vp, _u32]),
+        "tc_sim_set_sewing_rest_lengths": (_i, [_vp, _vp, _u32]),
+        "tc_mesh_build_arrays_sewn": (_i, [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _u32, _vp, _vp, _i, _vp,
+                                          C.c_size_t, C.POINTER(C.c_size_t)]),
         "tc_close_open_boundary_loops": (_i, [_vp, _u32, _vp, _u32, _vp, C.c_size_t, _vp, C.c_size_t,
                                               C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32)]),
         "tc_last_error": (C.c_char_p, []),

@@ -32,7 +32,18 @@ hipError_t launch_distance_jacobi(hipStream_t s, float4 *P, const float4 *X, con
                                   float *lam, uint32_t nd, int4 *acc, uint32_t nv, const SubstepArgs &a, bool first,
                                   bool split);
 hipError_t launch_bending(hipStream_t s, float4 *P, const float4 *X, const uint4 *rec, float *lam, uint32_t off,
-                          uint32_t cnt, const SubstepArgs &a, bool first, uint32_t block);
+                          uint32_t cnt, const SubstepArgs &a, bool first, uint32_t block, const uint4 *gate = nullptr);
+// sewing (tc_sewing.hip, sewing.wgsl / sewing_shrink.wgsl): rec {v0, v1, current rest bits, target rest bits}
+struct SewArgs {
+    float dt;              // substep length
+    float compliance;      // 0 for stiffness >= 5000, else 1 / max(stiffness, 1)
+    float lock;            // > 0.5: lock pairs closer than max(thickness sum, lock_dist)
+    float lock_dist;
+    float speed;           // shrink speed of the current rest lengths (m/s)
+};
+hipError_t launch_sew_shrink(hipStream_t s, uint4 *rec, float *lam, uint32_t n, const SewArgs &a);
+hipError_t launch_sewing(hipStream_t s, float4 *P, const float4 *X, uint4 *rec, float *lam, uint32_t off, uint32_t cnt,
+                         const SewArgs &a, uint32_t block);
 // SDF colliders (set_bone_sdf_colliders, crates/cloth_core/src/simulation/mod.rs:1630-1727): an
 // Rg16Float volume (pack2x16float(distance, weight) per texel, Z/Y/X order), 20 floats of
 // GpuBoneInfo per collider and its world / inverse-world matrices (16 + 16 floats, each given

@@ -28,3 +28,9 @@ static_assert(offsetof(tc_instance, faces) == 32, "tc_instance.faces");
 static_assert(offsetof(tc_instance, num_faces) == 40, "tc_instance.num_faces");
 static_assert(offsetof(tc_instance, inv_masses) == 48, "tc_instance.inv_masses");
 static_assert(offsetof(tc_instance, thicknesses) == 56, "tc_instance.thicknesses");
+
+static_assert(sizeof(tc_sewing_params) == 16, "tc_sewing_params size");
+static_assert(offsetof(tc_sewing_params, shrink_speed) == 0, "tc_sewing_params.shrink_speed");
+static_assert(offsetof(tc_sewing_params, stiffness) == 4, "tc_sewing_params.stiffness");
+static_assert(offsetof(tc_sewing_params, lock) == 8, "tc_sewing_params.lock");
+static_assert(offsetof(tc_sewing_params, lock_distance) == 12, "tc_sewing_params.lock_distance");

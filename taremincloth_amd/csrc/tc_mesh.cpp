@@ -4,8 +4,10 @@
 // the Welsh-Powell colouring (crates/cloth_core/src/coloring.rs:6-173), the
 // virtual-sample count that sizes the hash table
 // (crates/cloth_core/src/virtual_mesh.rs:75-200) and the hash cell/table rule
-// (crates/cloth_core/src/simulation/pipelines.rs:899-912).  Sewing springs and
-// the optional coarse (2-hop) constraints are not part of this build.
+// (crates/cloth_core/src/simulation/pipelines.rs:899-912).  Sewing springs give
+// seam-gated bending (mesh.rs:706-831), sewing constraints (mesh.rs:909-947) and
+// the contracted self-collision topology (mesh.rs:991-1191, 1291-1344, 1382-1490).
+// The optional coarse (2-hop) constraints are not part of this build.
 //
 // Constraint order decides the Gauss-Seidel order inside a colour only up to
 // independence, but the colour each constraint gets depends on the order the
@@ -16,6 +18,8 @@
 #include "tc_mesh.h"
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <deque>
@@ -115,6 +119,92 @@ uint32_t morton(const float *p, const float *bmin, const float *inv) {  // mesh.
     for (int a = 0; a < 3; ++a)
         q[a] = as_u32_sat(std::min(std::max((p[a] - bmin[a]) * inv[a], 0.0f), 1.0f) * 1023.0f);
     return (expand_bits_10(q[0]) << 2) | (expand_bits_10(q[1]) << 1) | expand_bits_10(q[2]);
+}
+
+// Union-find over the sewing pairs; every vertex maps to the smallest vertex of its group.
+std::vector<uint32_t> sew_groups(uint32_t nv, const uint32_t *sew, uint32_t nsew) {
+    std::vector<uint32_t> parent(nv);
+    std::iota(parent.begin(), parent.end(), 0u);
+    auto find = [&](uint32_t i) {
+        uint32_t r = i;
+        while (r != parent[r]) r = parent[r];
+        while (i != r) { const uint32_t n = parent[i]; parent[i] = r; i = n; }
+        return r;
+    };
+    for (uint32_t k = 0; k < nsew; ++k) {
+        const uint32_t u = sew[2 * k], v = sew[2 * k + 1];
+        if (u >= nv || v >= nv || u == v) continue;
+        const uint32_t ru = find(u), rv = find(v);
+        if (ru != rv) parent[std::max(ru, rv)] = std::min(ru, rv);
+    }
+    std::vector<uint32_t> rep(nv);
+    for (uint32_t i = 0; i < nv; ++i) rep[i] = find(i);
+    return rep;
+}
+
+inline void sub3(const float *a, const float *b, float *o) { o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2]; }
+inline float norm3(const float *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+// Distance of two opposite vertices once their triangles are unfolded flat about a shared seam
+// axis: vertex q over edge (e0, e1) at along-axis s and height h, likewise r over (f0, f1);
+// returns sqrt((s_q - s_r)^2 + (h_q + h_r)^2), or -1 when an edge is shorter than 1e-6
+// (mesh.rs:789-822, 1436-1458).
+float unfolded_length(const float *pos, uint32_t e0, uint32_t e1, uint32_t qv, uint32_t f0, uint32_t f1, uint32_t rv) {
+    float ea[3], eb[3];
+    sub3(pos + 3 * e1, pos + 3 * e0, ea);
+    sub3(pos + 3 * f1, pos + 3 * f0, eb);
+    const float la = norm3(ea), lb = norm3(eb);
+    if (!(la > 1e-6f && lb > 1e-6f)) return -1.0f;
+    const float ua[3] = {ea[0] / la, ea[1] / la, ea[2] / la};
+    float da[3];
+    sub3(pos + 3 * qv, pos + 3 * e0, da);
+    const float sa = da[0] * ua[0] + da[1] * ua[1] + da[2] * ua[2];
+    const float pa[3] = {da[0] - sa * ua[0], da[1] - sa * ua[1], da[2] - sa * ua[2]};
+    const float ha = norm3(pa);
+    const float ub[3] = {eb[0] / lb, eb[1] / lb, eb[2] / lb};
+    float db[3];
+    sub3(pos + 3 * rv, pos + 3 * f0, db);
+    const float sb = db[0] * ub[0] + db[1] * ub[1] + db[2] * ub[2];
+    const float pb[3] = {db[0] - sb * ub[0], db[1] - sb * ub[1], db[2] - sb * ub[2]};
+    const float hb = norm3(pb);
+    const float ds = sa - sb, dh = ha + hb;
+    return std::sqrt(ds * ds + dh * dh);
+}
+
+// Rest length of a true two-hop pair that only the sewing joins (compute_seam_two_hop_rest_length,
+// mesh.rs:1382-1490): over original neighbours m1 of i and m2 of nbr in one sewing group, the
+// direct distance when m1 == m2, else the unfolded length over a shared seam edge (m1-k1, m2-k2
+// with k1, k2 in one group), else |i - m1| + |nbr - m2|; the smallest, or 0 if none exceeds 1e-5.
+float seam_two_hop_rest(uint32_t i, uint32_t nbr, const float *pos, const std::vector<std::vector<uint32_t>> &orig,
+                        const std::vector<uint32_t> &rep) {
+    const uint32_t ri = rep[i], rn = rep[nbr];
+    float best = 3.4028235e38f;
+    for (uint32_t m1 : orig[i]) {
+        const uint32_t rm1 = rep[m1];
+        if (rm1 == ri || rm1 == rn) continue;
+        for (uint32_t m2 : orig[nbr]) {
+            if (rep[m2] != rm1) continue;
+            if (m1 == m2) {
+                best = std::min(best, dist_p0_minus_p1(pos, i, nbr));
+                continue;
+            }
+            bool found = false;
+            for (uint32_t k1 : orig[m1]) {
+                const uint32_t rk1 = rep[k1];
+                if (rk1 == rm1) continue;
+                for (uint32_t k2 : orig[m2]) {
+                    if (rep[k2] != rk1) continue;
+                    const float fl = unfolded_length(pos, m1, k1, i, m2, k2, nbr);
+                    if (fl >= 0.0f && fl < best) { best = fl; found = true; }
+                }
+            }
+            if (!found) {
+                const float slit = dist_p0_minus_p1(pos, i, m1) + dist_p0_minus_p1(pos, nbr, m2);
+                best = std::min(best, slit);
+            }
+        }
+    }
+    return (best < 3.4028235e38f && std::isfinite(best) && best > 1e-5f) ? best : 0.0f;
 }
 
 }  // namespace
@@ -261,6 +351,41 @@ std::string build_mesh(const MeshInput &in, Mesh &m) {
         if (o.size() != 2 || o[0] == o[1]) continue;
         bcs.push_back({uint32_t(k >> 32), uint32_t(k), o[0], o[1], dist_p0_minus_p1(pos, o[0], o[1]), bend_c, 0.f, 0.f});
     }
+    const bool sewing = in.sew && in.nsew;
+    // seam-gated bending (mesh.rs:706-831): two boundary edges whose end points are sewn pairwise
+    // (one group per end) bend about the seam: the opposite vertices keep their unfolded distance,
+    // only while closer than it and while the seam's first pair is within half of it (>= 1 cm)
+    if (sewing && in.faces && in.nf) {
+        const std::vector<uint32_t> rep = sew_groups(nv, in.sew, in.nsew);
+        std::vector<uint8_t> has_sew(nv, 0);
+        for (uint32_t k = 0; k < in.nsew; ++k) {
+            const uint32_t u = in.sew[2 * k], v = in.sew[2 * k + 1];
+            if (u < nv && v < nv && u != v) has_sew[u] = has_sew[v] = 1;
+        }
+        // boundary edges (one opposite vertex) in ascending key order, grouped by their pair of
+        // sewing groups, each aligned so that its first end is in the smaller group
+        std::map<uint64_t, std::vector<std::array<uint32_t, 3>>> seam;
+        for (uint64_t k : sorted_keys) {
+            const auto &o = edge_opp[k];
+            if (o.size() != 1) continue;
+            const uint32_t ea = uint32_t(k >> 32), eb = uint32_t(k);
+            if (!has_sew[ea] || !has_sew[eb]) continue;
+            const uint32_t ra = rep[ea], rb = rep[eb];
+            if (ra == rb) continue;
+            if (ra < rb) seam[key2(ra, rb)].push_back({ea, eb, o[0]});
+            else seam[key2(rb, ra)].push_back({eb, ea, o[0]});
+        }
+        for (const auto &kv : seam) {
+            if (kv.second.size() != 2) continue;
+            const auto &A = kv.second[0], &B = kv.second[1];
+            const uint32_t a0 = A[0], a1 = A[1], v2 = A[2], b0 = B[0], b1 = B[1], v3 = B[2];
+            if (a0 == b0 || a1 == b1 || v2 == v3 || rep[v2] == rep[v3]) continue;
+            const float flat = unfolded_length(pos, a0, a1, v2, b0, b1, v3);
+            if (flat < 0.0f || flat <= 1e-5f) continue;
+            bcs.push_back({a0, b0, v2, v3, flat, bend_c, 1.0f, std::max(flat * 0.5f, 0.01f)});
+            m.has_seams = true;
+        }
+    }
     {
         std::unordered_set<uint64_t> face_edges;
         if (in.faces)
@@ -331,6 +456,37 @@ std::string build_mesh(const MeshInput &in, Mesh &m) {
         }
     }
 
+    // --- sewing constraints (mesh.rs:909-947) and their colouring (coloring.rs:236-300) ---
+    m.sew_shrink_speed = std::max(in.sew_shrink_speed, 0.0f);
+    if (sewing) {
+        const float comp = compliance_of(in.sew_stiffness);
+        std::vector<uint32_t> sv0, sv1;
+        std::vector<float> srest;
+        for (uint32_t k = 0; k < in.nsew; ++k) {
+            const uint32_t a = in.sew[2 * k], b = in.sew[2 * k + 1];
+            if (!valid_edge(a, b)) continue;
+            sv0.push_back(a);
+            sv1.push_back(b);
+            srest.push_back(dist_p0_minus_p1(pos, a, b));
+        }
+        WPInput wp;
+        wp.reg.resize(sv0.size());
+        for (size_t i = 0; i < sv0.size(); ++i) wp.reg[i] = {sv0[i], sv1[i]};
+        wp.deg = wp.reg;
+        wp.use = wp.reg;
+        uint32_t ncol = 0;
+        const std::vector<uint32_t> col = welsh_powell(nv, wp, ncol);
+        const std::vector<uint32_t> perm = bucket_perm(col, ncol, m.s_col);
+        const size_t n = sv0.size();
+        m.s_v0.resize(n); m.s_v1.resize(n); m.s_rest.resize(n); m.s_target.assign(n, 0.0f);
+        m.s_speed.assign(n, m.sew_shrink_speed); m.s_comp.assign(n, comp); m.s_lock.assign(n, in.sew_lock ? 1.0f : 0.0f);
+        for (size_t q = 0; q < n; ++q) {
+            m.s_v0[q] = sv0[perm[q]];
+            m.s_v1[q] = sv1[perm[q]];
+            m.s_rest[q] = srest[perm[q]];
+        }
+    }
+
     // --- adjacency, local edge lengths (mesh.rs:949-1075) ---
     std::vector<std::vector<uint32_t>> adj(nv);
     std::vector<float> len_sum(nv, 0.0f);
@@ -351,6 +507,37 @@ std::string build_mesh(const MeshInput &in, Mesh &m) {
     }
     m.mean_edge = nvalid ? total / float(nvalid) : 0.0f;
     m.max_edge = mx;
+    // the original adjacency (edge order, before sewing) and the sewing groups
+    const std::vector<std::vector<uint32_t>> orig = adj;
+    std::vector<uint32_t> rep(nv);
+    std::iota(rep.begin(), rep.end(), 0u);
+    if (sewing) {
+        // sewn vertices are one node (mesh.rs:991-1063): a group's vertices are neighbours of each
+        // other, and every neighbour of a group vertex is a neighbour of the whole group (both ways)
+        rep = sew_groups(nv, in.sew, in.nsew);
+        std::vector<std::vector<uint32_t>> group(nv);
+        for (uint32_t i = 0; i < nv; ++i) group[rep[i]].push_back(i);
+        for (const auto &g : group)
+            if (g.size() > 1)
+                for (uint32_t u : g)
+                    for (uint32_t v : g)
+                        if (u != v) adj[u].push_back(v);
+        const std::vector<std::vector<uint32_t>> base = adj;
+        for (const auto &g : group) {
+            if (g.size() < 2) continue;
+            std::vector<uint32_t> shared;
+            for (uint32_t u : g)
+                for (uint32_t n : base[u])
+                    if (rep[n] != rep[u]) shared.push_back(n);
+            std::sort(shared.begin(), shared.end());
+            shared.erase(std::unique(shared.begin(), shared.end()), shared.end());
+            for (uint32_t u : g)
+                for (uint32_t n : shared) {
+                    adj[u].push_back(n);
+                    adj[n].push_back(u);
+                }
+        }
+    }
     m.local_len.resize(nv);
     for (uint32_t i = 0; i < nv; ++i)
         m.local_len[i] = len_cnt[i] ? len_sum[i] / float(len_cnt[i]) : std::max(in.thickness, 0.01f);
@@ -369,8 +556,23 @@ std::string build_mesh(const MeshInput &in, Mesh &m) {
     if (in.faces)
         for (uint32_t k = 0; k < 3 * in.nf; ++k)
             if (in.faces[k] < nv) is_face[in.faces[k]] = 1;
+    // one-hop neighbours of the contracted graph, as group representatives (mesh.rs:1103-1118)
+    std::vector<std::vector<uint32_t>> c1(sewing ? nv : 0);
+    if (sewing) {
+        for (uint32_t e = 0; e < in.ne; ++e) {
+            const uint32_t a = in.edges[2 * e], b = in.edges[2 * e + 1];
+            if (!valid_edge(a, b) || rep[a] == rep[b]) continue;
+            c1[rep[a]].push_back(rep[b]);
+            c1[rep[b]].push_back(rep[a]);
+        }
+        for (auto &l : c1) {
+            std::sort(l.begin(), l.end());
+            l.erase(std::unique(l.begin(), l.end()), l.end());
+        }
+    }
+    std::unordered_map<uint64_t, float> seam_memo;
     m.th_off.assign(nv + 1, 0);
-    std::vector<uint32_t> nbh;
+    std::vector<uint32_t> nbh, onb;
     for (uint32_t i = 0; i < nv; ++i) {
         nbh.clear();
         nbh.push_back(i);
@@ -380,11 +582,42 @@ std::string build_mesh(const MeshInput &in, Mesh &m) {
         }
         std::sort(nbh.begin(), nbh.end());
         nbh.erase(std::unique(nbh.begin(), nbh.end()), nbh.end());
+        if (sewing) {  // two-hop neighbours of the original mesh
+            onb.clear();
+            onb.push_back(i);
+            for (uint32_t u : orig[i]) {
+                onb.push_back(u);
+                onb.insert(onb.end(), orig[u].begin(), orig[u].end());
+            }
+            std::sort(onb.begin(), onb.end());
+            onb.erase(std::unique(onb.begin(), onb.end()), onb.end());
+        }
         for (uint32_t j : nbh) {
             m.th_idx.push_back(j);
-            const bool two_hop = j != i && is_face[i] && is_face[j] &&
-                                 !std::binary_search(adj[i].begin(), adj[i].end(), j);
-            m.th_rest.push_back(two_hop ? dist_p0_minus_p1(pos, i, j) : 0.0f);
+            float rest = 0.0f;
+            if (!sewing) {
+                const bool two_hop = j != i && is_face[i] && is_face[j] &&
+                                     !std::binary_search(adj[i].begin(), adj[i].end(), j);
+                if (two_hop) rest = dist_p0_minus_p1(pos, i, j);
+            } else {
+                // a true two-hop pair of the contracted graph: face vertices of different groups,
+                // not joined by an original edge, their groups not neighbours (mesh.rs:1160-1171)
+                const bool two_hop = j != i && is_face[i] && is_face[j] && rep[i] != rep[j] &&
+                                     std::find(orig[i].begin(), orig[i].end(), j) == orig[i].end() &&
+                                     !std::binary_search(c1[rep[i]].begin(), c1[rep[i]].end(), rep[j]);
+                if (two_hop) {
+                    if (std::binary_search(onb.begin(), onb.end(), j)) {
+                        rest = dist_p0_minus_p1(pos, i, j);
+                    } else {
+                        const uint64_t key = ukey(i, j);
+                        auto it = seam_memo.find(key);
+                        if (it == seam_memo.end())
+                            it = seam_memo.emplace(key, seam_two_hop_rest(i, j, pos, orig, rep)).first;
+                        rest = it->second;
+                    }
+                }
+            }
+            m.th_rest.push_back(rest);
         }
         m.th_off[i + 1] = uint32_t(m.th_idx.size());
     }
@@ -451,22 +684,35 @@ std::string build_mesh(const MeshInput &in, Mesh &m) {
             std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return code[a] < code[b]; });
             for (uint32_t k : idx) { m.e_v0.push_back(es[k].first); m.e_v1.push_back(es[k].second); }
         }
-        // open-boundary bits: an edge in at most one triangle (mesh.rs:1291-1344, no sewing)
+        // open-boundary bits (mesh.rs:1291-1344): an edge in no triangle, or in one triangle whose
+        // contracted edge (sewing groups) is in one triangle too; edges inside a group are never open
         const size_t ne = m.e_v0.size();
         m.e_bbits.assign((ne + 31) / 32, 0u);
-        std::unordered_map<uint64_t, uint32_t> tri_count;
+        std::unordered_map<uint64_t, uint32_t> tri_count, ctri_count;
         if (in.faces && in.nf)
             for (uint32_t f = 0; f < in.nf; ++f) {
                 const uint32_t f0 = in.faces[3 * f], f1 = in.faces[3 * f + 1], f2 = in.faces[3 * f + 2];
                 const uint32_t pr[3][2] = {{f0, f1}, {f1, f2}, {f2, f0}};
                 for (const auto &p : pr)
-                    if (p[0] < nv && p[1] < nv && p[0] != p[1]) tri_count[ukey(p[0], p[1])]++;
+                    if (p[0] < nv && p[1] < nv && p[0] != p[1]) {
+                        tri_count[ukey(p[0], p[1])]++;
+                        if (rep[p[0]] != rep[p[1]]) ctri_count[ukey(rep[p[0]], rep[p[1]])]++;
+                    }
             }
         for (size_t k = 0; k < ne; ++k) {
-            const bool open = !(in.faces && in.nf) || [&] {
-                auto it = tri_count.find(key2(m.e_v0[k], m.e_v1[k]));
-                return it == tri_count.end() || it->second <= 1;
-            }();
+            const uint32_t a = m.e_v0[k], b = m.e_v1[k];
+            if (rep[a] == rep[b]) continue;
+            bool open = true;
+            if (in.faces && in.nf) {
+                auto it = tri_count.find(key2(a, b));
+                const uint32_t raw = it == tri_count.end() ? 0u : it->second;
+                if (raw == 1) {
+                    auto ct = ctri_count.find(ukey(rep[a], rep[b]));
+                    open = ct != ctri_count.end() && ct->second == 1;
+                } else {
+                    open = raw == 0;
+                }
+            }
             if (open) m.e_bbits[k >> 5] |= 1u << (k & 31);
         }
     }

@@ -20,6 +20,13 @@ struct MeshInput {
     uint32_t layer_id = 0;
     float thickness = 0.005f;
     float tension_k = 1000.f, compression_k = 1000.f, shear_k = 500.f, bending_k = 10.f;
+    // sewing springs (vertex pairs, 2 * nsew); with sewing the sewn vertices count as one node of
+    // the self-collision topology and seams get gated bending constraints
+    const uint32_t *sew = nullptr;
+    uint32_t nsew = 0;
+    float sew_shrink_speed = 1.0f;  // m/s
+    float sew_stiffness = 10000.f;  // compliance stored per constraint (the solver uses the simulator's)
+    bool sew_lock = true;           // lock flag stored per constraint (likewise)
 };
 
 struct ColorSet {
@@ -42,6 +49,14 @@ struct Mesh {
     std::vector<uint32_t> b_v0, b_v1, b_v2, b_v3;
     std::vector<float> b_rest, b_comp, b_pad0, b_pad1;
     ColorSet b_col;
+
+    // sewing constraints in colour order: current (shrinking) and target rest lengths, shrink speed,
+    // stored compliance and lock flag (GpuSewingConstraint, mesh.rs:51-60)
+    std::vector<uint32_t> s_v0, s_v1;
+    std::vector<float> s_rest, s_target, s_speed, s_comp, s_lock;
+    ColorSet s_col;
+    float sew_shrink_speed = 1.0f;
+    bool has_seams = false;  // some bending constraint is seam-gated (_pad0 in (0.5, 1.5))
 
     // topology for self-collision
     std::vector<uint32_t> adj_off, adj_idx;

@@ -87,9 +87,10 @@ hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t coun
                 if (e != hipSuccess) return e;
             }
         }
+        if (!m.s_v0.empty() && (e = enqueue_sewing(s, s->P.p, a.dt)) != hipSuccess) return e;  // dispatch.rs:183-191
         for (size_t c = 0; c < m.b_col.count.size(); ++c) {
             e = tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p, m.b_col.offset[c],
-                                   m.b_col.count[c], a, f, B);
+                                   m.b_col.count[c], a, f, B, s->bgate.p);
             if (e != hipSuccess) return e;
         }
         s->prof_end(q);
@@ -142,8 +143,16 @@ hipError_t enqueue_substep_head(tc_sim *s, float dt_sub, bool gate_to_host, uint
     const tc::SubstepArgs a = s->args(dt_sub);
     const tc::Mesh &m = s->mesh;
     const uint32_t n = m.nv, B = s->block;
-    int q = s->prof_begin("predict");
-    hipError_t e = tc::launch_predict(s->stream, s->P.p, s->X.p, s->V.p, n, a, B);
+    int q;
+    hipError_t e;
+    if (!m.s_v0.empty()) {  // the rest lengths shrink once per substep; the sewing lambdas clear (dispatch.rs:117-134)
+        q = s->prof_begin("sew_shrink");
+        e = enqueue_sew_shrink(s, dt_sub);
+        s->prof_end(q);
+        if (e != hipSuccess) return e;
+    }
+    q = s->prof_begin("predict");
+    e = tc::launch_predict(s->stream, s->P.p, s->X.p, s->V.p, n, a, B);
     s->prof_end(q);
     if (e != hipSuccess) return e;
     if (s->wrinkle_on && s->wrinkle_tex.p) {  // dispatch.rs:151-165
@@ -283,6 +292,12 @@ hipError_t enqueue_substep_tail(tc_sim *s, float dt_sub, uint32_t pc, uint32_t l
         e = tc::launch_sc_apply(s->stream, Pc, s->local_len.p, s->accum.p, sa, B);
         s->prof_end(q);
         s->prof_end(scope);
+        if (e != hipSuccess) return e;
+    }
+    if (!m.s_v0.empty()) {  // the final sewing pass before the velocity update (dispatch.rs:318-339)
+        q = s->prof_begin("final_sewing");
+        e = enqueue_sewing(s, Pc, dt_sub);
+        s->prof_end(q);
         if (e != hipSuccess) return e;
     }
     if (update) {
@@ -558,9 +573,37 @@ static int check_topology(const float *positions, uint32_t nv, const uint32_t *e
     return TC_OK;
 }
 
-int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
-                  uint32_t nf, const float *inv_masses, const float *thicknesses, const uint32_t *layer_ids,
-                  const tc_sim_params *params, tc_sim **out) {
+void tc_sewing_params_default(tc_sewing_params *p) {
+    if (!p) return;
+    p->shrink_speed = 1.0f;
+    p->stiffness = -1.0f;
+    p->lock = -1;
+    p->lock_distance = -1.0f;
+}
+
+// Sewing inputs of the mesh build (lib.rs:1643-1665 validation, mesh.rs:909-947 defaults).
+static int sewing_input(uint32_t nv, const uint32_t *sew, uint32_t nsew, const tc_sewing_params *sp,
+                        tc::MeshInput &in) {
+    if (nsew && !sew) return fail(TC_ERR_VALUE, "sewing springs are null");
+    for (size_t k = 0; k < size_t(nsew) * 2; k += 2)
+        if (sew[k] >= nv || sew[k + 1] >= nv)
+            return fail(TC_ERR_VALUE, "sewing_springs contain an index >= number of vertices (" + std::to_string(nv) +
+                                          "): [" + std::to_string(sew[k]) + ", " + std::to_string(sew[k + 1]) + "]");
+    tc_sewing_params def;
+    tc_sewing_params_default(&def);
+    const tc_sewing_params &q = sp ? *sp : def;
+    in.sew = nsew ? sew : nullptr;
+    in.nsew = nsew;
+    in.sew_shrink_speed = q.shrink_speed;
+    in.sew_stiffness = q.stiffness >= 0.0f ? q.stiffness : 10000.0f;
+    in.sew_lock = q.lock != 0;
+    return TC_OK;
+}
+
+int tc_sim_create_sewn(const float *positions, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
+                       uint32_t nf, const float *inv_masses, const float *thicknesses, const uint32_t *layer_ids,
+                       const uint32_t *sewing, uint32_t num_sewing, const tc_sewing_params *sewing_params,
+                       const tc_sim_params *params, tc_sim **out) {
     g_err.clear();
     if (!out) return fail(TC_ERR_VALUE, "out handle pointer is null");
     *out = nullptr;
@@ -568,12 +611,22 @@ int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, ui
     tc_sim_params_default(&def);
     const tc_sim_params &pp = params ? *params : def;
     if (int rc = check_topology(positions, nv, edges, ne, faces, nf)) return rc;
-    const tc::MeshInput in = mesh_input(positions, nv, edges, ne, faces, nf, inv_masses, thicknesses, layer_ids, pp);
+    tc::MeshInput in = mesh_input(positions, nv, edges, ne, faces, nf, inv_masses, thicknesses, layer_ids, pp);
+    if (int rc = sewing_input(nv, sewing, num_sewing, sewing_params, in)) return rc;
 
     tc_sim *s = new tc_sim();
     std::string err = tc::build_mesh(in, s->mesh);
     if (!err.empty()) { delete s; return fail(TC_ERR_VALUE, err); }
+    if (sewing_params && sewing_params->lock_distance >= 0.0f)  // lib.rs:1749-1751
+        s->sew_lock_dist = sewing_params->lock_distance;
     return finish_create(s, pp, out);
+}
+
+int tc_sim_create(const float *positions, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
+                  uint32_t nf, const float *inv_masses, const float *thicknesses, const uint32_t *layer_ids,
+                  const tc_sim_params *params, tc_sim **out) {
+    return tc_sim_create_sewn(positions, nv, edges, ne, faces, nf, inv_masses, thicknesses, layer_ids, nullptr, 0,
+                              nullptr, params, out);
 }
 
 // Device state of a simulator whose mesh is built; takes ownership of s.
@@ -716,6 +769,9 @@ static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
         (he = hipMemsetAsync(s->blam2.p, 0, sizeof(float) * std::max<size_t>(m.b_v0.size(), 1), s->stream)) != hipSuccess ||
         (he = hipStreamSynchronize(s->stream)) != hipSuccess)
         return bail(fail(TC_ERR_RUNTIME, std::string("device init: ") + hipGetErrorString(he)));
+    if (int rc = upload_sewing(s)) return bail(rc);
+    if ((he = hipStreamSynchronize(s->stream)) != hipSuccess)
+        return bail(fail(TC_ERR_RUNTIME, std::string("device init: ") + hipGetErrorString(he)));
     build_plan(s);
     frames_init(s);
     *out = s;
@@ -820,7 +876,7 @@ static int time_iterations(tc_sim *s, float dt_sub, uint32_t its, bool fused, do
         }
         for (size_t c = 0; c < m.b_col.count.size(); ++c) {
             hipError_t r = tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p, m.b_col.offset[c],
-                                              m.b_col.count[c], a, it == 0, s->block);
+                                              m.b_col.count[c], a, it == 0, s->block, s->bgate.p);
             if (r != hipSuccess) return r;
         }
         return hipSuccess;
@@ -901,7 +957,8 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
                             hipError_t r = hipSuccess;
                             for (size_t c = 0; c < nbc && r == hipSuccess; ++c)
                                 r = tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p,
-                                                       m.b_col.offset[c], m.b_col.count[c], a, it == 0, B);
+                                                       m.b_col.offset[c], m.b_col.count[c], a, it == 0, B,
+                                                       s->bgate.p);
                             return r;
                         })) != hipSuccess) return e;
         }
@@ -1650,6 +1707,14 @@ int tc_sim_residual(tc_sim *s, float dt_sub, double *rel, double *max_rel) {
 int tc_mesh_build_arrays(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
                          uint32_t nf, const float *inv_masses, const float *thicknesses, const tc_sim_params *params,
                          int which, void *out, size_t cap, size_t *count) {
+    return tc_mesh_build_arrays_sewn(pos, nv, edges, ne, faces, nf, inv_masses, thicknesses, nullptr, 0, nullptr,
+                                     params, which, out, cap, count);
+}
+
+int tc_mesh_build_arrays_sewn(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne, const uint32_t *faces,
+                              uint32_t nf, const float *inv_masses, const float *thicknesses, const uint32_t *sewing,
+                              uint32_t num_sewing, const tc_sewing_params *sewing_params, const tc_sim_params *params,
+                              int which, void *out, size_t cap, size_t *count) {
     g_err.clear();
     if (nv > 0 && !pos) return fail(TC_ERR_VALUE, "positions is null");
     if (ne > 0 && !edges) return fail(TC_ERR_VALUE, "edges is null");
@@ -1670,6 +1735,7 @@ int tc_mesh_build_arrays(const float *pos, uint32_t nv, const uint32_t *edges, u
     in.compression_k = pp.compression_stiffness >= 0.0f ? pp.compression_stiffness : pp.stiffness;
     in.shear_k = pp.shear_stiffness >= 0.0f ? pp.shear_stiffness : pp.stiffness * 0.5f;
     in.bending_k = pp.bending_stiffness;
+    if (int rc = sewing_input(nv, sewing, num_sewing, sewing_params, in)) return rc;
     tc::Mesh m;
     const std::string err = tc::build_mesh(in, m);
     if (!err.empty()) return fail(TC_ERR_VALUE, err);
@@ -1719,6 +1785,15 @@ int tc_mesh_build_arrays(const float *pos, uint32_t nv, const uint32_t *edges, u
     case 35: return host(m.adj_off.data(), m.adj_off.size());
     case 36: return host(m.adj_idx.data(), m.adj_idx.size());
     case 37: return host(m.layer.data(), m.layer.size());
+    case 38: return host(m.b_v0.data(), m.b_v0.size());
+    case 39: return host(m.b_v1.data(), m.b_v1.size());
+    case 40: return host(m.b_pad0.data(), m.b_pad0.size());
+    case 41: return host(m.b_pad1.data(), m.b_pad1.size());
+    case 42: return host(m.s_v0.data(), m.s_v0.size());
+    case 43: return host(m.s_v1.data(), m.s_v1.size());
+    case 44: return host(m.s_rest.data(), m.s_rest.size());
+    case 45: return host(m.s_col.offset.data(), m.s_col.offset.size());
+    case 46: return host(m.s_col.count.data(), m.s_col.count.size());
     default: return fail(TC_ERR_VALUE, "unknown array id " + std::to_string(which));
     }
 }

@@ -159,6 +159,14 @@ struct tc_sim {
     // constraints
     DevBuf<uint4> drec, brec;
     DevBuf<float> dcc, dlam, blam, dlam2, blam2;
+    DevBuf<uint4> bgate;               // seam-gated bending: {v0, v1, gate distance bits, 1} per constraint (meshes with seams)
+    // sewing (mesh.rs:51-60, sewing.wgsl, sewing_shrink.wgsl): {v0, v1, current rest bits, target rest bits}
+    DevBuf<uint4> srec;
+    DevBuf<float> slam;
+    float sew_stiffness = 10000.0f;    // simulator fields (mod.rs:637-640); the constructor's stiffness and
+    bool sew_lock = true;              // lock flag are kept only in the records, which the solver does not read
+    float sew_lock_dist = 0.02f;
+    float sew_speed = 1.0f;
     bool split_compliance = false;
     std::vector<uint4> drec_h, brec_h;
     std::vector<float> dcc_h;
@@ -366,6 +374,7 @@ struct tc_sim {
     }
     bool fused_active() const {
         if (!fused_ready) return false;
+        if (!mesh.s_v0.empty() || mesh.has_seams) return false;  // sewing runs between the distance and bending colours
         if (solver_mode == 1 || solver_path == 1) return false;
         if (solver_path == 2) return true;
         return mesh.nv >= 16384;  // small meshes: the per-colour launches are cheap
@@ -399,6 +408,10 @@ struct tc_sim {
 };
 
 // Wrinkle field after predict (tc_wrinkle.hip); nothing when off or without a texture.
+// sewing (tc_sewing.hip): the per-substep shrink (also clears the sewing lambdas) and one pass over the colours
+hipError_t enqueue_sew_shrink(tc_sim *s, float dt_sub);
+hipError_t enqueue_sewing(tc_sim *s, float4 *P, float dt_sub);
+int upload_sewing(tc_sim *s);
 hipError_t enqueue_wrinkle(tc_sim *s, float dt_sub);
 
 // Collider-edge collision (tc_edge_collision.hip): the edges' rebuild after the triangles change,

@@ -86,10 +86,14 @@ __global__ void __launch_bounds__(256) k_distance(float4 *__restrict__ P, const 
     P[i1] = make_float4(q1.x + c1.x, q1.y + c1.y, q1.z + c1.z, w1);
 }
 
+// gate (meshes with sewing seams, else null): {v0, v1, gate distance bits, seam flag} per constraint;
+// a seam constraint (bending.wgsl:92-104) pulls only while its opposite vertices are closer than the
+// unfolded rest length and its first sewn pair is within the gate distance.
 template <bool FIRST, bool DAMP>
 __global__ void __launch_bounds__(256) k_bending(float4 *__restrict__ P, const float4 *__restrict__ X,
                                                  const uint4 *__restrict__ rec, float *__restrict__ lam,
-                                                 uint32_t off, uint32_t cnt, SubstepArgs a) {
+                                                 uint32_t off, uint32_t cnt, SubstepArgs a,
+                                                 const uint4 *__restrict__ gate) {
     if (a.run_flag && *a.run_flag == 0u) return;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cnt) return;
@@ -104,6 +108,14 @@ __global__ void __launch_bounds__(256) k_bending(float4 *__restrict__ P, const f
     const f3 delta = xyz(q2) - xyz(q3);
     const float dist = length(delta);
     if (dist <= 1e-7f) { if (FIRST) lam[k] = 0.0f; return; }
+    if (gate) {
+        const uint4 g = gate[k];
+        if (g.w != 0u) {
+            if (dist >= __uint_as_float(r.z)) { if (FIRST) lam[k] = 0.0f; return; }
+            const float gap = length(xyz(P[g.x]) - xyz(P[g.y]));
+            if (gap > __uint_as_float(g.z)) { if (FIRST) lam[k] = 0.0f; return; }
+        }
+    }
     const f3 dir = delta / dist;
     const float dt = fmaxf(a.dt, 1e-8f);
     const float alpha = comp / (dt * dt);
@@ -692,16 +704,16 @@ hipError_t launch_distance(hipStream_t s, float4 *P, const float4 *X, const uint
 }
 
 hipError_t launch_bending(hipStream_t s, float4 *P, const float4 *X, const uint4 *rec, float *lam, uint32_t off,
-                          uint32_t cnt, const SubstepArgs &a, bool first, uint32_t block) {
+                          uint32_t cnt, const SubstepArgs &a, bool first, uint32_t block, const uint4 *gate) {
     if (!cnt) return hipSuccess;
     const bool damp = a.bending_damping != 0.0f;
     const dim3 g(grid_for(cnt, block)), b(block);
     if (first) {
-        if (damp) hipLaunchKernelGGL((k_bending<true, true>), g, b, 0, s, P, X, rec, lam, off, cnt, a);
-        else hipLaunchKernelGGL((k_bending<true, false>), g, b, 0, s, P, X, rec, lam, off, cnt, a);
+        if (damp) hipLaunchKernelGGL((k_bending<true, true>), g, b, 0, s, P, X, rec, lam, off, cnt, a, gate);
+        else hipLaunchKernelGGL((k_bending<true, false>), g, b, 0, s, P, X, rec, lam, off, cnt, a, gate);
     } else {
-        if (damp) hipLaunchKernelGGL((k_bending<false, true>), g, b, 0, s, P, X, rec, lam, off, cnt, a);
-        else hipLaunchKernelGGL((k_bending<false, false>), g, b, 0, s, P, X, rec, lam, off, cnt, a);
+        if (damp) hipLaunchKernelGGL((k_bending<false, true>), g, b, 0, s, P, X, rec, lam, off, cnt, a, gate);
+        else hipLaunchKernelGGL((k_bending<false, false>), g, b, 0, s, P, X, rec, lam, off, cnt, a, gate);
     }
     return hipGetLastError();
 }

@@ -8,7 +8,7 @@ frame statistics (largest speed and displacement, NaN/Inf, bounding box, largest
 config hash), the dynamic pins (PinRecord: vertex_idx, target_pos, weight), on the frames where the
 SimConfig changed the whole SimConfig as param_deltas, and on the frames where base inverse masses
 changed their new values as inv_mass_deltas (debug_recorder.rs:478-505: a value counts as changed
-when its bits differ, +0 and -0 being the same).  What this build does not simulate (sewing, bone
+when its bits differ, +0 and -0 being the same).  What this build does not simulate (bone
 SDFs, pair counters) is written empty.
 
 `load(path)` reads such a file; `replay(path)` builds a simulator from the metadata and steps it
@@ -67,7 +67,8 @@ class DebugRecorder:
             "initial_positions": pos.tolist(),
             "thicknesses": None if c["thicknesses"] is None else np.asarray(c["thicknesses"]).tolist(),
             "layer_ids": None if c["layer_ids"] is None else np.asarray(c["layer_ids"]).tolist(),
-            "sewing_springs": None, "stiffness": c["stiffness"], "compression_stiffness": c["compression_stiffness"],
+            "sewing_springs": None if c.get("sewing_springs") is None else np.asarray(c["sewing_springs"]).tolist(),
+            "stiffness": c["stiffness"], "compression_stiffness": c["compression_stiffness"],
             "shear_stiffness": c["shear_stiffness"], "bending_stiffness": c["bending_stiffness"],
             "thickness": c["thickness"], "layer_id": c["layer_id"],
             "gravity": list(sim._cfg["gravity"]), "damping": sim._cfg["damping"], "solver_mode": sim.get_solver_mode(),
@@ -97,7 +98,9 @@ class DebugRecorder:
                "colliders": [dict(c) for c in sim._colliders],
                "stats": _stats(pos, vel, self._prev_pos, self._edges, self._rest, h),
                "param_deltas": json.loads(sim.get_config_json()) if h != self._prev_hash else None,
-               "elastic_scales": None, "bone_transforms": None, "sewing_rest_lengths": None, "sewing_priority": None,
+               "elastic_scales": None, "bone_transforms": None,
+               "sewing_rest_lengths": sim.get_sewing_current_rest_lengths() if sim.num_sewing_constraints else None,
+               "sewing_priority": None,
                "inv_mass_deltas": deltas or None}
         self._prev_pos = pos.copy()
         self._prev_hash = h
@@ -139,6 +142,8 @@ def replay(path: str):
     sim = ClothSimulator(np.asarray(meta["initial_positions"], np.float32), np.asarray(meta["edges"], np.uint32),
                          np.asarray(meta["faces"], np.uint32).reshape(-1, 3) if meta["faces"] else None,
                          np.asarray(meta["inv_masses"], np.float32),
+                         sewing_springs=None if meta.get("sewing_springs") is None else
+                         np.asarray(meta["sewing_springs"], np.uint32).reshape(-1, 2),
                          thicknesses=None if meta["thicknesses"] is None else np.asarray(meta["thicknesses"], np.float32),
                          layer_ids=None if meta["layer_ids"] is None else np.asarray(meta["layer_ids"], np.uint32),
                          layer_id=meta.get("layer_id", 0), thickness=meta["thickness"], stiffness=meta["stiffness"],

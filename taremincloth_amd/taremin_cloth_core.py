@@ -72,8 +72,17 @@ class ClothSimulator:
                 raise ValueError(f"faces contain an index >= number of vertices ({n})")
             if f.size == 0:
                 f = None
-        if sewing_springs is not None and np.asarray(sewing_springs).size:
-            raise NotImplementedError("sewing springs are not part of this build")
+        sew = None
+        if sewing_springs is not None:  # lib.rs:1643-1665
+            sa = np.asarray(sewing_springs)
+            if sa.size:
+                if sa.ndim != 2 or sa.shape[1] != 2:
+                    raise ValueError("each pair of sewing_springs must hold two vertices (v0, v1)")
+                sew = np.ascontiguousarray(sa.astype(np.uint32))
+                bad = (sew >= n).any(axis=1)
+                if bad.any():
+                    b = sew[bad][0]
+                    raise ValueError(f"sewing_springs contain an index >= number of vertices ({n}): [{b[0]}, {b[1]}]")
         if enable_coarse_constraints:
             raise NotImplementedError("coarse (two-hop) constraints are not part of this build")
         if self_collision_algorithm == 2:
@@ -105,9 +114,16 @@ class ClothSimulator:
         # uses its own tuned block size, so the value is accepted and not forwarded.
         p.workgroup_size = 0
         p.solver_mode = solver_mode
+        sp = _lib.TcSewingParams()
+        L.tc_sewing_params_default(C.byref(sp))
+        sp.shrink_speed = float(sewing_shrink_speed)
+        sp.stiffness = -1.0 if sewing_stiffness is None else float(sewing_stiffness)
+        sp.lock = -1 if enable_sewing_lock is None else (1 if enable_sewing_lock else 0)
+        sp.lock_distance = -1.0 if sewing_lock_distance is None else float(sewing_lock_distance)
         h = C.c_void_p()
-        _lib.check(L.tc_sim_create(_ptr(pos), n, _ptr(e), len(e), _ptr(f), 0 if f is None else len(f), _ptr(im),
-                                   _ptr(th), _ptr(ly), C.byref(p), C.byref(h)))
+        _lib.check(L.tc_sim_create_sewn(_ptr(pos), n, _ptr(e), len(e), _ptr(f), 0 if f is None else len(f), _ptr(im),
+                                        _ptr(th), _ptr(ly), _ptr(sew), 0 if sew is None else len(sew), C.byref(sp),
+                                        C.byref(p), C.byref(h)))
         self._h = h
         self._n = n
         self._keep = (pos, e, f)
@@ -121,7 +137,13 @@ class ClothSimulator:
             self.set_enable_pair_cache(enable_pair_cache)
         if self_collision_algorithm is not None:
             self.set_self_collision_algorithm(self_collision_algorithm)
+        if sewing_lock_distance is not None:
+            self._cfg["sewing_lock_distance"] = float(max(sewing_lock_distance, 0.0))
+        self._cfg["sewing_shrink_speed"] = float(max(sewing_shrink_speed, 0.0))
         self._ctor = dict(positions=pos, edges=e, faces=f, inv_masses=im, thicknesses=th, layer_ids=ly,
+                          sewing_springs=sew, sewing_shrink_speed=float(sewing_shrink_speed),
+                          sewing_stiffness=sewing_stiffness, enable_sewing_lock=enable_sewing_lock,
+                          sewing_lock_distance=sewing_lock_distance,
                           layer_id=int(layer_id), thickness=float(thickness), stiffness=float(stiffness),
                           bending_stiffness=float(bending_stiffness), compression_stiffness=compression_stiffness,
                           shear_stiffness=shear_stiffness)
@@ -280,9 +302,12 @@ class ClothSimulator:
         self.set_auto_compensate_iterations(c["auto_compensate_iterations"])
         self.set_strain_adaptive_options(c["enable_strain_adaptive"], c["strain_tolerance"])
         self.areal_density = float(min(max(c["areal_density"], 0.0), 100.0))
+        self.set_sewing_stiffness(c["sewing_stiffness"])
+        self.set_sewing_shrink_speed(c["sewing_shrink_speed"])
+        self.set_enable_sewing_lock(c["enable_sewing_lock"])
+        self.set_sewing_lock_distance(c["sewing_lock_distance"])
         # recorded as given: options of solvers this build does not have, kept at their inert values
-        for k in ("post_relaxation_iters", "auto_coupled_on_low_substeps", "sewing_stiffness", "sewing_shrink_speed",
-                  "enable_sewing_lock", "sewing_lock_distance", "sewing_priority_threshold",
+        for k in ("post_relaxation_iters", "auto_coupled_on_low_substeps", "sewing_priority_threshold",
                   "sewing_priority_merge_dist", "sewing_priority_ramp_frames", "sewing_priority_max_frames",
                   "areal_density", "version"):
             self._cfg[k] = c[k]
@@ -704,8 +729,46 @@ class ClothSimulator:
 
     @property
     def num_sewing_constraints(self) -> int:
-        """Sewing springs are rejected at construction, so there are none."""
-        return 0
+        return int(self._sewing_state()[4])
+
+    # --- sewing (lib.rs:2800-2947, sewing.wgsl, sewing_shrink.wgsl) ------------------------------
+    def _sewing_state(self):
+        out = np.zeros(5, np.float32)
+        _lib.check(self._L.tc_sim_get_sewing_state(self._h, _ptr(out)))
+        return out
+
+    def set_sewing_stiffness(self, stiffness):
+        _lib.check(self._L.tc_sim_set_sewing_stiffness(self._h, float(stiffness)))
+        self._cfg["sewing_stiffness"] = float(self._sewing_state()[0])
+
+    def set_sewing_shrink_speed(self, speed):
+        _lib.check(self._L.tc_sim_set_sewing_shrink_speed(self._h, float(speed)))
+        self._cfg["sewing_shrink_speed"] = float(self._sewing_state()[1])
+
+    def get_sewing_shrink_speed(self) -> float:
+        return float(self._sewing_state()[1])
+
+    def set_enable_sewing_lock(self, enabled):
+        _lib.check(self._L.tc_sim_set_enable_sewing_lock(self._h, 1 if enabled else 0))
+        self._cfg["enable_sewing_lock"] = bool(enabled)
+
+    def set_sewing_lock_distance(self, distance):
+        _lib.check(self._L.tc_sim_set_sewing_lock_distance(self._h, float(distance)))
+        self._cfg["sewing_lock_distance"] = float(self._sewing_state()[3])
+
+    def get_sewing_lock_distance(self) -> float:
+        return float(self._sewing_state()[3])
+
+    def get_sewing_current_rest_lengths(self) -> list:
+        """Current (shrinking) rest lengths, in the sewing constraints' colour order."""
+        n = self.num_sewing_constraints
+        out = np.empty(n, np.float32)
+        _lib.check(self._L.tc_sim_get_sewing_rest_lengths(self._h, _ptr(out), n))
+        return [float(x) for x in out]
+
+    def set_sewing_current_rest_lengths(self, values):
+        v = np.ascontiguousarray(np.asarray(values, np.float32).reshape(-1))
+        _lib.check(self._L.tc_sim_set_sewing_rest_lengths(self._h, _ptr(v), len(v)))
 
     # --- diagnostics: GPU pass timing (lib.rs:1975-1990, simulation/profile.rs) -----------------
     def set_profiling_enabled(self, enable) -> bool:
@@ -1132,19 +1195,21 @@ def close_open_boundary_loops(mesh_verts, mesh_tris):
     return ov, ot, int(loops.value)
 
 
-_MESH_FLOAT_ARRAYS = {8, 9, 24, 25, 32, 33, 34}
+_MESH_FLOAT_ARRAYS = {8, 9, 24, 25, 32, 33, 34, 40, 41, 44}
 MESH_ARRAYS = {
     "dist_v0": 0, "dist_v1": 1, "dist_col_off": 2, "dist_col_cnt": 3, "bend_v2": 4, "bend_v3": 5,
     "bend_col_off": 6, "bend_col_cnt": 7, "dist_rest": 8, "bend_rest": 9, "scalars": 17, "edges": 18,
     "th_off": 22, "th_idx": 23, "th_rest": 24, "local_len": 25, "star_off": 26, "star": 27, "edge_bbits": 28,
     "island": 29, "edge_to_constraint": 30, "dist_type": 31, "dist_tc": 32, "dist_cc": 33, "bend_comp": 34,
-    "adj_off": 35, "adj_idx": 36, "layer": 37,
+    "adj_off": 35, "adj_idx": 36, "layer": 37, "bend_v0": 38, "bend_v1": 39, "bend_pad0": 40, "bend_pad1": 41,
+    "sew_v0": 42, "sew_v1": 43, "sew_rest": 44, "sew_col_off": 45, "sew_col_cnt": 46,
 }
 
 
 def build_mesh_arrays(positions, edges, faces=None, inv_masses=None, thicknesses=None, thickness=0.005,
                       stiffness=1000.0, bending_stiffness=10.0, compression_stiffness=None,
-                      shear_stiffness=None, layer_id=0) -> dict:
+                      shear_stiffness=None, layer_id=0, sewing_springs=None, sewing_shrink_speed=1.0,
+                      sewing_stiffness=None, enable_sewing_lock=None) -> dict:
     """Host-side mesh build of the library (no GPU needed): constraints, colours, topology."""
     L = _lib.load()
     pos = _f32(positions, (3,), "positions")
@@ -1157,15 +1222,22 @@ def build_mesh_arrays(positions, edges, faces=None, inv_masses=None, thicknesses
     p.thickness, p.layer_id, p.stiffness, p.bending_stiffness = thickness, layer_id, stiffness, bending_stiffness
     p.compression_stiffness = -1.0 if compression_stiffness is None else compression_stiffness
     p.shear_stiffness = -1.0 if shear_stiffness is None else shear_stiffness
+    sew = None if sewing_springs is None or np.asarray(sewing_springs).size == 0 else _u32(sewing_springs, 2,
+                                                                                          "sewing_springs")
+    sp = _lib.TcSewingParams()
+    L.tc_sewing_params_default(C.byref(sp))
+    sp.shrink_speed = float(sewing_shrink_speed)
+    sp.stiffness = -1.0 if sewing_stiffness is None else float(sewing_stiffness)
+    sp.lock = -1 if enable_sewing_lock is None else (1 if enable_sewing_lock else 0)
     out = {}
     for name, which in MESH_ARRAYS.items():
         cnt = C.c_size_t(0)
         args = (_ptr(pos), len(pos), _ptr(e), len(e), _ptr(f), 0 if f is None else len(f), _ptr(im), _ptr(th),
-                C.byref(p), which)
-        _lib.check(L.tc_mesh_build_arrays(*args, None, 0, C.byref(cnt)))
+                _ptr(sew), 0 if sew is None else len(sew), C.byref(sp), C.byref(p), which)
+        _lib.check(L.tc_mesh_build_arrays_sewn(*args, None, 0, C.byref(cnt)))
         arr = np.empty(cnt.value, np.float32 if which in _MESH_FLOAT_ARRAYS else np.uint32)
         if cnt.value:
-            _lib.check(L.tc_mesh_build_arrays(*args, arr.ctypes.data_as(C.c_void_p), arr.size, C.byref(cnt)))
+            _lib.check(L.tc_mesh_build_arrays_sewn(*args, arr.ctypes.data_as(C.c_void_p), arr.size, C.byref(cnt)))
         out[name] = arr
     sc = out["scalars"]
     out["cell_size"] = float(sc[:1].view(np.float32)[0])

@@ -70,7 +70,7 @@ def test_host_mesh_build_rejects_bad_faces_through_abi():
 def test_unsupported_options_raise():
     pos, e, f = cases.grid(3, 3, 0.1)
     with pytest.raises(NotImplementedError):
-        T.ClothSimulator(pos, e, f, sewing_springs=np.array([[0, 8]], np.uint32))
+        T.ClothSimulator(pos, e, f, enable_coarse_constraints=True)
 
 
 def test_abi_struct_layout_matches_ctypes():
@@ -79,7 +79,8 @@ def test_abi_struct_layout_matches_ctypes():
     text = open(os.path.join(cases.ROOT, "taremincloth_amd", "csrc", "tc_layout.h")).read()
     sizes = dict(re.findall(r"static_assert\(sizeof\((\w+)\) == (\d+)", text))
     offs = re.findall(r"static_assert\(offsetof\((\w+), (\w+)\) == (\d+)", text)
-    classes = {"tc_sim_params": _lib.TcParams, "tc_instance": _lib.TcInstance}
+    classes = {"tc_sim_params": _lib.TcParams, "tc_instance": _lib.TcInstance,
+               "tc_sewing_params": _lib.TcSewingParams}
     assert set(sizes) == set(classes)
     for name, cls in classes.items():
         assert C.sizeof(cls) == int(sizes[name]), name
@@ -159,7 +160,8 @@ def test_rust_extern_block_matches_the_header():
 
 def test_rust_structs_and_constants_match_the_header():
     rs = open(os.path.join(cases.ROOT, "include", "tc_ffi.rs")).read()
-    for rname, cls in (("TcParams", _lib.TcParams), ("TcInstance", _lib.TcInstance)):
+    for rname, cls in (("TcParams", _lib.TcParams), ("TcInstance", _lib.TcInstance),
+                       ("TcSewingParams", _lib.TcSewingParams)):
         body = re.search(r"pub struct %s \{(.*?)\}" % rname, rs, flags=re.S).group(1)
         assert re.findall(r"pub (\w+):", body) == [f for f, *_ in cls._fields_], rname
     hdr = open(HEADER).read()
