@@ -74,6 +74,11 @@ extern "C" {
     pub fn tc_sim_get_sewing_state(sim: *const TcSim, out: *mut f32) -> i32;
     pub fn tc_sim_get_sewing_rest_lengths(sim: *mut TcSim, out: *mut f32, n: u32) -> i32;
     pub fn tc_sim_set_sewing_rest_lengths(sim: *mut TcSim, values: *const f32, n: u32) -> i32;
+    pub fn tc_sewing_closure_ratio(positions: *const f32, num_vertices: u32, pairs: *const u32, num_pairs: u32, merge_dist: f32, ratio: *mut f32) -> i32;
+    pub fn tc_sim_set_sewing_priority_options(sim: *mut TcSim, enabled: i32, threshold: f32, merge_dist: f32, ramp_frames: u32, max_frames: u32) -> i32;
+    pub fn tc_sim_set_sewing_priority_state(sim: *mut TcSim, latched: i32, frame: u32, ramp_t: f32, ratio: f32) -> i32;
+    pub fn tc_sim_get_sewing_priority_state(sim: *const TcSim, out: *mut f64) -> i32;
+    pub fn tc_sim_update_sewing_priority(sim: *mut TcSim, positions: *const f32, num_vertices: u32, out: *mut f32) -> i32;
     pub fn tc_sim_create_batch(instances: *const TcInstance, num_instances: u32, params: *const TcParams, out: *mut *mut TcSim) -> i32;
     pub fn tc_sim_instance_count(sim: *const TcSim, num_instances: *mut u32) -> i32;
     pub fn tc_sim_instance_range(sim: *const TcSim, instance: u32, first_vertex: *mut u32, num_vertices: *mut u32) -> i32;

@@ -100,6 +100,29 @@ int tc_sim_get_sewing_state(const tc_sim *sim, float *out);
 int tc_sim_get_sewing_rest_lengths(tc_sim *sim, float *out, uint32_t n);
 int tc_sim_set_sewing_rest_lengths(tc_sim *sim, const float *values, uint32_t n);
 
+/* Sewing priority (mod.rs:1262-1420, lib.rs:2842-2923): a one-way latch advanced on the host that
+ * scales gravity and the wrinkle field's blend (dispatch.rs:13-21, 53-60, 151-165) by 0 until a share
+ * `threshold` of the sewing pairs is closer than `merge_dist` (or `max_frames` updates have passed,
+ * 0 = never), then by smoothstep over `ramp_frames` updates.  Off, or without sewing pairs, the scale
+ * is 1.
+ *   tc_sewing_closure_ratio: the share of closed pairs among those whose indices fall inside the
+ *     positions (num_vertices x 3 floats); 1 when none count (mod.rs:1381-1420).  Needs no GPU.
+ *   set_options: clamps threshold to [0, 1], merge_dist to [0.0005, 0.1], ramp_frames to 120 and
+ *     max_frames to 100000; a change resets the latch, an unchanged setting keeps it.
+ *   set_state: restores (latched, frame, ramp_t, ratio), ramp_t and ratio clamped to [0, 1].
+ *   get_state out[12]: enabled, threshold, merge_dist, ramp_frames, max_frames, latched, frame,
+ *     ramp_t, ratio (cached), scale, closure ratio (1 when off), active (latch not yet reached).
+ *   update: measures the closure on the given positions (no GPU read-back), advances the latch and
+ *     ramp; out[3] = ratio, scale, latched (1, 1, 1 when off or without sewing pairs).
+ *   tc_sim_reset clears the latch (dispatch.rs:1414-1417). */
+int tc_sewing_closure_ratio(const float *positions, uint32_t num_vertices, const uint32_t *pairs, uint32_t num_pairs,
+                            float merge_dist, float *ratio);
+int tc_sim_set_sewing_priority_options(tc_sim *sim, int enabled, float threshold, float merge_dist,
+                                       uint32_t ramp_frames, uint32_t max_frames);
+int tc_sim_set_sewing_priority_state(tc_sim *sim, int latched, uint32_t frame, float ramp_t, float ratio);
+int tc_sim_get_sewing_priority_state(const tc_sim *sim, double *out);
+int tc_sim_update_sewing_priority(tc_sim *sim, const float *positions, uint32_t num_vertices, float *out);
+
 /* One cloth instance of a batch: the arguments of tc_sim_create for that cloth (vertex indices are
  * local to the instance; inv_masses / thicknesses / faces may be NULL). */
 typedef struct {

@@ -48,6 +48,8 @@ EXPORTS = [
     "tc_sewing_params_default", "tc_sim_create_sewn", "tc_sim_set_sewing_stiffness", "tc_sim_set_sewing_shrink_speed",
     "tc_sim_set_enable_sewing_lock", "tc_sim_set_sewing_lock_distance", "tc_sim_get_sewing_state",
     "tc_sim_get_sewing_rest_lengths", "tc_sim_set_sewing_rest_lengths", "tc_mesh_build_arrays_sewn",
+    "tc_sewing_closure_ratio", "tc_sim_set_sewing_priority_options", "tc_sim_set_sewing_priority_state",
+    "tc_sim_get_sewing_priority_state", "tc_sim_update_sewing_priority",
 ]
 
 TC_OK, TC_ERR_VALUE, TC_ERR_RUNTIME, TC_ERR_UNSUPPORTED, TC_ERR_IO = 0, 1, 2, 3, 4
@@ -228,6 +230,11 @@ def load() -> C.CDLL:
         "tc_sim_get_sewing_state": (_i, [_vp, _vp]),
         "tc_sim_get_sewing_rest_lengths": (_i, [_vp, _vp, _u32]),
         "tc_sim_set_sewing_rest_lengths": (_i, [_vp, _vp, _u32]),
+        "tc_sewing_closure_ratio": (_i, [_vp, _u32, _vp, _u32, C.c_float, _vp]),
+        "tc_sim_set_sewing_priority_options": (_i, [_vp, _i, C.c_float, C.c_float, _u32, _u32]),
+        "tc_sim_set_sewing_priority_state": (_i, [_vp, _i, _u32, C.c_float, C.c_float]),
+        "tc_sim_get_sewing_priority_state": (_i, [_vp, _vp]),
+        "tc_sim_update_sewing_priority": (_i, [_vp, _vp, _u32, _vp]),
         "tc_mesh_build_arrays_sewn": (_i, [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _u32, _vp, _vp, _i, _vp,
                                           C.c_size_t, C.POINTER(C.c_size_t)]),
         "tc_close_open_boundary_loops": (_i, [_vp, _u32, _vp, _u32, _vp, C.c_size_t, _vp, C.c_size_t,

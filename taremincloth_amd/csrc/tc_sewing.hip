@@ -205,4 +205,106 @@ int tc_sim_set_sewing_rest_lengths(tc_sim *s, const float *values, uint32_t n) {
     return TC_OK;
 }
 
+// mod.rs:1381-1420: the share of sewing pairs closer than merge_dist; pairs with an index outside
+// the positions are skipped, and with none counted the share is 1
+int tc_sewing_closure_ratio(const float *positions, uint32_t num_vertices, const uint32_t *pairs, uint32_t num_pairs,
+                            float merge_dist, float *ratio) {
+    if (!ratio || (num_vertices && !positions) || (num_pairs && !pairs)) return fail(TC_ERR_VALUE, "null argument");
+    const float m2 = merge_dist * merge_dist;
+    uint32_t closed = 0, total = 0;
+    for (uint32_t k = 0; k < num_pairs; ++k) {
+        const uint32_t a = pairs[2 * k], b = pairs[2 * k + 1];
+        if (a >= num_vertices || b >= num_vertices) continue;
+        ++total;
+        const float dx = positions[3 * a] - positions[3 * b];
+        const float dy = positions[3 * a + 1] - positions[3 * b + 1];
+        const float dz = positions[3 * a + 2] - positions[3 * b + 2];
+        if (dx * dx + dy * dy + dz * dz <= m2) ++closed;
+    }
+    *ratio = total ? float(closed) / float(total) : 1.0f;
+    return TC_OK;
+}
+
+int tc_sim_set_sewing_priority_options(tc_sim *s, int enabled, float threshold, float merge_dist,
+                                       uint32_t ramp_frames, uint32_t max_frames) {  // mod.rs:1262-1292
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    const bool on = enabled != 0;
+    const float th = std::min(std::max(threshold, 0.0f), 1.0f);
+    const float md = std::min(std::max(merge_dist, 0.0005f), 0.1f);
+    const uint32_t rf = std::min(ramp_frames, 120u), mf = std::min(max_frames, 100000u);
+    if (s->prio_on == on && s->prio_threshold == th && s->prio_merge == md && s->prio_ramp == rf && s->prio_max == mf)
+        return TC_OK;  // an unchanged setting keeps the latch
+    s->prio_on = on;
+    s->prio_threshold = th;
+    s->prio_merge = md;
+    s->prio_ramp = rf;
+    s->prio_max = mf;
+    s->prio_latched = false;
+    s->prio_frame = 0;
+    s->prio_ramp_t = 0.0f;
+    s->prio_ratio = 0.0f;
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_set_sewing_priority_state(tc_sim *s, int latched, uint32_t frame, float ramp_t,
+                                     float ratio) {  // mod.rs:1307-1318
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    s->prio_latched = latched != 0;
+    s->prio_frame = frame;
+    s->prio_ramp_t = std::min(std::max(ramp_t, 0.0f), 1.0f);
+    s->prio_ratio = std::min(std::max(ratio, 0.0f), 1.0f);
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_get_sewing_priority_state(const tc_sim *s, double *out) {
+    if (!s || !out) return fail(TC_ERR_VALUE, "null argument");
+    const bool live = s->prio_on && !s->mesh.s_v0.empty();
+    out[0] = s->prio_on ? 1.0 : 0.0;
+    out[1] = s->prio_threshold;
+    out[2] = s->prio_merge;
+    out[3] = s->prio_ramp;
+    out[4] = s->prio_max;
+    out[5] = s->prio_latched ? 1.0 : 0.0;
+    out[6] = s->prio_frame;
+    out[7] = s->prio_ramp_t;
+    out[8] = s->prio_ratio;
+    out[9] = s->priority_scale();
+    out[10] = live ? s->prio_ratio : 1.0;                  // mod.rs:1379-1384
+    out[11] = live && !s->prio_latched ? 1.0 : 0.0;       // mod.rs:1387-1391
+    return TC_OK;
+}
+
+int tc_sim_update_sewing_priority(tc_sim *s, const float *positions, uint32_t num_vertices,
+                                  float *out) {  // mod.rs:1323-1361
+    if (!s || !out) return fail(TC_ERR_VALUE, "null argument");
+    if (!s->prio_on || s->mesh.s_v0.empty()) {
+        out[0] = 1.0f; out[1] = 1.0f; out[2] = 1.0f;
+        return TC_OK;
+    }
+    const tc::Mesh &m = s->mesh;
+    std::vector<uint32_t> pairs(2 * m.s_v0.size());
+    for (size_t k = 0; k < m.s_v0.size(); ++k) { pairs[2 * k] = m.s_v0[k]; pairs[2 * k + 1] = m.s_v1[k]; }
+    float ratio = 1.0f;
+    if (int rc = tc_sewing_closure_ratio(positions, num_vertices, pairs.data(), uint32_t(m.s_v0.size()),
+                                         s->prio_merge, &ratio))
+        return rc;
+    const float before = s->priority_scale();
+    s->prio_ratio = ratio;
+    if (s->prio_frame != UINT32_MAX) ++s->prio_frame;
+    if (!s->prio_latched && (ratio >= s->prio_threshold || (s->prio_max > 0 && s->prio_frame >= s->prio_max)))
+        s->prio_latched = true;
+    if (s->prio_latched && s->prio_ramp > 0)
+        s->prio_ramp_t = std::min(s->prio_ramp_t + 1.0f / float(s->prio_ramp), 1.0f);
+    else
+        s->prio_ramp_t = s->prio_latched ? 1.0f : 0.0f;
+    const float after = s->priority_scale();
+    if (after != before) s->touch();
+    out[0] = ratio;
+    out[1] = after;
+    out[2] = s->prio_latched ? 1.0f : 0.0f;
+    return TC_OK;
+}
+
 }  // extern "C"

@@ -155,7 +155,7 @@ hipError_t enqueue_substep_head(tc_sim *s, float dt_sub, bool gate_to_host, uint
     e = tc::launch_predict(s->stream, s->P.p, s->X.p, s->V.p, n, a, B);
     s->prof_end(q);
     if (e != hipSuccess) return e;
-    if (s->wrinkle_on && s->wrinkle_tex.p) {  // dispatch.rs:151-165
+    if (s->wrinkle_on && s->wrinkle_tex.p && s->priority_scale() > 0.0f) {  // dispatch.rs:151-165
         q = s->prof_begin("wrinkle_field");
         e = enqueue_wrinkle(s, dt_sub);
         s->prof_end(q);
@@ -1166,6 +1166,11 @@ int tc_sim_reset(tc_sim *s) {
     // masses of set_vertex_inv_masses are kept for later calls)
     s->pins.clear();
     if (int rc = upload_pins(s)) return rc;
+    s->prio_latched = false;
+    s->prio_frame = 0;
+    s->prio_ramp_t = 0.0f;
+    s->prio_ratio = 0.0f;
+    s->touch();
     for (uint32_t i = 0; i < n; ++i) s->inv_eff[i] = s->P0h[i].w;
     s->last_p = 0;
     return ws_clear_history(s);

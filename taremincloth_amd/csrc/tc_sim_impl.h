@@ -167,6 +167,22 @@ struct tc_sim {
     bool sew_lock = true;              // lock flag are kept only in the records, which the solver does not read
     float sew_lock_dist = 0.02f;
     float sew_speed = 1.0f;
+    // sewing priority (mod.rs:1262-1397): a one-way latch, advanced on the host from positions the
+    // caller holds, that keeps gravity and the wrinkle field's blend at zero until enough sewing pairs
+    // have closed (or max_frames have passed), then ramps them back over ramp_frames by smoothstep
+    bool prio_on = false;
+    float prio_threshold = 0.9f, prio_merge = 0.005f;
+    uint32_t prio_ramp = 3, prio_max = 600;
+    bool prio_latched = false;
+    uint32_t prio_frame = 0;
+    float prio_ramp_t = 0.0f, prio_ratio = 0.0f;
+    float priority_scale() const {  // mod.rs:1364-1376
+        if (!prio_on || mesh.s_v0.empty()) return 1.0f;
+        if (!prio_latched) return 0.0f;
+        if (prio_ramp == 0) return 1.0f;
+        const float x = std::min(std::max(prio_ramp_t, 0.0f), 1.0f);
+        return x * x * (3.0f - 2.0f * x);
+    }
     bool split_compliance = false;
     std::vector<uint4> drec_h, brec_h;
     std::vector<float> dcc_h;
@@ -351,7 +367,8 @@ struct tc_sim {
     tc::SubstepArgs args(float dt_sub) const {
         tc::SubstepArgs a;
         a.dt = dt_sub;
-        a.gx = gravity[0]; a.gy = gravity[1]; a.gz = gravity[2];
+        const float k = priority_scale();  // dispatch.rs:13-21
+        a.gx = gravity[0] * k; a.gy = gravity[1] * k; a.gz = gravity[2] * k;
         a.damping = damping;
         a.tension_damping = tension_damping; a.compression_damping = compression_damping;
         a.shear_damping = shear_damping; a.bending_damping = bending_damping;

@@ -95,8 +95,10 @@ __global__ void __launch_bounds__(256) k_wrinkle(float4 *__restrict__ P, float4 
 hipError_t enqueue_wrinkle(tc_sim *s, float dt_sub) {
     if (!s->wrinkle_on || !s->wrinkle_tex.p || !s->mesh.nv) return hipSuccess;
     const uint32_t n = s->mesh.nv;
+    tc::WrinkleArgs w = s->wrinkle;
+    w.blend *= s->priority_scale();  // dispatch.rs:53-60: the sewing priority scales the blend
     hipLaunchKernelGGL(tc::k_wrinkle, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->P.p, s->V.p, n,
-                       s->wrinkle_tex.p, s->wrinkle_w, s->wrinkle_h, s->wrinkle, dt_sub);
+                       s->wrinkle_tex.p, s->wrinkle_w, s->wrinkle_h, w, dt_sub);
     return hipGetLastError();
 }
 

@@ -42,6 +42,14 @@ def _stats(pos, vel, prev, edges, rest, cfg_hash):
             "vt_count": 0, "ee_count": 0, "vt_saturated": False, "max_strain": strain, "config_hash": cfg_hash}
 
 
+def _priority_record(sim):
+    """[latched01, frame, ramp_t, ratio] of the sewing priority latch, while it is on (recording.rs:226-232)."""
+    if not sim._cfg.get("sewing_priority_enabled") or not sim.num_sewing_constraints:
+        return None
+    latched, frame, ramp_t, ratio = sim.get_sewing_priority_state()
+    return [1.0 if latched else 0.0, float(np.float32(frame)), ramp_t, ratio]
+
+
 class DebugRecorder:
     """Frames of one simulator, kept in memory up to max_frames (the oldest dropped first)."""
 
@@ -100,7 +108,7 @@ class DebugRecorder:
                "param_deltas": json.loads(sim.get_config_json()) if h != self._prev_hash else None,
                "elastic_scales": None, "bone_transforms": None,
                "sewing_rest_lengths": sim.get_sewing_current_rest_lengths() if sim.num_sewing_constraints else None,
-               "sewing_priority": None,
+               "sewing_priority": _priority_record(sim),
                "inv_mass_deltas": deltas or None}
         self._prev_pos = pos.copy()
         self._prev_hash = h
@@ -180,6 +188,10 @@ def replay(path: str):
                 sim.release_pins_batch(gone)
             if pins:
                 sim.set_pins_batch(want[0], np.asarray(want[1], np.float32).reshape(-1, 3), want[2])
+        # the latch state recorded with the frame is the state the frame stepped with (replayer.py:537-558)
+        st = fr.get("sewing_priority")
+        if isinstance(st, list) and len(st) >= 4:
+            sim.set_sewing_priority_state(float(st[0]) > 0.5, int(st[1]), float(st[2]), float(st[3]))
         # the recorded substeps and iterations are what ran; the adaptive policy is not re-run
         sim.set_adaptive_substep_options(False, sim._cfg["min_substeps"], sim._cfg["max_substeps"])
         sim.set_solver_iterations(fr["solver_iterations"])

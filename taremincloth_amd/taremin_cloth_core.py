@@ -40,6 +40,14 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+def _u32_arg(v):
+    """An unsigned 32-bit argument, rejected with OverflowError outside that range as PyO3 does."""
+    i = int(v)
+    if not 0 <= i < 2 ** 32:
+        raise OverflowError("argument must fit an unsigned 32-bit integer")
+    return i
+
+
 def _vec3(v, name):
     a = np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(-1))
     if a.size != 3:
@@ -272,8 +280,7 @@ class ClothSimulator:
         """Applies every field through its setter (config_impl.rs:91-166).  Options whose solver is not
         part of this build raise NotImplementedError unless they keep their inert value."""
         c = _sc.parse(json_str)
-        inert = {"coupled_mode": 0, "coupled_collider": False, "substep_interval": 1, "ee_substep_interval": 1,
-                 "sewing_priority_enabled": False}
+        inert = {"coupled_mode": 0, "coupled_collider": False, "substep_interval": 1, "ee_substep_interval": 1}
         if c["self_collision_algorithm"] == 2:
             raise NotImplementedError("SimConfig self_collision_algorithm=2: that solver option is not part of this build")
         for k, v in inert.items():
@@ -306,10 +313,11 @@ class ClothSimulator:
         self.set_sewing_shrink_speed(c["sewing_shrink_speed"])
         self.set_enable_sewing_lock(c["enable_sewing_lock"])
         self.set_sewing_lock_distance(c["sewing_lock_distance"])
+        self.set_sewing_priority_options(c["sewing_priority_enabled"], c["sewing_priority_threshold"],
+                                         c["sewing_priority_merge_dist"], c["sewing_priority_ramp_frames"],
+                                         c["sewing_priority_max_frames"])
         # recorded as given: options of solvers this build does not have, kept at their inert values
-        for k in ("post_relaxation_iters", "auto_coupled_on_low_substeps", "sewing_priority_threshold",
-                  "sewing_priority_merge_dist", "sewing_priority_ramp_frames", "sewing_priority_max_frames",
-                  "areal_density", "version"):
+        for k in ("post_relaxation_iters", "auto_coupled_on_low_substeps", "areal_density", "version"):
             self._cfg[k] = c[k]
 
     # --- debug recording (lib.rs:2956-3012, debug_recorder.rs) ----------------------------------
@@ -770,6 +778,54 @@ class ClothSimulator:
         v = np.ascontiguousarray(np.asarray(values, np.float32).reshape(-1))
         _lib.check(self._L.tc_sim_set_sewing_rest_lengths(self._h, _ptr(v), len(v)))
 
+    # --- sewing priority (lib.rs:2842-2923, mod.rs:1262-1420) ------------------------------------
+    def _priority(self):
+        out = np.empty(12, np.float64)
+        _lib.check(self._L.tc_sim_get_sewing_priority_state(self._h, _ptr(out)))
+        return out
+
+    def set_sewing_priority_options(self, enabled, threshold=0.9, merge_dist=0.005, ramp_frames=3, max_frames=600):
+        """Holds gravity and the wrinkle field at zero until a share `threshold` of the sewing pairs is
+        closer than `merge_dist`, then ramps them back over `ramp_frames` updates."""
+        _lib.check(self._L.tc_sim_set_sewing_priority_options(self._h, 1 if enabled else 0, float(threshold),
+                                                              float(merge_dist), _u32_arg(ramp_frames),
+                                                              _u32_arg(max_frames)))
+        st = self._priority()
+        self._cfg["sewing_priority_enabled"] = bool(st[0])
+        self._cfg["sewing_priority_threshold"] = float(np.float32(st[1]))
+        self._cfg["sewing_priority_merge_dist"] = float(np.float32(st[2]))
+        self._cfg["sewing_priority_ramp_frames"] = int(st[3])
+        self._cfg["sewing_priority_max_frames"] = int(st[4])
+
+    def set_sewing_priority_state(self, latched, frame, ramp_t, ratio):
+        _lib.check(self._L.tc_sim_set_sewing_priority_state(self._h, 1 if latched else 0, _u32_arg(frame),
+                                                            float(ramp_t), float(ratio)))
+
+    def get_sewing_priority_state(self) -> tuple:
+        """(latched, frame, ramp_t, ratio): the latch's internal state (mod.rs:1296-1303)."""
+        st = self._priority()
+        return bool(st[5]), int(st[6]), float(st[7]), float(st[8])
+
+    def update_sewing_priority(self, positions) -> tuple:
+        """Measures the closure on the given positions (N x 3, no read-back) and advances the latch;
+        returns (closure ratio, gravity scale, latched)."""
+        p = np.ascontiguousarray(np.asarray(positions, np.float32))
+        if p.ndim != 2 or p.shape[1] < 3:
+            raise ValueError("positions must be an N x 3 array")
+        p = np.ascontiguousarray(p[:, :3])
+        out = np.empty(3, np.float32)
+        _lib.check(self._L.tc_sim_update_sewing_priority(self._h, _ptr(p), len(p), _ptr(out)))
+        return float(out[0]), float(out[1]), bool(out[2])
+
+    def get_sewing_priority_scale(self) -> float:
+        return float(np.float32(self._priority()[9]))
+
+    def get_sewing_closure_ratio(self) -> float:
+        return float(np.float32(self._priority()[10]))
+
+    def is_sewing_priority_active(self) -> bool:
+        return bool(self._priority()[11])
+
     # --- diagnostics: GPU pass timing (lib.rs:1975-1990, simulation/profile.rs) -----------------
     def set_profiling_enabled(self, enable) -> bool:
         """Times each pass of the coming frames with HIP events; returns whether it is on."""
@@ -1204,6 +1260,19 @@ MESH_ARRAYS = {
     "adj_off": 35, "adj_idx": 36, "layer": 37, "bend_v0": 38, "bend_v1": 39, "bend_pad0": 40, "bend_pad1": 41,
     "sew_v0": 42, "sew_v1": 43, "sew_rest": 44, "sew_col_off": 45, "sew_col_cnt": 46,
 }
+
+
+def sewing_closure_ratio(positions, pairs, merge_dist=0.005) -> float:
+    """Share of sewing pairs closer than merge_dist (mod.rs:1381-1420); pairs indexing outside the
+    positions are skipped, and with none counted the share is 1.  Host only, no GPU."""
+    p = np.ascontiguousarray(np.asarray(positions, np.float32).reshape(-1, 3))
+    q = np.ascontiguousarray(np.asarray(pairs, np.int64).reshape(-1, 2))
+    keep = ((q >= 0) & (q < 2 ** 32)).all(axis=1)
+    q = np.ascontiguousarray(q[keep].astype(np.uint32))
+    out = C.c_float(0.0)
+    _lib.check(_lib.load().tc_sewing_closure_ratio(_ptr(p), len(p), _ptr(q), len(q), float(merge_dist),
+                                                   C.byref(out)))
+    return float(out.value)
 
 
 def build_mesh_arrays(positions, edges, faces=None, inv_masses=None, thicknesses=None, thickness=0.005,

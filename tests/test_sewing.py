@@ -23,6 +23,7 @@ draped with self-collision, and reproduces the reference's simulator tests (test
 2998): flap closure without crossing, a flat seam closing below 1 mm, a shrink speed changed
 mid-run keeping the rest length already reached.
 """
+import json
 import sys
 
 import numpy as np
@@ -326,6 +327,22 @@ def test_oracle_rest_length_setter_round_trip():
     assert s.sewing_rest_lengths()[0] == pytest.approx(0.25, abs=1e-6)
 
 
+def test_closure_ratio_pure():
+    """tests.rs:637-654 through the C ABI (host only)."""
+    pos = np.array([[0, 0, 0], [0.001, 0, 0], [0, 0, 0], [1.0, 0, 0]], np.float32)
+    assert T.sewing_closure_ratio(pos, [[0, 1], [2, 3]], 0.005) == pytest.approx(0.5, abs=1e-6)
+    assert T.sewing_closure_ratio(pos, np.zeros((0, 2)), 0.005) == 1.0
+    assert T.sewing_closure_ratio(pos, [[0, 99]], 0.005) == 1.0
+    # brute force on random pairs, including out-of-range ones
+    rng = np.random.default_rng(5)
+    p = rng.uniform(-0.01, 0.01, (50, 3)).astype(np.float32)
+    q = rng.integers(0, 60, (200, 2))
+    ok = (q < 50).all(axis=1)
+    d2 = ((p[q[ok, 0]] - p[q[ok, 1]]) ** 2).sum(axis=1)
+    want = float(np.float32((d2 <= np.float32(0.01) ** 2).sum()) / np.float32(ok.sum()))
+    assert T.sewing_closure_ratio(p, q, 0.01) == pytest.approx(want, abs=1e-6)
+
+
 # ------------------------------------------------------------------ GPU: product against the oracle
 def _gpos(sim):
     out = np.empty(3 * sim.num_vertices, np.float32)
@@ -486,3 +503,161 @@ def test_gpu_sewing_debug_record_replay(require_gpu, tmp_path):
         assert np.array_equal(p, np.asarray(fr["positions"], np.float32)), fr["frame_index"]
         n += 1
     assert n == 4
+
+
+# ------------------------------------------------------------------ GPU: sewing priority (mod.rs:1262-1420)
+def _prio_pair():
+    """tests.rs:657-665: two vertices, one sewing pair, 1 m apart."""
+    pos = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]], np.float32)
+    return pos, T.ClothSimulator(pos, np.zeros((0, 2), np.uint32), None, None,
+                                 sewing_springs=np.array([[0, 1]], np.uint32), stiffness=10000.0,
+                                 compression_stiffness=10.0, shear_stiffness=5000.0, bending_stiffness=0.0)
+
+
+@pytest.mark.gpu
+def test_gpu_sewing_priority_latch_and_ramp(require_gpu):
+    pos, g = _prio_pair()
+    assert g.num_sewing_constraints == 1
+    g.set_sewing_priority_options(True, 0.9, 0.005, 2, 0)
+    r0, s0, lat0 = g.update_sewing_priority(pos)
+    assert (r0, s0, lat0) == (0.0, 0.0, False)
+    assert g.is_sewing_priority_active()
+    closed = np.array([[0.0, 0.0, 0.0], [0.001, 0.0, 0.0]], np.float32)
+    r1, s1, lat1 = g.update_sewing_priority(closed)
+    assert r1 == 1.0 and lat1 and abs(s1 - 0.5) < 1e-6      # smoothstep(0.5)
+    _, s2, _ = g.update_sewing_priority(closed)
+    assert abs(s2 - 1.0) < 1e-6
+    _, s3, lat3 = g.update_sewing_priority(pos)              # one-way latch
+    assert lat3 and s3 == 1.0
+    assert g.get_sewing_priority_state() == (True, 4, 1.0, 0.0)
+    # an unchanged setting keeps the latch, a changed one resets it
+    g.set_sewing_priority_options(True, 0.9, 0.005, 2, 0)
+    assert g.get_sewing_priority_state()[0]
+    g.set_sewing_priority_options(True, 0.8, 0.005, 2, 0)
+    assert g.get_sewing_priority_state() == (False, 0, 0.0, 0.0)
+    # reset clears it too
+    g.update_sewing_priority(closed)
+    g.reset()
+    assert g.get_sewing_priority_state() == (False, 0, 0.0, 0.0)
+
+
+@pytest.mark.gpu
+def test_gpu_sewing_priority_max_frames_and_disabled(require_gpu):
+    pos, g = _prio_pair()
+    g.set_sewing_priority_options(True, 1.0, 0.005, 0, 2)       # tests.rs:698-718
+    _, s0, lat0 = g.update_sewing_priority(pos)
+    assert not lat0 and s0 == 0.0
+    _, s1, lat1 = g.update_sewing_priority(pos)
+    assert lat1 and s1 == 1.0
+    pos, g = _prio_pair()                                       # tests.rs:721-738
+    assert g.get_sewing_priority_scale() == 1.0 and not g.is_sewing_priority_active()
+    assert g.update_sewing_priority(pos) == (1.0, 1.0, True)
+    assert g.get_sewing_closure_ratio() == 1.0
+    # clamps (mod.rs:1270-1273) and the configuration record
+    g.set_sewing_priority_options(True, 1.5, 1.0, 500, 10 ** 6)
+    c = json.loads(g.get_config_json())
+    assert (c["sewing_priority_threshold"], c["sewing_priority_ramp_frames"], c["sewing_priority_max_frames"]) == \
+        (1.0, 120, 100000)
+    assert abs(c["sewing_priority_merge_dist"] - 0.1) < 1e-7
+    with pytest.raises(OverflowError):
+        g.set_sewing_priority_options(True, 0.9, 0.005, -1, 0)
+
+
+@pytest.mark.gpu
+def test_gpu_sewing_priority_scales_gravity_bit_exact(require_gpu):
+    """Gravity times the latch's scale, frame by frame: the product with the priority on equals the FP32
+    oracle stepped with that scaled gravity; while the latch holds, the panels do not fall."""
+    n = 10
+    pos, e, f, sew = sewn_panels(n, spacing=0.015, gap=0.03, seed=7)
+    kw = dict(thickness=0.003, sewing_shrink_speed=0.6)
+    g = T.ClothSimulator(pos, e, f, None, sewing_springs=sew, **kw)
+    o = O.OracleSim(pos, e, f, None, sewing_springs=sew, **kw)
+    for s in (g, o):
+        s.set_gravity(0.0, 0.0, -9.81)
+        s.set_enable_self_collision(True)
+    g.set_sewing_priority_options(True, 0.9, 0.004, 3, 0)
+    prev = pos.copy()
+    scales = []
+    for fr in range(10):
+        _, sc, _ = g.update_sewing_priority(prev)
+        scales.append(sc)
+        gz = np.float32(-9.81) * np.float32(sc)
+        o.set_gravity(0.0 * np.float32(sc), 0.0 * np.float32(sc), float(gz))
+        g.step(1.0 / 60.0, 5, 6)
+        o.step(1.0 / 60.0, 5, 6)
+        prev = _gpos(g)
+        assert np.array_equal(prev, o.get_positions().reshape(-1, 3)), fr
+        if sc == 0.0:
+            assert float(np.abs(prev[:, 2] - pos[:, 2]).max()) < 1e-3, fr
+    _say(f"priority scales per frame: {scales}")
+    assert scales[0] == 0.0 and scales[-1] == 1.0 and any(0.0 < x < 1.0 for x in scales)
+
+
+@pytest.mark.gpu
+def test_gpu_sewing_priority_suppresses_wrinkle_field(require_gpu):
+    """tests.rs:741-776 made exact: with the wrinkle field on, the unlatched frames skip the field and the
+    ramping frames scale its blend by the latch's scale; the product equals the FP32 oracle stepped
+    without the field, then with the blend scaled, frame by frame."""
+    pos = np.array([[0.07, 0.0, 0.0], [-0.07, 0.0, 0.0]], np.float32)   # both inside the field's range
+    sew = np.array([[0, 1]], np.uint32)
+    g = T.ClothSimulator(pos, np.zeros((0, 2), np.uint32), None, None, sewing_springs=sew)
+    o = O.OracleSim(pos, np.zeros((0, 2), np.uint32), None, None, sewing_springs=sew)
+    for s in (g, o):
+        s.set_gravity(0.0, 0.0, 0.0)
+        s.set_enable_self_collision(False)
+    tw, th = 16, 8
+    tex = np.random.default_rng(3).integers(0, 256, size=(th, tw, 4), dtype=np.uint8)
+    g.set_enable_wrinkle_field(True)
+    g.set_wrinkle_field_params([0.0, 0.0, 0.0], [0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0],
+                               influence_radius=0.05, bone_radius=0.05, stiffness=3.0, blend_weight=0.9)
+    g.set_wrinkle_field_texture_2d(tw, th, tex.tobytes(), -0.2, 0.2, 0.02, 0.16)
+    g.set_sewing_priority_options(True, 0.9, 0.005, 2, 0)
+    wp = g._wrinkle_params()
+    closed = np.array([[0.0, 0.0, 0.0], [0.001, 0.0, 0.0]], np.float32)
+    seen = []
+    for fr, upd in enumerate([pos, pos, closed, closed, closed]):
+        _, sc, _ = g.update_sewing_priority(upd)
+        seen.append(sc)
+        w = wp.copy()
+        w[15] = np.float32(w[15]) * np.float32(sc)
+        o.set_wrinkle(sc > 0.0, w, tex.tobytes(), tw, th)
+        g.step(0.016, 2, 3)
+        o.step(0.016, 2, 3)
+        assert np.array_equal(_gpos(g), o.get_positions().reshape(-1, 3)), fr
+    assert seen == [0.0, 0.0, 0.5, 1.0, 1.0]
+    # the field acts on these vertices: the same run with the priority off ends elsewhere
+    h = T.ClothSimulator(pos, np.zeros((0, 2), np.uint32), None, None, sewing_springs=sew)
+    h.set_gravity(0.0, 0.0, 0.0)
+    h.set_enable_self_collision(False)
+    h.set_enable_wrinkle_field(True)
+    h.set_wrinkle_field_params([0.0, 0.0, 0.0], [0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0],
+                               influence_radius=0.05, bone_radius=0.05, stiffness=3.0, blend_weight=0.9)
+    h.set_wrinkle_field_texture_2d(tw, th, tex.tobytes(), -0.2, 0.2, 0.02, 0.16)
+    for _ in range(5):
+        h.step(0.016, 2, 3)
+    assert not np.array_equal(_gpos(h), _gpos(g))
+
+
+@pytest.mark.gpu
+def test_gpu_sewing_priority_record_replay(require_gpu, tmp_path):
+    from taremincloth_amd import debug_recorder as DR
+    pos, e, f, sew = sewn_panels(6, gap=0.02, seed=4)
+    g = T.ClothSimulator(pos, e, f, None, sewing_springs=sew, sewing_shrink_speed=0.8)
+    g.set_sewing_priority_options(True, 0.9, 0.004, 2, 0)
+    g.set_solver_iterations(5)
+    g.start_debug_recording("sewn", max_frames=20)
+    prev = pos
+    for _ in range(6):
+        g.update_sewing_priority(prev)
+        g.step(1.0 / 60.0, 4)
+        prev = _gpos(g)
+    g.stop_debug_recording()
+    path = g.save_debug_recording(str(tmp_path / "prio.jsonl.gz"))
+    meta, frames = DR.load(path)
+    assert meta["config"]["sewing_priority_enabled"] is True
+    assert all(fr["sewing_priority"] is not None for fr in frames)
+    n = 0
+    for fr, p in DR.replay(path):
+        assert np.array_equal(p, np.asarray(fr["positions"], np.float32)), fr["frame_index"]
+        n += 1
+    assert n == 6
