@@ -112,6 +112,12 @@ extern "C" {
     pub fn tc_sim_clear_colliders(sim: *mut TcSim) -> i32;
     pub fn tc_sim_set_sdf_colliders(sim: *mut TcSim, width: u32, height: u32, depth: u32, texels: *const c_void, nbytes: usize, infos: *const f32, nb: u32) -> i32;
     pub fn tc_sim_update_bone_transforms(sim: *mut TcSim, world: *const f32, inv_world: *const f32, n: u32) -> i32;
+    pub fn tc_sim_clear_bone_sdf_colliders(sim: *mut TcSim) -> i32;
+    pub fn tc_sim_setup_dynamic_bone_sdf(sim: *mut TcSim, width: u32, height: u32, depth: u32, res: u32, infos: *const f32, num_bones: u32, rest_positions: *const f32, num_verts: u32, bone_indices: *const u32, bone_weights: *const f32, tri_sources: *const u32, tri_weights: *const f32, num_tris: u32, bind_inv: *const f32, num_bind: u32, update_interval: u32) -> i32;
+    pub fn tc_sim_set_dynamic_bone_sdf_dirty_bones(sim: *mut TcSim, bones: *const u32, n: u32, has_list: i32) -> i32;
+    pub fn tc_sim_set_dynamic_bone_sdf_enabled(sim: *mut TcSim, enabled: i32) -> i32;
+    pub fn tc_sim_set_dynamic_bone_sdf_update_interval(sim: *mut TcSim, interval: u32) -> i32;
+    pub fn tc_sim_dynamic_bone_sdf_stats(sim: *const TcSim, out: *mut u32) -> i32;
     pub fn tc_sim_set_mesh_triangles(sim: *mut TcSim, tris: *const f32, num_tris: u32, attrs: *const f32, friction: f32, thickness: f32, restitution: f32, single_sided: i32) -> i32;
     pub fn tc_sim_set_gravity(sim: *mut TcSim, gx: f32, gy: f32, gz: f32) -> i32;
     pub fn tc_sim_set_damping(sim: *mut TcSim, damping: f32) -> i32;

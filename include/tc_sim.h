@@ -238,6 +238,38 @@ int tc_sim_set_sdf_colliders(tc_sim *sim, uint32_t width, uint32_t height, uint3
 /* ClothSimulator.update_bone_transforms(world, inv_world), lib.rs:2287-2330: n 4x4 matrices each,
  * row-major as given (the reference uploads them unchanged); n must equal the collider count. */
 int tc_sim_update_bone_transforms(tc_sim *sim, const float *world, const float *inv_world, uint32_t n);
+/* ClothSimulator.clear_bone_sdf_colliders, lib.rs:2342-2345 -> simulation/mod.rs:1941-1948: removes the
+ * SDF colliders and the dynamic bone SDF. */
+int tc_sim_clear_bone_sdf_colliders(tc_sim *sim);
+/* ClothSimulator.setup_dynamic_bone_sdf, lib.rs:2362-2588 -> simulation/mod.rs:1998-2312.  A skinned
+ * collider mesh whose bone-SDF atlas (width x height x depth texels, res^3 tiles, bone b's tile at
+ * column b % (width/res), row (b / (width/res)) % (height/res), layer b / ((width/res)(height/res)))
+ * is re-baked on the device at the start of every update_interval-th step / step_async /
+ * step_buffered frame.  infos: num_bones x 20 floats (the collider infos; the bake box is the
+ * local min/max of slots 0-2 and 4-6); rest_positions: num_verts x 3; bone_indices / bone_weights:
+ * num_verts x 4 (the four skin influences); tri_sources: num_tris x 4 (i0, i1, i2, bone), each
+ * bone's triangles contiguous; tri_weights: num_tris x 3 (the corners' weights); bind_inv: num_bind
+ * row-major 4x4 bind matrices (world -> bone), num_bind >= num_bones.  TC_ERR_VALUE for a zero
+ * size, num_bind < num_bones, a weighted bone index, triangle vertex or triangle bone out of range,
+ * or a tile outside the atlas (the reference reads or writes out of range there).  An empty mesh,
+ * triangle list or bone list changes nothing.  The atlas stays zero until the first update, which
+ * bakes every bone. */
+int tc_sim_setup_dynamic_bone_sdf(tc_sim *sim, uint32_t width, uint32_t height, uint32_t depth, uint32_t res,
+                                  const float *infos, uint32_t num_bones, const float *rest_positions,
+                                  uint32_t num_verts, const uint32_t *bone_indices, const float *bone_weights,
+                                  const uint32_t *tri_sources, const float *tri_weights, uint32_t num_tris,
+                                  const float *bind_inv, uint32_t num_bind, uint32_t update_interval);
+/* set_dynamic_bone_sdf_dirty_bones(dirty_bone_indices=None), lib.rs:2332-2340 -> simulation/mod.rs:
+ * 2314-2317: has_list 0 re-bakes every bone at each update; otherwise only the listed bones (an empty
+ * list skips the bake).  update_bone_transforms in Python sets it as well. */
+int tc_sim_set_dynamic_bone_sdf_dirty_bones(tc_sim *sim, const uint32_t *bones, uint32_t n, int has_list);
+/* set_dynamic_bone_sdf_enabled, lib.rs:2591-2594 (on only after a setup) and
+ * set_dynamic_bone_sdf_update_interval, lib.rs:2596-2599 (at least 1). */
+int tc_sim_set_dynamic_bone_sdf_enabled(tc_sim *sim, int enabled);
+int tc_sim_set_dynamic_bone_sdf_update_interval(tc_sim *sim, uint32_t interval);
+/* Diagnostics (no reference counterpart): out[0] enabled, [1] baked at least once, [2] bone tiles
+ * baked so far, [3] frame counter, [4] updates skipped by an empty dirty list. */
+int tc_sim_dynamic_bone_sdf_stats(const tc_sim *sim, uint32_t *out);
 
 /* set_mesh_collider_triangles(tris f32[T,3,3], friction, thickness, restitution, single_sided, attributes=None),
  * lib.rs:2175-2233 -> upload_colliders (simulation/mod.rs:2582-2765). attrs: NULL or T*4 floats
@@ -360,6 +392,9 @@ uint32_t tc_sim_num_bending_constraints(const tc_sim *sim);
  * 17 scalars [cell_size(f32 bits), table_size, n_virtual, num_edges]       18 Morton-sorted edges (u32 pairs)
  * 19 edge hash keys                 20 edge hash cell starts                 21 edge hash sorted indices
  * 22 inverse masses the solver reads (f32; the base values scaled by the pins)
+ * 23 SDF texels (u32, width x height x depth)  24 SDF collider infos (f32, 20 per collider)
+ * 25 dynamic bone SDF skinned positions (f32 xyz)  26 dynamic bone SDF bake triangles (16 f32 each:
+ *    p0, w0, p1, w1, p2, w2, normal, open-edge mask)
  * *count receives the element count; out may be NULL to query it. */
 int tc_sim_get_array(tc_sim *sim, int which, void *out, size_t capacity, size_t *count);
 /* Rebuild the spatial hashes from the current predicted positions (parity checks of the hash). */

@@ -120,6 +120,17 @@ def lib(precision: str = "f32") -> C.CDLL:
                                        C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     L.orc_sim_set_sdf.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _u32p, _f32p, C.c_uint32]
     L.orc_sim_set_bone_transforms.argtypes = [_vp, _f32p, _f32p, C.c_uint32]
+    L.orc_sim_clear_bone_sdf.argtypes = [_vp]
+    L.orc_sim_setup_dynamic_bone_sdf.restype = C.c_int
+    L.orc_sim_setup_dynamic_bone_sdf.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _vp,
+                                                 C.c_uint32, _vp, C.c_uint32, _vp, _vp, _vp, _vp, C.c_uint32, _vp,
+                                                 C.c_uint32, C.c_uint32]
+    L.orc_sim_set_dynamic_dirty.argtypes = [_vp, _vp, C.c_uint32, C.c_int]
+    L.orc_sim_set_dynamic_enabled.argtypes = [_vp, C.c_int]
+    L.orc_sim_set_dynamic_interval.argtypes = [_vp, C.c_uint32]
+    L.orc_sim_dynamic_arrays.argtypes = [_vp, _vp, _vp]
+    L.orc_sim_sdf_arrays.argtypes = [_vp, _vp, _vp]
+    L.orc_bone_bake_tiles.argtypes = [_vp, C.c_uint32, _vp, _vp]
     L.orc_sim_set_pin_table.argtypes = [_vp, _vp, _vp, _vp, C.c_uint32]
     L.orc_sim_set_inv_masses.argtypes = [_vp, _vp, _vp, C.c_uint32]
     L.orc_sim_get_inv_masses.argtypes = [_vp, C.c_int, _f32p]
@@ -379,10 +390,58 @@ class OracleSim:
         infos = np.ascontiguousarray(np.asarray(bone_infos, np.float32)[:, :20])
         self._L.orc_sim_set_sdf(self._h, width, height, depth, tex, infos, len(infos))
 
-    def update_bone_transforms(self, world_matrices, inv_world_matrices):
+    def update_bone_transforms(self, world_matrices, inv_world_matrices, dirty_bone_indices=None):
+        """lib.rs:2287-2330: transforms, the bounding spheres in the infos, and the dirty bones."""
         w = np.ascontiguousarray(world_matrices, np.float32)
         iw = np.ascontiguousarray(inv_world_matrices, np.float32)
         self._L.orc_sim_set_bone_transforms(self._h, w, iw, len(w))
+        self.set_dynamic_bone_sdf_dirty_bones(dirty_bone_indices)
+
+    def clear_bone_sdf_colliders(self):
+        self._L.orc_sim_clear_bone_sdf(self._h)
+
+    def setup_dynamic_bone_sdf(self, width, height, depth, res, bone_infos, rest_verts, bone_indices, bone_weights,
+                               tri_sources, tri_weights, bone_bind_inv_matrices, update_interval=1):
+        """setup_dynamic_bone_sdf (lib.rs:2362-2588); raises ValueError where the build rejects the input."""
+        infos = np.ascontiguousarray(np.asarray(bone_infos, np.float32)[:, :20])
+        rv = np.ascontiguousarray(np.asarray(rest_verts, np.float32)[:, :3])
+        bi = np.ascontiguousarray(np.asarray(bone_indices, np.uint32)[:, :4])
+        bw = np.ascontiguousarray(np.asarray(bone_weights, np.float32)[:, :4])
+        ts = np.ascontiguousarray(np.asarray(tri_sources, np.uint32).reshape(-1, 4)[:, :4])
+        tw = np.ascontiguousarray(np.asarray(tri_weights, np.float32).reshape(-1, 3)[:, :3])
+        inv = np.ascontiguousarray(bone_bind_inv_matrices, np.float32)
+        r = self._L.orc_sim_setup_dynamic_bone_sdf(self._h, width, height, depth, res, _ptr(infos), len(infos),
+                                                   _ptr(rv), len(rv), _ptr(bi), _ptr(bw), _ptr(ts), _ptr(tw), len(ts),
+                                                   _ptr(inv), len(inv), update_interval)
+        if r:
+            raise ValueError("setup_dynamic_bone_sdf: input rejected")
+
+    def set_dynamic_bone_sdf_dirty_bones(self, dirty_bone_indices=None):
+        if dirty_bone_indices is None:
+            self._L.orc_sim_set_dynamic_dirty(self._h, None, 0, 0)
+        else:
+            d = np.ascontiguousarray(np.asarray(dirty_bone_indices, np.uint32).reshape(-1))
+            self._L.orc_sim_set_dynamic_dirty(self._h, _ptr(d), len(d), 1)
+
+    def set_dynamic_bone_sdf_enabled(self, enabled):
+        self._L.orc_sim_set_dynamic_enabled(self._h, 1 if enabled else 0)
+
+    def set_dynamic_bone_sdf_update_interval(self, interval):
+        self._L.orc_sim_set_dynamic_interval(self._h, int(interval))
+
+    def dynamic_arrays(self, nv, nt):
+        """(skinned positions (nv, 3), bake triangles (nt, 16)) of the last dynamic update."""
+        sk = np.zeros((nv, 3), np.float32)
+        tr = np.zeros((nt, 16), np.float32)
+        self._L.orc_sim_dynamic_arrays(self._h, _ptr(sk), _ptr(tr))
+        return sk, tr
+
+    def sdf_arrays(self, nvox, nb):
+        """(texels (nvox,) u32, infos (nb, 20)) of the SDF colliders."""
+        tex = np.zeros(nvox, np.uint32)
+        info = np.zeros((nb, 20), np.float32)
+        self._L.orc_sim_sdf_arrays(self._h, _ptr(tex), _ptr(info))
+        return tex, info
 
     def reset(self):
         """dispatch.rs:1411-1427: initial state and inverse masses, no pins; the base inverse masses

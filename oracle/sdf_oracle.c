@@ -850,6 +850,40 @@ static uint32_t bone_voxel(const obparams *P, const obtri *tr, uint32_t res, uin
     return (uint32_t)orc_f32_to_f16((float)best_sd) | ((uint32_t)orc_f32_to_f16((float)best_alpha) << 16);
 }
 
+/* bake_sdf.wgsl over bones [0, nbake) of a parameter array: params holds 16 words per bone in the
+ * GpuBakeParams layout (local min[3], tri_start, local max[3], tri_count, col, row, layer, res, width,
+ * height, depth, pitch), tris 16 floats per triangle in the GpuBakeTriangle layout (p0, w0, p1, w1, p2,
+ * w2, normal, open-edge mask); each bone's res^3 tile is written into tex (width x height rows, Z/Y/X). */
+void orc_bone_bake_tiles(const uint32_t *params, uint32_t nbake, const float *tris, uint32_t *tex) {
+    for (uint32_t b = 0; b < nbake; b++) {
+        const uint32_t *q = params + 16 * (size_t)b;
+        obparams P;
+        memcpy(P.lmin, q, 12); memcpy(P.lmax, q + 4, 12);
+        P.start = q[3]; P.count = q[7]; P.col = q[8]; P.row = q[9]; P.layer = q[10];
+        const uint32_t res = q[11], tw = q[12], th = q[13], pitch = q[15] ? q[15] : tw;
+        obtri *tr = (obtri *)malloc(sizeof(obtri) * (P.count ? P.count : 1));
+        for (uint32_t i = 0; i < P.count; i++) {
+            const float *t = tris + 16 * (size_t)(P.start + i);
+            for (int c = 0; c < 3; c++) {
+                for (int k = 0; k < 3; k++) tr[i].p[c][k] = t[4 * c + k];
+                tr[i].w[c] = t[4 * c + 3];
+            }
+            for (int k = 0; k < 3; k++) tr[i].n[k] = t[12 + k];
+            tr[i].bmask = t[15];
+        }
+        obparams L = P;
+        L.start = 0;
+        for (uint32_t vz = 0; vz < res; vz++)
+            for (uint32_t vy = 0; vy < res; vy++)
+                for (uint32_t vx = 0; vx < res; vx++) {
+                    const size_t gx = (size_t)P.col * res + vx, gy = (size_t)P.row * res + vy,
+                                 gz = (size_t)P.layer * res + vz;
+                    tex[(gz * th + gy) * pitch + gx] = bone_voxel(&L, tr, res, vx, vy, vz);
+                }
+        free(tr);
+    }
+}
+
 /* bake_bone_sdf_gpu_ext (sdf_baker.rs:494-930).  weights: nbones x nv (bone-major); binds: nbones
  * row-major 4x4 bind (world -> bone) matrices.  Returns NULL when no bone has a weight above the
  * threshold or resolution is 0. */

@@ -1349,6 +1349,14 @@ typedef struct {
     real *s_cur, *s_tgt, *lam_s;
     float sew_stiffness, sew_lock_dist, sew_speed;
     int sew_lock;
+    /* dynamic bone SDF (setup_dynamic_bone_sdf): rest mesh with four skin influences per vertex,
+     * triangle sources {i0, i1, i2, bone} with corner weights and open-edge masks, bind matrices,
+     * bake parameters (16 words per bone, GpuBakeParams layout) and the device parameter buffer's
+     * contents; counters follow dispatch_dynamic_bone_sdf_update */
+    int dyn_ready, dyn_enabled, dyn_baked, dyn_dirty_set;
+    uint32_t dyn_interval, dyn_counter, dyn_res, dyn_nb, dyn_nv, dyn_nt, dyn_ndirty;
+    float *dyn_rest, *dyn_w, *dyn_tw, *dyn_bind, *dyn_tris, *dyn_skinned;
+    uint32_t *dyn_bi, *dyn_src, *dyn_bmask, *dyn_params, *dyn_params_dev, *dyn_dirty;
 } orc_sim;
 
 orc_sim *orc_sim_create_sewn(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne,
@@ -1441,6 +1449,9 @@ void orc_sim_free(orc_sim *s) {
     free(s->pin_v); free(s->pin_t); free(s->pin_w); free(s->jac); free(s->pair_vt); free(s->pair_ee);
     free(s->cedges); free(s->cedge_flags); free(s->cstarts); free(s->csorted); free(s->wtex);
     free(s->s_cur); free(s->s_tgt); free(s->lam_s);
+    free(s->dyn_rest); free(s->dyn_w); free(s->dyn_tw); free(s->dyn_bind); free(s->dyn_tris); free(s->dyn_skinned);
+    free(s->dyn_bi); free(s->dyn_src); free(s->dyn_bmask); free(s->dyn_params); free(s->dyn_params_dev);
+    free(s->dyn_dirty);
     free(s);
 }
 
@@ -1504,12 +1515,252 @@ void orc_sim_set_sdf(orc_sim *s, uint32_t w, uint32_t h, uint32_t d, const uint3
             for (int k = 0; k < 4; k++) s->sdf_xf[(size_t)b * 32 + m * 16 + k * 5] = 1.0f;
     s->sdf_w = w; s->sdf_h = h; s->sdf_d = d; s->sdf_nb = nb;
 }
-/* update_bone_transforms (lib.rs:2287-2330): world and inverse-world 4x4 matrices, uploaded as given */
+/* update_bone_transforms (lib.rs:2287-2330 -> simulation/mod.rs:1865-1938): world and inverse-world
+ * 4x4 matrices, uploaded as given (WGSL reads each given row as a column); each collider's world
+ * bounding sphere goes to info slots 3, 7, 11 (centre) and 15 (radius): the centre of the local box
+ * (slots 0-2, 4-6) through the world matrix, radius = half the box diagonal times the largest
+ * column length, plus 0.1 m. */
 void orc_sim_set_bone_transforms(orc_sim *s, const float *world, const float *inv, uint32_t n) {
     for (uint32_t b = 0; b < n && b < s->sdf_nb; b++) {
         memcpy(&s->sdf_xf[(size_t)b * 32], world + (size_t)b * 16, 64);
         memcpy(&s->sdf_xf[(size_t)b * 32 + 16], inv + (size_t)b * 16, 64);
+        float *I = &s->sdf_info[(size_t)b * 20];
+        const float *M = world + (size_t)b * 16;   /* M[4r + c] = m[r][c] */
+        const float cl[3] = {(I[0] + I[4]) * 0.5f, (I[1] + I[5]) * 0.5f, (I[2] + I[6]) * 0.5f};
+        const float dx = I[4] - I[0], dy = I[5] - I[1], dz = I[6] - I[2];
+        const float rl = 0.5f * sqrtf(dx * dx + dy * dy + dz * dz);
+        float cw[3], sc[3];
+        for (int k = 0; k < 3; k++) cw[k] = M[k] * cl[0] + M[4 + k] * cl[1] + M[8 + k] * cl[2] + M[12 + k];
+        for (int r = 0; r < 3; r++) sc[r] = sqrtf(M[4 * r] * M[4 * r] + M[4 * r + 1] * M[4 * r + 1] + M[4 * r + 2] * M[4 * r + 2]);
+        float ms = sc[0];
+        if (sc[1] > ms) ms = sc[1];
+        if (sc[2] > ms) ms = sc[2];
+        I[3] = cw[0]; I[7] = cw[1]; I[11] = cw[2]; I[15] = rl * ms + 0.1f;
     }
+}
+
+void orc_bone_bake_tiles(const uint32_t *params, uint32_t nbake, const float *tris, uint32_t *tex);
+
+/* clear_bone_sdf_colliders (simulation/mod.rs:1941-1948) */
+void orc_sim_clear_bone_sdf(orc_sim *s) {
+    orc_sim_set_sdf(s, 0, 0, 0, NULL, NULL, 0);
+    s->dyn_ready = s->dyn_enabled = 0;
+}
+
+static int cmp_u64(const void *a, const void *b) {
+    const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : x > y;
+}
+static int has_u64(const uint64_t *k, size_t n, uint64_t key) {  /* binary search in sorted keys */
+    size_t lo = 0, hi = n;
+    while (lo < hi) { size_t mid = (lo + hi) / 2; if (k[mid] < key) lo = mid + 1; else hi = mid; }
+    return lo < n && k[lo] == key;
+}
+
+/* setup_dynamic_bone_sdf (lib.rs:2362-2588 -> simulation/mod.rs:1998-2312).  Returns 0, or 1 for the
+ * inputs the build rejects (zero sizes, too few bind matrices, indices out of range, tiles outside the
+ * atlas).  rest: nv x 3; bi / bw: nv x 4; src: nt x 4 (i0, i1, i2, bone); tw: nt x 3; bind: nbind x 16. */
+int orc_sim_setup_dynamic_bone_sdf(orc_sim *s, uint32_t w, uint32_t h, uint32_t d, uint32_t res, const float *infos,
+                                   uint32_t nb, const float *rest, uint32_t nv, const uint32_t *bi, const float *bw,
+                                   const uint32_t *src, const float *tw, uint32_t nt, const float *bind,
+                                   uint32_t nbind, uint32_t interval) {
+    if (!res || !w || !h || !d || nbind < nb) return 1;
+    for (size_t i = 0; i < (size_t)nv * 4; i++) if (bw[i] > 0.0f && bi[i] >= nb) return 1;
+    for (uint32_t t = 0; t < nt; t++)
+        if (src[4 * t] >= nv || src[4 * t + 1] >= nv || src[4 * t + 2] >= nv || src[4 * t + 3] >= nb) return 1;
+    if (!nv || !nt || !nb) { s->dyn_interval = interval ? interval : 1; return 0; }
+    const uint32_t cols = w / res ? w / res : 1, rows = h / res ? h / res : 1;
+    for (uint32_t b = 0; b < nb; b++) {
+        const uint32_t l = b / (cols * rows), rem = b % (cols * rows), r = rem / cols, c = rem % cols;
+        if ((uint64_t)(c + 1) * res > w || (uint64_t)(r + 1) * res > h || (uint64_t)(l + 1) * res > d) return 1;
+    }
+    if (w > 2048 || h > 2048 || d > 2048) return 1;
+    /* undirected edges used by exactly one distinct triangle (the sources deduplicated by their sorted
+     * vertex triple, collect_open_boundary_undirected_edges, sdf_baker.rs:203-240) */
+    uint64_t *tk = (uint64_t *)malloc(sizeof(uint64_t) * nt);
+    uint32_t *first = (uint32_t *)malloc(sizeof(uint32_t) * nt);
+    uint64_t *ek = (uint64_t *)malloc(sizeof(uint64_t) * 3 * (size_t)nt);
+    size_t ne = 0;
+    for (uint32_t t = 0; t < nt; t++) {  /* first occurrence of each sorted triple (n^2 for clarity) */
+        uint32_t a = src[4 * t], b2 = src[4 * t + 1], c = src[4 * t + 2], x;
+        if (a > b2) { x = a; a = b2; b2 = x; }
+        if (b2 > c) { x = b2; b2 = c; c = x; }
+        if (a > b2) { x = a; a = b2; b2 = x; }
+        tk[t] = ((uint64_t)a << 42) ^ ((uint64_t)b2 << 21) ^ (uint64_t)c;
+        first[t] = 1;
+        for (uint32_t u = 0; u < t; u++) {
+            uint32_t p = src[4 * u], q = src[4 * u + 1], r2 = src[4 * u + 2], y;
+            if (p > q) { y = p; p = q; q = y; }
+            if (q > r2) { y = q; q = r2; r2 = y; }
+            if (p > q) { y = p; p = q; q = y; }
+            if (p == a && q == b2 && r2 == c) { first[t] = 0; break; }
+        }
+        const uint32_t i0 = src[4 * t], i1 = src[4 * t + 1], i2 = src[4 * t + 2];
+        if (!first[t] || i0 > 0x7FFFFFFFu || i1 > 0x7FFFFFFFu || i2 > 0x7FFFFFFFu || i0 == i1 || i1 == i2 || i2 == i0)
+            continue;
+        const uint32_t e[3][2] = {{i0, i1}, {i1, i2}, {i2, i0}};
+        for (int k = 0; k < 3; k++) {
+            const uint32_t lo = e[k][0] < e[k][1] ? e[k][0] : e[k][1], hi = e[k][0] < e[k][1] ? e[k][1] : e[k][0];
+            ek[ne++] = ((uint64_t)lo << 32) | hi;
+        }
+    }
+    qsort(ek, ne, sizeof(uint64_t), cmp_u64);
+    uint64_t *open = (uint64_t *)malloc(sizeof(uint64_t) * (ne ? ne : 1));
+    size_t no = 0;
+    for (size_t i = 0; i < ne;) {
+        size_t j = i;
+        while (j < ne && ek[j] == ek[i]) j++;
+        if (j - i == 1) open[no++] = ek[i];
+        i = j;
+    }
+    free(tk); free(first); free(ek);
+    free(s->dyn_rest); free(s->dyn_w); free(s->dyn_tw); free(s->dyn_bind); free(s->dyn_tris); free(s->dyn_skinned);
+    free(s->dyn_bi); free(s->dyn_src); free(s->dyn_bmask); free(s->dyn_params); free(s->dyn_params_dev);
+    s->dyn_rest = (float *)malloc(12 * (size_t)nv); memcpy(s->dyn_rest, rest, 12 * (size_t)nv);
+    s->dyn_bi = (uint32_t *)malloc(16 * (size_t)nv); memcpy(s->dyn_bi, bi, 16 * (size_t)nv);
+    s->dyn_w = (float *)malloc(16 * (size_t)nv); memcpy(s->dyn_w, bw, 16 * (size_t)nv);
+    s->dyn_src = (uint32_t *)malloc(16 * (size_t)nt); memcpy(s->dyn_src, src, 16 * (size_t)nt);
+    s->dyn_tw = (float *)malloc(12 * (size_t)nt); memcpy(s->dyn_tw, tw, 12 * (size_t)nt);
+    s->dyn_bind = (float *)malloc(64 * (size_t)nb); memcpy(s->dyn_bind, bind, 64 * (size_t)nb);
+    s->dyn_bmask = (uint32_t *)malloc(4 * (size_t)nt);
+    for (uint32_t t = 0; t < nt; t++) {
+        const uint32_t i0 = src[4 * t], i1 = src[4 * t + 1], i2 = src[4 * t + 2];
+        uint32_t m = 0;
+        if (i0 <= 0x7FFFFFFFu && i1 <= 0x7FFFFFFFu && i2 <= 0x7FFFFFFFu && i0 != i1 && i1 != i2 && i2 != i0) {
+            const uint32_t e[3][2] = {{i0, i1}, {i1, i2}, {i2, i0}};
+            for (int k = 0; k < 3; k++) {
+                const uint32_t lo = e[k][0] < e[k][1] ? e[k][0] : e[k][1], hi = e[k][0] < e[k][1] ? e[k][1] : e[k][0];
+                if (has_u64(open, no, ((uint64_t)lo << 32) | hi)) m |= 1u << k;
+            }
+        }
+        s->dyn_bmask[t] = m;
+    }
+    free(open);
+    /* bake parameters: the bone's local box, its triangle range (first index, count) and tile */
+    s->dyn_params = (uint32_t *)calloc(16 * (size_t)nb, 4);
+    for (uint32_t b = 0; b < nb; b++) {
+        uint32_t *q = s->dyn_params + 16 * (size_t)b;
+        const float *I = infos + 20 * (size_t)b;
+        uint32_t start = 0, count = 0;
+        for (uint32_t t = 0; t < nt; t++)
+            if (src[4 * t + 3] == b) { if (!count) start = t; count++; }
+        memcpy(q, I, 12); q[3] = start; memcpy(q + 4, I + 4, 12); q[7] = count;
+        const uint32_t l = b / (cols * rows), rem = b % (cols * rows);
+        q[8] = rem % cols; q[9] = rem / cols; q[10] = l; q[11] = res; q[12] = w; q[13] = h; q[14] = d; q[15] = 0;
+    }
+    s->dyn_params_dev = (uint32_t *)malloc(64 * (size_t)nb);
+    memcpy(s->dyn_params_dev, s->dyn_params, 64 * (size_t)nb);
+    s->dyn_tris = (float *)calloc(16 * (size_t)nt, 4);
+    s->dyn_skinned = (float *)calloc(3 * (size_t)nv, 4);
+    /* the texture (zero until the first bake), the infos, identity transforms appended */
+    const uint32_t have = s->sdf_nb;
+    float *xf = (float *)calloc(32 * (size_t)nb, 4);
+    for (uint32_t b = 0; b < nb; b++) {
+        if (b < have) memcpy(xf + 32 * (size_t)b, s->sdf_xf + 32 * (size_t)b, 128);
+        else for (int m = 0; m < 2; m++) for (int k = 0; k < 4; k++) xf[32 * (size_t)b + 16 * m + 5 * k] = 1.0f;
+    }
+    uint32_t *zero = (uint32_t *)calloc((size_t)w * h * d, 4);
+    orc_sim_set_sdf(s, w, h, d, zero, infos, nb);
+    memcpy(s->sdf_xf, xf, 128 * (size_t)nb);
+    free(zero); free(xf);
+    s->dyn_nb = nb; s->dyn_nv = nv; s->dyn_nt = nt; s->dyn_res = res;
+    s->dyn_ready = 1; s->dyn_enabled = 1; s->dyn_baked = 0; s->dyn_dirty_set = 0; s->dyn_ndirty = 0;
+    s->dyn_interval = interval ? interval : 1;
+    return 0;
+}
+void orc_sim_set_dynamic_dirty(orc_sim *s, const uint32_t *bones, uint32_t n, int has_list) {
+    free(s->dyn_dirty);
+    s->dyn_dirty = NULL;
+    s->dyn_dirty_set = has_list != 0;
+    s->dyn_ndirty = has_list ? n : 0;
+    if (s->dyn_ndirty) { s->dyn_dirty = (uint32_t *)malloc(4 * (size_t)n); memcpy(s->dyn_dirty, bones, 4 * (size_t)n); }
+}
+void orc_sim_set_dynamic_enabled(orc_sim *s, int on) { s->dyn_enabled = on && s->dyn_ready; }
+void orc_sim_set_dynamic_interval(orc_sim *s, uint32_t k) { s->dyn_interval = k ? k : 1; }
+/* the skinned positions (nv x 3) and bake triangles (nt x 16 floats) of the last update */
+void orc_sim_dynamic_arrays(const orc_sim *s, float *skinned, float *tris) {
+    if (skinned) memcpy(skinned, s->dyn_skinned, 12 * (size_t)s->dyn_nv);
+    if (tris) memcpy(tris, s->dyn_tris, 64 * (size_t)s->dyn_nt);
+}
+void orc_sim_sdf_arrays(const orc_sim *s, uint32_t *tex, float *info) {
+    if (tex && s->sdf_nb) memcpy(tex, s->sdf_tex, 4 * (size_t)s->sdf_w * s->sdf_h * s->sdf_d);
+    if (info && s->sdf_nb) memcpy(info, s->sdf_info, 80 * (size_t)s->sdf_nb);
+}
+
+/* (m * (x, y, z, 1)).r for a matrix stored as WGSL reads it (column c at m[4c .. 4c+3]) */
+static void wgsl_apply(const float *m, const real *v, real *o) {
+    for (int r = 0; r < 3; r++) o[r] = (real)m[r] * v[0] + (real)m[4 + r] * v[1] + (real)m[8 + r] * v[2] + (real)m[12 + r];
+}
+
+/* dispatch_dynamic_bone_sdf_update (simulation/mod.rs:2320-2457): at the start of a step frame */
+static void dynamic_sdf_update(orc_sim *s) {
+    if (!s->dyn_enabled) return;
+    s->dyn_counter += 1u;
+    if ((s->dyn_counter - 1u) % s->dyn_interval != 0u) return;
+    const uint32_t nb = s->dyn_nb;
+    if (s->sdf_nb < nb) return;
+    /* skin matrices S = W B in the reference's host f32 arithmetic, laid out for WGSL (transposed) */
+    float *skin = (float *)malloc(64 * (size_t)nb);
+    for (uint32_t b = 0; b < nb; b++) {
+        const float *W = s->sdf_xf + 32 * (size_t)b, *B = s->dyn_bind + 16 * (size_t)b;
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) {  /* w_row[i][k] = W[4k + i]; s_row[i][j]; stored at [4j + i] */
+                const float v = W[i] * B[j] + W[4 + i] * B[4 + j] + W[8 + i] * B[8 + j] + W[12 + i] * B[12 + j];
+                skin[16 * (size_t)b + 4 * j + i] = v;
+            }
+    }
+    /* skinning.wgsl:17-43 */
+    for (uint32_t i = 0; i < s->dyn_nv; i++) {
+        const real pos[3] = {s->dyn_rest[3 * i], s->dyn_rest[3 * i + 1], s->dyn_rest[3 * i + 2]};
+        real p[3] = {0, 0, 0}, ws = 0;
+        for (int k = 0; k < 4; k++) {
+            const real wk = s->dyn_w[4 * i + k];
+            if (wk > 0) {
+                real t[3];
+                wgsl_apply(skin + 16 * (size_t)s->dyn_bi[4 * i + k], pos, t);
+                for (int c = 0; c < 3; c++) p[c] = p[c] + wk * t[c];
+                ws = ws + wk;
+            }
+        }
+        if (ws <= (real)1e-6f) for (int c = 0; c < 3; c++) p[c] = pos[c];
+        for (int c = 0; c < 3; c++) s->dyn_skinned[3 * i + c] = (float)p[c];
+    }
+    /* prep_bone_triangles.wgsl:31-67 */
+    for (uint32_t t = 0; t < s->dyn_nt; t++) {
+        const uint32_t *q = s->dyn_src + 4 * (size_t)t;
+        const float *inv = s->sdf_xf + 32 * (size_t)q[3] + 16;
+        real P[3][3];
+        for (int c = 0; c < 3; c++) {
+            const real v[3] = {s->dyn_skinned[3 * q[c]], s->dyn_skinned[3 * q[c] + 1], s->dyn_skinned[3 * q[c] + 2]};
+            wgsl_apply(inv, v, P[c]);
+        }
+        const real e1[3] = {P[1][0] - P[0][0], P[1][1] - P[0][1], P[1][2] - P[0][2]};
+        const real e2[3] = {P[2][0] - P[0][0], P[2][1] - P[0][1], P[2][2] - P[0][2]};
+        const real cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const real ls = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2];
+        real n[3] = {0, 1, 0};
+        if (ls > (real)1e-12f) { const real il = 1 / (real)sqrt((double)ls); for (int c = 0; c < 3; c++) n[c] = cr[c] * il; }
+        float *o = s->dyn_tris + 16 * (size_t)t;
+        for (int c = 0; c < 3; c++) {
+            for (int k = 0; k < 3; k++) o[4 * c + k] = (float)P[c][k];
+            o[4 * c + 3] = s->dyn_tw[3 * t + c];
+        }
+        for (int k = 0; k < 3; k++) o[12 + k] = (float)n[k];
+        o[15] = (float)s->dyn_bmask[t];
+    }
+    free(skin);
+    uint32_t count = nb;
+    if (s->dyn_baked && s->dyn_dirty_set) {
+        if (!s->dyn_ndirty) return;
+        count = s->dyn_ndirty;
+        uint32_t j = 0;
+        for (uint32_t k = 0; k < s->dyn_ndirty; k++)
+            if (s->dyn_dirty[k] < nb && j < nb) memcpy(s->dyn_params_dev + 16 * (size_t)j++, s->dyn_params + 16 * (size_t)s->dyn_dirty[k], 64);
+    } else {
+        memcpy(s->dyn_params_dev, s->dyn_params, 64 * (size_t)nb);
+    }
+    orc_bone_bake_tiles(s->dyn_params_dev, count < nb ? count : nb, s->dyn_tris, s->sdf_tex);
+    s->dyn_baked = 1;
 }
 
 typedef struct { uint32_t code, idx; } tkey;
@@ -3395,6 +3646,7 @@ static void encode_steps(orc_sim *s, real dt, uint32_t substeps) {
 /* ClothSimulator.step (lib.rs:1770-1786 -> dispatch.rs:610-671); returns effective substeps */
 uint32_t orc_sim_step(orc_sim *s, float dt, uint32_t substeps) {
     uint32_t S = substeps > 0 ? substeps : 20;
+    dynamic_sdf_update(s);
     encode_steps(s, (real)dt, S);
     return S;
 }

@@ -42,7 +42,9 @@ EXPORTS = [
     "tc_sim_set_warm_start", "tc_sim_warm_start_stats", "tc_sim_ws_reset_stats", "tc_sim_ws_record",
     "tc_sim_ws_num_records", "tc_sim_ws_get_record", "tc_sim_ws_last_output", "tc_sim_ws_history",
     "tc_ws_eval", "tc_sim_ws_time", "tc_ws_net_forward_noalloc",
-    "tc_sim_set_sdf_colliders", "tc_sim_update_bone_transforms", "tc_bake_mesh_sdf", "tc_sdf_result_info",
+    "tc_sim_set_sdf_colliders", "tc_sim_update_bone_transforms", "tc_sim_clear_bone_sdf_colliders",
+    "tc_sim_setup_dynamic_bone_sdf", "tc_sim_set_dynamic_bone_sdf_dirty_bones", "tc_sim_set_dynamic_bone_sdf_enabled",
+    "tc_sim_set_dynamic_bone_sdf_update_interval", "tc_sim_dynamic_bone_sdf_stats", "tc_bake_mesh_sdf", "tc_sdf_result_info",
     "tc_sdf_result_timing", "tc_sdf_result_texels", "tc_sdf_result_atlas", "tc_sdf_result_nearest", "tc_sdf_result_free", "tc_close_open_boundary_loops",
     "tc_bake_bone_sdf", "tc_bone_sdf_result_info", "tc_bone_sdf_result_arrays", "tc_bone_sdf_result_free",
     "tc_sewing_params_default", "tc_sim_create_sewn", "tc_sim_set_sewing_stiffness", "tc_sim_set_sewing_shrink_speed",
@@ -208,6 +210,13 @@ def load() -> C.CDLL:
         "tc_sim_ws_time": (_i, [_vp, _f32, _u32, C.POINTER(C.c_double)]),
         "tc_sim_set_sdf_colliders": (_i, [_vp, _u32, _u32, _u32, _vp, C.c_size_t, _vp, _u32]),
         "tc_sim_update_bone_transforms": (_i, [_vp, _vp, _vp, _u32]),
+        "tc_sim_clear_bone_sdf_colliders": (_i, [_vp]),
+        "tc_sim_setup_dynamic_bone_sdf": (_i, [_vp, _u32, _u32, _u32, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp,
+                                               _u32, _vp, _u32, _u32]),
+        "tc_sim_set_dynamic_bone_sdf_dirty_bones": (_i, [_vp, _vp, _u32, _i]),
+        "tc_sim_set_dynamic_bone_sdf_enabled": (_i, [_vp, _i]),
+        "tc_sim_set_dynamic_bone_sdf_update_interval": (_i, [_vp, _u32]),
+        "tc_sim_dynamic_bone_sdf_stats": (_i, [_vp, _vp]),
         "tc_bake_mesh_sdf": (_i, [_vp, _u32, _vp, _u32, _f32, _f32, _f32, _f32, _f32, _i, _i, C.POINTER(_vp)]),
         "tc_sdf_result_info": (_i, [_vp, _vp, _vp, C.POINTER(_f32)]),
         "tc_sdf_result_timing": (_i, [_vp, _vp, _vp]),

@@ -16,7 +16,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libtcloth_hip.so"
 ARCH = os.environ.get("TC_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["tc_sim.hip", "tc_xpbd.hip", "tc_selfcol.hip", "tc_fused.hip", "tc_warmstart.hip", "tc_ws_api.hip", "tc_sdf_bake.hip", "tc_frames.hip", "tc_pins.hip", "tc_jacobi.hip", "tc_edge_collision.hip", "tc_wrinkle.hip", "tc_bone_sdf.hip", "tc_sewing.hip"]
+HIP_SOURCES = ["tc_sim.hip", "tc_xpbd.hip", "tc_selfcol.hip", "tc_fused.hip", "tc_warmstart.hip", "tc_ws_api.hip", "tc_sdf_bake.hip", "tc_frames.hip", "tc_pins.hip", "tc_jacobi.hip", "tc_edge_collision.hip", "tc_wrinkle.hip", "tc_bone_sdf.hip", "tc_sewing.hip", "tc_dyn_sdf.hip"]
 CXX_SOURCES = ["tc_mesh.cpp", "tc_tiles.cpp", "tc_batch.cpp"]
 
 # Strict IEEE fp32 (no FMA contraction) so results follow the reference's

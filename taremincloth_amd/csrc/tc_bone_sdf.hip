@@ -26,19 +26,6 @@
 namespace tc {
 namespace bone {
 
-struct BakeTri {  // GpuBakeTriangle: corners with their bone weights, face normal, open-edge mask
-    float p0[3], w0, p1[3], w1, p2[3], w2, n[3], bmask;
-};
-static_assert(sizeof(BakeTri) == 64, "GpuBakeTriangle is 64 bytes");
-struct Params {  // GpuBakeParams
-    float lmin[3];
-    uint32_t tri_start;
-    float lmax[3];
-    uint32_t tri_count;
-    uint32_t col, row, layer, res, tw, th, td, pitch;
-};
-static_assert(sizeof(Params) == 64, "GpuBakeParams is 64 bytes");
-
 __device__ __forceinline__ f3 v3(const float *p) { return mk3(p[0], p[1], p[2]); }
 __device__ __forceinline__ f3 normalize3(f3 v) { return v / sqrtf(dot(v, v)); }
 
@@ -138,6 +125,14 @@ __global__ void __launch_bounds__(64) k_bone_bake(const Params *__restrict__ par
     __builtin_memcpy(&bd, &hd, 2);
     __builtin_memcpy(&ba, &ha, 2);
     out[flat] = uint32_t(bd) | (uint32_t(ba) << 16);
+}
+
+hipError_t launch_bone_bake(hipStream_t st, const Params *params, uint32_t nparams, uint32_t res, const BakeTri *tris,
+                            uint32_t *out) {
+    if (nparams == 0 || res == 0) return hipSuccess;
+    const uint32_t bpa = (res + 3u) / 4u;
+    hipLaunchKernelGGL(k_bone_bake, dim3(nparams * bpa * bpa * bpa), dim3(64), 0, st, params, tris, out, bpa);
+    return hipGetLastError();
 }
 
 // compute_3d_atlas_layout (sdf_baker.rs:137-163): (cols, rows, layers, res, width, height, depth)
@@ -319,11 +314,8 @@ int tc_bake_bone_sdf(const float *verts, uint32_t nv, const int32_t *tris, uint3
     if (e == hipSuccess) e = hipEventCreate(&ev0);
     if (e == hipSuccess) e = hipEventCreate(&ev1);
     if (e == hipSuccess) {
-        const uint32_t bpa = uint32_t((res + 3) / 4);
-        const uint32_t blocks = uint32_t(params.size()) * bpa * bpa * bpa;
         e = hipEventRecord(ev0, nullptr);
-        if (e == hipSuccess) hipLaunchKernelGGL(k_bone_bake, dim3(blocks), dim3(64), 0, nullptr, dp, dt, dout, bpa);
-        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = launch_bone_bake(nullptr, dp, uint32_t(params.size()), uint32_t(res), dt, dout);
         if (e == hipSuccess) e = hipEventRecord(ev1, nullptr);
         if (e == hipSuccess) e = hipEventSynchronize(ev1);
         if (e == hipSuccess) (void)hipEventElapsedTime(&r->kernel_ms, ev0, ev1);

@@ -227,6 +227,8 @@ int policy_frame(tc_sim *s, float dt, uint32_t requested, uint32_t *substeps_out
     const uint32_t S = effective_substeps(s, dt, requested);
     effective_iterations(s, S);
     if (substeps_out) *substeps_out = S;
+    const int r = dyn_sdf_frame(s);
+    if (r != TC_OK) return r;
     return tc_run_frame(s, dt, S);
 }
 

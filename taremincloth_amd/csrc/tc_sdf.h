@@ -107,4 +107,29 @@ struct ResolveArgs {
 hipError_t launch_bake_resolve(hipStream_t st, const ResolveArgs &a);
 
 }  // namespace sdf
+
+namespace bone {
+
+// GpuBakeTriangle (sdf_baker.rs:28-38): corners in bone space with their bone weights, the unit
+// face normal and the open-edge mask (bit 0: p0p1, bit 1: p1p2, bit 2: p2p0) stored as a float.
+struct BakeTri {
+    float p0[3], w0, p1[3], w1, p2[3], w2, n[3], bmask;
+};
+static_assert(sizeof(BakeTri) == 64, "GpuBakeTriangle is 64 bytes");
+// GpuBakeParams (sdf_baker.rs:12-25): one bone's local box, triangle range and atlas tile.
+struct Params {
+    float lmin[3];
+    uint32_t tri_start;
+    float lmax[3];
+    uint32_t tri_count;
+    uint32_t col, row, layer, res, tw, th, td, pitch;
+};
+static_assert(sizeof(Params) == 64, "GpuBakeParams is 64 bytes");
+
+// bake_sdf.wgsl over `nparams` bones (all of the same resolution `res`): each bone's res^3 tile of
+// the atlas `out` (row pitch `pitch` texels, or the atlas width when 0).
+hipError_t launch_bone_bake(hipStream_t st, const Params *params, uint32_t nparams, uint32_t res,
+                            const BakeTri *tris, uint32_t *out);
+
+}  // namespace bone
 }  // namespace tc

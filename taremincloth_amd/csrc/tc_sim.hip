@@ -1224,9 +1224,13 @@ int tc_sim_set_sdf_colliders(tc_sim *s, uint32_t w, uint32_t h, uint32_t d, cons
                                       std::to_string(nvox * 4) + " needed (Rg16Float, 4 bytes per texel)");
     TC_HIP(hipStreamSynchronize(s->stream));
     s->sdf_nb = 0;
+    s->sdf_info_h.clear();
+    s->sdf_xf_h.clear();
+    s->dyn.clear();  // a static atlas replaces the dynamic one
     if (nb > 0) {
         TC_HIP(s->sdf_tex.alloc(nvox));
         TC_HIP(hipMemcpy(s->sdf_tex.p, texels, nvox * 4, hipMemcpyHostToDevice));
+        s->sdf_info_h.assign(infos, infos + size_t(nb) * 20);
         TC_HIP(s->sdf_info.alloc(size_t(nb) * 20));
         TC_HIP(hipMemcpy(s->sdf_info.p, infos, size_t(nb) * 20 * 4, hipMemcpyHostToDevice));
         // identity transforms until update_bone_transforms (simulation/mod.rs:1700-1719)
@@ -1236,9 +1240,21 @@ int tc_sim_set_sdf_colliders(tc_sim *s, uint32_t w, uint32_t h, uint32_t d, cons
                 for (int k = 0; k < 4; ++k) xf[size_t(b) * 32 + m * 16 + k * 5] = 1.0f;
         TC_HIP(s->sdf_xf.alloc(xf.size()));
         TC_HIP(hipMemcpy(s->sdf_xf.p, xf.data(), 4 * xf.size(), hipMemcpyHostToDevice));
+        s->sdf_xf_h = std::move(xf);
         s->sdf_w = w; s->sdf_h = h; s->sdf_d = d;
         s->sdf_nb = nb;
     }
+    s->touch();
+    return TC_OK;
+}
+
+int tc_sim_clear_bone_sdf_colliders(tc_sim *s) {
+    if (!s) return fail(TC_ERR_VALUE, "null simulator");
+    TC_HIP(hipStreamSynchronize(s->stream));
+    s->sdf_nb = 0;
+    s->sdf_info_h.clear();
+    s->sdf_xf_h.clear();
+    s->dyn.clear();
     s->touch();
     return TC_OK;
 }
@@ -1254,8 +1270,32 @@ int tc_sim_update_bone_transforms(tc_sim *s, const float *world, const float *in
         std::memcpy(&xf[size_t(b) * 32], world + size_t(b) * 16, 64);
         std::memcpy(&xf[size_t(b) * 32 + 16], inv_world + size_t(b) * 16, 64);
     }
+    // each collider's bounding sphere in world space goes into the infos' spare slots 3, 7, 11
+    // (centre) and 15 (radius), where the resolve's prefilter reads it (simulation/mod.rs:1886-1925):
+    // the local box's centre through the world matrix (the matrix as given, its rows taken as the
+    // columns), radius = half the box diagonal x the largest column length + 0.1
+    for (uint32_t b = 0; b < n; ++b) {
+        float *I = &s->sdf_info_h[size_t(b) * 20];
+        const float *m = world + size_t(b) * 16;
+        const float c0 = (I[0] + I[4]) * 0.5f, c1 = (I[1] + I[5]) * 0.5f, c2 = (I[2] + I[6]) * 0.5f;
+        const float ex = I[4] - I[0], ey = I[5] - I[1], ez = I[6] - I[2];
+        const float r_local = 0.5f * std::sqrt(ex * ex + ey * ey + ez * ez);
+        const float cw0 = m[0] * c0 + m[4] * c1 + m[8] * c2 + m[12];
+        const float cw1 = m[1] * c0 + m[5] * c1 + m[9] * c2 + m[13];
+        const float cw2 = m[2] * c0 + m[6] * c1 + m[10] * c2 + m[14];
+        const float sx = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        const float sy = std::sqrt(m[4] * m[4] + m[5] * m[5] + m[6] * m[6]);
+        const float sz = std::sqrt(m[8] * m[8] + m[9] * m[9] + m[10] * m[10]);
+        const float max_scale = std::fmax(std::fmax(sx, sy), sz);  // f32::max
+        I[3] = cw0; I[7] = cw1; I[11] = cw2;
+        I[15] = r_local * max_scale + 0.1f;
+    }
     TC_HIP(hipStreamSynchronize(s->stream));
-    if (n) TC_HIP(hipMemcpy(s->sdf_xf.p, xf.data(), 4 * xf.size(), hipMemcpyHostToDevice));
+    if (n) {
+        TC_HIP(hipMemcpy(s->sdf_xf.p, xf.data(), 4 * xf.size(), hipMemcpyHostToDevice));
+        TC_HIP(hipMemcpy(s->sdf_info.p, s->sdf_info_h.data(), 4 * s->sdf_info_h.size(), hipMemcpyHostToDevice));
+    }
+    s->sdf_xf_h = std::move(xf);
     return TC_OK;
 }
 
@@ -1691,6 +1731,21 @@ int tc_sim_get_array(tc_sim *s, int which, void *out, size_t cap, size_t *count)
         for (uint32_t i = 0; i < m.nv; ++i) o[i] = tmp[i].w;
         return TC_OK;
     }
+    case 23: return dev(s->sdf_tex.p, s->sdf_nb ? size_t(s->sdf_w) * s->sdf_h * s->sdf_d : 0, 4);
+    case 24: return host(s->sdf_info_h.data(), s->sdf_info_h.size(), 4);
+    case 25: {  // the dynamic bone SDF's skinned positions (xyz)
+        const size_t n = s->dyn.ready ? size_t(s->dyn.nv) : 0;
+        if (count) *count = 3 * n;
+        if (!out) return TC_OK;
+        if (cap < 3 * n) return fail(TC_ERR_VALUE, "capacity too small");
+        std::vector<float4> tmp(n);
+        TC_HIP(hipStreamSynchronize(s->stream));
+        if (n) TC_HIP(hipMemcpy(tmp.data(), s->dyn.skinned.p, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        float *o = static_cast<float *>(out);
+        for (size_t i = 0; i < n; ++i) { o[3 * i] = tmp[i].x; o[3 * i + 1] = tmp[i].y; o[3 * i + 2] = tmp[i].z; }
+        return TC_OK;
+    }
+    case 26: return dev(s->dyn.tris.p, s->dyn.ready ? size_t(s->dyn.nt) * 16 : 0, 4);
     default: return fail(TC_ERR_VALUE, "unknown array id " + std::to_string(which));
     }
 }

@@ -12,6 +12,7 @@
 #include "../../include/tc_sim.h"
 #include "tc_kernels.h"
 #include "tc_mesh.h"
+#include "tc_sdf.h"
 #include "tc_tiles.h"
 #include "tc_ws.h"
 
@@ -208,10 +209,41 @@ struct tc_sim {
     DevBuf<uint32_t> col_order;        // vertices in Morton order of their rest positions (collider lanes)
     int single_sided_recovery = 1;
 
-    // SDF colliders (set_bone_sdf_colliders)
+    // SDF colliders (set_bone_sdf_colliders); host copies of the infos and of the (world, inverse
+    // world) transforms, 32 floats per collider, for update_bone_transforms and the dynamic bake
     DevBuf<uint32_t> sdf_tex;
     DevBuf<float> sdf_info, sdf_xf;
+    std::vector<float> sdf_info_h, sdf_xf_h;
     uint32_t sdf_w = 0, sdf_h = 0, sdf_d = 0, sdf_nb = 0;
+    // dynamic bone SDF (setup_dynamic_bone_sdf, simulation/mod.rs:1998-2312): every update_interval-th
+    // frame the rest mesh is skinned, its triangles moved into their bones' frames and the bones'
+    // atlas tiles re-baked straight into sdf_tex (tc_dyn_sdf.hip)
+    struct DynSdf {
+        bool ready = false;     // dynamic_sdf_setup is Some
+        bool enabled = false;   // enable_dynamic_bone_sdf
+        bool baked = false;     // dynamic_sdf_initial_baked
+        uint32_t interval = 1, counter = 0;
+        uint32_t res = 0, nbones = 0, nv = 0, nt = 0;
+        std::vector<float> bind_inv;                   // 16 per bone, row-major
+        std::vector<tc::bone::Params> params;          // bake_params, one per bone
+        std::vector<tc::bone::Params> params_dev;      // what the device's parameter buffer holds
+        bool dirty_set = false;                        // active_dirty_bone_indices is Some
+        std::vector<uint32_t> dirty;
+        DevBuf<float4> rest;                           // 4 per vertex: position, normal, indices (bits), weights
+        DevBuf<float4> skinned;                        // skinned_positions
+        DevBuf<uint4> src;                             // 2 per triangle: (i0, i1, i2, bone), (w0, w1, w2, mask) bits
+        DevBuf<tc::bone::BakeTri> tris;                // bone-space bake triangles
+        DevBuf<tc::bone::Params> dparams;
+        DevBuf<float> mats;                            // skin matrices, then inverse world matrices (16 per bone)
+        float *mats_host = nullptr;                    // pinned staging of `mats`
+        hipEvent_t mats_ev = nullptr;                  // the last upload from mats_host
+        uint32_t bakes = 0, skips = 0;                 // counters: tile bakes launched, frames skipped
+        void clear() {
+            ready = enabled = baked = dirty_set = false;
+            counter = 0;
+            dirty.clear();
+        }
+    } dyn;
     tc::SdfArgs sdf_args() const {
         tc::SdfArgs a;
         a.tex = sdf_tex.p; a.w = sdf_w; a.h = sdf_h; a.d = sdf_d;
@@ -360,6 +392,8 @@ struct tc_sim {
             if (b->ev) (void)hipEventDestroy(b->ev);
         }
         if (ring_host) (void)hipHostFree(ring_host);
+        if (dyn.mats_ev) (void)hipEventDestroy(dyn.mats_ev);
+        if (dyn.mats_host) (void)hipHostFree(dyn.mats_host);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void touch() { ++epoch; }
@@ -444,6 +478,10 @@ float effective_inv_mass(const tc_sim *s, uint32_t i);
 
 // One frame of `substeps` substeps on the simulator's stream (graph replay when it applies).
 int tc_run_frame(tc_sim *s, float dt, uint32_t substeps);
+// The dynamic bone SDF's update at the start of a step / step_async / step_buffered frame
+// (dispatch_dynamic_bone_sdf_update, simulation/mod.rs:2320-2457), enqueued on the simulator's
+// stream ahead of the frame (tc_dyn_sdf.hip).
+int dyn_sdf_frame(tc_sim *s);
 // The policy's mesh constants (simulation/mod.rs:380-420), set once at creation (tc_frames.hip).
 void frames_init(tc_sim *s);
 

@@ -849,7 +849,13 @@ class ClothSimulator:
         _lib.check(self._L.tc_sim_save_profile_trace(self._h, str(path).encode()))
 
     # --- introspection used by parity checks ---------------------------------------------------
-    _FLOAT_ARRAYS = {8, 9, 13, 14, 15, 16, 22}
+    _FLOAT_ARRAYS = {8, 9, 13, 14, 15, 16, 22, 24, 25, 26}
+
+    def sdf_collider_arrays(self) -> dict[str, np.ndarray]:
+        """The SDF colliders as the resolve reads them: texels (u32) and infos (N, 20), and the dynamic
+        bone SDF's last skinned positions (V, 3) and bone-space bake triangles (T, 16)."""
+        return {"texels": self._array(23), "infos": self._array(24).reshape(-1, 20),
+                "skinned": self._array(25).reshape(-1, 3), "bake_triangles": self._array(26).reshape(-1, 16)}
 
     def device_inv_masses(self) -> np.ndarray:
         """The inverse masses the solver reads (base values scaled by the pins)."""
@@ -920,13 +926,82 @@ class ClothSimulator:
         self._sdf_texture = tex
 
     def update_bone_transforms(self, world_matrices, inv_world_matrices, dirty_bone_indices=None):
-        """Per-collider world and inverse-world 4x4 matrices (lib.rs:2287-2330)."""
+        """Per-collider world and inverse-world 4x4 matrices (lib.rs:2287-2330); also sets the dynamic
+        bone SDF's dirty bones (None: every bone), as the reference does."""
         w = np.ascontiguousarray(world_matrices, np.float32)
         iw = np.ascontiguousarray(inv_world_matrices, np.float32)
         if w.ndim != 3 or w.shape[1:] != (4, 4) or iw.shape != w.shape:
             raise ValueError("world_matrices and inv_world_matrices must both have shape (N, 4, 4)")
         _lib.check(self._L.tc_sim_update_bone_transforms(self._h, w.ctypes.data_as(C.c_void_p),
                                                          iw.ctypes.data_as(C.c_void_p), len(w)))
+        self.set_dynamic_bone_sdf_dirty_bones(dirty_bone_indices)
+
+    def clear_bone_sdf_colliders(self):
+        """Removes the SDF colliders and the dynamic bone SDF (lib.rs:2342-2345)."""
+        _lib.check(self._L.tc_sim_clear_bone_sdf_colliders(self._h))
+        self._sdf_texture = None
+
+    def setup_dynamic_bone_sdf(self, width, height, depth, res, bone_infos, rest_verts, bone_indices, bone_weights,
+                               tri_sources, tri_weights, bone_bind_inv_matrices, update_interval=1):
+        """Skinned collider mesh whose bone-SDF atlas is re-baked on the device every update_interval
+        frames (lib.rs:2347-2588).  bone_infos (N_bones, >=20); rest_verts (V, >=3) (columns 3-5, the
+        rest normals, are not used by any pass); bone_indices / bone_weights (V, >=4); tri_sources
+        (T, >=4) = (i0, i1, i2, bone), each bone's triangles contiguous; tri_weights (T, >=3);
+        bone_bind_inv_matrices (>=N_bones, 4, 4) row-major bind matrices (world -> bone)."""
+        if int(res) == 0 or int(width) == 0 or int(height) == 0 or int(depth) == 0:
+            raise ValueError("width, height, depth and res must all be 1 or more")
+        infos = np.asarray(bone_infos, dtype=np.float32)
+        if infos.ndim != 2 or (infos.shape[0] and infos.shape[1] < 20):
+            raise ValueError("each bone_infos row needs at least 20 elements")
+        rv = np.asarray(rest_verts, dtype=np.float32)
+        bi = np.asarray(bone_indices, dtype=np.uint32)
+        bw = np.asarray(bone_weights, dtype=np.float32)
+        if rv.ndim != 2 or bi.ndim != 2 or bw.ndim != 2 or rv.shape[1] < 3 or bi.shape[1] < 4 or bw.shape[1] < 4:
+            raise ValueError("rest_verts needs >= 3 columns, bone_indices and bone_weights >= 4")
+        if bi.shape[0] != rv.shape[0] or bw.shape[0] != rv.shape[0]:
+            raise ValueError("rest_verts, bone_indices and bone_weights must have the same number of rows")
+        ts = np.asarray(tri_sources, dtype=np.uint32)
+        tw = np.asarray(tri_weights, dtype=np.float32)
+        if ts.ndim != 2 or tw.ndim != 2 or ts.shape[0] != tw.shape[0]:
+            raise ValueError("tri_sources and tri_weights must have the same number of rows")
+        if ts.shape[0] and (ts.shape[1] < 4 or tw.shape[1] < 3):
+            raise ValueError("tri_sources needs 4 columns (i0, i1, i2, bone) and tri_weights 3 (w0, w1, w2)")
+        inv = np.asarray(bone_bind_inv_matrices, dtype=np.float32)
+        n_bones = infos.shape[0]
+        if inv.ndim != 3 or inv.shape[0] < n_bones or inv.shape[1:] != (4, 4):
+            raise ValueError("bone_bind_inv_matrices must have shape (N_bones, 4, 4)")
+        infos = np.ascontiguousarray(infos[:, :20]) if n_bones else np.zeros((0, 20), np.float32)
+        rv = np.ascontiguousarray(rv[:, :3])
+        bi = np.ascontiguousarray(bi[:, :4])
+        bw = np.ascontiguousarray(bw[:, :4])
+        ts = np.ascontiguousarray(ts[:, :4]) if len(ts) else np.zeros((0, 4), np.uint32)
+        tw = np.ascontiguousarray(tw[:, :3]) if len(tw) else np.zeros((0, 3), np.float32)
+        inv = np.ascontiguousarray(inv)
+        vp = C.c_void_p
+        _lib.check(self._L.tc_sim_setup_dynamic_bone_sdf(
+            self._h, int(width), int(height), int(depth), int(res), infos.ctypes.data_as(vp), n_bones,
+            rv.ctypes.data_as(vp), len(rv), bi.ctypes.data_as(vp), bw.ctypes.data_as(vp), ts.ctypes.data_as(vp),
+            tw.ctypes.data_as(vp), len(ts), inv.ctypes.data_as(vp), inv.shape[0], int(update_interval)))
+
+    def set_dynamic_bone_sdf_dirty_bones(self, dirty_bone_indices=None):
+        """Bones whose tiles the next updates re-bake (None: every bone; empty: none) (lib.rs:2332-2340)."""
+        if dirty_bone_indices is None:
+            _lib.check(self._L.tc_sim_set_dynamic_bone_sdf_dirty_bones(self._h, None, 0, 0))
+            return
+        d = np.ascontiguousarray(np.asarray(dirty_bone_indices, dtype=np.uint32).reshape(-1))
+        _lib.check(self._L.tc_sim_set_dynamic_bone_sdf_dirty_bones(self._h, d.ctypes.data_as(C.c_void_p), len(d), 1))
+
+    def set_dynamic_bone_sdf_enabled(self, enabled):
+        _lib.check(self._L.tc_sim_set_dynamic_bone_sdf_enabled(self._h, 1 if enabled else 0))
+
+    def set_dynamic_bone_sdf_update_interval(self, interval):
+        _lib.check(self._L.tc_sim_set_dynamic_bone_sdf_update_interval(self._h, int(interval)))
+
+    def dynamic_bone_sdf_stats(self) -> dict:
+        """Diagnostics (no reference counterpart): enabled, baked, tiles baked, frame counter, skipped."""
+        out = (C.c_uint32 * 5)()
+        _lib.check(self._L.tc_sim_dynamic_bone_sdf_stats(self._h, out))
+        return dict(zip(("enabled", "baked", "tiles_baked", "frame_counter", "skipped"), list(out)))
 
     # --- learned warm start (no reference counterpart; see taremincloth_amd.warmstart) -------
     def set_warm_start(self, net=None, mode=0, iterations=8, fallback_iterations=56, residual_threshold=1e-4,
