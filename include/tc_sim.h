@@ -283,9 +283,10 @@ int tc_sim_set_damping(tc_sim *sim, float damping);
 int tc_sim_set_damping_all(tc_sim *sim, float tension, float compression, float shear, float bending);
 int tc_sim_set_solver_iterations(tc_sim *sim, uint32_t iterations);
 int tc_sim_set_enable_self_collision(tc_sim *sim, int enable);
-/* set_self_collision_options, simulation/mod.rs:920-953. Normal untangling only acts on multi-layer
- * meshes (several islands or layer ids); it is not part of this build, so stepping a multi-layer mesh
- * with self-collision and untangling enabled returns TC_ERR_UNSUPPORTED. */
+/* set_self_collision_options, simulation/mod.rs:920-953. Normal untangling acts only on multi-layer
+ * meshes (several islands or layer ids; simulation/mod.rs:885-890): it adds the vertex-normal pass
+ * and the untangle pass (compute_normals.wgsl, self_collision_untangle.wgsl) to each substep's
+ * self-collision and exempts outer-behind-inner contacts from the V-V, V-T and E-E responses. */
 int tc_sim_set_self_collision_options(tc_sim *sim, float relief_factor, float max_displacement_ratio,
                                       int exclude_neighbors, int enable_normal_untangling, uint32_t max_iterations);
 /* set_stiffness_all, simulation/mod.rs:1106-1168 */

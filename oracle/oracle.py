@@ -82,7 +82,7 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.orc_sim_set_iterations.argtypes = [_vp, C.c_uint32]
     L.orc_sim_set_self_collision.argtypes = [_vp, C.c_int]
     L.orc_sim_set_hash_cell.argtypes = [_vp, C.c_float]
-    L.orc_sim_set_self_collision_options.argtypes = [_vp, C.c_float, C.c_float, C.c_int, C.c_uint32]
+    L.orc_sim_set_self_collision_options.argtypes = [_vp, C.c_float, C.c_float, C.c_int, C.c_int, C.c_uint32]
     L.orc_sim_set_compliances_raw.argtypes = [_vp, C.c_float, C.c_float, C.c_float]
     L.orc_sim_add_sphere.argtypes = [_vp, _f32p, C.c_float, C.c_float, C.c_float]
     L.orc_sim_add_capsule.argtypes = [_vp, _f32p, _f32p, C.c_float, C.c_float, C.c_float]
@@ -121,6 +121,7 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.orc_sim_set_sdf.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _u32p, _f32p, C.c_uint32]
     L.orc_sim_set_bone_transforms.argtypes = [_vp, _f32p, _f32p, C.c_uint32]
     L.orc_sim_clear_bone_sdf.argtypes = [_vp]
+    L.orc_sim_normals.argtypes = [_vp, _vp]
     L.orc_sim_setup_dynamic_bone_sdf.restype = C.c_int
     L.orc_sim_setup_dynamic_bone_sdf.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _vp,
                                                  C.c_uint32, _vp, C.c_uint32, _vp, _vp, _vp, _vp, C.c_uint32, _vp,
@@ -356,7 +357,8 @@ class OracleSim:
     def set_self_collision_options(self, relief_factor, max_displacement_ratio, exclude_neighbors=True,
                                    enable_normal_untangling=True, max_iterations=256):
         self._L.orc_sim_set_self_collision_options(self._h, relief_factor, max_displacement_ratio,
-                                                   1 if exclude_neighbors else 0, max_iterations)
+                                                   1 if exclude_neighbors else 0, 1 if enable_normal_untangling else 0,
+                                                   max_iterations)
 
     def set_hash_cell_size(self, cell):
         self._L.orc_sim_set_hash_cell(self._h, cell)
@@ -399,6 +401,12 @@ class OracleSim:
 
     def clear_bone_sdf_colliders(self):
         self._L.orc_sim_clear_bone_sdf(self._h)
+
+    def vertex_normals(self):
+        """compute_normals.wgsl at the current positions: (nv, 3)."""
+        out = np.zeros((len(self.state(0).reshape(-1, 3)), 3), np.float32)
+        self._L.orc_sim_normals(self._h, _ptr(out))
+        return out
 
     def setup_dynamic_bone_sdf(self, width, height, depth, res, bone_infos, rest_verts, bone_indices, bone_weights,
                                tri_sources, tri_weights, bone_bind_inv_matrices, update_interval=1):

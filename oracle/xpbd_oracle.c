@@ -1357,6 +1357,9 @@ typedef struct {
     uint32_t dyn_interval, dyn_counter, dyn_res, dyn_nb, dyn_nv, dyn_nt, dyn_ndirty;
     float *dyn_rest, *dyn_w, *dyn_tw, *dyn_bind, *dyn_tris, *dyn_skinned;
     uint32_t *dyn_bi, *dyn_src, *dyn_bmask, *dyn_params, *dyn_params_dev, *dyn_dirty;
+    /* multi-layer normal untangling: the user flag and the vertex normals of the substep's start */
+    int normal_untangling;
+    v3 *normals;
 } orc_sim;
 
 orc_sim *orc_sim_create_sewn(const float *pos, uint32_t nv, const uint32_t *edges, uint32_t ne,
@@ -1436,6 +1439,8 @@ orc_sim *orc_sim_create_sewn(const float *pos, uint32_t nv, const uint32_t *edge
     s->single_sided_recovery = 1;
     s->relief_factor = (real)0.2f; s->max_disp_ratio = (real)0.2f;
     s->enable_relief = 1; s->exclude_neighbors = 1;
+    s->normal_untangling = 1;
+    s->normals = (v3 *)calloc(nv ? nv : 1, sizeof(v3));
     return s;
 }
 
@@ -1451,7 +1456,7 @@ void orc_sim_free(orc_sim *s) {
     free(s->s_cur); free(s->s_tgt); free(s->lam_s);
     free(s->dyn_rest); free(s->dyn_w); free(s->dyn_tw); free(s->dyn_bind); free(s->dyn_tris); free(s->dyn_skinned);
     free(s->dyn_bi); free(s->dyn_src); free(s->dyn_bmask); free(s->dyn_params); free(s->dyn_params_dev);
-    free(s->dyn_dirty);
+    free(s->dyn_dirty); free(s->normals);
     free(s);
 }
 
@@ -1464,14 +1469,17 @@ void orc_sim_set_damping_all(orc_sim *s, float t, float c, float sh, float b) {
 }
 void orc_sim_set_iterations(orc_sim *s, uint32_t it) { s->iterations = it; }
 void orc_sim_set_self_collision(orc_sim *s, int on) { s->self_collision = on; }
-/* mod.rs:920-953 (multi-layer untangling is not part of this build) */
+/* mod.rs:920-953 */
 void orc_sim_set_self_collision_options(orc_sim *s, float relief, float ratio, int exclude_neighbors,
-                                        uint32_t max_iterations) {
+                                        int normal_untangling, uint32_t max_iterations) {
     s->relief_factor = relief; s->max_disp_ratio = ratio;
     s->enable_relief = relief < 0.999f;
     s->exclude_neighbors = exclude_neighbors;
+    s->normal_untangling = normal_untangling != 0;
     s->max_search_iterations = max_iterations;
 }
+/* effective_normal_untangling (simulation/mod.rs:885-890) */
+static int untangle_on(const orc_sim *s) { return s->normal_untangling && s->m->has_multiple_layers; }
 void orc_sim_set_hash_cell(orc_sim *s, float cell) { if (cell > 0.0f) s->cell_size = cell; }
 /* raw compliance override per constraint kind (no reference counterpart: the
  * reference maps stiffness >= 5000 N/m to zero compliance) */
@@ -2569,7 +2577,10 @@ static int near_folded_sq(const orc_mesh *m, uint32_t a, uint32_t b, real cur_sq
     return 0;
 }
 
-/* sc_solve: self_collision.wgsl:317-681 (single layer: untangling branches inactive) */
+/* sc_solve: self_collision.wgsl:317-681.  On a multi-layer cloth with untangling on (the branches
+ * guarded by enable_normal_untangling): each free outer vertex records its nearest inner vertex
+ * outside its two-hop ring (lines 340-386, 678-681); V-V and V-T responses are skipped where the outer
+ * vertex sits behind the inner one (408-427, 546-556, 613-627). */
 static void k_sc_solve(orc_sim *s) {
     const orc_mesh *m = s->m;
     const real EPS = (real)1e-7f;
@@ -2583,6 +2594,10 @@ static void k_sc_solve(orc_sim *s) {
         real thick_i = (real)m->thick[index];
         real llen_i = (real)m->local_len[index];
         int32_t cc[3]; cell_of(p_i, cell, cc);
+        const int ut = untangle_on(s);
+        const int track = ut && !pinned_i && m->layer[index] > 0;
+        real best_inner = (real)1e20f;
+        uint32_t best_inner_j = 0xFFFFFFFFu;
         for (int32_t cx = cc[0] - 1; cx <= cc[0] + 1; cx++)
         for (int32_t cy = cc[1] - 1; cy <= cc[1] + 1; cy++)
         for (int32_t cz = cc[2] - 1; cz <= cc[2] + 1; cz++) {
@@ -2601,6 +2616,10 @@ static void k_sc_solve(orc_sim *s) {
                 real edge_margin = rmin(llen_i, llen_j) * (real)0.06f;
                 real eff = rmax(min_dist, edge_margin);
                 real max_geom = eff + (llen_i + llen_j) * (real)1.3f;
+                if (track && m->layer[j] < m->layer[index] && dsq < best_inner) {
+                    real r = eff * (real)2.5f + llen_j * (real)1.3f;
+                    if (dsq <= r * r && !within_two_hops(m, index, j)) { best_inner = dsq; best_inner_j = j; }
+                }
                 if (!(dsq <= max_geom * max_geom)) continue;
                 real dvv_len = RSQRT(rmax(dsq, 0));
                 if (near_folded(m, index, j, dvv_len)) continue;
@@ -2609,7 +2628,23 @@ static void k_sc_solve(orc_sim *s) {
                 real eff_sq = eff * eff;
                 /* 1. V-V */
                 if (!pinned_i && (!both_dynamic || index < j)) {
-                    if (dsq < eff_sq && dsq > EPS * EPS) {
+                    int untangling = 0;
+                    if (ut && dsq < eff_sq && dsq > EPS * EPS && m->layer[index] != m->layer[j]) {
+                        v3 ni = s->normals[index], nj = s->normals[j];
+                        real nil = len3(ni), njl = len3(nj);
+                        if (m->layer[index] > m->layer[j]) {
+                            if (njl > (real)0.5f) {
+                                v3 nju = dvs(nj, njl);
+                                int allow = s->w[j] <= 0 || nil <= (real)0.5f || dot(dvs(ni, rmax(nil, EPS)), nju) > 0;
+                                untangling = allow && dot(dvv, nju) < 0;
+                            }
+                        } else if (both_dynamic && nil > (real)0.5f) {
+                            v3 niu = dvs(ni, nil);
+                            int allow = njl <= (real)0.5f || dot(dvs(nj, rmax(njl, EPS)), niu) > 0;
+                            untangling = allow && dot(neg(dvv), niu) < 0;
+                        }
+                    }
+                    if (!untangling && dsq < eff_sq && dsq > EPS * EPS) {
                         s->stats[0]++;
                         v3 n = dvs(dvv, dvv_len);
                         real pen = eff - dvv_len;
@@ -2670,6 +2705,12 @@ static void k_sc_solve(orc_sim *s) {
                         real tl = len3(tcr);
                         v3 tn = tl > (real)1e-12f ? dvs(tcr, tl) : mk(0, 1, 0);
                         real h_old_raw = dot(sub(rel_old, p_j), tn);
+                        if (ut && !pinned_i && m->layer[index] > m->layer[j] && tl > (real)1e-12f) {
+                            v3 ni = s->normals[index];
+                            real nil = len3(ni);
+                            int allow = tri_pin || nil <= (real)0.5f || dot(dvs(ni, rmax(nil, EPS)), tn) > 0;
+                            if (allow && h_old_raw < 0 && dot(move, tn) > 0) continue;
+                        }
                         real side = fabs((double)h_old_raw) > 1e-6 ? h_old_raw : dot(neg(move), tn);
                         v3 n_side = side >= 0 ? tn : neg(tn);
                         real h_old = rmax(dot(sub(rel_old, hit_pt), n_side), 0);
@@ -2713,8 +2754,20 @@ static void k_sc_solve(orc_sim *s) {
                         v3 dvt = sub(p_i, r.pt);
                         real dvsq = dot(dvt, dvt);
                         if (!pinned_i && dvsq < eff_sq && dvsq > EPS * EPS) {
-                            s->stats[2]++;
                             real dvt_len = RSQRT(dvsq);
+                            if (ut && m->layer[index] > m->layer[j]) {
+                                v3 tcr = cross(sub(p_v0, p_j), sub(p_v1, p_j));
+                                real tl = len3(tcr);
+                                if (tl > EPS) {
+                                    v3 tn = dvs(tcr, tl);
+                                    if (dot(dvt, tn) < 0) {
+                                        v3 ni = s->normals[index];
+                                        real nil = len3(ni);
+                                        if (tri_pin || nil <= (real)0.5f || dot(dvs(ni, rmax(nil, EPS)), tn) > 0) continue;
+                                    }
+                                }
+                            }
+                            s->stats[2]++;
                             v3 nvt = dvs(dvt, dvt_len);
                             real pen = eff - dvt_len;
                             if (tri_pin) {
@@ -2737,6 +2790,7 @@ static void k_sc_solve(orc_sim *s) {
                 }
             }
         }
+        if (track && best_inner_j < 0x3FFFFEu) s->acc[index].untangle = best_inner_j + 1u;
     }
 }
 
@@ -2784,6 +2838,44 @@ static real seg_volume(v3 a0o, v3 a0, v3 a1o, v3 a1, v3 b0o, v3 b0, v3 b1o, v3 b
     return dot(cross(sub(A1, A0), sub(B1, B0)), sub(B0, A0));
 }
 static int bbit(const orc_mesh *m, uint32_t e) { return (m->e_bbits[e >> 5] >> (e & 31)) & 1; }
+
+/* is_outer_edge_behind_inner (self_collision_ee_direct.wgsl:123-148); delta_ab from edge b to edge a */
+static int outer_edge_behind_inner(const orc_sim *s, uint32_t la, uint32_t lb, uint32_t a0, uint32_t a1, uint32_t b0,
+                                   uint32_t b1, int a_fixed, int b_fixed, v3 delta_ab) {
+    const real EPS = (real)1e-7f;
+    if (la == lb) return 0;
+    v3 na = add(s->normals[a0], s->normals[a1]), nb = add(s->normals[b0], s->normals[b1]);
+    real nal = len3(na), nbl = len3(nb);
+    if (la > lb) {
+        if (nbl <= (real)0.5f) return 0;
+        v3 ni = dvs(nb, nbl);
+        int allow = b_fixed || nal <= (real)0.5f || dot(dvs(na, rmax(nal, EPS)), ni) > 0;
+        return allow && dot(delta_ab, ni) < 0;
+    }
+    if (nal <= (real)0.5f) return 0;
+    v3 ni = dvs(na, nal);
+    int allow = a_fixed || nbl <= (real)0.5f || dot(dvs(nb, rmax(nbl, EPS)), ni) > 0;
+    return allow && dot(neg(delta_ab), ni) < 0;
+}
+/* the E-E CCD's untangling exemption (ee_direct.wgsl:506-524, solve_ee.wgsl:386-408) */
+static int ee_ccd_untangling(const orc_sim *s, uint32_t la, uint32_t lb, uint32_t u0, uint32_t j, int a_fixed,
+                             int b_fixed, v3 a0o, v3 a1o, v3 b0o, v3 b1o, v3 a0, v3 a1, v3 b0, v3 b1) {
+    if (la == lb) return 0;
+    v3 d0 = mul(sub(add(a0o, a1o), add(b0o, b1o)), (real)0.5f);
+    v3 d1 = mul(sub(add(a0, a1), add(b0, b1)), (real)0.5f);
+    v3 mv = sub(d1, d0);
+    if (la > lb && !a_fixed) {
+        v3 ni = s->normals[j], no = s->normals[u0];
+        int allow = b_fixed || dot(no, no) <= (real)0.25f;
+        return allow && dot(ni, ni) > (real)0.25f && dot(d0, ni) < 0 && dot(mv, ni) > 0;
+    }
+    if (lb > la && !b_fixed) {
+        v3 ni = s->normals[u0], no = s->normals[j];
+        int allow = a_fixed || dot(no, no) <= (real)0.25f;
+        return allow && dot(ni, ni) > (real)0.25f && dot(neg(d0), ni) < 0 && dot(neg(mv), ni) > 0;
+    }
+    return 0;
+}
 
 static void k_sc_solve_ee(orc_sim *s) {
     const orc_mesh *m = s->m;
@@ -2899,6 +2991,10 @@ static void k_sc_solve_ee(orc_sim *s) {
                         if (near_folded_sq(m, u0, j, dot(t0, t0)) || near_folded_sq(m, u0, vj, dot(t1, t1)) ||
                             near_folded_sq(m, u1, j, dot(t2, t2)) || near_folded_sq(m, u1, vj, dot(t3, t3))) continue;
                     }
+                    if (untangle_on(s) &&
+                        ee_ccd_untangling(s, m->layer[u0], m->layer[j], u0, j, s->w[u0] == 0 && s->w[u1] == 0,
+                                          s->w[j] == 0 && s->w[vj] == 0, p_u0o, p_u1o, p_jo, p_vjo, p_u0, p_u1, p_j, p_vj))
+                        continue;
                     st2 st0 = closest_segs(p_u0o, p_u1o, p_jo, p_vjo);
                     v3 delta0 = sub(vmix(p_u0o, p_u1o, st0.s), vmix(p_jo, p_vjo, st0.t));
                     real dist0 = len3(delta0);
@@ -2971,8 +3067,12 @@ static void k_sc_solve_ee(orc_sim *s) {
                     if (near_folded_sq(m, u0, j, dot(d0, d0)) || near_folded_sq(m, u0, vj, dot(d1, d1)) ||
                         near_folded_sq(m, u1, j, dot(d2, d2)) || near_folded_sq(m, u1, vj, dot(d3, d3))) continue;
                 }
-                s->stats[4]++;
                 real dist = RSQRT(dsq);
+                if (untangle_on(s) && outer_edge_behind_inner(s, m->layer[u0], m->layer[j], u0, u1, j, vj,
+                                                              s->w[u0] <= 0 && s->w[u1] <= 0,
+                                                              s->w[j] <= 0 && s->w[vj] <= 0, delta_ee))
+                    continue;
+                s->stats[4]++;
                 v3 n = dvs(delta_ee, dist);
                 real pen = eff - dist;
                 real sp = st.s, tp = st.t;
@@ -3267,7 +3367,16 @@ static void k_pair_collect(orc_sim *s) {
     }
     free(cand);
 }
-/* self_collision_solve_vt.wgsl:227-414 (single layer) */
+/* self_collision_solve_vt.wgsl:227-414 */
+/* record_untangle_cand (self_collision_solve_vt.wgsl:97-107): atomic max of
+ * (1023 - distance rank) << 22 | (j + 1) */
+static void record_untangle(orc_sim *s, uint32_t vi, uint32_t j, real dist, real cap) {
+    if (vi >= s->nv || j >= 0x3FFFFEu || dist >= cap) return;
+    real ratio = rclamp(dist / rmax(cap, (real)1e-7f), 0, 1);
+    uint32_t rank = 1023u - (uint32_t)(float)(ratio * (real)1023.0f);
+    uint32_t packed = (rank << 22) | ((j + 1u) & 0x3FFFFFu);
+    if (packed > s->acc[vi].untangle) s->acc[vi].untangle = packed;
+}
 static void k_pair_solve_vt(orc_sim *s) {
     const orc_mesh *m = s->m;
     const real EPS = (real)1e-7f;
@@ -3286,6 +3395,13 @@ static void k_pair_solve_vt(orc_sim *s) {
         v3 tri_move = dvs(add(add(sub(p_j, pj_old), sub(p_v0, p_v0o)), sub(p_v1, p_v1o)), (real)3.0f);
         real wi = s->w[index], wj = s->w[j], w0 = s->w[v0], w1 = s->w[v1];
         int tri_pin = wj <= 0 || w0 <= 0 || w1 <= 0;
+        const int ut = untangle_on(s) && !pinned_i && m->layer[index] > m->layer[j];
+        v3 ni_unit = mk(0, 0, 0);
+        real nil = 0;
+        if (ut) {
+            nil = len3(s->normals[index]);
+            if (nil > (real)0.5f) ni_unit = dvs(s->normals[index], rmax(nil, EPS));
+        }
         v3 rel_old = add(xo_i, tri_move);
         real ht = 0, hu = 0, hv = 0;
         if (seg_tri(rel_old, p_i, p_j, p_v0, p_v1, &ht, &hu, &hv)) {
@@ -3295,6 +3411,11 @@ static void k_pair_solve_vt(orc_sim *s) {
             real tl = len3(tcr);
             v3 tn = tl > (real)1e-12f ? dvs(tcr, tl) : mk(0, 1, 0);
             real h_old_raw = dot(sub(rel_old, p_j), tn);
+            if (ut && tl > (real)1e-12f && (tri_pin || nil <= (real)0.5f || dot(ni_unit, tn) > 0) && h_old_raw < 0 &&
+                dot(move, tn) > 0) {
+                record_untangle(s, index, j, len3(sub(p_i, hit_pt)), eff * (real)2.5f);
+                continue;
+            }
             real side = (real)fabs((double)h_old_raw) > (real)1e-6f ? h_old_raw : dot(neg(move), tn);
             v3 n_side = side >= 0 ? tn : neg(tn);
             real h_old = rmax(dot(sub(rel_old, hit_pt), n_side), 0);
@@ -3334,6 +3455,16 @@ static void k_pair_solve_vt(orc_sim *s) {
         cbary cb = closest_bary(p_i, p_j, p_v0, p_v1);
         v3 dvt = sub(p_i, cb.pt);
         real dist = len3(dvt);
+        if (ut) {
+            v3 tcr = cross(sub(p_v0, p_j), sub(p_v1, p_j));
+            real tl = len3(tcr);
+            v3 tn = tl > EPS ? dvs(tcr, rmax(tl, EPS)) : mk(0, 0, 0);
+            real h = tl > EPS ? dot(dvt, tn) : 0;
+            if (tl > EPS && (tri_pin || nil <= (real)0.5f || dot(ni_unit, tn) > 0) && h < 0) {
+                record_untangle(s, index, j, dist, eff * (real)2.5f);
+                continue;
+            }
+        }
         if (pinned_i || !(dist < eff) || !(dist > EPS)) continue;
         v3 nvt = dvs(dvt, dist);
         real pen = eff - dist;
@@ -3350,7 +3481,7 @@ static void k_pair_solve_vt(orc_sim *s) {
         }
     }
 }
-/* self_collision_solve_ee.wgsl:231-589 (single layer) */
+/* self_collision_solve_ee.wgsl:231-589 */
 static void k_pair_solve_ee(orc_sim *s) {
     const orc_mesh *m = s->m;
     const real EPS = (real)1e-7f;
@@ -3416,6 +3547,10 @@ static void k_pair_solve_ee(orc_sim *s) {
             real ss = st_star.s, ts = st_star.t;
             real eff_star = rmix(th_u0, th_u1, ss) + rmix(th_j, th_vj, ts);
             if (dsq_star >= eff_star * eff_star) continue;
+            if (untangle_on(s) &&
+                ee_ccd_untangling(s, m->layer[u0], m->layer[j], u0, j, s->w[u0] == 0 && s->w[u1] == 0,
+                                  s->w[j] == 0 && s->w[vj] == 0, p_u0o, p_u1o, p_jo, p_vjo, p_u0, p_u1, p_j, p_vj))
+                continue;
             st2 st0 = closest_segs(p_u0o, p_u1o, p_jo, p_vjo);
             v3 delta0 = sub(vmix(p_u0o, p_u1o, st0.s), vmix(p_jo, p_vjo, st0.t));
             real dist0 = len3(delta0);
@@ -3462,6 +3597,9 @@ static void k_pair_solve_ee(orc_sim *s) {
         if (done) continue;
         if (dsqc >= effc * effc || dsqc <= EPS * EPS) continue;
         real dist = RSQRT(dsqc);
+        if (untangle_on(s) && outer_edge_behind_inner(s, m->layer[u0], m->layer[j], u0, u1, j, vj,
+                                                      s->w[u0] <= 0 && s->w[u1] <= 0, s->w[j] <= 0 && s->w[vj] <= 0, d1c))
+            continue;
         v3 n = dvs(d1c, dist);
         real pen = effc - dist;
         real sp = st1c.s, tp = st1c.t;
@@ -3496,8 +3634,157 @@ static void k_pair_solve_ee(orc_sim *s) {
     }
 }
 
+/* compute_normals.wgsl:25-58: normalised sum of the star triangles' cross products at x */
+static void k_compute_normals(orc_sim *s) {
+    const orc_mesh *m = s->m;
+    for (uint32_t i = 0; i < s->nv; i++) {
+        v3 p_i = s->x[i], sum = mk(0, 0, 0);
+        for (uint32_t k = m->star_off[i]; k < m->star_off[i + 1]; k++)
+            sum = add(sum, cross(sub(s->x[m->star_v0[k]], p_i), sub(s->x[m->star_v1[k]], p_i)));
+        real l = len3(sum);
+        s->normals[i] = l > (real)1e-7f ? dvs(sum, l) : mk(0, 0, 0);
+    }
+}
+
+/* closest point on a triangle, position only (self_collision_untangle.wgsl:85-129) */
+static v3 closest_pos(v3 p, v3 a, v3 b, v3 c) {
+    v3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+    real d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) return a;
+    v3 bp = sub(p, b);
+    real d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) return b;
+    real vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) return add(a, mul(ab, d1 / (d1 - d3)));
+    v3 cp = sub(p, c);
+    real d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) return c;
+    real vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) return add(a, mul(ac, d2 / (d2 - d6)));
+    real va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) return add(b, mul(sub(c, b), (d4 - d3) / ((d4 - d3) + (d5 - d6))));
+    real den = 1 / (va + vb + vc);
+    real v = vb * den, w = vc * den;
+    return add(add(a, mul(ab, v)), mul(ac, w));
+}
+
+/* self_collision_untangle.wgsl:163-394: a free outer-layer vertex with a recorded inner candidate j1
+ * takes j1's nearest 1-ring inner neighbour j2 outside its two-hop ring, the nearest triangle of the
+ * two stars (ties within 1e-10: the front side, then the larger |height|), and if it lies behind it
+ * inside the 45-degree cone and within half a thickness of its normal line, with a compatible vertex
+ * normal and behind the interpolated pseudo-normal, is pushed out through the triangle's normal. */
+static void k_sc_untangle(orc_sim *s) {
+    const orc_mesh *m = s->m;
+    const real EPS = (real)1e-7f;
+    if (!untangle_on(s)) return;
+    for (uint32_t index = 0; index < s->nv; index++) {
+        uint32_t cand = s->acc[index].untangle;
+        if (!cand) continue;
+        s->acc[index].untangle = 0;
+        if (s->w[index] <= 0 || m->layer[index] == 0) continue;
+        uint32_t j1e = cand & 0x3FFFFFu;
+        if (!j1e) continue;
+        uint32_t j1 = j1e - 1u;
+        if (j1 >= s->nv || j1 == index) continue;
+        v3 p_i = s->p[index];
+        real thick_i = (real)m->thick[index], llen_i = (real)m->local_len[index];
+        uint32_t j2 = 0xFFFFFFFFu;
+        real best_nb = (real)1e20f;
+        for (uint32_t a = m->adj_off[j1]; a < m->adj_off[j1 + 1]; a++) {
+            uint32_t nb = m->adj_idx[a];
+            if (nb < s->nv && nb != index && m->layer[nb] < m->layer[index]) {
+                v3 d = sub(p_i, s->p[nb]);
+                real dsq = dot(d, d);
+                if (dsq < best_nb && !within_two_hops(m, index, nb)) { best_nb = dsq; j2 = nb; }
+            }
+        }
+        real best_dsq = (real)1e20f, best_h = 0;
+        uint32_t bj = 0xFFFFFFFFu, bv0 = 0, bv1 = 0;
+        for (int c = 0; c < 2; c++) {
+            uint32_t j = c == 0 ? j1 : j2;
+            if (j == 0xFFFFFFFFu) continue;
+            v3 p_j = s->p[j];
+            real eff = rmax(thick_i + (real)m->thick[j], rmin(llen_i, (real)m->local_len[j]) * (real)0.06f);
+            real cap = eff * (real)2.5f, cap_sq = cap * cap;
+            for (uint32_t st = m->star_off[j]; st < m->star_off[j + 1]; st++) {
+                uint32_t v0 = m->star_v0[st], v1 = m->star_v1[st];
+                if (v0 == index || v1 == index) continue;
+                if (c == 1 && (v0 == j1 || v1 == j1)) continue;
+                v3 p_v0 = s->p[v0], p_v1 = s->p[v1];
+                v3 tmin = vmin(vmin(p_j, p_v0), p_v1), tmax = vmax(vmax(p_j, p_v0), p_v1), z = mk(0, 0, 0);
+                v3 ad = add(vmax(sub(tmin, p_i), z), vmax(sub(p_i, tmax), z));
+                real lim = rmin(cap_sq, best_dsq + (real)1e-10f);
+                if (dot(ad, ad) >= lim) continue;
+                v3 d = sub(p_i, closest_pos(p_i, p_j, p_v0, p_v1));
+                real dsq = dot(d, d);
+                if (dsq >= cap_sq) continue;
+                if (bj != 0xFFFFFFFFu && dsq > best_dsq + (real)1e-10f) continue;
+                v3 tcr = cross(sub(p_v0, p_j), sub(p_v1, p_j));
+                real tl = len3(tcr);
+                if (tl <= EPS) continue;
+                real h = dot(d, dvs(tcr, tl));
+                int same_d = (real)fabs((double)(dsq - best_dsq)) <= (real)1e-10f;
+                int better_tie = (h >= 0 && best_h < 0) ||
+                                 (((h >= 0) == (best_h >= 0)) && (real)fabs((double)h) > (real)fabs((double)best_h));
+                if (bj == 0xFFFFFFFFu || dsq < best_dsq - (real)1e-10f || (same_d && better_tie)) {
+                    if (within_two_hops(m, index, j) || within_two_hops(m, index, v0) || within_two_hops(m, index, v1))
+                        continue;
+                    best_dsq = dsq; best_h = h; bj = j; bv0 = v0; bv1 = v1;
+                }
+            }
+        }
+        if (bj == 0xFFFFFFFFu || !(best_h < 0)) continue;
+        v3 p_j = s->p[bj], p_v0 = s->p[bv0], p_v1 = s->p[bv1];
+        cbary r = closest_bary(p_i, p_j, p_v0, p_v1);
+        v3 d = sub(p_i, r.pt);
+        real dist = len3(d);
+        v3 tcr = cross(sub(p_v0, p_j), sub(p_v1, p_j));
+        real tl = len3(tcr);
+        if (!(tl > EPS && dist > EPS)) continue;
+        v3 tn = dvs(tcr, tl);
+        real h = dot(d, tn);
+        if (!(h < 0)) continue;
+        v3 ni = s->normals[index];
+        real nil = len3(ni);
+        int tri_pin = s->w[bj] <= 0 || s->w[bv0] <= 0 || s->w[bv1] <= 0;
+        int allow = tri_pin || nil <= (real)0.5f || dot(dvs(ni, rmax(nil, EPS)), tn) > 0;
+        real eff = rmax(thick_i + (real)m->thick[bj], rmin(llen_i, (real)m->local_len[bj]) * (real)0.06f);
+        real cos_behind = -h / dist;
+        real r_tan = RSQRT(rmax(dist * dist - h * h, 0));
+        real r_lim = rmax(eff * (real)0.5f, EPS);
+        if (!(allow && cos_behind > (real)0.707f && r_tan < r_lim)) continue;
+        v3 bc = r.bary;
+        v3 nint = add(add(mul(s->normals[bj], bc.x), mul(s->normals[bv0], bc.y)), mul(s->normals[bv1], bc.z));
+        real nl = len3(nint);
+        if (!(nl <= (real)0.5f || dot(d, nint) < 0)) continue;
+        real cone_w = rclamp((cos_behind - (real)0.707f) / (real)0.2f, 0, 1);
+        real r_w = rclamp(1 - r_tan / r_lim, 0, 1);
+        real pen = rmin(eff - h, eff * (real)2.0f) * cone_w * r_w;
+        if (!(pen > EPS)) continue;
+        if (tri_pin) { add_disp(&s->acc[index], mul(tn, pen)); continue; }
+        real wi = s->w[index], wj = s->w[bj], w0 = s->w[bv0], w1 = s->w[bv1];
+        real wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+        real wtot = wi + wt;
+        if (!(wtot > EPS)) continue;
+        v3 disp = mul(tn, pen * (real)0.5f);
+        add_disp(&s->acc[index], mul(disp, wi / wtot));
+        if (wj > 0) add_disp(&s->acc[bj], mul(neg(disp), bc.x * wj / wtot));
+        if (w0 > 0) add_disp(&s->acc[bv0], mul(neg(disp), bc.y * w0 / wtot));
+        if (w1 > 0) add_disp(&s->acc[bv1], mul(neg(disp), bc.z * w1 / wtot));
+    }
+}
+
+/* the vertex normals compute_normals.wgsl gives at the current positions x (nv x 3) */
+void orc_sim_normals(orc_sim *s, float *out) {
+    k_compute_normals(s);
+    for (uint32_t i = 0; i < s->nv; i++) {
+        out[3 * i] = (float)s->normals[i].x; out[3 * i + 1] = (float)s->normals[i].y; out[3 * i + 2] = (float)s->normals[i].z;
+    }
+}
+
 static void self_collision_pass(orc_sim *s) {
     const orc_mesh *m = s->m;
+    if (untangle_on(s)) k_compute_normals(s);
     hash_build(s->p, s->nv, s->cell_size, s->table_size, s->hkeys, s->hstarts, s->hsorted);
     if (m->ne > 0) {
         v3 *mid = (v3 *)malloc(m->ne * sizeof(v3));
@@ -3506,6 +3793,7 @@ static void self_collision_pass(orc_sim *s) {
         free(mid);
     }
     k_sc_solve(s);
+    k_sc_untangle(s);
     if (m->ne > 0) k_sc_solve_ee(s);
     k_sc_apply(s);
 }
@@ -3602,12 +3890,14 @@ static void k_edge_collision(orc_sim *s) {
  * last substep takes the direct path. */
 static void self_collision_outer(orc_sim *s, int first, int last) {
     if (!s->pair_cache || (s->pair_final_fallback && last)) { self_collision_pass(s); return; }
+    if (untangle_on(s)) k_compute_normals(s);
     if (first) {
         hash_build(s->p, s->nv, s->cell_size, s->table_size, s->hkeys, s->hstarts, s->hsorted);
         k_pair_collect(s);
     }
     k_pair_solve_vt(s);
     k_pair_solve_ee(s);
+    k_sc_untangle(s);
     k_sc_apply(s);
 }
 

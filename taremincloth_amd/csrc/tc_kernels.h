@@ -119,6 +119,7 @@ struct SelfColArgs {
     float relief_factor, max_disp_ratio;
     int enable_relief, exclude_neighbors;
     uint32_t max_search_iterations;
+    int untangle;  // normal untangling in effect: enabled and the cloth has more than one layer
 };
 struct SelfColTopo {
     const float *local_len;
@@ -128,7 +129,17 @@ struct SelfColTopo {
     const uint2 *star;
     const uint2 *edges;
     const uint32_t *edge_bbits;
+    // multi-layer cloth (read only when SelfColArgs::untangle is set): layer ids, the area-weighted
+    // vertex normals of the substep's start positions, the 1-ring adjacency
+    const uint32_t *layer;
+    const float4 *normals;
+    const uint32_t *adj_off, *adj_idx;
 };
+// compute_normals.wgsl: per vertex the normalised sum of its star triangles' cross products at X
+hipError_t launch_compute_normals(hipStream_t s, const float4 *X, const SelfColTopo &t, float4 *normals, uint32_t nv);
+// self_collision_untangle.wgsl: an outer-layer vertex recorded behind an inner triangle is pushed out
+hipError_t launch_sc_untangle(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t, Accum *acc,
+                              const SelfColArgs &a);
 hipError_t launch_sc_solve(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t,
                            const HashBufs &vh, Accum *acc, const SelfColArgs &a);
 hipError_t launch_sc_solve_ee(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t,
@@ -177,8 +188,8 @@ hipError_t launch_pair_collect(hipStream_t s, const float4 *P, const float4 *X, 
                                const PairTopo &PT, const HashBufs &vh, uint4 *vt, uint4 *ee, uint32_t *counters,
                                const PairArgs &a);
 hipError_t launch_pair_solve_vt(hipStream_t s, const float4 *P, const float4 *X, const uint4 *vt, uint32_t slots,
-                                Accum *acc, uint32_t nv);
+                                Accum *acc, uint32_t nv, const SelfColTopo &T, int untangle);
 hipError_t launch_pair_solve_ee(hipStream_t s, const float4 *P, const float4 *X, const uint4 *ee, uint32_t slots,
-                                Accum *acc, uint32_t nv);
+                                Accum *acc, uint32_t nv, const SelfColTopo &T, int untangle);
 
 }  // namespace tc

@@ -372,9 +372,28 @@ __device__ __forceinline__ void vv_pair(const float4 *__restrict__ P, const floa
     const float wi = qi.w, wj = qj.w;
     const bool both_dynamic = wj > 0.0f;
     const float eff_sq = eff * eff;
+    const bool ut = A.untangle != 0;
+    const uint32_t layer_i = ut ? T.layer[index] : 0u, layer_j = ut ? T.layer[j] : 0u;
     // 1. vertex-vertex
     if (!pinned_i && (!both_dynamic || index < j)) {
-        if (dsq < eff_sq && dsq > EPS * EPS) {
+        bool untangling = false;  // the outer vertex sits behind the inner one: the untangle pass moves it
+        if (ut && dsq < eff_sq && dsq > EPS * EPS && layer_i != layer_j) {
+            const float4 ni4 = T.normals[index], nj4 = T.normals[j];
+            const f3 ni = xyz(ni4), nj = xyz(nj4);
+            const float nil = length(ni), njl = length(nj);
+            if (layer_i > layer_j) {
+                if (njl > 0.5f) {
+                    const f3 nju = nj / njl;
+                    const bool allow = wj <= 0.0f || nil <= 0.5f || dot(ni / fmaxf(nil, EPS), nju) > 0.0f;
+                    untangling = allow && dot(dvv, nju) < 0.0f;
+                }
+            } else if (both_dynamic && nil > 0.5f) {
+                const f3 niu = ni / nil;
+                const bool allow = njl <= 0.5f || dot(nj / fmaxf(njl, EPS), niu) > 0.0f;
+                untangling = allow && dot(-dvv, niu) < 0.0f;
+            }
+        }
+        if (!untangling && dsq < eff_sq && dsq > EPS * EPS) {
             const f3 n = dvv / dvv_len;
             const float pen = eff - dvv_len;
             const float ws = wi + wj;
@@ -436,6 +455,13 @@ __device__ __forceinline__ void vv_pair(const float4 *__restrict__ P, const floa
             const float tl = length(tcr);
             const f3 tn = tl > 1e-12f ? tcr / tl : mk3(0.0f, 1.0f, 0.0f);
             const float h_old_raw = dot(rel_old - p_j, tn);
+            if (ut && !pinned_i && layer_i > layer_j && tl > 1e-12f) {
+                // a free outer vertex returning from behind the triangle to its front is let through
+                const f3 ni = xyz(T.normals[index]);
+                const float nil = length(ni);
+                const bool allow = tri_pin || nil <= 0.5f || dot(ni / fmaxf(nil, EPS), tn) > 0.0f;
+                if (allow && h_old_raw < 0.0f && dot(move, tn) > 0.0f) continue;
+            }
             const float side = fabsf(h_old_raw) > 1e-6f ? h_old_raw : dot(-move, tn);
             const f3 n_side = side >= 0.0f ? tn : -tn;
             const float h_old = fmaxf(dot(rel_old - hit_pt, n_side), 0.0f);
@@ -477,6 +503,18 @@ __device__ __forceinline__ void vv_pair(const float4 *__restrict__ P, const floa
             const float dvsq = dot(dvt, dvt);
             if (!pinned_i && dvsq < eff_sq && dvsq > EPS * EPS) {
                 const float dl = sqrtf(dvsq);
+                if (ut && layer_i > layer_j) {  // behind the inner triangle: left to the untangle pass
+                    const f3 tcr = cross(p_v0 - p_j, p_v1 - p_j);
+                    const float tl = length(tcr);
+                    if (tl > EPS) {
+                        const f3 tn = tcr / tl;
+                        if (dot(dvt, tn) < 0.0f) {
+                            const f3 ni = xyz(T.normals[index]);
+                            const float nil = length(ni);
+                            if (tri_pin || nil <= 0.5f || dot(ni / fmaxf(nil, EPS), tn) > 0.0f) continue;
+                        }
+                    }
+                }
                 const f3 nvt = dvt / dl;
                 const float pen = eff - dl;
                 if (tri_pin) {
@@ -497,8 +535,12 @@ __device__ __forceinline__ void vv_pair(const float4 *__restrict__ P, const floa
     }
 }
 
-// sc_solve over the 27 neighbour cells of each vertex (single-layer cloth: the
-// multi-layer untangling branches of the reference are inactive).
+// sc_solve over the 27 neighbour cells of each vertex.  On a multi-layer cloth with untangling on,
+// each free outer-layer vertex also records its nearest inner-layer vertex outside its two-hop
+// ring within 2.5 effective thicknesses plus 1.3 local edge lengths (self_collision.wgsl:340-386,
+// 678-681): every lane keeps the first minimum of its cell's bucket, and the half-wave reduction
+// keeps the minimum by (distance^2, cell in the reference's x-outer order), which is the first
+// minimum of the reference's sequential walk.
 __global__ void __launch_bounds__(SC_BLOCK) k_sc_solve(const float4 *__restrict__ P, const float4 *__restrict__ X,
                                                        SelfColTopo T, const uint32_t *__restrict__ starts,
                                                        const uint32_t *__restrict__ sorted, Accum *__restrict__ acc,
@@ -513,6 +555,9 @@ __global__ void __launch_bounds__(SC_BLOCK) k_sc_solve(const float4 *__restrict_
         f3 p_i = mk3(0.0f, 0.0f, 0.0f);
         float thick_i = 0.0f, llen_i = 0.0f;
         i3 nc = {0, 0, 0};
+        bool track = false;           // untangle candidate search of this vertex
+        uint32_t layer_i = 0u, best_j = SC_NONE;
+        float best_dsq = 1e20f;
         if (index < A.nv && ci < 27u) {
             const float4 qi = P[index], xi = X[index];
             p_i = xyz(qi);
@@ -524,6 +569,10 @@ __global__ void __launch_bounds__(SC_BLOCK) k_sc_solve(const float4 *__restrict_
                 const uint32_t h = hash_cell(nc, A.table);
                 q = starts[h];
                 qe = starts[h + 1];
+                if (A.untangle) {
+                    layer_i = T.layer[index];
+                    track = !(qi.w <= 0.0f) && layer_i > 0u;
+                }
             }
         }
         auto accept = [&](uint32_t k) -> uint32_t {
@@ -538,12 +587,183 @@ __global__ void __launch_bounds__(SC_BLOCK) k_sc_solve(const float4 *__restrict_
             const float dsq = dot(dvv, dvv);
             const float thick_j = xj.w, llen_j = T.local_len[j];
             const float eff = fmaxf(thick_i + thick_j, fminf(llen_i, llen_j) * 0.06f);
+            if (track && dsq < best_dsq && T.layer[j] < layer_i) {
+                const float r = eff * 2.5f + llen_j * 1.3f;
+                if (dsq <= r * r && !within_two_hops(T, index, j)) { best_dsq = dsq; best_j = j; }
+            }
             const float max_geom = eff + (llen_i + llen_j) * 1.3f;
             return dsq <= max_geom * max_geom ? j : SC_NONE;
         };
         sc_walk(s_queue[wv], qn, g + 1u == SC_GROUPS, lane, index, q, qe, accept,
                 [&](uint32_t i, uint32_t j) { vv_pair(P, X, T, acc, A, i, j); });
+        if (A.untangle) {  // the vertex's nearest candidate over its 27 cells (lanes of one half)
+            uint32_t kd = __float_as_uint(best_dsq);
+            uint32_t ko = ci < 27u ? 9u * (ci % 3u) + 3u * ((ci / 3u) % 3u) + ci / 9u : 31u;
+            for (uint32_t o = 16u; o != 0u; o >>= 1) {
+                const uint32_t od = __shfl_xor(kd, int(o), 32), oo = __shfl_xor(ko, int(o), 32);
+                const uint32_t oj = __shfl_xor(best_j, int(o), 32);
+                if (od < kd || (od == kd && oo < ko)) { kd = od; ko = oo; best_j = oj; }
+            }
+            if (ci == 0u && track && best_j < 0x3FFFFEu) acc[index].untangle = best_j + 1u;
+        }
     }
+}
+
+// compute_normals.wgsl:25-58: per vertex the normalised sum of (p_a - p_i) x (p_b - p_i) over its
+// star pairs at the substep's start positions X; zero for a vertex without triangles.
+__global__ void __launch_bounds__(256) k_compute_normals(const float4 *__restrict__ X, SelfColTopo T,
+                                                         float4 *__restrict__ normals, uint32_t nv) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    const f3 p_i = xyz(X[i]);
+    f3 sum = mk3(0.0f, 0.0f, 0.0f);
+    for (uint32_t k = T.star_off[i]; k < T.star_off[i + 1]; ++k) {
+        const uint2 pr = T.star[k];
+        sum = sum + cross(xyz(X[pr.x]) - p_i, xyz(X[pr.y]) - p_i);
+    }
+    const float l = length(sum);
+    normals[i] = l > 1e-7f ? make_float4(sum.x / l, sum.y / l, sum.z / l, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// closest point on a triangle, position only (self_collision_untangle.wgsl:85-129)
+__device__ __forceinline__ f3 closest_pos(f3 p, f3 a, f3 b, f3 c) {
+    const f3 ab = b - a, ac = c - a, ap = p - a;
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) return a;
+    const f3 bp = p - b;
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) return b;
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) return a + ab * (d1 / (d1 - d3));
+    const f3 cp = p - c;
+    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) return c;
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) return a + ac * (d2 / (d2 - d6));
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        return b + (c - b) * w;
+    }
+    const float denom = 1.0f / (va + vb + vc);
+    const float v = vb * denom, w = vc * denom;
+    return a + ab * v + ac * w;
+}
+
+// self_collision_untangle.wgsl:163-394: a free outer-layer vertex with a recorded inner candidate j1
+// takes j1's nearest 1-ring inner neighbour j2 outside its two-hop ring, picks the nearest triangle
+// of j1's and j2's stars (ties: the front side, then the larger height), and if it lies behind it,
+// inside a 45-degree cone under the triangle and within half a thickness of its normal line, with
+// the vertex normal not opposed (or a pinned triangle) and behind the interpolated pseudo-normal too,
+// it is pushed to the front, mass-weighted against the triangle unless the triangle holds a pin.
+__global__ void __launch_bounds__(256) k_sc_untangle(const float4 *__restrict__ P, const float4 *__restrict__ X,
+                                                     SelfColTopo T, Accum *__restrict__ acc, SelfColArgs A) {
+    const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
+    if (index >= A.nv || !A.untangle) return;
+    const float EPS = 1e-7f;
+    const uint32_t cand = acc[index].untangle;
+    if (cand == 0u) return;
+    acc[index].untangle = 0u;
+    const float4 qi = P[index];
+    const uint32_t layer_i = T.layer[index];
+    if (qi.w <= 0.0f || layer_i == 0u) return;
+    const uint32_t j1e = cand & 0x3FFFFFu;
+    if (j1e == 0u) return;
+    const uint32_t j1 = j1e - 1u;
+    if (j1 >= A.nv || j1 == index) return;
+    const f3 p_i = xyz(qi);
+    const float thick_i = X[index].w;
+    const float llen_i = T.local_len[index];
+    // j2: j1's nearest 1-ring neighbour on an inner layer, outside the vertex's two-hop ring
+    uint32_t j2 = SC_NONE;
+    float best_nb = 1e20f;
+    for (uint32_t a = T.adj_off[j1]; a < T.adj_off[j1 + 1]; ++a) {
+        const uint32_t nb = T.adj_idx[a];
+        if (nb < A.nv && nb != index && T.layer[nb] < layer_i) {
+            const f3 d = p_i - xyz(P[nb]);
+            const float dsq = dot(d, d);
+            if (dsq < best_nb && !within_two_hops(T, index, nb)) { best_nb = dsq; j2 = nb; }
+        }
+    }
+    float best_dsq = 1e20f, best_h = 0.0f;
+    uint32_t bj = SC_NONE, bv0 = 0u, bv1 = 0u;
+    for (uint32_t c = 0; c < 2u; ++c) {
+        const uint32_t j = c == 0u ? j1 : j2;
+        if (j == SC_NONE) continue;
+        const float4 qj = P[j];
+        const f3 p_j = xyz(qj);
+        const float eff = fmaxf(thick_i + X[j].w, fminf(llen_i, T.local_len[j]) * 0.06f);
+        const float cap = eff * 2.5f;
+        const float cap_sq = cap * cap;
+        for (uint32_t st = T.star_off[j]; st < T.star_off[j + 1]; ++st) {
+            const uint2 pr = T.star[st];
+            const uint32_t v0 = pr.x, v1 = pr.y;
+            if (v0 == index || v1 == index) continue;
+            if (c == 1u && (v0 == j1 || v1 == j1)) continue;
+            const f3 p_v0 = xyz(P[v0]), p_v1 = xyz(P[v1]);
+            const f3 tmin = min3(min3(p_j, p_v0), p_v1), tmax = max3(max3(p_j, p_v0), p_v1);
+            const f3 z = mk3(0.0f, 0.0f, 0.0f);
+            const f3 ad = max3(tmin - p_i, z) + max3(p_i - tmax, z);
+            const float lim = fminf(cap_sq, best_dsq + 1e-10f);
+            if (dot(ad, ad) >= lim) continue;
+            const f3 d = p_i - closest_pos(p_i, p_j, p_v0, p_v1);
+            const float dsq = dot(d, d);
+            if (dsq >= cap_sq) continue;
+            if (bj != SC_NONE && dsq > best_dsq + 1e-10f) continue;
+            const f3 tcr = cross(p_v0 - p_j, p_v1 - p_j);
+            const float tl = length(tcr);
+            if (tl <= EPS) continue;
+            const float h = dot(d, tcr / tl);
+            const bool same_d = fabsf(dsq - best_dsq) <= 1e-10f;
+            const bool better_tie = (h >= 0.0f && best_h < 0.0f) || (((h >= 0.0f) == (best_h >= 0.0f)) && fabsf(h) > fabsf(best_h));
+            if (bj == SC_NONE || dsq < best_dsq - 1e-10f || (same_d && better_tie)) {
+                if (within_two_hops(T, index, j) || within_two_hops(T, index, v0) || within_two_hops(T, index, v1))
+                    continue;
+                best_dsq = dsq; best_h = h; bj = j; bv0 = v0; bv1 = v1;
+            }
+        }
+    }
+    if (bj == SC_NONE || !(best_h < 0.0f)) return;
+    const float4 qj = P[bj], q0 = P[bv0], q1 = P[bv1];
+    const f3 p_j = xyz(qj), p_v0 = xyz(q0), p_v1 = xyz(q1);
+    f3 bc;
+    const f3 d = p_i - closest_bary(p_i, p_j, p_v0, p_v1, bc);
+    const float dist = length(d);
+    const f3 tcr = cross(p_v0 - p_j, p_v1 - p_j);
+    const float tl = length(tcr);
+    if (!(tl > EPS && dist > EPS)) return;
+    const f3 tn = tcr / tl;
+    const float h = dot(d, tn);
+    if (!(h < 0.0f)) return;
+    const f3 ni = xyz(T.normals[index]);
+    const float nil = length(ni);
+    const bool tri_pin = qj.w <= 0.0f || q0.w <= 0.0f || q1.w <= 0.0f;
+    const bool allow = tri_pin || nil <= 0.5f || dot(ni / fmaxf(nil, EPS), tn) > 0.0f;
+    const float eff = fmaxf(thick_i + X[bj].w, fminf(llen_i, T.local_len[bj]) * 0.06f);
+    const float cos_behind = -h / dist;
+    const float r_tan = sqrtf(fmaxf(dist * dist - h * h, 0.0f));
+    const float r_lim = fmaxf(eff * 0.5f, EPS);
+    if (!(allow && cos_behind > 0.707f && r_tan < r_lim)) return;
+    const f3 nint = xyz(T.normals[bj]) * bc.x + xyz(T.normals[bv0]) * bc.y + xyz(T.normals[bv1]) * bc.z;
+    const float nl = length(nint);
+    if (!(nl <= 0.5f || dot(d, nint) < 0.0f)) return;
+    const float cone_w = clampf((cos_behind - 0.707f) / 0.2f, 0.0f, 1.0f);
+    const float r_w = clampf(1.0f - r_tan / r_lim, 0.0f, 1.0f);
+    const float pen = fminf(eff - h, eff * 2.0f) * cone_w * r_w;
+    if (!(pen > EPS)) return;
+    if (tri_pin) {
+        add_disp(&acc[index], tn * pen);
+        return;
+    }
+    const float wi = qi.w, wj = qj.w, w0 = q0.w, w1 = q1.w;
+    const float wt = bc.x * bc.x * wj + bc.y * bc.y * w0 + bc.z * bc.z * w1;
+    const float wtot = wi + wt;
+    if (!(wtot > EPS)) return;
+    const f3 disp = tn * (pen * 0.5f);
+    add_disp(&acc[index], disp * (wi / wtot));
+    if (wj > 0.0f) add_disp(&acc[bj], (-disp) * (bc.x * wj / wtot));
+    if (w0 > 0.0f) add_disp(&acc[bv0], (-disp) * (bc.y * w0 / wtot));
+    if (w1 > 0.0f) add_disp(&acc[bv1], (-disp) * (bc.z * w1 / wtot));
 }
 
 // ---------------------------------------------------------------------------
@@ -579,6 +799,49 @@ __device__ __forceinline__ float sweep_volume(const Sweep &w, float tau) {
     return dot(cross(A1 - A0, B1 - B0), B0 - A0);
 }
 __device__ __forceinline__ bool edge_bit(const uint32_t *bits, uint32_t e) { return ((bits[e >> 5] >> (e & 31u)) & 1u) != 0u; }
+
+// is_outer_edge_behind_inner (self_collision_ee_direct.wgsl:123-148): for edges of different layers,
+// whether the outer one lies behind the inner one along the inner edge's summed vertex normals, with
+// the outer normals not opposed (or the inner edge fixed); delta_ab points from edge b to edge a.
+__device__ __forceinline__ bool outer_edge_behind_inner(const SelfColTopo &T, uint32_t la, uint32_t lb, uint32_t a0,
+                                                        uint32_t a1, uint32_t b0, uint32_t b1, bool a_fixed,
+                                                        bool b_fixed, f3 delta_ab) {
+    const float EPS = 1e-7f;
+    if (la == lb) return false;
+    const f3 na = xyz(T.normals[a0]) + xyz(T.normals[a1]), nb = xyz(T.normals[b0]) + xyz(T.normals[b1]);
+    const float nal = length(na), nbl = length(nb);
+    if (la > lb) {
+        if (nbl <= 0.5f) return false;
+        const f3 ni = nb / nbl;
+        const bool allow = b_fixed || nal <= 0.5f || dot(na / fmaxf(nal, EPS), ni) > 0.0f;
+        return allow && dot(delta_ab, ni) < 0.0f;
+    }
+    if (nal <= 0.5f) return false;
+    const f3 ni = na / nal;
+    const bool allow = a_fixed || nbl <= 0.5f || dot(nb / fmaxf(nbl, EPS), ni) > 0.0f;
+    return allow && dot(-delta_ab, ni) < 0.0f;
+}
+// The E-E CCD's untangling exemption (self_collision_ee_direct.wgsl:506-524, solve_ee.wgsl:386-408):
+// an outer edge (not fixed) that starts behind the inner edge's normal and moves to its front.
+__device__ __forceinline__ bool ee_ccd_untangling(const SelfColTopo &T, uint32_t la, uint32_t lb, uint32_t u0,
+                                                  uint32_t j, bool a_fixed, bool b_fixed, f3 a0o, f3 a1o, f3 b0o,
+                                                  f3 b1o, f3 a0, f3 a1, f3 b0, f3 b1) {
+    if (la == lb) return false;
+    const f3 d0 = ((a0o + a1o) - (b0o + b1o)) * 0.5f;
+    const f3 d1 = ((a0 + a1) - (b0 + b1)) * 0.5f;
+    const f3 mv = d1 - d0;
+    if (la > lb && !a_fixed) {
+        const f3 ni = xyz(T.normals[j]), no = xyz(T.normals[u0]);
+        const bool allow = b_fixed || dot(no, no) <= 0.25f;
+        return allow && dot(ni, ni) > 0.25f && dot(d0, ni) < 0.0f && dot(mv, ni) > 0.0f;
+    }
+    if (lb > la && !b_fixed) {
+        const f3 ni = xyz(T.normals[u0]), no = xyz(T.normals[j]);
+        const bool allow = a_fixed || dot(no, no) <= 0.25f;
+        return allow && dot(ni, ni) > 0.25f && dot(-d0, ni) < 0.0f && dot(-mv, ni) > 0.0f;
+    }
+    return false;
+}
 
 // E-E response of edge ea against edge eb for a pair that passed the box test
 // (self_collision_ee_direct.wgsl:266-776, the candidate loop's body after the
@@ -670,6 +933,10 @@ __device__ __forceinline__ void ee_pair(const float4 *__restrict__ P, const floa
                 near_folded_sq(T, u1, j, dot(t2, t2)) || near_folded_sq(T, u1, vj, dot(t3, t3)))
                 return;
         }
+        if (A.untangle &&
+            ee_ccd_untangling(T, T.layer[u0], T.layer[j], u0, j, wu0 == 0.0f && wu1 == 0.0f, wj == 0.0f && wvj == 0.0f,
+                              p_u0o, p_u1o, p_jo, p_vjo, p_u0, p_u1, p_j, p_vj))
+            return;
         const ST st0 = closest_segs(p_u0o, p_u1o, p_jo, p_vjo);
         const f3 delta0 = mix3(p_u0o, p_u1o, st0.s) - mix3(p_jo, p_vjo, st0.t);
         const float dist0 = length(delta0);
@@ -742,6 +1009,9 @@ __device__ __forceinline__ void ee_pair(const float4 *__restrict__ P, const floa
             return;
     }
     const float dist = sqrtf(dsq);
+    if (A.untangle && outer_edge_behind_inner(T, T.layer[u0], T.layer[j], u0, u1, j, vj, wu0 <= 0.0f && wu1 <= 0.0f,
+                                              wj <= 0.0f && wvj <= 0.0f, delta_ee))
+        return;
     const f3 n = delta_ee / dist;
     const float pen = eff - dist;
     const float sp = st.s, tp = st.t;
@@ -1137,10 +1407,21 @@ __global__ void __launch_bounds__(64) k_pair_collect(const float4 *__restrict__ 
     if (drop) atomicAdd(&counters[VT ? 2 : 3], drop);
 }
 
-// self_collision_solve_vt.wgsl:227-414 for one cached triple (single-layer: no untangling)
+// record_untangle_cand (self_collision_solve_vt.wgsl:97-107): the nearest candidate wins through an
+// atomic max of (1023 - distance rank) << 22 | (j + 1)
+__device__ __forceinline__ void record_untangle(Accum *__restrict__ acc, uint32_t nv, uint32_t vi, uint32_t j, float dist,
+                                                float cap) {
+    if (vi >= nv || j >= 0x3FFFFEu || dist >= cap) return;
+    const float ratio = clampf(dist / fmaxf(cap, 1e-7f), 0.0f, 1.0f);
+    const uint32_t rank = 1023u - uint32_t(ratio * 1023.0f);
+    atomicMax(&acc[vi].untangle, (rank << 22) | ((j + 1u) & 0x3FFFFFu));
+}
+
+// self_collision_solve_vt.wgsl:227-414 for one cached triple
 __global__ void __launch_bounds__(256) k_pair_solve_vt(const float4 *__restrict__ P, const float4 *__restrict__ X,
                                                        const uint4 *__restrict__ pairs, uint32_t slots,
-                                                       Accum *__restrict__ acc, uint32_t nv) {
+                                                       Accum *__restrict__ acc, uint32_t nv, SelfColTopo T,
+                                                       int untangle) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= slots) return;
     const float EPS = 1e-7f;
@@ -1158,6 +1439,14 @@ __global__ void __launch_bounds__(256) k_pair_solve_vt(const float4 *__restrict_
     const f3 tri_move = (((p_j - pj_old) + (p_v0 - p_v0o)) + (p_v1 - p_v1o)) / 3.0f;
     const float wi = qi.w, wj = qj.w, w0 = q0.w, w1 = q1.w;
     const bool tri_pin = wj <= 0.0f || w0 <= 0.0f || w1 <= 0.0f;
+    const bool ut = untangle && !pinned_i && T.layer[index] > T.layer[j];
+    f3 ni_unit = mk3(0.0f, 0.0f, 0.0f);
+    float nil = 0.0f;
+    if (ut) {
+        const f3 ni = xyz(T.normals[index]);
+        nil = length(ni);
+        if (nil > 0.5f) ni_unit = ni / fmaxf(nil, EPS);
+    }
     const f3 rel_old = xo_i + tri_move;
     float ht = 0.0f, hu = 0.0f, hv = 0.0f;
     if (seg_tri_uv(rel_old, p_i, p_j, p_v0, p_v1, ht, hu, hv)) {
@@ -1167,6 +1456,11 @@ __global__ void __launch_bounds__(256) k_pair_solve_vt(const float4 *__restrict_
         const float tl = length(tcr);
         const f3 tn = tl > 1e-12f ? tcr / tl : mk3(0.0f, 1.0f, 0.0f);
         const float h_old_raw = dot(rel_old - p_j, tn);
+        if (ut && tl > 1e-12f && (tri_pin || nil <= 0.5f || dot(ni_unit, tn) > 0.0f) && h_old_raw < 0.0f &&
+            dot(move, tn) > 0.0f) {  // returning to the front: no response, the candidate is kept
+            record_untangle(acc, nv, index, j, length(p_i - hit_pt), eff * 2.5f);
+            return;
+        }
         const float side = fabsf(h_old_raw) > 1e-6f ? h_old_raw : dot(-move, tn);
         const f3 n_side = side >= 0.0f ? tn : -tn;
         const float h_old = fmaxf(dot(rel_old - hit_pt, n_side), 0.0f);
@@ -1207,6 +1501,16 @@ __global__ void __launch_bounds__(256) k_pair_solve_vt(const float4 *__restrict_
     const f3 cp = closest_bary(p_i, p_j, p_v0, p_v1, bc);
     const f3 dvt = p_i - cp;
     const float dist = length(dvt);
+    if (ut) {  // behind an inner triangle: the candidate goes to the untangle pass, no response
+        const f3 tcr = cross(p_v0 - p_j, p_v1 - p_j);
+        const float tl = length(tcr);
+        const f3 tn = tl > EPS ? tcr / fmaxf(tl, EPS) : mk3(0.0f, 0.0f, 0.0f);
+        const float h = tl > EPS ? dot(dvt, tn) : 0.0f;
+        if (tl > EPS && (tri_pin || nil <= 0.5f || dot(ni_unit, tn) > 0.0f) && h < 0.0f) {
+            record_untangle(acc, nv, index, j, dist, eff * 2.5f);
+            return;
+        }
+    }
     if (pinned_i || !(dist < eff) || !(dist > EPS)) return;
     const f3 nvt = dvt / dist;
     const float pen = eff - dist;
@@ -1225,10 +1529,11 @@ __global__ void __launch_bounds__(256) k_pair_solve_vt(const float4 *__restrict_
     }
 }
 
-// self_collision_solve_ee.wgsl:231-589 for one cached quadruple (single-layer: no untangling)
+// self_collision_solve_ee.wgsl:231-589 for one cached quadruple
 __global__ void __launch_bounds__(256) k_pair_solve_ee(const float4 *__restrict__ P, const float4 *__restrict__ X,
                                                        const uint4 *__restrict__ pairs, uint32_t slots,
-                                                       Accum *__restrict__ acc, uint32_t nv) {
+                                                       Accum *__restrict__ acc, uint32_t nv, SelfColTopo T,
+                                                       int untangle) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= slots) return;
     const float EPS = 1e-7f;
@@ -1296,6 +1601,10 @@ __global__ void __launch_bounds__(256) k_pair_solve_ee(const float4 *__restrict_
         const float ss = st_star.s, ts = st_star.t;
         const float eff_star = mixf(th_u0, th_u1, ss) + mixf(th_j, th_vj, ts);
         if (dsq_star >= eff_star * eff_star) return;
+        if (untangle &&
+            ee_ccd_untangling(T, T.layer[u0], T.layer[j], u0, j, wu0 == 0.0f && wu1 == 0.0f, wj == 0.0f && wvj == 0.0f,
+                              p_u0o, p_u1o, p_jo, p_vjo, p_u0, p_u1, p_j, p_vj))
+            return;
         const ST st0 = closest_segs(p_u0o, p_u1o, p_jo, p_vjo);
         const f3 delta0 = mix3(p_u0o, p_u1o, st0.s) - mix3(p_jo, p_vjo, st0.t);
         const float dist0 = length(delta0);
@@ -1344,6 +1653,9 @@ __global__ void __launch_bounds__(256) k_pair_solve_ee(const float4 *__restrict_
     }
     if (dsqc >= effc * effc || dsqc <= EPS * EPS) return;
     const float dist = sqrtf(dsqc);
+    if (untangle && outer_edge_behind_inner(T, T.layer[u0], T.layer[j], u0, u1, j, vj, wu0 <= 0.0f && wu1 <= 0.0f,
+                                            wj <= 0.0f && wvj <= 0.0f, d1c))
+        return;
     const f3 n = d1c / dist;
     const float pen = effc - dist;
     const float sp = st1c.s, tp = st1c.t;
@@ -1391,16 +1703,31 @@ hipError_t launch_pair_collect(hipStream_t s, const float4 *P, const float4 *X, 
 }
 
 hipError_t launch_pair_solve_vt(hipStream_t s, const float4 *P, const float4 *X, const uint4 *vt, uint32_t slots,
-                                Accum *acc, uint32_t nv) {
+                                Accum *acc, uint32_t nv, const SelfColTopo &T, int untangle) {
     if (!slots) return hipSuccess;
-    hipLaunchKernelGGL(k_pair_solve_vt, dim3(grid_for(slots, 256)), dim3(256), 0, s, P, X, vt, slots, acc, nv);
+    hipLaunchKernelGGL(k_pair_solve_vt, dim3(grid_for(slots, 256)), dim3(256), 0, s, P, X, vt, slots, acc, nv, T,
+                       untangle);
     return hipGetLastError();
 }
 
 hipError_t launch_pair_solve_ee(hipStream_t s, const float4 *P, const float4 *X, const uint4 *ee, uint32_t slots,
-                                Accum *acc, uint32_t nv) {
+                                Accum *acc, uint32_t nv, const SelfColTopo &T, int untangle) {
     if (!slots) return hipSuccess;
-    hipLaunchKernelGGL(k_pair_solve_ee, dim3(grid_for(slots, 256)), dim3(256), 0, s, P, X, ee, slots, acc, nv);
+    hipLaunchKernelGGL(k_pair_solve_ee, dim3(grid_for(slots, 256)), dim3(256), 0, s, P, X, ee, slots, acc, nv, T,
+                       untangle);
+    return hipGetLastError();
+}
+
+hipError_t launch_compute_normals(hipStream_t s, const float4 *X, const SelfColTopo &t, float4 *normals, uint32_t nv) {
+    if (!nv) return hipSuccess;
+    hipLaunchKernelGGL(k_compute_normals, dim3(grid_for(nv, 256)), dim3(256), 0, s, X, t, normals, nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_sc_untangle(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t, Accum *acc,
+                              const SelfColArgs &a) {
+    if (!a.nv || !a.untangle) return hipSuccess;
+    hipLaunchKernelGGL(k_sc_untangle, dim3(grid_for(a.nv, 256)), dim3(256), 0, s, P, X, t, acc, a);
     return hipGetLastError();
 }
 

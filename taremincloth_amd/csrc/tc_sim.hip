@@ -246,6 +246,12 @@ hipError_t enqueue_substep_tail(tc_sim *s, float dt_sub, uint32_t pc, uint32_t l
         const int scope = s->prof_begin("self_collision_Outer");
         const tc::SelfColArgs sa = s->sc_args();
         const tc::SelfColTopo t = s->topo();
+        if (sa.untangle) {
+            q = s->prof_begin("sc_normals");
+            e = tc::launch_compute_normals(s->stream, s->X.p, t, s->normals.p, n);
+            s->prof_end(q);
+            if (e != hipSuccess) return e;
+        }
         if (s->sub_first) {
             q = s->prof_begin("hash_build");
             e = enqueue_hashes(s, Pc);
@@ -260,13 +266,21 @@ hipError_t enqueue_substep_tail(tc_sim *s, float dt_sub, uint32_t pc, uint32_t l
         }
         const tc::PairArgs pa = s->pair_args();
         q = s->prof_begin("pair_vt");
-        e = tc::launch_pair_solve_vt(s->stream, Pc, s->X.p, s->pair_vt.p, n * pa.quota_vt, s->accum.p, n);
+        e = tc::launch_pair_solve_vt(s->stream, Pc, s->X.p, s->pair_vt.p, n * pa.quota_vt, s->accum.p, n, t,
+                                     sa.untangle);
         s->prof_end(q);
         if (e != hipSuccess) return e;
         q = s->prof_begin("pair_ee");
-        e = tc::launch_pair_solve_ee(s->stream, Pc, s->X.p, s->pair_ee.p, n * pa.quota_ee, s->accum.p, n);
+        e = tc::launch_pair_solve_ee(s->stream, Pc, s->X.p, s->pair_ee.p, n * pa.quota_ee, s->accum.p, n, t,
+                                     sa.untangle);
         s->prof_end(q);
         if (e != hipSuccess) return e;
+        if (sa.untangle) {
+            q = s->prof_begin("sc_untangle");
+            e = tc::launch_sc_untangle(s->stream, Pc, s->X.p, t, s->accum.p, sa);
+            s->prof_end(q);
+            if (e != hipSuccess) return e;
+        }
         q = s->prof_begin("pair_apply");
         e = tc::launch_sc_apply(s->stream, Pc, s->local_len.p, s->accum.p, sa, B);
         s->prof_end(q);
@@ -274,16 +288,28 @@ hipError_t enqueue_substep_tail(tc_sim *s, float dt_sub, uint32_t pc, uint32_t l
         if (e != hipSuccess) return e;
     } else if (self_collide && s->self_collision) {
         const int scope = s->prof_begin("self_collision_Outer");
+        const tc::SelfColArgs sa = s->sc_args();
+        const tc::SelfColTopo t = s->topo();
+        if (sa.untangle) {
+            q = s->prof_begin("sc_normals");
+            e = tc::launch_compute_normals(s->stream, s->X.p, t, s->normals.p, n);
+            s->prof_end(q);
+            if (e != hipSuccess) return e;
+        }
         q = s->prof_begin("hash_build");
         e = enqueue_hashes(s, Pc);
         s->prof_end(q);
         if (e != hipSuccess) return e;
-        const tc::SelfColArgs sa = s->sc_args();
-        const tc::SelfColTopo t = s->topo();
         q = s->prof_begin("sc_solve");
         e = tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa);
         s->prof_end(q);
         if (e != hipSuccess) return e;
+        if (sa.untangle) {
+            q = s->prof_begin("sc_untangle");
+            e = tc::launch_sc_untangle(s->stream, Pc, s->X.p, t, s->accum.p, sa);
+            s->prof_end(q);
+            if (e != hipSuccess) return e;
+        }
         q = s->prof_begin("sc_solve_ee");
         e = tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa);
         s->prof_end(q);
@@ -402,10 +428,6 @@ int run_frame_host_gated(tc_sim *s, float dt, uint32_t substeps) {
 }
 
 int run_frame(tc_sim *s, float dt, uint32_t substeps) {
-    if (s->self_collision && s->normal_untangling && s->mesh.has_multiple_layers)
-        return fail(TC_ERR_UNSUPPORTED,
-                    "multi-layer cloth with normal untangling is not part of this build; disable it with "
-                    "set_self_collision_options(..., enable_normal_untangling=False) or disable self-collision");
     const uint32_t iters = s->iters_effective();
     if (s->fused_active() && iters > 0) {  // host bookkeeping of the buffer parity the graph bakes in
         s->last_p = iters & 1u;
@@ -751,6 +773,12 @@ static int finish_create(tc_sim *s, const tc_sim_params &pp, tc_sim **out) {
     UP(s->star, star.data(), star.size());
     UP(s->edges, edg.data(), edg.size());
     UP(s->edge_bbits, m.e_bbits.data(), m.e_bbits.size());
+    if (m.has_multiple_layers) {  // the untangling pass's inputs
+        UP(s->layer_d, m.layer.data(), m.layer.size());
+        AL(s->normals, nv);
+        UP(s->adj_off, m.adj_off.data(), m.adj_off.size());
+        UP(s->adj_idx, m.adj_idx.data(), m.adj_idx.size());
+    }
     UP(s->col_order, col_order.data(), col_order.size());
     AL(s->vkeys, nv); AL(s->vsorted, nv); AL(s->vcounts, T + (T + 1023) / 1024); AL(s->vstarts, T + 1);
     AL(s->ekeys, NE); AL(s->esorted, NE); AL(s->ecounts, T + (T + 1023) / 1024); AL(s->estarts, T + 1);
@@ -842,8 +870,6 @@ int tc_sim_step_single_substep(tc_sim *s, float dt_sub) {
 int tc_sim_substep_parts(tc_sim *s, float dt_sub, int collide, int self_collide, int update) {
     if (!s) return fail(TC_ERR_VALUE, "null simulator");
     if (s->mesh.nv == 0) return TC_OK;
-    if (self_collide && s->self_collision && s->normal_untangling && s->mesh.has_multiple_layers)
-        return fail(TC_ERR_UNSUPPORTED, "multi-layer cloth with normal untangling is not part of this build");
     s->rec_slot = -1;
     s->sub_first = s->sub_last = true;  // one substep: a frame of its own
     s->pair_dt_frame = dt_sub;
@@ -968,11 +994,14 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
                                                                s->single_sided_recovery, dt_sub, B, s->sdf_args()); })) != hipSuccess) return e;
         if (s->self_collision)
             if ((e = mark(6, [&] {
-                     hipError_t r = enqueue_hashes(s, Pc);
-                     if (r != hipSuccess) return r;
                      const tc::SelfColArgs sa = s->sc_args();
                      const tc::SelfColTopo t = s->topo();
+                     hipError_t r = sa.untangle ? tc::launch_compute_normals(s->stream, s->X.p, t, s->normals.p, n)
+                                                : hipSuccess;
+                     if (r != hipSuccess) return r;
+                     if ((r = enqueue_hashes(s, Pc)) != hipSuccess) return r;
                      if ((r = tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa)) != hipSuccess) return r;
+                     if ((r = tc::launch_sc_untangle(s->stream, Pc, s->X.p, t, s->accum.p, sa)) != hipSuccess) return r;
                      if ((r = tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa)) != hipSuccess) return r;
                      return tc::launch_sc_apply(s->stream, Pc, s->local_len.p, s->accum.p, sa, B);
                  })) != hipSuccess) return e;
@@ -1520,8 +1549,10 @@ static int pair_cache_alloc(tc_sim *s) {
     TC_HIP(s->pair_vt.alloc(slots));
     TC_HIP(s->pair_ee.alloc(slots));
     TC_HIP(s->pair_counters.alloc(4));
-    TC_HIP(s->adj_off.upload(m.adj_off.data(), m.adj_off.size(), s->stream));
-    TC_HIP(s->adj_idx.upload(m.adj_idx.data(), m.adj_idx.size(), s->stream));
+    if (!s->adj_off.p) {
+        TC_HIP(s->adj_off.upload(m.adj_off.data(), m.adj_off.size(), s->stream));
+        TC_HIP(s->adj_idx.upload(m.adj_idx.data(), m.adj_idx.size(), s->stream));
+    }
     TC_HIP(s->island.upload(m.island.data(), m.island.size(), s->stream));
     std::vector<uint4> none(slots, make_uint4(0xFFFFFFFFu, 0u, 0u, 0u));
     TC_HIP(hipMemcpyAsync(s->pair_vt.p, none.data(), sizeof(uint4) * slots, hipMemcpyHostToDevice, s->stream));

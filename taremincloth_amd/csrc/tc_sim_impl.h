@@ -291,6 +291,8 @@ struct tc_sim {
     DevBuf<uint4> pair_vt, pair_ee;     // nv * 8 slots each (quota slots in use per vertex)
     DevBuf<uint32_t> pair_counters;     // 4: V-T / E-E evaluated, V-T / E-E dropped
     DevBuf<uint32_t> adj_off, adj_idx, island;
+    DevBuf<uint32_t> layer_d;          // multi-layer cloth: layer ids, and the vertex normals of the
+    DevBuf<float4> normals;            // substep's start positions (compute_normals.wgsl)
     DevBuf<int4> jacobi_acc;      // per-vertex sums of solver_mode 1, zero between iterations
     bool fused_ready = false;     // plan built and uploaded
     std::string fused_reason;     // why the plan is unavailable
@@ -415,14 +417,18 @@ struct tc_sim {
         a.relief_factor = relief_factor; a.max_disp_ratio = max_disp_ratio;
         a.enable_relief = enable_relief; a.exclude_neighbors = exclude_neighbors;
         a.max_search_iterations = max_search_iterations;
+        a.untangle = untangle_active() ? 1 : 0;
         return a;
     }
     tc::SelfColTopo topo() const {
         tc::SelfColTopo t;
         t.local_len = local_len.p; t.th_off = th_off.p; t.th_idx = th_idx.p; t.th_rest = th_rest.p;
         t.star_off = star_off.p; t.star = star.p; t.edges = edges.p; t.edge_bbits = edge_bbits.p;
+        t.layer = layer_d.p; t.normals = normals.p; t.adj_off = adj_off.p; t.adj_idx = adj_idx.p;
         return t;
     }
+    // effective_normal_untangling (simulation/mod.rs:885-890): on, and the cloth has several layers
+    bool untangle_active() const { return normal_untangling && mesh.has_multiple_layers && layer_d.p && normals.p; }
     bool fused_active() const {
         if (!fused_ready) return false;
         if (!mesh.s_v0.empty() || mesh.has_seams) return false;  // sewing runs between the distance and bending colours

@@ -604,8 +604,6 @@ int tc_sim_ws_time(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
     for (int k = 0; k < 16; ++k) out[k] = 0.0;
     if (!s->ws_active()) return fail(TC_ERR_VALUE, "warm start is not enabled on this simulator");
     if (s->mesh.nv == 0 || reps == 0) return TC_OK;
-    if (s->self_collision && s->normal_untangling && s->mesh.has_multiple_layers)
-        return fail(TC_ERR_UNSUPPORTED, "multi-layer cloth with normal untangling is not part of this build");
     const tc::Mesh &m = s->mesh;
     const uint32_t nv = m.nv, B = s->block, R = s->ws_net ? s->ws_net->nl.rounds : 0;
     const bool net = s->ws_mode == 2;
@@ -656,7 +654,9 @@ int tc_sim_ws_time(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
                                      s->table_size, s->vhash(), s->ehash()));
             const tc::SelfColArgs sa = s->sc_args();
             const tc::SelfColTopo t = s->topo();
+            if (sa.untangle) TC_HIP(tc::launch_compute_normals(s->stream, s->X.p, t, s->normals.p, nv));
             TC_HIP(tc::launch_sc_solve(s->stream, Pc, s->X.p, t, s->vhash(), s->accum.p, sa));
+            TC_HIP(tc::launch_sc_untangle(s->stream, Pc, s->X.p, t, s->accum.p, sa));
             TC_HIP(tc::launch_sc_solve_ee(s->stream, Pc, s->X.p, t, s->ehash(), s->accum.p, sa));
             TC_HIP(tc::launch_sc_apply(s->stream, Pc, s->local_len.p, s->accum.p, sa, B));
         }

@@ -297,7 +297,7 @@ def test_gpu_pair_lists_bit_exact(require_gpu, scene, margin_mode):
     for s in (g, o):
         s.set_gravity(0.0, 0.0, -9.81)
         s.set_positions_and_velocities(pos, vel)
-        # two separate sheets count as layers: the multi-layer untangling branches are not built
+        # two separate sheets count as layers; these lists are compared without the untangling branches
         s.set_self_collision_options(0.2, 0.2, True, False, 256)
     _say(f"lists {scene} margin_mode={margin_mode}: {len(pos)} vertices, quota {int(st[2])}")
     held = [0, 0]
