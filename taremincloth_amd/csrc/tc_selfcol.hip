@@ -346,7 +346,9 @@ __device__ __forceinline__ void sc_walk(uint2 *queue, uint32_t &qn, bool flush, 
 }
 
 // V-V and V-T response of vertex `index` against vertex j and j's star, for a
-// pair that passed the cell and reach tests (self_collision.wgsl:388-681).
+// pair that passed the cell and reach tests (self_collision.wgsl:388-681).  UT: the multi-layer
+// untangling exemptions (a separate instantiation keeps single-layer cloth at its register count).
+template <bool UT>
 __device__ __forceinline__ void vv_pair(const float4 *__restrict__ P, const float4 *__restrict__ X,
                                         const SelfColTopo &T, Accum *__restrict__ acc, const SelfColArgs &A,
                                         uint32_t index, uint32_t j) {
@@ -372,7 +374,7 @@ __device__ __forceinline__ void vv_pair(const float4 *__restrict__ P, const floa
     const float wi = qi.w, wj = qj.w;
     const bool both_dynamic = wj > 0.0f;
     const float eff_sq = eff * eff;
-    const bool ut = A.untangle != 0;
+    constexpr bool ut = UT;
     const uint32_t layer_i = ut ? T.layer[index] : 0u, layer_j = ut ? T.layer[j] : 0u;
     // 1. vertex-vertex
     if (!pinned_i && (!both_dynamic || index < j)) {
@@ -541,6 +543,7 @@ __device__ __forceinline__ void vv_pair(const float4 *__restrict__ P, const floa
 // 678-681): every lane keeps the first minimum of its cell's bucket, and the half-wave reduction
 // keeps the minimum by (distance^2, cell in the reference's x-outer order), which is the first
 // minimum of the reference's sequential walk.
+template <bool UT>
 __global__ void __launch_bounds__(SC_BLOCK) k_sc_solve(const float4 *__restrict__ P, const float4 *__restrict__ X,
                                                        SelfColTopo T, const uint32_t *__restrict__ starts,
                                                        const uint32_t *__restrict__ sorted, Accum *__restrict__ acc,
@@ -569,7 +572,7 @@ __global__ void __launch_bounds__(SC_BLOCK) k_sc_solve(const float4 *__restrict_
                 const uint32_t h = hash_cell(nc, A.table);
                 q = starts[h];
                 qe = starts[h + 1];
-                if (A.untangle) {
+                if (UT) {
                     layer_i = T.layer[index];
                     track = !(qi.w <= 0.0f) && layer_i > 0u;
                 }
@@ -587,7 +590,7 @@ __global__ void __launch_bounds__(SC_BLOCK) k_sc_solve(const float4 *__restrict_
             const float dsq = dot(dvv, dvv);
             const float thick_j = xj.w, llen_j = T.local_len[j];
             const float eff = fmaxf(thick_i + thick_j, fminf(llen_i, llen_j) * 0.06f);
-            if (track && dsq < best_dsq && T.layer[j] < layer_i) {
+            if (UT && track && dsq < best_dsq && T.layer[j] < layer_i) {
                 const float r = eff * 2.5f + llen_j * 1.3f;
                 if (dsq <= r * r && !within_two_hops(T, index, j)) { best_dsq = dsq; best_j = j; }
             }
@@ -595,8 +598,8 @@ __global__ void __launch_bounds__(SC_BLOCK) k_sc_solve(const float4 *__restrict_
             return dsq <= max_geom * max_geom ? j : SC_NONE;
         };
         sc_walk(s_queue[wv], qn, g + 1u == SC_GROUPS, lane, index, q, qe, accept,
-                [&](uint32_t i, uint32_t j) { vv_pair(P, X, T, acc, A, i, j); });
-        if (A.untangle) {  // the vertex's nearest candidate over its 27 cells (lanes of one half)
+                [&](uint32_t i, uint32_t j) { vv_pair<UT>(P, X, T, acc, A, i, j); });
+        if (UT) {  // the vertex's nearest candidate over its 27 cells (lanes of one half)
             uint32_t kd = __float_as_uint(best_dsq);
             uint32_t ko = ci < 27u ? 9u * (ci % 3u) + 3u * ((ci / 3u) % 3u) + ci / 9u : 31u;
             for (uint32_t o = 16u; o != 0u; o >>= 1) {
@@ -1157,8 +1160,12 @@ __global__ void __launch_bounds__(256) k_sc_apply(float4 *__restrict__ P, const 
 hipError_t launch_sc_solve(hipStream_t s, const float4 *P, const float4 *X, const SelfColTopo &t, const HashBufs &vh,
                            Accum *acc, const SelfColArgs &a) {
     if (!a.nv) return hipSuccess;
-    hipLaunchKernelGGL(k_sc_solve, dim3(grid_for(a.nv, SC_PER_BLOCK)), dim3(SC_BLOCK), 0, s, P, X, t, vh.starts,
-                       vh.sorted, acc, a);
+    if (a.untangle)
+        hipLaunchKernelGGL(k_sc_solve<true>, dim3(grid_for(a.nv, SC_PER_BLOCK)), dim3(SC_BLOCK), 0, s, P, X, t,
+                           vh.starts, vh.sorted, acc, a);
+    else
+        hipLaunchKernelGGL(k_sc_solve<false>, dim3(grid_for(a.nv, SC_PER_BLOCK)), dim3(SC_BLOCK), 0, s, P, X, t,
+                           vh.starts, vh.sorted, acc, a);
     return hipGetLastError();
 }
 
