@@ -434,6 +434,270 @@ def batch_bench(T, frames: int, warmup: int, cpu_sample: bool):
     return out
 
 
+def _timed_frames(sim, frames: int, per_frame: int = 10, warmup: int = 3, before_frame=None):
+    """Substeps/s of `frames` frames of `per_frame` substeps after `warmup` untimed frames;
+    before_frame(k), when given, runs ahead of frame k (pose or pin updates) inside the timing."""
+    dt_frame = 1.0 / 60.0
+    for k in range(warmup):
+        if before_frame:
+            before_frame(k)
+        sim.step(dt_frame, per_frame)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for k in range(frames):
+        if before_frame:
+            before_frame(warmup + k)
+        sim.step(dt_frame, per_frame)
+    sim.synchronize()
+    el = time.perf_counter() - t0
+    return frames * per_frame / el
+
+
+def _pair(rate_on: float, rate_off: float) -> dict:
+    return {"substeps_per_s": rate_on, "without_feature_substeps_per_s": rate_off,
+            "feature_ms_per_substep": 1e3 / rate_on - 1e3 / rate_off}
+
+
+def features_bench(T, frames: int = 30):
+    """The widened rows of the hot-path table, each on its own scene, with and without the feature:
+    dynamic pins, the Jacobi solver mode, the pair cache, multi-layer untangling, collider-edge
+    collision, the wrinkle field, sewing, the bone-SDF bake and the dynamic bone SDF."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import cases
+    out = {}
+    per_frame = 10
+
+    def cloth(n, spacing, height, seed, **kw):
+        pos, edges, faces = grid_512(n, spacing, height, seed=seed)
+        inv = T.compute_combined_inv_masses(pos, faces, areal_density=0.15)
+        sim = T.ClothSimulator(pos, edges, faces, inv, **kw)
+        sim.set_gravity(0.0, 0.0, -9.81)
+        sim.set_compliances_raw(1e-8, 1e-6, 1e-5)
+        return sim, pos, edges, faces, inv
+
+    # dynamic pins: a 256x256 cloth (1 cm) whose first row is held by 256 pins swinging each frame
+    def pins_scene(on):
+        sim, pos, _, _, _ = cloth(256, 0.01, 0.5, 11, thickness=0.005)
+        sim.set_enable_self_collision(False)
+        sim.add_sphere_collider([0.0, 0.0, 0.0], 0.3, 0.3, 0.0)
+        sim.set_solver_iterations(20)
+        row = np.arange(256, dtype=np.int64)
+
+        def move(k):
+            if on:
+                tgt = pos[:256].copy()
+                tgt[:, 2] += np.float32(0.05 * np.sin(0.2 * k))
+                sim.set_pins_batch(row, tgt, np.ones(256, np.float32))
+        return _timed_frames(sim, frames, per_frame, before_frame=move)
+    out["dynamic_pins"] = dict(_pair(pins_scene(True), pins_scene(False)),
+                               workload="256x256 cloth (1 cm) on a sphere, 256 pins re-set every frame, 10 x 20")
+
+    # solver_mode 1 (fixed-point Jacobi distance pass) against the coloured per-iteration path
+    def jacobi_scene(mode):
+        sim, *_ = cloth(256, 0.01, 0.5, 12, thickness=0.005)
+        sim.set_enable_self_collision(False)
+        sim.add_sphere_collider([0.0, 0.0, 0.0], 0.3, 0.3, 0.0)
+        sim.set_solver_iterations(20)
+        sim.set_solver_mode(mode)
+        return _timed_frames(sim, frames, per_frame)
+    out["solver_mode_1"] = dict(_pair(jacobi_scene(1), jacobi_scene(0)),
+                                workload="256x256 cloth (1 cm) on a sphere, 10 x 20; the baseline is solver_mode 0")
+
+    # pair cache (self_collision_algorithm 1) against DIRECT on the 256x256 self-collision drop
+    def selfcol_scene(pair_cache):
+        kw = dict(thickness=0.002, enable_pair_cache=True) if pair_cache else dict(thickness=0.002)
+        sim, *_ = cloth(256, 0.004, 0.5, 5, **kw)
+        sim.add_sphere_collider([0.0, 0.0, 0.0], 0.3, 0.3, 0.0)
+        sim.add_capsule_collider([-0.3, 0.35, 0.1], [0.3, 0.35, 0.1], 0.1, 0.3, 0.0)
+        sim.set_enable_self_collision(True)
+        sim.set_hash_cell_size(0.004)
+        sim.set_solver_iterations(20)
+        return _timed_frames(sim, frames, per_frame)
+    out["pair_cache"] = dict(_pair(selfcol_scene(True), selfcol_scene(False)),
+                             workload="256x256 cloth (4 mm) with self-collision on a sphere and a capsule, 10 x 20; "
+                                      "the baseline is the DIRECT algorithm")
+
+    # multi-layer untangling: two 128x128 sheets (islands, hence two layers) 3 mm apart falling together
+    def layers_scene(untangle):
+        p1, e1, f1 = grid_512(128, 0.01, 0.5, seed=21)
+        p2 = p1 + np.float32([0.0, 0.0, 0.003])
+        pos = np.vstack([p1, p2])
+        edges = np.vstack([e1, e1 + len(p1)])
+        faces = np.vstack([f1, f1 + len(p1)])
+        inv = T.compute_combined_inv_masses(pos, faces, areal_density=0.15)
+        sim = T.ClothSimulator(pos, edges, faces, inv, thickness=0.002)
+        sim.set_gravity(0.0, 0.0, -9.81)
+        sim.set_compliances_raw(1e-8, 1e-6, 1e-5)
+        sim.add_sphere_collider([0.0, 0.0, 0.0], 0.3, 0.3, 0.0)
+        sim.set_enable_self_collision(True)
+        sim.set_self_collision_options(0.2, 0.2, True, untangle, 256)
+        sim.set_solver_iterations(20)
+        return _timed_frames(sim, frames, per_frame)
+    out["untangling"] = dict(_pair(layers_scene(True), layers_scene(False)),
+                             workload="two 128x128 sheets (1 cm, 3 mm apart, two layers) on a sphere with "
+                                      "self-collision, 10 x 20; the baseline has untangling off")
+
+    # collider-edge collision: a 128x128 cloth on the icosphere-4 mesh collider (5,120 triangles)
+    v4, f4 = cases.icosphere(4, radius=0.3, noise_amp=0.05, seed=0)
+    tris4 = v4[f4.astype(np.int64)]
+
+    def edges_scene(on):
+        sim, *_ = cloth(128, 0.01, 0.45, 3, thickness=0.005)
+        sim.set_enable_self_collision(False)
+        sim.set_mesh_collider_triangles(tris4, friction=0.4, thickness=0.002, restitution=0.0)
+        sim.set_enable_edge_collision(on)
+        sim.set_solver_iterations(20)
+        return _timed_frames(sim, frames, per_frame)
+    out["edge_collision"] = dict(_pair(edges_scene(True), edges_scene(False)),
+                                 workload=f"128x128 cloth (1 cm) on an icosphere mesh collider ({len(f4)} triangles, "
+                                          f"{3 * len(f4) // 2} unique edges), 10 x 20")
+
+    # wrinkle field: a cloth tube (96 around x 64 along) around a bone with a random RGBA8 texture
+    def wrinkle_scene(on):
+        n_around, n_along, radius, length = 96, 64, 0.09, 0.3
+        th = np.arange(n_around) * 2 * np.pi / n_around
+        zs = np.linspace(-length / 2, length / 2, n_along)
+        pos = np.array([[radius * np.cos(t), radius * np.sin(t), z] for z in zs for t in th], np.float32)
+        idx = lambda i, j: j * n_around + (i % n_around)
+        edges, faces = set(), []
+        for j in range(n_along - 1):
+            for i in range(n_around):
+                a, b, c, d = idx(i, j), idx(i + 1, j), idx(i, j + 1), idx(i + 1, j + 1)
+                faces += [[a, b, d], [a, d, c]]
+                for u, v in ((a, b), (a, c), (a, d), (b, d), (c, d)):
+                    edges.add((min(u, v), max(u, v)))
+        e = np.array(sorted(edges), np.uint32)
+        f = np.array(faces, np.uint32)
+        inv = T.compute_combined_inv_masses(pos, f, areal_density=0.15)
+        sim = T.ClothSimulator(pos, e, f, inv, thickness=0.003)
+        sim.set_gravity(0.0, 0.0, -9.81)
+        sim.set_compliances_raw(1e-7, 1e-6, 1e-5)
+        sim.set_enable_self_collision(False)
+        sim.set_solver_iterations(20)
+        if on:
+            tex = np.random.default_rng(5).integers(0, 256, size=(8, 16, 4), dtype=np.uint8)
+            sim.set_enable_wrinkle_field(True)
+            sim.set_wrinkle_field_params([0.0, 0.0, 0.0], [0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0],
+                                         influence_radius=0.05, bone_radius=0.05, stiffness=3.0, blend_weight=0.9,
+                                         valley_window=0.01, crest_window=0.02)
+            sim.set_wrinkle_field_texture_2d(16, 8, tex.tobytes(), -0.2, 0.2, 0.02, 0.16)
+        return _timed_frames(sim, frames, per_frame)
+    out["wrinkle_field"] = dict(_pair(wrinkle_scene(True), wrinkle_scene(False)),
+                                workload="cloth tube (96 x 64 vertices) around a bone with a 16x8 RGBA8 wrinkle texture, "
+                                         "10 x 20")
+
+    # sewing: two 128 x 64 panels 2 cm apart, sewn along their facing edges (128 springs), shrinking
+    def sewing_scene(on):
+        pa, ea, fa = grid_512(128, 0.01, 0.5, seed=31)
+        pa = pa.reshape(128, 128, 3)[:64].reshape(-1, 3)
+        keep = np.arange(128 * 128).reshape(128, 128)[:64].ravel()
+        remap = -np.ones(128 * 128, np.int64)
+        remap[keep] = np.arange(len(keep))
+        ea = remap[ea.astype(np.int64)]
+        ea = ea[(ea >= 0).all(1)].astype(np.uint32)
+        fa = remap[fa.astype(np.int64)]
+        fa = fa[(fa >= 0).all(1)].astype(np.uint32)
+        pb = pa + np.float32([0.0, 0.66, 0.0])
+        pos = np.vstack([pa, pb])
+        edges = np.vstack([ea, ea + len(pa)])
+        faces = np.vstack([fa, fa + len(pa)])
+        sew = np.stack([np.arange(63 * 128, 64 * 128), len(pa) + np.arange(128)], 1).astype(np.uint32) if on else None
+        inv = T.compute_combined_inv_masses(pos, faces, areal_density=0.15)
+        sim = T.ClothSimulator(pos, edges, faces, inv, sewing_springs=sew, thickness=0.003)
+        sim.set_gravity(0.0, 0.0, -9.81)
+        sim.set_compliances_raw(1e-8, 1e-6, 1e-5)
+        sim.add_sphere_collider([0.0, 0.0, 0.0], 0.3, 0.3, 0.0)
+        sim.set_enable_self_collision(False)
+        sim.set_solver_iterations(20)
+        return _timed_frames(sim, frames, per_frame)
+    out["sewing"] = dict(_pair(sewing_scene(True), sewing_scene(False)),
+                         workload="two 128x64 panels (1 cm) sewn along their facing edges (128 springs, shrink speed "
+                                  "1), on a sphere, 10 x 20")
+
+    # bone-SDF atlas bake: 64 boxes on 64 bones at 64^3 per tile, and a lumpy icosphere on 4 bones
+    n = 64
+    h = 0.0625
+    box = np.array([[-h, -h, -h], [h, -h, -h], [h, h, -h], [-h, h, -h],
+                    [-h, -h, h], [h, -h, h], [h, h, h], [-h, h, h]], np.float32)
+    box_t = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4],
+                      [1, 2, 6], [1, 6, 5], [2, 3, 7], [2, 7, 6], [3, 0, 4], [3, 4, 7]], np.int32)
+    bv = np.vstack([box + np.float32([b, 0, 0]) for b in range(n)])
+    bt = np.vstack([box_t + 8 * b for b in range(n)]).astype(np.int32)
+    bw = np.zeros((n, 8 * n), np.float32)
+    bm = np.stack([np.eye(4, dtype=np.float32)] * n)
+    for b in range(n):
+        bw[b, 8 * b:8 * b + 8] = 1.0
+        bm[b, 0, 3] = -float(b)
+    t0 = time.perf_counter()
+    g = T.bake_bone_sdf_gpu(bv, bt, [str(b) for b in range(n)], list(bw), bm, resolution=64, margin=0.25,
+                            thickness=0.0, close_open_boundaries=False)
+    call_ms = (time.perf_counter() - t0) * 1e3
+    vox = g["width"] * g["height"] * g["depth"]
+    out["bone_sdf_bake"] = {"workload": f"64 boxes on 64 bones, 64^3 texels per tile ({g['width']}x{g['height']}x"
+                                        f"{g['depth']} atlas, 12 triangles per bone)",
+                            "kernel_ms": g["kernel_ms"], "call_ms": call_ms,
+                            "texels_per_s": vox / (g["kernel_ms"] * 1e-3),
+                            "point_triangle_tests_per_s": n * 64 ** 3 * 12 / (g["kernel_ms"] * 1e-3)}
+
+    # dynamic bone SDF: a skinned icosphere-4 on 4 bones (res 32 tiles) re-baked every frame under a
+    # 128x128 cloth, against the same atlas kept static
+    vs, fs = cases.icosphere(4, radius=0.25, noise_amp=0.05, seed=1)
+    centres = np.array([[0.1, 0, 0], [-0.1, 0, 0], [0, 0.1, 0], [0, -0.1, 0]], np.float64)
+    d2 = ((vs[:, None, :].astype(np.float64) - centres[None]) ** 2).sum(-1)
+    wv = np.exp(-d2 / 0.01)
+    wv = (wv / wv.sum(1, keepdims=True)).astype(np.float32)
+    binds = np.stack([np.eye(4, dtype=np.float32)] * 4)
+    for b in range(4):
+        binds[b, :3, 3] = -centres[b]
+    st = T.bake_bone_sdf_gpu(vs, fs.astype(np.int32), ["a", "b", "c", "d"], list(wv.T), binds, resolution=32,
+                             margin=0.2, thickness=0.004)
+    res = int(round(float(st["bone_infos"][0][7])))
+    src, tw = [], []
+    for b in range(4):
+        for tri in fs:
+            src.append([tri[0], tri[1], tri[2], b])
+            tw.append(wv[tri, b])
+    order = np.argsort(-wv, axis=1)[:, :4]
+    bi = order.astype(np.uint32)
+    bwv = np.take_along_axis(wv, order, 1).astype(np.float32)
+
+    def pose(k):
+        world, invw = [], []
+        for b in range(4):
+            m = np.linalg.inv(binds[b].astype(np.float64))
+            r = np.eye(4)
+            a = 0.02 * np.sin(0.3 * k) if b == 0 else 0.0
+            r[:3, :3] = [[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]]
+            wm = r @ m
+            world.append(wm.T.astype(np.float32))
+            invw.append(np.linalg.inv(wm).T.astype(np.float32))
+        return np.stack(world), np.stack(invw)
+
+    def dyn_scene(dynamic):
+        sim, *_ = cloth(128, 0.008, 0.4, 41, thickness=0.004)
+        sim.set_enable_self_collision(False)
+        sim.set_solver_iterations(20)
+        if dynamic:
+            sim.setup_dynamic_bone_sdf(st["width"], st["height"], st["depth"], res, st["bone_infos"], vs, bi, bwv,
+                                       np.array(src, np.uint32), np.array(tw, np.float32), binds)
+        else:
+            sim.set_bone_sdf_colliders(st["width"], st["height"], st["depth"], st["texture_bytes"], st["bone_infos"])
+
+        def move(k):
+            sim.update_bone_transforms(*pose(k))
+        rate = _timed_frames(sim, frames, per_frame, before_frame=move)
+        return rate, (sim.dynamic_bone_sdf_stats() if dynamic else None)
+    r_dyn, stats = dyn_scene(True)
+    r_static, _ = dyn_scene(False)
+    out["dynamic_bone_sdf"] = dict(_pair(r_dyn, r_static),
+                                   workload=f"skinned icosphere-4 ({len(vs)} vertices, {len(fs)} triangles) on 4 bones, "
+                                            f"{res}^3 tiles ({st['width']}x{st['height']}x{st['depth']} atlas), "
+                                            f"re-skinned and re-baked every frame under a 128x128 cloth, 10 x 20; the "
+                                            f"baseline keeps the same atlas static",
+                                   feature_ms_per_frame=(1e3 / r_dyn - 1e3 / r_static) * per_frame, stats=stats)
+    return out
+
+
 def kernel_pass(T, kind: str, substeps: int = 3):
     """One kernel path alone, launched kernel by kernel (no graphs), for counter passes:
     'tile': plain 64-iteration substeps of the 512x512 workload (k_tile_iteration); 'tile_fast': the
@@ -519,6 +783,10 @@ def main():
                     help="measure only the batch of 64 instances and print its JSON object (for profiler runs)")
     ap.add_argument("--sdf-only", action="store_true",
                     help="measure only the mesh SDF bakes and print their JSON object (for profiler runs)")
+    ap.add_argument("--features-only", action="store_true",
+                    help="only the widened rows: pins, Jacobi mode, pair cache, untangling, edge collision, wrinkle "
+                         "field, sewing, bone-SDF bake, dynamic bone SDF (each with and without the feature)")
+    ap.add_argument("--no-features", action="store_true", help="skip the widened rows' measurements")
     ap.add_argument("--kernel-pass", choices=["tile", "tile_fast", "round", "batch"],
                     help="run one kernel path alone, kernel by kernel, and print what ran (for counter passes)")
     args = ap.parse_args()
@@ -546,6 +814,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps(batch_bench(T, 30, 5, not args.no_cpu_baseline)), flush=True)
+        return
+    if args.features_only:
+        print(json.dumps(features_bench(T)), flush=True)
         return
 
     n = args.grid
@@ -779,6 +1050,7 @@ def main():
     drape = None if args.no_drape else drape_bench(T)
     selfcol = None if args.no_selfcol else selfcol_bench(T)
     cfg1 = None if args.no_cfg1 else config1_bench(T, not args.no_cpu_baseline)
+    features = None if (args.no_features or args.light) else features_bench(T)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -863,6 +1135,7 @@ def main():
         "batch": batch,
         "mesh_collider_drape": drape,
         "self_collision_drop": selfcol,
+        "widened_rows": features,
     }
     print(json.dumps(line), flush=True)
 
