@@ -383,11 +383,18 @@ int replay_part(tc_sim *s, tc_sim::PartGraph &pg, const uint64_t key[4], Enqueue
 
 // A frame of warm-started substeps with a fallback, from graphs of the substeps' parts.  Captured
 // whole, a substep carries every fallback chunk, and a chunk whose gate has passed still costs its
-// launches (about 2.7 us each for the per-iteration kernel, 56 of them); here the host waits for the
-// first check and launches chunks only while the gate stays open.  The kernels that run, and their
-// order, are those of the captured frame.
+// launches (about 2.7 us each for the per-iteration kernel, 56 of them); here the host launches
+// chunks only while the gate stays open, one chunk ahead of the check it waits for: the next chunk
+// is queued before the host waits for the previous check, so the device never idles between
+// chunks, and a queued chunk whose gate has closed returns at once (its kernels are gated on the
+// device flag).  The chunks that run, and their order, are those of the captured frame.  The host
+// flag is written by every check, a closed gate stays closed, and everything queued before a
+// substep's head has finished when the host reads the head's flag, so any value read after the
+// awaited check reflects that check or a later one of the same substep.
 int run_frame_host_gated(tc_sim *s, float dt, uint32_t substeps) {
     if (!s->gate_host) TC_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->gate_host), 16, hipHostMallocDefault));
+    for (hipEvent_t &ev : s->gate_ev)
+        if (!ev) TC_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     const float dt_sub = dt / float(substeps);
     uint32_t dtb;
     std::memcpy(&dtb, &dt_sub, 4);
@@ -407,18 +414,34 @@ int run_frame_host_gated(tc_sim *s, float dt, uint32_t substeps) {
         if (r != TC_OK) return r;
         // the buffers holding P and lambda after the warm iterations (chunks have an even size)
         p0 = l0 = (fused && s->ws_iters) ? (s->ws_iters & 1u) : 0u;
-        TC_HIP(hipStreamSynchronize(s->stream));
-        for (uint32_t done = 0; done < s->ws_fallback && *s->gate_host != 0u;) {
-            const uint32_t cnt = std::min(chunk, s->ws_fallback - done);
-            const bool last = done + cnt >= s->ws_fallback;
+        TC_HIP(hipEventRecord(s->gate_ev[0], s->stream));
+        uint32_t launched = 0, ev_next = 1;
+        bool last_queued = s->ws_fallback == 0;
+        // queues the next chunk and records its completion event; returns the event of the chunk
+        auto queue_chunk = [&](hipEvent_t &out) -> int {
+            const uint32_t cnt = std::min(chunk, s->ws_fallback - launched);
+            const bool last = launched + cnt >= s->ws_fallback;
             const uint64_t kc[4] = {dtb, cnt, fused | (uint64_t(p0) << 1), s->epoch};
             uint32_t pc = p0, lc = l0;
-            r = replay_part(s, last ? s->pg_last : s->pg_chunk, kc,
-                            [&] { return enqueue_fallback_chunk(s, dt_sub, cnt, last, pc, lc); });
-            if (r != TC_OK) return r;
-            done += cnt;
-            if (last) break;
-            TC_HIP(hipStreamSynchronize(s->stream));
+            const int rr = replay_part(s, last ? s->pg_last : s->pg_chunk, kc,
+                                       [&] { return enqueue_fallback_chunk(s, dt_sub, cnt, last, pc, lc); });
+            if (rr != TC_OK) return rr;
+            launched += cnt;
+            last_queued = last;
+            out = s->gate_ev[ev_next];
+            ev_next = ev_next == 1u ? 2u : 1u;
+            TC_HIP(hipEventRecord(out, s->stream));
+            return TC_OK;
+        };
+        hipEvent_t awaited = s->gate_ev[0], queued = nullptr;
+        if (!last_queued && (r = queue_chunk(queued)) != TC_OK) return r;
+        for (;;) {
+            TC_HIP(hipEventSynchronize(awaited));
+            if (*s->gate_host == 0u || last_queued) break;  // the gate closed, or the budget is queued
+            hipEvent_t next = nullptr;
+            if ((r = queue_chunk(next)) != TC_OK) return r;
+            awaited = queued;
+            queued = next;
         }
         const uint64_t kt[4] = {dtb, uint64_t(p0) | (uint64_t(l0) << 1) | (kind << 2), fused, s->epoch};
         r = replay_part(s, s->pg_tail, kt, [&] { return enqueue_substep_tail(s, dt_sub, p0, l0, true, true, true); });

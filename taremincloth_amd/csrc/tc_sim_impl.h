@@ -348,6 +348,7 @@ struct tc_sim {
     };
     PartGraph pg_head, pg_chunk, pg_last, pg_tail;
     uint32_t *gate_host = nullptr;  // pinned host copy of the gate's fallback flag
+    hipEvent_t gate_ev[3] = {nullptr, nullptr, nullptr};  // completion of the head and of queued chunks
     uint64_t epoch = 1;  // bumped whenever a parameter or buffer baked into the graph changes
     bool use_graph = true;  // TCLOTH_USE_GRAPH=0 launches kernel by kernel (profilers)
     TcProfiler prof;        // set_profiling_enabled: frames launch kernel by kernel with pass events
@@ -389,6 +390,8 @@ struct tc_sim {
         pg_last.clear();
         pg_tail.clear();
         if (gate_host) (void)hipHostFree(gate_host);
+        for (hipEvent_t &ev : gate_ev)
+            if (ev) (void)hipEventDestroy(ev);
         for (HostReadback *b : {&fb[0], &fb[1], &as[0], &as[1]}) {
             if (b->host) (void)hipHostFree(b->host);
             if (b->ev) (void)hipEventDestroy(b->ev);
