@@ -82,6 +82,8 @@ hipError_t launch_residual(hipStream_t s, const float4 *P, const float4 *X, cons
 // --- one iteration per launch over cone tiles (tc_fused.hip, plan from tc_tiles.cpp) ---
 struct FusedArgs {
     uint32_t ntiles, nphases, nd_phases, max_ext;
+    uint32_t ns_phases;                          // sewing colours after the distance colours
+    float sew_comp, sew_lock, sew_lock_dist;     // as SewArgs::compliance, lock, lock_dist
     // 1: fast projection (hardware reciprocal square root and reciprocal, one Newton-Raphson step
     // each, instead of the IEEE square root and divisions); 0: the exact operation sequence
     uint32_t fast;
@@ -90,11 +92,15 @@ struct FusedArgs {
 };
 struct FusedPlanDev {
     const uint32_t *ext_off, *ext_vid, *n_own, *ph_off, *items;
+    uint4 *srec;          // sewing records (current rest length written by the lock), or null
+    const uint4 *brec;    // bending records: the rest length of seam-gated items
+    const uint4 *bgate;   // seam gate table (hinge vertices, gap threshold), or null
 };
+// Sewing lambdas (slam_in / slam_out) are double-buffered like the others.
 hipError_t launch_tile_iteration(hipStream_t s, const float4 *Pin, float4 *Pout, const float4 *X,
                                  const FusedPlanDev &p, const float *dlam_in, float *dlam_out, const float *blam_in,
-                                 float *blam_out, const FusedArgs &fa, const SubstepArgs &a, bool first,
-                                 uint32_t block);
+                                 float *blam_out, const float *slam_in, float *slam_out, const FusedArgs &fa,
+                                 const SubstepArgs &a, bool first, uint32_t block);
 hipError_t set_tile_lds_limit(size_t bytes);
 // fills the alpha~ tables of `fa` for substep length dt_sub (same float operations as the kernels)
 void fused_alphas(FusedArgs &fa, float dt_sub);

@@ -96,11 +96,11 @@ hipError_t enqueue_sew_shrink(tc_sim *s, float dt_sub) {
     return tc::launch_sew_shrink(s->stream, s->srec.p, s->slam.p, uint32_t(s->mesh.s_v0.size()), sew_args(s, dt_sub));
 }
 
-hipError_t enqueue_sewing(tc_sim *s, float4 *P, float dt_sub) {
+hipError_t enqueue_sewing(tc_sim *s, float4 *P, float dt_sub, float *lam) {
     const tc::Mesh &m = s->mesh;
     const tc::SewArgs a = sew_args(s, dt_sub);
     for (size_t c = 0; c < m.s_col.count.size(); ++c) {
-        const hipError_t e = tc::launch_sewing(s->stream, P, s->X.p, s->srec.p, s->slam.p, m.s_col.offset[c],
+        const hipError_t e = tc::launch_sewing(s->stream, P, s->X.p, s->srec.p, lam, m.s_col.offset[c],
                                                m.s_col.count[c], a, s->block);
         if (e != hipSuccess) return e;
     }
@@ -121,7 +121,9 @@ int upload_sewing(tc_sim *s) {
         }
         TC_HIP(s->srec.upload(rec.data(), n, s->stream));
         TC_HIP(s->slam.alloc(n));
+        TC_HIP(s->slam2.alloc(n));
         TC_HIP(hipMemsetAsync(s->slam.p, 0, 4 * n, s->stream));
+        TC_HIP(hipMemsetAsync(s->slam2.p, 0, 4 * n, s->stream));
     }
     if (m.has_seams) {
         const size_t nb = m.b_v0.size();

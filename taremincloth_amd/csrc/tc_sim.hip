@@ -63,8 +63,9 @@ hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t coun
             const int q = s->prof_begin("solver_iter");
             if (s->npins && (e = enqueue_pins(s, s->Pbuf(pc), a)) != hipSuccess) return e;
             e = tc::launch_tile_iteration(s->stream, s->Pbuf(pc), s->Pbuf(pc ^ 1u), s->X.p, pd, s->dlam_buf(lc),
-                                          s->dlam_buf(lc ^ 1u), s->blam_buf(lc), s->blam_buf(lc ^ 1u), fa, a,
-                                          first && it == 0, s->fused_block);
+                                          s->dlam_buf(lc ^ 1u), s->blam_buf(lc), s->blam_buf(lc ^ 1u),
+                                          s->slam_buf(lc), s->slam_buf(lc ^ 1u), fa, a, first && it == 0,
+                                          s->fused_block);
             s->prof_end(q);
             if (e != hipSuccess) return e;
             pc ^= 1u;
@@ -87,7 +88,7 @@ hipError_t enqueue_iterations(tc_sim *s, const tc::SubstepArgs &a, uint32_t coun
                 if (e != hipSuccess) return e;
             }
         }
-        if (!m.s_v0.empty() && (e = enqueue_sewing(s, s->P.p, a.dt)) != hipSuccess) return e;  // dispatch.rs:183-191
+        if (!m.s_v0.empty() && (e = enqueue_sewing(s, s->P.p, a.dt, s->slam.p)) != hipSuccess) return e;  // dispatch.rs:183-191
         for (size_t c = 0; c < m.b_col.count.size(); ++c) {
             e = tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p, m.b_col.offset[c],
                                    m.b_col.count[c], a, f, B, s->bgate.p);
@@ -322,7 +323,7 @@ hipError_t enqueue_substep_tail(tc_sim *s, float dt_sub, uint32_t pc, uint32_t l
     }
     if (!m.s_v0.empty()) {  // the final sewing pass before the velocity update (dispatch.rs:318-339)
         q = s->prof_begin("final_sewing");
-        e = enqueue_sewing(s, Pc, dt_sub);
+        e = enqueue_sewing(s, Pc, dt_sub, s->slam_buf(lc));  // the lambdas the iterations left
         s->prof_end(q);
         if (e != hipSuccess) return e;
     }
@@ -913,7 +914,8 @@ static int time_iterations(tc_sim *s, float dt_sub, uint32_t its, bool fused, do
         if (fused) {
             hipError_t r = tc::launch_tile_iteration(s->stream, s->Pbuf(pc), s->Pbuf(pc ^ 1u), s->X.p, s->plan_dev(),
                                                      s->dlam_buf(pc), s->dlam_buf(pc ^ 1u), s->blam_buf(pc),
-                                                     s->blam_buf(pc ^ 1u), s->fused_args(), a, it == 0, s->fused_block);
+                                                     s->blam_buf(pc ^ 1u), s->slam_buf(pc), s->slam_buf(pc ^ 1u),
+                                                     s->fused_args(), a, it == 0, s->fused_block);
             pc ^= 1u;
             return r;
         }
@@ -923,6 +925,10 @@ static int time_iterations(tc_sim *s, float dt_sub, uint32_t its, bool fused, do
                                                s->block);
             if (r != hipSuccess) return r;
         }
+        if (!m.s_v0.empty()) {
+            hipError_t r = enqueue_sewing(s, s->P.p, dt_sub, s->slam.p);
+            if (r != hipSuccess) return r;
+        }
         for (size_t c = 0; c < m.b_col.count.size(); ++c) {
             hipError_t r = tc::launch_bending(s->stream, s->P.p, s->X.p, s->brec.p, s->blam.p, m.b_col.offset[c],
                                               m.b_col.count[c], a, it == 0, s->block, s->bgate.p);
@@ -930,11 +936,18 @@ static int time_iterations(tc_sim *s, float dt_sub, uint32_t its, bool fused, do
         }
         return hipSuccess;
     };
+    // the sewing lock rewrites rest lengths: time on a copy of the records and put them back after
+    DevBuf<uint4> srec_keep;
+    const size_t ns = m.s_v0.size();
+    if (ns && e == hipSuccess) e = srec_keep.alloc(ns);
+    if (ns && e == hipSuccess) e = hipMemcpyAsync(srec_keep.p, s->srec.p, 16 * ns, hipMemcpyDeviceToDevice, s->stream);
     if (e == hipSuccess) e = one(0);  // warm-up
     if (e == hipSuccess) e = hipEventRecord(e0, s->stream);
     for (uint32_t it = 1; e == hipSuccess && it <= its; ++it) e = one(it);
     if (e == hipSuccess) e = hipEventRecord(e1, s->stream);
+    if (ns && e == hipSuccess) e = hipMemcpyAsync(s->srec.p, srec_keep.p, 16 * ns, hipMemcpyDeviceToDevice, s->stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     float ms = 0.0f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
@@ -985,7 +998,8 @@ int tc_sim_time_kernels(tc_sim *s, float dt_sub, uint32_t reps, double *out) {
                      for (uint32_t it = 0; it < s->iterations && r == hipSuccess; ++it) {
                          r = tc::launch_tile_iteration(s->stream, s->Pbuf(pc), s->Pbuf(pc ^ 1u), s->X.p, pd,
                                                        s->dlam_buf(pc), s->dlam_buf(pc ^ 1u), s->blam_buf(pc),
-                                                       s->blam_buf(pc ^ 1u), fa, a, it == 0, s->fused_block);
+                                                       s->blam_buf(pc ^ 1u), s->slam_buf(pc), s->slam_buf(pc ^ 1u),
+                                                       fa, a, it == 0, s->fused_block);
                          pc ^= 1u;
                      }
                      return r;

@@ -163,7 +163,7 @@ struct tc_sim {
     DevBuf<uint4> bgate;               // seam-gated bending: {v0, v1, gate distance bits, 1} per constraint (meshes with seams)
     // sewing (mesh.rs:51-60, sewing.wgsl, sewing_shrink.wgsl): {v0, v1, current rest bits, target rest bits}
     DevBuf<uint4> srec;
-    DevBuf<float> slam;
+    DevBuf<float> slam, slam2;         // sewing lambdas; the second buffer for the one-launch-per-iteration path
     float sew_stiffness = 10000.0f;    // simulator fields (mod.rs:637-640); the constructor's stiffness and
     bool sew_lock = true;              // lock flag are kept only in the records, which the solver does not read
     float sew_lock_dist = 0.02f;
@@ -434,7 +434,6 @@ struct tc_sim {
     bool untangle_active() const { return normal_untangling && mesh.has_multiple_layers && layer_d.p && normals.p; }
     bool fused_active() const {
         if (!fused_ready) return false;
-        if (!mesh.s_v0.empty() || mesh.has_seams) return false;  // sewing runs between the distance and bending colours
         if (solver_mode == 1 || solver_path == 1) return false;
         // one launch per iteration at every size: on the 1,985-vertex curtain it runs 2.5x the
         // substeps of one launch per colour (16 colour launches per iteration)
@@ -443,16 +442,21 @@ struct tc_sim {
     float4 *Pbuf(uint32_t k) const { return k ? Palt.p : P.p; }
     float *dlam_buf(uint32_t k) const { return k ? dlam2.p : dlam.p; }
     float *blam_buf(uint32_t k) const { return k ? blam2.p : blam.p; }
+    float *slam_buf(uint32_t k) const { return k ? slam2.p : slam.p; }
     tc::FusedArgs fused_args() const {
         tc::FusedArgs f{};
         f.ntiles = plan.ntiles; f.nphases = plan.nphases; f.nd_phases = plan.nd_phases; f.max_ext = plan.max_ext;
         f.fast = fast_projection ? 1u : 0u;
+        f.ns_phases = plan.ns_phases;
+        f.sew_comp = sew_stiffness >= 5000.0f ? 0.0f : 1.0f / std::max(sew_stiffness, 1.0f);  // dispatch.rs:29-33
+        f.sew_lock = sew_lock ? 1.0f : 0.0f;
+        f.sew_lock_dist = sew_lock_dist;
         for (size_t c = 0; c < plan.dcls_tc.size(); ++c) { f.dtc[c] = plan.dcls_tc[c]; f.dcc[c] = plan.dcls_cc[c]; }
         for (size_t c = 0; c < plan.bcls.size(); ++c) f.bcomp[c] = plan.bcls[c];
         return f;
     }
     tc::FusedPlanDev plan_dev() const {
-        return {pl_ext_off.p, pl_ext_vid.p, pl_n_own.p, pl_ph_off.p, pl_items.p};
+        return {pl_ext_off.p, pl_ext_vid.p, pl_n_own.p, pl_ph_off.p, pl_items.p, srec.p, brec.p, bgate.p};
     }
     tc::PairArgs pair_args() const {
         tc::PairArgs a;
@@ -471,7 +475,7 @@ struct tc_sim {
 // Wrinkle field after predict (tc_wrinkle.hip); nothing when off or without a texture.
 // sewing (tc_sewing.hip): the per-substep shrink (also clears the sewing lambdas) and one pass over the colours
 hipError_t enqueue_sew_shrink(tc_sim *s, float dt_sub);
-hipError_t enqueue_sewing(tc_sim *s, float4 *P, float dt_sub);
+hipError_t enqueue_sewing(tc_sim *s, float4 *P, float dt_sub, float *lam);
 int upload_sewing(tc_sim *s);
 hipError_t enqueue_wrinkle(tc_sim *s, float dt_sub);
 

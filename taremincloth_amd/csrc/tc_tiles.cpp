@@ -46,10 +46,13 @@ std::string build_tile_plan(const Mesh &m, uint32_t tile_vertices, uint32_t max_
     const uint32_t nv = m.nv;
     const size_t nd = m.d_v0.size(), nb = m.b_v0.size();
     if (nv == 0) return "empty mesh";
-    if (nd >= (1u << 26) || nb >= (1u << 26)) return "too many constraints for the item encoding";
-    // seam-gated bending reads its hinge vertices; this build generates none, but refuse if present
+    const size_t ns = m.s_v0.size();
+    if (nd >= (1u << 26) || nb >= (1u << 26) || ns >= (1u << 26)) return "too many constraints for the item encoding";
+    // seam-gated bending (gated only where the mesh has seams, as in the per-colour sweep) also reads
+    // its hinge vertices v0, v1: their values enter the cone, they are not written
+    auto gated = [&](size_t k) { return m.has_seams && m.b_pad0[k] > 0.5f && m.b_pad0[k] < 1.5f; };
     for (size_t k = 0; k < nb; ++k)
-        if (m.b_pad0[k] > 0.5f && m.b_pad0[k] < 1.5f) return "seam-gated bending is not supported by the fused path";
+        if (gated(k) && (m.b_v0[k] >= nv || m.b_v1[k] >= nv)) return "seam-gated bending with a hinge vertex out of range";
 
     // compliance classes
     std::vector<uint32_t> dcls(nd), bcls(nb);
@@ -75,21 +78,29 @@ std::string build_tile_plan(const Mesh &m, uint32_t tile_vertices, uint32_t max_
         bcls[k] = c;
     }
 
-    // phases of one iteration and, per phase, the constraint touching each vertex
-    const uint32_t ndp = uint32_t(m.d_col.count.size()), nbp = uint32_t(m.b_col.count.size());
-    const uint32_t L = ndp + nbp;
+    // phases of one iteration (distance colours, sewing colours, bending colours, the order of the
+    // per-colour sweep) and, per phase, the constraint writing each vertex
+    const uint32_t ndp = uint32_t(m.d_col.count.size()), nsp = uint32_t(m.s_col.count.size()),
+                   nbp = uint32_t(m.b_col.count.size());
+    const uint32_t nds = ndp + nsp, L = nds + nbp;
     P.nphases = L;
     P.nd_phases = ndp;
+    P.ns_phases = nsp;
     std::vector<uint32_t> touch(size_t(L) * nv, UINT32_MAX);
     for (uint32_t c = 0; c < ndp; ++c)
         for (uint32_t k = m.d_col.offset[c]; k < m.d_col.offset[c] + m.d_col.count[c]; ++k) {
             touch[size_t(c) * nv + m.d_v0[k]] = k;
             touch[size_t(c) * nv + m.d_v1[k]] = k;
         }
+    for (uint32_t c = 0; c < nsp; ++c)
+        for (uint32_t k = m.s_col.offset[c]; k < m.s_col.offset[c] + m.s_col.count[c]; ++k) {
+            touch[size_t(ndp + c) * nv + m.s_v0[k]] = k;
+            touch[size_t(ndp + c) * nv + m.s_v1[k]] = k;
+        }
     for (uint32_t c = 0; c < nbp; ++c)
         for (uint32_t k = m.b_col.offset[c]; k < m.b_col.offset[c] + m.b_col.count[c]; ++k) {
-            touch[size_t(ndp + c) * nv + m.b_v2[k]] = k;
-            touch[size_t(ndp + c) * nv + m.b_v3[k]] = k;
+            touch[size_t(nds + c) * nv + m.b_v2[k]] = k;
+            touch[size_t(nds + c) * nv + m.b_v3[k]] = k;
         }
 
     const uint32_t T = std::max<uint32_t>(1, tile_vertices);
@@ -110,13 +121,13 @@ std::string build_tile_plan(const Mesh &m, uint32_t tile_vertices, uint32_t max_
         for (uint32_t r = part_begin[t]; r < part_begin[t + 1]; ++r) tile_of[order[r]] = t;
 
     std::vector<uint32_t> vstamp(nv, UINT32_MAX), local(nv, 0);
-    std::vector<uint32_t> dsel(nd, UINT32_MAX), bsel(nb, UINT32_MAX);
+    std::vector<uint32_t> dsel(nd, UINT32_MAX), ssel(ns, UINT32_MAX), bsel(nb, UINT32_MAX);
     std::vector<std::vector<uint32_t>> phase_sel(L);
     std::vector<uint32_t> cone;
     P.ext_off.assign(P.ntiles + 1, 0);
     P.n_own.assign(P.ntiles, 0);
     P.ph_off.assign(size_t(P.ntiles) * (L + 1), 0);
-    P.plain_items = uint64_t(nd) + nb;
+    P.plain_items = uint64_t(nd) + ns + nb;
 
     for (uint32_t t = 0; t < P.ntiles; ++t) {
         cone.clear();
@@ -138,20 +149,24 @@ std::string build_tile_plan(const Mesh &m, uint32_t tile_vertices, uint32_t max_
         for (uint32_t c = L; c-- > 0;) {
             auto &sel = phase_sel[c];
             sel.clear();
-            const bool dist = c < ndp;
+            const uint32_t kind = c < ndp ? 0u : c < nds ? 1u : 2u;
             const size_t base = size_t(c) * nv;
             const size_t n_now = cone.size();  // vertices needed after phase c
             for (size_t q = 0; q < n_now; ++q) {
                 const uint32_t k = touch[base + cone[q]];
                 if (k == UINT32_MAX) continue;
-                uint32_t &st = dist ? dsel[k] : bsel[k];
+                uint32_t &st = kind == 0u ? dsel[k] : kind == 1u ? ssel[k] : bsel[k];
                 if (st == t) continue;
                 st = t;
                 sel.push_back(k);
             }
             for (uint32_t k : sel) {
-                if (dist) { need(m.d_v0[k]); need(m.d_v1[k]); }
-                else { need(m.b_v2[k]); need(m.b_v3[k]); }
+                if (kind == 0u) { need(m.d_v0[k]); need(m.d_v1[k]); }
+                else if (kind == 1u) { need(m.s_v0[k]); need(m.s_v1[k]); }
+                else {
+                    need(m.b_v2[k]); need(m.b_v3[k]);
+                    if (gated(k)) { need(m.b_v0[k]); need(m.b_v1[k]); }
+                }
             }
         }
         if (cone.size() > max_lds_vertices || cone.size() > 65535)
@@ -163,19 +178,27 @@ std::string build_tile_plan(const Mesh &m, uint32_t tile_vertices, uint32_t max_
             auto &sel = phase_sel[c];
             std::sort(sel.begin(), sel.end());
             P.ph_off[size_t(t) * (L + 1) + c] = uint32_t(P.items.size() / 3);
-            const bool dist = c < ndp;
+            const uint32_t kind = c < ndp ? 0u : c < nds ? 1u : 2u;
             for (uint32_t k : sel) {
-                const uint32_t a = dist ? m.d_v0[k] : m.b_v2[k];
-                const uint32_t b = dist ? m.d_v1[k] : m.b_v3[k];
+                const uint32_t a = kind == 0u ? m.d_v0[k] : kind == 1u ? m.s_v0[k] : m.b_v2[k];
+                const uint32_t b = kind == 0u ? m.d_v1[k] : kind == 1u ? m.s_v1[k] : m.b_v3[k];
                 const bool owner = tile_of[a] == t;
-                const uint32_t cls = dist ? dcls[k] : bcls[k];
-                const uint32_t shear = dist && m.d_type[k] == 1u ? 1u : 0u;
-                const float rest = dist ? m.d_rest[k] : m.b_rest[k];
-                uint32_t rb;
-                std::memcpy(&rb, &rest, 4);
+                uint32_t w1 = k | (owner ? 0x80000000u : 0u), w2 = 0u;
+                if (kind == 0u) {
+                    std::memcpy(&w2, &m.d_rest[k], 4);
+                    w1 |= (dcls[k] << 26) | (m.d_type[k] == 1u ? 1u << 30 : 0u);
+                } else if (kind == 2u) {
+                    w1 |= bcls[k] << 26;
+                    if (gated(k)) {  // the rest length is read from the bending record
+                        w1 |= 1u << 30;
+                        w2 = local[m.b_v0[k]] | (local[m.b_v1[k]] << 16);
+                    } else {
+                        std::memcpy(&w2, &m.b_rest[k], 4);
+                    }
+                }  // sewing: the current rest length lives in the sewing record (it shrinks and locks)
                 P.items.push_back(local[a] | (local[b] << 16));
-                P.items.push_back(k | (cls << 26) | (shear << 30) | (owner ? 0x80000000u : 0u));
-                P.items.push_back(rb);
+                P.items.push_back(w1);
+                P.items.push_back(w2);
             }
         }
         P.ph_off[size_t(t) * (L + 1) + L] = uint32_t(P.items.size() / 3);

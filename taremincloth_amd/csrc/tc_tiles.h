@@ -1,5 +1,5 @@
 // Cone-tile plan for running one full solver iteration (every distance colour,
-// then every bending colour) in a single kernel launch.
+// every sewing colour, then every bending colour) in a single kernel launch.
 //
 // The vertices are cut into tiles of spatially close vertices (recursive
 // coordinate bisection of the rest positions).  For each tile the planner walks the iteration's colour
@@ -22,8 +22,9 @@ namespace tc {
 
 struct TilePlan {
     uint32_t ntiles = 0;
-    uint32_t nphases = 0;       // distance colours + bending colours
+    uint32_t nphases = 0;       // distance colours + sewing colours + bending colours
     uint32_t nd_phases = 0;     // the first nd_phases phases are distance colours
+    uint32_t ns_phases = 0;     // then ns_phases sewing colours
     uint32_t max_ext = 0;       // largest cone (vertices) -> LDS size
     std::vector<uint32_t> ext_off;   // ntiles + 1
     std::vector<uint32_t> ext_vid;   // cone vertex ids; a tile's own vertices come first

@@ -378,12 +378,15 @@ def test_gpu_dynamic_shrink_speed_preserves_rest_length(require_gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("path", [1, 2])   # one launch per colour; one launch per iteration
 @pytest.mark.parametrize("stiffness,lock,d0,thick", [(10000.0, True, 1.0, None), (300.0, True, 1.0, None),
                                                      (2000.0, False, 0.015, None), (10000.0, True, 0.015, None),
                                                      (10000.0, True, 0.08, 0.05)])
-def test_gpu_pair_bit_exact(require_gpu, stiffness, lock, d0, thick):
+def test_gpu_pair_bit_exact(require_gpu, stiffness, lock, d0, thick, path):
     th = None if thick is None else np.full(2, thick, np.float32)
     g = _gpair(d0, thick=th, sewing_shrink_speed=0.3)
+    g.set_solver_path(path)
+    assert g.solver_info()["path"] == path
     o = _pair_sim(d0, precision="f32", thick=th, sewing_shrink_speed=0.3)
     for s in (g, o):
         s.set_sewing_stiffness(stiffness)
@@ -404,8 +407,11 @@ def _setup_drape(s):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,stride", [(10, 1), (24, 3)])
-def test_gpu_sewn_panels_drape_bit_exact(require_gpu, n, stride):
+@pytest.mark.parametrize("path", [1, 2])   # one launch per colour; one launch per iteration
+@pytest.mark.parametrize("n,stride,damp", [(10, 1, False), (24, 3, False), (12, 1, True)])
+def test_gpu_sewn_panels_drape_bit_exact(require_gpu, n, stride, damp, path):
+    """stride 1 sews every boundary vertex, so the seam edges carry seam-gated bending (mesh.rs:706-831);
+    both solver paths give the oracle's bits, with and without constraint damping."""
     pos, e, f, sew = sewn_panels(n, spacing=0.015, gap=0.04, stride=stride, seed=n)
     inv = np.ones(len(pos), np.float32)
     inv[[0, n * (n - 1), n * n + n - 1, 2 * n * n - 1]] = 0.0   # the outer corners (away from the seam) held
@@ -414,7 +420,16 @@ def test_gpu_sewn_panels_drape_bit_exact(require_gpu, n, stride):
     o = O.OracleSim(pos, e, f, inv, sewing_springs=sew, **kw)
     for s in (g, o):
         _setup_drape(s)
-    _say(f"sewn panels {n}x{n} x2, {len(sew)} sewing pairs, self-collision on")
+        if damp:
+            s.set_damping_all(0.3, 0.2, 0.25, 0.4)
+    g.set_solver_path(path)
+    info = g.solver_info()
+    assert info["path"] == path
+    if path == 2:
+        assert info["plan_available"] == 1.0, info["plan_note"]
+    if stride == 1:
+        assert int(np.count_nonzero((o.mesh()["bend_pad0"] > 0.5) & (o.mesh()["bend_pad0"] < 1.5))) > 0
+    _say(f"sewn panels {n}x{n} x2, {len(sew)} sewing pairs, self-collision on, path {path}, damping {damp}")
     for fr in range(6):
         g.step(1.0 / 60.0, 5, 6)
         o.step(1.0 / 60.0, 5, 6)
